@@ -1,0 +1,181 @@
+// Oracle math: plain-C++ restatement of the GLM 0.9.9.8 scalar operations that the
+// Physecs step uses, with the SAME association order, so that results are bit-identical
+// to a GLM build compiled with -ffp-contract=off.
+//
+// TEST INFRASTRUCTURE ONLY. This header belongs to the CPU oracle. Nothing in the
+// product (physecs_amd/) may include it.
+//
+// GLM sources restated (paths relative to the reference's vendor/glm 0.9.9.8/glm):
+//   dot (vec3)          detail/func_geometric.inl:45-53   ((x+y)+z)
+//   cross               detail/func_geometric.inl:65-77
+//   normalize           detail/func_geometric.inl:79-88   v * (1/sqrt(dot))
+//   length/distance     detail/func_geometric.inl:6-22
+//   min/max (ternary)   detail/func_common.inl:15-29
+//   clamp               detail/func_common.inl:503-523   min(max(x,lo),hi)
+//   mix                 detail/func_common.inl:124-132   x*(1-a) + y*a
+//   quat * vec3         detail/type_quat.inl:343-350
+//   quat * quat         detail/type_quat.inl:281-292
+//   quat dot            detail/type_quat.inl:16-23       (w*w+x*x)+(y*y+z*z)
+//   quat normalize      ext/quaternion_geometric.inl:17-24
+//   quat inverse        ext/quaternion_common.inl:119-122 conjugate / dot
+//   mat3_cast           gtc/quaternion.inl:41-66
+//   mat3 * vec3         detail/type_mat3x3.inl:468-474
+//   mat3 * mat3         detail/type_mat3x3.inl:486-519
+//   inverse(mat3)       detail/func_matrix.inl:268-291
+//   matrixCross3        gtx/matrix_cross_product.inl:6-19
+#pragma once
+#include <cmath>
+#include <cstdint>
+
+namespace orc {
+
+struct V2 { float x, y; };
+struct V3 {
+    float x, y, z;
+    float& operator[](int i) { return (&x)[i]; }
+    float operator[](int i) const { return (&x)[i]; }
+};
+struct Q { float x, y, z, w; };              // memory order x,y,z,w (GLM default)
+struct M3 {                                   // column-major: c[i] is column i, m[i][j] = c[i][j]
+    V3 c[3];
+    V3& operator[](int i) { return c[i]; }
+    const V3& operator[](int i) const { return c[i]; }
+};
+
+inline V3 v3(float x, float y, float z) { return {x, y, z}; }
+inline V3 v3s(float s) { return {s, s, s}; }
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+inline V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator*(float s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline V3 operator*(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+inline V3 operator/(V3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+inline V3 operator/(V3 a, V3 b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+inline V3 operator-(V3 a, float s) { return {a.x - s, a.y - s, a.z - s}; }
+inline V3 operator+(V3 a, float s) { return {a.x + s, a.y + s, a.z + s}; }
+
+inline float gmin(float x, float y) { return (y < x) ? y : x; }
+inline float gmax(float x, float y) { return (x < y) ? y : x; }
+inline float gabs(float x) { return x >= 0.f ? x : -x; }
+inline float gclamp(float x, float lo, float hi) { return gmin(gmax(x, lo), hi); }
+inline float gmix(float x, float y, float a) { return x * (1.f - a) + y * a; }
+inline float gsign(float x) { return (float)(0.f < x) - (float)(x < 0.f); }
+inline V3 vmin(V3 a, V3 b) { return {gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z)}; }
+inline V3 vmax(V3 a, V3 b) { return {gmax(a.x, b.x), gmax(a.y, b.y), gmax(a.z, b.z)}; }
+inline V3 vabs(V3 a) { return {gabs(a.x), gabs(a.y), gabs(a.z)}; }
+inline V3 vclamp(V3 x, V3 lo, V3 hi) { return vmin(vmax(x, lo), hi); }
+
+inline float dot(V3 a, V3 b) { V3 t = a * b; return t.x + t.y + t.z; }
+inline float dot2(V2 a, V2 b) { return a.x * b.x + a.y * b.y; }
+inline V3 cross(V3 x, V3 y) {
+    return {x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y};
+}
+inline float length(V3 v) { return std::sqrt(dot(v, v)); }
+inline float distance(V3 a, V3 b) { return length(b - a); }
+inline float distance2(V3 a, V3 b) { V3 d = b - a; return dot(d, d); }
+inline V3 normalize(V3 v) { return v * (1.f / std::sqrt(dot(v, v))); }
+
+inline V2 operator+(V2 a, V2 b) { return {a.x + b.x, a.y + b.y}; }
+inline V2 operator-(V2 a, V2 b) { return {a.x - b.x, a.y - b.y}; }
+inline V2 operator-(V2 a) { return {-a.x, -a.y}; }
+inline V2 operator*(float s, V2 a) { return {s * a.x, s * a.y}; }
+// determinant(mat2(c0, c1)) = c0.x*c1.y - c1.x*c0.y   (detail/func_matrix.inl:176-182)
+inline float det2(V2 c0, V2 c1) { return c0.x * c1.y - c1.x * c0.y; }
+
+// quaternion
+inline Q qmake(float w, float x, float y, float z) { return {x, y, z, w}; }
+inline Q qmul(Q p, Q q) {
+    Q r;
+    r.w = p.w * q.w - p.x * q.x - p.y * q.y - p.z * q.z;
+    r.x = p.w * q.x + p.x * q.w + p.y * q.z - p.z * q.y;
+    r.y = p.w * q.y + p.y * q.w + p.z * q.x - p.x * q.z;
+    r.z = p.w * q.z + p.z * q.w + p.x * q.y - p.y * q.x;
+    return r;
+}
+inline V3 qrot(Q q, V3 v) {
+    V3 qv = {q.x, q.y, q.z};
+    V3 uv = cross(qv, v);
+    V3 uuv = cross(qv, uv);
+    return v + ((uv * q.w) + uuv) * 2.f;
+}
+inline float qdot(Q a, Q b) {
+    float tx = a.w * b.w, ty = a.x * b.x, tz = a.y * b.y, tw = a.z * b.z;
+    return (tx + ty) + (tz + tw);
+}
+inline Q qinverse(Q q) {
+    float d = qdot(q, q);
+    return qmake(q.w / d, -q.x / d, -q.y / d, -q.z / d);
+}
+inline Q qnormalize(Q q) {
+    float len = std::sqrt(qdot(q, q));
+    if (len <= 0.f) return qmake(1.f, 0.f, 0.f, 0.f);
+    float o = 1.f / len;
+    return qmake(q.w * o, q.x * o, q.y * o, q.z * o);
+}
+inline Q qadd(Q q, Q p) { return qmake(q.w + p.w, q.x + p.x, q.y + p.y, q.z + p.z); }
+
+inline M3 mat3_cast(Q q) {
+    float qxx = q.x * q.x, qyy = q.y * q.y, qzz = q.z * q.z;
+    float qxz = q.x * q.z, qxy = q.x * q.y, qyz = q.y * q.z;
+    float qwx = q.w * q.x, qwy = q.w * q.y, qwz = q.w * q.z;
+    M3 r;
+    r[0][0] = 1.f - 2.f * (qyy + qzz);
+    r[0][1] = 2.f * (qxy + qwz);
+    r[0][2] = 2.f * (qxz - qwy);
+    r[1][0] = 2.f * (qxy - qwz);
+    r[1][1] = 1.f - 2.f * (qxx + qzz);
+    r[1][2] = 2.f * (qyz + qwx);
+    r[2][0] = 2.f * (qxz + qwy);
+    r[2][1] = 2.f * (qyz - qwx);
+    r[2][2] = 1.f - 2.f * (qxx + qyy);
+    return r;
+}
+inline M3 mzero() { M3 m; for (int i = 0; i < 3; ++i) m[i] = v3s(0.f); return m; }
+inline M3 mdiag(float s) { M3 m = mzero(); m[0][0] = m[1][1] = m[2][2] = s; return m; }
+inline V3 mulmv(const M3& m, V3 v) {
+    return {m[0][0] * v.x + m[1][0] * v.y + m[2][0] * v.z,
+            m[0][1] * v.x + m[1][1] * v.y + m[2][1] * v.z,
+            m[0][2] * v.x + m[1][2] * v.y + m[2][2] * v.z};
+}
+inline M3 mulmm(const M3& a, const M3& b) {
+    M3 r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            r[i][j] = a[0][j] * b[i][0] + a[1][j] * b[i][1] + a[2][j] * b[i][2];
+    return r;
+}
+inline M3 transpose(const M3& m) {
+    M3 r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r[i][j] = m[j][i];
+    return r;
+}
+inline M3 madd(const M3& a, const M3& b) { M3 r; for (int i = 0; i < 3; ++i) r[i] = a[i] + b[i]; return r; }
+inline M3 msub(const M3& a, const M3& b) { M3 r; for (int i = 0; i < 3; ++i) r[i] = a[i] - b[i]; return r; }
+inline M3 mscale(float s, const M3& a) { M3 r; for (int i = 0; i < 3; ++i) r[i] = a[i] * s; return r; }
+inline M3 inverse(const M3& m) {
+    float ood = 1.f / (+m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2])
+                       - m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2])
+                       + m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]));
+    M3 r;
+    r[0][0] = +(m[1][1] * m[2][2] - m[2][1] * m[1][2]) * ood;
+    r[1][0] = -(m[1][0] * m[2][2] - m[2][0] * m[1][2]) * ood;
+    r[2][0] = +(m[1][0] * m[2][1] - m[2][0] * m[1][1]) * ood;
+    r[0][1] = -(m[0][1] * m[2][2] - m[2][1] * m[0][2]) * ood;
+    r[1][1] = +(m[0][0] * m[2][2] - m[2][0] * m[0][2]) * ood;
+    r[2][1] = -(m[0][0] * m[2][1] - m[2][0] * m[0][1]) * ood;
+    r[0][2] = +(m[0][1] * m[1][2] - m[1][1] * m[0][2]) * ood;
+    r[1][2] = -(m[0][0] * m[1][2] - m[1][0] * m[0][2]) * ood;
+    r[2][2] = +(m[0][0] * m[1][1] - m[1][0] * m[0][1]) * ood;
+    return r;
+}
+inline M3 matrixCross3(V3 x) {
+    M3 r = mzero();
+    r[0][1] = x.z; r[1][0] = -x.z;
+    r[0][2] = -x.y; r[2][0] = x.y;
+    r[1][2] = x.x; r[2][1] = -x.x;
+    return r;
+}
+
+}  // namespace orc

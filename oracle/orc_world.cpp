@@ -1,0 +1,697 @@
+// Oracle world step: CPU restatement of physecs::Scene::simulate (src/Physecs.cpp:112-561)
+// for primitive shapes, with a C ABI for the tests and for the CPU-baseline timing.
+//
+// TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load this library, and only as the checker or the timed CPU baseline.
+//
+// What each part follows:
+//   SAP sort + sweep             src/Physecs.cpp:121-172
+//   narrowphase + contact setup  src/Physecs.cpp:181-317 (ThreadPool order: src/ThreadPool.cpp:3-10)
+//   substep loop                 src/Physecs.cpp:363-531
+//   contact rows                 src/ContactConstraints.cpp:4-124
+//   cache swap + bounds          src/Physecs.cpp:553-559, 79-90
+//   joint rows are not restated here (no joints in the plain configurations).
+// Differences by design: trigger pairs are skipped (no trigger events), and the order of
+// contact constraints can be injected (set_order) so that a solver fed the GPU's graph
+// color order can be compared with it.
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+#include "orc_collision.h"
+
+using namespace orc;
+
+namespace {
+
+struct Actor { uint32_t entity; V3 pos; Q quat; int dyn; };
+struct Dyn { int actor; bool kinematic; V3 v, w; float invMass; V3 com; M3 invI; };
+struct Collider {
+    int actor, index; V3 lpos; Q lquat; Geom geom;
+    float friction, restitution, damping; bool trigger, enable; int data;
+};
+struct BPEntry { int collider; uint32_t entity; int colliderIndex; Bounds bounds; bool enable; bool isDynamic; };
+struct Pair { uint32_t e0; int c0; uint32_t e1; int c1; int g0, g1; };
+struct PointC {
+    V3 r0, r1, r0xn, r1xn, t, r0xt, r1xt;
+    float tv, c, lN, lT;
+    V3 r0xnt, r1xnt, r0xtt, r1xtt;
+    float invEffN, invEffT;
+};
+struct CC {
+    int a0, a1, d0, d1, b0, b1;
+    V3 n; float friction; bool isSoft; float freq, damp; int numPoints;
+    PointC pts[4];
+    int32_t key[5];          // e0, c0, e1, c1, triangleIndex
+    Manifold raw;            // narrowphase result (world points), for comparisons
+};
+struct Key {
+    uint32_t e0; int c0; uint32_t e1; int c1; int tri;
+    bool operator==(const Key& o) const { return e0 == o.e0 && c0 == o.c0 && e1 == o.e1 && c1 == o.c1 && tri == o.tri; }
+};
+struct KeyHash {
+    size_t operator()(const Key& k) const {
+        uint64_t h = ((uint64_t)k.e0 << 32) ^ (uint64_t)(uint32_t)k.e1;
+        h ^= (uint64_t)(uint32_t)k.c0 * 0x9E3779B97F4A7C15ull;
+        h ^= (uint64_t)(uint32_t)k.c1 * 0xC2B2AE3D27D4EB4Full;
+        h ^= (uint64_t)(uint32_t)k.tri * 0x165667B19E3779F9ull;
+        return (size_t)h;
+    }
+};
+struct CacheData { int numPoints; V3 lp[4]; float tv[4]; };
+
+}  // namespace
+
+struct orc_world {
+    std::vector<Actor> actors;
+    std::vector<Dyn> dyn;
+    std::vector<Collider> cols;
+    std::vector<BPEntry> bp;
+    std::vector<Pair> pairs;
+    std::vector<CC> ccs;
+    std::unordered_map<Key, CacheData, KeyHash> cache, cacheTemp;
+    std::unordered_set<uint64_t> nonColliding;
+    std::vector<int32_t> order;   // injected manifold order (5 ints per manifold)
+    int numSubSteps = 8, numIterations = 2, numThreads = 0;
+    float g = 9.81f;
+    std::mutex mtx;
+};
+
+extern "C" {
+
+struct orc_scene {
+    int n_actors;
+    const uint32_t* entity; const float* pos; const float* quat; const int32_t* dyn_of_actor;
+    int n_dyn;
+    const int32_t* actor_of_dyn; const uint8_t* kinematic; const float* vel; const float* angvel;
+    const float* inv_mass; const float* com; const float* inv_inertia;
+    int n_colliders;
+    const int32_t* col_actor; const int32_t* col_index; const float* col_pos; const float* col_quat;
+    const int32_t* col_type; const float* col_params; const float* col_material;
+    const uint8_t* col_trigger; const uint8_t* col_enable; const int32_t* col_data;
+};
+
+orc_world* orc_world_create(const orc_scene* s) {
+    auto* w = new orc_world();
+    w->actors.resize(s->n_actors);
+    for (int i = 0; i < s->n_actors; ++i) {
+        auto& a = w->actors[i];
+        a.entity = s->entity[i];
+        a.pos = v3(s->pos[3 * i], s->pos[3 * i + 1], s->pos[3 * i + 2]);
+        a.quat = {s->quat[4 * i], s->quat[4 * i + 1], s->quat[4 * i + 2], s->quat[4 * i + 3]};
+        a.dyn = s->dyn_of_actor[i];
+    }
+    w->dyn.resize(s->n_dyn);
+    for (int i = 0; i < s->n_dyn; ++i) {
+        auto& d = w->dyn[i];
+        d.actor = s->actor_of_dyn[i];
+        d.kinematic = s->kinematic[i] != 0;
+        d.v = v3(s->vel[3 * i], s->vel[3 * i + 1], s->vel[3 * i + 2]);
+        d.w = v3(s->angvel[3 * i], s->angvel[3 * i + 1], s->angvel[3 * i + 2]);
+        d.invMass = s->inv_mass[i];
+        d.com = v3(s->com[3 * i], s->com[3 * i + 1], s->com[3 * i + 2]);
+        for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r) d.invI[c][r] = s->inv_inertia[9 * i + 3 * c + r];
+    }
+    w->cols.resize(s->n_colliders);
+    w->bp.resize(s->n_colliders);
+    for (int i = 0; i < s->n_colliders; ++i) {
+        auto& c = w->cols[i];
+        c.actor = s->col_actor[i];
+        c.index = s->col_index[i];
+        c.lpos = v3(s->col_pos[3 * i], s->col_pos[3 * i + 1], s->col_pos[3 * i + 2]);
+        c.lquat = {s->col_quat[4 * i], s->col_quat[4 * i + 1], s->col_quat[4 * i + 2], s->col_quat[4 * i + 3]};
+        c.geom.type = s->col_type[i];
+        for (int k = 0; k < 4; ++k) c.geom.p[k] = s->col_params[4 * i + k];
+        c.friction = s->col_material[3 * i];
+        c.restitution = s->col_material[3 * i + 1];
+        c.damping = s->col_material[3 * i + 2];
+        c.trigger = s->col_trigger[i] != 0;
+        c.enable = s->col_enable[i] != 0;
+        c.data = s->col_data[i];
+        // src/Physecs.cpp:25-37: creation-time bounds carry no margin
+        const Actor& a = w->actors[c.actor];
+        int dy = a.dyn;
+        bool isDynamic = dy >= 0 ? !w->dyn[dy].kinematic : false;
+        Bounds b = getBounds(a.pos + qrot(a.quat, c.lpos), qmul(a.quat, c.lquat), c.geom);
+        w->bp[i] = {i, a.entity, c.index, b, c.enable, isDynamic};
+    }
+    return w;
+}
+
+void orc_world_destroy(orc_world* w) { delete w; }
+
+void orc_world_set_params(orc_world* w, int substeps, int iterations, float gravity, int threads) {
+    w->numSubSteps = substeps;
+    w->numIterations = iterations;
+    w->g = gravity;
+    w->numThreads = threads;
+}
+
+// src/Physecs.cpp:788-793 (setCanCollide(e0, e1, false))
+void orc_world_set_non_colliding(orc_world* w, uint32_t e0, uint32_t e1) {
+    uint32_t a = e0 < e1 ? e0 : e1, b = e0 < e1 ? e1 : e0;
+    w->nonColliding.insert(((uint64_t)a << 32) | b);
+}
+
+void orc_world_set_order(orc_world* w, const int32_t* keys, int n) {
+    w->order.assign(keys, keys + 5 * (size_t)n);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- step pieces
+static void sap(orc_world* w) {
+    auto& e = w->bp;
+    // src/Physecs.cpp:121-133: stable insertion sort on min.x
+    for (size_t i = 1; i < e.size(); ++i) {
+        BPEntry x = e[i];
+        size_t j = i;
+        while (j > 0 && e[j - 1].bounds.min.x > x.bounds.min.x) { e[j] = e[j - 1]; --j; }
+        if (j < i) e[j] = x;
+    }
+    // src/Physecs.cpp:134-172
+    w->pairs.clear();
+    for (size_t i = 0; i + 1 < e.size(); ++i) {
+        auto& e0 = e[i];
+        if (!e0.enable) continue;
+        for (size_t j = i + 1; j < e.size(); ++j) {
+            auto& e1 = e[j];
+            if (!e1.enable) continue;
+            if (e0.entity == e1.entity) continue;
+            if (!(e0.isDynamic || e1.isDynamic)) continue;
+            const Bounds& b0 = e0.bounds;
+            const Bounds& b1 = e1.bounds;
+            if (b0.max.x < b1.min.x) break;
+            if (b0.max.y < b1.min.y || b0.min.y > b1.max.y) continue;
+            if (b0.max.z < b1.min.z || b0.min.z > b1.max.z) continue;
+            if (e0.entity < e1.entity) w->pairs.push_back({e0.entity, e0.colliderIndex, e1.entity, e1.colliderIndex, e0.collider, e1.collider});
+            else w->pairs.push_back({e1.entity, e1.colliderIndex, e0.entity, e0.colliderIndex, e1.collider, e0.collider});
+        }
+    }
+}
+
+// src/Physecs.cpp:183-316, one potential contact
+static void narrowTask(orc_world* w, int index, std::vector<CC>& outCC, std::vector<std::pair<Key, CacheData>>& outCache) {
+    const Pair& cp = w->pairs[index];
+    const Collider& col0 = w->cols[cp.g0];
+    const Collider& col1 = w->cols[cp.g1];
+    const Actor& t0 = w->actors[col0.actor];
+    const Actor& t1 = w->actors[col1.actor];
+    V3 pos0 = t0.pos + qrot(t0.quat, col0.lpos);
+    Q or0 = qmul(t0.quat, col0.lquat);
+    V3 pos1 = t1.pos + qrot(t1.quat, col1.lpos);
+    Q or1 = qmul(t1.quat, col1.lquat);
+    if (col0.trigger || col1.trigger) return;   // defaultContactFilter -> TRIGGER (no contact)
+    uint64_t ek = ((uint64_t)cp.e0 << 32) | cp.e1;
+    if (w->nonColliding.count(ek)) return;
+    std::vector<Manifold> buf;
+    if (!collision(pos0, or0, col0.geom, pos1, or1, col1.geom, buf)) return;
+    const Dyn* dyn0 = t0.dyn >= 0 ? &w->dyn[t0.dyn] : nullptr;
+    const Dyn* dyn1 = t1.dyn >= 0 ? &w->dyn[t1.dyn] : nullptr;
+    V3 com0 = v3s(0), v0 = v3s(0), w0 = v3s(0);
+    if (dyn0 && !dyn0->kinematic) { com0 = t0.pos + qrot(t0.quat, dyn0->com); v0 = dyn0->v; w0 = dyn0->w; }
+    V3 com1 = v3s(0), v1 = v3s(0), w1 = v3s(0);
+    if (dyn1 && !dyn1->kinematic) { com1 = t1.pos + qrot(t1.quat, dyn1->com); v1 = dyn1->v; w1 = dyn1->w; }
+    for (auto& res : buf) {
+        if (!res.numPoints) continue;
+        V3 n = res.normal;
+        Key key{cp.e0, cp.c0, cp.e1, cp.c1, res.triangleIndex};
+        auto it = w->cache.find(key);
+        const CacheData* prev = it != w->cache.end() ? &it->second : nullptr;
+        CacheData cur{res.numPoints, {}, {}};
+        float friction = (col0.friction + col1.friction) * 0.5f;
+        bool isSoft; float frequency, damping, restitution;
+        if (col0.damping != 0.f || col1.damping != 0.f) {
+            isSoft = true;
+            if (col0.damping != 0.f && col1.damping != 0.f) {
+                frequency = gmin(col0.restitution, col1.restitution);
+                damping = gmin(col0.damping, col1.damping);
+            } else if (col0.damping != 0.f) {
+                frequency = col0.restitution; damping = col0.damping;
+            } else {
+                frequency = col1.restitution; damping = col1.damping;
+            }
+            restitution = 0;
+        } else {
+            isSoft = false; frequency = 0; damping = 0;
+            restitution = (col0.restitution + col1.restitution) * 0.5f;
+        }
+        CC cc;
+        cc.a0 = col0.actor; cc.a1 = col1.actor;
+        cc.d0 = t0.dyn; cc.d1 = t1.dyn;
+        cc.b0 = dyn0 && !dyn0->kinematic ? t0.dyn : -1;
+        cc.b1 = dyn1 && !dyn1->kinematic ? t1.dyn : -1;
+        cc.n = n; cc.friction = friction; cc.isSoft = isSoft; cc.freq = frequency; cc.damp = damping;
+        cc.numPoints = res.numPoints;
+        cc.key[0] = (int32_t)cp.e0; cc.key[1] = cp.c0; cc.key[2] = (int32_t)cp.e1; cc.key[3] = cp.c1; cc.key[4] = res.triangleIndex;
+        cc.raw = res;
+        for (int k = 0; k < res.numPoints; ++k) {
+            V3 r0 = res.points[k].p0 - com0;
+            V3 r1 = res.points[k].p1 - com1;
+            V3 rel = v1 + cross(w1, r1) - v0 - cross(w0, r0);
+            float relN = dot(rel, n);
+            r0 = qrot(qinverse(t0.quat), r0);
+            r1 = qrot(qinverse(t1.quat), r1);
+            float tv = 0;
+            bool found = false;
+            if (prev) {
+                for (int i = 0; i < prev->numPoints; ++i) {
+                    // glm::distance(r0, prev) < 0.1 compares in double (Physecs.cpp:294)
+                    if ((double)distance(r0, prev->lp[i]) < 0.1) { found = true; tv = prev->tv[i]; break; }
+                }
+            }
+            if (!found) tv = -restitution * relN;
+            cur.lp[k] = r0; cur.tv[k] = tv;
+            PointC& pc = cc.pts[k];
+            std::memset(&pc, 0, sizeof(pc));
+            pc.r0 = r0; pc.r1 = r1; pc.tv = tv;
+        }
+        outCC.push_back(cc);
+        outCache.push_back({key, cur});
+    }
+}
+
+static void narrowphase(orc_world* w) {
+    w->ccs.clear();
+    w->cacheTemp.clear();
+    int count = (int)w->pairs.size();
+    if (w->numThreads <= 0) {
+        // ThreadPool with no workers: the caller drains indices in descending order
+        std::vector<CC> cc;
+        std::vector<std::pair<Key, CacheData>> cd;
+        for (int i = count - 1; i >= 0; --i) {
+            cc.clear(); cd.clear();
+            narrowTask(w, i, cc, cd);
+            for (auto& x : cc) w->ccs.push_back(x);
+            for (auto& x : cd) w->cacheTemp[x.first] = x.second;
+        }
+        return;
+    }
+    // src/ThreadPool.cpp:3-10,30-46: chunk = max(1, i / ((T+1)*2)), claimed by CAS, descending
+    int T = w->numThreads;
+    std::atomic<int> current(count);
+    auto worker = [&]() {
+        std::vector<CC> cc;
+        std::vector<std::pair<Key, CacheData>> cd;
+        int i = current.load();
+        while (i) {
+            int n = std::max(1, i / ((T + 1) * 2));
+            if (!current.compare_exchange_weak(i, i - n)) continue;
+            for (int j = 0; j < n; ++j) {
+                cc.clear(); cd.clear();
+                narrowTask(w, --i, cc, cd);
+                if (!cc.empty()) {
+                    std::lock_guard<std::mutex> lk(w->mtx);
+                    for (auto& x : cc) w->ccs.push_back(x);
+                    for (auto& x : cd) w->cacheTemp[x.first] = x.second;
+                }
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t) th.emplace_back(worker);
+    worker();
+    for (auto& t : th) t.join();
+}
+
+static void applyOrder(orc_world* w) {
+    if (w->order.empty()) return;
+    size_t n = w->order.size() / 5;
+    std::unordered_map<Key, size_t, KeyHash> pos;
+    for (size_t i = 0; i < w->ccs.size(); ++i) {
+        auto& k = w->ccs[i].key;
+        pos[{(uint32_t)k[0], k[1], (uint32_t)k[2], k[3], k[4]}] = i;
+    }
+    std::vector<CC> out;
+    out.reserve(w->ccs.size());
+    std::vector<char> used(w->ccs.size(), 0);
+    for (size_t i = 0; i < n; ++i) {
+        const int32_t* k = &w->order[5 * i];
+        auto it = pos.find({(uint32_t)k[0], k[1], (uint32_t)k[2], k[3], k[4]});
+        if (it == pos.end()) continue;
+        out.push_back(w->ccs[it->second]);
+        used[it->second] = 1;
+    }
+    for (size_t i = 0; i < w->ccs.size(); ++i)
+        if (!used[i]) out.push_back(w->ccs[i]);
+    w->ccs.swap(out);
+    w->order.clear();
+}
+
+struct VelocityData { V3 v, w; };
+struct PseudoVelocityData { V3 pv, pw; int count; };
+struct MassData { float invMass; M3 invI; };
+
+// src/ContactConstraints.cpp:4-31
+static void preSolve(CC& c, const MassData* masses) {
+    float im0 = 0; M3 I0 = mzero();
+    if (c.b0 >= 0) { im0 = masses[c.b0].invMass; I0 = masses[c.b0].invI; }
+    float im1 = 0; M3 I1 = mzero();
+    if (c.b1 >= 0) { im1 = masses[c.b1].invMass; I1 = masses[c.b1].invI; }
+    for (int i = 0; i < c.numPoints; ++i) {
+        PointC& p = c.pts[i];
+        p.r0xnt = mulmv(I0, p.r0xn);
+        p.r1xnt = mulmv(I1, p.r1xn);
+        p.r0xtt = mulmv(I0, p.r0xt);
+        p.r1xtt = mulmv(I1, p.r1xt);
+        p.invEffN = dot(c.n, c.n) * (im0 + im1) + dot(p.r0xn, p.r0xnt) + dot(p.r1xn, p.r1xnt);
+        p.invEffT = dot(p.t, p.t) * (im0 + im1) + dot(p.r0xt, p.r0xtt) + dot(p.r1xt, p.r1xtt);
+    }
+}
+
+// src/ContactConstraints.cpp:33-124
+static void solveContact(orc_world* w, CC& c, VelocityData* vel, bool useBias, float h) {
+    const Dyn* dyn0 = c.d0 >= 0 ? &w->dyn[c.d0] : nullptr;
+    const Dyn* dyn1 = c.d1 >= 0 ? &w->dyn[c.d1] : nullptr;
+    for (int i = 0; i < c.numPoints; ++i) {
+        PointC& p = c.pts[i];
+        if (p.invEffN == 0.f) continue;
+        V3 v0 = v3s(0), w0 = v3s(0), v1 = v3s(0), w1 = v3s(0);
+        if (c.b0 >= 0) { v0 = vel[c.b0].v; w0 = vel[c.b0].w; }
+        if (c.b1 >= 0) { v1 = vel[c.b1].v; w1 = vel[c.b1].w; }
+        float rv = dot(-c.n, v0) + dot(-p.r0xn, w0) + dot(c.n, v1) + dot(p.r1xn, w1);
+        float effMass = 1.f / p.invEffN;
+        float lambda;
+        if (c.isSoft) {
+            float af = 2.f * 3.14159265358979323846264338327950288f * c.freq;
+            float stiffness = af * af * effMass;
+            float damping = 2.f * af * c.damp * effMass;
+            float gamma = 1.f / (damping + h * stiffness);
+            float beta = h * stiffness / (damping + h * stiffness);
+            lambda = (rv + beta * p.c / h) / (p.invEffN + gamma / h);
+        } else {
+            lambda = (rv - p.tv + (useBias ? 0.1f * p.c / h : 0.f)) * effMass;
+        }
+        float prev = p.lN;
+        p.lN += lambda;
+        p.lN = gmin(p.lN, 0.f);
+        lambda = p.lN - prev;
+        if (c.b0 >= 0) { vel[c.b0].v = vel[c.b0].v + lambda * dyn0->invMass * c.n; vel[c.b0].w = vel[c.b0].w + lambda * p.r0xnt; }
+        if (c.b1 >= 0) { vel[c.b1].v = vel[c.b1].v - lambda * dyn1->invMass * c.n; vel[c.b1].w = vel[c.b1].w - lambda * p.r1xnt; }
+    }
+    for (int i = 0; i < c.numPoints; ++i) {
+        PointC& p = c.pts[i];
+        if (p.invEffT == 0.f) continue;
+        V3 v0 = v3s(0), w0 = v3s(0), v1 = v3s(0), w1 = v3s(0);
+        if (c.b0 >= 0) { v0 = dyn0->v; w0 = dyn0->w; }     // component velocities (quirk)
+        if (c.b1 >= 0) { v1 = dyn1->v; w1 = dyn1->w; }
+        float rv = dot(-p.t, v0) + dot(-p.r0xt, w0) + dot(p.t, v1) + dot(p.r1xt, w1);
+        float lambda = rv / p.invEffT;
+        float limit = c.friction * p.lN;
+        float prev = p.lT;
+        p.lT += lambda;
+        p.lT = gclamp(p.lT, limit, -limit);
+        lambda = p.lT - prev;
+        if (c.b0 >= 0) { vel[c.b0].v = vel[c.b0].v + lambda * dyn0->invMass * p.t; vel[c.b0].w = vel[c.b0].w + lambda * p.r0xtt; }
+        if (c.b1 >= 0) { vel[c.b1].v = vel[c.b1].v - lambda * dyn1->invMass * p.t; vel[c.b1].w = vel[c.b1].w - lambda * p.r1xtt; }
+    }
+}
+
+// src/MathUtil.h:10-21
+static V3 solve33(const M3& A, V3 b) {
+    V3 c12 = cross(A[1], A[2]);
+    float det = dot(A[0], c12);
+    if (det == 0.f) return v3s(0);
+    float id = 1.f / det;
+    return v3(id * dot(b, c12), id * dot(A[0], cross(b, A[2])), id * dot(A[0], cross(A[1], b)));
+}
+
+
+// src/Physecs.cpp:450-467 for one body
+static void integrateVelocity(Dyn& d, Q quat, float h, float g, MassData& md) {
+    M3 rot = mat3_cast(quat);
+    M3 invRot = transpose(rot);
+    d.v = d.v + h * v3(0, -g, 0);
+    V3 wl = mulmv(invRot, d.w);
+    M3 I = inverse(d.invI);
+    V3 f = h * cross(wl, mulmv(I, wl));
+    M3 J = madd(I, mscale(h, msub(mulmm(matrixCross3(wl), I), matrixCross3(mulmv(I, wl)))));
+    wl = wl - solve33(J, f);
+    d.w = mulmv(rot, wl);
+    md = {d.invMass, mulmm(mulmm(rot, d.invI), invRot)};
+}
+
+// src/Physecs.cpp:501-510 for one body
+static void integratePosition(V3& pos, Q& quat, const VelocityData& vel, const PseudoVelocityData& pvel, V3 com, float h) {
+    float s = pvel.count ? 1.f / pvel.count : 1.f;
+    pos = pos + (h * vel.v + s * pvel.pv);
+    V3 prevCom = qrot(quat, com);
+    V3 hv = 0.5f * (h * vel.w + s * pvel.pw);
+    quat = qadd(quat, qmul(qmake(0.f, hv.x, hv.y, hv.z), quat));
+    quat = qnormalize(quat);
+    pos = pos + (prevCom - qrot(quat, com));
+}
+
+static void substeps(orc_world* w, float timeStep) {
+    size_t nd = w->dyn.size();
+    std::vector<VelocityData> vel(nd);
+    std::vector<PseudoVelocityData> pvel(nd);
+    std::vector<MassData> mass(nd);
+    float h = timeStep / w->numSubSteps;
+    for (int m = 0; m < w->numSubSteps; ++m) {
+        // update contact constraints, src/Physecs.cpp:368-426
+        for (auto& c : w->ccs) {
+            const Actor& t0 = w->actors[c.a0];
+            const Actor& t1 = w->actors[c.a1];
+            const Dyn* dyn0 = c.d0 >= 0 ? &w->dyn[c.d0] : nullptr;
+            const Dyn* dyn1 = c.d1 >= 0 ? &w->dyn[c.d1] : nullptr;
+            V3 com0 = v3s(0), v0 = v3s(0), w0 = v3s(0);
+            if (dyn0 && !dyn0->kinematic) { com0 = t0.pos + qrot(t0.quat, dyn0->com); v0 = dyn0->v; w0 = dyn0->w; }
+            V3 com1 = v3s(0), v1 = v3s(0), w1 = v3s(0);
+            if (dyn1 && !dyn1->kinematic) { com1 = t1.pos + qrot(t1.quat, dyn1->com); v1 = dyn1->v; w1 = dyn1->w; }
+            for (int k = 0; k < c.numPoints; ++k) {
+                PointC& p = c.pts[k];
+                V3 r0 = qrot(t0.quat, p.r0);
+                V3 r1 = qrot(t1.quat, p.r1);
+                V3 cp0 = com0 + r0;
+                V3 cp1 = com1 + r1;
+                float cn = dot(cp1 - cp0, c.n);
+                V3 r0xn = cross(r0, c.n);
+                V3 r1xn = cross(r1, c.n);
+                V3 rel = v1 + cross(w1, r1) - v0 - cross(w0, r0);
+                float relN = dot(rel, c.n);
+                V3 t = rel - relN * c.n;
+                float tl = length(t);
+                if (tl != 0.f) t = t / tl;
+                p.r0xn = r0xn; p.r1xn = r1xn; p.t = t;
+                p.r0xt = cross(r0, t); p.r1xt = cross(r1, t);
+                p.c = cn; p.lN = 0; p.lT = 0;
+            }
+        }
+        // integrate velocities, src/Physecs.cpp:443-468
+        for (size_t i = 0; i < nd; ++i) {
+            Dyn& d = w->dyn[i];
+            if (d.kinematic) continue;
+            integrateVelocity(d, w->actors[d.actor].quat, h, w->g, mass[i]);
+            vel[i] = {d.v, d.w};
+            pvel[i] = {v3s(0), v3s(0), 0};
+        }
+        for (auto& c : w->ccs) preSolve(c, mass.data());
+        for (int it = 0; it < w->numIterations; ++it)
+            for (auto& c : w->ccs) solveContact(w, c, vel.data(), true, h);
+        // integrate positions, src/Physecs.cpp:494-511
+        for (size_t i = 0; i < nd; ++i) {
+            Dyn& d = w->dyn[i];
+            if (d.kinematic) continue;
+            Actor& tr = w->actors[d.actor];
+            integratePosition(tr.pos, tr.quat, vel[i], pvel[i], d.com, h);
+        }
+        // relaxation, src/Physecs.cpp:516-519
+        for (auto& c : w->ccs) {
+            if (c.isSoft) continue;
+            solveContact(w, c, vel.data(), false, 0.f);
+        }
+        // write back, src/Physecs.cpp:523-530
+        for (size_t i = 0; i < nd; ++i) {
+            Dyn& d = w->dyn[i];
+            if (d.kinematic) continue;
+            d.v = vel[i].v; d.w = vel[i].w;
+        }
+    }
+}
+
+extern "C" {
+
+// physecs::Scene::simulate restated: SAP, narrowphase, (injected order), substeps,
+// cache swap, bounds update with the 0.01 margin for non-kinematic dynamics.
+void orc_world_step(orc_world* w, float timeStep) {
+    sap(w);
+    narrowphase(w);
+    applyOrder(w);
+    substeps(w, timeStep);
+    w->cache.swap(w->cacheTemp);
+    for (auto& e : w->bp) {
+        const Collider& c = w->cols[e.collider];
+        const Actor& a = w->actors[c.actor];
+        if (a.dyn < 0 || w->dyn[a.dyn].kinematic) continue;
+        Bounds b = getBounds(a.pos + qrot(a.quat, c.lpos), qmul(a.quat, c.lquat), c.geom);
+        b.max = b.max + v3s(0.01f);
+        b.min = b.min - v3s(0.01f);
+        e.bounds = b;
+    }
+}
+
+void orc_world_get_state(const orc_world* w, float* pos, float* quat, float* vel, float* angvel) {
+    for (size_t i = 0; i < w->actors.size(); ++i) {
+        const Actor& a = w->actors[i];
+        pos[3 * i] = a.pos.x; pos[3 * i + 1] = a.pos.y; pos[3 * i + 2] = a.pos.z;
+        quat[4 * i] = a.quat.x; quat[4 * i + 1] = a.quat.y; quat[4 * i + 2] = a.quat.z; quat[4 * i + 3] = a.quat.w;
+    }
+    for (size_t i = 0; i < w->dyn.size(); ++i) {
+        const Dyn& d = w->dyn[i];
+        vel[3 * i] = d.v.x; vel[3 * i + 1] = d.v.y; vel[3 * i + 2] = d.v.z;
+        angvel[3 * i] = d.w.x; angvel[3 * i + 1] = d.w.y; angvel[3 * i + 2] = d.w.z;
+    }
+}
+
+// bounds of every collider, by collider index: min xyz, max xyz
+void orc_world_get_bounds(const orc_world* w, float* out) {
+    for (auto& e : w->bp) {
+        float* o = out + 6 * e.collider;
+        o[0] = e.bounds.min.x; o[1] = e.bounds.min.y; o[2] = e.bounds.min.z;
+        o[3] = e.bounds.max.x; o[4] = e.bounds.max.y; o[5] = e.bounds.max.z;
+    }
+}
+
+int orc_world_num_pairs(const orc_world* w) { return (int)w->pairs.size(); }
+// pairs of the last step as collider indices (g0 = collider of entity0)
+void orc_world_get_pairs(const orc_world* w, int32_t* out) {
+    for (size_t i = 0; i < w->pairs.size(); ++i) { out[2 * i] = w->pairs[i].g0; out[2 * i + 1] = w->pairs[i].g1; }
+}
+
+int orc_world_num_manifolds(const orc_world* w) { return (int)w->ccs.size(); }
+// per manifold in solve order: keys[5]; ints[4] = b0, b1, numPoints, isSoft;
+// floats[4 + 4*6 + 4*7] = normal(3), friction, raw points 4 x (p0, p1), rows 4 x (r0l, r1l, tv)
+void orc_world_get_manifolds(const orc_world* w, int32_t* keys, int32_t* ints, float* f) {
+    for (size_t i = 0; i < w->ccs.size(); ++i) {
+        const CC& c = w->ccs[i];
+        for (int k = 0; k < 5; ++k) keys[5 * i + k] = c.key[k];
+        ints[4 * i] = c.b0; ints[4 * i + 1] = c.b1; ints[4 * i + 2] = c.numPoints; ints[4 * i + 3] = c.isSoft;
+        float* o = f + 56 * i;
+        o[0] = c.n.x; o[1] = c.n.y; o[2] = c.n.z; o[3] = c.friction;
+        for (int k = 0; k < 4; ++k) {
+            const ContactPoint& p = c.raw.points[k];
+            float* q = o + 4 + 6 * k;
+            bool on = k < c.numPoints;
+            q[0] = on ? p.p0.x : 0; q[1] = on ? p.p0.y : 0; q[2] = on ? p.p0.z : 0;
+            q[3] = on ? p.p1.x : 0; q[4] = on ? p.p1.y : 0; q[5] = on ? p.p1.z : 0;
+            const PointC& r = c.pts[k];
+            float* s = o + 28 + 7 * k;
+            s[0] = on ? r.r0.x : 0; s[1] = on ? r.r0.y : 0; s[2] = on ? r.r0.z : 0;
+            s[3] = on ? r.r1.x : 0; s[4] = on ? r.r1.y : 0; s[5] = on ? r.r1.z : 0;
+            s[6] = on ? r.tv : 0;
+        }
+    }
+}
+
+
+// Single-manifold and single-body entry points with the same layouts as the reference
+// harness (oracle/ref_harness.cpp), used to pin this restatement against the reference.
+int orc_contact_solve(int b0, int b1, int numPoints, const float* in_f, const float* masses,
+                      const float* vel, const float* comp, float h, int iters, int relax, float* out) {
+    orc_world w;
+    w.dyn.resize(2);
+    for (int k = 0; k < 2; ++k) {
+        w.dyn[k].invMass = masses[10 * k];
+        w.dyn[k].v = v3(comp[6 * k], comp[6 * k + 1], comp[6 * k + 2]);
+        w.dyn[k].w = v3(comp[6 * k + 3], comp[6 * k + 4], comp[6 * k + 5]);
+        w.dyn[k].kinematic = false;
+    }
+    CC c{};
+    c.d0 = 0; c.d1 = 1;
+    c.b0 = b0 >= 0 ? 0 : -1; c.b1 = b1 >= 0 ? 1 : -1;
+    c.n = v3(in_f[0], in_f[1], in_f[2]); c.friction = in_f[3]; c.isSoft = in_f[4] != 0.f;
+    c.freq = in_f[5]; c.damp = in_f[6]; c.numPoints = numPoints;
+    for (int k = 0; k < 4; ++k) {
+        const float* p = in_f + 7 + 17 * k;
+        PointC& pc = c.pts[k];
+        pc.r0xn = v3(p[0], p[1], p[2]); pc.r1xn = v3(p[3], p[4], p[5]); pc.t = v3(p[6], p[7], p[8]);
+        pc.r0xt = v3(p[9], p[10], p[11]); pc.r1xt = v3(p[12], p[13], p[14]);
+        pc.c = p[15]; pc.tv = p[16];
+    }
+    MassData md[2];
+    VelocityData vd[2];
+    for (int k = 0; k < 2; ++k) {
+        md[k].invMass = masses[10 * k];
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) md[k].invI[i][j] = masses[10 * k + 1 + 3 * i + j];
+        vd[k].v = v3(vel[6 * k], vel[6 * k + 1], vel[6 * k + 2]);
+        vd[k].w = v3(vel[6 * k + 3], vel[6 * k + 4], vel[6 * k + 5]);
+    }
+    preSolve(c, md);
+    for (int i = 0; i < iters; ++i) solveContact(&w, c, vd, true, h);
+    for (int i = 0; i < relax; ++i) solveContact(&w, c, vd, false, 0.f);
+    for (int k = 0; k < 2; ++k) {
+        out[6 * k] = vd[k].v.x; out[6 * k + 1] = vd[k].v.y; out[6 * k + 2] = vd[k].v.z;
+        out[6 * k + 3] = vd[k].w.x; out[6 * k + 4] = vd[k].w.y; out[6 * k + 5] = vd[k].w.z;
+    }
+    for (int k = 0; k < 4; ++k) { out[12 + 2 * k] = c.pts[k].lN; out[13 + 2 * k] = c.pts[k].lT; }
+    return 0;
+}
+
+int orc_integrate_velocity(const float* quat, const float* v, const float* w, const float* invI, float h, float g, float* out) {
+    Dyn d;
+    d.v = v3(v[0], v[1], v[2]); d.w = v3(w[0], w[1], w[2]); d.invMass = 1.f;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) d.invI[i][j] = invI[3 * i + j];
+    MassData md;
+    integrateVelocity(d, {quat[0], quat[1], quat[2], quat[3]}, h, g, md);
+    out[0] = d.v.x; out[1] = d.v.y; out[2] = d.v.z; out[3] = d.w.x; out[4] = d.w.y; out[5] = d.w.z;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) out[6 + 3 * i + j] = md.invI[i][j];
+    return 0;
+}
+
+int orc_integrate_position(const float* pos, const float* quat, const float* v, const float* w,
+                           const float* pv, const float* pw, int count, const float* com, float h, float* out) {
+    V3 p = v3(pos[0], pos[1], pos[2]);
+    Q q = {quat[0], quat[1], quat[2], quat[3]};
+    VelocityData vd{v3(v[0], v[1], v[2]), v3(w[0], w[1], w[2])};
+    PseudoVelocityData pd{v3(pv[0], pv[1], pv[2]), v3(pw[0], pw[1], pw[2]), count};
+    integratePosition(p, q, vd, pd, v3(com[0], com[1], com[2]), h);
+    out[0] = p.x; out[1] = p.y; out[2] = p.z; out[3] = q.x; out[4] = q.y; out[5] = q.z; out[6] = q.w;
+    return 0;
+}
+
+// single narrowphase call on primitive shapes (for pinning against the reference).
+// out: numPoints, normal(3), 4 x (p0, p1); returns the collision() bool.
+int orc_collision(int type0, const float* prm0, const float* pos0, const float* q0,
+                  int type1, const float* prm1, const float* pos1, const float* q1, float* out) {
+    Geom g0{type0, {prm0[0], prm0[1], prm0[2], prm0[3]}};
+    Geom g1{type1, {prm1[0], prm1[1], prm1[2], prm1[3]}};
+    std::vector<Manifold> buf;
+    bool hit = collision(v3(pos0[0], pos0[1], pos0[2]), {q0[0], q0[1], q0[2], q0[3]}, g0,
+                         v3(pos1[0], pos1[1], pos1[2]), {q1[0], q1[1], q1[2], q1[3]}, g1, buf);
+    std::memset(out, 0, sizeof(float) * 28);
+    const Manifold& m = buf.back();
+    out[0] = (float)m.numPoints;
+    out[1] = m.normal.x; out[2] = m.normal.y; out[3] = m.normal.z;
+    for (int k = 0; k < m.numPoints && k < 4; ++k) {
+        float* o = out + 4 + 6 * k;
+        o[0] = m.points[k].p0.x; o[1] = m.points[k].p0.y; o[2] = m.points[k].p0.z;
+        o[3] = m.points[k].p1.x; o[4] = m.points[k].p1.y; o[5] = m.points[k].p1.z;
+    }
+    return hit ? 1 : 0;
+}
+
+int orc_bounds(int type, const float* prm, const float* pos, const float* q, float* out) {
+    Geom g{type, {prm[0], prm[1], prm[2], prm[3]}};
+    Bounds b = getBounds(v3(pos[0], pos[1], pos[2]), {q[0], q[1], q[2], q[3]}, g);
+    out[0] = b.min.x; out[1] = b.min.y; out[2] = b.min.z;
+    out[3] = b.max.x; out[4] = b.max.y; out[5] = b.max.z;
+    return 0;
+}
+
+// out: sqrDist, t, q(3)
+int orc_sqr_dist_segment_aabb(const float* p, const float* dir, float mn, float mx, const float* he, float* out) {
+    V3 qq = v3s(0);
+    float t = 0;
+    float sq = sqrDistSegmentAABB(v3(p[0], p[1], p[2]), v3(dir[0], dir[1], dir[2]), mn, mx, v3(he[0], he[1], he[2]), t, qq);
+    out[0] = sq; out[1] = t; out[2] = qq.x; out[3] = qq.y; out[4] = qq.z;
+    return 0;
+}
+
+}  // extern "C"

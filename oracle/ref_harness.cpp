@@ -1,0 +1,182 @@
+// Reference harness: a C ABI over functions of the reference itself, compiled from the
+// reference's own sources where they lie (see oracle/Makefile), so that the tests can
+// pin the CPU oracle against the real reference code. TEST INFRASTRUCTURE ONLY.
+//
+// Calls into the reference:
+//   physecs::getBounds            src/BoundsUtil.cpp:23-33
+//   physecs::collision            src/Collision.cpp:895-1016
+//   sqrDistSegmentAABB            src/GeomUtil.cpp:455-467
+//   ContactConstraints::preSolve / solve   src/ContactConstraints.cpp:4-124
+//   physecs::setMassProps         src/MassUtil.cpp:185-192
+//   solve33                       src/MathUtil.h:10-21
+// The two integrate functions below are GLM-written restatements of src/Physecs.cpp
+// (that file does not compile here): they pin the oracle's GLM-free arithmetic against
+// GLM's own operator order.
+#include <cstring>
+#include <vector>
+#include <glm/glm.hpp>
+#include <glm/gtx/quaternion.hpp>
+#include <glm/gtx/matrix_cross_product.hpp>
+#include "BoundsUtil.h"
+#include "Collision.h"
+#include "ContactConstraints.h"
+#include "ContactManifold.h"
+#include "GeomUtil.h"
+#include "MassUtil.h"
+#include "MathUtil.h"
+#include "SolverData.h"
+
+using namespace physecs;
+
+static Geometry makeGeom(int type, const float* p) {
+    Geometry g;
+    std::memset(&g, 0, sizeof(g));
+    g.type = (GeometryType)type;
+    switch (type) {
+        case SPHERE: g.sphere.radius = p[0]; break;
+        case CAPSULE: g.capsule.halfHeight = p[0]; g.capsule.radius = p[1]; break;
+        case BOX: g.box.halfExtents = glm::vec3(p[0], p[1], p[2]); break;
+        default: break;
+    }
+    return g;
+}
+static glm::vec3 V(const float* p) { return glm::vec3(p[0], p[1], p[2]); }
+static glm::quat Qt(const float* q) { return glm::quat(q[3], q[0], q[1], q[2]); }
+
+extern "C" {
+
+int ref_bounds(int type, const float* prm, const float* pos, const float* q, float* out) {
+    Geometry g = makeGeom(type, prm);
+    Bounds b = getBounds(V(pos), Qt(q), g);
+    out[0] = b.min.x; out[1] = b.min.y; out[2] = b.min.z;
+    out[3] = b.max.x; out[4] = b.max.y; out[5] = b.max.z;
+    return 0;
+}
+
+// out: numPoints, normal(3), 4 x (p0, p1); returns collision()'s bool
+int ref_collision(int type0, const float* prm0, const float* pos0, const float* q0,
+                  int type1, const float* prm1, const float* pos1, const float* q1, float* out) {
+    Geometry g0 = makeGeom(type0, prm0), g1 = makeGeom(type1, prm1);
+    std::vector<ContactManifold> res;
+    bool hit = collision(V(pos0), Qt(q0), g0, V(pos1), Qt(q1), g1, res);
+    std::memset(out, 0, sizeof(float) * 28);
+    const ContactManifold& m = res.back();
+    out[0] = (float)m.numPoints;
+    if (m.numPoints) { out[1] = m.normal.x; out[2] = m.normal.y; out[3] = m.normal.z; }
+    for (int k = 0; k < m.numPoints && k < 4; ++k) {
+        float* o = out + 4 + 6 * k;
+        o[0] = m.points[k].position0.x; o[1] = m.points[k].position0.y; o[2] = m.points[k].position0.z;
+        o[3] = m.points[k].position1.x; o[4] = m.points[k].position1.y; o[5] = m.points[k].position1.z;
+    }
+    return hit ? 1 : 0;
+}
+
+int ref_sqr_dist_segment_aabb(const float* p, const float* dir, float mn, float mx, const float* he, float* out) {
+    float t = 0;
+    glm::vec3 q(0);
+    float sq = sqrDistSegmentAABB(V(p), V(dir), mn, mx, V(he), t, q);
+    out[0] = sq; out[1] = t; out[2] = q.x; out[3] = q.y; out[4] = q.z;
+    return 0;
+}
+
+// mass properties of a one-collider body: out invMass, com(3), invInertia(9, column-major)
+int ref_mass_props(int type, const float* prm, const float* lpos, const float* lq, float mass, float* out) {
+    RigidBodyCollisionComponent col;
+    Collider c;
+    std::memset(&c, 0, sizeof(c));
+    c.position = V(lpos);
+    c.orientation = Qt(lq);
+    c.geometry = makeGeom(type, prm);
+    c.enableSimulation = true;
+    col.colliders.push_back(c);
+    RigidBodyDynamicComponent d{};
+    setMassProps(d, col, mass);
+    out[0] = d.invMass;
+    out[1] = d.com.x; out[2] = d.com.y; out[3] = d.com.z;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) out[4 + 3 * i + j] = d.invInertiaTensor[i][j];
+    return 0;
+}
+
+// One ContactConstraints (up to 4 points) run through the reference's preSolve and
+// solve: `iters` biased passes with timeStep h, then `relax` unbiased passes.
+// in_f layout: n(3) friction isSoft freq damping | per point x4: r0xn r1xn t r0xt r1xt (15) c tv
+// masses: invMass0, invI0(9), invMass1, invI1(9); vel: v0 w0 v1 w1 (12, solver velocities);
+// comp: v0 w0 v1 w1 (12, component velocities). out: vel(12) then per point lN lT (8)
+int ref_contact_solve(int b0, int b1, int numPoints, const float* in_f, const float* masses,
+                      const float* vel, const float* comp, float h, int iters, int relax, float* out) {
+    TransformComponent tr0{}, tr1{};
+    RigidBodyDynamicComponent d0{}, d1{};
+    d0.invMass = masses[0]; d1.invMass = masses[10];
+    d0.velocity = V(comp); d0.angularVelocity = V(comp + 3);
+    d1.velocity = V(comp + 6); d1.angularVelocity = V(comp + 9);
+    int ib0 = b0 >= 0 ? 0 : -1, ib1 = b1 >= 0 ? 1 : -1;
+    ContactConstraints cc{tr0, tr1, &d0, &d1, ib0, ib1, V(in_f), in_f[3], in_f[4] != 0.f, in_f[5], in_f[6], numPoints, {}};
+    for (int k = 0; k < 4; ++k) {
+        const float* p = in_f + 7 + 17 * k;
+        auto& pc = cc.contactPointConstraints[k];
+        pc.r0 = glm::vec3(0); pc.r1 = glm::vec3(0);
+        pc.r0xn = V(p); pc.r1xn = V(p + 3); pc.t = V(p + 6); pc.r0xt = V(p + 9); pc.r1xt = V(p + 12);
+        pc.c = p[15]; pc.targetVelocity = p[16];
+        pc.totalLambdaN = 0; pc.totalLambdaT = 0;
+    }
+    MassData md[2];
+    md[0].invMass = masses[0]; md[1].invMass = masses[10];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) { md[0].invInertiaTensor[i][j] = masses[1 + 3 * i + j]; md[1].invInertiaTensor[i][j] = masses[11 + 3 * i + j]; }
+    VelocityData vd[2];
+    vd[0].velocity = V(vel); vd[0].angularVelocity = V(vel + 3);
+    vd[1].velocity = V(vel + 6); vd[1].angularVelocity = V(vel + 9);
+    cc.preSolve(md);
+    for (int i = 0; i < iters; ++i) cc.solve(vd, true, h);
+    for (int i = 0; i < relax; ++i) cc.solve(vd, false);
+    float* o = out;
+    o[0] = vd[0].velocity.x; o[1] = vd[0].velocity.y; o[2] = vd[0].velocity.z;
+    o[3] = vd[0].angularVelocity.x; o[4] = vd[0].angularVelocity.y; o[5] = vd[0].angularVelocity.z;
+    o[6] = vd[1].velocity.x; o[7] = vd[1].velocity.y; o[8] = vd[1].velocity.z;
+    o[9] = vd[1].angularVelocity.x; o[10] = vd[1].angularVelocity.y; o[11] = vd[1].angularVelocity.z;
+    for (int k = 0; k < 4; ++k) { o[12 + 2 * k] = cc.contactPointConstraints[k].totalLambdaN; o[13 + 2 * k] = cc.contactPointConstraints[k].totalLambdaT; }
+    return 0;
+}
+
+// GLM restatement of src/Physecs.cpp:450-467 for one body.
+// io: pos(3) quat(4 xyzw) v(3) w(3); invI(9); out: v(3) w(3) worldInvI(9)
+int ref_integrate_velocity(const float* quat, const float* v, const float* w, const float* invI, float h, float g, float* out) {
+    glm::quat q = Qt(quat);
+    glm::mat3 invInertia;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) invInertia[i][j] = invI[3 * i + j];
+    glm::vec3 vel = V(v), ang = V(w);
+    glm::mat3 rot = glm::toMat3(q);
+    glm::mat3 invRot = glm::transpose(rot);
+    vel += h * glm::vec3(0, -g, 0);
+    glm::vec3 omegaLocal = invRot * ang;
+    glm::mat3x3 I = glm::inverse(invInertia);
+    glm::vec3 f = h * glm::cross(omegaLocal, I * omegaLocal);
+    glm::mat3x3 J = I + h * (glm::matrixCross3(omegaLocal) * I - glm::matrixCross3(I * omegaLocal));
+    omegaLocal = omegaLocal - solve33(J, f);
+    ang = rot * omegaLocal;
+    glm::mat3 wI = rot * invInertia * invRot;
+    out[0] = vel.x; out[1] = vel.y; out[2] = vel.z;
+    out[3] = ang.x; out[4] = ang.y; out[5] = ang.z;
+    for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) out[6 + 3 * i + j] = wI[i][j];
+    return 0;
+}
+
+// GLM restatement of src/Physecs.cpp:501-510 for one body.
+// in: pos(3) quat(4) v(3) w(3) pv(3) pw(3) count com(3); out: pos(3) quat(4)
+int ref_integrate_position(const float* pos, const float* quat, const float* v, const float* w,
+                           const float* pv, const float* pw, int count, const float* com, float h, float* out) {
+    glm::vec3 position = V(pos);
+    glm::quat orientation = Qt(quat);
+    float s = count ? 1.f / count : 1.f;
+    position += h * V(v) + s * V(pv);
+    glm::vec3 prevComWorld = orientation * V(com);
+    orientation += glm::quat(0, 0.5f * (h * V(w) + s * V(pw))) * orientation;
+    orientation = glm::normalize(orientation);
+    position += prevComWorld - orientation * V(com);
+    out[0] = position.x; out[1] = position.y; out[2] = position.z;
+    out[3] = orientation.x; out[4] = orientation.y; out[5] = orientation.z; out[6] = orientation.w;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+"""Pins the CPU oracle against the reference's own compiled functions (oracle/_ref),
+bit for bit, on seeded random inputs. Skipped where the reference harness is not built
+(it needs the reference sources, so it is built only where they are present)."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+pytestmark = pytest.mark.skipif(not ol.ref_available(), reason="oracle/_ref not built here")
+
+
+def rq(rng):
+    q = rng.normal(size=4)
+    return (q / np.linalg.norm(q)).astype(np.float32)
+
+
+def shape_params(t, rng):
+    if t == 0:
+        return [rng.uniform(0.2, 0.5)]
+    if t == 1:
+        return [rng.uniform(0.2, 0.6), rng.uniform(0.15, 0.3)]
+    return list(rng.uniform(0.2, 0.6, 3))
+
+
+def bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_collision_bitexact(seed):
+    """src/Collision.cpp:895-1016 for all 9 ordered primitive pairs."""
+    orc, ref = ol.orc(), ol.ref()
+    rng = np.random.default_rng(seed)
+    hits = 0
+    for _ in range(40000):
+        t0, t1 = int(rng.integers(0, 3)), int(rng.integers(0, 3))
+        p0, p1 = shape_params(t0, rng), shape_params(t1, rng)
+        pos0 = rng.uniform(-1, 1, 3)
+        pos1 = pos0 + rng.uniform(-1.2, 1.2, 3)
+        q0, q1 = rq(rng), rq(rng)
+        mode = rng.integers(0, 5)
+        if mode == 1:
+            q0 = np.array([0, 0, 0, 1], np.float32)
+            q1 = q0.copy()
+        elif mode == 2:
+            q1 = q0.copy()
+        elif mode == 3:   # stacked
+            pos1 = pos0 + np.array([rng.uniform(-.05, .05), rng.uniform(.3, 1.2), rng.uniform(-.05, .05)])
+        elif mode == 4:   # deep
+            pos1 = pos0 + rng.uniform(-.1, .1, 3)
+        h1, o1 = ol.collision(orc, "orc", t0, p0, pos0, q0, t1, p1, pos1, q1)
+        h2, o2 = ol.collision(ref, "ref", t0, p0, pos0, q0, t1, p1, pos1, q1)
+        assert h1 == h2 and o1[0] == o2[0], (t0, t1, mode)
+        if h2 and o2[0] > 0:
+            hits += 1
+            assert np.array_equal(bits(o1), bits(o2)), (t0, t1, mode, o1, o2)
+    assert hits > 10000
+
+
+def test_bounds_bitexact():
+    """src/BoundsUtil.cpp:35-59."""
+    orc, ref = ol.orc(), ol.ref()
+    rng = np.random.default_rng(3)
+    for _ in range(20000):
+        t = int(rng.integers(0, 3))
+        p = np.zeros(4, np.float32)
+        p[:3] = shape_params(t, rng) + [0] * (3 - len(shape_params(t, rng)))
+        pos = rng.uniform(-50, 50, 3).astype(np.float32)
+        q = rq(rng)
+        a = np.zeros(6, np.float32)
+        b = np.zeros(6, np.float32)
+        orc.orc_bounds(t, ol.fptr(p), ol.fptr(pos), ol.fptr(q), ol.fptr(a))
+        ref.ref_bounds(t, ol.fptr(p), ol.fptr(pos), ol.fptr(q), ol.fptr(b))
+        assert np.array_equal(bits(a), bits(b))
+
+
+def test_sqr_dist_segment_aabb_bitexact():
+    """src/GeomUtil.cpp:129-467, including the region-5 sum kept from GeomUtil.cpp:273,326."""
+    orc, ref = ol.orc(), ol.ref()
+    rng = np.random.default_rng(4)
+    for _ in range(50000):
+        p = rng.uniform(-2, 2, 3).astype(np.float32)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        d = d.astype(np.float32)
+        k = rng.integers(0, 4)
+        if k == 1:
+            d[rng.integers(0, 3)] = 0
+        elif k == 2:
+            d[:] = 0
+            d[rng.integers(0, 3)] = rng.choice([-1, 1])
+        he = rng.uniform(0.2, 0.6, 3).astype(np.float32)
+        hh = np.float32(rng.uniform(0.2, 0.6))
+        a = np.zeros(5, np.float32)
+        b = np.zeros(5, np.float32)
+        orc.orc_sqr_dist_segment_aabb(ol.fptr(p), ol.fptr(d), -hh, hh, ol.fptr(he), ol.fptr(a))
+        ref.ref_sqr_dist_segment_aabb(ol.fptr(p), ol.fptr(d), -hh, hh, ol.fptr(he), ol.fptr(b))
+        assert np.array_equal(bits(a), bits(b)), (p, d, he, a, b)
+
+
+def random_contact_inputs(rng, soft, static0):
+    inf = np.zeros(7 + 17 * 4, np.float32)
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    inf[0:3] = n
+    inf[3] = rng.uniform(0.2, 1.0)
+    inf[4] = 1.0 if soft else 0.0
+    inf[5] = rng.uniform(1, 30) if soft else 0
+    inf[6] = rng.uniform(0.1, 1) if soft else 0
+    for k in range(4):
+        p = inf[7 + 17 * k: 7 + 17 * (k + 1)]
+        p[:15] = rng.normal(size=15) * 0.5
+        p[15] = rng.uniform(-0.05, 0.01)
+        p[16] = rng.normal() * 0.1
+    masses = np.zeros(20, np.float32)
+    for b in range(2):
+        masses[10 * b] = 0.0 if (b == 0 and static0) else rng.uniform(0.001, 2)
+        A = rng.normal(size=(3, 3)).astype(np.float32)
+        masses[10 * b + 1: 10 * b + 10] = (A @ A.T * 0.3).ravel()
+    vel = rng.normal(size=12).astype(np.float32)
+    comp = rng.normal(size=12).astype(np.float32)
+    return inf, masses, vel, comp
+
+
+@pytest.mark.parametrize("soft", [False, True])
+def test_contact_solver_bitexact(soft):
+    """src/ContactConstraints.cpp:4-124: preSolve, biased iterations, relaxation."""
+    orc, ref = ol.orc(), ol.ref()
+    rng = np.random.default_rng(5 + soft)
+    for it in range(5000):
+        static0 = it % 3 == 0
+        inf, masses, vel, comp = random_contact_inputs(rng, soft, static0)
+        npts = int(rng.integers(1, 5))
+        b0 = -1 if static0 else 0
+        a = np.zeros(20, np.float32)
+        b = np.zeros(20, np.float32)
+        h = np.float32(1 / 240)
+        for lib, out, pre in ((orc, a, "orc"), (ref, b, "ref")):
+            getattr(lib, pre + "_contact_solve")(b0, 1, npts, ol.fptr(inf), ol.fptr(masses), ol.fptr(vel),
+                                                 ol.fptr(comp), h, 2, 0 if soft else 1, ol.fptr(out))
+        assert np.array_equal(bits(a), bits(b)), (it, a, b)
+
+
+def test_integrators_bitexact():
+    """src/Physecs.cpp:450-467 (gyroscopic velocity update) and :501-510 (positions),
+    against a GLM build of the same lines."""
+    orc, ref = ol.orc(), ol.ref()
+    rng = np.random.default_rng(6)
+    h = np.float32(1 / 240)
+    for _ in range(5000):
+        q = rq(rng)
+        v = rng.normal(size=3).astype(np.float32)
+        w = (rng.normal(size=3) * 3).astype(np.float32)
+        A = rng.normal(size=(3, 3))
+        invI = (np.linalg.inv(A @ A.T + np.eye(3))).astype(np.float32).ravel()
+        a = np.zeros(15, np.float32)
+        b = np.zeros(15, np.float32)
+        orc.orc_integrate_velocity(ol.fptr(q), ol.fptr(v), ol.fptr(w), ol.fptr(invI), h, np.float32(9.81), ol.fptr(a))
+        ref.ref_integrate_velocity(ol.fptr(q), ol.fptr(v), ol.fptr(w), ol.fptr(invI), h, np.float32(9.81), ol.fptr(b))
+        assert np.array_equal(bits(a), bits(b))
+        pos = rng.uniform(-5, 5, 3).astype(np.float32)
+        pv = rng.normal(size=3).astype(np.float32) * 0.01
+        pw = rng.normal(size=3).astype(np.float32) * 0.01
+        com = rng.normal(size=3).astype(np.float32) * 0.1
+        cnt = int(rng.integers(0, 3))
+        a = np.zeros(7, np.float32)
+        b = np.zeros(7, np.float32)
+        orc.orc_integrate_position(ol.fptr(pos), ol.fptr(q), ol.fptr(v), ol.fptr(w), ol.fptr(pv), ol.fptr(pw), cnt,
+                                   ol.fptr(com), h, ol.fptr(a))
+        ref.ref_integrate_position(ol.fptr(pos), ol.fptr(q), ol.fptr(v), ol.fptr(w), ol.fptr(pv), ol.fptr(pw), cnt,
+                                   ol.fptr(com), h, ol.fptr(b))
+        assert np.array_equal(bits(a), bits(b))
