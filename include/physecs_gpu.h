@@ -1,0 +1,147 @@
+/*
+ * physecs_gpu.h: C ABI of the MI355X (gfx950) rigid-body step.
+ *
+ * This is the drop-in boundary under physecs::Scene::simulate(float): host code gathers
+ * the EnTT components (TransformComponent, RigidBodyDynamicComponent,
+ * RigidBodyCollisionComponent) into the scene descriptor below, creates a world, and
+ * steps it; the world keeps all state resident in HBM between steps.
+ *
+ * Conventions: every function returns an int status (PGX_OK = 0, negative on error;
+ * pgx_last_error() describes the last error of the calling thread). No C++ exception
+ * crosses this boundary. Host pointers are borrowed for the duration of the call only.
+ * Device memory is owned by the world handle. One HIP stream per world; a world is not
+ * thread-safe; distinct worlds are independent.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   pgx_world_create        Scene::Scene + the registry signal hooks that register
+ *                           bodies (include/Physecs/Physecs.h:199, src/Physecs.cpp:25-37,92-98)
+ *   pgx_world_set_params    Scene::setNumSubSteps / setNumIterations / setGravity
+ *                           (include/Physecs/Physecs.h:201-203)
+ *   pgx_world_step          Scene::simulate(float timeStep) (include/Physecs/Physecs.h:204,
+ *                           src/Physecs.cpp:112-561)
+ *   pgx_world_set_can_collide  Scene::setCanCollide (include/Physecs/Physecs.h:223)
+ *   pgx_world_read_state / pgx_world_write_state
+ *                           the in-place component writes that simulate performs
+ *                           (Transform position/orientation, Dynamic velocity/angularVelocity)
+ */
+#ifndef PHYSECS_GPU_H
+#define PHYSECS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PGX_OK 0
+#define PGX_ERR_INVALID -1      /* bad argument */
+#define PGX_ERR_HIP -2          /* HIP runtime error (no device, out of memory, launch) */
+#define PGX_ERR_CAPACITY -3     /* a per-step buffer overflowed; raise the capacity */
+#define PGX_ERR_UNSUPPORTED -4  /* shape or feature not supported on the device path */
+
+/* Geometry types, include/Physecs/Colliders.h:9 */
+#define PGX_SPHERE 0
+#define PGX_CAPSULE 1
+#define PGX_BOX 2
+#define PGX_CONVEX_MESH 3
+#define PGX_TRIANGLE_MESH 4
+
+typedef struct pgx_world pgx_world;
+
+typedef struct pgx_config {
+    int num_substeps;       /* Scene::numSubSteps, default 8 (Physecs.h:156) */
+    int num_iterations;     /* Scene::numIterations, default 2 (Physecs.h:157) */
+    float gravity;          /* Scene::g, default 9.81 along -y (Physecs.h:159) */
+    int device;             /* HIP device ordinal */
+    int pair_capacity;      /* candidate pairs per step; 0 = 12 x colliders */
+    int block_size;         /* solver block size (autotuned; 0 = default 256) */
+} pgx_config;
+
+/* Scene descriptor. Actors are entities with a TransformComponent; the dynamic storage
+ * holds RigidBodyDynamicComponent entries in storage order (its index is the body index
+ * `b` the reference's solver uses); colliders are listed in broadphase-entry creation
+ * order. Arrays are tightly packed float32 / int32. Quaternions are x,y,z,w. Inverse
+ * inertia tensors are 3x3 column-major (glm::mat3). Material = friction, restitution,
+ * damping. col_params: sphere {r}, capsule {halfHeight, r}, box {hx, hy, hz}.
+ * actor_world may be NULL (one world); otherwise worlds never interact. */
+typedef struct pgx_scene_desc {
+    int n_actors;
+    const uint32_t* entity;       /* entt::entity value, n_actors */
+    const float* pos;             /* 3 x n_actors */
+    const float* quat;            /* 4 x n_actors */
+    const int32_t* dyn_of_actor;  /* dynamic-storage index or -1 */
+    int n_dyn;
+    const int32_t* actor_of_dyn;
+    const uint8_t* kinematic;
+    const float* vel;             /* 3 x n_dyn */
+    const float* angvel;          /* 3 x n_dyn */
+    const float* inv_mass;        /* n_dyn */
+    const float* com;             /* 3 x n_dyn, body-local centre of mass */
+    const float* inv_inertia;     /* 9 x n_dyn, body-local */
+    int n_colliders;
+    const int32_t* col_actor;
+    const int32_t* col_index;     /* index within the actor's collider list */
+    const float* col_pos;         /* 3 x n_colliders, actor-local */
+    const float* col_quat;        /* 4 x n_colliders */
+    const int32_t* col_type;
+    const float* col_params;      /* 4 x n_colliders */
+    const float* col_material;    /* 3 x n_colliders */
+    const uint8_t* col_trigger;
+    const uint8_t* col_enable;    /* Collider::enableSimulation */
+    const int32_t* col_data;
+    int n_worlds;
+    const int32_t* actor_world;   /* world of each actor, or NULL */
+} pgx_scene_desc;
+
+typedef struct pgx_counts {
+    int pairs;        /* broadphase candidate pairs of the last step */
+    int manifolds;    /* contact manifolds with at least one point */
+    int points;       /* contact points */
+    int colors;       /* graph colors used by the contact solve */
+    int joint_rows;   /* joint constraint rows */
+    int coloring_rounds;
+} pgx_counts;
+
+/* Library / device */
+const char* pgx_last_error(void);
+int pgx_device_count(int* count);
+
+/* World lifetime. The descriptor is copied to the device; creation-time bounds carry no
+ * margin (src/Physecs.cpp:32). cfg may be NULL for the defaults. */
+int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* scene, pgx_world** out);
+int pgx_world_destroy(pgx_world* w);
+int pgx_world_set_params(pgx_world* w, int num_substeps, int num_iterations, float gravity);
+int pgx_world_set_can_collide(pgx_world* w, uint32_t entity0, uint32_t entity1, int can_collide);
+
+/* One Scene::simulate(time_step) on the device. Asynchronous; pgx_world_sync waits. */
+int pgx_world_step(pgx_world* w, float time_step);
+int pgx_world_sync(pgx_world* w);
+
+/* Host <-> device body state, in the descriptor's actor and dynamic-storage order.
+ * pos: 3 x n_actors, quat: 4 x n_actors, vel/angvel: 3 x n_dyn. Any pointer may be NULL. */
+int pgx_world_read_state(pgx_world* w, float* pos, float* quat, float* vel, float* angvel);
+int pgx_world_write_state(pgx_world* w, const float* pos, const float* quat, const float* vel, const float* angvel);
+
+/* Introspection of the last step (for tests and tools). */
+int pgx_world_get_counts(pgx_world* w, pgx_counts* out);
+/* Bounds of every collider, descriptor order: min xyz, max xyz. */
+int pgx_world_get_bounds(pgx_world* w, float* out);
+/* Candidate pairs as descriptor collider indices (collider of the smaller entity first),
+ * in the device's canonical order. Writes at most cap pairs; returns count in *n. */
+int pgx_world_get_pairs(pgx_world* w, int32_t* out, int cap, int* n);
+/* Manifolds in solve order (graph color order). keys: e0, c0, e1, c1, triangleIndex;
+ * ints: b0, b1, numPoints, color; floats (56): normal(3), friction, 4 x (p0, p1) world
+ * points, 4 x (r0 local, r1 local, targetVelocity). Any pointer may be NULL. */
+int pgx_world_get_manifolds(pgx_world* w, int32_t* keys, int32_t* ints, float* floats, int cap, int* n);
+
+/* Timing of the phases of the last step on the device, in microseconds (HIP events on
+ * the world's stream; enabled by pgx_world_enable_timing). */
+int pgx_world_enable_timing(pgx_world* w, int enable);
+int pgx_world_get_timing(pgx_world* w, float* broadphase_us, float* narrowphase_us, float* coloring_us,
+                         float* solver_us, float* total_us);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHYSECS_GPU_H */

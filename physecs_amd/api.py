@@ -1,0 +1,190 @@
+"""ctypes binding of libphysecs_gpu.so (include/physecs_gpu.h).
+
+The library is built in-tree (physecs_amd/lib/libphysecs_gpu.so) by physecs_amd.build.
+There is no CPU fallback: loading fails loudly when the library is missing, and creating
+a world fails with the HIP error when no GPU is present.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libphysecs_gpu.so")
+
+F = ctypes.POINTER(ctypes.c_float)
+I32 = ctypes.POINTER(ctypes.c_int32)
+U32 = ctypes.POINTER(ctypes.c_uint32)
+U8 = ctypes.POINTER(ctypes.c_uint8)
+
+PGX_OK = 0
+ERRORS = {-1: "PGX_ERR_INVALID", -2: "PGX_ERR_HIP", -3: "PGX_ERR_CAPACITY", -4: "PGX_ERR_UNSUPPORTED"}
+
+# every symbol declared in include/physecs_gpu.h
+EXPORTS = [
+    "pgx_last_error", "pgx_device_count", "pgx_world_create", "pgx_world_destroy", "pgx_world_set_params",
+    "pgx_world_set_can_collide", "pgx_world_step", "pgx_world_sync", "pgx_world_read_state",
+    "pgx_world_write_state", "pgx_world_get_counts", "pgx_world_get_bounds", "pgx_world_get_pairs",
+    "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing",
+]
+
+
+class PgxConfig(ctypes.Structure):
+    _fields_ = [("num_substeps", ctypes.c_int), ("num_iterations", ctypes.c_int), ("gravity", ctypes.c_float),
+                ("device", ctypes.c_int), ("pair_capacity", ctypes.c_int), ("block_size", ctypes.c_int)]
+
+
+class PgxSceneDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_actors", ctypes.c_int), ("entity", U32), ("pos", F), ("quat", F), ("dyn_of_actor", I32),
+        ("n_dyn", ctypes.c_int), ("actor_of_dyn", I32), ("kinematic", U8), ("vel", F), ("angvel", F),
+        ("inv_mass", F), ("com", F), ("inv_inertia", F),
+        ("n_colliders", ctypes.c_int), ("col_actor", I32), ("col_index", I32), ("col_pos", F), ("col_quat", F),
+        ("col_type", I32), ("col_params", F), ("col_material", F), ("col_trigger", U8), ("col_enable", U8),
+        ("col_data", I32), ("n_worlds", ctypes.c_int), ("actor_world", I32),
+    ]
+
+
+class PgxCounts(ctypes.Structure):
+    _fields_ = [("pairs", ctypes.c_int), ("manifolds", ctypes.c_int), ("points", ctypes.c_int),
+                ("colors", ctypes.c_int), ("joint_rows", ctypes.c_int), ("coloring_rounds", ctypes.c_int)]
+
+
+class PgxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m physecs_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.pgx_last_error.restype = ctypes.c_char_p
+        L.pgx_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        L.pgx_world_create.argtypes = [ctypes.POINTER(PgxConfig), ctypes.POINTER(PgxSceneDesc), ctypes.POINTER(vp)]
+        L.pgx_world_destroy.argtypes = [vp]
+        L.pgx_world_set_params.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float]
+        L.pgx_world_set_can_collide.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+        L.pgx_world_step.argtypes = [vp, ctypes.c_float]
+        L.pgx_world_sync.argtypes = [vp]
+        L.pgx_world_read_state.argtypes = [vp, F, F, F, F]
+        L.pgx_world_write_state.argtypes = [vp, F, F, F, F]
+        L.pgx_world_get_counts.argtypes = [vp, ctypes.POINTER(PgxCounts)]
+        L.pgx_world_get_bounds.argtypes = [vp, F]
+        L.pgx_world_get_pairs.argtypes = [vp, I32, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.pgx_world_get_manifolds.argtypes = [vp, I32, I32, F, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.pgx_world_enable_timing.argtypes = [vp, ctypes.c_int]
+        L.pgx_world_get_timing.argtypes = [vp, F, F, F, F, F]
+        for name in EXPORTS:
+            if name != "pgx_last_error":
+                getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != PGX_OK:
+        msg = lib().pgx_last_error().decode(errors="replace")
+        raise PgxError(f"{ERRORS.get(rc, rc)}: {msg}")
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib().pgx_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def _f(a):
+    return None if a is None else a.ctypes.data_as(F)
+
+
+class World:
+    """One pgx_world: a scene resident on the GPU, stepped by Scene::simulate semantics."""
+
+    def __init__(self, scene, substeps=4, iterations=2, gravity=9.81, device=0, pair_capacity=0):
+        self.L = lib()
+        self.scene = scene
+        self._arr = scene.arrays()
+        desc = scene.struct(PgxSceneDesc)
+        if scene.n_worlds <= 1:
+            desc.actor_world = ctypes.cast(None, I32)
+            desc.n_worlds = 1
+        cfg = PgxConfig(substeps, iterations, gravity, device, pair_capacity, 0)
+        h = ctypes.c_void_p()
+        check(self.L.pgx_world_create(ctypes.byref(cfg), ctypes.byref(desc), ctypes.byref(h)))
+        self.h = h
+        check(self.L.pgx_world_set_params(self.h, substeps, iterations, gravity))
+        for e0, e1 in getattr(scene, "non_colliding", []):
+            check(self.L.pgx_world_set_can_collide(self.h, e0, e1, 0))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pgx_world_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, dt):
+        check(self.L.pgx_world_step(self.h, dt))
+
+    def sync(self):
+        check(self.L.pgx_world_sync(self.h))
+
+    def state(self):
+        s = self.scene
+        pos = np.zeros((s.n_actors, 3), np.float32)
+        quat = np.zeros((s.n_actors, 4), np.float32)
+        vel = np.zeros((s.n_dyn, 3), np.float32)
+        ang = np.zeros((s.n_dyn, 3), np.float32)
+        check(self.L.pgx_world_read_state(self.h, _f(pos), _f(quat), _f(vel), _f(ang)))
+        return pos, quat, vel, ang
+
+    def write_state(self, pos=None, quat=None, vel=None, ang=None):
+        c = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
+        pos, quat, vel, ang = c(pos), c(quat), c(vel), c(ang)
+        check(self.L.pgx_world_write_state(self.h, _f(pos), _f(quat), _f(vel), _f(ang)))
+
+    def counts(self):
+        c = PgxCounts()
+        check(self.L.pgx_world_get_counts(self.h, ctypes.byref(c)))
+        return {k: getattr(c, k) for k, _ in PgxCounts._fields_}
+
+    def bounds(self):
+        out = np.zeros((self.scene.n_colliders, 6), np.float32)
+        check(self.L.pgx_world_get_bounds(self.h, _f(out)))
+        return out
+
+    def pairs(self):
+        n = ctypes.c_int(0)
+        check(self.L.pgx_world_get_pairs(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros((n.value, 2), np.int32)
+        if n.value:
+            check(self.L.pgx_world_get_pairs(self.h, out.ctypes.data_as(I32), n.value, ctypes.byref(n)))
+        return out
+
+    def manifolds(self):
+        n = ctypes.c_int(0)
+        check(self.L.pgx_world_get_manifolds(self.h, None, None, None, 0, ctypes.byref(n)))
+        m = n.value
+        keys = np.zeros((m, 5), np.int32)
+        ints = np.zeros((m, 4), np.int32)
+        f = np.zeros((m, 56), np.float32)
+        if m:
+            check(self.L.pgx_world_get_manifolds(self.h, keys.ctypes.data_as(I32), ints.ctypes.data_as(I32),
+                                                 _f(f), m, ctypes.byref(n)))
+        return keys, ints, f
+
+    def enable_timing(self, on=True):
+        check(self.L.pgx_world_enable_timing(self.h, 1 if on else 0))
+
+    def timing(self):
+        v = [ctypes.c_float() for _ in range(5)]
+        check(self.L.pgx_world_get_timing(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["broadphase_us", "narrowphase_us", "coloring_us", "solver_us", "total_us"],
+                        [x.value for x in v]))

@@ -1,0 +1,398 @@
+// Broadphase: per-collider AABBs, 30-bit Morton codes (world id in the high bits), an LBVH
+// (Karras 2012) rebuilt every step over the radix-sorted codes, and a stackless traversal
+// over escape links that emits candidate pairs with wave-aggregated appends.
+//
+// It replaces the reference's sort-and-sweep (src/Physecs.cpp:119-172) and must produce
+// the same pair SET: the same filters (enableSimulation, same entity, both non-dynamic)
+// and the same inclusive AABB overlap test (Physecs.cpp:152-154). The bounds follow
+// src/BoundsUtil.cpp:35-59 with the reference's margin rule: none at creation, +0.01 for
+// the colliders of non-kinematic dynamic bodies after every step (Physecs.cpp:32,79-90).
+#include "pgx_internal.h"
+
+namespace pgx {
+
+// ------------------------------------------------------------------ bounds
+__device__ __forceinline__ void collider_world_pose(const float4* __restrict__ bpos, const float4* __restrict__ bquat,
+                                                    const float4* __restrict__ clpos,
+                                                    const float4* __restrict__ clquat, int body, int c, f3& pos,
+                                                    qf& q) {
+    f3 p = xyz(bpos[body]);
+    qf bq = q4(bquat[body]);
+    pos = add(p, qrot(bq, xyz(clpos[c])));
+    q = qmul(bq, q4(clquat[c]));
+}
+
+__device__ __forceinline__ void shape_bounds(int type, float4 prm, f3 pos, qf ori, f3& mn, f3& mx) {
+    if (type == PGX_SPHERE) {
+        float r = prm.x;
+        mn = sub(pos, s3(r));
+        mx = add(pos, s3(r));
+    } else if (type == PGX_CAPSULE) {
+        float hh = prm.x, r = prm.y;
+        f3 p0 = add(pos, qrot(ori, mk3(0.f, hh, 0.f)));
+        f3 p1 = add(pos, qrot(ori, mk3(0.f, -hh, 0.f)));
+        mn = subs(vmin(p0, p1), r);
+        mx = adds(vmax(p0, p1), r);
+    } else {   // box
+        m3 u = mat3_cast(ori);
+        m3 au = mk_m3(vabs(u.c0), vabs(u.c1), vabs(u.c2));
+        f3 wh = mulmv(au, mk3(prm.x, prm.y, prm.z));
+        mn = sub(pos, wh);
+        mx = add(pos, wh);
+    }
+}
+
+__global__ void k_bounds(int C, const int32_t* __restrict__ cbody, const int32_t* __restrict__ ctype,
+                         const int32_t* __restrict__ cflags, const float4* __restrict__ cparams,
+                         const float4* __restrict__ clpos, const float4* __restrict__ clquat,
+                         const float4* __restrict__ bpos, const float4* __restrict__ bquat, int dynamicOnly,
+                         float4* __restrict__ bmin, float4* __restrict__ bmax) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    if (dynamicOnly && !(cflags[c] & CF_DYNAMIC)) return;
+    f3 pos;
+    qf q;
+    collider_world_pose(bpos, bquat, clpos, clquat, cbody[c], c, pos, q);
+    f3 mn, mx;
+    shape_bounds(ctype[c], cparams[c], pos, q, mn, mx);
+    if (dynamicOnly) {   // Bounds::addMargin(vec3(0.01f)), src/Bounds.cpp:23-26
+        mx = add(mx, s3(0.01f));
+        mn = sub(mn, s3(0.01f));
+    }
+    bmin[c] = f4(mn, 0.f);
+    bmax[c] = f4(mx, 0.f);
+}
+
+void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin) {
+    if (w.C == 0) return;
+    hipLaunchKernelGGL(k_bounds, dim3(div_up(w.C, 256)), dim3(256), 0, w.stream, w.C, w.cbody.p, w.ctype.p,
+                       w.cflags.p, w.cparams.p, w.clpos.p, w.clquat.p, w.bpos.p, w.bquat.p,
+                       dynamicOnlyWithMargin ? 1 : 0, w.cbmin.p, w.cbmax.p);
+    PGX_LAUNCH_CHECK();
+}
+
+// ------------------------------------------------------------------ Morton codes
+__device__ __forceinline__ uint32_t ord_enc(float f) {   // order-preserving float -> uint
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord_dec(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void k_world_bounds_init(int W, uint32_t* wb) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= W) return;
+    for (int k = 0; k < 3; ++k) {
+        wb[6 * i + k] = 0xffffffffu;   // min
+        wb[6 * i + 3 + k] = 0u;        // max
+    }
+}
+
+__global__ void k_world_bounds(int C, const int32_t* __restrict__ cworld, const float4* __restrict__ bmin,
+                               const float4* __restrict__ bmax, uint32_t* __restrict__ wb) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    float4 a = bmin[c], b = bmax[c];
+    float cx = 0.5f * (a.x + b.x), cy = 0.5f * (a.y + b.y), cz = 0.5f * (a.z + b.z);
+    uint32_t* o = wb + 6 * cworld[c];
+    atomicMin(o + 0, ord_enc(cx));
+    atomicMin(o + 1, ord_enc(cy));
+    atomicMin(o + 2, ord_enc(cz));
+    atomicMax(o + 3, ord_enc(cx));
+    atomicMax(o + 4, ord_enc(cy));
+    atomicMax(o + 5, ord_enc(cz));
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void k_morton(int C, const int32_t* __restrict__ cworld, const float4* __restrict__ bmin,
+                         const float4* __restrict__ bmax, const uint32_t* __restrict__ wb,
+                         uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    int wd = cworld[c];
+    const uint32_t* o = wb + 6 * wd;
+    float4 a = bmin[c], b = bmax[c];
+    float cc[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+        float lo = ord_dec(o[k]), hi = ord_dec(o[3 + k]);
+        float ext = hi - lo;
+        float t = ext > 0.f ? (cc[k] - lo) / ext : 0.5f;
+        t = fminf(fmaxf(t, 0.f), 1.f);
+        q[k] = (uint32_t)fminf(t * 1024.f, 1023.f);
+    }
+    uint32_t m = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    keys[c] = ((uint64_t)wd << 30) | m;
+    vals[c] = c;
+}
+
+// ------------------------------------------------------------------ LBVH build
+__device__ __forceinline__ int lcp(const uint64_t* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = k[i], b = k[j];
+    if (a != b) return __clzll((long long)(a ^ b));
+    return 64 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+}
+
+// Internal node i of the n-1 nodes: range, split, children (Karras 2012, Alg. 2).
+__global__ void k_build(int n, const uint64_t* __restrict__ keys, int32_t* __restrict__ child,
+                        int2* __restrict__ range, int32_t* __restrict__ nodeParent,
+                        int32_t* __restrict__ leafParent, int32_t* __restrict__ worldRoot, int worldShift) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (lcp(keys, n, i, i + 1) - lcp(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    int dmin = lcp(keys, n, i, i - d);
+    int lmax = 2;
+    while (lcp(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (lcp(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int first = min(i, j), last = max(i, j);
+    int dnode = lcp(keys, n, first, last);
+    int split = first, step = last - first;
+    do {
+        step = (step + 1) >> 1;
+        int ns = split + step;
+        if (ns < last && lcp(keys, n, first, ns) > dnode) split = ns;
+    } while (step > 1);
+    int left = (split == first) ? (split | LEAF_BIT) : split;
+    int right = (split + 1 == last) ? ((split + 1) | LEAF_BIT) : (split + 1);
+    child[2 * i] = left;
+    child[2 * i + 1] = right;
+    range[i] = make_int2(first, last);
+    if (left & LEAF_BIT) leafParent[split] = i; else nodeParent[split] = i;
+    if (right & LEAF_BIT) leafParent[split + 1] = i; else nodeParent[split + 1] = i;
+    // A child whose leaves all belong to one world while this node spans two worlds is
+    // that world's root (the world id forms the high bits of every key).
+    int wFirst = (int)(keys[first] >> worldShift), wLast = (int)(keys[last] >> worldShift);
+    if (wFirst != wLast) {
+        int wl = (int)(keys[first] >> worldShift), wsplit = (int)(keys[split] >> worldShift);
+        if (wl == wsplit) worldRoot[wl] = left;
+        int wr = (int)(keys[split + 1] >> worldShift), wlast = (int)(keys[last] >> worldShift);
+        if (wr == wlast) worldRoot[wr] = right;
+    } else if (i == 0) {
+        worldRoot[wFirst] = 0;
+    }
+}
+
+// Leaf data in sorted order; bottom-up refit with one atomic counter per internal node.
+__global__ void k_leaves(int n, const int32_t* __restrict__ order, const float4* __restrict__ bmin,
+                         const float4* __restrict__ bmax, const int32_t* __restrict__ cflags,
+                         float4* __restrict__ lmin, float4* __restrict__ lmax, int32_t* __restrict__ lflags) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int c = order[s];
+    lmin[s] = bmin[c];
+    lmax[s] = bmax[c];
+    lflags[s] = cflags[c];
+}
+
+__global__ void k_refit(int n, const int32_t* __restrict__ child, const int32_t* __restrict__ nodeParent,
+                        const int32_t* __restrict__ leafParent, const float4* __restrict__ lmin,
+                        const float4* __restrict__ lmax, const int32_t* __restrict__ lflags,
+                        float4* __restrict__ nmin, float4* __restrict__ nmax, int32_t* __restrict__ nflags,
+                        int32_t* __restrict__ counter) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int node = leafParent[s];
+    for (int guard = 0; node >= 0 && guard < n; ++guard) {
+        __threadfence();   // release this subtree's box before signalling the parent
+        if (atomicAdd(&counter[node], 1) == 0) return;   // first child to arrive stops
+        __threadfence();   // acquire the sibling subtree's box (written on another CU / XCD)
+        int a = child[2 * node], b = child[2 * node + 1];
+        float4 amin, amax, bmn, bmx;
+        int af, bf;
+        if (a & LEAF_BIT) { int x = a & ~LEAF_BIT; amin = lmin[x]; amax = lmax[x]; af = (lflags[x] & CF_DYNAMIC) ? 0 : 1; }
+        else { amin = nmin[a]; amax = nmax[a]; af = nflags[a]; }
+        if (b & LEAF_BIT) { int x = b & ~LEAF_BIT; bmn = lmin[x]; bmx = lmax[x]; bf = (lflags[x] & CF_DYNAMIC) ? 0 : 1; }
+        else { bmn = nmin[b]; bmx = nmax[b]; bf = nflags[b]; }
+        nmin[node] = make_float4(fminf(amin.x, bmn.x), fminf(amin.y, bmn.y), fminf(amin.z, bmn.z), 0.f);
+        nmax[node] = make_float4(fmaxf(amax.x, bmx.x), fmaxf(amax.y, bmx.y), fmaxf(amax.z, bmx.z), 0.f);
+        nflags[node] = af | bf;
+        node = nodeParent[node];
+    }
+}
+
+// Escape link of every node: the next node in depth-first pre-order after its subtree.
+__global__ void k_escape(int n, const int32_t* __restrict__ child, const int32_t* __restrict__ nodeParent,
+                         const int32_t* __restrict__ leafParent, int32_t* __restrict__ nodeEsc,
+                         int32_t* __restrict__ leafEsc) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int total = 2 * n - 1;
+    if (t >= total) return;
+    bool isLeaf = t >= n - 1;
+    int self = isLeaf ? ((t - (n - 1)) | LEAF_BIT) : t;
+    int cur = self;
+    int esc = -1;
+    for (int guard = 0; guard < n; ++guard) {
+        int p = (cur & LEAF_BIT) ? leafParent[cur & ~LEAF_BIT] : nodeParent[cur];
+        if (p < 0) break;
+        if (child[2 * p] == cur) { esc = child[2 * p + 1]; break; }
+        cur = p;
+    }
+    if (isLeaf) leafEsc[t - (n - 1)] = esc; else nodeEsc[t] = esc;
+}
+
+__global__ void k_world_stop(int W, const int32_t* __restrict__ worldRoot, const int32_t* __restrict__ nodeEsc,
+                             const int32_t* __restrict__ leafEsc, int32_t* __restrict__ worldStop) {
+    int wd = blockIdx.x * blockDim.x + threadIdx.x;
+    if (wd >= W) return;
+    int r = worldRoot[wd];
+    if (r == -1) { worldStop[wd] = -1; return; }
+    worldStop[wd] = (r & LEAF_BIT) ? leafEsc[r & ~LEAF_BIT] : nodeEsc[r];
+}
+
+// ------------------------------------------------------------------ traversal
+__device__ __forceinline__ bool overlaps(float4 amin, float4 amax, float4 bmin, float4 bmax) {
+    // inclusive, as the sweep's `max < min` rejections (src/Physecs.cpp:152-154)
+    return !(amax.x < bmin.x || amin.x > bmax.x || amax.y < bmin.y || amin.y > bmax.y || amax.z < bmin.z ||
+             amin.z > bmax.z);
+}
+
+__global__ __launch_bounds__(256) void k_traverse(
+    int n, const uint64_t* __restrict__ mkeys, int worldShift, const int32_t* __restrict__ order,
+    const float4* __restrict__ lmin, const float4* __restrict__ lmax, const int32_t* __restrict__ lflags,
+    const int32_t* __restrict__ child, const int2* __restrict__ range, const float4* __restrict__ nmin,
+    const float4* __restrict__ nmax, const int32_t* __restrict__ nflags, const int32_t* __restrict__ nodeEsc,
+    const int32_t* __restrict__ leafEsc, const int32_t* __restrict__ worldRoot,
+    const int32_t* __restrict__ worldStop, const uint32_t* __restrict__ centity,
+    const int32_t* __restrict__ clocal, int localBits, int32_t* __restrict__ pairCount, int cap,
+    uint64_t* __restrict__ pkeys) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = false;
+    float4 qmin, qmax;
+    int node = -1, stop = -1;
+    int myCol = 0;
+    uint32_t myEnt = 0;
+    int wd = 0;
+    if (s < n) {
+        int f = lflags[s];
+        if ((f & CF_ENABLE) && (f & CF_DYNAMIC)) {
+            active = true;
+            qmin = lmin[s];
+            qmax = lmax[s];
+            wd = (int)(mkeys[s] >> worldShift);
+            node = worldRoot[wd];
+            stop = worldStop[wd];
+            myCol = order[s];
+            myEnt = centity[myCol];
+        }
+    }
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int visits = 0;
+    const int maxVisits = 2 * n;   // a depth-first walk visits at most 2n - 1 nodes
+    while (__ballot(active && node != stop)) {
+        bool emit = false;
+        uint64_t key = 0;
+        if (active && node != stop && ++visits > maxVisits) {
+            atomicOr(&pairCount[6], 1);   // malformed tree: stop and report
+            active = false;
+        }
+        if (active && node != stop) {
+            if (node & LEAF_BIT) {
+                int s2 = node & ~LEAF_BIT;
+                int f2 = lflags[s2];
+                if (s2 != s && (f2 & CF_ENABLE) && (s2 > s || !(f2 & CF_DYNAMIC)) &&
+                    overlaps(qmin, qmax, lmin[s2], lmax[s2])) {
+                    int c2 = order[s2];
+                    uint32_t e2 = centity[c2];
+                    if (e2 != myEnt) {
+                        int c0 = myEnt < e2 ? myCol : c2;
+                        int c1 = myEnt < e2 ? c2 : myCol;
+                        key = ((uint64_t)wd << (2 * localBits)) | ((uint64_t)clocal[c0] << localBits) |
+                              (uint64_t)clocal[c1];
+                        emit = true;
+                    }
+                }
+                node = leafEsc[s2];
+            } else {
+                int2 r = range[node];
+                // subtrees whose leaves all sit at or before s and are all dynamic hold no
+                // pair that this leaf owns (a dynamic pair is emitted by its lower leaf)
+                bool skip = (r.y <= s) && nflags[node] == 0;
+                if (!skip && overlaps(qmin, qmax, nmin[node], nmax[node])) node = child[2 * node];
+                else node = nodeEsc[node];
+            }
+        }
+        uint64_t m = __ballot(emit);
+        if (m) {
+            int leader = __ffsll((long long)m) - 1;
+            int base = 0;
+            if (lane == leader) base = atomicAdd(pairCount, __popcll(m));
+            base = __shfl(base, leader);
+            if (emit) {
+                int idx = base + __popcll(m & lanesBelow);
+                if (idx < cap) pkeys[idx] = key;
+            }
+        }
+    }
+}
+
+__global__ void k_single_leaf_root(int32_t* worldRoot, int W, const uint64_t* mkeys, int worldShift) {
+    // n == 1: the only leaf is its world's root
+    if (threadIdx.x == 0 && blockIdx.x == 0) worldRoot[(int)(mkeys[0] >> worldShift)] = 0 | LEAF_BIT;
+}
+
+int bp_find_pairs(World& w) {
+    hipStream_t s = w.stream;
+    int n = w.C;
+    if (n == 0) return 0;
+    int W = w.W;
+    int worldShift = 30;
+    // world AABB of collider centres -> Morton codes -> sort
+    hipLaunchKernelGGL(k_world_bounds_init, dim3(div_up(W, 256)), dim3(256), 0, s, W, w.worldBoundsI.p);
+    hipLaunchKernelGGL(k_world_bounds, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.cworld.p, w.cbmin.p, w.cbmax.p,
+                       w.worldBoundsI.p);
+    hipLaunchKernelGGL(k_morton, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.cworld.p, w.cbmin.p, w.cbmax.p,
+                       w.worldBoundsI.p, w.mkeys.p, w.mvals.p);
+    PGX_LAUNCH_CHECK();
+    radix_sort_u64(s, w.mkeys.p, w.mvals.p, n, 30 + w.worldBits, w.sortT);
+    fill_i32(s, w.worldRoot.p, -1, W);
+    hipLaunchKernelGGL(k_leaves, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mvals.p, w.cbmin.p, w.cbmax.p,
+                       w.cflags.p, w.leafMin.p, w.leafMax.p, w.leafFlags.p);
+    if (n == 1) {
+        hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(64), 0, s, w.worldRoot.p, W, w.mkeys.p, worldShift);
+        fill_i32(s, w.leafEsc.p, -1, 1);
+        fill_i32(s, w.leafParent.p, -1, 1);
+    } else {
+        fill_i32(s, w.nodeParent.p, -1, n - 1);
+        fill_i32(s, w.nodeCounter.p, 0, n - 1);
+        hipLaunchKernelGGL(k_build, dim3(div_up(n - 1, 256)), dim3(256), 0, s, n, w.mkeys.p, w.nodeChild.p,
+                           w.nodeRange.p, w.nodeParent.p, w.leafParent.p, w.worldRoot.p, worldShift);
+        hipLaunchKernelGGL(k_refit, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.nodeChild.p, w.nodeParent.p,
+                           w.leafParent.p, w.leafMin.p, w.leafMax.p, w.leafFlags.p, w.nodeMin.p, w.nodeMax.p,
+                           w.nodeFlags.p, w.nodeCounter.p);
+        hipLaunchKernelGGL(k_escape, dim3(div_up(2 * n - 1, 256)), dim3(256), 0, s, n, w.nodeChild.p,
+                           w.nodeParent.p, w.leafParent.p, w.nodeEsc.p, w.leafEsc.p);
+    }
+    hipLaunchKernelGGL(k_world_stop, dim3(div_up(W, 256)), dim3(256), 0, s, W, w.worldRoot.p, w.nodeEsc.p,
+                       w.leafEsc.p, w.worldStop.p);
+    fill_i32(s, w.counters.p, 0, 8);
+    hipLaunchKernelGGL(k_traverse, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift, w.mvals.p,
+                       w.leafMin.p, w.leafMax.p, w.leafFlags.p, w.nodeChild.p, w.nodeRange.p, w.nodeMin.p,
+                       w.nodeMax.p, w.nodeFlags.p, w.nodeEsc.p, w.leafEsc.p, w.worldRoot.p, w.worldStop.p,
+                       w.centity.p, w.clocal.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
+    PGX_LAUNCH_CHECK();
+    int cnt[8] = {0};
+    PGX_HIP(hipMemcpyAsync(cnt, w.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    PGX_HIP(hipStreamSynchronize(s));
+    int nPairs = cnt[0];
+    if (cnt[6]) throw PgxError{PGX_ERR_INVALID, "broadphase traversal exceeded its visit bound (malformed tree)"};
+    if (nPairs > w.pairCap) throw PgxError{PGX_ERR_CAPACITY, "pair capacity exceeded: " + std::to_string(nPairs) +
+                                                                 " > " + std::to_string(w.pairCap)};
+    // canonical order: (world, local collider of the smaller entity, other local collider)
+    radix_sort_u64(s, w.pkeys.p, w.pvals.p, nPairs, w.worldBits + 2 * w.localBits, w.sortT);
+    return nPairs;
+}
+
+}  // namespace pgx

@@ -1,0 +1,704 @@
+// Narrowphase: one candidate pair per lane, writing a manifold slot (SoA) per pair.
+//
+// The pair routines follow the reference's src/Collision.cpp (dispatcher :895-1016,
+// sphere/capsule/box closed forms :109-133,428-486, box-box SAT + face clipping
+// :54-107,135-350, capsule-box :501-692), src/GeomUtil.cpp (segment distances and the
+// segment-AABB regions :7-59,129-467), src/Clipping.cpp (:10-71) and
+// src/ContactsUtil.cpp (:9-107), operation for operation, so the manifold is bit-identical
+// to the reference's for the same poses. The polygon clip buffers live in registers
+// (a quad clipped by four edges has at most 8 vertices).
+#include "pgx_internal.h"
+
+namespace pgx {
+
+struct CPt { f3 p0, p1; };
+struct Mf {
+    int numPoints;
+    CPt pts[4];
+    f3 normal;
+};
+
+// ------------------------------------------------------------------ GeomUtil
+__device__ f3 closest_point_on_segment(f3 point, f3 o, f3 d, float mn, float mx) {
+    f3 r = sub(o, point);
+    float t = -dot(r, d);
+    t = gclamp(t, mn, mx);
+    return add(o, mul(t, d));
+}
+
+__device__ void closest_points_segments(f3 o0, f3 d0, float mn0, float mx0, f3 o1, f3 d1, float mn1, float mx1,
+                                        f3& p0, f3& p1) {
+    float v1v2 = dot(d0, d1);
+    f3 r = sub(o1, o0);
+    float rv1 = dot(r, d0);
+    float rv2 = dot(r, d1);
+    float t1 = gabs(v1v2) > 1.f - 0.0001f ? 0.f : (rv1 * v1v2 - rv2) / (1.f - v1v2 * v1v2);
+    float t0 = rv1 + t1 * v1v2;
+    if (t0 < mn0 || t0 > mx0) {
+        t0 = t0 < mn0 ? mn0 : mx0;
+        p0 = add(o0, mul(t0, d0));
+        t1 = -dot(sub(o1, p0), d1);
+    } else {
+        p0 = add(o0, mul(t0, d0));
+    }
+    if (t1 < mn1 || t1 > mx1) {
+        t1 = t1 < mn1 ? mn1 : mx1;
+        p1 = add(o1, mul(t1, d1));
+        t0 = -dot(sub(o0, p1), d0);
+        t0 = gclamp(t0, mn0, mx0);
+        p0 = add(o0, mul(t0, d0));
+    } else {
+        p1 = add(o1, mul(t1, d1));
+    }
+}
+
+__device__ float case_two_zeros(f3 p, int axis, f3 he, float& t, f3& q) {
+    float sq = 0.f;
+    t = at(he, axis) - at(p, axis);
+    set(q, axis, at(p, axis) + t);
+    for (int k = 1; k <= 2; ++k) {
+        int a = (axis + k) % 3;
+        float pa = at(p, a), ha = at(he, a);
+        if (pa > ha) { float d = pa - ha; sq += d * d; set(q, a, ha); }
+        else if (pa < -ha) { float d = pa + ha; sq += d * d; set(q, a, -ha); }
+        else set(q, a, pa);
+    }
+    return sq;
+}
+
+__device__ float case_one_zero(f3 p, int z0, f3 dir, f3 he, float& t, f3& q) {
+    float sq = 0.f;
+    set(q, z0, at(p, z0));
+    f3 pme = sub(p, he);
+    int x = (z0 + 1) % 3, y = (z0 + 2) % 3;
+    float prod0 = at(dir, y) * at(pme, x);
+    float prod1 = at(dir, x) * at(pme, y);
+    if (prod0 >= prod1) {
+        set(q, x, at(he, x));
+        float eY = at(p, y) + at(he, y);
+        float delta = prod0 - at(dir, x) * eY;
+        if (delta >= 0) {
+            sq += delta * delta;
+            set(q, y, -at(he, y));
+            t = -(at(pme, x) * at(dir, x) + eY * at(dir, y));
+        } else {
+            set(q, y, at(p, y) - prod0 / at(dir, x));
+            t = -at(pme, x) / at(dir, x);
+        }
+    } else {
+        set(q, y, at(he, y));
+        float eX = at(p, x) + at(he, x);
+        float delta = prod1 - at(dir, y) * eX;
+        if (delta >= 0) {
+            sq += delta * delta;
+            set(q, x, -at(he, x));
+            t = -(eX * at(dir, x) + at(pme, y) * at(dir, y));
+        } else {
+            set(q, x, at(p, x) - prod1 / at(dir, y));
+            t = -at(pme, y) / at(dir, y);
+        }
+    }
+    float pz = at(p, z0), hz = at(he, z0);
+    if (pz < -hz) { float d = pz + hz; sq += d * d; set(q, z0, -hz); }
+    else if (pz > hz) { float d = pz - hz; sq += d * d; set(q, z0, hz); }
+    return sq;
+}
+
+// Region 4/5 and 1/2/3 helpers of the face case. "Region 5" keeps the reference's
+// pme[y] + pme[y] term (src/GeomUtil.cpp:273,326).
+__device__ __forceinline__ float region45(f3 p, f3 dir, f3 he, int x, int y, int z, f3 pme, f3 ppe, float npY,
+                                          float& t, f3& q) {
+    float dx = at(dir, x), dy = at(dir, y), dz = at(dir, z);
+    float pmx = at(pme, x), pmy = at(pme, y), ppz = at(ppe, z);
+    if (npY <= at(he, y)) {
+        float tmp = at(p, y) - npY;
+        float delta = dx * pmx + dy * tmp + dz * ppz;
+        t = -delta;
+        set(q, x, at(he, x)); set(q, y, npY); set(q, z, -at(he, z));
+        return pmx * pmx + tmp * tmp + ppz * ppz - delta * delta;
+    }
+    float delta = dx * pmx + dy * pmy + dz * ppz;
+    t = -delta;
+    set(q, x, at(he, x)); set(q, y, at(he, y)); set(q, z, -at(he, z));
+    return pmx * pmx + pmy + pmy + ppz * ppz - delta * delta;
+}
+__device__ __forceinline__ float region12(f3 p, f3 dir, f3 he, int x, int y, int z, f3 pme, f3 ppe, float npZ,
+                                          float& t, f3& q) {
+    float dx = at(dir, x), dy = at(dir, y), dz = at(dir, z);
+    float pmx = at(pme, x), ppy = at(ppe, y), pmz = at(pme, z);
+    if (npZ <= at(he, z)) {
+        float tmp = at(p, z) - npZ;
+        float delta = dx * pmx + dy * ppy + dz * tmp;
+        t = -delta;
+        set(q, x, at(he, x)); set(q, y, -at(he, y)); set(q, z, npZ);
+        return pmx * pmx + ppy * ppy + tmp * tmp - delta * delta;
+    }
+    float delta = dx * pmx + dy * ppy + dz * pmz;
+    t = -delta;
+    set(q, x, at(he, x)); set(q, y, -at(he, y)); set(q, z, at(he, z));
+    return pmx * pmx + ppy * ppy + pmz * pmz - delta * delta;
+}
+
+__device__ float line_aabb_face(f3 p, f3 dir, f3 he, int x, f3 pme, float& t, f3& q) {
+    q = p;
+    t = 0.f;
+    int y = (x + 1) % 3, z = (y + 1) % 3;
+    f3 ppe = add(p, he);
+    float dx = at(dir, x), dy = at(dir, y), dz = at(dir, z);
+    float pmx = at(pme, x);
+    if (dx * at(ppe, y) >= dy * pmx) {
+        if (dx * at(ppe, z) >= dz * pmx) {
+            set(q, x, at(he, x));
+            set(q, y, at(q, y) - pmx * dy / dx);
+            set(q, z, at(q, z) - pmx * dz / dx);
+            t = -pmx / dx;
+            return 0.f;
+        }
+        float sqrLen = dx * dx + dz * dz;
+        float npY = at(p, y) - dy * (dx * pmx + dz * at(ppe, z)) / sqrLen;
+        return region45(p, dir, he, x, y, z, pme, ppe, npY, t, q);
+    }
+    if (dx * at(ppe, z) >= dz * pmx) {
+        float sqrLen = dx * dx + dy * dy;
+        float npZ = at(p, z) - dz * (dx * pmx + dy * at(ppe, y)) / sqrLen;
+        return region12(p, dir, he, x, y, z, pme, ppe, npZ, t, q);
+    }
+    float npY = at(p, y) - dy * (dx * pmx + dz * at(ppe, z)) / (dx * dx + dz * dz);
+    if (npY >= -at(he, y)) return region45(p, dir, he, x, y, z, pme, ppe, npY, t, q);
+    float npZ = at(p, z) - dz * (dx * pmx + dy * at(ppe, y)) / (dx * dx + dy * dy);
+    if (npZ >= -at(he, z)) return region12(p, dir, he, x, y, z, pme, ppe, npZ, t, q);
+    float delta = dx * pmx + dy * at(ppe, y) + dz * at(ppe, z);
+    t = -delta;
+    set(q, x, at(he, x)); set(q, y, -at(he, y)); set(q, z, -at(he, z));
+    return pmx * pmx + at(ppe, y) * at(ppe, y) + at(ppe, z) * at(ppe, z) - delta * delta;
+}
+
+__device__ float case_no_zeroes(f3 p, f3 dir, f3 he, float& t, f3& q) {
+    f3 pme = sub(p, he);
+    float dxEy = dir.x * pme.y, dyEx = dir.y * pme.x;
+    if (dyEx >= dxEy) {
+        float dzEx = dir.z * pme.x, dxEz = dir.x * pme.z;
+        return line_aabb_face(p, dir, he, dzEx >= dxEz ? 0 : 2, pme, t, q);
+    }
+    float dzEy = dir.z * pme.y, dyEz = dir.y * pme.z;
+    return line_aabb_face(p, dir, he, dzEy >= dyEz ? 1 : 2, pme, t, q);
+}
+
+__device__ float point_aabb(f3 p, f3 he, f3& q) {
+    q = vclamp(p, neg(he), he);
+    f3 d = sub(p, q);
+    return dot(d, d);
+}
+
+__device__ float segment_aabb(f3 p, f3 dir, float mn, float mx, f3 he, float& t, f3& q) {
+    bool refl[3];
+    f3 pp = p, dd = dir;
+    for (int i = 0; i < 3; ++i) {
+        if (at(dd, i) < 0.f) { set(pp, i, -at(pp, i)); set(dd, i, -at(dd, i)); refl[i] = true; }
+        else refl[i] = false;
+    }
+    float sq;
+    bool px = dd.x > 0, py = dd.y > 0, pz = dd.z > 0;
+    if (px && py && pz) sq = case_no_zeroes(pp, dd, he, t, q);
+    else if (px && py) sq = case_one_zero(pp, 2, dd, he, t, q);
+    else if (px && pz) sq = case_one_zero(pp, 1, dd, he, t, q);
+    else if (py && pz) sq = case_one_zero(pp, 0, dd, he, t, q);
+    else if (px) sq = case_two_zeros(pp, 0, he, t, q);
+    else if (py) sq = case_two_zeros(pp, 1, he, t, q);
+    else if (pz) sq = case_two_zeros(pp, 2, he, t, q);
+    else { sq = point_aabb(pp, he, q); t = 0.f; }
+    for (int i = 0; i < 3; ++i)
+        if (refl[i]) set(q, i, -at(q, i));
+    if (t < mn) { t = mn; sq = point_aabb(add(p, mul(dir, t)), he, q); }
+    else if (t > mx) { t = mx; sq = point_aabb(add(p, mul(dir, t)), he, q); }
+    return sq;
+}
+
+// ------------------------------------------------------------------ clipping
+constexpr int MAXPOLY = 12;
+
+__device__ __forceinline__ bool inside(f2 pt, f2 a, f2 b) {
+    return (b.y - a.y) * (pt.x - a.x) - (b.x - a.x) * (pt.y - a.y) > 0;
+}
+__device__ __forceinline__ f2 intersect2(f2 a1, f2 b1, f2 a2, f2 b2) {
+    f2 r1 = sub2(b1, a1), r2 = sub2(b2, a2);
+    float t = det2(sub2(a1, a2), r2) / det2(mk2(-r1.x, -r1.y), r2);
+    return add2(a1, mk2(t * r1.x, t * r1.y));
+}
+// Sutherland-Hodgman against an axis-aligned rectangle given as 4 corners (Clipping.cpp:21-53)
+__device__ void clip_polygon(f2* poly, int& n, const f2* clip) {
+    f2 out[MAXPOLY];
+    for (int e = 0; e < 4; ++e) {
+        f2 a = clip[e], b = clip[(e + 1) & 3];
+        int m = 0;
+        for (int i = 0; i < n; ++i) {
+            int j = (i + 1) == n ? 0 : i + 1;
+            bool ii = inside(poly[i], a, b), ji = inside(poly[j], a, b);
+            if (ii && ji) { if (m < MAXPOLY) out[m++] = poly[j]; }
+            else if (ii) { if (m < MAXPOLY) out[m++] = intersect2(poly[i], poly[j], a, b); }
+            else if (ji) {
+                if (m < MAXPOLY) out[m++] = intersect2(poly[i], poly[j], a, b);
+                if (m < MAXPOLY) out[m++] = poly[j];
+            }
+        }
+        n = m;
+        for (int i = 0; i < m; ++i) poly[i] = out[i];
+    }
+}
+__device__ bool clip_line(f2& p0, f2& p1, const f2* clip) {
+    for (int i = 0; i < 4; ++i) {
+        int j = (i + 1) & 3;
+        bool in0 = inside(p0, clip[i], clip[j]), in1 = inside(p1, clip[i], clip[j]);
+        if (in0 && in1) continue;
+        if (in0) p1 = intersect2(p0, p1, clip[i], clip[j]);
+        else if (in1) p0 = intersect2(p0, p1, clip[i], clip[j]);
+        else return false;
+    }
+    return true;
+}
+
+// src/ContactsUtil.cpp:9-107
+__device__ void polygon_box_face_contacts(f3 boxCenter, const m3& basis, int axis, int axisSign, f3 he,
+                                          f3 incOrig, f3 incNormal, const f2* poly, int n, int cx, int cy,
+                                          CPt* cps, int& numPoints) {
+    f3 pen[MAXPOLY];
+    int np = 0;
+    f3 dir = s3(0.f);
+    set(dir, axis, (float)axisSign);
+    float denom = dot(dir, incNormal);
+    float lim = at(he, axis) + 1e-4f;
+    for (int i = 0; i < n; ++i) {
+        f3 p3 = s3(0.f);
+        set(p3, cx, poly[i].x);
+        set(p3, cy, poly[i].y);
+        float distance = dot(sub(incOrig, p3), incNormal) / denom;
+        if (distance < lim) {
+            set(p3, axis, axisSign * distance);
+            pen[np++] = p3;
+        }
+    }
+    auto emit = [&](int slot, int idx) {
+        f3 on = pen[idx];
+        set(on, axis, axisSign * at(he, axis));
+        cps[slot].p0 = add(boxCenter, mulmv(basis, on));
+        cps[slot].p1 = add(boxCenter, mulmv(basis, pen[idx]));
+    };
+    if (np > 4) {
+        emit(0, 0);
+        float maxD = 0.f;
+        int maxDi = 0;
+        for (int i = 0; i < np; ++i) {
+            f3 d = sub(pen[i], pen[0]);
+            float dd = dot(d, d);
+            if (dd > maxD) { maxD = dd; maxDi = i; }
+        }
+        emit(1, maxDi);
+        float maxA = 0.f, minA = 0.f;
+        int maxAi = 0, minAi = 0;
+        for (int i = 0; i < np; ++i) {
+            f3 ca = sub(pen[0], pen[i]);
+            f3 cb = sub(pen[maxDi], pen[i]);
+            float area = det2(mk2(at(ca, cx), at(ca, cy)), mk2(at(cb, cx), at(cb, cy)));
+            if (area > maxA) { maxA = area; maxAi = i; }
+        }
+        emit(2, maxAi);
+        for (int i = 0; i < np; ++i) {
+            f3 da = sub(pen[0], pen[i]);
+            f3 db = sub(pen[maxDi], pen[i]);
+            float area = det2(mk2(at(da, cx), at(da, cy)), mk2(at(db, cx), at(db, cy)));
+            if (area < minA) { minA = area; minAi = i; }
+        }
+        emit(3, minAi);
+        numPoints = 4;
+    } else {
+        for (int i = 0; i < np; ++i) emit(i, i);
+        numPoints = np;
+    }
+}
+
+// ------------------------------------------------------------------ pair routines
+// src/Collision.cpp:54-107
+__device__ void box_face_contacts(const m3& uRef, f3 posRef, f3 heRef, const m3& uInc, f3 posInc, f3 heInc,
+                                  int refAxis, CPt* cps, int& numPoints) {
+    f3 axis = col(uRef, refAxis);
+    float refSign = dot(axis, sub(posInc, posRef)) < 0 ? -1.f : 1.f;
+    float maxDot = 0.f, incSign = 1.f;
+    int incAxis = 0;
+    for (int i = 0; i < 3; ++i) {
+        float d = dot(mul(-refSign, axis), col(uInc, i));
+        float ad = gabs(d);
+        if (ad > maxDot) { maxDot = ad; incAxis = i; incSign = d < 0 ? -1.f : 1.f; }
+    }
+    int ia1 = (incAxis + 1) % 3, ia2 = (incAxis + 2) % 3;
+    m3 inv = inverse(uRef);
+    f3 ui = col(uInc, incAxis), u1 = col(uInc, ia1), u2 = col(uInc, ia2);
+    float h1 = at(heInc, ia1), h2 = at(heInc, ia2);
+    f3 incOrig = mulmv(inv, sub(add(posInc, mul(mul(ui, at(heInc, incAxis)), incSign)), posRef));
+    f3 a = mul(u1, h1), b = mul(u2, h2);
+    f3 na = mul(neg(u1), h1);
+    f3 poly3[4] = {add(incOrig, mulmv(inv, add(a, b))), add(incOrig, mulmv(inv, add(na, b))),
+                   add(incOrig, mulmv(inv, sub(na, b))), add(incOrig, mulmv(inv, sub(a, b)))};
+    int cx = (refAxis + 1) % 3, cy = (refAxis + 2) % 3;
+    f2 poly[MAXPOLY];
+    for (int i = 0; i < 4; ++i) poly[i] = mk2(at(poly3[i], cx), at(poly3[i], cy));
+    float hx = at(heRef, cx), hy = at(heRef, cy);
+    f2 clip[4] = {mk2(hx, hy), mk2(hx, -hy), mk2(-hx, -hy), mk2(-hx, hy)};
+    int n = 4;
+    clip_polygon(poly, n, clip);
+    f3 incNormal = mulmv(inv, ui);
+    polygon_box_face_contacts(posRef, uRef, refAxis, (int)refSign, heRef, incOrig, incNormal, poly, n, cx, cy, cps,
+                              numPoints);
+}
+
+__device__ bool sphere_sphere(f3 p0, float r0, f3 p1, float r1, Mf& m) {
+    f3 d = sub(p1, p0);
+    float rs = r0 + r1;
+    if (dot(d, d) > rs * rs) return false;
+    float dl = length(d);
+    m.normal = dl != 0.f ? divs(d, dl) : mk3(0.f, 1.f, 0.f);
+    m.numPoints = 1;
+    m.pts[0].p0 = add(p0, mul(m.normal, r0));
+    m.pts[0].p1 = sub(p1, mul(m.normal, r1));
+    return true;
+}
+
+__device__ bool capsule_capsule(f3 p0, qf q0, float hh0, float r0, f3 p1, qf q1, float hh1, float r1, Mf& m) {
+    f3 up = mk3(0.f, 1.f, 0.f);
+    f3 v0 = qrot(q0, up), v1 = qrot(q1, up);
+    f3 a, b;
+    closest_points_segments(p0, v0, -hh0, hh0, p1, v1, -hh1, hh1, a, b);
+    f3 d = sub(b, a);
+    float rs = r0 + r1;
+    if (dot(d, d) > rs * rs) return false;
+    float dl = length(d);
+    m.normal = dl != 0.f ? divs(d, dl) : up;
+    m.numPoints = 1;
+    m.pts[0].p0 = add(a, mul(m.normal, r0));
+    m.pts[0].p1 = sub(b, mul(m.normal, r1));
+    return true;
+}
+
+__device__ bool sphere_capsule(f3 p0, float r0, f3 p1, qf q1, float hh1, float r1, Mf& m) {
+    f3 up = mk3(0.f, 1.f, 0.f);
+    f3 dir = qrot(q1, up);
+    f3 q = closest_point_on_segment(p0, p1, dir, -hh1, hh1);
+    f3 v = sub(q, p0);
+    float rs = r0 + r1;
+    if (dot(v, v) > rs * rs) return false;
+    float vl = length(v);
+    m.normal = vl != 0.f ? divs(v, vl) : up;
+    m.numPoints = 1;
+    m.pts[0].p0 = add(p0, mul(m.normal, r0));
+    m.pts[0].p1 = sub(q, mul(m.normal, r1));
+    return true;
+}
+
+__device__ bool sphere_box(f3 p0, float r0, f3 p1, qf q1, f3 he, Mf& m) {
+    m3 u = mat3_cast(q1);
+    f3 d = sub(p0, p1);
+    d = mk3(dot(d, u.c0), dot(d, u.c1), dot(d, u.c2));
+    f3 q = add(p1, mulmv(u, vclamp(d, neg(he), he)));
+    f3 v = sub(q, p0);
+    if (dot(v, v) > r0 * r0) return false;
+    float vl = length(v);
+    m.numPoints = 1;
+    if (vl != 0.f) {
+        m.normal = divs(v, vl);
+        m.pts[0].p0 = add(p0, mul(m.normal, r0));
+        m.pts[0].p1 = q;
+    } else {
+        float mn = 3.402823466e+38f;
+        int axis = 0;
+        for (int i = 0; i < 3; ++i) {
+            float depth = at(he, i) - gabs(at(d, i));
+            if (depth < mn) { mn = depth; axis = i; }
+        }
+        float sign = at(d, axis) != 0.f ? gsign(at(d, axis)) : 1.f;
+        f3 bp = d;
+        set(bp, axis, sign * at(he, axis));
+        m.normal = mul(-sign, col(u, axis));
+        m.pts[0].p0 = p0;
+        m.pts[0].p1 = add(p1, mulmv(u, bp));
+    }
+    return true;
+}
+
+__device__ void capsule_box_contacts(f3 p0L, f3 p1L, float radius, f3 normal, f3 boxCenter, f3 he,
+                                     const m3& basis, Mf& m) {
+    int axis = 0;
+    float axisSign = 0.f, maxDot = 0.f;
+    for (int i = 0; i < 3; ++i) {
+        float d = dot(normal, col(basis, i));
+        float ad = gabs(d);
+        if (ad > maxDot) { maxDot = ad; axis = i; axisSign = d < 0 ? 1.f : -1.f; }
+    }
+    int cx = (axis + 1) % 3, cy = (axis + 2) % 3;
+    f2 l0 = mk2(at(p0L, cx), at(p0L, cy)), l1 = mk2(at(p1L, cx), at(p1L, cy));
+    float hx = at(he, cx), hy = at(he, cy);
+    f2 clip[4] = {mk2(hx, hy), mk2(hx, -hy), mk2(-hx, -hy), mk2(-hx, hy)};
+    if (!clip_line(l0, l1, clip)) { m.numPoints = 0; return; }
+    float onAxis = axisSign * at(he, axis);
+    f3 a = s3(0.f), b = s3(0.f);
+    float distX = at(p0L, cx) - at(p1L, cx), distY = at(p0L, cy) - at(p1L, cy);
+    float adx = gabs(distX), ady = gabs(distY);
+    if (adx > ady) {
+        set(a, axis, gmix(at(p0L, axis), at(p1L, axis), (at(p0L, cx) - l0.x) / distX));
+        set(b, axis, gmix(at(p1L, axis), at(p0L, axis), (at(p1L, cx) - l1.x) / -distX));
+    } else if (ady > adx) {
+        set(a, axis, gmix(at(p0L, axis), at(p1L, axis), (at(p0L, cy) - l0.y) / distY));
+        set(b, axis, gmix(at(p1L, axis), at(p0L, axis), (at(p1L, cy) - l1.y) / -distY));
+    } else {
+        set(a, axis, at(p0L, axis));
+        set(b, axis, at(p1L, axis));
+    }
+    set(a, axis, at(a, axis) - axisSign * radius);
+    set(b, axis, at(b, axis) - axisSign * radius);
+    int np = 0;
+    float ha = at(he, axis);
+    if (gabs(at(a, axis)) < ha) {
+        set(a, cx, l0.x); set(a, cy, l0.y);
+        f3 ob = a; set(ob, axis, onAxis);
+        m.pts[np].p0 = add(boxCenter, mulmv(basis, a));
+        m.pts[np].p1 = add(boxCenter, mulmv(basis, ob));
+        ++np;
+    }
+    if (gabs(at(b, axis)) < ha) {
+        set(b, cx, l1.x); set(b, cy, l1.y);
+        f3 ob = b; set(ob, axis, onAxis);
+        m.pts[np].p0 = add(boxCenter, mulmv(basis, b));
+        m.pts[np].p1 = add(boxCenter, mulmv(basis, ob));
+        ++np;
+    }
+    m.numPoints = np;
+}
+
+__device__ bool capsule_box(f3 p0, qf q0, float hh0, float r0, f3 p1, qf q1, f3 he, Mf& m) {
+    m3 u = mat3_cast(q1);
+    f3 p = sub(p0, p1);
+    p = mk3(dot(p, u.c0), dot(p, u.c1), dot(p, u.c2));
+    f3 dir = qrot(q0, mk3(0.f, 1.f, 0.f));
+    f3 dl = mk3(dot(dir, u.c0), dot(dir, u.c1), dot(dir, u.c2));
+    float t = 0.f;
+    f3 q = s3(0.f);
+    float sq = segment_aabb(p, dl, -hh0, hh0, he, t, q);
+    if (sq >= r0 * r0) return false;
+    f3 onSeg = add(p0, mul(dir, t));
+    f3 onBox = add(p1, mulmv(u, q));
+    f3 nrm = sub(onBox, onSeg);
+    float nl = length(nrm);
+    if (nl != 0.f) {
+        m.normal = divs(nrm, nl);
+    } else {
+        f3 r, ar;
+        r = mk3(dot(dir, u.c0), dot(dir, u.c1), dot(dir, u.c2));
+        ar = vabs(r);
+        float mx = -3.402823466e+38f;
+        bool edge = false;
+        int axis = 0;
+        for (int i = 0; i < 3; ++i) {
+            float d = gabs(at(p, i)) - (hh0 * at(ar, i) + r0) - at(he, i);
+            if (d > mx) { mx = d; axis = i; }
+        }
+        float edgeOffset = 0.1f;
+        {
+            float rb = he.y * ar.z + he.z * ar.y;
+            float d = gabs(p.z * r.y - p.y * r.z) - r0 - rb;
+            if (ar.x < 0.999f && d > mx + edgeOffset) { mx = d; edge = true; axis = 0; edgeOffset = 0.f; }
+        }
+        {
+            float rb = he.x * ar.z + he.z * ar.x;
+            float d = gabs(p.x * r.z - p.z * r.x) - r0 - rb;
+            if (ar.y < 0.999f && d > mx + edgeOffset) { mx = d; edge = true; axis = 1; edgeOffset = 0.f; }
+        }
+        {
+            float rb = he.x * ar.y + he.y * ar.x;
+            float d = gabs(p.y * r.x - p.x * r.y) - r0 - rb;
+            if (ar.z < 0.999f && d > mx + edgeOffset) { mx = d; edge = true; axis = 2; }
+        }
+        f3 sep = edge ? cross(dir, col(u, axis)) : col(u, axis);
+        m.normal = normalize(dot(sep, sub(p1, p0)) > 0 ? sep : neg(sep));
+    }
+    capsule_box_contacts(sub(p, mul(dl, hh0)), add(p, mul(dl, hh0)), r0, m.normal, p1, he, u, m);
+    if (!m.numPoints) {
+        m.numPoints = 1;
+        m.pts[0].p0 = add(onSeg, mul(m.normal, r0));
+        m.pts[0].p1 = onBox;
+    }
+    return true;
+}
+
+// src/Collision.cpp:135-350
+__device__ bool box_box(f3 p0, qf q0, f3 he0, f3 p1, qf q1, f3 he1, Mf& m) {
+    float edgeOffset = 0.1f;
+    const float edgeLimit = 0.999f;
+    bool isEdge = false;
+    int faceBox = 0, faceAxis = 0, e0 = 0, e1 = 0;
+    float mx = -3.402823466e+38f;
+    m3 u0 = mat3_cast(q0), u1 = mat3_cast(q1);
+    f3 t = sub(p1, p0);
+    t = mk3(dot(t, u0.c0), dot(t, u0.c1), dot(t, u0.c2));
+    float r[3][3], ar[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            r[i][j] = dot(col(u0, i), col(u1, j));
+            ar[i][j] = gabs(r[i][j]);
+        }
+    float h0[3] = {he0.x, he0.y, he0.z}, h1[3] = {he1.x, he1.y, he1.z}, tt[3] = {t.x, t.y, t.z};
+    for (int i = 0; i < 3; ++i) {
+        float rb = h1[0] * ar[i][0] + h1[1] * ar[i][1] + h1[2] * ar[i][2];
+        float d = gabs(tt[i]) - h0[i] - rb;
+        if (d > 0) return false;
+        if (d > mx) { mx = d; faceBox = 0; faceAxis = i; }
+    }
+    for (int i = 0; i < 3; ++i) {
+        float ra = h0[0] * ar[0][i] + h0[1] * ar[1][i] + h0[2] * ar[2][i];
+        float l = gabs(tt[0] * r[0][i] + tt[1] * r[1][i] + tt[2] * r[2][i]);
+        float d = l - ra - h1[i];
+        if (d > 0) return false;
+        if (d > mx) { mx = d; faceBox = 1; faceAxis = i; }
+    }
+    // A_i x B_j in the reference's order; each radius sum lists the lower axis first
+    for (int i = 0; i < 3; ++i) {
+        int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+        int lo = i1 < i2 ? i1 : i2, hi = i1 < i2 ? i2 : i1;
+        for (int j = 0; j < 3; ++j) {
+            int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            int jlo = j1 < j2 ? j1 : j2, jhi = j1 < j2 ? j2 : j1;
+            float ra = h0[lo] * ar[lo == i1 ? i2 : i1][j] + h0[hi] * ar[hi == i1 ? i2 : i1][j];
+            float rb = h1[jlo] * ar[i][jlo == j1 ? j2 : j1] + h1[jhi] * ar[i][jhi == j1 ? j2 : j1];
+            float l = gabs(tt[i2] * r[i1][j] - tt[i1] * r[i2][j]);
+            float d = l - ra - rb;
+            if (d > 0) return false;
+            if (ar[i][j] < edgeLimit && d > mx + edgeOffset) {
+                mx = d; isEdge = true; e0 = i; e1 = j; edgeOffset = 0.f;
+            }
+        }
+    }
+    if (!isEdge) {
+        CPt cps[4];
+        int np = 0;
+        f3 axis;
+        if (faceBox) {
+            axis = col(u1, faceAxis);
+            box_face_contacts(u1, p1, he1, u0, p0, he0, faceAxis, cps, np);
+            for (int i = 0; i < np; ++i) { m.pts[i].p0 = cps[i].p1; m.pts[i].p1 = cps[i].p0; }
+        } else {
+            axis = col(u0, faceAxis);
+            box_face_contacts(u0, p0, he0, u1, p1, he1, faceAxis, cps, np);
+            for (int i = 0; i < np; ++i) m.pts[i] = cps[i];
+        }
+        m.numPoints = np;
+        m.normal = dot(axis, sub(p1, p0)) < 0 ? neg(axis) : axis;
+    } else {
+        f3 a0 = col(u0, e0), a1 = col(u1, e1);
+        f3 axis = cross(a0, a1);
+        m.normal = normalize(dot(axis, sub(p1, p0)) < 0 ? neg(axis) : axis);
+        float s0[3] = {-1.f, -1.f, -1.f}, s1[3] = {-1.f, -1.f, -1.f};
+        s0[(e0 + 1) % 3] = dot(col(u0, (e0 + 1) % 3), m.normal) < 0 ? -1.f : 1.f;
+        s0[(e0 + 2) % 3] = dot(col(u0, (e0 + 2) % 3), m.normal) < 0 ? -1.f : 1.f;
+        f3 c0 = add(add(add(p0, mul(s0[0] * h0[0], u0.c0)), mul(s0[1] * h0[1], u0.c1)), mul(s0[2] * h0[2], u0.c2));
+        s1[(e1 + 1) % 3] = dot(col(u1, (e1 + 1) % 3), m.normal) > 0 ? -1.f : 1.f;
+        s1[(e1 + 2) % 3] = dot(col(u1, (e1 + 2) % 3), m.normal) > 0 ? -1.f : 1.f;
+        f3 c1 = add(add(add(p1, mul(s1[0] * h1[0], u1.c0)), mul(s1[1] * h1[1], u1.c1)), mul(s1[2] * h1[2], u1.c2));
+        m.numPoints = 1;
+        f3 a, b;
+        closest_points_segments(c0, a0, 0.f, h0[e0] * 2, c1, a1, 0.f, h1[e1] * 2, a, b);
+        m.pts[0].p0 = a;
+        m.pts[0].p1 = b;
+    }
+    return true;
+}
+
+__device__ void flip(Mf& m) {
+    m.normal = neg(m.normal);
+    for (int i = 0; i < m.numPoints; ++i) { f3 a = m.pts[i].p0; m.pts[i].p0 = m.pts[i].p1; m.pts[i].p1 = a; }
+}
+
+// src/Collision.cpp:895-1016 for primitive shapes
+__device__ bool collide(int t0, float4 g0, f3 p0, qf q0, int t1, float4 g1, f3 p1, qf q1, Mf& m) {
+    m.numPoints = 0;
+    bool hit = false;
+    f3 he0 = mk3(g0.x, g0.y, g0.z), he1 = mk3(g1.x, g1.y, g1.z);
+    if (t0 == PGX_SPHERE) {
+        if (t1 == PGX_SPHERE) return sphere_sphere(p0, g0.x, p1, g1.x, m);
+        if (t1 == PGX_CAPSULE) return sphere_capsule(p0, g0.x, p1, q1, g1.x, g1.y, m);
+        return sphere_box(p0, g0.x, p1, q1, he1, m);
+    }
+    if (t0 == PGX_CAPSULE) {
+        if (t1 == PGX_SPHERE) { hit = sphere_capsule(p1, g1.x, p0, q0, g0.x, g0.y, m); flip(m); return hit; }
+        if (t1 == PGX_CAPSULE) return capsule_capsule(p0, q0, g0.x, g0.y, p1, q1, g1.x, g1.y, m);
+        return capsule_box(p0, q0, g0.x, g0.y, p1, q1, he1, m);
+    }
+    if (t1 == PGX_SPHERE) { hit = sphere_box(p1, g1.x, p0, q0, he0, m); flip(m); return hit; }
+    if (t1 == PGX_CAPSULE) { hit = capsule_box(p1, q1, g1.x, g1.y, p0, q0, he0, m); flip(m); return hit; }
+    return box_box(p0, q0, he0, p1, q1, he1, m);
+}
+
+// ------------------------------------------------------------------ kernel
+__device__ __forceinline__ bool in_sorted(const uint64_t* __restrict__ a, int n, uint64_t key) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (a[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo < n && a[lo] == key;
+}
+
+__global__ __launch_bounds__(256) void k_narrowphase(
+    int nPairs, const uint64_t* __restrict__ pkeys, int localBits, const int32_t* __restrict__ worldColStart,
+    const int32_t* __restrict__ colOfLocal, const int32_t* __restrict__ cbody, const int32_t* __restrict__ ctype,
+    const int32_t* __restrict__ cflags, const float4* __restrict__ cparams, const float4* __restrict__ clpos,
+    const float4* __restrict__ clquat, const uint32_t* __restrict__ centity, const float4* __restrict__ bpos,
+    const float4* __restrict__ bquat, const uint64_t* __restrict__ nonColliding, int nNonColliding,
+    int32_t* __restrict__ npNum, float4* __restrict__ npNormal, float4* __restrict__ npPts) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nPairs) return;
+    uint64_t key = pkeys[p];
+    uint64_t lmask = (1ull << localBits) - 1;
+    int wd = (int)(key >> (2 * localBits));
+    int base = worldColStart[wd];
+    int c0 = colOfLocal[base + (int)((key >> localBits) & lmask)];
+    int c1 = colOfLocal[base + (int)(key & lmask)];
+    int num = 0;
+    // defaultContactFilter: a trigger on either side makes a trigger pair, not a contact
+    // (src/Physecs.cpp:20-23,200-207); joint-connected / setCanCollide pairs are skipped (:209)
+    bool skip = ((cflags[c0] | cflags[c1]) & CF_TRIGGER) != 0;
+    if (!skip && nNonColliding) {
+        uint64_t ek = ((uint64_t)centity[c0] << 32) | centity[c1];
+        skip = in_sorted(nonColliding, nNonColliding, ek);
+    }
+    Mf m;
+    m.numPoints = 0;
+    if (!skip) {
+        int b0 = cbody[c0], b1 = cbody[c1];
+        f3 bp0 = xyz(bpos[b0]), bp1 = xyz(bpos[b1]);
+        qf bq0 = q4(bquat[b0]), bq1 = q4(bquat[b1]);
+        f3 p0 = add(bp0, qrot(bq0, xyz(clpos[c0])));
+        qf q0 = qmul(bq0, q4(clquat[c0]));
+        f3 p1 = add(bp1, qrot(bq1, xyz(clpos[c1])));
+        qf q1 = qmul(bq1, q4(clquat[c1]));
+        bool hit = collide(ctype[c0], cparams[c0], p0, q0, ctype[c1], cparams[c1], p1, q1, m);
+        num = hit ? m.numPoints : 0;
+    }
+    npNum[p] = num;
+    if (num) {
+        npNormal[p] = f4(m.normal, 0.f);
+        for (int k = 0; k < 4; ++k) {
+            if (k < num) {
+                npPts[8 * (size_t)p + 2 * k] = f4(m.pts[k].p0, 0.f);
+                npPts[8 * (size_t)p + 2 * k + 1] = f4(m.pts[k].p1, 0.f);
+            }
+        }
+    }
+}
+
+void np_run(World& w, int nPairs) {
+    if (nPairs == 0) return;
+    hipLaunchKernelGGL(k_narrowphase, dim3(div_up(nPairs, 256)), dim3(256), 0, w.stream, nPairs, w.pkeys.p,
+                       w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.ctype.p, w.cflags.p,
+                       w.cparams.p, w.clpos.p, w.clquat.p, w.centity.p, w.bpos.p, w.bquat.p, w.nonColliding.p,
+                       w.nNonColliding, w.npNum.p, w.npNormal.p, w.npPts.p);
+    PGX_LAUNCH_CHECK();
+}
+
+}  // namespace pgx

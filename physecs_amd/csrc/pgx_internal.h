@@ -1,0 +1,185 @@
+// Internal declarations shared by the HIP translation units of libphysecs_gpu.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <algorithm>
+#include <string>
+#include <vector>
+#include "../../include/physecs_gpu.h"
+#include "pgx_math.h"
+
+namespace pgx {
+
+struct HipError {
+    hipError_t err;
+    const char* expr;
+    const char* file;
+    int line;
+};
+struct PgxError {
+    int code;
+    std::string msg;
+};
+
+#define PGX_HIP(x)                                                      \
+    do {                                                                \
+        hipError_t e_ = (x);                                            \
+        if (e_ != hipSuccess) throw ::pgx::HipError{e_, #x, __FILE__, __LINE__}; \
+    } while (0)
+#define PGX_LAUNCH_CHECK() PGX_HIP(hipGetLastError())
+
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t cnt) {
+        if (cnt <= n && p) return;
+        release();
+        PGX_HIP(hipMalloc((void**)&p, std::max<size_t>(cnt, 1) * sizeof(T)));
+        n = cnt;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    operator T*() const { return p; }
+};
+
+constexpr int WAVE = 64;
+inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ utilities (util.hip)
+struct SortTemp {
+    DBuf<uint32_t> hist;
+    DBuf<uint64_t> kalt;
+    DBuf<int32_t> valt;
+};
+// Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys (8-bit digits).
+// The result is left in keys/vals.
+void radix_sort_u64(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t);
+
+struct ScanTemp {
+    DBuf<int32_t> sums;
+};
+// out[i] = sum(in[0..i-1]); *total (device) = sum(in[0..n-1])
+void exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int n, int32_t* total, ScanTemp& t);
+
+void fill_i32(hipStream_t s, int32_t* p, int32_t v, int n);
+void fill_u64(hipStream_t s, uint64_t* p, uint64_t v, int n);
+
+// ------------------------------------------------------------------ world state
+// Collider flag bits
+enum { CF_ENABLE = 1, CF_TRIGGER = 2, CF_DYNAMIC = 4 };
+constexpr int LEAF_BIT = 0x40000000;
+
+struct World {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int A = 0, D = 0, C = 0, W = 1;       // actors, dynamic bodies, colliders, worlds
+    int substeps = 8, iterations = 2;
+    float g = 9.81f;
+    int pairCap = 0, manifoldCap = 0;
+    int localBits = 1, worldBits = 0;
+    int blockSize = 256;
+
+    // host-side maps
+    std::vector<int> actorToBody, bodyToActor;   // descriptor actor <-> internal body
+    std::vector<uint32_t> hostEntity;            // per internal body
+    std::vector<int> hostColBody;
+    std::vector<int> hostColIndex;
+    std::vector<uint64_t> hostNonColliding;       // sorted (min entity << 32 | max entity)
+    bool nonCollidingDirty = false;
+
+    // bodies: internal body b; b < D are the dynamic-storage entries (same index)
+    DBuf<float4> bpos, bquat;            // A
+    DBuf<uint32_t> bentity;              // A
+    DBuf<float4> dvel, dang;             // D component velocity (dvel.w = invMass)
+    DBuf<float4> dcom;                   // D local com (w = kinematic ? 1 : 0)
+    DBuf<float4> dinvI;                  // 3 x D columns of the local inverse inertia
+    DBuf<float4> tvel, tang;             // D solver velocities (velocityTemp)
+    DBuf<float4> pvel, pang;             // D pseudo velocities (pang.w = constraint count)
+    DBuf<float4> winvI;                  // 3 x D world inverse inertia (massTemp)
+
+    // colliders, descriptor order
+    DBuf<int32_t> cbody, cworld, clocal, cflags, ctype;
+    DBuf<uint32_t> centity;
+    DBuf<int32_t> cindex;
+    DBuf<float4> cparams, clpos, clquat, cmat;
+    DBuf<float4> cbmin, cbmax;
+    DBuf<int32_t> worldColStart, colOfLocal;   // (world, local) -> collider
+
+    // broadphase
+    DBuf<uint32_t> worldBoundsI;         // W x 6 orderable-int encoded min/max of centres
+    DBuf<uint64_t> mkeys;
+    DBuf<int32_t> mvals;
+    DBuf<float4> leafMin, leafMax;       // sorted leaves
+    DBuf<int32_t> leafFlags, leafParent, leafEsc;
+    DBuf<int32_t> nodeChild;             // 2 x (C-1)
+    DBuf<int32_t> nodeParent, nodeEsc, nodeFlags, nodeCounter;
+    DBuf<int2> nodeRange;
+    DBuf<float4> nodeMin, nodeMax;
+    DBuf<int32_t> worldRoot, worldStop;
+    DBuf<uint64_t> pkeys;
+    DBuf<int32_t> pvals;
+    DBuf<int32_t> counters;              // [0] pairs, [1] manifolds, [2] colors, [3] uncolored, [4] points, [5] overflow
+    SortTemp sortT;
+    ScanTemp scanT;
+
+    // narrowphase output, one slot per pair
+    DBuf<int32_t> npNum;                 // numPoints (0 = no manifold)
+    DBuf<float4> npNormal;               // normal
+    DBuf<float4> npPts;                  // 8 per pair: p0,p1 for 4 points
+    DBuf<int32_t> npIndex;               // exclusive scan of (numPoints > 0)
+
+    // manifolds, canonical order (sorted by pair key)
+    DBuf<uint64_t> mfKey;                // pair key
+    DBuf<int32_t> mfPair;                // pair slot
+    DBuf<int32_t> mfB0, mfB1, mfColor;
+    DBuf<uint64_t> claim;                // D coloring claims
+    DBuf<uint64_t> usedMask;             // 2 x D color masks
+    DBuf<uint64_t> colorKeys;
+    DBuf<int32_t> colorPerm;
+    DBuf<int32_t> colorCount;            // colour start offsets (129 + 1)
+    DBuf<int32_t> worldMStart;           // W first manifold of each world
+
+    // manifolds, solve (colour) order
+    DBuf<int32_t> sB0, sB1, sFlags, sA0, sA1;   // sFlags: numPoints | soft<<8 | color<<16
+    DBuf<int32_t> sCanon;                // canonical index
+    DBuf<float4> sN;                     // normal, friction
+    DBuf<float2> sSoft;                  // frequency, damping
+    DBuf<float4> sR0, sR1;               // 4 x M: r0 local + targetVelocity, r1 local
+    DBuf<float4> sRow;                   // 9 x 4 x M per-substep rows
+
+    // contact cache (canonical order, double-buffered)
+    DBuf<uint64_t> cacheKey[2];
+    DBuf<int32_t> cacheNum[2];
+    DBuf<float4> cacheLp[2];             // 4 x M: local r0 + targetVelocity
+    int cacheCur = 0;
+    int cacheCount[2] = {0, 0};
+
+    DBuf<uint64_t> nonColliding;
+    int nNonColliding = 0;
+
+    // last step stats
+    int lastPairs = 0, lastManifolds = 0, lastColors = 0, lastRounds = 0, lastPoints = 0;
+    std::vector<int> colorStartHost;
+    bool timing = false;
+    hipEvent_t ev[6] = {};
+    float tBroad = 0, tNarrow = 0, tColor = 0, tSolve = 0, tTotal = 0;
+};
+
+// broadphase.hip
+void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin);
+int bp_find_pairs(World& w);          // returns pair count (host)
+
+// narrowphase.hip
+void np_run(World& w, int nPairs);
+
+// solver.hip
+int sv_setup_manifolds(World& w, int nPairs);          // returns manifold count
+int sv_color(World& w, int nManifolds);                 // returns colour count
+void sv_substeps(World& w, int nManifolds, float dt);
+void sv_finish_step(World& w, int nManifolds);
+
+}  // namespace pgx

@@ -1,0 +1,591 @@
+// Contact setup, graph colouring and the TGS soft-step substep loop.
+//
+// Reference semantics (src/Physecs.cpp, src/ContactConstraints.cpp):
+//   contact setup (material mix, body indices, body-local anchors, restitution target
+//   with the (pair, triangle) cache matched within 0.1 m)      Physecs.cpp:215-315
+//   update contact rows each substep                           Physecs.cpp:368-426
+//   integrate velocities (gravity, implicit gyroscopic term)   Physecs.cpp:443-468
+//   preSolve / solve (normal + one friction row per point)     ContactConstraints.cpp:4-124
+//   integrate positions (COM pivot, pseudo velocities)         Physecs.cpp:494-511
+//   relaxation (hard contacts, no bias) and write-back         Physecs.cpp:515-530
+//
+// The reference solves contact constraints sequentially in push order. Here the
+// manifolds are graph-coloured on the device so that no two manifolds of one colour share
+// a dynamic body; each colour is one kernel launch and is solved fully in parallel with
+// the same result as a sequential pass over it in any order. The solve order is
+// "colour 0, colour 1, ..." and is exported (pgx_world_get_manifolds) so that a
+// sequential solver fed this order reproduces the device result bit for bit.
+#include "pgx_internal.h"
+
+namespace pgx {
+
+// ------------------------------------------------------------------ compaction
+__global__ void k_flag_manifolds(int nPairs, const int32_t* __restrict__ npNum, int32_t* __restrict__ flag) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nPairs) flag[p] = npNum[p] > 0 ? 1 : 0;
+}
+
+__device__ __forceinline__ void pair_colliders(uint64_t key, int localBits, const int32_t* __restrict__ worldColStart,
+                                               const int32_t* __restrict__ colOfLocal, int& c0, int& c1, int& wd) {
+    uint64_t lmask = (1ull << localBits) - 1;
+    wd = (int)(key >> (2 * localBits));
+    int base = worldColStart[wd];
+    c0 = colOfLocal[base + (int)((key >> localBits) & lmask)];
+    c1 = colOfLocal[base + (int)(key & lmask)];
+}
+
+__global__ void k_compact(int nPairs, const int32_t* __restrict__ npNum, const int32_t* __restrict__ npIndex,
+                          const uint64_t* __restrict__ pkeys, int localBits, const int32_t* __restrict__ worldColStart,
+                          const int32_t* __restrict__ colOfLocal, const int32_t* __restrict__ cbody,
+                          const int32_t* __restrict__ cflags, int32_t* __restrict__ mfPair,
+                          uint64_t* __restrict__ mfKey, int32_t* __restrict__ mfB0, int32_t* __restrict__ mfB1) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nPairs || npNum[p] == 0) return;
+    int m = npIndex[p];
+    uint64_t key = pkeys[p];
+    int c0, c1, wd;
+    pair_colliders(key, localBits, worldColStart, colOfLocal, c0, c1, wd);
+    mfPair[m] = p;
+    mfKey[m] = key;
+    mfB0[m] = (cflags[c0] & CF_DYNAMIC) ? cbody[c0] : -1;
+    mfB1[m] = (cflags[c1] & CF_DYNAMIC) ? cbody[c1] : -1;
+}
+
+// ------------------------------------------------------------------ colouring
+// Jones-Plassmann with largest-priority-first selection and smallest-available colour.
+// Priorities are a bijective hash of the manifold's index within its world, so a world's
+// colouring does not depend on the other worlds sharing the buffers.
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352du;
+    x ^= x >> 15; x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__global__ void k_world_mstart(int M, const uint64_t* __restrict__ mfKey, int worldShift, int32_t* __restrict__ wstart) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    int wd = (int)(mfKey[m] >> worldShift);
+    if (m == 0 || (int)(mfKey[m - 1] >> worldShift) != wd) wstart[wd] = m;
+}
+
+__global__ void k_color_claim(int M, int round, const uint64_t* __restrict__ mfKey, int worldShift,
+                              const int32_t* __restrict__ wstart, const int32_t* __restrict__ color,
+                              const int32_t* __restrict__ b0, const int32_t* __restrict__ b1,
+                              unsigned long long* __restrict__ claim) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M || color[m] >= 0) return;
+    int wd = (int)(mfKey[m] >> worldShift);
+    unsigned long long v = ((unsigned long long)round << 32) | mix32((uint32_t)(m - wstart[wd]));
+    int a = b0[m], b = b1[m];
+    if (a >= 0) atomicMax(&claim[a], v);
+    if (b >= 0) atomicMax(&claim[b], v);
+}
+
+__global__ void k_color_pick(int M, int round, const uint64_t* __restrict__ mfKey, int worldShift,
+                             const int32_t* __restrict__ wstart, int32_t* __restrict__ color,
+                             const int32_t* __restrict__ b0, const int32_t* __restrict__ b1,
+                             const unsigned long long* __restrict__ claim, uint64_t* __restrict__ used,
+                             int32_t* __restrict__ counters) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M || color[m] >= 0) return;
+    int wd = (int)(mfKey[m] >> worldShift);
+    unsigned long long v = ((unsigned long long)round << 32) | mix32((uint32_t)(m - wstart[wd]));
+    int a = b0[m], b = b1[m];
+    bool won = (a < 0 || claim[a] == v) && (b < 0 || claim[b] == v);
+    if (!won) {
+        atomicAdd(&counters[3], 1);
+        return;
+    }
+    uint64_t lo = 0, hi = 0;
+    if (a >= 0) { lo |= used[2 * a]; hi |= used[2 * a + 1]; }
+    if (b >= 0) { lo |= used[2 * b]; hi |= used[2 * b + 1]; }
+    int c;
+    if (~lo) c = __ffsll((long long)~lo) - 1;
+    else if (~hi) c = 64 + __ffsll((long long)~hi) - 1;
+    else { c = 127; atomicAdd(&counters[5], 1); }   // overflow: reported as an error by the host
+    color[m] = c;
+    uint64_t bl = c < 64 ? (1ull << c) : 0ull, bh = c >= 64 ? (1ull << (c - 64)) : 0ull;
+    if (a >= 0) { used[2 * a] |= bl; used[2 * a + 1] |= bh; }
+    if (b >= 0) { used[2 * b] |= bl; used[2 * b + 1] |= bh; }
+}
+
+__global__ void k_color_keys(int M, const int32_t* __restrict__ color, uint64_t* __restrict__ keys,
+                             int32_t* __restrict__ vals) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    keys[m] = (uint64_t)color[m];
+    vals[m] = m;
+}
+
+__global__ void k_color_start(int M, const uint64_t* __restrict__ keys, int32_t* __restrict__ start) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    int c = (int)keys[s];
+    if (s == 0 || (int)keys[s - 1] != c) start[c] = s;
+    if (s == M - 1) start[c + 1] = M;
+}
+
+// ------------------------------------------------------------------ setup (solve order)
+__device__ __forceinline__ uint64_t cache_key(uint64_t pairKey) { return pairKey; }
+
+__global__ __launch_bounds__(256) void k_setup(
+    int M, int Mcap, const int32_t* __restrict__ perm, const int32_t* __restrict__ mfPair,
+    const uint64_t* __restrict__ mfKey, int localBits, const int32_t* __restrict__ worldColStart,
+    const int32_t* __restrict__ colOfLocal, const int32_t* __restrict__ cbody, const int32_t* __restrict__ cflags,
+    const float4* __restrict__ cmat, const int32_t* __restrict__ npNum, const float4* __restrict__ npNormal,
+    const float4* __restrict__ npPts, const int32_t* __restrict__ mfColor, const int32_t* __restrict__ mfB0,
+    const int32_t* __restrict__ mfB1, const float4* __restrict__ bpos, const float4* __restrict__ bquat,
+    const float4* __restrict__ dvel, const float4* __restrict__ dang, const float4* __restrict__ dcom, int D,
+    const uint64_t* __restrict__ prevKey, const int32_t* __restrict__ prevNum, const float4* __restrict__ prevLp,
+    int prevCount, int prevCap, uint64_t* __restrict__ curKey, int32_t* __restrict__ curNum,
+    float4* __restrict__ curLp, int32_t* __restrict__ sB0, int32_t* __restrict__ sB1, int32_t* __restrict__ sA0,
+    int32_t* __restrict__ sA1, int32_t* __restrict__ sFlags, int32_t* __restrict__ sCanon, float4* __restrict__ sN,
+    float2* __restrict__ sSoft, float4* __restrict__ sR0, float4* __restrict__ sR1) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    int m = perm[s];
+    int p = mfPair[m];
+    uint64_t key = mfKey[m];
+    int c0, c1, wd;
+    pair_colliders(key, localBits, worldColStart, colOfLocal, c0, c1, wd);
+    int a0 = cbody[c0], a1 = cbody[c1];
+    int b0 = mfB0[m], b1 = mfB1[m];
+    float4 m0 = cmat[c0], m1 = cmat[c1];   // friction, restitution, damping
+    float friction = (m0.x + m1.x) * 0.5f;
+    bool soft;
+    float freq, damp, rest;
+    if (m0.z != 0.f || m1.z != 0.f) {
+        soft = true;
+        if (m0.z != 0.f && m1.z != 0.f) { freq = gmin(m0.y, m1.y); damp = gmin(m0.z, m1.z); }
+        else if (m0.z != 0.f) { freq = m0.y; damp = m0.z; }
+        else { freq = m1.y; damp = m1.z; }
+        rest = 0.f;
+    } else {
+        soft = false; freq = 0.f; damp = 0.f;
+        rest = (m0.y + m1.y) * 0.5f;
+    }
+    f3 n = xyz(npNormal[p]);
+    int num = npNum[p];
+    qf q0 = q4(bquat[a0]), q1 = q4(bquat[a1]);
+    f3 com0 = s3(0.f), v0 = s3(0.f), w0 = s3(0.f), com1 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
+    if (b0 >= 0) { com0 = add(xyz(bpos[a0]), qrot(q0, xyz(dcom[b0]))); v0 = xyz(dvel[b0]); w0 = xyz(dang[b0]); }
+    if (b1 >= 0) { com1 = add(xyz(bpos[a1]), qrot(q1, xyz(dcom[b1]))); v1 = xyz(dvel[b1]); w1 = xyz(dang[b1]); }
+    // previous step's manifold of this (pair, triangle), src/Physecs.cpp:237
+    int prev = -1;
+    {
+        uint64_t ck = cache_key(key);
+        int lo = 0, hi = prevCount;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (prevKey[mid] < ck) lo = mid + 1; else hi = mid;
+        }
+        if (lo < prevCount && prevKey[lo] == ck) prev = lo;
+    }
+    qf iq0 = qinverse(q0), iq1 = qinverse(q1);
+    for (int k = 0; k < 4; ++k) {
+        float4 r0o = make_float4(0.f, 0.f, 0.f, 0.f), r1o = r0o;
+        if (k < num) {
+            f3 p0 = xyz(npPts[8 * (size_t)p + 2 * k]), p1 = xyz(npPts[8 * (size_t)p + 2 * k + 1]);
+            f3 r0 = sub(p0, com0), r1 = sub(p1, com1);
+            f3 rel = sub(sub(add(v1, cross(w1, r1)), v0), cross(w0, r0));
+            float relN = dot(rel, n);
+            r0 = qrot(iq0, r0);
+            r1 = qrot(iq1, r1);
+            float tv = 0.f;
+            bool found = false;
+            if (prev >= 0) {
+                int pn = prevNum[prev];
+                for (int i = 0; i < pn; ++i) {
+                    float4 lp = prevLp[(size_t)i * prevCap + prev];
+                    f3 d = sub(xyz(lp), r0);
+                    // glm::distance(...) < 0.1 is a double comparison (Physecs.cpp:294)
+                    if ((double)sqrtf(dot(d, d)) < 0.1) { found = true; tv = lp.w; break; }
+                }
+            }
+            if (!found) tv = -rest * relN;
+            r0o = f4(r0, tv);
+            r1o = f4(r1, 0.f);
+            curLp[(size_t)k * Mcap + m] = r0o;
+        }
+        sR0[(size_t)k * Mcap + s] = r0o;
+        sR1[(size_t)k * Mcap + s] = r1o;
+    }
+    curKey[m] = cache_key(key);
+    curNum[m] = num;
+    sB0[s] = b0;
+    sB1[s] = b1;
+    sA0[s] = a0;
+    sA1[s] = a1;
+    sFlags[s] = num | (soft ? 256 : 0) | (mfColor[m] << 16);
+    sCanon[s] = m;
+    sN[s] = f4(n, friction);
+    sSoft[s] = make_float2(freq, damp);
+}
+
+// ------------------------------------------------------------------ substep kernels
+// per-point row storage: field f of point k of manifold s at ((k * 9 + f) * Mcap + s)
+enum { R_R0XN = 0, R_R1XN, R_T, R_R0XT, R_R1XT, R_R0XNT, R_R1XNT, R_R0XTT, R_R1XTT, R_NF };
+#define ROW(f, k) sRow[((size_t)((k) * R_NF + (f))) * Mcap + s]
+
+__global__ __launch_bounds__(256) void k_update_contacts(
+    int M, int Mcap, const int32_t* __restrict__ sB0, const int32_t* __restrict__ sB1,
+    const int32_t* __restrict__ sA0, const int32_t* __restrict__ sA1, const int32_t* __restrict__ sFlags,
+    const float4* __restrict__ sN, const float4* __restrict__ sR0, const float4* __restrict__ sR1,
+    const float4* __restrict__ bpos, const float4* __restrict__ bquat, const float4* __restrict__ dvel,
+    const float4* __restrict__ dang, const float4* __restrict__ dcom, float4* __restrict__ sRow) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    int b0 = sB0[s], b1 = sB1[s], a0 = sA0[s], a1 = sA1[s];
+    int num = sFlags[s] & 255;
+    f3 n = xyz(sN[s]);
+    qf q0 = q4(bquat[a0]), q1 = q4(bquat[a1]);
+    f3 com0 = s3(0.f), v0 = s3(0.f), w0 = s3(0.f), com1 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
+    if (b0 >= 0) { com0 = add(xyz(bpos[a0]), qrot(q0, xyz(dcom[b0]))); v0 = xyz(dvel[b0]); w0 = xyz(dang[b0]); }
+    if (b1 >= 0) { com1 = add(xyz(bpos[a1]), qrot(q1, xyz(dcom[b1]))); v1 = xyz(dvel[b1]); w1 = xyz(dang[b1]); }
+    for (int k = 0; k < num; ++k) {
+        f3 r0 = qrot(q0, xyz(sR0[(size_t)k * Mcap + s]));
+        f3 r1 = qrot(q1, xyz(sR1[(size_t)k * Mcap + s]));
+        f3 cp0 = add(com0, r0), cp1 = add(com1, r1);
+        float cn = dot(sub(cp1, cp0), n);
+        f3 r0xn = cross(r0, n), r1xn = cross(r1, n);
+        f3 rel = sub(sub(add(v1, cross(w1, r1)), v0), cross(w0, r0));
+        float relN = dot(rel, n);
+        f3 t = sub(rel, mul(relN, n));
+        float tl = length(t);
+        if (tl != 0.f) t = divs(t, tl);
+        ROW(R_R0XN, k) = f4(r0xn, cn);
+        ROW(R_R1XN, k) = f4(r1xn, 0.f);
+        ROW(R_T, k) = f4(t, 0.f);
+        ROW(R_R0XT, k) = f4(cross(r0, t), 0.f);     // .w = lambdaN
+        ROW(R_R1XT, k) = f4(cross(r1, t), 0.f);     // .w = lambdaT
+    }
+}
+
+__device__ __forceinline__ f3 solve33(const m3& A, f3 b) {
+    f3 c12 = cross(A.c1, A.c2);
+    float det = dot(A.c0, c12);
+    if (det == 0.f) return s3(0.f);
+    float id = 1.f / det;
+    return mk3(id * dot(b, c12), id * dot(A.c0, cross(b, A.c2)), id * dot(A.c0, cross(A.c1, b)));
+}
+
+__global__ __launch_bounds__(256) void k_integrate_velocities(int D, float h, float g, const float4* __restrict__ bquat,
+                                                              float4* __restrict__ dvel, float4* __restrict__ dang,
+                                                              const float4* __restrict__ dcom,
+                                                              const float4* __restrict__ dinvI,
+                                                              float4* __restrict__ tvel, float4* __restrict__ tang,
+                                                              float4* __restrict__ pvel, float4* __restrict__ pang,
+                                                              float4* __restrict__ winvI) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    if (dcom[d].w != 0.f) return;   // kinematic
+    m3 rot = mat3_cast(q4(bquat[d]));
+    m3 invRot = transpose(rot);
+    float4 vv = dvel[d];
+    f3 v = add(xyz(vv), mul(h, mk3(0.f, -g, 0.f)));
+    m3 invI = mk_m3(xyz(dinvI[3 * d]), xyz(dinvI[3 * d + 1]), xyz(dinvI[3 * d + 2]));
+    f3 wl = mulmv(invRot, xyz(dang[d]));
+    m3 I = inverse(invI);
+    f3 Iw = mulmv(I, wl);
+    f3 f = mul(h, cross(wl, Iw));
+    m3 J = madd(I, mscale(h, msub(mulmm(matrixCross3(wl), I), matrixCross3(Iw))));
+    wl = sub(wl, solve33(J, f));
+    f3 w = mulmv(rot, wl);
+    dvel[d] = f4(v, vv.w);
+    dang[d] = f4(w, 0.f);
+    tvel[d] = f4(v, vv.w);
+    tang[d] = f4(w, 0.f);
+    pvel[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+    pang[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+    m3 wI = mulmm(mulmm(rot, invI), invRot);
+    winvI[3 * d] = f4(wI.c0, 0.f);
+    winvI[3 * d + 1] = f4(wI.c1, 0.f);
+    winvI[3 * d + 2] = f4(wI.c2, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_presolve(int M, int Mcap, const int32_t* __restrict__ sB0,
+                                                  const int32_t* __restrict__ sB1, const int32_t* __restrict__ sFlags,
+                                                  const float4* __restrict__ sN, const float4* __restrict__ dvel,
+                                                  const float4* __restrict__ winvI, float4* __restrict__ sRow) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    int b0 = sB0[s], b1 = sB1[s];
+    int num = sFlags[s] & 255;
+    f3 n = xyz(sN[s]);
+    float im0 = 0.f, im1 = 0.f;
+    m3 I0 = mzero(), I1 = mzero();
+    if (b0 >= 0) { im0 = dvel[b0].w; I0 = mk_m3(xyz(winvI[3 * b0]), xyz(winvI[3 * b0 + 1]), xyz(winvI[3 * b0 + 2])); }
+    if (b1 >= 0) { im1 = dvel[b1].w; I1 = mk_m3(xyz(winvI[3 * b1]), xyz(winvI[3 * b1 + 1]), xyz(winvI[3 * b1 + 2])); }
+    float nn = dot(n, n);
+    for (int k = 0; k < num; ++k) {
+        float4 a = ROW(R_R0XN, k), b = ROW(R_R1XN, k), t = ROW(R_T, k), c = ROW(R_R0XT, k), e = ROW(R_R1XT, k);
+        f3 r0xn = xyz(a), r1xn = xyz(b), tt = xyz(t), r0xt = xyz(c), r1xt = xyz(e);
+        f3 r0xnt = mulmv(I0, r0xn), r1xnt = mulmv(I1, r1xn), r0xtt = mulmv(I0, r0xt), r1xtt = mulmv(I1, r1xt);
+        float invN = nn * (im0 + im1) + dot(r0xn, r0xnt) + dot(r1xn, r1xnt);
+        float invT = dot(tt, tt) * (im0 + im1) + dot(r0xt, r0xtt) + dot(r1xt, r1xtt);
+        ROW(R_R1XN, k) = f4(r1xn, invN);
+        ROW(R_T, k) = f4(tt, invT);
+        ROW(R_R0XNT, k) = f4(r0xnt, 0.f);
+        ROW(R_R1XNT, k) = f4(r1xnt, 0.f);
+        ROW(R_R0XTT, k) = f4(r0xtt, 0.f);
+        ROW(R_R1XTT, k) = f4(r1xtt, 0.f);
+    }
+}
+
+// One colour: every manifold of [begin, end) touches bodies no other manifold of the
+// range touches, so the range is solved in parallel exactly as a sequential pass would.
+__global__ __launch_bounds__(256) void k_solve(int begin, int end, int Mcap, int useBias, float h, int skipSoft,
+                                               const int32_t* __restrict__ sB0, const int32_t* __restrict__ sB1,
+                                               const int32_t* __restrict__ sFlags, const float4* __restrict__ sN,
+                                               const float2* __restrict__ sSoft, const float4* __restrict__ sR0,
+                                               const float4* __restrict__ dvel, const float4* __restrict__ dang,
+                                               float4* __restrict__ tvel, float4* __restrict__ tang,
+                                               float4* __restrict__ sRow) {
+    int s = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= end) return;
+    int flags = sFlags[s];
+    bool soft = (flags & 256) != 0;
+    if (skipSoft && soft) return;
+    int num = flags & 255;
+    int b0 = sB0[s], b1 = sB1[s];
+    float4 nf = sN[s];
+    f3 n = xyz(nf);
+    float friction = nf.w;
+    float im0 = b0 >= 0 ? dvel[b0].w : 0.f, im1 = b1 >= 0 ? dvel[b1].w : 0.f;
+    f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
+    if (b0 >= 0) { v0 = xyz(tvel[b0]); w0 = xyz(tang[b0]); }
+    if (b1 >= 0) { v1 = xyz(tvel[b1]); w1 = xyz(tang[b1]); }
+    float2 sp = sSoft[s];
+    float lN[4], lT[4];
+    // normal rows, src/ContactConstraints.cpp:84-133
+    for (int k = 0; k < num; ++k) {
+        float4 a = ROW(R_R0XN, k), b = ROW(R_R1XN, k);
+        lN[k] = ROW(R_R0XT, k).w;
+        float invN = b.w;
+        if (invN == 0.f) continue;
+        f3 r0xn = xyz(a), r1xn = xyz(b);
+        float c = a.w;
+        float rv = dot(neg(n), v0) + dot(neg(r0xn), w0) + dot(n, v1) + dot(r1xn, w1);
+        float effMass = 1.f / invN;
+        float lambda;
+        if (soft) {
+            float af = 2.f * 3.14159265358979323846f * sp.x;
+            float stiffness = af * af * effMass;
+            float damping = 2.f * af * sp.y * effMass;
+            float gamma = 1.f / (damping + h * stiffness);
+            float beta = h * stiffness / (damping + h * stiffness);
+            lambda = (rv + beta * c / h) / (invN + gamma / h);
+        } else {
+            float tv = sR0[(size_t)k * Mcap + s].w;
+            lambda = (rv - tv + (useBias ? 0.1f * c / h : 0.f)) * effMass;
+        }
+        float prev = lN[k];
+        lN[k] = lN[k] + lambda;
+        lN[k] = gmin(lN[k], 0.f);
+        lambda = lN[k] - prev;
+        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, n)); w0 = add(w0, mul(lambda, xyz(ROW(R_R0XNT, k)))); }
+        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, n)); w1 = sub(w1, mul(lambda, xyz(ROW(R_R1XNT, k)))); }
+    }
+    // friction rows read the component velocities, src/ContactConstraints.cpp:136-173
+    f3 cv0 = s3(0.f), cw0 = s3(0.f), cv1 = s3(0.f), cw1 = s3(0.f);
+    if (b0 >= 0) { cv0 = xyz(dvel[b0]); cw0 = xyz(dang[b0]); }
+    if (b1 >= 0) { cv1 = xyz(dvel[b1]); cw1 = xyz(dang[b1]); }
+    for (int k = 0; k < num; ++k) {
+        float4 t4 = ROW(R_T, k);
+        lT[k] = ROW(R_R1XT, k).w;
+        float invT = t4.w;
+        if (invT == 0.f) continue;
+        f3 t = xyz(t4), r0xt = xyz(ROW(R_R0XT, k)), r1xt = xyz(ROW(R_R1XT, k));
+        float rv = dot(neg(t), cv0) + dot(neg(r0xt), cw0) + dot(t, cv1) + dot(r1xt, cw1);
+        float lambda = rv / invT;
+        float limit = friction * lN[k];
+        float prev = lT[k];
+        lT[k] = lT[k] + lambda;
+        lT[k] = gclamp(lT[k], limit, -limit);
+        lambda = lT[k] - prev;
+        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, t)); w0 = add(w0, mul(lambda, xyz(ROW(R_R0XTT, k)))); }
+        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, t)); w1 = sub(w1, mul(lambda, xyz(ROW(R_R1XTT, k)))); }
+    }
+    for (int k = 0; k < num; ++k) {
+        float4 c = ROW(R_R0XT, k), e = ROW(R_R1XT, k);
+        c.w = lN[k];
+        e.w = lT[k];
+        ROW(R_R0XT, k) = c;
+        ROW(R_R1XT, k) = e;
+    }
+    if (b0 >= 0) { tvel[b0] = f4(v0, 0.f); tang[b0] = f4(w0, 0.f); }
+    if (b1 >= 0) { tvel[b1] = f4(v1, 0.f); tang[b1] = f4(w1, 0.f); }
+}
+
+__global__ __launch_bounds__(256) void k_integrate_positions(int D, float h, float4* __restrict__ bpos,
+                                                             float4* __restrict__ bquat,
+                                                             const float4* __restrict__ dcom,
+                                                             const float4* __restrict__ tvel,
+                                                             const float4* __restrict__ tang,
+                                                             const float4* __restrict__ pvel,
+                                                             const float4* __restrict__ pang) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    float4 cm = dcom[d];
+    if (cm.w != 0.f) return;
+    float4 pa = pang[d];
+    float cnt = pa.w;
+    float sc = cnt != 0.f ? 1.f / cnt : 1.f;
+    f3 com = xyz(cm);
+    f3 pos = xyz(bpos[d]);
+    qf q = q4(bquat[d]);
+    pos = add(pos, add(mul(h, xyz(tvel[d])), mul(sc, xyz(pvel[d]))));
+    f3 prevCom = qrot(q, com);
+    f3 hv = mul(0.5f, add(mul(h, xyz(tang[d])), mul(sc, xyz(pa))));
+    qf dq = qmul(mkq(0.f, hv.x, hv.y, hv.z), q);
+    q = mkq(q.w + dq.w, q.x + dq.x, q.y + dq.y, q.z + dq.z);
+    q = qnormalize(q);
+    pos = add(pos, sub(prevCom, qrot(q, com)));
+    bpos[d] = f4(pos, 0.f);
+    bquat[d] = f4q(q);
+}
+
+__global__ void k_writeback(int D, float4* __restrict__ dvel, float4* __restrict__ dang,
+                            const float4* __restrict__ dcom, const float4* __restrict__ tvel,
+                            const float4* __restrict__ tang) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    if (dcom[d].w != 0.f) return;
+    float4 v = tvel[d];
+    dvel[d] = make_float4(v.x, v.y, v.z, dvel[d].w);
+    dang[d] = tang[d];
+}
+
+// ------------------------------------------------------------------ host orchestration
+int sv_setup_manifolds(World& w, int nPairs) {
+    hipStream_t s = w.stream;
+    if (nPairs == 0) return 0;
+    // manifold index of every pair with points: exclusive scan of the hit flags (the pair
+    // sort's value buffer is free at this point and holds the flags)
+    hipLaunchKernelGGL(k_flag_manifolds, dim3(div_up(nPairs, 256)), dim3(256), 0, s, nPairs, w.npNum.p, w.pvals.p);
+    exclusive_scan_i32(s, w.pvals.p, w.npIndex.p, nPairs, w.counters.p + 1, w.scanT);
+    int M = 0;
+    PGX_HIP(hipMemcpyAsync(&M, w.counters.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_compact, dim3(div_up(nPairs, 256)), dim3(256), 0, s, nPairs, w.npNum.p, w.npIndex.p,
+                       w.pkeys.p, w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.cflags.p,
+                       w.mfPair.p, w.mfKey.p, w.mfB0.p, w.mfB1.p);
+    PGX_LAUNCH_CHECK();
+    PGX_HIP(hipStreamSynchronize(s));
+    return M;
+}
+
+int sv_color(World& w, int M) {
+    hipStream_t s = w.stream;
+    w.lastRounds = 0;
+    if (M == 0) {
+        w.colorStartHost.assign(1, 0);
+        return 0;
+    }
+    int worldShift = 2 * w.localBits;
+    int nb = div_up(M, 256);
+    hipLaunchKernelGGL(k_world_mstart, dim3(nb), dim3(256), 0, s, M, w.mfKey.p, worldShift, w.worldMStart.p);
+    fill_i32(s, w.mfColor.p, -1, M);
+    fill_u64(s, w.claim.p, 0ull, w.D);
+    fill_u64(s, w.usedMask.p, 0ull, 2 * w.D);
+    fill_i32(s, w.counters.p + 5, 0, 1);
+    int round = 0;
+    const int batch = 8;
+    while (true) {
+        for (int r = 0; r < batch; ++r) {
+            ++round;
+            fill_i32(s, w.counters.p + 3, 0, 1);
+            hipLaunchKernelGGL(k_color_claim, dim3(nb), dim3(256), 0, s, M, round, w.mfKey.p, worldShift,
+                               w.worldMStart.p, w.mfColor.p, w.mfB0.p, w.mfB1.p,
+                               (unsigned long long*)w.claim.p);
+            hipLaunchKernelGGL(k_color_pick, dim3(nb), dim3(256), 0, s, M, round, w.mfKey.p, worldShift,
+                               w.worldMStart.p, w.mfColor.p, w.mfB0.p, w.mfB1.p,
+                               (const unsigned long long*)w.claim.p, w.usedMask.p, w.counters.p);
+        }
+        PGX_LAUNCH_CHECK();
+        int rem[3] = {0, 0, 0};
+        PGX_HIP(hipMemcpyAsync(rem, w.counters.p + 3, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipStreamSynchronize(s));
+        if (rem[2]) throw PgxError{PGX_ERR_CAPACITY, "a body touches more than 127 contact colours"};
+        if (rem[0] == 0) break;
+        if (round > 4096) throw PgxError{PGX_ERR_INVALID, "graph colouring did not converge"};
+    }
+    // stable sort by colour -> solve order
+    hipLaunchKernelGGL(k_color_keys, dim3(nb), dim3(256), 0, s, M, w.mfColor.p, w.colorKeys.p, w.colorPerm.p);
+    radix_sort_u64(s, w.colorKeys.p, w.colorPerm.p, M, 8, w.sortT);
+    hipLaunchKernelGGL(k_color_start, dim3(nb), dim3(256), 0, s, M, w.colorKeys.p, w.colorCount.p);
+    PGX_LAUNCH_CHECK();
+    uint64_t lastKey = 0;
+    PGX_HIP(hipMemcpyAsync(&lastKey, w.colorKeys.p + (M - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    PGX_HIP(hipStreamSynchronize(s));
+    int nColors = (int)lastKey + 1;
+    w.colorStartHost.resize(nColors + 1);
+    PGX_HIP(hipMemcpyAsync(w.colorStartHost.data(), w.colorCount.p, sizeof(int) * (nColors + 1),
+                           hipMemcpyDeviceToHost, s));
+    PGX_HIP(hipStreamSynchronize(s));
+    w.lastRounds = round;
+    // contact setup, written in solve order
+    int cur = w.cacheCur ^ 1, prv = w.cacheCur;
+    hipLaunchKernelGGL(k_setup, dim3(nb), dim3(256), 0, s, M, w.manifoldCap, w.colorPerm.p, w.mfPair.p, w.mfKey.p,
+                       w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.cflags.p, w.cmat.p, w.npNum.p,
+                       w.npNormal.p, w.npPts.p, w.mfColor.p, w.mfB0.p, w.mfB1.p, w.bpos.p, w.bquat.p, w.dvel.p,
+                       w.dang.p, w.dcom.p, w.D, w.cacheKey[prv].p, w.cacheNum[prv].p, w.cacheLp[prv].p,
+                       w.cacheCount[prv], w.manifoldCap, w.cacheKey[cur].p, w.cacheNum[cur].p, w.cacheLp[cur].p,
+                       w.sB0.p, w.sB1.p, w.sA0.p, w.sA1.p, w.sFlags.p, w.sCanon.p, w.sN.p, w.sSoft.p, w.sR0.p,
+                       w.sR1.p);
+    PGX_LAUNCH_CHECK();
+    return nColors;
+}
+
+void sv_substeps(World& w, int M, float dt) {
+    hipStream_t s = w.stream;
+    int D = w.D;
+    int Mcap = w.manifoldCap;
+    float h = dt / w.substeps;
+    int nbM = div_up(std::max(M, 1), 256), nbD = div_up(std::max(D, 1), 256);
+    int nColors = (int)w.colorStartHost.size() - 1;
+    for (int sub = 0; sub < w.substeps; ++sub) {
+        if (M)
+            hipLaunchKernelGGL(k_update_contacts, dim3(nbM), dim3(256), 0, s, M, Mcap, w.sB0.p, w.sB1.p, w.sA0.p,
+                               w.sA1.p, w.sFlags.p, w.sN.p, w.sR0.p, w.sR1.p, w.bpos.p, w.bquat.p, w.dvel.p,
+                               w.dang.p, w.dcom.p, w.sRow.p);
+        if (D)
+            hipLaunchKernelGGL(k_integrate_velocities, dim3(nbD), dim3(256), 0, s, D, h, w.g, w.bquat.p, w.dvel.p,
+                               w.dang.p, w.dcom.p, w.dinvI.p, w.tvel.p, w.tang.p, w.pvel.p, w.pang.p, w.winvI.p);
+        if (M)
+            hipLaunchKernelGGL(k_presolve, dim3(nbM), dim3(256), 0, s, M, Mcap, w.sB0.p, w.sB1.p, w.sFlags.p, w.sN.p,
+                               w.dvel.p, w.winvI.p, w.sRow.p);
+        for (int it = 0; it < w.iterations; ++it) {
+            for (int c = 0; c < nColors; ++c) {
+                int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
+                if (e <= b) continue;
+                hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 256)), dim3(256), 0, s, b, e, Mcap, 1, h, 0, w.sB0.p,
+                                   w.sB1.p, w.sFlags.p, w.sN.p, w.sSoft.p, w.sR0.p, w.dvel.p, w.dang.p, w.tvel.p,
+                                   w.tang.p, w.sRow.p);
+            }
+        }
+        if (D)
+            hipLaunchKernelGGL(k_integrate_positions, dim3(nbD), dim3(256), 0, s, D, h, w.bpos.p, w.bquat.p, w.dcom.p,
+                               w.tvel.p, w.tang.p, w.pvel.p, w.pang.p);
+        for (int c = 0; c < nColors; ++c) {
+            int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
+            if (e <= b) continue;
+            hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 256)), dim3(256), 0, s, b, e, Mcap, 0, 0.f, 1, w.sB0.p,
+                               w.sB1.p, w.sFlags.p, w.sN.p, w.sSoft.p, w.sR0.p, w.dvel.p, w.dang.p, w.tvel.p,
+                               w.tang.p, w.sRow.p);
+        }
+        if (D)
+            hipLaunchKernelGGL(k_writeback, dim3(nbD), dim3(256), 0, s, D, w.dvel.p, w.dang.p, w.dcom.p, w.tvel.p,
+                               w.tang.p);
+        PGX_LAUNCH_CHECK();
+    }
+}
+
+void sv_finish_step(World& w, int M) {
+    // contact cache of this step becomes the previous one (Physecs.cpp:553)
+    w.cacheCur ^= 1;
+    w.cacheCount[w.cacheCur] = M;
+    bp_compute_bounds(w, true);
+}
+
+}  // namespace pgx

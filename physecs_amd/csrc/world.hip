@@ -1,0 +1,518 @@
+// C ABI of libphysecs_gpu.so (include/physecs_gpu.h): world lifetime, upload/download of
+// the component state, and the per-step orchestration of Scene::simulate
+// (src/Physecs.cpp:112-561) on one HIP stream.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include "pgx_internal.h"
+
+using namespace pgx;
+
+struct pgx_world {
+    World w;
+};
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+template <class F>
+static int guarded(F&& f) {
+    try {
+        return f();
+    } catch (const HipError& e) {
+        return fail(PGX_ERR_HIP, "HIP error %d (%s) at %s:%d: %s", (int)e.err, hipGetErrorString(e.err), e.file,
+                    e.line, e.expr);
+    } catch (const PgxError& e) {
+        return fail(e.code, "%s", e.msg.c_str());
+    } catch (const std::bad_alloc&) {
+        return fail(PGX_ERR_INVALID, "host out of memory");
+    } catch (...) {
+        return fail(PGX_ERR_INVALID, "unknown error");
+    }
+}
+
+template <class T>
+static void upload(DBuf<T>& b, const std::vector<T>& v, hipStream_t s) {
+    b.alloc(v.size());
+    if (!v.empty()) PGX_HIP(hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, s));
+}
+
+static int bits_for(long n) {
+    int b = 1;
+    while ((1L << b) < n) ++b;
+    return b;
+}
+
+extern "C" {
+
+const char* pgx_last_error(void) { return g_err.c_str(); }
+
+int pgx_device_count(int* count) {
+    if (!count) return fail(PGX_ERR_INVALID, "count is NULL");
+    return guarded([&] {
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        *count = e == hipSuccess ? n : 0;
+        return PGX_OK;
+    });
+}
+
+int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world** out) {
+    if (!sc || !out) return fail(PGX_ERR_INVALID, "scene or out is NULL");
+    *out = nullptr;
+    if (sc->n_actors < 0 || sc->n_dyn < 0 || sc->n_colliders < 0)
+        return fail(PGX_ERR_INVALID, "negative counts");
+    // validate the descriptor on the host before anything reaches the device
+    for (int d = 0; d < sc->n_dyn; ++d) {
+        int a = sc->actor_of_dyn[d];
+        if (a < 0 || a >= sc->n_actors || sc->dyn_of_actor[a] != d)
+            return fail(PGX_ERR_INVALID, "dynamic entry %d: actor_of_dyn / dyn_of_actor mismatch", d);
+    }
+    int W = sc->actor_world ? std::max(1, sc->n_worlds) : 1;
+    for (int a = 0; a < sc->n_actors; ++a) {
+        int wd = sc->actor_world ? sc->actor_world[a] : 0;
+        if (wd < 0 || wd >= W) return fail(PGX_ERR_INVALID, "actor %d: world %d out of range", a, wd);
+        int d = sc->dyn_of_actor[a];
+        if (d < -1 || d >= sc->n_dyn) return fail(PGX_ERR_INVALID, "actor %d: dyn index %d out of range", a, d);
+    }
+    for (int c = 0; c < sc->n_colliders; ++c) {
+        int a = sc->col_actor[c];
+        if (a < 0 || a >= sc->n_actors) return fail(PGX_ERR_INVALID, "collider %d: actor out of range", c);
+        int t = sc->col_type[c];
+        if (t != PGX_SPHERE && t != PGX_CAPSULE && t != PGX_BOX)
+            return fail(PGX_ERR_UNSUPPORTED, "collider %d: geometry type %d is not supported on the device path", c, t);
+    }
+    return guarded([&] {
+        auto* h = new pgx_world();
+        World& w = h->w;
+        try {
+            w.device = cfg ? cfg->device : 0;
+            PGX_HIP(hipSetDevice(w.device));
+            PGX_HIP(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
+            if (cfg) {
+                if (cfg->num_substeps > 0) w.substeps = cfg->num_substeps;
+                if (cfg->num_iterations >= 0) w.iterations = cfg->num_iterations;
+                if (cfg->gravity != 0.f) w.g = cfg->gravity;
+            }
+            const int A = sc->n_actors, D = sc->n_dyn, C = sc->n_colliders;
+            w.A = A; w.D = D; w.C = C; w.W = W;
+            // internal body order: dynamic storage first (body b == dynamic index b), then the rest
+            w.actorToBody.assign(A, -1);
+            w.bodyToActor.clear();
+            for (int d = 0; d < D; ++d) { w.actorToBody[sc->actor_of_dyn[d]] = d; w.bodyToActor.push_back(sc->actor_of_dyn[d]); }
+            for (int a = 0; a < A; ++a)
+                if (sc->dyn_of_actor[a] < 0) { w.actorToBody[a] = (int)w.bodyToActor.size(); w.bodyToActor.push_back(a); }
+            std::vector<float4> bpos(A), bquat(A);
+            std::vector<uint32_t> bent(A);
+            for (int b = 0; b < A; ++b) {
+                int a = w.bodyToActor[b];
+                bpos[b] = make_float4(sc->pos[3 * a], sc->pos[3 * a + 1], sc->pos[3 * a + 2], 0.f);
+                bquat[b] = make_float4(sc->quat[4 * a], sc->quat[4 * a + 1], sc->quat[4 * a + 2], sc->quat[4 * a + 3]);
+                bent[b] = sc->entity[a];
+            }
+            std::vector<float4> dvel(D), dang(D), dcom(D), dinvI(3 * (size_t)D);
+            for (int d = 0; d < D; ++d) {
+                dvel[d] = make_float4(sc->vel[3 * d], sc->vel[3 * d + 1], sc->vel[3 * d + 2], sc->inv_mass[d]);
+                dang[d] = make_float4(sc->angvel[3 * d], sc->angvel[3 * d + 1], sc->angvel[3 * d + 2], 0.f);
+                dcom[d] = make_float4(sc->com[3 * d], sc->com[3 * d + 1], sc->com[3 * d + 2], sc->kinematic[d] ? 1.f : 0.f);
+                for (int k = 0; k < 3; ++k)
+                    dinvI[3 * d + k] = make_float4(sc->inv_inertia[9 * d + 3 * k], sc->inv_inertia[9 * d + 3 * k + 1],
+                                                   sc->inv_inertia[9 * d + 3 * k + 2], 0.f);
+            }
+            // colliders (descriptor order), world-local indices
+            std::vector<int32_t> cbody(C), cworld(C), clocal(C), cflags(C), ctype(C), cindex(C);
+            std::vector<uint32_t> cent(C);
+            std::vector<float4> cparams(C), clpos(C), clquat(C), cmat(C);
+            std::vector<int32_t> perWorld(W, 0);
+            for (int c = 0; c < C; ++c) {
+                int a = sc->col_actor[c];
+                int wd = sc->actor_world ? sc->actor_world[a] : 0;
+                cbody[c] = w.actorToBody[a];
+                cworld[c] = wd;
+                clocal[c] = perWorld[wd]++;
+                int d = sc->dyn_of_actor[a];
+                bool dyn = d >= 0 && !sc->kinematic[d];
+                cflags[c] = (sc->col_enable[c] ? CF_ENABLE : 0) | (sc->col_trigger[c] ? CF_TRIGGER : 0) | (dyn ? CF_DYNAMIC : 0);
+                ctype[c] = sc->col_type[c];
+                cindex[c] = sc->col_index[c];
+                cent[c] = sc->entity[a];
+                cparams[c] = make_float4(sc->col_params[4 * c], sc->col_params[4 * c + 1], sc->col_params[4 * c + 2],
+                                         sc->col_params[4 * c + 3]);
+                clpos[c] = make_float4(sc->col_pos[3 * c], sc->col_pos[3 * c + 1], sc->col_pos[3 * c + 2], 0.f);
+                clquat[c] = make_float4(sc->col_quat[4 * c], sc->col_quat[4 * c + 1], sc->col_quat[4 * c + 2],
+                                        sc->col_quat[4 * c + 3]);
+                cmat[c] = make_float4(sc->col_material[3 * c], sc->col_material[3 * c + 1], sc->col_material[3 * c + 2], 0.f);
+            }
+            std::vector<int32_t> wstart(W + 1, 0), colOfLocal(C);
+            for (int i = 0; i < W; ++i) wstart[i + 1] = wstart[i] + perWorld[i];
+            for (int c = 0; c < C; ++c) colOfLocal[wstart[cworld[c]] + clocal[c]] = c;
+            int maxPerWorld = W ? *std::max_element(perWorld.begin(), perWorld.end()) : 1;
+            w.localBits = bits_for(std::max(2, maxPerWorld));
+            w.worldBits = W > 1 ? bits_for(W) : 0;
+            if (w.worldBits + 30 > 64 || w.worldBits + 2 * w.localBits > 64)
+                throw PgxError{PGX_ERR_INVALID, "too many worlds or colliders per world for 64-bit keys"};
+            w.hostColBody = cbody;
+            w.hostColIndex = cindex;
+            w.hostEntity = bent;
+
+            hipStream_t s = w.stream;
+            upload(w.bpos, bpos, s); upload(w.bquat, bquat, s); upload(w.bentity, bent, s);
+            upload(w.dvel, dvel, s); upload(w.dang, dang, s); upload(w.dcom, dcom, s); upload(w.dinvI, dinvI, s);
+            upload(w.cbody, cbody, s); upload(w.cworld, cworld, s); upload(w.clocal, clocal, s);
+            upload(w.cflags, cflags, s); upload(w.ctype, ctype, s); upload(w.cindex, cindex, s);
+            upload(w.centity, cent, s); upload(w.cparams, cparams, s); upload(w.clpos, clpos, s);
+            upload(w.clquat, clquat, s); upload(w.cmat, cmat, s);
+            upload(w.worldColStart, wstart, s); upload(w.colOfLocal, colOfLocal, s);
+            w.tvel.alloc(D); w.tang.alloc(D); w.pvel.alloc(D); w.pang.alloc(D); w.winvI.alloc(3 * (size_t)D);
+            w.cbmin.alloc(C); w.cbmax.alloc(C);
+            // broadphase buffers
+            w.worldBoundsI.alloc(6 * (size_t)W);
+            w.mkeys.alloc(C); w.mvals.alloc(C);
+            w.leafMin.alloc(C); w.leafMax.alloc(C); w.leafFlags.alloc(C); w.leafParent.alloc(C); w.leafEsc.alloc(C);
+            size_t ni = C > 1 ? C - 1 : 1;
+            w.nodeChild.alloc(2 * ni); w.nodeParent.alloc(ni); w.nodeEsc.alloc(ni); w.nodeFlags.alloc(ni);
+            w.nodeCounter.alloc(ni); w.nodeRange.alloc(ni); w.nodeMin.alloc(ni); w.nodeMax.alloc(ni);
+            w.worldRoot.alloc(W); w.worldStop.alloc(W); w.worldMStart.alloc(W);
+            w.pairCap = (cfg && cfg->pair_capacity > 0) ? cfg->pair_capacity : std::max(1024, 12 * C);
+            w.manifoldCap = w.pairCap;
+            w.pkeys.alloc(w.pairCap); w.pvals.alloc(w.pairCap);
+            w.counters.alloc(16);
+            w.npNum.alloc(w.pairCap); w.npNormal.alloc(w.pairCap); w.npPts.alloc(8 * (size_t)w.pairCap);
+            w.npIndex.alloc(w.pairCap);
+            size_t M = w.manifoldCap;
+            w.mfKey.alloc(M); w.mfPair.alloc(M); w.mfB0.alloc(M); w.mfB1.alloc(M); w.mfColor.alloc(M);
+            w.claim.alloc(std::max(D, 1)); w.usedMask.alloc(2 * (size_t)std::max(D, 1));
+            w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorCount.alloc(160);
+            w.sB0.alloc(M); w.sB1.alloc(M); w.sFlags.alloc(M); w.sA0.alloc(M); w.sA1.alloc(M); w.sCanon.alloc(M);
+            w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(36 * M);
+            for (int k = 0; k < 2; ++k) { w.cacheKey[k].alloc(M); w.cacheNum[k].alloc(M); w.cacheLp[k].alloc(4 * M); }
+            w.nonColliding.alloc(1);
+            // creation-time bounds: no margin (src/Physecs.cpp:32)
+            bp_compute_bounds(w, false);
+            PGX_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+        return PGX_OK;
+    });
+}
+
+static void release_world(World& w) {
+    DBuf<float4>* f4s[] = {&w.bpos, &w.bquat, &w.dvel, &w.dang, &w.dcom, &w.dinvI, &w.tvel, &w.tang, &w.pvel,
+                           &w.pang, &w.winvI, &w.cparams, &w.clpos, &w.clquat, &w.cmat, &w.cbmin, &w.cbmax,
+                           &w.leafMin, &w.leafMax, &w.nodeMin, &w.nodeMax, &w.npNormal, &w.npPts, &w.sN, &w.sR0,
+                           &w.sR1, &w.sRow, &w.cacheLp[0], &w.cacheLp[1]};
+    for (auto* b : f4s) b->release();
+    DBuf<int32_t>* i32s[] = {&w.cbody, &w.cworld, &w.clocal, &w.cflags, &w.ctype, &w.cindex, &w.worldColStart,
+                             &w.colOfLocal, &w.mvals, &w.leafFlags, &w.leafParent, &w.leafEsc, &w.nodeChild,
+                             &w.nodeParent, &w.nodeEsc, &w.nodeFlags, &w.nodeCounter, &w.worldRoot, &w.worldStop,
+                             &w.pvals, &w.counters, &w.npNum, &w.npIndex, &w.mfPair, &w.mfB0, &w.mfB1, &w.mfColor,
+                             &w.colorPerm, &w.colorCount, &w.worldMStart, &w.sB0, &w.sB1, &w.sFlags, &w.sA0,
+                             &w.sA1, &w.sCanon, &w.cacheNum[0], &w.cacheNum[1], &w.sortT.valt, &w.scanT.sums};
+    for (auto* b : i32s) b->release();
+    DBuf<uint64_t>* u64s[] = {&w.mkeys, &w.pkeys, &w.mfKey, &w.claim, &w.usedMask, &w.colorKeys, &w.cacheKey[0],
+                              &w.cacheKey[1], &w.nonColliding, &w.sortT.kalt};
+    for (auto* b : u64s) b->release();
+    w.bentity.release(); w.centity.release(); w.worldBoundsI.release(); w.sortT.hist.release();
+    w.nodeRange.release(); w.sSoft.release();
+    for (auto& e : w.ev)
+        if (e) (void)hipEventDestroy(e);
+    if (w.stream) (void)hipStreamDestroy(w.stream);
+    w.stream = nullptr;
+}
+
+int pgx_world_destroy(pgx_world* h) {
+    if (!h) return PGX_OK;
+    int rc = guarded([&] {
+        PGX_HIP(hipSetDevice(h->w.device));
+        if (h->w.stream) PGX_HIP(hipStreamSynchronize(h->w.stream));
+        return PGX_OK;
+    });
+    release_world(h->w);
+    delete h;
+    return rc;
+}
+
+int pgx_world_set_params(pgx_world* h, int substeps, int iterations, float gravity) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    if (substeps <= 0 || iterations < 0) return fail(PGX_ERR_INVALID, "substeps must be > 0, iterations >= 0");
+    h->w.substeps = substeps;
+    h->w.iterations = iterations;
+    h->w.g = gravity;
+    return PGX_OK;
+}
+
+int pgx_world_set_can_collide(pgx_world* h, uint32_t e0, uint32_t e1, int canCollide) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    uint64_t k = ((uint64_t)std::min(e0, e1) << 32) | std::max(e0, e1);
+    auto& v = h->w.hostNonColliding;
+    auto it = std::lower_bound(v.begin(), v.end(), k);
+    bool present = it != v.end() && *it == k;
+    if (!canCollide && !present) v.insert(it, k);
+    if (canCollide && present) v.erase(it);
+    h->w.nonCollidingDirty = true;
+    return PGX_OK;
+}
+
+int pgx_world_step(pgx_world* h, float dt) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    return guarded([&] {
+        World& w = h->w;
+        PGX_HIP(hipSetDevice(w.device));
+        hipStream_t s = w.stream;
+        if (w.nonCollidingDirty) {
+            w.nonColliding.alloc(std::max<size_t>(1, w.hostNonColliding.size()));
+            if (!w.hostNonColliding.empty())
+                PGX_HIP(hipMemcpyAsync(w.nonColliding.p, w.hostNonColliding.data(),
+                                       sizeof(uint64_t) * w.hostNonColliding.size(), hipMemcpyHostToDevice, s));
+            w.nNonColliding = (int)w.hostNonColliding.size();
+            w.nonCollidingDirty = false;
+        }
+        if (w.timing && !w.ev[0])
+            for (auto& e : w.ev) PGX_HIP(hipEventCreate(&e));
+        if (w.timing) PGX_HIP(hipEventRecord(w.ev[0], s));
+        int nPairs = bp_find_pairs(w);
+        if (w.timing) PGX_HIP(hipEventRecord(w.ev[1], s));
+        np_run(w, nPairs);
+        int M = sv_setup_manifolds(w, nPairs);
+        if (w.timing) PGX_HIP(hipEventRecord(w.ev[2], s));
+        int nColors = sv_color(w, M);
+        if (w.timing) PGX_HIP(hipEventRecord(w.ev[3], s));
+        sv_substeps(w, M, dt);
+        if (w.timing) PGX_HIP(hipEventRecord(w.ev[4], s));
+        sv_finish_step(w, M);
+        if (w.timing) {
+            PGX_HIP(hipEventRecord(w.ev[5], s));
+            PGX_HIP(hipEventSynchronize(w.ev[5]));
+            PGX_HIP(hipEventElapsedTime(&w.tBroad, w.ev[0], w.ev[1]));
+            PGX_HIP(hipEventElapsedTime(&w.tNarrow, w.ev[1], w.ev[2]));
+            PGX_HIP(hipEventElapsedTime(&w.tColor, w.ev[2], w.ev[3]));
+            PGX_HIP(hipEventElapsedTime(&w.tSolve, w.ev[3], w.ev[4]));
+            PGX_HIP(hipEventElapsedTime(&w.tTotal, w.ev[0], w.ev[5]));
+        }
+        w.lastPairs = nPairs;
+        w.lastManifolds = M;
+        w.lastColors = nColors;
+        return PGX_OK;
+    });
+}
+
+int pgx_world_sync(pgx_world* h) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(h->w.device));
+        PGX_HIP(hipStreamSynchronize(h->w.stream));
+        return PGX_OK;
+    });
+}
+
+int pgx_world_read_state(pgx_world* h, float* pos, float* quat, float* vel, float* angvel) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    return guarded([&] {
+        World& w = h->w;
+        PGX_HIP(hipSetDevice(w.device));
+        std::vector<float4> bp(w.A), bq(w.A), dv(w.D), da(w.D);
+        hipStream_t s = w.stream;
+        if (w.A) {
+            PGX_HIP(hipMemcpyAsync(bp.data(), w.bpos.p, sizeof(float4) * w.A, hipMemcpyDeviceToHost, s));
+            PGX_HIP(hipMemcpyAsync(bq.data(), w.bquat.p, sizeof(float4) * w.A, hipMemcpyDeviceToHost, s));
+        }
+        if (w.D) {
+            PGX_HIP(hipMemcpyAsync(dv.data(), w.dvel.p, sizeof(float4) * w.D, hipMemcpyDeviceToHost, s));
+            PGX_HIP(hipMemcpyAsync(da.data(), w.dang.p, sizeof(float4) * w.D, hipMemcpyDeviceToHost, s));
+        }
+        PGX_HIP(hipStreamSynchronize(s));
+        for (int a = 0; a < w.A; ++a) {
+            int b = w.actorToBody[a];
+            if (pos) { pos[3 * a] = bp[b].x; pos[3 * a + 1] = bp[b].y; pos[3 * a + 2] = bp[b].z; }
+            if (quat) { quat[4 * a] = bq[b].x; quat[4 * a + 1] = bq[b].y; quat[4 * a + 2] = bq[b].z; quat[4 * a + 3] = bq[b].w; }
+        }
+        for (int d = 0; d < w.D; ++d) {
+            if (vel) { vel[3 * d] = dv[d].x; vel[3 * d + 1] = dv[d].y; vel[3 * d + 2] = dv[d].z; }
+            if (angvel) { angvel[3 * d] = da[d].x; angvel[3 * d + 1] = da[d].y; angvel[3 * d + 2] = da[d].z; }
+        }
+        return PGX_OK;
+    });
+}
+
+int pgx_world_write_state(pgx_world* h, const float* pos, const float* quat, const float* vel, const float* angvel) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    return guarded([&] {
+        World& w = h->w;
+        PGX_HIP(hipSetDevice(w.device));
+        hipStream_t s = w.stream;
+        std::vector<float4> bp(w.A), bq(w.A), dv(w.D), da(w.D);
+        PGX_HIP(hipMemcpyAsync(bp.data(), w.bpos.p, sizeof(float4) * w.A, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(bq.data(), w.bquat.p, sizeof(float4) * w.A, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(dv.data(), w.dvel.p, sizeof(float4) * w.D, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(da.data(), w.dang.p, sizeof(float4) * w.D, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipStreamSynchronize(s));
+        for (int a = 0; a < w.A; ++a) {
+            int b = w.actorToBody[a];
+            if (pos) bp[b] = make_float4(pos[3 * a], pos[3 * a + 1], pos[3 * a + 2], 0.f);
+            if (quat) bq[b] = make_float4(quat[4 * a], quat[4 * a + 1], quat[4 * a + 2], quat[4 * a + 3]);
+        }
+        for (int d = 0; d < w.D; ++d) {
+            if (vel) dv[d] = make_float4(vel[3 * d], vel[3 * d + 1], vel[3 * d + 2], dv[d].w);
+            if (angvel) da[d] = make_float4(angvel[3 * d], angvel[3 * d + 1], angvel[3 * d + 2], 0.f);
+        }
+        upload(w.bpos, bp, s); upload(w.bquat, bq, s); upload(w.dvel, dv, s); upload(w.dang, da, s);
+        PGX_HIP(hipStreamSynchronize(s));
+        return PGX_OK;
+    });
+}
+
+int pgx_world_get_counts(pgx_world* h, pgx_counts* out) {
+    if (!h || !out) return fail(PGX_ERR_INVALID, "NULL argument");
+    out->pairs = h->w.lastPairs;
+    out->manifolds = h->w.lastManifolds;
+    out->colors = h->w.lastColors;
+    out->joint_rows = 0;
+    out->coloring_rounds = h->w.lastRounds;
+    out->points = 0;
+    return guarded([&] {
+        World& w = h->w;
+        int M = w.lastManifolds;
+        if (M) {
+            std::vector<int32_t> f(M);
+            PGX_HIP(hipMemcpy(f.data(), w.sFlags.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
+            long pts = 0;
+            for (int v : f) pts += v & 255;
+            out->points = (int)pts;
+        }
+        return PGX_OK;
+    });
+}
+
+int pgx_world_get_bounds(pgx_world* h, float* out) {
+    if (!h || !out) return fail(PGX_ERR_INVALID, "NULL argument");
+    return guarded([&] {
+        World& w = h->w;
+        std::vector<float4> mn(w.C), mx(w.C);
+        if (w.C) {
+            PGX_HIP(hipMemcpy(mn.data(), w.cbmin.p, sizeof(float4) * w.C, hipMemcpyDeviceToHost));
+            PGX_HIP(hipMemcpy(mx.data(), w.cbmax.p, sizeof(float4) * w.C, hipMemcpyDeviceToHost));
+        }
+        for (int c = 0; c < w.C; ++c) {
+            out[6 * c] = mn[c].x; out[6 * c + 1] = mn[c].y; out[6 * c + 2] = mn[c].z;
+            out[6 * c + 3] = mx[c].x; out[6 * c + 4] = mx[c].y; out[6 * c + 5] = mx[c].z;
+        }
+        return PGX_OK;
+    });
+}
+
+int pgx_world_get_pairs(pgx_world* h, int32_t* out, int cap, int* n) {
+    if (!h || !n) return fail(PGX_ERR_INVALID, "NULL argument");
+    return guarded([&] {
+        World& w = h->w;
+        int P = w.lastPairs;
+        *n = P;
+        if (!out || P == 0) return PGX_OK;
+        std::vector<uint64_t> keys(P);
+        std::vector<int32_t> ws(w.W + 1), col(w.C);
+        PGX_HIP(hipMemcpy(keys.data(), w.pkeys.p, sizeof(uint64_t) * P, hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(ws.data(), w.worldColStart.p, sizeof(int32_t) * (w.W + 1), hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(col.data(), w.colOfLocal.p, sizeof(int32_t) * w.C, hipMemcpyDeviceToHost));
+        uint64_t lm = (1ull << w.localBits) - 1;
+        for (int i = 0; i < P && i < cap; ++i) {
+            uint64_t k = keys[i];
+            int wd = (int)(k >> (2 * w.localBits));
+            out[2 * i] = col[ws[wd] + (int)((k >> w.localBits) & lm)];
+            out[2 * i + 1] = col[ws[wd] + (int)(k & lm)];
+        }
+        return PGX_OK;
+    });
+}
+
+int pgx_world_get_manifolds(pgx_world* h, int32_t* keys, int32_t* ints, float* floats, int cap, int* n) {
+    if (!h || !n) return fail(PGX_ERR_INVALID, "NULL argument");
+    return guarded([&] {
+        World& w = h->w;
+        int M = w.lastManifolds;
+        *n = M;
+        if (M == 0) return PGX_OK;
+        size_t Mc = w.manifoldCap;
+        std::vector<int32_t> canon(M), b0(M), b1(M), flags(M), mfPair(M);
+        std::vector<uint64_t> mkey(M);
+        std::vector<float4> sn(M), r0(4 * Mc), r1(4 * Mc);
+        PGX_HIP(hipMemcpy(canon.data(), w.sCanon.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(b0.data(), w.sB0.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(b1.data(), w.sB1.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(flags.data(), w.sFlags.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(mfPair.data(), w.mfPair.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(mkey.data(), w.mfKey.p, sizeof(uint64_t) * M, hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(sn.data(), w.sN.p, sizeof(float4) * M, hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(r0.data(), w.sR0.p, sizeof(float4) * 4 * Mc, hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(r1.data(), w.sR1.p, sizeof(float4) * 4 * Mc, hipMemcpyDeviceToHost));
+        std::vector<float4> pts(8 * (size_t)w.lastPairs);
+        if (w.lastPairs)
+            PGX_HIP(hipMemcpy(pts.data(), w.npPts.p, sizeof(float4) * 8 * (size_t)w.lastPairs, hipMemcpyDeviceToHost));
+        std::vector<int32_t> ws(w.W + 1), col(w.C);
+        PGX_HIP(hipMemcpy(ws.data(), w.worldColStart.p, sizeof(int32_t) * (w.W + 1), hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(col.data(), w.colOfLocal.p, sizeof(int32_t) * w.C, hipMemcpyDeviceToHost));
+        uint64_t lm = (1ull << w.localBits) - 1;
+        for (int s = 0; s < M && s < cap; ++s) {
+            int m = canon[s];
+            uint64_t k = mkey[m];
+            int wd = (int)(k >> (2 * w.localBits));
+            int c0 = col[ws[wd] + (int)((k >> w.localBits) & lm)];
+            int c1 = col[ws[wd] + (int)(k & lm)];
+            int num = flags[s] & 255;
+            if (keys) {
+                keys[5 * s] = (int32_t)w.hostEntity[w.hostColBody[c0]];
+                keys[5 * s + 1] = w.hostColIndex[c0];
+                keys[5 * s + 2] = (int32_t)w.hostEntity[w.hostColBody[c1]];
+                keys[5 * s + 3] = w.hostColIndex[c1];
+                keys[5 * s + 4] = -1;
+            }
+            if (ints) {
+                ints[4 * s] = b0[s]; ints[4 * s + 1] = b1[s]; ints[4 * s + 2] = num; ints[4 * s + 3] = flags[s] >> 16;
+            }
+            if (floats) {
+                float* o = floats + 56 * (size_t)s;
+                std::memset(o, 0, 56 * sizeof(float));
+                o[0] = sn[s].x; o[1] = sn[s].y; o[2] = sn[s].z; o[3] = sn[s].w;
+                int p = mfPair[m];
+                for (int k2 = 0; k2 < num; ++k2) {
+                    float4 a = pts[8 * (size_t)p + 2 * k2], b = pts[8 * (size_t)p + 2 * k2 + 1];
+                    float* q = o + 4 + 6 * k2;
+                    q[0] = a.x; q[1] = a.y; q[2] = a.z; q[3] = b.x; q[4] = b.y; q[5] = b.z;
+                    float4 ra = r0[(size_t)k2 * Mc + s], rb = r1[(size_t)k2 * Mc + s];
+                    float* t = o + 28 + 7 * k2;
+                    t[0] = ra.x; t[1] = ra.y; t[2] = ra.z; t[3] = rb.x; t[4] = rb.y; t[5] = rb.z; t[6] = ra.w;
+                }
+            }
+        }
+        return PGX_OK;
+    });
+}
+
+int pgx_world_enable_timing(pgx_world* h, int enable) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    h->w.timing = enable != 0;
+    return PGX_OK;
+}
+
+int pgx_world_get_timing(pgx_world* h, float* b, float* n, float* c, float* s, float* t) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    if (b) *b = h->w.tBroad * 1000.f;
+    if (n) *n = h->w.tNarrow * 1000.f;
+    if (c) *c = h->w.tColor * 1000.f;
+    if (s) *s = h->w.tSolve * 1000.f;
+    if (t) *t = h->w.tTotal * 1000.f;
+    return PGX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+"""CPU-side checks of the C ABI boundary and host logic (no GPU needed)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "physecs_gpu.h")).read()
+    return sorted(set(re.findall(r"\b(pgx_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from physecs_amd import api
+    L = api.lib()
+    syms = header_symbols()
+    assert len(syms) >= 10
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+    assert sorted(api.EXPORTS) == syms
+
+
+def test_no_device_reports_error_not_crash():
+    from physecs_amd import api, scenes
+    if api.device_count() > 0:
+        pytest.skip("a GPU is present")
+    sc = scenes.box_pyramid(0, rows=2)
+    with pytest.raises(api.PgxError):
+        api.World(sc)
+
+
+def test_invalid_descriptor_rejected_on_host():
+    from physecs_amd import api, scenes
+    sc = scenes.box_pyramid(0, rows=2)
+    arr = sc.arrays()
+    arr["col_type"][0] = 7
+    desc = sc.struct(api.PgxSceneDesc)
+    desc.actor_world = ctypes.cast(None, api.I32)
+    h = ctypes.c_void_p()
+    rc = api.lib().pgx_world_create(None, ctypes.byref(desc), ctypes.byref(h))
+    assert rc == -4
+    assert b"geometry type 7" in api.lib().pgx_last_error()
+
+
+def test_scene_generators_shapes():
+    from physecs_amd import scenes
+    p = scenes.box_pyramid(0)
+    assert p.n_dyn == 210 and p.n_colliders == 211
+    m = scenes.mixed_lattice(0, side=4)
+    assert m.n_dyn == 64 and m.n_colliders == 69
+    kinds = np.bincount(m.arrays()["col_type"][5:], minlength=3)
+    assert abs(kinds[0] - kinds[1]) <= 1 and abs(kinds[1] - kinds[2]) <= 1
+    b = scenes.batched_worlds(0, n_worlds=16)
+    assert b.n_worlds == 16
+    assert b.n_colliders == b.n_actors
+
+
+def test_mass_props_match_reference_formulas():
+    import oracle_lib as ol
+    from physecs_amd import scenes
+    if not ol.ref_available():
+        pytest.skip("oracle/_ref not built here")
+    ref = ol.ref()
+    for t, p in ((0, [0.4]), (1, [0.5, 0.2]), (2, [0.3, 0.5, 0.2])):
+        pr = np.zeros(4, np.float32)
+        pr[:len(p)] = p
+        mass = 1000.0 * scenes.true_volume(t, pr)
+        im, com, invI = scenes.mass_props(t, pr, mass)
+        out = np.zeros(13, np.float32)
+        z = np.zeros(3, np.float32)
+        q = np.array([0, 0, 0, 1], np.float32)
+        ref.ref_mass_props(t, ol.fptr(pr), ol.fptr(z), ol.fptr(q), np.float32(mass), ol.fptr(out))
+        assert abs(out[0] - im) <= 1e-6 * abs(im)
+        np.testing.assert_allclose(out[4:], invI, rtol=2e-5)
