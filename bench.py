@@ -1,0 +1,130 @@
+"""Benchmark of the rigid-body step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain32k|pyramid|batched]
+
+A step is one Scene::simulate(1/60) with 4 substeps and 2 iterations over the whole scene,
+inputs resident in HBM. The default workload is the 32,768-body mixed scene (1/3 spheres,
+capsules, boxes on a jittered 32^3 lattice at 1.5 m in an open 5-box container, 600 steps:
+10 warm-up + 590 timed). Prints one JSON line (metric body-substeps/s, higher is better).
+
+roofline: the dominant kernel is the contact solve (k_solve, one launch per graph colour and
+pass). Its algorithmic bytes per launch are 284 B per contact point + 12 B per manifold of
+the colour (rows 108 + scalars 24 + both bodies' solver v,w 48 + component v,w 48 read;
+lambdas 8 + both bodies' v,w 48 written); its duration is measured with HIP events around
+each launch of the first substep of every timed step, on the world's stream.
+
+cpu_baseline: the CPU oracle (a restatement of the reference's Scene::simulate with the
+reference's ThreadPool narrowphase chunking, nproc-1 workers + caller) on the same scene,
+started from the device state after the warm-up, timed for about 15 s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def make_scene(name):
+    from physecs_amd import scenes
+    if name == "plain32k":
+        return scenes.mixed_lattice(0, side=32), {"workload": "mixed 32768 bodies (sphere/capsule/box) in 5-box container",
+                                                  "bodies": 32768, "substeps": 4, "iterations": 2, "dt": 1 / 60}
+    if name == "pyramid":
+        return scenes.box_pyramid(0), {"workload": "20-layer box pyramid (210 boxes)", "bodies": 210,
+                                       "substeps": 4, "iterations": 2, "dt": 1 / 60}
+    if name == "batched":
+        s = scenes.batched_worlds(0, n_worlds=4096)
+        return s, {"workload": "4096 uneven worlds (8..1024 bodies, log-uniform)", "bodies": s.n_dyn,
+                   "substeps": 4, "iterations": 2, "dt": 1 / 60}
+    raise SystemExit(f"unknown config {name}")
+
+
+def cpu_baseline(scene, state, substeps, iterations, dt, budget_s=15.0):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    pos, quat, vel, ang = state
+    arr = scene.arrays()
+    saved = {k: arr[k].copy() for k in ("pos", "quat", "vel", "angvel")}
+    arr["pos"][:] = pos
+    arr["quat"][:] = quat
+    arr["vel"][:] = vel
+    arr["angvel"][:] = ang
+    threads = max(0, (os.cpu_count() or 1) - 1)
+    o = ol.OracleWorld(scene, substeps=substeps, iterations=iterations, threads=threads)
+    for k, v in saved.items():
+        arr[k][:] = v
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(dt)
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s or steps >= 200:
+            break
+    el = time.perf_counter() - t0
+    return {"value": scene.n_dyn * substeps * steps / el, "unit": "body-substeps/s", "cores": threads + 1,
+            "kind": "port", "sample": f"{steps} steps of the same scene from the device state after warm-up, "
+                                      f"{el:.1f} s; narrowphase on {threads} workers + caller, solver single-threaded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=590)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="plain32k")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+    from physecs_amd import api
+    scene, cfg = make_scene(args.config)
+    sub, it, dt = cfg["substeps"], cfg["iterations"], cfg["dt"]
+    w = api.World(scene, substeps=sub, iterations=it)
+    for _ in range(args.warmup):
+        w.step(dt)
+    w.sync()
+    warm_state = w.state()
+    w.enable_timing(True)
+    w.reset_stats()
+    lat = []
+    w.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        w.step(dt)
+        w.sync()
+        lat.append(time.perf_counter() - ts)
+    w.sync()
+    el = time.perf_counter() - t0
+    st = w.stats()
+    counts = w.counts()
+    bss = scene.n_dyn * sub * args.steps / el
+    achieved = st["solve_bytes"] / (st["solve_us"] * 1e-6) / 1e9 if st["solve_us"] > 0 else 0.0
+    line = {
+        "metric": "body-substeps/s (32k bodies, 1 GPU)" if args.config == "plain32k" else "body-substeps/s",
+        "value": bss, "unit": "body-substeps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000 * el / args.steps, "p50_step_ms": 1000 * float(np.percentile(lat, 50)),
+        "p99_step_ms": 1000 * float(np.percentile(lat, 99)), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
+        "config": dict(cfg, parallelism="single GPU", last_step_counts=counts),
+        "roofline": {"bound": "hbm", "kernel": "k_solve", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": st["solve_us"] / max(1, st["solve_launches"]),
+                     "bytes_per_launch": st["solve_bytes"] / max(1, st["solve_launches"]),
+                     "launches_timed": st["solve_launches"]},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(scene, warm_state, sub, it, dt)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -140,6 +140,20 @@ int pgx_world_enable_timing(pgx_world* w, int enable);
 int pgx_world_get_timing(pgx_world* w, float* broadphase_us, float* narrowphase_us, float* coloring_us,
                          float* solver_us, float* total_us);
 
+/* Cumulative timing since pgx_world_reset_stats, collected while timing is enabled:
+ * every contact-solve launch of the first substep of each step is bracketed by HIP events
+ * on the world's stream; solve_bytes is their algorithmic traffic (284 B per contact point
+ * + 12 B per manifold of the launched colour). */
+typedef struct pgx_stats {
+    long steps;
+    long solve_launches;
+    double solve_us;
+    double solve_bytes;
+    double step_us;
+} pgx_stats;
+int pgx_world_reset_stats(pgx_world* w);
+int pgx_world_get_stats(pgx_world* w, pgx_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

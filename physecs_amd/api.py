@@ -25,7 +25,8 @@ EXPORTS = [
     "pgx_last_error", "pgx_device_count", "pgx_world_create", "pgx_world_destroy", "pgx_world_set_params",
     "pgx_world_set_can_collide", "pgx_world_step", "pgx_world_sync", "pgx_world_read_state",
     "pgx_world_write_state", "pgx_world_get_counts", "pgx_world_get_bounds", "pgx_world_get_pairs",
-    "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing",
+    "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
+    "pgx_world_get_stats",
 ]
 
 
@@ -48,6 +49,11 @@ class PgxSceneDesc(ctypes.Structure):
 class PgxCounts(ctypes.Structure):
     _fields_ = [("pairs", ctypes.c_int), ("manifolds", ctypes.c_int), ("points", ctypes.c_int),
                 ("colors", ctypes.c_int), ("joint_rows", ctypes.c_int), ("coloring_rounds", ctypes.c_int)]
+
+
+class PgxStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_long), ("solve_launches", ctypes.c_long), ("solve_us", ctypes.c_double),
+                ("solve_bytes", ctypes.c_double), ("step_us", ctypes.c_double)]
 
 
 class PgxError(RuntimeError):
@@ -80,6 +86,8 @@ def lib():
         L.pgx_world_get_manifolds.argtypes = [vp, I32, I32, F, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
         L.pgx_world_enable_timing.argtypes = [vp, ctypes.c_int]
         L.pgx_world_get_timing.argtypes = [vp, F, F, F, F, F]
+        L.pgx_world_reset_stats.argtypes = [vp]
+        L.pgx_world_get_stats.argtypes = [vp, ctypes.POINTER(PgxStats)]
         for name in EXPORTS:
             if name != "pgx_last_error":
                 getattr(L, name).restype = ctypes.c_int
@@ -182,6 +190,14 @@ class World:
 
     def enable_timing(self, on=True):
         check(self.L.pgx_world_enable_timing(self.h, 1 if on else 0))
+
+    def reset_stats(self):
+        check(self.L.pgx_world_reset_stats(self.h))
+
+    def stats(self):
+        st = PgxStats()
+        check(self.L.pgx_world_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in PgxStats._fields_}
 
     def timing(self):
         v = [ctypes.c_float() for _ in range(5)]

@@ -167,6 +167,13 @@ struct World {
     bool timing = false;
     hipEvent_t ev[6] = {};
     float tBroad = 0, tNarrow = 0, tColor = 0, tSolve = 0, tTotal = 0;
+    // per-launch timing of the contact solve (first substep of each timed step)
+    DBuf<int32_t> colorPoints;           // contact points per colour (this step)
+    std::vector<hipEvent_t> solveEv;
+    std::vector<int> solveEvColor;
+    int solveEvUsed = 0;
+    long statSteps = 0, statSolveLaunches = 0;
+    double statSolveUs = 0, statSolveBytes = 0, statStepUs = 0;
 };
 
 // broadphase.hip

@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_setup(
     int prevCount, int prevCap, uint64_t* __restrict__ curKey, int32_t* __restrict__ curNum,
     float4* __restrict__ curLp, int32_t* __restrict__ sB0, int32_t* __restrict__ sB1, int32_t* __restrict__ sA0,
     int32_t* __restrict__ sA1, int32_t* __restrict__ sFlags, int32_t* __restrict__ sCanon, float4* __restrict__ sN,
-    float2* __restrict__ sSoft, float4* __restrict__ sR0, float4* __restrict__ sR1) {
+    float2* __restrict__ sSoft, float4* __restrict__ sR0, float4* __restrict__ sR1, int32_t* __restrict__ colorPoints) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     int m = perm[s];
@@ -221,6 +221,7 @@ __global__ __launch_bounds__(256) void k_setup(
     sCanon[s] = m;
     sN[s] = f4(n, friction);
     sSoft[s] = make_float2(freq, damp);
+    atomicAdd(&colorPoints[mfColor[m]], num);
 }
 
 // ------------------------------------------------------------------ substep kernels
@@ -526,15 +527,33 @@ int sv_color(World& w, int M) {
     w.lastRounds = round;
     // contact setup, written in solve order
     int cur = w.cacheCur ^ 1, prv = w.cacheCur;
+    fill_i32(s, w.colorPoints.p, 0, 160);
     hipLaunchKernelGGL(k_setup, dim3(nb), dim3(256), 0, s, M, w.manifoldCap, w.colorPerm.p, w.mfPair.p, w.mfKey.p,
                        w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.cflags.p, w.cmat.p, w.npNum.p,
                        w.npNormal.p, w.npPts.p, w.mfColor.p, w.mfB0.p, w.mfB1.p, w.bpos.p, w.bquat.p, w.dvel.p,
                        w.dang.p, w.dcom.p, w.D, w.cacheKey[prv].p, w.cacheNum[prv].p, w.cacheLp[prv].p,
                        w.cacheCount[prv], w.manifoldCap, w.cacheKey[cur].p, w.cacheNum[cur].p, w.cacheLp[cur].p,
                        w.sB0.p, w.sB1.p, w.sA0.p, w.sA1.p, w.sFlags.p, w.sCanon.p, w.sN.p, w.sSoft.p, w.sR0.p,
-                       w.sR1.p);
+                       w.sR1.p, w.colorPoints.p);
     PGX_LAUNCH_CHECK();
     return nColors;
+}
+
+// HIP events around one k_solve launch (timing mode, first substep only)
+static void solve_event(World& w, bool rec, int color, bool after) {
+    if (!rec) return;
+    int idx = 2 * w.solveEvUsed + (after ? 1 : 0);
+    while ((int)w.solveEv.size() <= idx) {
+        hipEvent_t e;
+        PGX_HIP(hipEventCreate(&e));
+        w.solveEv.push_back(e);
+    }
+    PGX_HIP(hipEventRecord(w.solveEv[idx], w.stream));
+    if (after) {
+        if ((int)w.solveEvColor.size() <= w.solveEvUsed) w.solveEvColor.resize(w.solveEvUsed + 1);
+        w.solveEvColor[w.solveEvUsed] = color;
+        ++w.solveEvUsed;
+    }
 }
 
 void sv_substeps(World& w, int M, float dt) {
@@ -544,7 +563,9 @@ void sv_substeps(World& w, int M, float dt) {
     float h = dt / w.substeps;
     int nbM = div_up(std::max(M, 1), 256), nbD = div_up(std::max(D, 1), 256);
     int nColors = (int)w.colorStartHost.size() - 1;
+    w.solveEvUsed = 0;
     for (int sub = 0; sub < w.substeps; ++sub) {
+        const bool rec = w.timing && sub == 0;
         if (M)
             hipLaunchKernelGGL(k_update_contacts, dim3(nbM), dim3(256), 0, s, M, Mcap, w.sB0.p, w.sB1.p, w.sA0.p,
                                w.sA1.p, w.sFlags.p, w.sN.p, w.sR0.p, w.sR1.p, w.bpos.p, w.bquat.p, w.dvel.p,
@@ -559,9 +580,11 @@ void sv_substeps(World& w, int M, float dt) {
             for (int c = 0; c < nColors; ++c) {
                 int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
                 if (e <= b) continue;
+                solve_event(w, rec, c, false);
                 hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 256)), dim3(256), 0, s, b, e, Mcap, 1, h, 0, w.sB0.p,
                                    w.sB1.p, w.sFlags.p, w.sN.p, w.sSoft.p, w.sR0.p, w.dvel.p, w.dang.p, w.tvel.p,
                                    w.tang.p, w.sRow.p);
+                solve_event(w, rec, c, true);
             }
         }
         if (D)
@@ -570,9 +593,11 @@ void sv_substeps(World& w, int M, float dt) {
         for (int c = 0; c < nColors; ++c) {
             int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
             if (e <= b) continue;
+            solve_event(w, rec, c, false);
             hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 256)), dim3(256), 0, s, b, e, Mcap, 0, 0.f, 1, w.sB0.p,
                                w.sB1.p, w.sFlags.p, w.sN.p, w.sSoft.p, w.sR0.p, w.dvel.p, w.dang.p, w.tvel.p,
                                w.tang.p, w.sRow.p);
+            solve_event(w, rec, c, true);
         }
         if (D)
             hipLaunchKernelGGL(k_writeback, dim3(nbD), dim3(256), 0, s, D, w.dvel.p, w.dang.p, w.dcom.p, w.tvel.p,

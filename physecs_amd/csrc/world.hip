@@ -192,7 +192,7 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
             size_t M = w.manifoldCap;
             w.mfKey.alloc(M); w.mfPair.alloc(M); w.mfB0.alloc(M); w.mfB1.alloc(M); w.mfColor.alloc(M);
             w.claim.alloc(std::max(D, 1)); w.usedMask.alloc(2 * (size_t)std::max(D, 1));
-            w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorCount.alloc(160);
+            w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorCount.alloc(160); w.colorPoints.alloc(160);
             w.sB0.alloc(M); w.sB1.alloc(M); w.sFlags.alloc(M); w.sA0.alloc(M); w.sA1.alloc(M); w.sCanon.alloc(M);
             w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(36 * M);
             for (int k = 0; k < 2; ++k) { w.cacheKey[k].alloc(M); w.cacheNum[k].alloc(M); w.cacheLp[k].alloc(4 * M); }
@@ -229,6 +229,9 @@ static void release_world(World& w) {
     w.nodeRange.release(); w.sSoft.release();
     for (auto& e : w.ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : w.solveEv) (void)hipEventDestroy(e);
+    w.solveEv.clear();
+    w.colorPoints.release();
     if (w.stream) (void)hipStreamDestroy(w.stream);
     w.stream = nullptr;
 }
@@ -301,6 +304,20 @@ int pgx_world_step(pgx_world* h, float dt) {
             PGX_HIP(hipEventElapsedTime(&w.tColor, w.ev[2], w.ev[3]));
             PGX_HIP(hipEventElapsedTime(&w.tSolve, w.ev[3], w.ev[4]));
             PGX_HIP(hipEventElapsedTime(&w.tTotal, w.ev[0], w.ev[5]));
+            // per-launch k_solve durations and their algorithmic bytes
+            std::vector<int32_t> pts(160, 0);
+            PGX_HIP(hipMemcpy(pts.data(), w.colorPoints.p, sizeof(int32_t) * 160, hipMemcpyDeviceToHost));
+            for (int j = 0; j < w.solveEvUsed; ++j) {
+                float ms = 0.f;
+                PGX_HIP(hipEventElapsedTime(&ms, w.solveEv[2 * j], w.solveEv[2 * j + 1]));
+                int c = w.solveEvColor[j];
+                int nm = w.colorStartHost[c + 1] - w.colorStartHost[c];
+                w.statSolveUs += 1000.0 * ms;
+                w.statSolveBytes += 284.0 * pts[c] + 12.0 * nm;
+                ++w.statSolveLaunches;
+            }
+            w.statStepUs += 1000.0 * w.tTotal;
+            ++w.statSteps;
         }
         w.lastPairs = nPairs;
         w.lastManifolds = M;
@@ -497,6 +514,25 @@ int pgx_world_get_manifolds(pgx_world* h, int32_t* keys, int32_t* ints, float* f
         }
         return PGX_OK;
     });
+}
+
+int pgx_world_reset_stats(pgx_world* h) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    World& w = h->w;
+    w.statSteps = w.statSolveLaunches = 0;
+    w.statSolveUs = w.statSolveBytes = w.statStepUs = 0;
+    return PGX_OK;
+}
+
+int pgx_world_get_stats(pgx_world* h, pgx_stats* out) {
+    if (!h || !out) return fail(PGX_ERR_INVALID, "NULL argument");
+    World& w = h->w;
+    out->steps = w.statSteps;
+    out->solve_launches = w.statSolveLaunches;
+    out->solve_us = w.statSolveUs;
+    out->solve_bytes = w.statSolveBytes;
+    out->step_us = w.statStepUs;
+    return PGX_OK;
 }
 
 int pgx_world_enable_timing(pgx_world* h, int enable) {

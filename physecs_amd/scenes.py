@@ -53,7 +53,7 @@ def mass_props(t, p, mass):
         inertia = _inertia_capsule(mass, p[0], p[1])
     else:
         inertia = _inertia_box(mass, p[:3])
-    inv = np.linalg.inv(inertia)
+    inv = np.diag(1.0 / np.diag(inertia))          # diagonal for one collider at the origin
     return np.float32(1.0 / mass), np.zeros(3, np.float32), inv.T.astype(np.float32).ravel()
 
 
@@ -78,6 +78,7 @@ class Scene:
         self.col_type, self.col_params, self.col_material = [], [], []
         self.col_trigger, self.col_enable, self.col_data = [], [], []
         self.non_colliding = []
+        self._ncol = {}
         self.n_worlds = 1
         self._frozen = None
 
@@ -106,7 +107,8 @@ class Scene:
 
     def add_collider(self, actor, gtype, params, material=(0.6, 0.0, 0.0), lpos=(0, 0, 0), lquat=(0, 0, 0, 1),
                      trigger=False, enable=True, data=0):
-        idx = sum(1 for a in self.col_actor if a == actor)
+        idx = self._ncol.get(actor, 0)
+        self._ncol[actor] = idx + 1
         self.col_actor.append(actor)
         self.col_index.append(idx)
         self.col_pos.append(np.asarray(lpos, np.float32))
