@@ -140,10 +140,12 @@ int pgx_world_enable_timing(pgx_world* w, int enable);
 int pgx_world_get_timing(pgx_world* w, float* broadphase_us, float* narrowphase_us, float* coloring_us,
                          float* solver_us, float* total_us);
 
-/* Cumulative timing since pgx_world_reset_stats, collected while timing is enabled:
- * every contact-solve launch of the first substep of each step is bracketed by HIP events
- * on the world's stream; solve_bytes is their algorithmic traffic (284 B per contact point
- * + 12 B per manifold of the launched colour). */
+/* Cumulative timing since pgx_world_reset_stats, collected while timing is enabled: the
+ * substep-loop kernel (one cooperative launch per step) is bracketed by HIP events on the
+ * world's stream; solve_bytes is its algorithmic traffic per the phase model (update rows
+ * 164 B/manifold + 96 B/point, integrate velocities 168 B/body, preSolve 80 B/manifold +
+ * 116 B/point, each solve pass 284 B/point + 12 B/manifold, integrate positions 120 B/body,
+ * write-back 48 B/body; per substep). */
 typedef struct pgx_stats {
     long steps;
     long solve_launches;
@@ -152,6 +154,10 @@ typedef struct pgx_stats {
     double step_us;
 } pgx_stats;
 int pgx_world_reset_stats(pgx_world* w);
+/* Execution mode of the substep loop: persistent = 1 runs all substeps of a step as one
+ * cooperative launch with grid barriers between colours (default); 0 launches one kernel
+ * per colour and phase. blocks = grid size of the persistent launch (0 = one per CU). */
+int pgx_world_set_exec(pgx_world* w, int persistent, int blocks);
 int pgx_world_get_stats(pgx_world* w, pgx_stats* out);
 
 #ifdef __cplusplus

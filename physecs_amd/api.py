@@ -26,7 +26,7 @@ EXPORTS = [
     "pgx_world_set_can_collide", "pgx_world_step", "pgx_world_sync", "pgx_world_read_state",
     "pgx_world_write_state", "pgx_world_get_counts", "pgx_world_get_bounds", "pgx_world_get_pairs",
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
-    "pgx_world_get_stats",
+    "pgx_world_get_stats", "pgx_world_set_exec",
 ]
 
 
@@ -88,6 +88,7 @@ def lib():
         L.pgx_world_get_timing.argtypes = [vp, F, F, F, F, F]
         L.pgx_world_reset_stats.argtypes = [vp]
         L.pgx_world_get_stats.argtypes = [vp, ctypes.POINTER(PgxStats)]
+        L.pgx_world_set_exec.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         for name in EXPORTS:
             if name != "pgx_last_error":
                 getattr(L, name).restype = ctypes.c_int
@@ -190,6 +191,9 @@ class World:
 
     def enable_timing(self, on=True):
         check(self.L.pgx_world_enable_timing(self.h, 1 if on else 0))
+
+    def set_exec(self, persistent=True, blocks=0):
+        check(self.L.pgx_world_set_exec(self.h, 1 if persistent else 0, blocks))
 
     def reset_stats(self):
         check(self.L.pgx_world_reset_stats(self.h))

@@ -72,6 +72,8 @@ void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin) {
 }
 
 // ------------------------------------------------------------------ Morton codes
+// Quantisation box: the AABB of all dynamic collider centres (the pair set does not depend
+// on it; worlds are kept apart by the world id in the key's high bits).
 __device__ __forceinline__ uint32_t ord_enc(float f) {   // order-preserving float -> uint
     uint32_t u = __float_as_uint(f);
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
@@ -80,28 +82,36 @@ __device__ __forceinline__ float ord_dec(uint32_t u) {
     return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
-__global__ void k_world_bounds_init(int W, uint32_t* wb) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= W) return;
-    for (int k = 0; k < 3; ++k) {
-        wb[6 * i + k] = 0xffffffffu;   // min
-        wb[6 * i + 3 + k] = 0u;        // max
-    }
+__global__ void k_scene_bounds_init(uint32_t* wb) {
+    if (threadIdx.x < 3) { wb[threadIdx.x] = 0xffffffffu; wb[3 + threadIdx.x] = 0u; }
 }
 
-__global__ void k_world_bounds(int C, const int32_t* __restrict__ cworld, const float4* __restrict__ bmin,
-                               const float4* __restrict__ bmax, uint32_t* __restrict__ wb) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C) return;
-    float4 a = bmin[c], b = bmax[c];
-    float cx = 0.5f * (a.x + b.x), cy = 0.5f * (a.y + b.y), cz = 0.5f * (a.z + b.z);
-    uint32_t* o = wb + 6 * cworld[c];
-    atomicMin(o + 0, ord_enc(cx));
-    atomicMin(o + 1, ord_enc(cy));
-    atomicMin(o + 2, ord_enc(cz));
-    atomicMax(o + 3, ord_enc(cx));
-    atomicMax(o + 4, ord_enc(cy));
-    atomicMax(o + 5, ord_enc(cz));
+// one block of 1024 threads reduces its share in LDS, then 6 atomics per block
+__global__ __launch_bounds__(1024) void k_scene_bounds(int n, const int32_t* __restrict__ dynCols,
+                                                       const float4* __restrict__ bmin,
+                                                       const float4* __restrict__ bmax, uint32_t* __restrict__ wb) {
+    __shared__ uint32_t red[6];
+    if (threadIdx.x < 3) { red[threadIdx.x] = 0xffffffffu; red[3 + threadIdx.x] = 0u; }
+    __syncthreads();
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int c = dynCols[i];
+        float4 a = bmin[c], b = bmax[c];
+        uint32_t e[3] = {ord_enc(0.5f * (a.x + b.x)), ord_enc(0.5f * (a.y + b.y)), ord_enc(0.5f * (a.z + b.z))};
+        for (int k = 0; k < 3; ++k) { mn[k] = min(mn[k], e[k]); mx[k] = max(mx[k], e[k]); }
+    }
+    for (int off = WAVE / 2; off >= 1; off >>= 1)
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = min(mn[k], (uint32_t)__shfl_xor((int)mn[k], off));
+            mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], off));
+        }
+    if ((threadIdx.x & (WAVE - 1)) == 0)
+        for (int k = 0; k < 3; ++k) { atomicMin(&red[k], mn[k]); atomicMax(&red[3 + k], mx[k]); }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        atomicMin(&wb[threadIdx.x], red[threadIdx.x]);
+        atomicMax(&wb[3 + threadIdx.x], red[3 + threadIdx.x]);
+    }
 }
 
 __device__ __forceinline__ uint32_t spread10(uint32_t v) {
@@ -112,26 +122,25 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
     return v;
 }
 
-__global__ void k_morton(int C, const int32_t* __restrict__ cworld, const float4* __restrict__ bmin,
-                         const float4* __restrict__ bmax, const uint32_t* __restrict__ wb,
-                         uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= C) return;
-    int wd = cworld[c];
-    const uint32_t* o = wb + 6 * wd;
+__global__ void k_morton(int n, const int32_t* __restrict__ dynCols, const int32_t* __restrict__ cworld,
+                         const float4* __restrict__ bmin, const float4* __restrict__ bmax,
+                         const uint32_t* __restrict__ wb, uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c = dynCols[i];
     float4 a = bmin[c], b = bmax[c];
     float cc[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
     uint32_t q[3];
     for (int k = 0; k < 3; ++k) {
-        float lo = ord_dec(o[k]), hi = ord_dec(o[3 + k]);
+        float lo = ord_dec(wb[k]), hi = ord_dec(wb[3 + k]);
         float ext = hi - lo;
         float t = ext > 0.f ? (cc[k] - lo) / ext : 0.5f;
         t = fminf(fmaxf(t, 0.f), 1.f);
         q[k] = (uint32_t)fminf(t * 1024.f, 1023.f);
     }
     uint32_t m = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
-    keys[c] = ((uint64_t)wd << 30) | m;
-    vals[c] = c;
+    keys[i] = ((uint64_t)cworld[c] << 30) | m;
+    vals[i] = c;
 }
 
 // ------------------------------------------------------------------ LBVH build
@@ -184,7 +193,7 @@ __global__ void k_build(int n, const uint64_t* __restrict__ keys, int32_t* __res
     }
 }
 
-// Leaf data in sorted order; bottom-up refit with one atomic counter per internal node.
+// Leaf data in sorted order.
 __global__ void k_leaves(int n, const int32_t* __restrict__ order, const float4* __restrict__ bmin,
                          const float4* __restrict__ bmax, const int32_t* __restrict__ cflags,
                          float4* __restrict__ lmin, float4* __restrict__ lmax, int32_t* __restrict__ lflags) {
@@ -196,29 +205,88 @@ __global__ void k_leaves(int n, const int32_t* __restrict__ order, const float4*
     lflags[s] = cflags[c];
 }
 
-__global__ void k_refit(int n, const int32_t* __restrict__ child, const int32_t* __restrict__ nodeParent,
-                        const int32_t* __restrict__ leafParent, const float4* __restrict__ lmin,
-                        const float4* __restrict__ lmax, const int32_t* __restrict__ lflags,
-                        float4* __restrict__ nmin, float4* __restrict__ nmax, int32_t* __restrict__ nflags,
-                        int32_t* __restrict__ counter) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+// Bottom-up refit, stage 1. Block b owns leaves [b*1024, b*1024+1023]; an internal node
+// whose leaf range lies inside that span is "local" (its index lies in the span too), and
+// is completed in LDS by the second of its two children to arrive, with workgroup-scope
+// ordering only. A climb that reaches a non-local node counts one finished child for it.
+constexpr int REFIT_BLOCK = 1024;
+
+__device__ __forceinline__ float4 fmin4(float4 a, float4 b) {
+    return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.f);
+}
+__device__ __forceinline__ float4 fmax4(float4 a, float4 b) {
+    return make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), 0.f);
+}
+
+__global__ __launch_bounds__(REFIT_BLOCK) void k_refit_local(
+    int n, const int32_t* __restrict__ child, const int2* __restrict__ range, const int32_t* __restrict__ nodeParent,
+    const int32_t* __restrict__ leafParent, const float4* __restrict__ lmin, const float4* __restrict__ lmax,
+    float4* __restrict__ nmin, float4* __restrict__ nmax, int32_t* __restrict__ gcount) {
+    __shared__ int cnt[REFIT_BLOCK];
+    __shared__ float4 smin[REFIT_BLOCK], smax[REFIT_BLOCK];
+    const int B0 = blockIdx.x * REFIT_BLOCK;
+    const int B1 = min(n, B0 + REFIT_BLOCK) - 1;
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int s = B0 + threadIdx.x;
     if (s >= n) return;
     int node = leafParent[s];
-    for (int guard = 0; node >= 0 && guard < n; ++guard) {
-        __threadfence();   // release this subtree's box before signalling the parent
-        if (atomicAdd(&counter[node], 1) == 0) return;   // first child to arrive stops
-        __threadfence();   // acquire the sibling subtree's box (written on another CU / XCD)
+    for (int guard = 0; node >= 0 && guard < REFIT_BLOCK; ++guard) {
+        int2 r = range[node];
+        if (r.x < B0 || r.y > B1) {
+            atomicAdd(&gcount[node], 1);   // consumed by stage 2 after this kernel ends
+            return;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (atomicAdd(&cnt[node - B0], 1) == 0) return;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         int a = child[2 * node], b = child[2 * node + 1];
-        float4 amin, amax, bmn, bmx;
-        int af, bf;
-        if (a & LEAF_BIT) { int x = a & ~LEAF_BIT; amin = lmin[x]; amax = lmax[x]; af = (lflags[x] & CF_DYNAMIC) ? 0 : 1; }
-        else { amin = nmin[a]; amax = nmax[a]; af = nflags[a]; }
-        if (b & LEAF_BIT) { int x = b & ~LEAF_BIT; bmn = lmin[x]; bmx = lmax[x]; bf = (lflags[x] & CF_DYNAMIC) ? 0 : 1; }
-        else { bmn = nmin[b]; bmx = nmax[b]; bf = nflags[b]; }
-        nmin[node] = make_float4(fminf(amin.x, bmn.x), fminf(amin.y, bmn.y), fminf(amin.z, bmn.z), 0.f);
-        nmax[node] = make_float4(fmaxf(amax.x, bmx.x), fmaxf(amax.y, bmx.y), fmaxf(amax.z, bmx.z), 0.f);
-        nflags[node] = af | bf;
+        float4 amn = (a & LEAF_BIT) ? lmin[a & ~LEAF_BIT] : smin[a - B0];
+        float4 amx = (a & LEAF_BIT) ? lmax[a & ~LEAF_BIT] : smax[a - B0];
+        float4 bmn = (b & LEAF_BIT) ? lmin[b & ~LEAF_BIT] : smin[b - B0];
+        float4 bmx = (b & LEAF_BIT) ? lmax[b & ~LEAF_BIT] : smax[b - B0];
+        float4 mn = fmin4(amn, bmn), mx = fmax4(amx, bmx);
+        smin[node - B0] = mn;
+        smax[node - B0] = mx;
+        nmin[node] = mn;
+        nmax[node] = mx;
         node = nodeParent[node];
+    }
+}
+
+// Stage 2a: list the non-local nodes whose two children were both finished in stage 1.
+__global__ void k_refit_ready(int n, const int32_t* __restrict__ gcount, int32_t* __restrict__ ready,
+                              int32_t* __restrict__ nReady) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n - 1 && gcount[i] == 2) ready[atomicAdd(nReady, 1)] = i;
+}
+
+// Stage 2b, one workgroup: climbs from the ready nodes; a parent (non-local as well) is
+// completed by whichever climb brings its second child.
+__global__ __launch_bounds__(1024) void k_refit_global(int n, const int32_t* __restrict__ child,
+                                                       const int32_t* __restrict__ nodeParent,
+                                                       const float4* __restrict__ lmin,
+                                                       const float4* __restrict__ lmax, float4* __restrict__ nmin,
+                                                       float4* __restrict__ nmax, int32_t* __restrict__ gcount,
+                                                       const int32_t* __restrict__ ready,
+                                                       const int32_t* __restrict__ nReady) {
+    int nr = *nReady;
+    for (int k = threadIdx.x; k < nr; k += blockDim.x) {
+        int node = ready[k];
+        for (int guard = 0; node >= 0 && guard < n; ++guard) {
+            int a = child[2 * node], b = child[2 * node + 1];
+            float4 amn = (a & LEAF_BIT) ? lmin[a & ~LEAF_BIT] : nmin[a];
+            float4 amx = (a & LEAF_BIT) ? lmax[a & ~LEAF_BIT] : nmax[a];
+            float4 bmn = (b & LEAF_BIT) ? lmin[b & ~LEAF_BIT] : nmin[b];
+            float4 bmx = (b & LEAF_BIT) ? lmax[b & ~LEAF_BIT] : nmax[b];
+            nmin[node] = fmin4(amn, bmn);
+            nmax[node] = fmax4(amx, bmx);
+            int p = nodeParent[node];
+            if (p < 0) break;
+            int old = __hip_atomic_fetch_add(&gcount[p], 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old != 1) break;
+            node = p;
+        }
     }
 }
 
@@ -258,31 +326,37 @@ __device__ __forceinline__ bool overlaps(float4 amin, float4 amax, float4 bmin, 
              amin.z > bmax.z);
 }
 
+// Each enabled dynamic leaf s walks its world's subtree over escape links and takes the
+// dynamic leaves after it (so each dynamic pair is emitted once), then tests the world's
+// non-dynamic colliders (static and kinematic; their bounds never move during a step).
 __global__ __launch_bounds__(256) void k_traverse(
     int n, const uint64_t* __restrict__ mkeys, int worldShift, const int32_t* __restrict__ order,
     const float4* __restrict__ lmin, const float4* __restrict__ lmax, const int32_t* __restrict__ lflags,
     const int32_t* __restrict__ child, const int2* __restrict__ range, const float4* __restrict__ nmin,
-    const float4* __restrict__ nmax, const int32_t* __restrict__ nflags, const int32_t* __restrict__ nodeEsc,
-    const int32_t* __restrict__ leafEsc, const int32_t* __restrict__ worldRoot,
-    const int32_t* __restrict__ worldStop, const uint32_t* __restrict__ centity,
-    const int32_t* __restrict__ clocal, int localBits, int32_t* __restrict__ pairCount, int cap,
-    uint64_t* __restrict__ pkeys) {
+    const float4* __restrict__ nmax, const int32_t* __restrict__ nodeEsc, const int32_t* __restrict__ leafEsc,
+    const int32_t* __restrict__ worldRoot, const int32_t* __restrict__ worldStop,
+    const int32_t* __restrict__ worldStaticStart, const int32_t* __restrict__ staticCols,
+    const float4* __restrict__ cbmin, const float4* __restrict__ cbmax, const int32_t* __restrict__ cflags,
+    const uint32_t* __restrict__ centity, const int32_t* __restrict__ clocal, int localBits,
+    int32_t* __restrict__ pairCount, int cap, uint64_t* __restrict__ pkeys) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     bool active = false;
-    float4 qmin, qmax;
-    int node = -1, stop = -1;
+    float4 qmin = make_float4(0.f, 0.f, 0.f, 0.f), qmax = qmin;
+    int node = -1, stop = -1, st = 0, stEnd = 0;
     int myCol = 0;
     uint32_t myEnt = 0;
     int wd = 0;
     if (s < n) {
         int f = lflags[s];
-        if ((f & CF_ENABLE) && (f & CF_DYNAMIC)) {
+        if (f & CF_ENABLE) {
             active = true;
             qmin = lmin[s];
             qmax = lmax[s];
             wd = (int)(mkeys[s] >> worldShift);
             node = worldRoot[wd];
             stop = worldStop[wd];
+            st = worldStaticStart[wd];
+            stEnd = worldStaticStart[wd + 1];
             myCol = order[s];
             myEnt = centity[myCol];
         }
@@ -291,19 +365,17 @@ __global__ __launch_bounds__(256) void k_traverse(
     const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int visits = 0;
     const int maxVisits = 2 * n;   // a depth-first walk visits at most 2n - 1 nodes
-    while (__ballot(active && node != stop)) {
+    while (__ballot(active && (node != stop || st < stEnd))) {
         bool emit = false;
         uint64_t key = 0;
         if (active && node != stop && ++visits > maxVisits) {
             atomicOr(&pairCount[6], 1);   // malformed tree: stop and report
-            active = false;
+            node = stop;
         }
         if (active && node != stop) {
             if (node & LEAF_BIT) {
                 int s2 = node & ~LEAF_BIT;
-                int f2 = lflags[s2];
-                if (s2 != s && (f2 & CF_ENABLE) && (s2 > s || !(f2 & CF_DYNAMIC)) &&
-                    overlaps(qmin, qmax, lmin[s2], lmax[s2])) {
+                if (s2 > s && (lflags[s2] & CF_ENABLE) && overlaps(qmin, qmax, lmin[s2], lmax[s2])) {
                     int c2 = order[s2];
                     uint32_t e2 = centity[c2];
                     if (e2 != myEnt) {
@@ -316,12 +388,21 @@ __global__ __launch_bounds__(256) void k_traverse(
                 }
                 node = leafEsc[s2];
             } else {
-                int2 r = range[node];
-                // subtrees whose leaves all sit at or before s and are all dynamic hold no
-                // pair that this leaf owns (a dynamic pair is emitted by its lower leaf)
-                bool skip = (r.y <= s) && nflags[node] == 0;
-                if (!skip && overlaps(qmin, qmax, nmin[node], nmax[node])) node = child[2 * node];
+                // subtrees whose leaves all sit at or before s hold no pair this leaf owns
+                if (range[node].y > s && overlaps(qmin, qmax, nmin[node], nmax[node])) node = child[2 * node];
                 else node = nodeEsc[node];
+            }
+        } else if (active && st < stEnd) {
+            int c2 = staticCols[st++];
+            if ((cflags[c2] & CF_ENABLE) && overlaps(qmin, qmax, cbmin[c2], cbmax[c2])) {
+                uint32_t e2 = centity[c2];
+                if (e2 != myEnt) {
+                    int c0 = myEnt < e2 ? myCol : c2;
+                    int c1 = myEnt < e2 ? c2 : myCol;
+                    key = ((uint64_t)wd << (2 * localBits)) | ((uint64_t)clocal[c0] << localBits) |
+                          (uint64_t)clocal[c1];
+                    emit = true;
+                }
             }
         }
         uint64_t m = __ballot(emit);
@@ -345,44 +426,51 @@ __global__ void k_single_leaf_root(int32_t* worldRoot, int W, const uint64_t* mk
 
 int bp_find_pairs(World& w) {
     hipStream_t s = w.stream;
-    int n = w.C;
-    if (n == 0) return 0;
+    int n = w.nDynCol;
     int W = w.W;
     int worldShift = 30;
-    // world AABB of collider centres -> Morton codes -> sort
-    hipLaunchKernelGGL(k_world_bounds_init, dim3(div_up(W, 256)), dim3(256), 0, s, W, w.worldBoundsI.p);
-    hipLaunchKernelGGL(k_world_bounds, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.cworld.p, w.cbmin.p, w.cbmax.p,
-                       w.worldBoundsI.p);
-    hipLaunchKernelGGL(k_morton, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.cworld.p, w.cbmin.p, w.cbmax.p,
-                       w.worldBoundsI.p, w.mkeys.p, w.mvals.p);
-    PGX_LAUNCH_CHECK();
-    radix_sort_u64(s, w.mkeys.p, w.mvals.p, n, 30 + w.worldBits, w.sortT);
-    fill_i32(s, w.worldRoot.p, -1, W);
-    hipLaunchKernelGGL(k_leaves, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mvals.p, w.cbmin.p, w.cbmax.p,
-                       w.cflags.p, w.leafMin.p, w.leafMax.p, w.leafFlags.p);
-    if (n == 1) {
-        hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(64), 0, s, w.worldRoot.p, W, w.mkeys.p, worldShift);
-        fill_i32(s, w.leafEsc.p, -1, 1);
-        fill_i32(s, w.leafParent.p, -1, 1);
-    } else {
-        fill_i32(s, w.nodeParent.p, -1, n - 1);
-        fill_i32(s, w.nodeCounter.p, 0, n - 1);
-        hipLaunchKernelGGL(k_build, dim3(div_up(n - 1, 256)), dim3(256), 0, s, n, w.mkeys.p, w.nodeChild.p,
-                           w.nodeRange.p, w.nodeParent.p, w.leafParent.p, w.worldRoot.p, worldShift);
-        hipLaunchKernelGGL(k_refit, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.nodeChild.p, w.nodeParent.p,
-                           w.leafParent.p, w.leafMin.p, w.leafMax.p, w.leafFlags.p, w.nodeMin.p, w.nodeMax.p,
-                           w.nodeFlags.p, w.nodeCounter.p);
-        hipLaunchKernelGGL(k_escape, dim3(div_up(2 * n - 1, 256)), dim3(256), 0, s, n, w.nodeChild.p,
-                           w.nodeParent.p, w.leafParent.p, w.nodeEsc.p, w.leafEsc.p);
-    }
-    hipLaunchKernelGGL(k_world_stop, dim3(div_up(W, 256)), dim3(256), 0, s, W, w.worldRoot.p, w.nodeEsc.p,
-                       w.leafEsc.p, w.worldStop.p);
     fill_i32(s, w.counters.p, 0, 8);
-    hipLaunchKernelGGL(k_traverse, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift, w.mvals.p,
-                       w.leafMin.p, w.leafMax.p, w.leafFlags.p, w.nodeChild.p, w.nodeRange.p, w.nodeMin.p,
-                       w.nodeMax.p, w.nodeFlags.p, w.nodeEsc.p, w.leafEsc.p, w.worldRoot.p, w.worldStop.p,
-                       w.centity.p, w.clocal.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
-    PGX_LAUNCH_CHECK();
+    if (n > 0) {
+        // quantisation box -> Morton codes of the dynamic colliders -> sort
+        hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p);
+        hipLaunchKernelGGL(k_scene_bounds, dim3(std::min(64, div_up(n, 1024))), dim3(1024), 0, s, n, w.dynCols.p,
+                           w.cbmin.p, w.cbmax.p, w.worldBoundsI.p);
+        hipLaunchKernelGGL(k_morton, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.dynCols.p, w.cworld.p, w.cbmin.p,
+                           w.cbmax.p, w.worldBoundsI.p, w.mkeys.p, w.mvals.p);
+        PGX_LAUNCH_CHECK();
+        radix_sort_u64(s, w.mkeys.p, w.mvals.p, n, 30 + w.worldBits, w.sortT);
+        fill_i32(s, w.worldRoot.p, -1, W);
+        hipLaunchKernelGGL(k_leaves, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mvals.p, w.cbmin.p, w.cbmax.p,
+                           w.cflags.p, w.leafMin.p, w.leafMax.p, w.leafFlags.p);
+        if (n == 1) {
+            hipLaunchKernelGGL(k_single_leaf_root, dim3(1), dim3(64), 0, s, w.worldRoot.p, W, w.mkeys.p, worldShift);
+            fill_i32(s, w.leafEsc.p, -1, 1);
+            fill_i32(s, w.leafParent.p, -1, 1);
+        } else {
+            fill_i32(s, w.nodeParent.p, -1, n - 1);
+            fill_i32(s, w.nodeCounter.p, 0, n - 1);
+            hipLaunchKernelGGL(k_build, dim3(div_up(n - 1, 256)), dim3(256), 0, s, n, w.mkeys.p, w.nodeChild.p,
+                               w.nodeRange.p, w.nodeParent.p, w.leafParent.p, w.worldRoot.p, worldShift);
+            hipLaunchKernelGGL(k_refit_local, dim3(div_up(n, REFIT_BLOCK)), dim3(REFIT_BLOCK), 0, s, n, w.nodeChild.p,
+                               w.nodeRange.p, w.nodeParent.p, w.leafParent.p, w.leafMin.p, w.leafMax.p, w.nodeMin.p,
+                               w.nodeMax.p, w.nodeCounter.p);
+            hipLaunchKernelGGL(k_refit_ready, dim3(div_up(n - 1, 256)), dim3(256), 0, s, n, w.nodeCounter.p,
+                               w.refitReady.p, w.counters.p + 7);
+            hipLaunchKernelGGL(k_refit_global, dim3(1), dim3(1024), 0, s, n, w.nodeChild.p, w.nodeParent.p,
+                               w.leafMin.p, w.leafMax.p, w.nodeMin.p, w.nodeMax.p, w.nodeCounter.p, w.refitReady.p,
+                               w.counters.p + 7);
+            hipLaunchKernelGGL(k_escape, dim3(div_up(2 * n - 1, 256)), dim3(256), 0, s, n, w.nodeChild.p,
+                               w.nodeParent.p, w.leafParent.p, w.nodeEsc.p, w.leafEsc.p);
+        }
+        hipLaunchKernelGGL(k_world_stop, dim3(div_up(W, 256)), dim3(256), 0, s, W, w.worldRoot.p, w.nodeEsc.p,
+                           w.leafEsc.p, w.worldStop.p);
+        hipLaunchKernelGGL(k_traverse, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift, w.mvals.p,
+                           w.leafMin.p, w.leafMax.p, w.leafFlags.p, w.nodeChild.p, w.nodeRange.p, w.nodeMin.p,
+                           w.nodeMax.p, w.nodeEsc.p, w.leafEsc.p, w.worldRoot.p, w.worldStop.p,
+                           w.worldStaticStart.p, w.staticCols.p, w.cbmin.p, w.cbmax.p, w.cflags.p, w.centity.p,
+                           w.clocal.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
+        PGX_LAUNCH_CHECK();
+    }
     int cnt[8] = {0};
     PGX_HIP(hipMemcpyAsync(cnt, w.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
     PGX_HIP(hipStreamSynchronize(s));

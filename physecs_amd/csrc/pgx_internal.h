@@ -97,6 +97,7 @@ struct World {
     DBuf<float4> dvel, dang;             // D component velocity (dvel.w = invMass)
     DBuf<float4> dcom;                   // D local com (w = kinematic ? 1 : 0)
     DBuf<float4> dinvI;                  // 3 x D columns of the local inverse inertia
+    DBuf<float4> cvel, cang;             // D component velocity after gravity (cvel.w = invMass)
     DBuf<float4> tvel, tang;             // D solver velocities (velocityTemp)
     DBuf<float4> pvel, pang;             // D pseudo velocities (pang.w = constraint count)
     DBuf<float4> winvI;                  // 3 x D world inverse inertia (massTemp)
@@ -108,6 +109,11 @@ struct World {
     DBuf<float4> cparams, clpos, clquat, cmat;
     DBuf<float4> cbmin, cbmax;
     DBuf<int32_t> worldColStart, colOfLocal;   // (world, local) -> collider
+    int nDynCol = 0;                     // colliders of non-kinematic dynamic bodies (the LBVH leaves)
+    DBuf<int32_t> dynCols;               // those colliders
+    DBuf<int32_t> staticCols;            // the other colliders, grouped by world
+    DBuf<int32_t> worldStaticStart;      // W + 1 offsets into staticCols
+    DBuf<int32_t> refitReady;
 
     // broadphase
     DBuf<uint32_t> worldBoundsI;         // W x 6 orderable-int encoded min/max of centres
@@ -122,7 +128,7 @@ struct World {
     DBuf<int32_t> worldRoot, worldStop;
     DBuf<uint64_t> pkeys;
     DBuf<int32_t> pvals;
-    DBuf<int32_t> counters;              // [0] pairs, [1] manifolds, [2] colors, [3] uncolored, [4] points, [5] overflow
+    DBuf<int32_t> counters;              // [0] pairs, [1] manifolds, [3] uncolored, [5] colour overflow, [6] bvh error, [7] refit ready
     SortTemp sortT;
     ScanTemp scanT;
 
@@ -165,7 +171,9 @@ struct World {
     int lastPairs = 0, lastManifolds = 0, lastColors = 0, lastRounds = 0, lastPoints = 0;
     std::vector<int> colorStartHost;
     bool timing = false;
-    hipEvent_t ev[6] = {};
+    bool persistent = true;              // substep loop as one cooperative launch per step
+    int persistentBlocks = 0, persistentBlocksWanted = 0;
+    hipEvent_t ev[8] = {};
     float tBroad = 0, tNarrow = 0, tColor = 0, tSolve = 0, tTotal = 0;
     // per-launch timing of the contact solve (first substep of each timed step)
     DBuf<int32_t> colorPoints;           // contact points per colour (this step)
@@ -174,6 +182,7 @@ struct World {
     int solveEvUsed = 0;
     long statSteps = 0, statSolveLaunches = 0;
     double statSolveUs = 0, statSolveBytes = 0, statStepUs = 0;
+    long lastPointsHost = 0;
 };
 
 // broadphase.hip
@@ -188,5 +197,6 @@ int sv_setup_manifolds(World& w, int nPairs);          // returns manifold count
 int sv_color(World& w, int nManifolds);                 // returns colour count
 void sv_substeps(World& w, int nManifolds, float dt);
 void sv_finish_step(World& w, int nManifolds);
+double sv_step_bytes(const World& w, int M, long points);
 
 }  // namespace pgx

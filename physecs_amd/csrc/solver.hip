@@ -15,7 +15,10 @@
 // the same result as a sequential pass over it in any order. The solve order is
 // "colour 0, colour 1, ..." and is exported (pgx_world_get_manifolds) so that a
 // sequential solver fed this order reproduces the device result bit for bit.
+#include <hip/hip_cooperative_groups.h>
 #include "pgx_internal.h"
+
+namespace cg = cooperative_groups;
 
 namespace pgx {
 
@@ -224,29 +227,58 @@ __global__ __launch_bounds__(256) void k_setup(
     atomicAdd(&colorPoints[mfColor[m]], num);
 }
 
-// ------------------------------------------------------------------ substep kernels
+// ------------------------------------------------------------------ substep pieces
 // per-point row storage: field f of point k of manifold s at ((k * 9 + f) * Mcap + s)
 enum { R_R0XN = 0, R_R1XN, R_T, R_R0XT, R_R1XT, R_R0XNT, R_R1XNT, R_R0XTT, R_R1XTT, R_NF };
-#define ROW(f, k) sRow[((size_t)((k) * R_NF + (f))) * Mcap + s]
+#define ROW(f, k) a.sRow[((size_t)((k) * R_NF + (f))) * a.Mcap + s]
 
-__global__ __launch_bounds__(256) void k_update_contacts(
-    int M, int Mcap, const int32_t* __restrict__ sB0, const int32_t* __restrict__ sB1,
-    const int32_t* __restrict__ sA0, const int32_t* __restrict__ sA1, const int32_t* __restrict__ sFlags,
-    const float4* __restrict__ sN, const float4* __restrict__ sR0, const float4* __restrict__ sR1,
-    const float4* __restrict__ bpos, const float4* __restrict__ bquat, const float4* __restrict__ dvel,
-    const float4* __restrict__ dang, const float4* __restrict__ dcom, float4* __restrict__ sRow) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    int b0 = sB0[s], b1 = sB1[s], a0 = sA0[s], a1 = sA1[s];
-    int num = sFlags[s] & 255;
-    f3 n = xyz(sN[s]);
-    qf q0 = q4(bquat[a0]), q1 = q4(bquat[a1]);
+struct SubArgs {
+    int M, Mcap, D, nColors, substeps, iterations;
+    float h, g;
+    const int32_t *sB0, *sB1, *sA0, *sA1, *sFlags, *colorStart;
+    const float4 *sN, *sR0, *sR1;
+    const float2* sSoft;
+    float4 *sRow;
+    float4 *bpos, *bquat;                 // body poses (dynamic bodies come first)
+    float4 *dvel, *dang;                  // component velocity at the start of the substep (dvel.w = invMass)
+    float4 *cvel, *cang;                  // component velocity after gravity + gyroscopic term (friction reads it)
+    const float4 *dcom, *dinvI;
+    float4 *tvel, *tang;                  // solver velocities (velocityTemp)
+    float4 *pvel, *pang;                  // pseudo velocities (pang.w = constraint count)
+};
+
+__device__ __forceinline__ m3 world_inv_inertia(const SubArgs& a, int b) {
+    m3 rot = mat3_cast(q4(a.bquat[b]));
+    m3 invI = mk_m3(xyz(a.dinvI[3 * b]), xyz(a.dinvI[3 * b + 1]), xyz(a.dinvI[3 * b + 2]));
+    return mulmm(mulmm(rot, invI), transpose(rot));   // rot * invI * invRot, src/Physecs.cpp:467
+}
+
+// update contact rows (src/Physecs.cpp:368-426) then preSolve (src/ContactConstraints.cpp:4-31)
+// of one manifold; the world inverse inertia is rebuilt with the same operations as massTemp.
+__device__ void prepare_manifold(const SubArgs& a, int s) {
+    int b0 = a.sB0[s], b1 = a.sB1[s], a0 = a.sA0[s], a1 = a.sA1[s];
+    int num = a.sFlags[s] & 255;
+    f3 n = xyz(a.sN[s]);
+    qf q0 = q4(a.bquat[a0]), q1 = q4(a.bquat[a1]);
     f3 com0 = s3(0.f), v0 = s3(0.f), w0 = s3(0.f), com1 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
-    if (b0 >= 0) { com0 = add(xyz(bpos[a0]), qrot(q0, xyz(dcom[b0]))); v0 = xyz(dvel[b0]); w0 = xyz(dang[b0]); }
-    if (b1 >= 0) { com1 = add(xyz(bpos[a1]), qrot(q1, xyz(dcom[b1]))); v1 = xyz(dvel[b1]); w1 = xyz(dang[b1]); }
+    float im0 = 0.f, im1 = 0.f;
+    m3 I0 = mzero(), I1 = mzero();
+    if (b0 >= 0) {
+        com0 = add(xyz(a.bpos[a0]), qrot(q0, xyz(a.dcom[b0])));
+        float4 vv = a.dvel[b0];
+        v0 = xyz(vv); im0 = vv.w; w0 = xyz(a.dang[b0]);
+        I0 = world_inv_inertia(a, b0);
+    }
+    if (b1 >= 0) {
+        com1 = add(xyz(a.bpos[a1]), qrot(q1, xyz(a.dcom[b1])));
+        float4 vv = a.dvel[b1];
+        v1 = xyz(vv); im1 = vv.w; w1 = xyz(a.dang[b1]);
+        I1 = world_inv_inertia(a, b1);
+    }
+    float nn = dot(n, n);
     for (int k = 0; k < num; ++k) {
-        f3 r0 = qrot(q0, xyz(sR0[(size_t)k * Mcap + s]));
-        f3 r1 = qrot(q1, xyz(sR1[(size_t)k * Mcap + s]));
+        f3 r0 = qrot(q0, xyz(a.sR0[(size_t)k * a.Mcap + s]));
+        f3 r1 = qrot(q1, xyz(a.sR1[(size_t)k * a.Mcap + s]));
         f3 cp0 = add(com0, r0), cp1 = add(com1, r1);
         float cn = dot(sub(cp1, cp0), n);
         f3 r0xn = cross(r0, n), r1xn = cross(r1, n);
@@ -255,14 +287,23 @@ __global__ __launch_bounds__(256) void k_update_contacts(
         f3 t = sub(rel, mul(relN, n));
         float tl = length(t);
         if (tl != 0.f) t = divs(t, tl);
+        f3 r0xt = cross(r0, t), r1xt = cross(r1, t);
+        f3 r0xnt = mulmv(I0, r0xn), r1xnt = mulmv(I1, r1xn), r0xtt = mulmv(I0, r0xt), r1xtt = mulmv(I1, r1xt);
+        float invN = nn * (im0 + im1) + dot(r0xn, r0xnt) + dot(r1xn, r1xnt);
+        float invT = dot(t, t) * (im0 + im1) + dot(r0xt, r0xtt) + dot(r1xt, r1xtt);
         ROW(R_R0XN, k) = f4(r0xn, cn);
-        ROW(R_R1XN, k) = f4(r1xn, 0.f);
-        ROW(R_T, k) = f4(t, 0.f);
-        ROW(R_R0XT, k) = f4(cross(r0, t), 0.f);     // .w = lambdaN
-        ROW(R_R1XT, k) = f4(cross(r1, t), 0.f);     // .w = lambdaT
+        ROW(R_R1XN, k) = f4(r1xn, invN);
+        ROW(R_T, k) = f4(t, invT);
+        ROW(R_R0XT, k) = f4(r0xt, 0.f);     // .w = lambdaN
+        ROW(R_R1XT, k) = f4(r1xt, 0.f);     // .w = lambdaT
+        ROW(R_R0XNT, k) = f4(r0xnt, 0.f);
+        ROW(R_R1XNT, k) = f4(r1xnt, 0.f);
+        ROW(R_R0XTT, k) = f4(r0xtt, 0.f);
+        ROW(R_R1XTT, k) = f4(r1xtt, 0.f);
     }
 }
 
+// src/MathUtil.h:10-21
 __device__ __forceinline__ f3 solve33(const m3& A, f3 b) {
     f3 c12 = cross(A.c1, A.c2);
     float det = dot(A.c0, c12);
@@ -271,102 +312,63 @@ __device__ __forceinline__ f3 solve33(const m3& A, f3 b) {
     return mk3(id * dot(b, c12), id * dot(A.c0, cross(b, A.c2)), id * dot(A.c0, cross(A.c1, b)));
 }
 
-__global__ __launch_bounds__(256) void k_integrate_velocities(int D, float h, float g, const float4* __restrict__ bquat,
-                                                              float4* __restrict__ dvel, float4* __restrict__ dang,
-                                                              const float4* __restrict__ dcom,
-                                                              const float4* __restrict__ dinvI,
-                                                              float4* __restrict__ tvel, float4* __restrict__ tang,
-                                                              float4* __restrict__ pvel, float4* __restrict__ pang,
-                                                              float4* __restrict__ winvI) {
-    int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= D) return;
-    if (dcom[d].w != 0.f) return;   // kinematic
-    m3 rot = mat3_cast(q4(bquat[d]));
+// integrate velocities of one body, src/Physecs.cpp:443-468
+__device__ void integrate_velocity(const SubArgs& a, int d) {
+    if (a.dcom[d].w != 0.f) return;   // kinematic
+    m3 rot = mat3_cast(q4(a.bquat[d]));
     m3 invRot = transpose(rot);
-    float4 vv = dvel[d];
-    f3 v = add(xyz(vv), mul(h, mk3(0.f, -g, 0.f)));
-    m3 invI = mk_m3(xyz(dinvI[3 * d]), xyz(dinvI[3 * d + 1]), xyz(dinvI[3 * d + 2]));
-    f3 wl = mulmv(invRot, xyz(dang[d]));
+    float4 vv = a.dvel[d];
+    f3 v = add(xyz(vv), mul(a.h, mk3(0.f, -a.g, 0.f)));
+    m3 invI = mk_m3(xyz(a.dinvI[3 * d]), xyz(a.dinvI[3 * d + 1]), xyz(a.dinvI[3 * d + 2]));
+    f3 wl = mulmv(invRot, xyz(a.dang[d]));
     m3 I = inverse(invI);
     f3 Iw = mulmv(I, wl);
-    f3 f = mul(h, cross(wl, Iw));
-    m3 J = madd(I, mscale(h, msub(mulmm(matrixCross3(wl), I), matrixCross3(Iw))));
+    f3 f = mul(a.h, cross(wl, Iw));
+    m3 J = madd(I, mscale(a.h, msub(mulmm(matrixCross3(wl), I), matrixCross3(Iw))));
     wl = sub(wl, solve33(J, f));
     f3 w = mulmv(rot, wl);
-    dvel[d] = f4(v, vv.w);
-    dang[d] = f4(w, 0.f);
-    tvel[d] = f4(v, vv.w);
-    tang[d] = f4(w, 0.f);
-    pvel[d] = make_float4(0.f, 0.f, 0.f, 0.f);
-    pang[d] = make_float4(0.f, 0.f, 0.f, 0.f);
-    m3 wI = mulmm(mulmm(rot, invI), invRot);
-    winvI[3 * d] = f4(wI.c0, 0.f);
-    winvI[3 * d + 1] = f4(wI.c1, 0.f);
-    winvI[3 * d + 2] = f4(wI.c2, 0.f);
+    a.cvel[d] = f4(v, vv.w);
+    a.cang[d] = f4(w, 0.f);
+    a.tvel[d] = f4(v, 0.f);
+    a.tang[d] = f4(w, 0.f);
+    a.pvel[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+    a.pang[d] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-__global__ __launch_bounds__(256) void k_presolve(int M, int Mcap, const int32_t* __restrict__ sB0,
-                                                  const int32_t* __restrict__ sB1, const int32_t* __restrict__ sFlags,
-                                                  const float4* __restrict__ sN, const float4* __restrict__ dvel,
-                                                  const float4* __restrict__ winvI, float4* __restrict__ sRow) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    int b0 = sB0[s], b1 = sB1[s];
-    int num = sFlags[s] & 255;
-    f3 n = xyz(sN[s]);
-    float im0 = 0.f, im1 = 0.f;
-    m3 I0 = mzero(), I1 = mzero();
-    if (b0 >= 0) { im0 = dvel[b0].w; I0 = mk_m3(xyz(winvI[3 * b0]), xyz(winvI[3 * b0 + 1]), xyz(winvI[3 * b0 + 2])); }
-    if (b1 >= 0) { im1 = dvel[b1].w; I1 = mk_m3(xyz(winvI[3 * b1]), xyz(winvI[3 * b1 + 1]), xyz(winvI[3 * b1 + 2])); }
-    float nn = dot(n, n);
-    for (int k = 0; k < num; ++k) {
-        float4 a = ROW(R_R0XN, k), b = ROW(R_R1XN, k), t = ROW(R_T, k), c = ROW(R_R0XT, k), e = ROW(R_R1XT, k);
-        f3 r0xn = xyz(a), r1xn = xyz(b), tt = xyz(t), r0xt = xyz(c), r1xt = xyz(e);
-        f3 r0xnt = mulmv(I0, r0xn), r1xnt = mulmv(I1, r1xn), r0xtt = mulmv(I0, r0xt), r1xtt = mulmv(I1, r1xt);
-        float invN = nn * (im0 + im1) + dot(r0xn, r0xnt) + dot(r1xn, r1xnt);
-        float invT = dot(tt, tt) * (im0 + im1) + dot(r0xt, r0xtt) + dot(r1xt, r1xtt);
-        ROW(R_R1XN, k) = f4(r1xn, invN);
-        ROW(R_T, k) = f4(tt, invT);
-        ROW(R_R0XNT, k) = f4(r0xnt, 0.f);
-        ROW(R_R1XNT, k) = f4(r1xnt, 0.f);
-        ROW(R_R0XTT, k) = f4(r0xtt, 0.f);
-        ROW(R_R1XTT, k) = f4(r1xtt, 0.f);
-    }
-}
-
-// One colour: every manifold of [begin, end) touches bodies no other manifold of the
-// range touches, so the range is solved in parallel exactly as a sequential pass would.
-__global__ __launch_bounds__(256) void k_solve(int begin, int end, int Mcap, int useBias, float h, int skipSoft,
-                                               const int32_t* __restrict__ sB0, const int32_t* __restrict__ sB1,
-                                               const int32_t* __restrict__ sFlags, const float4* __restrict__ sN,
-                                               const float2* __restrict__ sSoft, const float4* __restrict__ sR0,
-                                               const float4* __restrict__ dvel, const float4* __restrict__ dang,
-                                               float4* __restrict__ tvel, float4* __restrict__ tang,
-                                               float4* __restrict__ sRow) {
-    int s = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= end) return;
-    int flags = sFlags[s];
+// One manifold's normal then friction rows, src/ContactConstraints.cpp:33-124. Manifolds of
+// one colour touch disjoint dynamic bodies, so a colour is solved in parallel exactly as a
+// sequential pass over it would be.
+__device__ void solve_manifold(const SubArgs& a, int s, bool useBias, float h, bool skipSoft) {
+    int flags = a.sFlags[s];
     bool soft = (flags & 256) != 0;
     if (skipSoft && soft) return;
     int num = flags & 255;
-    int b0 = sB0[s], b1 = sB1[s];
-    float4 nf = sN[s];
+    int b0 = a.sB0[s], b1 = a.sB1[s];
+    float4 nf = a.sN[s];
     f3 n = xyz(nf);
     float friction = nf.w;
-    float im0 = b0 >= 0 ? dvel[b0].w : 0.f, im1 = b1 >= 0 ? dvel[b1].w : 0.f;
+    float im0 = 0.f, im1 = 0.f;
     f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
-    if (b0 >= 0) { v0 = xyz(tvel[b0]); w0 = xyz(tang[b0]); }
-    if (b1 >= 0) { v1 = xyz(tvel[b1]); w1 = xyz(tang[b1]); }
-    float2 sp = sSoft[s];
+    f3 cv0 = s3(0.f), cw0 = s3(0.f), cv1 = s3(0.f), cw1 = s3(0.f);
+    if (b0 >= 0) {
+        float4 c = a.cvel[b0];
+        im0 = c.w; cv0 = xyz(c); cw0 = xyz(a.cang[b0]);
+        v0 = xyz(a.tvel[b0]); w0 = xyz(a.tang[b0]);
+    }
+    if (b1 >= 0) {
+        float4 c = a.cvel[b1];
+        im1 = c.w; cv1 = xyz(c); cw1 = xyz(a.cang[b1]);
+        v1 = xyz(a.tvel[b1]); w1 = xyz(a.tang[b1]);
+    }
+    float2 sp = a.sSoft[s];
     float lN[4], lT[4];
-    // normal rows, src/ContactConstraints.cpp:84-133
     for (int k = 0; k < num; ++k) {
-        float4 a = ROW(R_R0XN, k), b = ROW(R_R1XN, k);
+        float4 r0 = ROW(R_R0XN, k), r1 = ROW(R_R1XN, k);
         lN[k] = ROW(R_R0XT, k).w;
-        float invN = b.w;
+        float invN = r1.w;
         if (invN == 0.f) continue;
-        f3 r0xn = xyz(a), r1xn = xyz(b);
-        float c = a.w;
+        f3 r0xn = xyz(r0), r1xn = xyz(r1);
+        float c = r0.w;
         float rv = dot(neg(n), v0) + dot(neg(r0xn), w0) + dot(n, v1) + dot(r1xn, w1);
         float effMass = 1.f / invN;
         float lambda;
@@ -378,7 +380,7 @@ __global__ __launch_bounds__(256) void k_solve(int begin, int end, int Mcap, int
             float beta = h * stiffness / (damping + h * stiffness);
             lambda = (rv + beta * c / h) / (invN + gamma / h);
         } else {
-            float tv = sR0[(size_t)k * Mcap + s].w;
+            float tv = a.sR0[(size_t)k * a.Mcap + s].w;
             lambda = (rv - tv + (useBias ? 0.1f * c / h : 0.f)) * effMass;
         }
         float prev = lN[k];
@@ -388,10 +390,7 @@ __global__ __launch_bounds__(256) void k_solve(int begin, int end, int Mcap, int
         if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, n)); w0 = add(w0, mul(lambda, xyz(ROW(R_R0XNT, k)))); }
         if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, n)); w1 = sub(w1, mul(lambda, xyz(ROW(R_R1XNT, k)))); }
     }
-    // friction rows read the component velocities, src/ContactConstraints.cpp:136-173
-    f3 cv0 = s3(0.f), cw0 = s3(0.f), cv1 = s3(0.f), cw1 = s3(0.f);
-    if (b0 >= 0) { cv0 = xyz(dvel[b0]); cw0 = xyz(dang[b0]); }
-    if (b1 >= 0) { cv1 = xyz(dvel[b1]); cw1 = xyz(dang[b1]); }
+    // friction rows read the component velocities (src/ContactConstraints.cpp:142-152)
     for (int k = 0; k < num; ++k) {
         float4 t4 = ROW(R_T, k);
         lT[k] = ROW(R_R1XT, k).w;
@@ -415,47 +414,89 @@ __global__ __launch_bounds__(256) void k_solve(int begin, int end, int Mcap, int
         ROW(R_R0XT, k) = c;
         ROW(R_R1XT, k) = e;
     }
-    if (b0 >= 0) { tvel[b0] = f4(v0, 0.f); tang[b0] = f4(w0, 0.f); }
-    if (b1 >= 0) { tvel[b1] = f4(v1, 0.f); tang[b1] = f4(w1, 0.f); }
+    if (b0 >= 0) { a.tvel[b0] = f4(v0, 0.f); a.tang[b0] = f4(w0, 0.f); }
+    if (b1 >= 0) { a.tvel[b1] = f4(v1, 0.f); a.tang[b1] = f4(w1, 0.f); }
 }
 
-__global__ __launch_bounds__(256) void k_integrate_positions(int D, float h, float4* __restrict__ bpos,
-                                                             float4* __restrict__ bquat,
-                                                             const float4* __restrict__ dcom,
-                                                             const float4* __restrict__ tvel,
-                                                             const float4* __restrict__ tang,
-                                                             const float4* __restrict__ pvel,
-                                                             const float4* __restrict__ pang) {
-    int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= D) return;
-    float4 cm = dcom[d];
+// integrate positions of one body, src/Physecs.cpp:494-511
+__device__ void integrate_position(const SubArgs& a, int d) {
+    float4 cm = a.dcom[d];
     if (cm.w != 0.f) return;
-    float4 pa = pang[d];
+    float4 pa = a.pang[d];
     float cnt = pa.w;
     float sc = cnt != 0.f ? 1.f / cnt : 1.f;
     f3 com = xyz(cm);
-    f3 pos = xyz(bpos[d]);
-    qf q = q4(bquat[d]);
-    pos = add(pos, add(mul(h, xyz(tvel[d])), mul(sc, xyz(pvel[d]))));
+    f3 pos = xyz(a.bpos[d]);
+    qf q = q4(a.bquat[d]);
+    pos = add(pos, add(mul(a.h, xyz(a.tvel[d])), mul(sc, xyz(a.pvel[d]))));
     f3 prevCom = qrot(q, com);
-    f3 hv = mul(0.5f, add(mul(h, xyz(tang[d])), mul(sc, xyz(pa))));
+    f3 hv = mul(0.5f, add(mul(a.h, xyz(a.tang[d])), mul(sc, xyz(pa))));
     qf dq = qmul(mkq(0.f, hv.x, hv.y, hv.z), q);
     q = mkq(q.w + dq.w, q.x + dq.x, q.y + dq.y, q.z + dq.z);
     q = qnormalize(q);
     pos = add(pos, sub(prevCom, qrot(q, com)));
-    bpos[d] = f4(pos, 0.f);
-    bquat[d] = f4q(q);
+    a.bpos[d] = f4(pos, 0.f);
+    a.bquat[d] = f4q(q);
 }
 
-__global__ void k_writeback(int D, float4* __restrict__ dvel, float4* __restrict__ dang,
-                            const float4* __restrict__ dcom, const float4* __restrict__ tvel,
-                            const float4* __restrict__ tang) {
+// write back, src/Physecs.cpp:523-530
+__device__ void write_back(const SubArgs& a, int d) {
+    if (a.dcom[d].w != 0.f) return;
+    float4 v = a.tvel[d];
+    a.dvel[d] = make_float4(v.x, v.y, v.z, a.dvel[d].w);
+    a.dang[d] = a.tang[d];
+}
+
+// ------------------------------------------------------------------ discrete kernels
+__global__ __launch_bounds__(256) void k_prepare(SubArgs a) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.M) prepare_manifold(a, i);
+    else if (i - a.M < a.D) integrate_velocity(a, i - a.M);
+}
+__global__ __launch_bounds__(256) void k_solve(SubArgs a, int begin, int end, int useBias, float h, int skipSoft) {
+    int s = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < end) solve_manifold(a, s, useBias != 0, h, skipSoft != 0);
+}
+__global__ __launch_bounds__(256) void k_integrate_positions(SubArgs a) {
     int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= D) return;
-    if (dcom[d].w != 0.f) return;
-    float4 v = tvel[d];
-    dvel[d] = make_float4(v.x, v.y, v.z, dvel[d].w);
-    dang[d] = tang[d];
+    if (d < a.D) integrate_position(a, d);
+}
+__global__ __launch_bounds__(256) void k_writeback(SubArgs a) {
+    int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < a.D) write_back(a, d);
+}
+
+// ------------------------------------------------------------------ persistent kernel
+// Every substep of one step in a single cooperative launch; grid-wide barriers separate the
+// phases and the colours. Blocks are co-resident (cooperative launch), every block reaches
+// every barrier the same number of times, and the kernel ends after the last substep.
+__global__ __launch_bounds__(256) void k_step_persistent(SubArgs a) {
+    cg::grid_group grid = cg::this_grid();
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nth = gridDim.x * blockDim.x;
+    for (int sub = 0; sub < a.substeps; ++sub) {
+        for (int i = tid; i < a.M + a.D; i += nth) {
+            if (i < a.M) prepare_manifold(a, i);
+            else integrate_velocity(a, i - a.M);
+        }
+        grid.sync();
+        for (int it = 0; it < a.iterations; ++it) {
+            for (int c = 0; c < a.nColors; ++c) {
+                int b = a.colorStart[c], e = a.colorStart[c + 1];
+                for (int s = b + tid; s < e; s += nth) solve_manifold(a, s, true, a.h, false);
+                grid.sync();
+            }
+        }
+        for (int d = tid; d < a.D; d += nth) integrate_position(a, d);
+        grid.sync();
+        for (int c = 0; c < a.nColors; ++c) {
+            int b = a.colorStart[c], e = a.colorStart[c + 1];
+            for (int s = b + tid; s < e; s += nth) solve_manifold(a, s, false, 0.f, true);
+            grid.sync();
+        }
+        for (int d = tid; d < a.D; d += nth) write_back(a, d);
+        grid.sync();
+    }
 }
 
 // ------------------------------------------------------------------ host orchestration
@@ -539,71 +580,68 @@ int sv_color(World& w, int M) {
     return nColors;
 }
 
-// HIP events around one k_solve launch (timing mode, first substep only)
-static void solve_event(World& w, bool rec, int color, bool after) {
-    if (!rec) return;
-    int idx = 2 * w.solveEvUsed + (after ? 1 : 0);
-    while ((int)w.solveEv.size() <= idx) {
-        hipEvent_t e;
-        PGX_HIP(hipEventCreate(&e));
-        w.solveEv.push_back(e);
-    }
-    PGX_HIP(hipEventRecord(w.solveEv[idx], w.stream));
-    if (after) {
-        if ((int)w.solveEvColor.size() <= w.solveEvUsed) w.solveEvColor.resize(w.solveEvUsed + 1);
-        w.solveEvColor[w.solveEvUsed] = color;
-        ++w.solveEvUsed;
-    }
+static SubArgs make_args(World& w, int M, float dt) {
+    SubArgs a;
+    a.M = M; a.Mcap = w.manifoldCap; a.D = w.D;
+    a.nColors = (int)w.colorStartHost.size() - 1;
+    a.substeps = w.substeps; a.iterations = w.iterations;
+    a.h = dt / w.substeps; a.g = w.g;
+    a.sB0 = w.sB0.p; a.sB1 = w.sB1.p; a.sA0 = w.sA0.p; a.sA1 = w.sA1.p; a.sFlags = w.sFlags.p;
+    a.colorStart = w.colorCount.p;
+    a.sN = w.sN.p; a.sR0 = w.sR0.p; a.sR1 = w.sR1.p; a.sSoft = w.sSoft.p; a.sRow = w.sRow.p;
+    a.bpos = w.bpos.p; a.bquat = w.bquat.p; a.dvel = w.dvel.p; a.dang = w.dang.p;
+    a.cvel = w.cvel.p; a.cang = w.cang.p; a.dcom = w.dcom.p; a.dinvI = w.dinvI.p;
+    a.tvel = w.tvel.p; a.tang = w.tang.p; a.pvel = w.pvel.p; a.pang = w.pang.p;
+    return a;
+}
+
+// Largest co-resident grid for the persistent kernel (cached per world).
+static int persistent_blocks(World& w) {
+    if (w.persistentBlocks > 0) return w.persistentBlocks;
+    int perCU = 0, cus = 0;
+    PGX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void*)k_step_persistent, 256, 0));
+    PGX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, w.device));
+    int target = w.persistentBlocksWanted > 0 ? w.persistentBlocksWanted : cus;   // one block per CU by default
+    w.persistentBlocks = std::max(1, std::min(target, perCU * cus));
+    return w.persistentBlocks;
 }
 
 void sv_substeps(World& w, int M, float dt) {
     hipStream_t s = w.stream;
-    int D = w.D;
-    int Mcap = w.manifoldCap;
-    float h = dt / w.substeps;
-    int nbM = div_up(std::max(M, 1), 256), nbD = div_up(std::max(D, 1), 256);
-    int nColors = (int)w.colorStartHost.size() - 1;
-    w.solveEvUsed = 0;
+    SubArgs a = make_args(w, M, dt);
+    if (w.persistent) {
+        int blocks = persistent_blocks(w);
+        void* args[] = {&a};
+        if (w.timing) PGX_HIP(hipEventRecord(w.ev[6], s));
+        PGX_HIP(hipLaunchCooperativeKernel((const void*)k_step_persistent, dim3(blocks), dim3(256), args, 0, s));
+        if (w.timing) PGX_HIP(hipEventRecord(w.ev[7], s));
+        return;
+    }
+    int nbP = div_up(std::max(M + w.D, 1), 256), nbD = div_up(std::max(w.D, 1), 256);
+    int nColors = a.nColors;
     for (int sub = 0; sub < w.substeps; ++sub) {
-        const bool rec = w.timing && sub == 0;
-        if (M)
-            hipLaunchKernelGGL(k_update_contacts, dim3(nbM), dim3(256), 0, s, M, Mcap, w.sB0.p, w.sB1.p, w.sA0.p,
-                               w.sA1.p, w.sFlags.p, w.sN.p, w.sR0.p, w.sR1.p, w.bpos.p, w.bquat.p, w.dvel.p,
-                               w.dang.p, w.dcom.p, w.sRow.p);
-        if (D)
-            hipLaunchKernelGGL(k_integrate_velocities, dim3(nbD), dim3(256), 0, s, D, h, w.g, w.bquat.p, w.dvel.p,
-                               w.dang.p, w.dcom.p, w.dinvI.p, w.tvel.p, w.tang.p, w.pvel.p, w.pang.p, w.winvI.p);
-        if (M)
-            hipLaunchKernelGGL(k_presolve, dim3(nbM), dim3(256), 0, s, M, Mcap, w.sB0.p, w.sB1.p, w.sFlags.p, w.sN.p,
-                               w.dvel.p, w.winvI.p, w.sRow.p);
-        for (int it = 0; it < w.iterations; ++it) {
+        hipLaunchKernelGGL(k_prepare, dim3(nbP), dim3(256), 0, s, a);
+        for (int it = 0; it < w.iterations; ++it)
             for (int c = 0; c < nColors; ++c) {
                 int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
-                if (e <= b) continue;
-                solve_event(w, rec, c, false);
-                hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 256)), dim3(256), 0, s, b, e, Mcap, 1, h, 0, w.sB0.p,
-                                   w.sB1.p, w.sFlags.p, w.sN.p, w.sSoft.p, w.sR0.p, w.dvel.p, w.dang.p, w.tvel.p,
-                                   w.tang.p, w.sRow.p);
-                solve_event(w, rec, c, true);
+                if (e > b) hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 256)), dim3(256), 0, s, a, b, e, 1, a.h, 0);
             }
-        }
-        if (D)
-            hipLaunchKernelGGL(k_integrate_positions, dim3(nbD), dim3(256), 0, s, D, h, w.bpos.p, w.bquat.p, w.dcom.p,
-                               w.tvel.p, w.tang.p, w.pvel.p, w.pang.p);
+        hipLaunchKernelGGL(k_integrate_positions, dim3(nbD), dim3(256), 0, s, a);
         for (int c = 0; c < nColors; ++c) {
             int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
-            if (e <= b) continue;
-            solve_event(w, rec, c, false);
-            hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 256)), dim3(256), 0, s, b, e, Mcap, 0, 0.f, 1, w.sB0.p,
-                               w.sB1.p, w.sFlags.p, w.sN.p, w.sSoft.p, w.sR0.p, w.dvel.p, w.dang.p, w.tvel.p,
-                               w.tang.p, w.sRow.p);
-            solve_event(w, rec, c, true);
+            if (e > b) hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 256)), dim3(256), 0, s, a, b, e, 0, 0.f, 1);
         }
-        if (D)
-            hipLaunchKernelGGL(k_writeback, dim3(nbD), dim3(256), 0, s, D, w.dvel.p, w.dang.p, w.dcom.p, w.tvel.p,
-                               w.tang.p);
+        hipLaunchKernelGGL(k_writeback, dim3(nbD), dim3(256), 0, s, a);
         PGX_LAUNCH_CHECK();
     }
+}
+
+// algorithmic bytes of one step's substep loop (per-phase model in the measurement notes)
+double sv_step_bytes(const World& w, int M, long points) {
+    double N = w.D, C = (double)points, Mm = M;
+    double perSub = 164 * Mm + 96 * C + 168 * N + 80 * Mm + 116 * C + (w.iterations + 1) * (284 * C + 12 * Mm) +
+                    120 * N + 48 * N;
+    return w.substeps * perSub;
 }
 
 void sv_finish_step(World& w, int M) {

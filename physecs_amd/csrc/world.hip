@@ -156,6 +156,18 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
             std::vector<int32_t> wstart(W + 1, 0), colOfLocal(C);
             for (int i = 0; i < W; ++i) wstart[i + 1] = wstart[i] + perWorld[i];
             for (int c = 0; c < C; ++c) colOfLocal[wstart[cworld[c]] + clocal[c]] = c;
+            // LBVH leaves: colliders of non-kinematic dynamic bodies; the rest per world
+            std::vector<int32_t> dynCols, staticCols, wss(W + 1, 0);
+            for (int c = 0; c < C; ++c)
+                if (cflags[c] & CF_DYNAMIC) dynCols.push_back(c); else ++wss[cworld[c] + 1];
+            for (int i = 0; i < W; ++i) wss[i + 1] += wss[i];
+            staticCols.resize(wss[W]);
+            {
+                std::vector<int32_t> fillp(wss.begin(), wss.end() - 1);
+                for (int c = 0; c < C; ++c)
+                    if (!(cflags[c] & CF_DYNAMIC)) staticCols[fillp[cworld[c]]++] = c;
+            }
+            w.nDynCol = (int)dynCols.size();
             int maxPerWorld = W ? *std::max_element(perWorld.begin(), perWorld.end()) : 1;
             w.localBits = bits_for(std::max(2, maxPerWorld));
             w.worldBits = W > 1 ? bits_for(W) : 0;
@@ -173,10 +185,12 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
             upload(w.centity, cent, s); upload(w.cparams, cparams, s); upload(w.clpos, clpos, s);
             upload(w.clquat, clquat, s); upload(w.cmat, cmat, s);
             upload(w.worldColStart, wstart, s); upload(w.colOfLocal, colOfLocal, s);
-            w.tvel.alloc(D); w.tang.alloc(D); w.pvel.alloc(D); w.pang.alloc(D); w.winvI.alloc(3 * (size_t)D);
+            upload(w.dynCols, dynCols, s); upload(w.staticCols, staticCols, s); upload(w.worldStaticStart, wss, s);
+            w.refitReady.alloc(std::max(1, w.nDynCol));
+            w.cvel.alloc(D); w.cang.alloc(D); w.tvel.alloc(D); w.tang.alloc(D); w.pvel.alloc(D); w.pang.alloc(D); w.winvI.alloc(3 * (size_t)D);
             w.cbmin.alloc(C); w.cbmax.alloc(C);
             // broadphase buffers
-            w.worldBoundsI.alloc(6 * (size_t)W);
+            w.worldBoundsI.alloc(8);
             w.mkeys.alloc(C); w.mvals.alloc(C);
             w.leafMin.alloc(C); w.leafMax.alloc(C); w.leafFlags.alloc(C); w.leafParent.alloc(C); w.leafEsc.alloc(C);
             size_t ni = C > 1 ? C - 1 : 1;
@@ -210,7 +224,7 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
 }
 
 static void release_world(World& w) {
-    DBuf<float4>* f4s[] = {&w.bpos, &w.bquat, &w.dvel, &w.dang, &w.dcom, &w.dinvI, &w.tvel, &w.tang, &w.pvel,
+    DBuf<float4>* f4s[] = {&w.bpos, &w.bquat, &w.dvel, &w.dang, &w.dcom, &w.dinvI, &w.cvel, &w.cang, &w.tvel, &w.tang, &w.pvel,
                            &w.pang, &w.winvI, &w.cparams, &w.clpos, &w.clquat, &w.cmat, &w.cbmin, &w.cbmax,
                            &w.leafMin, &w.leafMax, &w.nodeMin, &w.nodeMax, &w.npNormal, &w.npPts, &w.sN, &w.sR0,
                            &w.sR1, &w.sRow, &w.cacheLp[0], &w.cacheLp[1]};
@@ -220,7 +234,8 @@ static void release_world(World& w) {
                              &w.nodeParent, &w.nodeEsc, &w.nodeFlags, &w.nodeCounter, &w.worldRoot, &w.worldStop,
                              &w.pvals, &w.counters, &w.npNum, &w.npIndex, &w.mfPair, &w.mfB0, &w.mfB1, &w.mfColor,
                              &w.colorPerm, &w.colorCount, &w.worldMStart, &w.sB0, &w.sB1, &w.sFlags, &w.sA0,
-                             &w.sA1, &w.sCanon, &w.cacheNum[0], &w.cacheNum[1], &w.sortT.valt, &w.scanT.sums};
+                             &w.sA1, &w.sCanon, &w.cacheNum[0], &w.cacheNum[1], &w.sortT.valt, &w.scanT.sums,
+                             &w.dynCols, &w.staticCols, &w.worldStaticStart, &w.refitReady};
     for (auto* b : i32s) b->release();
     DBuf<uint64_t>* u64s[] = {&w.mkeys, &w.pkeys, &w.mfKey, &w.claim, &w.usedMask, &w.colorKeys, &w.cacheKey[0],
                               &w.cacheKey[1], &w.nonColliding, &w.sortT.kalt};
@@ -304,16 +319,18 @@ int pgx_world_step(pgx_world* h, float dt) {
             PGX_HIP(hipEventElapsedTime(&w.tColor, w.ev[2], w.ev[3]));
             PGX_HIP(hipEventElapsedTime(&w.tSolve, w.ev[3], w.ev[4]));
             PGX_HIP(hipEventElapsedTime(&w.tTotal, w.ev[0], w.ev[5]));
-            // per-launch k_solve durations and their algorithmic bytes
+            // the substep-loop kernel: duration (HIP events on the world's stream) and the
+            // algorithmic bytes of this step's substep loop
             std::vector<int32_t> pts(160, 0);
             PGX_HIP(hipMemcpy(pts.data(), w.colorPoints.p, sizeof(int32_t) * 160, hipMemcpyDeviceToHost));
-            for (int j = 0; j < w.solveEvUsed; ++j) {
+            long points = 0;
+            for (int c = 0; c < nColors; ++c) points += pts[c];
+            w.lastPointsHost = points;
+            if (w.persistent && M > 0) {
                 float ms = 0.f;
-                PGX_HIP(hipEventElapsedTime(&ms, w.solveEv[2 * j], w.solveEv[2 * j + 1]));
-                int c = w.solveEvColor[j];
-                int nm = w.colorStartHost[c + 1] - w.colorStartHost[c];
+                PGX_HIP(hipEventElapsedTime(&ms, w.ev[6], w.ev[7]));
                 w.statSolveUs += 1000.0 * ms;
-                w.statSolveBytes += 284.0 * pts[c] + 12.0 * nm;
+                w.statSolveBytes += sv_step_bytes(w, M, points);
                 ++w.statSolveLaunches;
             }
             w.statStepUs += 1000.0 * w.tTotal;
@@ -532,6 +549,15 @@ int pgx_world_get_stats(pgx_world* h, pgx_stats* out) {
     out->solve_us = w.statSolveUs;
     out->solve_bytes = w.statSolveBytes;
     out->step_us = w.statStepUs;
+    return PGX_OK;
+}
+
+int pgx_world_set_exec(pgx_world* h, int persistent, int blocks) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    if (blocks < 0) return fail(PGX_ERR_INVALID, "blocks must be >= 0");
+    h->w.persistent = persistent != 0;
+    h->w.persistentBlocksWanted = blocks;
+    h->w.persistentBlocks = 0;
     return PGX_OK;
 }
 

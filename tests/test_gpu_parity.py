@@ -78,17 +78,19 @@ def test_free_fall_closed_form(api):
     assert abs(gs[2][0, 1] - (-9.81 * 30 / 60)) < 1e-3
 
 
-def test_pyramid_lockstep_bitexact(api):
-    """20-layer box pyramid, dt 1/60, 4 substeps, 2 iterations: 120 steps in lockstep."""
-    log("[lockstep] box pyramid 120 steps")
-    lockstep(scenes.box_pyramid(0), 120)
+@pytest.mark.parametrize("persistent", [True, False])
+def test_pyramid_lockstep_bitexact(api, persistent):
+    """20-layer box pyramid, dt 1/60, 4 substeps, 2 iterations: 120 steps in lockstep, with
+    the persistent substep kernel and with one launch per colour."""
+    log(f"[lockstep] box pyramid 120 steps persistent={persistent}")
+    lockstep(scenes.box_pyramid(0), 120, persistent=persistent)
 
 
 @pytest.mark.parametrize("seed", [0, 1])
 def test_mixed_lockstep_bitexact(api, seed):
     """Spheres, capsules and boxes in a container: every pair type, 80 steps."""
     log(f"[lockstep] mixed lattice side=7 seed={seed}")
-    lockstep(scenes.mixed_lattice(seed, side=7), 80)
+    lockstep(scenes.mixed_lattice(seed, side=7), 80, persistent=seed == 0)
 
 
 def test_cluster_lockstep_with_restitution_and_soft(api):
