@@ -7,13 +7,13 @@ inputs resident in HBM. The default workload is the 32,768-body mixed scene (1/3
 capsules, boxes on a jittered 32^3 lattice at 1.5 m in an open 5-box container, 600 steps:
 10 warm-up + 590 timed). Prints one JSON line (metric body-substeps/s, higher is better).
 
-roofline: the dominant kernel is k_step_persistent, the whole substep loop of one step as a
-single cooperative launch. Its algorithmic bytes per launch follow the per-phase model, per
-substep: update rows 164 B/manifold + 96 B/point, integrate velocities 168 B/body,
-preSolve 80 B/manifold + 116 B/point, each solve pass (iterations + relaxation) 284 B/point
-+ 12 B/manifold, integrate positions 120 B/body, write-back 48 B/body; the counts are the
-step's own. Its duration is measured with HIP events around every launch in the timed
-region, on the world's stream.
+roofline: the dominant kernel is k_solve, the contact solve of one graph colour (one launch
+per colour and pass: numIterations biased passes + 1 relaxation, every substep). Its
+algorithmic bytes per launch are 284 B per contact point + 12 B per manifold of the colour
+(rows 108 + scalars 24 + both bodies' solver and component v,w 96 read; lambdas 8 + both
+bodies' v,w 48 written). Its duration is measured with HIP events around each launch of the
+first substep of every timed step, on the world's stream. (--persistent times the
+single-launch substep kernel k_step_persistent instead, with the per-substep phase model.)
 
 cpu_baseline: the CPU oracle (a restatement of the reference's Scene::simulate with the
 reference's ThreadPool narrowphase chunking, nproc-1 workers + caller) on the same scene,
@@ -83,7 +83,7 @@ def main():
     ap.add_argument("--config", default="plain32k")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks", type=int, default=0, help="persistent grid size (0 = one block per CU)")
-    ap.add_argument("--discrete", action="store_true", help="one launch per colour instead of the persistent kernel")
+    ap.add_argument("--persistent", action="store_true", help="substep loop as one cooperative launch per step")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -92,8 +92,8 @@ def main():
     scene, cfg = make_scene(args.config)
     sub, it, dt = cfg["substeps"], cfg["iterations"], cfg["dt"]
     w = api.World(scene, substeps=sub, iterations=it)
-    if args.blocks or args.discrete:
-        w.set_exec(persistent=not args.discrete, blocks=args.blocks)
+    if args.persistent:
+        w.set_exec(persistent=True, blocks=args.blocks)
     for _ in range(args.warmup):
         w.step(dt)
     w.sync()
@@ -122,7 +122,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
         "config": dict(cfg, parallelism="single GPU", last_step_counts=counts,
                        phase_us_last_step=w.timing()),
-        "roofline": {"bound": "hbm", "kernel": "k_step_persistent", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": "k_step_persistent" if args.persistent else "k_solve", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_us": st["solve_us"] / max(1, st["solve_launches"]),
                      "bytes_per_launch": st["solve_bytes"] / max(1, st["solve_launches"]),

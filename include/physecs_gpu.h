@@ -140,9 +140,13 @@ int pgx_world_enable_timing(pgx_world* w, int enable);
 int pgx_world_get_timing(pgx_world* w, float* broadphase_us, float* narrowphase_us, float* coloring_us,
                          float* solver_us, float* total_us);
 
-/* Cumulative timing since pgx_world_reset_stats, collected while timing is enabled: the
- * substep-loop kernel (one cooperative launch per step) is bracketed by HIP events on the
- * world's stream; solve_bytes is its algorithmic traffic per the phase model (update rows
+/* Cumulative timing since pgx_world_reset_stats, collected while timing is enabled.
+ * Default (per-colour) mode: every k_solve launch of the first substep of each step is
+ * bracketed by HIP events on the world's stream; solve_bytes is their algorithmic traffic,
+ * 284 B per contact point + 12 B per manifold of the launched colour (rows 108 + scalars
+ * 24 + solver and component velocities of both bodies 96 read; lambdas 8 + velocities 48
+ * written). Persistent mode: the substep-loop kernel (one cooperative launch per step) is
+ * bracketed instead, and solve_bytes follows the per-substep phase model (update rows
  * 164 B/manifold + 96 B/point, integrate velocities 168 B/body, preSolve 80 B/manifold +
  * 116 B/point, each solve pass 284 B/point + 12 B/manifold, integrate positions 120 B/body,
  * write-back 48 B/body; per substep). */

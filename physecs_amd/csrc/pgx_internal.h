@@ -171,7 +171,7 @@ struct World {
     int lastPairs = 0, lastManifolds = 0, lastColors = 0, lastRounds = 0, lastPoints = 0;
     std::vector<int> colorStartHost;
     bool timing = false;
-    bool persistent = true;              // substep loop as one cooperative launch per step
+    bool persistent = false;             // substep loop as one cooperative launch per step
     int persistentBlocks = 0, persistentBlocksWanted = 0;
     hipEvent_t ev[8] = {};
     float tBroad = 0, tNarrow = 0, tColor = 0, tSolve = 0, tTotal = 0;

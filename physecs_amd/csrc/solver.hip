@@ -361,10 +361,26 @@ __device__ void solve_manifold(const SubArgs& a, int s, bool useBias, float h, b
         v1 = xyz(a.tvel[b1]); w1 = xyz(a.tang[b1]);
     }
     float2 sp = a.sSoft[s];
+    // all rows of the manifold are loaded before any dependent arithmetic (memory-level
+    // parallelism); unused point slots are never read back
+    float4 rA[4], rB[4], rT[4], rC[4], rE[4], rNT0[4], rNT1[4], rTT0[4], rTT1[4];
+    float tvk[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < num) {
+            rA[k] = ROW(R_R0XN, k); rB[k] = ROW(R_R1XN, k); rT[k] = ROW(R_T, k);
+            rC[k] = ROW(R_R0XT, k); rE[k] = ROW(R_R1XT, k);
+            rNT0[k] = ROW(R_R0XNT, k); rNT1[k] = ROW(R_R1XNT, k);
+            rTT0[k] = ROW(R_R0XTT, k); rTT1[k] = ROW(R_R1XTT, k);
+            tvk[k] = a.sR0[(size_t)k * a.Mcap + s].w;
+        }
+    }
     float lN[4], lT[4];
-    for (int k = 0; k < num; ++k) {
-        float4 r0 = ROW(R_R0XN, k), r1 = ROW(R_R1XN, k);
-        lN[k] = ROW(R_R0XT, k).w;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= num) break;
+        float4 r0 = rA[k], r1 = rB[k];
+        lN[k] = rC[k].w;
         float invN = r1.w;
         if (invN == 0.f) continue;
         f3 r0xn = xyz(r0), r1xn = xyz(r1);
@@ -380,23 +396,24 @@ __device__ void solve_manifold(const SubArgs& a, int s, bool useBias, float h, b
             float beta = h * stiffness / (damping + h * stiffness);
             lambda = (rv + beta * c / h) / (invN + gamma / h);
         } else {
-            float tv = a.sR0[(size_t)k * a.Mcap + s].w;
-            lambda = (rv - tv + (useBias ? 0.1f * c / h : 0.f)) * effMass;
+            lambda = (rv - tvk[k] + (useBias ? 0.1f * c / h : 0.f)) * effMass;
         }
         float prev = lN[k];
         lN[k] = lN[k] + lambda;
         lN[k] = gmin(lN[k], 0.f);
         lambda = lN[k] - prev;
-        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, n)); w0 = add(w0, mul(lambda, xyz(ROW(R_R0XNT, k)))); }
-        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, n)); w1 = sub(w1, mul(lambda, xyz(ROW(R_R1XNT, k)))); }
+        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, n)); w0 = add(w0, mul(lambda, xyz(rNT0[k]))); }
+        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, n)); w1 = sub(w1, mul(lambda, xyz(rNT1[k]))); }
     }
     // friction rows read the component velocities (src/ContactConstraints.cpp:142-152)
-    for (int k = 0; k < num; ++k) {
-        float4 t4 = ROW(R_T, k);
-        lT[k] = ROW(R_R1XT, k).w;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= num) break;
+        float4 t4 = rT[k];
+        lT[k] = rE[k].w;
         float invT = t4.w;
         if (invT == 0.f) continue;
-        f3 t = xyz(t4), r0xt = xyz(ROW(R_R0XT, k)), r1xt = xyz(ROW(R_R1XT, k));
+        f3 t = xyz(t4), r0xt = xyz(rC[k]), r1xt = xyz(rE[k]);
         float rv = dot(neg(t), cv0) + dot(neg(r0xt), cw0) + dot(t, cv1) + dot(r1xt, cw1);
         float lambda = rv / invT;
         float limit = friction * lN[k];
@@ -404,11 +421,13 @@ __device__ void solve_manifold(const SubArgs& a, int s, bool useBias, float h, b
         lT[k] = lT[k] + lambda;
         lT[k] = gclamp(lT[k], limit, -limit);
         lambda = lT[k] - prev;
-        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, t)); w0 = add(w0, mul(lambda, xyz(ROW(R_R0XTT, k)))); }
-        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, t)); w1 = sub(w1, mul(lambda, xyz(ROW(R_R1XTT, k)))); }
+        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, t)); w0 = add(w0, mul(lambda, xyz(rTT0[k]))); }
+        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, t)); w1 = sub(w1, mul(lambda, xyz(rTT1[k]))); }
     }
-    for (int k = 0; k < num; ++k) {
-        float4 c = ROW(R_R0XT, k), e = ROW(R_R1XT, k);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k >= num) break;
+        float4 c = rC[k], e = rE[k];
         c.w = lN[k];
         e.w = lT[k];
         ROW(R_R0XT, k) = c;
@@ -453,7 +472,7 @@ __global__ __launch_bounds__(256) void k_prepare(SubArgs a) {
     if (i < a.M) prepare_manifold(a, i);
     else if (i - a.M < a.D) integrate_velocity(a, i - a.M);
 }
-__global__ __launch_bounds__(256) void k_solve(SubArgs a, int begin, int end, int useBias, float h, int skipSoft) {
+__global__ __launch_bounds__(64) void k_solve(SubArgs a, int begin, int end, int useBias, float h, int skipSoft) {
     int s = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (s < end) solve_manifold(a, s, useBias != 0, h, skipSoft != 0);
 }
@@ -532,7 +551,7 @@ int sv_color(World& w, int M) {
     fill_u64(s, w.usedMask.p, 0ull, 2 * w.D);
     fill_i32(s, w.counters.p + 5, 0, 1);
     int round = 0;
-    const int batch = 8;
+    const int batch = 16;
     while (true) {
         for (int r = 0; r < batch; ++r) {
             ++round;
@@ -606,8 +625,26 @@ static int persistent_blocks(World& w) {
     return w.persistentBlocks;
 }
 
+// HIP events around one k_solve launch (timing mode, first substep of the step)
+static void solve_event(World& w, bool rec, int color, bool after) {
+    if (!rec) return;
+    int idx = 2 * w.solveEvUsed + (after ? 1 : 0);
+    while ((int)w.solveEv.size() <= idx) {
+        hipEvent_t e;
+        PGX_HIP(hipEventCreate(&e));
+        w.solveEv.push_back(e);
+    }
+    PGX_HIP(hipEventRecord(w.solveEv[idx], w.stream));
+    if (after) {
+        if ((int)w.solveEvColor.size() <= w.solveEvUsed) w.solveEvColor.resize(w.solveEvUsed + 1);
+        w.solveEvColor[w.solveEvUsed] = color;
+        ++w.solveEvUsed;
+    }
+}
+
 void sv_substeps(World& w, int M, float dt) {
     hipStream_t s = w.stream;
+    w.solveEvUsed = 0;
     SubArgs a = make_args(w, M, dt);
     if (w.persistent) {
         int blocks = persistent_blocks(w);
@@ -620,16 +657,23 @@ void sv_substeps(World& w, int M, float dt) {
     int nbP = div_up(std::max(M + w.D, 1), 256), nbD = div_up(std::max(w.D, 1), 256);
     int nColors = a.nColors;
     for (int sub = 0; sub < w.substeps; ++sub) {
+        const bool rec = w.timing && sub == 0;
         hipLaunchKernelGGL(k_prepare, dim3(nbP), dim3(256), 0, s, a);
         for (int it = 0; it < w.iterations; ++it)
             for (int c = 0; c < nColors; ++c) {
                 int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
-                if (e > b) hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 256)), dim3(256), 0, s, a, b, e, 1, a.h, 0);
+                if (e <= b) continue;
+                solve_event(w, rec, c, false);
+                hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 64)), dim3(64), 0, s, a, b, e, 1, a.h, 0);
+                solve_event(w, rec, c, true);
             }
         hipLaunchKernelGGL(k_integrate_positions, dim3(nbD), dim3(256), 0, s, a);
         for (int c = 0; c < nColors; ++c) {
             int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
-            if (e > b) hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 256)), dim3(256), 0, s, a, b, e, 0, 0.f, 1);
+            if (e <= b) continue;
+            solve_event(w, rec, c, false);
+            hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 64)), dim3(64), 0, s, a, b, e, 0, 0.f, 1);
+            solve_event(w, rec, c, true);
         }
         hipLaunchKernelGGL(k_writeback, dim3(nbD), dim3(256), 0, s, a);
         PGX_LAUNCH_CHECK();

@@ -333,6 +333,16 @@ int pgx_world_step(pgx_world* h, float dt) {
                 w.statSolveBytes += sv_step_bytes(w, M, points);
                 ++w.statSolveLaunches;
             }
+            for (int j = 0; !w.persistent && j < w.solveEvUsed; ++j) {
+                // k_solve: 284 B per contact point + 12 B per manifold of the launched colour
+                float ms = 0.f;
+                PGX_HIP(hipEventElapsedTime(&ms, w.solveEv[2 * j], w.solveEv[2 * j + 1]));
+                int c = w.solveEvColor[j];
+                int nm = w.colorStartHost[c + 1] - w.colorStartHost[c];
+                w.statSolveUs += 1000.0 * ms;
+                w.statSolveBytes += 284.0 * pts[c] + 12.0 * nm;
+                ++w.statSolveLaunches;
+            }
             w.statStepUs += 1000.0 * w.tTotal;
             ++w.statSteps;
         }
