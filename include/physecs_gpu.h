@@ -20,6 +20,8 @@
  *   pgx_world_step          Scene::simulate(float timeStep) (include/Physecs/Physecs.h:204,
  *                           src/Physecs.cpp:112-561)
  *   pgx_world_set_can_collide  Scene::setCanCollide (include/Physecs/Physecs.h:223)
+ *   pgx_world_add_joints    Scene::createJoint<T> for the built-in joint types
+ *                           (include/Physecs/Physecs.h:209-214, src/Physecs.cpp:690-710)
  *   pgx_world_read_state / pgx_world_write_state
  *                           the in-place component writes that simulate performs
  *                           (Transform position/orientation, Dynamic velocity/angularVelocity)
@@ -93,6 +95,24 @@ typedef struct pgx_scene_desc {
     const int32_t* actor_world;   /* world of each actor, or NULL */
 } pgx_scene_desc;
 
+/* Built-in joint types (include/Physecs/Joints/*.h) */
+#define PGX_JOINT_SPHERICAL 0
+#define PGX_JOINT_UNIVERSAL 1
+#define PGX_JOINT_REVOLUTE 2
+#define PGX_JOINT_FIXED 3
+
+/* One Scene::createJoint<T>(entity0, anchor0Pos, anchor0Or, entity1, anchor1Pos, anchor1Or).
+ * Anchors are in body space: position xyz, orientation xyzw. drive_* apply to revolute
+ * joints (RevoluteJoint::setDriveEnabled / setDriveVelocity / setDriveMaxTorque). */
+typedef struct pgx_joint_desc {
+    int32_t type;
+    int32_t actor0, actor1;   /* descriptor actor indices */
+    float anchor0[7];
+    float anchor1[7];
+    int32_t drive_enabled;
+    float drive_velocity, drive_max_torque;
+} pgx_joint_desc;
+
 typedef struct pgx_counts {
     int pairs;        /* broadphase candidate pairs of the last step */
     int manifolds;    /* contact manifolds with at least one point */
@@ -112,6 +132,9 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* scene, pgx_wor
 int pgx_world_destroy(pgx_world* w);
 int pgx_world_set_params(pgx_world* w, int num_substeps, int num_iterations, float gravity);
 int pgx_world_set_can_collide(pgx_world* w, uint32_t entity0, uint32_t entity1, int can_collide);
+/* Appends joints in creation order. Each takes the lowest joint colour free on both of its
+ * entities (8 colours, then the overflow colour), and the pair stops colliding. */
+int pgx_world_add_joints(pgx_world* w, int n, const pgx_joint_desc* joints);
 
 /* One Scene::simulate(time_step) on the device. Asynchronous; pgx_world_sync waits. */
 int pgx_world_step(pgx_world* w, float time_step);

@@ -15,6 +15,8 @@
 // contact constraints can be injected (set_order) so that a solver fed the GPU's graph
 // color order can be compared with it.
 #include <algorithm>
+#include <cmath>
+#include <limits>
 #include <atomic>
 #include <cstdint>
 #include <cstring>
@@ -65,6 +67,31 @@ struct KeyHash {
 };
 struct CacheData { int numPoints; V3 lp[4]; float tv[4]; };
 
+// ---- joints (include/Physecs/Joint.h, src/Joints/*.cpp, include/Physecs/Constraint1D*.h)
+enum JointType { J_SPHERICAL = 0, J_UNIVERSAL = 1, J_REVOLUTE = 2, J_FIXED = 3 };
+enum RowFlags { F_NONE = 0, F_SOFT = 1, F_ANGULAR = 2, F_LIMITED = 4 };
+// flag-type order of Constraint1DContainer's collections (Constraint1DContainer.h:117)
+static const int kFlagOrder[6] = {F_NONE, F_ANGULAR, F_SOFT, F_LIMITED, F_ANGULAR | F_SOFT, F_ANGULAR | F_LIMITED};
+struct Joint {
+    int type, a0, a1;                  // actor indices
+    V3 p0; Q q0; V3 p1; Q q1;          // anchors in body space
+    int driveEnabled; float driveVelocity, driveMaxTorque;
+    int color;
+};
+struct Row {
+    int flags, b0, b1;
+    V3 linear, angular0, angular1;
+    float tv, c, mn, mx, freq, damp;
+    V3 linear0t, linear1t, angular0t, angular1t;
+    float invEff, lambda;
+};
+struct JointRows {                     // one joint's rows, in solve order
+    int joint;
+    V3 r0, r1;                         // anchor relative to the COM (or the anchor) in body space
+    int nRows;
+    Row rows[6];
+};
+
 }  // namespace
 
 struct orc_world {
@@ -77,6 +104,9 @@ struct orc_world {
     std::unordered_map<Key, CacheData, KeyHash> cache, cacheTemp;
     std::unordered_set<uint64_t> nonColliding;
     std::vector<int32_t> order;   // injected manifold order (5 ints per manifold)
+    std::vector<Joint> joints;
+    std::unordered_map<uint32_t, unsigned char> jointBitsets;   // per entity, src/Physecs.cpp:696-705
+    std::vector<JointRows> jrows[9];                           // per colour, joints in creation order
     int numSubSteps = 8, numIterations = 2, numThreads = 0;
     float g = 9.81f;
     std::mutex mtx;
@@ -162,6 +192,8 @@ void orc_world_set_non_colliding(orc_world* w, uint32_t e0, uint32_t e1) {
 void orc_world_set_order(orc_world* w, const int32_t* keys, int n) {
     w->order.assign(keys, keys + 5 * (size_t)n);
 }
+
+
 
 }  // extern "C"
 
@@ -344,9 +376,257 @@ static void applyOrder(orc_world* w) {
     w->order.clear();
 }
 
+
+// ---------------------------------------------------------------- joints
+// Joint colouring at creation, src/Physecs.cpp:690-710: the lowest colour free on both
+// entities (8 SIMD colours, colour 8 = scalar overflow); static entities take colours too.
+static void addJoint(orc_world* w, const Joint& jin) {
+    Joint j = jin;
+    uint32_t e0 = w->actors[j.a0].entity, e1 = w->actors[j.a1].entity;
+    uint32_t lo = e0 < e1 ? e0 : e1, hi = e0 < e1 ? e1 : e0;
+    w->nonColliding.insert(((uint64_t)lo << 32) | hi);
+    unsigned char& c0 = w->jointBitsets[e0];
+    unsigned char& c1 = w->jointBitsets[e1];
+    unsigned int un = (unsigned int)(c0 | c1);
+    unsigned int inv = ~un;
+    int color = 0;
+    while (!((inv >> color) & 1u)) ++color;     // _BitScanForward(~colorsUnion)
+    c0 = (unsigned char)(c0 | (1u << color));
+    c1 = (unsigned char)(c1 | (1u << color));
+    j.color = color;
+    w->joints.push_back(j);
+}
+
+static int rowLayout(const Joint& j, int* flags) {
+    int n = 0;
+    for (int k = 0; k < 3; ++k) flags[n++] = F_NONE;                           // point-to-point
+    if (j.type == J_UNIVERSAL) flags[n++] = F_ANGULAR;                         // UniversalJoint.cpp:102-105
+    if (j.type == J_REVOLUTE) {                                                // RevoluteJoint.cpp:52-55
+        flags[n++] = F_ANGULAR; flags[n++] = F_ANGULAR;
+        if (j.driveEnabled) flags[n++] = F_ANGULAR | F_LIMITED;
+    }
+    if (j.type == J_FIXED) for (int k = 0; k < 3; ++k) flags[n++] = F_ANGULAR;  // FixedJoint.cpp:36-39
+    return n;
+}
+
+// "create joint constraints", src/Physecs.cpp:321-355: rows are rebuilt every step (defaults
+// from include/Physecs/Constraint1D.h:17-31, so the accumulated lambda restarts at 0).
+static void buildJointRows(orc_world* w) {
+    for (auto& c : w->jrows) c.clear();
+    for (size_t i = 0; i < w->joints.size(); ++i) {
+        const Joint& j = w->joints[i];
+        JointRows jr{};
+        jr.joint = (int)i;
+        const Actor& t0 = w->actors[j.a0];
+        const Actor& t1 = w->actors[j.a1];
+        const Dyn* d0 = t0.dyn >= 0 ? &w->dyn[t0.dyn] : nullptr;
+        const Dyn* d1 = t1.dyn >= 0 ? &w->dyn[t1.dyn] : nullptr;
+        int b0 = d0 && !d0->kinematic ? t0.dyn : -1;
+        int b1 = d1 && !d1->kinematic ? t1.dyn : -1;
+        jr.r0 = b0 >= 0 ? j.p0 - d0->com : j.p0;
+        jr.r1 = b1 >= 0 ? j.p1 - d1->com : j.p1;
+        int fl[6];
+        jr.nRows = rowLayout(j, fl);
+        for (int r = 0; r < jr.nRows; ++r) {
+            Row& row = jr.rows[r];
+            std::memset(&row, 0, sizeof(row));
+            row.flags = fl[r];
+            row.b0 = b0; row.b1 = b1;
+            row.mn = std::numeric_limits<float>::lowest();
+            row.mx = std::numeric_limits<float>::max();
+        }
+        w->jrows[j.color].push_back(jr);
+    }
+}
+
+// calculateWorldSpaceData (src/Joint.cpp:5-14) + makeConstraints of each type
+// (src/JointsUtil.h:8-28, src/Joints/{Spherical,Universal,Revolute,Fixed}Joint.cpp)
+static void updateJointRows(orc_world* w, JointRows& jr) {
+    const Joint& j = w->joints[jr.joint];
+    const Actor& t0 = w->actors[j.a0];
+    const Actor& t1 = w->actors[j.a1];
+    M3 u0 = mat3_cast(qmul(t0.quat, j.q0));
+    M3 u1 = mat3_cast(qmul(t1.quat, j.q1));
+    V3 r0 = qrot(t0.quat, jr.r0);
+    V3 r1 = qrot(t1.quat, jr.r1);
+    V3 p0 = t0.pos + qrot(t0.quat, j.p0);
+    V3 p1 = t1.pos + qrot(t1.quat, j.p1);
+    V3 d = p1 - p0;
+    Row* R = jr.rows;
+    R[0].linear = v3(1, 0, 0); R[0].angular0 = v3(0, r0.z, -r0.y); R[0].angular1 = v3(0, r1.z, -r1.y); R[0].c = d.x;
+    R[1].linear = v3(0, 1, 0); R[1].angular0 = v3(-r0.z, 0, r0.x); R[1].angular1 = v3(-r1.z, 0, r1.x); R[1].c = d.y;
+    R[2].linear = v3(0, 0, 1); R[2].angular0 = v3(r0.y, -r0.x, 0); R[2].angular1 = v3(r1.y, -r1.x, 0); R[2].c = d.z;
+    auto angularRow = [&](Row& row, V3 a, V3 b) {   // c = dot(a, b), axis = cross(b, a)
+        V3 ax = cross(b, a);
+        row.angular0 = ax; row.angular1 = ax; row.c = dot(a, b);
+    };
+    if (j.type == J_UNIVERSAL) angularRow(R[3], u0[2], u1[2]);
+    if (j.type == J_REVOLUTE || j.type == J_FIXED) {
+        angularRow(R[3], u0[0], u1[1]);
+        angularRow(R[4], u0[0], u1[2]);
+        if (j.type == J_FIXED) angularRow(R[5], u0[1], u1[2]);
+        if (j.type == J_REVOLUTE && j.driveEnabled) {
+            R[5].angular0 = u0[0]; R[5].angular1 = u0[0];
+            R[5].tv = j.driveVelocity; R[5].mx = j.driveMaxTorque; R[5].mn = -j.driveMaxTorque;
+        }
+    }
+}
+
+struct VelocityData;
+struct PseudoVelocityData;
+struct MassData;
+
 struct VelocityData { V3 v, w; };
 struct PseudoVelocityData { V3 pv, pw; int count; };
 struct MassData { float invMass; M3 invI; };
+
+// Constraint1D<flags>::preSolve (src/Constraint1D.cpp:5-52) and the per-lane meaning of
+// Constraint1DW<flags>::preSolve (src/Constraint1DW.cpp:5-115): the same arithmetic; the
+// SIMD form clamps LIMITED rows with _mm_min_ps/_mm_max_ps.
+static void rowPreSolve(Row& r, const MassData* masses, PseudoVelocityData* pv, bool simd) {
+    float im0 = 0, im1 = 0;
+    M3 I0 = mzero(), I1 = mzero();
+    if (r.b0 >= 0) { im0 = masses[r.b0].invMass; I0 = masses[r.b0].invI; }
+    if (r.b1 >= 0) { im1 = masses[r.b1].invMass; I1 = masses[r.b1].invI; }
+    bool ang = r.flags & F_ANGULAR;
+    r.angular0t = mulmv(I0, r.angular0);
+    r.angular1t = mulmv(I1, r.angular1);
+    r.invEff = dot(r.angular0, r.angular0t) + dot(r.angular1, r.angular1t);
+    if (!ang) {
+        r.invEff = r.invEff + dot(r.linear, r.linear) * (im0 + im1);
+        r.linear0t = im0 * r.linear;
+        r.linear1t = im1 * r.linear;
+    }
+    if (r.flags & F_SOFT) return;
+    if (r.c == 0.f || r.invEff == 0.f) return;
+    float lambda = r.c / r.invEff;
+    if (r.flags & F_LIMITED) {
+        if (simd) { lambda = lambda > r.mn ? lambda : r.mn; lambda = lambda < r.mx ? lambda : r.mx; }
+        else lambda = gclamp(lambda, r.mn, r.mx);
+    }
+    if (r.b0 >= 0) {
+        if (!ang) pv[r.b0].pv = pv[r.b0].pv + lambda * r.linear0t;
+        pv[r.b0].pw = pv[r.b0].pw + lambda * r.angular0t;
+        ++pv[r.b0].count;
+    }
+    if (r.b1 >= 0) {
+        if (!ang) pv[r.b1].pv = pv[r.b1].pv - lambda * r.linear1t;
+        pv[r.b1].pw = pv[r.b1].pw - lambda * r.angular1t;
+        ++pv[r.b1].count;
+    }
+}
+
+// Constraint1DW<flags>::solve for one lane (src/Constraint1DW.cpp:117-233): strict warm-start
+// test against (float)1e-4 and 10000, the warm impulse goes into the written-back
+// velocities, bias (float)(0.2 / h), LIMITED clamp to [min*h, max*h].
+static void rowSolveSimd(Row& r, VelocityData* vel, float h, bool warm) {
+    bool ang = r.flags & F_ANGULAR;
+    V3 v0 = v3s(0), w0 = v3s(0), v1 = v3s(0), w1 = v3s(0);
+    if (r.b0 >= 0) { if (!ang) v0 = vel[r.b0].v; w0 = vel[r.b0].w; }
+    if (r.b1 >= 0) { if (!ang) v1 = vel[r.b1].v; w1 = vel[r.b1].w; }
+    if (!(r.flags & F_SOFT) && warm) {
+        float ac = std::fabs(r.c), al = std::fabs(r.lambda);
+        if (ac < 1e-4f && al < 10000.f) {
+            float lam = r.lambda * 0.5f;
+            if (!ang) { v0 = v0 + lam * r.linear0t; v1 = v1 - lam * r.linear1t; }
+            w0 = w0 + lam * r.angular0t;
+            w1 = w1 - lam * r.angular1t;
+            r.lambda = lam;
+        }
+    }
+    if (r.invEff == 0.f) return;
+    float rv = dot(r.angular1, w1) - dot(r.angular0, w0);
+    if (!ang) rv = rv + (dot(r.linear, v1) - dot(r.linear, v0));
+    float effMass = 1.f / r.invEff;
+    float lambda;
+    if (r.flags & F_SOFT) {
+        float twoPi = 2.f * 3.14159265358979323846f;
+        float af = twoPi * r.freq;
+        float stiffness = af * af * effMass;
+        float damping = 2.f * af * r.damp * effMass;
+        float gamma = 1.f / (damping + h * stiffness);
+        float beta = h * stiffness * gamma;
+        lambda = (rv + beta * r.c / h) / (r.invEff + gamma / h);
+    } else {
+        float bias = (float)(0.2 / (double)h);
+        lambda = (rv - r.tv + bias * r.c) * effMass;
+    }
+    float prev = r.lambda;
+    if (r.flags & F_LIMITED) {
+        float t = r.lambda + lambda;
+        float lo = r.mn * h, hi = r.mx * h;
+        t = t > lo ? t : lo;
+        t = t < hi ? t : hi;
+        r.lambda = t;
+        lambda = r.lambda - prev;
+    } else {
+        r.lambda = r.lambda + lambda;
+    }
+    if (!ang) { v0 = v0 + lambda * r.linear0t; v1 = v1 - lambda * r.linear1t; }
+    w0 = w0 + lambda * r.angular0t;
+    w1 = w1 - lambda * r.angular1t;
+    if (r.b0 >= 0) { if (!ang) vel[r.b0].v = v0; vel[r.b0].w = w0; }
+    if (r.b1 >= 0) { if (!ang) vel[r.b1].v = v1; vel[r.b1].w = w1; }
+}
+
+// Constraint1D<flags>::solve (src/Constraint1D.cpp:54-127): non-strict warm-start test
+// (|c| compared with the double 1e-4), the warm impulse only biases local copies, bias
+// 0.2f * c / h, LIMITED clamp to [min, max].
+static void rowSolveScalar(Row& r, VelocityData* vel, float h, bool warm) {
+    if (r.invEff == 0.f) return;
+    bool ang = r.flags & F_ANGULAR;
+    V3 v0 = v3s(0), w0 = v3s(0), v1 = v3s(0), w1 = v3s(0);
+    if (r.b0 >= 0) { if (!ang) v0 = vel[r.b0].v; w0 = vel[r.b0].w; }
+    if (r.b1 >= 0) { if (!ang) v1 = vel[r.b1].v; w1 = vel[r.b1].w; }
+    if (!(r.flags & F_SOFT)) {
+        if (warm && !((double)gabs(r.c) > 1e-4 || gabs(r.lambda) > 10000.f)) {
+            r.lambda = r.lambda * 0.5f;
+            if (r.b0 >= 0) { if (!ang) v0 = v0 + r.lambda * r.linear0t; w0 = w0 + r.lambda * r.angular0t; }
+            if (r.b1 >= 0) { if (!ang) v1 = v1 - r.lambda * r.linear1t; w1 = w1 - r.lambda * r.angular1t; }
+        }
+    }
+    float rv = dot(r.angular1, w1) - dot(r.angular0, w0);
+    if (!ang) rv = rv + (dot(r.linear, v1) - dot(r.linear, v0));
+    float lambda;
+    if (r.flags & F_SOFT) {
+        float af = 2.f * 3.14159265358979323846f * r.freq;
+        float stiffness = af * af / r.invEff;
+        float damping = 2.f * af * r.damp / r.invEff;
+        float gamma = 1.f / (damping + h * stiffness);
+        float beta = h * stiffness / (damping + h * stiffness);
+        lambda = (rv + beta * r.c / h) / (r.invEff + gamma / h);
+    } else {
+        lambda = (rv - r.tv + 0.2f * r.c / h) / r.invEff;
+    }
+    if (r.flags & F_LIMITED) {
+        float prev = r.lambda;
+        r.lambda = r.lambda + lambda;
+        r.lambda = gclamp(r.lambda, r.mn, r.mx);
+        lambda = r.lambda - prev;
+    } else {
+        r.lambda = r.lambda + lambda;
+    }
+    if (r.b0 >= 0) { if (!ang) vel[r.b0].v = vel[r.b0].v + lambda * r.linear0t; vel[r.b0].w = vel[r.b0].w + lambda * r.angular0t; }
+    if (r.b1 >= 0) { if (!ang) vel[r.b1].v = vel[r.b1].v - lambda * r.linear1t; vel[r.b1].w = vel[r.b1].w - lambda * r.angular1t; }
+}
+
+// Joint rows of one colour in the reference's container order: colours 0-7 are SIMD rows
+// of body-disjoint joints (per joint, its rows in flag-type order); colour 8 is the scalar
+// overflow container, walked flag type by flag type over all its joints.
+template <class F>
+static void forColorRows(std::vector<JointRows>& jr, int color, F&& f) {
+    if (color < 8) {
+        for (auto& j : jr)
+            for (int k = 0; k < 6; ++k)
+                for (int r = 0; r < j.nRows; ++r)
+                    if (j.rows[r].flags == kFlagOrder[k]) f(j.rows[r]);
+    } else {
+        for (int k = 0; k < 6; ++k)
+            for (auto& j : jr)
+                for (int r = 0; r < j.nRows; ++r)
+                    if (j.rows[r].flags == kFlagOrder[k]) f(j.rows[r]);
+    }
+}
 
 // src/ContactConstraints.cpp:4-31
 static void preSolve(CC& c, const MassData* masses) {
@@ -484,6 +764,9 @@ static void substeps(orc_world* w, float timeStep) {
                 p.c = cn; p.lN = 0; p.lT = 0;
             }
         }
+        // update joint constraints, src/Physecs.cpp:430-439
+        for (auto& col : w->jrows)
+            for (auto& jr : col) updateJointRows(w, jr);
         // integrate velocities, src/Physecs.cpp:443-468
         for (size_t i = 0; i < nd; ++i) {
             Dyn& d = w->dyn[i];
@@ -493,8 +776,16 @@ static void substeps(orc_world* w, float timeStep) {
             pvel[i] = {v3s(0), v3s(0), 0};
         }
         for (auto& c : w->ccs) preSolve(c, mass.data());
-        for (int it = 0; it < w->numIterations; ++it)
+        for (int color = 0; color < 9; ++color)
+            forColorRows(w->jrows[color], color, [&](Row& r) { rowPreSolve(r, mass.data(), pvel.data(), color < 8); });
+        for (int it = 0; it < w->numIterations; ++it) {
             for (auto& c : w->ccs) solveContact(w, c, vel.data(), true, h);
+            for (int color = 0; color < 9; ++color)
+                forColorRows(w->jrows[color], color, [&](Row& r) {
+                    if (color < 8) rowSolveSimd(r, vel.data(), h, it == 0);
+                    else rowSolveScalar(r, vel.data(), h, it == 0);
+                });
+        }
         // integrate positions, src/Physecs.cpp:494-511
         for (size_t i = 0; i < nd; ++i) {
             Dyn& d = w->dyn[i];
@@ -524,6 +815,7 @@ void orc_world_step(orc_world* w, float timeStep) {
     sap(w);
     narrowphase(w);
     applyOrder(w);
+    buildJointRows(w);
     substeps(w, timeStep);
     w->cache.swap(w->cacheTemp);
     for (auto& e : w->bp) {
@@ -691,6 +983,68 @@ int orc_sqr_dist_segment_aabb(const float* p, const float* dir, float mn, float 
     float t = 0;
     float sq = sqrDistSegmentAABB(v3(p[0], p[1], p[2]), v3(dir[0], dir[1], dir[2]), mn, mx, v3(he[0], he[1], he[2]), t, qq);
     out[0] = sq; out[1] = t; out[2] = qq.x; out[3] = qq.y; out[4] = qq.z;
+    return 0;
+}
+
+// Scene::createJoint<T> for the built-in types (include/Physecs/Physecs.h:209-214)
+struct orc_joint {
+    int32_t type, actor0, actor1;
+    float anchor0[7];    // position xyz, orientation xyzw
+    float anchor1[7];
+    int32_t drive_enabled;
+    float drive_velocity, drive_max_torque;
+};
+
+int orc_world_add_joints(orc_world* w, int n, const orc_joint* js) {
+    for (int i = 0; i < n; ++i) {
+        const orc_joint& s = js[i];
+        Joint j{};
+        j.type = s.type; j.a0 = s.actor0; j.a1 = s.actor1;
+        j.p0 = v3(s.anchor0[0], s.anchor0[1], s.anchor0[2]);
+        j.q0 = {s.anchor0[3], s.anchor0[4], s.anchor0[5], s.anchor0[6]};
+        j.p1 = v3(s.anchor1[0], s.anchor1[1], s.anchor1[2]);
+        j.q1 = {s.anchor1[3], s.anchor1[4], s.anchor1[5], s.anchor1[6]};
+        j.driveEnabled = s.drive_enabled; j.driveVelocity = s.drive_velocity; j.driveMaxTorque = s.drive_max_torque;
+        addJoint(w, j);
+    }
+    return 0;
+}
+
+int orc_world_joint_color(const orc_world* w, int i) { return w->joints[i].color; }
+
+// One joint row through this restatement, same layout as ref_row_scalar (oracle/ref_harness.cpp);
+// simd != 0 runs the per-lane Constraint1DW semantics instead of Constraint1D's.
+int orc_row(int simd, int flags, int b0, int b1, const float* in, const float* masses, float* vel, float* pv, float h,
+            int iters, float* out) {
+    Row r{};
+    r.flags = flags; r.b0 = b0 >= 0 ? 0 : -1; r.b1 = b1 >= 0 ? 1 : -1;
+    r.linear = v3(in[0], in[1], in[2]); r.angular0 = v3(in[3], in[4], in[5]); r.angular1 = v3(in[6], in[7], in[8]);
+    r.tv = in[9]; r.c = in[10]; r.mn = in[11]; r.mx = in[12]; r.freq = in[13]; r.damp = in[14];
+    MassData md[2];
+    VelocityData vd[2];
+    PseudoVelocityData pd[2];
+    for (int k = 0; k < 2; ++k) {
+        md[k].invMass = masses[10 * k];
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) md[k].invI[i][j] = masses[10 * k + 1 + 3 * i + j];
+        vd[k].v = v3(vel[6 * k], vel[6 * k + 1], vel[6 * k + 2]);
+        vd[k].w = v3(vel[6 * k + 3], vel[6 * k + 4], vel[6 * k + 5]);
+        pd[k].pv = v3(pv[7 * k], pv[7 * k + 1], pv[7 * k + 2]);
+        pd[k].pw = v3(pv[7 * k + 3], pv[7 * k + 4], pv[7 * k + 5]);
+        pd[k].count = (int)pv[7 * k + 6];
+    }
+    rowPreSolve(r, md, pd, simd != 0);
+    for (int i = 0; i < iters; ++i) {
+        if (simd) rowSolveSimd(r, vd, h, i == 0);
+        else rowSolveScalar(r, vd, h, i == 0);
+    }
+    for (int k = 0; k < 2; ++k) {
+        vel[6 * k] = vd[k].v.x; vel[6 * k + 1] = vd[k].v.y; vel[6 * k + 2] = vd[k].v.z;
+        vel[6 * k + 3] = vd[k].w.x; vel[6 * k + 4] = vd[k].w.y; vel[6 * k + 5] = vd[k].w.z;
+        pv[7 * k] = pd[k].pv.x; pv[7 * k + 1] = pd[k].pv.y; pv[7 * k + 2] = pd[k].pv.z;
+        pv[7 * k + 3] = pd[k].pw.x; pv[7 * k + 4] = pd[k].pw.y; pv[7 * k + 5] = pd[k].pw.z; pv[7 * k + 6] = (float)pd[k].count;
+    }
+    out[0] = r.invEff;
+    out[1] = r.lambda;
     return 0;
 }
 

@@ -25,6 +25,8 @@
 #include "MassUtil.h"
 #include "MathUtil.h"
 #include "SolverData.h"
+#include "Constraint1D.h"
+#include "Constraint1D.cpp"   // the row templates, compiled from the reference source
 
 using namespace physecs;
 
@@ -42,6 +44,39 @@ static Geometry makeGeom(int type, const float* p) {
 }
 static glm::vec3 V(const float* p) { return glm::vec3(p[0], p[1], p[2]); }
 static glm::quat Qt(const float* q) { return glm::quat(q[3], q[0], q[1], q[2]); }
+
+// One Constraint1D<flags> row (src/Constraint1D.cpp): preSolve, then `iters` solves, the
+// first with warm start. in: linear(3) angular0(3) angular1(3) tv c min max freq damp;
+// masses: invMass0, invI0(9), invMass1, invI1(9); vel: v0 w0 v1 w1 (12, updated);
+// pv: pv0 pw0 count0 pv1 pw1 count1 (14, updated). out: invEff, lambda.
+template <int F>
+static void runRow(int b0, int b1, const float* in, const float* masses, float* vel, float* pv, float h, int iters,
+                   float* out) {
+    Constraint1D<F> r(b0 >= 0 ? 0 : -1, b1 >= 0 ? 1 : -1);
+    r.linear = V(in); r.angular0 = V(in + 3); r.angular1 = V(in + 6);
+    r.targetVelocity = in[9]; r.c = in[10]; r.min = in[11]; r.max = in[12]; r.frequency = in[13]; r.dampingRatio = in[14];
+    MassData md[2];
+    VelocityData vd[2];
+    PseudoVelocityData pd[2];
+    for (int k = 0; k < 2; ++k) {
+        md[k].invMass = masses[10 * k];
+        for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) md[k].invInertiaTensor[i][j] = masses[10 * k + 1 + 3 * i + j];
+        vd[k].velocity = V(vel + 6 * k); vd[k].angularVelocity = V(vel + 6 * k + 3);
+        pd[k].pseudoVelocity = V(pv + 7 * k); pd[k].pseudoAngularVelocity = V(pv + 7 * k + 3);
+        pd[k].constraintCount = (int)pv[7 * k + 6];
+    }
+    r.preSolve(md, pd);
+    for (int i = 0; i < iters; ++i) r.solve(vd, h, i == 0);
+    for (int k = 0; k < 2; ++k) {
+        vel[6 * k] = vd[k].velocity.x; vel[6 * k + 1] = vd[k].velocity.y; vel[6 * k + 2] = vd[k].velocity.z;
+        vel[6 * k + 3] = vd[k].angularVelocity.x; vel[6 * k + 4] = vd[k].angularVelocity.y; vel[6 * k + 5] = vd[k].angularVelocity.z;
+        pv[7 * k] = pd[k].pseudoVelocity.x; pv[7 * k + 1] = pd[k].pseudoVelocity.y; pv[7 * k + 2] = pd[k].pseudoVelocity.z;
+        pv[7 * k + 3] = pd[k].pseudoAngularVelocity.x; pv[7 * k + 4] = pd[k].pseudoAngularVelocity.y;
+        pv[7 * k + 5] = pd[k].pseudoAngularVelocity.z; pv[7 * k + 6] = (float)pd[k].constraintCount;
+    }
+    out[0] = r.invEffMass;
+    out[1] = r.totalLambda;
+}
 
 extern "C" {
 
@@ -177,6 +212,19 @@ int ref_integrate_position(const float* pos, const float* quat, const float* v, 
     out[0] = position.x; out[1] = position.y; out[2] = position.z;
     out[3] = orientation.x; out[4] = orientation.y; out[5] = orientation.z; out[6] = orientation.w;
     return 0;
+}
+
+int ref_row_scalar(int flags, int b0, int b1, const float* in, const float* masses, float* vel, float* pv, float h,
+                   int iters, float* out) {
+    switch (flags) {
+        case NONE: runRow<NONE>(b0, b1, in, masses, vel, pv, h, iters, out); return 0;
+        case ANGULAR: runRow<ANGULAR>(b0, b1, in, masses, vel, pv, h, iters, out); return 0;
+        case SOFT: runRow<SOFT>(b0, b1, in, masses, vel, pv, h, iters, out); return 0;
+        case LIMITED: runRow<LIMITED>(b0, b1, in, masses, vel, pv, h, iters, out); return 0;
+        case ANGULAR | SOFT: runRow<ANGULAR | SOFT>(b0, b1, in, masses, vel, pv, h, iters, out); return 0;
+        case ANGULAR | LIMITED: runRow<ANGULAR | LIMITED>(b0, b1, in, masses, vel, pv, h, iters, out); return 0;
+    }
+    return -1;
 }
 
 }  // extern "C"

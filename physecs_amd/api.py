@@ -26,7 +26,7 @@ EXPORTS = [
     "pgx_world_set_can_collide", "pgx_world_step", "pgx_world_sync", "pgx_world_read_state",
     "pgx_world_write_state", "pgx_world_get_counts", "pgx_world_get_bounds", "pgx_world_get_pairs",
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
-    "pgx_world_get_stats", "pgx_world_set_exec",
+    "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints",
 ]
 
 
@@ -44,6 +44,12 @@ class PgxSceneDesc(ctypes.Structure):
         ("col_type", I32), ("col_params", F), ("col_material", F), ("col_trigger", U8), ("col_enable", U8),
         ("col_data", I32), ("n_worlds", ctypes.c_int), ("actor_world", I32),
     ]
+
+
+class PgxJointDesc(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("actor0", ctypes.c_int32), ("actor1", ctypes.c_int32),
+                ("anchor0", ctypes.c_float * 7), ("anchor1", ctypes.c_float * 7), ("drive_enabled", ctypes.c_int32),
+                ("drive_velocity", ctypes.c_float), ("drive_max_torque", ctypes.c_float)]
 
 
 class PgxCounts(ctypes.Structure):
@@ -89,6 +95,7 @@ def lib():
         L.pgx_world_reset_stats.argtypes = [vp]
         L.pgx_world_get_stats.argtypes = [vp, ctypes.POINTER(PgxStats)]
         L.pgx_world_set_exec.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.pgx_world_add_joints.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PgxJointDesc)]
         for name in EXPORTS:
             if name != "pgx_last_error":
                 getattr(L, name).restype = ctypes.c_int
@@ -130,6 +137,9 @@ class World:
         check(self.L.pgx_world_set_params(self.h, substeps, iterations, gravity))
         for e0, e1 in getattr(scene, "non_colliding", []):
             check(self.L.pgx_world_set_can_collide(self.h, e0, e1, 0))
+        if getattr(scene, "joints", None):
+            arr = scene.joint_array(PgxJointDesc)
+            check(self.L.pgx_world_add_joints(self.h, len(scene.joints), arr))
 
     def close(self):
         if getattr(self, "h", None):

@@ -1,0 +1,375 @@
+// Joint rows on the device: row creation per step, world-space rows per substep, preSolve
+// (NGS pseudo-velocities) and solve, in the reference's joint colours.
+//
+// Reference semantics:
+//   colouring at creation            src/Physecs.cpp:690-710 (host, pgx_world_add_joints)
+//   rows created every step          src/Physecs.cpp:321-355, include/Physecs/Constraint1DContainer.h:133-198
+//   world-space data, row values     src/Joint.cpp:5-14, src/JointsUtil.h:8-28,
+//                                    src/Joints/{Spherical,Universal,Revolute,Fixed}Joint.cpp
+//   colours 0-7 (SIMD rows)          src/Constraint1DW.cpp:5-233, per lane
+//   colour 8 (scalar overflow)       src/Constraint1D.cpp:5-127
+// Joints of one colour 0-7 share no entity, so each is one thread that walks its rows in
+// flag-type order; that is exactly what the reference's 4-wide rows compute lane by lane.
+// Colour 8 may share bodies and is walked by one thread, flag type by flag type over all
+// its joints, as the reference's overflow container does.
+#include "pgx_internal.h"
+
+namespace pgx {
+
+enum { JF_NONE = 0, JF_SOFT = 1, JF_ANGULAR = 2, JF_LIMITED = 4 };
+enum { JT_SPHERICAL = 0, JT_UNIVERSAL = 1, JT_REVOLUTE = 2, JT_FIXED = 3 };
+__constant__ int kFlagOrder[6] = {JF_NONE, JF_ANGULAR, JF_SOFT, JF_LIMITED, JF_ANGULAR | JF_SOFT,
+                                  JF_ANGULAR | JF_LIMITED};
+
+// row fields (float4) of row r of joint j: jRow[(r * 8 + f) * Jcap + j]
+enum { JR_LIN = 0, JR_ANG0, JR_ANG1, JR_LIN0T, JR_LIN1T, JR_ANG0T, JR_ANG1T, JR_EXTRA, JR_NF };
+// JR_LIN.w = targetVelocity, JR_ANG0.w = c, JR_ANG1.w = invEffMass, JR_LIN0T.w = lambda,
+// JR_LIN1T.w = min, JR_ANG0T.w = max, JR_ANG1T.w = frequency, JR_EXTRA.x = dampingRatio
+
+struct JointArgs {
+    int J, Jcap;
+    const int32_t *jType, *jA0, *jA1, *jRowFlags, *jNRows;   // jRowFlags: 4 bits per row
+    const float4 *jP0, *jQ0, *jP1, *jQ1, *jDrive;            // anchors (body space), drive (enabled, v, maxTorque)
+    int32_t *jB0, *jB1;
+    float4 *jR0, *jR1;                                        // anchor - com (or anchor), body space
+    float4* jRow;
+    const float4 *bpos, *bquat, *dcom, *dinvI, *dvel;
+    float4 *tvel, *tang, *pvel, *pang;
+    const int32_t* order;                                     // joints sorted by colour
+    float h;
+};
+
+#define JROW(r, f) a.jRow[((size_t)((r) * JR_NF + (f))) * a.Jcap + j]
+
+__device__ __forceinline__ int row_flags(const JointArgs& a, int j, int r) { return (a.jRowFlags[j] >> (4 * r)) & 15; }
+
+// src/Physecs.cpp:332-352: body indices and the anchors relative to the COM; fresh rows
+// (Constraint1D defaults: zeros, min = lowest, max = max, lambda 0)
+__global__ void k_joint_setup(JointArgs a) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.J) return;
+    int A0 = a.jA0[j], A1 = a.jA1[j];
+    // dynamic bodies are the first ones; a kinematic body carries a nonzero dcom.w
+    // (the descriptor's actor order is mapped to these internal indices on the host)
+    int b0 = a.jB0[j], b1 = a.jB1[j];
+    f3 p0 = xyz(a.jP0[j]), p1 = xyz(a.jP1[j]);
+    a.jR0[j] = f4(b0 >= 0 ? sub(p0, xyz(a.dcom[b0])) : p0, 0.f);
+    a.jR1[j] = f4(b1 >= 0 ? sub(p1, xyz(a.dcom[b1])) : p1, 0.f);
+    (void)A0; (void)A1;
+    int n = a.jNRows[j];
+    for (int r = 0; r < n; ++r) {
+        for (int f = 0; f < JR_NF; ++f) JROW(r, f) = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 e = JROW(r, JR_LIN1T); e.w = -3.402823466e+38f; JROW(r, JR_LIN1T) = e;
+        float4 m = JROW(r, JR_ANG0T); m.w = 3.402823466e+38f; JROW(r, JR_ANG0T) = m;
+    }
+}
+
+__device__ __forceinline__ void set_xyz(float4& d, f3 v) { d.x = v.x; d.y = v.y; d.z = v.z; }
+
+// world-space data and the rows' values for one joint, every substep
+__device__ void joint_update(const JointArgs& a, int j) {
+    int A0 = a.jA0[j], A1 = a.jA1[j];
+    qf t0 = q4(a.bquat[A0]), t1 = q4(a.bquat[A1]);
+    m3 u0 = mat3_cast(qmul(t0, q4(a.jQ0[j])));
+    m3 u1 = mat3_cast(qmul(t1, q4(a.jQ1[j])));
+    f3 r0 = qrot(t0, xyz(a.jR0[j]));
+    f3 r1 = qrot(t1, xyz(a.jR1[j]));
+    f3 p0 = add(xyz(a.bpos[A0]), qrot(t0, xyz(a.jP0[j])));
+    f3 p1 = add(xyz(a.bpos[A1]), qrot(t1, xyz(a.jP1[j])));
+    f3 d = sub(p1, p0);
+    f3 lin[3] = {mk3(1.f, 0.f, 0.f), mk3(0.f, 1.f, 0.f), mk3(0.f, 0.f, 1.f)};
+    f3 a0[3] = {mk3(0.f, r0.z, -r0.y), mk3(-r0.z, 0.f, r0.x), mk3(r0.y, -r0.x, 0.f)};
+    f3 a1[3] = {mk3(0.f, r1.z, -r1.y), mk3(-r1.z, 0.f, r1.x), mk3(r1.y, -r1.x, 0.f)};
+    float dc[3] = {d.x, d.y, d.z};
+    for (int k = 0; k < 3; ++k) {
+        float4 l = JROW(k, JR_LIN), g0 = JROW(k, JR_ANG0), g1 = JROW(k, JR_ANG1);
+        set_xyz(l, lin[k]); set_xyz(g0, a0[k]); g0.w = dc[k]; set_xyz(g1, a1[k]);
+        JROW(k, JR_LIN) = l; JROW(k, JR_ANG0) = g0; JROW(k, JR_ANG1) = g1;
+    }
+    auto angular_row = [&](int r, f3 x, f3 y) {   // c = dot(x, y), axis = cross(y, x)
+        f3 ax = cross(y, x);
+        float4 g0 = JROW(r, JR_ANG0), g1 = JROW(r, JR_ANG1);
+        set_xyz(g0, ax); g0.w = dot(x, y); set_xyz(g1, ax);
+        JROW(r, JR_ANG0) = g0; JROW(r, JR_ANG1) = g1;
+    };
+    int type = a.jType[j];
+    if (type == JT_UNIVERSAL) angular_row(3, u0.c2, u1.c2);
+    if (type == JT_REVOLUTE || type == JT_FIXED) {
+        angular_row(3, u0.c0, u1.c1);
+        angular_row(4, u0.c0, u1.c2);
+        if (type == JT_FIXED) angular_row(5, u0.c1, u1.c2);
+        float4 dr = a.jDrive[j];
+        if (type == JT_REVOLUTE && dr.x != 0.f) {
+            float4 l = JROW(5, JR_LIN), g0 = JROW(5, JR_ANG0), g1 = JROW(5, JR_ANG1);
+            float4 mn = JROW(5, JR_LIN1T), mx = JROW(5, JR_ANG0T);
+            set_xyz(g0, u0.c0); set_xyz(g1, u0.c0);
+            l.w = dr.y; mx.w = dr.z; mn.w = -dr.z;
+            JROW(5, JR_LIN) = l; JROW(5, JR_ANG0) = g0; JROW(5, JR_ANG1) = g1;
+            JROW(5, JR_LIN1T) = mn; JROW(5, JR_ANG0T) = mx;
+        }
+    }
+}
+
+__global__ void k_joint_update(JointArgs a) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < a.J) joint_update(a, j);
+}
+
+__device__ __forceinline__ m3 world_inv_inertia_j(const JointArgs& a, int b) {
+    m3 rot = mat3_cast(q4(a.bquat[b]));
+    m3 invI = mk_m3(xyz(a.dinvI[3 * b]), xyz(a.dinvI[3 * b + 1]), xyz(a.dinvI[3 * b + 2]));
+    return mulmm(mulmm(rot, invI), transpose(rot));
+}
+
+// Constraint1D<flags>::preSolve / Constraint1DW<flags>::preSolve for one row
+__device__ void row_presolve(const JointArgs& a, int j, int r, int flags, bool simd) {
+    int b0 = a.jB0[j], b1 = a.jB1[j];
+    float im0 = 0.f, im1 = 0.f;
+    m3 I0 = mzero(), I1 = mzero();
+    if (b0 >= 0) { im0 = a.dvel[b0].w; I0 = world_inv_inertia_j(a, b0); }
+    if (b1 >= 0) { im1 = a.dvel[b1].w; I1 = world_inv_inertia_j(a, b1); }
+    bool ang = flags & JF_ANGULAR;
+    float4 L = JROW(r, JR_LIN), G0 = JROW(r, JR_ANG0), G1 = JROW(r, JR_ANG1);
+    f3 lin = xyz(L), ang0 = xyz(G0), ang1 = xyz(G1);
+    float c = G0.w;
+    f3 ang0t = mulmv(I0, ang0), ang1t = mulmv(I1, ang1);
+    float invEff = dot(ang0, ang0t) + dot(ang1, ang1t);
+    f3 lin0t = s3(0.f), lin1t = s3(0.f);
+    float4 L0 = JROW(r, JR_LIN0T), L1 = JROW(r, JR_LIN1T);
+    if (!ang) {
+        invEff = invEff + dot(lin, lin) * (im0 + im1);
+        lin0t = mul(im0, lin);
+        lin1t = mul(im1, lin);
+        set_xyz(L0, lin0t);
+        set_xyz(L1, lin1t);
+    }
+    float4 T0 = JROW(r, JR_ANG0T), T1 = JROW(r, JR_ANG1T);
+    set_xyz(T0, ang0t);
+    set_xyz(T1, ang1t);
+    G1.w = invEff;
+    JROW(r, JR_ANG1) = G1; JROW(r, JR_LIN0T) = L0; JROW(r, JR_LIN1T) = L1; JROW(r, JR_ANG0T) = T0;
+    JROW(r, JR_ANG1T) = T1;
+    if (flags & JF_SOFT) return;
+    if (c == 0.f || invEff == 0.f) return;
+    float lambda = c / invEff;
+    if (flags & JF_LIMITED) {
+        float mn = L1.w, mx = T0.w;
+        if (simd) { lambda = lambda > mn ? lambda : mn; lambda = lambda < mx ? lambda : mx; }
+        else lambda = gclamp(lambda, mn, mx);
+    }
+    if (b0 >= 0) {
+        float4 pv = a.pvel[b0], pw = a.pang[b0];
+        if (!ang) set_xyz(pv, add(xyz(pv), mul(lambda, lin0t)));
+        set_xyz(pw, add(xyz(pw), mul(lambda, ang0t)));
+        pw.w = pw.w + 1.f;
+        a.pvel[b0] = pv; a.pang[b0] = pw;
+    }
+    if (b1 >= 0) {
+        float4 pv = a.pvel[b1], pw = a.pang[b1];
+        if (!ang) set_xyz(pv, sub(xyz(pv), mul(lambda, lin1t)));
+        set_xyz(pw, sub(xyz(pw), mul(lambda, ang1t)));
+        pw.w = pw.w + 1.f;
+        a.pvel[b1] = pv; a.pang[b1] = pw;
+    }
+}
+
+// Constraint1DW<flags>::solve for one lane (src/Constraint1DW.cpp:117-233)
+__device__ void row_solve_simd(const JointArgs& a, int j, int r, int flags, bool warm) {
+    int b0 = a.jB0[j], b1 = a.jB1[j];
+    bool ang = flags & JF_ANGULAR;
+    float h = a.h;
+    f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
+    if (b0 >= 0) { if (!ang) v0 = xyz(a.tvel[b0]); w0 = xyz(a.tang[b0]); }
+    if (b1 >= 0) { if (!ang) v1 = xyz(a.tvel[b1]); w1 = xyz(a.tang[b1]); }
+    float4 L = JROW(r, JR_LIN), G0 = JROW(r, JR_ANG0), G1 = JROW(r, JR_ANG1);
+    float4 L0 = JROW(r, JR_LIN0T), L1 = JROW(r, JR_LIN1T), T0 = JROW(r, JR_ANG0T), T1 = JROW(r, JR_ANG1T);
+    float4 EX = JROW(r, JR_EXTRA);
+    f3 lin0t = xyz(L0), lin1t = xyz(L1), ang0t = xyz(T0), ang1t = xyz(T1);
+    float c = G0.w, invEff = G1.w, lam = L0.w;
+    if (!(flags & JF_SOFT) && warm) {
+        if (fabsf(c) < 1e-4f && fabsf(lam) < 10000.f) {
+            float l2 = lam * 0.5f;
+            if (!ang) { v0 = add(v0, mul(l2, lin0t)); v1 = sub(v1, mul(l2, lin1t)); }
+            w0 = add(w0, mul(l2, ang0t));
+            w1 = sub(w1, mul(l2, ang1t));
+            lam = l2;
+        }
+    }
+    if (invEff == 0.f) {
+        L0.w = lam;
+        JROW(r, JR_LIN0T) = L0;
+        return;
+    }
+    float rv = dot(xyz(G1), w1) - dot(xyz(G0), w0);
+    if (!ang) rv = rv + (dot(xyz(L), v1) - dot(xyz(L), v0));
+    float effMass = 1.f / invEff;
+    float lambda;
+    if (flags & JF_SOFT) {
+        float twoPi = 2.f * 3.14159265358979323846f;
+        float af = twoPi * T1.w;
+        float stiffness = af * af * effMass;
+        float damping = 2.f * af * EX.x * effMass;
+        float gamma = 1.f / (damping + h * stiffness);
+        float beta = h * stiffness * gamma;
+        lambda = (rv + beta * c / h) / (invEff + gamma / h);
+    } else {
+        float bias = (float)(0.2 / (double)h);
+        lambda = (rv - L.w + bias * c) * effMass;
+    }
+    float prev = lam;
+    if (flags & JF_LIMITED) {
+        float t = lam + lambda;
+        float lo = L1.w * h, hi = T0.w * h;
+        t = t > lo ? t : lo;
+        t = t < hi ? t : hi;
+        lam = t;
+        lambda = lam - prev;
+    } else {
+        lam = lam + lambda;
+    }
+    if (!ang) { v0 = add(v0, mul(lambda, lin0t)); v1 = sub(v1, mul(lambda, lin1t)); }
+    w0 = add(w0, mul(lambda, ang0t));
+    w1 = sub(w1, mul(lambda, ang1t));
+    L0.w = lam;
+    JROW(r, JR_LIN0T) = L0;
+    if (b0 >= 0) {
+        if (!ang) { float4 t = a.tvel[b0]; set_xyz(t, v0); a.tvel[b0] = t; }
+        float4 t = a.tang[b0]; set_xyz(t, w0); a.tang[b0] = t;
+    }
+    if (b1 >= 0) {
+        if (!ang) { float4 t = a.tvel[b1]; set_xyz(t, v1); a.tvel[b1] = t; }
+        float4 t = a.tang[b1]; set_xyz(t, w1); a.tang[b1] = t;
+    }
+}
+
+// Constraint1D<flags>::solve (src/Constraint1D.cpp:54-127)
+__device__ void row_solve_scalar(const JointArgs& a, int j, int r, int flags, bool warm) {
+    float4 G1 = JROW(r, JR_ANG1);
+    float invEff = G1.w;
+    if (invEff == 0.f) return;
+    int b0 = a.jB0[j], b1 = a.jB1[j];
+    bool ang = flags & JF_ANGULAR;
+    float h = a.h;
+    f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
+    if (b0 >= 0) { if (!ang) v0 = xyz(a.tvel[b0]); w0 = xyz(a.tang[b0]); }
+    if (b1 >= 0) { if (!ang) v1 = xyz(a.tvel[b1]); w1 = xyz(a.tang[b1]); }
+    float4 L = JROW(r, JR_LIN), G0 = JROW(r, JR_ANG0);
+    float4 L0 = JROW(r, JR_LIN0T), L1 = JROW(r, JR_LIN1T), T0 = JROW(r, JR_ANG0T), T1 = JROW(r, JR_ANG1T);
+    float4 EX = JROW(r, JR_EXTRA);
+    f3 lin0t = xyz(L0), lin1t = xyz(L1), ang0t = xyz(T0), ang1t = xyz(T1);
+    float c = G0.w, lam = L0.w;
+    if (!(flags & JF_SOFT)) {
+        if (warm && !((double)gabs(c) > 1e-4 || gabs(lam) > 10000.f)) {
+            lam = lam * 0.5f;
+            if (b0 >= 0) { if (!ang) v0 = add(v0, mul(lam, lin0t)); w0 = add(w0, mul(lam, ang0t)); }
+            if (b1 >= 0) { if (!ang) v1 = sub(v1, mul(lam, lin1t)); w1 = sub(w1, mul(lam, ang1t)); }
+        }
+    }
+    float rv = dot(xyz(G1), w1) - dot(xyz(G0), w0);
+    if (!ang) rv = rv + (dot(xyz(L), v1) - dot(xyz(L), v0));
+    float lambda;
+    if (flags & JF_SOFT) {
+        float af = 2.f * 3.14159265358979323846f * T1.w;
+        float stiffness = af * af / invEff;
+        float damping = 2.f * af * EX.x / invEff;
+        float gamma = 1.f / (damping + h * stiffness);
+        float beta = h * stiffness / (damping + h * stiffness);
+        lambda = (rv + beta * c / h) / (invEff + gamma / h);
+    } else {
+        lambda = (rv - L.w + 0.2f * c / h) / invEff;
+    }
+    if (flags & JF_LIMITED) {
+        float prev = lam;
+        lam = lam + lambda;
+        lam = gclamp(lam, L1.w, T0.w);
+        lambda = lam - prev;
+    } else {
+        lam = lam + lambda;
+    }
+    L0.w = lam;
+    JROW(r, JR_LIN0T) = L0;
+    if (b0 >= 0) {
+        if (!ang) { float4 t = a.tvel[b0]; set_xyz(t, add(xyz(t), mul(lambda, lin0t))); a.tvel[b0] = t; }
+        float4 t = a.tang[b0]; set_xyz(t, add(xyz(t), mul(lambda, ang0t))); a.tang[b0] = t;
+    }
+    if (b1 >= 0) {
+        if (!ang) { float4 t = a.tvel[b1]; set_xyz(t, sub(xyz(t), mul(lambda, lin1t))); a.tvel[b1] = t; }
+        float4 t = a.tang[b1]; set_xyz(t, sub(xyz(t), mul(lambda, ang1t))); a.tang[b1] = t;
+    }
+}
+
+// one SIMD colour: a thread per joint, its rows in flag-type order
+__global__ void k_joint_color(JointArgs a, int begin, int end, int phase, int warm) {
+    int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= end) return;
+    int j = a.order[k];
+    int n = a.jNRows[j];
+    for (int f = 0; f < 6; ++f)
+        for (int r = 0; r < n; ++r) {
+            int fl = row_flags(a, j, r);
+            if (fl != kFlagOrder[f]) continue;
+            if (phase == 0) row_presolve(a, j, r, fl, true);
+            else row_solve_simd(a, j, r, fl, warm != 0);
+        }
+}
+
+// the overflow colour: one thread, flag type by flag type over all its joints
+__global__ void k_joint_overflow(JointArgs a, int begin, int end, int phase, int warm) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int f = 0; f < 6; ++f)
+        for (int k = begin; k < end; ++k) {
+            int j = a.order[k];
+            int n = a.jNRows[j];
+            for (int r = 0; r < n; ++r) {
+                int fl = row_flags(a, j, r);
+                if (fl != kFlagOrder[f]) continue;
+                if (phase == 0) row_presolve(a, j, r, fl, false);
+                else row_solve_scalar(a, j, r, fl, warm != 0);
+            }
+        }
+}
+
+static JointArgs make_jargs(World& w, float h) {
+    JointArgs a;
+    a.J = w.J; a.Jcap = w.jCap;
+    a.jType = w.jType.p; a.jA0 = w.jA0.p; a.jA1 = w.jA1.p; a.jRowFlags = w.jRowFlags.p; a.jNRows = w.jNRows.p;
+    a.jP0 = w.jP0.p; a.jQ0 = w.jQ0.p; a.jP1 = w.jP1.p; a.jQ1 = w.jQ1.p; a.jDrive = w.jDrive.p;
+    a.jB0 = w.jB0.p; a.jB1 = w.jB1.p; a.jR0 = w.jR0.p; a.jR1 = w.jR1.p; a.jRow = w.jRow.p;
+    a.bpos = w.bpos.p; a.bquat = w.bquat.p; a.dcom = w.dcom.p; a.dinvI = w.dinvI.p; a.dvel = w.dvel.p;
+    a.tvel = w.tvel.p; a.tang = w.tang.p; a.pvel = w.pvel.p; a.pang = w.pang.p;
+    a.order = w.jOrder.p;
+    a.h = h;
+    return a;
+}
+
+void jt_step_setup(World& w) {
+    if (!w.J) return;
+    JointArgs a = make_jargs(w, 0.f);
+    hipLaunchKernelGGL(k_joint_setup, dim3(div_up(w.J, 256)), dim3(256), 0, w.stream, a);
+    PGX_LAUNCH_CHECK();
+}
+
+void jt_update(World& w, float h) {
+    if (!w.J) return;
+    JointArgs a = make_jargs(w, h);
+    hipLaunchKernelGGL(k_joint_update, dim3(div_up(w.J, 256)), dim3(256), 0, w.stream, a);
+    PGX_LAUNCH_CHECK();
+}
+
+// phase 0 = preSolve, 1 = solve (warm start on the first iteration)
+void jt_colors(World& w, float h, int phase, bool warm) {
+    if (!w.J) return;
+    JointArgs a = make_jargs(w, h);
+    for (int c = 0; c < 9; ++c) {
+        int b = w.jColorStart[c], e = w.jColorStart[c + 1];
+        if (e <= b) continue;
+        if (c < 8)
+            hipLaunchKernelGGL(k_joint_color, dim3(div_up(e - b, 64)), dim3(64), 0, w.stream, a, b, e, phase,
+                               warm ? 1 : 0);
+        else
+            hipLaunchKernelGGL(k_joint_overflow, dim3(1), dim3(64), 0, w.stream, a, b, e, phase, warm ? 1 : 0);
+    }
+    PGX_LAUNCH_CHECK();
+}
+
+}  // namespace pgx

@@ -88,6 +88,7 @@ struct World {
     std::vector<uint32_t> hostEntity;            // per internal body
     std::vector<int> hostColBody;
     std::vector<int> hostColIndex;
+    std::vector<uint8_t> hostKinematic;
     std::vector<uint64_t> hostNonColliding;       // sorted (min entity << 32 | max entity)
     bool nonCollidingDirty = false;
 
@@ -167,6 +168,15 @@ struct World {
     DBuf<uint64_t> nonColliding;
     int nNonColliding = 0;
 
+    // joints (descriptor order = creation order), colours as Scene::addJoint assigns them
+    int J = 0, jCap = 0, jRowsTotal = 0;
+    std::vector<int32_t> hjType, hjA0, hjA1, hjB0, hjB1, hjRowFlags, hjNRows, hjColor;
+    std::vector<float4> hjP0, hjQ0, hjP1, hjQ1, hjDrive;
+    std::vector<unsigned char> hjBits;           // per internal body: colours taken (src/Physecs.cpp:696-705)
+    std::vector<int> jColorStart;                // 10 offsets into jOrder
+    DBuf<int32_t> jType, jA0, jA1, jB0, jB1, jRowFlags, jNRows, jOrder;
+    DBuf<float4> jP0, jQ0, jP1, jQ1, jDrive, jR0, jR1, jRow;
+
     // last step stats
     int lastPairs = 0, lastManifolds = 0, lastColors = 0, lastRounds = 0, lastPoints = 0;
     std::vector<int> colorStartHost;
@@ -198,5 +208,10 @@ int sv_color(World& w, int nManifolds);                 // returns colour count
 void sv_substeps(World& w, int nManifolds, float dt);
 void sv_finish_step(World& w, int nManifolds);
 double sv_step_bytes(const World& w, int M, long points);
+
+// joints.hip
+void jt_step_setup(World& w);
+void jt_update(World& w, float h);
+void jt_colors(World& w, float h, int phase, bool warm);
 
 }  // namespace pgx

@@ -646,7 +646,7 @@ void sv_substeps(World& w, int M, float dt) {
     hipStream_t s = w.stream;
     w.solveEvUsed = 0;
     SubArgs a = make_args(w, M, dt);
-    if (w.persistent) {
+    if (w.persistent && w.J == 0) {
         int blocks = persistent_blocks(w);
         void* args[] = {&a};
         if (w.timing) PGX_HIP(hipEventRecord(w.ev[6], s));
@@ -656,10 +656,15 @@ void sv_substeps(World& w, int M, float dt) {
     }
     int nbP = div_up(std::max(M + w.D, 1), 256), nbD = div_up(std::max(w.D, 1), 256);
     int nColors = a.nColors;
+    jt_step_setup(w);
     for (int sub = 0; sub < w.substeps; ++sub) {
         const bool rec = w.timing && sub == 0;
+        // rows + preSolve of the contacts, velocity integration; then the joint rows, whose
+        // preSolve follows the contacts' (src/Physecs.cpp:430-479)
         hipLaunchKernelGGL(k_prepare, dim3(nbP), dim3(256), 0, s, a);
-        for (int it = 0; it < w.iterations; ++it)
+        jt_update(w, a.h);
+        jt_colors(w, a.h, 0, false);
+        for (int it = 0; it < w.iterations; ++it) {
             for (int c = 0; c < nColors; ++c) {
                 int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
                 if (e <= b) continue;
@@ -667,6 +672,8 @@ void sv_substeps(World& w, int M, float dt) {
                 hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 64)), dim3(64), 0, s, a, b, e, 1, a.h, 0);
                 solve_event(w, rec, c, true);
             }
+            jt_colors(w, a.h, 1, it == 0);   // joint colours after the contacts (Physecs.cpp:482-490)
+        }
         hipLaunchKernelGGL(k_integrate_positions, dim3(nbD), dim3(256), 0, s, a);
         for (int c = 0; c < nColors; ++c) {
             int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];

@@ -174,6 +174,7 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
             if (w.worldBits + 30 > 64 || w.worldBits + 2 * w.localBits > 64)
                 throw PgxError{PGX_ERR_INVALID, "too many worlds or colliders per world for 64-bit keys"};
             w.hostColBody = cbody;
+            w.hostKinematic.assign(sc->kinematic, sc->kinematic + D);
             w.hostColIndex = cindex;
             w.hostEntity = bent;
 
@@ -242,6 +243,10 @@ static void release_world(World& w) {
     for (auto* b : u64s) b->release();
     w.bentity.release(); w.centity.release(); w.worldBoundsI.release(); w.sortT.hist.release();
     w.nodeRange.release(); w.sSoft.release();
+    DBuf<int32_t>* ji[] = {&w.jType, &w.jA0, &w.jA1, &w.jB0, &w.jB1, &w.jRowFlags, &w.jNRows, &w.jOrder};
+    for (auto* b : ji) b->release();
+    DBuf<float4>* jf[] = {&w.jP0, &w.jQ0, &w.jP1, &w.jQ1, &w.jDrive, &w.jR0, &w.jR1, &w.jRow};
+    for (auto* b : jf) b->release();
     for (auto& e : w.ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : w.solveEv) (void)hipEventDestroy(e);
@@ -282,6 +287,74 @@ int pgx_world_set_can_collide(pgx_world* h, uint32_t e0, uint32_t e1, int canCol
     if (canCollide && present) v.erase(it);
     h->w.nonCollidingDirty = true;
     return PGX_OK;
+}
+
+int pgx_world_add_joints(pgx_world* h, int n, const pgx_joint_desc* js) {
+    if (!h || (n > 0 && !js)) return fail(PGX_ERR_INVALID, "NULL argument");
+    World& w = h->w;
+    for (int i = 0; i < n; ++i) {
+        const pgx_joint_desc& d = js[i];
+        if (d.type < PGX_JOINT_SPHERICAL || d.type > PGX_JOINT_FIXED)
+            return fail(PGX_ERR_UNSUPPORTED, "joint %d: type %d is not supported on the device path", i, d.type);
+        if (d.actor0 < 0 || d.actor0 >= w.A || d.actor1 < 0 || d.actor1 >= w.A)
+            return fail(PGX_ERR_INVALID, "joint %d: actor out of range", i);
+    }
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(w.device));
+        if (w.hjBits.empty()) w.hjBits.assign(w.A, 0);
+        for (int i = 0; i < n; ++i) {
+            const pgx_joint_desc& d = js[i];
+            int a0 = w.actorToBody[d.actor0], a1 = w.actorToBody[d.actor1];
+            // Scene::addJoint (src/Physecs.cpp:690-710)
+            uint32_t e0 = w.hostEntity[a0], e1 = w.hostEntity[a1];
+            uint64_t k = ((uint64_t)std::min(e0, e1) << 32) | std::max(e0, e1);
+            auto it = std::lower_bound(w.hostNonColliding.begin(), w.hostNonColliding.end(), k);
+            if (it == w.hostNonColliding.end() || *it != k) w.hostNonColliding.insert(it, k);
+            w.nonCollidingDirty = true;
+            unsigned int inv = ~(unsigned int)(w.hjBits[a0] | w.hjBits[a1]);
+            int color = 0;
+            while (!((inv >> color) & 1u)) ++color;
+            w.hjBits[a0] = (unsigned char)(w.hjBits[a0] | (1u << color));
+            w.hjBits[a1] = (unsigned char)(w.hjBits[a1] | (1u << color));
+            // rows in creation order, which is also flag-type order for these types
+            int fl[6], nr = 0;
+            for (int r = 0; r < 3; ++r) fl[nr++] = 0;                 // point-to-point: NONE
+            if (d.type == PGX_JOINT_UNIVERSAL) fl[nr++] = 2;          // ANGULAR
+            if (d.type == PGX_JOINT_REVOLUTE) { fl[nr++] = 2; fl[nr++] = 2; if (d.drive_enabled) fl[nr++] = 2 | 4; }
+            if (d.type == PGX_JOINT_FIXED) for (int r = 0; r < 3; ++r) fl[nr++] = 2;
+            int packed = 0;
+            for (int r = 0; r < nr; ++r) packed |= fl[r] << (4 * r);
+            auto dyn = [&](int b) { return b < w.D && !w.hostKinematic[b] ? b : -1; };
+            w.hjType.push_back(d.type); w.hjA0.push_back(a0); w.hjA1.push_back(a1);
+            w.hjB0.push_back(dyn(a0)); w.hjB1.push_back(dyn(a1));
+            w.hjRowFlags.push_back(packed); w.hjNRows.push_back(nr); w.hjColor.push_back(color);
+            w.hjP0.push_back(make_float4(d.anchor0[0], d.anchor0[1], d.anchor0[2], 0.f));
+            w.hjQ0.push_back(make_float4(d.anchor0[3], d.anchor0[4], d.anchor0[5], d.anchor0[6]));
+            w.hjP1.push_back(make_float4(d.anchor1[0], d.anchor1[1], d.anchor1[2], 0.f));
+            w.hjQ1.push_back(make_float4(d.anchor1[3], d.anchor1[4], d.anchor1[5], d.anchor1[6]));
+            w.hjDrive.push_back(make_float4(d.drive_enabled ? 1.f : 0.f, d.drive_velocity, d.drive_max_torque, 0.f));
+            w.jRowsTotal += nr;
+        }
+        w.J = (int)w.hjType.size();
+        // joints grouped by colour, creation order inside a colour
+        std::vector<int32_t> order(w.J);
+        w.jColorStart.assign(10, 0);
+        for (int j = 0; j < w.J; ++j) ++w.jColorStart[w.hjColor[j] + 1];
+        for (int c = 0; c < 9; ++c) w.jColorStart[c + 1] += w.jColorStart[c];
+        std::vector<int> fillp(w.jColorStart.begin(), w.jColorStart.end() - 1);
+        for (int j = 0; j < w.J; ++j) order[fillp[w.hjColor[j]]++] = j;
+        hipStream_t s = w.stream;
+        w.jCap = w.J;
+        upload(w.jType, w.hjType, s); upload(w.jA0, w.hjA0, s); upload(w.jA1, w.hjA1, s);
+        upload(w.jB0, w.hjB0, s); upload(w.jB1, w.hjB1, s); upload(w.jRowFlags, w.hjRowFlags, s);
+        upload(w.jNRows, w.hjNRows, s); upload(w.jOrder, order, s);
+        upload(w.jP0, w.hjP0, s); upload(w.jQ0, w.hjQ0, s); upload(w.jP1, w.hjP1, s); upload(w.jQ1, w.hjQ1, s);
+        upload(w.jDrive, w.hjDrive, s);
+        w.jR0.alloc(w.J); w.jR1.alloc(w.J);
+        w.jRow.alloc((size_t)6 * 8 * w.J);
+        PGX_HIP(hipStreamSynchronize(s));
+        return PGX_OK;
+    });
 }
 
 int pgx_world_step(pgx_world* h, float dt) {
@@ -423,7 +496,7 @@ int pgx_world_get_counts(pgx_world* h, pgx_counts* out) {
     out->pairs = h->w.lastPairs;
     out->manifolds = h->w.lastManifolds;
     out->colors = h->w.lastColors;
-    out->joint_rows = 0;
+    out->joint_rows = h->w.jRowsTotal;
     out->coloring_rounds = h->w.lastRounds;
     out->points = 0;
     return guarded([&] {

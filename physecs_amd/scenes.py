@@ -16,6 +16,7 @@ import math
 import numpy as np
 
 SPHERE, CAPSULE, BOX, CONVEX_MESH, TRIANGLE_MESH = range(5)
+SPHERICAL, UNIVERSAL, REVOLUTE, FIXED = range(4)
 
 
 def _inertia_sphere(m, r):
@@ -78,6 +79,7 @@ class Scene:
         self.col_type, self.col_params, self.col_material = [], [], []
         self.col_trigger, self.col_enable, self.col_data = [], [], []
         self.non_colliding = []
+        self.joints = []        # (type, actor0, actor1, anchor0[7], anchor1[7], drive_enabled, drive_v, drive_max)
         self._ncol = {}
         self.n_worlds = 1
         self._frozen = None
@@ -135,6 +137,25 @@ class Scene:
         a = self.add_actor(pos, quat, world)
         self.add_collider(a, gtype, params, material)
         return a
+
+    def add_joint(self, jtype, a0, a1, p0, p1, q0=(0, 0, 0, 1), q1=(0, 0, 0, 1), drive=None):
+        """Scene::createJoint<T>(entity0, anchor0Pos, anchor0Or, entity1, anchor1Pos, anchor1Or);
+        anchors in body space. drive = (velocity, max torque) enables a revolute drive."""
+        an0 = list(p0) + list(q0)
+        an1 = list(p1) + list(q1)
+        d = (1, drive[0], drive[1]) if drive else (0, 0.0, 0.0)
+        self.joints.append((jtype, a0, a1, an0, an1) + d)
+
+    def joint_array(self, cls):
+        """ctypes array of joint descriptors (same field layout for the oracle and the C ABI)."""
+        arr = (cls * max(1, len(self.joints)))()
+        for i, (t, a0, a1, an0, an1, de, dv, dm) in enumerate(self.joints):
+            arr[i].type, arr[i].actor0, arr[i].actor1 = t, a0, a1
+            for k in range(7):
+                arr[i].anchor0[k] = an0[k]
+                arr[i].anchor1[k] = an1[k]
+            arr[i].drive_enabled, arr[i].drive_velocity, arr[i].drive_max_torque = de, dv, dm
+        return arr
 
     # -- views
     @property
@@ -292,6 +313,78 @@ def batched_worlds(seed=0, n_worlds=4096, lo=8, hi=1024, spacing=1.5):
             ix, iz, iy = i % side, (i // side) % side, i // (side * side)
             p = [ix * spacing - off, 1.0 + iy * spacing, iz * spacing - off]
             s.add_body(int(kinds[i]), _mixed_shape(rng, int(kinds[i])), p, quats[i], vel=vels[i], world=w)
+    return s
+
+
+def capsule_chain(s, origin, n=8, radius=0.1, half_height=0.3, axis="x", world=0, material=(0.6, 0.0, 0.0),
+                  drive=None):
+    """n capsules end to end, joined by n-1 revolute joints at their touching ends. The
+    capsules lie along world x (axis="x") or hang along -y (axis="y")."""
+    seg = 2 * (half_height + radius)
+    if axis == "x":
+        q = (0.0, 0.0, -math.sqrt(0.5), math.sqrt(0.5))   # local +y -> world +x
+        step = np.array([seg, 0.0, 0.0])
+    else:
+        q = (0.0, 0.0, 0.0, 1.0)
+        step = np.array([0.0, -seg, 0.0])
+    actors = []
+    for i in range(n):
+        p = np.asarray(origin, float) + i * step
+        actors.append(s.add_body(CAPSULE, [half_height, radius], p, q, world=world, material=material))
+    end = half_height + radius
+    for i in range(n - 1):
+        if axis == "x":
+            s.add_joint(REVOLUTE, actors[i], actors[i + 1], (0, end, 0), (0, -end, 0), drive=drive)
+        else:
+            s.add_joint(REVOLUTE, actors[i], actors[i + 1], (0, -end, 0), (0, end, 0), drive=drive)
+    return actors
+
+
+def ragdoll(s, origin, world=0, material=(0.6, 0.0, 0.0)):
+    """A 15-capsule ragdoll with 14 joints (spherical, universal, revolute)."""
+    o = np.asarray(origin, float)
+    r, hh = 0.09, 0.12
+    parts = {}
+
+    def body(name, p, horizontal=False, rr=r, h=hh):
+        q = (0.0, 0.0, -math.sqrt(0.5), math.sqrt(0.5)) if horizontal else (0.0, 0.0, 0.0, 1.0)
+        parts[name] = s.add_body(CAPSULE, [h, rr], o + np.asarray(p, float), q, world=world, material=material)
+
+    body("pelvis", (0, 1.0, 0)); body("abdomen", (0, 1.3, 0)); body("chest", (0, 1.6, 0))
+    body("neck", (0, 1.86, 0), h=0.03, rr=0.05); body("head", (0, 2.05, 0), rr=0.11, h=0.04)
+    for sgn, side in ((1, "l"), (-1, "r")):
+        body("uarm" + side, (sgn * 0.42, 1.7, 0), horizontal=True)
+        body("farm" + side, (sgn * 0.84, 1.7, 0), horizontal=True)
+        body("hand" + side, (sgn * 1.13, 1.7, 0), horizontal=True, h=0.03, rr=0.06)
+        body("thigh" + side, (sgn * 0.15, 0.62, 0))
+        body("shin" + side, (sgn * 0.15, 0.2, 0))
+    J = s.add_joint
+    J(UNIVERSAL, parts["pelvis"], parts["abdomen"], (0, 0.15, 0), (0, -0.15, 0))
+    J(UNIVERSAL, parts["abdomen"], parts["chest"], (0, 0.15, 0), (0, -0.15, 0))
+    J(SPHERICAL, parts["chest"], parts["neck"], (0, 0.21, 0), (0, -0.05, 0))
+    J(SPHERICAL, parts["neck"], parts["head"], (0, 0.05, 0), (0, -0.14, 0))
+    for sgn, side in ((1, "l"), (-1, "r")):
+        # horizontal capsules: local +y points along world x
+        J(SPHERICAL, parts["chest"], parts["uarm" + side], (sgn * 0.21, 0.1, 0), (0, -sgn * 0.21, 0))
+        J(REVOLUTE, parts["uarm" + side], parts["farm" + side], (0, sgn * 0.21, 0), (0, -sgn * 0.21, 0))
+        J(UNIVERSAL, parts["farm" + side], parts["hand" + side], (0, sgn * 0.21, 0), (0, -sgn * 0.08, 0))
+        J(SPHERICAL, parts["pelvis"], parts["thigh" + side], (sgn * 0.15, -0.1, 0), (0, 0.21, 0))
+        J(REVOLUTE, parts["thigh" + side], parts["shin" + side], (0, -0.21, 0), (0, 0.21, 0))
+    return parts
+
+
+def joint_scene(seed=0, chains=6, ragdolls=2, ground=True):
+    """Capsule chains and ragdolls dropped on a ground box, for joint parity tests."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    if ground:
+        s.add_static(BOX, [30, 0.5, 30], [0, -0.5, 0])
+    for c in range(chains):
+        horizontal = c % 2 == 0
+        top = rng.uniform(1.0, 4.0) if horizontal else rng.uniform(7.0, 9.0)   # a hanging chain is 6.4 m long
+        capsule_chain(s, [rng.uniform(-6, 3), top, rng.uniform(-6, 6)], n=8, axis="x" if horizontal else "y")
+    for r in range(ragdolls):
+        ragdoll(s, [rng.uniform(-5, 5), rng.uniform(0.5, 2.0), rng.uniform(-5, 5)])
     return s
 
 

@@ -117,6 +117,12 @@ def collision(lib, prefix, t0, p0, pos0, q0, t1, p1, pos1, q1):
     return hit, out
 
 
+class OrcJoint(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("actor0", ctypes.c_int32), ("actor1", ctypes.c_int32),
+                ("anchor0", ctypes.c_float * 7), ("anchor1", ctypes.c_float * 7), ("drive_enabled", ctypes.c_int32),
+                ("drive_velocity", ctypes.c_float), ("drive_max_torque", ctypes.c_float)]
+
+
 class OracleWorld:
     """The oracle's restated Scene::simulate over a scene descriptor (see physecs_amd.scenes)."""
 
@@ -129,6 +135,14 @@ class OracleWorld:
         self.lib.orc_world_set_params(self.h, substeps, iterations, gravity, threads)
         for e0, e1 in getattr(scene, "non_colliding", []):
             self.lib.orc_world_set_non_colliding(self.h, e0, e1)
+        if getattr(scene, "joints", None):
+            arr = scene.joint_array(OrcJoint)
+            self.lib.orc_world_add_joints.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcJoint)]
+            self.lib.orc_world_add_joints(self.h, len(scene.joints), arr)
+
+    def joint_colors(self):
+        self.lib.orc_world_joint_color.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        return [self.lib.orc_world_joint_color(self.h, i) for i in range(len(self.scene.joints))]
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -171,3 +185,27 @@ class OracleWorld:
         if n:
             self.lib.orc_world_get_manifolds(self.h, iptr(keys), iptr(ints), fptr(f))
         return keys, ints, f
+
+
+def _row_args(lib):
+    lib.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, F, F, F, F, ctypes.c_float, ctypes.c_int, F]
+
+
+def row_solve(which, flags, b0, b1, inp, masses, vel, pv, h, iters):
+    """One joint row: which = 'ref' (reference Constraint1D), 'scalar' or 'simd' (oracle)."""
+    vel = np.array(vel, np.float32)
+    pv = np.array(pv, np.float32)
+    out = np.zeros(2, np.float32)
+    inp = np.ascontiguousarray(inp, np.float32)
+    masses = np.ascontiguousarray(masses, np.float32)
+    if which == "ref":
+        fn = ref().ref_row_scalar
+        fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, F, F, F, F, ctypes.c_float, ctypes.c_int, F]
+        fn(flags, b0, b1, fptr(inp), fptr(masses), fptr(vel), fptr(pv), np.float32(h), iters, fptr(out))
+    else:
+        fn = orc().orc_row
+        fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, F, F, F, F, ctypes.c_float,
+                       ctypes.c_int, F]
+        fn(1 if which == "simd" else 0, flags, b0, b1, fptr(inp), fptr(masses), fptr(vel), fptr(pv), np.float32(h),
+           iters, fptr(out))
+    return vel, pv, out

@@ -76,3 +76,45 @@ def test_mass_props_match_reference_formulas():
         ref.ref_mass_props(t, ol.fptr(pr), ol.fptr(z), ol.fptr(q), np.float32(mass), ol.fptr(out))
         assert abs(out[0] - im) <= 1e-6 * abs(im)
         np.testing.assert_allclose(out[4:], invI, rtol=2e-5)
+
+
+def test_joint_colours_follow_add_joint():
+    """Scene::addJoint colouring (src/Physecs.cpp:690-710): lowest colour free on both
+    entities, static bodies included, colour 8 once eight are taken."""
+    import oracle_lib as ol
+    from physecs_amd import scenes
+    sc = scenes.Scene()
+    ground = sc.add_static(scenes.BOX, [5, 0.5, 5], [0, -0.5, 0])
+    bodies = [sc.add_body(scenes.SPHERE, [0.2], [i, 1, 0]) for i in range(12)]
+    for b in bodies[:10]:
+        sc.add_joint(scenes.SPHERICAL, ground, b, (0, 0.5, 0), (0, 0, 0))
+    sc.add_joint(scenes.SPHERICAL, bodies[0], bodies[1], (0, 0, 0), (0, 0, 0))
+    o = ol.OracleWorld(sc)
+    assert o.joint_colors() == [0, 1, 2, 3, 4, 5, 6, 7, 8, 8, 2]
+
+
+def test_oracle_joint_pendulum_closed_form():
+    """A point mass on a spherical joint to a static anchor keeps its distance to the anchor
+    (within the solver's position error) and swings with the closed-form small-angle period."""
+    import oracle_lib as ol
+    from physecs_amd import scenes
+    sc = scenes.Scene()
+    anchor = sc.add_static(scenes.SPHERE, [0.05], [0, 10, 0])
+    L = 2.0
+    theta0 = 0.1
+    b = sc.add_body(scenes.SPHERE, [0.05], [L * np.sin(theta0), 10 - L * np.cos(theta0), 0])
+    sc.col_enable[0] = 0
+    sc.col_enable[1] = 0
+    sc._frozen = None
+    sc.add_joint(scenes.SPHERICAL, anchor, b, (0, 0, 0), (-L * np.sin(theta0), L * np.cos(theta0), 0))
+    o = ol.OracleWorld(sc, substeps=8, iterations=2)
+    xs = []
+    for _ in range(600):
+        o.step(1 / 120)
+        p = o.state()[0][1]
+        xs.append(p[0])
+        assert abs(np.linalg.norm(p - np.array([0, 10, 0])) - L) < 5e-3
+    xs = np.array(xs)
+    crossings = np.where(np.diff(np.sign(xs)) != 0)[0]
+    period = 2 * np.mean(np.diff(crossings)) / 120
+    assert abs(period - 2 * np.pi * np.sqrt(L / 9.81)) < 0.05

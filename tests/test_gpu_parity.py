@@ -107,3 +107,36 @@ def test_cluster_lockstep_with_restitution_and_soft(api):
             mat[i] = np.array([0.6, 20.0, 0.7], np.float32)     # soft: restitution field is a frequency
     sc._frozen = None
     lockstep(sc, 60)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_joints_lockstep_bitexact(api, seed):
+    """Revolute chains and ragdolls (spherical / universal / revolute rows, SIMD colours)."""
+    log(f"[lockstep] joint scene seed={seed}")
+    lockstep(scenes.joint_scene(seed, chains=6, ragdolls=3), 120)
+
+
+def star_scene(spokes=11):
+    """One hub body jointed to `spokes` bodies: colours 0-7 fill up and the rest go to the
+    scalar overflow colour, whose joints share the hub."""
+    sc = scenes.Scene()
+    sc.add_static(scenes.BOX, [20, 0.5, 20], [0, -0.5, 0])
+    hub = sc.add_body(scenes.SPHERE, [0.3], [0, 3.0, 0])
+    for i in range(spokes):
+        ang = 2 * np.pi * i / spokes
+        d = np.array([np.cos(ang), 0.0, np.sin(ang)])
+        b = sc.add_body(scenes.BOX, [0.15, 0.15, 0.15], [0, 3.0, 0] + 1.2 * d)
+        jt = [scenes.SPHERICAL, scenes.UNIVERSAL, scenes.REVOLUTE, scenes.FIXED][i % 4]
+        drive = (1.5, 50.0) if jt == scenes.REVOLUTE and i % 8 == 2 else None
+        sc.add_joint(jt, hub, b, tuple(0.3 * d), tuple(-1.05 * d), drive=drive)
+    return sc
+
+
+def test_joint_overflow_colour_lockstep(api):
+    """A hub with 11 joints: 8 SIMD colours + the sequential overflow colour, fixed joints and
+    a revolute drive (ANGULAR | LIMITED rows)."""
+    log("[lockstep] joint star (overflow colour)")
+    sc = star_scene(11)
+    o = ol.OracleWorld(sc, substeps=4)
+    assert max(o.joint_colors()) == 8
+    lockstep(sc, 90)

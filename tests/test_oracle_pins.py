@@ -170,3 +170,30 @@ def test_integrators_bitexact():
         ref.ref_integrate_position(ol.fptr(pos), ol.fptr(q), ol.fptr(v), ol.fptr(w), ol.fptr(pv), ol.fptr(pw), cnt,
                                    ol.fptr(com), h, ol.fptr(b))
         assert np.array_equal(bits(a), bits(b))
+
+
+@pytest.mark.parametrize("flags", [0, 2, 1, 4, 3, 6])
+def test_joint_row_scalar_bitexact(flags):
+    """Constraint1D<flags>::preSolve/solve (src/Constraint1D.cpp), the scalar overflow rows,
+    including warm start on the first iteration, soft and limited rows."""
+    rng = np.random.default_rng(100 + flags)
+    h = np.float32(1 / 240)
+    for it in range(3000):
+        inp = np.zeros(15, np.float32)
+        inp[0:9] = rng.normal(size=9)
+        inp[9] = rng.normal() * 0.1
+        inp[10] = rng.choice([0.0, rng.normal() * 0.01, rng.normal() * 1e-5])
+        inp[11], inp[12] = -abs(rng.normal()), abs(rng.normal())
+        inp[13], inp[14] = rng.uniform(1, 30), rng.uniform(0.1, 1)
+        masses = np.zeros(20, np.float32)
+        for b in range(2):
+            masses[10 * b] = rng.uniform(0.01, 2)
+            A = rng.normal(size=(3, 3))
+            masses[10 * b + 1: 10 * b + 10] = (A @ A.T * 0.3).astype(np.float32).ravel()
+        vel = rng.normal(size=12).astype(np.float32)
+        pv = np.zeros(14, np.float32)
+        b0 = -1 if it % 4 == 0 else 0
+        a = ol.row_solve("ref", flags, b0, 1, inp, masses, vel, pv, h, 3)
+        b = ol.row_solve("scalar", flags, b0, 1, inp, masses, vel, pv, h, 3)
+        for x, y in zip(a, b):
+            assert np.array_equal(bits(x), bits(y)), (it, flags, x, y)
