@@ -93,6 +93,30 @@ typedef struct pgx_scene_desc {
     const int32_t* col_data;
     int n_worlds;
     const int32_t* actor_world;   /* world of each actor, or NULL */
+    /* Convex meshes (include/Physecs/ConvexMesh.h:7-34). A CONVEX_MESH collider's params
+     * are {scale.x, scale.y, scale.z, mesh index}. Mesh m owns vertices
+     * [convex_vert_start[m], convex_vert_start[m+1]) and faces
+     * [convex_face_start[m], convex_face_start[m+1]); face f lists the mesh-local vertex
+     * indices face_idx[face_idx_start[f] .. face_idx_start[f+1]), clockwise seen from
+     * outside (the winding the reference's clipping keeps, src/Clipping.cpp:9-11), with
+     * its outward unit normal and centroid. */
+    int n_convex;
+    const int32_t* convex_vert_start;  /* n_convex + 1 */
+    const float* convex_verts;         /* 3 x vertices */
+    const int32_t* convex_face_start;  /* n_convex + 1 */
+    const int32_t* face_idx_start;     /* faces + 1 */
+    const int32_t* face_idx;
+    const float* face_normal;          /* 3 x faces */
+    const float* face_centroid;        /* 3 x faces */
+    /* Static triangle meshes (include/Physecs/TriangleMesh.h). A TRIANGLE_MESH collider's
+     * params are {0, 0, 0, mesh index}; mesh m owns vertices [tri_vert_start[m],
+     * tri_vert_start[m+1]) and triangles [tri_start[m], tri_start[m+1]) given as three
+     * mesh-local vertex indices each, in the order TriangleMesh's constructor takes them. */
+    int n_trimesh;
+    const int32_t* tri_vert_start;     /* n_trimesh + 1 */
+    const float* tri_verts;            /* 3 x vertices */
+    const int32_t* tri_start;          /* n_trimesh + 1 */
+    const int32_t* tri_idx;            /* 3 x triangles */
 } pgx_scene_desc;
 
 /* Built-in joint types (include/Physecs/Joints/*.h) */

@@ -30,6 +30,16 @@ Bounds getBounds(V3 pos, Q ori, const Geom& g) {
             V3 w = mulmv(au, he);
             return {pos - w, pos + w};
         }
+        case CONVEX_MESH: {   // src/BoundsUtil.cpp:61-69, over the padded vertex buffer
+            V3 mn = v3s(std::numeric_limits<float>::max()), mx = v3s(std::numeric_limits<float>::lowest());
+            V3 sc = v3(g.p[0], g.p[1], g.p[2]);
+            for (const V3& v : g.mesh->verts) {
+                V3 t = pos + qrot(ori, sc * v);
+                mn = vmin(mn, t);
+                mx = vmax(mx, t);
+            }
+            return {mn, mx};
+        }
     }
     return {v3s(0), v3s(0)};
 }
@@ -340,7 +350,7 @@ bool clipLine(V2& p0, V2& p1, const V2* clip, int clipSize) {
 
 // ---------------------------------------------------------------- ContactsUtil
 // src/ContactsUtil.cpp:9-107 (epsilon 1e-4f, :6)
-static void generateContactsPolygonBoxFace(V3 boxCenter, const M3& boxBasis, int boxAxis, int boxAxisSign,
+void generateContactsPolygonBoxFace(V3 boxCenter, const M3& boxBasis, int boxAxis, int boxAxisSign,
                                            V3 he, V3 incPlaneOrig, V3 incPlaneNormal,
                                            const std::vector<V2>& polygon, int clipX, int clipY,
                                            ContactPoint* cps, int& numPoints) {
@@ -700,7 +710,8 @@ static void flipContacts(Manifold& m) {
     for (int i = 0; i < m.numPoints; ++i) m.points[i] = {m.points[i].p1, m.points[i].p0};
 }
 
-// src/Collision.cpp:895-1016 (primitive shapes)
+// src/Collision.cpp:895-1016 (primitive and convex-mesh shapes; triangle meshes are
+// dispatched by the caller)
 bool collision(V3 pos0, Q or0, const Geom& g0, V3 pos1, Q or1, const Geom& g1, std::vector<Manifold>& out) {
     out.push_back(Manifold());
     Manifold& m = out.back();
@@ -712,6 +723,7 @@ bool collision(V3 pos0, Q or0, const Geom& g0, V3 pos1, Q or1, const Geom& g1, s
                 case SPHERE: return sphereSphere(pos0, g0.p[0], pos1, g1.p[0], m);
                 case CAPSULE: return sphereCapsule(pos0, g0.p[0], pos1, or1, g1.p[0], g1.p[1], m);
                 case BOX: return sphereBox(pos0, g0.p[0], pos1, or1, he(g1), m);
+                case CONVEX_MESH: return collisionSphereConvex(pos0, g0.p[0], pos1, or1, g1.mesh, he(g1), m);
             }
             break;
         case CAPSULE:
@@ -722,6 +734,8 @@ bool collision(V3 pos0, Q or0, const Geom& g0, V3 pos1, Q or1, const Geom& g1, s
                     return hit;
                 case CAPSULE: return capsuleCapsule(pos0, or0, g0.p[0], g0.p[1], pos1, or1, g1.p[0], g1.p[1], m);
                 case BOX: return capsuleBox(pos0, or0, g0.p[0], g0.p[1], pos1, or1, he(g1), m);
+                case CONVEX_MESH:
+                    return collisionCapsuleConvex(pos0, or0, g0.p[0], g0.p[1], pos1, or1, g1.mesh, he(g1), m);
             }
             break;
         case BOX:
@@ -735,6 +749,25 @@ bool collision(V3 pos0, Q or0, const Geom& g0, V3 pos1, Q or1, const Geom& g1, s
                     flipContacts(m);
                     return hit;
                 case BOX: return boxBox(pos0, or0, he(g0), pos1, or1, he(g1), m);
+                case CONVEX_MESH: return collisionBoxConvex(pos0, or0, he(g0), pos1, or1, g1.mesh, he(g1), m);
+            }
+            break;
+        case CONVEX_MESH:
+            switch (g1.type) {
+                case SPHERE:
+                    hit = collisionSphereConvex(pos1, g1.p[0], pos0, or0, g0.mesh, he(g0), m);
+                    flipContacts(m);
+                    return hit;
+                case CAPSULE:
+                    hit = collisionCapsuleConvex(pos1, or1, g1.p[0], g1.p[1], pos0, or0, g0.mesh, he(g0), m);
+                    flipContacts(m);
+                    return hit;
+                case BOX:
+                    hit = collisionBoxConvex(pos1, or1, he(g1), pos0, or0, g0.mesh, he(g0), m);
+                    flipContacts(m);
+                    return hit;
+                case CONVEX_MESH:
+                    return collisionConvexConvex(pos0, or0, g0.mesh, he(g0), pos1, or1, g1.mesh, he(g1), m);
             }
             break;
     }

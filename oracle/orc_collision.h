@@ -9,8 +9,17 @@ namespace orc {
 
 enum GeomType { SPHERE = 0, CAPSULE = 1, BOX = 2, CONVEX_MESH = 3, TRIANGLE_MESH = 4 };
 
-// params: sphere {r}, capsule {halfHeight, r}, box {hx, hy, hz}
-struct Geom { int type; float p[4]; };
+// include/Physecs/ConvexMesh.h:7-34. Vertices are padded to a multiple of 4 with copies
+// of the last vertex (src/ConvexMesh.cpp:3-11); `count` is the unpadded number.
+struct ConvexFace { std::vector<int> idx; V3 normal, centroid; };
+struct ConvexMesh {
+    int count = 0;
+    std::vector<V3> verts;
+    std::vector<ConvexFace> faces;
+};
+
+// params: sphere {r}, capsule {halfHeight, r}, box {hx, hy, hz}, convex {sx, sy, sz} (scale)
+struct Geom { int type; float p[4]; const ConvexMesh* mesh = nullptr; };
 
 struct ContactPoint { V3 p0, p1; };
 struct Manifold {                          // src/ContactManifold.h:11-16
@@ -35,5 +44,18 @@ void closestPointsBetweenSegments(V3 o0, V3 d0, float mn0, float mx0, V3 o1, V3 
 float sqrDistSegmentAABB(V3 p, V3 dir, float mn, float mx, V3 he, float& t, V3& q);
 void suthHodgClip(std::vector<V2>& polygon, const std::vector<V2>& clip);
 bool clipLine(V2& p0, V2& p1, const V2* clip, int clipSize);
+void generateContactsPolygonBoxFace(V3 boxCenter, const M3& boxBasis, int boxAxis, int boxAxisSign, V3 he,
+                                    V3 incPlaneOrig, V3 incPlaneNormal, const std::vector<V2>& polygon,
+                                    int clipX, int clipY, ContactPoint* cps, int& numPoints);
+
+// convex-mesh pairs (orc_convex.cpp): src/Collision.cpp:352-426,488-499,694-887
+bool collisionSphereConvex(V3 pos0, float r0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1, Manifold& res);
+bool collisionCapsuleConvex(V3 pos0, Q or0, float hh0, float r0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1,
+                            Manifold& res);
+bool collisionBoxConvex(V3 pos0, Q or0, V3 he0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1, Manifold& res);
+bool collisionConvexConvex(V3 pos0, Q or0, const ConvexMesh* m0, V3 s0, V3 pos1, Q or1, const ConvexMesh* m1,
+                           V3 s1, Manifold& res);
+// counters of EPA runs that needed the whole 100 iterations / hit a zero face normal
+extern thread_local long epaExhausted, epaZeroNormal;
 
 }  // namespace orc

@@ -170,6 +170,30 @@ inline M3 inverse(const M3& m) {
     r[2][2] = +(m[0][0] * m[1][1] - m[1][0] * m[0][1]) * ood;
     return r;
 }
+// double vectors (glm::dvec3), used by GJK/EPA: same association order as the float ones
+struct D3 {
+    double x, y, z;
+    double& operator[](int i) { return (&x)[i]; }
+    double operator[](int i) const { return (&x)[i]; }
+};
+inline D3 d3(V3 v) { return {v.x, v.y, v.z}; }
+inline D3 d3(double x, double y, double z) { return {x, y, z}; }
+inline V3 f3(D3 d) { return {(float)d.x, (float)d.y, (float)d.z}; }   // glm::vec3(dvec3): round to nearest
+inline D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 operator-(D3 a) { return {-a.x, -a.y, -a.z}; }
+inline D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline D3 operator*(double s, D3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline D3 operator+(D3 a, double s) { return {a.x + s, a.y + s, a.z + s}; }
+inline D3 operator/(D3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+inline double ddot(D3 a, D3 b) { D3 t = {a.x * b.x, a.y * b.y, a.z * b.z}; return t.x + t.y + t.z; }
+inline D3 dcross(D3 x, D3 y) {
+    return {x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y};
+}
+inline double dlength(D3 v) { return std::sqrt(ddot(v, v)); }
+inline D3 dnormalize(D3 v) { return v * (1.0 / std::sqrt(ddot(v, v))); }
+inline double dsign(double x) { return (double)(0.0 < x) - (double)(x < 0.0); }
+
 inline M3 matrixCross3(V3 x) {
     M3 r = mzero();
     r[0][1] = x.z; r[1][0] = -x.z;

@@ -95,6 +95,7 @@ struct JointRows {                     // one joint's rows, in solve order
 }  // namespace
 
 struct orc_world {
+    std::vector<ConvexMesh> convex;
     std::vector<Actor> actors;
     std::vector<Dyn> dyn;
     std::vector<Collider> cols;
@@ -124,7 +125,35 @@ struct orc_scene {
     const int32_t* col_actor; const int32_t* col_index; const float* col_pos; const float* col_quat;
     const int32_t* col_type; const float* col_params; const float* col_material;
     const uint8_t* col_trigger; const uint8_t* col_enable; const int32_t* col_data;
+    int n_worlds; const int32_t* actor_world;       // unused by the oracle (one world)
+    // convex meshes; a CONVEX_MESH collider's params are {scale xyz, mesh index}
+    int n_convex;
+    const int32_t* convex_vert_start; const float* convex_verts;
+    const int32_t* convex_face_start; const int32_t* face_idx_start; const int32_t* face_idx;
+    const float* face_normal; const float* face_centroid;
+    // triangle meshes; a TRIANGLE_MESH collider's params are {0, 0, 0, mesh index}
+    int n_trimesh;
+    const int32_t* tri_vert_start; const float* tri_verts; const int32_t* tri_start; const int32_t* tri_idx;
 };
+
+// ConvexMesh(vertices, faces) (include/Physecs/ConvexMesh.h:29-34, src/ConvexMesh.cpp:3-11):
+// the vertex buffer is padded to a multiple of 4 with copies of the last vertex.
+static ConvexMesh makeConvex(int nv, const float* verts, int nf, const int32_t* fstart, const int32_t* fidx,
+                             const float* normals, const float* centroids) {
+    ConvexMesh m;
+    m.count = nv;
+    for (int i = 0; i < nv; ++i) m.verts.push_back(v3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]));
+    int padded = (nv + 3) / 4 * 4;
+    V3 last = m.verts.back();
+    m.verts.resize(padded, last);
+    m.faces.resize(nf);
+    for (int i = 0; i < nf; ++i) {
+        m.faces[i].idx.assign(fidx + fstart[i], fidx + fstart[i + 1]);
+        m.faces[i].normal = v3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
+        m.faces[i].centroid = v3(centroids[3 * i], centroids[3 * i + 1], centroids[3 * i + 2]);
+    }
+    return m;
+}
 
 orc_world* orc_world_create(const orc_scene* s) {
     auto* w = new orc_world();
@@ -148,6 +177,12 @@ orc_world* orc_world_create(const orc_scene* s) {
         for (int c = 0; c < 3; ++c)
             for (int r = 0; r < 3; ++r) d.invI[c][r] = s->inv_inertia[9 * i + 3 * c + r];
     }
+    for (int m = 0; m < s->n_convex; ++m) {
+        int v0 = s->convex_vert_start[m], f0 = s->convex_face_start[m];
+        w->convex.push_back(makeConvex(s->convex_vert_start[m + 1] - v0, s->convex_verts + 3 * v0,
+                                       s->convex_face_start[m + 1] - f0, s->face_idx_start + f0, s->face_idx,
+                                       s->face_normal + 3 * f0, s->face_centroid + 3 * f0));
+    }
     w->cols.resize(s->n_colliders);
     w->bp.resize(s->n_colliders);
     for (int i = 0; i < s->n_colliders; ++i) {
@@ -158,6 +193,7 @@ orc_world* orc_world_create(const orc_scene* s) {
         c.lquat = {s->col_quat[4 * i], s->col_quat[4 * i + 1], s->col_quat[4 * i + 2], s->col_quat[4 * i + 3]};
         c.geom.type = s->col_type[i];
         for (int k = 0; k < 4; ++k) c.geom.p[k] = s->col_params[4 * i + k];
+        if (c.geom.type == CONVEX_MESH) c.geom.mesh = &w->convex.at((size_t)c.geom.p[3]);
         c.friction = s->col_material[3 * i];
         c.restitution = s->col_material[3 * i + 1];
         c.damping = s->col_material[3 * i + 2];
@@ -948,12 +984,29 @@ int orc_integrate_position(const float* pos, const float* quat, const float* v, 
     return 0;
 }
 
-// single narrowphase call on primitive shapes (for pinning against the reference).
+// convex meshes for the single-call pin entry points below (params[3] = index)
+static std::vector<ConvexMesh*> g_convex;
+
+int orc_set_convex_mesh(int n_verts, const float* verts, int n_faces, const int32_t* face_start,
+                        const int32_t* face_idx, const float* normals, const float* centroids) {
+    g_convex.push_back(new ConvexMesh(makeConvex(n_verts, verts, n_faces, face_start, face_idx, normals, centroids)));
+    return (int)g_convex.size() - 1;
+}
+
+static Geom pinGeom(int type, const float* prm) {
+    Geom g{type, {prm[0], prm[1], prm[2], prm[3]}};
+    if (type == CONVEX_MESH) g.mesh = g_convex.at((size_t)prm[3]);
+    return g;
+}
+
+long orc_epa_exhausted() { return epaExhausted; }
+
+// single narrowphase call (for pinning against the reference).
 // out: numPoints, normal(3), 4 x (p0, p1); returns the collision() bool.
 int orc_collision(int type0, const float* prm0, const float* pos0, const float* q0,
                   int type1, const float* prm1, const float* pos1, const float* q1, float* out) {
-    Geom g0{type0, {prm0[0], prm0[1], prm0[2], prm0[3]}};
-    Geom g1{type1, {prm1[0], prm1[1], prm1[2], prm1[3]}};
+    Geom g0 = pinGeom(type0, prm0);
+    Geom g1 = pinGeom(type1, prm1);
     std::vector<Manifold> buf;
     bool hit = collision(v3(pos0[0], pos0[1], pos0[2]), {q0[0], q0[1], q0[2], q0[3]}, g0,
                          v3(pos1[0], pos1[1], pos1[2]), {q1[0], q1[1], q1[2], q1[3]}, g1, buf);
@@ -970,7 +1023,7 @@ int orc_collision(int type0, const float* prm0, const float* pos0, const float* 
 }
 
 int orc_bounds(int type, const float* prm, const float* pos, const float* q, float* out) {
-    Geom g{type, {prm[0], prm[1], prm[2], prm[3]}};
+    Geom g = pinGeom(type, prm);
     Bounds b = getBounds(v3(pos[0], pos[1], pos[2]), {q[0], q[1], q[2], q[3]}, g);
     out[0] = b.min.x; out[1] = b.min.y; out[2] = b.min.z;
     out[3] = b.max.x; out[4] = b.max.y; out[5] = b.max.z;

@@ -9,6 +9,7 @@
 //   ContactConstraints::preSolve / solve   src/ContactConstraints.cpp:4-124
 //   physecs::setMassProps         src/MassUtil.cpp:185-192
 //   solve33                       src/MathUtil.h:10-21
+//   ConvexMesh(vertices, faces)   include/Physecs/ConvexMesh.h:29-34, src/ConvexMesh.cpp
 // The two integrate functions below are GLM-written restatements of src/Physecs.cpp
 // (that file does not compile here): they pin the oracle's GLM-free arithmetic against
 // GLM's own operator order.
@@ -27,8 +28,12 @@
 #include "SolverData.h"
 #include "Constraint1D.h"
 #include "Constraint1D.cpp"   // the row templates, compiled from the reference source
+#include "ConvexMesh.cpp"     // ConvexMeshVertices<4>'s constructor template (not instantiated there)
 
 using namespace physecs;
+
+// convex meshes registered by ref_set_convex_mesh (include/Physecs/ConvexMesh.h:7-34)
+static std::vector<ConvexMesh*> g_convex;
 
 static Geometry makeGeom(int type, const float* p) {
     Geometry g;
@@ -38,6 +43,10 @@ static Geometry makeGeom(int type, const float* p) {
         case SPHERE: g.sphere.radius = p[0]; break;
         case CAPSULE: g.capsule.halfHeight = p[0]; g.capsule.radius = p[1]; break;
         case BOX: g.box.halfExtents = glm::vec3(p[0], p[1], p[2]); break;
+        case CONVEX_MESH:   // params: scale xyz, mesh index
+            g.convex.mesh = g_convex.at((size_t)p[3]);
+            g.convex.scale = glm::vec3(p[0], p[1], p[2]);
+            break;
         default: break;
     }
     return g;
@@ -104,6 +113,22 @@ int ref_collision(int type0, const float* prm0, const float* pos0, const float* 
         o[3] = m.points[k].position1.x; o[4] = m.points[k].position1.y; o[5] = m.points[k].position1.z;
     }
     return hit ? 1 : 0;
+}
+
+// Registers a ConvexMesh built by the reference's own constructor; returns its index.
+// faces: face_start[n_faces+1] offsets into face_idx; normals/centroids xyz per face.
+int ref_set_convex_mesh(int n_verts, const float* verts, int n_faces, const int* face_start, const int* face_idx,
+                        const float* normals, const float* centroids) {
+    std::vector<glm::vec3> v(n_verts);
+    for (int i = 0; i < n_verts; ++i) v[i] = V(verts + 3 * i);
+    std::vector<ConvexMeshFace> f(n_faces);
+    for (int i = 0; i < n_faces; ++i) {
+        f[i].indices.assign(face_idx + face_start[i], face_idx + face_start[i + 1]);
+        f[i].normal = V(normals + 3 * i);
+        f[i].centroid = V(centroids + 3 * i);
+    }
+    g_convex.push_back(new ConvexMesh(std::move(v), std::move(f)));
+    return (int)g_convex.size() - 1;
 }
 
 int ref_sqr_dist_segment_aabb(const float* p, const float* dir, float mn, float mx, const float* he, float* out) {

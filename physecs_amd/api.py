@@ -43,6 +43,9 @@ class PgxSceneDesc(ctypes.Structure):
         ("n_colliders", ctypes.c_int), ("col_actor", I32), ("col_index", I32), ("col_pos", F), ("col_quat", F),
         ("col_type", I32), ("col_params", F), ("col_material", F), ("col_trigger", U8), ("col_enable", U8),
         ("col_data", I32), ("n_worlds", ctypes.c_int), ("actor_world", I32),
+        ("n_convex", ctypes.c_int), ("convex_vert_start", I32), ("convex_verts", F), ("convex_face_start", I32),
+        ("face_idx_start", I32), ("face_idx", I32), ("face_normal", F), ("face_centroid", F),
+        ("n_trimesh", ctypes.c_int), ("tri_vert_start", I32), ("tri_verts", F), ("tri_start", I32), ("tri_idx", I32),
     ]
 
 

@@ -67,62 +67,186 @@ def random_quats(rng, n):
     return q.astype(np.float32)
 
 
+def convex_hull(points):
+    """ConvexMesh data for the convex hull of `points` (scipy's Qhull): vertices (the hull's
+    points only), faces as polygons of coplanar hull facets ordered clockwise seen from
+    outside, outward unit normals and face centroids (vertex means)."""
+    from scipy.spatial import ConvexHull
+    pts = np.asarray(points, np.float64)
+    h = ConvexHull(pts)
+    used = np.unique(h.simplices)
+    remap = {int(v): i for i, v in enumerate(used)}
+    verts = pts[used]
+    inner = verts.mean(axis=0)
+    groups = {}
+    for simplex, eq in zip(h.simplices, h.equations):
+        key = tuple(np.round(eq, 6))
+        groups.setdefault(key, set()).update(int(v) for v in simplex)
+    faces, normals, centroids = [], [], []
+    for key, vs in groups.items():
+        n = np.asarray(key[:3], np.float64)
+        n /= np.linalg.norm(n)
+        idx = [remap[v] for v in vs]
+        c = verts[idx].mean(axis=0)
+        # order around the face; clockwise seen from outside = counter-clockwise about -n
+        u = verts[idx[0]] - c
+        u /= np.linalg.norm(u)
+        w = np.cross(n, u)
+        ang = [math.atan2(np.dot(verts[i] - c, w), np.dot(verts[i] - c, u)) for i in idx]
+        order = [idx[k] for k in np.argsort(ang)[::-1]]
+        if np.dot(n, c - inner) < 0:
+            n = -n
+        faces.append(order)
+        normals.append(n)
+        centroids.append(c)
+    return verts.astype(np.float32), faces, np.asarray(normals, np.float32), np.asarray(centroids, np.float32)
+
+
+def ellipsoid_hull(rng, n_points=24, lo=0.3, hi=0.8):
+    """Hull of n_points uniform on an ellipsoid with semi-axes U(lo, hi)^3."""
+    axes = rng.uniform(lo, hi, 3)
+    d = rng.normal(size=(n_points, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return convex_hull(d * axes)
+
+
+def hull_volume(verts, faces):
+    c = verts.mean(axis=0)
+    vol = 0.0
+    for f in faces:
+        for i in range(1, len(f) - 1):
+            vol += abs(np.dot(np.cross(verts[f[0]] - c, verts[f[i]] - c), verts[f[i + 1]] - c)) / 6.0
+    return vol
+
+
+def _inertia_tetrahedron(m, v):
+    """src/MassUtil.cpp:31-72 (about the origin)."""
+    d = np.zeros(3)
+    for i in range(4):
+        for j in range(i + 1):
+            d += v[j] * v[i]
+    a, b, c = (d[1] + d[2]) / 10, (d[0] + d[2]) / 10, (d[0] + d[1]) / 10
+    ap = bp = cp = 0.0
+    for i in range(4):
+        for j in range(4):
+            k = 2 if i == j else 1
+            ap += k * v[i][1] * v[j][2]
+            bp += k * v[i][0] * v[j][2]
+            cp += k * v[i][0] * v[j][1]
+    ap, bp, cp = ap / 20, bp / 20, cp / 20
+    I = np.array([[a, -bp, -cp], [-bp, b, -ap], [-cp, -ap, c]])
+    return m * I
+
+
+def convex_mass_props(verts, faces, scale, mass):
+    """(com, invInertia column-major) for one convex collider at the body origin, as
+    src/MassUtil.cpp:96-123,156-177 computes them: the fan centre is the mean over the
+    vertex buffer padded to a multiple of 4 with the last vertex, divided by the
+    unpadded count."""
+    sv = np.asarray(verts, np.float64) * np.asarray(scale, np.float64)
+    n = len(sv)
+    padded = np.concatenate([sv, np.repeat(sv[-1:], (-n) % 4, axis=0)])
+    center = padded.sum(axis=0) / n
+    com = np.zeros(3)
+    total = 0.0
+    tets = []
+    for f in faces:
+        v0 = sv[f[0]]
+        for i in range(1, len(f) - 1):
+            v1, v2 = sv[f[i]], sv[f[i + 1]]
+            vol = abs(np.dot(np.cross(v0 - center, v1 - center), v2 - center)) / 6.0
+            com += (center + v0 + v1 + v2) / 4.0 * vol
+            total += vol
+            tets.append((vol, v0, v1, v2))
+    com /= total
+    I = np.zeros((3, 3))
+    d = -com
+    for vol, v0, v1, v2 in tets:
+        m = mass * vol / total
+        I += _inertia_tetrahedron(m, [center + d, v0 + d, v1 + d, v2 + d])
+    inv = np.linalg.inv(I)
+    return com.astype(np.float32), inv.T.astype(np.float32).ravel()
+
+
+_FIELDS = {
+    # name: (dtype, per-row shape)
+    "entity": (np.uint32, ()), "pos": (np.float32, (3,)), "quat": (np.float32, (4,)),
+    "dyn_of_actor": (np.int32, ()), "world": (np.int32, ()),
+    "actor_of_dyn": (np.int32, ()), "kinematic": (np.uint8, ()), "vel": (np.float32, (3,)),
+    "angvel": (np.float32, (3,)), "inv_mass": (np.float32, ()), "com": (np.float32, (3,)),
+    "inv_inertia": (np.float32, (9,)),
+    "col_actor": (np.int32, ()), "col_index": (np.int32, ()), "col_pos": (np.float32, (3,)),
+    "col_quat": (np.float32, (4,)), "col_type": (np.int32, ()), "col_params": (np.float32, (4,)),
+    "col_material": (np.float32, (3,)), "col_trigger": (np.uint8, ()), "col_enable": (np.uint8, ()),
+    "col_data": (np.int32, ()),
+}
+
+
 class Scene:
     """Scene descriptor builder. Actors, dynamic storage and colliders are appended in
-    creation order; entity ids are the creation index unless given."""
+    creation order; entity ids are the creation index unless given. Fields are kept as
+    lists of numpy chunks (one row per single append, many rows per bulk append)."""
 
     def __init__(self):
-        self.entity, self.pos, self.quat, self.dyn_of_actor, self.world = [], [], [], [], []
-        self.actor_of_dyn, self.kinematic, self.vel, self.angvel = [], [], [], []
-        self.inv_mass, self.com, self.inv_inertia = [], [], []
-        self.col_actor, self.col_index, self.col_pos, self.col_quat = [], [], [], []
-        self.col_type, self.col_params, self.col_material = [], [], []
-        self.col_trigger, self.col_enable, self.col_data = [], [], []
+        self._chunks = {k: [] for k in _FIELDS}
+        self._n = {"actor": 0, "dyn": 0, "col": 0}
+        self._dyn_patch = {}          # actor -> dyn index, for actors added before their dynamic part
         self.non_colliding = []
         self.joints = []        # (type, actor0, actor1, anchor0[7], anchor1[7], drive_enabled, drive_v, drive_max)
         self._ncol = {}
         self.n_worlds = 1
         self._frozen = None
+        self.convex = []        # (verts (n,3), faces [list of vertex indices], normals (f,3), centroids (f,3))
+        self.trimesh = []       # (verts (n,3), tris (t,3))
+
+    def _put(self, name, rows):
+        dt, shp = _FIELDS[name]
+        a = np.asarray(rows, dt).reshape((-1,) + shp)
+        self._chunks[name].append(a)
+        self._frozen = None
 
     # -- building
     def add_actor(self, pos, quat=(0, 0, 0, 1), world=0, entity=None):
-        a = len(self.entity)
-        self.entity.append(a if entity is None else entity)
-        self.pos.append(np.asarray(pos, np.float32))
-        self.quat.append(np.asarray(quat, np.float32))
-        self.dyn_of_actor.append(-1)
-        self.world.append(world)
+        a = self._n["actor"]
+        self._n["actor"] += 1
+        self._put("entity", [a if entity is None else entity])
+        self._put("pos", pos)
+        self._put("quat", quat)
+        self._put("dyn_of_actor", [-1])
+        self._put("world", [world])
         self.n_worlds = max(self.n_worlds, world + 1)
         return a
 
     def add_dynamic(self, actor, inv_mass, com, inv_inertia, vel=(0, 0, 0), angvel=(0, 0, 0), kinematic=False):
-        d = len(self.actor_of_dyn)
-        self.dyn_of_actor[actor] = d
-        self.actor_of_dyn.append(actor)
-        self.kinematic.append(1 if kinematic else 0)
-        self.vel.append(np.asarray(vel, np.float32))
-        self.angvel.append(np.asarray(angvel, np.float32))
-        self.inv_mass.append(np.float32(inv_mass))
-        self.com.append(np.asarray(com, np.float32))
-        self.inv_inertia.append(np.asarray(inv_inertia, np.float32).ravel())
+        d = self._n["dyn"]
+        self._n["dyn"] += 1
+        self._dyn_patch[actor] = d
+        self._put("actor_of_dyn", [actor])
+        self._put("kinematic", [1 if kinematic else 0])
+        self._put("vel", vel)
+        self._put("angvel", angvel)
+        self._put("inv_mass", [inv_mass])
+        self._put("com", com)
+        self._put("inv_inertia", np.asarray(inv_inertia, np.float32).ravel())
         return d
 
     def add_collider(self, actor, gtype, params, material=(0.6, 0.0, 0.0), lpos=(0, 0, 0), lquat=(0, 0, 0, 1),
                      trigger=False, enable=True, data=0):
         idx = self._ncol.get(actor, 0)
         self._ncol[actor] = idx + 1
-        self.col_actor.append(actor)
-        self.col_index.append(idx)
-        self.col_pos.append(np.asarray(lpos, np.float32))
-        self.col_quat.append(np.asarray(lquat, np.float32))
-        self.col_type.append(gtype)
+        self._n["col"] += 1
         p = np.zeros(4, np.float32)
         p[: len(params)] = params
-        self.col_params.append(p)
-        self.col_material.append(np.asarray(material, np.float32))
-        self.col_trigger.append(1 if trigger else 0)
-        self.col_enable.append(1 if enable else 0)
-        self.col_data.append(data)
+        self._put("col_actor", [actor])
+        self._put("col_index", [idx])
+        self._put("col_pos", lpos)
+        self._put("col_quat", lquat)
+        self._put("col_type", [gtype])
+        self._put("col_params", p)
+        self._put("col_material", material)
+        self._put("col_trigger", [1 if trigger else 0])
+        self._put("col_enable", [1 if enable else 0])
+        self._put("col_data", [data])
 
     def add_body(self, gtype, params, pos, quat=(0, 0, 0, 1), density=1000.0, vel=(0, 0, 0), angvel=(0, 0, 0),
                  material=(0.6, 0.0, 0.0), world=0):
@@ -136,6 +260,89 @@ class Scene:
     def add_static(self, gtype, params, pos, quat=(0, 0, 0, 1), material=(0.6, 0.0, 0.0), world=0):
         a = self.add_actor(pos, quat, world)
         self.add_collider(a, gtype, params, material)
+        return a
+
+    def add_bodies_bulk(self, gtypes, params, pos, quats, vels, worlds, density=1000.0, material=(0.6, 0.0, 0.0)):
+        """Appends many one-collider dynamic bodies at once (the same fields add_body writes)."""
+        n = len(gtypes)
+        gtypes = np.asarray(gtypes, np.int32)
+        params = np.asarray(params, np.float64).reshape(n, 4)
+        a0, d0 = self._n["actor"], self._n["dyn"]
+        actors = np.arange(a0, a0 + n)
+        self._put("entity", actors.astype(np.uint32))
+        self._put("pos", pos)
+        self._put("quat", quats)
+        self._put("dyn_of_actor", np.arange(d0, d0 + n))
+        self._put("world", worlds)
+        self.n_worlds = max(self.n_worlds, int(np.max(worlds)) + 1 if n else 1)
+        vol = np.where(gtypes == SPHERE, 4.0 / 3.0 * np.pi * params[:, 0] ** 3,
+                       np.where(gtypes == CAPSULE, np.pi * params[:, 1] ** 2 * 2 * params[:, 0]
+                                + 4.0 / 3.0 * np.pi * params[:, 1] ** 3, 8.0 * params[:, 0] * params[:, 1] * params[:, 2]))
+        m = density * vol
+        r = params[:, 0]
+        isph = 2.0 * m * r * r / 5.0
+        cr, ch = params[:, 1], params[:, 0]
+        vc = 2 * cr * cr * np.pi * ch
+        vs = 4 * cr ** 3 * np.pi / 3
+        with np.errstate(invalid="ignore", divide="ignore"):   # rows that are not capsules
+            mc, ms = vc * m / (vc + vs), vs * m / (vc + vs)
+        cxx = mc * (ch * ch / 3 + cr * cr / 4) + ms * (ch * ch + 3 * ch * cr / 4 + 2 * cr * cr / 5)
+        cyy = mc * cr * cr / 2 + ms * 2 * cr * cr / 5
+        x2, y2, z2 = params[:, 0] ** 2, params[:, 1] ** 2, params[:, 2] ** 2
+        bxx, byy, bzz = m / 12 * (y2 + z2), m / 12 * (x2 + z2), m / 12 * (x2 + y2)
+        ixx = np.where(gtypes == SPHERE, isph, np.where(gtypes == CAPSULE, cxx, bxx))
+        iyy = np.where(gtypes == SPHERE, isph, np.where(gtypes == CAPSULE, cyy, byy))
+        izz = np.where(gtypes == SPHERE, isph, np.where(gtypes == CAPSULE, cxx, bzz))
+        invI = np.zeros((n, 9))
+        invI[:, 0], invI[:, 4], invI[:, 8] = 1 / ixx, 1 / iyy, 1 / izz
+        self._put("actor_of_dyn", actors)
+        self._put("kinematic", np.zeros(n))
+        self._put("vel", vels)
+        self._put("angvel", np.zeros((n, 3)))
+        self._put("inv_mass", 1.0 / m)
+        self._put("com", np.zeros((n, 3)))
+        self._put("inv_inertia", invI)
+        self._put("col_actor", actors)
+        self._put("col_index", np.zeros(n))
+        self._put("col_pos", np.zeros((n, 3)))
+        self._put("col_quat", np.tile([0, 0, 0, 1], (n, 1)))
+        self._put("col_type", gtypes)
+        self._put("col_params", params)
+        self._put("col_material", np.tile(material, (n, 1)))
+        self._put("col_trigger", np.zeros(n))
+        self._put("col_enable", np.ones(n))
+        self._put("col_data", np.zeros(n))
+        for a in range(a0, a0 + n):
+            self._ncol[a] = 1
+        self._n["actor"] += n
+        self._n["dyn"] += n
+        self._n["col"] += n
+        return actors
+
+    def add_convex_mesh(self, verts, faces, normals, centroids):
+        """Registers a ConvexMesh (faces clockwise seen from outside); returns its index."""
+        self.convex.append((np.asarray(verts, np.float32).reshape(-1, 3), [list(map(int, f)) for f in faces],
+                            np.asarray(normals, np.float32).reshape(-1, 3),
+                            np.asarray(centroids, np.float32).reshape(-1, 3)))
+        self._frozen = None
+        return len(self.convex) - 1
+
+    def add_triangle_mesh(self, verts, tris):
+        """Registers a static TriangleMesh; returns its index."""
+        self.trimesh.append((np.asarray(verts, np.float32).reshape(-1, 3), np.asarray(tris, np.int32).reshape(-1, 3)))
+        self._frozen = None
+        return len(self.trimesh) - 1
+
+    def add_convex_body(self, mesh, pos, quat=(0, 0, 0, 1), scale=(1, 1, 1), density=1000.0, vel=(0, 0, 0),
+                        angvel=(0, 0, 0), material=(0.6, 0.0, 0.0), world=0):
+        """A dynamic body with one convex-mesh collider; mass = density x hull volume,
+        centre of mass and inertia from src/MassUtil.cpp's convex branch."""
+        verts, faces, _, _ = self.convex[mesh]
+        a = self.add_actor(pos, quat, world)
+        mass = density * hull_volume(verts * np.asarray(scale, np.float64), faces)
+        com, invI = convex_mass_props(verts, faces, scale, mass)
+        self.add_dynamic(a, np.float32(1.0 / mass), com, invI, vel, angvel)
+        self.add_collider(a, CONVEX_MESH, [scale[0], scale[1], scale[2], float(mesh)], material)
         return a
 
     def add_joint(self, jtype, a0, a1, p0, p1, q0=(0, 0, 0, 1), q1=(0, 0, 0, 1), drive=None):
@@ -160,38 +367,69 @@ class Scene:
     # -- views
     @property
     def n_actors(self):
-        return len(self.entity)
+        return self._n["actor"]
 
     @property
     def n_dyn(self):
-        return len(self.actor_of_dyn)
+        return self._n["dyn"]
 
     @property
     def n_colliders(self):
-        return len(self.col_actor)
+        return self._n["col"]
+
+    @property
+    def col_material(self):
+        """Writable view of the colliders' materials (friction, restitution, damping)."""
+        return self.arrays()["col_material"]
+
+    @property
+    def col_enable(self):
+        return self.arrays()["col_enable"]
 
     def arrays(self):
         if self._frozen is None:
-            def f(x, shape, dt=np.float32):
-                return np.ascontiguousarray(np.asarray(x, dt).reshape(shape)) if len(x) else np.zeros(shape, dt)
-            A, D, C = self.n_actors, self.n_dyn, self.n_colliders
-            self._frozen = dict(
-                entity=f(self.entity, (A,), np.uint32), pos=f(self.pos, (A, 3)), quat=f(self.quat, (A, 4)),
-                dyn_of_actor=f(self.dyn_of_actor, (A,), np.int32), world=f(self.world, (A,), np.int32),
-                actor_of_dyn=f(self.actor_of_dyn, (D,), np.int32), kinematic=f(self.kinematic, (D,), np.uint8),
-                vel=f(self.vel, (D, 3)), angvel=f(self.angvel, (D, 3)), inv_mass=f(self.inv_mass, (D,)),
-                com=f(self.com, (D, 3)), inv_inertia=f(self.inv_inertia, (D, 9)),
-                col_actor=f(self.col_actor, (C,), np.int32), col_index=f(self.col_index, (C,), np.int32),
-                col_pos=f(self.col_pos, (C, 3)), col_quat=f(self.col_quat, (C, 4)),
-                col_type=f(self.col_type, (C,), np.int32), col_params=f(self.col_params, (C, 4)),
-                col_material=f(self.col_material, (C, 3)), col_trigger=f(self.col_trigger, (C,), np.uint8),
-                col_enable=f(self.col_enable, (C,), np.uint8), col_data=f(self.col_data, (C,), np.int32))
+            out = {}
+            for k, (dt, shp) in _FIELDS.items():
+                ch = self._chunks[k]
+                out[k] = np.ascontiguousarray(np.concatenate(ch) if ch else np.zeros((0,) + shp, dt))
+            for a, d in self._dyn_patch.items():
+                out["dyn_of_actor"][a] = d
+            out.update(self._mesh_arrays())
+            self._chunks = {k: [v] for k, v in out.items()}   # keep the frozen arrays as the single chunk
+            self._dyn_patch = {}
+            self._frozen = out
         return self._frozen
+
+    def _mesh_arrays(self):
+        cvs, cfs, fis, fi, fn, fc = [0], [0], [0], [], [], []
+        verts = []
+        for v, faces, normals, centroids in self.convex:
+            verts.append(v)
+            cvs.append(cvs[-1] + len(v))
+            cfs.append(cfs[-1] + len(faces))
+            for f in faces:
+                fi.extend(f)
+                fis.append(fis[-1] + len(f))
+            fn.append(normals)
+            fc.append(centroids)
+        tvs, ts, tv, ti = [0], [0], [], []
+        for v, t in self.trimesh:
+            tv.append(v)
+            ti.append(t)
+            tvs.append(tvs[-1] + len(v))
+            ts.append(ts[-1] + len(t))
+        f32 = lambda xs: np.ascontiguousarray(np.concatenate(xs) if xs else np.zeros((1, 3)), np.float32)
+        i32 = lambda xs: np.ascontiguousarray(xs if len(xs) else [0], np.int32)
+        return {
+            "convex_vert_start": i32(cvs), "convex_verts": f32(verts), "convex_face_start": i32(cfs),
+            "face_idx_start": i32(fis), "face_idx": i32(fi), "face_normal": f32(fn), "face_centroid": f32(fc),
+            "tri_vert_start": i32(tvs), "tri_verts": f32(tv), "tri_start": i32(ts),
+            "tri_idx": np.ascontiguousarray(np.concatenate(ti) if ti else np.zeros((1, 3)), np.int32),
+        }
 
     def struct(self, cls):
         """Fill a ctypes Structure whose field names match the arrays (see tests/oracle_lib.py
         and physecs_amd/api.py)."""
-        import ctypes
         arr = self.arrays()
         kw = {}
         for name, ctype in cls._fields_:
@@ -203,6 +441,10 @@ class Scene:
                 kw[name] = self.n_colliders
             elif name == "n_worlds":
                 kw[name] = self.n_worlds
+            elif name == "n_convex":
+                kw[name] = len(self.convex)
+            elif name == "n_trimesh":
+                kw[name] = len(self.trimesh)
             elif name == "actor_world":
                 kw[name] = arr["world"].ctypes.data_as(ctype)
             else:
@@ -210,25 +452,28 @@ class Scene:
         return cls(**kw)
 
     def world_subscene(self, w):
-        """The actors, dynamics and colliders of world w as a standalone Scene (same order)."""
+        """The actors, dynamics, colliders and joints of world w as a standalone Scene, in the
+        same relative order and with the same entity ids."""
         arr = self.arrays()
         s = Scene()
+        s.convex = list(self.convex)
+        s.trimesh = list(self.trimesh)
         amap = {}
-        for a in range(self.n_actors):
-            if arr["world"][a] != w:
-                continue
-            amap[a] = s.add_actor(arr["pos"][a], arr["quat"][a], 0, entity=int(arr["entity"][a]))
+        for a in np.nonzero(arr["world"] == w)[0]:
+            amap[int(a)] = s.add_actor(arr["pos"][a], arr["quat"][a], 0, entity=int(arr["entity"][a]))
         for d in range(self.n_dyn):
             a = int(arr["actor_of_dyn"][d])
             if a in amap:
                 s.add_dynamic(amap[a], arr["inv_mass"][d], arr["com"][d], arr["inv_inertia"][d], arr["vel"][d],
                               arr["angvel"][d], bool(arr["kinematic"][d]))
-        for c in range(self.n_colliders):
+        for c in np.nonzero(np.isin(arr["col_actor"], list(amap)))[0]:
             a = int(arr["col_actor"][c])
-            if a in amap:
-                s.add_collider(amap[a], int(arr["col_type"][c]), arr["col_params"][c][:3], arr["col_material"][c],
-                               arr["col_pos"][c], arr["col_quat"][c], bool(arr["col_trigger"][c]),
-                               bool(arr["col_enable"][c]), int(arr["col_data"][c]))
+            s.add_collider(amap[a], int(arr["col_type"][c]), arr["col_params"][c][:3], arr["col_material"][c],
+                           arr["col_pos"][c], arr["col_quat"][c], bool(arr["col_trigger"][c]),
+                           bool(arr["col_enable"][c]), int(arr["col_data"][c]))
+        for (t, a0, a1, an0, an1, de, dv, dm) in self.joints:
+            if a0 in amap and a1 in amap:
+                s.joints.append((t, amap[a0], amap[a1], an0, an1, de, dv, dm))
         return s
 
 
@@ -293,26 +538,38 @@ def mixed_lattice(seed=0, side=32, spacing=1.5, jitter=0.05, with_container=True
     return s
 
 
-def batched_worlds(seed=0, n_worlds=4096, lo=8, hi=1024, spacing=1.5):
+def batched_worlds(seed=0, n_worlds=4096, lo=8, hi=1024, spacing=1.5, ragdoll_frac=0.0):
     """n_worlds independent worlds with body counts floor(exp(U(ln lo, ln hi))), each a
-    seeded mix of the mixed-lattice shapes on a lattice above its own static ground box.
-    All worlds share coordinates; the world id keeps them apart."""
+    seeded mix of the mixed-lattice shapes on a lattice above its own static ground box;
+    a fraction of the worlds also holds a 15-capsule ragdoll. All worlds share coordinates;
+    the world id keeps them apart. Worlds are laid out one after another (ground, bodies,
+    ragdoll)."""
     rng = np.random.default_rng(seed)
     s = Scene()
     counts = np.floor(np.exp(rng.uniform(math.log(lo), math.log(hi), n_worlds))).astype(int)
+    has_rd = rng.random(n_worlds) < ragdoll_frac
     for w in range(n_worlds):
         n = int(counts[w])
         side = int(math.ceil(n ** (1 / 3)))
         half = (side - 1) * spacing / 2 + 1.0
-        s.add_static(BOX, [half + 1, 0.5, half + 1], [0, -0.5, 0], world=w)
+        s.add_static(BOX, [half + 2, 0.5, half + 2], [0, -0.5, 0], world=w)
         kinds = rng.integers(0, 3, n)
+        prm = np.zeros((n, 4))
+        prm[kinds == SPHERE, 0] = rng.uniform(0.2, 0.5, int(np.sum(kinds == SPHERE)))
+        nc = int(np.sum(kinds == CAPSULE))
+        prm[kinds == CAPSULE, 1] = rng.uniform(0.15, 0.3, nc)
+        prm[kinds == CAPSULE, 0] = rng.uniform(0.2, 0.6, nc)
+        prm[kinds == BOX, :3] = rng.uniform(0.2, 0.6, (int(np.sum(kinds == BOX)), 3))
         quats = random_quats(rng, n)
-        vels = rng.normal(size=(n, 3)).astype(np.float32)
+        vels = rng.normal(size=(n, 3))
+        i = np.arange(n)
         off = (side - 1) * spacing / 2
-        for i in range(n):
-            ix, iz, iy = i % side, (i // side) % side, i // (side * side)
-            p = [ix * spacing - off, 1.0 + iy * spacing, iz * spacing - off]
-            s.add_body(int(kinds[i]), _mixed_shape(rng, int(kinds[i])), p, quats[i], vel=vels[i], world=w)
+        pos = np.stack([(i % side) * spacing - off, 1.0 + (i // (side * side)) * spacing,
+                        ((i // side) % side) * spacing - off], axis=1)
+        s.add_bodies_bulk(kinds, prm, pos, quats, vels, np.full(n, w))
+        if has_rd[w]:
+            top = 1.0 + (n // (side * side) + 1) * spacing + 1.0
+            ragdoll(s, [0.0, top, 0.0], world=w)
     return s
 
 

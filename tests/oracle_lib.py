@@ -31,7 +31,10 @@ class SceneStruct(ctypes.Structure):
         ("inv_mass", F), ("com", F), ("inv_inertia", F),
         ("n_colliders", ctypes.c_int), ("col_actor", I32), ("col_index", I32), ("col_pos", F), ("col_quat", F),
         ("col_type", I32), ("col_params", F), ("col_material", F), ("col_trigger", U8), ("col_enable", U8),
-        ("col_data", I32),
+        ("col_data", I32), ("n_worlds", ctypes.c_int), ("actor_world", I32),
+        ("n_convex", ctypes.c_int), ("convex_vert_start", I32), ("convex_verts", F), ("convex_face_start", I32),
+        ("face_idx_start", I32), ("face_idx", I32), ("face_normal", F), ("face_centroid", F),
+        ("n_trimesh", ctypes.c_int), ("tri_vert_start", I32), ("tri_verts", F), ("tri_start", I32), ("tri_idx", I32),
     ]
 
 
@@ -209,3 +212,31 @@ def row_solve(which, flags, b0, b1, inp, masses, vel, pv, h, iters):
         fn(1 if which == "simd" else 0, flags, b0, b1, fptr(inp), fptr(masses), fptr(vel), fptr(pv), np.float32(h),
            iters, fptr(out))
     return vel, pv, out
+
+
+_convex_ids = {}
+
+
+def register_convex(mesh):
+    """Registers a convex mesh (verts, faces, normals, centroids) with both the oracle and,
+    where built, the reference harness; returns the index both use (params[3])."""
+    key = id(mesh)
+    if key in _convex_ids:
+        return _convex_ids[key][1]
+    verts, faces, normals, centroids = mesh
+    v = np.ascontiguousarray(verts, np.float32)
+    fs = np.cumsum([0] + [len(f) for f in faces]).astype(np.int32)
+    fi = np.ascontiguousarray(np.concatenate([np.asarray(f, np.int32) for f in faces]), np.int32)
+    n = np.ascontiguousarray(normals, np.float32)
+    c = np.ascontiguousarray(centroids, np.float32)
+    args = (len(v), fptr(v), len(faces), iptr(fs), iptr(fi), fptr(n), fptr(c))
+    lib = orc()
+    lib.orc_set_convex_mesh.argtypes = [ctypes.c_int, F, ctypes.c_int, I32, I32, F, F]
+    i0 = lib.orc_set_convex_mesh(*args)
+    if ref_available():
+        r = ref()
+        r.ref_set_convex_mesh.argtypes = [ctypes.c_int, F, ctypes.c_int, I32, I32, F, F]
+        i1 = r.ref_set_convex_mesh(*args)
+        assert i0 == i1, (i0, i1)
+    _convex_ids[key] = (mesh, i0)
+    return i0

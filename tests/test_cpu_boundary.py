@@ -118,3 +118,27 @@ def test_oracle_joint_pendulum_closed_form():
     crossings = np.where(np.diff(np.sign(xs)) != 0)[0]
     period = 2 * np.mean(np.diff(crossings)) / 120
     assert abs(period - 2 * np.pi * np.sqrt(L / 9.81)) < 0.05
+
+
+def test_bulk_bodies_match_single_appends():
+    """Scene.add_bodies_bulk writes the same descriptor fields as add_body, one by one."""
+    from physecs_amd import scenes
+    rng = np.random.default_rng(7)
+    n = 60
+    kinds = rng.integers(0, 3, n)
+    prm = np.zeros((n, 4))
+    for i, k in enumerate(kinds):
+        prm[i, :3] = [rng.uniform(0.2, 0.5), rng.uniform(0.15, 0.3), rng.uniform(0.2, 0.6)] if k == 2 else \
+            ([rng.uniform(0.2, 0.6), rng.uniform(0.15, 0.3), 0] if k == 1 else [rng.uniform(0.2, 0.5), 0, 0])
+    pos = rng.normal(size=(n, 3))
+    q = scenes.random_quats(rng, n)
+    v = rng.normal(size=(n, 3))
+    a, b = scenes.Scene(), scenes.Scene()
+    a.add_static(scenes.BOX, [5, 0.5, 5], [0, -0.5, 0])
+    b.add_static(scenes.BOX, [5, 0.5, 5], [0, -0.5, 0])
+    for i in range(n):
+        a.add_body(int(kinds[i]), list(prm[i, :3]), pos[i], q[i], vel=v[i])
+    b.add_bodies_bulk(kinds, prm, pos, q, v, np.zeros(n, int))
+    A, B = a.arrays(), b.arrays()
+    for k in A:
+        np.testing.assert_allclose(A[k], B[k], rtol=2e-7, err_msg=k)

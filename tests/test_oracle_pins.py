@@ -197,3 +197,93 @@ def test_joint_row_scalar_bitexact(flags):
         b = ol.row_solve("scalar", flags, b0, 1, inp, masses, vel, pv, h, 3)
         for x, y in zip(a, b):
             assert np.array_equal(bits(x), bits(y)), (it, flags, x, y)
+
+
+def _convex_meshes():
+    from physecs_amd import scenes
+    rng = np.random.default_rng(11)
+    cube = scenes.convex_hull([[x, y, z] for x in (-.5, .5) for y in (-.5, .5) for z in (-.5, .5)])
+    meshes = [cube] + [scenes.ellipsoid_hull(rng) for _ in range(6)]
+    return meshes, [ol.register_convex(m) for m in meshes]
+
+
+def test_convex_cube_face_contacts():
+    """A unit-cube hull resting on a box: the reference's clipping keeps a 4-point face
+    manifold only when faces wind clockwise seen from outside (the hull builder's
+    convention); the oracle agrees."""
+    meshes, ids = _convex_meshes()
+    cube = ids[0]
+    for lib, pre in ((ol.ref(), "ref"), (ol.orc(), "orc")):
+        q = np.array([0, 0, 0, 1], np.float32)
+        hit, out = ol.collision(lib, pre, 2, [2, 0.5, 2], [0, -0.5, 0], q, 3, [1, 1, 1, cube], [0, 0.49, 0], q)
+        assert hit and out[0] == 4, (pre, out)
+        hit, out = ol.collision(lib, pre, 3, [1, 1, 1, cube], [0.1, 0, 0.05], q, 3, [1, 1, 1, cube], [0.2, 0.98, 0], q)
+        assert hit and out[0] == 4, (pre, out)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_convex_collision_bitexact(seed):
+    """GJK + EPA (src/GJK.h, src/EPA.h) and the convex contact generators
+    (src/Collision.cpp:352-426,488-499,694-887) for every pair with a convex mesh on either
+    side, double-precision simplex included."""
+    orc, ref = ol.orc(), ol.ref()
+    meshes, ids = _convex_meshes()
+    rng = np.random.default_rng(20 + seed)
+    hits = 0
+    multi = 0
+    for it in range(6000):
+        t0, t1 = int(rng.integers(0, 4)), int(rng.integers(0, 4))
+        if t0 != 3 and t1 != 3:
+            t1 = 3
+        def prm(t):
+            if t == 3:
+                sc = rng.uniform(0.6, 1.4, 3) if rng.random() < 0.3 else np.ones(3)
+                return [sc[0], sc[1], sc[2], ids[int(rng.integers(0, len(ids)))]]
+            return shape_params(t, rng)
+        p0, p1 = prm(t0), prm(t1)
+        pos0 = rng.uniform(-1, 1, 3)
+        pos1 = pos0 + rng.uniform(-1.3, 1.3, 3)
+        q0, q1 = rq(rng), rq(rng)
+        mode = rng.integers(0, 4)
+        if mode == 1:
+            pos1 = pos0 + np.array([rng.uniform(-.1, .1), rng.uniform(.4, 1.2), rng.uniform(-.1, .1)])
+        elif mode == 2:
+            q1 = q0.copy()
+        h1, o1 = ol.collision(orc, "orc", t0, p0, pos0, q0, t1, p1, pos1, q1)
+        h2, o2 = ol.collision(ref, "ref", t0, p0, pos0, q0, t1, p1, pos1, q1)
+        assert h1 == h2 and o1[0] == o2[0], (it, t0, t1, h1, h2, o1[0], o2[0])
+        if h2 and o2[0] > 0:
+            hits += 1
+            multi += o2[0] > 1
+            assert np.array_equal(bits(o1), bits(o2)), (it, t0, t1, o1, o2)
+    assert hits > 1500 and multi > 300, (hits, multi)
+
+
+def test_convex_bounds_and_mass_bitexact():
+    """getBoundsConvexMesh (src/BoundsUtil.cpp:61-69) over the padded vertex buffer, and the
+    host-side convex mass properties against src/MassUtil.cpp."""
+    from physecs_amd import scenes
+    orc, ref = ol.orc(), ol.ref()
+    meshes, ids = _convex_meshes()
+    rng = np.random.default_rng(9)
+    for _ in range(2000):
+        k = int(rng.integers(0, len(ids)))
+        p = np.array([*rng.uniform(0.5, 1.5, 3), ids[k]], np.float32)
+        pos = rng.uniform(-50, 50, 3).astype(np.float32)
+        q = rq(rng)
+        a = np.zeros(6, np.float32)
+        b = np.zeros(6, np.float32)
+        orc.orc_bounds(3, ol.fptr(p), ol.fptr(pos), ol.fptr(q), ol.fptr(a))
+        ref.ref_bounds(3, ol.fptr(p), ol.fptr(pos), ol.fptr(q), ol.fptr(b))
+        assert np.array_equal(bits(a), bits(b))
+    for k, mesh in enumerate(meshes):
+        scale = np.array([1.0, 1.3, 0.8], np.float32)
+        mass = 1000.0 * scenes.hull_volume(mesh[0] * scale, mesh[1])
+        com, invI = scenes.convex_mass_props(mesh[0], mesh[1], scale, mass)
+        out = np.zeros(13, np.float32)
+        prm = np.array([*scale, ids[k]], np.float32)
+        z = np.zeros(3, np.float32)
+        q = np.array([0, 0, 0, 1], np.float32)
+        ref.ref_mass_props(3, ol.fptr(prm), ol.fptr(z), ol.fptr(q), np.float32(mass), ol.fptr(out))
+        np.testing.assert_allclose(out[1:4], com, atol=2e-5)
+        np.testing.assert_allclose(out[4:], invI, rtol=2e-3, atol=1e-6 * np.abs(invI).max())
