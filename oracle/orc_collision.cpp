@@ -30,6 +30,15 @@ Bounds getBounds(V3 pos, Q ori, const Geom& g) {
             V3 w = mulmv(au, he);
             return {pos - w, pos + w};
         }
+        case TRIANGLE_MESH: {   // src/BoundsUtil.cpp:77-85
+            V3 mn = v3s(std::numeric_limits<float>::max()), mx = v3s(std::numeric_limits<float>::lowest());
+            for (const V3& v : g.tmesh->verts) {
+                V3 t = pos + qrot(ori, v);
+                mn = vmin(mn, t);
+                mx = vmax(mx, t);
+            }
+            return {mn, mx};
+        }
         case CONVEX_MESH: {   // src/BoundsUtil.cpp:61-69, over the padded vertex buffer
             V3 mn = v3s(std::numeric_limits<float>::max()), mx = v3s(std::numeric_limits<float>::lowest());
             V3 sc = v3(g.p[0], g.p[1], g.p[2]);
@@ -710,9 +719,15 @@ static void flipContacts(Manifold& m) {
     for (int i = 0; i < m.numPoints; ++i) m.points[i] = {m.points[i].p1, m.points[i].p0};
 }
 
-// src/Collision.cpp:895-1016 (primitive and convex-mesh shapes; triangle meshes are
-// dispatched by the caller)
+// src/Collision.cpp:895-1016
 bool collision(V3 pos0, Q or0, const Geom& g0, V3 pos1, Q or1, const Geom& g1, std::vector<Manifold>& out) {
+    if (g0.type == TRIANGLE_MESH) {
+        size_t first = out.size();
+        bool hit = collisionTriangleMesh(pos1, or1, g1, pos0, or0, *g0.tmesh, out);
+        for (size_t i = first; i < out.size(); ++i) flipContacts(out[i]);
+        return hit;
+    }
+    if (g1.type == TRIANGLE_MESH) return collisionTriangleMesh(pos0, or0, g0, pos1, or1, *g1.tmesh, out);
     out.push_back(Manifold());
     Manifold& m = out.back();
     auto he = [](const Geom& g) { return v3(g.p[0], g.p[1], g.p[2]); };

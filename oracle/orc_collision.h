@@ -18,8 +18,22 @@ struct ConvexMesh {
     std::vector<ConvexFace> faces;
 };
 
+struct Bounds { V3 min, max; };
+
+// include/Physecs/TriangleMesh.h: triangles are permuted by the BVH build; a manifold's
+// triangleIndex is the post-build index
+struct Tri { uint32_t idx[3]; Bounds bounds; V3 normal, centroid; };
+struct TriNode { Bounds bounds; int triCount; int index; };
+struct TriMesh {
+    std::vector<V3> verts;
+    std::vector<Tri> tris;
+    std::vector<TriNode> bvh;
+};
+TriMesh buildTriMesh(const std::vector<V3>& verts, const std::vector<uint32_t>& indices);   // src/TriangleMesh.cpp:139-161
+std::vector<int> overlapBvh(const TriMesh& m, const Bounds& b);                         // src/TriangleMesh.cpp:163-192
+
 // params: sphere {r}, capsule {halfHeight, r}, box {hx, hy, hz}, convex {sx, sy, sz} (scale)
-struct Geom { int type; float p[4]; const ConvexMesh* mesh = nullptr; };
+struct Geom { int type; float p[4]; const ConvexMesh* mesh = nullptr; const TriMesh* tmesh = nullptr; };
 
 struct ContactPoint { V3 p0, p1; };
 struct Manifold {                          // src/ContactManifold.h:11-16
@@ -34,7 +48,6 @@ struct Manifold {                          // src/ContactManifold.h:11-16
 bool collision(V3 pos0, Q or0, const Geom& g0, V3 pos1, Q or1, const Geom& g1,
                std::vector<Manifold>& out);
 
-struct Bounds { V3 min, max; };
 Bounds getBounds(V3 pos, Q ori, const Geom& g);          // src/BoundsUtil.cpp:23-59
 
 // geometry helpers (exported for unit pinning)
@@ -55,6 +68,16 @@ bool collisionCapsuleConvex(V3 pos0, Q or0, float hh0, float r0, V3 pos1, Q or1,
 bool collisionBoxConvex(V3 pos0, Q or0, V3 he0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1, Manifold& res);
 bool collisionConvexConvex(V3 pos0, Q or0, const ConvexMesh* m0, V3 s0, V3 pos1, Q or1, const ConvexMesh* m1,
                            V3 s1, Manifold& res);
+// GJK + EPA of a convex mesh against one triangle (src/CollisionTriangleMesh.cpp:616-620)
+bool gjkEpaConvexTriangle(V3 pos, Q ori, const ConvexMesh* m, V3 scale, V3 a, V3 b, V3 c, V3 startDir, V3& normal,
+                          V3& cpBody, V3& cpTri);
+void contactsPolygonPolygonFace(V3 refPos, const M3& refToWorld, V3 refPlaneOrigin, V3 refPlaneNormal,
+                                V3 incPlaneOrigin, V3 incPlaneNormal, const std::vector<V2>& polygon, int clipX,
+                                int clipY, ContactPoint* cps, int& numPoints);
+// physecs::collisionTriangleMesh (src/CollisionTriangleMesh.cpp:882-956): appends one
+// manifold per contacting triangle
+bool collisionTriangleMesh(V3 pos0, Q or0, const Geom& g0, V3 pos1, Q or1, const TriMesh& mesh,
+                           std::vector<Manifold>& out);
 // counters of EPA runs that needed the whole 100 iterations / hit a zero face normal
 extern thread_local long epaExhausted, epaZeroNormal;
 

@@ -249,9 +249,32 @@ static V3 epa(const S0& s0, const S1& s1, GjkSimplex& s, V3& cp0, V3& cp1) {
     return f3(normal);
 }
 
+// TriangleSupportFunction, src/GJK.h:109-125
+struct SupTriangle {
+    V3 a, b, c;
+    V3 operator()(V3 d) const {
+        if (dot(d, a) > dot(d, b)) {
+            if (dot(d, a) > dot(d, c)) return a;
+            return c;
+        }
+        if (dot(d, b) > dot(d, c)) return b;
+        return c;
+    }
+};
+
+bool gjkEpaConvexTriangle(V3 pos, Q ori, const ConvexMesh* m, V3 scale, V3 a, V3 b, V3 c, V3 startDir, V3& normal,
+                          V3& cpBody, V3& cpTri) {
+    SupConvex s0{pos, mat3_cast(ori), m, scale};
+    SupTriangle s1{a, b, c};
+    GjkSimplex s;
+    if (!gjk(s0, s1, startDir, s)) return false;
+    normal = epa(s0, s1, s, cpBody, cpTri);
+    return true;
+}
+
 // ---------------------------------------------------------------- contact generation
 // src/ContactsUtil.cpp:109-204 (epsilon 1e-4f)
-static void contactsPolygonPolygonFace(V3 refPos, const M3& refToWorld, V3 refPlaneOrigin, V3 refPlaneNormal,
+void contactsPolygonPolygonFace(V3 refPos, const M3& refToWorld, V3 refPlaneOrigin, V3 refPlaneNormal,
                                        V3 incPlaneOrigin, V3 incPlaneNormal, const std::vector<V2>& polygon,
                                        int clipX, int clipY, ContactPoint* cps, int& numPoints) {
     std::vector<V3> pen;

@@ -151,6 +151,7 @@ inline M3 transpose(const M3& m) {
         for (int j = 0; j < 3; ++j) r[i][j] = m[j][i];
     return r;
 }
+inline M3 mneg(const M3& a) { M3 r; for (int i = 0; i < 3; ++i) r[i] = -a[i]; return r; }   // unary minus
 inline M3 madd(const M3& a, const M3& b) { M3 r; for (int i = 0; i < 3; ++i) r[i] = a[i] + b[i]; return r; }
 inline M3 msub(const M3& a, const M3& b) { M3 r; for (int i = 0; i < 3; ++i) r[i] = a[i] - b[i]; return r; }
 inline M3 mscale(float s, const M3& a) { M3 r; for (int i = 0; i < 3; ++i) r[i] = a[i] * s; return r; }

@@ -96,6 +96,7 @@ struct JointRows {                     // one joint's rows, in solve order
 
 struct orc_world {
     std::vector<ConvexMesh> convex;
+    std::vector<TriMesh> trimesh;
     std::vector<Actor> actors;
     std::vector<Dyn> dyn;
     std::vector<Collider> cols;
@@ -183,6 +184,13 @@ orc_world* orc_world_create(const orc_scene* s) {
                                        s->convex_face_start[m + 1] - f0, s->face_idx_start + f0, s->face_idx,
                                        s->face_normal + 3 * f0, s->face_centroid + 3 * f0));
     }
+    for (int m = 0; m < s->n_trimesh; ++m) {
+        int v0 = s->tri_vert_start[m], v1 = s->tri_vert_start[m + 1];
+        std::vector<V3> verts;
+        for (int i = v0; i < v1; ++i) verts.push_back(v3(s->tri_verts[3 * i], s->tri_verts[3 * i + 1], s->tri_verts[3 * i + 2]));
+        std::vector<uint32_t> idx(s->tri_idx + 3 * s->tri_start[m], s->tri_idx + 3 * s->tri_start[m + 1]);
+        w->trimesh.push_back(buildTriMesh(verts, idx));
+    }
     w->cols.resize(s->n_colliders);
     w->bp.resize(s->n_colliders);
     for (int i = 0; i < s->n_colliders; ++i) {
@@ -194,6 +202,7 @@ orc_world* orc_world_create(const orc_scene* s) {
         c.geom.type = s->col_type[i];
         for (int k = 0; k < 4; ++k) c.geom.p[k] = s->col_params[4 * i + k];
         if (c.geom.type == CONVEX_MESH) c.geom.mesh = &w->convex.at((size_t)c.geom.p[3]);
+        if (c.geom.type == TRIANGLE_MESH) c.geom.tmesh = &w->trimesh.at((size_t)c.geom.p[3]);
         c.friction = s->col_material[3 * i];
         c.restitution = s->col_material[3 * i + 1];
         c.damping = s->col_material[3 * i + 2];
@@ -993,10 +1002,62 @@ int orc_set_convex_mesh(int n_verts, const float* verts, int n_faces, const int3
     return (int)g_convex.size() - 1;
 }
 
+static std::vector<TriMesh*> g_trimesh;
+
+// builds a TriangleMesh for the pin entry points; returns its index (params[3])
+int orc_set_tri_mesh(int n_verts, const float* verts, int n_tris, const uint32_t* idx) {
+    std::vector<V3> v;
+    for (int i = 0; i < n_verts; ++i) v.push_back(v3(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]));
+    g_trimesh.push_back(new TriMesh(buildTriMesh(v, std::vector<uint32_t>(idx, idx + 3 * n_tris))));
+    return (int)g_trimesh.size() - 1;
+}
+
+// the built mesh: triangle vertex indices after the BVH permutation (3 per triangle) and
+// the BVH nodes {min xyz, max xyz, triCount, index}
+int orc_tri_mesh_layout(int mesh, uint32_t* triIdx, float* nodes) {
+    const TriMesh& m = *g_trimesh.at((size_t)mesh);
+    for (size_t t = 0; t < m.tris.size(); ++t)
+        for (int k = 0; k < 3; ++k) triIdx[3 * t + k] = m.tris[t].idx[k];
+    for (size_t n = 0; n < m.bvh.size(); ++n) {
+        const TriNode& b = m.bvh[n];
+        float* o = nodes + 8 * n;
+        o[0] = b.bounds.min.x; o[1] = b.bounds.min.y; o[2] = b.bounds.min.z;
+        o[3] = b.bounds.max.x; o[4] = b.bounds.max.y; o[5] = b.bounds.max.z;
+        o[6] = (float)b.triCount; o[7] = (float)b.index;
+    }
+    return (int)m.bvh.size();
+}
+
 static Geom pinGeom(int type, const float* prm) {
     Geom g{type, {prm[0], prm[1], prm[2], prm[3]}};
     if (type == CONVEX_MESH) g.mesh = g_convex.at((size_t)prm[3]);
+    if (type == TRIANGLE_MESH) g.tmesh = g_trimesh.at((size_t)prm[3]);
     return g;
+}
+
+// every manifold of one collision() call: returns the count; per manifold 30 floats
+// {numPoints, normal(3), 4 x (p0, p1), triangleIndex, hit}
+int orc_collision_multi(int type0, const float* prm0, const float* pos0, const float* q0, int type1,
+                        const float* prm1, const float* pos1, const float* q1, int cap, float* out) {
+    Geom g0 = pinGeom(type0, prm0), g1 = pinGeom(type1, prm1);
+    std::vector<Manifold> buf;
+    bool hit = collision(v3(pos0[0], pos0[1], pos0[2]), {q0[0], q0[1], q0[2], q0[3]}, g0,
+                         v3(pos1[0], pos1[1], pos1[2]), {q1[0], q1[1], q1[2], q1[3]}, g1, buf);
+    int n = 0;
+    for (const Manifold& m : buf) {
+        if (n >= cap) break;
+        float* o = out + 30 * n++;
+        std::memset(o, 0, sizeof(float) * 30);
+        o[0] = (float)m.numPoints;
+        o[1] = m.normal.x; o[2] = m.normal.y; o[3] = m.normal.z;
+        for (int k = 0; k < m.numPoints && k < 4; ++k) {
+            o[4 + 6 * k] = m.points[k].p0.x; o[5 + 6 * k] = m.points[k].p0.y; o[6 + 6 * k] = m.points[k].p0.z;
+            o[7 + 6 * k] = m.points[k].p1.x; o[8 + 6 * k] = m.points[k].p1.y; o[9 + 6 * k] = m.points[k].p1.z;
+        }
+        o[28] = (float)m.triangleIndex;
+        o[29] = hit ? 1.f : 0.f;
+    }
+    return hit ? n : -1;
 }
 
 long orc_epa_exhausted() { return epaExhausted; }

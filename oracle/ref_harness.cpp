@@ -10,6 +10,7 @@
 //   physecs::setMassProps         src/MassUtil.cpp:185-192
 //   solve33                       src/MathUtil.h:10-21
 //   ConvexMesh(vertices, faces)   include/Physecs/ConvexMesh.h:29-34, src/ConvexMesh.cpp
+//   TriangleMesh(vertices, indices) src/TriangleMesh.cpp:139-161
 // The two integrate functions below are GLM-written restatements of src/Physecs.cpp
 // (that file does not compile here): they pin the oracle's GLM-free arithmetic against
 // GLM's own operator order.
@@ -26,6 +27,7 @@
 #include "MassUtil.h"
 #include "MathUtil.h"
 #include "SolverData.h"
+#include "TriangleMesh.h"
 #include "Constraint1D.h"
 #include "Constraint1D.cpp"   // the row templates, compiled from the reference source
 #include "ConvexMesh.cpp"     // ConvexMeshVertices<4>'s constructor template (not instantiated there)
@@ -34,6 +36,8 @@ using namespace physecs;
 
 // convex meshes registered by ref_set_convex_mesh (include/Physecs/ConvexMesh.h:7-34)
 static std::vector<ConvexMesh*> g_convex;
+// triangle meshes registered by ref_set_tri_mesh (include/Physecs/TriangleMesh.h)
+static std::vector<TriangleMesh*> g_tri;
 
 static Geometry makeGeom(int type, const float* p) {
     Geometry g;
@@ -46,6 +50,9 @@ static Geometry makeGeom(int type, const float* p) {
         case CONVEX_MESH:   // params: scale xyz, mesh index
             g.convex.mesh = g_convex.at((size_t)p[3]);
             g.convex.scale = glm::vec3(p[0], p[1], p[2]);
+            break;
+        case TRIANGLE_MESH:   // params: -, -, -, mesh index
+            g.triangleMesh.mesh = g_tri.at((size_t)p[3]);
             break;
         default: break;
     }
@@ -129,6 +136,54 @@ int ref_set_convex_mesh(int n_verts, const float* verts, int n_faces, const int*
     }
     g_convex.push_back(new ConvexMesh(std::move(v), std::move(f)));
     return (int)g_convex.size() - 1;
+}
+
+// Builds a TriangleMesh with the reference's constructor (BVH included); returns its index.
+int ref_set_tri_mesh(int n_verts, const float* verts, int n_tris, const uint32_t* idx) {
+    std::vector<glm::vec3> v(n_verts);
+    for (int i = 0; i < n_verts; ++i) v[i] = V(verts + 3 * i);
+    std::vector<unsigned int> ind(idx, idx + 3 * n_tris);
+    g_tri.push_back(new TriangleMesh(v, ind));
+    return (int)g_tri.size() - 1;
+}
+
+// triangle vertex indices after the build's permutation, and BVH nodes {min, max, triCount, index}
+int ref_tri_mesh_layout(int mesh, uint32_t* triIdx, float* nodes) {
+    const TriangleMesh& m = *g_tri.at((size_t)mesh);
+    for (size_t t = 0; t < m.triangles.size(); ++t)
+        for (int k = 0; k < 3; ++k) triIdx[3 * t + k] = m.triangles[t].indices[k];
+    for (size_t n = 0; n < m.bvh.size(); ++n) {
+        const TriangleMeshBVHNode& b = m.bvh[n];
+        float* o = nodes + 8 * n;
+        o[0] = b.bounds.min.x; o[1] = b.bounds.min.y; o[2] = b.bounds.min.z;
+        o[3] = b.bounds.max.x; o[4] = b.bounds.max.y; o[5] = b.bounds.max.z;
+        o[6] = (float)b.triCount; o[7] = (float)b.index;
+    }
+    return (int)m.bvh.size();
+}
+
+// every manifold of one collision() call (see orc_collision_multi for the layout)
+int ref_collision_multi(int type0, const float* prm0, const float* pos0, const float* q0, int type1,
+                        const float* prm1, const float* pos1, const float* q1, int cap, float* out) {
+    Geometry g0 = makeGeom(type0, prm0), g1 = makeGeom(type1, prm1);
+    std::vector<ContactManifold> res;
+    bool hit = collision(V(pos0), Qt(q0), g0, V(pos1), Qt(q1), g1, res);
+    int n = 0;
+    for (const ContactManifold& m : res) {
+        if (n >= cap) break;
+        float* o = out + 30 * n++;
+        std::memset(o, 0, sizeof(float) * 30);
+        o[0] = (float)m.numPoints;
+        o[1] = m.normal.x; o[2] = m.normal.y; o[3] = m.normal.z;
+        for (int k = 0; k < m.numPoints && k < 4; ++k) {
+            o[4 + 6 * k] = m.points[k].position0.x; o[5 + 6 * k] = m.points[k].position0.y;
+            o[6 + 6 * k] = m.points[k].position0.z; o[7 + 6 * k] = m.points[k].position1.x;
+            o[8 + 6 * k] = m.points[k].position1.y; o[9 + 6 * k] = m.points[k].position1.z;
+        }
+        o[28] = (float)m.triangleIndex;
+        o[29] = hit ? 1.f : 0.f;
+    }
+    return hit ? n : -1;
 }
 
 int ref_sqr_dist_segment_aabb(const float* p, const float* dir, float mn, float mx, const float* he, float* out) {

@@ -22,8 +22,24 @@ __device__ __forceinline__ void collider_world_pose(const float4* __restrict__ b
     q = qmul(bq, q4(clquat[c]));
 }
 
-__device__ __forceinline__ void shape_bounds(int type, float4 prm, f3 pos, qf ori, f3& mn, f3& mx) {
-    if (type == PGX_SPHERE) {
+__device__ __forceinline__ void shape_bounds(int type, float4 prm, f3 pos, qf ori, const float4* __restrict__ cvVerts,
+                                             const int4* __restrict__ cvMesh, const float4* __restrict__ tmVerts,
+                                             const int4* __restrict__ tmMesh, f3& mn, f3& mx) {
+    if (type == PGX_CONVEX_MESH || type == PGX_TRIANGLE_MESH) {
+        // src/BoundsUtil.cpp:61-69 (convex: the padded vertex buffer) and :77-85 (triangle mesh)
+        bool cvx = type == PGX_CONVEX_MESH;
+        int4 mi = cvx ? cvMesh[(int)prm.w] : tmMesh[(int)prm.w];
+        const float4* v = (cvx ? cvVerts : tmVerts) + mi.x;
+        int n = cvx ? ((mi.y + 3) & ~3) : mi.y;
+        f3 sc = cvx ? mk3(prm.x, prm.y, prm.z) : s3(1.f);
+        mn = s3(3.402823466e+38f);
+        mx = s3(-3.402823466e+38f);
+        for (int i = 0; i < n; ++i) {
+            f3 t = add(pos, qrot(ori, cvx ? mulv(sc, xyz(v[i])) : xyz(v[i])));
+            mn = vmin(mn, t);
+            mx = vmax(mx, t);
+        }
+    } else if (type == PGX_SPHERE) {
         float r = prm.x;
         mn = sub(pos, s3(r));
         mx = add(pos, s3(r));
@@ -46,6 +62,8 @@ __global__ void k_bounds(int C, const int32_t* __restrict__ cbody, const int32_t
                          const int32_t* __restrict__ cflags, const float4* __restrict__ cparams,
                          const float4* __restrict__ clpos, const float4* __restrict__ clquat,
                          const float4* __restrict__ bpos, const float4* __restrict__ bquat, int dynamicOnly,
+                         const float4* __restrict__ cvVerts, const int4* __restrict__ cvMesh,
+                         const float4* __restrict__ tmVerts, const int4* __restrict__ tmMesh,
                          float4* __restrict__ bmin, float4* __restrict__ bmax) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
@@ -54,7 +72,7 @@ __global__ void k_bounds(int C, const int32_t* __restrict__ cbody, const int32_t
     qf q;
     collider_world_pose(bpos, bquat, clpos, clquat, cbody[c], c, pos, q);
     f3 mn, mx;
-    shape_bounds(ctype[c], cparams[c], pos, q, mn, mx);
+    shape_bounds(ctype[c], cparams[c], pos, q, cvVerts, cvMesh, tmVerts, tmMesh, mn, mx);
     if (dynamicOnly) {   // Bounds::addMargin(vec3(0.01f)), src/Bounds.cpp:23-26
         mx = add(mx, s3(0.01f));
         mn = sub(mn, s3(0.01f));
@@ -67,7 +85,8 @@ void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin) {
     if (w.C == 0) return;
     hipLaunchKernelGGL(k_bounds, dim3(div_up(w.C, 256)), dim3(256), 0, w.stream, w.C, w.cbody.p, w.ctype.p,
                        w.cflags.p, w.cparams.p, w.clpos.p, w.clquat.p, w.bpos.p, w.bquat.p,
-                       dynamicOnlyWithMargin ? 1 : 0, w.cbmin.p, w.cbmax.p);
+                       dynamicOnlyWithMargin ? 1 : 0, w.cvVerts.p, w.cvMesh.p, w.tmVerts.p, w.tmMesh.p, w.cbmin.p,
+                       w.cbmax.p);
     PGX_LAUNCH_CHECK();
 }
 
@@ -429,7 +448,7 @@ int bp_find_pairs(World& w) {
     int n = w.nDynCol;
     int W = w.W;
     int worldShift = 30;
-    fill_i32(s, w.counters.p, 0, 8);
+    fill_i32(s, w.counters.p, 0, 10);   // [8] convex pairs, [9] EPA capacity errors
     if (n > 0) {
         // quantisation box -> Morton codes of the dynamic colliders -> sort
         hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p);

@@ -258,10 +258,11 @@ __device__ bool clip_line(f2& p0, f2& p1, const f2* clip) {
 }
 
 // src/ContactsUtil.cpp:9-107
-__device__ void polygon_box_face_contacts(f3 boxCenter, const m3& basis, int axis, int axisSign, f3 he,
-                                          f3 incOrig, f3 incNormal, const f2* poly, int n, int cx, int cy,
-                                          CPt* cps, int& numPoints) {
-    f3 pen[MAXPOLY];
+template <int CAP>
+__device__ void polygon_box_face_contacts_n(f3 boxCenter, const m3& basis, int axis, int axisSign, f3 he,
+                                            f3 incOrig, f3 incNormal, const f2* poly, int n, int cx, int cy,
+                                            CPt* cps, int& numPoints) {
+    f3 pen[CAP];
     int np = 0;
     f3 dir = s3(0.f);
     set(dir, axis, (float)axisSign);
@@ -314,6 +315,13 @@ __device__ void polygon_box_face_contacts(f3 boxCenter, const m3& basis, int axi
         for (int i = 0; i < np; ++i) emit(i, i);
         numPoints = np;
     }
+}
+
+__device__ __forceinline__ void polygon_box_face_contacts(f3 boxCenter, const m3& basis, int axis, int axisSign,
+                                                          f3 he, f3 incOrig, f3 incNormal, const f2* poly, int n,
+                                                          int cx, int cy, CPt* cps, int& numPoints) {
+    polygon_box_face_contacts_n<MAXPOLY>(boxCenter, basis, axis, axisSign, he, incOrig, incNormal, poly, n, cx, cy,
+                                         cps, numPoints);
 }
 
 // ------------------------------------------------------------------ pair routines
@@ -634,6 +642,12 @@ __device__ bool collide(int t0, float4 g0, f3 p0, qf q0, int t1, float4 g1, f3 p
     return box_box(p0, q0, he0, p1, q1, he1, m);
 }
 
+}  // namespace pgx
+
+#include "np_convex.h"
+
+namespace pgx {
+
 // ------------------------------------------------------------------ kernel
 __device__ __forceinline__ bool in_sorted(const uint64_t* __restrict__ a, int n, uint64_t key) {
     int lo = 0, hi = n;
@@ -650,7 +664,8 @@ __global__ __launch_bounds__(256) void k_narrowphase(
     const int32_t* __restrict__ cflags, const float4* __restrict__ cparams, const float4* __restrict__ clpos,
     const float4* __restrict__ clquat, const uint32_t* __restrict__ centity, const float4* __restrict__ bpos,
     const float4* __restrict__ bquat, const uint64_t* __restrict__ nonColliding, int nNonColliding,
-    int32_t* __restrict__ npNum, float4* __restrict__ npNormal, float4* __restrict__ npPts) {
+    int32_t* __restrict__ npNum, float4* __restrict__ npNormal, float4* __restrict__ npPts,
+    int32_t* __restrict__ heavy, int32_t* __restrict__ nHeavy) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nPairs) return;
     uint64_t key = pkeys[p];
@@ -669,6 +684,10 @@ __global__ __launch_bounds__(256) void k_narrowphase(
     }
     Mf m;
     m.numPoints = 0;
+    if (!skip && (ctype[c0] == PGX_CONVEX_MESH || ctype[c1] == PGX_CONVEX_MESH)) {
+        heavy[atomicAdd(nHeavy, 1)] = p;   // solved by k_narrowphase_convex
+        skip = true;
+    }
     if (!skip) {
         int b0 = cbody[c0], b1 = cbody[c1];
         f3 bp0 = xyz(bpos[b0]), bp1 = xyz(bpos[b1]);
@@ -692,13 +711,70 @@ __global__ __launch_bounds__(256) void k_narrowphase(
     }
 }
 
+// Pairs with a convex mesh: a fixed grid of `lanes` threads walks the list; lane l owns
+// EPA workspace l. The list length is read on the device, so no host round trip.
+__global__ __launch_bounds__(64) void k_narrowphase_convex(
+    const int32_t* __restrict__ nHeavyP, const int32_t* __restrict__ heavy, const uint64_t* __restrict__ pkeys,
+    int localBits, const int32_t* __restrict__ worldColStart, const int32_t* __restrict__ colOfLocal,
+    const int32_t* __restrict__ cbody, const int32_t* __restrict__ ctype, const float4* __restrict__ cparams,
+    const float4* __restrict__ clpos, const float4* __restrict__ clquat, const float4* __restrict__ bpos,
+    const float4* __restrict__ bquat, ConvexView cv, double* epaV, double* epaFN, int32_t* epaFI, int32_t* epaE,
+    int lanes, int32_t* __restrict__ npNum, float4* __restrict__ npNormal, float4* __restrict__ npPts,
+    int32_t* __restrict__ errors) {
+    int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= lanes) return;
+    int nHeavy = *nHeavyP;
+    EpaWs ws;
+    ws.vd = epaV + lane; ws.fn = epaFN + lane; ws.fi = epaFI + lane; ws.ed = epaE + lane;
+    ws.L = (size_t)lanes;
+    ws.overflow = false;
+    uint64_t lmask = (1ull << localBits) - 1;
+    for (int h = lane; h < nHeavy; h += lanes) {
+        int p = heavy[h];
+        uint64_t key = pkeys[p];
+        int wd = (int)(key >> (2 * localBits));
+        int base = worldColStart[wd];
+        int c0 = colOfLocal[base + (int)((key >> localBits) & lmask)];
+        int c1 = colOfLocal[base + (int)(key & lmask)];
+        int b0 = cbody[c0], b1 = cbody[c1];
+        qf bq0 = q4(bquat[b0]), bq1 = q4(bquat[b1]);
+        f3 p0 = add(xyz(bpos[b0]), qrot(bq0, xyz(clpos[c0])));
+        qf q0 = qmul(bq0, q4(clquat[c0]));
+        f3 p1 = add(xyz(bpos[b1]), qrot(bq1, xyz(clpos[c1])));
+        qf q1 = qmul(bq1, q4(clquat[c1]));
+        Mf m;
+        bool ovf = false;
+        bool hit = collide_convex(ctype[c0], cparams[c0], p0, q0, ctype[c1], cparams[c1], p1, q1, cv, ws, m, ovf);
+        if (ovf) atomicAdd(errors, 1);
+        int num = hit ? m.numPoints : 0;
+        npNum[p] = num;
+        if (num) {
+            npNormal[p] = f4(m.normal, 0.f);
+            for (int k = 0; k < 4; ++k)
+                if (k < num) {
+                    npPts[8 * (size_t)p + 2 * k] = f4(m.pts[k].p0, 0.f);
+                    npPts[8 * (size_t)p + 2 * k + 1] = f4(m.pts[k].p1, 0.f);
+                }
+        }
+    }
+}
+
 void np_run(World& w, int nPairs) {
     if (nPairs == 0) return;
     hipLaunchKernelGGL(k_narrowphase, dim3(div_up(nPairs, 256)), dim3(256), 0, w.stream, nPairs, w.pkeys.p,
                        w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.ctype.p, w.cflags.p,
                        w.cparams.p, w.clpos.p, w.clquat.p, w.centity.p, w.bpos.p, w.bquat.p, w.nonColliding.p,
-                       w.nNonColliding, w.npNum.p, w.npNormal.p, w.npPts.p);
+                       w.nNonColliding, w.npNum.p, w.npNormal.p, w.npPts.p, w.heavy.p, w.counters.p + 8);
     PGX_LAUNCH_CHECK();
+    if (w.nConvex > 0) {
+        ConvexView cv{w.cvVerts.p, w.cvMesh.p, w.cvFace.p, w.cvFaceIdx.p, w.cvFaceN.p, w.cvFaceC.p};
+        hipLaunchKernelGGL(k_narrowphase_convex, dim3(div_up(w.epaLanes, 64)), dim3(64), 0, w.stream,
+                           w.counters.p + 8, w.heavy.p, w.pkeys.p, w.localBits, w.worldColStart.p, w.colOfLocal.p,
+                           w.cbody.p, w.ctype.p, w.cparams.p, w.clpos.p, w.clquat.p, w.bpos.p, w.bquat.p, cv,
+                           w.epaV.p, w.epaFN.p, w.epaFI.p, w.epaE.p, w.epaLanes, w.npNum.p, w.npNormal.p, w.npPts.p,
+                           w.counters.p + 9);
+        PGX_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace pgx

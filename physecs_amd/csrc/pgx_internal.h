@@ -44,9 +44,19 @@ struct DBuf {
         n = 0;
     }
     operator T*() const { return p; }
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
 };
 
 constexpr int WAVE = 64;
+// convex narrowphase: EPA polytope capacity per lane and lanes of the convex kernel
+constexpr int EPA_MAXV = 104;          // 4 simplex vertices + at most 100 insertions
+constexpr int EPA_MAXF = 256;          // > 2 * EPA_MAXV - 4, the face count of a closed triangulation
+constexpr int EPA_MAXE = 128;          // horizon (loose) edges of one insertion
+constexpr int EPA_LANES = 32768;
+constexpr int CVX_MAX_FACE_VERTS = 16;
 inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
 // ------------------------------------------------------------------ utilities (util.hip)
@@ -167,6 +177,23 @@ struct World {
 
     DBuf<uint64_t> nonColliding;
     int nNonColliding = 0;
+
+    // convex meshes: padded vertex buffers, {first vertex, count, first face, faces} per
+    // mesh, {first index, count} per face, face normals / centroids
+    int nConvex = 0;
+    DBuf<float4> cvVerts, cvFaceN, cvFaceC;
+    DBuf<int4> cvMesh;
+    DBuf<int2> cvFace;
+    DBuf<int32_t> cvFaceIdx;
+    DBuf<int32_t> heavy;               // pairs with a convex mesh (filled by the primitive kernel)
+    int epaLanes = 0;                  // lanes of the convex kernel = EPA workspaces
+    DBuf<double> epaV, epaFN;
+    DBuf<int32_t> epaFI, epaE;
+    // static triangle meshes (vertices, {first vertex, count, first triangle, count})
+    int nTriMesh = 0;
+    DBuf<float4> tmVerts;
+    DBuf<int4> tmMesh;
+    DBuf<int4> tmTris;                 // vertex indices (mesh-local) + pad
 
     // joints (descriptor order = creation order), colours as Scene::addJoint assigns them
     int J = 0, jCap = 0, jRowsTotal = 0;

@@ -33,6 +33,8 @@ PGX_HD f3 mul(float s, f3 a) { return mk3(s * a.x, s * a.y, s * a.z); }
 PGX_HD f3 divs(f3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
 PGX_HD f3 subs(f3 a, float s) { return mk3(a.x - s, a.y - s, a.z - s); }
 PGX_HD f3 adds(f3 a, float s) { return mk3(a.x + s, a.y + s, a.z + s); }
+PGX_HD f3 mulv(f3 a, f3 b) { return mk3(a.x * b.x, a.y * b.y, a.z * b.z); }
+PGX_HD f3 divv(f3 a, f3 b) { return mk3(a.x / b.x, a.y / b.y, a.z / b.z); }
 PGX_HD float at(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 PGX_HD void set(f3& v, int i, float s) { if (i == 0) v.x = s; else if (i == 1) v.y = s; else v.z = s; }
 
@@ -134,6 +136,26 @@ PGX_HD m3 matrixCross3(f3 x) {
 }
 
 // float4 helpers for the SoA buffers
+// double vectors (glm::dvec3) for GJK/EPA, same association order as the float ones
+struct d3 { double x, y, z; };
+PGX_HD d3 mkd(double x, double y, double z) { d3 r; r.x = x; r.y = y; r.z = z; return r; }
+PGX_HD d3 wide(f3 v) { return mkd(v.x, v.y, v.z); }
+PGX_HD f3 narrow(d3 v) { return mk3((float)v.x, (float)v.y, (float)v.z); }
+PGX_HD d3 dadd(d3 a, d3 b) { return mkd(a.x + b.x, a.y + b.y, a.z + b.z); }
+PGX_HD d3 dsub(d3 a, d3 b) { return mkd(a.x - b.x, a.y - b.y, a.z - b.z); }
+PGX_HD d3 dneg(d3 a) { return mkd(-a.x, -a.y, -a.z); }
+PGX_HD d3 dmul(double s, d3 a) { return mkd(s * a.x, s * a.y, s * a.z); }
+PGX_HD d3 dmul(d3 a, double s) { return mkd(a.x * s, a.y * s, a.z * s); }
+PGX_HD d3 dadds(d3 a, double s) { return mkd(a.x + s, a.y + s, a.z + s); }
+PGX_HD d3 ddivs(d3 a, double s) { return mkd(a.x / s, a.y / s, a.z / s); }
+PGX_HD double ddot(d3 a, d3 b) { double x = a.x * b.x, y = a.y * b.y, z = a.z * b.z; return x + y + z; }
+PGX_HD d3 dcross(d3 a, d3 b) {
+    return mkd(a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y);
+}
+PGX_HD double dlength(d3 v) { return sqrt(ddot(v, v)); }
+PGX_HD d3 dnormalize(d3 v) { return dmul(v, 1.0 / sqrt(ddot(v, v))); }
+PGX_HD double dsign(double x) { return (double)(0.0 < x) - (double)(x < 0.0); }
+
 PGX_HD f3 xyz(float4 a) { return mk3(a.x, a.y, a.z); }
 PGX_HD float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 PGX_HD qf q4(float4 a) { return mkq(a.w, a.x, a.y, a.z); }

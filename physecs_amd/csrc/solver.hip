@@ -526,13 +526,16 @@ int sv_setup_manifolds(World& w, int nPairs) {
     // sort's value buffer is free at this point and holds the flags)
     hipLaunchKernelGGL(k_flag_manifolds, dim3(div_up(nPairs, 256)), dim3(256), 0, s, nPairs, w.npNum.p, w.pvals.p);
     exclusive_scan_i32(s, w.pvals.p, w.npIndex.p, nPairs, w.counters.p + 1, w.scanT);
-    int M = 0;
+    int M = 0, epaErrors = 0;
     PGX_HIP(hipMemcpyAsync(&M, w.counters.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (w.nConvex) PGX_HIP(hipMemcpyAsync(&epaErrors, w.counters.p + 9, sizeof(int), hipMemcpyDeviceToHost, s));
     hipLaunchKernelGGL(k_compact, dim3(div_up(nPairs, 256)), dim3(256), 0, s, nPairs, w.npNum.p, w.npIndex.p,
                        w.pkeys.p, w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.cflags.p,
                        w.mfPair.p, w.mfKey.p, w.mfB0.p, w.mfB1.p);
     PGX_LAUNCH_CHECK();
     PGX_HIP(hipStreamSynchronize(s));
+    if (epaErrors)
+        throw PgxError{PGX_ERR_CAPACITY, "EPA polytope capacity exceeded in " + std::to_string(epaErrors) + " pairs"};
     return M;
 }
 

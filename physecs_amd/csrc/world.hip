@@ -89,8 +89,27 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
         int a = sc->col_actor[c];
         if (a < 0 || a >= sc->n_actors) return fail(PGX_ERR_INVALID, "collider %d: actor out of range", c);
         int t = sc->col_type[c];
-        if (t != PGX_SPHERE && t != PGX_CAPSULE && t != PGX_BOX)
+        if (t == PGX_CONVEX_MESH) {
+            float mf = sc->col_params[4 * c + 3];
+            if (!(mf >= 0.f) || mf != (float)(int)mf || (int)mf >= sc->n_convex)
+                return fail(PGX_ERR_INVALID, "collider %d: convex mesh index %g out of range", c, (double)mf);
+        } else if (t != PGX_SPHERE && t != PGX_CAPSULE && t != PGX_BOX) {
             return fail(PGX_ERR_UNSUPPORTED, "collider %d: geometry type %d is not supported on the device path", c, t);
+        }
+    }
+    for (int m = 0; m < sc->n_convex; ++m) {
+        int nv = sc->convex_vert_start[m + 1] - sc->convex_vert_start[m];
+        int f0 = sc->convex_face_start[m], f1 = sc->convex_face_start[m + 1];
+        if (nv < 1 || f1 <= f0) return fail(PGX_ERR_INVALID, "convex mesh %d: no vertices or faces", m);
+        for (int f = f0; f < f1; ++f) {
+            int k0 = sc->face_idx_start[f], k1 = sc->face_idx_start[f + 1];
+            if (k1 - k0 < 3 || k1 - k0 > CVX_MAX_FACE_VERTS)
+                return fail(PGX_ERR_INVALID, "convex mesh %d face %d: %d vertices (3..%d supported)", m, f - f0,
+                            k1 - k0, CVX_MAX_FACE_VERTS);
+            for (int k = k0; k < k1; ++k)
+                if (sc->face_idx[k] < 0 || sc->face_idx[k] >= nv)
+                    return fail(PGX_ERR_INVALID, "convex mesh %d face %d: vertex index out of range", m, f - f0);
+        }
     }
     return guarded([&] {
         auto* h = new pgx_world();
@@ -212,6 +231,41 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
             w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(36 * M);
             for (int k = 0; k < 2; ++k) { w.cacheKey[k].alloc(M); w.cacheNum[k].alloc(M); w.cacheLp[k].alloc(4 * M); }
             w.nonColliding.alloc(1);
+            w.heavy.alloc(w.pairCap);
+            if (sc->n_convex > 0) {
+                // ConvexMesh vertex buffers padded to a multiple of 4 with the last vertex
+                // (src/ConvexMesh.cpp:3-11); faces, normals and centroids as given
+                std::vector<float4> cvv, cvn, cvc;
+                std::vector<int4> cvm;
+                std::vector<int2> cvf;
+                std::vector<int32_t> cvfi;
+                for (int m = 0; m < sc->n_convex; ++m) {
+                    int v0 = sc->convex_vert_start[m], nv = sc->convex_vert_start[m + 1] - v0;
+                    int first = (int)cvv.size();
+                    for (int i = 0; i < nv; ++i)
+                        cvv.push_back(make_float4(sc->convex_verts[3 * (v0 + i)], sc->convex_verts[3 * (v0 + i) + 1],
+                                                  sc->convex_verts[3 * (v0 + i) + 2], 0.f));
+                    while ((cvv.size() - first) % 4) cvv.push_back(cvv.back());
+                    int f0 = sc->convex_face_start[m], nf = sc->convex_face_start[m + 1] - f0;
+                    cvm.push_back(make_int4(first, nv, (int)cvf.size(), nf));
+                    for (int f = f0; f < f0 + nf; ++f) {
+                        int k0 = sc->face_idx_start[f], k1 = sc->face_idx_start[f + 1];
+                        cvf.push_back(make_int2((int)cvfi.size(), k1 - k0));
+                        for (int k = k0; k < k1; ++k) cvfi.push_back(sc->face_idx[k]);
+                        cvn.push_back(make_float4(sc->face_normal[3 * f], sc->face_normal[3 * f + 1],
+                                                  sc->face_normal[3 * f + 2], 0.f));
+                        cvc.push_back(make_float4(sc->face_centroid[3 * f], sc->face_centroid[3 * f + 1],
+                                                  sc->face_centroid[3 * f + 2], 0.f));
+                    }
+                }
+                w.nConvex = sc->n_convex;
+                upload(w.cvVerts, cvv, s); upload(w.cvMesh, cvm, s); upload(w.cvFace, cvf, s);
+                upload(w.cvFaceIdx, cvfi, s); upload(w.cvFaceN, cvn, s); upload(w.cvFaceC, cvc, s);
+                w.epaLanes = EPA_LANES;
+                size_t L = (size_t)w.epaLanes;
+                w.epaV.alloc(9 * EPA_MAXV * L); w.epaFN.alloc(3 * EPA_MAXF * L);
+                w.epaFI.alloc(3 * EPA_MAXF * L); w.epaE.alloc(2 * EPA_MAXE * L);
+            }
             // creation-time bounds: no margin (src/Physecs.cpp:32)
             bp_compute_bounds(w, false);
             PGX_HIP(hipStreamSynchronize(s));

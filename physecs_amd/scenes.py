@@ -630,6 +630,97 @@ def ragdoll(s, origin, world=0, material=(0.6, 0.0, 0.0)):
     return parts
 
 
+def _value_noise(rng, n, cells):
+    """Value noise on an (n x n) vertex grid from a (cells+1)^2 lattice of U(-1,1) values,
+    bilinear with smoothstep weights."""
+    lat = rng.uniform(-1, 1, (cells + 1, cells + 1))
+    t = np.linspace(0, cells, n, endpoint=True)
+    i0 = np.minimum(np.floor(t).astype(int), cells - 1)
+    f = t - i0
+    f = f * f * (3 - 2 * f)
+    a = lat[i0][:, i0]
+    b = lat[i0 + 1][:, i0]
+    c = lat[i0][:, i0 + 1]
+    d = lat[i0 + 1][:, i0 + 1]
+    fx, fz = f[:, None], f[None, :]
+    return (a * (1 - fx) + b * fx) * (1 - fz) + (c * (1 - fx) + d * fx) * fz
+
+
+def terrain_mesh(seed=0, quads=256, spacing=1.0, amplitude=8.0, octaves=5, lacunarity=2.0, gain=0.5):
+    """A heightfield of quads x quads cells (two triangles each, normals +y) centred on the
+    origin; heights = amplitude * fBm of value noise (octaves, lacunarity, gain)."""
+    rng = np.random.default_rng(seed)
+    n = quads + 1
+    h = np.zeros((n, n))
+    amp, cells, norm = 1.0, 4, 0.0
+    for _ in range(octaves):
+        h += amp * _value_noise(rng, n, min(cells, quads))
+        norm += amp
+        amp *= gain
+        cells = int(cells * lacunarity)
+    h = amplitude * h / norm
+    half = quads * spacing / 2
+    xs = np.arange(n) * spacing - half
+    verts = np.zeros((n, n, 3), np.float32)
+    verts[:, :, 0] = xs[:, None]
+    verts[:, :, 2] = xs[None, :]
+    verts[:, :, 1] = h
+    i, j = np.meshgrid(np.arange(quads), np.arange(quads), indexing="ij")
+    v00 = (i * n + j).ravel()
+    v01 = (i * n + j + 1).ravel()
+    v10 = ((i + 1) * n + j).ravel()
+    v11 = ((i + 1) * n + j + 1).ravel()
+    tris = np.stack([np.stack([v00, v01, v10], 1), np.stack([v10, v01, v11], 1)], 1).reshape(-1, 3)
+    return verts.reshape(-1, 3), tris.astype(np.int32), h
+
+
+def terrain_drop(seed=0, quads=12, n=40, convex_frac=0.4, amplitude=2.0, heights=(0.5, 3.0)):
+    """Bodies of every shape dropped onto a small static triangle-mesh terrain."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    verts, tris, h = terrain_mesh(seed, quads=quads, amplitude=amplitude)
+    tm = s.add_triangle_mesh(verts, tris)
+    a = s.add_actor([0, 0, 0])
+    s.add_collider(a, TRIANGLE_MESH, [0, 0, 0, float(tm)])
+    meshes = [s.add_convex_mesh(*ellipsoid_hull(rng, lo=0.25, hi=0.5)) for _ in range(4)]
+    half = quads / 2 - 1.5
+    for _ in range(n):
+        x, z = rng.uniform(-half, half, 2)
+        y = float(np.max(h)) + rng.uniform(*heights)
+        q = random_quats(rng, 1)[0]
+        v = rng.normal(size=3) * 0.3
+        if rng.random() < convex_frac:
+            s.add_convex_body(meshes[int(rng.integers(0, 4))], [x, y, z], q, vel=v)
+        else:
+            k = int(rng.integers(0, 3))
+            prm = {SPHERE: [rng.uniform(0.2, 0.4)], CAPSULE: [rng.uniform(0.2, 0.4), rng.uniform(0.1, 0.2)],
+                   BOX: list(rng.uniform(0.15, 0.4, 3))}[k]
+            s.add_body(k, prm, [x, y, z], q, vel=v)
+    return s
+
+
+def convex_pile(seed=0, n=60, n_meshes=6, extent=2.5, convex_frac=0.6, height=1.0):
+    """Convex hulls (24 points on random ellipsoids) mixed with spheres, capsules and boxes,
+    in a loose pile above a static ground box, for convex-mesh parity tests."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    s.add_static(BOX, [20, 0.5, 20], [0, -0.5, 0])
+    meshes = [s.add_convex_mesh(*ellipsoid_hull(rng)) for _ in range(n_meshes)]
+    for i in range(n):
+        pos = [rng.uniform(-extent, extent), height + rng.uniform(0, 2 * extent), rng.uniform(-extent, extent)]
+        q = random_quats(rng, 1)[0]
+        v = rng.normal(size=3) * 0.5
+        if rng.random() < convex_frac:
+            sc = (1.0, 1.0, 1.0) if rng.random() < 0.7 else tuple(rng.uniform(0.7, 1.3, 3))
+            s.add_convex_body(meshes[int(rng.integers(0, n_meshes))], pos, q, scale=sc, vel=v)
+        else:
+            k = int(rng.integers(0, 3))
+            prm = {SPHERE: [rng.uniform(0.2, 0.5)], CAPSULE: [rng.uniform(0.2, 0.6), rng.uniform(0.15, 0.3)],
+                   BOX: list(rng.uniform(0.2, 0.6, 3))}[k]
+            s.add_body(k, prm, pos, q, vel=v)
+    return s
+
+
 def joint_scene(seed=0, chains=6, ragdolls=2, ground=True):
     """Capsule chains and ragdolls dropped on a ground box, for joint parity tests."""
     rng = np.random.default_rng(seed)

@@ -240,3 +240,40 @@ def register_convex(mesh):
         assert i0 == i1, (i0, i1)
     _convex_ids[key] = (mesh, i0)
     return i0
+
+
+def register_trimesh(verts, tris):
+    """Builds the triangle mesh in the oracle and, where built, in the reference harness
+    (its own TriangleMesh constructor); returns the index both use (params[3])."""
+    v = np.ascontiguousarray(verts, np.float32)
+    t = np.ascontiguousarray(tris, np.uint32)
+    U = ctypes.POINTER(ctypes.c_uint32)
+    lib = orc()
+    lib.orc_set_tri_mesh.argtypes = [ctypes.c_int, F, ctypes.c_int, U]
+    i0 = lib.orc_set_tri_mesh(len(v), fptr(v), len(t), t.ctypes.data_as(U))
+    if ref_available():
+        r = ref()
+        r.ref_set_tri_mesh.argtypes = [ctypes.c_int, F, ctypes.c_int, U]
+        i1 = r.ref_set_tri_mesh(len(v), fptr(v), len(t), t.ctypes.data_as(U))
+        assert i0 == i1, (i0, i1)
+    return i0
+
+
+def trimesh_layout(which, mesh, n_tris):
+    U = ctypes.POINTER(ctypes.c_uint32)
+    idx = np.zeros((n_tris, 3), np.uint32)
+    nodes = np.zeros((2 * n_tris + 1, 8), np.float32)
+    fn = orc().orc_tri_mesh_layout if which == "orc" else ref().ref_tri_mesh_layout
+    fn.argtypes = [ctypes.c_int, U, F]
+    n = fn(mesh, idx.ctypes.data_as(U), fptr(nodes))
+    return idx, nodes[:n]
+
+
+def collision_multi(which, t0, p0, pos0, q0, t1, p1, pos1, q1, cap=64):
+    out = np.zeros((cap, 30), np.float32)
+    args = [_f32(p0, 4), _f32(pos0, 3), _f32(q0, 4), _f32(p1, 4), _f32(pos1, 3), _f32(q1, 4)]
+    fn = orc().orc_collision_multi if which == "orc" else ref().ref_collision_multi
+    fn.argtypes = [ctypes.c_int, F, F, F, ctypes.c_int, F, F, F, ctypes.c_int, F]
+    n = fn(t0, fptr(args[0]), fptr(args[1]), fptr(args[2]), t1, fptr(args[3]), fptr(args[4]), fptr(args[5]), cap,
+           fptr(out))
+    return n, out[:max(n, 0)]

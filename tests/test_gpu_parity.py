@@ -140,3 +140,30 @@ def test_joint_overflow_colour_lockstep(api):
     o = ol.OracleWorld(sc, substeps=4)
     assert max(o.joint_colors()) == 8
     lockstep(sc, 90)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_convex_manifolds_first_steps(api, seed):
+    """Convex hulls among primitives: GJK/EPA in double precision on the device; pair sets
+    and manifolds bit-identical to the oracle's over the first steps."""
+    log(f"[convex] pile seed={seed}: manifolds")
+    sc = scenes.convex_pile(seed, n=70, extent=1.6)
+    g = api.World(sc, substeps=4)
+    o = ol.OracleWorld(sc, substeps=4)
+    for k in range(6):
+        g.step(1 / 60)
+        keys, ints, f = g.manifolds()
+        o.set_order(keys)
+        o.step(1 / 60)
+        assert pair_set(g.pairs()) == pair_set(o.pairs()), k
+        gm = manifold_dict(keys, ints, f)
+        om = manifold_dict(*o.manifolds())
+        assert set(gm) == set(om), k
+        for key in gm:
+            assert np.array_equal(bits(gm[key][1]), bits(om[key][1])), (k, key)
+    assert len(gm) > 40
+
+
+def test_convex_lockstep_bitexact(api):
+    log("[lockstep] convex pile 90 steps")
+    lockstep(scenes.convex_pile(3, n=50, extent=2.0), 90, persistent=False)

@@ -287,3 +287,75 @@ def test_convex_bounds_and_mass_bitexact():
         ref.ref_mass_props(3, ol.fptr(prm), ol.fptr(z), ol.fptr(q), np.float32(mass), ol.fptr(out))
         np.testing.assert_allclose(out[1:4], com, atol=2e-5)
         np.testing.assert_allclose(out[4:], invI, rtol=2e-3, atol=1e-6 * np.abs(invI).max())
+
+
+def test_trimesh_bvh_build_matches_reference():
+    """TriangleMesh's constructor (src/TriangleMesh.cpp:7-161): the triangle permutation and
+    every BVH node (bounds, count, index) bit for bit, on an fBm terrain."""
+    from physecs_amd import scenes
+    verts, tris, _ = scenes.terrain_mesh(3, quads=48, amplitude=6.0)
+    m = ol.register_trimesh(verts, tris)
+    a_idx, a_nodes = ol.trimesh_layout("orc", m, len(tris))
+    b_idx, b_nodes = ol.trimesh_layout("ref", m, len(tris))
+    assert np.array_equal(a_idx, b_idx)
+    assert len(a_nodes) == len(b_nodes) > len(tris) // 4
+    assert np.array_equal(bits(a_nodes), bits(b_nodes))
+
+
+def test_trimesh_collision_bitexact():
+    """collisionTriangleMesh (src/CollisionTriangleMesh.cpp:882-956) for spheres, capsules,
+    boxes and convex hulls against a terrain, with the mesh on either side of the pair: the
+    manifold list (order, triangle index, points, normal) bit for bit."""
+    from physecs_amd import scenes
+    verts, tris, h = scenes.terrain_mesh(5, quads=24, amplitude=3.0)
+    m = ol.register_trimesh(verts, tris)
+    meshes, ids = _convex_meshes()
+    rng = np.random.default_rng(31)
+    total = multi = 0
+    flat_ties = 0
+    for it in range(3000):
+        t = int(rng.integers(0, 4))
+        prm = shape_params(t, rng) if t < 3 else [1, 1, 1, ids[int(rng.integers(0, len(ids)))]]
+        x, z = rng.uniform(-10, 10, 2)
+        i, j = int(round(x + 12)), int(round(z + 12))
+        y = h[min(i, 24), min(j, 24)] + rng.uniform(-0.3, 0.6)
+        q = rq(rng) if rng.random() < 0.7 else np.array([0, 0, 0, 1], np.float32)
+        mq = np.array([0, 0, 0, 1], np.float32)
+        mpos = np.zeros(3, np.float32)
+        mesh_first = rng.random() < 0.3
+        if mesh_first:
+            args = (4, [0, 0, 0, m], mpos, mq, t, prm, [x, y, z], q)
+        else:
+            args = (t, prm, [x, y, z], q, 4, [0, 0, 0, m], mpos, mq)
+        na, oa = ol.collision_multi("orc", *args)
+        nb, ob = ol.collision_multi("ref", *args)
+        assert na == nb, (it, t, na, nb)
+        if nb > 0:
+            total += 1
+            multi += nb > 1
+            assert np.array_equal(bits(oa), bits(ob)), (it, t, mesh_first, oa, ob)
+    assert total > 1000 and multi > 200, (total, multi)
+
+
+def test_trimesh_flat_ties_bitexact():
+    """Boxes resting on a flat triangle grid: many edge/vertex contacts at equal distance, so
+    the sort's tie order and the vertex voiding decide which manifolds survive."""
+    from physecs_amd import scenes
+    verts, tris, _ = scenes.terrain_mesh(0, quads=8, amplitude=0.0)
+    m = ol.register_trimesh(verts, tris)
+    rng = np.random.default_rng(41)
+    seen = 0
+    for it in range(800):
+        he = list(rng.choice([0.5, 1.0, 1.5], 3))
+        x, z = rng.choice([-1.0, -0.5, 0.0, 0.5, 1.0], 2)
+        y = he[1] - rng.choice([0.0, 0.01, 0.05])
+        ang = rng.choice([0.0, np.pi / 4, np.pi / 2])
+        q = np.array([0, np.sin(ang / 2), 0, np.cos(ang / 2)], np.float32)
+        args = (2, he, [x, y, z], q, 4, [0, 0, 0, m], np.zeros(3), np.array([0, 0, 0, 1], np.float32))
+        na, oa = ol.collision_multi("orc", *args)
+        nb, ob = ol.collision_multi("ref", *args)
+        assert na == nb
+        if nb > 0:
+            seen += 1
+            assert np.array_equal(bits(oa), bits(ob)), (it, oa, ob)
+    assert seen > 500
