@@ -448,7 +448,9 @@ int bp_find_pairs(World& w) {
     int n = w.nDynCol;
     int W = w.W;
     int worldShift = 30;
-    fill_i32(s, w.counters.p, 0, 10);   // [8] convex pairs, [9] EPA capacity errors
+    // [8] convex pairs, [9] EPA capacity errors, [10] mesh pairs, [11] mesh manifold slots,
+    // [12] mesh capacity errors
+    fill_i32(s, w.counters.p, 0, 13);
     if (n > 0) {
         // quantisation box -> Morton codes of the dynamic colliders -> sort
         hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p);

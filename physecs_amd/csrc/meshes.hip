@@ -1,0 +1,163 @@
+// Host-side build of the static triangle meshes: the triangles and the BVH that the
+// reference's TriangleMesh constructor builds (src/TriangleMesh.cpp:7-161), built once at
+// world creation and uploaded. The narrowphase queries the BVH on the device.
+//
+// The build follows the reference step for step -- 6 SAH buckets per axis over the node's
+// bounds, the in-place two-sided partition, depth-first child creation -- because the
+// triangle order it leaves is observable: a contact manifold carries the post-build
+// triangle index (it keys the restitution cache and the solve order), so a different
+// build would give different manifolds.
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <vector>
+#include "pgx_internal.h"
+
+namespace pgx {
+
+namespace {
+struct HBounds { f3 mn, mx; };
+struct HTri { int32_t idx[3]; HBounds b; f3 n, c; };
+struct HNode { HBounds b; int triCount; int index; };
+
+HBounds hunion(const HBounds& a, const HBounds& b) { return {vmin(a.mn, b.mn), vmax(a.mx, b.mx)}; }
+HBounds hempty() { return {s3(FLT_MAX), s3(-FLT_MAX)}; }
+float harea(const HBounds& b) {                                  // Bounds::area, src/Bounds.cpp:28-31
+    f3 d = sub(b.mx, b.mn);
+    return 2.f * (d.x * d.y + d.y * d.z + d.z * d.x);
+}
+
+struct Builder {
+    std::vector<HTri>& tris;
+    std::vector<HNode>& bvh;
+    static constexpr int NB = 6;
+
+    void node_bounds(int id) {                                   // :7-14
+        HBounds b = hempty();
+        for (int i = 0; i < bvh[id].triCount; ++i) b = hunion(b, tris[bvh[id].index + i].b);
+        bvh[id].b = b;
+    }
+    bool fill(int id, int axis, HBounds* bb, int* bc) {          // :16-38
+        for (int i = 0; i < NB; ++i) { bb[i] = hempty(); bc[i] = 0; }
+        float start = at(bvh[id].b.mn, axis), end = at(bvh[id].b.mx, axis);
+        float len = end - start;
+        if (!len) return false;
+        for (int i = 0; i < bvh[id].triCount; ++i) {
+            const HTri& t = tris[bvh[id].index + i];
+            int k = std::min(NB - 1, static_cast<int>((at(t.c, axis) - start) / len * NB));
+            bb[k] = hunion(bb[k], t.b);
+            bc[k]++;
+        }
+        return true;
+    }
+    bool choose(int id, int& axisOut, int& split, HBounds& bl, HBounds& br, int& cl, int& cr) {   // :40-94
+        float best = FLT_MAX;
+        for (int axis = 0; axis < 3; ++axis) {
+            HBounds bb[NB];
+            int bc[NB];
+            if (!fill(id, axis, bb, bc)) continue;
+            for (int s = 1; s < NB; ++s) {
+                HBounds l = hempty(), r = hempty();
+                int nl = 0, nr = 0;
+                for (int i = 0; i < NB; ++i) {
+                    if (i < s) { l = hunion(l, bb[i]); nl += bc[i]; }
+                    else { r = hunion(r, bb[i]); nr += bc[i]; }
+                }
+                if (!nl || !nr) continue;
+                float cost = nl * harea(l) + nr * harea(r);
+                if (cost < best) { best = cost; axisOut = axis; split = s; bl = l; br = r; cl = nl; cr = nr; }
+            }
+        }
+        return cl && cr;
+    }
+    void subdivide(int root) {                                   // :96-137, with an explicit stack
+        std::vector<int> todo{root};
+        while (!todo.empty()) {
+            int id = todo.back();
+            todo.pop_back();
+            int axis = 0, split = 0, cl = 0, cr = 0;
+            HBounds bl{}, br{};
+            if (!choose(id, axis, split, bl, br, cl, cr)) continue;
+            float start = at(bvh[id].b.mn, axis), end = at(bvh[id].b.mx, axis);
+            float len = end - start;
+            int i = bvh[id].index, j = i + bvh[id].triCount - 1;
+            while (i <= j) {
+                int k = static_cast<int>((at(tris[i].c, axis) - start) / len * NB);
+                if (k < split) ++i;
+                else std::swap(tris[i], tris[j--]);
+            }
+            int first = bvh[id].index;
+            bvh.push_back({bl, cl, first});
+            bvh.push_back({br, cr, i});
+            bvh[id].triCount = 0;
+            bvh[id].index = (int)bvh.size() - 2;
+            // the reference recurses into the left child first, then the right: the left
+            // subtree's nodes are created before the right child's
+            todo.push_back(bvh[id].index + 1);
+            todo.push_back(bvh[id].index);
+        }
+    }
+};
+}  // namespace
+
+// Builds every mesh of the descriptor and uploads vertices, triangles (post-build order),
+// triangle normals / centroids and BVH nodes. Returns the deepest BVH path (stack bound).
+int upload_triangle_meshes(World& w, const pgx_scene_desc* sc, hipStream_t s) {
+    std::vector<float4> verts, triN, triC, nodes;
+    std::vector<int4> tris, meshes;
+    int maxDepth = 0;
+    for (int m = 0; m < sc->n_trimesh; ++m) {
+        int v0 = sc->tri_vert_start[m], nv = sc->tri_vert_start[m + 1] - v0;
+        int t0 = sc->tri_start[m], nt = sc->tri_start[m + 1] - t0;
+        std::vector<f3> v(nv);
+        for (int i = 0; i < nv; ++i)
+            v[i] = mk3(sc->tri_verts[3 * (v0 + i)], sc->tri_verts[3 * (v0 + i) + 1], sc->tri_verts[3 * (v0 + i) + 2]);
+        std::vector<HTri> ht(nt);
+        for (int t = 0; t < nt; ++t) {
+            HTri& h = ht[t];
+            for (int k = 0; k < 3; ++k) h.idx[k] = sc->tri_idx[3 * (t0 + t) + k];
+            f3 a = v[h.idx[0]], b = v[h.idx[1]], c = v[h.idx[2]];
+            h.b = {vmin(vmin(a, b), c), vmax(vmax(a, b), c)};        // getBoundsTriangle
+            h.n = normalize(cross(sub(b, a), sub(c, a)));
+            h.c = divs(add(add(a, b), c), 3.f);
+        }
+        std::vector<HNode> bvh;
+        bvh.reserve(2 * (size_t)nt + 1);
+        bvh.push_back({hempty(), nt, 0});
+        Builder bld{ht, bvh};
+        bld.node_bounds(0);
+        bld.subdivide(0);
+        // depth of the tree (bounds the device traversal stack)
+        std::vector<int> depth(bvh.size(), 0);
+        for (size_t n = 0; n < bvh.size(); ++n)
+            if (!bvh[n].triCount) {
+                depth[bvh[n].index] = depth[bvh[n].index + 1] = depth[n] + 1;
+                maxDepth = std::max(maxDepth, depth[n] + 1);
+            }
+        int vbase = (int)verts.size(), tbase = (int)tris.size(), nbase = (int)nodes.size() / 2;
+        meshes.push_back(make_int4(vbase, nv, tbase, nbase));
+        for (const f3& p : v) verts.push_back(f4(p, 0.f));
+        for (const HTri& h : ht) {
+            tris.push_back(make_int4(h.idx[0], h.idx[1], h.idx[2], 0));
+            triN.push_back(f4(h.n, 0.f));
+            triC.push_back(f4(h.c, 0.f));
+        }
+        auto asf = [](int i) { float f; std::memcpy(&f, &i, sizeof f); return f; };
+        for (const HNode& n : bvh) {   // {min, triCount bits}, {max, index bits}
+            nodes.push_back(make_float4(n.b.mn.x, n.b.mn.y, n.b.mn.z, asf(n.triCount)));
+            nodes.push_back(make_float4(n.b.mx.x, n.b.mx.y, n.b.mx.z, asf(n.index)));
+        }
+    }
+    auto up = [&](auto& buf, const auto& vec) {
+        buf.alloc(vec.size());
+        if (!vec.empty())
+            PGX_HIP(hipMemcpyAsync(buf.p, vec.data(), sizeof(vec[0]) * vec.size(), hipMemcpyHostToDevice, s));
+    };
+    up(w.tmVerts, verts); up(w.tmTris, tris); up(w.tmTriN, triN); up(w.tmTriC, triC); up(w.tmNodes, nodes);
+    up(w.tmMesh, meshes);
+    PGX_HIP(hipStreamSynchronize(s));   // the host vectors above end with this function
+    w.nTriMesh = sc->n_trimesh;
+    return maxDepth;
+}
+
+}  // namespace pgx

@@ -645,6 +645,7 @@ __device__ bool collide(int t0, float4 g0, f3 p0, qf q0, int t1, float4 g1, f3 p
 }  // namespace pgx
 
 #include "np_convex.h"
+#include "np_trimesh.h"
 
 namespace pgx {
 
@@ -665,9 +666,11 @@ __global__ __launch_bounds__(256) void k_narrowphase(
     const float4* __restrict__ clquat, const uint32_t* __restrict__ centity, const float4* __restrict__ bpos,
     const float4* __restrict__ bquat, const uint64_t* __restrict__ nonColliding, int nNonColliding,
     int32_t* __restrict__ npNum, float4* __restrict__ npNormal, float4* __restrict__ npPts,
-    int32_t* __restrict__ heavy, int32_t* __restrict__ nHeavy) {
+    int32_t* __restrict__ heavy, int32_t* __restrict__ nHeavy, int32_t* __restrict__ meshPairs,
+    int32_t* __restrict__ nMesh, int32_t* __restrict__ meshCount) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nPairs) return;
+    meshCount[p] = 0;
     uint64_t key = pkeys[p];
     uint64_t lmask = (1ull << localBits) - 1;
     int wd = (int)(key >> (2 * localBits));
@@ -684,8 +687,11 @@ __global__ __launch_bounds__(256) void k_narrowphase(
     }
     Mf m;
     m.numPoints = 0;
-    if (!skip && (ctype[c0] == PGX_CONVEX_MESH || ctype[c1] == PGX_CONVEX_MESH)) {
-        heavy[atomicAdd(nHeavy, 1)] = p;   // solved by k_narrowphase_convex
+    if (!skip && (ctype[c0] == PGX_TRIANGLE_MESH || ctype[c1] == PGX_TRIANGLE_MESH)) {
+        meshPairs[atomicAdd(nMesh, 1)] = p;   // solved by k_narrowphase_mesh
+        skip = true;
+    } else if (!skip && (ctype[c0] == PGX_CONVEX_MESH || ctype[c1] == PGX_CONVEX_MESH)) {
+        heavy[atomicAdd(nHeavy, 1)] = p;      // solved by k_narrowphase_convex
         skip = true;
     }
     if (!skip) {
@@ -759,12 +765,72 @@ __global__ __launch_bounds__(64) void k_narrowphase_convex(
     }
 }
 
+// Pairs with a static triangle mesh: a fixed grid of `lanes` threads walks the list; lane
+// l owns mesh workspace l (and EPA workspace l for convex bodies). A pair's manifolds go to
+// a block of data slots after the pair slots, reserved with one atomic add.
+__global__ __launch_bounds__(64) void k_narrowphase_mesh(
+    const int32_t* __restrict__ nMeshP, const int32_t* __restrict__ meshPairs, const uint64_t* __restrict__ pkeys,
+    int localBits, const int32_t* __restrict__ worldColStart, const int32_t* __restrict__ colOfLocal,
+    const int32_t* __restrict__ cbody, const int32_t* __restrict__ ctype, const float4* __restrict__ cparams,
+    const float4* __restrict__ clpos, const float4* __restrict__ clquat, const float4* __restrict__ bpos,
+    const float4* __restrict__ bquat, ConvexView cv, TriMeshView tv, double* epaV, double* epaFN, int32_t* epaFI,
+    int32_t* epaE, int epaLanes, int32_t* tmWs, int lanes, int32_t* __restrict__ npNum,
+    float4* __restrict__ npNormal, float4* __restrict__ npPts, int32_t* __restrict__ npTri,
+    int32_t* __restrict__ meshFirst, int32_t* __restrict__ meshCount, int32_t* __restrict__ slotCounter, int mmCap,
+    int pairCap, int32_t* __restrict__ errors) {
+    int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= lanes) return;
+    int nMesh = *nMeshP;
+    EpaWs ews;
+    ews.vd = epaV + lane; ews.fn = epaFN + lane; ews.fi = epaFI + lane; ews.ed = epaE + lane;
+    ews.L = (size_t)epaLanes;
+    ews.overflow = false;
+    MeshWs ws{tmWs + lane, (size_t)lanes};
+    uint64_t lmask = (1ull << localBits) - 1;
+    for (int h = lane; h < nMesh; h += lanes) {
+        int p = meshPairs[h];
+        uint64_t key = pkeys[p];
+        int wd = (int)(key >> (2 * localBits));
+        int base = worldColStart[wd];
+        int c0 = colOfLocal[base + (int)((key >> localBits) & lmask)];
+        int c1 = colOfLocal[base + (int)(key & lmask)];
+        int t0 = ctype[c0], t1 = ctype[c1];
+        meshCount[p] = 0;
+        if (t0 == PGX_TRIANGLE_MESH && t1 == PGX_TRIANGLE_MESH) continue;   // no mesh-mesh routine
+        // src/Collision.cpp:897-906: a mesh on side 0 is solved as (other, mesh) and flipped
+        bool flip = t0 == PGX_TRIANGLE_MESH;
+        int cb = flip ? c1 : c0, cm = flip ? c0 : c1;
+        int bb = cbody[cb], bm = cbody[cm];
+        qf qb = q4(bquat[bb]), qm = q4(bquat[bm]);
+        f3 pos0 = add(xyz(bpos[bb]), qrot(qb, xyz(clpos[cb])));
+        qf or0 = qmul(qb, q4(clquat[cb]));
+        f3 pos1 = add(xyz(bpos[bm]), qrot(qm, xyz(clpos[cm])));
+        qf or1 = qmul(qm, q4(clquat[cm]));
+        int tb = ctype[cb];
+        float4 g = cparams[cb];
+        MeshCtx mc{&tv, tv.mesh[(int)cparams[cm].w]};
+        f3 lpos;
+        qf lor;
+        bool epaErr = false;
+        int nct = mesh_contacts(tb, g, pos0, or0, pos1, or1, mc, cv, ews, ws, lpos, lor, epaErr);
+        if (epaErr || nct < 0) atomicAdd(errors, 1);
+        if (nct <= 0) continue;
+        int slot = atomicAdd(slotCounter, nct);
+        if (slot + nct > mmCap) { atomicAdd(errors, 1); continue; }
+        MeshOut out{npNum, npNormal, npPts, npTri, pairCap + slot, slot, 0, flip};
+        mesh_emit(tb, g, lpos, lor, pos1, or1, mc, cv, ws, nct, out);
+        meshFirst[p] = slot;
+        meshCount[p] = out.n;
+    }
+}
+
 void np_run(World& w, int nPairs) {
     if (nPairs == 0) return;
     hipLaunchKernelGGL(k_narrowphase, dim3(div_up(nPairs, 256)), dim3(256), 0, w.stream, nPairs, w.pkeys.p,
                        w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.ctype.p, w.cflags.p,
                        w.cparams.p, w.clpos.p, w.clquat.p, w.centity.p, w.bpos.p, w.bquat.p, w.nonColliding.p,
-                       w.nNonColliding, w.npNum.p, w.npNormal.p, w.npPts.p, w.heavy.p, w.counters.p + 8);
+                       w.nNonColliding, w.npNum.p, w.npNormal.p, w.npPts.p, w.heavy.p, w.counters.p + 8,
+                       w.meshPairs.p, w.counters.p + 10, w.meshCount.p);
     PGX_LAUNCH_CHECK();
     if (w.nConvex > 0) {
         ConvexView cv{w.cvVerts.p, w.cvMesh.p, w.cvFace.p, w.cvFaceIdx.p, w.cvFaceN.p, w.cvFaceC.p};
@@ -773,6 +839,17 @@ void np_run(World& w, int nPairs) {
                            w.cbody.p, w.ctype.p, w.cparams.p, w.clpos.p, w.clquat.p, w.bpos.p, w.bquat.p, cv,
                            w.epaV.p, w.epaFN.p, w.epaFI.p, w.epaE.p, w.epaLanes, w.npNum.p, w.npNormal.p, w.npPts.p,
                            w.counters.p + 9);
+        PGX_LAUNCH_CHECK();
+    }
+    if (w.nTriMesh > 0) {
+        ConvexView cv{w.cvVerts.p, w.cvMesh.p, w.cvFace.p, w.cvFaceIdx.p, w.cvFaceN.p, w.cvFaceC.p};
+        TriMeshView tv{w.tmVerts.p, w.tmTris.p, w.tmTriN.p, w.tmTriC.p, w.tmNodes.p, w.tmMesh.p};
+        hipLaunchKernelGGL(k_narrowphase_mesh, dim3(div_up(w.meshLanes, 64)), dim3(64), 0, w.stream,
+                           w.counters.p + 10, w.meshPairs.p, w.pkeys.p, w.localBits, w.worldColStart.p,
+                           w.colOfLocal.p, w.cbody.p, w.ctype.p, w.cparams.p, w.clpos.p, w.clquat.p, w.bpos.p,
+                           w.bquat.p, cv, tv, w.epaV.p, w.epaFN.p, w.epaFI.p, w.epaE.p, w.epaLanes, w.tmWs.p,
+                           w.meshLanes, w.npNum.p, w.npNormal.p, w.npPts.p, w.npTri.p, w.meshFirst.p,
+                           w.meshCount.p, w.counters.p + 11, w.mmCap, w.pairCap, w.counters.p + 12);
         PGX_LAUNCH_CHECK();
     }
 }

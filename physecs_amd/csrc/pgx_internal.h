@@ -57,6 +57,12 @@ constexpr int EPA_MAXF = 256;          // > 2 * EPA_MAXV - 4, the face count of 
 constexpr int EPA_MAXE = 128;          // horizon (loose) edges of one insertion
 constexpr int EPA_LANES = 32768;
 constexpr int CVX_MAX_FACE_VERTS = 16;
+// triangle-mesh narrowphase: per-lane capacities and lanes
+constexpr int TM_MAXCAND = 256;        // triangles overlapping one body's local bounds
+constexpr int TM_STACK = 64;           // BVH traversal stack (the build's depth is checked)
+constexpr int TM_CI_WORDS = 14;        // one triangle contact record
+constexpr int TM_WS_WORDS = TM_MAXCAND * (2 + TM_CI_WORDS + 3);   // cand, order, contacts, voided
+constexpr int TM_LANES = 16384;
 inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
 // ------------------------------------------------------------------ utilities (util.hip)
@@ -170,6 +176,7 @@ struct World {
 
     // contact cache (canonical order, double-buffered)
     DBuf<uint64_t> cacheKey[2];
+    DBuf<int32_t> cacheTri[2];           // triangle index of the cached manifold (-1: none)
     DBuf<int32_t> cacheNum[2];
     DBuf<float4> cacheLp[2];             // 4 x M: local r0 + targetVelocity
     int cacheCur = 0;
@@ -189,11 +196,20 @@ struct World {
     int epaLanes = 0;                  // lanes of the convex kernel = EPA workspaces
     DBuf<double> epaV, epaFN;
     DBuf<int32_t> epaFI, epaE;
-    // static triangle meshes (vertices, {first vertex, count, first triangle, count})
-    int nTriMesh = 0;
-    DBuf<float4> tmVerts;
+    // static triangle meshes: vertices, triangles in the BVH build's order (mesh-local
+    // vertex indices), triangle normals / centroids, BVH nodes (2 float4 each: {min,
+    // triCount bits}, {max, index bits}), {first vertex, vertices, first triangle, first node}
+    int nTriMesh = 0, tmMaxDepth = 0;
+    DBuf<float4> tmVerts, tmTriN, tmTriC, tmNodes;
     DBuf<int4> tmMesh;
-    DBuf<int4> tmTris;                 // vertex indices (mesh-local) + pad
+    DBuf<int4> tmTris;
+    DBuf<int32_t> meshPairs;           // pairs with a triangle mesh (filled by the primitive kernel)
+    int meshLanes = 0;                 // lanes of the mesh kernel
+    DBuf<int32_t> tmWs;                // per-lane candidates, contacts, order, voided vertices
+    int mmCap = 0;                     // manifolds of mesh pairs, stored after the pair slots
+    int lastMeshSlots = 0;
+    DBuf<int32_t> npTri, meshFirst, meshCount;
+    DBuf<int32_t> mfSrc, mfTri;        // per manifold: narrowphase data slot, triangle index
 
     // joints (descriptor order = creation order), colours as Scene::addJoint assigns them
     int J = 0, jCap = 0, jRowsTotal = 0;
@@ -228,6 +244,7 @@ int bp_find_pairs(World& w);          // returns pair count (host)
 
 // narrowphase.hip
 void np_run(World& w, int nPairs);
+int upload_triangle_meshes(World& w, const pgx_scene_desc* sc, hipStream_t s);   // returns the BVH depth
 
 // solver.hip
 int sv_setup_manifolds(World& w, int nPairs);          // returns manifold count

@@ -23,9 +23,11 @@ namespace cg = cooperative_groups;
 namespace pgx {
 
 // ------------------------------------------------------------------ compaction
-__global__ void k_flag_manifolds(int nPairs, const int32_t* __restrict__ npNum, int32_t* __restrict__ flag) {
+// manifolds of pair p: one for a pair with points, meshCount[p] for a triangle-mesh pair
+__global__ void k_flag_manifolds(int nPairs, const int32_t* __restrict__ npNum, const int32_t* __restrict__ meshCount,
+                                 int32_t* __restrict__ flag) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p < nPairs) flag[p] = npNum[p] > 0 ? 1 : 0;
+    if (p < nPairs) flag[p] = (npNum[p] > 0 ? 1 : 0) + meshCount[p];
 }
 
 __device__ __forceinline__ void pair_colliders(uint64_t key, int localBits, const int32_t* __restrict__ worldColStart,
@@ -37,21 +39,37 @@ __device__ __forceinline__ void pair_colliders(uint64_t key, int localBits, cons
     c1 = colOfLocal[base + (int)(key & lmask)];
 }
 
+// Manifold list in pair order; a triangle-mesh pair contributes its manifolds in the order
+// collisionTriangleMesh emitted them. mfSrc is the narrowphase data slot (the pair's own
+// slot, or pairCap + the mesh manifold's slot), mfTri the triangle index (-1 otherwise).
 __global__ void k_compact(int nPairs, const int32_t* __restrict__ npNum, const int32_t* __restrict__ npIndex,
+                          const int32_t* __restrict__ meshFirst, const int32_t* __restrict__ meshCount,
+                          const int32_t* __restrict__ npTri, int pairCap,
                           const uint64_t* __restrict__ pkeys, int localBits, const int32_t* __restrict__ worldColStart,
                           const int32_t* __restrict__ colOfLocal, const int32_t* __restrict__ cbody,
                           const int32_t* __restrict__ cflags, int32_t* __restrict__ mfPair,
-                          uint64_t* __restrict__ mfKey, int32_t* __restrict__ mfB0, int32_t* __restrict__ mfB1) {
+                          uint64_t* __restrict__ mfKey, int32_t* __restrict__ mfB0, int32_t* __restrict__ mfB1,
+                          int32_t* __restrict__ mfSrc, int32_t* __restrict__ mfTri) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nPairs || npNum[p] == 0) return;
+    if (p >= nPairs) return;
+    int nm = meshCount[p];
+    if (npNum[p] == 0 && nm == 0) return;
     int m = npIndex[p];
     uint64_t key = pkeys[p];
     int c0, c1, wd;
     pair_colliders(key, localBits, worldColStart, colOfLocal, c0, c1, wd);
-    mfPair[m] = p;
-    mfKey[m] = key;
-    mfB0[m] = (cflags[c0] & CF_DYNAMIC) ? cbody[c0] : -1;
-    mfB1[m] = (cflags[c1] & CF_DYNAMIC) ? cbody[c1] : -1;
+    int b0 = (cflags[c0] & CF_DYNAMIC) ? cbody[c0] : -1;
+    int b1 = (cflags[c1] & CF_DYNAMIC) ? cbody[c1] : -1;
+    if (nm == 0) {
+        mfPair[m] = p; mfKey[m] = key; mfB0[m] = b0; mfB1[m] = b1; mfSrc[m] = p; mfTri[m] = -1;
+        return;
+    }
+    int first = meshFirst[p];
+    for (int j = 0; j < nm; ++j) {
+        mfPair[m + j] = p; mfKey[m + j] = key; mfB0[m + j] = b0; mfB1[m + j] = b1;
+        mfSrc[m + j] = pairCap + first + j;
+        mfTri[m + j] = npTri[first + j];
+    }
 }
 
 // ------------------------------------------------------------------ colouring
@@ -130,25 +148,26 @@ __global__ void k_color_start(int M, const uint64_t* __restrict__ keys, int32_t*
 }
 
 // ------------------------------------------------------------------ setup (solve order)
-__device__ __forceinline__ uint64_t cache_key(uint64_t pairKey) { return pairKey; }
 
 __global__ __launch_bounds__(256) void k_setup(
-    int M, int Mcap, const int32_t* __restrict__ perm, const int32_t* __restrict__ mfPair,
-    const uint64_t* __restrict__ mfKey, int localBits, const int32_t* __restrict__ worldColStart,
+    int M, int Mcap, const int32_t* __restrict__ perm, const int32_t* __restrict__ mfSrc,
+    const int32_t* __restrict__ mfTri, const uint64_t* __restrict__ mfKey, int localBits, const int32_t* __restrict__ worldColStart,
     const int32_t* __restrict__ colOfLocal, const int32_t* __restrict__ cbody, const int32_t* __restrict__ cflags,
     const float4* __restrict__ cmat, const int32_t* __restrict__ npNum, const float4* __restrict__ npNormal,
     const float4* __restrict__ npPts, const int32_t* __restrict__ mfColor, const int32_t* __restrict__ mfB0,
     const int32_t* __restrict__ mfB1, const float4* __restrict__ bpos, const float4* __restrict__ bquat,
     const float4* __restrict__ dvel, const float4* __restrict__ dang, const float4* __restrict__ dcom, int D,
-    const uint64_t* __restrict__ prevKey, const int32_t* __restrict__ prevNum, const float4* __restrict__ prevLp,
-    int prevCount, int prevCap, uint64_t* __restrict__ curKey, int32_t* __restrict__ curNum,
+    const uint64_t* __restrict__ prevKey, const int32_t* __restrict__ prevTri, const int32_t* __restrict__ prevNum,
+    const float4* __restrict__ prevLp, int prevCount, int prevCap, uint64_t* __restrict__ curKey,
+    int32_t* __restrict__ curTri, int32_t* __restrict__ curNum,
     float4* __restrict__ curLp, int32_t* __restrict__ sB0, int32_t* __restrict__ sB1, int32_t* __restrict__ sA0,
     int32_t* __restrict__ sA1, int32_t* __restrict__ sFlags, int32_t* __restrict__ sCanon, float4* __restrict__ sN,
     float2* __restrict__ sSoft, float4* __restrict__ sR0, float4* __restrict__ sR1, int32_t* __restrict__ colorPoints) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     int m = perm[s];
-    int p = mfPair[m];
+    int p = mfSrc[m];
+    int tri = mfTri[m];
     uint64_t key = mfKey[m];
     int c0, c1, wd;
     pair_colliders(key, localBits, worldColStart, colOfLocal, c0, c1, wd);
@@ -174,16 +193,17 @@ __global__ __launch_bounds__(256) void k_setup(
     f3 com0 = s3(0.f), v0 = s3(0.f), w0 = s3(0.f), com1 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
     if (b0 >= 0) { com0 = add(xyz(bpos[a0]), qrot(q0, xyz(dcom[b0]))); v0 = xyz(dvel[b0]); w0 = xyz(dang[b0]); }
     if (b1 >= 0) { com1 = add(xyz(bpos[a1]), qrot(q1, xyz(dcom[b1]))); v1 = xyz(dvel[b1]); w1 = xyz(dang[b1]); }
-    // previous step's manifold of this (pair, triangle), src/Physecs.cpp:237
+    // previous step's manifold of this (pair, triangle), src/Physecs.cpp:237: the cache is
+    // in manifold order, so a pair's manifolds are one run of equal keys
     int prev = -1;
     {
-        uint64_t ck = cache_key(key);
         int lo = 0, hi = prevCount;
         while (lo < hi) {
             int mid = (lo + hi) >> 1;
-            if (prevKey[mid] < ck) lo = mid + 1; else hi = mid;
+            if (prevKey[mid] < key) lo = mid + 1; else hi = mid;
         }
-        if (lo < prevCount && prevKey[lo] == ck) prev = lo;
+        for (; lo < prevCount && prevKey[lo] == key; ++lo)
+            if (prevTri[lo] == tri) { prev = lo; break; }
     }
     qf iq0 = qinverse(q0), iq1 = qinverse(q1);
     for (int k = 0; k < 4; ++k) {
@@ -214,7 +234,8 @@ __global__ __launch_bounds__(256) void k_setup(
         sR0[(size_t)k * Mcap + s] = r0o;
         sR1[(size_t)k * Mcap + s] = r1o;
     }
-    curKey[m] = cache_key(key);
+    curKey[m] = key;
+    curTri[m] = tri;
     curNum[m] = num;
     sB0[s] = b0;
     sB1[s] = b1;
@@ -524,18 +545,25 @@ int sv_setup_manifolds(World& w, int nPairs) {
     if (nPairs == 0) return 0;
     // manifold index of every pair with points: exclusive scan of the hit flags (the pair
     // sort's value buffer is free at this point and holds the flags)
-    hipLaunchKernelGGL(k_flag_manifolds, dim3(div_up(nPairs, 256)), dim3(256), 0, s, nPairs, w.npNum.p, w.pvals.p);
+    hipLaunchKernelGGL(k_flag_manifolds, dim3(div_up(nPairs, 256)), dim3(256), 0, s, nPairs, w.npNum.p,
+                       w.meshCount.p, w.pvals.p);
     exclusive_scan_i32(s, w.pvals.p, w.npIndex.p, nPairs, w.counters.p + 1, w.scanT);
     int M = 0, epaErrors = 0;
+    int meshInfo[3] = {0, 0, 0};   // mesh pairs, mesh manifold slots, mesh capacity errors
     PGX_HIP(hipMemcpyAsync(&M, w.counters.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
     if (w.nConvex) PGX_HIP(hipMemcpyAsync(&epaErrors, w.counters.p + 9, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (w.nTriMesh) PGX_HIP(hipMemcpyAsync(meshInfo, w.counters.p + 10, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     hipLaunchKernelGGL(k_compact, dim3(div_up(nPairs, 256)), dim3(256), 0, s, nPairs, w.npNum.p, w.npIndex.p,
-                       w.pkeys.p, w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.cflags.p,
-                       w.mfPair.p, w.mfKey.p, w.mfB0.p, w.mfB1.p);
+                       w.meshFirst.p, w.meshCount.p, w.npTri.p, w.pairCap, w.pkeys.p, w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.cflags.p,
+                       w.mfPair.p, w.mfKey.p, w.mfB0.p, w.mfB1.p, w.mfSrc.p, w.mfTri.p);
     PGX_LAUNCH_CHECK();
     PGX_HIP(hipStreamSynchronize(s));
     if (epaErrors)
         throw PgxError{PGX_ERR_CAPACITY, "EPA polytope capacity exceeded in " + std::to_string(epaErrors) + " pairs"};
+    if (meshInfo[2])
+        throw PgxError{PGX_ERR_CAPACITY, "triangle-mesh narrowphase capacity exceeded in " +
+                                             std::to_string(meshInfo[2]) + " pairs (candidates, stack or slots)"};
+    w.lastMeshSlots = meshInfo[1];
     return M;
 }
 
@@ -591,11 +619,13 @@ int sv_color(World& w, int M) {
     // contact setup, written in solve order
     int cur = w.cacheCur ^ 1, prv = w.cacheCur;
     fill_i32(s, w.colorPoints.p, 0, 160);
-    hipLaunchKernelGGL(k_setup, dim3(nb), dim3(256), 0, s, M, w.manifoldCap, w.colorPerm.p, w.mfPair.p, w.mfKey.p,
+    hipLaunchKernelGGL(k_setup, dim3(nb), dim3(256), 0, s, M, w.manifoldCap, w.colorPerm.p, w.mfSrc.p, w.mfTri.p,
+                       w.mfKey.p,
                        w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.cflags.p, w.cmat.p, w.npNum.p,
                        w.npNormal.p, w.npPts.p, w.mfColor.p, w.mfB0.p, w.mfB1.p, w.bpos.p, w.bquat.p, w.dvel.p,
-                       w.dang.p, w.dcom.p, w.D, w.cacheKey[prv].p, w.cacheNum[prv].p, w.cacheLp[prv].p,
-                       w.cacheCount[prv], w.manifoldCap, w.cacheKey[cur].p, w.cacheNum[cur].p, w.cacheLp[cur].p,
+                       w.dang.p, w.dcom.p, w.D, w.cacheKey[prv].p, w.cacheTri[prv].p, w.cacheNum[prv].p,
+                       w.cacheLp[prv].p, w.cacheCount[prv], w.manifoldCap, w.cacheKey[cur].p, w.cacheTri[cur].p,
+                       w.cacheNum[cur].p, w.cacheLp[cur].p,
                        w.sB0.p, w.sB1.p, w.sA0.p, w.sA1.p, w.sFlags.p, w.sCanon.p, w.sN.p, w.sSoft.p, w.sR0.p,
                        w.sR1.p, w.colorPoints.p);
     PGX_LAUNCH_CHECK();

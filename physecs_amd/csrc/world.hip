@@ -93,9 +93,21 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
             float mf = sc->col_params[4 * c + 3];
             if (!(mf >= 0.f) || mf != (float)(int)mf || (int)mf >= sc->n_convex)
                 return fail(PGX_ERR_INVALID, "collider %d: convex mesh index %g out of range", c, (double)mf);
+        } else if (t == PGX_TRIANGLE_MESH) {
+            float mf = sc->col_params[4 * c + 3];
+            if (!(mf >= 0.f) || mf != (float)(int)mf || (int)mf >= sc->n_trimesh)
+                return fail(PGX_ERR_INVALID, "collider %d: triangle mesh index %g out of range", c, (double)mf);
         } else if (t != PGX_SPHERE && t != PGX_CAPSULE && t != PGX_BOX) {
             return fail(PGX_ERR_UNSUPPORTED, "collider %d: geometry type %d is not supported on the device path", c, t);
         }
+    }
+    for (int m = 0; m < sc->n_trimesh; ++m) {
+        int nv = sc->tri_vert_start[m + 1] - sc->tri_vert_start[m];
+        int t0 = sc->tri_start[m], t1 = sc->tri_start[m + 1];
+        if (nv < 3 || t1 <= t0) return fail(PGX_ERR_INVALID, "triangle mesh %d: no vertices or triangles", m);
+        for (long k = 3L * t0; k < 3L * t1; ++k)
+            if (sc->tri_idx[k] < 0 || sc->tri_idx[k] >= nv)
+                return fail(PGX_ERR_INVALID, "triangle mesh %d: vertex index out of range", m);
     }
     for (int m = 0; m < sc->n_convex; ++m) {
         int nv = sc->convex_vert_start[m + 1] - sc->convex_vert_start[m];
@@ -218,18 +230,25 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
             w.nodeCounter.alloc(ni); w.nodeRange.alloc(ni); w.nodeMin.alloc(ni); w.nodeMax.alloc(ni);
             w.worldRoot.alloc(W); w.worldStop.alloc(W); w.worldMStart.alloc(W);
             w.pairCap = (cfg && cfg->pair_capacity > 0) ? cfg->pair_capacity : std::max(1024, 12 * C);
-            w.manifoldCap = w.pairCap;
+            // manifolds of triangle-mesh pairs are stored after the pair slots
+            w.mmCap = sc->n_trimesh > 0 ? std::max(65536, 16 * w.nDynCol) : 0;
+            w.manifoldCap = w.pairCap + w.mmCap;
             w.pkeys.alloc(w.pairCap); w.pvals.alloc(w.pairCap);
             w.counters.alloc(16);
-            w.npNum.alloc(w.pairCap); w.npNormal.alloc(w.pairCap); w.npPts.alloc(8 * (size_t)w.pairCap);
+            size_t ND = (size_t)w.pairCap + w.mmCap;
+            w.npNum.alloc(ND); w.npNormal.alloc(ND); w.npPts.alloc(8 * ND);
             w.npIndex.alloc(w.pairCap);
+            w.meshCount.alloc(w.pairCap); w.meshFirst.alloc(w.pairCap); w.npTri.alloc(std::max(1, w.mmCap));
             size_t M = w.manifoldCap;
             w.mfKey.alloc(M); w.mfPair.alloc(M); w.mfB0.alloc(M); w.mfB1.alloc(M); w.mfColor.alloc(M);
+            w.mfSrc.alloc(M); w.mfTri.alloc(M);
             w.claim.alloc(std::max(D, 1)); w.usedMask.alloc(2 * (size_t)std::max(D, 1));
             w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorCount.alloc(160); w.colorPoints.alloc(160);
             w.sB0.alloc(M); w.sB1.alloc(M); w.sFlags.alloc(M); w.sA0.alloc(M); w.sA1.alloc(M); w.sCanon.alloc(M);
             w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(36 * M);
-            for (int k = 0; k < 2; ++k) { w.cacheKey[k].alloc(M); w.cacheNum[k].alloc(M); w.cacheLp[k].alloc(4 * M); }
+            for (int k = 0; k < 2; ++k) {
+                w.cacheKey[k].alloc(M); w.cacheTri[k].alloc(M); w.cacheNum[k].alloc(M); w.cacheLp[k].alloc(4 * M);
+            }
             w.nonColliding.alloc(1);
             w.heavy.alloc(w.pairCap);
             if (sc->n_convex > 0) {
@@ -265,6 +284,21 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
                 size_t L = (size_t)w.epaLanes;
                 w.epaV.alloc(9 * EPA_MAXV * L); w.epaFN.alloc(3 * EPA_MAXF * L);
                 w.epaFI.alloc(3 * EPA_MAXF * L); w.epaE.alloc(2 * EPA_MAXE * L);
+            }
+            if (sc->n_trimesh > 0) {
+                w.tmMaxDepth = upload_triangle_meshes(w, sc, s);
+                if (w.tmMaxDepth + 2 > TM_STACK)
+                    throw PgxError{PGX_ERR_UNSUPPORTED, "triangle mesh BVH depth " + std::to_string(w.tmMaxDepth) +
+                                                            " exceeds the device traversal stack"};
+                w.meshPairs.alloc(w.pairCap);
+                w.meshLanes = TM_LANES;
+                w.tmWs.alloc((size_t)TM_WS_WORDS * w.meshLanes);
+                if (w.epaLanes < w.meshLanes) {   // convex bodies against triangles need EPA state per lane
+                    w.epaLanes = w.meshLanes;
+                    size_t L = (size_t)w.epaLanes;
+                    w.epaV.alloc(9 * EPA_MAXV * L); w.epaFN.alloc(3 * EPA_MAXF * L);
+                    w.epaFI.alloc(3 * EPA_MAXF * L); w.epaE.alloc(2 * EPA_MAXE * L);
+                }
             }
             // creation-time bounds: no margin (src/Physecs.cpp:32)
             bp_compute_bounds(w, false);
@@ -615,7 +649,7 @@ int pgx_world_get_manifolds(pgx_world* h, int32_t* keys, int32_t* ints, float* f
         *n = M;
         if (M == 0) return PGX_OK;
         size_t Mc = w.manifoldCap;
-        std::vector<int32_t> canon(M), b0(M), b1(M), flags(M), mfPair(M);
+        std::vector<int32_t> canon(M), b0(M), b1(M), flags(M), mfPair(M), mfSrc(M), mfTri(M);
         std::vector<uint64_t> mkey(M);
         std::vector<float4> sn(M), r0(4 * Mc), r1(4 * Mc);
         PGX_HIP(hipMemcpy(canon.data(), w.sCanon.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
@@ -623,13 +657,19 @@ int pgx_world_get_manifolds(pgx_world* h, int32_t* keys, int32_t* ints, float* f
         PGX_HIP(hipMemcpy(b1.data(), w.sB1.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
         PGX_HIP(hipMemcpy(flags.data(), w.sFlags.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
         PGX_HIP(hipMemcpy(mfPair.data(), w.mfPair.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(mfSrc.data(), w.mfSrc.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
+        PGX_HIP(hipMemcpy(mfTri.data(), w.mfTri.p, sizeof(int32_t) * M, hipMemcpyDeviceToHost));
         PGX_HIP(hipMemcpy(mkey.data(), w.mfKey.p, sizeof(uint64_t) * M, hipMemcpyDeviceToHost));
         PGX_HIP(hipMemcpy(sn.data(), w.sN.p, sizeof(float4) * M, hipMemcpyDeviceToHost));
         PGX_HIP(hipMemcpy(r0.data(), w.sR0.p, sizeof(float4) * 4 * Mc, hipMemcpyDeviceToHost));
         PGX_HIP(hipMemcpy(r1.data(), w.sR1.p, sizeof(float4) * 4 * Mc, hipMemcpyDeviceToHost));
-        std::vector<float4> pts(8 * (size_t)w.lastPairs);
+        // narrowphase points: the pair slots, then the triangle-mesh manifold slots
+        std::vector<float4> pts(8 * ((size_t)w.lastPairs + w.lastMeshSlots));
         if (w.lastPairs)
             PGX_HIP(hipMemcpy(pts.data(), w.npPts.p, sizeof(float4) * 8 * (size_t)w.lastPairs, hipMemcpyDeviceToHost));
+        if (w.lastMeshSlots)
+            PGX_HIP(hipMemcpy(pts.data() + 8 * (size_t)w.lastPairs, w.npPts.p + 8 * (size_t)w.pairCap,
+                              sizeof(float4) * 8 * (size_t)w.lastMeshSlots, hipMemcpyDeviceToHost));
         std::vector<int32_t> ws(w.W + 1), col(w.C);
         PGX_HIP(hipMemcpy(ws.data(), w.worldColStart.p, sizeof(int32_t) * (w.W + 1), hipMemcpyDeviceToHost));
         PGX_HIP(hipMemcpy(col.data(), w.colOfLocal.p, sizeof(int32_t) * w.C, hipMemcpyDeviceToHost));
@@ -646,7 +686,7 @@ int pgx_world_get_manifolds(pgx_world* h, int32_t* keys, int32_t* ints, float* f
                 keys[5 * s + 1] = w.hostColIndex[c0];
                 keys[5 * s + 2] = (int32_t)w.hostEntity[w.hostColBody[c1]];
                 keys[5 * s + 3] = w.hostColIndex[c1];
-                keys[5 * s + 4] = -1;
+                keys[5 * s + 4] = mfTri[m];
             }
             if (ints) {
                 ints[4 * s] = b0[s]; ints[4 * s + 1] = b1[s]; ints[4 * s + 2] = num; ints[4 * s + 3] = flags[s] >> 16;
@@ -655,7 +695,7 @@ int pgx_world_get_manifolds(pgx_world* h, int32_t* keys, int32_t* ints, float* f
                 float* o = floats + 56 * (size_t)s;
                 std::memset(o, 0, 56 * sizeof(float));
                 o[0] = sn[s].x; o[1] = sn[s].y; o[2] = sn[s].z; o[3] = sn[s].w;
-                int p = mfPair[m];
+                int p = mfSrc[m] < w.pairCap ? mfSrc[m] : w.lastPairs + (mfSrc[m] - w.pairCap);
                 for (int k2 = 0; k2 < num; ++k2) {
                     float4 a = pts[8 * (size_t)p + 2 * k2], b = pts[8 * (size_t)p + 2 * k2 + 1];
                     float* q = o + 4 + 6 * k2;

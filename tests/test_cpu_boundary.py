@@ -142,3 +142,49 @@ def test_bulk_bodies_match_single_appends():
     A, B = a.arrays(), b.arrays()
     for k in A:
         np.testing.assert_allclose(A[k], B[k], rtol=2e-7, err_msg=k)
+
+
+def test_device_introsort_matches_std_sort(tmp_path):
+    """physecs_amd/csrc/introsort.h repeats std::sort's exact sequence of compares and moves:
+    on arrays full of equal keys (where an unstable sort's output order is implementation
+    detail) it leaves the same permutation as this C++ library's std::sort, at every size
+    from the insertion-sort cutoff through the heap-sort fallback."""
+    import subprocess
+    src = tmp_path / "t.cpp"
+    src.write_text(r'''
+#include <algorithm>
+#include <cstdio>
+#include <random>
+#include <vector>
+#include "introsort.h"
+struct Arr { std::vector<int>& v; const std::vector<float>& key;
+  int get(int i) { return v[i]; } void set(int i, int x) { v[i] = x; }
+  bool less(int a, int b) { return key[a] < key[b]; } };
+struct E { float k; int id; bool operator<(const E& o) const { return k < o.k; } };
+int main() {
+  std::mt19937 rng(7);
+  for (int trial = 0; trial < 4000; ++trial) {
+    int n = 1 + rng() % 300;
+    int distinct = 1 + rng() % (trial % 3 == 0 ? 4 : 40);
+    std::vector<float> key(n);
+    for (auto& k : key) k = (float)(rng() % distinct);
+    if (trial % 7 == 0) std::sort(key.begin(), key.end());          // presorted runs
+    if (trial % 11 == 0) std::reverse(key.begin(), key.end());
+    std::vector<E> e(n);
+    for (int i = 0; i < n; ++i) e[i] = {key[i], i};
+    std::sort(e.begin(), e.end());
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    Arr a{idx, key};
+    pgx::introsort(a, n);
+    for (int i = 0; i < n; ++i) if (idx[i] != e[i].id) { printf("mismatch trial %d n %d at %d\n", trial, n, i); return 1; }
+  }
+  printf("ok\n");
+  return 0;
+}
+''')
+    exe = tmp_path / "t"
+    inc = os.path.join(ROOT, "physecs_amd", "csrc")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", inc, str(src), "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout

@@ -167,3 +167,33 @@ def test_convex_manifolds_first_steps(api, seed):
 def test_convex_lockstep_bitexact(api):
     log("[lockstep] convex pile 90 steps")
     lockstep(scenes.convex_pile(3, n=50, extent=2.0), 90, persistent=False)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_trimesh_manifolds_first_steps(api, seed):
+    """Spheres, capsules, boxes and convex hulls on a triangle-mesh terrain: the device BVH
+    query, per-triangle tests and face/edge/vertex voiding give the oracle's manifolds,
+    triangle indices included, bit for bit."""
+    log(f"[trimesh] terrain drop seed={seed}: manifolds")
+    sc = scenes.terrain_drop(seed, quads=12, n=40, heights=(0.0, 0.6))
+    g = api.World(sc, substeps=4)
+    o = ol.OracleWorld(sc, substeps=4)
+    ntri = 0
+    for k in range(20):
+        g.step(1 / 60)
+        keys, ints, f = g.manifolds()
+        o.set_order(keys)
+        o.step(1 / 60)
+        assert pair_set(g.pairs()) == pair_set(o.pairs()), k
+        gm = manifold_dict(keys, ints, f)
+        om = manifold_dict(*o.manifolds())
+        assert set(gm) == set(om), (k, len(set(gm) - set(om)), len(set(om) - set(gm)))
+        for key in gm:
+            assert np.array_equal(bits(gm[key][1]), bits(om[key][1])), (k, key)
+        ntri = max(ntri, int(np.sum(keys[:, 4] >= 0)))
+    assert ntri > 20
+
+
+def test_trimesh_lockstep_bitexact(api):
+    log("[lockstep] terrain drop 120 steps")
+    lockstep(scenes.terrain_drop(2, quads=12, n=40), 120, persistent=False)
