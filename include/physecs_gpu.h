@@ -25,6 +25,9 @@
  *   pgx_world_read_state / pgx_world_write_state
  *                           the in-place component writes that simulate performs
  *                           (Transform position/orientation, Dynamic velocity/angularVelocity)
+ *   pgx_server_*            no single reference counterpart: many independent Scenes, each
+ *                           simulate()d once per server step (include/Physecs/Physecs.h:199-204),
+ *                           batched into one device world with continuous admission
  */
 #ifndef PHYSECS_GPU_H
 #define PHYSECS_GPU_H
@@ -210,6 +213,45 @@ int pgx_world_reset_stats(pgx_world* w);
  * per colour and phase. blocks = grid size of the persistent launch (0 = one per CU). */
 int pgx_world_set_exec(pgx_world* w, int persistent, int blocks);
 int pgx_world_get_stats(pgx_world* w, pgx_stats* out);
+
+/* ---------------------------------------------------------------------------------
+ * Serving: a request loop that keeps many independent scenes ("requests") resident in one
+ * batched device world. pgx_server_submit queues a scene with a horizon (number of steps);
+ * every pgx_server_step first retires requests that reached their horizon and admits
+ * queued ones (first come, first served) while the HBM footprint of the admitted set
+ * stays within the budget, then steps every admitted request once. Requests admitted
+ * together, or while others are running, step in the same launches. Admission keeps
+ * the state of continuing requests (bodies, contact cache) on the device, so a request's
+ * result is the same whether it ran alone or in the mix. Scenes with convex or triangle
+ * meshes are not accepted by the server. */
+typedef struct pgx_server pgx_server;
+
+typedef struct pgx_server_config {
+    double hbm_budget_bytes;   /* footprint budget of the admitted set (0: 64 GiB) */
+    int max_worlds;            /* admitted requests at most (0: 8192) */
+} pgx_server_config;
+
+typedef struct pgx_server_step_info {
+    int active;                /* requests stepped by this call */
+    int admitted, retired;     /* at this call's step boundary */
+    int queued;                /* still waiting */
+    long long bodies;          /* dynamic bodies stepped */
+    double footprint_bytes;    /* estimated HBM footprint of the admitted set */
+} pgx_server_step_info;
+
+int pgx_server_create(const pgx_config* cfg, const pgx_server_config* scfg, pgx_server** out);
+int pgx_server_destroy(pgx_server* s);
+/* Queues a copy of the scene (and its joints); returns its request id. */
+int pgx_server_submit(pgx_server* s, const pgx_scene_desc* scene, const pgx_joint_desc* joints, int n_joints,
+                      int horizon_steps, int* request_id);
+int pgx_server_step(pgx_server* s, float dt, pgx_server_step_info* info);
+/* 1 when the request finished (its state is available), 0 while queued or running. */
+int pgx_server_done(pgx_server* s, int request_id);
+/* Final state of a finished request in its descriptor's order (pos/quat per actor,
+ * vel/angvel per dynamic body); releases it. */
+int pgx_server_take_result(pgx_server* s, int request_id, float* pos, float* quat, float* vel, float* angvel);
+/* Estimated HBM footprint of one request (bytes). */
+int pgx_server_footprint(const pgx_scene_desc* scene, double* bytes);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,8 @@ EXPORTS = [
     "pgx_world_write_state", "pgx_world_get_counts", "pgx_world_get_bounds", "pgx_world_get_pairs",
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints",
+    "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
+    "pgx_server_take_result", "pgx_server_footprint",
 ]
 
 
@@ -65,6 +67,15 @@ class PgxStats(ctypes.Structure):
                 ("solve_bytes", ctypes.c_double), ("step_us", ctypes.c_double)]
 
 
+class PgxServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_double), ("max_worlds", ctypes.c_int)]
+
+
+class PgxServerStepInfo(ctypes.Structure):
+    _fields_ = [("active", ctypes.c_int), ("admitted", ctypes.c_int), ("retired", ctypes.c_int),
+                ("queued", ctypes.c_int), ("bodies", ctypes.c_longlong), ("footprint_bytes", ctypes.c_double)]
+
+
 class PgxError(RuntimeError):
     pass
 
@@ -99,6 +110,14 @@ def lib():
         L.pgx_world_get_stats.argtypes = [vp, ctypes.POINTER(PgxStats)]
         L.pgx_world_set_exec.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.pgx_world_add_joints.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PgxJointDesc)]
+        L.pgx_server_create.argtypes = [ctypes.POINTER(PgxConfig), ctypes.POINTER(PgxServerConfig), ctypes.POINTER(vp)]
+        L.pgx_server_destroy.argtypes = [vp]
+        L.pgx_server_submit.argtypes = [vp, ctypes.POINTER(PgxSceneDesc), ctypes.POINTER(PgxJointDesc), ctypes.c_int,
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.pgx_server_step.argtypes = [vp, ctypes.c_float, ctypes.POINTER(PgxServerStepInfo)]
+        L.pgx_server_done.argtypes = [vp, ctypes.c_int]
+        L.pgx_server_take_result.argtypes = [vp, ctypes.c_int, F, F, F, F]
+        L.pgx_server_footprint.argtypes = [ctypes.POINTER(PgxSceneDesc), ctypes.POINTER(ctypes.c_double)]
         for name in EXPORTS:
             if name != "pgx_last_error":
                 getattr(L, name).restype = ctypes.c_int
@@ -221,3 +240,70 @@ class World:
         check(self.L.pgx_world_get_timing(self.h, *[ctypes.byref(x) for x in v]))
         return dict(zip(["broadphase_us", "narrowphase_us", "coloring_us", "solver_us", "total_us"],
                         [x.value for x in v]))
+
+
+def _one_world_desc(scene):
+    desc = scene.struct(PgxSceneDesc)
+    desc.actor_world = ctypes.cast(None, I32)
+    desc.n_worlds = 1
+    return desc
+
+
+class Server:
+    """pgx_server: continuous batching of independent scenes ("requests") in one device
+    world. submit() queues a scene with a horizon; every step() retires finished requests,
+    admits queued ones within the HBM footprint budget and steps all admitted requests once."""
+
+    def __init__(self, substeps=4, iterations=2, gravity=9.81, device=0, budget_bytes=0.0, max_worlds=0):
+        self.L = lib()
+        cfg = PgxConfig(substeps, iterations, gravity, device, 0, 0)
+        scfg = PgxServerConfig(budget_bytes, max_worlds)
+        h = ctypes.c_void_p()
+        check(self.L.pgx_server_create(ctypes.byref(cfg), ctypes.byref(scfg), ctypes.byref(h)))
+        self.h = h
+        self.scenes = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.pgx_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def footprint(scene):
+        b = ctypes.c_double(0)
+        desc = _one_world_desc(scene)
+        check(lib().pgx_server_footprint(ctypes.byref(desc), ctypes.byref(b)))
+        return b.value
+
+    def submit(self, scene, horizon):
+        scene.arrays()
+        desc = _one_world_desc(scene)
+        nj = len(scene.joints)
+        js = scene.joint_array(PgxJointDesc) if nj else None
+        rid = ctypes.c_int(0)
+        check(self.L.pgx_server_submit(self.h, ctypes.byref(desc), js, nj, int(horizon), ctypes.byref(rid)))
+        self.scenes[rid.value] = scene
+        return rid.value
+
+    def step(self, dt):
+        info = PgxServerStepInfo()
+        check(self.L.pgx_server_step(self.h, dt, ctypes.byref(info)))
+        return {k: getattr(info, k) for k, _ in PgxServerStepInfo._fields_}
+
+    def done(self, rid):
+        rc = self.L.pgx_server_done(self.h, rid)
+        if rc < 0:
+            check(rc)
+        return rc == 1
+
+    def take_result(self, rid):
+        s = self.scenes.pop(rid)
+        pos = np.zeros((s.n_actors, 3), np.float32)
+        quat = np.zeros((s.n_actors, 4), np.float32)
+        vel = np.zeros((s.n_dyn, 3), np.float32)
+        ang = np.zeros((s.n_dyn, 3), np.float32)
+        check(self.L.pgx_server_take_result(self.h, rid, _f(pos), _f(quat), _f(vel), _f(ang)))
+        return pos, quat, vel, ang

@@ -62,12 +62,14 @@ __global__ void k_bounds(int C, const int32_t* __restrict__ cbody, const int32_t
                          const int32_t* __restrict__ cflags, const float4* __restrict__ cparams,
                          const float4* __restrict__ clpos, const float4* __restrict__ clquat,
                          const float4* __restrict__ bpos, const float4* __restrict__ bquat, int dynamicOnly,
+                         const int32_t* __restrict__ cworld, const uint8_t* __restrict__ worldMask,
                          const float4* __restrict__ cvVerts, const int4* __restrict__ cvMesh,
                          const float4* __restrict__ tmVerts, const int4* __restrict__ tmMesh,
                          float4* __restrict__ bmin, float4* __restrict__ bmax) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     if (dynamicOnly && !(cflags[c] & CF_DYNAMIC)) return;
+    if (worldMask && !worldMask[cworld[c]]) return;
     f3 pos;
     qf q;
     collider_world_pose(bpos, bquat, clpos, clquat, cbody[c], c, pos, q);
@@ -81,11 +83,11 @@ __global__ void k_bounds(int C, const int32_t* __restrict__ cbody, const int32_t
     bmax[c] = f4(mx, 0.f);
 }
 
-void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin) {
+void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin, const uint8_t* worldMask) {
     if (w.C == 0) return;
     hipLaunchKernelGGL(k_bounds, dim3(div_up(w.C, 256)), dim3(256), 0, w.stream, w.C, w.cbody.p, w.ctype.p,
                        w.cflags.p, w.cparams.p, w.clpos.p, w.clquat.p, w.bpos.p, w.bquat.p,
-                       dynamicOnlyWithMargin ? 1 : 0, w.cvVerts.p, w.cvMesh.p, w.tmVerts.p, w.tmMesh.p, w.cbmin.p,
+                       dynamicOnlyWithMargin ? 1 : 0, w.cworld.p, worldMask, w.cvVerts.p, w.cvMesh.p, w.tmVerts.p, w.tmMesh.p, w.cbmin.p,
                        w.cbmax.p);
     PGX_LAUNCH_CHECK();
 }

@@ -239,12 +239,18 @@ struct World {
 };
 
 // broadphase.hip
-void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin);
+// bounds of every collider (creation: no margin) or, with the margin, of the dynamic
+// colliders; worldMask (device, one byte per world) limits the update to the marked worlds
+void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin, const uint8_t* worldMask = nullptr);
 int bp_find_pairs(World& w);          // returns pair count (host)
 
 // narrowphase.hip
 void np_run(World& w, int nPairs);
 int upload_triangle_meshes(World& w, const pgx_scene_desc* sc, hipStream_t s);   // returns the BVH depth
+void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc);      // world.hip
+void world_add_joints(World& w, int n, const pgx_joint_desc* js);
+void world_step(World& w, float dt);
+int report_error(int code, const std::string& msg);   // sets pgx_last_error(), returns code
 
 // solver.hip
 int sv_setup_manifolds(World& w, int nPairs);          // returns manifold count

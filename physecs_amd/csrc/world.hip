@@ -67,9 +67,10 @@ int pgx_device_count(int* count) {
     });
 }
 
-int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world** out) {
-    if (!sc || !out) return fail(PGX_ERR_INVALID, "scene or out is NULL");
-    *out = nullptr;
+}  // extern "C"
+
+static int validate_scene(const pgx_scene_desc* sc) {
+    if (!sc) return fail(PGX_ERR_INVALID, "scene is NULL");
     if (sc->n_actors < 0 || sc->n_dyn < 0 || sc->n_colliders < 0)
         return fail(PGX_ERR_INVALID, "negative counts");
     // validate the descriptor on the host before anything reaches the device
@@ -123,6 +124,210 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
                     return fail(PGX_ERR_INVALID, "convex mesh %d face %d: vertex index out of range", m, f - f0);
         }
     }
+    return PGX_OK;
+}
+
+namespace pgx {
+
+// (Re)initialises a world from a validated descriptor on w's device and stream. Device
+// buffers are reused where they are large enough; host state of a previous scene (joints,
+// non-colliding pairs, contact cache, statistics) is cleared.
+void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
+    w.J = 0; w.jCap = 0; w.jRowsTotal = 0;
+    for (auto* v : {&w.hjType, &w.hjA0, &w.hjA1, &w.hjB0, &w.hjB1, &w.hjRowFlags, &w.hjNRows, &w.hjColor}) v->clear();
+    for (auto* v : {&w.hjP0, &w.hjQ0, &w.hjP1, &w.hjQ1, &w.hjDrive}) v->clear();
+    w.hjBits.clear();
+    w.jColorStart.clear();
+    w.hostNonColliding.clear();
+    w.nonCollidingDirty = true;
+    w.cacheCur = 0; w.cacheCount[0] = w.cacheCount[1] = 0;
+    w.nConvex = 0; w.nTriMesh = 0; w.mmCap = 0; w.lastMeshSlots = 0;
+    w.lastPairs = w.lastManifolds = w.lastColors = w.lastRounds = w.lastPoints = 0;
+    int W = sc->actor_world ? std::max(1, sc->n_worlds) : 1;
+    if (cfg) {
+        if (cfg->num_substeps > 0) w.substeps = cfg->num_substeps;
+        if (cfg->num_iterations >= 0) w.iterations = cfg->num_iterations;
+        if (cfg->gravity != 0.f) w.g = cfg->gravity;
+    }
+    const int A = sc->n_actors, D = sc->n_dyn, C = sc->n_colliders;
+    w.A = A; w.D = D; w.C = C; w.W = W;
+    // internal body order: dynamic storage first (body b == dynamic index b), then the rest
+    w.actorToBody.assign(A, -1);
+    w.bodyToActor.clear();
+    for (int d = 0; d < D; ++d) { w.actorToBody[sc->actor_of_dyn[d]] = d; w.bodyToActor.push_back(sc->actor_of_dyn[d]); }
+    for (int a = 0; a < A; ++a)
+        if (sc->dyn_of_actor[a] < 0) { w.actorToBody[a] = (int)w.bodyToActor.size(); w.bodyToActor.push_back(a); }
+    std::vector<float4> bpos(A), bquat(A);
+    std::vector<uint32_t> bent(A);
+    for (int b = 0; b < A; ++b) {
+        int a = w.bodyToActor[b];
+        bpos[b] = make_float4(sc->pos[3 * a], sc->pos[3 * a + 1], sc->pos[3 * a + 2], 0.f);
+        bquat[b] = make_float4(sc->quat[4 * a], sc->quat[4 * a + 1], sc->quat[4 * a + 2], sc->quat[4 * a + 3]);
+        bent[b] = sc->entity[a];
+    }
+    std::vector<float4> dvel(D), dang(D), dcom(D), dinvI(3 * (size_t)D);
+    for (int d = 0; d < D; ++d) {
+        dvel[d] = make_float4(sc->vel[3 * d], sc->vel[3 * d + 1], sc->vel[3 * d + 2], sc->inv_mass[d]);
+        dang[d] = make_float4(sc->angvel[3 * d], sc->angvel[3 * d + 1], sc->angvel[3 * d + 2], 0.f);
+        dcom[d] = make_float4(sc->com[3 * d], sc->com[3 * d + 1], sc->com[3 * d + 2], sc->kinematic[d] ? 1.f : 0.f);
+        for (int k = 0; k < 3; ++k)
+            dinvI[3 * d + k] = make_float4(sc->inv_inertia[9 * d + 3 * k], sc->inv_inertia[9 * d + 3 * k + 1],
+                                           sc->inv_inertia[9 * d + 3 * k + 2], 0.f);
+    }
+    // colliders (descriptor order), world-local indices
+    std::vector<int32_t> cbody(C), cworld(C), clocal(C), cflags(C), ctype(C), cindex(C);
+    std::vector<uint32_t> cent(C);
+    std::vector<float4> cparams(C), clpos(C), clquat(C), cmat(C);
+    std::vector<int32_t> perWorld(W, 0);
+    for (int c = 0; c < C; ++c) {
+        int a = sc->col_actor[c];
+        int wd = sc->actor_world ? sc->actor_world[a] : 0;
+        cbody[c] = w.actorToBody[a];
+        cworld[c] = wd;
+        clocal[c] = perWorld[wd]++;
+        int d = sc->dyn_of_actor[a];
+        bool dyn = d >= 0 && !sc->kinematic[d];
+        cflags[c] = (sc->col_enable[c] ? CF_ENABLE : 0) | (sc->col_trigger[c] ? CF_TRIGGER : 0) | (dyn ? CF_DYNAMIC : 0);
+        ctype[c] = sc->col_type[c];
+        cindex[c] = sc->col_index[c];
+        cent[c] = sc->entity[a];
+        cparams[c] = make_float4(sc->col_params[4 * c], sc->col_params[4 * c + 1], sc->col_params[4 * c + 2],
+                                 sc->col_params[4 * c + 3]);
+        clpos[c] = make_float4(sc->col_pos[3 * c], sc->col_pos[3 * c + 1], sc->col_pos[3 * c + 2], 0.f);
+        clquat[c] = make_float4(sc->col_quat[4 * c], sc->col_quat[4 * c + 1], sc->col_quat[4 * c + 2],
+                                sc->col_quat[4 * c + 3]);
+        cmat[c] = make_float4(sc->col_material[3 * c], sc->col_material[3 * c + 1], sc->col_material[3 * c + 2], 0.f);
+    }
+    std::vector<int32_t> wstart(W + 1, 0), colOfLocal(C);
+    for (int i = 0; i < W; ++i) wstart[i + 1] = wstart[i] + perWorld[i];
+    for (int c = 0; c < C; ++c) colOfLocal[wstart[cworld[c]] + clocal[c]] = c;
+    // LBVH leaves: colliders of non-kinematic dynamic bodies; the rest per world
+    std::vector<int32_t> dynCols, staticCols, wss(W + 1, 0);
+    for (int c = 0; c < C; ++c)
+        if (cflags[c] & CF_DYNAMIC) dynCols.push_back(c); else ++wss[cworld[c] + 1];
+    for (int i = 0; i < W; ++i) wss[i + 1] += wss[i];
+    staticCols.resize(wss[W]);
+    {
+        std::vector<int32_t> fillp(wss.begin(), wss.end() - 1);
+        for (int c = 0; c < C; ++c)
+            if (!(cflags[c] & CF_DYNAMIC)) staticCols[fillp[cworld[c]]++] = c;
+    }
+    w.nDynCol = (int)dynCols.size();
+    int maxPerWorld = W ? *std::max_element(perWorld.begin(), perWorld.end()) : 1;
+    w.localBits = bits_for(std::max(2, maxPerWorld));
+    w.worldBits = W > 1 ? bits_for(W) : 0;
+    if (w.worldBits + 30 > 64 || w.worldBits + 2 * w.localBits > 64)
+        throw PgxError{PGX_ERR_INVALID, "too many worlds or colliders per world for 64-bit keys"};
+    w.hostColBody = cbody;
+    w.hostKinematic.assign(sc->kinematic, sc->kinematic + D);
+    w.hostColIndex = cindex;
+    w.hostEntity = bent;
+
+    hipStream_t s = w.stream;
+    upload(w.bpos, bpos, s); upload(w.bquat, bquat, s); upload(w.bentity, bent, s);
+    upload(w.dvel, dvel, s); upload(w.dang, dang, s); upload(w.dcom, dcom, s); upload(w.dinvI, dinvI, s);
+    upload(w.cbody, cbody, s); upload(w.cworld, cworld, s); upload(w.clocal, clocal, s);
+    upload(w.cflags, cflags, s); upload(w.ctype, ctype, s); upload(w.cindex, cindex, s);
+    upload(w.centity, cent, s); upload(w.cparams, cparams, s); upload(w.clpos, clpos, s);
+    upload(w.clquat, clquat, s); upload(w.cmat, cmat, s);
+    upload(w.worldColStart, wstart, s); upload(w.colOfLocal, colOfLocal, s);
+    upload(w.dynCols, dynCols, s); upload(w.staticCols, staticCols, s); upload(w.worldStaticStart, wss, s);
+    w.refitReady.alloc(std::max(1, w.nDynCol));
+    w.cvel.alloc(D); w.cang.alloc(D); w.tvel.alloc(D); w.tang.alloc(D); w.pvel.alloc(D); w.pang.alloc(D); w.winvI.alloc(3 * (size_t)D);
+    w.cbmin.alloc(C); w.cbmax.alloc(C);
+    // broadphase buffers
+    w.worldBoundsI.alloc(8);
+    w.mkeys.alloc(C); w.mvals.alloc(C);
+    w.leafMin.alloc(C); w.leafMax.alloc(C); w.leafFlags.alloc(C); w.leafParent.alloc(C); w.leafEsc.alloc(C);
+    size_t ni = C > 1 ? C - 1 : 1;
+    w.nodeChild.alloc(2 * ni); w.nodeParent.alloc(ni); w.nodeEsc.alloc(ni); w.nodeFlags.alloc(ni);
+    w.nodeCounter.alloc(ni); w.nodeRange.alloc(ni); w.nodeMin.alloc(ni); w.nodeMax.alloc(ni);
+    w.worldRoot.alloc(W); w.worldStop.alloc(W); w.worldMStart.alloc(W);
+    w.pairCap = (cfg && cfg->pair_capacity > 0) ? cfg->pair_capacity : std::max(1024, 12 * C);
+    // manifolds of triangle-mesh pairs are stored after the pair slots
+    w.mmCap = sc->n_trimesh > 0 ? std::max(65536, 16 * w.nDynCol) : 0;
+    w.manifoldCap = w.pairCap + w.mmCap;
+    w.pkeys.alloc(w.pairCap); w.pvals.alloc(w.pairCap);
+    w.counters.alloc(16);
+    size_t ND = (size_t)w.pairCap + w.mmCap;
+    w.npNum.alloc(ND); w.npNormal.alloc(ND); w.npPts.alloc(8 * ND);
+    w.npIndex.alloc(w.pairCap);
+    w.meshCount.alloc(w.pairCap); w.meshFirst.alloc(w.pairCap); w.npTri.alloc(std::max(1, w.mmCap));
+    size_t M = w.manifoldCap;
+    w.mfKey.alloc(M); w.mfPair.alloc(M); w.mfB0.alloc(M); w.mfB1.alloc(M); w.mfColor.alloc(M);
+    w.mfSrc.alloc(M); w.mfTri.alloc(M);
+    w.claim.alloc(std::max(D, 1)); w.usedMask.alloc(2 * (size_t)std::max(D, 1));
+    w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorCount.alloc(160); w.colorPoints.alloc(160);
+    w.sB0.alloc(M); w.sB1.alloc(M); w.sFlags.alloc(M); w.sA0.alloc(M); w.sA1.alloc(M); w.sCanon.alloc(M);
+    w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(36 * M);
+    for (int k = 0; k < 2; ++k) {
+        w.cacheKey[k].alloc(M); w.cacheTri[k].alloc(M); w.cacheNum[k].alloc(M); w.cacheLp[k].alloc(4 * M);
+    }
+    w.nonColliding.alloc(1);
+    w.heavy.alloc(w.pairCap);
+    if (sc->n_convex > 0) {
+        // ConvexMesh vertex buffers padded to a multiple of 4 with the last vertex
+        // (src/ConvexMesh.cpp:3-11); faces, normals and centroids as given
+        std::vector<float4> cvv, cvn, cvc;
+        std::vector<int4> cvm;
+        std::vector<int2> cvf;
+        std::vector<int32_t> cvfi;
+        for (int m = 0; m < sc->n_convex; ++m) {
+            int v0 = sc->convex_vert_start[m], nv = sc->convex_vert_start[m + 1] - v0;
+            int first = (int)cvv.size();
+            for (int i = 0; i < nv; ++i)
+                cvv.push_back(make_float4(sc->convex_verts[3 * (v0 + i)], sc->convex_verts[3 * (v0 + i) + 1],
+                                          sc->convex_verts[3 * (v0 + i) + 2], 0.f));
+            while ((cvv.size() - first) % 4) cvv.push_back(cvv.back());
+            int f0 = sc->convex_face_start[m], nf = sc->convex_face_start[m + 1] - f0;
+            cvm.push_back(make_int4(first, nv, (int)cvf.size(), nf));
+            for (int f = f0; f < f0 + nf; ++f) {
+                int k0 = sc->face_idx_start[f], k1 = sc->face_idx_start[f + 1];
+                cvf.push_back(make_int2((int)cvfi.size(), k1 - k0));
+                for (int k = k0; k < k1; ++k) cvfi.push_back(sc->face_idx[k]);
+                cvn.push_back(make_float4(sc->face_normal[3 * f], sc->face_normal[3 * f + 1],
+                                          sc->face_normal[3 * f + 2], 0.f));
+                cvc.push_back(make_float4(sc->face_centroid[3 * f], sc->face_centroid[3 * f + 1],
+                                          sc->face_centroid[3 * f + 2], 0.f));
+            }
+        }
+        w.nConvex = sc->n_convex;
+        upload(w.cvVerts, cvv, s); upload(w.cvMesh, cvm, s); upload(w.cvFace, cvf, s);
+        upload(w.cvFaceIdx, cvfi, s); upload(w.cvFaceN, cvn, s); upload(w.cvFaceC, cvc, s);
+        w.epaLanes = EPA_LANES;
+        size_t L = (size_t)w.epaLanes;
+        w.epaV.alloc(9 * EPA_MAXV * L); w.epaFN.alloc(3 * EPA_MAXF * L);
+        w.epaFI.alloc(3 * EPA_MAXF * L); w.epaE.alloc(2 * EPA_MAXE * L);
+    }
+    if (sc->n_trimesh > 0) {
+        w.tmMaxDepth = upload_triangle_meshes(w, sc, s);
+        if (w.tmMaxDepth + 2 > TM_STACK)
+            throw PgxError{PGX_ERR_UNSUPPORTED, "triangle mesh BVH depth " + std::to_string(w.tmMaxDepth) +
+                                                    " exceeds the device traversal stack"};
+        w.meshPairs.alloc(w.pairCap);
+        w.meshLanes = TM_LANES;
+        w.tmWs.alloc((size_t)TM_WS_WORDS * w.meshLanes);
+        if (w.epaLanes < w.meshLanes) {   // convex bodies against triangles need EPA state per lane
+            w.epaLanes = w.meshLanes;
+            size_t L = (size_t)w.epaLanes;
+            w.epaV.alloc(9 * EPA_MAXV * L); w.epaFN.alloc(3 * EPA_MAXF * L);
+            w.epaFI.alloc(3 * EPA_MAXF * L); w.epaE.alloc(2 * EPA_MAXE * L);
+        }
+    }
+    // creation-time bounds: no margin (src/Physecs.cpp:32)
+    bp_compute_bounds(w, false);
+    PGX_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace pgx
+
+extern "C" {
+
+int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world** out) {
+    if (!sc || !out) return fail(PGX_ERR_INVALID, "scene or out is NULL");
+    *out = nullptr;
+    int rc = validate_scene(sc);
+    if (rc != PGX_OK) return rc;
     return guarded([&] {
         auto* h = new pgx_world();
         World& w = h->w;
@@ -130,179 +335,7 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
             w.device = cfg ? cfg->device : 0;
             PGX_HIP(hipSetDevice(w.device));
             PGX_HIP(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));
-            if (cfg) {
-                if (cfg->num_substeps > 0) w.substeps = cfg->num_substeps;
-                if (cfg->num_iterations >= 0) w.iterations = cfg->num_iterations;
-                if (cfg->gravity != 0.f) w.g = cfg->gravity;
-            }
-            const int A = sc->n_actors, D = sc->n_dyn, C = sc->n_colliders;
-            w.A = A; w.D = D; w.C = C; w.W = W;
-            // internal body order: dynamic storage first (body b == dynamic index b), then the rest
-            w.actorToBody.assign(A, -1);
-            w.bodyToActor.clear();
-            for (int d = 0; d < D; ++d) { w.actorToBody[sc->actor_of_dyn[d]] = d; w.bodyToActor.push_back(sc->actor_of_dyn[d]); }
-            for (int a = 0; a < A; ++a)
-                if (sc->dyn_of_actor[a] < 0) { w.actorToBody[a] = (int)w.bodyToActor.size(); w.bodyToActor.push_back(a); }
-            std::vector<float4> bpos(A), bquat(A);
-            std::vector<uint32_t> bent(A);
-            for (int b = 0; b < A; ++b) {
-                int a = w.bodyToActor[b];
-                bpos[b] = make_float4(sc->pos[3 * a], sc->pos[3 * a + 1], sc->pos[3 * a + 2], 0.f);
-                bquat[b] = make_float4(sc->quat[4 * a], sc->quat[4 * a + 1], sc->quat[4 * a + 2], sc->quat[4 * a + 3]);
-                bent[b] = sc->entity[a];
-            }
-            std::vector<float4> dvel(D), dang(D), dcom(D), dinvI(3 * (size_t)D);
-            for (int d = 0; d < D; ++d) {
-                dvel[d] = make_float4(sc->vel[3 * d], sc->vel[3 * d + 1], sc->vel[3 * d + 2], sc->inv_mass[d]);
-                dang[d] = make_float4(sc->angvel[3 * d], sc->angvel[3 * d + 1], sc->angvel[3 * d + 2], 0.f);
-                dcom[d] = make_float4(sc->com[3 * d], sc->com[3 * d + 1], sc->com[3 * d + 2], sc->kinematic[d] ? 1.f : 0.f);
-                for (int k = 0; k < 3; ++k)
-                    dinvI[3 * d + k] = make_float4(sc->inv_inertia[9 * d + 3 * k], sc->inv_inertia[9 * d + 3 * k + 1],
-                                                   sc->inv_inertia[9 * d + 3 * k + 2], 0.f);
-            }
-            // colliders (descriptor order), world-local indices
-            std::vector<int32_t> cbody(C), cworld(C), clocal(C), cflags(C), ctype(C), cindex(C);
-            std::vector<uint32_t> cent(C);
-            std::vector<float4> cparams(C), clpos(C), clquat(C), cmat(C);
-            std::vector<int32_t> perWorld(W, 0);
-            for (int c = 0; c < C; ++c) {
-                int a = sc->col_actor[c];
-                int wd = sc->actor_world ? sc->actor_world[a] : 0;
-                cbody[c] = w.actorToBody[a];
-                cworld[c] = wd;
-                clocal[c] = perWorld[wd]++;
-                int d = sc->dyn_of_actor[a];
-                bool dyn = d >= 0 && !sc->kinematic[d];
-                cflags[c] = (sc->col_enable[c] ? CF_ENABLE : 0) | (sc->col_trigger[c] ? CF_TRIGGER : 0) | (dyn ? CF_DYNAMIC : 0);
-                ctype[c] = sc->col_type[c];
-                cindex[c] = sc->col_index[c];
-                cent[c] = sc->entity[a];
-                cparams[c] = make_float4(sc->col_params[4 * c], sc->col_params[4 * c + 1], sc->col_params[4 * c + 2],
-                                         sc->col_params[4 * c + 3]);
-                clpos[c] = make_float4(sc->col_pos[3 * c], sc->col_pos[3 * c + 1], sc->col_pos[3 * c + 2], 0.f);
-                clquat[c] = make_float4(sc->col_quat[4 * c], sc->col_quat[4 * c + 1], sc->col_quat[4 * c + 2],
-                                        sc->col_quat[4 * c + 3]);
-                cmat[c] = make_float4(sc->col_material[3 * c], sc->col_material[3 * c + 1], sc->col_material[3 * c + 2], 0.f);
-            }
-            std::vector<int32_t> wstart(W + 1, 0), colOfLocal(C);
-            for (int i = 0; i < W; ++i) wstart[i + 1] = wstart[i] + perWorld[i];
-            for (int c = 0; c < C; ++c) colOfLocal[wstart[cworld[c]] + clocal[c]] = c;
-            // LBVH leaves: colliders of non-kinematic dynamic bodies; the rest per world
-            std::vector<int32_t> dynCols, staticCols, wss(W + 1, 0);
-            for (int c = 0; c < C; ++c)
-                if (cflags[c] & CF_DYNAMIC) dynCols.push_back(c); else ++wss[cworld[c] + 1];
-            for (int i = 0; i < W; ++i) wss[i + 1] += wss[i];
-            staticCols.resize(wss[W]);
-            {
-                std::vector<int32_t> fillp(wss.begin(), wss.end() - 1);
-                for (int c = 0; c < C; ++c)
-                    if (!(cflags[c] & CF_DYNAMIC)) staticCols[fillp[cworld[c]]++] = c;
-            }
-            w.nDynCol = (int)dynCols.size();
-            int maxPerWorld = W ? *std::max_element(perWorld.begin(), perWorld.end()) : 1;
-            w.localBits = bits_for(std::max(2, maxPerWorld));
-            w.worldBits = W > 1 ? bits_for(W) : 0;
-            if (w.worldBits + 30 > 64 || w.worldBits + 2 * w.localBits > 64)
-                throw PgxError{PGX_ERR_INVALID, "too many worlds or colliders per world for 64-bit keys"};
-            w.hostColBody = cbody;
-            w.hostKinematic.assign(sc->kinematic, sc->kinematic + D);
-            w.hostColIndex = cindex;
-            w.hostEntity = bent;
-
-            hipStream_t s = w.stream;
-            upload(w.bpos, bpos, s); upload(w.bquat, bquat, s); upload(w.bentity, bent, s);
-            upload(w.dvel, dvel, s); upload(w.dang, dang, s); upload(w.dcom, dcom, s); upload(w.dinvI, dinvI, s);
-            upload(w.cbody, cbody, s); upload(w.cworld, cworld, s); upload(w.clocal, clocal, s);
-            upload(w.cflags, cflags, s); upload(w.ctype, ctype, s); upload(w.cindex, cindex, s);
-            upload(w.centity, cent, s); upload(w.cparams, cparams, s); upload(w.clpos, clpos, s);
-            upload(w.clquat, clquat, s); upload(w.cmat, cmat, s);
-            upload(w.worldColStart, wstart, s); upload(w.colOfLocal, colOfLocal, s);
-            upload(w.dynCols, dynCols, s); upload(w.staticCols, staticCols, s); upload(w.worldStaticStart, wss, s);
-            w.refitReady.alloc(std::max(1, w.nDynCol));
-            w.cvel.alloc(D); w.cang.alloc(D); w.tvel.alloc(D); w.tang.alloc(D); w.pvel.alloc(D); w.pang.alloc(D); w.winvI.alloc(3 * (size_t)D);
-            w.cbmin.alloc(C); w.cbmax.alloc(C);
-            // broadphase buffers
-            w.worldBoundsI.alloc(8);
-            w.mkeys.alloc(C); w.mvals.alloc(C);
-            w.leafMin.alloc(C); w.leafMax.alloc(C); w.leafFlags.alloc(C); w.leafParent.alloc(C); w.leafEsc.alloc(C);
-            size_t ni = C > 1 ? C - 1 : 1;
-            w.nodeChild.alloc(2 * ni); w.nodeParent.alloc(ni); w.nodeEsc.alloc(ni); w.nodeFlags.alloc(ni);
-            w.nodeCounter.alloc(ni); w.nodeRange.alloc(ni); w.nodeMin.alloc(ni); w.nodeMax.alloc(ni);
-            w.worldRoot.alloc(W); w.worldStop.alloc(W); w.worldMStart.alloc(W);
-            w.pairCap = (cfg && cfg->pair_capacity > 0) ? cfg->pair_capacity : std::max(1024, 12 * C);
-            // manifolds of triangle-mesh pairs are stored after the pair slots
-            w.mmCap = sc->n_trimesh > 0 ? std::max(65536, 16 * w.nDynCol) : 0;
-            w.manifoldCap = w.pairCap + w.mmCap;
-            w.pkeys.alloc(w.pairCap); w.pvals.alloc(w.pairCap);
-            w.counters.alloc(16);
-            size_t ND = (size_t)w.pairCap + w.mmCap;
-            w.npNum.alloc(ND); w.npNormal.alloc(ND); w.npPts.alloc(8 * ND);
-            w.npIndex.alloc(w.pairCap);
-            w.meshCount.alloc(w.pairCap); w.meshFirst.alloc(w.pairCap); w.npTri.alloc(std::max(1, w.mmCap));
-            size_t M = w.manifoldCap;
-            w.mfKey.alloc(M); w.mfPair.alloc(M); w.mfB0.alloc(M); w.mfB1.alloc(M); w.mfColor.alloc(M);
-            w.mfSrc.alloc(M); w.mfTri.alloc(M);
-            w.claim.alloc(std::max(D, 1)); w.usedMask.alloc(2 * (size_t)std::max(D, 1));
-            w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorCount.alloc(160); w.colorPoints.alloc(160);
-            w.sB0.alloc(M); w.sB1.alloc(M); w.sFlags.alloc(M); w.sA0.alloc(M); w.sA1.alloc(M); w.sCanon.alloc(M);
-            w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(36 * M);
-            for (int k = 0; k < 2; ++k) {
-                w.cacheKey[k].alloc(M); w.cacheTri[k].alloc(M); w.cacheNum[k].alloc(M); w.cacheLp[k].alloc(4 * M);
-            }
-            w.nonColliding.alloc(1);
-            w.heavy.alloc(w.pairCap);
-            if (sc->n_convex > 0) {
-                // ConvexMesh vertex buffers padded to a multiple of 4 with the last vertex
-                // (src/ConvexMesh.cpp:3-11); faces, normals and centroids as given
-                std::vector<float4> cvv, cvn, cvc;
-                std::vector<int4> cvm;
-                std::vector<int2> cvf;
-                std::vector<int32_t> cvfi;
-                for (int m = 0; m < sc->n_convex; ++m) {
-                    int v0 = sc->convex_vert_start[m], nv = sc->convex_vert_start[m + 1] - v0;
-                    int first = (int)cvv.size();
-                    for (int i = 0; i < nv; ++i)
-                        cvv.push_back(make_float4(sc->convex_verts[3 * (v0 + i)], sc->convex_verts[3 * (v0 + i) + 1],
-                                                  sc->convex_verts[3 * (v0 + i) + 2], 0.f));
-                    while ((cvv.size() - first) % 4) cvv.push_back(cvv.back());
-                    int f0 = sc->convex_face_start[m], nf = sc->convex_face_start[m + 1] - f0;
-                    cvm.push_back(make_int4(first, nv, (int)cvf.size(), nf));
-                    for (int f = f0; f < f0 + nf; ++f) {
-                        int k0 = sc->face_idx_start[f], k1 = sc->face_idx_start[f + 1];
-                        cvf.push_back(make_int2((int)cvfi.size(), k1 - k0));
-                        for (int k = k0; k < k1; ++k) cvfi.push_back(sc->face_idx[k]);
-                        cvn.push_back(make_float4(sc->face_normal[3 * f], sc->face_normal[3 * f + 1],
-                                                  sc->face_normal[3 * f + 2], 0.f));
-                        cvc.push_back(make_float4(sc->face_centroid[3 * f], sc->face_centroid[3 * f + 1],
-                                                  sc->face_centroid[3 * f + 2], 0.f));
-                    }
-                }
-                w.nConvex = sc->n_convex;
-                upload(w.cvVerts, cvv, s); upload(w.cvMesh, cvm, s); upload(w.cvFace, cvf, s);
-                upload(w.cvFaceIdx, cvfi, s); upload(w.cvFaceN, cvn, s); upload(w.cvFaceC, cvc, s);
-                w.epaLanes = EPA_LANES;
-                size_t L = (size_t)w.epaLanes;
-                w.epaV.alloc(9 * EPA_MAXV * L); w.epaFN.alloc(3 * EPA_MAXF * L);
-                w.epaFI.alloc(3 * EPA_MAXF * L); w.epaE.alloc(2 * EPA_MAXE * L);
-            }
-            if (sc->n_trimesh > 0) {
-                w.tmMaxDepth = upload_triangle_meshes(w, sc, s);
-                if (w.tmMaxDepth + 2 > TM_STACK)
-                    throw PgxError{PGX_ERR_UNSUPPORTED, "triangle mesh BVH depth " + std::to_string(w.tmMaxDepth) +
-                                                            " exceeds the device traversal stack"};
-                w.meshPairs.alloc(w.pairCap);
-                w.meshLanes = TM_LANES;
-                w.tmWs.alloc((size_t)TM_WS_WORDS * w.meshLanes);
-                if (w.epaLanes < w.meshLanes) {   // convex bodies against triangles need EPA state per lane
-                    w.epaLanes = w.meshLanes;
-                    size_t L = (size_t)w.epaLanes;
-                    w.epaV.alloc(9 * EPA_MAXV * L); w.epaFN.alloc(3 * EPA_MAXF * L);
-                    w.epaFI.alloc(3 * EPA_MAXF * L); w.epaE.alloc(2 * EPA_MAXE * L);
-                }
-            }
-            // creation-time bounds: no margin (src/Physecs.cpp:32)
-            bp_compute_bounds(w, false);
-            PGX_HIP(hipStreamSynchronize(s));
+            world_init(w, cfg, sc);
         } catch (...) {
             delete h;
             throw;
@@ -377,6 +410,71 @@ int pgx_world_set_can_collide(pgx_world* h, uint32_t e0, uint32_t e1, int canCol
     return PGX_OK;
 }
 
+}  // extern "C"
+
+namespace pgx {
+
+// Scene::createJoint for a batch of built-in joints (actor indices of the descriptor the
+// world was initialised from); colours as Scene::addJoint assigns them.
+void world_add_joints(World& w, int n, const pgx_joint_desc* js) {
+    if (w.hjBits.empty()) w.hjBits.assign(w.A, 0);
+    for (int i = 0; i < n; ++i) {
+        const pgx_joint_desc& d = js[i];
+        int a0 = w.actorToBody[d.actor0], a1 = w.actorToBody[d.actor1];
+        // Scene::addJoint (src/Physecs.cpp:690-710)
+        uint32_t e0 = w.hostEntity[a0], e1 = w.hostEntity[a1];
+        uint64_t k = ((uint64_t)std::min(e0, e1) << 32) | std::max(e0, e1);
+        auto it = std::lower_bound(w.hostNonColliding.begin(), w.hostNonColliding.end(), k);
+        if (it == w.hostNonColliding.end() || *it != k) w.hostNonColliding.insert(it, k);
+        w.nonCollidingDirty = true;
+        unsigned int inv = ~(unsigned int)(w.hjBits[a0] | w.hjBits[a1]);
+        int color = 0;
+        while (!((inv >> color) & 1u)) ++color;
+        w.hjBits[a0] = (unsigned char)(w.hjBits[a0] | (1u << color));
+        w.hjBits[a1] = (unsigned char)(w.hjBits[a1] | (1u << color));
+        // rows in creation order, which is also flag-type order for these types
+        int fl[6], nr = 0;
+        for (int r = 0; r < 3; ++r) fl[nr++] = 0;                 // point-to-point: NONE
+        if (d.type == PGX_JOINT_UNIVERSAL) fl[nr++] = 2;          // ANGULAR
+        if (d.type == PGX_JOINT_REVOLUTE) { fl[nr++] = 2; fl[nr++] = 2; if (d.drive_enabled) fl[nr++] = 2 | 4; }
+        if (d.type == PGX_JOINT_FIXED) for (int r = 0; r < 3; ++r) fl[nr++] = 2;
+        int packed = 0;
+        for (int r = 0; r < nr; ++r) packed |= fl[r] << (4 * r);
+        auto dyn = [&](int b) { return b < w.D && !w.hostKinematic[b] ? b : -1; };
+        w.hjType.push_back(d.type); w.hjA0.push_back(a0); w.hjA1.push_back(a1);
+        w.hjB0.push_back(dyn(a0)); w.hjB1.push_back(dyn(a1));
+        w.hjRowFlags.push_back(packed); w.hjNRows.push_back(nr); w.hjColor.push_back(color);
+        w.hjP0.push_back(make_float4(d.anchor0[0], d.anchor0[1], d.anchor0[2], 0.f));
+        w.hjQ0.push_back(make_float4(d.anchor0[3], d.anchor0[4], d.anchor0[5], d.anchor0[6]));
+        w.hjP1.push_back(make_float4(d.anchor1[0], d.anchor1[1], d.anchor1[2], 0.f));
+        w.hjQ1.push_back(make_float4(d.anchor1[3], d.anchor1[4], d.anchor1[5], d.anchor1[6]));
+        w.hjDrive.push_back(make_float4(d.drive_enabled ? 1.f : 0.f, d.drive_velocity, d.drive_max_torque, 0.f));
+        w.jRowsTotal += nr;
+    }
+    w.J = (int)w.hjType.size();
+    // joints grouped by colour, creation order inside a colour
+    std::vector<int32_t> order(w.J);
+    w.jColorStart.assign(10, 0);
+    for (int j = 0; j < w.J; ++j) ++w.jColorStart[w.hjColor[j] + 1];
+    for (int c = 0; c < 9; ++c) w.jColorStart[c + 1] += w.jColorStart[c];
+    std::vector<int> fillp(w.jColorStart.begin(), w.jColorStart.end() - 1);
+    for (int j = 0; j < w.J; ++j) order[fillp[w.hjColor[j]]++] = j;
+    hipStream_t s = w.stream;
+    w.jCap = w.J;
+    upload(w.jType, w.hjType, s); upload(w.jA0, w.hjA0, s); upload(w.jA1, w.hjA1, s);
+    upload(w.jB0, w.hjB0, s); upload(w.jB1, w.hjB1, s); upload(w.jRowFlags, w.hjRowFlags, s);
+    upload(w.jNRows, w.hjNRows, s); upload(w.jOrder, order, s);
+    upload(w.jP0, w.hjP0, s); upload(w.jQ0, w.hjQ0, s); upload(w.jP1, w.hjP1, s); upload(w.jQ1, w.hjQ1, s);
+    upload(w.jDrive, w.hjDrive, s);
+    w.jR0.alloc(w.J); w.jR1.alloc(w.J);
+    w.jRow.alloc((size_t)6 * 8 * w.J);
+    PGX_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace pgx
+
+extern "C" {
+
 int pgx_world_add_joints(pgx_world* h, int n, const pgx_joint_desc* js) {
     if (!h || (n > 0 && !js)) return fail(PGX_ERR_INVALID, "NULL argument");
     World& w = h->w;
@@ -389,127 +487,90 @@ int pgx_world_add_joints(pgx_world* h, int n, const pgx_joint_desc* js) {
     }
     return guarded([&] {
         PGX_HIP(hipSetDevice(w.device));
-        if (w.hjBits.empty()) w.hjBits.assign(w.A, 0);
-        for (int i = 0; i < n; ++i) {
-            const pgx_joint_desc& d = js[i];
-            int a0 = w.actorToBody[d.actor0], a1 = w.actorToBody[d.actor1];
-            // Scene::addJoint (src/Physecs.cpp:690-710)
-            uint32_t e0 = w.hostEntity[a0], e1 = w.hostEntity[a1];
-            uint64_t k = ((uint64_t)std::min(e0, e1) << 32) | std::max(e0, e1);
-            auto it = std::lower_bound(w.hostNonColliding.begin(), w.hostNonColliding.end(), k);
-            if (it == w.hostNonColliding.end() || *it != k) w.hostNonColliding.insert(it, k);
-            w.nonCollidingDirty = true;
-            unsigned int inv = ~(unsigned int)(w.hjBits[a0] | w.hjBits[a1]);
-            int color = 0;
-            while (!((inv >> color) & 1u)) ++color;
-            w.hjBits[a0] = (unsigned char)(w.hjBits[a0] | (1u << color));
-            w.hjBits[a1] = (unsigned char)(w.hjBits[a1] | (1u << color));
-            // rows in creation order, which is also flag-type order for these types
-            int fl[6], nr = 0;
-            for (int r = 0; r < 3; ++r) fl[nr++] = 0;                 // point-to-point: NONE
-            if (d.type == PGX_JOINT_UNIVERSAL) fl[nr++] = 2;          // ANGULAR
-            if (d.type == PGX_JOINT_REVOLUTE) { fl[nr++] = 2; fl[nr++] = 2; if (d.drive_enabled) fl[nr++] = 2 | 4; }
-            if (d.type == PGX_JOINT_FIXED) for (int r = 0; r < 3; ++r) fl[nr++] = 2;
-            int packed = 0;
-            for (int r = 0; r < nr; ++r) packed |= fl[r] << (4 * r);
-            auto dyn = [&](int b) { return b < w.D && !w.hostKinematic[b] ? b : -1; };
-            w.hjType.push_back(d.type); w.hjA0.push_back(a0); w.hjA1.push_back(a1);
-            w.hjB0.push_back(dyn(a0)); w.hjB1.push_back(dyn(a1));
-            w.hjRowFlags.push_back(packed); w.hjNRows.push_back(nr); w.hjColor.push_back(color);
-            w.hjP0.push_back(make_float4(d.anchor0[0], d.anchor0[1], d.anchor0[2], 0.f));
-            w.hjQ0.push_back(make_float4(d.anchor0[3], d.anchor0[4], d.anchor0[5], d.anchor0[6]));
-            w.hjP1.push_back(make_float4(d.anchor1[0], d.anchor1[1], d.anchor1[2], 0.f));
-            w.hjQ1.push_back(make_float4(d.anchor1[3], d.anchor1[4], d.anchor1[5], d.anchor1[6]));
-            w.hjDrive.push_back(make_float4(d.drive_enabled ? 1.f : 0.f, d.drive_velocity, d.drive_max_torque, 0.f));
-            w.jRowsTotal += nr;
-        }
-        w.J = (int)w.hjType.size();
-        // joints grouped by colour, creation order inside a colour
-        std::vector<int32_t> order(w.J);
-        w.jColorStart.assign(10, 0);
-        for (int j = 0; j < w.J; ++j) ++w.jColorStart[w.hjColor[j] + 1];
-        for (int c = 0; c < 9; ++c) w.jColorStart[c + 1] += w.jColorStart[c];
-        std::vector<int> fillp(w.jColorStart.begin(), w.jColorStart.end() - 1);
-        for (int j = 0; j < w.J; ++j) order[fillp[w.hjColor[j]]++] = j;
-        hipStream_t s = w.stream;
-        w.jCap = w.J;
-        upload(w.jType, w.hjType, s); upload(w.jA0, w.hjA0, s); upload(w.jA1, w.hjA1, s);
-        upload(w.jB0, w.hjB0, s); upload(w.jB1, w.hjB1, s); upload(w.jRowFlags, w.hjRowFlags, s);
-        upload(w.jNRows, w.hjNRows, s); upload(w.jOrder, order, s);
-        upload(w.jP0, w.hjP0, s); upload(w.jQ0, w.hjQ0, s); upload(w.jP1, w.hjP1, s); upload(w.jQ1, w.hjQ1, s);
-        upload(w.jDrive, w.hjDrive, s);
-        w.jR0.alloc(w.J); w.jR1.alloc(w.J);
-        w.jRow.alloc((size_t)6 * 8 * w.J);
-        PGX_HIP(hipStreamSynchronize(s));
+        world_add_joints(w, n, js);
         return PGX_OK;
     });
 }
 
+}  // extern "C"
+
+namespace pgx {
+
+// one Scene::simulate(dt) of every world in w (throws on error)
+void world_step(World& w, float dt) {
+    PGX_HIP(hipSetDevice(w.device));
+    hipStream_t s = w.stream;
+    if (w.nonCollidingDirty) {
+        w.nonColliding.alloc(std::max<size_t>(1, w.hostNonColliding.size()));
+        if (!w.hostNonColliding.empty())
+            PGX_HIP(hipMemcpyAsync(w.nonColliding.p, w.hostNonColliding.data(),
+                                   sizeof(uint64_t) * w.hostNonColliding.size(), hipMemcpyHostToDevice, s));
+        w.nNonColliding = (int)w.hostNonColliding.size();
+        w.nonCollidingDirty = false;
+    }
+    if (w.timing && !w.ev[0])
+        for (auto& e : w.ev) PGX_HIP(hipEventCreate(&e));
+    if (w.timing) PGX_HIP(hipEventRecord(w.ev[0], s));
+    int nPairs = bp_find_pairs(w);
+    if (w.timing) PGX_HIP(hipEventRecord(w.ev[1], s));
+    np_run(w, nPairs);
+    int M = sv_setup_manifolds(w, nPairs);
+    if (w.timing) PGX_HIP(hipEventRecord(w.ev[2], s));
+    int nColors = sv_color(w, M);
+    if (w.timing) PGX_HIP(hipEventRecord(w.ev[3], s));
+    sv_substeps(w, M, dt);
+    if (w.timing) PGX_HIP(hipEventRecord(w.ev[4], s));
+    sv_finish_step(w, M);
+    if (w.timing) {
+        PGX_HIP(hipEventRecord(w.ev[5], s));
+        PGX_HIP(hipEventSynchronize(w.ev[5]));
+        PGX_HIP(hipEventElapsedTime(&w.tBroad, w.ev[0], w.ev[1]));
+        PGX_HIP(hipEventElapsedTime(&w.tNarrow, w.ev[1], w.ev[2]));
+        PGX_HIP(hipEventElapsedTime(&w.tColor, w.ev[2], w.ev[3]));
+        PGX_HIP(hipEventElapsedTime(&w.tSolve, w.ev[3], w.ev[4]));
+        PGX_HIP(hipEventElapsedTime(&w.tTotal, w.ev[0], w.ev[5]));
+        // the substep-loop kernel: duration (HIP events on the world's stream) and the
+        // algorithmic bytes of this step's substep loop
+        std::vector<int32_t> pts(160, 0);
+        PGX_HIP(hipMemcpy(pts.data(), w.colorPoints.p, sizeof(int32_t) * 160, hipMemcpyDeviceToHost));
+        long points = 0;
+        for (int c = 0; c < nColors; ++c) points += pts[c];
+        w.lastPointsHost = points;
+        if (w.persistent && M > 0) {
+            float ms = 0.f;
+            PGX_HIP(hipEventElapsedTime(&ms, w.ev[6], w.ev[7]));
+            w.statSolveUs += 1000.0 * ms;
+            w.statSolveBytes += sv_step_bytes(w, M, points);
+            ++w.statSolveLaunches;
+        }
+        for (int j = 0; !w.persistent && j < w.solveEvUsed; ++j) {
+            // k_solve: 284 B per contact point + 12 B per manifold of the launched colour
+            float ms = 0.f;
+            PGX_HIP(hipEventElapsedTime(&ms, w.solveEv[2 * j], w.solveEv[2 * j + 1]));
+            int c = w.solveEvColor[j];
+            int nm = w.colorStartHost[c + 1] - w.colorStartHost[c];
+            w.statSolveUs += 1000.0 * ms;
+            w.statSolveBytes += 284.0 * pts[c] + 12.0 * nm;
+            ++w.statSolveLaunches;
+        }
+        w.statStepUs += 1000.0 * w.tTotal;
+        ++w.statSteps;
+    }
+    w.lastPairs = nPairs;
+    w.lastManifolds = M;
+    w.lastColors = nColors;
+}
+
+int report_error(int code, const std::string& msg) { return fail(code, "%s", msg.c_str()); }
+
+}  // namespace pgx
+
+extern "C" {
+
 int pgx_world_step(pgx_world* h, float dt) {
     if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
     return guarded([&] {
-        World& w = h->w;
-        PGX_HIP(hipSetDevice(w.device));
-        hipStream_t s = w.stream;
-        if (w.nonCollidingDirty) {
-            w.nonColliding.alloc(std::max<size_t>(1, w.hostNonColliding.size()));
-            if (!w.hostNonColliding.empty())
-                PGX_HIP(hipMemcpyAsync(w.nonColliding.p, w.hostNonColliding.data(),
-                                       sizeof(uint64_t) * w.hostNonColliding.size(), hipMemcpyHostToDevice, s));
-            w.nNonColliding = (int)w.hostNonColliding.size();
-            w.nonCollidingDirty = false;
-        }
-        if (w.timing && !w.ev[0])
-            for (auto& e : w.ev) PGX_HIP(hipEventCreate(&e));
-        if (w.timing) PGX_HIP(hipEventRecord(w.ev[0], s));
-        int nPairs = bp_find_pairs(w);
-        if (w.timing) PGX_HIP(hipEventRecord(w.ev[1], s));
-        np_run(w, nPairs);
-        int M = sv_setup_manifolds(w, nPairs);
-        if (w.timing) PGX_HIP(hipEventRecord(w.ev[2], s));
-        int nColors = sv_color(w, M);
-        if (w.timing) PGX_HIP(hipEventRecord(w.ev[3], s));
-        sv_substeps(w, M, dt);
-        if (w.timing) PGX_HIP(hipEventRecord(w.ev[4], s));
-        sv_finish_step(w, M);
-        if (w.timing) {
-            PGX_HIP(hipEventRecord(w.ev[5], s));
-            PGX_HIP(hipEventSynchronize(w.ev[5]));
-            PGX_HIP(hipEventElapsedTime(&w.tBroad, w.ev[0], w.ev[1]));
-            PGX_HIP(hipEventElapsedTime(&w.tNarrow, w.ev[1], w.ev[2]));
-            PGX_HIP(hipEventElapsedTime(&w.tColor, w.ev[2], w.ev[3]));
-            PGX_HIP(hipEventElapsedTime(&w.tSolve, w.ev[3], w.ev[4]));
-            PGX_HIP(hipEventElapsedTime(&w.tTotal, w.ev[0], w.ev[5]));
-            // the substep-loop kernel: duration (HIP events on the world's stream) and the
-            // algorithmic bytes of this step's substep loop
-            std::vector<int32_t> pts(160, 0);
-            PGX_HIP(hipMemcpy(pts.data(), w.colorPoints.p, sizeof(int32_t) * 160, hipMemcpyDeviceToHost));
-            long points = 0;
-            for (int c = 0; c < nColors; ++c) points += pts[c];
-            w.lastPointsHost = points;
-            if (w.persistent && M > 0) {
-                float ms = 0.f;
-                PGX_HIP(hipEventElapsedTime(&ms, w.ev[6], w.ev[7]));
-                w.statSolveUs += 1000.0 * ms;
-                w.statSolveBytes += sv_step_bytes(w, M, points);
-                ++w.statSolveLaunches;
-            }
-            for (int j = 0; !w.persistent && j < w.solveEvUsed; ++j) {
-                // k_solve: 284 B per contact point + 12 B per manifold of the launched colour
-                float ms = 0.f;
-                PGX_HIP(hipEventElapsedTime(&ms, w.solveEv[2 * j], w.solveEv[2 * j + 1]));
-                int c = w.solveEvColor[j];
-                int nm = w.colorStartHost[c + 1] - w.colorStartHost[c];
-                w.statSolveUs += 1000.0 * ms;
-                w.statSolveBytes += 284.0 * pts[c] + 12.0 * nm;
-                ++w.statSolveLaunches;
-            }
-            w.statStepUs += 1000.0 * w.tTotal;
-            ++w.statSteps;
-        }
-        w.lastPairs = nPairs;
-        w.lastManifolds = M;
-        w.lastColors = nColors;
+        world_step(h->w, dt);
         return PGX_OK;
     });
 }

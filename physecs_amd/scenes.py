@@ -119,23 +119,36 @@ def hull_volume(verts, faces):
     return vol
 
 
-def _inertia_tetrahedron(m, v):
-    """src/MassUtil.cpp:31-72 (about the origin)."""
-    d = np.zeros(3)
+def _fan(verts, faces, center):
+    """(v0, v1, v2) arrays of the fan triangulation of every face, in face order."""
+    a, b, c = [], [], []
+    for f in faces:
+        for i in range(1, len(f) - 1):
+            a.append(f[0]); b.append(f[i]); c.append(f[i + 1])
+    return verts[a], verts[b], verts[c]
+
+
+def _inertia_tetrahedra(m, v):
+    """src/MassUtil.cpp:31-72 for a batch: m (k,), v (k, 4, 3) -> (k, 3, 3) about the origin."""
+    d = np.zeros((len(m), 3))
     for i in range(4):
         for j in range(i + 1):
-            d += v[j] * v[i]
-    a, b, c = (d[1] + d[2]) / 10, (d[0] + d[2]) / 10, (d[0] + d[1]) / 10
-    ap = bp = cp = 0.0
+            d += v[:, j] * v[:, i]
+    a, b, c = (d[:, 1] + d[:, 2]) / 10, (d[:, 0] + d[:, 2]) / 10, (d[:, 0] + d[:, 1]) / 10
+    ap = np.zeros(len(m)); bp = np.zeros(len(m)); cp = np.zeros(len(m))
     for i in range(4):
         for j in range(4):
             k = 2 if i == j else 1
-            ap += k * v[i][1] * v[j][2]
-            bp += k * v[i][0] * v[j][2]
-            cp += k * v[i][0] * v[j][1]
+            ap += k * v[:, i, 1] * v[:, j, 2]
+            bp += k * v[:, i, 0] * v[:, j, 2]
+            cp += k * v[:, i, 0] * v[:, j, 1]
     ap, bp, cp = ap / 20, bp / 20, cp / 20
-    I = np.array([[a, -bp, -cp], [-bp, b, -ap], [-cp, -ap, c]])
-    return m * I
+    I = np.zeros((len(m), 3, 3))
+    I[:, 0, 0], I[:, 1, 1], I[:, 2, 2] = a, b, c
+    I[:, 0, 1] = I[:, 1, 0] = -bp
+    I[:, 0, 2] = I[:, 2, 0] = -cp
+    I[:, 1, 2] = I[:, 2, 1] = -ap
+    return m[:, None, None] * I
 
 
 def convex_mass_props(verts, faces, scale, mass):
@@ -147,23 +160,13 @@ def convex_mass_props(verts, faces, scale, mass):
     n = len(sv)
     padded = np.concatenate([sv, np.repeat(sv[-1:], (-n) % 4, axis=0)])
     center = padded.sum(axis=0) / n
-    com = np.zeros(3)
-    total = 0.0
-    tets = []
-    for f in faces:
-        v0 = sv[f[0]]
-        for i in range(1, len(f) - 1):
-            v1, v2 = sv[f[i]], sv[f[i + 1]]
-            vol = abs(np.dot(np.cross(v0 - center, v1 - center), v2 - center)) / 6.0
-            com += (center + v0 + v1 + v2) / 4.0 * vol
-            total += vol
-            tets.append((vol, v0, v1, v2))
-    com /= total
-    I = np.zeros((3, 3))
+    v0, v1, v2 = _fan(sv, faces, center)
+    vol = np.abs(np.einsum("ij,ij->i", np.cross(v0 - center, v1 - center), v2 - center)) / 6.0
+    total = vol.sum()
+    com = ((center + v0 + v1 + v2) / 4.0 * vol[:, None]).sum(axis=0) / total
     d = -com
-    for vol, v0, v1, v2 in tets:
-        m = mass * vol / total
-        I += _inertia_tetrahedron(m, [center + d, v0 + d, v1 + d, v2 + d])
+    tets = np.stack([np.broadcast_to(center + d, v0.shape), v0 + d, v1 + d, v2 + d], axis=1)
+    I = _inertia_tetrahedra(mass * vol / total, tets).sum(axis=0)
     inv = np.linalg.inv(I)
     return com.astype(np.float32), inv.T.astype(np.float32).ravel()
 
@@ -573,6 +576,39 @@ def batched_worlds(seed=0, n_worlds=4096, lo=8, hi=1024, spacing=1.5, ragdoll_fr
     return s
 
 
+def served_request(seed, n_bodies=None, chain=None, spacing=1.5):
+    """One serving request: a seeded scene of n_bodies (log-uniform in [16, 4096] when not
+    given) of the mixed-lattice shapes above its own static ground box; 10% of requests
+    (or chain=True) also hold an 8-capsule revolute chain."""
+    rng = np.random.default_rng(seed)
+    if n_bodies is None:
+        n_bodies = int(np.floor(np.exp(rng.uniform(math.log(16), math.log(4096)))))
+    if chain is None:
+        chain = rng.random() < 0.1
+    s = Scene()
+    n = n_bodies
+    side = int(math.ceil(n ** (1 / 3)))
+    half = (side - 1) * spacing / 2 + 1.0
+    s.add_static(BOX, [half + 6, 0.5, half + 6], [0, -0.5, 0])
+    kinds = rng.integers(0, 3, n)
+    prm = np.zeros((n, 4))
+    prm[kinds == SPHERE, 0] = rng.uniform(0.2, 0.5, int(np.sum(kinds == SPHERE)))
+    nc = int(np.sum(kinds == CAPSULE))
+    prm[kinds == CAPSULE, 1] = rng.uniform(0.15, 0.3, nc)
+    prm[kinds == CAPSULE, 0] = rng.uniform(0.2, 0.6, nc)
+    prm[kinds == BOX, :3] = rng.uniform(0.2, 0.6, (int(np.sum(kinds == BOX)), 3))
+    i = np.arange(n)
+    off = (side - 1) * spacing / 2
+    pos = np.stack([(i % side) * spacing - off, 1.0 + (i // (side * side)) * spacing,
+                    ((i // side) % side) * spacing - off], axis=1)
+    pos += rng.uniform(-0.05, 0.05, pos.shape)
+    s.add_bodies_bulk(kinds, prm, pos, random_quats(rng, n), rng.normal(size=(n, 3)), np.zeros(n, int))
+    if chain:
+        top = 1.0 + (n // (side * side) + 1) * spacing + 1.0
+        capsule_chain(s, [-3.2, top, 0.0], n=8, axis="x")
+    return s
+
+
 def capsule_chain(s, origin, n=8, radius=0.1, half_height=0.3, axis="x", world=0, material=(0.6, 0.0, 0.0),
                   drive=None):
     """n capsules end to end, joined by n-1 revolute joints at their touching ends. The
@@ -696,6 +732,36 @@ def terrain_drop(seed=0, quads=12, n=40, convex_frac=0.4, amplitude=2.0, heights
             prm = {SPHERE: [rng.uniform(0.2, 0.4)], CAPSULE: [rng.uniform(0.2, 0.4), rng.uniform(0.1, 0.2)],
                    BOX: list(rng.uniform(0.15, 0.4, 3))}[k]
             s.add_body(k, prm, [x, y, z], q, vel=v)
+    return s
+
+
+def convex_terrain(seed=0, n_convex=8192, n_chains=512, quads=256, amplitude=8.0, drop=(2.0, 40.0)):
+    """8,192 convex bodies (each the hull of 24 points on an ellipsoid with semi-axes
+    U(0.3, 0.8)) and 512 chains of 8 capsules (r 0.1, half-height 0.3, 7 revolute joints)
+    dropped from U(2, 40) m above a static quads x quads fBm terrain (two triangles per
+    quad, 1 m spacing, heights 8 * fBm of value noise, 5 octaves)."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    verts, tris, h = terrain_mesh(seed, quads=quads, amplitude=amplitude)
+    tm = s.add_triangle_mesh(verts, tris)
+    a = s.add_actor([0, 0, 0])
+    s.add_collider(a, TRIANGLE_MESH, [0, 0, 0, float(tm)])
+    half = quads / 2 - 4.0
+
+    def ground(x, z):
+        i = int(np.clip(round(x + quads / 2), 0, quads))
+        j = int(np.clip(round(z + quads / 2), 0, quads))
+        return float(h[max(0, i - 1):i + 2, max(0, j - 1):j + 2].max())
+
+    for _ in range(n_convex):
+        mesh = s.add_convex_mesh(*ellipsoid_hull(rng))
+        x, z = rng.uniform(-half, half, 2)
+        y = ground(x, z) + rng.uniform(*drop)
+        s.add_convex_body(mesh, [x, y, z], random_quats(rng, 1)[0])
+    for _ in range(n_chains):
+        x, z = rng.uniform(-half, half - 7.0, 2)
+        top = max(ground(x + k, z) for k in range(0, 8)) + rng.uniform(*drop)
+        capsule_chain(s, [x, top, z], n=8, radius=0.1, half_height=0.3, axis="x")
     return s
 
 
