@@ -1,6 +1,6 @@
 """Benchmark of the rigid-body step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain32k|pyramid|batched]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain32k|pyramid|batched|convex|served]
 
 A step is one Scene::simulate(1/60) with 4 substeps and 2 iterations over the whole scene,
 inputs resident in HBM. The default workload is the 32,768-body mixed scene (1/3 spheres,
@@ -42,9 +42,15 @@ def make_scene(name):
         return scenes.box_pyramid(0), {"workload": "20-layer box pyramid (210 boxes)", "bodies": 210,
                                        "substeps": 4, "iterations": 2, "dt": 1 / 60}
     if name == "batched":
-        s = scenes.batched_worlds(0, n_worlds=4096)
-        return s, {"workload": "4096 uneven worlds (8..1024 bodies, log-uniform)", "bodies": s.n_dyn,
-                   "substeps": 4, "iterations": 2, "dt": 1 / 60}
+        s = scenes.batched_worlds(0, n_worlds=4096, ragdoll_frac=0.25)
+        return s, {"workload": "4096 uneven worlds (8..1024 bodies, log-uniform; 25% with a 15-capsule ragdoll)",
+                   "bodies": s.n_dyn, "worlds": 4096, "joints": len(s.joints), "substeps": 4, "iterations": 2,
+                   "dt": 1 / 60}
+    if name == "convex":
+        s = scenes.convex_terrain(0)
+        return s, {"workload": "8192 convex hulls + 512 revolute capsule chains on a 131072-triangle fBm terrain",
+                   "bodies": s.n_dyn, "joints": len(s.joints), "triangles": 131072, "substeps": 4,
+                   "iterations": 2, "dt": 1 / 60}
     raise SystemExit(f"unknown config {name}")
 
 
@@ -75,6 +81,74 @@ def cpu_baseline(scene, state, substeps, iterations, dt, budget_s=15.0):
                                       f"{el:.1f} s; narrowphase on {threads} workers + caller, solver single-threaded"}
 
 
+def run_served(args):
+    """Poisson request stream (2000 req/s) of seeded scenes with 16..4096 bodies (log-uniform,
+    10% with an 8-capsule revolute chain) and horizons U(60, 600) steps, served by one
+    pgx_server; the request loop runs in wall-clock time for --seconds. A request is the
+    scene of one of 64 pre-built seeds (scene building is host work outside the loop)."""
+    from physecs_amd import api, scenes
+    rng = np.random.default_rng(0)
+    pool = [scenes.served_request(1000 + k) for k in range(64)]
+    for sc in pool:
+        sc.arrays()
+    rate, seconds = 2000.0, args.seconds
+    n_req = int(rate * seconds * 1.2) + 16
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    horizons = rng.integers(60, 601, n_req)
+    picks = rng.integers(0, len(pool), n_req)
+    budget = 16e9
+    srv = api.Server(substeps=4, iterations=2, budget_bytes=budget, max_worlds=4096, admit_interval=args.admit_interval)
+    # warm the device path with a first batch outside the timed window
+    warm = [srv.submit(pool[k], 2) for k in range(8)]
+    for _ in range(4):
+        srv.step(1 / 60)
+    for r in warm:
+        srv.take_result(r)
+    lat, advanced, outstanding = [], 0, []
+    nxt = 0
+    t0 = time.perf_counter()
+    steps = 0
+    completed = 0
+    active_sum = 0
+    while True:
+        now = time.perf_counter() - t0
+        if now >= seconds:
+            break
+        while nxt < n_req and arrivals[nxt] <= now and len(outstanding) < 20000:
+            outstanding.append(srv.submit(pool[picks[nxt]], int(horizons[nxt])))
+            nxt += 1
+        ts = time.perf_counter()
+        info = srv.step(1 / 60)
+        lat.append(time.perf_counter() - ts)
+        advanced += info["bodies"]
+        active_sum += info["active"]
+        steps += 1
+        if steps % 32 == 0:
+            keep = []
+            for r in outstanding:
+                if srv.done(r):
+                    srv.take_result(r)
+                    completed += 1
+                else:
+                    keep.append(r)
+            outstanding = keep
+    el = time.perf_counter() - t0
+    srv.close()
+    bss = advanced * 4 / el
+    line = {
+        "metric": "body-substeps/s (served: Poisson 2000 req/s, 16..4096 bodies, 1 GPU)", "value": bss,
+        "unit": "body-substeps/s", "n_gpus": 1, "steps": steps, "warmup": 4, "ms_per_step": 1000 * el / max(1, steps),
+        "p50_step_ms": 1000 * float(np.percentile(lat, 50)), "p99_step_ms": 1000 * float(np.percentile(lat, 99)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded request scenes)",
+        "config": {"workload": "served stream", "seconds": seconds, "requests_arrived": int(nxt),
+                   "requests_completed": completed, "mean_active_requests": active_sum / max(1, steps),
+                   "hbm_budget_bytes": budget, "admit_interval": args.admit_interval, "substeps": 4,
+                   "iterations": 2, "dt": 1 / 60, "parallelism": "single GPU"},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,10 +158,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks", type=int, default=0, help="persistent grid size (0 = one block per CU)")
     ap.add_argument("--persistent", action="store_true", help="substep loop as one cooperative launch per step")
+    ap.add_argument("--seconds", type=float, default=20.0, help="served: length of the request stream")
+    ap.add_argument("--admit-interval", type=int, default=8, help="served: steps between admission boundaries")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.config == "served":
+        run_served(args)
+        return
     from physecs_amd import api
     scene, cfg = make_scene(args.config)
     sub, it, dt = cfg["substeps"], cfg["iterations"], cfg["dt"]

@@ -217,25 +217,28 @@ int pgx_world_get_stats(pgx_world* w, pgx_stats* out);
 /* ---------------------------------------------------------------------------------
  * Serving: a request loop that keeps many independent scenes ("requests") resident in one
  * batched device world. pgx_server_submit queues a scene with a horizon (number of steps);
- * every pgx_server_step first retires requests that reached their horizon and admits
- * queued ones (first come, first served) while the HBM footprint of the admitted set
- * stays within the budget, then steps every admitted request once. Requests admitted
- * together, or while others are running, step in the same launches. Admission keeps
- * the state of continuing requests (bodies, contact cache) on the device, so a request's
- * result is the same whether it ran alone or in the mix. Scenes with convex or triangle
- * meshes are not accepted by the server. */
+ * pgx_server_step steps every admitted request once, in the same launches. A request's
+ * state is copied out right after its last step. At most every `admit_interval` steps
+ * (and whenever nothing is running) the server retires finished requests and admits
+ * queued ones, first come first served, while the HBM footprint of the admitted set stays
+ * within the budget; until then a finished request keeps riding in the batch, unobserved.
+ * Admission keeps the state of continuing requests (bodies, contact cache) on the device,
+ * so a request's result is the same whether it ran alone or in the mix. Scenes with convex
+ * or triangle meshes are not accepted by the server. */
 typedef struct pgx_server pgx_server;
 
 typedef struct pgx_server_config {
     double hbm_budget_bytes;   /* footprint budget of the admitted set (0: 64 GiB) */
     int max_worlds;            /* admitted requests at most (0: 8192) */
+    int admit_interval;        /* steps between admission boundaries (0: 1) */
 } pgx_server_config;
 
 typedef struct pgx_server_step_info {
-    int active;                /* requests stepped by this call */
-    int admitted, retired;     /* at this call's step boundary */
+    int active;                /* requests advanced by this call (before their horizon) */
+    int admitted, retired;     /* at this call's admission boundary */
+    int finished;              /* requests whose last step this call was */
     int queued;                /* still waiting */
-    long long bodies;          /* dynamic bodies stepped */
+    long long bodies;          /* dynamic bodies of the advanced requests */
     double footprint_bytes;    /* estimated HBM footprint of the admitted set */
 } pgx_server_step_info;
 

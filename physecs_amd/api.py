@@ -68,12 +68,13 @@ class PgxStats(ctypes.Structure):
 
 
 class PgxServerConfig(ctypes.Structure):
-    _fields_ = [("hbm_budget_bytes", ctypes.c_double), ("max_worlds", ctypes.c_int)]
+    _fields_ = [("hbm_budget_bytes", ctypes.c_double), ("max_worlds", ctypes.c_int), ("admit_interval", ctypes.c_int)]
 
 
 class PgxServerStepInfo(ctypes.Structure):
     _fields_ = [("active", ctypes.c_int), ("admitted", ctypes.c_int), ("retired", ctypes.c_int),
-                ("queued", ctypes.c_int), ("bodies", ctypes.c_longlong), ("footprint_bytes", ctypes.c_double)]
+                ("finished", ctypes.c_int), ("queued", ctypes.c_int), ("bodies", ctypes.c_longlong),
+                ("footprint_bytes", ctypes.c_double)]
 
 
 class PgxError(RuntimeError):
@@ -254,10 +255,11 @@ class Server:
     world. submit() queues a scene with a horizon; every step() retires finished requests,
     admits queued ones within the HBM footprint budget and steps all admitted requests once."""
 
-    def __init__(self, substeps=4, iterations=2, gravity=9.81, device=0, budget_bytes=0.0, max_worlds=0):
+    def __init__(self, substeps=4, iterations=2, gravity=9.81, device=0, budget_bytes=0.0, max_worlds=0,
+                 admit_interval=1):
         self.L = lib()
         cfg = PgxConfig(substeps, iterations, gravity, device, 0, 0)
-        scfg = PgxServerConfig(budget_bytes, max_worlds)
+        scfg = PgxServerConfig(budget_bytes, max_worlds, admit_interval)
         h = ctypes.c_void_p()
         check(self.L.pgx_server_create(ctypes.byref(cfg), ctypes.byref(scfg), ctypes.byref(h)))
         self.h = h

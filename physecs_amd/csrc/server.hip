@@ -46,7 +46,7 @@ struct Request {
     uint32_t entityBase = 0;
     int horizon = 0, stepsDone = 0;
     double footprint = 0;
-    int state = 0;                      // 0 queued, 1 active, 2 done
+    int state = 0;                      // 0 queued, 1 active, 2 done (may still ride in the batch)
     int dynStart = 0;                   // placement in the current batched world
     std::vector<float4> rpos, rquat, rvel, rang;   // final dynamic state (done)
 };
@@ -111,6 +111,8 @@ struct pgx_server {
     pgx_config cfg{};
     double budget = 64.0 * (1ull << 30);
     int maxWorlds = 8192;
+    int admitInterval = 1;
+    int sinceBoundary = 0;
     int nextId = 1;
     uint32_t nextEntity = 1;
     std::deque<int> queue;
@@ -344,6 +346,7 @@ int pgx_server_create(const pgx_config* cfg, const pgx_server_config* scfg, pgx_
             else { S->cfg.num_substeps = 4; S->cfg.num_iterations = 2; S->cfg.gravity = 9.81f; }
             if (scfg && scfg->hbm_budget_bytes > 0) S->budget = scfg->hbm_budget_bytes;
             if (scfg && scfg->max_worlds > 0) S->maxWorlds = scfg->max_worlds;
+            if (scfg && scfg->admit_interval > 0) S->admitInterval = scfg->admit_interval;
             S->w.device = S->cfg.device;
             PGX_HIP(hipSetDevice(S->w.device));
             PGX_HIP(hipStreamCreateWithFlags(&S->w.stream, hipStreamNonBlocking));
@@ -425,57 +428,68 @@ int pgx_server_step(pgx_server* S, float dt, pgx_server_step_info* info) {
     return srv_guarded([&] {
         World& w = S->w;
         PGX_HIP(hipSetDevice(w.device));
-        // retire requests that reached their horizon
-        std::vector<int> keep;
-        std::vector<std::pair<int, int>> continuing;   // (old world, request)
-        int retired = 0;
-        for (size_t i = 0; i < S->active.size(); ++i) {
-            Request& r = S->reqs[S->active[i]];
-            if (r.stepsDone >= r.horizon) {
-                read_result(*S, r);
-                r.state = 2;
-                S->footprint -= r.footprint;
-                ++retired;
-            } else {
+        int admitted = 0, retired = 0, finished = 0;
+        // admission boundary: every admit_interval steps, or whenever nothing is running
+        bool anyRunning = false;
+        for (int id : S->active) anyRunning |= S->reqs[id].state == 1;
+        if (S->sinceBoundary >= S->admitInterval || !anyRunning) {
+            std::vector<int> keep;
+            std::vector<std::pair<int, int>> continuing;   // (old world, request)
+            for (size_t i = 0; i < S->active.size(); ++i) {
+                Request& r = S->reqs[S->active[i]];
+                if (r.state == 2) {
+                    S->footprint -= r.footprint;
+                    ++retired;
+                } else {
+                    keep.push_back(r.id);
+                    continuing.push_back({(int)i, r.id});
+                }
+            }
+            while (!S->queue.empty() && (int)keep.size() < S->maxWorlds) {
+                Request& r = S->reqs[S->queue.front()];
+                if (!keep.empty() && S->footprint + r.footprint > S->budget) break;
+                S->queue.pop_front();
+                r.state = 1;
+                S->footprint += r.footprint;
                 keep.push_back(r.id);
-                continuing.push_back({(int)i, r.id});
+                ++admitted;
             }
-        }
-        if (retired) PGX_HIP(hipStreamSynchronize(w.stream));
-        // admit in arrival order while the budget allows
-        int admitted = 0;
-        while (!S->queue.empty() && (int)keep.size() < S->maxWorlds) {
-            Request& r = S->reqs[S->queue.front()];
-            if (!keep.empty() && S->footprint + r.footprint > S->budget) break;
-            S->queue.pop_front();
-            r.state = 1;
-            S->footprint += r.footprint;
-            keep.push_back(r.id);
-            ++admitted;
-        }
-        int oldW = (int)S->active.size();
-        int oldLocalBits = w.localBits;
-        S->active.swap(keep);
-        if (retired || admitted) {
-            if (S->active.empty()) S->initialised = false;
-            else {
-                rebuild(*S, continuing, oldW, oldLocalBits);
-                S->initialised = true;
+            int oldW = (int)S->active.size();
+            int oldLocalBits = w.localBits;
+            S->active.swap(keep);
+            if (retired || admitted) {
+                if (S->active.empty()) S->initialised = false;
+                else {
+                    rebuild(*S, continuing, oldW, oldLocalBits);
+                    S->initialised = true;
+                }
             }
+            S->sinceBoundary = 0;
         }
         long long bodies = 0;
+        int advanced = 0;
         if (S->initialised) {
             world_step(w, dt);
+            ++S->sinceBoundary;
             for (int id : S->active) {
                 Request& r = S->reqs[id];
+                if (r.state != 1) continue;
                 ++r.stepsDone;
+                ++advanced;
                 bodies += r.sc.nD;
+                if (r.stepsDone == r.horizon) {   // state right after the last step
+                    read_result(*S, r);
+                    r.state = 2;
+                    ++finished;
+                }
             }
+            if (finished) PGX_HIP(hipStreamSynchronize(w.stream));
         }
         if (info) {
-            info->active = (int)S->active.size();
+            info->active = advanced;
             info->admitted = admitted;
             info->retired = retired;
+            info->finished = finished;
             info->queued = (int)S->queue.size();
             info->bodies = bodies;
             info->footprint_bytes = S->footprint;

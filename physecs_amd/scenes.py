@@ -711,7 +711,8 @@ def terrain_mesh(seed=0, quads=256, spacing=1.0, amplitude=8.0, octaves=5, lacun
 
 
 def terrain_drop(seed=0, quads=12, n=40, convex_frac=0.4, amplitude=2.0, heights=(0.5, 3.0)):
-    """Bodies of every shape dropped onto a small static triangle-mesh terrain."""
+    """Bodies of every shape dropped onto a small static triangle-mesh terrain, from
+    heights U(heights) above the terrain under them."""
     rng = np.random.default_rng(seed)
     s = Scene()
     verts, tris, h = terrain_mesh(seed, quads=quads, amplitude=amplitude)
@@ -722,7 +723,8 @@ def terrain_drop(seed=0, quads=12, n=40, convex_frac=0.4, amplitude=2.0, heights
     half = quads / 2 - 1.5
     for _ in range(n):
         x, z = rng.uniform(-half, half, 2)
-        y = float(np.max(h)) + rng.uniform(*heights)
+        i, j = int(round(x + quads / 2)), int(round(z + quads / 2))
+        y = float(h[max(0, i - 1):i + 2, max(0, j - 1):j + 2].max()) + 0.6 + rng.uniform(*heights)
         q = random_quats(rng, 1)[0]
         v = rng.normal(size=3) * 0.3
         if rng.random() < convex_frac:

@@ -175,11 +175,11 @@ def test_trimesh_manifolds_first_steps(api, seed):
     query, per-triangle tests and face/edge/vertex voiding give the oracle's manifolds,
     triangle indices included, bit for bit."""
     log(f"[trimesh] terrain drop seed={seed}: manifolds")
-    sc = scenes.terrain_drop(seed, quads=12, n=40, heights=(0.0, 0.6))
+    sc = scenes.terrain_drop(seed, quads=12, n=40, heights=(0.0, 0.4))
     g = api.World(sc, substeps=4)
     o = ol.OracleWorld(sc, substeps=4)
     ntri = 0
-    for k in range(20):
+    for k in range(40):
         g.step(1 / 60)
         keys, ints, f = g.manifolds()
         o.set_order(keys)

@@ -25,12 +25,13 @@ def alone(api, scene, steps):
     return w.state()
 
 
-def test_served_results_same_alone_and_in_mix(api):
+@pytest.mark.parametrize("interval", [1, 4])
+def test_served_results_same_alone_and_in_mix(api, interval):
     plan = [  # (submit at server step, seed, bodies, chain, horizon)
         (0, 11, 40, False, 30), (0, 12, 120, True, 24), (5, 13, 16, False, 20),
         (5, 14, 300, False, 12), (9, 15, 64, True, 25), (20, 16, 200, False, 10),
     ]
-    srv = api.Server(substeps=4, iterations=2)
+    srv = api.Server(substeps=4, iterations=2, admit_interval=interval)
     ids = {}
     step = 0
     pending = list(plan)
@@ -40,7 +41,7 @@ def test_served_results_same_alone_and_in_mix(api):
             ids[item] = srv.submit(sc, item[4])
             pending.remove(item)
         info = srv.step(1 / 60)
-        log(f"[serving] step {step}: {info}")
+        log(f"[serving] interval {interval} step {step}: {info}")
         step += 1
         assert step < 200
     for item, rid in ids.items():
