@@ -1,6 +1,7 @@
 """Benchmark of the rigid-body step on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain32k|pyramid|batched|convex|served]
+    python bench.py --copy-calibration      # a plain 1 GiB device copy, for scaling PMC byte counters
 
 A step is one Scene::simulate(1/60) with 4 substeps and 2 iterations over the whole scene,
 inputs resident in HBM. The default workload is the 32,768-body mixed scene (1/3 spheres,
@@ -11,9 +12,12 @@ roofline: the dominant kernel is k_solve, the contact solve of one graph colour 
 per colour and pass: numIterations biased passes + 1 relaxation, every substep). Its
 algorithmic bytes per launch are 284 B per contact point + 12 B per manifold of the colour
 (rows 108 + scalars 24 + both bodies' solver and component v,w 96 read; lambdas 8 + both
-bodies' v,w 48 written). Its duration is measured with HIP events around each launch of the
+bodies' v,w 48 written). Its duration comes from HIP start/stop events that
+hipExtLaunchKernelGGL stamps from each launch's own dispatch packet, for every launch of the
 first substep of every timed step, on the world's stream. (--persistent times the
 single-launch substep kernel k_step_persistent instead, with the per-substep phase model.)
+traffic: HBM bytes per k_solve launch from the committed PMC summary (profiles/), default
+workload only.
 
 cpu_baseline: the CPU oracle (a restatement of the reference's Scene::simulate with the
 reference's ThreadPool narrowphase chunking, nproc-1 workers + caller) on the same scene,
@@ -31,6 +35,34 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_solve_plain32k_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per k_solve launch on the default workload, from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same counter's reading for a
+    copy of known size); None when the summary is absent."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def copy_calibration():
+    """Ten device-to-device copies of 1 GiB (reads 1 GiB, writes 1 GiB each): the reference
+    reading for FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(10):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": True, "bytes_per_copy": 4 * n, "copies": 10}), flush=True)
 
 
 def make_scene(name):
@@ -160,10 +192,14 @@ def main():
     ap.add_argument("--persistent", action="store_true", help="substep loop as one cooperative launch per step")
     ap.add_argument("--seconds", type=float, default=20.0, help="served: length of the request stream")
     ap.add_argument("--admit-interval", type=int, default=8, help="served: steps between admission boundaries")
+    ap.add_argument("--copy-calibration", action="store_true", help="run only the 1 GiB copy used to scale PMC counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.config == "served":
         run_served(args)
         return
@@ -182,11 +218,13 @@ def main():
     lat = []
     w.sync()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.steps):
         ts = time.perf_counter()
         w.step(dt)
         w.sync()
         lat.append(time.perf_counter() - ts)
+        if (k + 1) % 100 == 0:
+            print(f"timed step {k + 1}/{args.steps}", file=sys.stderr, flush=True)
     w.sync()
     el = time.perf_counter() - t0
     st = w.stats()
@@ -202,7 +240,8 @@ def main():
         "config": dict(cfg, parallelism="single GPU", last_step_counts=counts,
                        phase_us_last_step=w.timing()),
         "roofline": {"bound": "hbm", "kernel": "k_step_persistent" if args.persistent else "k_solve", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if args.config == "plain32k" and not args.persistent else None,
                      "avg_launch_us": st["solve_us"] / max(1, st["solve_launches"]),
                      "bytes_per_launch": st["solve_bytes"] / max(1, st["solve_launches"]),
                      "launches_timed": st["solve_launches"]},

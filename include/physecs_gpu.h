@@ -22,6 +22,9 @@
  *   pgx_world_set_can_collide  Scene::setCanCollide (include/Physecs/Physecs.h:223)
  *   pgx_world_add_joints    Scene::createJoint<T> for the built-in joint types
  *                           (include/Physecs/Physecs.h:209-214, src/Physecs.cpp:690-710)
+ *   pgx_world_set_joint_params  the joints' setters (include/Physecs/Joints/*.h)
+ *   pgx_world_destroy_joint Scene::destroyJoint (include/Physecs/Physecs.h:215,
+ *                           src/Physecs.cpp:712-725)
  *   pgx_world_read_state / pgx_world_write_state
  *                           the in-place component writes that simulate performs
  *                           (Transform position/orientation, Dynamic velocity/angularVelocity)
@@ -127,17 +130,30 @@ typedef struct pgx_scene_desc {
 #define PGX_JOINT_UNIVERSAL 1
 #define PGX_JOINT_REVOLUTE 2
 #define PGX_JOINT_FIXED 3
+#define PGX_JOINT_PRISMATIC 4
+#define PGX_JOINT_GEAR 5
+#define PGX_JOINT_SERVO 6
 
 /* One Scene::createJoint<T>(entity0, anchor0Pos, anchor0Or, entity1, anchor1Pos, anchor1Or).
- * Anchors are in body space: position xyz, orientation xyzw. drive_* apply to revolute
- * joints (RevoluteJoint::setDriveEnabled / setDriveVelocity / setDriveMaxTorque). */
+ * Anchors are in body space: position xyz, orientation xyzw. The other fields are the
+ * joint's data members, as its setters set them; a field a type does not have is ignored.
+ * Reference defaults: PrismaticJoint.h:8-17 (upper 1, lower 0, stiffness 5, damping 1),
+ * ServoJoint.h:7-11 (target 0, stiffness 30, damping 1), GearJoint.h:7-14 (ratio 1),
+ * RevoluteJoint.h:8-12 (drive off, 0, 0). */
 typedef struct pgx_joint_desc {
     int32_t type;
     int32_t actor0, actor1;   /* descriptor actor indices */
     float anchor0[7];
     float anchor1[7];
-    int32_t drive_enabled;
-    float drive_velocity, drive_max_torque;
+    int32_t drive_enabled;    /* revolute, prismatic: setDriveEnabled */
+    float drive_velocity;     /* revolute: setDriveVelocity */
+    float drive_max_torque;   /* revolute: setDriveMaxTorque */
+    float lower_limit;        /* prismatic: setLowerLimit */
+    float upper_limit;        /* prismatic: setUpperLimit */
+    float target;             /* prismatic: setTargetPosition; servo: setTargetAngle */
+    float drive_stiffness;    /* prismatic, servo: setDriveStiffness */
+    float drive_damping;      /* prismatic, servo: setDriveDamping */
+    float gear_ratio;         /* gear: setGearRatio */
 } pgx_joint_desc;
 
 typedef struct pgx_counts {
@@ -162,6 +178,14 @@ int pgx_world_set_can_collide(pgx_world* w, uint32_t entity0, uint32_t entity1, 
 /* Appends joints in creation order. Each takes the lowest joint colour free on both of its
  * entities (8 colours, then the overflow colour), and the pair stops colliding. */
 int pgx_world_add_joints(pgx_world* w, int n, const pgx_joint_desc* joints);
+/* Sets the data members of joint `index` (creation order) from `joint`'s type-specific
+ * fields, as the joint's setters do; type, actors and anchors must be unchanged. Takes
+ * effect from the next step. */
+int pgx_world_set_joint_params(pgx_world* w, int index, const pgx_joint_desc* joint);
+/* Scene::destroyJoint: the joint leaves its colour, its colour bit is freed on both
+ * entities and the entity pair collides again (even where another joint still joins
+ * them, as the reference does). Other joints keep their indices. */
+int pgx_world_destroy_joint(pgx_world* w, int index);
 
 /* One Scene::simulate(time_step) on the device. Asynchronous; pgx_world_sync waits. */
 int pgx_world_step(pgx_world* w, float time_step);

@@ -131,6 +131,12 @@ inline M3 mat3_cast(Q q) {
     r[2][2] = 1.f - 2.f * (qxx + qyy);
     return r;
 }
+// glm::atan(y, x) and glm::acos call the platform's atan2f / acosf, whose last bit differs
+// between C libraries (glibc 2.35's differ from the correctly rounded value on about 1 in 7
+// inputs). Both are restated here as the correctly rounded function: evaluated in double and
+// rounded once to float.
+inline float atan2_cr(float y, float x) { return (float)std::atan2((double)y, (double)x); }
+inline float acos_cr(float x) { return (float)std::acos((double)x); }
 inline M3 mzero() { M3 m; for (int i = 0; i < 3; ++i) m[i] = v3s(0.f); return m; }
 inline M3 mdiag(float s) { M3 m = mzero(); m[0][0] = m[1][1] = m[2][2] = s; return m; }
 inline V3 mulmv(const M3& m, V3 v) {

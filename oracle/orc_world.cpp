@@ -68,7 +68,7 @@ struct KeyHash {
 struct CacheData { int numPoints; V3 lp[4]; float tv[4]; };
 
 // ---- joints (include/Physecs/Joint.h, src/Joints/*.cpp, include/Physecs/Constraint1D*.h)
-enum JointType { J_SPHERICAL = 0, J_UNIVERSAL = 1, J_REVOLUTE = 2, J_FIXED = 3 };
+enum JointType { J_SPHERICAL = 0, J_UNIVERSAL = 1, J_REVOLUTE = 2, J_FIXED = 3, J_PRISMATIC = 4, J_GEAR = 5, J_SERVO = 6 };
 enum RowFlags { F_NONE = 0, F_SOFT = 1, F_ANGULAR = 2, F_LIMITED = 4 };
 // flag-type order of Constraint1DContainer's collections (Constraint1DContainer.h:117)
 static const int kFlagOrder[6] = {F_NONE, F_ANGULAR, F_SOFT, F_LIMITED, F_ANGULAR | F_SOFT, F_ANGULAR | F_LIMITED};
@@ -76,6 +76,13 @@ struct Joint {
     int type, a0, a1;                  // actor indices
     V3 p0; Q q0; V3 p1; Q q1;          // anchors in body space
     int driveEnabled; float driveVelocity, driveMaxTorque;
+    // PrismaticJointData (include/Physecs/Joints/PrismaticJoint.h:8-17), ServoJointData
+    // (ServoJoint.h:7-11), GearJointData (GearJoint.h:7-14)
+    float lowerLimit, upperLimit, target, stiffness, damping, gearRatio;
+    bool makeUpperLimit, makeLowerLimit;
+    float persistentAngle0, persistentAngle1, virtualAngle0, virtualAngle1;
+    bool gearInitialized;
+    bool alive;
     int color;
 };
 struct Row {
@@ -89,7 +96,7 @@ struct JointRows {                     // one joint's rows, in solve order
     int joint;
     V3 r0, r1;                         // anchor relative to the COM (or the anchor) in body space
     int nRows;
-    Row rows[6];
+    Row rows[7];
 };
 
 }  // namespace
@@ -442,8 +449,27 @@ static void addJoint(orc_world* w, const Joint& jin) {
     w->joints.push_back(j);
 }
 
-static int rowLayout(const Joint& j, int* flags) {
+// The rows getSolverDesc declares, in creation order (src/Joints/*.cpp). A prismatic joint
+// decides its limit row from the step-start pose (PrismaticJoint.cpp:123-150).
+static int rowLayout(const orc_world* w, Joint& j, int* flags) {
     int n = 0;
+    if (j.type == J_PRISMATIC) {
+        const Actor& t0 = w->actors[j.a0];
+        const Actor& t1 = w->actors[j.a1];
+        V3 p0 = t0.pos + qrot(t0.quat, j.p0);
+        V3 p1 = t1.pos + qrot(t1.quat, j.p1);
+        V3 d = p1 - p0;
+        V3 u00 = qrot(qmul(t0.quat, j.q0), v3(1, 0, 0));
+        float dx = dot(d, u00);
+        flags[n++] = F_NONE; flags[n++] = F_NONE;
+        for (int k = 0; k < 3; ++k) flags[n++] = F_ANGULAR;
+        if (dx > j.upperLimit) { j.makeUpperLimit = true; j.makeLowerLimit = false; flags[n++] = F_LIMITED; }
+        else if (dx < j.lowerLimit) { j.makeUpperLimit = false; j.makeLowerLimit = true; flags[n++] = F_LIMITED; }
+        else { j.makeUpperLimit = false; j.makeLowerLimit = false; }
+        if (j.driveEnabled) flags[n++] = F_SOFT;
+        return n;
+    }
+    if (j.type == J_GEAR) { flags[n++] = F_ANGULAR; return n; }                 // GearJoint.cpp:56-58
     for (int k = 0; k < 3; ++k) flags[n++] = F_NONE;                           // point-to-point
     if (j.type == J_UNIVERSAL) flags[n++] = F_ANGULAR;                         // UniversalJoint.cpp:102-105
     if (j.type == J_REVOLUTE) {                                                // RevoluteJoint.cpp:52-55
@@ -451,6 +477,9 @@ static int rowLayout(const Joint& j, int* flags) {
         if (j.driveEnabled) flags[n++] = F_ANGULAR | F_LIMITED;
     }
     if (j.type == J_FIXED) for (int k = 0; k < 3; ++k) flags[n++] = F_ANGULAR;  // FixedJoint.cpp:36-39
+    if (j.type == J_SERVO) {                                                   // ServoJoint.cpp:50-53
+        flags[n++] = F_ANGULAR; flags[n++] = F_ANGULAR; flags[n++] = F_ANGULAR | F_SOFT;
+    }
     return n;
 }
 
@@ -459,7 +488,8 @@ static int rowLayout(const Joint& j, int* flags) {
 static void buildJointRows(orc_world* w) {
     for (auto& c : w->jrows) c.clear();
     for (size_t i = 0; i < w->joints.size(); ++i) {
-        const Joint& j = w->joints[i];
+        Joint& j = w->joints[i];
+        if (!j.alive) continue;
         JointRows jr{};
         jr.joint = (int)i;
         const Actor& t0 = w->actors[j.a0];
@@ -470,8 +500,8 @@ static void buildJointRows(orc_world* w) {
         int b1 = d1 && !d1->kinematic ? t1.dyn : -1;
         jr.r0 = b0 >= 0 ? j.p0 - d0->com : j.p0;
         jr.r1 = b1 >= 0 ? j.p1 - d1->com : j.p1;
-        int fl[6];
-        jr.nRows = rowLayout(j, fl);
+        int fl[7];
+        jr.nRows = rowLayout(w, j, fl);
         for (int r = 0; r < jr.nRows; ++r) {
             Row& row = jr.rows[r];
             std::memset(&row, 0, sizeof(row));
@@ -484,10 +514,19 @@ static void buildJointRows(orc_world* w) {
     }
 }
 
+// GearJoint.cpp:5-8: the signed difference of two angles, wrapped to [-pi, pi)
+static float angleDiff(float angle0, float angle1) {
+    const float pi = 3.14159265358979323846f, twoPi = 6.28318530717958647692f;
+    const float diff = std::fmod(angle1 - angle0 + pi, twoPi) - pi;
+    return diff < -pi ? diff + twoPi : diff;
+}
+
+static M3 mcols(V3 a, V3 b, V3 c) { M3 m; m[0] = a; m[1] = b; m[2] = c; return m; }
+
 // calculateWorldSpaceData (src/Joint.cpp:5-14) + makeConstraints of each type
-// (src/JointsUtil.h:8-28, src/Joints/{Spherical,Universal,Revolute,Fixed}Joint.cpp)
+// (src/JointsUtil.h:8-28, src/Joints/*.cpp)
 static void updateJointRows(orc_world* w, JointRows& jr) {
-    const Joint& j = w->joints[jr.joint];
+    Joint& j = w->joints[jr.joint];
     const Actor& t0 = w->actors[j.a0];
     const Actor& t1 = w->actors[j.a1];
     M3 u0 = mat3_cast(qmul(t0.quat, j.q0));
@@ -498,21 +537,85 @@ static void updateJointRows(orc_world* w, JointRows& jr) {
     V3 p1 = t1.pos + qrot(t1.quat, j.p1);
     V3 d = p1 - p0;
     Row* R = jr.rows;
-    R[0].linear = v3(1, 0, 0); R[0].angular0 = v3(0, r0.z, -r0.y); R[0].angular1 = v3(0, r1.z, -r1.y); R[0].c = d.x;
-    R[1].linear = v3(0, 1, 0); R[1].angular0 = v3(-r0.z, 0, r0.x); R[1].angular1 = v3(-r1.z, 0, r1.x); R[1].c = d.y;
-    R[2].linear = v3(0, 0, 1); R[2].angular0 = v3(r0.y, -r0.x, 0); R[2].angular1 = v3(r1.y, -r1.x, 0); R[2].c = d.z;
     auto angularRow = [&](Row& row, V3 a, V3 b) {   // c = dot(a, b), axis = cross(b, a)
         V3 ax = cross(b, a);
         row.angular0 = ax; row.angular1 = ax; row.c = dot(a, b);
     };
+    if (j.type == J_PRISMATIC) {                    // PrismaticJoint.cpp:6-92
+        for (int k = 0; k < 2; ++k) {
+            V3 u = u0[k + 1];
+            R[k].linear = u; R[k].angular0 = cross(r0, u); R[k].angular1 = cross(r1, u); R[k].c = dot(d, u);
+        }
+        angularRow(R[2], u0[0], u1[1]);
+        angularRow(R[3], u0[0], u1[2]);
+        angularRow(R[4], u0[1], u1[2]);
+        float dx = dot(d, u0[0]);
+        V3 r0xx = cross(r0, u0[0]), r1xx = cross(r1, u0[0]);
+        int r = 5;
+        if (j.makeUpperLimit || j.makeLowerLimit) {
+            Row& L = R[r++];
+            L.linear = u0[0]; L.angular0 = r0xx; L.angular1 = r1xx;
+            if (j.makeUpperLimit) { L.c = dx - j.upperLimit; L.mn = 0; }
+            else { L.c = dx - j.lowerLimit; L.mx = 0; }
+        }
+        if (j.driveEnabled) {
+            Row& S = R[r++];
+            S.linear = u0[0]; S.angular0 = r0xx; S.angular1 = r1xx; S.c = dx - j.target;
+            S.freq = j.stiffness; S.damp = j.damping;
+        }
+        return;
+    }
+    if (j.type == J_GEAR) {                         // GearJoint.cpp:10-47
+        float angle0, angle1;
+        {
+            const V3 p1Proj = p1 + dot(p0 - p1, u0[0]) * u0[0];
+            const V3 dir = normalize(p0 - p1Proj);
+            const V3 n = cross(u0[0], dir);
+            const M3 m = mcols(u0[0], -n, dir);
+            const M3 u0t = mulmm(transpose(m), u0);
+            angle0 = atan2_cr(u0t[1][2], u0t[2][2]);
+        }
+        {
+            const V3 p0Proj = p0 + dot(p1 - p0, u1[0]) * u1[0];
+            const V3 dir = normalize(p0Proj - p1);
+            const V3 n = cross(u1[0], dir);
+            const M3 m = mcols(u1[0], -n, dir);
+            const M3 u1t = mulmm(transpose(u1), m);
+            angle1 = atan2_cr(u1t[1][2], u1t[2][2]);
+        }
+        if (!j.gearInitialized) {
+            j.persistentAngle0 = angle0;
+            j.persistentAngle1 = angle1;
+            j.gearInitialized = true;
+        }
+        j.virtualAngle0 += angleDiff(angle0, j.persistentAngle0);
+        j.virtualAngle1 += angleDiff(angle1, j.persistentAngle1);
+        j.persistentAngle0 = angle0;
+        j.persistentAngle1 = angle1;
+        R[0].angular0 = u0[0] * j.gearRatio;
+        R[0].angular1 = -u1[0];
+        R[0].c = j.virtualAngle0 * j.gearRatio - j.virtualAngle1;
+        return;
+    }
+    R[0].linear = v3(1, 0, 0); R[0].angular0 = v3(0, r0.z, -r0.y); R[0].angular1 = v3(0, r1.z, -r1.y); R[0].c = d.x;
+    R[1].linear = v3(0, 1, 0); R[1].angular0 = v3(-r0.z, 0, r0.x); R[1].angular1 = v3(-r1.z, 0, r1.x); R[1].c = d.y;
+    R[2].linear = v3(0, 0, 1); R[2].angular0 = v3(r0.y, -r0.x, 0); R[2].angular1 = v3(r1.y, -r1.x, 0); R[2].c = d.z;
     if (j.type == J_UNIVERSAL) angularRow(R[3], u0[2], u1[2]);
-    if (j.type == J_REVOLUTE || j.type == J_FIXED) {
+    if (j.type == J_REVOLUTE || j.type == J_FIXED || j.type == J_SERVO) {
         angularRow(R[3], u0[0], u1[1]);
         angularRow(R[4], u0[0], u1[2]);
         if (j.type == J_FIXED) angularRow(R[5], u0[1], u1[2]);
         if (j.type == J_REVOLUTE && j.driveEnabled) {
             R[5].angular0 = u0[0]; R[5].angular1 = u0[0];
             R[5].tv = j.driveVelocity; R[5].mx = j.driveMaxTorque; R[5].mn = -j.driveMaxTorque;
+        }
+        if (j.type == J_SERVO) {                    // ServoJoint.cpp:9-37, glm::orientedAngle
+            V3 x = u0[2], y = u1[2], ref = u0[0];
+            const float angle = acos_cr(gclamp(dot(x, y), -1.f, 1.f));
+            const float oriented = dot(ref, cross(x, y)) < 0.f ? -angle : angle;
+            R[5].angular0 = u0[0]; R[5].angular1 = u0[0];
+            R[5].c = oriented - j.target;
+            R[5].freq = j.stiffness; R[5].damp = j.damping;
         }
     }
 }
@@ -1100,14 +1203,22 @@ int orc_sqr_dist_segment_aabb(const float* p, const float* dir, float mn, float 
     return 0;
 }
 
-// Scene::createJoint<T> for the built-in types (include/Physecs/Physecs.h:209-214)
+// Scene::createJoint<T> for the built-in types (include/Physecs/Physecs.h:209-214); the
+// type-specific fields are the joints' data members, set through their setters
 struct orc_joint {
     int32_t type, actor0, actor1;
     float anchor0[7];    // position xyz, orientation xyzw
     float anchor1[7];
     int32_t drive_enabled;
     float drive_velocity, drive_max_torque;
+    float lower_limit, upper_limit, target, drive_stiffness, drive_damping, gear_ratio;
 };
+
+static void setParams(Joint& j, const orc_joint& s) {
+    j.driveEnabled = s.drive_enabled; j.driveVelocity = s.drive_velocity; j.driveMaxTorque = s.drive_max_torque;
+    j.lowerLimit = s.lower_limit; j.upperLimit = s.upper_limit; j.target = s.target;
+    j.stiffness = s.drive_stiffness; j.damping = s.drive_damping; j.gearRatio = s.gear_ratio;
+}
 
 int orc_world_add_joints(orc_world* w, int n, const orc_joint* js) {
     for (int i = 0; i < n; ++i) {
@@ -1118,9 +1229,32 @@ int orc_world_add_joints(orc_world* w, int n, const orc_joint* js) {
         j.q0 = {s.anchor0[3], s.anchor0[4], s.anchor0[5], s.anchor0[6]};
         j.p1 = v3(s.anchor1[0], s.anchor1[1], s.anchor1[2]);
         j.q1 = {s.anchor1[3], s.anchor1[4], s.anchor1[5], s.anchor1[6]};
-        j.driveEnabled = s.drive_enabled; j.driveVelocity = s.drive_velocity; j.driveMaxTorque = s.drive_max_torque;
+        setParams(j, s);
+        j.alive = true;
         addJoint(w, j);
     }
+    return 0;
+}
+
+// The joints' setters (PrismaticJoint::setUpperLimit, ServoJoint::setTargetAngle,
+// GearJoint::setGearRatio, RevoluteJoint::setDriveEnabled, ...): take effect next step.
+int orc_world_set_joint_params(orc_world* w, int i, const orc_joint* s) {
+    if (i < 0 || i >= (int)w->joints.size() || !w->joints[i].alive) return -1;
+    setParams(w->joints[i], *s);
+    return 0;
+}
+
+// Scene::destroyJoint (src/Physecs.cpp:712-725): out of its colour, the entity pair may
+// collide again, and the colour bit is freed on both entities.
+int orc_world_destroy_joint(orc_world* w, int i) {
+    if (i < 0 || i >= (int)w->joints.size() || !w->joints[i].alive) return -1;
+    Joint& j = w->joints[i];
+    j.alive = false;
+    uint32_t e0 = w->actors[j.a0].entity, e1 = w->actors[j.a1].entity;
+    uint32_t lo = e0 < e1 ? e0 : e1, hi = e0 < e1 ? e1 : e0;
+    w->nonColliding.erase(((uint64_t)lo << 32) | hi);
+    w->jointBitsets[e0] = (unsigned char)(w->jointBitsets[e0] & ~(1u << j.color));
+    w->jointBitsets[e1] = (unsigned char)(w->jointBitsets[e1] & ~(1u << j.color));
     return 0;
 }
 

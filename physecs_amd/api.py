@@ -26,7 +26,8 @@ EXPORTS = [
     "pgx_world_set_can_collide", "pgx_world_step", "pgx_world_sync", "pgx_world_read_state",
     "pgx_world_write_state", "pgx_world_get_counts", "pgx_world_get_bounds", "pgx_world_get_pairs",
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
-    "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints",
+    "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
+    "pgx_world_destroy_joint",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint",
 ]
@@ -54,7 +55,9 @@ class PgxSceneDesc(ctypes.Structure):
 class PgxJointDesc(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("actor0", ctypes.c_int32), ("actor1", ctypes.c_int32),
                 ("anchor0", ctypes.c_float * 7), ("anchor1", ctypes.c_float * 7), ("drive_enabled", ctypes.c_int32),
-                ("drive_velocity", ctypes.c_float), ("drive_max_torque", ctypes.c_float)]
+                ("drive_velocity", ctypes.c_float), ("drive_max_torque", ctypes.c_float),
+                ("lower_limit", ctypes.c_float), ("upper_limit", ctypes.c_float), ("target", ctypes.c_float),
+                ("drive_stiffness", ctypes.c_float), ("drive_damping", ctypes.c_float), ("gear_ratio", ctypes.c_float)]
 
 
 class PgxCounts(ctypes.Structure):
@@ -111,6 +114,8 @@ def lib():
         L.pgx_world_get_stats.argtypes = [vp, ctypes.POINTER(PgxStats)]
         L.pgx_world_set_exec.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.pgx_world_add_joints.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PgxJointDesc)]
+        L.pgx_world_set_joint_params.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PgxJointDesc)]
+        L.pgx_world_destroy_joint.argtypes = [vp, ctypes.c_int]
         L.pgx_server_create.argtypes = [ctypes.POINTER(PgxConfig), ctypes.POINTER(PgxServerConfig), ctypes.POINTER(vp)]
         L.pgx_server_destroy.argtypes = [vp]
         L.pgx_server_submit.argtypes = [vp, ctypes.POINTER(PgxSceneDesc), ctypes.POINTER(PgxJointDesc), ctypes.c_int,
@@ -191,6 +196,25 @@ class World:
         c = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
         pos, quat, vel, ang = c(pos), c(quat), c(vel), c(ang)
         check(self.L.pgx_world_write_state(self.h, _f(pos), _f(quat), _f(vel), _f(ang)))
+
+    def set_joint_params(self, index, **params):
+        """The joint's setters: updates the data members named in params (JOINT_FIELDS) of joint
+        `index` (creation order); takes effect from the next step."""
+        from . import scenes
+        if not hasattr(self, "_joints"):
+            self._joints = [tuple(j[:5]) + (dict(j[5]),) for j in self.scene.joints]
+        t, a0, a1, an0, an1, p = self._joints[index]
+        for k, v in params.items():
+            if k not in scenes.JOINT_FIELDS:
+                raise ValueError(f"unknown joint parameter {k}")
+            p[k] = v
+        d = PgxJointDesc()
+        scenes.Scene.fill_joint(d, t, a0, a1, an0, an1, p)
+        check(self.L.pgx_world_set_joint_params(self.h, index, ctypes.byref(d)))
+
+    def destroy_joint(self, index):
+        """Scene::destroyJoint for joint `index` (creation order)."""
+        check(self.L.pgx_world_destroy_joint(self.h, index))
 
     def counts(self):
         c = PgxCounts()

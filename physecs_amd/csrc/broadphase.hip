@@ -340,6 +340,65 @@ __global__ void k_world_stop(int W, const int32_t* __restrict__ worldRoot, const
     worldStop[wd] = (r & LEAF_BIT) ? leafEsc[r & ~LEAF_BIT] : nodeEsc[r];
 }
 
+// Traversal records: a visited node (or leaf) is one round of independent loads.
+__global__ void k_pack(int n, const float4* __restrict__ nmin, const float4* __restrict__ nmax,
+                       const int32_t* __restrict__ child, const int32_t* __restrict__ nodeEsc,
+                       const int2* __restrict__ range, const float4* __restrict__ lmin, const float4* __restrict__ lmax,
+                       const int32_t* __restrict__ leafEsc, const int32_t* __restrict__ lflags,
+                       const int32_t* __restrict__ order, const uint32_t* __restrict__ centity,
+                       const int32_t* __restrict__ clocal, float4* __restrict__ nodeA, float4* __restrict__ nodeB,
+                       int32_t* __restrict__ nodeLast, float4* __restrict__ leafA, float4* __restrict__ leafB,
+                       int2* __restrict__ leafI) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n - 1) {
+        float4 a = nmin[t], b = nmax[t];
+        nodeA[t] = make_float4(a.x, a.y, a.z, __int_as_float(child[2 * t]));
+        nodeB[t] = make_float4(b.x, b.y, b.z, __int_as_float(nodeEsc[t]));
+        nodeLast[t] = range[t].y;
+    }
+    if (t < n) {
+        float4 a = lmin[t], b = lmax[t];
+        leafA[t] = make_float4(a.x, a.y, a.z, __int_as_float(leafEsc[t]));
+        leafB[t] = make_float4(b.x, b.y, b.z, __int_as_float(lflags[t]));
+        int c = order[t];
+        leafI[t] = make_int2((int)centity[c], clocal[c]);
+    }
+}
+
+// The subtrees at depth CUT_DEPTH under the root of a one-world tree (a leaf reached
+// earlier stands for itself), in depth-first order, with the escape link that ends each
+// subtree's walk. Their leaf sets partition the tree, so walking each of them for a query
+// finds exactly the pairs one walk from the root finds.
+constexpr int CUT_DEPTH = 3, CUT_MAX = 1 << CUT_DEPTH;
+__global__ void k_cut(const int32_t* __restrict__ worldRoot, const int32_t* __restrict__ child,
+                      const int32_t* __restrict__ nodeEsc, const int32_t* __restrict__ leafEsc,
+                      int32_t* __restrict__ cutNode, int32_t* __restrict__ cutStop) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int cur[CUT_MAX], nxt[CUT_MAX];
+    int nc = 0;
+    if (worldRoot[0] != -1) cur[nc++] = worldRoot[0];
+    for (int d = 0; d < CUT_DEPTH; ++d) {
+        int nn = 0;
+        for (int i = 0; i < nc; ++i) {
+            int v = cur[i];
+            if (v & LEAF_BIT) nxt[nn++] = v;
+            else { nxt[nn++] = child[2 * v]; nxt[nn++] = child[2 * v + 1]; }
+        }
+        for (int i = 0; i < nn; ++i) cur[i] = nxt[i];
+        nc = nn;
+    }
+    for (int i = 0; i < CUT_MAX; ++i) {
+        if (i < nc) {
+            int v = cur[i];
+            cutNode[i] = v;
+            cutStop[i] = (v & LEAF_BIT) ? leafEsc[v & ~LEAF_BIT] : nodeEsc[v];
+        } else {
+            cutNode[i] = -1;
+            cutStop[i] = -1;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ traversal
 __device__ __forceinline__ bool overlaps(float4 amin, float4 amax, float4 bmin, float4 bmax) {
     // inclusive, as the sweep's `max < min` rejections (src/Physecs.cpp:152-154)
@@ -352,34 +411,45 @@ __device__ __forceinline__ bool overlaps(float4 amin, float4 amax, float4 bmin, 
 // non-dynamic colliders (static and kinematic; their bounds never move during a step).
 __global__ __launch_bounds__(256) void k_traverse(
     int n, const uint64_t* __restrict__ mkeys, int worldShift, const int32_t* __restrict__ order,
-    const float4* __restrict__ lmin, const float4* __restrict__ lmax, const int32_t* __restrict__ lflags,
-    const int32_t* __restrict__ child, const int2* __restrict__ range, const float4* __restrict__ nmin,
-    const float4* __restrict__ nmax, const int32_t* __restrict__ nodeEsc, const int32_t* __restrict__ leafEsc,
+    const float4* __restrict__ leafA, const float4* __restrict__ leafB, const int2* __restrict__ leafI,
+    const float4* __restrict__ nodeA, const float4* __restrict__ nodeB, const int32_t* __restrict__ nodeLast,
     const int32_t* __restrict__ worldRoot, const int32_t* __restrict__ worldStop,
     const int32_t* __restrict__ worldStaticStart, const int32_t* __restrict__ staticCols,
     const float4* __restrict__ cbmin, const float4* __restrict__ cbmax, const int32_t* __restrict__ cflags,
     const uint32_t* __restrict__ centity, const int32_t* __restrict__ clocal, int localBits,
-    int32_t* __restrict__ pairCount, int cap, uint64_t* __restrict__ pkeys) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t* __restrict__ pairCount, int cap, uint64_t* __restrict__ pkeys, const int32_t* __restrict__ cutNode,
+    const int32_t* __restrict__ cutStop, int parts) {
+    // `parts` lanes per query: lane p walks cut subtree p (parts = 1: the world's whole tree)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = parts > 1 ? t / parts : t;
+    const int part = parts > 1 ? t % parts : 0;
     bool active = false;
     float4 qmin = make_float4(0.f, 0.f, 0.f, 0.f), qmax = qmin;
     int node = -1, stop = -1, st = 0, stEnd = 0;
-    int myCol = 0;
+    int myLocal = 0;
     uint32_t myEnt = 0;
     int wd = 0;
     if (s < n) {
-        int f = lflags[s];
-        if (f & CF_ENABLE) {
+        float4 a = leafA[s], b = leafB[s];
+        if (__float_as_int(b.w) & CF_ENABLE) {
             active = true;
-            qmin = lmin[s];
-            qmax = lmax[s];
+            qmin = a;
+            qmax = b;
             wd = (int)(mkeys[s] >> worldShift);
-            node = worldRoot[wd];
-            stop = worldStop[wd];
-            st = worldStaticStart[wd];
-            stEnd = worldStaticStart[wd + 1];
-            myCol = order[s];
-            myEnt = centity[myCol];
+            if (parts > 1) {
+                node = cutNode[part];
+                stop = cutStop[part];
+            } else {
+                node = worldRoot[wd];
+                stop = worldStop[wd];
+            }
+            if (part == 0) {   // the world's non-dynamic colliders: once per query
+                st = worldStaticStart[wd];
+                stEnd = worldStaticStart[wd + 1];
+            }
+            int2 me = leafI[s];
+            myEnt = (uint32_t)me.x;
+            myLocal = me.y;
         }
     }
     const int lane = threadIdx.x & (WAVE - 1);
@@ -396,32 +466,32 @@ __global__ __launch_bounds__(256) void k_traverse(
         if (active && node != stop) {
             if (node & LEAF_BIT) {
                 int s2 = node & ~LEAF_BIT;
-                if (s2 > s && (lflags[s2] & CF_ENABLE) && overlaps(qmin, qmax, lmin[s2], lmax[s2])) {
-                    int c2 = order[s2];
-                    uint32_t e2 = centity[c2];
+                float4 a = leafA[s2], b = leafB[s2];
+                if (s2 > s && (__float_as_int(b.w) & CF_ENABLE) && overlaps(qmin, qmax, a, b)) {
+                    int2 o = leafI[s2];
+                    uint32_t e2 = (uint32_t)o.x;
                     if (e2 != myEnt) {
-                        int c0 = myEnt < e2 ? myCol : c2;
-                        int c1 = myEnt < e2 ? c2 : myCol;
-                        key = ((uint64_t)wd << (2 * localBits)) | ((uint64_t)clocal[c0] << localBits) |
-                              (uint64_t)clocal[c1];
+                        int l0 = myEnt < e2 ? myLocal : o.y;
+                        int l1 = myEnt < e2 ? o.y : myLocal;
+                        key = ((uint64_t)wd << (2 * localBits)) | ((uint64_t)l0 << localBits) | (uint64_t)l1;
                         emit = true;
                     }
                 }
-                node = leafEsc[s2];
+                node = __float_as_int(a.w);
             } else {
                 // subtrees whose leaves all sit at or before s hold no pair this leaf owns
-                if (range[node].y > s && overlaps(qmin, qmax, nmin[node], nmax[node])) node = child[2 * node];
-                else node = nodeEsc[node];
+                float4 a = nodeA[node], b = nodeB[node];
+                int last = nodeLast[node];
+                node = (last > s && overlaps(qmin, qmax, a, b)) ? __float_as_int(a.w) : __float_as_int(b.w);
             }
         } else if (active && st < stEnd) {
             int c2 = staticCols[st++];
             if ((cflags[c2] & CF_ENABLE) && overlaps(qmin, qmax, cbmin[c2], cbmax[c2])) {
                 uint32_t e2 = centity[c2];
                 if (e2 != myEnt) {
-                    int c0 = myEnt < e2 ? myCol : c2;
-                    int c1 = myEnt < e2 ? c2 : myCol;
-                    key = ((uint64_t)wd << (2 * localBits)) | ((uint64_t)clocal[c0] << localBits) |
-                          (uint64_t)clocal[c1];
+                    int l0 = myEnt < e2 ? myLocal : clocal[c2];
+                    int l1 = myEnt < e2 ? clocal[c2] : myLocal;
+                    key = ((uint64_t)wd << (2 * localBits)) | ((uint64_t)l0 << localBits) | (uint64_t)l1;
                     emit = true;
                 }
             }
@@ -487,11 +557,22 @@ int bp_find_pairs(World& w) {
         }
         hipLaunchKernelGGL(k_world_stop, dim3(div_up(W, 256)), dim3(256), 0, s, W, w.worldRoot.p, w.nodeEsc.p,
                            w.leafEsc.p, w.worldStop.p);
-        hipLaunchKernelGGL(k_traverse, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift, w.mvals.p,
-                           w.leafMin.p, w.leafMax.p, w.leafFlags.p, w.nodeChild.p, w.nodeRange.p, w.nodeMin.p,
-                           w.nodeMax.p, w.nodeEsc.p, w.leafEsc.p, w.worldRoot.p, w.worldStop.p,
-                           w.worldStaticStart.p, w.staticCols.p, w.cbmin.p, w.cbmax.p, w.cflags.p, w.centity.p,
-                           w.clocal.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
+        hipLaunchKernelGGL(k_pack, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.nodeMin.p, w.nodeMax.p, w.nodeChild.p,
+                           w.nodeEsc.p, w.nodeRange.p, w.leafMin.p, w.leafMax.p, w.leafEsc.p, w.leafFlags.p, w.mvals.p,
+                           w.centity.p, w.clocal.p, w.nodeA.p, w.nodeB.p, w.nodeLast.p, w.leafA.p, w.leafB.p,
+                           w.leafI.p);
+        // one world with few queries: several lanes per query keep enough waves in flight to
+        // hide the node loads' latency
+        int parts = 1;   // one lane per query: splitting a query over subtrees multiplies the waves
+                         // while each wave still runs as long as its longest walk
+        if (parts > 1)
+            hipLaunchKernelGGL(k_cut, dim3(1), dim3(64), 0, s, w.worldRoot.p, w.nodeChild.p, w.nodeEsc.p, w.leafEsc.p,
+                               w.cutNode.p, w.cutStop.p);
+        hipLaunchKernelGGL(k_traverse, dim3(div_up((long)n * parts, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift,
+                           w.mvals.p, w.leafA.p, w.leafB.p, w.leafI.p, w.nodeA.p, w.nodeB.p, w.nodeLast.p,
+                           w.worldRoot.p, w.worldStop.p, w.worldStaticStart.p, w.staticCols.p, w.cbmin.p, w.cbmax.p,
+                           w.cflags.p, w.centity.p, w.clocal.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p,
+                           w.cutNode.p, w.cutStop.p, parts);
         PGX_LAUNCH_CHECK();
     }
     int cnt[8] = {0};

@@ -4,8 +4,9 @@
 // Reference semantics:
 //   colouring at creation            src/Physecs.cpp:690-710 (host, pgx_world_add_joints)
 //   rows created every step          src/Physecs.cpp:321-355, include/Physecs/Constraint1DContainer.h:133-198
-//   world-space data, row values     src/Joint.cpp:5-14, src/JointsUtil.h:8-28,
-//                                    src/Joints/{Spherical,Universal,Revolute,Fixed}Joint.cpp
+//   world-space data, row values     src/Joint.cpp:5-14, src/JointsUtil.h:8-28, src/Joints/*.cpp
+//   prismatic limit row per step     src/Joints/PrismaticJoint.cpp:123-150 (getSolverDesc)
+//   gear angles, state per substep   src/Joints/GearJoint.cpp:5-47
 //   colours 0-7 (SIMD rows)          src/Constraint1DW.cpp:5-233, per lane
 //   colour 8 (scalar overflow)       src/Constraint1D.cpp:5-127
 // Joints of one colour 0-7 share no entity, so each is one thread that walks its rows in
@@ -17,7 +18,9 @@
 namespace pgx {
 
 enum { JF_NONE = 0, JF_SOFT = 1, JF_ANGULAR = 2, JF_LIMITED = 4 };
-enum { JT_SPHERICAL = 0, JT_UNIVERSAL = 1, JT_REVOLUTE = 2, JT_FIXED = 3 };
+enum { JT_SPHERICAL = 0, JT_UNIVERSAL = 1, JT_REVOLUTE = 2, JT_FIXED = 3, JT_PRISMATIC = 4, JT_GEAR = 5, JT_SERVO = 6 };
+// jRowFlags: 4 flag bits per row (7 rows), then the prismatic limit row's kind
+constexpr int JL_SHIFT = 28, JL_UPPER = 1, JL_LOWER = 2;
 __constant__ int kFlagOrder[6] = {JF_NONE, JF_ANGULAR, JF_SOFT, JF_LIMITED, JF_ANGULAR | JF_SOFT,
                                   JF_ANGULAR | JF_LIMITED};
 
@@ -28,8 +31,12 @@ enum { JR_LIN = 0, JR_ANG0, JR_ANG1, JR_LIN0T, JR_LIN1T, JR_ANG0T, JR_ANG1T, JR_
 
 struct JointArgs {
     int J, Jcap;
-    const int32_t *jType, *jA0, *jA1, *jRowFlags, *jNRows;   // jRowFlags: 4 bits per row
+    const int32_t *jType, *jA0, *jA1;
+    int32_t *jRowFlags, *jNRows;                              // per step (a prismatic joint's limit row)
     const float4 *jP0, *jQ0, *jP1, *jQ1, *jDrive;            // anchors (body space), drive (enabled, v, maxTorque)
+    const float4 *jPar0, *jPar1;                              // (lower, upper, target, stiffness), (damping, ratio)
+    float4* jGear;                                            // persistent angle 0, 1, virtual angle 0, 1
+    int32_t* jGearInit;
     int32_t *jB0, *jB1;
     float4 *jR0, *jR1;                                        // anchor - com (or anchor), body space
     float4* jRow;
@@ -43,19 +50,39 @@ struct JointArgs {
 
 __device__ __forceinline__ int row_flags(const JointArgs& a, int j, int r) { return (a.jRowFlags[j] >> (4 * r)) & 15; }
 
+// The rows PrismaticJoint::getSolverDesc declares for this step (PrismaticJoint.cpp:123-150):
+// 2 linear, 3 angular, a limit row when the slide is beyond a limit at the step's start
+// pose, a soft drive row when the drive is on.
+__device__ void prismatic_layout(const JointArgs& a, int j) {
+    int A0 = a.jA0[j], A1 = a.jA1[j];
+    qf t0 = q4(a.bquat[A0]), t1 = q4(a.bquat[A1]);
+    f3 p0 = add(xyz(a.bpos[A0]), qrot(t0, xyz(a.jP0[j])));
+    f3 p1 = add(xyz(a.bpos[A1]), qrot(t1, xyz(a.jP1[j])));
+    f3 d = sub(p1, p0);
+    f3 u00 = qrot(qmul(t0, q4(a.jQ0[j])), mk3(1.f, 0.f, 0.f));
+    float dx = dot(d, u00);
+    float4 par = a.jPar0[j];
+    int flags = (JF_ANGULAR << 8) | (JF_ANGULAR << 12) | (JF_ANGULAR << 16), n = 5, kind = 0;
+    if (dx > par.y) kind = JL_UPPER;
+    else if (dx < par.x) kind = JL_LOWER;
+    if (kind) flags |= JF_LIMITED << (4 * n++);
+    if (a.jDrive[j].x != 0.f) flags |= JF_SOFT << (4 * n++);
+    a.jRowFlags[j] = flags | (kind << JL_SHIFT);
+    a.jNRows[j] = n;
+}
+
 // src/Physecs.cpp:332-352: body indices and the anchors relative to the COM; fresh rows
 // (Constraint1D defaults: zeros, min = lowest, max = max, lambda 0)
 __global__ void k_joint_setup(JointArgs a) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= a.J) return;
-    int A0 = a.jA0[j], A1 = a.jA1[j];
     // dynamic bodies are the first ones; a kinematic body carries a nonzero dcom.w
     // (the descriptor's actor order is mapped to these internal indices on the host)
     int b0 = a.jB0[j], b1 = a.jB1[j];
     f3 p0 = xyz(a.jP0[j]), p1 = xyz(a.jP1[j]);
     a.jR0[j] = f4(b0 >= 0 ? sub(p0, xyz(a.dcom[b0])) : p0, 0.f);
     a.jR1[j] = f4(b1 >= 0 ? sub(p1, xyz(a.dcom[b1])) : p1, 0.f);
-    (void)A0; (void)A1;
+    if (a.jType[j] == JT_PRISMATIC) prismatic_layout(a, j);
     int n = a.jNRows[j];
     for (int r = 0; r < n; ++r) {
         for (int f = 0; f < JR_NF; ++f) JROW(r, f) = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -65,6 +92,31 @@ __global__ void k_joint_setup(JointArgs a) {
 }
 
 __device__ __forceinline__ void set_xyz(float4& d, f3 v) { d.x = v.x; d.y = v.y; d.z = v.z; }
+
+// glm::atan(y, x) and glm::acos call the platform's atan2f / acosf, whose last bit differs
+// between C libraries; both are computed here as the correctly rounded function, evaluated
+// in double and rounded once to float.
+__device__ __forceinline__ float atan2_cr(float y, float x) { return (float)atan2((double)y, (double)x); }
+__device__ __forceinline__ float acos_cr(float x) { return (float)acos((double)x); }
+
+// GearJoint.cpp:5-8: the signed difference of two angles, wrapped to [-pi, pi)
+__device__ __forceinline__ float angle_diff(float angle0, float angle1) {
+    const float pi = 3.14159265358979323846f, twoPi = 6.28318530717958647692f;
+    const float diff = fmodf(angle1 - angle0 + pi, twoPi) - pi;
+    return diff < -pi ? diff + twoPi : diff;
+}
+
+// row r's linear, angular and c, and optionally its limits or soft parameters
+__device__ __forceinline__ void write_row(const JointArgs& a, int j, int r, f3 lin, f3 ang0, f3 ang1, float c) {
+    float4 l = JROW(r, JR_LIN), g0 = JROW(r, JR_ANG0), g1 = JROW(r, JR_ANG1);
+    set_xyz(l, lin); set_xyz(g0, ang0); g0.w = c; set_xyz(g1, ang1);
+    JROW(r, JR_LIN) = l; JROW(r, JR_ANG0) = g0; JROW(r, JR_ANG1) = g1;
+}
+__device__ __forceinline__ void write_soft(const JointArgs& a, int j, int r, float freq, float damp) {
+    float4 t1 = JROW(r, JR_ANG1T), ex = JROW(r, JR_EXTRA);
+    t1.w = freq; ex.x = damp;
+    JROW(r, JR_ANG1T) = t1; JROW(r, JR_EXTRA) = ex;
+}
 
 // world-space data and the rows' values for one joint, every substep
 __device__ void joint_update(const JointArgs& a, int j) {
@@ -77,15 +129,6 @@ __device__ void joint_update(const JointArgs& a, int j) {
     f3 p0 = add(xyz(a.bpos[A0]), qrot(t0, xyz(a.jP0[j])));
     f3 p1 = add(xyz(a.bpos[A1]), qrot(t1, xyz(a.jP1[j])));
     f3 d = sub(p1, p0);
-    f3 lin[3] = {mk3(1.f, 0.f, 0.f), mk3(0.f, 1.f, 0.f), mk3(0.f, 0.f, 1.f)};
-    f3 a0[3] = {mk3(0.f, r0.z, -r0.y), mk3(-r0.z, 0.f, r0.x), mk3(r0.y, -r0.x, 0.f)};
-    f3 a1[3] = {mk3(0.f, r1.z, -r1.y), mk3(-r1.z, 0.f, r1.x), mk3(r1.y, -r1.x, 0.f)};
-    float dc[3] = {d.x, d.y, d.z};
-    for (int k = 0; k < 3; ++k) {
-        float4 l = JROW(k, JR_LIN), g0 = JROW(k, JR_ANG0), g1 = JROW(k, JR_ANG1);
-        set_xyz(l, lin[k]); set_xyz(g0, a0[k]); g0.w = dc[k]; set_xyz(g1, a1[k]);
-        JROW(k, JR_LIN) = l; JROW(k, JR_ANG0) = g0; JROW(k, JR_ANG1) = g1;
-    }
     auto angular_row = [&](int r, f3 x, f3 y) {   // c = dot(x, y), axis = cross(y, x)
         f3 ax = cross(y, x);
         float4 g0 = JROW(r, JR_ANG0), g1 = JROW(r, JR_ANG1);
@@ -93,8 +136,67 @@ __device__ void joint_update(const JointArgs& a, int j) {
         JROW(r, JR_ANG0) = g0; JROW(r, JR_ANG1) = g1;
     };
     int type = a.jType[j];
+    if (type == JT_PRISMATIC) {                    // PrismaticJoint.cpp:6-92
+        write_row(a, j, 0, u0.c1, cross(r0, u0.c1), cross(r1, u0.c1), dot(d, u0.c1));
+        write_row(a, j, 1, u0.c2, cross(r0, u0.c2), cross(r1, u0.c2), dot(d, u0.c2));
+        angular_row(2, u0.c0, u1.c1);
+        angular_row(3, u0.c0, u1.c2);
+        angular_row(4, u0.c1, u1.c2);
+        float dx = dot(d, u0.c0);
+        f3 r0xx = cross(r0, u0.c0), r1xx = cross(r1, u0.c0);
+        float4 par = a.jPar0[j];
+        int kind = (a.jRowFlags[j] >> JL_SHIFT) & 3;
+        int r = 5;
+        if (kind) {
+            write_row(a, j, r, u0.c0, r0xx, r1xx, kind == JL_UPPER ? dx - par.y : dx - par.x);
+            if (kind == JL_UPPER) { float4 e = JROW(r, JR_LIN1T); e.w = 0.f; JROW(r, JR_LIN1T) = e; }
+            else { float4 m = JROW(r, JR_ANG0T); m.w = 0.f; JROW(r, JR_ANG0T) = m; }
+            ++r;
+        }
+        if (a.jDrive[j].x != 0.f) {
+            write_row(a, j, r, u0.c0, r0xx, r1xx, dx - par.z);
+            write_soft(a, j, r, par.w, a.jPar1[j].x);
+        }
+        return;
+    }
+    if (type == JT_GEAR) {                         // GearJoint.cpp:10-47
+        float angle0, angle1;
+        {
+            const f3 p1Proj = add(p1, mul(dot(sub(p0, p1), u0.c0), u0.c0));
+            const f3 dir = normalize(sub(p0, p1Proj));
+            const f3 n = cross(u0.c0, dir);
+            const m3 u0t = mulmm(transpose(mk_m3(u0.c0, neg(n), dir)), u0);
+            angle0 = atan2_cr(u0t.c1.z, u0t.c2.z);
+        }
+        {
+            const f3 p0Proj = add(p0, mul(dot(sub(p1, p0), u1.c0), u1.c0));
+            const f3 dir = normalize(sub(p0Proj, p1));
+            const f3 n = cross(u1.c0, dir);
+            const m3 u1t = mulmm(transpose(u1), mk_m3(u1.c0, neg(n), dir));
+            angle1 = atan2_cr(u1t.c1.z, u1t.c2.z);
+        }
+        float4 g = a.jGear[j];
+        if (!a.jGearInit[j]) {
+            g.x = angle0;
+            g.y = angle1;
+            a.jGearInit[j] = 1;
+        }
+        g.z = g.z + angle_diff(angle0, g.x);
+        g.w = g.w + angle_diff(angle1, g.y);
+        g.x = angle0;
+        g.y = angle1;
+        a.jGear[j] = g;
+        float ratio = a.jPar1[j].y;
+        write_row(a, j, 0, s3(0.f), mul(u0.c0, ratio), neg(u1.c0), g.z * ratio - g.w);
+        return;
+    }
+    f3 lin[3] = {mk3(1.f, 0.f, 0.f), mk3(0.f, 1.f, 0.f), mk3(0.f, 0.f, 1.f)};
+    f3 a0[3] = {mk3(0.f, r0.z, -r0.y), mk3(-r0.z, 0.f, r0.x), mk3(r0.y, -r0.x, 0.f)};
+    f3 a1[3] = {mk3(0.f, r1.z, -r1.y), mk3(-r1.z, 0.f, r1.x), mk3(r1.y, -r1.x, 0.f)};
+    float dc[3] = {d.x, d.y, d.z};
+    for (int k = 0; k < 3; ++k) write_row(a, j, k, lin[k], a0[k], a1[k], dc[k]);
     if (type == JT_UNIVERSAL) angular_row(3, u0.c2, u1.c2);
-    if (type == JT_REVOLUTE || type == JT_FIXED) {
+    if (type == JT_REVOLUTE || type == JT_FIXED || type == JT_SERVO) {
         angular_row(3, u0.c0, u1.c1);
         angular_row(4, u0.c0, u1.c2);
         if (type == JT_FIXED) angular_row(5, u0.c1, u1.c2);
@@ -106,6 +208,16 @@ __device__ void joint_update(const JointArgs& a, int j) {
             l.w = dr.y; mx.w = dr.z; mn.w = -dr.z;
             JROW(5, JR_LIN) = l; JROW(5, JR_ANG0) = g0; JROW(5, JR_ANG1) = g1;
             JROW(5, JR_LIN1T) = mn; JROW(5, JR_ANG0T) = mx;
+        }
+        if (type == JT_SERVO) {                    // ServoJoint.cpp:9-37, glm::orientedAngle
+            const f3 x = u0.c2, y = u1.c2, ref = u0.c0;
+            const float angle = acos_cr(gclamp(dot(x, y), -1.f, 1.f));
+            const float oriented = dot(ref, cross(x, y)) < 0.f ? -angle : angle;
+            float4 par = a.jPar0[j];
+            float4 g0 = JROW(5, JR_ANG0), g1 = JROW(5, JR_ANG1);
+            set_xyz(g0, u0.c0); g0.w = oriented - par.z; set_xyz(g1, u0.c0);
+            JROW(5, JR_ANG0) = g0; JROW(5, JR_ANG1) = g1;
+            write_soft(a, j, 5, par.w, a.jPar1[j].x);
         }
     }
 }
@@ -334,6 +446,7 @@ static JointArgs make_jargs(World& w, float h) {
     a.J = w.J; a.Jcap = w.jCap;
     a.jType = w.jType.p; a.jA0 = w.jA0.p; a.jA1 = w.jA1.p; a.jRowFlags = w.jRowFlags.p; a.jNRows = w.jNRows.p;
     a.jP0 = w.jP0.p; a.jQ0 = w.jQ0.p; a.jP1 = w.jP1.p; a.jQ1 = w.jQ1.p; a.jDrive = w.jDrive.p;
+    a.jPar0 = w.jPar0.p; a.jPar1 = w.jPar1.p; a.jGear = w.jGear.p; a.jGearInit = w.jGearInit.p;
     a.jB0 = w.jB0.p; a.jB1 = w.jB1.p; a.jR0 = w.jR0.p; a.jR1 = w.jR1.p; a.jRow = w.jRow.p;
     a.bpos = w.bpos.p; a.bquat = w.bquat.p; a.dcom = w.dcom.p; a.dinvI = w.dinvI.p; a.dvel = w.dvel.p;
     a.tvel = w.tvel.p; a.tang = w.tang.p; a.pvel = w.pvel.p; a.pang = w.pang.p;

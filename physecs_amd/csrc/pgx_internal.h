@@ -51,6 +51,7 @@ struct DBuf {
 };
 
 constexpr int WAVE = 64;
+constexpr int JOINT_MAX_ROWS = 7;      // a prismatic joint with its limit and drive rows
 // convex narrowphase: EPA polytope capacity per lane and lanes of the convex kernel
 constexpr int EPA_MAXV = 104;          // 4 simplex vertices + at most 100 insertions
 constexpr int EPA_MAXF = 256;          // > 2 * EPA_MAXV - 4, the face count of a closed triangulation
@@ -67,7 +68,7 @@ inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
 // ------------------------------------------------------------------ utilities (util.hip)
 struct SortTemp {
-    DBuf<uint32_t> hist;
+    DBuf<uint8_t> temp;                  // library sort workspace
     DBuf<uint64_t> kalt;
     DBuf<int32_t> valt;
 };
@@ -142,6 +143,12 @@ struct World {
     DBuf<int32_t> nodeParent, nodeEsc, nodeFlags, nodeCounter;
     DBuf<int2> nodeRange;
     DBuf<float4> nodeMin, nodeMax;
+    // traversal records: one load round per visited node or leaf
+    DBuf<float4> nodeA, nodeB;           // {min, left child bits}, {max, escape bits}
+    DBuf<int32_t> nodeLast;              // last leaf of the node's subtree
+    DBuf<float4> leafA, leafB;           // {min, escape bits}, {max, flags bits}
+    DBuf<int2> leafI;                    // {entity, world-local collider index}
+    DBuf<int32_t> cutNode, cutStop;      // subtrees walked by the lanes of one query
     DBuf<int32_t> worldRoot, worldStop;
     DBuf<uint64_t> pkeys;
     DBuf<int32_t> pvals;
@@ -159,7 +166,8 @@ struct World {
     DBuf<uint64_t> mfKey;                // pair key
     DBuf<int32_t> mfPair;                // pair slot
     DBuf<int32_t> mfB0, mfB1, mfColor;
-    DBuf<uint64_t> claim;                // D coloring claims
+    DBuf<uint64_t> claim, claim2;        // D coloring claims, by round parity
+    DBuf<int32_t> colorUncol;            // manifolds left uncoloured after each round
     DBuf<uint64_t> usedMask;             // 2 x D color masks
     DBuf<uint64_t> colorKeys;
     DBuf<int32_t> colorPerm;
@@ -213,12 +221,15 @@ struct World {
 
     // joints (descriptor order = creation order), colours as Scene::addJoint assigns them
     int J = 0, jCap = 0, jRowsTotal = 0;
-    std::vector<int32_t> hjType, hjA0, hjA1, hjB0, hjB1, hjRowFlags, hjNRows, hjColor;
-    std::vector<float4> hjP0, hjQ0, hjP1, hjQ1, hjDrive;
+    std::vector<int32_t> hjType, hjA0, hjA1, hjB0, hjB1, hjRowFlags, hjNRows, hjColor, hjAlive;
+    std::vector<float4> hjP0, hjQ0, hjP1, hjQ1, hjDrive, hjPar0, hjPar1;
     std::vector<unsigned char> hjBits;           // per internal body: colours taken (src/Physecs.cpp:696-705)
     std::vector<int> jColorStart;                // 10 offsets into jOrder
     DBuf<int32_t> jType, jA0, jA1, jB0, jB1, jRowFlags, jNRows, jOrder;
     DBuf<float4> jP0, jQ0, jP1, jQ1, jDrive, jR0, jR1, jRow;
+    DBuf<float4> jPar0, jPar1;           // (lower, upper, target, stiffness), (damping, gear ratio)
+    DBuf<float4> jGear;                  // gear state: persistent angle 0, 1, virtual angle 0, 1
+    DBuf<int32_t> jGearInit;
 
     // last step stats
     int lastPairs = 0, lastManifolds = 0, lastColors = 0, lastRounds = 0, lastPoints = 0;
@@ -253,7 +264,8 @@ void world_step(World& w, float dt);
 int report_error(int code, const std::string& msg);   // sets pgx_last_error(), returns code
 
 // solver.hip
-int sv_setup_manifolds(World& w, int nPairs);          // returns manifold count
+int sv_setup_manifolds(World& w, int nPairs);
+void sv_color_points(World& w, int nColors);        // contact points per colour -> colorPoints          // returns manifold count
 int sv_color(World& w, int nManifolds);                 // returns colour count
 void sv_substeps(World& w, int nManifolds, float dt);
 void sv_finish_step(World& w, int nManifolds);

@@ -48,6 +48,7 @@ struct Request {
     double footprint = 0;
     int state = 0;                      // 0 queued, 1 active, 2 done (may still ride in the batch)
     int dynStart = 0;                   // placement in the current batched world
+    int jointStart = 0;                 // first joint in the current batched world
     std::vector<float4> rpos, rquat, rvel, rang;   // final dynamic state (done)
 };
 
@@ -175,6 +176,7 @@ void build_combined(pgx_server& S, Combined& c) {
         c.colTrigger.insert(c.colTrigger.end(), h.colTrigger.begin(), h.colTrigger.end());
         c.colEnable.insert(c.colEnable.end(), h.colEnable.begin(), h.colEnable.end());
         c.colData.insert(c.colData.end(), h.colData.begin(), h.colData.end());
+        r.jointStart = (int)c.joints.size();
         for (pgx_joint_desc j : h.joints) {
             j.actor0 += aOff;
             j.actor1 += aOff;
@@ -243,6 +245,17 @@ void rebuild(pgx_server& S, const std::vector<std::pair<int, int>>& continuing, 
                            w.bpos.p, w.bquat.p, w.dvel.p, w.dang.p, S.tPos.p, S.tQuat.p, S.tVel.p, S.tAng.p);
         PGX_LAUNCH_CHECK();
     }
+    // gear joints keep angles across steps (GearJoint.h:7-14): stage every joint's state
+    std::vector<float4> gearOld;
+    std::vector<int32_t> gearInitOld;
+    std::vector<int> oldJointStart(continuing.size());
+    for (size_t i = 0; i < continuing.size(); ++i) oldJointStart[i] = S.reqs[continuing[i].second].jointStart;
+    if (!continuing.empty() && w.J > 0) {
+        gearOld.resize(w.J);
+        gearInitOld.resize(w.J);
+        PGX_HIP(hipMemcpyAsync(gearOld.data(), w.jGear.p, sizeof(float4) * w.J, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(gearInitOld.data(), w.jGearInit.p, sizeof(int32_t) * w.J, hipMemcpyDeviceToHost, s));
+    }
     int nCache = continuing.empty() ? 0 : w.cacheCount[w.cacheCur];
     if (nCache) {
         int cur = w.cacheCur;
@@ -260,6 +273,20 @@ void rebuild(pgx_server& S, const std::vector<std::pair<int, int>>& continuing, 
     build_combined(S, c);
     world_init(w, &S.cfg, &c.desc);
     if (!c.joints.empty()) world_add_joints(w, (int)c.joints.size(), c.joints.data());
+    if (!gearOld.empty() && w.J > 0) {
+        std::vector<float4> gearNew(w.J, make_float4(0.f, 0.f, 0.f, 0.f));
+        std::vector<int32_t> gearInitNew(w.J, 0);
+        for (size_t i = 0; i < continuing.size(); ++i) {
+            const Request& r = S.reqs[continuing[i].second];
+            for (size_t k = 0; k < r.sc.joints.size(); ++k) {
+                gearNew[r.jointStart + k] = gearOld[oldJointStart[i] + k];
+                gearInitNew[r.jointStart + k] = gearInitOld[oldJointStart[i] + k];
+            }
+        }
+        PGX_HIP(hipMemcpyAsync(w.jGear.p, gearNew.data(), sizeof(float4) * w.J, hipMemcpyHostToDevice, s));
+        PGX_HIP(hipMemcpyAsync(w.jGearInit.p, gearInitNew.data(), sizeof(int32_t) * w.J, hipMemcpyHostToDevice, s));
+        PGX_HIP(hipStreamSynchronize(s));   // the host vectors end with this scope
+    }
     // 3. put the staged state back at the continuing requests' new placements
     std::vector<int> newWorldOf(oldW, -1);
     std::vector<uint8_t> mask(std::max(1, w.W), 0);
@@ -391,7 +418,7 @@ int pgx_server_submit(pgx_server* S, const pgx_scene_desc* sc, const pgx_joint_d
             return srv_fail(PGX_ERR_INVALID, "collider actor out of range");
     for (int j = 0; j < n_joints; ++j)
         if (joints[j].actor0 < 0 || joints[j].actor0 >= sc->n_actors || joints[j].actor1 < 0 ||
-            joints[j].actor1 >= sc->n_actors || joints[j].type < PGX_JOINT_SPHERICAL || joints[j].type > PGX_JOINT_FIXED)
+            joints[j].actor1 >= sc->n_actors || joints[j].type < PGX_JOINT_SPHERICAL || joints[j].type > PGX_JOINT_SERVO)
             return srv_fail(PGX_ERR_INVALID, "joint " + std::to_string(j) + " invalid");
     return srv_guarded([&] {
         Request r;

@@ -16,6 +16,7 @@
 // "colour 0, colour 1, ..." and is exported (pgx_world_get_manifolds) so that a
 // sequential solver fed this order reproduces the device result bit for bit.
 #include <hip/hip_cooperative_groups.h>
+#include <hip/hip_ext.h>
 #include "pgx_internal.h"
 
 namespace cg = cooperative_groups;
@@ -103,19 +104,29 @@ __global__ void k_color_claim(int M, int round, const uint64_t* __restrict__ mfK
     if (b >= 0) atomicMax(&claim[b], v);
 }
 
-__global__ void k_color_pick(int M, int round, const uint64_t* __restrict__ mfKey, int worldShift,
-                             const int32_t* __restrict__ wstart, int32_t* __restrict__ color,
-                             const int32_t* __restrict__ b0, const int32_t* __restrict__ b1,
-                             const unsigned long long* __restrict__ claim, uint64_t* __restrict__ used,
-                             int32_t* __restrict__ counters) {
+// One Jones-Plassmann round r: a manifold whose round-r claim (priority tagged with r)
+// holds on both its dynamic bodies takes the smallest colour free on both; every other
+// uncoloured manifold claims its bodies for round r+1 in the other claim buffer and is
+// marked in uncol[r]. Rounds after the last one that left manifolds uncoloured exit at
+// once, so a batch of rounds can be queued without asking the host in between.
+__global__ void k_color_round(int M, int round, const uint64_t* __restrict__ mfKey, int worldShift,
+                              const int32_t* __restrict__ wstart, int32_t* __restrict__ color,
+                              const int32_t* __restrict__ b0, const int32_t* __restrict__ b1,
+                              const unsigned long long* __restrict__ claimIn, unsigned long long* __restrict__ claimOut,
+                              uint64_t* __restrict__ used, int32_t* __restrict__ uncol, int32_t* __restrict__ counters) {
+    if (round > 1 && uncol[round - 1] == 0) return;
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M || color[m] >= 0) return;
     int wd = (int)(mfKey[m] >> worldShift);
-    unsigned long long v = ((unsigned long long)round << 32) | mix32((uint32_t)(m - wstart[wd]));
+    uint32_t prio = mix32((uint32_t)(m - wstart[wd]));
+    unsigned long long v = ((unsigned long long)round << 32) | prio;
     int a = b0[m], b = b1[m];
-    bool won = (a < 0 || claim[a] == v) && (b < 0 || claim[b] == v);
+    bool won = (a < 0 || claimIn[a] == v) && (b < 0 || claimIn[b] == v);
     if (!won) {
-        atomicAdd(&counters[3], 1);
+        unsigned long long vn = ((unsigned long long)(round + 1) << 32) | prio;
+        if (a >= 0) atomicMax(&claimOut[a], vn);
+        if (b >= 0) atomicMax(&claimOut[b], vn);
+        uncol[round] = 1;   // any writer writes 1: no atomic needed
         return;
     }
     uint64_t lo = 0, hi = 0;
@@ -162,7 +173,7 @@ __global__ __launch_bounds__(256) void k_setup(
     int32_t* __restrict__ curTri, int32_t* __restrict__ curNum,
     float4* __restrict__ curLp, int32_t* __restrict__ sB0, int32_t* __restrict__ sB1, int32_t* __restrict__ sA0,
     int32_t* __restrict__ sA1, int32_t* __restrict__ sFlags, int32_t* __restrict__ sCanon, float4* __restrict__ sN,
-    float2* __restrict__ sSoft, float4* __restrict__ sR0, float4* __restrict__ sR1, int32_t* __restrict__ colorPoints) {
+    float2* __restrict__ sSoft, float4* __restrict__ sR0, float4* __restrict__ sR1) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     int m = perm[s];
@@ -245,7 +256,30 @@ __global__ __launch_bounds__(256) void k_setup(
     sCanon[s] = m;
     sN[s] = f4(n, friction);
     sSoft[s] = make_float2(freq, damp);
-    atomicAdd(&colorPoints[mfColor[m]], num);
+}
+
+// contact points per colour (statistics): one block per colour sums its manifolds' counts
+__global__ __launch_bounds__(256) void k_color_points(const int32_t* __restrict__ colorStart,
+                                                      const int32_t* __restrict__ sFlags, int32_t* __restrict__ out) {
+    __shared__ int32_t part[256];
+    int c = blockIdx.x;
+    int b = colorStart[c], e = colorStart[c + 1];
+    int sum = 0;
+    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) sum += sFlags[i] & 255;
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = part[0];
+}
+
+void sv_color_points(World& w, int nColors) {
+    if (nColors <= 0) return;
+    hipLaunchKernelGGL(k_color_points, dim3(nColors), dim3(256), 0, w.stream, w.colorCount.p, w.sFlags.p,
+                       w.colorPoints.p);
+    PGX_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------------ substep pieces
@@ -581,26 +615,44 @@ int sv_color(World& w, int M) {
     fill_u64(s, w.claim.p, 0ull, w.D);
     fill_u64(s, w.usedMask.p, 0ull, 2 * w.D);
     fill_i32(s, w.counters.p + 5, 0, 1);
+    constexpr int MAX_ROUNDS = 1024;
+    w.colorUncol.alloc(MAX_ROUNDS + 1);
+    fill_i32(s, w.colorUncol.p, 0, MAX_ROUNDS + 1);
+    w.claim2.alloc(std::max(w.D, 1));
+    fill_u64(s, w.claim2.p, 0ull, w.D);
+    unsigned long long* claims[2] = {(unsigned long long*)w.claim.p, (unsigned long long*)w.claim2.p};
+    // round-1 claims
+    hipLaunchKernelGGL(k_color_claim, dim3(nb), dim3(256), 0, s, M, 1, w.mfKey.p, worldShift, w.worldMStart.p,
+                       w.mfColor.p, w.mfB0.p, w.mfB1.p, claims[1]);
     int round = 0;
-    const int batch = 16;
+    int batch = 32;
     while (true) {
-        for (int r = 0; r < batch; ++r) {
+        for (int r = 0; r < batch && round < MAX_ROUNDS; ++r) {
             ++round;
-            fill_i32(s, w.counters.p + 3, 0, 1);
-            hipLaunchKernelGGL(k_color_claim, dim3(nb), dim3(256), 0, s, M, round, w.mfKey.p, worldShift,
+            hipLaunchKernelGGL(k_color_round, dim3(nb), dim3(256), 0, s, M, round, w.mfKey.p, worldShift,
                                w.worldMStart.p, w.mfColor.p, w.mfB0.p, w.mfB1.p,
-                               (unsigned long long*)w.claim.p);
-            hipLaunchKernelGGL(k_color_pick, dim3(nb), dim3(256), 0, s, M, round, w.mfKey.p, worldShift,
-                               w.worldMStart.p, w.mfColor.p, w.mfB0.p, w.mfB1.p,
-                               (const unsigned long long*)w.claim.p, w.usedMask.p, w.counters.p);
+                               (const unsigned long long*)claims[round & 1], claims[(round + 1) & 1], w.usedMask.p,
+                               w.colorUncol.p, w.counters.p);
         }
         PGX_LAUNCH_CHECK();
-        int rem[3] = {0, 0, 0};
-        PGX_HIP(hipMemcpyAsync(rem, w.counters.p + 3, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+        int rem = 0, ovf = 0;
+        PGX_HIP(hipMemcpyAsync(&rem, w.colorUncol.p + round, sizeof(int), hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(&ovf, w.counters.p + 5, sizeof(int), hipMemcpyDeviceToHost, s));
         PGX_HIP(hipStreamSynchronize(s));
-        if (rem[2]) throw PgxError{PGX_ERR_CAPACITY, "a body touches more than 127 contact colours"};
-        if (rem[0] == 0) break;
-        if (round > 4096) throw PgxError{PGX_ERR_INVALID, "graph colouring did not converge"};
+        if (ovf) throw PgxError{PGX_ERR_CAPACITY, "a body touches more than 127 contact colours"};
+        if (rem == 0) break;
+        if (round >= MAX_ROUNDS) throw PgxError{PGX_ERR_INVALID, "graph colouring did not converge"};
+        batch = 16;
+    }
+    // rounds actually used: the first whose count of uncoloured manifolds is zero
+    {
+        std::vector<int> uc(round + 1, 0);
+        PGX_HIP(hipMemcpyAsync(uc.data(), w.colorUncol.p, sizeof(int) * (round + 1), hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipStreamSynchronize(s));
+        int used = round;
+        for (int r = 1; r <= round; ++r)
+            if (uc[r] == 0) { used = r; break; }
+        round = used;
     }
     // stable sort by colour -> solve order
     hipLaunchKernelGGL(k_color_keys, dim3(nb), dim3(256), 0, s, M, w.mfColor.p, w.colorKeys.p, w.colorPerm.p);
@@ -608,17 +660,15 @@ int sv_color(World& w, int M) {
     hipLaunchKernelGGL(k_color_start, dim3(nb), dim3(256), 0, s, M, w.colorKeys.p, w.colorCount.p);
     PGX_LAUNCH_CHECK();
     uint64_t lastKey = 0;
+    std::vector<int32_t> starts(130, 0);
     PGX_HIP(hipMemcpyAsync(&lastKey, w.colorKeys.p + (M - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    PGX_HIP(hipMemcpyAsync(starts.data(), w.colorCount.p, sizeof(int32_t) * 130, hipMemcpyDeviceToHost, s));
     PGX_HIP(hipStreamSynchronize(s));
     int nColors = (int)lastKey + 1;
-    w.colorStartHost.resize(nColors + 1);
-    PGX_HIP(hipMemcpyAsync(w.colorStartHost.data(), w.colorCount.p, sizeof(int) * (nColors + 1),
-                           hipMemcpyDeviceToHost, s));
-    PGX_HIP(hipStreamSynchronize(s));
+    w.colorStartHost.assign(starts.begin(), starts.begin() + nColors + 1);
     w.lastRounds = round;
     // contact setup, written in solve order
     int cur = w.cacheCur ^ 1, prv = w.cacheCur;
-    fill_i32(s, w.colorPoints.p, 0, 160);
     hipLaunchKernelGGL(k_setup, dim3(nb), dim3(256), 0, s, M, w.manifoldCap, w.colorPerm.p, w.mfSrc.p, w.mfTri.p,
                        w.mfKey.p,
                        w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.cflags.p, w.cmat.p, w.npNum.p,
@@ -627,7 +677,7 @@ int sv_color(World& w, int M) {
                        w.cacheLp[prv].p, w.cacheCount[prv], w.manifoldCap, w.cacheKey[cur].p, w.cacheTri[cur].p,
                        w.cacheNum[cur].p, w.cacheLp[cur].p,
                        w.sB0.p, w.sB1.p, w.sA0.p, w.sA1.p, w.sFlags.p, w.sCanon.p, w.sN.p, w.sSoft.p, w.sR0.p,
-                       w.sR1.p, w.colorPoints.p);
+                       w.sR1.p);
     PGX_LAUNCH_CHECK();
     return nColors;
 }
@@ -658,21 +708,22 @@ static int persistent_blocks(World& w) {
     return w.persistentBlocks;
 }
 
-// HIP events around one k_solve launch (timing mode, first substep of the step)
-static void solve_event(World& w, bool rec, int color, bool after) {
-    if (!rec) return;
-    int idx = 2 * w.solveEvUsed + (after ? 1 : 0);
-    while ((int)w.solveEv.size() <= idx) {
+// The start/stop events of one timed k_solve launch (timing mode, first substep of the
+// step). They go to hipExtLaunchKernelGGL, which stamps them from the kernel's own dispatch
+// packet: the kernel's execution time, without the launch latency an event recorded on the
+// stream before and after the launch would add.
+static void solve_events(World& w, int color, hipEvent_t* start, hipEvent_t* stop) {
+    int idx = 2 * w.solveEvUsed;
+    while ((int)w.solveEv.size() <= idx + 1) {
         hipEvent_t e;
         PGX_HIP(hipEventCreate(&e));
         w.solveEv.push_back(e);
     }
-    PGX_HIP(hipEventRecord(w.solveEv[idx], w.stream));
-    if (after) {
-        if ((int)w.solveEvColor.size() <= w.solveEvUsed) w.solveEvColor.resize(w.solveEvUsed + 1);
-        w.solveEvColor[w.solveEvUsed] = color;
-        ++w.solveEvUsed;
-    }
+    *start = w.solveEv[idx];
+    *stop = w.solveEv[idx + 1];
+    if ((int)w.solveEvColor.size() <= w.solveEvUsed) w.solveEvColor.resize(w.solveEvUsed + 1);
+    w.solveEvColor[w.solveEvUsed] = color;
+    ++w.solveEvUsed;
 }
 
 void sv_substeps(World& w, int M, float dt) {
@@ -689,6 +740,22 @@ void sv_substeps(World& w, int M, float dt) {
     }
     int nbP = div_up(std::max(M + w.D, 1), 256), nbD = div_up(std::max(w.D, 1), 256);
     int nColors = a.nColors;
+    // one launch per colour: a colour's cost is the latency of one manifold solve, which a
+    // launch across the chip pays once for all of the colour's manifolds
+    auto pass = [&](bool rec, int useBias, float h, int skipSoft) {
+        for (int c = 0; c < nColors; ++c) {
+            int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
+            if (e <= b) continue;
+            if (rec) {
+                hipEvent_t e0, e1;
+                solve_events(w, c, &e0, &e1);
+                hipExtLaunchKernelGGL(k_solve, dim3(div_up(e - b, 64)), dim3(64), 0, s, e0, e1, 0, a, b, e, useBias,
+                                      h, skipSoft);
+            } else {
+                hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 64)), dim3(64), 0, s, a, b, e, useBias, h, skipSoft);
+            }
+        }
+    };
     jt_step_setup(w);
     for (int sub = 0; sub < w.substeps; ++sub) {
         const bool rec = w.timing && sub == 0;
@@ -698,23 +765,11 @@ void sv_substeps(World& w, int M, float dt) {
         jt_update(w, a.h);
         jt_colors(w, a.h, 0, false);
         for (int it = 0; it < w.iterations; ++it) {
-            for (int c = 0; c < nColors; ++c) {
-                int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
-                if (e <= b) continue;
-                solve_event(w, rec, c, false);
-                hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 64)), dim3(64), 0, s, a, b, e, 1, a.h, 0);
-                solve_event(w, rec, c, true);
-            }
+            pass(rec, 1, a.h, 0);
             jt_colors(w, a.h, 1, it == 0);   // joint colours after the contacts (Physecs.cpp:482-490)
         }
         hipLaunchKernelGGL(k_integrate_positions, dim3(nbD), dim3(256), 0, s, a);
-        for (int c = 0; c < nColors; ++c) {
-            int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
-            if (e <= b) continue;
-            solve_event(w, rec, c, false);
-            hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 64)), dim3(64), 0, s, a, b, e, 0, 0.f, 1);
-            solve_event(w, rec, c, true);
-        }
+        pass(rec, 0, 0.f, 1);
         hipLaunchKernelGGL(k_writeback, dim3(nbD), dim3(256), 0, s, a);
         PGX_LAUNCH_CHECK();
     }

@@ -1,136 +1,24 @@
-// Device-wide primitives: stable LSD radix sort of 64-bit keys with 32-bit values, and an
-// exclusive scan. Both are written for 64-lane CDNA wavefronts.
-//
-// Radix sort, per 8-bit digit:
-//   1. hist:    each block counts its tile's digits in LDS  -> hist[digit * nBlocks + block]
-//   2. scan:    exclusive scan of hist (digit-major), one block
-//   3. scatter: each block ranks its tile stably. A tile is walked in rounds of 256 keys
-//               (4 waves x 64 lanes); inside a wave, lanes holding the same digit are found
-//               with 8 ballots (one per digit bit), so a key's rank is a popcount over the
-//               lanes below it; per-wave digit counts in LDS give the offsets across waves
-//               and rounds. Keys keep their input order within a digit (stable).
+// Device-wide primitives: stable radix sort of 64-bit keys with 32-bit values (rocPRIM's
+// onesweep radix sort through hipCUB, which is stable: equal keys keep their input order),
+// and an exclusive scan written for 64-lane CDNA wavefronts.
+#include <hipcub/hipcub.hpp>
 #include "pgx_internal.h"
 
 namespace pgx {
 
-constexpr int SORT_THREADS = 256;
-constexpr int SORT_ITEMS = 8;
-constexpr int SORT_TILE = SORT_THREADS * SORT_ITEMS;   // 2048 keys per block
-constexpr int SORT_WAVES = SORT_THREADS / WAVE;
-
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_hist(const uint64_t* __restrict__ keys, int n, int shift,
-                                                            uint32_t* __restrict__ hist, int nBlocks) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    int base = blockIdx.x * SORT_TILE;
-    for (int i = threadIdx.x; i < SORT_TILE; i += SORT_THREADS) {
-        int idx = base + i;
-        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    hist[threadIdx.x * nBlocks + blockIdx.x] = h[threadIdx.x];
-}
-
-// single block exclusive scan of m uint32 values (in place)
-__global__ __launch_bounds__(1024) void k_scan_single_u32(uint32_t* __restrict__ a, int m) {
-    __shared__ uint32_t partial[1024];
-    int per = (m + 1023) / 1024;
-    int beg = threadIdx.x * per;
-    int end = min(beg + per, m);
-    uint32_t s = 0;
-    for (int i = beg; i < end; ++i) s += a[i];
-    partial[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t v = threadIdx.x >= off ? partial[threadIdx.x - off] : 0;
-        __syncthreads();
-        partial[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = partial[threadIdx.x] - s;
-    for (int i = beg; i < end; ++i) {
-        uint32_t v = a[i];
-        a[i] = run;
-        run += v;
-    }
-}
-
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_scatter(const uint64_t* __restrict__ keys,
-                                                               const int32_t* __restrict__ vals, int n, int shift,
-                                                               const uint32_t* __restrict__ hist, int nBlocks,
-                                                               uint64_t* __restrict__ okeys,
-                                                               int32_t* __restrict__ ovals) {
-    __shared__ uint32_t blockOff[256];      // global offset of this block's digits + running count
-    __shared__ uint32_t waveCnt[SORT_WAVES][256];
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int wv = threadIdx.x / WAVE;
-    blockOff[threadIdx.x] = hist[threadIdx.x * nBlocks + blockIdx.x];
-    for (int k = 0; k < SORT_WAVES; ++k) waveCnt[k][threadIdx.x] = 0;
-    __syncthreads();
-    const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int base = blockIdx.x * SORT_TILE;
-    for (int r = 0; r < SORT_ITEMS; ++r) {
-        int idx = base + r * SORT_THREADS + threadIdx.x;
-        bool valid = idx < n;
-        uint64_t key = valid ? keys[idx] : 0;
-        int32_t val = valid ? vals[idx] : 0;
-        uint32_t digit = valid ? (uint32_t)((key >> shift) & 255u) : 0u;
-        // lanes of this wave with the same digit
-        uint64_t peers = __ballot(valid);
-        for (int b = 0; b < 8; ++b) {
-            uint64_t m = __ballot((digit >> b) & 1u);
-            peers &= ((digit >> b) & 1u) ? m : ~m;
-        }
-        int rankInWave = __popcll(peers & lanesBelow);
-        if (valid && rankInWave == 0) waveCnt[wv][digit] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        // thread t owns digit t: exclusive prefix across waves, then advance the block offset
-        {
-            uint32_t run = blockOff[threadIdx.x];
-            for (int k = 0; k < SORT_WAVES; ++k) {
-                uint32_t c = waveCnt[k][threadIdx.x];
-                waveCnt[k][threadIdx.x] = run;
-                run += c;
-            }
-            blockOff[threadIdx.x] = run;
-        }
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = waveCnt[wv][digit] + (uint32_t)rankInWave;
-            okeys[pos] = key;
-            ovals[pos] = val;
-        }
-        __syncthreads();
-        for (int k = 0; k < SORT_WAVES; ++k) waveCnt[k][threadIdx.x] = 0;
-        __syncthreads();
-    }
-}
-
 void radix_sort_u64(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t) {
     if (n <= 1 || bits <= 0) return;
-    int nBlocks = div_up(n, SORT_TILE);
-    t.hist.alloc((size_t)256 * nBlocks);
     t.kalt.alloc(n);
     t.valt.alloc(n);
-    uint64_t* ka = keys;
-    int32_t* va = vals;
-    uint64_t* kb = t.kalt.p;
-    int32_t* vb = t.valt.p;
-    int passes = (bits + 7) / 8;
-    for (int p = 0; p < passes; ++p) {
-        int shift = 8 * p;
-        hipLaunchKernelGGL(k_sort_hist, dim3(nBlocks), dim3(SORT_THREADS), 0, s, ka, n, shift, t.hist.p, nBlocks);
-        hipLaunchKernelGGL(k_scan_single_u32, dim3(1), dim3(1024), 0, s, t.hist.p, 256 * nBlocks);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nBlocks), dim3(SORT_THREADS), 0, s, ka, va, n, shift, t.hist.p,
-                           nBlocks, kb, vb);
-        std::swap(ka, kb);
-        std::swap(va, vb);
-    }
-    PGX_LAUNCH_CHECK();
-    if (ka != keys) {
-        PGX_HIP(hipMemcpyAsync(keys, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
-        PGX_HIP(hipMemcpyAsync(vals, va, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+    hipcub::DoubleBuffer<uint64_t> dk(keys, t.kalt.p);
+    hipcub::DoubleBuffer<int32_t> dv(vals, t.valt.p);
+    size_t bytes = 0;
+    PGX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, dk, dv, n, 0, bits, s));
+    t.temp.alloc(std::max<size_t>(bytes, 1));
+    PGX_HIP(hipcub::DeviceRadixSort::SortPairs(t.temp.p, bytes, dk, dv, n, 0, bits, s));
+    if (dk.Current() != keys) {
+        PGX_HIP(hipMemcpyAsync(keys, dk.Current(), sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
+        PGX_HIP(hipMemcpyAsync(vals, dv.Current(), sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
     }
 }
 

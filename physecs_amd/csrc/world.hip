@@ -134,8 +134,9 @@ namespace pgx {
 // non-colliding pairs, contact cache, statistics) is cleared.
 void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     w.J = 0; w.jCap = 0; w.jRowsTotal = 0;
-    for (auto* v : {&w.hjType, &w.hjA0, &w.hjA1, &w.hjB0, &w.hjB1, &w.hjRowFlags, &w.hjNRows, &w.hjColor}) v->clear();
-    for (auto* v : {&w.hjP0, &w.hjQ0, &w.hjP1, &w.hjQ1, &w.hjDrive}) v->clear();
+    for (auto* v : {&w.hjType, &w.hjA0, &w.hjA1, &w.hjB0, &w.hjB1, &w.hjRowFlags, &w.hjNRows, &w.hjColor, &w.hjAlive})
+        v->clear();
+    for (auto* v : {&w.hjP0, &w.hjQ0, &w.hjP1, &w.hjQ1, &w.hjDrive, &w.hjPar0, &w.hjPar1}) v->clear();
     w.hjBits.clear();
     w.jColorStart.clear();
     w.hostNonColliding.clear();
@@ -242,6 +243,9 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     size_t ni = C > 1 ? C - 1 : 1;
     w.nodeChild.alloc(2 * ni); w.nodeParent.alloc(ni); w.nodeEsc.alloc(ni); w.nodeFlags.alloc(ni);
     w.nodeCounter.alloc(ni); w.nodeRange.alloc(ni); w.nodeMin.alloc(ni); w.nodeMax.alloc(ni);
+    w.nodeA.alloc(ni); w.nodeB.alloc(ni); w.nodeLast.alloc(ni);
+    w.leafA.alloc(C); w.leafB.alloc(C); w.leafI.alloc(C);
+    w.cutNode.alloc(8); w.cutStop.alloc(8);
     w.worldRoot.alloc(W); w.worldStop.alloc(W); w.worldMStart.alloc(W);
     w.pairCap = (cfg && cfg->pair_capacity > 0) ? cfg->pair_capacity : std::max(1024, 12 * C);
     // manifolds of triangle-mesh pairs are stored after the pair slots
@@ -362,7 +366,7 @@ static void release_world(World& w) {
     DBuf<uint64_t>* u64s[] = {&w.mkeys, &w.pkeys, &w.mfKey, &w.claim, &w.usedMask, &w.colorKeys, &w.cacheKey[0],
                               &w.cacheKey[1], &w.nonColliding, &w.sortT.kalt};
     for (auto* b : u64s) b->release();
-    w.bentity.release(); w.centity.release(); w.worldBoundsI.release(); w.sortT.hist.release();
+    w.bentity.release(); w.centity.release(); w.worldBoundsI.release(); w.sortT.temp.release();
     w.nodeRange.release(); w.sSoft.release();
     DBuf<int32_t>* ji[] = {&w.jType, &w.jA0, &w.jA1, &w.jB0, &w.jB1, &w.jRowFlags, &w.jNRows, &w.jOrder};
     for (auto* b : ji) b->release();
@@ -414,10 +418,86 @@ int pgx_world_set_can_collide(pgx_world* h, uint32_t e0, uint32_t e1, int canCol
 
 namespace pgx {
 
+// The rows a joint declares, in creation order, as packed 4-bit flags (src/Joints/*.cpp
+// getSolverDesc). A prismatic joint's limit row is decided on the device every step; the
+// host layout holds its fixed rows and drive.
+static int joint_layout(int type, bool drive, int* packed) {
+    int fl[7], nr = 0;
+    if (type == PGX_JOINT_PRISMATIC) {
+        fl[nr++] = 0; fl[nr++] = 0; fl[nr++] = 2; fl[nr++] = 2; fl[nr++] = 2;
+        if (drive) fl[nr++] = 1;                               // SOFT
+    } else if (type == PGX_JOINT_GEAR) {
+        fl[nr++] = 2;
+    } else {
+        for (int r = 0; r < 3; ++r) fl[nr++] = 0;              // point-to-point: NONE
+        if (type == PGX_JOINT_UNIVERSAL) fl[nr++] = 2;         // ANGULAR
+        if (type == PGX_JOINT_REVOLUTE) { fl[nr++] = 2; fl[nr++] = 2; if (drive) fl[nr++] = 2 | 4; }
+        if (type == PGX_JOINT_FIXED) for (int r = 0; r < 3; ++r) fl[nr++] = 2;
+        if (type == PGX_JOINT_SERVO) { fl[nr++] = 2; fl[nr++] = 2; fl[nr++] = 2 | 1; }
+    }
+    *packed = 0;
+    for (int r = 0; r < nr; ++r) *packed |= fl[r] << (4 * r);
+    return nr;
+}
+
+static void joint_params(World& w, int i, const pgx_joint_desc& d) {
+    w.hjDrive[i] = make_float4(d.drive_enabled ? 1.f : 0.f, d.drive_velocity, d.drive_max_torque, 0.f);
+    w.hjPar0[i] = make_float4(d.lower_limit, d.upper_limit, d.target, d.drive_stiffness);
+    w.hjPar1[i] = make_float4(d.drive_damping, d.gear_ratio, 0.f, 0.f);
+    int packed;
+    w.hjNRows[i] = joint_layout(w.hjType[i], d.drive_enabled != 0, &packed);
+    w.hjRowFlags[i] = packed;
+}
+
+// Grows a per-joint device buffer to n entries, keeping the first `keep` (state that lives
+// on the device, such as the gear angles) and zeroing the rest.
+template <class T>
+static void grow_keep(DBuf<T>& b, int keep, int n, hipStream_t s) {
+    T* np = nullptr;
+    PGX_HIP(hipMalloc((void**)&np, sizeof(T) * std::max(n, 1)));
+    PGX_HIP(hipMemsetAsync(np, 0, sizeof(T) * std::max(n, 1), s));
+    if (keep > 0 && b.p) PGX_HIP(hipMemcpyAsync(np, b.p, sizeof(T) * keep, hipMemcpyDeviceToDevice, s));
+    PGX_HIP(hipStreamSynchronize(s));
+    b.release();
+    b.p = np;
+    b.n = (size_t)std::max(n, 1);
+}
+
+// The live joints grouped by colour, creation order inside a colour, and the per-joint
+// arrays, to the device.
+static void upload_joints(World& w, int keepState) {
+    std::vector<int32_t> order;
+    order.reserve(w.J);
+    w.jColorStart.assign(10, 0);
+    w.jRowsTotal = 0;
+    for (int j = 0; j < w.J; ++j)
+        if (w.hjAlive[j]) { ++w.jColorStart[w.hjColor[j] + 1]; w.jRowsTotal += w.hjNRows[j]; }
+    for (int c = 0; c < 9; ++c) w.jColorStart[c + 1] += w.jColorStart[c];
+    order.assign(w.jColorStart[9], 0);
+    std::vector<int> fillp(w.jColorStart.begin(), w.jColorStart.end() - 1);
+    for (int j = 0; j < w.J; ++j)
+        if (w.hjAlive[j]) order[fillp[w.hjColor[j]]++] = j;
+    hipStream_t s = w.stream;
+    w.jCap = w.J;
+    upload(w.jType, w.hjType, s); upload(w.jA0, w.hjA0, s); upload(w.jA1, w.hjA1, s);
+    upload(w.jB0, w.hjB0, s); upload(w.jB1, w.hjB1, s); upload(w.jRowFlags, w.hjRowFlags, s);
+    upload(w.jNRows, w.hjNRows, s); upload(w.jOrder, order, s);
+    upload(w.jP0, w.hjP0, s); upload(w.jQ0, w.hjQ0, s); upload(w.jP1, w.hjP1, s); upload(w.jQ1, w.hjQ1, s);
+    upload(w.jDrive, w.hjDrive, s); upload(w.jPar0, w.hjPar0, s); upload(w.jPar1, w.hjPar1, s);
+    if (keepState < w.J) {
+        grow_keep(w.jGear, keepState, w.J, s);
+        grow_keep(w.jGearInit, keepState, w.J, s);
+    }
+    w.jR0.alloc(w.J); w.jR1.alloc(w.J);
+    w.jRow.alloc((size_t)JOINT_MAX_ROWS * 8 * w.J);
+    PGX_HIP(hipStreamSynchronize(s));
+}
+
 // Scene::createJoint for a batch of built-in joints (actor indices of the descriptor the
 // world was initialised from); colours as Scene::addJoint assigns them.
 void world_add_joints(World& w, int n, const pgx_joint_desc* js) {
     if (w.hjBits.empty()) w.hjBits.assign(w.A, 0);
+    int before = w.J;
     for (int i = 0; i < n; ++i) {
         const pgx_joint_desc& d = js[i];
         int a0 = w.actorToBody[d.actor0], a1 = w.actorToBody[d.actor1];
@@ -432,43 +512,21 @@ void world_add_joints(World& w, int n, const pgx_joint_desc* js) {
         while (!((inv >> color) & 1u)) ++color;
         w.hjBits[a0] = (unsigned char)(w.hjBits[a0] | (1u << color));
         w.hjBits[a1] = (unsigned char)(w.hjBits[a1] | (1u << color));
-        // rows in creation order, which is also flag-type order for these types
-        int fl[6], nr = 0;
-        for (int r = 0; r < 3; ++r) fl[nr++] = 0;                 // point-to-point: NONE
-        if (d.type == PGX_JOINT_UNIVERSAL) fl[nr++] = 2;          // ANGULAR
-        if (d.type == PGX_JOINT_REVOLUTE) { fl[nr++] = 2; fl[nr++] = 2; if (d.drive_enabled) fl[nr++] = 2 | 4; }
-        if (d.type == PGX_JOINT_FIXED) for (int r = 0; r < 3; ++r) fl[nr++] = 2;
-        int packed = 0;
-        for (int r = 0; r < nr; ++r) packed |= fl[r] << (4 * r);
         auto dyn = [&](int b) { return b < w.D && !w.hostKinematic[b] ? b : -1; };
         w.hjType.push_back(d.type); w.hjA0.push_back(a0); w.hjA1.push_back(a1);
         w.hjB0.push_back(dyn(a0)); w.hjB1.push_back(dyn(a1));
-        w.hjRowFlags.push_back(packed); w.hjNRows.push_back(nr); w.hjColor.push_back(color);
+        w.hjColor.push_back(color);
+        w.hjAlive.push_back(1);
         w.hjP0.push_back(make_float4(d.anchor0[0], d.anchor0[1], d.anchor0[2], 0.f));
         w.hjQ0.push_back(make_float4(d.anchor0[3], d.anchor0[4], d.anchor0[5], d.anchor0[6]));
         w.hjP1.push_back(make_float4(d.anchor1[0], d.anchor1[1], d.anchor1[2], 0.f));
         w.hjQ1.push_back(make_float4(d.anchor1[3], d.anchor1[4], d.anchor1[5], d.anchor1[6]));
-        w.hjDrive.push_back(make_float4(d.drive_enabled ? 1.f : 0.f, d.drive_velocity, d.drive_max_torque, 0.f));
-        w.jRowsTotal += nr;
+        w.hjDrive.emplace_back(); w.hjPar0.emplace_back(); w.hjPar1.emplace_back();
+        w.hjRowFlags.push_back(0); w.hjNRows.push_back(0);
+        joint_params(w, (int)w.hjType.size() - 1, d);
     }
     w.J = (int)w.hjType.size();
-    // joints grouped by colour, creation order inside a colour
-    std::vector<int32_t> order(w.J);
-    w.jColorStart.assign(10, 0);
-    for (int j = 0; j < w.J; ++j) ++w.jColorStart[w.hjColor[j] + 1];
-    for (int c = 0; c < 9; ++c) w.jColorStart[c + 1] += w.jColorStart[c];
-    std::vector<int> fillp(w.jColorStart.begin(), w.jColorStart.end() - 1);
-    for (int j = 0; j < w.J; ++j) order[fillp[w.hjColor[j]]++] = j;
-    hipStream_t s = w.stream;
-    w.jCap = w.J;
-    upload(w.jType, w.hjType, s); upload(w.jA0, w.hjA0, s); upload(w.jA1, w.hjA1, s);
-    upload(w.jB0, w.hjB0, s); upload(w.jB1, w.hjB1, s); upload(w.jRowFlags, w.hjRowFlags, s);
-    upload(w.jNRows, w.hjNRows, s); upload(w.jOrder, order, s);
-    upload(w.jP0, w.hjP0, s); upload(w.jQ0, w.hjQ0, s); upload(w.jP1, w.hjP1, s); upload(w.jQ1, w.hjQ1, s);
-    upload(w.jDrive, w.hjDrive, s);
-    w.jR0.alloc(w.J); w.jR1.alloc(w.J);
-    w.jRow.alloc((size_t)6 * 8 * w.J);
-    PGX_HIP(hipStreamSynchronize(s));
+    upload_joints(w, before);
 }
 
 }  // namespace pgx
@@ -480,14 +538,51 @@ int pgx_world_add_joints(pgx_world* h, int n, const pgx_joint_desc* js) {
     World& w = h->w;
     for (int i = 0; i < n; ++i) {
         const pgx_joint_desc& d = js[i];
-        if (d.type < PGX_JOINT_SPHERICAL || d.type > PGX_JOINT_FIXED)
-            return fail(PGX_ERR_UNSUPPORTED, "joint %d: type %d is not supported on the device path", i, d.type);
+        if (d.type < PGX_JOINT_SPHERICAL || d.type > PGX_JOINT_SERVO)
+            return fail(PGX_ERR_INVALID, "joint %d: unknown type %d", i, d.type);
         if (d.actor0 < 0 || d.actor0 >= w.A || d.actor1 < 0 || d.actor1 >= w.A)
             return fail(PGX_ERR_INVALID, "joint %d: actor out of range", i);
     }
     return guarded([&] {
         PGX_HIP(hipSetDevice(w.device));
         world_add_joints(w, n, js);
+        return PGX_OK;
+    });
+}
+
+int pgx_world_set_joint_params(pgx_world* h, int index, const pgx_joint_desc* d) {
+    if (!h || !d) return fail(PGX_ERR_INVALID, "NULL argument");
+    World& w = h->w;
+    if (index < 0 || index >= w.J || !w.hjAlive[index]) return fail(PGX_ERR_INVALID, "joint %d does not exist", index);
+    if (d->type != w.hjType[index]) return fail(PGX_ERR_INVALID, "joint %d: the type cannot change", index);
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(w.device));
+        PGX_HIP(hipStreamSynchronize(w.stream));   // the previous step reads the joint arrays
+        joint_params(w, index, *d);
+        upload_joints(w, w.J);
+        return PGX_OK;
+    });
+}
+
+int pgx_world_destroy_joint(pgx_world* h, int index) {
+    if (!h) return fail(PGX_ERR_INVALID, "NULL argument");
+    World& w = h->w;
+    if (index < 0 || index >= w.J || !w.hjAlive[index]) return fail(PGX_ERR_INVALID, "joint %d does not exist", index);
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(w.device));
+        PGX_HIP(hipStreamSynchronize(w.stream));
+        // Scene::destroyJoint (src/Physecs.cpp:712-725)
+        w.hjAlive[index] = 0;
+        int a0 = w.hjA0[index], a1 = w.hjA1[index];
+        uint32_t e0 = w.hostEntity[a0], e1 = w.hostEntity[a1];
+        uint64_t k = ((uint64_t)std::min(e0, e1) << 32) | std::max(e0, e1);
+        auto it = std::lower_bound(w.hostNonColliding.begin(), w.hostNonColliding.end(), k);
+        if (it != w.hostNonColliding.end() && *it == k) w.hostNonColliding.erase(it);
+        w.nonCollidingDirty = true;
+        unsigned char bit = (unsigned char)(1u << w.hjColor[index]);   // colour 8 sets no bit
+        w.hjBits[a0] = (unsigned char)(w.hjBits[a0] & ~bit);
+        w.hjBits[a1] = (unsigned char)(w.hjBits[a1] & ~bit);
+        upload_joints(w, w.J);
         return PGX_OK;
     });
 }
@@ -532,7 +627,9 @@ void world_step(World& w, float dt) {
         // the substep-loop kernel: duration (HIP events on the world's stream) and the
         // algorithmic bytes of this step's substep loop
         std::vector<int32_t> pts(160, 0);
-        PGX_HIP(hipMemcpy(pts.data(), w.colorPoints.p, sizeof(int32_t) * 160, hipMemcpyDeviceToHost));
+        sv_color_points(w, nColors);
+        PGX_HIP(hipMemcpyAsync(pts.data(), w.colorPoints.p, sizeof(int32_t) * 160, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipStreamSynchronize(s));
         long points = 0;
         for (int c = 0; c < nColors; ++c) points += pts[c];
         w.lastPointsHost = points;
@@ -650,6 +747,8 @@ int pgx_world_get_counts(pgx_world* h, pgx_counts* out) {
     out->points = 0;
     return guarded([&] {
         World& w = h->w;
+        PGX_HIP(hipSetDevice(w.device));
+        PGX_HIP(hipStreamSynchronize(w.stream));   // the queries read the last step's results
         int M = w.lastManifolds;
         if (M) {
             std::vector<int32_t> f(M);
@@ -666,6 +765,8 @@ int pgx_world_get_bounds(pgx_world* h, float* out) {
     if (!h || !out) return fail(PGX_ERR_INVALID, "NULL argument");
     return guarded([&] {
         World& w = h->w;
+        PGX_HIP(hipSetDevice(w.device));
+        PGX_HIP(hipStreamSynchronize(w.stream));   // the queries read the last step's results
         std::vector<float4> mn(w.C), mx(w.C);
         if (w.C) {
             PGX_HIP(hipMemcpy(mn.data(), w.cbmin.p, sizeof(float4) * w.C, hipMemcpyDeviceToHost));
@@ -683,6 +784,8 @@ int pgx_world_get_pairs(pgx_world* h, int32_t* out, int cap, int* n) {
     if (!h || !n) return fail(PGX_ERR_INVALID, "NULL argument");
     return guarded([&] {
         World& w = h->w;
+        PGX_HIP(hipSetDevice(w.device));
+        PGX_HIP(hipStreamSynchronize(w.stream));   // the queries read the last step's results
         int P = w.lastPairs;
         *n = P;
         if (!out || P == 0) return PGX_OK;
@@ -706,6 +809,8 @@ int pgx_world_get_manifolds(pgx_world* h, int32_t* keys, int32_t* ints, float* f
     if (!h || !n) return fail(PGX_ERR_INVALID, "NULL argument");
     return guarded([&] {
         World& w = h->w;
+        PGX_HIP(hipSetDevice(w.device));
+        PGX_HIP(hipStreamSynchronize(w.stream));   // the queries read the last step's results
         int M = w.lastManifolds;
         *n = M;
         if (M == 0) return PGX_OK;

@@ -16,7 +16,18 @@ import math
 import numpy as np
 
 SPHERE, CAPSULE, BOX, CONVEX_MESH, TRIANGLE_MESH = range(5)
-SPHERICAL, UNIVERSAL, REVOLUTE, FIXED = range(4)
+SPHERICAL, UNIVERSAL, REVOLUTE, FIXED, PRISMATIC, GEAR, SERVO = range(7)
+
+# Defaults of the joints' data members (include/Physecs/Joints/*.h)
+JOINT_DEFAULTS = {
+    REVOLUTE: dict(drive_enabled=0, drive_velocity=0.0, drive_max_torque=0.0),
+    PRISMATIC: dict(upper_limit=1.0, lower_limit=0.0, drive_enabled=0, target=0.0, drive_stiffness=5.0,
+                    drive_damping=1.0),
+    GEAR: dict(gear_ratio=1.0),
+    SERVO: dict(target=0.0, drive_stiffness=30.0, drive_damping=1.0),
+}
+JOINT_FIELDS = ("drive_enabled", "drive_velocity", "drive_max_torque", "lower_limit", "upper_limit", "target",
+                "drive_stiffness", "drive_damping", "gear_ratio")
 
 
 def _inertia_sphere(m, r):
@@ -195,7 +206,7 @@ class Scene:
         self._n = {"actor": 0, "dyn": 0, "col": 0}
         self._dyn_patch = {}          # actor -> dyn index, for actors added before their dynamic part
         self.non_colliding = []
-        self.joints = []        # (type, actor0, actor1, anchor0[7], anchor1[7], drive_enabled, drive_v, drive_max)
+        self.joints = []        # (type, actor0, actor1, anchor0[7], anchor1[7], params dict)
         self._ncol = {}
         self.n_worlds = 1
         self._frozen = None
@@ -348,23 +359,38 @@ class Scene:
         self.add_collider(a, CONVEX_MESH, [scale[0], scale[1], scale[2], float(mesh)], material)
         return a
 
-    def add_joint(self, jtype, a0, a1, p0, p1, q0=(0, 0, 0, 1), q1=(0, 0, 0, 1), drive=None):
+    def add_joint(self, jtype, a0, a1, p0, p1, q0=(0, 0, 0, 1), q1=(0, 0, 0, 1), drive=None, **params):
         """Scene::createJoint<T>(entity0, anchor0Pos, anchor0Or, entity1, anchor1Pos, anchor1Or);
-        anchors in body space. drive = (velocity, max torque) enables a revolute drive."""
+        anchors in body space. drive = (velocity, max torque) enables a revolute drive; other
+        keyword arguments set the joint's data members (JOINT_FIELDS), defaults as the
+        reference's. Returns the joint's index."""
         an0 = list(p0) + list(q0)
         an1 = list(p1) + list(q1)
-        d = (1, drive[0], drive[1]) if drive else (0, 0.0, 0.0)
-        self.joints.append((jtype, a0, a1, an0, an1) + d)
+        p = dict(JOINT_DEFAULTS.get(jtype, {}))
+        if drive:
+            p.update(drive_enabled=1, drive_velocity=drive[0], drive_max_torque=drive[1])
+        for k, v in params.items():
+            if k not in JOINT_FIELDS:
+                raise ValueError(f"unknown joint parameter {k}")
+            p[k] = v
+        self.joints.append((jtype, a0, a1, an0, an1, p))
+        return len(self.joints) - 1
+
+    @staticmethod
+    def fill_joint(d, t, a0, a1, an0, an1, p):
+        """Fills one ctypes joint descriptor (the same layout for the oracle and the C ABI)."""
+        d.type, d.actor0, d.actor1 = t, a0, a1
+        for k in range(7):
+            d.anchor0[k] = an0[k]
+            d.anchor1[k] = an1[k]
+        for f in JOINT_FIELDS:
+            setattr(d, f, p.get(f, 0))
 
     def joint_array(self, cls):
         """ctypes array of joint descriptors (same field layout for the oracle and the C ABI)."""
         arr = (cls * max(1, len(self.joints)))()
-        for i, (t, a0, a1, an0, an1, de, dv, dm) in enumerate(self.joints):
-            arr[i].type, arr[i].actor0, arr[i].actor1 = t, a0, a1
-            for k in range(7):
-                arr[i].anchor0[k] = an0[k]
-                arr[i].anchor1[k] = an1[k]
-            arr[i].drive_enabled, arr[i].drive_velocity, arr[i].drive_max_torque = de, dv, dm
+        for i, j in enumerate(self.joints):
+            self.fill_joint(arr[i], *j)
         return arr
 
     # -- views
@@ -474,9 +500,9 @@ class Scene:
             s.add_collider(amap[a], int(arr["col_type"][c]), arr["col_params"][c][:3], arr["col_material"][c],
                            arr["col_pos"][c], arr["col_quat"][c], bool(arr["col_trigger"][c]),
                            bool(arr["col_enable"][c]), int(arr["col_data"][c]))
-        for (t, a0, a1, an0, an1, de, dv, dm) in self.joints:
+        for (t, a0, a1, an0, an1, p) in self.joints:
             if a0 in amap and a1 in amap:
-                s.joints.append((t, amap[a0], amap[a1], an0, an1, de, dv, dm))
+                s.joints.append((t, amap[a0], amap[a1], an0, an1, dict(p)))
         return s
 
 
@@ -801,6 +827,59 @@ def joint_scene(seed=0, chains=6, ragdolls=2, ground=True):
         capsule_chain(s, [rng.uniform(-6, 3), top, rng.uniform(-6, 6)], n=8, axis="x" if horizontal else "y")
     for r in range(ragdolls):
         ragdoll(s, [rng.uniform(-5, 5), rng.uniform(0.5, 2.0), rng.uniform(-5, 5)])
+    return s
+
+
+def _axis_quat(axis, angle):
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    s = np.sin(angle / 2)
+    return [float(axis[0] * s), float(axis[1] * s), float(axis[2] * s), float(np.cos(angle / 2))]
+
+
+def mechanism_scene(seed=0, ground=True):
+    """Prismatic sliders and pistons, servo arms and gear pairs above a ground box, with
+    boxes dropped onto them: every joint type of the reference in one scene, anchor frames
+    tilted, limits and drives on. Returns the scene; its joints are in creation order."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    if ground:
+        s.add_static(BOX, [30, 0.5, 30], [0, -0.5, 0])
+    for i in range(4):
+        # a slider on a tilted rail from a static post; gravity pulls it to its lower limit
+        x = -8.0 + 2.5 * i
+        post = s.add_static(BOX, [0.1, 0.1, 0.1], [x, 3.0, -4.0])
+        q = _axis_quat([0, 0, 1], -np.pi / 2 + rng.uniform(0.3, 1.2))   # slide axis x tilted towards -y
+        body = s.add_body(BOX, [0.25, 0.15, 0.15], [x, 3.0, -4.0], random_quats(rng, 1)[0] if i == 3 else (0, 0, 0, 1),
+                          vel=rng.normal(size=3))
+        drive = dict(drive_enabled=1, target=float(rng.uniform(-1.0, 0.5))) if i % 2 else {}
+        s.add_joint(PRISMATIC, post, body, (0, 0, 0), (0, 0, 0), q, q, lower_limit=-1.5, upper_limit=0.8, **drive)
+    for i in range(3):
+        # a piston between two dynamic bodies
+        x = -6.0 + 3.0 * i
+        a = s.add_body(BOX, [0.3, 0.3, 0.3], [x, 1.0, 0.0])
+        b = s.add_body(CAPSULE, [0.4, 0.15], [x, 2.2, 0.0])
+        q = _axis_quat([0, 0, 1], np.pi / 2)                             # slide axis x -> y
+        s.add_joint(PRISMATIC, a, b, (0, 0.3, 0), (0, -0.6, 0), q, q, lower_limit=0.1, upper_limit=1.0,
+                    drive_enabled=int(i == 1), target=0.5, drive_stiffness=8.0, drive_damping=0.5)
+    for i in range(3):
+        # a servo arm on a static base
+        base = s.add_static(SPHERE, [0.1], [3.0 + 2.0 * i, 2.0, 3.0])
+        arm = s.add_body(BOX, [0.1, 0.6, 0.1], [3.0 + 2.0 * i, 1.4, 3.0])
+        s.add_joint(SERVO, base, arm, (0, 0, 0), (0, 0.6, 0), target=float(rng.uniform(-1.0, 1.0)),
+                    drive_stiffness=float(rng.uniform(5.0, 30.0)))
+    for i in range(2):
+        # two wheels on revolute joints to a frame, geared; one spinning
+        z = 6.0 + 3.0 * i
+        frame = s.add_static(BOX, [0.1, 0.1, 0.1], [0.0, 2.0, z])
+        w0 = s.add_body(CAPSULE, [0.05, 0.5], [0.5, 2.0, z - 0.6], (0, 0, 0, 1), angvel=(6.0 * (i + 1), 0, 0))
+        w1 = s.add_body(CAPSULE, [0.05, 0.5], [0.5, 2.0, z + 0.6])
+        s.add_joint(REVOLUTE, frame, w0, (0.5, 0, -0.6), (0, 0, 0))
+        s.add_joint(REVOLUTE, frame, w1, (0.5, 0, 0.6), (0, 0, 0))
+        s.add_joint(GEAR, w0, w1, (0, 0, 0), (0, 0, 0), gear_ratio=float(rng.uniform(0.5, 3.0)))
+    for i in range(12):
+        p = [rng.uniform(-8, 8), rng.uniform(3.5, 6.0), rng.uniform(-5, 9)]
+        s.add_body(BOX, list(rng.uniform(0.15, 0.35, 3)), p, random_quats(rng, 1)[0])
     return s
 
 

@@ -188,3 +188,28 @@ int main() {
     subprocess.run(["g++", "-O1", "-std=c++17", "-I", inc, str(src), "-o", str(exe)], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 0 and r.stdout.strip() == "ok", r.stdout
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror of a header struct has the C compiler's size and field offsets."""
+    import subprocess
+    from physecs_amd import api
+    mirrors = {"pgx_config": api.PgxConfig, "pgx_scene_desc": api.PgxSceneDesc, "pgx_joint_desc": api.PgxJointDesc,
+               "pgx_counts": api.PgxCounts, "pgx_stats": api.PgxStats, "pgx_server_config": api.PgxServerConfig,
+               "pgx_server_step_info": api.PgxServerStepInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "physecs_gpu.h"', "int main(void) {"]
+    for cname, cls in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(line.split()[:2]): int(line.split()[2]) for line in out if line}
+    for cname, cls in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)

@@ -143,6 +143,28 @@ def test_joint_overflow_colour_lockstep(api):
 
 
 @pytest.mark.parametrize("seed", [0, 1])
+def test_all_joint_types_lockstep_bitexact(api, seed):
+    """Prismatic sliders (limit rows decided per step, drives) and pistons, servo arms, gear
+    pairs on revolute wheels, boxes falling onto them. At step 40 the joints' setters change
+    a drive, a servo target and a gear ratio; at step 80 a piston is destroyed. Every step
+    bit-identical to the oracle."""
+    log(f"[lockstep] mechanism scene seed={seed}")
+    sc = scenes.mechanism_scene(seed)
+    assert {j[0] for j in sc.joints} == {scenes.PRISMATIC, scenes.SERVO, scenes.REVOLUTE, scenes.GEAR}
+
+    def hook(k, g, o):
+        if k == 39:
+            for w in (g, o):
+                w.set_joint_params(0, drive_enabled=1, target=-0.5)
+                w.set_joint_params(8, target=0.2)
+                w.set_joint_params(12, gear_ratio=1.5)
+        if k == 79:
+            for w in (g, o):
+                w.destroy_joint(5)
+    lockstep(sc, 120, on_step=hook)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
 def test_convex_manifolds_first_steps(api, seed):
     """Convex hulls among primitives: GJK/EPA in double precision on the device; pair sets
     and manifolds bit-identical to the oracle's over the first steps."""

@@ -52,6 +52,31 @@ def test_served_results_same_alone_and_in_mix(api, interval):
     srv.close()
 
 
+def test_served_mechanisms_keep_gear_state_across_boundaries(api):
+    """Requests with prismatic, servo and gear joints: a gear joint keeps angles across steps
+    (GearJoint.h:7-14), and the server carries them over when it rebuilds the batch for an
+    admission, so each result still equals the scene stepped alone."""
+    plan = [(0, 21, 40), (3, 22, 30), (6, 23, 18), (11, 24, 25)]   # (submit at step, seed, horizon)
+    srv = api.Server(substeps=4, iterations=2, admit_interval=2)
+    ids = {}
+    step = 0
+    pending = list(plan)
+    while pending or any(not srv.done(r) for r in ids.values()):
+        for item in [p for p in pending if p[0] == step]:
+            ids[item] = srv.submit(scenes.mechanism_scene(item[1]), item[2])
+            pending.remove(item)
+        info = srv.step(1 / 60)
+        log(f"[serving] mechanisms step {step}: {info}")
+        step += 1
+        assert step < 200
+    for item, rid in ids.items():
+        got = srv.take_result(rid)
+        ref = alone(api, scenes.mechanism_scene(item[1]), item[2])
+        for name, a, b in zip(("pos", "quat", "vel", "angvel"), got, ref):
+            assert np.array_equal(bits(a), bits(b)), (item, name)
+    srv.close()
+
+
 def test_server_budget_limits_admission(api):
     sc = scenes.served_request(3, n_bodies=64)
     fp = api.Server.footprint(sc)
