@@ -14,8 +14,11 @@ algorithmic bytes per launch are 284 B per contact point + 12 B per manifold of 
 (rows 108 + scalars 24 + both bodies' solver and component v,w 96 read; lambdas 8 + both
 bodies' v,w 48 written). Its duration comes from HIP start/stop events that
 hipExtLaunchKernelGGL stamps from each launch's own dispatch packet, for every launch of the
-first substep of every timed step, on the world's stream. (--persistent times the
-single-launch substep kernel k_step_persistent instead, with the per-substep phase model.)
+first substep of every timed step, on the world's stream. --exec flow launches one
+k_solve_flow per pass instead (every colour in one launch: a manifold solves once its
+bodies have taken the lower colours' solves), so a launch covers all of the pass's points
+and manifolds; --exec cooperative times the single-launch substep kernel k_step_persistent
+with the per-substep phase model.
 traffic: HBM bytes per k_solve launch from the committed PMC summary (profiles/), default
 workload only.
 
@@ -35,15 +38,17 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_solve_plain32k_traffic.md")
+TRAFFIC_SUMMARY = {"colors": os.path.join(ROOT, "profiles", "k_solve_plain32k_traffic.md")}
 
 
-def measured_traffic():
-    """HBM bytes per k_solve launch on the default workload, from the committed PMC summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same counter's reading for a
-    copy of known size); None when the summary is absent."""
+def measured_traffic(mode):
+    """HBM bytes per launch of the roofline kernel on the default workload, from the
+    committed PMC summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same
+    counter's reading for a copy of known size); None when there is no summary."""
+    if mode not in TRAFFIC_SUMMARY:
+        return None
     try:
-        with open(TRAFFIC_SUMMARY) as f:
+        with open(TRAFFIC_SUMMARY[mode]) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -188,8 +193,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="plain32k")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--blocks", type=int, default=0, help="persistent grid size (0 = one block per CU)")
-    ap.add_argument("--persistent", action="store_true", help="substep loop as one cooperative launch per step")
+    ap.add_argument("--blocks", type=int, default=0, help="cooperative grid size (0 = one block per CU)")
+    ap.add_argument("--exec", default="colors", choices=["flow", "colors", "cooperative"],
+                    help="how the contact passes launch")
     ap.add_argument("--seconds", type=float, default=20.0, help="served: length of the request stream")
     ap.add_argument("--admit-interval", type=int, default=8, help="served: steps between admission boundaries")
     ap.add_argument("--copy-calibration", action="store_true", help="run only the 1 GiB copy used to scale PMC counters")
@@ -207,8 +213,7 @@ def main():
     scene, cfg = make_scene(args.config)
     sub, it, dt = cfg["substeps"], cfg["iterations"], cfg["dt"]
     w = api.World(scene, substeps=sub, iterations=it)
-    if args.persistent:
-        w.set_exec(persistent=True, blocks=args.blocks)
+    w.set_exec(args.exec, blocks=args.blocks)
     for _ in range(args.warmup):
         w.step(dt)
     w.sync()
@@ -239,9 +244,10 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
         "config": dict(cfg, parallelism="single GPU", last_step_counts=counts,
                        phase_us_last_step=w.timing()),
-        "roofline": {"bound": "hbm", "kernel": "k_step_persistent" if args.persistent else "k_solve", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "roofline": {"bound": "hbm", "kernel": {"flow": "k_solve_flow", "colors": "k_solve",
+                                                    "cooperative": "k_step_persistent"}[args.exec], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic() if args.config == "plain32k" and not args.persistent else None,
+                     "traffic": measured_traffic(args.exec) if args.config == "plain32k" else None,
                      "avg_launch_us": st["solve_us"] / max(1, st["solve_launches"]),
                      "bytes_per_launch": st["solve_bytes"] / max(1, st["solve_launches"]),
                      "launches_timed": st["solve_launches"]},

@@ -232,10 +232,17 @@ typedef struct pgx_stats {
     double step_us;
 } pgx_stats;
 int pgx_world_reset_stats(pgx_world* w);
-/* Execution mode of the substep loop: persistent = 1 runs all substeps of a step as one
- * cooperative launch with grid barriers between colours (default); 0 launches one kernel
- * per colour and phase. blocks = grid size of the persistent launch (0 = one per CU). */
-int pgx_world_set_exec(pgx_world* w, int persistent, int blocks);
+/* Execution of the substep loop's contact passes (the results are the same in every mode):
+ *   PGX_EXEC_COLORS (default) one launch per colour and pass
+ *   PGX_EXEC_FLOW            one launch per pass; each manifold solves as soon as its
+ *                            bodies have taken every lower colour's solves
+ *   PGX_EXEC_COOPERATIVE     every substep of a step in one cooperative launch, grid
+ *                            barriers between colours (worlds without joints; others use
+ *                            PGX_EXEC_FLOW). blocks = its grid size (0 = one per CU). */
+#define PGX_EXEC_COLORS 0
+#define PGX_EXEC_COOPERATIVE 1
+#define PGX_EXEC_FLOW 2
+int pgx_world_set_exec(pgx_world* w, int mode, int blocks);
 int pgx_world_get_stats(pgx_world* w, pgx_stats* out);
 
 /* ---------------------------------------------------------------------------------

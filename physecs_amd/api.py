@@ -249,8 +249,13 @@ class World:
     def enable_timing(self, on=True):
         check(self.L.pgx_world_enable_timing(self.h, 1 if on else 0))
 
-    def set_exec(self, persistent=True, blocks=0):
-        check(self.L.pgx_world_set_exec(self.h, 1 if persistent else 0, blocks))
+    EXEC_MODES = {"colors": 0, "cooperative": 1, "flow": 2}
+
+    def set_exec(self, mode="colors", blocks=0):
+        """How the contact passes launch: "colors" (one launch per colour, the default),
+        "flow" (one dependency-driven launch per pass) or "cooperative" (one launch per
+        step)."""
+        check(self.L.pgx_world_set_exec(self.h, self.EXEC_MODES[mode], blocks))
 
     def reset_stats(self):
         check(self.L.pgx_world_reset_stats(self.h))

@@ -365,40 +365,6 @@ __global__ void k_pack(int n, const float4* __restrict__ nmin, const float4* __r
     }
 }
 
-// The subtrees at depth CUT_DEPTH under the root of a one-world tree (a leaf reached
-// earlier stands for itself), in depth-first order, with the escape link that ends each
-// subtree's walk. Their leaf sets partition the tree, so walking each of them for a query
-// finds exactly the pairs one walk from the root finds.
-constexpr int CUT_DEPTH = 3, CUT_MAX = 1 << CUT_DEPTH;
-__global__ void k_cut(const int32_t* __restrict__ worldRoot, const int32_t* __restrict__ child,
-                      const int32_t* __restrict__ nodeEsc, const int32_t* __restrict__ leafEsc,
-                      int32_t* __restrict__ cutNode, int32_t* __restrict__ cutStop) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int cur[CUT_MAX], nxt[CUT_MAX];
-    int nc = 0;
-    if (worldRoot[0] != -1) cur[nc++] = worldRoot[0];
-    for (int d = 0; d < CUT_DEPTH; ++d) {
-        int nn = 0;
-        for (int i = 0; i < nc; ++i) {
-            int v = cur[i];
-            if (v & LEAF_BIT) nxt[nn++] = v;
-            else { nxt[nn++] = child[2 * v]; nxt[nn++] = child[2 * v + 1]; }
-        }
-        for (int i = 0; i < nn; ++i) cur[i] = nxt[i];
-        nc = nn;
-    }
-    for (int i = 0; i < CUT_MAX; ++i) {
-        if (i < nc) {
-            int v = cur[i];
-            cutNode[i] = v;
-            cutStop[i] = (v & LEAF_BIT) ? leafEsc[v & ~LEAF_BIT] : nodeEsc[v];
-        } else {
-            cutNode[i] = -1;
-            cutStop[i] = -1;
-        }
-    }
-}
-
 // ------------------------------------------------------------------ traversal
 __device__ __forceinline__ bool overlaps(float4 amin, float4 amax, float4 bmin, float4 bmax) {
     // inclusive, as the sweep's `max < min` rejections (src/Physecs.cpp:152-154)
@@ -406,9 +372,12 @@ __device__ __forceinline__ bool overlaps(float4 amin, float4 amax, float4 bmin, 
              amin.z > bmax.z);
 }
 
+constexpr int TRAV_SLOTS = 8;   // pairs a lane holds before its wave appends them
+
 // Each enabled dynamic leaf s walks its world's subtree over escape links and takes the
 // dynamic leaves after it (so each dynamic pair is emitted once), then tests the world's
 // non-dynamic colliders (static and kinematic; their bounds never move during a step).
+
 __global__ __launch_bounds__(256) void k_traverse(
     int n, const uint64_t* __restrict__ mkeys, int worldShift, const int32_t* __restrict__ order,
     const float4* __restrict__ leafA, const float4* __restrict__ leafB, const int2* __restrict__ leafI,
@@ -417,12 +386,8 @@ __global__ __launch_bounds__(256) void k_traverse(
     const int32_t* __restrict__ worldStaticStart, const int32_t* __restrict__ staticCols,
     const float4* __restrict__ cbmin, const float4* __restrict__ cbmax, const int32_t* __restrict__ cflags,
     const uint32_t* __restrict__ centity, const int32_t* __restrict__ clocal, int localBits,
-    int32_t* __restrict__ pairCount, int cap, uint64_t* __restrict__ pkeys, const int32_t* __restrict__ cutNode,
-    const int32_t* __restrict__ cutStop, int parts) {
-    // `parts` lanes per query: lane p walks cut subtree p (parts = 1: the world's whole tree)
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int s = parts > 1 ? t / parts : t;
-    const int part = parts > 1 ? t % parts : 0;
+    int32_t* __restrict__ pairCount, int cap, uint64_t* __restrict__ pkeys) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
     bool active = false;
     float4 qmin = make_float4(0.f, 0.f, 0.f, 0.f), qmax = qmin;
     int node = -1, stop = -1, st = 0, stEnd = 0;
@@ -436,24 +401,35 @@ __global__ __launch_bounds__(256) void k_traverse(
             qmin = a;
             qmax = b;
             wd = (int)(mkeys[s] >> worldShift);
-            if (parts > 1) {
-                node = cutNode[part];
-                stop = cutStop[part];
-            } else {
-                node = worldRoot[wd];
-                stop = worldStop[wd];
-            }
-            if (part == 0) {   // the world's non-dynamic colliders: once per query
-                st = worldStaticStart[wd];
-                stEnd = worldStaticStart[wd + 1];
-            }
+            node = worldRoot[wd];
+            stop = worldStop[wd];
+            st = worldStaticStart[wd];   // then the world's non-dynamic colliders
+            stEnd = worldStaticStart[wd + 1];
             int2 me = leafI[s];
             myEnt = (uint32_t)me.x;
             myLocal = me.y;
         }
     }
     const int lane = threadIdx.x & (WAVE - 1);
-    const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // found pairs wait in the lane's own LDS slots; when a lane's slots are full (or the
+    // walk ends) the wave appends all of its lanes' pairs with one atomic on the counter
+    __shared__ uint64_t found[256 * TRAV_SLOTS];
+    uint64_t* mine = found + threadIdx.x * TRAV_SLOTS;
+    int held = 0;
+    auto flush = [&]() {
+        int incl = held;   // inclusive prefix sum of held over the wave
+        for (int o = 1; o < WAVE; o <<= 1) {
+            int t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        int total = __shfl(incl, WAVE - 1);
+        int base = 0;
+        if (lane == WAVE - 1 && total) base = atomicAdd(pairCount, total);
+        base = __shfl(base, WAVE - 1) + incl - held;
+        for (int j = 0; j < held; ++j)
+            if (base + j < cap) pkeys[base + j] = mine[j];
+        held = 0;
+    };
     int visits = 0;
     const int maxVisits = 2 * n;   // a depth-first walk visits at most 2n - 1 nodes
     while (__ballot(active && (node != stop || st < stEnd))) {
@@ -496,18 +472,10 @@ __global__ __launch_bounds__(256) void k_traverse(
                 }
             }
         }
-        uint64_t m = __ballot(emit);
-        if (m) {
-            int leader = __ffsll((long long)m) - 1;
-            int base = 0;
-            if (lane == leader) base = atomicAdd(pairCount, __popcll(m));
-            base = __shfl(base, leader);
-            if (emit) {
-                int idx = base + __popcll(m & lanesBelow);
-                if (idx < cap) pkeys[idx] = key;
-            }
-        }
+        if (emit) mine[held++] = key;
+        if (__ballot(held == TRAV_SLOTS)) flush();
     }
+    if (__ballot(held > 0)) flush();
 }
 
 __global__ void k_single_leaf_root(int32_t* worldRoot, int W, const uint64_t* mkeys, int worldShift) {
@@ -563,16 +531,10 @@ int bp_find_pairs(World& w) {
                            w.leafI.p);
         // one world with few queries: several lanes per query keep enough waves in flight to
         // hide the node loads' latency
-        int parts = 1;   // one lane per query: splitting a query over subtrees multiplies the waves
-                         // while each wave still runs as long as its longest walk
-        if (parts > 1)
-            hipLaunchKernelGGL(k_cut, dim3(1), dim3(64), 0, s, w.worldRoot.p, w.nodeChild.p, w.nodeEsc.p, w.leafEsc.p,
-                               w.cutNode.p, w.cutStop.p);
-        hipLaunchKernelGGL(k_traverse, dim3(div_up((long)n * parts, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift,
+        hipLaunchKernelGGL(k_traverse, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift,
                            w.mvals.p, w.leafA.p, w.leafB.p, w.leafI.p, w.nodeA.p, w.nodeB.p, w.nodeLast.p,
                            w.worldRoot.p, w.worldStop.p, w.worldStaticStart.p, w.staticCols.p, w.cbmin.p, w.cbmax.p,
-                           w.cflags.p, w.centity.p, w.clocal.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p,
-                           w.cutNode.p, w.cutStop.p, parts);
+                           w.cflags.p, w.centity.p, w.clocal.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
         PGX_LAUNCH_CHECK();
     }
     int cnt[8] = {0};

@@ -52,6 +52,9 @@ struct DBuf {
 
 constexpr int WAVE = 64;
 constexpr int JOINT_MAX_ROWS = 7;      // a prismatic joint with its limit and drive rows
+// substep-loop execution: one launch per colour and pass; one cooperative launch per step
+// (grid barriers between colours); one dependency-driven launch per pass
+enum { EXEC_COLORS = 0, EXEC_COOPERATIVE = 1, EXEC_FLOW = 2 };
 // convex narrowphase: EPA polytope capacity per lane and lanes of the convex kernel
 constexpr int EPA_MAXV = 104;          // 4 simplex vertices + at most 100 insertions
 constexpr int EPA_MAXF = 256;          // > 2 * EPA_MAXV - 4, the face count of a closed triangulation
@@ -148,7 +151,6 @@ struct World {
     DBuf<int32_t> nodeLast;              // last leaf of the node's subtree
     DBuf<float4> leafA, leafB;           // {min, escape bits}, {max, flags bits}
     DBuf<int2> leafI;                    // {entity, world-local collider index}
-    DBuf<int32_t> cutNode, cutStop;      // subtrees walked by the lanes of one query
     DBuf<int32_t> worldRoot, worldStop;
     DBuf<uint64_t> pkeys;
     DBuf<int32_t> pvals;
@@ -235,8 +237,13 @@ struct World {
     int lastPairs = 0, lastManifolds = 0, lastColors = 0, lastRounds = 0, lastPoints = 0;
     std::vector<int> colorStartHost;
     bool timing = false;
-    bool persistent = false;             // substep loop as one cooperative launch per step
+    int execMode = EXEC_COLORS;          // how the substep loop's contact passes are launched
     int persistentBlocks = 0, persistentBlocksWanted = 0;
+    // dependency-driven passes
+    uint32_t flowEpoch = 0;              // per launch: versions of the published velocities
+    DBuf<uint64_t> flowPub;              // 6 x D published velocity words
+    DBuf<int32_t> tickets;               // work tickets, one per pass of a substep
+    DBuf<int32_t> flowErr;               // set when a dependency wait runs out
     hipEvent_t ev[8] = {};
     float tBroad = 0, tNarrow = 0, tColor = 0, tSolve = 0, tTotal = 0;
     // per-launch timing of the contact solve (first substep of each timed step)

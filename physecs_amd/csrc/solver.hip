@@ -300,6 +300,11 @@ struct SubArgs {
     const float4 *dcom, *dinvI;
     float4 *tvel, *tang;                  // solver velocities (velocityTemp)
     float4 *pvel, *pang;                  // pseudo velocities (pang.w = constraint count)
+    // dependency-driven passes: per body, the colours of its manifolds (2 x 64 bits); a work
+    // ticket per pass; a flag set if a wait ran out (never expected)
+    const uint64_t* used;
+    int32_t* tickets;
+    int32_t* flowErr;
 };
 
 __device__ __forceinline__ m3 world_inv_inertia(const SubArgs& a, int b) {
@@ -393,49 +398,62 @@ __device__ void integrate_velocity(const SubArgs& a, int d) {
 // One manifold's normal then friction rows, src/ContactConstraints.cpp:33-124. Manifolds of
 // one colour touch disjoint dynamic bodies, so a colour is solved in parallel exactly as a
 // sequential pass over it would be.
-__device__ void solve_manifold(const SubArgs& a, int s, bool useBias, float h, bool skipSoft) {
-    int flags = a.sFlags[s];
-    bool soft = (flags & 256) != 0;
-    if (skipSoft && soft) return;
-    int num = flags & 255;
-    int b0 = a.sB0[s], b1 = a.sB1[s];
-    float4 nf = a.sN[s];
-    f3 n = xyz(nf);
-    float friction = nf.w;
-    float im0 = 0.f, im1 = 0.f;
-    f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
-    f3 cv0 = s3(0.f), cw0 = s3(0.f), cv1 = s3(0.f), cw1 = s3(0.f);
-    if (b0 >= 0) {
-        float4 c = a.cvel[b0];
-        im0 = c.w; cv0 = xyz(c); cw0 = xyz(a.cang[b0]);
-        v0 = xyz(a.tvel[b0]); w0 = xyz(a.tang[b0]);
-    }
-    if (b1 >= 0) {
-        float4 c = a.cvel[b1];
-        im1 = c.w; cv1 = xyz(c); cw1 = xyz(a.cang[b1]);
-        v1 = xyz(a.tvel[b1]); w1 = xyz(a.tang[b1]);
-    }
-    float2 sp = a.sSoft[s];
-    // all rows of the manifold are loaded before any dependent arithmetic (memory-level
-    // parallelism); unused point slots are never read back
+// Everything one manifold's solve reads except the bodies' solver velocities: the
+// manifold's rows, normal, material, cached target velocities, and both bodies' inverse
+// masses and component velocities. All of it is fixed for the whole pass.
+struct MInputs {
+    int num, b0, b1;
+    bool soft;
+    f3 n;
+    float friction, im0, im1;
+    f3 cv0, cw0, cv1, cw1;
+    float2 sp;
     float4 rA[4], rB[4], rT[4], rC[4], rE[4], rNT0[4], rNT1[4], rTT0[4], rTT1[4];
     float tvk[4];
+};
+
+__device__ __forceinline__ void load_inputs(const SubArgs& a, int s, MInputs& in) {
+    int flags = a.sFlags[s];
+    in.soft = (flags & 256) != 0;
+    in.num = flags & 255;
+    in.b0 = a.sB0[s];
+    in.b1 = a.sB1[s];
+    float4 nf = a.sN[s];
+    in.n = xyz(nf);
+    in.friction = nf.w;
+    in.im0 = 0.f; in.im1 = 0.f;
+    in.cv0 = s3(0.f); in.cw0 = s3(0.f); in.cv1 = s3(0.f); in.cw1 = s3(0.f);
+    if (in.b0 >= 0) { float4 c = a.cvel[in.b0]; in.im0 = c.w; in.cv0 = xyz(c); in.cw0 = xyz(a.cang[in.b0]); }
+    if (in.b1 >= 0) { float4 c = a.cvel[in.b1]; in.im1 = c.w; in.cv1 = xyz(c); in.cw1 = xyz(a.cang[in.b1]); }
+    in.sp = a.sSoft[s];
+    // all rows of the manifold are loaded before any dependent arithmetic (memory-level
+    // parallelism); unused point slots are never read back
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (k < num) {
-            rA[k] = ROW(R_R0XN, k); rB[k] = ROW(R_R1XN, k); rT[k] = ROW(R_T, k);
-            rC[k] = ROW(R_R0XT, k); rE[k] = ROW(R_R1XT, k);
-            rNT0[k] = ROW(R_R0XNT, k); rNT1[k] = ROW(R_R1XNT, k);
-            rTT0[k] = ROW(R_R0XTT, k); rTT1[k] = ROW(R_R1XTT, k);
-            tvk[k] = a.sR0[(size_t)k * a.Mcap + s].w;
+        if (k < in.num) {
+            in.rA[k] = ROW(R_R0XN, k); in.rB[k] = ROW(R_R1XN, k); in.rT[k] = ROW(R_T, k);
+            in.rC[k] = ROW(R_R0XT, k); in.rE[k] = ROW(R_R1XT, k);
+            in.rNT0[k] = ROW(R_R0XNT, k); in.rNT1[k] = ROW(R_R1XNT, k);
+            in.rTT0[k] = ROW(R_R0XTT, k); in.rTT1[k] = ROW(R_R1XTT, k);
+            in.tvk[k] = a.sR0[(size_t)k * a.Mcap + s].w;
         }
     }
+}
+
+// ContactConstraints::solve (src/ContactConstraints.cpp:33-124) of one manifold on the
+// bodies' solver velocities v0, w0, v1, w1 (updated in place); stores the accumulated
+// lambdas in the manifold's rows.
+__device__ __forceinline__ void solve_core(const SubArgs& a, int s, const MInputs& in, bool useBias, float h, f3& v0,
+                                           f3& w0, f3& v1, f3& w1) {
+    const int num = in.num, b0 = in.b0, b1 = in.b1;
+    const f3 n = in.n;
+    const float im0 = in.im0, im1 = in.im1;
     float lN[4], lT[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= num) break;
-        float4 r0 = rA[k], r1 = rB[k];
-        lN[k] = rC[k].w;
+        float4 r0 = in.rA[k], r1 = in.rB[k];
+        lN[k] = in.rC[k].w;
         float invN = r1.w;
         if (invN == 0.f) continue;
         f3 r0xn = xyz(r0), r1xn = xyz(r1);
@@ -443,53 +461,65 @@ __device__ void solve_manifold(const SubArgs& a, int s, bool useBias, float h, b
         float rv = dot(neg(n), v0) + dot(neg(r0xn), w0) + dot(n, v1) + dot(r1xn, w1);
         float effMass = 1.f / invN;
         float lambda;
-        if (soft) {
-            float af = 2.f * 3.14159265358979323846f * sp.x;
+        if (in.soft) {
+            float af = 2.f * 3.14159265358979323846f * in.sp.x;
             float stiffness = af * af * effMass;
-            float damping = 2.f * af * sp.y * effMass;
+            float damping = 2.f * af * in.sp.y * effMass;
             float gamma = 1.f / (damping + h * stiffness);
             float beta = h * stiffness / (damping + h * stiffness);
             lambda = (rv + beta * c / h) / (invN + gamma / h);
         } else {
-            lambda = (rv - tvk[k] + (useBias ? 0.1f * c / h : 0.f)) * effMass;
+            lambda = (rv - in.tvk[k] + (useBias ? 0.1f * c / h : 0.f)) * effMass;
         }
         float prev = lN[k];
         lN[k] = lN[k] + lambda;
         lN[k] = gmin(lN[k], 0.f);
         lambda = lN[k] - prev;
-        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, n)); w0 = add(w0, mul(lambda, xyz(rNT0[k]))); }
-        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, n)); w1 = sub(w1, mul(lambda, xyz(rNT1[k]))); }
+        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, n)); w0 = add(w0, mul(lambda, xyz(in.rNT0[k]))); }
+        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, n)); w1 = sub(w1, mul(lambda, xyz(in.rNT1[k]))); }
     }
     // friction rows read the component velocities (src/ContactConstraints.cpp:142-152)
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= num) break;
-        float4 t4 = rT[k];
-        lT[k] = rE[k].w;
+        float4 t4 = in.rT[k];
+        lT[k] = in.rE[k].w;
         float invT = t4.w;
         if (invT == 0.f) continue;
-        f3 t = xyz(t4), r0xt = xyz(rC[k]), r1xt = xyz(rE[k]);
-        float rv = dot(neg(t), cv0) + dot(neg(r0xt), cw0) + dot(t, cv1) + dot(r1xt, cw1);
+        f3 t = xyz(t4), r0xt = xyz(in.rC[k]), r1xt = xyz(in.rE[k]);
+        float rv = dot(neg(t), in.cv0) + dot(neg(r0xt), in.cw0) + dot(t, in.cv1) + dot(r1xt, in.cw1);
         float lambda = rv / invT;
-        float limit = friction * lN[k];
+        float limit = in.friction * lN[k];
         float prev = lT[k];
         lT[k] = lT[k] + lambda;
         lT[k] = gclamp(lT[k], limit, -limit);
         lambda = lT[k] - prev;
-        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, t)); w0 = add(w0, mul(lambda, xyz(rTT0[k]))); }
-        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, t)); w1 = sub(w1, mul(lambda, xyz(rTT1[k]))); }
+        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, t)); w0 = add(w0, mul(lambda, xyz(in.rTT0[k]))); }
+        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, t)); w1 = sub(w1, mul(lambda, xyz(in.rTT1[k]))); }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= num) break;
-        float4 c = rC[k], e = rE[k];
+        float4 c = in.rC[k], e = in.rE[k];
         c.w = lN[k];
         e.w = lT[k];
         ROW(R_R0XT, k) = c;
         ROW(R_R1XT, k) = e;
     }
-    if (b0 >= 0) { a.tvel[b0] = f4(v0, 0.f); a.tang[b0] = f4(w0, 0.f); }
-    if (b1 >= 0) { a.tvel[b1] = f4(v1, 0.f); a.tang[b1] = f4(w1, 0.f); }
+}
+
+// one manifold of a colour launch: every other manifold touching its bodies is in another
+// launch, so plain accesses see the previous colours' results
+__device__ void solve_manifold(const SubArgs& a, int s, bool useBias, float h, bool skipSoft) {
+    MInputs in;
+    load_inputs(a, s, in);
+    if (skipSoft && in.soft) return;
+    f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
+    if (in.b0 >= 0) { v0 = xyz(a.tvel[in.b0]); w0 = xyz(a.tang[in.b0]); }
+    if (in.b1 >= 0) { v1 = xyz(a.tvel[in.b1]); w1 = xyz(a.tang[in.b1]); }
+    solve_core(a, s, in, useBias, h, v0, w0, v1, w1);
+    if (in.b0 >= 0) { a.tvel[in.b0] = f4(v0, 0.f); a.tang[in.b0] = f4(w0, 0.f); }
+    if (in.b1 >= 0) { a.tvel[in.b1] = f4(v1, 0.f); a.tang[in.b1] = f4(w1, 0.f); }
 }
 
 // integrate positions of one body, src/Physecs.cpp:494-511
@@ -522,8 +552,9 @@ __device__ void write_back(const SubArgs& a, int d) {
 }
 
 // ------------------------------------------------------------------ discrete kernels
-__global__ __launch_bounds__(256) void k_prepare(SubArgs a) {
+__global__ __launch_bounds__(256) void k_prepare(SubArgs a, int sub) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= a.iterations) a.tickets[i] = 0;
     if (i < a.M) prepare_manifold(a, i);
     else if (i - a.M < a.D) integrate_velocity(a, i - a.M);
 }
@@ -531,6 +562,139 @@ __global__ __launch_bounds__(64) void k_solve(SubArgs a, int begin, int end, int
     int s = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (s < end) solve_manifold(a, s, useBias != 0, h, skipSoft != 0);
 }
+
+// colours of body b, and its colours below colour c
+__device__ __forceinline__ int colours_of(const uint64_t* used, int b) {
+    return __popcll(used[2 * b]) + __popcll(used[2 * b + 1]);
+}
+__device__ __forceinline__ int colours_below(const uint64_t* used, int b, int c) {
+    uint64_t lo = used[2 * b], hi = used[2 * b + 1];
+    if (c < 64) return __popcll(lo & ((1ull << c) - 1ull));
+    return __popcll(lo) + __popcll(hi & ((1ull << (c - 64)) - 1ull));
+}
+
+// ------------------------------------------------------------------ dependency-driven pass
+// A pass (one of the substep's biased iterations, or the relaxation) over every colour in
+// ONE launch. A manifold of colour c needs the velocities its bodies hold after their
+// manifolds of lower colours in this pass; with r = (colours of body b below c) such
+// manifolds before it, the r-th of them publishes body b's velocities under version
+// (epoch, r). The first manifold of a body (r = 0) reads the plain solver velocities, which
+// are complete when the launch starts (previous pass, joints); its last manifold of the
+// pass writes them, their only writer in the launch.
+//
+// Published velocities: 6 words of {value, version} per body, each word readable on its
+// own, so a reader accepts a record only when all six carry the version it waits for and
+// needs no ordering between the words' stores. Records travel through the device-wide
+// coherence point (agent-scope atomic accesses): the waves of one launch run on every XCD,
+// and on this device only device-scope accesses see another CU's stores within a launch.
+//
+// The solve order is the colour order's, so the arithmetic is exactly that of one launch
+// per colour. Waves take 64 manifolds at a time, in solve order, from the pass's ticket: a
+// manifold waits only on manifolds of lower solve positions held by waves already running,
+// so the waits resolve in order. Every wait is bounded; one that runs out sets flowErr and
+// gives up (the host reports the error) rather than hang.
+constexpr int FLOW_SPIN_LIMIT = 1 << 18;   // ~0.5 s of polling: a wait this long is a bug
+
+struct FlowArgs {
+    int pass;
+    int useBias, skipSoft;
+    float h;
+    uint32_t epoch;                        // unique per launch (24 bits used)
+    uint64_t* pub;                         // 6 words per body
+};
+
+__device__ __forceinline__ uint64_t pub_word(float v, uint32_t ver) {
+    return (uint64_t)__float_as_uint(v) | ((uint64_t)ver << 32);
+}
+
+// body b's published velocities under version ver, if all six words carry it
+__device__ __forceinline__ bool read_pub(const FlowArgs& f, int b, uint32_t ver, f3& v, f3& w) {
+    uint64_t q[6];
+    uint64_t* p = f.pub + 6 * (size_t)b;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = __hip_atomic_load(p + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) ok = ok && (uint32_t)(q[k] >> 32) == ver;
+    v = mk3(__uint_as_float((uint32_t)q[0]), __uint_as_float((uint32_t)q[1]), __uint_as_float((uint32_t)q[2]));
+    w = mk3(__uint_as_float((uint32_t)q[3]), __uint_as_float((uint32_t)q[4]), __uint_as_float((uint32_t)q[5]));
+    return ok;
+}
+
+// Hands body b's velocities on after one of its manifolds: to the next manifold of the
+// pass (published under version ver), or, after its last manifold of the pass, to the
+// kernels after the pass (the plain solver velocities, written back at the launch's end).
+__device__ __forceinline__ void hand_on(const SubArgs& a, const FlowArgs& f, int b, bool last, uint32_t ver, f3 v,
+                                        f3 w) {
+    if (last) {
+        a.tvel[b] = f4(v, 0.f);
+        a.tang[b] = f4(w, 0.f);
+        return;
+    }
+    uint64_t* p = f.pub + 6 * (size_t)b;
+    __hip_atomic_store(p + 0, pub_word(v.x, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 1, pub_word(v.y, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 2, pub_word(v.z, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 3, pub_word(w.x, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 4, pub_word(w.y, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p + 5, pub_word(w.z, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(256) void k_solve_flow(SubArgs a, FlowArgs f) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const uint32_t vbase = (f.epoch & 0xffffffu) << 8;
+    const bool useBias = f.useBias != 0, skipSoft = f.skipSoft != 0;
+    for (;;) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(a.tickets + f.pass, WAVE);
+        base = __shfl(base, 0);
+        if (base >= a.M) break;
+        const int s = base + lane;
+        bool pending = s < a.M;
+        MInputs in;
+        int r0 = 0, r1 = 0;
+        bool last0 = false, last1 = false;
+        f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
+        bool have0 = true, have1 = true;
+        if (pending) {
+            // everything but the bodies' solver velocities is fixed for the pass: loaded
+            // before the wait, off the critical path
+            load_inputs(a, s, in);
+            const int c = a.sFlags[s] >> 16;
+            if (in.b0 >= 0) {
+                r0 = colours_below(a.used, in.b0, c);
+                last0 = r0 == colours_of(a.used, in.b0) - 1;
+                have0 = r0 == 0;
+                if (have0) { v0 = xyz(a.tvel[in.b0]); w0 = xyz(a.tang[in.b0]); }
+            }
+            if (in.b1 >= 0) {
+                r1 = colours_below(a.used, in.b1, c);
+                last1 = r1 == colours_of(a.used, in.b1) - 1;
+                have1 = r1 == 0;
+                if (have1) { v1 = xyz(a.tvel[in.b1]); w1 = xyz(a.tang[in.b1]); }
+            }
+        }
+        int spins = 0;
+        while (__ballot(pending)) {
+            if (pending) {
+                if (!have0) have0 = read_pub(f, in.b0, vbase | (uint32_t)r0, v0, w0);
+                if (!have1) have1 = read_pub(f, in.b1, vbase | (uint32_t)r1, v1, w1);
+                if (have0 && have1) {
+                    // a soft manifold in the relaxation passes its bodies' velocities on
+                    if (!(skipSoft && in.soft)) solve_core(a, s, in, useBias, f.h, v0, w0, v1, w1);
+                    if (in.b0 >= 0) hand_on(a, f, in.b0, last0, vbase | (uint32_t)(r0 + 1), v0, w0);
+                    if (in.b1 >= 0) hand_on(a, f, in.b1, last1, vbase | (uint32_t)(r1 + 1), v1, w1);
+                    pending = false;
+                } else if (++spins > FLOW_SPIN_LIMIT) {
+                    atomicOr(a.flowErr, 1);
+                    pending = false;
+                }
+            }
+            if (__ballot(pending)) __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_integrate_positions(SubArgs a) {
     int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d < a.D) integrate_position(a, d);
@@ -582,9 +746,11 @@ int sv_setup_manifolds(World& w, int nPairs) {
     hipLaunchKernelGGL(k_flag_manifolds, dim3(div_up(nPairs, 256)), dim3(256), 0, s, nPairs, w.npNum.p,
                        w.meshCount.p, w.pvals.p);
     exclusive_scan_i32(s, w.pvals.p, w.npIndex.p, nPairs, w.counters.p + 1, w.scanT);
-    int M = 0, epaErrors = 0;
+    int M = 0, epaErrors = 0, flowErr = 0;
     int meshInfo[3] = {0, 0, 0};   // mesh pairs, mesh manifold slots, mesh capacity errors
     PGX_HIP(hipMemcpyAsync(&M, w.counters.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    // the previous step's dependency-driven passes: a wait that ran out
+    PGX_HIP(hipMemcpyAsync(&flowErr, w.flowErr.p, sizeof(int), hipMemcpyDeviceToHost, s));
     if (w.nConvex) PGX_HIP(hipMemcpyAsync(&epaErrors, w.counters.p + 9, sizeof(int), hipMemcpyDeviceToHost, s));
     if (w.nTriMesh) PGX_HIP(hipMemcpyAsync(meshInfo, w.counters.p + 10, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     hipLaunchKernelGGL(k_compact, dim3(div_up(nPairs, 256)), dim3(256), 0, s, nPairs, w.npNum.p, w.npIndex.p,
@@ -592,6 +758,8 @@ int sv_setup_manifolds(World& w, int nPairs) {
                        w.mfPair.p, w.mfKey.p, w.mfB0.p, w.mfB1.p, w.mfSrc.p, w.mfTri.p);
     PGX_LAUNCH_CHECK();
     PGX_HIP(hipStreamSynchronize(s));
+    if (flowErr)
+        throw PgxError{PGX_ERR_HIP, "contact solve: a dependency wait of the previous step ran out"};
     if (epaErrors)
         throw PgxError{PGX_ERR_CAPACITY, "EPA polytope capacity exceeded in " + std::to_string(epaErrors) + " pairs"};
     if (meshInfo[2])
@@ -694,6 +862,7 @@ static SubArgs make_args(World& w, int M, float dt) {
     a.bpos = w.bpos.p; a.bquat = w.bquat.p; a.dvel = w.dvel.p; a.dang = w.dang.p;
     a.cvel = w.cvel.p; a.cang = w.cang.p; a.dcom = w.dcom.p; a.dinvI = w.dinvI.p;
     a.tvel = w.tvel.p; a.tang = w.tang.p; a.pvel = w.pvel.p; a.pang = w.pang.p;
+    a.used = w.usedMask.p; a.tickets = w.tickets.p; a.flowErr = w.flowErr.p;
     return a;
 }
 
@@ -729,8 +898,9 @@ static void solve_events(World& w, int color, hipEvent_t* start, hipEvent_t* sto
 void sv_substeps(World& w, int M, float dt) {
     hipStream_t s = w.stream;
     w.solveEvUsed = 0;
+    w.tickets.alloc(w.iterations + 1);   // k_prepare clears them
     SubArgs a = make_args(w, M, dt);
-    if (w.persistent && w.J == 0) {
+    if (w.execMode == EXEC_COOPERATIVE && w.J == 0) {
         int blocks = persistent_blocks(w);
         void* args[] = {&a};
         if (w.timing) PGX_HIP(hipEventRecord(w.ev[6], s));
@@ -740,9 +910,29 @@ void sv_substeps(World& w, int M, float dt) {
     }
     int nbP = div_up(std::max(M + w.D, 1), 256), nbD = div_up(std::max(w.D, 1), 256);
     int nColors = a.nColors;
-    // one launch per colour: a colour's cost is the latency of one manifold solve, which a
-    // launch across the chip pays once for all of the colour's manifolds
+    const bool flow = w.execMode == EXEC_FLOW;
+    FlowArgs f{};
+    f.pub = w.flowPub.p;
+    // waves of the dependency-driven pass: enough to hold every manifold at once, at most
+    // what the chip keeps resident comfortably
+    const int flowBlocks = std::max(1, std::min(div_up(std::max(M, 1), 256), 2048));
+    int passIndex = 0;
     auto pass = [&](bool rec, int useBias, float h, int skipSoft) {
+        const int p = passIndex++;
+        if (M == 0) return;
+        if (flow) {
+            f.pass = p; f.useBias = useBias; f.skipSoft = skipSoft; f.h = h;
+            f.epoch = ++w.flowEpoch;
+            if (rec) {
+                hipEvent_t e0, e1;
+                solve_events(w, -1, &e0, &e1);
+                hipExtLaunchKernelGGL(k_solve_flow, dim3(flowBlocks), dim3(256), 0, s, e0, e1, 0, a, f);
+            } else {
+                hipLaunchKernelGGL(k_solve_flow, dim3(flowBlocks), dim3(256), 0, s, a, f);
+            }
+            return;
+        }
+        // one launch per colour: a colour's manifolds share no dynamic body
         for (int c = 0; c < nColors; ++c) {
             int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
             if (e <= b) continue;
@@ -759,9 +949,10 @@ void sv_substeps(World& w, int M, float dt) {
     jt_step_setup(w);
     for (int sub = 0; sub < w.substeps; ++sub) {
         const bool rec = w.timing && sub == 0;
+        passIndex = 0;
         // rows + preSolve of the contacts, velocity integration; then the joint rows, whose
         // preSolve follows the contacts' (src/Physecs.cpp:430-479)
-        hipLaunchKernelGGL(k_prepare, dim3(nbP), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_prepare, dim3(nbP), dim3(256), 0, s, a, sub);
         jt_update(w, a.h);
         jt_colors(w, a.h, 0, false);
         for (int it = 0; it < w.iterations; ++it) {

@@ -245,7 +245,6 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     w.nodeCounter.alloc(ni); w.nodeRange.alloc(ni); w.nodeMin.alloc(ni); w.nodeMax.alloc(ni);
     w.nodeA.alloc(ni); w.nodeB.alloc(ni); w.nodeLast.alloc(ni);
     w.leafA.alloc(C); w.leafB.alloc(C); w.leafI.alloc(C);
-    w.cutNode.alloc(8); w.cutStop.alloc(8);
     w.worldRoot.alloc(W); w.worldStop.alloc(W); w.worldMStart.alloc(W);
     w.pairCap = (cfg && cfg->pair_capacity > 0) ? cfg->pair_capacity : std::max(1024, 12 * C);
     // manifolds of triangle-mesh pairs are stored after the pair slots
@@ -261,6 +260,12 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     w.mfKey.alloc(M); w.mfPair.alloc(M); w.mfB0.alloc(M); w.mfB1.alloc(M); w.mfColor.alloc(M);
     w.mfSrc.alloc(M); w.mfTri.alloc(M);
     w.claim.alloc(std::max(D, 1)); w.usedMask.alloc(2 * (size_t)std::max(D, 1));
+    w.flowPub.alloc(6 * (size_t)std::max(D, 1));
+    // versions start at zero; a launch's versions carry its epoch, which starts at one
+    PGX_HIP(hipMemsetAsync(w.flowPub.p, 0, sizeof(uint64_t) * 6 * std::max(D, 1), w.stream));
+    w.flowErr.alloc(1);
+    PGX_HIP(hipMemsetAsync(w.flowErr.p, 0, sizeof(int32_t), w.stream));
+
     w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorCount.alloc(160); w.colorPoints.alloc(160);
     w.sB0.alloc(M); w.sB1.alloc(M); w.sFlags.alloc(M); w.sA0.alloc(M); w.sA1.alloc(M); w.sCanon.alloc(M);
     w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(36 * M);
@@ -633,21 +638,24 @@ void world_step(World& w, float dt) {
         long points = 0;
         for (int c = 0; c < nColors; ++c) points += pts[c];
         w.lastPointsHost = points;
-        if (w.persistent && M > 0) {
+        const bool coop = w.execMode == EXEC_COOPERATIVE && w.J == 0;
+        if (coop && M > 0) {
             float ms = 0.f;
             PGX_HIP(hipEventElapsedTime(&ms, w.ev[6], w.ev[7]));
             w.statSolveUs += 1000.0 * ms;
             w.statSolveBytes += sv_step_bytes(w, M, points);
             ++w.statSolveLaunches;
         }
-        for (int j = 0; !w.persistent && j < w.solveEvUsed; ++j) {
-            // k_solve: 284 B per contact point + 12 B per manifold of the launched colour
+        for (int j = 0; !coop && j < w.solveEvUsed; ++j) {
+            // k_solve: 284 B per contact point + 12 B per manifold of the launched colour;
+            // k_solve_flow (colour -1): the same over every colour of the pass
             float ms = 0.f;
             PGX_HIP(hipEventElapsedTime(&ms, w.solveEv[2 * j], w.solveEv[2 * j + 1]));
             int c = w.solveEvColor[j];
-            int nm = w.colorStartHost[c + 1] - w.colorStartHost[c];
+            double bytes = c >= 0 ? 284.0 * pts[c] + 12.0 * (w.colorStartHost[c + 1] - w.colorStartHost[c])
+                                  : 284.0 * points + 12.0 * M;
             w.statSolveUs += 1000.0 * ms;
-            w.statSolveBytes += 284.0 * pts[c] + 12.0 * nm;
+            w.statSolveBytes += bytes;
             ++w.statSolveLaunches;
         }
         w.statStepUs += 1000.0 * w.tTotal;
@@ -895,10 +903,11 @@ int pgx_world_get_stats(pgx_world* h, pgx_stats* out) {
     return PGX_OK;
 }
 
-int pgx_world_set_exec(pgx_world* h, int persistent, int blocks) {
+int pgx_world_set_exec(pgx_world* h, int mode, int blocks) {
     if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
     if (blocks < 0) return fail(PGX_ERR_INVALID, "blocks must be >= 0");
-    h->w.persistent = persistent != 0;
+    if (mode < PGX_EXEC_COLORS || mode > PGX_EXEC_FLOW) return fail(PGX_ERR_INVALID, "unknown execution mode %d", mode);
+    h->w.execMode = mode;
     h->w.persistentBlocksWanted = blocks;
     h->w.persistentBlocks = 0;
     return PGX_OK;

@@ -78,19 +78,20 @@ def test_free_fall_closed_form(api):
     assert abs(gs[2][0, 1] - (-9.81 * 30 / 60)) < 1e-3
 
 
-@pytest.mark.parametrize("persistent", [True, False])
-def test_pyramid_lockstep_bitexact(api, persistent):
-    """20-layer box pyramid, dt 1/60, 4 substeps, 2 iterations: 120 steps in lockstep, with
-    the persistent substep kernel and with one launch per colour."""
-    log(f"[lockstep] box pyramid 120 steps persistent={persistent}")
-    lockstep(scenes.box_pyramid(0), 120, persistent=persistent)
+@pytest.mark.parametrize("mode", ["flow", "colors", "cooperative"])
+def test_pyramid_lockstep_bitexact(api, mode):
+    """20-layer box pyramid, dt 1/60, 4 substeps, 2 iterations: 120 steps in lockstep, in
+    every execution mode of the contact passes (dependency-driven, one launch per colour,
+    one cooperative launch per step)."""
+    log(f"[lockstep] box pyramid 120 steps exec={mode}")
+    lockstep(scenes.box_pyramid(0), 120, mode=mode)
 
 
 @pytest.mark.parametrize("seed", [0, 1])
 def test_mixed_lockstep_bitexact(api, seed):
     """Spheres, capsules and boxes in a container: every pair type, 80 steps."""
     log(f"[lockstep] mixed lattice side=7 seed={seed}")
-    lockstep(scenes.mixed_lattice(seed, side=7), 80, persistent=seed == 0)
+    lockstep(scenes.mixed_lattice(seed, side=7), 80, mode="flow" if seed == 0 else "colors")
 
 
 def test_cluster_lockstep_with_restitution_and_soft(api):
@@ -188,7 +189,7 @@ def test_convex_manifolds_first_steps(api, seed):
 
 def test_convex_lockstep_bitexact(api):
     log("[lockstep] convex pile 90 steps")
-    lockstep(scenes.convex_pile(3, n=50, extent=2.0), 90, persistent=False)
+    lockstep(scenes.convex_pile(3, n=50, extent=2.0), 90)
 
 
 @pytest.mark.parametrize("seed", [0, 1])
@@ -218,4 +219,4 @@ def test_trimesh_manifolds_first_steps(api, seed):
 
 def test_trimesh_lockstep_bitexact(api):
     log("[lockstep] terrain drop 120 steps")
-    lockstep(scenes.terrain_drop(2, quads=12, n=40), 120, persistent=False)
+    lockstep(scenes.terrain_drop(2, quads=12, n=40), 120)
