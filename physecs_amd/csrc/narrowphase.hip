@@ -659,6 +659,18 @@ __device__ __forceinline__ bool in_sorted(const uint64_t* __restrict__ a, int n,
     return lo < n && a[lo] == key;
 }
 
+// Index in a list of this lane's entry when the lanes with `want` append one entry each:
+// one atomic on the list's counter per wave.
+__device__ __forceinline__ int wave_append(int32_t* counter, bool want) {
+    const uint64_t m = __ballot(want);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    return base + __popcll(m & ((1ull << lane) - 1ull));
+}
+
 __global__ __launch_bounds__(256) void k_narrowphase(
     int nPairs, const uint64_t* __restrict__ pkeys, int localBits, const int32_t* __restrict__ worldColStart,
     const int32_t* __restrict__ colOfLocal, const int32_t* __restrict__ cbody, const int32_t* __restrict__ ctype,
@@ -687,13 +699,19 @@ __global__ __launch_bounds__(256) void k_narrowphase(
     }
     Mf m;
     m.numPoints = 0;
-    if (!skip && (ctype[c0] == PGX_TRIANGLE_MESH || ctype[c1] == PGX_TRIANGLE_MESH)) {
-        meshPairs[atomicAdd(nMesh, 1)] = p;   // solved by k_narrowphase_mesh
-        skip = true;
-    } else if (!skip && (ctype[c0] == PGX_CONVEX_MESH || ctype[c1] == PGX_CONVEX_MESH)) {
-        heavy[atomicAdd(nHeavy, 1)] = p;      // solved by k_narrowphase_convex
-        skip = true;
+    // pairs with a triangle mesh go to k_narrowphase_mesh, pairs with a convex mesh to
+    // k_narrowphase_convex: appended with one atomic per wave
+    const bool toMesh = !skip && (ctype[c0] == PGX_TRIANGLE_MESH || ctype[c1] == PGX_TRIANGLE_MESH);
+    const bool toHeavy = !skip && !toMesh && (ctype[c0] == PGX_CONVEX_MESH || ctype[c1] == PGX_CONVEX_MESH);
+    if (__ballot(toMesh)) {
+        int i = wave_append(nMesh, toMesh);
+        if (toMesh) meshPairs[i] = p;
     }
+    if (__ballot(toHeavy)) {
+        int i = wave_append(nHeavy, toHeavy);
+        if (toHeavy) heavy[i] = p;
+    }
+    skip = skip || toMesh || toHeavy;
     if (!skip) {
         int b0 = cbody[c0], b1 = cbody[c1];
         f3 bp0 = xyz(bpos[b0]), bp1 = xyz(bpos[b1]);

@@ -771,6 +771,7 @@ int sv_setup_manifolds(World& w, int nPairs) {
 
 int sv_color(World& w, int M) {
     hipStream_t s = w.stream;
+    const int prevRounds = w.lastRounds;
     w.lastRounds = 0;
     if (M == 0) {
         w.colorStartHost.assign(1, 0);
@@ -792,8 +793,14 @@ int sv_color(World& w, int M) {
     // round-1 claims
     hipLaunchKernelGGL(k_color_claim, dim3(nb), dim3(256), 0, s, M, 1, w.mfKey.p, worldShift, w.worldMStart.p,
                        w.mfColor.p, w.mfB0.p, w.mfB1.p, claims[1]);
+    // Rounds are queued in batches sized from the last step's count (a round after the last
+    // one needed exits at once); the sort into colour order and the colour starts follow,
+    // and one read-back tells whether the batch finished, with the counts and starts. Only
+    // a batch that ran short is followed by another (and the sort is redone).
     int round = 0;
-    int batch = 32;
+    int batch = std::min(MAX_ROUNDS, std::max(8, prevRounds + 4));
+    std::vector<int32_t> starts(130, 0), uc;
+    uint64_t lastKey = 0;
     while (true) {
         for (int r = 0; r < batch && round < MAX_ROUNDS; ++r) {
             ++round;
@@ -802,39 +809,30 @@ int sv_color(World& w, int M) {
                                (const unsigned long long*)claims[round & 1], claims[(round + 1) & 1], w.usedMask.p,
                                w.colorUncol.p, w.counters.p);
         }
+        // stable sort by colour -> solve order
+        hipLaunchKernelGGL(k_color_keys, dim3(nb), dim3(256), 0, s, M, w.mfColor.p, w.colorKeys.p, w.colorPerm.p);
+        radix_sort_u64(s, w.colorKeys.p, w.colorPerm.p, M, 8, w.sortT);
+        hipLaunchKernelGGL(k_color_start, dim3(nb), dim3(256), 0, s, M, w.colorKeys.p, w.colorCount.p);
         PGX_LAUNCH_CHECK();
-        int rem = 0, ovf = 0;
-        PGX_HIP(hipMemcpyAsync(&rem, w.colorUncol.p + round, sizeof(int), hipMemcpyDeviceToHost, s));
+        int ovf = 0;
+        uc.assign(round + 1, 0);
+        PGX_HIP(hipMemcpyAsync(uc.data(), w.colorUncol.p, sizeof(int) * (round + 1), hipMemcpyDeviceToHost, s));
         PGX_HIP(hipMemcpyAsync(&ovf, w.counters.p + 5, sizeof(int), hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(&lastKey, w.colorKeys.p + (M - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(starts.data(), w.colorCount.p, sizeof(int32_t) * 130, hipMemcpyDeviceToHost, s));
         PGX_HIP(hipStreamSynchronize(s));
         if (ovf) throw PgxError{PGX_ERR_CAPACITY, "a body touches more than 127 contact colours"};
-        if (rem == 0) break;
+        if (uc[round] == 0) break;
         if (round >= MAX_ROUNDS) throw PgxError{PGX_ERR_INVALID, "graph colouring did not converge"};
         batch = 16;
     }
     // rounds actually used: the first whose count of uncoloured manifolds is zero
-    {
-        std::vector<int> uc(round + 1, 0);
-        PGX_HIP(hipMemcpyAsync(uc.data(), w.colorUncol.p, sizeof(int) * (round + 1), hipMemcpyDeviceToHost, s));
-        PGX_HIP(hipStreamSynchronize(s));
-        int used = round;
-        for (int r = 1; r <= round; ++r)
-            if (uc[r] == 0) { used = r; break; }
-        round = used;
-    }
-    // stable sort by colour -> solve order
-    hipLaunchKernelGGL(k_color_keys, dim3(nb), dim3(256), 0, s, M, w.mfColor.p, w.colorKeys.p, w.colorPerm.p);
-    radix_sort_u64(s, w.colorKeys.p, w.colorPerm.p, M, 8, w.sortT);
-    hipLaunchKernelGGL(k_color_start, dim3(nb), dim3(256), 0, s, M, w.colorKeys.p, w.colorCount.p);
-    PGX_LAUNCH_CHECK();
-    uint64_t lastKey = 0;
-    std::vector<int32_t> starts(130, 0);
-    PGX_HIP(hipMemcpyAsync(&lastKey, w.colorKeys.p + (M - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    PGX_HIP(hipMemcpyAsync(starts.data(), w.colorCount.p, sizeof(int32_t) * 130, hipMemcpyDeviceToHost, s));
-    PGX_HIP(hipStreamSynchronize(s));
+    int used = round;
+    for (int r = 1; r <= round; ++r)
+        if (uc[r] == 0) { used = r; break; }
     int nColors = (int)lastKey + 1;
     w.colorStartHost.assign(starts.begin(), starts.begin() + nColors + 1);
-    w.lastRounds = round;
+    w.lastRounds = used;
     // contact setup, written in solve order
     int cur = w.cacheCur ^ 1, prv = w.cacheCur;
     hipLaunchKernelGGL(k_setup, dim3(nb), dim3(256), 0, s, M, w.manifoldCap, w.colorPerm.p, w.mfSrc.p, w.mfTri.p,
