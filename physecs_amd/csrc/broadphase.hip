@@ -489,8 +489,8 @@ int bp_find_pairs(World& w) {
     int W = w.W;
     int worldShift = 30;
     // [8] convex pairs, [9] EPA capacity errors, [10] mesh pairs, [11] mesh manifold slots,
-    // [12] mesh capacity errors
-    fill_i32(s, w.counters.p, 0, 13);
+    // [12] mesh capacity errors, [13] mesh candidate triangles
+    fill_i32(s, w.counters.p, 0, 14);
     if (n > 0) {
         // quantisation box -> Morton codes of the dynamic colliders -> sort
         hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p);

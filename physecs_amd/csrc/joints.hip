@@ -233,8 +233,10 @@ __device__ __forceinline__ m3 world_inv_inertia_j(const JointArgs& a, int b) {
     return mulmm(mulmm(rot, invI), transpose(rot));
 }
 
-// Constraint1D<flags>::preSolve / Constraint1DW<flags>::preSolve for one row
-__device__ void row_presolve(const JointArgs& a, int j, int r, int flags, bool simd) {
+// Constraint1D<flags>::preSolve / Constraint1DW<flags>::preSolve for one row, on the
+// bodies' pseudo velocities held by the caller (pv, pw; pw.w = constraint count)
+__device__ void row_presolve(const JointArgs& a, int j, int r, int flags, bool simd, float4& pv0, float4& pw0,
+                             float4& pv1, float4& pw1) {
     int b0 = a.jB0[j], b1 = a.jB1[j];
     float im0 = 0.f, im1 = 0.f;
     m3 I0 = mzero(), I1 = mzero();
@@ -270,29 +272,27 @@ __device__ void row_presolve(const JointArgs& a, int j, int r, int flags, bool s
         else lambda = gclamp(lambda, mn, mx);
     }
     if (b0 >= 0) {
-        float4 pv = a.pvel[b0], pw = a.pang[b0];
-        if (!ang) set_xyz(pv, add(xyz(pv), mul(lambda, lin0t)));
-        set_xyz(pw, add(xyz(pw), mul(lambda, ang0t)));
-        pw.w = pw.w + 1.f;
-        a.pvel[b0] = pv; a.pang[b0] = pw;
+        if (!ang) set_xyz(pv0, add(xyz(pv0), mul(lambda, lin0t)));
+        set_xyz(pw0, add(xyz(pw0), mul(lambda, ang0t)));
+        pw0.w = pw0.w + 1.f;
     }
     if (b1 >= 0) {
-        float4 pv = a.pvel[b1], pw = a.pang[b1];
-        if (!ang) set_xyz(pv, sub(xyz(pv), mul(lambda, lin1t)));
-        set_xyz(pw, sub(xyz(pw), mul(lambda, ang1t)));
-        pw.w = pw.w + 1.f;
-        a.pvel[b1] = pv; a.pang[b1] = pw;
+        if (!ang) set_xyz(pv1, sub(xyz(pv1), mul(lambda, lin1t)));
+        set_xyz(pw1, sub(xyz(pw1), mul(lambda, ang1t)));
+        pw1.w = pw1.w + 1.f;
     }
 }
 
-// Constraint1DW<flags>::solve for one lane (src/Constraint1DW.cpp:117-233)
-__device__ void row_solve_simd(const JointArgs& a, int j, int r, int flags, bool warm) {
+// Constraint1DW<flags>::solve for one lane (src/Constraint1DW.cpp:117-233), on the bodies'
+// solver velocities held by the caller (bv, bw; an angular row reads and writes only bw)
+__device__ void row_solve_simd(const JointArgs& a, int j, int r, int flags, bool warm, f3& bv0, f3& bw0, f3& bv1,
+                               f3& bw1) {
     int b0 = a.jB0[j], b1 = a.jB1[j];
     bool ang = flags & JF_ANGULAR;
     float h = a.h;
     f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
-    if (b0 >= 0) { if (!ang) v0 = xyz(a.tvel[b0]); w0 = xyz(a.tang[b0]); }
-    if (b1 >= 0) { if (!ang) v1 = xyz(a.tvel[b1]); w1 = xyz(a.tang[b1]); }
+    if (b0 >= 0) { if (!ang) v0 = bv0; w0 = bw0; }
+    if (b1 >= 0) { if (!ang) v1 = bv1; w1 = bw1; }
     float4 L = JROW(r, JR_LIN), G0 = JROW(r, JR_ANG0), G1 = JROW(r, JR_ANG1);
     float4 L0 = JROW(r, JR_LIN0T), L1 = JROW(r, JR_LIN1T), T0 = JROW(r, JR_ANG0T), T1 = JROW(r, JR_ANG1T);
     float4 EX = JROW(r, JR_EXTRA);
@@ -345,12 +345,12 @@ __device__ void row_solve_simd(const JointArgs& a, int j, int r, int flags, bool
     L0.w = lam;
     JROW(r, JR_LIN0T) = L0;
     if (b0 >= 0) {
-        if (!ang) { float4 t = a.tvel[b0]; set_xyz(t, v0); a.tvel[b0] = t; }
-        float4 t = a.tang[b0]; set_xyz(t, w0); a.tang[b0] = t;
+        if (!ang) bv0 = v0;
+        bw0 = w0;
     }
     if (b1 >= 0) {
-        if (!ang) { float4 t = a.tvel[b1]; set_xyz(t, v1); a.tvel[b1] = t; }
-        float4 t = a.tang[b1]; set_xyz(t, w1); a.tang[b1] = t;
+        if (!ang) bv1 = v1;
+        bw1 = w1;
     }
 }
 
@@ -410,33 +410,64 @@ __device__ void row_solve_scalar(const JointArgs& a, int j, int r, int flags, bo
     }
 }
 
-// one SIMD colour: a thread per joint, its rows in flag-type order
+// one SIMD colour: a thread per joint, its rows in flag-type order. The joint's bodies
+// belong to no other joint of the colour, so their velocities (or pseudo velocities) stay
+// in registers across the joint's rows: one load and one store per body.
 __global__ void k_joint_color(JointArgs a, int begin, int end, int phase, int warm) {
     int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= end) return;
     int j = a.order[k];
     int n = a.jNRows[j];
+    int b0 = a.jB0[j], b1 = a.jB1[j];
+    float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (phase == 0) {
+        float4 pv0 = z, pw0 = z, pv1 = z, pw1 = z;
+        if (b0 >= 0) { pv0 = a.pvel[b0]; pw0 = a.pang[b0]; }
+        if (b1 >= 0) { pv1 = a.pvel[b1]; pw1 = a.pang[b1]; }
+        for (int f = 0; f < 6; ++f)
+            for (int r = 0; r < n; ++r) {
+                int fl = row_flags(a, j, r);
+                if (fl == kFlagOrder[f]) row_presolve(a, j, r, fl, true, pv0, pw0, pv1, pw1);
+            }
+        if (b0 >= 0) { a.pvel[b0] = pv0; a.pang[b0] = pw0; }
+        if (b1 >= 0) { a.pvel[b1] = pv1; a.pang[b1] = pw1; }
+        return;
+    }
+    float4 tv0 = z, tw0 = z, tv1 = z, tw1 = z;
+    if (b0 >= 0) { tv0 = a.tvel[b0]; tw0 = a.tang[b0]; }
+    if (b1 >= 0) { tv1 = a.tvel[b1]; tw1 = a.tang[b1]; }
+    f3 v0 = xyz(tv0), w0 = xyz(tw0), v1 = xyz(tv1), w1 = xyz(tw1);
     for (int f = 0; f < 6; ++f)
         for (int r = 0; r < n; ++r) {
             int fl = row_flags(a, j, r);
-            if (fl != kFlagOrder[f]) continue;
-            if (phase == 0) row_presolve(a, j, r, fl, true);
-            else row_solve_simd(a, j, r, fl, warm != 0);
+            if (fl == kFlagOrder[f]) row_solve_simd(a, j, r, fl, warm != 0, v0, w0, v1, w1);
         }
+    if (b0 >= 0) { set_xyz(tv0, v0); set_xyz(tw0, w0); a.tvel[b0] = tv0; a.tang[b0] = tw0; }
+    if (b1 >= 0) { set_xyz(tv1, v1); set_xyz(tw1, w1); a.tvel[b1] = tv1; a.tang[b1] = tw1; }
 }
 
-// the overflow colour: one thread, flag type by flag type over all its joints
+// the overflow colour: one thread, flag type by flag type over all its joints (which may
+// share bodies), each row on the bodies' velocities in memory
 __global__ void k_joint_overflow(JointArgs a, int begin, int end, int phase, int warm) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     for (int f = 0; f < 6; ++f)
         for (int k = begin; k < end; ++k) {
             int j = a.order[k];
             int n = a.jNRows[j];
+            int b0 = a.jB0[j], b1 = a.jB1[j];
             for (int r = 0; r < n; ++r) {
                 int fl = row_flags(a, j, r);
                 if (fl != kFlagOrder[f]) continue;
-                if (phase == 0) row_presolve(a, j, r, fl, false);
-                else row_solve_scalar(a, j, r, fl, warm != 0);
+                if (phase == 0) {
+                    float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+                    float4 pv0 = b0 >= 0 ? a.pvel[b0] : z, pw0 = b0 >= 0 ? a.pang[b0] : z;
+                    float4 pv1 = b1 >= 0 ? a.pvel[b1] : z, pw1 = b1 >= 0 ? a.pang[b1] : z;
+                    row_presolve(a, j, r, fl, false, pv0, pw0, pv1, pw1);
+                    if (b0 >= 0) { a.pvel[b0] = pv0; a.pang[b0] = pw0; }
+                    if (b1 >= 0) { a.pvel[b1] = pv1; a.pang[b1] = pw1; }
+                } else {
+                    row_solve_scalar(a, j, r, fl, warm != 0);
+                }
             }
         }
 }

@@ -783,62 +783,168 @@ __global__ __launch_bounds__(64) void k_narrowphase_convex(
     }
 }
 
-// Pairs with a static triangle mesh: a fixed grid of `lanes` threads walks the list; lane
-// l owns mesh workspace l (and EPA workspace l for convex bodies). A pair's manifolds go to
-// a block of data slots after the pair slots, reserved with one atomic add.
-__global__ __launch_bounds__(64) void k_narrowphase_mesh(
-    const int32_t* __restrict__ nMeshP, const int32_t* __restrict__ meshPairs, const uint64_t* __restrict__ pkeys,
-    int localBits, const int32_t* __restrict__ worldColStart, const int32_t* __restrict__ colOfLocal,
-    const int32_t* __restrict__ cbody, const int32_t* __restrict__ ctype, const float4* __restrict__ cparams,
-    const float4* __restrict__ clpos, const float4* __restrict__ clquat, const float4* __restrict__ bpos,
-    const float4* __restrict__ bquat, ConvexView cv, TriMeshView tv, double* epaV, double* epaFN, int32_t* epaFI,
-    int32_t* epaE, int epaLanes, int32_t* tmWs, int lanes, int32_t* __restrict__ npNum,
-    float4* __restrict__ npNormal, float4* __restrict__ npPts, int32_t* __restrict__ npTri,
-    int32_t* __restrict__ meshFirst, int32_t* __restrict__ meshCount, int32_t* __restrict__ slotCounter, int mmCap,
-    int pairCap, int32_t* __restrict__ errors) {
-    int lane = blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= lanes) return;
-    int nMesh = *nMeshP;
+// ------------------------------------------------------------------ triangle-mesh pairs
+// Everything the three mesh passes read.
+struct MeshArgs {
+    const int32_t* nMeshP;                 // number of mesh pairs (device)
+    const int32_t* meshPairs;              // their pair slots
+    const uint64_t* pkeys;
+    int localBits;
+    const int32_t *worldColStart, *colOfLocal, *cbody, *ctype;
+    const float4 *cparams, *clpos, *clquat, *bpos, *bquat;
+    ConvexView cv;
+    TriMeshView tv;
+    int32_t* errors;                       // capacity and EPA errors
+};
+
+struct MeshPair {
+    int p, tb;
+    bool flip;
+    float4 g;
+    f3 pos0, pos1, lpos;
+    qf or0, or1, lor;
+    int4 mi;
+};
+
+// mesh pair h: the body collider and the mesh, their world poses, the body in the mesh's
+// frame. src/Collision.cpp:897-906: a mesh on side 0 is solved as (other, mesh) and flipped.
+// False for a mesh-mesh pair (no routine).
+__device__ bool mesh_pair(const MeshArgs& A, int h, MeshPair& m, SupConvex& sc) {
+    m.p = A.meshPairs[h];
+    uint64_t key = A.pkeys[m.p];
+    uint64_t lmask = (1ull << A.localBits) - 1;
+    int wd = (int)(key >> (2 * A.localBits));
+    int base = A.worldColStart[wd];
+    int c0 = A.colOfLocal[base + (int)((key >> A.localBits) & lmask)];
+    int c1 = A.colOfLocal[base + (int)(key & lmask)];
+    int t0 = A.ctype[c0], t1 = A.ctype[c1];
+    if (t0 == PGX_TRIANGLE_MESH && t1 == PGX_TRIANGLE_MESH) return false;
+    m.flip = t0 == PGX_TRIANGLE_MESH;
+    int cb = m.flip ? c1 : c0, cm = m.flip ? c0 : c1;
+    int bb = A.cbody[cb], bm = A.cbody[cm];
+    qf qb = q4(A.bquat[bb]), qm = q4(A.bquat[bm]);
+    m.pos0 = add(xyz(A.bpos[bb]), qrot(qb, xyz(A.clpos[cb])));
+    m.or0 = qmul(qb, q4(A.clquat[cb]));
+    m.pos1 = add(xyz(A.bpos[bm]), qrot(qm, xyz(A.clpos[cm])));
+    m.or1 = qmul(qm, q4(A.clquat[cm]));
+    m.tb = A.ctype[cb];
+    m.g = A.cparams[cb];
+    m.mi = A.tv.mesh[(int)A.cparams[cm].w];
+    mesh_local(m.tb, m.g, m.pos0, m.or0, m.pos1, m.or1, A.cv, m.lpos, m.lor, sc);
+    return true;
+}
+
+// exclusive prefix of n over the wave and the wave's total
+__device__ __forceinline__ int wave_scan(int n, int& total) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    int incl = n;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    total = __shfl(incl, WAVE - 1);
+    return incl - n;
+}
+
+// pass 1, one lane per mesh pair: the BVH query, and a block of candidate slots (triangle,
+// pair) reserved for the pair with one atomic per wave
+__global__ __launch_bounds__(64) void k_mesh_query(MeshArgs A, int32_t* tmWs, int lanes, int32_t* __restrict__ candTotal,
+                                                   int candCap, int32_t* __restrict__ cand,
+                                                   int32_t* __restrict__ pairOf, int32_t* __restrict__ candOff,
+                                                   int32_t* __restrict__ candCount, int32_t* __restrict__ meshCount) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nMesh = *A.nMeshP;
+    MeshWs ws{tmWs + lane, (size_t)lanes};
+    for (int h0 = lane & ~(WAVE - 1); h0 < nMesh; h0 += lanes) {   // the same trip count for a whole wave
+        const int h = h0 + (lane & (WAVE - 1));
+        int nc = 0;
+        if (h < nMesh) {
+            MeshPair m;
+            SupConvex sc;
+            meshCount[A.meshPairs[h]] = 0;
+            if (mesh_pair(A, h, m, sc)) {
+                MeshCtx mc{&A.tv, m.mi};
+                nc = mesh_query(m.tb, m.g, m.lpos, m.lor, A.cv, mc, ws);
+                if (nc < 0) { atomicAdd(A.errors, 1); nc = 0; }
+            }
+        }
+        int total;
+        int off = wave_scan(nc, total);
+        int base = 0;
+        if ((lane & (WAVE - 1)) == WAVE - 1 && total) base = atomicAdd(candTotal, total);
+        base = __shfl(base, WAVE - 1) + off;
+        if (h < nMesh) {
+            if (base + nc > candCap) { atomicAdd(A.errors, 1); nc = 0; }
+            for (int i = 0; i < nc; ++i) {
+                cand[base + i] = ws.cand(i);
+                pairOf[base + i] = h;
+            }
+            candOff[h] = base;
+            candCount[h] = nc;
+        }
+    }
+}
+
+// pass 2, one lane per (pair, candidate triangle): the per-shape test, its contact record
+__global__ __launch_bounds__(64) void k_mesh_test(MeshArgs A, double* epaV, double* epaFN, int32_t* epaFI,
+                                                  int32_t* epaE, int epaLanes, const int32_t* __restrict__ candTotal,
+                                                  int candCap, const int32_t* __restrict__ cand,
+                                                  const int32_t* __restrict__ pairOf, MeshRecs recs) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= epaLanes) return;
+    const int total = min(*candTotal, candCap);
     EpaWs ews;
     ews.vd = epaV + lane; ews.fn = epaFN + lane; ews.fi = epaFI + lane; ews.ed = epaE + lane;
     ews.L = (size_t)epaLanes;
-    ews.overflow = false;
-    MeshWs ws{tmWs + lane, (size_t)lanes};
-    uint64_t lmask = (1ull << localBits) - 1;
-    for (int h = lane; h < nMesh; h += lanes) {
-        int p = meshPairs[h];
-        uint64_t key = pkeys[p];
-        int wd = (int)(key >> (2 * localBits));
-        int base = worldColStart[wd];
-        int c0 = colOfLocal[base + (int)((key >> localBits) & lmask)];
-        int c1 = colOfLocal[base + (int)(key & lmask)];
-        int t0 = ctype[c0], t1 = ctype[c1];
-        meshCount[p] = 0;
-        if (t0 == PGX_TRIANGLE_MESH && t1 == PGX_TRIANGLE_MESH) continue;   // no mesh-mesh routine
-        // src/Collision.cpp:897-906: a mesh on side 0 is solved as (other, mesh) and flipped
-        bool flip = t0 == PGX_TRIANGLE_MESH;
-        int cb = flip ? c1 : c0, cm = flip ? c0 : c1;
-        int bb = cbody[cb], bm = cbody[cm];
-        qf qb = q4(bquat[bb]), qm = q4(bquat[bm]);
-        f3 pos0 = add(xyz(bpos[bb]), qrot(qb, xyz(clpos[cb])));
-        qf or0 = qmul(qb, q4(clquat[cb]));
-        f3 pos1 = add(xyz(bpos[bm]), qrot(qm, xyz(clpos[cm])));
-        qf or1 = qmul(qm, q4(clquat[cm]));
-        int tb = ctype[cb];
-        float4 g = cparams[cb];
-        MeshCtx mc{&tv, tv.mesh[(int)cparams[cm].w]};
-        f3 lpos;
-        qf lor;
+    for (int k = lane; k < total; k += epaLanes) {
+        MeshPair m;
+        SupConvex sc;
+        mesh_pair(A, pairOf[k], m, sc);
+        MeshCtx mc{&A.tv, m.mi};
+        ews.overflow = false;
+        TCI ci;
         bool epaErr = false;
-        int nct = mesh_contacts(tb, g, pos0, or0, pos1, or1, mc, cv, ews, ws, lpos, lor, epaErr);
-        if (epaErr || nct < 0) atomicAdd(errors, 1);
-        if (nct <= 0) continue;
-        int slot = atomicAdd(slotCounter, nct);
-        if (slot + nct > mmCap) { atomicAdd(errors, 1); continue; }
-        MeshOut out{npNum, npNormal, npPts, npTri, pairCap + slot, slot, 0, flip};
-        mesh_emit(tb, g, lpos, lor, pos1, or1, mc, cv, ws, nct, out);
-        meshFirst[p] = slot;
-        meshCount[p] = out.n;
+        bool hit = mesh_test(m.tb, m.g, m.lpos, m.lor, sc, mc, cand[k], ews, ci, epaErr);
+        if (epaErr) atomicAdd(A.errors, 1);
+        recs.store(k, hit, ci);
+    }
+}
+
+// pass 3, one lane per mesh pair: the pair's hit records in candidate order, a block of
+// manifold slots (one atomic per wave), the emission passes
+__global__ __launch_bounds__(64) void k_mesh_emit(MeshArgs A, int32_t* tmWs, int lanes,
+                                                  const int32_t* __restrict__ candOff,
+                                                  const int32_t* __restrict__ candCount, MeshRecs recs,
+                                                  int32_t* __restrict__ npNum, float4* __restrict__ npNormal,
+                                                  float4* __restrict__ npPts, int32_t* __restrict__ npTri,
+                                                  int32_t* __restrict__ meshFirst, int32_t* __restrict__ meshCount,
+                                                  int32_t* __restrict__ slotCounter, int mmCap, int pairCap) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nMesh = *A.nMeshP;
+    MeshWs ws{tmWs + lane, (size_t)lanes};
+    for (int h0 = lane & ~(WAVE - 1); h0 < nMesh; h0 += lanes) {
+        const int h = h0 + (lane & (WAVE - 1));
+        int nct = 0;
+        MeshPair m;
+        SupConvex sc;
+        bool ok = h < nMesh && mesh_pair(A, h, m, sc);
+        if (ok) {
+            const int off = candOff[h], nc = candCount[h];
+            for (int i = 0; i < nc; ++i)
+                if (recs.hit(off + i)) ws.order(nct++) = off + i;
+        }
+        int total;
+        int pre = wave_scan(nct, total);
+        int base = 0;
+        if ((lane & (WAVE - 1)) == WAVE - 1 && total) base = atomicAdd(slotCounter, total);
+        const int slot = __shfl(base, WAVE - 1) + pre;
+        if (!ok || nct == 0) continue;
+        if (slot + nct > mmCap) { atomicAdd(A.errors, 1); continue; }
+        MeshCtx mc{&A.tv, m.mi};
+        MeshOut out{npNum, npNormal, npPts, npTri, pairCap + slot, slot, 0, m.flip};
+        mesh_emit(m.tb, m.g, m.lpos, m.lor, m.pos1, m.or1, mc, A.cv, ws, recs, nct, out);
+        meshFirst[m.p] = slot;
+        meshCount[m.p] = out.n;
     }
 }
 
@@ -860,14 +966,23 @@ void np_run(World& w, int nPairs) {
         PGX_LAUNCH_CHECK();
     }
     if (w.nTriMesh > 0) {
-        ConvexView cv{w.cvVerts.p, w.cvMesh.p, w.cvFace.p, w.cvFaceIdx.p, w.cvFaceN.p, w.cvFaceC.p};
-        TriMeshView tv{w.tmVerts.p, w.tmTris.p, w.tmTriN.p, w.tmTriC.p, w.tmNodes.p, w.tmMesh.p};
-        hipLaunchKernelGGL(k_narrowphase_mesh, dim3(div_up(w.meshLanes, 64)), dim3(64), 0, w.stream,
-                           w.counters.p + 10, w.meshPairs.p, w.pkeys.p, w.localBits, w.worldColStart.p,
-                           w.colOfLocal.p, w.cbody.p, w.ctype.p, w.cparams.p, w.clpos.p, w.clquat.p, w.bpos.p,
-                           w.bquat.p, cv, tv, w.epaV.p, w.epaFN.p, w.epaFI.p, w.epaE.p, w.epaLanes, w.tmWs.p,
-                           w.meshLanes, w.npNum.p, w.npNormal.p, w.npPts.p, w.npTri.p, w.meshFirst.p,
-                           w.meshCount.p, w.counters.p + 11, w.mmCap, w.pairCap, w.counters.p + 12);
+        MeshArgs A;
+        A.nMeshP = w.counters.p + 10; A.meshPairs = w.meshPairs.p; A.pkeys = w.pkeys.p; A.localBits = w.localBits;
+        A.worldColStart = w.worldColStart.p; A.colOfLocal = w.colOfLocal.p; A.cbody = w.cbody.p; A.ctype = w.ctype.p;
+        A.cparams = w.cparams.p; A.clpos = w.clpos.p; A.clquat = w.clquat.p; A.bpos = w.bpos.p; A.bquat = w.bquat.p;
+        A.cv = ConvexView{w.cvVerts.p, w.cvMesh.p, w.cvFace.p, w.cvFaceIdx.p, w.cvFaceN.p, w.cvFaceC.p};
+        A.tv = TriMeshView{w.tmVerts.p, w.tmTris.p, w.tmTriN.p, w.tmTriC.p, w.tmNodes.p, w.tmMesh.p};
+        A.errors = w.counters.p + 12;
+        MeshRecs recs{w.tmRecs.p, (size_t)w.tmCandCap};
+        hipLaunchKernelGGL(k_mesh_query, dim3(div_up(w.meshLanes, 64)), dim3(64), 0, w.stream, A, w.tmWs.p,
+                           w.meshLanes, w.counters.p + 13, w.tmCandCap, w.tmCand.p, w.tmPairOf.p, w.tmCandOff.p,
+                           w.tmCandCount.p, w.meshCount.p);
+        hipLaunchKernelGGL(k_mesh_test, dim3(div_up(w.epaLanes, 64)), dim3(64), 0, w.stream, A, w.epaV.p, w.epaFN.p,
+                           w.epaFI.p, w.epaE.p, w.epaLanes, w.counters.p + 13, w.tmCandCap, w.tmCand.p, w.tmPairOf.p,
+                           recs);
+        hipLaunchKernelGGL(k_mesh_emit, dim3(div_up(w.meshLanes, 64)), dim3(64), 0, w.stream, A, w.tmWs.p,
+                           w.meshLanes, w.tmCandOff.p, w.tmCandCount.p, recs, w.npNum.p, w.npNormal.p, w.npPts.p,
+                           w.npTri.p, w.meshFirst.p, w.meshCount.p, w.counters.p + 11, w.mmCap, w.pairCap);
         PGX_LAUNCH_CHECK();
     }
 }

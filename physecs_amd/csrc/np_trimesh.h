@@ -7,10 +7,13 @@
 // std::sort's own introsort and emitted unless their feature's vertices were already
 // covered (:906-953). The triangle distance routines follow src/GeomUtil.cpp:61-127,469-665.
 //
-// Per-lane state (candidate triangles, contact records, their order, covered vertices)
-// lives in a lane-interleaved workspace (word w of lane l at w * lanes + l). A pair's
-// manifolds go to a block of slots reserved with one atomic add, in emission order.
-// Included by narrowphase.hip after np_convex.h.
+// Three passes: each pair queries the BVH and reserves a block of candidate triangles;
+// every (pair, candidate) is tested by a lane of its own (contact records stored flat,
+// one per candidate); each pair then emits its manifolds from its records. Per-lane state
+// of the query and emission passes (candidate triangles, the contact order, covered
+// vertices) lives in a lane-interleaved workspace (word w of lane l at w * lanes + l). A
+// pair's manifolds go to a block of slots reserved with one atomic add, in emission
+// order. Included by narrowphase.hip after np_convex.h.
 #pragma once
 #include "introsort.h"
 
@@ -35,40 +38,55 @@ struct TCI {                 // TriangleContactInfo (:22-36)
     int feature, featureIndex, boxFeature, boxAxis;
 };
 
+// per-lane workspace of the query and emission passes
 struct MeshWs {
     int32_t* w;              // lane's first word
     size_t L;
     __device__ int& cand(int i) { return w[(size_t)i * L]; }
     __device__ int& order(int i) { return w[(size_t)(TM_MAXCAND + i) * L]; }
-    __device__ int& rec(int c, int f) { return w[(size_t)(2 * TM_MAXCAND + c * TM_CI_WORDS + f) * L]; }
-    __device__ int& voided(int i) { return w[(size_t)((2 + TM_CI_WORDS) * TM_MAXCAND + i) * L]; }
-    __device__ void store(int c, const TCI& t) {
-        rec(c, 0) = t.tri;
-        rec(c, 1) = __float_as_int(t.normal.x); rec(c, 2) = __float_as_int(t.normal.y); rec(c, 3) = __float_as_int(t.normal.z);
-        rec(c, 4) = __float_as_int(t.cpBody.x); rec(c, 5) = __float_as_int(t.cpBody.y); rec(c, 6) = __float_as_int(t.cpBody.z);
-        rec(c, 7) = __float_as_int(t.cpTri.x); rec(c, 8) = __float_as_int(t.cpTri.y); rec(c, 9) = __float_as_int(t.cpTri.z);
-        rec(c, 10) = __float_as_int(t.distance);
-        rec(c, 11) = t.feature; rec(c, 12) = t.featureIndex; rec(c, 13) = t.boxFeature | (t.boxAxis << 8);
+    __device__ int& voided(int i) { return w[(size_t)(2 * TM_MAXCAND + i) * L]; }
+};
+
+// contact records, one per candidate triangle: word f of record k at f * cap + k (word
+// TM_CI_WORDS: 1 if the triangle test hit)
+struct MeshRecs {
+    int32_t* w;
+    size_t cap;
+    __device__ int& rec(int k, int f) const { return w[(size_t)f * cap + k]; }
+    __device__ bool hit(int k) const { return rec(k, TM_CI_WORDS) != 0; }
+    __device__ float dist(int k) const { return __int_as_float(rec(k, 10)); }
+    __device__ int feature(int k) const { return rec(k, 11); }
+    __device__ void store(int k, bool h, const TCI& t) const {
+        rec(k, TM_CI_WORDS) = h ? 1 : 0;
+        if (!h) return;
+        rec(k, 0) = t.tri;
+        rec(k, 1) = __float_as_int(t.normal.x); rec(k, 2) = __float_as_int(t.normal.y); rec(k, 3) = __float_as_int(t.normal.z);
+        rec(k, 4) = __float_as_int(t.cpBody.x); rec(k, 5) = __float_as_int(t.cpBody.y); rec(k, 6) = __float_as_int(t.cpBody.z);
+        rec(k, 7) = __float_as_int(t.cpTri.x); rec(k, 8) = __float_as_int(t.cpTri.y); rec(k, 9) = __float_as_int(t.cpTri.z);
+        rec(k, 10) = __float_as_int(t.distance);
+        rec(k, 11) = t.feature; rec(k, 12) = t.featureIndex; rec(k, 13) = t.boxFeature | (t.boxAxis << 8);
     }
-    __device__ TCI load(int c) {
+    __device__ TCI load(int k) const {
         TCI t;
-        t.tri = rec(c, 0);
-        t.normal = mk3(__int_as_float(rec(c, 1)), __int_as_float(rec(c, 2)), __int_as_float(rec(c, 3)));
-        t.cpBody = mk3(__int_as_float(rec(c, 4)), __int_as_float(rec(c, 5)), __int_as_float(rec(c, 6)));
-        t.cpTri = mk3(__int_as_float(rec(c, 7)), __int_as_float(rec(c, 8)), __int_as_float(rec(c, 9)));
-        t.distance = __int_as_float(rec(c, 10));
-        t.feature = rec(c, 11); t.featureIndex = rec(c, 12);
-        t.boxFeature = rec(c, 13) & 255; t.boxAxis = rec(c, 13) >> 8;
+        t.tri = rec(k, 0);
+        t.normal = mk3(__int_as_float(rec(k, 1)), __int_as_float(rec(k, 2)), __int_as_float(rec(k, 3)));
+        t.cpBody = mk3(__int_as_float(rec(k, 4)), __int_as_float(rec(k, 5)), __int_as_float(rec(k, 6)));
+        t.cpTri = mk3(__int_as_float(rec(k, 7)), __int_as_float(rec(k, 8)), __int_as_float(rec(k, 9)));
+        t.distance = __int_as_float(rec(k, 10));
+        t.feature = rec(k, 11); t.featureIndex = rec(k, 12);
+        t.boxFeature = rec(k, 13) & 255; t.boxAxis = rec(k, 13) >> 8;
         return t;
     }
 };
 
-// index array over the contact order, compared by distance (TriangleContactInfo::operator<)
+// index array over the contact order (record indices), compared by distance
+// (TriangleContactInfo::operator<)
 struct OrderArr {
     MeshWs* ws;
+    const MeshRecs* recs;
     __device__ int get(int i) { return ws->order(i); }
     __device__ void set(int i, int v) { ws->order(i) = v; }
-    __device__ bool less(int a, int b) { return __int_as_float(ws->rec(a, 10)) < __int_as_float(ws->rec(b, 10)); }
+    __device__ bool less(int a, int b) { return recs->dist(a) < recs->dist(b); }
 };
 
 // ------------------------------------------------------------------ GeomUtil (triangles)
@@ -673,16 +691,26 @@ struct MeshOut {
     }
 };
 
-// body collider (type t, params g, world pose pos0/or0) against static mesh mc at pos1/or1.
-// Returns the number of contact records (the slot block to reserve), -1 on a capacity error.
-__device__ int mesh_contacts(int t, float4 g, f3 pos0, qf or0, f3 pos1, qf or1, const MeshCtx& mc,
-                             const ConvexView& cv, EpaWs& ews, MeshWs& ws, f3& localPos, qf& localOr, bool& epaErr) {
+// The body collider (type t, params g, world pose pos0/or0) in the mesh's frame (mesh at
+// pos1/or1), and for a convex mesh its support function there.
+__device__ void mesh_local(int t, float4 g, f3 pos0, qf or0, f3 pos1, qf or1, const ConvexView& cv, f3& localPos,
+                           qf& localOr, SupConvex& sc) {
     qf inv1 = qinverse(or1);
     localPos = qrot(inv1, sub(pos0, pos1));
     localOr = qmul(inv1, or0);
-    // getBounds(localPos, localOr, geom0), src/BoundsUtil.cpp:35-69
+    if (t == PGX_CONVEX_MESH) {
+        int4 ci = cv.mesh[(int)g.w];
+        sc = SupConvex{localPos, mat3_cast(localOr), mk3(g.x, g.y, g.z), cv.verts + ci.x, ci.y};
+    }
+}
+
+// The body's bounds in the mesh frame (getBounds(localPos, localOr, geom0),
+// src/BoundsUtil.cpp:35-69) and the mesh's triangles whose BVH leaves they overlap, depth
+// first, left child first (overlapBvh, src/TriangleMesh.cpp:163-192). Returns the number
+// of candidates, -1 on a capacity error.
+__device__ int mesh_query(int t, float4 g, f3 localPos, qf localOr, const ConvexView& cv, const MeshCtx& mc,
+                          MeshWs& ws) {
     f3 mn, mx;
-    SupConvex sc;
     if (t == PGX_SPHERE) {
         mn = sub(localPos, s3(g.x)); mx = add(localPos, s3(g.x));
     } else if (t == PGX_CAPSULE) {
@@ -694,15 +722,14 @@ __device__ int mesh_contacts(int t, float4 g, f3 pos0, qf or0, f3 pos1, qf or1, 
         mn = sub(localPos, w); mx = add(localPos, w);
     } else {
         int4 ci = cv.mesh[(int)g.w];
-        sc = SupConvex{localPos, mat3_cast(localOr), mk3(g.x, g.y, g.z), cv.verts + ci.x, ci.y};
+        f3 scale = mk3(g.x, g.y, g.z);
         mn = s3(3.402823466e+38f); mx = s3(-3.402823466e+38f);
         int np = (ci.y + 3) & ~3;
         for (int i = 0; i < np; ++i) {
-            f3 p = add(localPos, qrot(localOr, mulv(sc.scale, xyz(cv.verts[ci.x + i]))));
+            f3 p = add(localPos, qrot(localOr, mulv(scale, xyz(cv.verts[ci.x + i]))));
             mn = vmin(mn, p); mx = vmax(mx, p);
         }
     }
-    // overlapBvh: depth first, left child first (src/TriangleMesh.cpp:163-192)
     int stack[TM_STACK];
     int sp = 0, nc = 0;
     stack[sp++] = 0;
@@ -724,28 +751,26 @@ __device__ int mesh_contacts(int t, float4 g, f3 pos0, qf or0, f3 pos1, qf or1, 
             stack[sp++] = index;
         }
     }
-    int nct = 0;
-    for (int k = 0; k < nc; ++k) {
-        int ti = ws.cand(k);
-        int4 ix = mc.tri(ti);
-        f3 a = mc.vert(ix.x), b = mc.vert(ix.y), c = mc.vert(ix.z);
-        f3 n = mc.n(ti);
-        TCI ci;
-        ci.featureIndex = 0; ci.boxFeature = 0; ci.boxAxis = 0;
-        bool hit;
-        if (t == PGX_SPHERE) hit = sphere_tri(localPos, g.x, a, b, c, n, ci);
-        else if (t == PGX_CAPSULE) hit = capsule_tri(localPos, localOr, g.x, g.y, a, b, c, n, ci);
-        else if (t == PGX_BOX) hit = box_tri(localPos, localOr, mk3(g.x, g.y, g.z), a, b, c, n, ci);
-        else {
-            hit = convex_tri(sc, a, b, c, mc.c(ti), ews, ci);
-            if (ews.overflow) epaErr = true;
-        }
-        if (hit) {
-            ci.tri = ti;
-            ws.store(nct++, ci);
-        }
+    return nc;
+}
+
+// the per-shape test of one candidate triangle ti (src/CollisionTriangleMesh.cpp:886-903)
+__device__ bool mesh_test(int t, float4 g, f3 localPos, qf localOr, const SupConvex& sc, const MeshCtx& mc, int ti,
+                          EpaWs& ews, TCI& ci, bool& epaErr) {
+    int4 ix = mc.tri(ti);
+    f3 a = mc.vert(ix.x), b = mc.vert(ix.y), c = mc.vert(ix.z);
+    f3 n = mc.n(ti);
+    ci.featureIndex = 0; ci.boxFeature = 0; ci.boxAxis = 0;
+    bool hit;
+    if (t == PGX_SPHERE) hit = sphere_tri(localPos, g.x, a, b, c, n, ci);
+    else if (t == PGX_CAPSULE) hit = capsule_tri(localPos, localOr, g.x, g.y, a, b, c, n, ci);
+    else if (t == PGX_BOX) hit = box_tri(localPos, localOr, mk3(g.x, g.y, g.z), a, b, c, n, ci);
+    else {
+        hit = convex_tri(sc, a, b, c, mc.c(ti), ews, ci);
+        if (ews.overflow) epaErr = true;
     }
-    return nct;
+    ci.tri = ti;
+    return hit;
 }
 
 __device__ __forceinline__ bool is_voided(MeshWs& ws, int nv, int v) {
@@ -754,20 +779,20 @@ __device__ __forceinline__ bool is_voided(MeshWs& ws, int nv, int v) {
     return false;
 }
 
-// the emission passes (:906-953) over nct stored contact records
+// the emission passes (:906-953) over the nct contact records ws.order(0 .. nct-1) lists
+// in candidate order
 __device__ void mesh_emit(int t, float4 g, f3 localPos, qf localOr, f3 pos1, qf or1, const MeshCtx& mc,
-                          const ConvexView& cv, MeshWs& ws, int nct, MeshOut& out) {
+                          const ConvexView& cv, MeshWs& ws, const MeshRecs& recs, int nct, MeshOut& out) {
     f3 he = mk3(g.x, g.y, g.z);
     MeshRef cm{&cv, make_int4(0, 0, 0, 0)};
     if (t == PGX_CONVEX_MESH) cm.info = cv.mesh[(int)g.w];
-    for (int i = 0; i < nct; ++i) ws.order(i) = i;
     int nv = 0;
     int n = nct;
     // first pass: face contacts, last to first, swap-removed
     for (int i = n - 1; i >= 0; i--) {
         int c = ws.order(i);
-        if (ws.rec(c, 11) != TF_FACE) continue;
-        TCI ci = ws.load(c);
+        if (recs.feature(c) != TF_FACE) continue;
+        TCI ci = recs.load(c);
         int4 ix = mc.tri(ci.tri);
         ws.voided(nv++) = ix.x; ws.voided(nv++) = ix.y; ws.voided(nv++) = ix.z;
         Mf m;
@@ -781,11 +806,11 @@ __device__ void mesh_emit(int t, float4 g, f3 localPos, qf localOr, f3 pos1, qf 
         ws.order(i) = last;
         --n;
     }
-    OrderArr arr{&ws};
+    OrderArr arr{&ws, &recs};
     introsort(arr, n);
     for (int k = 0; k < n; ++k) {
         int c = ws.order(k);
-        TCI ci = ws.load(c);
+        TCI ci = recs.load(c);
         int4 ix = mc.tri(ci.tri);
         int iv[3] = {ix.x, ix.y, ix.z};
         Mf m;

@@ -64,8 +64,8 @@ constexpr int CVX_MAX_FACE_VERTS = 16;
 // triangle-mesh narrowphase: per-lane capacities and lanes
 constexpr int TM_MAXCAND = 256;        // triangles overlapping one body's local bounds
 constexpr int TM_STACK = 64;           // BVH traversal stack (the build's depth is checked)
-constexpr int TM_CI_WORDS = 14;        // one triangle contact record
-constexpr int TM_WS_WORDS = TM_MAXCAND * (2 + TM_CI_WORDS + 3);   // cand, order, contacts, voided
+constexpr int TM_CI_WORDS = 14;        // one triangle contact record (+ 1 word: the test hit)
+constexpr int TM_WS_WORDS = TM_MAXCAND * (2 + 3);   // per lane: candidates, order, covered vertices
 constexpr int TM_LANES = 16384;
 inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -215,7 +215,11 @@ struct World {
     DBuf<int4> tmTris;
     DBuf<int32_t> meshPairs;           // pairs with a triangle mesh (filled by the primitive kernel)
     int meshLanes = 0;                 // lanes of the mesh kernel
-    DBuf<int32_t> tmWs;                // per-lane candidates, contacts, order, voided vertices
+    DBuf<int32_t> tmWs;                // per-lane candidates, order, voided vertices
+    int tmCandCap = 0;                 // candidate triangles per step (all mesh pairs)
+    DBuf<int32_t> tmCand, tmPairOf;    // per candidate: triangle, mesh pair
+    DBuf<int32_t> tmRecs;              // per candidate: contact record (TM_CI_WORDS + hit)
+    DBuf<int32_t> tmCandOff, tmCandCount;   // per mesh pair: its candidates
     int mmCap = 0;                     // manifolds of mesh pairs, stored after the pair slots
     int lastMeshSlots = 0;
     DBuf<int32_t> npTri, meshFirst, meshCount;

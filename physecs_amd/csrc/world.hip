@@ -316,6 +316,11 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
         w.meshPairs.alloc(w.pairCap);
         w.meshLanes = TM_LANES;
         w.tmWs.alloc((size_t)TM_WS_WORDS * w.meshLanes);
+        // candidate triangles of all mesh pairs of a step, one contact record each
+        w.tmCandCap = std::max(1 << 18, 4 * w.mmCap);
+        w.tmCand.alloc(w.tmCandCap); w.tmPairOf.alloc(w.tmCandCap);
+        w.tmRecs.alloc((size_t)(TM_CI_WORDS + 1) * w.tmCandCap);
+        w.tmCandOff.alloc(w.pairCap); w.tmCandCount.alloc(w.pairCap);
         if (w.epaLanes < w.meshLanes) {   // convex bodies against triangles need EPA state per lane
             w.epaLanes = w.meshLanes;
             size_t L = (size_t)w.epaLanes;
