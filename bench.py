@@ -194,7 +194,7 @@ def main():
     ap.add_argument("--config", default="plain32k")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks", type=int, default=0, help="cooperative grid size (0 = one block per CU)")
-    ap.add_argument("--exec", default="colors", choices=["flow", "colors", "cooperative"],
+    ap.add_argument("--exec", default="colors", choices=["flow", "xcd", "colors", "cooperative"],
                     help="how the contact passes launch")
     ap.add_argument("--seconds", type=float, default=20.0, help="served: length of the request stream")
     ap.add_argument("--admit-interval", type=int, default=8, help="served: steps between admission boundaries")
@@ -244,7 +244,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
         "config": dict(cfg, parallelism="single GPU", last_step_counts=counts,
                        phase_us_last_step=w.timing()),
-        "roofline": {"bound": "hbm", "kernel": {"flow": "k_solve_flow", "colors": "k_solve",
+        "roofline": {"bound": "hbm", "kernel": {"flow": "k_solve_flow", "xcd": "k_solve_flow", "colors": "k_solve",
                                                     "cooperative": "k_step_persistent"}[args.exec], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.exec) if args.config == "plain32k" else None,

@@ -236,12 +236,14 @@ int pgx_world_reset_stats(pgx_world* w);
  *   PGX_EXEC_COLORS (default) one launch per colour and pass
  *   PGX_EXEC_FLOW            one launch per pass; each manifold solves as soon as its
  *                            bodies have taken every lower colour's solves
+ *   PGX_EXEC_FLOW_XCD        the same with each world's manifolds on one XCD
  *   PGX_EXEC_COOPERATIVE     every substep of a step in one cooperative launch, grid
  *                            barriers between colours (worlds without joints; others use
  *                            PGX_EXEC_FLOW). blocks = its grid size (0 = one per CU). */
 #define PGX_EXEC_COLORS 0
 #define PGX_EXEC_COOPERATIVE 1
 #define PGX_EXEC_FLOW 2
+#define PGX_EXEC_FLOW_XCD 3
 int pgx_world_set_exec(pgx_world* w, int mode, int blocks);
 int pgx_world_get_stats(pgx_world* w, pgx_stats* out);
 

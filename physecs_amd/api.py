@@ -249,12 +249,12 @@ class World:
     def enable_timing(self, on=True):
         check(self.L.pgx_world_enable_timing(self.h, 1 if on else 0))
 
-    EXEC_MODES = {"colors": 0, "cooperative": 1, "flow": 2}
+    EXEC_MODES = {"colors": 0, "cooperative": 1, "flow": 2, "xcd": 3}
 
     def set_exec(self, mode="colors", blocks=0):
         """How the contact passes launch: "colors" (one launch per colour, the default),
-        "flow" (one dependency-driven launch per pass) or "cooperative" (one launch per
-        step)."""
+        "flow" (one dependency-driven launch per pass), "xcd" (the same with each world's
+        manifolds on one XCD) or "cooperative" (one launch per step)."""
         check(self.L.pgx_world_set_exec(self.h, self.EXEC_MODES[mode], blocks))
 
     def reset_stats(self):

@@ -54,7 +54,9 @@ constexpr int WAVE = 64;
 constexpr int JOINT_MAX_ROWS = 7;      // a prismatic joint with its limit and drive rows
 // substep-loop execution: one launch per colour and pass; one cooperative launch per step
 // (grid barriers between colours); one dependency-driven launch per pass
-enum { EXEC_COLORS = 0, EXEC_COOPERATIVE = 1, EXEC_FLOW = 2 };
+// (one device-wide queue), the same with a queue per XCD
+enum { EXEC_COLORS = 0, EXEC_COOPERATIVE = 1, EXEC_FLOW = 2, EXEC_FLOW_XCD = 3 };
+constexpr int FLOW_QUEUES = 8;         // queues of a dependency-driven pass: at most one per XCD
 // convex narrowphase: EPA polytope capacity per lane and lanes of the convex kernel
 constexpr int EPA_MAXV = 104;          // 4 simplex vertices + at most 100 insertions
 constexpr int EPA_MAXF = 256;          // > 2 * EPA_MAXV - 4, the face count of a closed triangulation
@@ -244,10 +246,14 @@ struct World {
     int execMode = EXEC_COLORS;          // how the substep loop's contact passes are launched
     int persistentBlocks = 0, persistentBlocksWanted = 0;
     // dependency-driven passes
+    int nXcd = 1;                        // XCDs of the device (CUs / 32)
     uint32_t flowEpoch = 0;              // per launch: versions of the published velocities
+    int lastQueues = 0;                  // queues of the last step's passes (0: no pass ran)
+    int lastFlowPasses = 0;              // dependency-driven launches of the last step
     DBuf<uint64_t> flowPub;              // 6 x D published velocity words
-    DBuf<int32_t> tickets;               // work tickets, one per pass of a substep
+    DBuf<int32_t> tickets;               // work tickets, FLOW_QUEUES per pass of a substep
     DBuf<int32_t> flowErr;               // set when a dependency wait runs out
+    DBuf<int32_t> qStart, qList, qDone;  // queues: offsets, solve positions (M), manifolds finished
     hipEvent_t ev[8] = {};
     float tBroad = 0, tNarrow = 0, tColor = 0, tSolve = 0, tTotal = 0;
     // per-launch timing of the contact solve (first substep of each timed step)

@@ -305,6 +305,7 @@ struct SubArgs {
     const uint64_t* used;
     int32_t* tickets;
     int32_t* flowErr;
+    int32_t* qDone;                       // manifolds finished per queue in this step
 };
 
 __device__ __forceinline__ m3 world_inv_inertia(const SubArgs& a, int b) {
@@ -554,7 +555,8 @@ __device__ void write_back(const SubArgs& a, int d) {
 // ------------------------------------------------------------------ discrete kernels
 __global__ __launch_bounds__(256) void k_prepare(SubArgs a, int sub) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i <= a.iterations) a.tickets[i] = 0;
+    if (i < (a.iterations + 1) * FLOW_QUEUES) a.tickets[i] = 0;
+    if (sub == 0 && i < FLOW_QUEUES) a.qDone[i] = 0;
     if (i < a.M) prepare_manifold(a, i);
     else if (i - a.M < a.D) integrate_velocity(a, i - a.M);
 }
@@ -584,9 +586,12 @@ __device__ __forceinline__ int colours_below(const uint64_t* used, int b, int c)
 //
 // Published velocities: 6 words of {value, version} per body, each word readable on its
 // own, so a reader accepts a record only when all six carry the version it waits for and
-// needs no ordering between the words' stores. Records travel through the device-wide
-// coherence point (agent-scope atomic accesses): the waves of one launch run on every XCD,
-// and on this device only device-scope accesses see another CU's stores within a launch.
+// needs no ordering between the words' stores. Records are agent-scope atomic accesses (a
+// load that hits the CU's own cache may never see another CU's store within a launch).
+// Queues: one device-wide queue, or one per XCD (world w on XCD w % XCDs; waves take
+// manifolds only from their own XCD's queue, read from the hardware register XCC_ID), so
+// that a world's hand-offs stay inside one XCD: a few hundred nanoseconds each, against
+// microseconds between XCDs.
 //
 // The solve order is the colour order's, so the arithmetic is exactly that of one launch
 // per colour. Waves take 64 manifolds at a time, in solve order, from the pass's ticket: a
@@ -601,6 +606,10 @@ struct FlowArgs {
     float h;
     uint32_t epoch;                        // unique per launch (24 bits used)
     uint64_t* pub;                         // 6 words per body
+    int xcdScope;                          // 1: a queue per XCD, 0: one queue
+    int nQueues;
+    const int32_t* qStart;                 // nQueues + 1 offsets into qList
+    const int32_t* qList;                  // solve positions by queue (nullptr: identity)
 };
 
 __device__ __forceinline__ uint64_t pub_word(float v, uint32_t ver) {
@@ -642,15 +651,23 @@ __device__ __forceinline__ void hand_on(const SubArgs& a, const FlowArgs& f, int
 
 __global__ __launch_bounds__(256) void k_solve_flow(SubArgs a, FlowArgs f) {
     const int lane = threadIdx.x & (WAVE - 1);
+    int queue = 0;
+    if (f.xcdScope) {
+        queue = (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15);   // hwreg(HW_REG_XCC_ID, 0, 4)
+        if (queue >= f.nQueues) return;
+    }
+    const int qb = f.qStart[queue], qn = f.qStart[queue + 1] - qb;
     const uint32_t vbase = (f.epoch & 0xffffffu) << 8;
     const bool useBias = f.useBias != 0, skipSoft = f.skipSoft != 0;
+    int done = 0;
     for (;;) {
         int base = 0;
-        if (lane == 0) base = atomicAdd(a.tickets + f.pass, WAVE);
+        if (lane == 0) base = atomicAdd(a.tickets + f.pass * FLOW_QUEUES + queue, WAVE);
         base = __shfl(base, 0);
-        if (base >= a.M) break;
-        const int s = base + lane;
-        bool pending = s < a.M;
+        if (base >= qn) break;
+        const int idx = base + lane;
+        bool pending = idx < qn;
+        const int s = pending ? (f.qList ? f.qList[qb + idx] : qb + idx) : 0;
         MInputs in;
         int r0 = 0, r1 = 0;
         bool last0 = false, last1 = false;
@@ -685,6 +702,7 @@ __global__ __launch_bounds__(256) void k_solve_flow(SubArgs a, FlowArgs f) {
                     if (in.b0 >= 0) hand_on(a, f, in.b0, last0, vbase | (uint32_t)(r0 + 1), v0, w0);
                     if (in.b1 >= 0) hand_on(a, f, in.b1, last1, vbase | (uint32_t)(r1 + 1), v1, w1);
                     pending = false;
+                    ++done;
                 } else if (++spins > FLOW_SPIN_LIMIT) {
                     atomicOr(a.flowErr, 1);
                     pending = false;
@@ -693,6 +711,35 @@ __global__ __launch_bounds__(256) void k_solve_flow(SubArgs a, FlowArgs f) {
             if (__ballot(pending)) __builtin_amdgcn_s_sleep(1);
         }
     }
+    // manifolds finished per queue: the host checks that every queue was served
+    for (int o = WAVE / 2; o > 0; o >>= 1) done += __shfl_xor(done, o);
+    if (lane == 0 && done) atomicAdd(a.qDone + queue, done);
+}
+
+// The queues of the dependency-driven pass. One world, or one device-wide queue: queue 0
+// is the whole solve order. Several worlds on XCD queues: world w goes to queue w % nq,
+// keeping solve order inside each queue (a stable sort by queue).
+__global__ void k_flow_identity(int M, int nq, int32_t* qStart) {
+    int i = threadIdx.x;
+    if (i <= nq) qStart[i] = i == 0 ? 0 : M;
+}
+__global__ void k_flow_keys(int M, const int32_t* __restrict__ sCanon, const uint64_t* __restrict__ mfKey,
+                            int worldShift, int nq, uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    keys[s] = (mfKey[sCanon[s]] >> worldShift) % (uint64_t)nq;
+    vals[s] = s;
+}
+__global__ void k_flow_starts(int M, int nq, const uint64_t* __restrict__ keys, int32_t* __restrict__ qStart) {
+    int q = threadIdx.x;
+    if (q > nq) return;
+    int lo = 0, hi = M;   // first position with key >= q
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (keys[mid] < (uint64_t)q) lo = mid + 1;
+        else hi = mid;
+    }
+    qStart[q] = lo;
 }
 
 __global__ __launch_bounds__(256) void k_integrate_positions(SubArgs a) {
@@ -749,8 +796,15 @@ int sv_setup_manifolds(World& w, int nPairs) {
     int M = 0, epaErrors = 0, flowErr = 0;
     int meshInfo[3] = {0, 0, 0};   // mesh pairs, mesh manifold slots, mesh capacity errors
     PGX_HIP(hipMemcpyAsync(&M, w.counters.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    // the previous step's dependency-driven passes: a wait that ran out
+    // the previous step's dependency-driven passes: a wait that ran out, and every queue
+    // served by waves of its XCD
+    int qDone[FLOW_QUEUES] = {}, qStart[FLOW_QUEUES + 1] = {};
+    const int lastQueues = w.lastQueues, lastPasses = w.lastFlowPasses;
     PGX_HIP(hipMemcpyAsync(&flowErr, w.flowErr.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (lastQueues) {
+        PGX_HIP(hipMemcpyAsync(qDone, w.qDone.p, sizeof(qDone), hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(qStart, w.qStart.p, sizeof(int) * (lastQueues + 1), hipMemcpyDeviceToHost, s));
+    }
     if (w.nConvex) PGX_HIP(hipMemcpyAsync(&epaErrors, w.counters.p + 9, sizeof(int), hipMemcpyDeviceToHost, s));
     if (w.nTriMesh) PGX_HIP(hipMemcpyAsync(meshInfo, w.counters.p + 10, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     hipLaunchKernelGGL(k_compact, dim3(div_up(nPairs, 256)), dim3(256), 0, s, nPairs, w.npNum.p, w.npIndex.p,
@@ -760,6 +814,12 @@ int sv_setup_manifolds(World& w, int nPairs) {
     PGX_HIP(hipStreamSynchronize(s));
     if (flowErr)
         throw PgxError{PGX_ERR_HIP, "contact solve: a dependency wait of the previous step ran out"};
+    for (int q = 0; q < lastQueues; ++q)
+        if (qDone[q] != lastPasses * (qStart[q + 1] - qStart[q]))
+            throw PgxError{PGX_ERR_HIP, "contact solve: queue " + std::to_string(q) + " of the previous step finished " +
+                                            std::to_string(qDone[q]) + " of " +
+                                            std::to_string(lastPasses * (qStart[q + 1] - qStart[q])) +
+                                            " manifold solves (no wave ran on its XCD)"};
     if (epaErrors)
         throw PgxError{PGX_ERR_CAPACITY, "EPA polytope capacity exceeded in " + std::to_string(epaErrors) + " pairs"};
     if (meshInfo[2])
@@ -860,7 +920,7 @@ static SubArgs make_args(World& w, int M, float dt) {
     a.bpos = w.bpos.p; a.bquat = w.bquat.p; a.dvel = w.dvel.p; a.dang = w.dang.p;
     a.cvel = w.cvel.p; a.cang = w.cang.p; a.dcom = w.dcom.p; a.dinvI = w.dinvI.p;
     a.tvel = w.tvel.p; a.tang = w.tang.p; a.pvel = w.pvel.p; a.pang = w.pang.p;
-    a.used = w.usedMask.p; a.tickets = w.tickets.p; a.flowErr = w.flowErr.p;
+    a.used = w.usedMask.p; a.tickets = w.tickets.p; a.flowErr = w.flowErr.p; a.qDone = w.qDone.p;
     return a;
 }
 
@@ -896,7 +956,7 @@ static void solve_events(World& w, int color, hipEvent_t* start, hipEvent_t* sto
 void sv_substeps(World& w, int M, float dt) {
     hipStream_t s = w.stream;
     w.solveEvUsed = 0;
-    w.tickets.alloc(w.iterations + 1);   // k_prepare clears them
+    w.tickets.alloc((size_t)(w.iterations + 1) * FLOW_QUEUES);   // k_prepare clears them
     SubArgs a = make_args(w, M, dt);
     if (w.execMode == EXEC_COOPERATIVE && w.J == 0) {
         int blocks = persistent_blocks(w);
@@ -908,12 +968,31 @@ void sv_substeps(World& w, int M, float dt) {
     }
     int nbP = div_up(std::max(M + w.D, 1), 256), nbD = div_up(std::max(w.D, 1), 256);
     int nColors = a.nColors;
-    const bool flow = w.execMode == EXEC_FLOW;
+    const bool flow = w.execMode == EXEC_FLOW || w.execMode == EXEC_FLOW_XCD;
     FlowArgs f{};
     f.pub = w.flowPub.p;
+    if (flow && M > 0) {
+        f.xcdScope = w.execMode == EXEC_FLOW_XCD;
+        f.nQueues = f.xcdScope ? std::min(w.W, w.nXcd) : 1;
+        f.qStart = w.qStart.p;
+        f.qList = nullptr;
+        if (f.nQueues > 1) {
+            // several worlds: a stable partition of the solve order by world % queues
+            hipLaunchKernelGGL(k_flow_keys, dim3(div_up(M, 256)), dim3(256), 0, s, M, w.sCanon.p, w.mfKey.p,
+                               2 * w.localBits, f.nQueues, w.colorKeys.p, w.qList.p);
+            radix_sort_u64(s, w.colorKeys.p, w.qList.p, M, 4, w.sortT);
+            hipLaunchKernelGGL(k_flow_starts, dim3(1), dim3(64), 0, s, M, f.nQueues, w.colorKeys.p, w.qStart.p);
+            f.qList = w.qList.p;
+        } else {
+            hipLaunchKernelGGL(k_flow_identity, dim3(1), dim3(64), 0, s, M, f.nQueues, w.qStart.p);
+        }
+        PGX_LAUNCH_CHECK();
+    }
+    w.lastQueues = flow && M > 0 ? f.nQueues : 0;
+    w.lastFlowPasses = flow && M > 0 ? w.substeps * (w.iterations + 1) : 0;
     // waves of the dependency-driven pass: enough to hold every manifold at once, at most
-    // what the chip keeps resident comfortably
-    const int flowBlocks = std::max(1, std::min(div_up(std::max(M, 1), 256), 2048));
+    // what the chip keeps resident comfortably; on XCD queues, 64 workgroups per XCD
+    const int flowBlocks = f.xcdScope ? 64 * w.nXcd : std::max(1, std::min(div_up(std::max(M, 1), 256), 2048));
     int passIndex = 0;
     auto pass = [&](bool rec, int useBias, float h, int skipSoft) {
         const int p = passIndex++;

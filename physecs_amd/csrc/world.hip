@@ -144,6 +144,7 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     w.cacheCur = 0; w.cacheCount[0] = w.cacheCount[1] = 0;
     w.nConvex = 0; w.nTriMesh = 0; w.mmCap = 0; w.lastMeshSlots = 0;
     w.lastPairs = w.lastManifolds = w.lastColors = w.lastRounds = w.lastPoints = 0;
+    w.lastQueues = 0; w.lastFlowPasses = 0;
     int W = sc->actor_world ? std::max(1, sc->n_worlds) : 1;
     if (cfg) {
         if (cfg->num_substeps > 0) w.substeps = cfg->num_substeps;
@@ -265,6 +266,13 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     PGX_HIP(hipMemsetAsync(w.flowPub.p, 0, sizeof(uint64_t) * 6 * std::max(D, 1), w.stream));
     w.flowErr.alloc(1);
     PGX_HIP(hipMemsetAsync(w.flowErr.p, 0, sizeof(int32_t), w.stream));
+    w.qStart.alloc(FLOW_QUEUES + 1); w.qList.alloc(M); w.qDone.alloc(FLOW_QUEUES);
+    PGX_HIP(hipMemsetAsync(w.qDone.p, 0, sizeof(int32_t) * FLOW_QUEUES, w.stream));
+    {
+        int cus = 0;
+        PGX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, w.device));
+        w.nXcd = std::max(1, std::min(FLOW_QUEUES, cus / 32));   // 32 CUs per XCD on CDNA3 and CDNA4
+    }
 
     w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorCount.alloc(160); w.colorPoints.alloc(160);
     w.sB0.alloc(M); w.sB1.alloc(M); w.sFlags.alloc(M); w.sA0.alloc(M); w.sA1.alloc(M); w.sCanon.alloc(M);
@@ -911,7 +919,7 @@ int pgx_world_get_stats(pgx_world* h, pgx_stats* out) {
 int pgx_world_set_exec(pgx_world* h, int mode, int blocks) {
     if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
     if (blocks < 0) return fail(PGX_ERR_INVALID, "blocks must be >= 0");
-    if (mode < PGX_EXEC_COLORS || mode > PGX_EXEC_FLOW) return fail(PGX_ERR_INVALID, "unknown execution mode %d", mode);
+    if (mode < PGX_EXEC_COLORS || mode > PGX_EXEC_FLOW_XCD) return fail(PGX_ERR_INVALID, "unknown execution mode %d", mode);
     h->w.execMode = mode;
     h->w.persistentBlocksWanted = blocks;
     h->w.persistentBlocks = 0;

@@ -25,11 +25,15 @@ def world_rows(scene, w):
     return actors, dyn
 
 
-def test_world_same_alone_and_in_mix(api):
+@pytest.mark.parametrize("mode", ["colors", "xcd"])
+def test_world_same_alone_and_in_mix(api, mode):
+    """A world steps bit-identically alone and among 23 others, with the contact passes
+    launched per colour or on per-XCD queues (worlds spread over the XCDs)."""
     sc = scenes.batched_worlds(3, n_worlds=24, lo=8, hi=160, ragdoll_frac=0.5)
-    log(f"[batched] 24 worlds, {sc.n_dyn} bodies, {len(sc.joints)} joints: mixed run")
+    log(f"[batched] 24 worlds, {sc.n_dyn} bodies, {len(sc.joints)} joints: mixed run, exec {mode}")
     steps = 40
     g = api.World(sc, substeps=4, iterations=2)
+    g.set_exec(mode)
     for _ in range(steps):
         g.step(1 / 60)
     pos, quat, vel, ang = g.state()

@@ -78,7 +78,7 @@ def test_free_fall_closed_form(api):
     assert abs(gs[2][0, 1] - (-9.81 * 30 / 60)) < 1e-3
 
 
-@pytest.mark.parametrize("mode", ["flow", "colors", "cooperative"])
+@pytest.mark.parametrize("mode", ["xcd", "flow", "colors", "cooperative"])
 def test_pyramid_lockstep_bitexact(api, mode):
     """20-layer box pyramid, dt 1/60, 4 substeps, 2 iterations: 120 steps in lockstep, in
     every execution mode of the contact passes (dependency-driven, one launch per colour,
