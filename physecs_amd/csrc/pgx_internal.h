@@ -44,6 +44,7 @@ struct DBuf {
         n = 0;
     }
     operator T*() const { return p; }
+    void swap(DBuf& o) { std::swap(p, o.p); std::swap(n, o.n); }
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
@@ -275,6 +276,19 @@ int bp_find_pairs(World& w);          // returns pair count (host)
 // narrowphase.hip
 void np_run(World& w, int nPairs);
 int upload_triangle_meshes(World& w, const pgx_scene_desc* sc, hipStream_t s);   // returns the BVH depth
+// device-layout arrays of a scene descriptor (world.hip)
+struct SceneArrays {
+    int A = 0, D = 0, C = 0, W = 1, maxPerWorld = 1;
+    std::vector<int> actorToBody, bodyToActor;
+    std::vector<float4> bpos, bquat, dvel, dang, dcom, dinvI, cparams, clpos, clquat, cmat;
+    std::vector<uint32_t> bent, cent;
+    std::vector<int32_t> cbody, cworld, clocal, cflags, ctype, cindex, wstart, colOfLocal, dynCols, staticCols, wss;
+    std::vector<uint8_t> kinematic;
+};
+void convert_scene(const pgx_scene_desc* sc, uint32_t entityBase, SceneArrays& o);
+void world_reset_host(World& w);
+void world_set_shape(World& w, const pgx_config* cfg, int A, int D, int C, int W, int nDynCol, int maxPerWorld);
+void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh);
 void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc);      // world.hip
 void world_add_joints(World& w, int n, const pgx_joint_desc* js);
 void world_step(World& w, float dt);

@@ -3,12 +3,15 @@
 // Every request is one world of a batched device world (worlds never interact; see the
 // world id in the broadphase keys). At each step boundary the server retires requests
 // that reached their horizon (their final state is copied out) and admits queued ones
-// while the footprint budget allows; when the set changed, the batched world is
-// re-initialised in place (device buffers are reused) from the concatenated descriptors,
-// and the persistent state of continuing requests -- body poses and velocities, the
-// restitution contact cache, the bounds margin of their dynamic colliders -- is carried
-// over on the device. Worlds keep their admission order, so a continuing request's
-// contact cache stays sorted after its world id is renumbered.
+// while the footprint budget allows. When the set changed, the next batched world's
+// arrays are assembled on the device: each continuing request's segment of every array is
+// gathered from the current world (body and collider indices renumbered), each admitted
+// request's segments come from one packed upload of its converted descriptor, and the
+// buffers swap. The persistent state of continuing requests -- body poses and velocities,
+// the restitution contact cache, the bounds margin of their dynamic colliders, gear joint
+// angles -- stays on the device. Worlds keep their admission order, so a continuing
+// request's contact cache stays sorted after its world id is renumbered. The assembled
+// world equals the one a descriptor concatenating the active requests would initialise.
 //
 // Entity ids of a request are shifted by a per-request base so that entity-keyed state
 // (non-colliding pairs) never aliases across requests; the shift keeps their order, the
@@ -47,8 +50,17 @@ struct Request {
     int horizon = 0, stepsDone = 0;
     double footprint = 0;
     int state = 0;                      // 0 queued, 1 active, 2 done (may still ride in the batch)
-    int dynStart = 0;                   // placement in the current batched world
-    int jointStart = 0;                 // first joint in the current batched world
+    bool placed = false;                // has a placement in the current batched world
+    // placement in the current batched world: world id, first dynamic body, first other
+    // body (counted after all dynamic bodies), first collider, first entry of the dynamic
+    // and of the other collider lists, first joint
+    int world = 0, dOff = 0, sOff = 0, cOff = 0, dcOff = 0, scOff = 0, jointStart = 0;
+    // device-layout facts kept from the conversion at admission: collider list lengths,
+    // entity id and kinematic flag per internal body, joints with internal body indices
+    int nDC = 0, nSC = 0;
+    std::vector<uint32_t> bodyEntity;
+    std::vector<uint8_t> bodyKinematic;
+    std::vector<pgx_joint_desc> bodyJoints;
     std::vector<float4> rpos, rquat, rvel, rang;   // final dynamic state (done)
 };
 
@@ -59,24 +71,47 @@ double footprint_of(int nA, int nD, int nC) {
     return 12.0 * nC * perPairSlot + 400.0 * nA + 320.0 * nD + 200.0 * nC;
 }
 
-__global__ void k_copy_segments(int nseg, const int4* __restrict__ seg, int total, const float4* __restrict__ s0,
-                                const float4* __restrict__ s1, const float4* __restrict__ s2,
-                                const float4* __restrict__ s3, float4* __restrict__ d0, float4* __restrict__ d1,
-                                float4* __restrict__ d2, float4* __restrict__ d3) {
-    // seg[i] = {source start, destination start, length, first flat index}
+// One segment of an assembled array: `count` elements from the current world (src 0) or
+// the staging upload (src 1) at srcStart to dstStart; flat = elements of the segments
+// before it. Body indices are renumbered from the source layout (a = dynamic bodies in
+// it, b / c = the segment owner's first dynamic / other body there) to the new one (d / e);
+// collider indices move by `delta`; `world` is the owner's new world id.
+struct Seg {
+    int src, srcStart, dstStart, count, flat;
+    int a, b, c, d, e, delta, world;
+};
+enum { FIX_NONE = 0, FIX_BODY = 1, FIX_COLLIDER = 2, FIX_WORLD = 3 };
+
+template <class T, int FIX>
+__global__ void k_gather(int nseg, const Seg* __restrict__ seg, int total, const T* __restrict__ cur,
+                         const T* __restrict__ stage, T* __restrict__ dst, int dstDyn) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;
-    int lo = 0, hi = nseg - 1;
+    int lo = 0, hi = nseg - 1;   // the last segment starting at or before i (none is empty)
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
-        if (seg[mid].w <= i) lo = mid; else hi = mid - 1;
+        if (seg[mid].flat <= i) lo = mid; else hi = mid - 1;
     }
-    int4 g = seg[lo];
-    int k = i - g.w;
-    d0[g.y + k] = s0[g.x + k];
-    d1[g.y + k] = s1[g.x + k];
-    d2[g.y + k] = s2[g.x + k];
-    d3[g.y + k] = s3[g.x + k];
+    const Seg g = seg[lo];
+    const int k = i - g.flat;
+    T v;
+    if constexpr (FIX == FIX_WORLD) {
+        v = (T)g.world;
+    } else {
+        v = (g.src ? stage : cur)[g.srcStart + k];
+        if constexpr (FIX == FIX_BODY) {
+            int x = (int)v;
+            v = (T)(x < g.a ? x - g.b + g.d : x - g.a - g.c + dstDyn + g.e);
+        } else if constexpr (FIX == FIX_COLLIDER) {
+            v = (T)((int)v + g.delta);
+        }
+    }
+    dst[g.dstStart + k] = v;
+}
+
+__global__ void k_iota(int n, int32_t* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = i;
 }
 
 __global__ void k_cache_flag(int n, const uint64_t* __restrict__ key, int oldLocalBits,
@@ -87,7 +122,8 @@ __global__ void k_cache_flag(int n, const uint64_t* __restrict__ key, int oldLoc
 }
 
 __global__ void k_cache_remap(int n, const uint64_t* __restrict__ key, const int32_t* __restrict__ tri,
-                              const int32_t* __restrict__ num, const float4* __restrict__ lp, int oldLocalBits,
+                              const int32_t* __restrict__ num, const float4* __restrict__ lp, size_t lpStride,
+                              int oldLocalBits,
                               int newLocalBits, const int32_t* __restrict__ worldMap,
                               const int32_t* __restrict__ flag, const int32_t* __restrict__ pos,
                               uint64_t* __restrict__ nKey, int32_t* __restrict__ nTri, int32_t* __restrict__ nNum,
@@ -102,7 +138,7 @@ __global__ void k_cache_remap(int n, const uint64_t* __restrict__ key, const int
     nKey[q] = ((uint64_t)nw << (2 * newLocalBits)) | (l0 << newLocalBits) | l1;
     nTri[q] = tri[e];
     nNum[q] = num[e];
-    for (int j = 0; j < 4; ++j) nLp[(size_t)j * newCap + q] = lp[(size_t)j * n + e];
+    for (int j = 0; j < 4; ++j) nLp[(size_t)j * newCap + q] = lp[(size_t)j * lpStride + e];
 }
 
 }  // namespace
@@ -121,86 +157,42 @@ struct pgx_server {
     std::unordered_map<int, Request> reqs;
     double footprint = 0;
     bool initialised = false;
-    // staging of the carried-over state
-    DBuf<float4> tPos, tQuat, tVel, tAng, tLp;
-    DBuf<uint64_t> tKey;
-    DBuf<int32_t> tTri, tNum, worldMap, flag, fpos;
+    int worlds = 0;                     // worlds of the current batched world
+    // assembly: spare arrays the next world's arrays are gathered into (then swapped with the
+    // world's), one packed upload (segment tables, admitted requests' arrays, world starts)
+    // through a pinned host buffer, the previous contact cache and joint state
+    DBuf<float4> xBpos, xBquat, xDvel, xDang, xDcom, xDinvI, xCparams, xClpos, xClquat, xCmat;
+    DBuf<uint32_t> xBent, xCent;
+    DBuf<int32_t> xCbody, xCworld, xClocal, xCflags, xCtype, xCindex, xDynCols, xStaticCols;
+    DBuf<uint8_t> stage;
+    uint8_t* hstage = nullptr;
+    size_t hstageCap = 0;
+    DBuf<float4> oLp, oGear;
+    DBuf<uint64_t> oKey;
+    DBuf<int32_t> oTri, oNum, oGearInit, worldMap, flag, fpos;
     DBuf<uint8_t> worldMask;
-    DBuf<int4> seg;
+    ~pgx_server() { if (hstage) (void)hipHostFree(hstage); }
 };
 
 static int srv_fail(int code, const std::string& msg) { return report_error(code, msg); }
 
 namespace {
 
-// concatenation of the active requests' descriptors (entity ids shifted, world ids in
-// admission order)
-struct Combined {
-    std::vector<uint32_t> entity;
-    std::vector<float> pos, quat, vel, angvel, invMass, com, invI, colPos, colQuat, colParams, colMat;
-    std::vector<int32_t> dynOfActor, actorOfDyn, colActor, colIndex, colType, colData, actorWorld;
-    std::vector<uint8_t> kinematic, colTrigger, colEnable;
-    std::vector<pgx_joint_desc> joints;
-    int32_t zero[2] = {0, 0};
-    float fzero[3] = {0, 0, 0};
-    pgx_scene_desc desc{};
-};
-
-void build_combined(pgx_server& S, Combined& c) {
-    int aOff = 0, dOff = 0, wd = 0;
-    for (int id : S.active) {
-        Request& r = S.reqs[id];
-        const HostScene& h = r.sc;
-        r.dynStart = dOff;
-        for (int a = 0; a < h.nA; ++a) {
-            c.entity.push_back(r.entityBase + h.entity[a]);
-            c.dynOfActor.push_back(h.dynOfActor[a] >= 0 ? h.dynOfActor[a] + dOff : -1);
-            c.actorWorld.push_back(wd);
-        }
-        c.pos.insert(c.pos.end(), h.pos.begin(), h.pos.end());
-        c.quat.insert(c.quat.end(), h.quat.begin(), h.quat.end());
-        for (int d = 0; d < h.nD; ++d) c.actorOfDyn.push_back(h.actorOfDyn[d] + aOff);
-        c.kinematic.insert(c.kinematic.end(), h.kinematic.begin(), h.kinematic.end());
-        c.vel.insert(c.vel.end(), h.vel.begin(), h.vel.end());
-        c.angvel.insert(c.angvel.end(), h.angvel.begin(), h.angvel.end());
-        c.invMass.insert(c.invMass.end(), h.invMass.begin(), h.invMass.end());
-        c.com.insert(c.com.end(), h.com.begin(), h.com.end());
-        c.invI.insert(c.invI.end(), h.invI.begin(), h.invI.end());
-        for (int k = 0; k < h.nC; ++k) c.colActor.push_back(h.colActor[k] + aOff);
-        c.colIndex.insert(c.colIndex.end(), h.colIndex.begin(), h.colIndex.end());
-        c.colPos.insert(c.colPos.end(), h.colPos.begin(), h.colPos.end());
-        c.colQuat.insert(c.colQuat.end(), h.colQuat.begin(), h.colQuat.end());
-        c.colType.insert(c.colType.end(), h.colType.begin(), h.colType.end());
-        c.colParams.insert(c.colParams.end(), h.colParams.begin(), h.colParams.end());
-        c.colMat.insert(c.colMat.end(), h.colMat.begin(), h.colMat.end());
-        c.colTrigger.insert(c.colTrigger.end(), h.colTrigger.begin(), h.colTrigger.end());
-        c.colEnable.insert(c.colEnable.end(), h.colEnable.begin(), h.colEnable.end());
-        c.colData.insert(c.colData.end(), h.colData.begin(), h.colData.end());
-        r.jointStart = (int)c.joints.size();
-        for (pgx_joint_desc j : h.joints) {
-            j.actor0 += aOff;
-            j.actor1 += aOff;
-            c.joints.push_back(j);
-        }
-        aOff += h.nA;
-        dOff += h.nD;
-        ++wd;
-    }
-    pgx_scene_desc& d = c.desc;
-    d.n_actors = aOff;
-    d.entity = c.entity.data(); d.pos = c.pos.data(); d.quat = c.quat.data(); d.dyn_of_actor = c.dynOfActor.data();
-    d.n_dyn = dOff;
-    d.actor_of_dyn = c.actorOfDyn.data(); d.kinematic = c.kinematic.data(); d.vel = c.vel.data();
-    d.angvel = c.angvel.data(); d.inv_mass = c.invMass.data(); d.com = c.com.data(); d.inv_inertia = c.invI.data();
-    d.n_colliders = (int)c.colType.size();
-    d.col_actor = c.colActor.data(); d.col_index = c.colIndex.data(); d.col_pos = c.colPos.data();
-    d.col_quat = c.colQuat.data(); d.col_type = c.colType.data(); d.col_params = c.colParams.data();
-    d.col_material = c.colMat.data(); d.col_trigger = c.colTrigger.data(); d.col_enable = c.colEnable.data();
-    d.col_data = c.colData.data();
-    d.n_worlds = wd;
-    d.actor_world = c.actorWorld.data();
-    d.n_convex = 0; d.n_trimesh = 0;
-    d.convex_vert_start = c.zero; d.convex_face_start = c.zero; d.tri_vert_start = c.zero; d.tri_start = c.zero;
+// the descriptor a request was submitted with (its own copy)
+pgx_scene_desc desc_of(const HostScene& h) {
+    static const int32_t zero[2] = {0, 0};
+    pgx_scene_desc d{};
+    d.n_actors = h.nA; d.n_dyn = h.nD; d.n_colliders = h.nC;
+    d.entity = h.entity.data(); d.pos = h.pos.data(); d.quat = h.quat.data(); d.dyn_of_actor = h.dynOfActor.data();
+    d.actor_of_dyn = h.actorOfDyn.data(); d.kinematic = h.kinematic.data(); d.vel = h.vel.data();
+    d.angvel = h.angvel.data(); d.inv_mass = h.invMass.data(); d.com = h.com.data(); d.inv_inertia = h.invI.data();
+    d.col_actor = h.colActor.data(); d.col_index = h.colIndex.data(); d.col_pos = h.colPos.data();
+    d.col_quat = h.colQuat.data(); d.col_type = h.colType.data(); d.col_params = h.colParams.data();
+    d.col_material = h.colMat.data(); d.col_trigger = h.colTrigger.data(); d.col_enable = h.colEnable.data();
+    d.col_data = h.colData.data();
+    d.n_worlds = 1;
+    d.convex_vert_start = zero; d.convex_face_start = zero; d.tri_vert_start = zero; d.tri_start = zero;
+    return d;
 }
 
 // final dynamic state of a request leaving the batch
@@ -209,122 +201,271 @@ void read_result(pgx_server& S, Request& r) {
     int n = r.sc.nD;
     r.rpos.resize(n); r.rquat.resize(n); r.rvel.resize(n); r.rang.resize(n);
     if (n) {
-        PGX_HIP(hipMemcpyAsync(r.rpos.data(), w.bpos.p + r.dynStart, sizeof(float4) * n, hipMemcpyDeviceToHost, w.stream));
-        PGX_HIP(hipMemcpyAsync(r.rquat.data(), w.bquat.p + r.dynStart, sizeof(float4) * n, hipMemcpyDeviceToHost, w.stream));
-        PGX_HIP(hipMemcpyAsync(r.rvel.data(), w.dvel.p + r.dynStart, sizeof(float4) * n, hipMemcpyDeviceToHost, w.stream));
-        PGX_HIP(hipMemcpyAsync(r.rang.data(), w.dang.p + r.dynStart, sizeof(float4) * n, hipMemcpyDeviceToHost, w.stream));
+        PGX_HIP(hipMemcpyAsync(r.rpos.data(), w.bpos.p + r.dOff, sizeof(float4) * n, hipMemcpyDeviceToHost, w.stream));
+        PGX_HIP(hipMemcpyAsync(r.rquat.data(), w.bquat.p + r.dOff, sizeof(float4) * n, hipMemcpyDeviceToHost, w.stream));
+        PGX_HIP(hipMemcpyAsync(r.rvel.data(), w.dvel.p + r.dOff, sizeof(float4) * n, hipMemcpyDeviceToHost, w.stream));
+        PGX_HIP(hipMemcpyAsync(r.rang.data(), w.dang.p + r.dOff, sizeof(float4) * n, hipMemcpyDeviceToHost, w.stream));
     }
 }
 
-// re-initialises the batched world for the current active set, carrying over the state
-// of the requests in `continuing` (old world index -> request id)
-void rebuild(pgx_server& S, const std::vector<std::pair<int, int>>& continuing, int oldW, int oldLocalBits) {
+// Byte regions of the packed upload, 256-byte aligned.
+struct Packing {
+    size_t size = 0;
+    template <class T>
+    size_t region(size_t n) {
+        size_t off = size;
+        size = (off + n * sizeof(T) + 255) & ~(size_t)255;
+        return off;
+    }
+};
+
+template <class T>
+void put(uint8_t* base, size_t region, size_t at, const std::vector<T>& v) {
+    if (!v.empty()) std::memcpy(base + region + at * sizeof(T), v.data(), v.size() * sizeof(T));
+}
+
+// Gathers one array of the next world into `spare` (from `live`, the current world's, and
+// the upload) and swaps the two.
+template <class T, int FIX>
+void gather(hipStream_t s, int nseg, const Seg* segs, int total, DBuf<T>& live, const uint8_t* stage, size_t region,
+            DBuf<T>& spare, int dstDyn) {
+    spare.alloc(total);
+    if (total > 0 && nseg > 0) {
+        hipLaunchKernelGGL((k_gather<T, FIX>), dim3(div_up(total, 256)), dim3(256), 0, s, nseg, segs, total,
+                           (const T*)live.p, (const T*)(stage + region), spare.p, dstDyn);
+        PGX_LAUNCH_CHECK();
+    }
+    live.swap(spare);
+}
+
+// Makes the batched world hold the active requests (S.active, in world order). Requests
+// placed in the current world keep their state; the others are converted and uploaded.
+void assemble(pgx_server& S) {
     World& w = S.w;
     hipStream_t s = w.stream;
-    // 1. stage the continuing requests' dynamic state and the contact cache
-    std::vector<int> oldDynStart(continuing.size()), nd(continuing.size());
-    int total = 0;
-    for (size_t i = 0; i < continuing.size(); ++i) {
-        const Request& r = S.reqs[continuing[i].second];
-        oldDynStart[i] = r.dynStart;
-        nd[i] = r.sc.nD;
-        total += r.sc.nD;
-    }
-    std::vector<int4> seg;
-    if (total) {
-        S.tPos.alloc(total); S.tQuat.alloc(total); S.tVel.alloc(total); S.tAng.alloc(total);
-        int flat = 0;
-        for (size_t i = 0; i < continuing.size(); ++i) {
-            if (!nd[i]) continue;
-            seg.push_back(make_int4(oldDynStart[i], flat, nd[i], flat));
-            flat += nd[i];
+    const int oldD = w.D, oldW = S.worlds, oldLocalBits = w.localBits;
+    const int W = (int)S.active.size();
+    // 1. conversion of the admitted requests (request-local layout, entity ids shifted)
+    std::vector<SceneArrays> lay(W);
+    for (int i = 0; i < W; ++i) {
+        Request& r = S.reqs[S.active[i]];
+        if (r.placed) continue;
+        pgx_scene_desc d = desc_of(r.sc);
+        convert_scene(&d, r.entityBase, lay[i]);
+        const SceneArrays& L = lay[i];
+        r.nDC = (int)L.dynCols.size();
+        r.nSC = (int)L.staticCols.size();
+        r.bodyEntity = L.bent;
+        r.bodyKinematic = L.kinematic;
+        r.bodyJoints = r.sc.joints;
+        for (pgx_joint_desc& j : r.bodyJoints) {
+            j.actor0 = L.actorToBody[j.actor0];
+            j.actor1 = L.actorToBody[j.actor1];
         }
-        S.seg.alloc(seg.size());
-        PGX_HIP(hipMemcpyAsync(S.seg.p, seg.data(), sizeof(int4) * seg.size(), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_copy_segments, dim3(div_up(total, 256)), dim3(256), 0, s, (int)seg.size(), S.seg.p, total,
-                           w.bpos.p, w.bquat.p, w.dvel.p, w.dang.p, S.tPos.p, S.tQuat.p, S.tVel.p, S.tAng.p);
+    }
+    // 2. placements in the next world and the segment tables; the upload holds the admitted
+    // requests one after the other (per request: dynamic bodies, then the others)
+    int Dt = 0, St = 0, Ct = 0, DCt = 0, SCt = 0, Jt = 0, maxC = 1;
+    for (int id : S.active) {
+        const Request& r = S.reqs[id];
+        Dt += r.sc.nD; St += r.sc.nA - r.sc.nD; Ct += r.sc.nC; DCt += r.nDC; SCt += r.nSC;
+        Jt += (int)r.bodyJoints.size();
+        maxC = std::max(maxC, r.sc.nC);
+    }
+    // checks the key widths before anything changes
+    world_set_shape(w, &S.cfg, Dt + St, Dt, Ct, W, DCt, maxC);
+    std::vector<Seg> gBody, gDyn, gDyn3, gCol, gDC, gSC, gJoint;
+    std::vector<int32_t> colStart(W + 1, 0), staticStart(W + 1, 0), worldMapH(std::max(oldW, 1), -1);
+    std::vector<uint8_t> mask(std::max(W, 1), 0);
+    std::vector<int> upA(W), upD(W), upC(W), upDC(W), upSC(W);
+    int uA = 0, uD = 0, uC = 0, uDC = 0, uSC = 0;
+    int nd = 0, ns = 0, nc = 0, ndc = 0, nsc = 0, nj = 0;
+    auto add = [](std::vector<Seg>& v, Seg g) {
+        if (g.count <= 0) return;
+        g.flat = v.empty() ? 0 : v.back().flat + v.back().count;
+        v.push_back(g);
+    };
+    bool anyContinuing = false;
+    for (int i = 0; i < W; ++i) {
+        Request& r = S.reqs[S.active[i]];
+        const int nD = r.sc.nD, nS = r.sc.nA - r.sc.nD, nC = r.sc.nC;
+        Seg b{};
+        b.world = i;
+        b.d = nd; b.e = ns;
+        if (r.placed) {
+            anyContinuing = true;
+            mask[i] = 1;
+            worldMapH[r.world] = i;
+            b.src = 0;
+            b.a = oldD; b.b = r.dOff; b.c = r.sOff;
+            b.delta = nc - r.cOff;
+            Seg t = b;
+            t.srcStart = r.dOff; t.dstStart = nd; t.count = nD; add(gBody, t);
+            t.srcStart = oldD + r.sOff; t.dstStart = Dt + ns; t.count = nS; add(gBody, t);
+            t.srcStart = r.dOff; t.dstStart = nd; t.count = nD; add(gDyn, t);
+            t.srcStart = 3 * r.dOff; t.dstStart = 3 * nd; t.count = 3 * nD; add(gDyn3, t);
+            t.srcStart = r.cOff; t.dstStart = nc; t.count = nC; add(gCol, t);
+            t.srcStart = r.dcOff; t.dstStart = ndc; t.count = r.nDC; add(gDC, t);
+            t.srcStart = r.scOff; t.dstStart = nsc; t.count = r.nSC; add(gSC, t);
+            t.srcStart = r.jointStart; t.dstStart = nj; t.count = (int)r.bodyJoints.size(); add(gJoint, t);
+        } else {
+            upA[i] = uA; upD[i] = uD; upC[i] = uC; upDC[i] = uDC; upSC[i] = uSC;
+            b.src = 1;
+            b.a = nD; b.b = 0; b.c = 0;   // uploaded collider bodies are request-local
+            b.delta = nc;
+            Seg t = b;
+            t.srcStart = uA; t.dstStart = nd; t.count = nD; add(gBody, t);
+            t.srcStart = uA + nD; t.dstStart = Dt + ns; t.count = nS; add(gBody, t);
+            t.srcStart = uD; t.dstStart = nd; t.count = nD; add(gDyn, t);
+            t.srcStart = 3 * uD; t.dstStart = 3 * nd; t.count = 3 * nD; add(gDyn3, t);
+            t.srcStart = uC; t.dstStart = nc; t.count = nC; add(gCol, t);
+            t.srcStart = uDC; t.dstStart = ndc; t.count = r.nDC; add(gDC, t);
+            t.srcStart = uSC; t.dstStart = nsc; t.count = r.nSC; add(gSC, t);
+            uA += r.sc.nA; uD += nD; uC += nC; uDC += r.nDC; uSC += r.nSC;
+        }
+        r.world = i;
+        r.dOff = nd; r.sOff = ns; r.cOff = nc; r.dcOff = ndc; r.scOff = nsc; r.jointStart = nj;
+        nd += nD; ns += nS; nc += nC; ndc += r.nDC; nsc += r.nSC; nj += (int)r.bodyJoints.size();
+        colStart[i + 1] = nc;
+        staticStart[i + 1] = nsc;
+    }
+    // 3. the packed upload
+    Packing P;
+    const size_t rSeg[7] = {P.region<Seg>(gBody.size()), P.region<Seg>(gDyn.size()), P.region<Seg>(gDyn3.size()),
+                            P.region<Seg>(gCol.size()), P.region<Seg>(gDC.size()), P.region<Seg>(gSC.size()),
+                            P.region<Seg>(gJoint.size())};
+    const size_t rColStart = P.region<int32_t>(W + 1), rStaticStart = P.region<int32_t>(W + 1);
+    const size_t rWorldMap = P.region<int32_t>(worldMapH.size()), rMask = P.region<uint8_t>(mask.size());
+    const size_t rBpos = P.region<float4>(uA), rBquat = P.region<float4>(uA), rBent = P.region<uint32_t>(uA);
+    const size_t rDvel = P.region<float4>(uD), rDang = P.region<float4>(uD), rDcom = P.region<float4>(uD);
+    const size_t rDinvI = P.region<float4>(3 * (size_t)uD);
+    const size_t rCbody = P.region<int32_t>(uC), rClocal = P.region<int32_t>(uC), rCflags = P.region<int32_t>(uC);
+    const size_t rCtype = P.region<int32_t>(uC), rCindex = P.region<int32_t>(uC), rCent = P.region<uint32_t>(uC);
+    const size_t rCparams = P.region<float4>(uC), rClpos = P.region<float4>(uC), rClquat = P.region<float4>(uC);
+    const size_t rCmat = P.region<float4>(uC);
+    const size_t rDynCols = P.region<int32_t>(uDC), rStaticCols = P.region<int32_t>(uSC);
+    if (P.size > S.hstageCap) {
+        if (S.hstage) PGX_HIP(hipHostFree(S.hstage));
+        S.hstage = nullptr;
+        S.hstageCap = 0;
+        PGX_HIP(hipHostMalloc((void**)&S.hstage, P.size, hipHostMallocDefault));
+        S.hstageCap = P.size;
+    }
+    uint8_t* h = S.hstage;
+    put(h, rSeg[0], 0, gBody); put(h, rSeg[1], 0, gDyn); put(h, rSeg[2], 0, gDyn3); put(h, rSeg[3], 0, gCol);
+    put(h, rSeg[4], 0, gDC); put(h, rSeg[5], 0, gSC); put(h, rSeg[6], 0, gJoint);
+    put(h, rColStart, 0, colStart); put(h, rStaticStart, 0, staticStart);
+    put(h, rWorldMap, 0, worldMapH); put(h, rMask, 0, mask);
+    for (int i = 0; i < W; ++i) {
+        const Request& r = S.reqs[S.active[i]];
+        if (r.placed) continue;
+        const SceneArrays& L = lay[i];
+        put(h, rBpos, upA[i], L.bpos); put(h, rBquat, upA[i], L.bquat); put(h, rBent, upA[i], L.bent);
+        put(h, rDvel, upD[i], L.dvel); put(h, rDang, upD[i], L.dang); put(h, rDcom, upD[i], L.dcom);
+        put(h, rDinvI, 3 * (size_t)upD[i], L.dinvI);
+        put(h, rCbody, upC[i], L.cbody); put(h, rClocal, upC[i], L.clocal); put(h, rCflags, upC[i], L.cflags);
+        put(h, rCtype, upC[i], L.ctype); put(h, rCindex, upC[i], L.cindex); put(h, rCent, upC[i], L.cent);
+        put(h, rCparams, upC[i], L.cparams); put(h, rClpos, upC[i], L.clpos); put(h, rClquat, upC[i], L.clquat);
+        put(h, rCmat, upC[i], L.cmat);
+        put(h, rDynCols, upDC[i], L.dynCols); put(h, rStaticCols, upSC[i], L.staticCols);
+    }
+    S.stage.alloc(P.size);
+    PGX_HIP(hipMemcpyAsync(S.stage.p, h, P.size, hipMemcpyHostToDevice, s));
+    const uint8_t* st = S.stage.p;
+    auto segs = [&](int k) { return (const Seg*)(st + rSeg[k]); };
+    // 4. the state to carry that the world's re-initialisation replaces: contact cache and
+    // joint state move to the server's buffers
+    const int nCache = anyContinuing ? w.cacheCount[w.cacheCur] : 0;
+    const size_t lpStride = w.manifoldCap;
+    if (nCache) {
+        const int cur = w.cacheCur;
+        S.oKey.swap(w.cacheKey[cur]); S.oTri.swap(w.cacheTri[cur]); S.oNum.swap(w.cacheNum[cur]);
+        S.oLp.swap(w.cacheLp[cur]);
+    }
+    if (!gJoint.empty()) { S.oGear.swap(w.jGear); S.oGearInit.swap(w.jGearInit); }
+    // 5. the next world's arrays
+    gather<float4, FIX_NONE>(s, (int)gBody.size(), segs(0), Dt + St, w.bpos, st, rBpos, S.xBpos, Dt);
+    gather<float4, FIX_NONE>(s, (int)gBody.size(), segs(0), Dt + St, w.bquat, st, rBquat, S.xBquat, Dt);
+    gather<uint32_t, FIX_NONE>(s, (int)gBody.size(), segs(0), Dt + St, w.bentity, st, rBent, S.xBent, Dt);
+    gather<float4, FIX_NONE>(s, (int)gDyn.size(), segs(1), Dt, w.dvel, st, rDvel, S.xDvel, Dt);
+    gather<float4, FIX_NONE>(s, (int)gDyn.size(), segs(1), Dt, w.dang, st, rDang, S.xDang, Dt);
+    gather<float4, FIX_NONE>(s, (int)gDyn.size(), segs(1), Dt, w.dcom, st, rDcom, S.xDcom, Dt);
+    gather<float4, FIX_NONE>(s, (int)gDyn3.size(), segs(2), 3 * Dt, w.dinvI, st, rDinvI, S.xDinvI, Dt);
+    gather<int32_t, FIX_BODY>(s, (int)gCol.size(), segs(3), Ct, w.cbody, st, rCbody, S.xCbody, Dt);
+    gather<int32_t, FIX_WORLD>(s, (int)gCol.size(), segs(3), Ct, w.cworld, st, 0, S.xCworld, Dt);
+    gather<int32_t, FIX_NONE>(s, (int)gCol.size(), segs(3), Ct, w.clocal, st, rClocal, S.xClocal, Dt);
+    gather<int32_t, FIX_NONE>(s, (int)gCol.size(), segs(3), Ct, w.cflags, st, rCflags, S.xCflags, Dt);
+    gather<int32_t, FIX_NONE>(s, (int)gCol.size(), segs(3), Ct, w.ctype, st, rCtype, S.xCtype, Dt);
+    gather<int32_t, FIX_NONE>(s, (int)gCol.size(), segs(3), Ct, w.cindex, st, rCindex, S.xCindex, Dt);
+    gather<uint32_t, FIX_NONE>(s, (int)gCol.size(), segs(3), Ct, w.centity, st, rCent, S.xCent, Dt);
+    gather<float4, FIX_NONE>(s, (int)gCol.size(), segs(3), Ct, w.cparams, st, rCparams, S.xCparams, Dt);
+    gather<float4, FIX_NONE>(s, (int)gCol.size(), segs(3), Ct, w.clpos, st, rClpos, S.xClpos, Dt);
+    gather<float4, FIX_NONE>(s, (int)gCol.size(), segs(3), Ct, w.clquat, st, rClquat, S.xClquat, Dt);
+    gather<float4, FIX_NONE>(s, (int)gCol.size(), segs(3), Ct, w.cmat, st, rCmat, S.xCmat, Dt);
+    gather<int32_t, FIX_COLLIDER>(s, (int)gDC.size(), segs(4), DCt, w.dynCols, st, rDynCols, S.xDynCols, Dt);
+    gather<int32_t, FIX_COLLIDER>(s, (int)gSC.size(), segs(5), SCt, w.staticCols, st, rStaticCols, S.xStaticCols, Dt);
+    w.worldColStart.alloc(W + 1); w.worldStaticStart.alloc(W + 1); w.colOfLocal.alloc(Ct);
+    PGX_HIP(hipMemcpyAsync(w.worldColStart.p, st + rColStart, sizeof(int32_t) * (W + 1), hipMemcpyDeviceToDevice, s));
+    PGX_HIP(hipMemcpyAsync(w.worldStaticStart.p, st + rStaticStart, sizeof(int32_t) * (W + 1),
+                           hipMemcpyDeviceToDevice, s));
+    if (Ct) {
+        hipLaunchKernelGGL(k_iota, dim3(div_up(Ct, 256)), dim3(256), 0, s, Ct, w.colOfLocal.p);   // colliders by world
         PGX_LAUNCH_CHECK();
     }
-    // gear joints keep angles across steps (GearJoint.h:7-14): stage every joint's state
-    std::vector<float4> gearOld;
-    std::vector<int32_t> gearInitOld;
-    std::vector<int> oldJointStart(continuing.size());
-    for (size_t i = 0; i < continuing.size(); ++i) oldJointStart[i] = S.reqs[continuing[i].second].jointStart;
-    if (!continuing.empty() && w.J > 0) {
-        gearOld.resize(w.J);
-        gearInitOld.resize(w.J);
-        PGX_HIP(hipMemcpyAsync(gearOld.data(), w.jGear.p, sizeof(float4) * w.J, hipMemcpyDeviceToHost, s));
-        PGX_HIP(hipMemcpyAsync(gearInitOld.data(), w.jGearInit.p, sizeof(int32_t) * w.J, hipMemcpyDeviceToHost, s));
+    // 6. host state and the buffers that follow from the shape
+    world_reset_host(w);
+    w.actorToBody.resize(Dt + St);
+    for (int b = 0; b < Dt + St; ++b) w.actorToBody[b] = b;   // joints name internal bodies
+    w.bodyToActor.clear();
+    w.hostEntity.resize(Dt + St);
+    w.hostKinematic.resize(Dt);
+    for (int i = 0; i < W; ++i) {
+        const Request& r = S.reqs[S.active[i]];
+        const int nD = r.sc.nD;
+        std::copy(r.bodyEntity.begin(), r.bodyEntity.begin() + nD, w.hostEntity.begin() + r.dOff);
+        std::copy(r.bodyEntity.begin() + nD, r.bodyEntity.end(), w.hostEntity.begin() + Dt + r.sOff);
+        std::copy(r.bodyKinematic.begin(), r.bodyKinematic.end(), w.hostKinematic.begin() + r.dOff);
     }
-    int nCache = continuing.empty() ? 0 : w.cacheCount[w.cacheCur];
-    if (nCache) {
-        int cur = w.cacheCur;
-        S.tKey.alloc(nCache); S.tTri.alloc(nCache); S.tNum.alloc(nCache); S.tLp.alloc(4 * (size_t)nCache);
-        PGX_HIP(hipMemcpyAsync(S.tKey.p, w.cacheKey[cur].p, sizeof(uint64_t) * nCache, hipMemcpyDeviceToDevice, s));
-        PGX_HIP(hipMemcpyAsync(S.tTri.p, w.cacheTri[cur].p, sizeof(int32_t) * nCache, hipMemcpyDeviceToDevice, s));
-        PGX_HIP(hipMemcpyAsync(S.tNum.p, w.cacheNum[cur].p, sizeof(int32_t) * nCache, hipMemcpyDeviceToDevice, s));
-        for (int j = 0; j < 4; ++j)
-            PGX_HIP(hipMemcpyAsync(S.tLp.p + (size_t)j * nCache, w.cacheLp[cur].p + (size_t)j * w.manifoldCap,
-                                   sizeof(float4) * nCache, hipMemcpyDeviceToDevice, s));
-    }
-    PGX_HIP(hipStreamSynchronize(s));
-    // 2. re-initialise from the concatenated descriptors (new placements set here)
-    Combined c;
-    build_combined(S, c);
-    world_init(w, &S.cfg, &c.desc);
-    if (!c.joints.empty()) world_add_joints(w, (int)c.joints.size(), c.joints.data());
-    if (!gearOld.empty() && w.J > 0) {
-        std::vector<float4> gearNew(w.J, make_float4(0.f, 0.f, 0.f, 0.f));
-        std::vector<int32_t> gearInitNew(w.J, 0);
-        for (size_t i = 0; i < continuing.size(); ++i) {
-            const Request& r = S.reqs[continuing[i].second];
-            for (size_t k = 0; k < r.sc.joints.size(); ++k) {
-                gearNew[r.jointStart + k] = gearOld[oldJointStart[i] + k];
-                gearInitNew[r.jointStart + k] = gearInitOld[oldJointStart[i] + k];
+    w.hostColBody.clear();   // per-contact reports are not served
+    w.hostColIndex.clear();
+    world_alloc_derived(w, &S.cfg, 0);
+    // creation-time bounds (no margin); a continuing request's dynamic colliders carry the
+    // margin of its last step
+    bp_compute_bounds(w, false);
+    if (anyContinuing) bp_compute_bounds(w, true, st + rMask);
+    // 7. joints (internal body indices) and the gear angles of continuing requests
+    if (Jt) {
+        std::vector<pgx_joint_desc> js;
+        js.reserve(Jt);
+        for (int i = 0; i < W; ++i) {
+            const Request& r = S.reqs[S.active[i]];
+            for (pgx_joint_desc j : r.bodyJoints) {
+                for (int* a : {&j.actor0, &j.actor1})
+                    *a = *a < r.sc.nD ? r.dOff + *a : Dt + r.sOff + (*a - r.sc.nD);
+                js.push_back(j);
             }
         }
-        PGX_HIP(hipMemcpyAsync(w.jGear.p, gearNew.data(), sizeof(float4) * w.J, hipMemcpyHostToDevice, s));
-        PGX_HIP(hipMemcpyAsync(w.jGearInit.p, gearInitNew.data(), sizeof(int32_t) * w.J, hipMemcpyHostToDevice, s));
-        PGX_HIP(hipStreamSynchronize(s));   // the host vectors end with this scope
-    }
-    // 3. put the staged state back at the continuing requests' new placements
-    std::vector<int> newWorldOf(oldW, -1);
-    std::vector<uint8_t> mask(std::max(1, w.W), 0);
-    {
-        std::unordered_map<int, int> widx;
-        for (size_t i = 0; i < S.active.size(); ++i) widx[S.active[i]] = (int)i;
-        int flat = 0;
-        seg.clear();
-        for (size_t i = 0; i < continuing.size(); ++i) {
-            int nw = widx[continuing[i].second];
-            newWorldOf[continuing[i].first] = nw;
-            mask[nw] = 1;
-            if (!nd[i]) continue;
-            seg.push_back(make_int4(flat, S.reqs[continuing[i].second].dynStart, nd[i], flat));
-            flat += nd[i];
+        world_add_joints(w, Jt, js.data());   // joint state starts at zero
+        if (!gJoint.empty()) {
+            const int n = gJoint.back().flat + gJoint.back().count;
+            hipLaunchKernelGGL((k_gather<float4, FIX_NONE>), dim3(div_up(n, 256)), dim3(256), 0, s,
+                               (int)gJoint.size(), segs(6), n, (const float4*)S.oGear.p, (const float4*)nullptr,
+                               w.jGear.p, 0);
+            hipLaunchKernelGGL((k_gather<int32_t, FIX_NONE>), dim3(div_up(n, 256)), dim3(256), 0, s,
+                               (int)gJoint.size(), segs(6), n, (const int32_t*)S.oGearInit.p,
+                               (const int32_t*)nullptr, w.jGearInit.p, 0);
+            PGX_LAUNCH_CHECK();
         }
     }
-    if (total) {
-        PGX_HIP(hipMemcpyAsync(S.seg.p, seg.data(), sizeof(int4) * seg.size(), hipMemcpyHostToDevice, s));
-        hipLaunchKernelGGL(k_copy_segments, dim3(div_up(total, 256)), dim3(256), 0, s, (int)seg.size(), S.seg.p, total,
-                           S.tPos.p, S.tQuat.p, S.tVel.p, S.tAng.p, w.bpos.p, w.bquat.p, w.dvel.p, w.dang.p);
-        PGX_LAUNCH_CHECK();
-    }
-    if (!continuing.empty()) {
-        // a continuing request's dynamic colliders carry the margin of its last step
-        S.worldMask.alloc(mask.size());
-        PGX_HIP(hipMemcpyAsync(S.worldMask.p, mask.data(), mask.size(), hipMemcpyHostToDevice, s));
-        bp_compute_bounds(w, true, S.worldMask.p);
-    }
+    // 8. the contact cache of continuing requests, world ids renumbered
     if (nCache) {
-        S.worldMap.alloc(oldW);
-        PGX_HIP(hipMemcpyAsync(S.worldMap.p, newWorldOf.data(), sizeof(int32_t) * oldW, hipMemcpyHostToDevice, s));
         S.flag.alloc(nCache); S.fpos.alloc(nCache);
-        hipLaunchKernelGGL(k_cache_flag, dim3(div_up(nCache, 256)), dim3(256), 0, s, nCache, S.tKey.p, oldLocalBits,
-                           S.worldMap.p, S.flag.p);
+        const int32_t* wmap = (const int32_t*)(st + rWorldMap);
+        hipLaunchKernelGGL(k_cache_flag, dim3(div_up(nCache, 256)), dim3(256), 0, s, nCache, S.oKey.p, oldLocalBits,
+                           wmap, S.flag.p);
         exclusive_scan_i32(s, S.flag.p, S.fpos.p, nCache, w.counters.p + 15, w.scanT);
-        hipLaunchKernelGGL(k_cache_remap, dim3(div_up(nCache, 256)), dim3(256), 0, s, nCache, S.tKey.p, S.tTri.p,
-                           S.tNum.p, S.tLp.p, oldLocalBits, w.localBits, S.worldMap.p, S.flag.p, S.fpos.p,
+        hipLaunchKernelGGL(k_cache_remap, dim3(div_up(nCache, 256)), dim3(256), 0, s, nCache, S.oKey.p, S.oTri.p,
+                           S.oNum.p, S.oLp.p, lpStride, oldLocalBits, w.localBits, wmap, S.flag.p, S.fpos.p,
                            w.cacheKey[0].p, w.cacheTri[0].p, w.cacheNum[0].p, w.cacheLp[0].p, w.manifoldCap);
         PGX_LAUNCH_CHECK();
         int kept = 0;
@@ -333,7 +474,9 @@ void rebuild(pgx_server& S, const std::vector<std::pair<int, int>>& continuing, 
         w.cacheCur = 0;
         w.cacheCount[0] = kept;
     }
-    PGX_HIP(hipStreamSynchronize(s));
+    PGX_HIP(hipStreamSynchronize(s));   // the pinned upload buffer is reused by the next assembly
+    for (int id : S.active) S.reqs[id].placed = true;
+    S.worlds = W;
 }
 
 template <class F>
@@ -461,7 +604,6 @@ int pgx_server_step(pgx_server* S, float dt, pgx_server_step_info* info) {
         for (int id : S->active) anyRunning |= S->reqs[id].state == 1;
         if (S->sinceBoundary >= S->admitInterval || !anyRunning) {
             std::vector<int> keep;
-            std::vector<std::pair<int, int>> continuing;   // (old world, request)
             for (size_t i = 0; i < S->active.size(); ++i) {
                 Request& r = S->reqs[S->active[i]];
                 if (r.state == 2) {
@@ -469,7 +611,6 @@ int pgx_server_step(pgx_server* S, float dt, pgx_server_step_info* info) {
                     ++retired;
                 } else {
                     keep.push_back(r.id);
-                    continuing.push_back({(int)i, r.id});
                 }
             }
             while (!S->queue.empty() && (int)keep.size() < S->maxWorlds) {
@@ -481,13 +622,13 @@ int pgx_server_step(pgx_server* S, float dt, pgx_server_step_info* info) {
                 keep.push_back(r.id);
                 ++admitted;
             }
-            int oldW = (int)S->active.size();
-            int oldLocalBits = w.localBits;
             S->active.swap(keep);
             if (retired || admitted) {
-                if (S->active.empty()) S->initialised = false;
-                else {
-                    rebuild(*S, continuing, oldW, oldLocalBits);
+                if (S->active.empty()) {
+                    S->initialised = false;
+                    S->worlds = 0;
+                } else {
+                    assemble(*S);
                     S->initialised = true;
                 }
             }

@@ -129,10 +129,82 @@ static int validate_scene(const pgx_scene_desc* sc) {
 
 namespace pgx {
 
-// (Re)initialises a world from a validated descriptor on w's device and stream. Device
-// buffers are reused where they are large enough; host state of a previous scene (joints,
-// non-colliding pairs, contact cache, statistics) is cleared.
-void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
+// The device-layout arrays of a descriptor: bodies in internal order (the dynamic storage
+// first, so body b == dynamic index b, then the other actors), colliders in descriptor
+// order with world-local indices, the dynamic colliders (LBVH leaves) and the others
+// grouped by world. Entity ids are shifted by entityBase.
+void convert_scene(const pgx_scene_desc* sc, uint32_t entityBase, SceneArrays& o) {
+    const int W = sc->actor_world ? std::max(1, sc->n_worlds) : 1;
+    const int A = sc->n_actors, D = sc->n_dyn, C = sc->n_colliders;
+    o.A = A; o.D = D; o.C = C; o.W = W;
+    o.actorToBody.assign(A, -1);
+    o.bodyToActor.clear();
+    o.bodyToActor.reserve(A);
+    for (int d = 0; d < D; ++d) { o.actorToBody[sc->actor_of_dyn[d]] = d; o.bodyToActor.push_back(sc->actor_of_dyn[d]); }
+    for (int a = 0; a < A; ++a)
+        if (sc->dyn_of_actor[a] < 0) { o.actorToBody[a] = (int)o.bodyToActor.size(); o.bodyToActor.push_back(a); }
+    o.bpos.resize(A); o.bquat.resize(A); o.bent.resize(A);
+    for (int b = 0; b < A; ++b) {
+        int a = o.bodyToActor[b];
+        o.bpos[b] = make_float4(sc->pos[3 * a], sc->pos[3 * a + 1], sc->pos[3 * a + 2], 0.f);
+        o.bquat[b] = make_float4(sc->quat[4 * a], sc->quat[4 * a + 1], sc->quat[4 * a + 2], sc->quat[4 * a + 3]);
+        o.bent[b] = sc->entity[a] + entityBase;
+    }
+    o.dvel.resize(D); o.dang.resize(D); o.dcom.resize(D); o.dinvI.resize(3 * (size_t)D);
+    for (int d = 0; d < D; ++d) {
+        o.dvel[d] = make_float4(sc->vel[3 * d], sc->vel[3 * d + 1], sc->vel[3 * d + 2], sc->inv_mass[d]);
+        o.dang[d] = make_float4(sc->angvel[3 * d], sc->angvel[3 * d + 1], sc->angvel[3 * d + 2], 0.f);
+        o.dcom[d] = make_float4(sc->com[3 * d], sc->com[3 * d + 1], sc->com[3 * d + 2], sc->kinematic[d] ? 1.f : 0.f);
+        for (int k = 0; k < 3; ++k)
+            o.dinvI[3 * d + k] = make_float4(sc->inv_inertia[9 * d + 3 * k], sc->inv_inertia[9 * d + 3 * k + 1],
+                                             sc->inv_inertia[9 * d + 3 * k + 2], 0.f);
+    }
+    o.kinematic.assign(sc->kinematic, sc->kinematic + D);
+    // colliders (descriptor order), world-local indices
+    o.cbody.resize(C); o.cworld.resize(C); o.clocal.resize(C); o.cflags.resize(C); o.ctype.resize(C);
+    o.cindex.resize(C); o.cent.resize(C); o.cparams.resize(C); o.clpos.resize(C); o.clquat.resize(C); o.cmat.resize(C);
+    std::vector<int32_t> perWorld(W, 0);
+    for (int c = 0; c < C; ++c) {
+        int a = sc->col_actor[c];
+        int wd = sc->actor_world ? sc->actor_world[a] : 0;
+        o.cbody[c] = o.actorToBody[a];
+        o.cworld[c] = wd;
+        o.clocal[c] = perWorld[wd]++;
+        int d = sc->dyn_of_actor[a];
+        bool dyn = d >= 0 && !sc->kinematic[d];
+        o.cflags[c] = (sc->col_enable[c] ? CF_ENABLE : 0) | (sc->col_trigger[c] ? CF_TRIGGER : 0) | (dyn ? CF_DYNAMIC : 0);
+        o.ctype[c] = sc->col_type[c];
+        o.cindex[c] = sc->col_index[c];
+        o.cent[c] = sc->entity[a] + entityBase;
+        o.cparams[c] = make_float4(sc->col_params[4 * c], sc->col_params[4 * c + 1], sc->col_params[4 * c + 2],
+                                   sc->col_params[4 * c + 3]);
+        o.clpos[c] = make_float4(sc->col_pos[3 * c], sc->col_pos[3 * c + 1], sc->col_pos[3 * c + 2], 0.f);
+        o.clquat[c] = make_float4(sc->col_quat[4 * c], sc->col_quat[4 * c + 1], sc->col_quat[4 * c + 2],
+                                  sc->col_quat[4 * c + 3]);
+        o.cmat[c] = make_float4(sc->col_material[3 * c], sc->col_material[3 * c + 1], sc->col_material[3 * c + 2], 0.f);
+    }
+    o.wstart.assign(W + 1, 0);
+    o.colOfLocal.resize(C);
+    for (int i = 0; i < W; ++i) o.wstart[i + 1] = o.wstart[i] + perWorld[i];
+    for (int c = 0; c < C; ++c) o.colOfLocal[o.wstart[o.cworld[c]] + o.clocal[c]] = c;
+    // LBVH leaves: colliders of non-kinematic dynamic bodies; the rest per world
+    o.dynCols.clear();
+    o.wss.assign(W + 1, 0);
+    for (int c = 0; c < C; ++c)
+        if (o.cflags[c] & CF_DYNAMIC) o.dynCols.push_back(c); else ++o.wss[o.cworld[c] + 1];
+    for (int i = 0; i < W; ++i) o.wss[i + 1] += o.wss[i];
+    o.staticCols.resize(o.wss[W]);
+    {
+        std::vector<int32_t> fillp(o.wss.begin(), o.wss.end() - 1);
+        for (int c = 0; c < C; ++c)
+            if (!(o.cflags[c] & CF_DYNAMIC)) o.staticCols[fillp[o.cworld[c]]++] = c;
+    }
+    o.maxPerWorld = W ? *std::max_element(perWorld.begin(), perWorld.end()) : 1;
+}
+
+// Host state of a previous scene (joints, non-colliding pairs, contact cache, statistics)
+// is cleared.
+void world_reset_host(World& w) {
     w.J = 0; w.jCap = 0; w.jRowsTotal = 0;
     for (auto* v : {&w.hjType, &w.hjA0, &w.hjA1, &w.hjB0, &w.hjB1, &w.hjRowFlags, &w.hjNRows, &w.hjColor, &w.hjAlive})
         v->clear();
@@ -145,95 +217,27 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     w.nConvex = 0; w.nTriMesh = 0; w.mmCap = 0; w.lastMeshSlots = 0;
     w.lastPairs = w.lastManifolds = w.lastColors = w.lastRounds = w.lastPoints = 0;
     w.lastQueues = 0; w.lastFlowPasses = 0;
-    int W = sc->actor_world ? std::max(1, sc->n_worlds) : 1;
+}
+
+// step parameters of the configuration; keys' bit widths of the world's shape
+void world_set_shape(World& w, const pgx_config* cfg, int A, int D, int C, int W, int nDynCol, int maxPerWorld) {
     if (cfg) {
         if (cfg->num_substeps > 0) w.substeps = cfg->num_substeps;
         if (cfg->num_iterations >= 0) w.iterations = cfg->num_iterations;
         if (cfg->gravity != 0.f) w.g = cfg->gravity;
     }
-    const int A = sc->n_actors, D = sc->n_dyn, C = sc->n_colliders;
     w.A = A; w.D = D; w.C = C; w.W = W;
-    // internal body order: dynamic storage first (body b == dynamic index b), then the rest
-    w.actorToBody.assign(A, -1);
-    w.bodyToActor.clear();
-    for (int d = 0; d < D; ++d) { w.actorToBody[sc->actor_of_dyn[d]] = d; w.bodyToActor.push_back(sc->actor_of_dyn[d]); }
-    for (int a = 0; a < A; ++a)
-        if (sc->dyn_of_actor[a] < 0) { w.actorToBody[a] = (int)w.bodyToActor.size(); w.bodyToActor.push_back(a); }
-    std::vector<float4> bpos(A), bquat(A);
-    std::vector<uint32_t> bent(A);
-    for (int b = 0; b < A; ++b) {
-        int a = w.bodyToActor[b];
-        bpos[b] = make_float4(sc->pos[3 * a], sc->pos[3 * a + 1], sc->pos[3 * a + 2], 0.f);
-        bquat[b] = make_float4(sc->quat[4 * a], sc->quat[4 * a + 1], sc->quat[4 * a + 2], sc->quat[4 * a + 3]);
-        bent[b] = sc->entity[a];
-    }
-    std::vector<float4> dvel(D), dang(D), dcom(D), dinvI(3 * (size_t)D);
-    for (int d = 0; d < D; ++d) {
-        dvel[d] = make_float4(sc->vel[3 * d], sc->vel[3 * d + 1], sc->vel[3 * d + 2], sc->inv_mass[d]);
-        dang[d] = make_float4(sc->angvel[3 * d], sc->angvel[3 * d + 1], sc->angvel[3 * d + 2], 0.f);
-        dcom[d] = make_float4(sc->com[3 * d], sc->com[3 * d + 1], sc->com[3 * d + 2], sc->kinematic[d] ? 1.f : 0.f);
-        for (int k = 0; k < 3; ++k)
-            dinvI[3 * d + k] = make_float4(sc->inv_inertia[9 * d + 3 * k], sc->inv_inertia[9 * d + 3 * k + 1],
-                                           sc->inv_inertia[9 * d + 3 * k + 2], 0.f);
-    }
-    // colliders (descriptor order), world-local indices
-    std::vector<int32_t> cbody(C), cworld(C), clocal(C), cflags(C), ctype(C), cindex(C);
-    std::vector<uint32_t> cent(C);
-    std::vector<float4> cparams(C), clpos(C), clquat(C), cmat(C);
-    std::vector<int32_t> perWorld(W, 0);
-    for (int c = 0; c < C; ++c) {
-        int a = sc->col_actor[c];
-        int wd = sc->actor_world ? sc->actor_world[a] : 0;
-        cbody[c] = w.actorToBody[a];
-        cworld[c] = wd;
-        clocal[c] = perWorld[wd]++;
-        int d = sc->dyn_of_actor[a];
-        bool dyn = d >= 0 && !sc->kinematic[d];
-        cflags[c] = (sc->col_enable[c] ? CF_ENABLE : 0) | (sc->col_trigger[c] ? CF_TRIGGER : 0) | (dyn ? CF_DYNAMIC : 0);
-        ctype[c] = sc->col_type[c];
-        cindex[c] = sc->col_index[c];
-        cent[c] = sc->entity[a];
-        cparams[c] = make_float4(sc->col_params[4 * c], sc->col_params[4 * c + 1], sc->col_params[4 * c + 2],
-                                 sc->col_params[4 * c + 3]);
-        clpos[c] = make_float4(sc->col_pos[3 * c], sc->col_pos[3 * c + 1], sc->col_pos[3 * c + 2], 0.f);
-        clquat[c] = make_float4(sc->col_quat[4 * c], sc->col_quat[4 * c + 1], sc->col_quat[4 * c + 2],
-                                sc->col_quat[4 * c + 3]);
-        cmat[c] = make_float4(sc->col_material[3 * c], sc->col_material[3 * c + 1], sc->col_material[3 * c + 2], 0.f);
-    }
-    std::vector<int32_t> wstart(W + 1, 0), colOfLocal(C);
-    for (int i = 0; i < W; ++i) wstart[i + 1] = wstart[i] + perWorld[i];
-    for (int c = 0; c < C; ++c) colOfLocal[wstart[cworld[c]] + clocal[c]] = c;
-    // LBVH leaves: colliders of non-kinematic dynamic bodies; the rest per world
-    std::vector<int32_t> dynCols, staticCols, wss(W + 1, 0);
-    for (int c = 0; c < C; ++c)
-        if (cflags[c] & CF_DYNAMIC) dynCols.push_back(c); else ++wss[cworld[c] + 1];
-    for (int i = 0; i < W; ++i) wss[i + 1] += wss[i];
-    staticCols.resize(wss[W]);
-    {
-        std::vector<int32_t> fillp(wss.begin(), wss.end() - 1);
-        for (int c = 0; c < C; ++c)
-            if (!(cflags[c] & CF_DYNAMIC)) staticCols[fillp[cworld[c]]++] = c;
-    }
-    w.nDynCol = (int)dynCols.size();
-    int maxPerWorld = W ? *std::max_element(perWorld.begin(), perWorld.end()) : 1;
+    w.nDynCol = nDynCol;
     w.localBits = bits_for(std::max(2, maxPerWorld));
     w.worldBits = W > 1 ? bits_for(W) : 0;
     if (w.worldBits + 30 > 64 || w.worldBits + 2 * w.localBits > 64)
         throw PgxError{PGX_ERR_INVALID, "too many worlds or colliders per world for 64-bit keys"};
-    w.hostColBody = cbody;
-    w.hostKinematic.assign(sc->kinematic, sc->kinematic + D);
-    w.hostColIndex = cindex;
-    w.hostEntity = bent;
+}
 
-    hipStream_t s = w.stream;
-    upload(w.bpos, bpos, s); upload(w.bquat, bquat, s); upload(w.bentity, bent, s);
-    upload(w.dvel, dvel, s); upload(w.dang, dang, s); upload(w.dcom, dcom, s); upload(w.dinvI, dinvI, s);
-    upload(w.cbody, cbody, s); upload(w.cworld, cworld, s); upload(w.clocal, clocal, s);
-    upload(w.cflags, cflags, s); upload(w.ctype, ctype, s); upload(w.cindex, cindex, s);
-    upload(w.centity, cent, s); upload(w.cparams, cparams, s); upload(w.clpos, clpos, s);
-    upload(w.clquat, clquat, s); upload(w.cmat, cmat, s);
-    upload(w.worldColStart, wstart, s); upload(w.colOfLocal, colOfLocal, s);
-    upload(w.dynCols, dynCols, s); upload(w.staticCols, staticCols, s); upload(w.worldStaticStart, wss, s);
+// Every buffer whose size follows from the world's shape (bodies, colliders, worlds,
+// dynamic colliders) and the pair capacity; reused where large enough.
+void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh) {
+    const int D = w.D, C = w.C, W = w.W;
     w.refitReady.alloc(std::max(1, w.nDynCol));
     w.cvel.alloc(D); w.cang.alloc(D); w.tvel.alloc(D); w.tang.alloc(D); w.pvel.alloc(D); w.pang.alloc(D); w.winvI.alloc(3 * (size_t)D);
     w.cbmin.alloc(C); w.cbmax.alloc(C);
@@ -249,7 +253,7 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     w.worldRoot.alloc(W); w.worldStop.alloc(W); w.worldMStart.alloc(W);
     w.pairCap = (cfg && cfg->pair_capacity > 0) ? cfg->pair_capacity : std::max(1024, 12 * C);
     // manifolds of triangle-mesh pairs are stored after the pair slots
-    w.mmCap = sc->n_trimesh > 0 ? std::max(65536, 16 * w.nDynCol) : 0;
+    w.mmCap = nTrimesh > 0 ? std::max(65536, 16 * w.nDynCol) : 0;
     w.manifoldCap = w.pairCap + w.mmCap;
     w.pkeys.alloc(w.pairCap); w.pvals.alloc(w.pairCap);
     w.counters.alloc(16);
@@ -282,8 +286,33 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     }
     w.nonColliding.alloc(1);
     w.heavy.alloc(w.pairCap);
-    if (sc->n_convex > 0) {
-        // ConvexMesh vertex buffers padded to a multiple of 4 with the last vertex
+}
+
+// (Re)initialises a world from a validated descriptor on w's device and stream. Device
+// buffers are reused where they are large enough.
+void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
+    world_reset_host(w);
+    SceneArrays o;
+    convert_scene(sc, 0, o);
+    world_set_shape(w, cfg, o.A, o.D, o.C, o.W, (int)o.dynCols.size(), o.maxPerWorld);
+    w.actorToBody = o.actorToBody;
+    w.bodyToActor = o.bodyToActor;
+    w.hostColBody = o.cbody;
+    w.hostKinematic = o.kinematic;
+    w.hostColIndex = o.cindex;
+    w.hostEntity = o.bent;
+
+    hipStream_t s = w.stream;
+    upload(w.bpos, o.bpos, s); upload(w.bquat, o.bquat, s); upload(w.bentity, o.bent, s);
+    upload(w.dvel, o.dvel, s); upload(w.dang, o.dang, s); upload(w.dcom, o.dcom, s); upload(w.dinvI, o.dinvI, s);
+    upload(w.cbody, o.cbody, s); upload(w.cworld, o.cworld, s); upload(w.clocal, o.clocal, s);
+    upload(w.cflags, o.cflags, s); upload(w.ctype, o.ctype, s); upload(w.cindex, o.cindex, s);
+    upload(w.centity, o.cent, s); upload(w.cparams, o.cparams, s); upload(w.clpos, o.clpos, s);
+    upload(w.clquat, o.clquat, s); upload(w.cmat, o.cmat, s);
+    upload(w.worldColStart, o.wstart, s); upload(w.colOfLocal, o.colOfLocal, s);
+    upload(w.dynCols, o.dynCols, s); upload(w.staticCols, o.staticCols, s); upload(w.worldStaticStart, o.wss, s);
+    world_alloc_derived(w, cfg, sc->n_trimesh);
+    if (sc->n_convex > 0) {        // ConvexMesh vertex buffers padded to a multiple of 4 with the last vertex
         // (src/ConvexMesh.cpp:3-11); faces, normals and centroids as given
         std::vector<float4> cvv, cvn, cvc;
         std::vector<int4> cvm;
