@@ -239,13 +239,19 @@ int pgx_world_reset_stats(pgx_world* w);
  *   PGX_EXEC_FLOW_XCD        the same with each world's manifolds on one XCD
  *   PGX_EXEC_COOPERATIVE     every substep of a step in one cooperative launch, grid
  *                            barriers between colours (worlds without joints; others use
- *                            PGX_EXEC_FLOW). blocks = its grid size (0 = one per CU). */
+ *                            PGX_EXEC_COLORS). blocks = its grid size (0 = one per CU). */
 #define PGX_EXEC_COLORS 0
 #define PGX_EXEC_COOPERATIVE 1
 #define PGX_EXEC_FLOW 2
 #define PGX_EXEC_FLOW_XCD 3
 int pgx_world_set_exec(pgx_world* w, int mode, int blocks);
 int pgx_world_get_stats(pgx_world* w, pgx_stats* out);
+/* In PGX_EXEC_COLORS, enabled != 0 launches each step's substep loop as one HIP graph:
+ * captured every step (its launches follow the step's colours), the instantiated graph
+ * updated in place when the launch sequence keeps its shape. Results are the same either
+ * way. Steps with per-launch timing (pgx_world_enable_timing) launch directly. No
+ * counterpart in the reference, whose substep loop is host code (src/Physecs.cpp:361-534). */
+int pgx_world_set_graph(pgx_world* w, int enabled);
 
 /* ---------------------------------------------------------------------------------
  * Serving: a request loop that keeps many independent scenes ("requests") resident in one
@@ -283,6 +289,8 @@ int pgx_server_submit(pgx_server* s, const pgx_scene_desc* scene, const pgx_join
 int pgx_server_step(pgx_server* s, float dt, pgx_server_step_info* info);
 /* 1 when the request finished (its state is available), 0 while queued or running. */
 int pgx_server_done(pgx_server* s, int request_id);
+/* the batched world's substep loop as one HIP graph launch per step (pgx_world_set_graph) */
+int pgx_server_set_graph(pgx_server* s, int enabled);
 /* Final state of a finished request in its descriptor's order (pos/quat per actor,
  * vel/angvel per dynamic body); releases it. */
 int pgx_server_take_result(pgx_server* s, int request_id, float* pos, float* quat, float* vel, float* angvel);

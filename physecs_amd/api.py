@@ -27,9 +27,9 @@ EXPORTS = [
     "pgx_world_write_state", "pgx_world_get_counts", "pgx_world_get_bounds", "pgx_world_get_pairs",
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
-    "pgx_world_destroy_joint",
+    "pgx_world_destroy_joint", "pgx_world_set_graph",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
-    "pgx_server_take_result", "pgx_server_footprint",
+    "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
 ]
 
 
@@ -116,6 +116,8 @@ def lib():
         L.pgx_world_add_joints.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PgxJointDesc)]
         L.pgx_world_set_joint_params.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PgxJointDesc)]
         L.pgx_world_destroy_joint.argtypes = [vp, ctypes.c_int]
+        L.pgx_world_set_graph.argtypes = [vp, ctypes.c_int]
+        L.pgx_server_set_graph.argtypes = [vp, ctypes.c_int]
         L.pgx_server_create.argtypes = [ctypes.POINTER(PgxConfig), ctypes.POINTER(PgxServerConfig), ctypes.POINTER(vp)]
         L.pgx_server_destroy.argtypes = [vp]
         L.pgx_server_submit.argtypes = [vp, ctypes.POINTER(PgxSceneDesc), ctypes.POINTER(PgxJointDesc), ctypes.c_int,
@@ -257,6 +259,10 @@ class World:
         manifolds on one XCD) or "cooperative" (one launch per step)."""
         check(self.L.pgx_world_set_exec(self.h, self.EXEC_MODES[mode], blocks))
 
+    def set_graph(self, enabled=True):
+        """Launch each step's substep loop as one HIP graph (colour mode); same results."""
+        check(self.L.pgx_world_set_graph(self.h, 1 if enabled else 0))
+
     def reset_stats(self):
         check(self.L.pgx_world_reset_stats(self.h))
 
@@ -318,6 +324,10 @@ class Server:
         check(self.L.pgx_server_submit(self.h, ctypes.byref(desc), js, nj, int(horizon), ctypes.byref(rid)))
         self.scenes[rid.value] = scene
         return rid.value
+
+    def set_graph(self, enabled=True):
+        """Launch the batched world's substep loop as one HIP graph per step."""
+        check(self.L.pgx_server_set_graph(self.h, 1 if enabled else 0))
 
     def step(self, dt):
         info = PgxServerStepInfo()

@@ -245,6 +245,9 @@ struct World {
     std::vector<int> colorStartHost;
     bool timing = false;
     int execMode = EXEC_COLORS;          // how the substep loop's contact passes are launched
+    bool graph = false;                  // substep loop as one HIP graph launch (colour mode)
+    hipGraphExec_t subExec = nullptr;    // the last instantiated substep-loop graph
+    long graphInstantiations = 0, graphUpdates = 0;
     int persistentBlocks = 0, persistentBlocksWanted = 0;
     // dependency-driven passes
     int nXcd = 1;                        // XCDs of the device (CUs / 32)
@@ -292,6 +295,7 @@ void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh);
 void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc);      // world.hip
 void world_add_joints(World& w, int n, const pgx_joint_desc* js);
 void world_step(World& w, float dt);
+void world_release_graph(World& w);
 int report_error(int code, const std::string& msg);   // sets pgx_last_error(), returns code
 
 // solver.hip

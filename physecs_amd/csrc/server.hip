@@ -413,7 +413,9 @@ void assemble(pgx_server& S) {
         PGX_LAUNCH_CHECK();
     }
     // 6. host state and the buffers that follow from the shape
+    const int rounds = w.lastRounds;
     world_reset_host(w);
+    w.lastRounds = rounds;   // sizes the next colouring's first batch of rounds only
     w.actorToBody.resize(Dt + St);
     for (int b = 0; b < Dt + St; ++b) w.actorToBody[b] = b;   // joints name internal bodies
     w.bodyToActor.clear();
@@ -536,6 +538,7 @@ int pgx_server_destroy(pgx_server* S) {
         (void)hipStreamSynchronize(S->w.stream);
     }
     hipStream_t st = S->w.stream;
+    world_release_graph(S->w);
     delete S;   // device buffers are released by their destructors
     if (st) (void)hipStreamDestroy(st);
     return PGX_OK;
@@ -664,6 +667,12 @@ int pgx_server_step(pgx_server* S, float dt, pgx_server_step_info* info) {
         }
         return PGX_OK;
     });
+}
+
+int pgx_server_set_graph(pgx_server* S, int enabled) {
+    if (!S) return srv_fail(PGX_ERR_INVALID, "server is NULL");
+    S->w.graph = enabled != 0;
+    return PGX_OK;
 }
 
 int pgx_server_done(pgx_server* S, int id) {

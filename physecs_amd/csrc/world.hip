@@ -419,6 +419,7 @@ static void release_world(World& w) {
     for (auto* b : ji) b->release();
     DBuf<float4>* jf[] = {&w.jP0, &w.jQ0, &w.jP1, &w.jQ1, &w.jDrive, &w.jR0, &w.jR1, &w.jRow};
     for (auto* b : jf) b->release();
+    world_release_graph(w);
     for (auto& e : w.ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : w.solveEv) (void)hipEventDestroy(e);
@@ -638,6 +639,53 @@ int pgx_world_destroy_joint(pgx_world* h, int index) {
 
 namespace pgx {
 
+// The substep loop as one HIP graph launch. The loop's launch sequence follows the step's
+// colours, so it is captured every step; the instantiated graph is updated in place when
+// the sequence has the shape of the previous step's and instantiated anew otherwise.
+static void substeps_graph(World& w, int M, float dt) {
+    hipStream_t s = w.stream;
+    w.tickets.alloc((size_t)(w.iterations + 1) * FLOW_QUEUES);   // no allocation under capture
+    PGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+        sv_substeps(w, M, dt);
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    PGX_HIP(hipStreamEndCapture(s, &g));
+    bool updated = false;
+    if (w.subExec) {
+        hipGraphNode_t errNode = nullptr;
+        hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+        updated = hipGraphExecUpdate(w.subExec, g, &errNode, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if (!updated) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(w.subExec);
+            w.subExec = nullptr;
+        }
+    }
+    if (updated) {
+        ++w.graphUpdates;
+    } else {
+        hipError_t e = hipGraphInstantiate(&w.subExec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            w.subExec = nullptr;
+            (void)hipGraphDestroy(g);
+            PGX_HIP(e);
+        }
+        ++w.graphInstantiations;
+    }
+    (void)hipGraphDestroy(g);   // the executable graph does not refer to it
+    PGX_HIP(hipGraphLaunch(w.subExec, s));
+}
+
+void world_release_graph(World& w) {
+    if (w.subExec) (void)hipGraphExecDestroy(w.subExec);
+    w.subExec = nullptr;
+}
+
 // one Scene::simulate(dt) of every world in w (throws on error)
 void world_step(World& w, float dt) {
     PGX_HIP(hipSetDevice(w.device));
@@ -660,7 +708,8 @@ void world_step(World& w, float dt) {
     if (w.timing) PGX_HIP(hipEventRecord(w.ev[2], s));
     int nColors = sv_color(w, M);
     if (w.timing) PGX_HIP(hipEventRecord(w.ev[3], s));
-    sv_substeps(w, M, dt);
+    if (w.graph && !w.timing && w.execMode == EXEC_COLORS) substeps_graph(w, M, dt);
+    else sv_substeps(w, M, dt);
     if (w.timing) PGX_HIP(hipEventRecord(w.ev[4], s));
     sv_finish_step(w, M);
     if (w.timing) {
@@ -952,6 +1001,12 @@ int pgx_world_set_exec(pgx_world* h, int mode, int blocks) {
     h->w.execMode = mode;
     h->w.persistentBlocksWanted = blocks;
     h->w.persistentBlocks = 0;
+    return PGX_OK;
+}
+
+int pgx_world_set_graph(pgx_world* h, int enabled) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    h->w.graph = enabled != 0;
     return PGX_OK;
 }
 
