@@ -570,9 +570,12 @@ def mixed_lattice(seed=0, side=32, spacing=1.5, jitter=0.05, with_container=True
 def batched_worlds(seed=0, n_worlds=4096, lo=8, hi=1024, spacing=1.5, ragdoll_frac=0.0):
     """n_worlds independent worlds with body counts floor(exp(U(ln lo, ln hi))), each a
     seeded mix of the mixed-lattice shapes on a lattice above its own static ground box;
-    a fraction of the worlds also holds a 15-capsule ragdoll. All worlds share coordinates;
-    the world id keeps them apart. Worlds are laid out one after another (ground, bodies,
-    ragdoll)."""
+    a fraction of the worlds also holds a 15-capsule ragdoll standing on a static pad of
+    its own, 8 m beside the ground box. (Dropped onto the pile, the light ragdoll links
+    end up between bodies up to a hundred times heavier, and the reference's solver with
+    its default two iterations blows such a ragdoll up within a second.) All worlds share
+    coordinates; the world id keeps them apart. Worlds are laid out one after another
+    (ground, bodies, then pad and ragdoll)."""
     rng = np.random.default_rng(seed)
     s = Scene()
     counts = np.floor(np.exp(rng.uniform(math.log(lo), math.log(hi), n_worlds))).astype(int)
@@ -597,8 +600,8 @@ def batched_worlds(seed=0, n_worlds=4096, lo=8, hi=1024, spacing=1.5, ragdoll_fr
                         ((i // side) % side) * spacing - off], axis=1)
         s.add_bodies_bulk(kinds, prm, pos, quats, vels, np.full(n, w))
         if has_rd[w]:
-            top = 1.0 + (n // (side * side) + 1) * spacing + 1.0
-            ragdoll(s, [0.0, top, 0.0], world=w)
+            s.add_static(BOX, [2.0, 0.5, 2.0], [half + 10.0, -0.5, 0.0], world=w)
+            ragdoll(s, [half + 10.0, 0.3, 0.0], world=w)
     return s
 
 
