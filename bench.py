@@ -201,6 +201,8 @@ def main():
     ap.add_argument("--exec", default="colors", choices=["flow", "xcd", "colors", "cooperative"],
                     help="how the contact passes launch")
     ap.add_argument("--graph", action="store_true", help="launch each step's substep loop as one HIP graph")
+    ap.add_argument("--tune", default="auto", help="launch shapes: 'auto' (pgx_world_autotune after the warm-up), "
+                    "'default', or a JSON object of pgx_tuning fields")
     ap.add_argument("--seconds", type=float, default=20.0, help="served: length of the request stream")
     ap.add_argument("--admit-interval", type=int, default=8, help="served: steps between admission boundaries")
     ap.add_argument("--copy-calibration", action="store_true", help="run only the 1 GiB copy used to scale PMC counters")
@@ -223,6 +225,11 @@ def main():
     for _ in range(args.warmup):
         w.step(dt)
     w.sync()
+    if args.tune == "auto":
+        chosen = w.autotune(dt, trial_steps=4)   # restores the world's state
+        print(f"autotune chose {chosen}", file=sys.stderr, flush=True)
+    elif args.tune != "default":
+        w.set_tuning(**json.loads(args.tune))
     warm_state = w.state()
     w.enable_timing(not args.graph)
     w.reset_stats()
@@ -231,7 +238,11 @@ def main():
     t0 = time.perf_counter()
     for k in range(args.steps):
         ts = time.perf_counter()
-        w.step(dt)
+        try:
+            w.step(dt)
+        except api.PgxError:
+            print(f"timed step {k + 1} (step {args.warmup + k + 1} of the scene) failed", file=sys.stderr, flush=True)
+            raise
         w.sync()
         lat.append(time.perf_counter() - ts)
         if (k + 1) % 100 == 0:
@@ -255,7 +266,8 @@ def main():
         "ms_per_step": 1000 * el / args.steps, "p50_step_ms": 1000 * float(np.percentile(lat, 50)),
         "p99_step_ms": 1000 * float(np.percentile(lat, 99)), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
-        "config": dict(cfg, parallelism="single GPU", graph=args.graph, last_step_counts=counts,
+        "config": dict(cfg, parallelism="single GPU", graph=args.graph, tuning=w.tuning(), tuned=args.tune,
+                       last_step_counts=counts,
                        phase_us_last_step=w.timing()),
         "roofline": {"bound": "hbm", "kernel": {"flow": "k_solve_flow", "xcd": "k_solve_flow", "colors": "k_solve",
                                                     "cooperative": "k_step_persistent"}[args.exec], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -253,6 +253,28 @@ int pgx_world_get_stats(pgx_world* w, pgx_stats* out);
  * counterpart in the reference, whose substep loop is host code (src/Physecs.cpp:361-534). */
 int pgx_world_set_graph(pgx_world* w, int enabled);
 
+/* Launch shapes of the step. Results are the same for every value; only speed changes.
+ * No counterpart in the reference (its CPU step has no launch shapes). */
+typedef struct pgx_tuning {
+    int solve_block;      /* threads per workgroup of the per-colour contact solve: 64, 128, 256 */
+    int color_batch;      /* a run of two or more consecutive colours of at most this many
+                             manifolds is solved by one single-workgroup launch that steps
+                             through them in order (0: one launch per colour) */
+    int pairs_per_lane;   /* narrowphase pairs per lane: 1, 2, 4 */
+    int color_rounds;     /* least graph-colouring rounds queued before the host reads back
+                             whether colouring finished: 1..1024 */
+} pgx_tuning;
+int pgx_world_set_tuning(pgx_world* w, const pgx_tuning* t);
+int pgx_world_get_tuning(pgx_world* w, pgx_tuning* t);
+/* Autotuner: times trial_steps steps (after one untimed step; the better of two runs)
+ * from the world's current state for each value of one parameter at a time (color_batch,
+ * solve_block, pairs_per_lane, color_rounds, in that order), keeps a value that is at
+ * least 3% faster than the current choice, and restores the
+ * world's state afterwards (poses, velocities, bounds, contact cache, gear angles,
+ * statistics), so the world steps on exactly as it would have. The chosen values stay set
+ * and are written to *chosen (may be NULL). */
+int pgx_world_autotune(pgx_world* w, float dt, int trial_steps, pgx_tuning* chosen);
+
 /* ---------------------------------------------------------------------------------
  * Serving: a request loop that keeps many independent scenes ("requests") resident in one
  * batched device world. pgx_server_submit queues a scene with a horizon (number of steps);
@@ -291,6 +313,9 @@ int pgx_server_step(pgx_server* s, float dt, pgx_server_step_info* info);
 int pgx_server_done(pgx_server* s, int request_id);
 /* the batched world's substep loop as one HIP graph launch per step (pgx_world_set_graph) */
 int pgx_server_set_graph(pgx_server* s, int enabled);
+/* launch shapes of the batched world (pgx_world_set_tuning), e.g. from pgx_world_autotune
+ * on a world of the same request mix */
+int pgx_server_set_tuning(pgx_server* s, const pgx_tuning* t);
 /* Final state of a finished request in its descriptor's order (pos/quat per actor,
  * vel/angvel per dynamic body); releases it. */
 int pgx_server_take_result(pgx_server* s, int request_id, float* pos, float* quat, float* vel, float* angvel);

@@ -27,9 +27,11 @@ EXPORTS = [
     "pgx_world_write_state", "pgx_world_get_counts", "pgx_world_get_bounds", "pgx_world_get_pairs",
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
-    "pgx_world_destroy_joint", "pgx_world_set_graph",
+    "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
+    "pgx_world_autotune",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
+    "pgx_server_set_tuning",
 ]
 
 
@@ -80,6 +82,11 @@ class PgxServerStepInfo(ctypes.Structure):
                 ("footprint_bytes", ctypes.c_double)]
 
 
+class PgxTuning(ctypes.Structure):
+    _fields_ = [("solve_block", ctypes.c_int), ("color_batch", ctypes.c_int), ("pairs_per_lane", ctypes.c_int),
+                ("color_rounds", ctypes.c_int)]
+
+
 class PgxError(RuntimeError):
     pass
 
@@ -118,6 +125,10 @@ def lib():
         L.pgx_world_destroy_joint.argtypes = [vp, ctypes.c_int]
         L.pgx_world_set_graph.argtypes = [vp, ctypes.c_int]
         L.pgx_server_set_graph.argtypes = [vp, ctypes.c_int]
+        L.pgx_world_set_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
+        L.pgx_world_get_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
+        L.pgx_world_autotune.argtypes = [vp, ctypes.c_float, ctypes.c_int, ctypes.POINTER(PgxTuning)]
+        L.pgx_server_set_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
         L.pgx_server_create.argtypes = [ctypes.POINTER(PgxConfig), ctypes.POINTER(PgxServerConfig), ctypes.POINTER(vp)]
         L.pgx_server_destroy.argtypes = [vp]
         L.pgx_server_submit.argtypes = [vp, ctypes.POINTER(PgxSceneDesc), ctypes.POINTER(PgxJointDesc), ctypes.c_int,
@@ -263,6 +274,24 @@ class World:
         """Launch each step's substep loop as one HIP graph (colour mode); same results."""
         check(self.L.pgx_world_set_graph(self.h, 1 if enabled else 0))
 
+    def tuning(self):
+        t = PgxTuning()
+        check(self.L.pgx_world_get_tuning(self.h, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in PgxTuning._fields_}
+
+    def set_tuning(self, **values):
+        """Launch shapes (solve_block, color_batch, pairs_per_lane, color_rounds); the
+        results do not depend on them."""
+        t = PgxTuning(**dict(self.tuning(), **values))
+        check(self.L.pgx_world_set_tuning(self.h, ctypes.byref(t)))
+
+    def autotune(self, dt, trial_steps=4):
+        """pgx_world_autotune: picks the fastest launch shapes from trial steps of this
+        world's current state, which it restores; returns the chosen values."""
+        t = PgxTuning()
+        check(self.L.pgx_world_autotune(self.h, dt, int(trial_steps), ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in PgxTuning._fields_}
+
     def reset_stats(self):
         check(self.L.pgx_world_reset_stats(self.h))
 
@@ -328,6 +357,11 @@ class Server:
     def set_graph(self, enabled=True):
         """Launch the batched world's substep loop as one HIP graph per step."""
         check(self.L.pgx_server_set_graph(self.h, 1 if enabled else 0))
+
+    def set_tuning(self, tuning):
+        """Launch shapes of the batched world, e.g. World.autotune()'s choice."""
+        t = PgxTuning(**tuning)
+        check(self.L.pgx_server_set_tuning(self.h, ctypes.byref(t)))
 
     def step(self, dt):
         info = PgxServerStepInfo()

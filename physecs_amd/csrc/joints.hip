@@ -41,7 +41,7 @@ struct JointArgs {
     float4 *jR0, *jR1;                                        // anchor - com (or anchor), body space
     float4* jRow;
     const float4 *bpos, *bquat, *dcom, *dinvI, *dvel;
-    float4 *tvel, *tang, *pvel, *pang;
+    float4 *bvel, *pvel, *pang;
     const int32_t* order;                                     // joints sorted by colour
     float h;
 };
@@ -363,8 +363,8 @@ __device__ void row_solve_scalar(const JointArgs& a, int j, int r, int flags, bo
     bool ang = flags & JF_ANGULAR;
     float h = a.h;
     f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
-    if (b0 >= 0) { if (!ang) v0 = xyz(a.tvel[b0]); w0 = xyz(a.tang[b0]); }
-    if (b1 >= 0) { if (!ang) v1 = xyz(a.tvel[b1]); w1 = xyz(a.tang[b1]); }
+    if (b0 >= 0) { if (!ang) v0 = xyz(a.bvel[bv(b0, BV_V)]); w0 = xyz(a.bvel[bv(b0, BV_W)]); }
+    if (b1 >= 0) { if (!ang) v1 = xyz(a.bvel[bv(b1, BV_V)]); w1 = xyz(a.bvel[bv(b1, BV_W)]); }
     float4 L = JROW(r, JR_LIN), G0 = JROW(r, JR_ANG0);
     float4 L0 = JROW(r, JR_LIN0T), L1 = JROW(r, JR_LIN1T), T0 = JROW(r, JR_ANG0T), T1 = JROW(r, JR_ANG1T);
     float4 EX = JROW(r, JR_EXTRA);
@@ -401,12 +401,12 @@ __device__ void row_solve_scalar(const JointArgs& a, int j, int r, int flags, bo
     L0.w = lam;
     JROW(r, JR_LIN0T) = L0;
     if (b0 >= 0) {
-        if (!ang) { float4 t = a.tvel[b0]; set_xyz(t, add(xyz(t), mul(lambda, lin0t))); a.tvel[b0] = t; }
-        float4 t = a.tang[b0]; set_xyz(t, add(xyz(t), mul(lambda, ang0t))); a.tang[b0] = t;
+        if (!ang) { float4 t = a.bvel[bv(b0, BV_V)]; set_xyz(t, add(xyz(t), mul(lambda, lin0t))); a.bvel[bv(b0, BV_V)] = t; }
+        float4 t = a.bvel[bv(b0, BV_W)]; set_xyz(t, add(xyz(t), mul(lambda, ang0t))); a.bvel[bv(b0, BV_W)] = t;
     }
     if (b1 >= 0) {
-        if (!ang) { float4 t = a.tvel[b1]; set_xyz(t, sub(xyz(t), mul(lambda, lin1t))); a.tvel[b1] = t; }
-        float4 t = a.tang[b1]; set_xyz(t, sub(xyz(t), mul(lambda, ang1t))); a.tang[b1] = t;
+        if (!ang) { float4 t = a.bvel[bv(b1, BV_V)]; set_xyz(t, sub(xyz(t), mul(lambda, lin1t))); a.bvel[bv(b1, BV_V)] = t; }
+        float4 t = a.bvel[bv(b1, BV_W)]; set_xyz(t, sub(xyz(t), mul(lambda, ang1t))); a.bvel[bv(b1, BV_W)] = t;
     }
 }
 
@@ -434,16 +434,16 @@ __global__ void k_joint_color(JointArgs a, int begin, int end, int phase, int wa
         return;
     }
     float4 tv0 = z, tw0 = z, tv1 = z, tw1 = z;
-    if (b0 >= 0) { tv0 = a.tvel[b0]; tw0 = a.tang[b0]; }
-    if (b1 >= 0) { tv1 = a.tvel[b1]; tw1 = a.tang[b1]; }
+    if (b0 >= 0) { tv0 = a.bvel[bv(b0, BV_V)]; tw0 = a.bvel[bv(b0, BV_W)]; }
+    if (b1 >= 0) { tv1 = a.bvel[bv(b1, BV_V)]; tw1 = a.bvel[bv(b1, BV_W)]; }
     f3 v0 = xyz(tv0), w0 = xyz(tw0), v1 = xyz(tv1), w1 = xyz(tw1);
     for (int f = 0; f < 6; ++f)
         for (int r = 0; r < n; ++r) {
             int fl = row_flags(a, j, r);
             if (fl == kFlagOrder[f]) row_solve_simd(a, j, r, fl, warm != 0, v0, w0, v1, w1);
         }
-    if (b0 >= 0) { set_xyz(tv0, v0); set_xyz(tw0, w0); a.tvel[b0] = tv0; a.tang[b0] = tw0; }
-    if (b1 >= 0) { set_xyz(tv1, v1); set_xyz(tw1, w1); a.tvel[b1] = tv1; a.tang[b1] = tw1; }
+    if (b0 >= 0) { set_xyz(tv0, v0); set_xyz(tw0, w0); a.bvel[bv(b0, BV_V)] = tv0; a.bvel[bv(b0, BV_W)] = tw0; }
+    if (b1 >= 0) { set_xyz(tv1, v1); set_xyz(tw1, w1); a.bvel[bv(b1, BV_V)] = tv1; a.bvel[bv(b1, BV_W)] = tw1; }
 }
 
 // the overflow colour: one thread, flag type by flag type over all its joints (which may
@@ -480,7 +480,7 @@ static JointArgs make_jargs(World& w, float h) {
     a.jPar0 = w.jPar0.p; a.jPar1 = w.jPar1.p; a.jGear = w.jGear.p; a.jGearInit = w.jGearInit.p;
     a.jB0 = w.jB0.p; a.jB1 = w.jB1.p; a.jR0 = w.jR0.p; a.jR1 = w.jR1.p; a.jRow = w.jRow.p;
     a.bpos = w.bpos.p; a.bquat = w.bquat.p; a.dcom = w.dcom.p; a.dinvI = w.dinvI.p; a.dvel = w.dvel.p;
-    a.tvel = w.tvel.p; a.tang = w.tang.p; a.pvel = w.pvel.p; a.pang = w.pang.p;
+    a.bvel = w.bvel.p; a.pvel = w.pvel.p; a.pang = w.pang.p;
     a.order = w.jOrder.p;
     a.h = h;
     return a;

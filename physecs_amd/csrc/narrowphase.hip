@@ -680,56 +680,57 @@ __global__ __launch_bounds__(256) void k_narrowphase(
     int32_t* __restrict__ npNum, float4* __restrict__ npNormal, float4* __restrict__ npPts,
     int32_t* __restrict__ heavy, int32_t* __restrict__ nHeavy, int32_t* __restrict__ meshPairs,
     int32_t* __restrict__ nMesh, int32_t* __restrict__ meshCount) {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nPairs) return;
-    meshCount[p] = 0;
-    uint64_t key = pkeys[p];
-    uint64_t lmask = (1ull << localBits) - 1;
-    int wd = (int)(key >> (2 * localBits));
-    int base = worldColStart[wd];
-    int c0 = colOfLocal[base + (int)((key >> localBits) & lmask)];
-    int c1 = colOfLocal[base + (int)(key & lmask)];
-    int num = 0;
-    // defaultContactFilter: a trigger on either side makes a trigger pair, not a contact
-    // (src/Physecs.cpp:20-23,200-207); joint-connected / setCanCollide pairs are skipped (:209)
-    bool skip = ((cflags[c0] | cflags[c1]) & CF_TRIGGER) != 0;
-    if (!skip && nNonColliding) {
-        uint64_t ek = ((uint64_t)centity[c0] << 32) | centity[c1];
-        skip = in_sorted(nonColliding, nNonColliding, ek);
-    }
-    Mf m;
-    m.numPoints = 0;
-    // pairs with a triangle mesh go to k_narrowphase_mesh, pairs with a convex mesh to
-    // k_narrowphase_convex: appended with one atomic per wave
-    const bool toMesh = !skip && (ctype[c0] == PGX_TRIANGLE_MESH || ctype[c1] == PGX_TRIANGLE_MESH);
-    const bool toHeavy = !skip && !toMesh && (ctype[c0] == PGX_CONVEX_MESH || ctype[c1] == PGX_CONVEX_MESH);
-    if (__ballot(toMesh)) {
-        int i = wave_append(nMesh, toMesh);
-        if (toMesh) meshPairs[i] = p;
-    }
-    if (__ballot(toHeavy)) {
-        int i = wave_append(nHeavy, toHeavy);
-        if (toHeavy) heavy[i] = p;
-    }
-    skip = skip || toMesh || toHeavy;
-    if (!skip) {
-        int b0 = cbody[c0], b1 = cbody[c1];
-        f3 bp0 = xyz(bpos[b0]), bp1 = xyz(bpos[b1]);
-        qf bq0 = q4(bquat[b0]), bq1 = q4(bquat[b1]);
-        f3 p0 = add(bp0, qrot(bq0, xyz(clpos[c0])));
-        qf q0 = qmul(bq0, q4(clquat[c0]));
-        f3 p1 = add(bp1, qrot(bq1, xyz(clpos[c1])));
-        qf q1 = qmul(bq1, q4(clquat[c1]));
-        bool hit = collide(ctype[c0], cparams[c0], p0, q0, ctype[c1], cparams[c1], p1, q1, m);
-        num = hit ? m.numPoints : 0;
-    }
-    npNum[p] = num;
-    if (num) {
-        npNormal[p] = f4(m.normal, 0.f);
-        for (int k = 0; k < 4; ++k) {
-            if (k < num) {
-                npPts[8 * (size_t)p + 2 * k] = f4(m.pts[k].p0, 0.f);
-                npPts[8 * (size_t)p + 2 * k + 1] = f4(m.pts[k].p1, 0.f);
+    // pairs per lane = pairs / (grid size): a lane takes every grid-size'th pair
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nPairs; p += gridDim.x * blockDim.x) {
+        meshCount[p] = 0;
+        uint64_t key = pkeys[p];
+        uint64_t lmask = (1ull << localBits) - 1;
+        int wd = (int)(key >> (2 * localBits));
+        int base = worldColStart[wd];
+        int c0 = colOfLocal[base + (int)((key >> localBits) & lmask)];
+        int c1 = colOfLocal[base + (int)(key & lmask)];
+        int num = 0;
+        // defaultContactFilter: a trigger on either side makes a trigger pair, not a contact
+        // (src/Physecs.cpp:20-23,200-207); joint-connected / setCanCollide pairs are skipped (:209)
+        bool skip = ((cflags[c0] | cflags[c1]) & CF_TRIGGER) != 0;
+        if (!skip && nNonColliding) {
+            uint64_t ek = ((uint64_t)centity[c0] << 32) | centity[c1];
+            skip = in_sorted(nonColliding, nNonColliding, ek);
+        }
+        Mf m;
+        m.numPoints = 0;
+        // pairs with a triangle mesh go to k_narrowphase_mesh, pairs with a convex mesh to
+        // k_narrowphase_convex: appended with one atomic per wave
+        const bool toMesh = !skip && (ctype[c0] == PGX_TRIANGLE_MESH || ctype[c1] == PGX_TRIANGLE_MESH);
+        const bool toHeavy = !skip && !toMesh && (ctype[c0] == PGX_CONVEX_MESH || ctype[c1] == PGX_CONVEX_MESH);
+        if (__ballot(toMesh)) {
+            int i = wave_append(nMesh, toMesh);
+            if (toMesh) meshPairs[i] = p;
+        }
+        if (__ballot(toHeavy)) {
+            int i = wave_append(nHeavy, toHeavy);
+            if (toHeavy) heavy[i] = p;
+        }
+        skip = skip || toMesh || toHeavy;
+        if (!skip) {
+            int b0 = cbody[c0], b1 = cbody[c1];
+            f3 bp0 = xyz(bpos[b0]), bp1 = xyz(bpos[b1]);
+            qf bq0 = q4(bquat[b0]), bq1 = q4(bquat[b1]);
+            f3 p0 = add(bp0, qrot(bq0, xyz(clpos[c0])));
+            qf q0 = qmul(bq0, q4(clquat[c0]));
+            f3 p1 = add(bp1, qrot(bq1, xyz(clpos[c1])));
+            qf q1 = qmul(bq1, q4(clquat[c1]));
+            bool hit = collide(ctype[c0], cparams[c0], p0, q0, ctype[c1], cparams[c1], p1, q1, m);
+            num = hit ? m.numPoints : 0;
+        }
+        npNum[p] = num;
+        if (num) {
+            npNormal[p] = f4(m.normal, 0.f);
+            for (int k = 0; k < 4; ++k) {
+                if (k < num) {
+                    npPts[8 * (size_t)p + 2 * k] = f4(m.pts[k].p0, 0.f);
+                    npPts[8 * (size_t)p + 2 * k + 1] = f4(m.pts[k].p1, 0.f);
+                }
             }
         }
     }
@@ -950,7 +951,8 @@ __global__ __launch_bounds__(64) void k_mesh_emit(MeshArgs A, int32_t* tmWs, int
 
 void np_run(World& w, int nPairs) {
     if (nPairs == 0) return;
-    hipLaunchKernelGGL(k_narrowphase, dim3(div_up(nPairs, 256)), dim3(256), 0, w.stream, nPairs, w.pkeys.p,
+    const int ppl = std::max(1, w.tune.pairsPerLane);
+    hipLaunchKernelGGL(k_narrowphase, dim3(div_up(nPairs, 256 * ppl)), dim3(256), 0, w.stream, nPairs, w.pkeys.p,
                        w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.ctype.p, w.cflags.p,
                        w.cparams.p, w.clpos.p, w.clquat.p, w.centity.p, w.bpos.p, w.bquat.p, w.nonColliding.p,
                        w.nNonColliding, w.npNum.p, w.npNormal.p, w.npPts.p, w.heavy.p, w.counters.p + 8,

@@ -52,6 +52,13 @@ struct DBuf {
 };
 
 constexpr int WAVE = 64;
+// What a contact or joint solve reads and writes for one dynamic body, in one 128-byte
+// record (one cache line): solver v and w (velocityTemp), component v after the velocity
+// integration (.w = inverse mass), component w, and the columns of the world inverse
+// inertia of the substep (massTemp's R * invI * R^T). Bodies are gathered by index, so
+// one line per body instead of one per array.
+enum { BV_V = 0, BV_W = 1, BV_CV = 2, BV_CW = 3, BV_I0 = 4, BV_I1 = 5, BV_I2 = 6, BV_WORDS = 8 };
+__host__ __device__ __forceinline__ int bv(int body, int field) { return BV_WORDS * body + field; }
 constexpr int JOINT_MAX_ROWS = 7;      // a prismatic joint with its limit and drive rows
 // substep-loop execution: one launch per colour and pass; one cooperative launch per step
 // (grid barriers between colours); one dependency-driven launch per pass
@@ -121,8 +128,7 @@ struct World {
     DBuf<float4> dvel, dang;             // D component velocity (dvel.w = invMass)
     DBuf<float4> dcom;                   // D local com (w = kinematic ? 1 : 0)
     DBuf<float4> dinvI;                  // 3 x D columns of the local inverse inertia
-    DBuf<float4> cvel, cang;             // D component velocity after gravity (cvel.w = invMass)
-    DBuf<float4> tvel, tang;             // D solver velocities (velocityTemp)
+    DBuf<float4> bvel;                   // BV_WORDS per dynamic body (record layout below)
     DBuf<float4> pvel, pang;             // D pseudo velocities (pang.w = constraint count)
     DBuf<float4> winvI;                  // 3 x D world inverse inertia (massTemp)
 
@@ -185,7 +191,8 @@ struct World {
     DBuf<float4> sN;                     // normal, friction
     DBuf<float2> sSoft;                  // frequency, damping
     DBuf<float4> sR0, sR1;               // 4 x M: r0 local + targetVelocity, r1 local
-    DBuf<float4> sRow;                   // 9 x 4 x M per-substep rows
+    DBuf<float4> sRow;                   // 5 x 4 x M per-substep rows
+    DBuf<float4> sLam;                   // 4 x M: lambda N, lambda T, target velocity
 
     // contact cache (canonical order, double-buffered)
     DBuf<uint64_t> cacheKey[2];
@@ -245,6 +252,14 @@ struct World {
     std::vector<int> colorStartHost;
     bool timing = false;
     int execMode = EXEC_COLORS;          // how the substep loop's contact passes are launched
+    // launch shapes, picked by pgx_world_autotune or set by the caller; results do not
+    // depend on them
+    struct Tuning {
+        int solveBlock = 64;             // threads per k_solve workgroup (64, 128, 256)
+        int colorBatch = 0;              // runs of colours of at most this many manifolds: one launch
+        int pairsPerLane = 1;            // narrowphase pairs per lane (1, 2, 4)
+        int colorRounds = 8;             // least colouring rounds queued per read-back
+    } tune;
     bool graph = false;                  // substep loop as one HIP graph launch (colour mode)
     hipGraphExec_t subExec = nullptr;    // the last instantiated substep-loop graph
     long graphInstantiations = 0, graphUpdates = 0;
@@ -296,6 +311,7 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc);     
 void world_add_joints(World& w, int n, const pgx_joint_desc* js);
 void world_step(World& w, float dt);
 void world_release_graph(World& w);
+void world_autotune(World& w, float dt, int trialSteps);
 int report_error(int code, const std::string& msg);   // sets pgx_last_error(), returns code
 
 // solver.hip

@@ -284,8 +284,14 @@ void sv_color_points(World& w, int nColors) {
 
 // ------------------------------------------------------------------ substep pieces
 // per-point row storage: field f of point k of manifold s at ((k * 9 + f) * Mcap + s)
-enum { R_R0XN = 0, R_R1XN, R_T, R_R0XT, R_R1XT, R_R0XNT, R_R1XNT, R_R0XTT, R_R1XTT, R_NF };
+// Contact rows of point k of manifold slot s: r0 x n (.w = c), r1 x n (.w = inverse
+// effective mass N), tangent (.w = inverse effective mass T), r0 x t, r1 x t. The rows
+// times the bodies' world inverse inertia are recomputed in the solve from the body
+// records, with the operations that produced them here. Per point also (lambda N,
+// lambda T, target velocity) in sLam.
+enum { R_R0XN = 0, R_R1XN, R_T, R_R0XT, R_R1XT, R_NF };
 #define ROW(f, k) a.sRow[((size_t)((k) * R_NF + (f))) * a.Mcap + s]
+#define LAM(k) a.sLam[(size_t)(k) * a.Mcap + s]
 
 struct SubArgs {
     int M, Mcap, D, nColors, substeps, iterations;
@@ -293,12 +299,12 @@ struct SubArgs {
     const int32_t *sB0, *sB1, *sA0, *sA1, *sFlags, *colorStart;
     const float4 *sN, *sR0, *sR1;
     const float2* sSoft;
-    float4 *sRow;
+    float4 *sRow, *sLam;
     float4 *bpos, *bquat;                 // body poses (dynamic bodies come first)
     float4 *dvel, *dang;                  // component velocity at the start of the substep (dvel.w = invMass)
-    float4 *cvel, *cang;                  // component velocity after gravity + gyroscopic term (friction reads it)
+    float4* bvel;                         // per dynamic body: solver v, w (velocityTemp), component
+                                          // v, w after gravity + gyroscopic term (friction reads them)
     const float4 *dcom, *dinvI;
-    float4 *tvel, *tang;                  // solver velocities (velocityTemp)
     float4 *pvel, *pang;                  // pseudo velocities (pang.w = constraint count)
     // dependency-driven passes: per body, the colours of its manifolds (2 x 64 bits); a work
     // ticket per pass; a flag set if a wait ran out (never expected)
@@ -355,12 +361,9 @@ __device__ void prepare_manifold(const SubArgs& a, int s) {
         ROW(R_R0XN, k) = f4(r0xn, cn);
         ROW(R_R1XN, k) = f4(r1xn, invN);
         ROW(R_T, k) = f4(t, invT);
-        ROW(R_R0XT, k) = f4(r0xt, 0.f);     // .w = lambdaN
-        ROW(R_R1XT, k) = f4(r1xt, 0.f);     // .w = lambdaT
-        ROW(R_R0XNT, k) = f4(r0xnt, 0.f);
-        ROW(R_R1XNT, k) = f4(r1xnt, 0.f);
-        ROW(R_R0XTT, k) = f4(r0xtt, 0.f);
-        ROW(R_R1XTT, k) = f4(r1xtt, 0.f);
+        ROW(R_R0XT, k) = f4(r0xt, 0.f);
+        ROW(R_R1XT, k) = f4(r1xt, 0.f);
+        LAM(k) = make_float4(0.f, 0.f, a.sR0[(size_t)k * a.Mcap + s].w, 0.f);   // lambdas start at 0
     }
 }
 
@@ -388,10 +391,14 @@ __device__ void integrate_velocity(const SubArgs& a, int d) {
     m3 J = madd(I, mscale(a.h, msub(mulmm(matrixCross3(wl), I), matrixCross3(Iw))));
     wl = sub(wl, solve33(J, f));
     f3 w = mulmv(rot, wl);
-    a.cvel[d] = f4(v, vv.w);
-    a.cang[d] = f4(w, 0.f);
-    a.tvel[d] = f4(v, 0.f);
-    a.tang[d] = f4(w, 0.f);
+    a.bvel[bv(d, BV_CV)] = f4(v, vv.w);
+    a.bvel[bv(d, BV_CW)] = f4(w, 0.f);
+    a.bvel[bv(d, BV_V)] = f4(v, 0.f);
+    a.bvel[bv(d, BV_W)] = f4(w, 0.f);
+    const m3 wI = world_inv_inertia(a, d);   // the inertia prepare_manifold uses this substep
+    a.bvel[bv(d, BV_I0)] = f4(wI.c0, 0.f);
+    a.bvel[bv(d, BV_I1)] = f4(wI.c1, 0.f);
+    a.bvel[bv(d, BV_I2)] = f4(wI.c2, 0.f);
     a.pvel[d] = make_float4(0.f, 0.f, 0.f, 0.f);
     a.pang[d] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
@@ -408,10 +415,14 @@ struct MInputs {
     f3 n;
     float friction, im0, im1;
     f3 cv0, cw0, cv1, cw1;
+    m3 I0, I1;                            // the bodies' world inverse inertia
     float2 sp;
-    float4 rA[4], rB[4], rT[4], rC[4], rE[4], rNT0[4], rNT1[4], rTT0[4], rTT1[4];
-    float tvk[4];
+    float4 rA[4], rB[4], rT[4], rC[4], rE[4], lam[4];
 };
+
+__device__ __forceinline__ m3 body_inertia(const SubArgs& a, int b) {
+    return mk_m3(xyz(a.bvel[bv(b, BV_I0)]), xyz(a.bvel[bv(b, BV_I1)]), xyz(a.bvel[bv(b, BV_I2)]));
+}
 
 __device__ __forceinline__ void load_inputs(const SubArgs& a, int s, MInputs& in) {
     int flags = a.sFlags[s];
@@ -424,8 +435,17 @@ __device__ __forceinline__ void load_inputs(const SubArgs& a, int s, MInputs& in
     in.friction = nf.w;
     in.im0 = 0.f; in.im1 = 0.f;
     in.cv0 = s3(0.f); in.cw0 = s3(0.f); in.cv1 = s3(0.f); in.cw1 = s3(0.f);
-    if (in.b0 >= 0) { float4 c = a.cvel[in.b0]; in.im0 = c.w; in.cv0 = xyz(c); in.cw0 = xyz(a.cang[in.b0]); }
-    if (in.b1 >= 0) { float4 c = a.cvel[in.b1]; in.im1 = c.w; in.cv1 = xyz(c); in.cw1 = xyz(a.cang[in.b1]); }
+    in.I0 = mzero(); in.I1 = mzero();
+    if (in.b0 >= 0) {
+        float4 c = a.bvel[bv(in.b0, BV_CV)];
+        in.im0 = c.w; in.cv0 = xyz(c); in.cw0 = xyz(a.bvel[bv(in.b0, BV_CW)]);
+        in.I0 = body_inertia(a, in.b0);
+    }
+    if (in.b1 >= 0) {
+        float4 c = a.bvel[bv(in.b1, BV_CV)];
+        in.im1 = c.w; in.cv1 = xyz(c); in.cw1 = xyz(a.bvel[bv(in.b1, BV_CW)]);
+        in.I1 = body_inertia(a, in.b1);
+    }
     in.sp = a.sSoft[s];
     // all rows of the manifold are loaded before any dependent arithmetic (memory-level
     // parallelism); unused point slots are never read back
@@ -434,9 +454,7 @@ __device__ __forceinline__ void load_inputs(const SubArgs& a, int s, MInputs& in
         if (k < in.num) {
             in.rA[k] = ROW(R_R0XN, k); in.rB[k] = ROW(R_R1XN, k); in.rT[k] = ROW(R_T, k);
             in.rC[k] = ROW(R_R0XT, k); in.rE[k] = ROW(R_R1XT, k);
-            in.rNT0[k] = ROW(R_R0XNT, k); in.rNT1[k] = ROW(R_R1XNT, k);
-            in.rTT0[k] = ROW(R_R0XTT, k); in.rTT1[k] = ROW(R_R1XTT, k);
-            in.tvk[k] = a.sR0[(size_t)k * a.Mcap + s].w;
+            in.lam[k] = LAM(k);
         }
     }
 }
@@ -454,7 +472,7 @@ __device__ __forceinline__ void solve_core(const SubArgs& a, int s, const MInput
     for (int k = 0; k < 4; ++k) {
         if (k >= num) break;
         float4 r0 = in.rA[k], r1 = in.rB[k];
-        lN[k] = in.rC[k].w;
+        lN[k] = in.lam[k].x;
         float invN = r1.w;
         if (invN == 0.f) continue;
         f3 r0xn = xyz(r0), r1xn = xyz(r1);
@@ -470,21 +488,21 @@ __device__ __forceinline__ void solve_core(const SubArgs& a, int s, const MInput
             float beta = h * stiffness / (damping + h * stiffness);
             lambda = (rv + beta * c / h) / (invN + gamma / h);
         } else {
-            lambda = (rv - in.tvk[k] + (useBias ? 0.1f * c / h : 0.f)) * effMass;
+            lambda = (rv - in.lam[k].z + (useBias ? 0.1f * c / h : 0.f)) * effMass;
         }
         float prev = lN[k];
         lN[k] = lN[k] + lambda;
         lN[k] = gmin(lN[k], 0.f);
         lambda = lN[k] - prev;
-        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, n)); w0 = add(w0, mul(lambda, xyz(in.rNT0[k]))); }
-        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, n)); w1 = sub(w1, mul(lambda, xyz(in.rNT1[k]))); }
+        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, n)); w0 = add(w0, mul(lambda, mulmv(in.I0, r0xn))); }
+        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, n)); w1 = sub(w1, mul(lambda, mulmv(in.I1, r1xn))); }
     }
     // friction rows read the component velocities (src/ContactConstraints.cpp:142-152)
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= num) break;
         float4 t4 = in.rT[k];
-        lT[k] = in.rE[k].w;
+        lT[k] = in.lam[k].y;
         float invT = t4.w;
         if (invT == 0.f) continue;
         f3 t = xyz(t4), r0xt = xyz(in.rC[k]), r1xt = xyz(in.rE[k]);
@@ -495,17 +513,13 @@ __device__ __forceinline__ void solve_core(const SubArgs& a, int s, const MInput
         lT[k] = lT[k] + lambda;
         lT[k] = gclamp(lT[k], limit, -limit);
         lambda = lT[k] - prev;
-        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, t)); w0 = add(w0, mul(lambda, xyz(in.rTT0[k]))); }
-        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, t)); w1 = sub(w1, mul(lambda, xyz(in.rTT1[k]))); }
+        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, t)); w0 = add(w0, mul(lambda, mulmv(in.I0, r0xt))); }
+        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, t)); w1 = sub(w1, mul(lambda, mulmv(in.I1, r1xt))); }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= num) break;
-        float4 c = in.rC[k], e = in.rE[k];
-        c.w = lN[k];
-        e.w = lT[k];
-        ROW(R_R0XT, k) = c;
-        ROW(R_R1XT, k) = e;
+        *reinterpret_cast<float2*>(&LAM(k)) = make_float2(lN[k], lT[k]);
     }
 }
 
@@ -516,11 +530,11 @@ __device__ void solve_manifold(const SubArgs& a, int s, bool useBias, float h, b
     load_inputs(a, s, in);
     if (skipSoft && in.soft) return;
     f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
-    if (in.b0 >= 0) { v0 = xyz(a.tvel[in.b0]); w0 = xyz(a.tang[in.b0]); }
-    if (in.b1 >= 0) { v1 = xyz(a.tvel[in.b1]); w1 = xyz(a.tang[in.b1]); }
+    if (in.b0 >= 0) { v0 = xyz(a.bvel[bv(in.b0, BV_V)]); w0 = xyz(a.bvel[bv(in.b0, BV_W)]); }
+    if (in.b1 >= 0) { v1 = xyz(a.bvel[bv(in.b1, BV_V)]); w1 = xyz(a.bvel[bv(in.b1, BV_W)]); }
     solve_core(a, s, in, useBias, h, v0, w0, v1, w1);
-    if (in.b0 >= 0) { a.tvel[in.b0] = f4(v0, 0.f); a.tang[in.b0] = f4(w0, 0.f); }
-    if (in.b1 >= 0) { a.tvel[in.b1] = f4(v1, 0.f); a.tang[in.b1] = f4(w1, 0.f); }
+    if (in.b0 >= 0) { a.bvel[bv(in.b0, BV_V)] = f4(v0, 0.f); a.bvel[bv(in.b0, BV_W)] = f4(w0, 0.f); }
+    if (in.b1 >= 0) { a.bvel[bv(in.b1, BV_V)] = f4(v1, 0.f); a.bvel[bv(in.b1, BV_W)] = f4(w1, 0.f); }
 }
 
 // integrate positions of one body, src/Physecs.cpp:494-511
@@ -533,9 +547,9 @@ __device__ void integrate_position(const SubArgs& a, int d) {
     f3 com = xyz(cm);
     f3 pos = xyz(a.bpos[d]);
     qf q = q4(a.bquat[d]);
-    pos = add(pos, add(mul(a.h, xyz(a.tvel[d])), mul(sc, xyz(a.pvel[d]))));
+    pos = add(pos, add(mul(a.h, xyz(a.bvel[bv(d, BV_V)])), mul(sc, xyz(a.pvel[d]))));
     f3 prevCom = qrot(q, com);
-    f3 hv = mul(0.5f, add(mul(a.h, xyz(a.tang[d])), mul(sc, xyz(pa))));
+    f3 hv = mul(0.5f, add(mul(a.h, xyz(a.bvel[bv(d, BV_W)])), mul(sc, xyz(pa))));
     qf dq = qmul(mkq(0.f, hv.x, hv.y, hv.z), q);
     q = mkq(q.w + dq.w, q.x + dq.x, q.y + dq.y, q.z + dq.z);
     q = qnormalize(q);
@@ -547,9 +561,9 @@ __device__ void integrate_position(const SubArgs& a, int d) {
 // write back, src/Physecs.cpp:523-530
 __device__ void write_back(const SubArgs& a, int d) {
     if (a.dcom[d].w != 0.f) return;
-    float4 v = a.tvel[d];
+    float4 v = a.bvel[bv(d, BV_V)];
     a.dvel[d] = make_float4(v.x, v.y, v.z, a.dvel[d].w);
-    a.dang[d] = a.tang[d];
+    a.dang[d] = a.bvel[bv(d, BV_W)];
 }
 
 // ------------------------------------------------------------------ discrete kernels
@@ -560,9 +574,24 @@ __global__ __launch_bounds__(256) void k_prepare(SubArgs a, int sub) {
     if (i < a.M) prepare_manifold(a, i);
     else if (i - a.M < a.D) integrate_velocity(a, i - a.M);
 }
-__global__ __launch_bounds__(64) void k_solve(SubArgs a, int begin, int end, int useBias, float h, int skipSoft) {
-    int s = begin + blockIdx.x * blockDim.x + threadIdx.x;
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_solve(SubArgs a, int begin, int end, int useBias, float h, int skipSoft) {
+    int s = begin + blockIdx.x * BLOCK + threadIdx.x;
     if (s < end) solve_manifold(a, s, useBias != 0, h, skipSoft != 0);
+}
+
+// Consecutive small colours c0..c1-1 in one launch of one workgroup: the colours are solved
+// in order with a workgroup barrier between them, so the solve order is the colour order,
+// exactly as with one launch per colour. The barrier makes one colour's global stores
+// visible to every thread of the workgroup before the next colour reads them.
+constexpr int BATCH_THREADS = 512;   // k_solve's register footprint allows 2 waves per SIMD
+__global__ __launch_bounds__(BATCH_THREADS) void k_solve_batch(SubArgs a, int c0, int c1, int useBias, float h,
+                                                               int skipSoft) {
+    for (int c = c0; c < c1; ++c) {
+        const int b = a.colorStart[c], e = a.colorStart[c + 1];
+        for (int s = b + (int)threadIdx.x; s < e; s += BATCH_THREADS) solve_manifold(a, s, useBias != 0, h, skipSoft != 0);
+        __syncthreads();
+    }
 }
 
 // colours of body b, and its colours below colour c
@@ -636,8 +665,8 @@ __device__ __forceinline__ bool read_pub(const FlowArgs& f, int b, uint32_t ver,
 __device__ __forceinline__ void hand_on(const SubArgs& a, const FlowArgs& f, int b, bool last, uint32_t ver, f3 v,
                                         f3 w) {
     if (last) {
-        a.tvel[b] = f4(v, 0.f);
-        a.tang[b] = f4(w, 0.f);
+        a.bvel[bv(b, BV_V)] = f4(v, 0.f);
+        a.bvel[bv(b, BV_W)] = f4(w, 0.f);
         return;
     }
     uint64_t* p = f.pub + 6 * (size_t)b;
@@ -682,13 +711,13 @@ __global__ __launch_bounds__(256) void k_solve_flow(SubArgs a, FlowArgs f) {
                 r0 = colours_below(a.used, in.b0, c);
                 last0 = r0 == colours_of(a.used, in.b0) - 1;
                 have0 = r0 == 0;
-                if (have0) { v0 = xyz(a.tvel[in.b0]); w0 = xyz(a.tang[in.b0]); }
+                if (have0) { v0 = xyz(a.bvel[bv(in.b0, BV_V)]); w0 = xyz(a.bvel[bv(in.b0, BV_W)]); }
             }
             if (in.b1 >= 0) {
                 r1 = colours_below(a.used, in.b1, c);
                 last1 = r1 == colours_of(a.used, in.b1) - 1;
                 have1 = r1 == 0;
-                if (have1) { v1 = xyz(a.tvel[in.b1]); w1 = xyz(a.tang[in.b1]); }
+                if (have1) { v1 = xyz(a.bvel[bv(in.b1, BV_V)]); w1 = xyz(a.bvel[bv(in.b1, BV_W)]); }
             }
         }
         int spins = 0;
@@ -858,7 +887,7 @@ int sv_color(World& w, int M) {
     // and one read-back tells whether the batch finished, with the counts and starts. Only
     // a batch that ran short is followed by another (and the sort is redone).
     int round = 0;
-    int batch = std::min(MAX_ROUNDS, std::max(8, prevRounds + 4));
+    int batch = std::min(MAX_ROUNDS, std::max(w.tune.colorRounds, prevRounds + 4));
     std::vector<int32_t> starts(130, 0), uc;
     uint64_t lastKey = 0;
     while (true) {
@@ -916,10 +945,10 @@ static SubArgs make_args(World& w, int M, float dt) {
     a.h = dt / w.substeps; a.g = w.g;
     a.sB0 = w.sB0.p; a.sB1 = w.sB1.p; a.sA0 = w.sA0.p; a.sA1 = w.sA1.p; a.sFlags = w.sFlags.p;
     a.colorStart = w.colorCount.p;
-    a.sN = w.sN.p; a.sR0 = w.sR0.p; a.sR1 = w.sR1.p; a.sSoft = w.sSoft.p; a.sRow = w.sRow.p;
+    a.sN = w.sN.p; a.sR0 = w.sR0.p; a.sR1 = w.sR1.p; a.sSoft = w.sSoft.p; a.sRow = w.sRow.p; a.sLam = w.sLam.p;
     a.bpos = w.bpos.p; a.bquat = w.bquat.p; a.dvel = w.dvel.p; a.dang = w.dang.p;
-    a.cvel = w.cvel.p; a.cang = w.cang.p; a.dcom = w.dcom.p; a.dinvI = w.dinvI.p;
-    a.tvel = w.tvel.p; a.tang = w.tang.p; a.pvel = w.pvel.p; a.pang = w.pang.p;
+    a.bvel = w.bvel.p; a.dcom = w.dcom.p; a.dinvI = w.dinvI.p;
+    a.pvel = w.pvel.p; a.pang = w.pang.p;
     a.used = w.usedMask.p; a.tickets = w.tickets.p; a.flowErr = w.flowErr.p; a.qDone = w.qDone.p;
     return a;
 }
@@ -951,6 +980,20 @@ static void solve_events(World& w, int color, hipEvent_t* start, hipEvent_t* sto
     if ((int)w.solveEvColor.size() <= w.solveEvUsed) w.solveEvColor.resize(w.solveEvUsed + 1);
     w.solveEvColor[w.solveEvUsed] = color;
     ++w.solveEvUsed;
+}
+
+// one colour [b, e) in workgroups of `block` threads; with events, stamped from the
+// launch's own dispatch packet
+static void launch_solve(int block, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const SubArgs& a, int b, int e,
+                         int useBias, float h, int skipSoft) {
+    auto go = [&](auto kernel, int bs) {
+        if (e0) hipExtLaunchKernelGGL(kernel, dim3(div_up(e - b, bs)), dim3(bs), 0, s, e0, e1, 0, a, b, e, useBias, h,
+                                      skipSoft);
+        else hipLaunchKernelGGL(kernel, dim3(div_up(e - b, bs)), dim3(bs), 0, s, a, b, e, useBias, h, skipSoft);
+    };
+    if (block == 256) go(k_solve<256>, 256);
+    else if (block == 128) go(k_solve<128>, 128);
+    else go(k_solve<64>, 64);
 }
 
 void sv_substeps(World& w, int M, float dt) {
@@ -993,6 +1036,22 @@ void sv_substeps(World& w, int M, float dt) {
     // waves of the dependency-driven pass: enough to hold every manifold at once, at most
     // what the chip keeps resident comfortably; on XCD queues, 64 workgroups per XCD
     const int flowBlocks = f.xcdScope ? 64 * w.nXcd : std::max(1, std::min(div_up(std::max(M, 1), 256), 2048));
+    // launch groups of the colour passes: (first colour, end colour, batched)
+    std::vector<int3> groups;
+    for (int c = 0; c < nColors;) {
+        const int n = w.colorStartHost[c + 1] - w.colorStartHost[c];
+        if (w.tune.colorBatch > 0 && n <= w.tune.colorBatch) {
+            int c1 = c + 1;
+            while (c1 < nColors && w.colorStartHost[c1 + 1] - w.colorStartHost[c1] <= w.tune.colorBatch) ++c1;
+            if (c1 - c >= 2) {
+                groups.push_back(make_int3(c, c1, 1));
+                c = c1;
+                continue;
+            }
+        }
+        if (n > 0) groups.push_back(make_int3(c, c + 1, 0));
+        ++c;
+    }
     int passIndex = 0;
     auto pass = [&](bool rec, int useBias, float h, int skipSoft) {
         const int p = passIndex++;
@@ -1009,18 +1068,17 @@ void sv_substeps(World& w, int M, float dt) {
             }
             return;
         }
-        // one launch per colour: a colour's manifolds share no dynamic body
-        for (int c = 0; c < nColors; ++c) {
-            int b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
-            if (e <= b) continue;
-            if (rec) {
-                hipEvent_t e0, e1;
-                solve_events(w, c, &e0, &e1);
-                hipExtLaunchKernelGGL(k_solve, dim3(div_up(e - b, 64)), dim3(64), 0, s, e0, e1, 0, a, b, e, useBias,
-                                      h, skipSoft);
-            } else {
-                hipLaunchKernelGGL(k_solve, dim3(div_up(e - b, 64)), dim3(64), 0, s, a, b, e, useBias, h, skipSoft);
+        // one launch per colour (a colour's manifolds share no dynamic body), runs of small
+        // colours in one single-workgroup launch
+        for (const int3& g : groups) {
+            if (g.z) {
+                hipLaunchKernelGGL(k_solve_batch, dim3(1), dim3(BATCH_THREADS), 0, s, a, g.x, g.y, useBias, h, skipSoft);
+                continue;
             }
+            const int c = g.x, b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (rec) solve_events(w, c, &e0, &e1);
+            launch_solve(w.tune.solveBlock, s, e0, e1, a, b, e, useBias, h, skipSoft);
         }
     };
     jt_step_setup(w);

@@ -1,6 +1,7 @@
 // C ABI of libphysecs_gpu.so (include/physecs_gpu.h): world lifetime, upload/download of
 // the component state, and the per-step orchestration of Scene::simulate
 // (src/Physecs.cpp:112-561) on one HIP stream.
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -239,7 +240,7 @@ void world_set_shape(World& w, const pgx_config* cfg, int A, int D, int C, int W
 void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh) {
     const int D = w.D, C = w.C, W = w.W;
     w.refitReady.alloc(std::max(1, w.nDynCol));
-    w.cvel.alloc(D); w.cang.alloc(D); w.tvel.alloc(D); w.tang.alloc(D); w.pvel.alloc(D); w.pang.alloc(D); w.winvI.alloc(3 * (size_t)D);
+    w.bvel.alloc(BV_WORDS * (size_t)D); w.pvel.alloc(D); w.pang.alloc(D); w.winvI.alloc(3 * (size_t)D);
     w.cbmin.alloc(C); w.cbmax.alloc(C);
     // broadphase buffers
     w.worldBoundsI.alloc(8);
@@ -280,7 +281,7 @@ void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh) {
 
     w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorCount.alloc(160); w.colorPoints.alloc(160);
     w.sB0.alloc(M); w.sB1.alloc(M); w.sFlags.alloc(M); w.sA0.alloc(M); w.sA1.alloc(M); w.sCanon.alloc(M);
-    w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(36 * M);
+    w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(20 * M); w.sLam.alloc(4 * M);
     for (int k = 0; k < 2; ++k) {
         w.cacheKey[k].alloc(M); w.cacheTri[k].alloc(M); w.cacheNum[k].alloc(M); w.cacheLp[k].alloc(4 * M);
     }
@@ -397,10 +398,10 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
 }
 
 static void release_world(World& w) {
-    DBuf<float4>* f4s[] = {&w.bpos, &w.bquat, &w.dvel, &w.dang, &w.dcom, &w.dinvI, &w.cvel, &w.cang, &w.tvel, &w.tang, &w.pvel,
+    DBuf<float4>* f4s[] = {&w.bpos, &w.bquat, &w.dvel, &w.dang, &w.dcom, &w.dinvI, &w.bvel, &w.pvel,
                            &w.pang, &w.winvI, &w.cparams, &w.clpos, &w.clquat, &w.cmat, &w.cbmin, &w.cbmax,
                            &w.leafMin, &w.leafMax, &w.nodeMin, &w.nodeMax, &w.npNormal, &w.npPts, &w.sN, &w.sR0,
-                           &w.sR1, &w.sRow, &w.cacheLp[0], &w.cacheLp[1]};
+                           &w.sR1, &w.sRow, &w.sLam, &w.cacheLp[0], &w.cacheLp[1]};
     for (auto* b : f4s) b->release();
     DBuf<int32_t>* i32s[] = {&w.cbody, &w.cworld, &w.clocal, &w.cflags, &w.ctype, &w.cindex, &w.worldColStart,
                              &w.colOfLocal, &w.mvals, &w.leafFlags, &w.leafParent, &w.leafEsc, &w.nodeChild,
@@ -638,6 +639,113 @@ int pgx_world_destroy_joint(pgx_world* h, int index) {
 }  // extern "C"
 
 namespace pgx {
+
+// Everything a step carries into the next: body poses and velocities, collider bounds
+// (the margin rule), the contact cache of the last step, gear joint angles, and the host
+// counters and statistics. Autotuning steps the world from a snapshot and restores it.
+struct Snapshot {
+    DBuf<float4> pos, quat, vel, ang, bmin, bmax, lp, gear;
+    DBuf<uint64_t> key;
+    DBuf<int32_t> tri, num, gearInit;
+    int cacheCur = 0, cacheCount = 0, lastRounds = 0, lastPairs = 0, lastManifolds = 0, lastColors = 0;
+    int lastMeshSlots = 0;
+    long statSteps = 0, statSolveLaunches = 0, lastPointsHost = 0;
+    double statSolveUs = 0, statSolveBytes = 0, statStepUs = 0;
+};
+
+template <class T>
+static void d2d(T* dst, const T* src, size_t n, hipStream_t s) {
+    if (n) PGX_HIP(hipMemcpyAsync(dst, src, sizeof(T) * n, hipMemcpyDeviceToDevice, s));
+}
+
+static void take_snapshot(World& w, Snapshot& z) {
+    hipStream_t s = w.stream;
+    const int cur = w.cacheCur, n = w.cacheCount[cur];
+    z.pos.alloc(w.A); z.quat.alloc(w.A); z.vel.alloc(w.D); z.ang.alloc(w.D); z.bmin.alloc(w.C); z.bmax.alloc(w.C);
+    d2d(z.pos.p, w.bpos.p, w.A, s); d2d(z.quat.p, w.bquat.p, w.A, s);
+    d2d(z.vel.p, w.dvel.p, w.D, s); d2d(z.ang.p, w.dang.p, w.D, s);
+    d2d(z.bmin.p, w.cbmin.p, w.C, s); d2d(z.bmax.p, w.cbmax.p, w.C, s);
+    z.key.alloc(n); z.tri.alloc(n); z.num.alloc(n); z.lp.alloc(4 * (size_t)n);
+    d2d(z.key.p, w.cacheKey[cur].p, n, s); d2d(z.tri.p, w.cacheTri[cur].p, n, s); d2d(z.num.p, w.cacheNum[cur].p, n, s);
+    for (int j = 0; j < 4; ++j) d2d(z.lp.p + (size_t)j * n, w.cacheLp[cur].p + (size_t)j * w.manifoldCap, n, s);
+    if (w.J > 0) {
+        z.gear.alloc(w.J); z.gearInit.alloc(w.J);
+        d2d(z.gear.p, w.jGear.p, w.J, s); d2d(z.gearInit.p, w.jGearInit.p, w.J, s);
+    }
+    z.cacheCur = cur; z.cacheCount = n;
+    z.lastRounds = w.lastRounds; z.lastPairs = w.lastPairs; z.lastManifolds = w.lastManifolds;
+    z.lastColors = w.lastColors; z.lastMeshSlots = w.lastMeshSlots; z.lastPointsHost = w.lastPointsHost;
+    z.statSteps = w.statSteps; z.statSolveLaunches = w.statSolveLaunches; z.statSolveUs = w.statSolveUs;
+    z.statSolveBytes = w.statSolveBytes; z.statStepUs = w.statStepUs;
+    PGX_HIP(hipStreamSynchronize(s));
+}
+
+static void restore_snapshot(World& w, const Snapshot& z) {
+    hipStream_t s = w.stream;
+    const int cur = z.cacheCur, n = z.cacheCount;
+    d2d(w.bpos.p, z.pos.p, w.A, s); d2d(w.bquat.p, z.quat.p, w.A, s);
+    d2d(w.dvel.p, z.vel.p, w.D, s); d2d(w.dang.p, z.ang.p, w.D, s);
+    d2d(w.cbmin.p, z.bmin.p, w.C, s); d2d(w.cbmax.p, z.bmax.p, w.C, s);
+    d2d(w.cacheKey[cur].p, z.key.p, n, s); d2d(w.cacheTri[cur].p, z.tri.p, n, s); d2d(w.cacheNum[cur].p, z.num.p, n, s);
+    for (int j = 0; j < 4; ++j) d2d(w.cacheLp[cur].p + (size_t)j * w.manifoldCap, z.lp.p + (size_t)j * n, n, s);
+    if (w.J > 0) { d2d(w.jGear.p, z.gear.p, w.J, s); d2d(w.jGearInit.p, z.gearInit.p, w.J, s); }
+    w.cacheCur = cur; w.cacheCount[cur] = n;
+    w.lastRounds = z.lastRounds; w.lastPairs = z.lastPairs; w.lastManifolds = z.lastManifolds;
+    w.lastColors = z.lastColors; w.lastMeshSlots = z.lastMeshSlots; w.lastPointsHost = z.lastPointsHost;
+    w.statSteps = z.statSteps; w.statSolveLaunches = z.statSolveLaunches; w.statSolveUs = z.statSolveUs;
+    w.statSolveBytes = z.statSolveBytes; w.statStepUs = z.statStepUs;
+    PGX_HIP(hipStreamSynchronize(s));
+}
+
+// Picks the launch shapes by timing trial steps from the world's current state (one
+// untimed step, then trial_steps timed, wall clock with the stream drained; the better of
+// two such runs), one parameter at a time, in the order below. A value replaces the
+// current one only when it is at least 3% faster, so timing noise does not move the
+// choice. The state is restored afterwards and between trials, so every trial steps the
+// same scene. The results never depend on these parameters.
+void world_autotune(World& w, float dt, int trialSteps) {
+    Snapshot z;
+    take_snapshot(w, z);
+    const bool timing = w.timing;
+    w.timing = false;
+    auto trial = [&]() {
+        double best = 1e30;
+        for (int rep = 0; rep < 2; ++rep) {
+            restore_snapshot(w, z);
+            world_step(w, dt);
+            PGX_HIP(hipStreamSynchronize(w.stream));
+            auto t0 = std::chrono::steady_clock::now();
+            for (int k = 0; k < trialSteps; ++k) world_step(w, dt);
+            PGX_HIP(hipStreamSynchronize(w.stream));
+            best = std::min(best, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        }
+        return best;
+    };
+    try {
+        struct Knob { int* field; std::vector<int> values; };
+        std::vector<Knob> knobs = {{&w.tune.colorBatch, {0, 256, 512, 1024, 2048, 4096}},
+                                   {&w.tune.solveBlock, {64, 128, 256}},
+                                   {&w.tune.pairsPerLane, {1, 2, 4}},
+                                   {&w.tune.colorRounds, {8, 16, 32}}};
+        double best = trial();
+        for (Knob& k : knobs) {
+            const int keep = *k.field;
+            int bestValue = keep;
+            for (int v : k.values) {
+                if (v == keep) continue;
+                *k.field = v;
+                double t = trial();
+                if (t < 0.97 * best) { best = t; bestValue = v; }
+            }
+            *k.field = bestValue;
+        }
+    } catch (...) {
+        w.timing = timing;
+        throw;
+    }
+    w.timing = timing;
+    restore_snapshot(w, z);
+}
 
 // The substep loop as one HIP graph launch. The loop's launch sequence follows the step's
 // colours, so it is captured every step; the instantiated graph is updated in place when
@@ -1002,6 +1110,57 @@ int pgx_world_set_exec(pgx_world* h, int mode, int blocks) {
     h->w.persistentBlocksWanted = blocks;
     h->w.persistentBlocks = 0;
     return PGX_OK;
+}
+
+static int check_tuning(const pgx_tuning* t) {
+    if (!t) return fail(PGX_ERR_INVALID, "tuning is NULL");
+    if (t->solve_block != 64 && t->solve_block != 128 && t->solve_block != 256)
+        return fail(PGX_ERR_INVALID, "solve_block must be 64, 128 or 256 (got %d)", t->solve_block);
+    if (t->color_batch < 0) return fail(PGX_ERR_INVALID, "color_batch must be >= 0");
+    if (t->pairs_per_lane != 1 && t->pairs_per_lane != 2 && t->pairs_per_lane != 4)
+        return fail(PGX_ERR_INVALID, "pairs_per_lane must be 1, 2 or 4 (got %d)", t->pairs_per_lane);
+    if (t->color_rounds < 1 || t->color_rounds > 1024) return fail(PGX_ERR_INVALID, "color_rounds must be in 1..1024");
+    return PGX_OK;
+}
+
+static void tuning_in(World& w, const pgx_tuning* t) {
+    w.tune.solveBlock = t->solve_block;
+    w.tune.colorBatch = t->color_batch;
+    w.tune.pairsPerLane = t->pairs_per_lane;
+    w.tune.colorRounds = t->color_rounds;
+}
+
+static void tuning_out(const World& w, pgx_tuning* t) {
+    t->solve_block = w.tune.solveBlock;
+    t->color_batch = w.tune.colorBatch;
+    t->pairs_per_lane = w.tune.pairsPerLane;
+    t->color_rounds = w.tune.colorRounds;
+}
+
+int pgx_world_set_tuning(pgx_world* h, const pgx_tuning* t) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    int rc = check_tuning(t);
+    if (rc != PGX_OK) return rc;
+    tuning_in(h->w, t);
+    return PGX_OK;
+}
+
+int pgx_world_get_tuning(pgx_world* h, pgx_tuning* t) {
+    if (!h || !t) return fail(PGX_ERR_INVALID, "NULL argument");
+    tuning_out(h->w, t);
+    return PGX_OK;
+}
+
+int pgx_world_autotune(pgx_world* h, float dt, int trial_steps, pgx_tuning* chosen) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    if (trial_steps < 1) return fail(PGX_ERR_INVALID, "trial_steps must be >= 1");
+    return guarded([&] {
+        World& w = h->w;
+        PGX_HIP(hipSetDevice(w.device));
+        world_autotune(w, dt, trial_steps);
+        if (chosen) tuning_out(w, chosen);
+        return PGX_OK;
+    });
 }
 
 int pgx_world_set_graph(pgx_world* h, int enabled) {

@@ -26,13 +26,16 @@ def manifold_dict(keys, ints, f):
     return out
 
 
-def lockstep(scene, steps, substeps=4, iterations=2, dt=1 / 60, check_every=1, on_step=None, mode="colors"):
+def lockstep(scene, steps, substeps=4, iterations=2, dt=1 / 60, check_every=1, on_step=None, mode="colors",
+             tuning=None):
     """Steps the GPU world and the oracle from the same scene; before each oracle step the
     oracle is fed the device's solve order. Returns (gpu, oracle) after asserting the
     states are bit-identical at every checked step."""
     from physecs_amd import api
     g = api.World(scene, substeps=substeps, iterations=iterations)
     g.set_exec(mode)
+    if tuning:
+        g.set_tuning(**tuning)
     o = ol.OracleWorld(scene, substeps=substeps, iterations=iterations)
     for k in range(steps):
         g.step(dt)
