@@ -217,7 +217,9 @@ def main():
         run_served(args)
         return
     from physecs_amd import api
+    print(f"building the {args.config} scene", file=sys.stderr, flush=True)
     scene, cfg = make_scene(args.config)
+    print(f"scene built: {scene.n_dyn} dynamic bodies", file=sys.stderr, flush=True)
     sub, it, dt = cfg["substeps"], cfg["iterations"], cfg["dt"]
     w = api.World(scene, substeps=sub, iterations=it)
     w.set_exec(args.exec, blocks=args.blocks)

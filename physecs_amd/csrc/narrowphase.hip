@@ -671,8 +671,28 @@ __device__ __forceinline__ int wave_append(int32_t* counter, bool want) {
     return base + __popcll(m & ((1ull << lane) - 1ull));
 }
 
+// Shape-pair class of every pair: the unordered pair of primitive types (6 classes), or
+// 15 for pairs that leave this kernel's collision routines (triangle or convex meshes,
+// triggers, non-colliding pairs route or skip quickly). Sorted by class, a wave of the
+// narrowphase runs one collision routine instead of every routine its pairs need.
+__global__ void k_pair_class(int nPairs, const uint64_t* __restrict__ pkeys, int localBits,
+                             const int32_t* __restrict__ worldColStart, const int32_t* __restrict__ colOfLocal,
+                             const int32_t* __restrict__ ctype, uint64_t* __restrict__ cls,
+                             int32_t* __restrict__ order) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nPairs) return;
+    uint64_t key = pkeys[p];
+    uint64_t lmask = (1ull << localBits) - 1;
+    int base = worldColStart[(int)(key >> (2 * localBits))];
+    int t0 = ctype[colOfLocal[base + (int)((key >> localBits) & lmask)]];
+    int t1 = ctype[colOfLocal[base + (int)(key & lmask)]];
+    bool prim = t0 <= PGX_BOX && t1 <= PGX_BOX;
+    cls[p] = prim ? (uint64_t)(min(t0, t1) * 3 + max(t0, t1)) : 15ull;
+    order[p] = p;
+}
+
 __global__ __launch_bounds__(256) void k_narrowphase(
-    int nPairs, const uint64_t* __restrict__ pkeys, int localBits, const int32_t* __restrict__ worldColStart,
+    int nPairs, const int32_t* __restrict__ order, const uint64_t* __restrict__ pkeys, int localBits, const int32_t* __restrict__ worldColStart,
     const int32_t* __restrict__ colOfLocal, const int32_t* __restrict__ cbody, const int32_t* __restrict__ ctype,
     const int32_t* __restrict__ cflags, const float4* __restrict__ cparams, const float4* __restrict__ clpos,
     const float4* __restrict__ clquat, const uint32_t* __restrict__ centity, const float4* __restrict__ bpos,
@@ -680,8 +700,10 @@ __global__ __launch_bounds__(256) void k_narrowphase(
     int32_t* __restrict__ npNum, float4* __restrict__ npNormal, float4* __restrict__ npPts,
     int32_t* __restrict__ heavy, int32_t* __restrict__ nHeavy, int32_t* __restrict__ meshPairs,
     int32_t* __restrict__ nMesh, int32_t* __restrict__ meshCount) {
-    // pairs per lane = pairs / (grid size): a lane takes every grid-size'th pair
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < nPairs; p += gridDim.x * blockDim.x) {
+    // pairs per lane = pairs / (grid size): a lane takes every grid-size'th pair of the
+    // class order; results are stored at the pair's own slot
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nPairs; i += gridDim.x * blockDim.x) {
+        const int p = order[i];
         meshCount[p] = 0;
         uint64_t key = pkeys[p];
         uint64_t lmask = (1ull << localBits) - 1;
@@ -952,7 +974,14 @@ __global__ __launch_bounds__(64) void k_mesh_emit(MeshArgs A, int32_t* tmWs, int
 void np_run(World& w, int nPairs) {
     if (nPairs == 0) return;
     const int ppl = std::max(1, w.tune.pairsPerLane);
-    hipLaunchKernelGGL(k_narrowphase, dim3(div_up(nPairs, 256 * ppl)), dim3(256), 0, w.stream, nPairs, w.pkeys.p,
+    w.npClass.alloc(w.pairCap);
+    w.npOrder.alloc(w.pairCap);
+    hipLaunchKernelGGL(k_pair_class, dim3(div_up(nPairs, 256)), dim3(256), 0, w.stream, nPairs, w.pkeys.p, w.localBits,
+                       w.worldColStart.p, w.colOfLocal.p, w.ctype.p, w.npClass.p, w.npOrder.p);
+    PGX_LAUNCH_CHECK();
+    radix_sort_u64(w.stream, w.npClass.p, w.npOrder.p, nPairs, 4, w.sortT);   // stable: slot order within a class
+    hipLaunchKernelGGL(k_narrowphase, dim3(div_up(nPairs, 256 * ppl)), dim3(256), 0, w.stream, nPairs, w.npOrder.p,
+                       w.pkeys.p,
                        w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.ctype.p, w.cflags.p,
                        w.cparams.p, w.clpos.p, w.clquat.p, w.centity.p, w.bpos.p, w.bquat.p, w.nonColliding.p,
                        w.nNonColliding, w.npNum.p, w.npNormal.p, w.npPts.p, w.heavy.p, w.counters.p + 8,

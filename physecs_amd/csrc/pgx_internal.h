@@ -172,6 +172,8 @@ struct World {
     DBuf<float4> npNormal;               // normal
     DBuf<float4> npPts;                  // 8 per pair: p0,p1 for 4 points
     DBuf<int32_t> npIndex;               // exclusive scan of (numPoints > 0)
+    DBuf<uint64_t> npClass;              // per pair: its shape-pair class (sort key)
+    DBuf<int32_t> npOrder;               // pair slots grouped by shape-pair class
 
     // manifolds, canonical order (sorted by pair key)
     DBuf<uint64_t> mfKey;                // pair key

@@ -258,34 +258,45 @@ __global__ __launch_bounds__(256) void k_setup(
     sSoft[s] = make_float2(freq, damp);
 }
 
-// contact points per colour (statistics): one block per colour sums its manifolds' counts
-__global__ __launch_bounds__(256) void k_color_points(const int32_t* __restrict__ colorStart,
-                                                      const int32_t* __restrict__ sFlags, int32_t* __restrict__ out) {
-    __shared__ int32_t part[256];
-    int c = blockIdx.x;
-    int b = colorStart[c], e = colorStart[c + 1];
-    int sum = 0;
-    for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x) sum += sFlags[i] & 255;
-    part[threadIdx.x] = sum;
+// contact points per colour (statistics). The manifolds are in colour order, so each
+// thread sums a run of POINTS_RUN consecutive manifolds and adds to its block's per-colour
+// counters in LDS when the colour changes; each block then adds its non-zero counters to
+// the output, which starts at zero.
+constexpr int POINTS_RUN = 16;
+__global__ __launch_bounds__(256) void k_color_points(int M, const int32_t* __restrict__ sFlags,
+                                                      int32_t* __restrict__ out) {
+    __shared__ int32_t cnt[128];
+    if (threadIdx.x < 128) cnt[threadIdx.x] = 0;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
-        __syncthreads();
+    const int first = (blockIdx.x * blockDim.x + threadIdx.x) * POINTS_RUN;
+    int colour = -1, sum = 0;
+    for (int i = first; i < min(M, first + POINTS_RUN); ++i) {
+        const int f = sFlags[i], c = (f >> 16) & 127;
+        if (c != colour) {
+            if (sum) atomicAdd(&cnt[colour], sum);
+            colour = c;
+            sum = 0;
+        }
+        sum += f & 255;
     }
-    if (threadIdx.x == 0) out[c] = part[0];
+    if (sum) atomicAdd(&cnt[colour], sum);
+    __syncthreads();
+    if (threadIdx.x < 128 && cnt[threadIdx.x]) atomicAdd(&out[threadIdx.x], cnt[threadIdx.x]);
 }
 
 void sv_color_points(World& w, int nColors) {
     if (nColors <= 0) return;
-    hipLaunchKernelGGL(k_color_points, dim3(nColors), dim3(256), 0, w.stream, w.colorCount.p, w.sFlags.p,
-                       w.colorPoints.p);
+    const int M = w.colorStartHost.back();
+    fill_i32(w.stream, w.colorPoints.p, 0, 160);
+    hipLaunchKernelGGL(k_color_points, dim3(std::max(1, div_up(M, 256 * POINTS_RUN))), dim3(256), 0, w.stream, M,
+                       w.sFlags.p, w.colorPoints.p);
     PGX_LAUNCH_CHECK();
 }
 
 // ------------------------------------------------------------------ substep pieces
-// per-point row storage: field f of point k of manifold s at ((k * 9 + f) * Mcap + s)
-// Contact rows of point k of manifold slot s: r0 x n (.w = c), r1 x n (.w = inverse
-// effective mass N), tangent (.w = inverse effective mass T), r0 x t, r1 x t. The rows
+// Contact rows of point k of manifold slot s, field f at ((k * R_NF + f) * Mcap + s):
+// r0 x n (.w = c), r1 x n (.w = inverse effective mass N), tangent (.w = inverse
+// effective mass T), r0 x t, r1 x t. The rows
 // times the bodies' world inverse inertia are recomputed in the solve from the body
 // records, with the operations that produced them here. Per point also (lambda N,
 // lambda T, target velocity) in sLam.
