@@ -233,23 +233,66 @@ __device__ __forceinline__ m3 world_inv_inertia_j(const JointArgs& a, int b) {
     return mulmm(mulmm(rot, invI), transpose(rot));
 }
 
+// The data of one row (JR_* fields), loaded at once so that a joint's next row can be in
+// flight while the current one is solved.
+struct JRow { float4 L, G0, G1, L0, L1, T0, T1, EX; };
+__device__ __forceinline__ JRow load_jrow(const JointArgs& a, int j, int r) {
+    JRow d;
+    d.L = JROW(r, JR_LIN); d.G0 = JROW(r, JR_ANG0); d.G1 = JROW(r, JR_ANG1);
+    d.L0 = JROW(r, JR_LIN0T); d.L1 = JROW(r, JR_LIN1T); d.T0 = JROW(r, JR_ANG0T); d.T1 = JROW(r, JR_ANG1T);
+    d.EX = JROW(r, JR_EXTRA);
+    return d;
+}
+
+// A joint's bodies as its rows' preSolve reads them: inverse mass and the world inverse
+// inertia of the substep (massTemp), zero for a static or kinematic side.
+struct JBodies { int b0, b1; float im0, im1; m3 I0, I1; };
+
+// From the body records the velocity integration wrote this substep (the same
+// R * invI * R^T operations as world_inv_inertia_j).
+__device__ __forceinline__ JBodies joint_bodies(const JointArgs& a, int j) {
+    JBodies B;
+    B.b0 = a.jB0[j]; B.b1 = a.jB1[j];
+    B.im0 = 0.f; B.im1 = 0.f; B.I0 = mzero(); B.I1 = mzero();
+    if (B.b0 >= 0) {
+        B.im0 = a.bvel[bv(B.b0, BV_CV)].w;
+        B.I0 = mk_m3(xyz(a.bvel[bv(B.b0, BV_I0)]), xyz(a.bvel[bv(B.b0, BV_I1)]), xyz(a.bvel[bv(B.b0, BV_I2)]));
+    }
+    if (B.b1 >= 0) {
+        B.im1 = a.bvel[bv(B.b1, BV_CV)].w;
+        B.I1 = mk_m3(xyz(a.bvel[bv(B.b1, BV_I0)]), xyz(a.bvel[bv(B.b1, BV_I1)]), xyz(a.bvel[bv(B.b1, BV_I2)]));
+    }
+    return B;
+}
+
+__device__ void row_presolve_d(const JointArgs& a, int j, int r, int flags, bool simd, const JRow& d,
+                               const JBodies& B, float4& pv0, float4& pw0, float4& pv1, float4& pw1);
+
 // Constraint1D<flags>::preSolve / Constraint1DW<flags>::preSolve for one row, on the
 // bodies' pseudo velocities held by the caller (pv, pw; pw.w = constraint count)
 __device__ void row_presolve(const JointArgs& a, int j, int r, int flags, bool simd, float4& pv0, float4& pw0,
                              float4& pv1, float4& pw1) {
-    int b0 = a.jB0[j], b1 = a.jB1[j];
-    float im0 = 0.f, im1 = 0.f;
-    m3 I0 = mzero(), I1 = mzero();
-    if (b0 >= 0) { im0 = a.dvel[b0].w; I0 = world_inv_inertia_j(a, b0); }
-    if (b1 >= 0) { im1 = a.dvel[b1].w; I1 = world_inv_inertia_j(a, b1); }
+    JBodies B;
+    B.b0 = a.jB0[j]; B.b1 = a.jB1[j];
+    B.im0 = 0.f; B.im1 = 0.f; B.I0 = mzero(); B.I1 = mzero();
+    if (B.b0 >= 0) { B.im0 = a.dvel[B.b0].w; B.I0 = world_inv_inertia_j(a, B.b0); }
+    if (B.b1 >= 0) { B.im1 = a.dvel[B.b1].w; B.I1 = world_inv_inertia_j(a, B.b1); }
+    row_presolve_d(a, j, r, flags, simd, load_jrow(a, j, r), B, pv0, pw0, pv1, pw1);
+}
+
+__device__ void row_presolve_d(const JointArgs& a, int j, int r, int flags, bool simd, const JRow& d,
+                               const JBodies& B, float4& pv0, float4& pw0, float4& pv1, float4& pw1) {
+    const int b0 = B.b0, b1 = B.b1;
+    const float im0 = B.im0, im1 = B.im1;
+    const m3 &I0 = B.I0, &I1 = B.I1;
     bool ang = flags & JF_ANGULAR;
-    float4 L = JROW(r, JR_LIN), G0 = JROW(r, JR_ANG0), G1 = JROW(r, JR_ANG1);
+    float4 L = d.L, G0 = d.G0, G1 = d.G1;
     f3 lin = xyz(L), ang0 = xyz(G0), ang1 = xyz(G1);
     float c = G0.w;
     f3 ang0t = mulmv(I0, ang0), ang1t = mulmv(I1, ang1);
     float invEff = dot(ang0, ang0t) + dot(ang1, ang1t);
     f3 lin0t = s3(0.f), lin1t = s3(0.f);
-    float4 L0 = JROW(r, JR_LIN0T), L1 = JROW(r, JR_LIN1T);
+    float4 L0 = d.L0, L1 = d.L1;
     if (!ang) {
         invEff = invEff + dot(lin, lin) * (im0 + im1);
         lin0t = mul(im0, lin);
@@ -257,7 +300,7 @@ __device__ void row_presolve(const JointArgs& a, int j, int r, int flags, bool s
         set_xyz(L0, lin0t);
         set_xyz(L1, lin1t);
     }
-    float4 T0 = JROW(r, JR_ANG0T), T1 = JROW(r, JR_ANG1T);
+    float4 T0 = d.T0, T1 = d.T1;
     set_xyz(T0, ang0t);
     set_xyz(T1, ang1t);
     G1.w = invEff;
@@ -285,17 +328,16 @@ __device__ void row_presolve(const JointArgs& a, int j, int r, int flags, bool s
 
 // Constraint1DW<flags>::solve for one lane (src/Constraint1DW.cpp:117-233), on the bodies'
 // solver velocities held by the caller (bv, bw; an angular row reads and writes only bw)
-__device__ void row_solve_simd(const JointArgs& a, int j, int r, int flags, bool warm, f3& bv0, f3& bw0, f3& bv1,
-                               f3& bw1) {
-    int b0 = a.jB0[j], b1 = a.jB1[j];
+__device__ void row_solve_simd(const JointArgs& a, int j, int r, int flags, bool warm, const JRow& d, int b0, int b1,
+                               f3& bv0, f3& bw0, f3& bv1, f3& bw1) {
     bool ang = flags & JF_ANGULAR;
     float h = a.h;
     f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
     if (b0 >= 0) { if (!ang) v0 = bv0; w0 = bw0; }
     if (b1 >= 0) { if (!ang) v1 = bv1; w1 = bw1; }
-    float4 L = JROW(r, JR_LIN), G0 = JROW(r, JR_ANG0), G1 = JROW(r, JR_ANG1);
-    float4 L0 = JROW(r, JR_LIN0T), L1 = JROW(r, JR_LIN1T), T0 = JROW(r, JR_ANG0T), T1 = JROW(r, JR_ANG1T);
-    float4 EX = JROW(r, JR_EXTRA);
+    float4 L = d.L, G0 = d.G0, G1 = d.G1;
+    float4 L0 = d.L0, L1 = d.L1, T0 = d.T0, T1 = d.T1;
+    float4 EX = d.EX;
     f3 lin0t = xyz(L0), lin1t = xyz(L1), ang0t = xyz(T0), ang1t = xyz(T1);
     float c = G0.w, invEff = G1.w, lam = L0.w;
     if (!(flags & JF_SOFT) && warm) {
@@ -419,16 +461,28 @@ __global__ void k_joint_color(JointArgs a, int begin, int end, int phase, int wa
     int j = a.order[k];
     int n = a.jNRows[j];
     int b0 = a.jB0[j], b1 = a.jB1[j];
+    // the joint's rows in solve order (flag type by flag type, creation order inside a
+    // type), 4 bits per row
+    const int rowFlags = a.jRowFlags[j];
+    uint32_t seq = 0;
+    int cnt = 0;
+    for (int f = 0; f < 6; ++f)
+        for (int r = 0; r < n; ++r)
+            if (((rowFlags >> (4 * r)) & 15) == kFlagOrder[f]) seq |= (uint32_t)r << (4 * cnt++);
     float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    JRow cur, nxt;
+    if (cnt) cur = load_jrow(a, j, seq & 15);
     if (phase == 0) {
+        const JBodies B = joint_bodies(a, j);
         float4 pv0 = z, pw0 = z, pv1 = z, pw1 = z;
         if (b0 >= 0) { pv0 = a.pvel[b0]; pw0 = a.pang[b0]; }
         if (b1 >= 0) { pv1 = a.pvel[b1]; pw1 = a.pang[b1]; }
-        for (int f = 0; f < 6; ++f)
-            for (int r = 0; r < n; ++r) {
-                int fl = row_flags(a, j, r);
-                if (fl == kFlagOrder[f]) row_presolve(a, j, r, fl, true, pv0, pw0, pv1, pw1);
-            }
+        for (int i = 0; i < cnt; ++i) {
+            const int r = (seq >> (4 * i)) & 15;
+            if (i + 1 < cnt) nxt = load_jrow(a, j, (seq >> (4 * (i + 1))) & 15);
+            row_presolve_d(a, j, r, (rowFlags >> (4 * r)) & 15, true, cur, B, pv0, pw0, pv1, pw1);
+            cur = nxt;
+        }
         if (b0 >= 0) { a.pvel[b0] = pv0; a.pang[b0] = pw0; }
         if (b1 >= 0) { a.pvel[b1] = pv1; a.pang[b1] = pw1; }
         return;
@@ -437,11 +491,12 @@ __global__ void k_joint_color(JointArgs a, int begin, int end, int phase, int wa
     if (b0 >= 0) { tv0 = a.bvel[bv(b0, BV_V)]; tw0 = a.bvel[bv(b0, BV_W)]; }
     if (b1 >= 0) { tv1 = a.bvel[bv(b1, BV_V)]; tw1 = a.bvel[bv(b1, BV_W)]; }
     f3 v0 = xyz(tv0), w0 = xyz(tw0), v1 = xyz(tv1), w1 = xyz(tw1);
-    for (int f = 0; f < 6; ++f)
-        for (int r = 0; r < n; ++r) {
-            int fl = row_flags(a, j, r);
-            if (fl == kFlagOrder[f]) row_solve_simd(a, j, r, fl, warm != 0, v0, w0, v1, w1);
-        }
+    for (int i = 0; i < cnt; ++i) {
+        const int r = (seq >> (4 * i)) & 15;
+        if (i + 1 < cnt) nxt = load_jrow(a, j, (seq >> (4 * (i + 1))) & 15);
+        row_solve_simd(a, j, r, (rowFlags >> (4 * r)) & 15, warm != 0, cur, b0, b1, v0, w0, v1, w1);
+        cur = nxt;
+    }
     if (b0 >= 0) { set_xyz(tv0, v0); set_xyz(tw0, w0); a.bvel[bv(b0, BV_V)] = tv0; a.bvel[bv(b0, BV_W)] = tw0; }
     if (b1 >= 0) { set_xyz(tv1, v1); set_xyz(tw1, w1); a.bvel[bv(b1, BV_V)] = tv1; a.bvel[bv(b1, BV_W)] = tw1; }
 }
