@@ -201,8 +201,9 @@ def main():
     ap.add_argument("--exec", default="colors", choices=["flow", "xcd", "colors", "cooperative"],
                     help="how the contact passes launch")
     ap.add_argument("--graph", action="store_true", help="launch each step's substep loop as one HIP graph")
-    ap.add_argument("--tune", default="auto", help="launch shapes: 'auto' (pgx_world_autotune after the warm-up), "
-                    "'default', or a JSON object of pgx_tuning fields")
+    ap.add_argument("--tune", default=None, help="launch shapes: 'auto' (pgx_world_autotune after the warm-up), "
+                    "'default', or a JSON object of pgx_tuning fields; by default 'auto' for the batched worlds "
+                    "and 'default' otherwise")
     ap.add_argument("--seconds", type=float, default=20.0, help="served: length of the request stream")
     ap.add_argument("--admit-interval", type=int, default=8, help="served: steps between admission boundaries")
     ap.add_argument("--copy-calibration", action="store_true", help="run only the 1 GiB copy used to scale PMC counters")
@@ -227,6 +228,8 @@ def main():
     for _ in range(args.warmup):
         w.step(dt)
     w.sync()
+    if args.tune is None:
+        args.tune = "auto" if args.config == "batched" else "default"
     if args.tune == "auto":
         chosen = w.autotune(dt, trial_steps=4)   # restores the world's state
         print(f"autotune chose {chosen}", file=sys.stderr, flush=True)

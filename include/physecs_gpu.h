@@ -263,12 +263,16 @@ typedef struct pgx_tuning {
     int pairs_per_lane;   /* narrowphase pairs per lane: 1, 2, 4 */
     int color_rounds;     /* least graph-colouring rounds queued before the host reads back
                              whether colouring finished: 1..1024 */
+    int eager_rows_max;   /* colours of at most this many manifolds issue all four point
+                             slots' row loads at once instead of after the point count:
+                             shorter latency, more bytes (>= 0) */
 } pgx_tuning;
 int pgx_world_set_tuning(pgx_world* w, const pgx_tuning* t);
 int pgx_world_get_tuning(pgx_world* w, pgx_tuning* t);
 /* Autotuner: times trial_steps steps (after one untimed step; the better of two runs)
  * from the world's current state for each value of one parameter at a time (color_batch,
- * solve_block, pairs_per_lane, color_rounds, in that order), keeps a value that is at
+ * solve_block, eager_rows_max, pairs_per_lane, color_rounds, in that order), keeps a value
+ * that is at
  * least 3% faster than the current choice, and restores the
  * world's state afterwards (poses, velocities, bounds, contact cache, gear angles,
  * statistics), so the world steps on exactly as it would have. The chosen values stay set

@@ -261,6 +261,7 @@ struct World {
         int colorBatch = 0;              // runs of colours of at most this many manifolds: one launch
         int pairsPerLane = 1;            // narrowphase pairs per lane (1, 2, 4)
         int colorRounds = 8;             // least colouring rounds queued per read-back
+        int eagerMax = 32768;            // colours of at most this many manifolds load rows eagerly
     } tune;
     bool graph = false;                  // substep loop as one HIP graph launch (colour mode)
     hipGraphExec_t subExec = nullptr;    // the last instantiated substep-loop graph

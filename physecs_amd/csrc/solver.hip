@@ -435,6 +435,10 @@ __device__ __forceinline__ m3 body_inertia(const SubArgs& a, int b) {
     return mk_m3(xyz(a.bvel[bv(b, BV_I0)]), xyz(a.bvel[bv(b, BV_I1)]), xyz(a.bvel[bv(b, BV_I2)]));
 }
 
+// EAGER loads the rows of all four point slots at once, without waiting for the point
+// count (slots beyond it hold stale data and are not used): one dependent memory round
+// trip less for colours too small to be bandwidth-bound, at the cost of unused loads.
+template <bool EAGER = false>
 __device__ __forceinline__ void load_inputs(const SubArgs& a, int s, MInputs& in) {
     int flags = a.sFlags[s];
     in.soft = (flags & 256) != 0;
@@ -462,7 +466,7 @@ __device__ __forceinline__ void load_inputs(const SubArgs& a, int s, MInputs& in
     // parallelism); unused point slots are never read back
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (k < in.num) {
+        if (EAGER || k < in.num) {
             in.rA[k] = ROW(R_R0XN, k); in.rB[k] = ROW(R_R1XN, k); in.rT[k] = ROW(R_T, k);
             in.rC[k] = ROW(R_R0XT, k); in.rE[k] = ROW(R_R1XT, k);
             in.lam[k] = LAM(k);
@@ -536,9 +540,10 @@ __device__ __forceinline__ void solve_core(const SubArgs& a, int s, const MInput
 
 // one manifold of a colour launch: every other manifold touching its bodies is in another
 // launch, so plain accesses see the previous colours' results
+template <bool EAGER = false>
 __device__ void solve_manifold(const SubArgs& a, int s, bool useBias, float h, bool skipSoft) {
     MInputs in;
-    load_inputs(a, s, in);
+    load_inputs<EAGER>(a, s, in);
     if (skipSoft && in.soft) return;
     f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
     if (in.b0 >= 0) { v0 = xyz(a.bvel[bv(in.b0, BV_V)]); w0 = xyz(a.bvel[bv(in.b0, BV_W)]); }
@@ -585,10 +590,10 @@ __global__ __launch_bounds__(256) void k_prepare(SubArgs a, int sub) {
     if (i < a.M) prepare_manifold(a, i);
     else if (i - a.M < a.D) integrate_velocity(a, i - a.M);
 }
-template <int BLOCK>
+template <int BLOCK, bool EAGER>
 __global__ __launch_bounds__(BLOCK) void k_solve(SubArgs a, int begin, int end, int useBias, float h, int skipSoft) {
     int s = begin + blockIdx.x * BLOCK + threadIdx.x;
-    if (s < end) solve_manifold(a, s, useBias != 0, h, skipSoft != 0);
+    if (s < end) solve_manifold<EAGER>(a, s, useBias != 0, h, skipSoft != 0);
 }
 
 // Consecutive small colours c0..c1-1 in one launch of one workgroup: the colours are solved
@@ -995,16 +1000,22 @@ static void solve_events(World& w, int color, hipEvent_t* start, hipEvent_t* sto
 
 // one colour [b, e) in workgroups of `block` threads; with events, stamped from the
 // launch's own dispatch packet
-static void launch_solve(int block, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const SubArgs& a, int b, int e,
-                         int useBias, float h, int skipSoft) {
+static void launch_solve(int block, bool eager, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const SubArgs& a, int b,
+                         int e, int useBias, float h, int skipSoft) {
     auto go = [&](auto kernel, int bs) {
         if (e0) hipExtLaunchKernelGGL(kernel, dim3(div_up(e - b, bs)), dim3(bs), 0, s, e0, e1, 0, a, b, e, useBias, h,
                                       skipSoft);
         else hipLaunchKernelGGL(kernel, dim3(div_up(e - b, bs)), dim3(bs), 0, s, a, b, e, useBias, h, skipSoft);
     };
-    if (block == 256) go(k_solve<256>, 256);
-    else if (block == 128) go(k_solve<128>, 128);
-    else go(k_solve<64>, 64);
+    if (eager) {
+        if (block == 256) go(k_solve<256, true>, 256);
+        else if (block == 128) go(k_solve<128, true>, 128);
+        else go(k_solve<64, true>, 64);
+    } else {
+        if (block == 256) go(k_solve<256, false>, 256);
+        else if (block == 128) go(k_solve<128, false>, 128);
+        else go(k_solve<64, false>, 64);
+    }
 }
 
 void sv_substeps(World& w, int M, float dt) {
@@ -1089,7 +1100,7 @@ void sv_substeps(World& w, int M, float dt) {
             const int c = g.x, b = w.colorStartHost[c], e = w.colorStartHost[c + 1];
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (rec) solve_events(w, c, &e0, &e1);
-            launch_solve(w.tune.solveBlock, s, e0, e1, a, b, e, useBias, h, skipSoft);
+            launch_solve(w.tune.solveBlock, e - b <= w.tune.eagerMax, s, e0, e1, a, b, e, useBias, h, skipSoft);
         }
     };
     jt_step_setup(w);

@@ -725,6 +725,7 @@ void world_autotune(World& w, float dt, int trialSteps) {
         struct Knob { int* field; std::vector<int> values; };
         std::vector<Knob> knobs = {{&w.tune.colorBatch, {0, 256, 512, 1024, 2048, 4096}},
                                    {&w.tune.solveBlock, {64, 128, 256}},
+                                   {&w.tune.eagerMax, {0, 8192, 32768, 131072, 1 << 30}},
                                    {&w.tune.pairsPerLane, {1, 2, 4}},
                                    {&w.tune.colorRounds, {8, 16, 32}}};
         double best = trial();
@@ -1120,6 +1121,7 @@ static int check_tuning(const pgx_tuning* t) {
     if (t->pairs_per_lane != 1 && t->pairs_per_lane != 2 && t->pairs_per_lane != 4)
         return fail(PGX_ERR_INVALID, "pairs_per_lane must be 1, 2 or 4 (got %d)", t->pairs_per_lane);
     if (t->color_rounds < 1 || t->color_rounds > 1024) return fail(PGX_ERR_INVALID, "color_rounds must be in 1..1024");
+    if (t->eager_rows_max < 0) return fail(PGX_ERR_INVALID, "eager_rows_max must be >= 0");
     return PGX_OK;
 }
 
@@ -1128,6 +1130,7 @@ static void tuning_in(World& w, const pgx_tuning* t) {
     w.tune.colorBatch = t->color_batch;
     w.tune.pairsPerLane = t->pairs_per_lane;
     w.tune.colorRounds = t->color_rounds;
+    w.tune.eagerMax = t->eager_rows_max;
 }
 
 static void tuning_out(const World& w, pgx_tuning* t) {
@@ -1135,6 +1138,7 @@ static void tuning_out(const World& w, pgx_tuning* t) {
     t->color_batch = w.tune.colorBatch;
     t->pairs_per_lane = w.tune.pairsPerLane;
     t->color_rounds = w.tune.colorRounds;
+    t->eager_rows_max = w.tune.eagerMax;
 }
 
 int pgx_world_set_tuning(pgx_world* h, const pgx_tuning* t) {

@@ -12,9 +12,9 @@ from physecs_amd import scenes
 pytestmark = pytest.mark.gpu
 
 TUNINGS = [
-    {"solve_block": 256, "color_batch": 1 << 30, "pairs_per_lane": 4, "color_rounds": 1},
-    {"solve_block": 128, "color_batch": 512, "pairs_per_lane": 2, "color_rounds": 32},
-    {"solve_block": 64, "color_batch": 64, "pairs_per_lane": 1, "color_rounds": 3},
+    {"solve_block": 256, "color_batch": 1 << 30, "pairs_per_lane": 4, "color_rounds": 1, "eager_rows_max": 0},
+    {"solve_block": 128, "color_batch": 512, "pairs_per_lane": 2, "color_rounds": 32, "eager_rows_max": 1 << 30},
+    {"solve_block": 64, "color_batch": 64, "pairs_per_lane": 1, "color_rounds": 3, "eager_rows_max": 100},
 ]
 
 
@@ -62,7 +62,7 @@ def test_batched_colours_match_the_oracle():
     oracle fed the device's solve order."""
     sc = scenes.box_pyramid(1, rows=10)
     log("[tuning] pyramid 10 rows, all colours batched, lockstep with the oracle")
-    lockstep(sc, 40, tuning={"color_batch": 1 << 30})
+    lockstep(sc, 40, tuning={"color_batch": 1 << 30, "eager_rows_max": 0})
 
 
 def test_autotune_restores_the_world(api):
@@ -90,7 +90,8 @@ def test_autotune_restores_the_world(api):
 def test_tuning_rejects_bad_values(api):
     w = api.World(scenes.box_pyramid(0, rows=3), substeps=4, iterations=2)
     start = w.tuning()
-    for bad in ({"solve_block": 96}, {"pairs_per_lane": 3}, {"color_rounds": 0}, {"color_batch": -1}):
+    for bad in ({"solve_block": 96}, {"pairs_per_lane": 3}, {"color_rounds": 0}, {"color_batch": -1},
+                {"eager_rows_max": -1}):
         with pytest.raises(api.PgxError):
             w.set_tuning(**bad)
     assert w.tuning() == start
