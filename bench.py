@@ -198,7 +198,7 @@ def main():
     ap.add_argument("--config", default="plain32k")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks", type=int, default=0, help="cooperative grid size (0 = one block per CU)")
-    ap.add_argument("--exec", default="colors", choices=["flow", "xcd", "colors", "cooperative"],
+    ap.add_argument("--exec", default="colors", choices=["flow", "xcd", "colors", "cooperative", "worlds"],
                     help="how the contact passes launch")
     ap.add_argument("--graph", action="store_true", help="launch each step's substep loop as one HIP graph")
     ap.add_argument("--tune", default=None, help="launch shapes: 'auto' (pgx_world_autotune after the warm-up), "
@@ -232,7 +232,7 @@ def main():
         args.tune = "auto" if args.config == "batched" else "default"
     if args.tune == "auto":
         chosen = w.autotune(dt, trial_steps=4)   # restores the world's state
-        print(f"autotune chose {chosen}", file=sys.stderr, flush=True)
+        print(f"autotune chose {chosen}, exec {w.exec_mode()}", file=sys.stderr, flush=True)
     elif args.tune != "default":
         w.set_tuning(**json.loads(args.tune))
     warm_state = w.state()
@@ -272,12 +272,14 @@ def main():
         "p99_step_ms": 1000 * float(np.percentile(lat, 99)), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
         "config": dict(cfg, parallelism="single GPU", graph=args.graph, tuning=w.tuning(), tuned=args.tune,
+                       exec=w.exec_mode(),
                        last_step_counts=counts,
                        phase_us_last_step=w.timing()),
         "roofline": {"bound": "hbm", "kernel": {"flow": "k_solve_flow", "xcd": "k_solve_flow", "colors": "k_solve",
-                                                    "cooperative": "k_step_persistent"}[args.exec], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                                    "cooperative": "k_step_persistent",
+                                                    "worlds": "k_world_pass"}[w.exec_mode()], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.exec) if args.config == "plain32k" else None,
+                     "traffic": measured_traffic(w.exec_mode()) if args.config == "plain32k" else None,
                      "avg_launch_us": st["solve_us"] / max(1, st["solve_launches"]),
                      "bytes_per_launch": st["solve_bytes"] / max(1, st["solve_launches"]),
                      "launches_timed": st["solve_launches"]},

@@ -239,12 +239,19 @@ int pgx_world_reset_stats(pgx_world* w);
  *   PGX_EXEC_FLOW_XCD        the same with each world's manifolds on one XCD
  *   PGX_EXEC_COOPERATIVE     every substep of a step in one cooperative launch, grid
  *                            barriers between colours (worlds without joints; others use
- *                            PGX_EXEC_COLORS). blocks = its grid size (0 = one per CU). */
+ *                            PGX_EXEC_COLORS). blocks = its grid size (0 = one per CU).
+ *   PGX_EXEC_WORLDS          one launch per pass, one workgroup per world: the world's
+ *                            solver velocities in LDS, its colours in order with
+ *                            workgroup barriers. Needs each world's dynamic bodies to be
+ *                            contiguous and to fit in LDS (5120 bodies on gfx950);
+ *                            otherwise the step uses PGX_EXEC_COLORS. */
 #define PGX_EXEC_COLORS 0
 #define PGX_EXEC_COOPERATIVE 1
 #define PGX_EXEC_FLOW 2
 #define PGX_EXEC_FLOW_XCD 3
+#define PGX_EXEC_WORLDS 4
 int pgx_world_set_exec(pgx_world* w, int mode, int blocks);
+int pgx_world_get_exec(pgx_world* w, int* mode);
 int pgx_world_get_stats(pgx_world* w, pgx_stats* out);
 /* In PGX_EXEC_COLORS, enabled != 0 launches each step's substep loop as one HIP graph:
  * captured every step (its launches follow the step's colours), the instantiated graph
@@ -272,11 +279,12 @@ int pgx_world_get_tuning(pgx_world* w, pgx_tuning* t);
 /* Autotuner: times trial_steps steps (after one untimed step; the better of two runs)
  * from the world's current state for each value of one parameter at a time (color_batch,
  * solve_block, eager_rows_max, pairs_per_lane, color_rounds, in that order), keeps a value
- * that is at
- * least 3% faster than the current choice, and restores the
- * world's state afterwards (poses, velocities, bounds, contact cache, gear angles,
- * statistics), so the world steps on exactly as it would have. The chosen values stay set
- * and are written to *chosen (may be NULL). */
+ * that is at least 3% faster than the current choice, and restores the world's state
+ * afterwards (poses, velocities, bounds, contact cache, gear angles, statistics), so the
+ * world steps on exactly as it would have. The chosen values stay set
+ * and are written to *chosen (may be NULL). A world in PGX_EXEC_COLORS or PGX_EXEC_WORLDS
+ * whose worlds fit the per-world passes also has the faster of those two modes chosen
+ * first (pgx_world_set_exec). */
 int pgx_world_autotune(pgx_world* w, float dt, int trial_steps, pgx_tuning* chosen);
 
 /* ---------------------------------------------------------------------------------

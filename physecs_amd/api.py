@@ -28,7 +28,7 @@ EXPORTS = [
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
-    "pgx_world_autotune",
+    "pgx_world_autotune", "pgx_world_get_exec",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -128,6 +128,7 @@ def lib():
         L.pgx_world_set_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
         L.pgx_world_get_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
         L.pgx_world_autotune.argtypes = [vp, ctypes.c_float, ctypes.c_int, ctypes.POINTER(PgxTuning)]
+        L.pgx_world_get_exec.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.pgx_server_set_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
         L.pgx_server_create.argtypes = [ctypes.POINTER(PgxConfig), ctypes.POINTER(PgxServerConfig), ctypes.POINTER(vp)]
         L.pgx_server_destroy.argtypes = [vp]
@@ -262,12 +263,13 @@ class World:
     def enable_timing(self, on=True):
         check(self.L.pgx_world_enable_timing(self.h, 1 if on else 0))
 
-    EXEC_MODES = {"colors": 0, "cooperative": 1, "flow": 2, "xcd": 3}
+    EXEC_MODES = {"colors": 0, "cooperative": 1, "flow": 2, "xcd": 3, "worlds": 4}
 
     def set_exec(self, mode="colors", blocks=0):
         """How the contact passes launch: "colors" (one launch per colour, the default),
         "flow" (one dependency-driven launch per pass), "xcd" (the same with each world's
-        manifolds on one XCD) or "cooperative" (one launch per step)."""
+        manifolds on one XCD), "cooperative" (one launch per step) or "worlds" (one
+        workgroup per world per pass, bodies in LDS)."""
         check(self.L.pgx_world_set_exec(self.h, self.EXEC_MODES[mode], blocks))
 
     def set_graph(self, enabled=True):
@@ -291,6 +293,11 @@ class World:
         t = PgxTuning()
         check(self.L.pgx_world_autotune(self.h, dt, int(trial_steps), ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in PgxTuning._fields_}
+
+    def exec_mode(self):
+        m = ctypes.c_int(0)
+        check(self.L.pgx_world_get_exec(self.h, ctypes.byref(m)))
+        return {v: k for k, v in self.EXEC_MODES.items()}[m.value]
 
     def reset_stats(self):
         check(self.L.pgx_world_reset_stats(self.h))

@@ -63,8 +63,10 @@ constexpr int JOINT_MAX_ROWS = 7;      // a prismatic joint with its limit and d
 // substep-loop execution: one launch per colour and pass; one cooperative launch per step
 // (grid barriers between colours); one dependency-driven launch per pass
 // (one device-wide queue), the same with a queue per XCD
-enum { EXEC_COLORS = 0, EXEC_COOPERATIVE = 1, EXEC_FLOW = 2, EXEC_FLOW_XCD = 3 };
+// (one launch per pass); one workgroup per world per pass, bodies in LDS
+enum { EXEC_COLORS = 0, EXEC_COOPERATIVE = 1, EXEC_FLOW = 2, EXEC_FLOW_XCD = 3, EXEC_WORLDS = 4 };
 constexpr int FLOW_QUEUES = 8;         // queues of a dependency-driven pass: at most one per XCD
+constexpr int WORLD_SMALL = 256;       // per-world passes: worlds up to this many dynamic bodies share a launch
 // convex narrowphase: EPA polytope capacity per lane and lanes of the convex kernel
 constexpr int EPA_MAXV = 104;          // 4 simplex vertices + at most 100 insertions
 constexpr int EPA_MAXF = 256;          // > 2 * EPA_MAXV - 4, the face count of a closed triangulation
@@ -264,6 +266,15 @@ struct World {
         int eagerMax = 32768;            // colours of at most this many manifolds load rows eagerly
     } tune;
     bool graph = false;                  // substep loop as one HIP graph launch (colour mode)
+    // per-world passes: first dynamic body of each world (W + 1), whether every world's
+    // dynamic bodies are contiguous, the largest world, the LDS limit; per solve slot its
+    // world; per (colour, world) its slots
+    std::vector<int32_t> wDynHost;
+    DBuf<int32_t> wDyn, sWorld, wcS, wcE;
+    std::vector<int32_t> worldListHost[2];   // worlds of at most WORLD_SMALL dynamic bodies, and the rest
+    DBuf<int32_t> worldList[2];
+    bool worldsContiguous = false;
+    int maxWorldDyn = 0, ldsMax = 0;
     hipGraphExec_t subExec = nullptr;    // the last instantiated substep-loop graph
     long graphInstantiations = 0, graphUpdates = 0;
     int persistentBlocks = 0, persistentBlocksWanted = 0;
@@ -305,11 +316,14 @@ struct SceneArrays {
     std::vector<uint32_t> bent, cent;
     std::vector<int32_t> cbody, cworld, clocal, cflags, ctype, cindex, wstart, colOfLocal, dynCols, staticCols, wss;
     std::vector<uint8_t> kinematic;
+    std::vector<int32_t> dynWorld;        // world of each dynamic body
 };
 void convert_scene(const pgx_scene_desc* sc, uint32_t entityBase, SceneArrays& o);
 void world_reset_host(World& w);
 void world_set_shape(World& w, const pgx_config* cfg, int A, int D, int C, int W, int nDynCol, int maxPerWorld);
 void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh);
+void world_set_dyn_ranges(World& w, const std::vector<int32_t>& dynWorld);
+void world_set_dyn_starts(World& w, const std::vector<int32_t>& starts, bool contiguous);
 void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc);      // world.hip
 void world_add_joints(World& w, int n, const pgx_joint_desc* js);
 void world_step(World& w, float dt);
@@ -324,6 +338,7 @@ int sv_color(World& w, int nManifolds);                 // returns colour count
 void sv_substeps(World& w, int nManifolds, float dt);
 void sv_finish_step(World& w, int nManifolds);
 double sv_step_bytes(const World& w, int M, long points);
+bool sv_world_pass_fits(World& w);                  // per-world passes possible for this world
 
 // joints.hip
 void jt_step_setup(World& w);

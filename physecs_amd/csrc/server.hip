@@ -428,6 +428,11 @@ void assemble(pgx_server& S) {
         std::copy(r.bodyEntity.begin() + nD, r.bodyEntity.end(), w.hostEntity.begin() + Dt + r.sOff);
         std::copy(r.bodyKinematic.begin(), r.bodyKinematic.end(), w.hostKinematic.begin() + r.dOff);
     }
+    {
+        std::vector<int32_t> starts(W + 1, Dt);   // each request's dynamic bodies are one range
+        for (int i = 0; i < W; ++i) starts[i] = S.reqs[S.active[i]].dOff;
+        world_set_dyn_starts(w, starts, true);
+    }
     w.hostColBody.clear();   // per-contact reports are not served
     w.hostColIndex.clear();
     world_alloc_derived(w, &S.cfg, 0);

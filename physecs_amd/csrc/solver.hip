@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void k_setup(
     int32_t* __restrict__ curTri, int32_t* __restrict__ curNum,
     float4* __restrict__ curLp, int32_t* __restrict__ sB0, int32_t* __restrict__ sB1, int32_t* __restrict__ sA0,
     int32_t* __restrict__ sA1, int32_t* __restrict__ sFlags, int32_t* __restrict__ sCanon, float4* __restrict__ sN,
-    float2* __restrict__ sSoft, float4* __restrict__ sR0, float4* __restrict__ sR1) {
+    float2* __restrict__ sSoft, float4* __restrict__ sR0, float4* __restrict__ sR1, int32_t* __restrict__ sWorld) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= M) return;
     int m = perm[s];
@@ -182,6 +182,7 @@ __global__ __launch_bounds__(256) void k_setup(
     uint64_t key = mfKey[m];
     int c0, c1, wd;
     pair_colliders(key, localBits, worldColStart, colOfLocal, c0, c1, wd);
+    sWorld[s] = wd;
     int a0 = cbody[c0], a1 = cbody[c1];
     int b0 = mfB0[m], b1 = mfB1[m];
     float4 m0 = cmat[c0], m1 = cmat[c1];   // friction, restitution, damping
@@ -610,6 +611,67 @@ __global__ __launch_bounds__(BATCH_THREADS) void k_solve_batch(SubArgs a, int c0
     }
 }
 
+// ------------------------------------------------------------------ per-world passes
+// Worlds never share a body, and a world's dynamic bodies are a contiguous index range, so
+// one workgroup can run a whole contact pass of one world: the solver velocities of its
+// bodies in LDS, its manifolds colour by colour with a workgroup barrier between colours.
+// Every body sees its manifolds in colour order, as with one launch per colour, so the
+// floats are the same. No launch per colour and no global body gathers inside the pass.
+struct WorldPassArgs {
+    int W, nColors;
+    const int32_t* wDyn;                  // W + 1: first dynamic body of each world
+    const int32_t *wcS, *wcE;             // per colour and world: its solve slots [S, E)
+    const int32_t* list;                  // the worlds of this launch (one workgroup each)
+};
+
+// the solve slots of every (colour, world) run; slots are in colour order and, within a
+// colour, in world order. Absent runs stay [0, 0).
+__global__ void k_world_ranges(int M, int W, const int32_t* __restrict__ sFlags, const int32_t* __restrict__ sWorld,
+                               int32_t* __restrict__ wcS, int32_t* __restrict__ wcE) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= M) return;
+    const int c = (sFlags[s] >> 16) & 127, wd = sWorld[s];
+    if (s == 0 || ((sFlags[s - 1] >> 16) & 127) != c || sWorld[s - 1] != wd) wcS[c * W + wd] = s;
+    if (s == M - 1 || ((sFlags[s + 1] >> 16) & 127) != c || sWorld[s + 1] != wd) wcE[c * W + wd] = s + 1;
+}
+
+// One wave per world: a world's colour holds a few dozen manifolds, and the solve's
+// register footprint allows 8 waves per CU, so one-wave workgroups keep 8 worlds resident
+// per CU (when their LDS fits). Rows are loaded eagerly: these colours are latency-bound.
+constexpr int WORLD_THREADS = 64;
+__global__ __launch_bounds__(WORLD_THREADS) void k_world_pass(SubArgs a, WorldPassArgs p, int useBias, float h,
+                                                               int skipSoft) {
+    extern __shared__ float4 wv[];        // [n] solver v, then [n] solver w of the world's bodies
+    const int wd = p.list[blockIdx.x];
+    const int d0 = p.wDyn[wd], n = p.wDyn[wd + 1] - d0;
+    if (n == 0) return;
+    for (int i = threadIdx.x; i < n; i += WORLD_THREADS) {
+        wv[i] = a.bvel[bv(d0 + i, BV_V)];
+        wv[n + i] = a.bvel[bv(d0 + i, BV_W)];
+    }
+    __syncthreads();
+    for (int c = 0; c < p.nColors; ++c) {
+        const int S = p.wcS[c * p.W + wd], E = p.wcE[c * p.W + wd];
+        for (int s = S + (int)threadIdx.x; s < E; s += WORLD_THREADS) {
+            MInputs in;
+            load_inputs<true>(a, s, in);
+            if (skipSoft && in.soft) continue;
+            const int l0 = in.b0 - d0, l1 = in.b1 - d0;
+            f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
+            if (in.b0 >= 0) { v0 = xyz(wv[l0]); w0 = xyz(wv[n + l0]); }
+            if (in.b1 >= 0) { v1 = xyz(wv[l1]); w1 = xyz(wv[n + l1]); }
+            solve_core(a, s, in, useBias != 0, h, v0, w0, v1, w1);
+            if (in.b0 >= 0) { wv[l0] = f4(v0, 0.f); wv[n + l0] = f4(w0, 0.f); }
+            if (in.b1 >= 0) { wv[l1] = f4(v1, 0.f); wv[n + l1] = f4(w1, 0.f); }
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += WORLD_THREADS) {
+        a.bvel[bv(d0 + i, BV_V)] = wv[i];
+        a.bvel[bv(d0 + i, BV_W)] = wv[n + i];
+    }
+}
+
 // colours of body b, and its colours below colour c
 __device__ __forceinline__ int colours_of(const uint64_t* used, int b) {
     return __popcll(used[2 * b]) + __popcll(used[2 * b + 1]);
@@ -948,7 +1010,7 @@ int sv_color(World& w, int M) {
                        w.cacheLp[prv].p, w.cacheCount[prv], w.manifoldCap, w.cacheKey[cur].p, w.cacheTri[cur].p,
                        w.cacheNum[cur].p, w.cacheLp[cur].p,
                        w.sB0.p, w.sB1.p, w.sA0.p, w.sA1.p, w.sFlags.p, w.sCanon.p, w.sN.p, w.sSoft.p, w.sR0.p,
-                       w.sR1.p);
+                       w.sR1.p, w.sWorld.p);
     PGX_LAUNCH_CHECK();
     return nColors;
 }
@@ -996,6 +1058,14 @@ static void solve_events(World& w, int color, hipEvent_t* start, hipEvent_t* sto
     if ((int)w.solveEvColor.size() <= w.solveEvUsed) w.solveEvColor.resize(w.solveEvUsed + 1);
     w.solveEvColor[w.solveEvUsed] = color;
     ++w.solveEvUsed;
+}
+
+// per-world passes need every world's dynamic bodies contiguous and their solver
+// velocities within one workgroup's LDS
+bool sv_world_pass_fits(World& w) {
+    if (!w.worldsContiguous) return false;
+    if (w.ldsMax == 0) PGX_HIP(hipDeviceGetAttribute(&w.ldsMax, hipDeviceAttributeMaxSharedMemoryPerBlock, w.device));
+    return 2 * sizeof(float4) * (size_t)std::max(1, w.maxWorldDyn) <= (size_t)w.ldsMax;
 }
 
 // one colour [b, e) in workgroups of `block` threads; with events, stamped from the
@@ -1053,6 +1123,26 @@ void sv_substeps(World& w, int M, float dt) {
         }
         PGX_LAUNCH_CHECK();
     }
+    // per-world passes: the (colour, world) runs once per step
+    const bool worlds = w.execMode == EXEC_WORLDS && sv_world_pass_fits(w) && M > 0;
+    WorldPassArgs wp{};
+    if (worlds) {
+        wp.W = w.W; wp.nColors = nColors; wp.wDyn = w.wDyn.p;
+        const size_t cells = (size_t)std::max(nColors, 1) * w.W;
+        w.wcS.alloc(cells); w.wcE.alloc(cells);
+        fill_i32(s, w.wcS.p, 0, (int)cells);
+        fill_i32(s, w.wcE.p, 0, (int)cells);
+        hipLaunchKernelGGL(k_world_ranges, dim3(div_up(M, 256)), dim3(256), 0, s, M, w.W, w.sFlags.p, w.sWorld.p,
+                           w.wcS.p, w.wcE.p);
+        PGX_LAUNCH_CHECK();
+        wp.wcS = w.wcS.p; wp.wcE = w.wcE.p;
+    }
+    // worlds by size: each size class in its own launch, with LDS for its largest world
+    const size_t worldLds[2] = {2 * sizeof(float4) * (size_t)WORLD_SMALL,
+                                2 * sizeof(float4) * (size_t)std::max(1, w.maxWorldDyn)};
+    if (worlds && worldLds[1] > 65536)
+        PGX_HIP(hipFuncSetAttribute((const void*)k_world_pass, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)worldLds[1]));
     w.lastQueues = flow && M > 0 ? f.nQueues : 0;
     w.lastFlowPasses = flow && M > 0 ? w.substeps * (w.iterations + 1) : 0;
     // waves of the dependency-driven pass: enough to hold every manifold at once, at most
@@ -1087,6 +1177,25 @@ void sv_substeps(World& w, int M, float dt) {
                 hipExtLaunchKernelGGL(k_solve_flow, dim3(flowBlocks), dim3(256), 0, s, e0, e1, 0, a, f);
             } else {
                 hipLaunchKernelGGL(k_solve_flow, dim3(flowBlocks), dim3(256), 0, s, a, f);
+            }
+            return;
+        }
+        if (worlds) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (rec) solve_events(w, -1, &e0, &e1);
+            for (int k = 0; k < 2; ++k) {
+                const int nw = (int)w.worldListHost[k].size();
+                if (!nw) continue;
+                WorldPassArgs q = wp;
+                q.list = w.worldList[k].p;
+                // the events bracket both launches: the first stamps the start, the last the end
+                const bool first = k == 0 || w.worldListHost[0].empty(), last = k == 1 || w.worldListHost[1].empty();
+                if (rec && (first || last))
+                    hipExtLaunchKernelGGL(k_world_pass, dim3(nw), dim3(WORLD_THREADS), worldLds[k], s,
+                                          first ? e0 : nullptr, last ? e1 : nullptr, 0, a, q, useBias, h, skipSoft);
+                else
+                    hipLaunchKernelGGL(k_world_pass, dim3(nw), dim3(WORLD_THREADS), worldLds[k], s, a, q, useBias, h,
+                                       skipSoft);
             }
             return;
         }

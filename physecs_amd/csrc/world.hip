@@ -161,6 +161,8 @@ void convert_scene(const pgx_scene_desc* sc, uint32_t entityBase, SceneArrays& o
                                              sc->inv_inertia[9 * d + 3 * k + 2], 0.f);
     }
     o.kinematic.assign(sc->kinematic, sc->kinematic + D);
+    o.dynWorld.resize(D);
+    for (int d = 0; d < D; ++d) o.dynWorld[d] = sc->actor_world ? sc->actor_world[sc->actor_of_dyn[d]] : 0;
     // colliders (descriptor order), world-local indices
     o.cbody.resize(C); o.cworld.resize(C); o.clocal.resize(C); o.cflags.resize(C); o.ctype.resize(C);
     o.cindex.resize(C); o.cent.resize(C); o.cparams.resize(C); o.clpos.resize(C); o.clquat.resize(C); o.cmat.resize(C);
@@ -235,6 +237,34 @@ void world_set_shape(World& w, const pgx_config* cfg, int A, int D, int C, int W
         throw PgxError{PGX_ERR_INVALID, "too many worlds or colliders per world for 64-bit keys"};
 }
 
+// First dynamic body of each world (W + 1 entries) for the per-world passes, which need
+// each world's dynamic bodies to be one contiguous range (`contiguous`).
+void world_set_dyn_starts(World& w, const std::vector<int32_t>& starts, bool contiguous) {
+    w.wDynHost = starts;
+    w.worldsContiguous = contiguous;
+    w.maxWorldDyn = 0;
+    for (int k = 0; k < 2; ++k) w.worldListHost[k].clear();
+    for (size_t i = 0; i + 1 < starts.size(); ++i) {
+        const int n = starts[i + 1] - starts[i];
+        w.maxWorldDyn = std::max(w.maxWorldDyn, n);
+        if (n > 0) w.worldListHost[n <= WORLD_SMALL ? 0 : 1].push_back((int32_t)i);
+    }
+    upload(w.wDyn, w.wDynHost, w.stream);
+    for (int k = 0; k < 2; ++k) upload(w.worldList[k], w.worldListHost[k], w.stream);
+}
+
+// the same from the world of every dynamic body
+void world_set_dyn_ranges(World& w, const std::vector<int32_t>& dynWorld) {
+    std::vector<int32_t> starts(w.W + 1, 0);
+    bool contiguous = true;
+    for (size_t d = 0; d < dynWorld.size(); ++d) {
+        ++starts[dynWorld[d] + 1];
+        if (d > 0 && dynWorld[d] < dynWorld[d - 1]) contiguous = false;
+    }
+    for (int i = 0; i < w.W; ++i) starts[i + 1] += starts[i];
+    world_set_dyn_starts(w, starts, contiguous);
+}
+
 // Every buffer whose size follows from the world's shape (bodies, colliders, worlds,
 // dynamic colliders) and the pair capacity; reused where large enough.
 void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh) {
@@ -264,7 +294,7 @@ void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh) {
     w.meshCount.alloc(w.pairCap); w.meshFirst.alloc(w.pairCap); w.npTri.alloc(std::max(1, w.mmCap));
     size_t M = w.manifoldCap;
     w.mfKey.alloc(M); w.mfPair.alloc(M); w.mfB0.alloc(M); w.mfB1.alloc(M); w.mfColor.alloc(M);
-    w.mfSrc.alloc(M); w.mfTri.alloc(M);
+    w.mfSrc.alloc(M); w.mfTri.alloc(M); w.sWorld.alloc(M);
     w.claim.alloc(std::max(D, 1)); w.usedMask.alloc(2 * (size_t)std::max(D, 1));
     w.flowPub.alloc(6 * (size_t)std::max(D, 1));
     // versions start at zero; a launch's versions carry its epoch, which starts at one
@@ -302,6 +332,7 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     w.hostKinematic = o.kinematic;
     w.hostColIndex = o.cindex;
     w.hostEntity = o.bent;
+    world_set_dyn_ranges(w, o.dynWorld);
 
     hipStream_t s = w.stream;
     upload(w.bpos, o.bpos, s); upload(w.bquat, o.bquat, s); upload(w.bentity, o.bent, s);
@@ -697,7 +728,8 @@ static void restore_snapshot(World& w, const Snapshot& z) {
     PGX_HIP(hipStreamSynchronize(s));
 }
 
-// Picks the launch shapes by timing trial steps from the world's current state (one
+// Picks the launch shapes (and between per-colour launches and per-world passes) by
+// timing trial steps from the world's current state (one
 // untimed step, then trial_steps timed, wall clock with the stream drained; the better of
 // two such runs), one parameter at a time, in the order below. A value replaces the
 // current one only when it is at least 3% faster, so timing noise does not move the
@@ -728,6 +760,9 @@ void world_autotune(World& w, float dt, int trialSteps) {
                                    {&w.tune.eagerMax, {0, 8192, 32768, 131072, 1 << 30}},
                                    {&w.tune.pairsPerLane, {1, 2, 4}},
                                    {&w.tune.colorRounds, {8, 16, 32}}};
+        // per-colour launches or per-world passes (the same floats either way)
+        if ((w.execMode == EXEC_COLORS || w.execMode == EXEC_WORLDS) && sv_world_pass_fits(w))
+            knobs.insert(knobs.begin(), Knob{&w.execMode, {EXEC_COLORS, EXEC_WORLDS}});
         double best = trial();
         for (Knob& k : knobs) {
             const int keep = *k.field;
@@ -1106,7 +1141,7 @@ int pgx_world_get_stats(pgx_world* h, pgx_stats* out) {
 int pgx_world_set_exec(pgx_world* h, int mode, int blocks) {
     if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
     if (blocks < 0) return fail(PGX_ERR_INVALID, "blocks must be >= 0");
-    if (mode < PGX_EXEC_COLORS || mode > PGX_EXEC_FLOW_XCD) return fail(PGX_ERR_INVALID, "unknown execution mode %d", mode);
+    if (mode < PGX_EXEC_COLORS || mode > PGX_EXEC_WORLDS) return fail(PGX_ERR_INVALID, "unknown execution mode %d", mode);
     h->w.execMode = mode;
     h->w.persistentBlocksWanted = blocks;
     h->w.persistentBlocks = 0;
@@ -1170,6 +1205,12 @@ int pgx_world_autotune(pgx_world* h, float dt, int trial_steps, pgx_tuning* chos
 int pgx_world_set_graph(pgx_world* h, int enabled) {
     if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
     h->w.graph = enabled != 0;
+    return PGX_OK;
+}
+
+int pgx_world_get_exec(pgx_world* h, int* mode) {
+    if (!h || !mode) return fail(PGX_ERR_INVALID, "NULL argument");
+    *mode = h->w.execMode;
     return PGX_OK;
 }
 

@@ -25,10 +25,11 @@ def world_rows(scene, w):
     return actors, dyn
 
 
-@pytest.mark.parametrize("mode", ["colors", "xcd"])
+@pytest.mark.parametrize("mode", ["colors", "xcd", "worlds"])
 def test_world_same_alone_and_in_mix(api, mode):
     """A world steps bit-identically alone and among 23 others, with the contact passes
-    launched per colour or on per-XCD queues (worlds spread over the XCDs)."""
+    launched per colour, on per-XCD queues (worlds spread over the XCDs) or one workgroup
+    per world."""
     sc = scenes.batched_worlds(3, n_worlds=24, lo=8, hi=160, ragdoll_frac=0.5)
     log(f"[batched] 24 worlds, {sc.n_dyn} bodies, {len(sc.joints)} joints: mixed run, exec {mode}")
     steps = 40
