@@ -1,6 +1,6 @@
 """Benchmark of the rigid-body step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain32k|pyramid|batched|convex|served]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain32k|pyramid|batched|convex|served|raycast]
     python bench.py --copy-calibration      # a plain 1 GiB device copy, for scaling PMC byte counters
 
 A step is one Scene::simulate(1/60) with 4 substeps and 2 iterations over the whole scene,
@@ -190,6 +190,47 @@ def run_served(args):
     print(json.dumps(line), flush=True)
 
 
+def run_raycast(args):
+    """Scene::raycastClosest for 1M rays (pgx_world_raycast) into the 32,768-body scene
+    after 100 steps: rays from a box around the container toward random points inside it,
+    half of them normalised, maxDistance 200. value = rays per second of device time (the
+    LBVH rebuild from the current bounds plus the ray kernel, HIP events); the wall time of
+    the whole call, host arrays and PCIe included, is reported beside it."""
+    from physecs_amd import api, scenes
+    sc = scenes.mixed_lattice(0, side=32)
+    w = api.World(sc, substeps=4, iterations=2)
+    for k in range(100):
+        w.step(1 / 60)
+    w.sync()
+    rng = np.random.default_rng(0)
+    n = 1 << 20
+    o = rng.uniform((-30, 0, -30), (30, 60, 30), (n, 3)).astype(np.float32)
+    tgt = rng.uniform((-25, 0, -25), (25, 50, 25), (n, 3))
+    d = tgt - o
+    half = rng.random(n) < 0.5
+    d[half] /= np.linalg.norm(d[half], axis=1, keepdims=True)
+    d = d.astype(np.float32)
+    maxd = np.full(n, 200.0, np.float32)
+    w.raycast(o[:1024], d[:1024], maxd[:1024])   # warm-up
+    dev, wall, hits = [], [], 0
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        col, t, us = w.raycast(o, d, maxd)
+        wall.append(time.perf_counter() - t0)
+        dev.append(us * 1e-6)
+        hits = int((col >= 0).sum())
+    value = n / float(np.median(dev))
+    line = {"metric": "rays/s (Scene::raycastClosest, 32k bodies, 1 GPU)", "value": value, "unit": "rays/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": 1000 * float(np.median(dev)),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded scene and rays)",
+            "config": {"workload": "1,048,576 rays into 32,768 mixed bodies after 100 steps", "rays": n,
+                       "hits": hits, "device_ms_median": 1000 * float(np.median(dev)),
+                       "wall_ms_median_with_pcie": 1000 * float(np.median(wall)), "parallelism": "single GPU"},
+            "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +257,11 @@ def main():
         return
     if args.config == "served":
         run_served(args)
+        return
+    if args.config == "raycast":
+        if args.steps == 590:
+            args.steps = 10
+        run_raycast(args)
         return
     from physecs_amd import api
     print(f"building the {args.config} scene", file=sys.stderr, flush=True)

@@ -253,6 +253,22 @@ int pgx_world_reset_stats(pgx_world* w);
 int pgx_world_set_exec(pgx_world* w, int mode, int blocks);
 int pgx_world_get_exec(pgx_world* w, int* mode);
 int pgx_world_get_stats(pgx_world* w, pgx_stats* out);
+
+/* Scene::raycastClosest (include/Physecs/Physecs.h:216-217, src/Physecs.cpp:571-621) for n
+ * rays at once, between steps, on the world's current poses and broadphase bounds. Ray i
+ * starts at origins[3i..], runs along dirs[3i..] (not normalised: distances are in units
+ * of |dir|, as in the reference) up to max_distance[i], in world ray_world[i] (NULL: world
+ * 0), and rejects the entity ignore_entity[i] (NULL or 0xffffffff: none), the form of the
+ * reference's filter argument this boundary takes; the reference's unfiltered overload
+ * passes an empty filter, which its leaf test never accepts, so it reports no hit.
+ * Results: hit_collider[i] = the closest hit collider (descriptor index) or -1, and
+ * hit_distance[i] its distance t (the reference's hitPos = origin + dir * t). Equal
+ * distances go to the lower collider index (the reference: its tree's shape). Triangle
+ * meshes are never hit (src/Raycast.cpp:148). device_us (may be NULL): the query's device
+ * time, tree build included, by HIP events. */
+int pgx_world_raycast(pgx_world* w, int n, const float* origins, const float* dirs, const float* max_distance,
+                      const int32_t* ray_world, const uint32_t* ignore_entity, int32_t* hit_collider,
+                      float* hit_distance, double* device_us);
 /* In PGX_EXEC_COLORS, enabled != 0 launches each step's substep loop as one HIP graph:
  * captured every step (its launches follow the step's colours), the instantiated graph
  * updated in place when the launch sequence keeps its shape. Results are the same either

@@ -1297,3 +1297,164 @@ int orc_row(int simd, int flags, int b0, int b1, const float* in, const float* m
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ scene raycast
+namespace {
+
+// intersectRayAABB (src/Raycast.cpp:5-30)
+bool rayAABB(V3 o, V3 d, V3 mn, V3 mx, float& t) {
+    float tMin = 0.f;
+    float tMax = std::numeric_limits<float>::max();
+    const float ov[3] = {o.x, o.y, o.z}, dv[3] = {d.x, d.y, d.z}, lo[3] = {mn.x, mn.y, mn.z}, hi[3] = {mx.x, mx.y, mx.z};
+    for (int i = 0; i < 3; ++i) {
+        if (gabs(dv[i]) < 0.001f) {
+            if (ov[i] < lo[i] || ov[i] > hi[i]) return false;   // parallel to the slab, outside it
+        } else {
+            float ood = 1.f / dv[i];
+            float t1 = (lo[i] - ov[i]) * ood;
+            float t2 = (hi[i] - ov[i]) * ood;
+            if (t1 > t2) std::swap(t1, t2);
+            if (t1 > tMin) tMin = t1;
+            if (t2 < tMax) tMax = t2;
+            if (tMin > tMax) return false;
+        }
+    }
+    t = tMin;
+    return true;
+}
+
+// intersectRaySphere (src/Raycast.cpp:32-46)
+bool raySphere(V3 o, V3 d, V3 pos, float radius, float& t) {
+    V3 m = o - pos;
+    float b = dot(m, d);
+    float c = dot(m, m) - radius * radius;
+    if (c > 0.f && b > 0.f) return false;
+    float discr = b * b - c;
+    if (discr < 0.f) return false;
+    t = -b - std::sqrt(discr);
+    if (t < 0.f) t = 0.f;
+    return true;
+}
+
+// intersectRayCapsule (src/Raycast.cpp:48-103): top hemisphere, cylinder, bottom hemisphere
+bool rayCapsule(V3 o, V3 d0, V3 pos, Q ori, float hh, float radius, float& t) {
+    Q inv = qinverse(ori);
+    V3 p = qrot(inv, o - pos);
+    V3 d = qrot(inv, d0);
+    float pd = dot(p, d);
+    float hdy = hh * d.y;
+    float thpy = 2.f * hh * p.y;
+    float phr = dot(p, p) + hh * hh - radius * radius;
+    float tMin = std::numeric_limits<float>::max();
+    bool hit = false;
+    float b = pd - hdy, c = phr - thpy, discr = b * b - c;
+    if (discr >= 0.f) {
+        t = -b - std::sqrt(discr);
+        if (t < 0.f) t = 0.f;
+        if ((p + t * d).y >= hh) { hit = true; tMin = t; }
+    }
+    float a = d.x * d.x + d.z * d.z;
+    b = p.x * d.x + p.z * d.z;
+    c = p.x * p.x + p.z * p.z - radius * radius;
+    discr = b * b - a * c;
+    if (discr >= 0.f) {
+        t = (-b - std::sqrt(discr)) / a;
+        if (t < 0.f) t = 0.f;
+        float y = (p + t * d).y;
+        if (-hh <= y && y < hh && t < tMin) { hit = true; tMin = t; }
+    }
+    b = pd + hdy;
+    c = phr + thpy;
+    discr = b * b - c;
+    if (discr >= 0.f) {
+        t = -b - std::sqrt(discr);
+        if (t < 0.f) t = 0.f;
+        if ((p + t * d).y < -hh && t < tMin) { hit = true; tMin = t; }
+    }
+    t = tMin;
+    return hit;
+}
+
+// intersectRayBox (src/Raycast.cpp:105-110)
+bool rayBox(V3 o, V3 d, V3 pos, Q ori, V3 he, float& t) {
+    Q inv = qinverse(ori);
+    return rayAABB(qrot(inv, o - pos), qrot(inv, d), -he, he, t);
+}
+
+// intersectRayConvexMesh (src/Raycast.cpp:112-139): the faces' half-spaces, in face order
+bool rayConvex(V3 o, V3 d, V3 pos, Q ori, const ConvexMesh* mesh, V3 scale, float& t) {
+    float tMin = 0.f;
+    float tMax = std::numeric_limits<float>::max();
+    for (const ConvexFace& f : mesh->faces) {
+        V3 planeOrig = pos + qrot(ori, scale * f.centroid);
+        V3 planeNormal = qrot(ori, (v3s(1.f) / scale) * f.normal);
+        float denom = dot(d, planeNormal);
+        float dist = dot(planeOrig - o, planeNormal);
+        if (denom == 0.f) {
+            if (dist > 0) return false;
+        } else {
+            t = dist / denom;
+            if (denom < 0) { if (t > tMin) tMin = t; }
+            else { if (t < tMax) tMax = t; }
+            if (tMin > tMax) return false;
+        }
+    }
+    t = tMin;
+    return true;
+}
+
+// intersectRayGeometry (src/Raycast.cpp:141-150); triangle meshes are never hit
+bool rayGeometry(V3 o, V3 d, V3 pos, Q ori, const Geom& g, float& t) {
+    switch (g.type) {
+        case SPHERE: return raySphere(o, d, pos, g.p[0], t);
+        case CAPSULE: return rayCapsule(o, d, pos, ori, g.p[0], g.p[1], t);
+        case BOX: return rayBox(o, d, pos, ori, v3(g.p[0], g.p[1], g.p[2]), t);
+        case CONVEX_MESH: return rayConvex(o, d, pos, ori, g.mesh, v3(g.p[0], g.p[1], g.p[2]), t);
+        default: return false;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_ray_aabb(const float* o, const float* d, const float* mn, const float* mx, float* t) {
+    return rayAABB(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), v3(mn[0], mn[1], mn[2]), v3(mx[0], mx[1], mx[2]), *t) ? 1 : 0;
+}
+
+int orc_ray_geometry(const float* o, const float* d, int type, const float* prm, const float* pos, const float* q,
+                     float* t) {
+    Geom g = pinGeom(type, prm);
+    return rayGeometry(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), v3(pos[0], pos[1], pos[2]),
+                       qmake(q[3], q[0], q[1], q[2]), g, *t) ? 1 : 0;
+}
+
+// Scene::raycastClosest (src/Physecs.cpp:571-621) over the world's colliders with their
+// current broadphase bounds. The reference walks its dynamic AABB tree; a leaf is reached
+// when the ray enters the leaf's own bounds within maxDistance (every ancestor's box holds
+// those bounds), so the accepted hits, and the closest of them, are the ones this loop
+// over every collider finds. Equal distances go to the lower collider index here; the
+// reference settles them by its tree's shape. The filter rejects one entity (`ignore`;
+// 0xffffffff rejects none); the reference's own unfiltered overload passes an empty
+// std::function, which its leaf test (`filter && filter(entity)`) never accepts.
+// Returns the collider index hit, or -1; *tOut is the hit distance along d.
+int orc_world_raycast(const orc_world* w, const float* o, const float* d, float maxDistance, uint32_t ignore,
+                      float* tOut) {
+    V3 ro = v3(o[0], o[1], o[2]), rd = v3(d[0], d[1], d[2]);
+    int best = -1;
+    float bestT = 0.f;
+    for (const BPEntry& e : w->bp) {
+        const Collider& c = w->cols[e.collider];
+        const Actor& a = w->actors[c.actor];
+        if (a.entity == ignore) continue;
+        float t;
+        if (!rayAABB(ro, rd, e.bounds.min, e.bounds.max, t) || t > maxDistance) continue;
+        if (!rayGeometry(ro, rd, a.pos + qrot(a.quat, c.lpos), qmul(a.quat, c.lquat), c.geom, t)) continue;
+        if (t > maxDistance) continue;
+        if (best < 0 || t < bestT || (t == bestT && e.collider < best)) { best = e.collider; bestT = t; }
+    }
+    if (best >= 0) *tOut = bestT;
+    return best;
+}
+
+}  // extern "C"

@@ -26,6 +26,7 @@
 #include "GeomUtil.h"
 #include "MassUtil.h"
 #include "MathUtil.h"
+#include "Raycast.h"
 #include "SolverData.h"
 #include "TriangleMesh.h"
 #include "Constraint1D.h"
@@ -305,6 +306,18 @@ int ref_row_scalar(int flags, int b0, int b1, const float* in, const float* mass
         case ANGULAR | LIMITED: runRow<ANGULAR | LIMITED>(b0, b1, in, masses, vel, pv, h, iters, out); return 0;
     }
     return -1;
+}
+
+// intersectRayAABB (src/Raycast.cpp:5-30): returns the hit, t when hit (else untouched)
+int ref_ray_aabb(const float* o, const float* d, const float* mn, const float* mx, float* t) {
+    return intersectRayAABB(V(o), V(d), V(mn), V(mx), *t) ? 1 : 0;
+}
+
+// intersectRayGeometry (src/Raycast.cpp:141-150)
+int ref_ray_geometry(const float* o, const float* d, int type, const float* prm, const float* pos, const float* q,
+                     float* t) {
+    Geometry g = makeGeom(type, prm);
+    return intersectRayGeometry(V(o), V(d), V(pos), Qt(q), g, *t) ? 1 : 0;
 }
 
 }  // extern "C"

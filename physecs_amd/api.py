@@ -28,7 +28,7 @@ EXPORTS = [
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
-    "pgx_world_autotune", "pgx_world_get_exec",
+    "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -129,6 +129,9 @@ def lib():
         L.pgx_world_get_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
         L.pgx_world_autotune.argtypes = [vp, ctypes.c_float, ctypes.c_int, ctypes.POINTER(PgxTuning)]
         L.pgx_world_get_exec.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.pgx_world_raycast.argtypes = [vp, ctypes.c_int, F, F, F, ctypes.POINTER(ctypes.c_int32),
+                                        ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int32), F,
+                                        ctypes.POINTER(ctypes.c_double)]
         L.pgx_server_set_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
         L.pgx_server_create.argtypes = [ctypes.POINTER(PgxConfig), ctypes.POINTER(PgxServerConfig), ctypes.POINTER(vp)]
         L.pgx_server_destroy.argtypes = [vp]
@@ -293,6 +296,24 @@ class World:
         t = PgxTuning()
         check(self.L.pgx_world_autotune(self.h, dt, int(trial_steps), ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in PgxTuning._fields_}
+
+    def raycast(self, origins, dirs, max_dist, world=None, ignore=None):
+        """pgx_world_raycast: Scene::raycastClosest for every ray. Returns (collider index
+        or -1, hit distance, device microseconds)."""
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = len(o)
+        md = np.ascontiguousarray(np.broadcast_to(np.asarray(max_dist, np.float32), (n,)))
+        wd = None if world is None else np.ascontiguousarray(np.broadcast_to(np.asarray(world, np.int32), (n,)))
+        ig = None if ignore is None else np.ascontiguousarray(np.asarray(ignore, np.uint32))
+        col = np.full(n, -1, np.int32)
+        t = np.zeros(n, np.float32)
+        us = ctypes.c_double(0)
+        check(self.L.pgx_world_raycast(self.h, n, _f(o), _f(d), _f(md),
+                                       None if wd is None else wd.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                       None if ig is None else ig.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                       col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _f(t), ctypes.byref(us)))
+        return col, t, us.value
 
     def exec_mode(self):
         m = ctypes.c_int(0)

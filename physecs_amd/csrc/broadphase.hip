@@ -483,14 +483,13 @@ __global__ void k_single_leaf_root(int32_t* worldRoot, int W, const uint64_t* mk
     if (threadIdx.x == 0 && blockIdx.x == 0) worldRoot[(int)(mkeys[0] >> worldShift)] = 0 | LEAF_BIT;
 }
 
-int bp_find_pairs(World& w) {
+// The LBVH of every world's dynamic colliders from the current bounds: Morton keys,
+// sorted leaves, Karras nodes, refit, escape links, packed traversal records.
+void bp_build(World& w) {
     hipStream_t s = w.stream;
     int n = w.nDynCol;
     int W = w.W;
     int worldShift = 30;
-    // [8] convex pairs, [9] EPA capacity errors, [10] mesh pairs, [11] mesh manifold slots,
-    // [12] mesh capacity errors, [13] mesh candidate triangles
-    fill_i32(s, w.counters.p, 0, 14);
     if (n > 0) {
         // quantisation box -> Morton codes of the dynamic colliders -> sort
         hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p);
@@ -529,8 +528,20 @@ int bp_find_pairs(World& w) {
                            w.nodeEsc.p, w.nodeRange.p, w.leafMin.p, w.leafMax.p, w.leafEsc.p, w.leafFlags.p, w.mvals.p,
                            w.centity.p, w.clocal.p, w.nodeA.p, w.nodeB.p, w.nodeLast.p, w.leafA.p, w.leafB.p,
                            w.leafI.p);
-        // one world with few queries: several lanes per query keep enough waves in flight to
-        // hide the node loads' latency
+        PGX_LAUNCH_CHECK();
+    }
+}
+
+int bp_find_pairs(World& w) {
+    hipStream_t s = w.stream;
+    int n = w.nDynCol;
+    int worldShift = 30;
+    // [6] traversal visit bound, [7] refit, [8] convex pairs, [9] EPA capacity errors,
+    // [10] mesh pairs, [11] mesh manifold slots, [12] mesh capacity errors, [13] mesh
+    // candidate triangles
+    fill_i32(s, w.counters.p, 0, 14);
+    bp_build(w);
+    if (n > 0) {
         hipLaunchKernelGGL(k_traverse, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift,
                            w.mvals.p, w.leafA.p, w.leafB.p, w.leafI.p, w.nodeA.p, w.nodeB.p, w.nodeLast.p,
                            w.worldRoot.p, w.worldStop.p, w.worldStaticStart.p, w.staticCols.p, w.cbmin.p, w.cbmax.p,

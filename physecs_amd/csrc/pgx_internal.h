@@ -271,6 +271,13 @@ struct World {
     // world; per (colour, world) its slots
     std::vector<int32_t> wDynHost;
     DBuf<int32_t> wDyn, sWorld, wcS, wcE;
+    // scene queries: rays (origin + maxDistance, direction + world), rejected entities,
+    // results, the events around a query's device work
+    DBuf<float4> qRays;
+    DBuf<uint32_t> qIgnore;
+    DBuf<int32_t> qHit;
+    DBuf<float> qT;
+    hipEvent_t qEv[2] = {};
     std::vector<int32_t> worldListHost[2];   // worlds of at most WORLD_SMALL dynamic bodies, and the rest
     DBuf<int32_t> worldList[2];
     bool worldsContiguous = false;
@@ -304,6 +311,11 @@ struct World {
 // colliders; worldMask (device, one byte per world) limits the update to the marked worlds
 void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin, const uint8_t* worldMask = nullptr);
 int bp_find_pairs(World& w);          // returns pair count (host)
+void bp_build(World& w);              // the LBVH from the current bounds (no pairs)
+// queries.hip: Scene::raycastClosest for n rays (host arrays)
+void qr_raycast(World& w, int n, const float* origins, const float* dirs, const float* maxDist,
+                const int32_t* rayWorld, const uint32_t* ignoreEntity, int32_t* hitCollider, float* hitDistance,
+                double* deviceUs);
 
 // narrowphase.hip
 void np_run(World& w, int nPairs);

@@ -1,0 +1,245 @@
+// Scene raycasts on the device: Scene::raycastClosest (src/Physecs.cpp:571-621) for a batch
+// of rays, each in one world. A ray walks the LBVH of its world's dynamic colliders (built
+// from the current bounds for the query) and then its world's non-dynamic colliders. The
+// reference walks its dynamic AABB tree; a leaf is reached when the ray enters the leaf's
+// own bounds within maxDistance, and every ancestor's box holds those bounds, so pruning
+// with the same ray-box test on the LBVH's boxes accepts the same hits. Of the accepted
+// hits the closest is kept; equal distances go to the lower collider index (the
+// reference settles them by its tree's shape). The filter rejects one entity per ray.
+#include <cstring>
+#include "pgx_internal.h"
+
+using namespace pgx;
+
+namespace {
+
+__device__ __forceinline__ float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+// intersectRayAABB (src/Raycast.cpp:5-30)
+__device__ bool ray_aabb(f3 o, f3 d, f3 mn, f3 mx, float& t) {
+    float tMin = 0.f;
+    float tMax = 3.402823466e+38f;   // std::numeric_limits<float>::max()
+    for (int i = 0; i < 3; ++i) {
+        const float oi = comp(o, i), di = comp(d, i), lo = comp(mn, i), hi = comp(mx, i);
+        if (gabs(di) < 0.001f) {
+            if (oi < lo || oi > hi) return false;   // parallel to the slab, outside it
+        } else {
+            const float ood = 1.f / di;
+            float t1 = (lo - oi) * ood;
+            float t2 = (hi - oi) * ood;
+            if (t1 > t2) { float x = t1; t1 = t2; t2 = x; }
+            if (t1 > tMin) tMin = t1;
+            if (t2 < tMax) tMax = t2;
+            if (tMin > tMax) return false;
+        }
+    }
+    t = tMin;
+    return true;
+}
+
+// intersectRaySphere (src/Raycast.cpp:32-46)
+__device__ bool ray_sphere(f3 o, f3 d, f3 pos, float radius, float& t) {
+    f3 m = sub(o, pos);
+    float b = dot(m, d);
+    float c = dot(m, m) - radius * radius;
+    if (c > 0.f && b > 0.f) return false;
+    float discr = b * b - c;
+    if (discr < 0.f) return false;
+    t = -b - sqrtf(discr);
+    if (t < 0.f) t = 0.f;
+    return true;
+}
+
+// intersectRayCapsule (src/Raycast.cpp:48-103): top hemisphere, cylinder, bottom hemisphere
+__device__ bool ray_capsule(f3 o, f3 d0, f3 pos, qf ori, float hh, float radius, float& t) {
+    qf inv = qinverse(ori);
+    f3 p = qrot(inv, sub(o, pos));
+    f3 d = qrot(inv, d0);
+    float pd = dot(p, d);
+    float hdy = hh * d.y;
+    float thpy = 2.f * hh * p.y;
+    float phr = dot(p, p) + hh * hh - radius * radius;
+    float tMin = 3.402823466e+38f;
+    bool hit = false;
+    float b = pd - hdy, c = phr - thpy, discr = b * b - c;
+    if (discr >= 0.f) {
+        t = -b - sqrtf(discr);
+        if (t < 0.f) t = 0.f;
+        if (add(p, mul(t, d)).y >= hh) { hit = true; tMin = t; }
+    }
+    float a = d.x * d.x + d.z * d.z;
+    b = p.x * d.x + p.z * d.z;
+    c = p.x * p.x + p.z * p.z - radius * radius;
+    discr = b * b - a * c;
+    if (discr >= 0.f) {
+        t = (-b - sqrtf(discr)) / a;
+        if (t < 0.f) t = 0.f;
+        float y = add(p, mul(t, d)).y;
+        if (-hh <= y && y < hh && t < tMin) { hit = true; tMin = t; }
+    }
+    b = pd + hdy;
+    c = phr + thpy;
+    discr = b * b - c;
+    if (discr >= 0.f) {
+        t = -b - sqrtf(discr);
+        if (t < 0.f) t = 0.f;
+        if (add(p, mul(t, d)).y < -hh && t < tMin) { hit = true; tMin = t; }
+    }
+    t = tMin;
+    return hit;
+}
+
+// intersectRayBox (src/Raycast.cpp:105-110)
+__device__ bool ray_box(f3 o, f3 d, f3 pos, qf ori, f3 he, float& t) {
+    qf inv = qinverse(ori);
+    return ray_aabb(qrot(inv, sub(o, pos)), qrot(inv, d), neg(he), he, t);
+}
+
+// intersectRayConvexMesh (src/Raycast.cpp:112-139): the faces' half-spaces, in face order
+// convex meshes: per mesh {first vertex, vertex count, first face, face count}; per face
+// outward normal and centroid
+struct ConvexFaces { const int4* mesh; const float4 *faceN, *faceC; };
+
+__device__ bool ray_convex(f3 o, f3 d, f3 pos, qf ori, const ConvexFaces& cv, int mesh, f3 scale, float& t) {
+    const int4 mi = cv.mesh[mesh];
+    float tMin = 0.f;
+    float tMax = 3.402823466e+38f;
+    for (int f = 0; f < mi.w; ++f) {
+        f3 planeOrig = add(pos, qrot(ori, mulv(scale, xyz(cv.faceC[mi.z + f]))));
+        f3 planeNormal = qrot(ori, mulv(divv(s3(1.f), scale), xyz(cv.faceN[mi.z + f])));
+        float denom = dot(d, planeNormal);
+        float dist = dot(sub(planeOrig, o), planeNormal);
+        if (denom == 0.f) {
+            if (dist > 0.f) return false;
+        } else {
+            t = dist / denom;
+            if (denom < 0.f) { if (t > tMin) tMin = t; }
+            else { if (t < tMax) tMax = t; }
+            if (tMin > tMax) return false;
+        }
+    }
+    t = tMin;
+    return true;
+}
+
+struct RayScene {
+    int W, nLeaves;
+    const int32_t *worldRoot, *worldStop, *worldStaticStart, *staticCols, *mvals;
+    const float4 *leafA, *nodeA, *nodeB;
+    const float4 *cbmin, *cbmax, *cparams, *clpos, *clquat, *bpos, *bquat;
+    const int32_t *cbody, *ctype;
+    const uint32_t* centity;
+    ConvexFaces cv;
+};
+
+// one collider: the reference's leaf test (its bounds, then its geometry, both within
+// maxDistance, and the filter), kept when closer than the best so far
+__device__ __forceinline__ void test_collider(const RayScene& S, int c, f3 o, f3 d, float maxDist, uint32_t ignore,
+                                              float& bestT, int& best) {
+    if (S.centity[c] == ignore) return;
+    float t;
+    if (!ray_aabb(o, d, xyz(S.cbmin[c]), xyz(S.cbmax[c]), t) || t > maxDist) return;
+    const int b = S.cbody[c];
+    const qf bq = q4(S.bquat[b]);
+    const f3 pos = add(xyz(S.bpos[b]), qrot(bq, xyz(S.clpos[c])));
+    const qf q = qmul(bq, q4(S.clquat[c]));
+    const float4 prm = S.cparams[c];
+    bool hit;
+    switch (S.ctype[c]) {
+        case PGX_SPHERE: hit = ray_sphere(o, d, pos, prm.x, t); break;
+        case PGX_CAPSULE: hit = ray_capsule(o, d, pos, q, prm.x, prm.y, t); break;
+        case PGX_BOX: hit = ray_box(o, d, pos, q, xyz(prm), t); break;
+        case PGX_CONVEX_MESH: hit = ray_convex(o, d, pos, q, S.cv, (int)prm.w, xyz(prm), t); break;
+        default: hit = false;   // triangle meshes are never hit (src/Raycast.cpp:148)
+    }
+    if (!hit || t > maxDist) return;
+    if (best < 0 || t < bestT || (t == bestT && c < best)) { best = c; bestT = t; }
+}
+
+// ray i: origin (.w = maxDistance), direction (.w = world id bits), entity to reject
+__global__ __launch_bounds__(256) void k_raycast(int n, const float4* __restrict__ ro, const float4* __restrict__ rd,
+                                                 const uint32_t* __restrict__ ignore, RayScene S,
+                                                 int32_t* __restrict__ hitCol, float* __restrict__ hitT) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 a = ro[i], b = rd[i];
+    const f3 o = xyz(a), d = xyz(b);
+    const float maxDist = a.w;
+    const int wd = __float_as_int(b.w);
+    const uint32_t ig = ignore[i];
+    float bestT = 0.f;
+    int best = -1;
+    int node = S.worldRoot[wd];
+    if (node != -1) {
+        const int stop = S.worldStop[wd];
+        int visits = 0;
+        while (node != stop && ++visits <= 2 * S.nLeaves) {   // a walk visits at most 2n - 1 nodes
+            if (node & LEAF_BIT) {
+                const int l = node & ~LEAF_BIT;
+                test_collider(S, S.mvals[l], o, d, maxDist, ig, bestT, best);
+                node = __float_as_int(S.leafA[l].w);
+            } else {
+                const float4 na = S.nodeA[node], nb = S.nodeB[node];
+                float tn;
+                node = (ray_aabb(o, d, xyz(na), xyz(nb), tn) && tn <= maxDist) ? __float_as_int(na.w)
+                                                                                  : __float_as_int(nb.w);
+            }
+        }
+    }
+    for (int st = S.worldStaticStart[wd]; st < S.worldStaticStart[wd + 1]; ++st)
+        test_collider(S, S.staticCols[st], o, d, maxDist, ig, bestT, best);
+    hitCol[i] = best;
+    hitT[i] = best >= 0 ? bestT : 0.f;
+}
+
+}  // namespace
+
+namespace pgx {
+
+void qr_raycast(World& w, int n, const float* origins, const float* dirs, const float* maxDist,
+                const int32_t* rayWorld, const uint32_t* ignoreEntity, int32_t* hitCollider, float* hitDistance,
+                double* deviceUs) {
+    hipStream_t s = w.stream;
+    std::vector<float4> hr(2 * (size_t)n);
+    std::vector<uint32_t> hig((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        int32_t wd = rayWorld ? rayWorld[i] : 0;
+        float wbits;
+        std::memcpy(&wbits, &wd, sizeof wbits);
+        hr[i] = make_float4(origins[3 * i], origins[3 * i + 1], origins[3 * i + 2], maxDist[i]);
+        hr[n + i] = make_float4(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2], wbits);
+        hig[i] = ignoreEntity ? ignoreEntity[i] : 0xffffffffu;
+    }
+    w.qRays.alloc(2 * (size_t)n); w.qIgnore.alloc(n); w.qHit.alloc(n); w.qT.alloc(n);
+    PGX_HIP(hipMemcpyAsync(w.qRays.p, hr.data(), sizeof(float4) * 2 * (size_t)n, hipMemcpyHostToDevice, s));
+    PGX_HIP(hipMemcpyAsync(w.qIgnore.p, hig.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    if (!w.qEv[0])
+        for (auto& e : w.qEv) PGX_HIP(hipEventCreate(&e));
+    PGX_HIP(hipEventRecord(w.qEv[0], s));
+    fill_i32(s, w.counters.p + 7, 0, 1);   // the refit's work counter
+    bp_build(w);
+    RayScene S;
+    S.W = w.W;
+    S.nLeaves = w.nDynCol;
+    S.worldRoot = w.worldRoot.p; S.worldStop = w.worldStop.p;
+    S.worldStaticStart = w.worldStaticStart.p; S.staticCols = w.staticCols.p; S.mvals = w.mvals.p;
+    S.leafA = w.leafA.p; S.nodeA = w.nodeA.p; S.nodeB = w.nodeB.p;
+    S.cbmin = w.cbmin.p; S.cbmax = w.cbmax.p; S.cparams = w.cparams.p; S.clpos = w.clpos.p; S.clquat = w.clquat.p;
+    S.bpos = w.bpos.p; S.bquat = w.bquat.p; S.cbody = w.cbody.p; S.ctype = w.ctype.p; S.centity = w.centity.p;
+    S.cv = ConvexFaces{w.cvMesh.p, w.cvFaceN.p, w.cvFaceC.p};
+    if (w.nDynCol == 0) fill_i32(s, w.worldRoot.p, -1, w.W);   // no tree: static lists only
+    hipLaunchKernelGGL(k_raycast, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.qRays.p, w.qRays.p + n, w.qIgnore.p,
+                       S, w.qHit.p, w.qT.p);
+    PGX_LAUNCH_CHECK();
+    PGX_HIP(hipEventRecord(w.qEv[1], s));
+    PGX_HIP(hipMemcpyAsync(hitCollider, w.qHit.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    PGX_HIP(hipMemcpyAsync(hitDistance, w.qT.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    PGX_HIP(hipStreamSynchronize(s));
+    if (deviceUs) {
+        float ms = 0.f;
+        PGX_HIP(hipEventElapsedTime(&ms, w.qEv[0], w.qEv[1]));
+        *deviceUs = 1000.0 * ms;
+    }
+}
+
+}  // namespace pgx

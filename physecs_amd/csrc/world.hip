@@ -452,6 +452,8 @@ static void release_world(World& w) {
     DBuf<float4>* jf[] = {&w.jP0, &w.jQ0, &w.jP1, &w.jQ1, &w.jDrive, &w.jR0, &w.jR1, &w.jRow};
     for (auto* b : jf) b->release();
     world_release_graph(w);
+    for (auto& e : w.qEv)
+        if (e) (void)hipEventDestroy(e);
     for (auto& e : w.ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : w.solveEv) (void)hipEventDestroy(e);
@@ -1206,6 +1208,24 @@ int pgx_world_set_graph(pgx_world* h, int enabled) {
     if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
     h->w.graph = enabled != 0;
     return PGX_OK;
+}
+
+int pgx_world_raycast(pgx_world* h, int n, const float* origins, const float* dirs, const float* max_distance,
+                      const int32_t* ray_world, const uint32_t* ignore_entity, int32_t* hit_collider,
+                      float* hit_distance, double* device_us) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    if (n < 0) return fail(PGX_ERR_INVALID, "ray count must be >= 0");
+    if (n > 0 && (!origins || !dirs || !max_distance || !hit_collider || !hit_distance))
+        return fail(PGX_ERR_INVALID, "NULL ray or result array");
+    World& w = h->w;
+    for (int i = 0; ray_world && i < n; ++i)
+        if (ray_world[i] < 0 || ray_world[i] >= w.W) return fail(PGX_ERR_INVALID, "ray %d: world %d out of range", i, ray_world[i]);
+    if (n == 0) return PGX_OK;
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(w.device));
+        qr_raycast(w, n, origins, dirs, max_distance, ray_world, ignore_entity, hit_collider, hit_distance, device_us);
+        return PGX_OK;
+    });
 }
 
 int pgx_world_get_exec(pgx_world* h, int* mode) {

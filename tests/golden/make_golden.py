@@ -3,7 +3,8 @@ reference's own compiled functions (oracle/_ref/libphysecs_ref.so, built from th
 sources by `make -C oracle ref`). The reference ships no test vectors of its own; these
 fixtures let the oracle be checked where the reference sources are absent.
 
-    python tests/golden/make_golden.py      # needs oracle/_ref built
+    python tests/golden/make_golden.py           # needs oracle/_ref built
+    python tests/golden/make_golden.py raycast   # the ray fixtures only
 
 Every output is stored as raw float32 bits, so a check is bit for bit.
 """
@@ -347,6 +348,50 @@ def gen_rows(n_per_flag=100):
                 vels=np.array(vels), outs=np.array(outs))
 
 
+def ray_case(rng, ids):
+    """A shape, its pose, and a ray aimed near it: from outside or inside, normalised or
+    not, some with direction components below the 0.001 parallel threshold."""
+    t = int(rng.integers(0, 4))
+    p = np.array(shape_params(t, rng) if t < 3 else [*rng.uniform(0.5, 1.5, 3), int(rng.integers(0, len(ids)))],
+                 np.float32)
+    pos = rng.uniform(-5, 5, 3).astype(np.float32)
+    q = rq(rng)
+    o = (pos + rng.uniform(-3, 3, 3) * (0.2 if rng.random() < 0.15 else 1.0)).astype(np.float32)
+    d = (pos + rng.uniform(-0.6, 0.6, 3) - o)
+    if rng.random() < 0.5:
+        d = d / np.linalg.norm(d)
+    if rng.random() < 0.15:
+        d[int(rng.integers(0, 3))] = rng.uniform(-0.0009, 0.0009)
+    return t, p, pos, q, o, d.astype(np.float32)
+
+
+def gen_raycast(ref, ids, n=4000):
+    """intersectRayGeometry (src/Raycast.cpp:141-150) for spheres, capsules, boxes and
+    scaled convex meshes, and intersectRayAABB (:5-30) on the shape's bounds."""
+    rng = np.random.default_rng(1011)
+    types, prm, poses, rays, hits, outs, bhits, bouts = [], [], [], [], [], [], [], []
+    for _ in range(n):
+        t, p, pos, q, o, d = ray_case(rng, ids)
+        live = p.copy()
+        if t == 3:
+            live[3] = ids[int(p[3])]
+        hit, tt = ol.ray_geometry("ref", o, d, t, live, pos, q)
+        b = np.zeros(6, np.float32)
+        ref.ref_bounds(t, ol.fptr(live), ol.fptr(pos), ol.fptr(q), ol.fptr(b))
+        bh, bt = ol.ray_aabb("ref", o, d, b[:3], b[3:])
+        types.append(t)
+        prm.append(p)
+        poses.append(np.concatenate([pos, q]))
+        rays.append(np.concatenate([o, d]))
+        hits.append(hit)
+        outs.append(bits(np.float32(tt if hit else 0.0)))
+        bhits.append(bh)
+        bouts.append(bits(np.float32(bt if bh else 0.0)))
+    return dict(types=np.array(types, np.int32), params=np.array(prm), poses=np.array(poses),
+                rays=np.array(rays), hits=np.array(hits, np.int32), outs=np.array(outs),
+                aabb_hits=np.array(bhits, np.int32), aabb_outs=np.array(bouts))
+
+
 def main():
     if not ol.ref_available():
         raise SystemExit("oracle/_ref is not built: run `make -C oracle ref` where the reference sources are")
@@ -357,6 +402,14 @@ def main():
     verts, tris, h = scenes.terrain_mesh(5, quads=24, amplitude=3.0)
     m = ol.register_trimesh(verts, tris)
     idx, nodes = ol.trimesh_layout("ref", m, len(tris))
+    only = sys.argv[1:]
+    if only == ["raycast"]:   # the ray fixtures alone (the others are unchanged)
+        np.savez_compressed(os.path.join(HERE, "raycast.npz"),
+                            **{"ray_" + k: v for k, v in gen_raycast(ref, ids).items()}, **tables)
+        print("raycast.npz", os.path.getsize(os.path.join(HERE, "raycast.npz")), "bytes")
+        return
+    np.savez_compressed(os.path.join(HERE, "raycast.npz"),
+                        **{"ray_" + k: v for k, v in gen_raycast(ref, ids).items()}, **tables)
     np.savez_compressed(os.path.join(HERE, "collision.npz"),
                         **{"prim_" + k: v for k, v in gen_collision_primitive(ref).items()},
                         **{"cvx_" + k: v for k, v in gen_collision_convex(ref, ids).items()},
@@ -371,7 +424,7 @@ def main():
                         **{"cs_" + k: v for k, v in gen_contact_solve(ref).items()},
                         **{"int_" + k: v for k, v in gen_integrators(ref).items()},
                         **{"row_" + k: v for k, v in gen_rows().items()}, **tables)
-    for f in ("collision.npz", "trimesh.npz", "kernels.npz"):
+    for f in ("collision.npz", "trimesh.npz", "kernels.npz", "raycast.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")
 
 

@@ -110,6 +110,27 @@ def _f32(x, n):
     return a
 
 
+def ray_aabb(which, o, d, mn, mx):
+    """intersectRayAABB of `which` ("orc" or "ref"): (hit, t)."""
+    lib = orc() if which == "orc" else ref()
+    fn = getattr(lib, which + "_ray_aabb")
+    fn.argtypes = [F, F, F, F, F]
+    t = np.full(1, -1.0, np.float32)
+    hit = fn(fptr(_f32(o, 3)), fptr(_f32(d, 3)), fptr(_f32(mn, 3)), fptr(_f32(mx, 3)), fptr(t))
+    return hit, t[0]
+
+
+def ray_geometry(which, o, d, t, prm, pos, q):
+    """intersectRayGeometry of `which`: (hit, t); convex params[3] = register_convex id."""
+    lib = orc() if which == "orc" else ref()
+    fn = getattr(lib, which + "_ray_geometry")
+    fn.argtypes = [F, F, ctypes.c_int, F, F, F, F]
+    out = np.full(1, -1.0, np.float32)
+    hit = fn(fptr(_f32(o, 3)), fptr(_f32(d, 3)), int(t), fptr(_f32(prm, 4)), fptr(_f32(pos, 3)), fptr(_f32(q, 4)),
+             fptr(out))
+    return hit, out[0]
+
+
 def collision(lib, prefix, t0, p0, pos0, q0, t1, p1, pos1, q1):
     """Returns (hit, out[28]) for one narrowphase call of `lib` (orc or ref)."""
     out = np.zeros(28, np.float32)
@@ -197,6 +218,24 @@ class OracleWorld:
         if n:
             self.lib.orc_world_get_pairs(self.h, iptr(out))
         return out
+
+    def raycast(self, origins, dirs, max_dist, ignore=None):
+        """Scene::raycastClosest for each ray: (collider index or -1, hit distance)."""
+        fn = self.lib.orc_world_raycast
+        fn.argtypes = [ctypes.c_void_p, F, F, ctypes.c_float, ctypes.c_uint32, F]
+        fn.restype = ctypes.c_int
+        o = np.ascontiguousarray(origins, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        md = np.broadcast_to(np.asarray(max_dist, np.float32), (len(o),))
+        ig = np.full(len(o), 0xFFFFFFFF, np.uint32) if ignore is None else np.asarray(ignore, np.uint32)
+        col = np.full(len(o), -1, np.int32)
+        t = np.zeros(len(o), np.float32)
+        for i in range(len(o)):
+            ti = np.zeros(1, np.float32)
+            col[i] = fn(self.h, fptr(np.ascontiguousarray(o[i])), fptr(np.ascontiguousarray(d[i])),
+                        np.float32(md[i]), int(ig[i]), fptr(ti))
+            t[i] = ti[0] if col[i] >= 0 else 0.0
+        return col, t
 
     def manifolds(self):
         n = self.lib.orc_world_num_manifolds(self.h)
