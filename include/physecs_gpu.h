@@ -269,6 +269,26 @@ int pgx_world_get_stats(pgx_world* w, pgx_stats* out);
 int pgx_world_raycast(pgx_world* w, int n, const float* origins, const float* dirs, const float* max_distance,
                       const int32_t* ray_world, const uint32_t* ignore_entity, int32_t* hit_collider,
                       float* hit_distance, double* device_us);
+/* Scene::overlap (include/Physecs/Physecs.h:207, src/Physecs.cpp:628-651) for n query
+ * shapes at once, between steps, on the world's current poses and broadphase bounds. Query
+ * i is a shape of types[i] (PGX_SPHERE .. PGX_TRIANGLE_MESH) with params[4i..] as in
+ * pgx_scene_desc.col_params (a convex mesh: scale xyz, then the index of one of the scene's
+ * convex meshes), at positions[3i..] with orientations[4i..] (x, y, z, w), in world
+ * query_world[i] (NULL: world 0). filters[i] != 0 keeps the colliders whose
+ * pgx_scene_desc.col_data has a bit in common with it (NULL or 0: all). A collider is hit
+ * when the query's bounds meet its broadphase bounds (inclusive) and physecs::overlap
+ * (src/Overlap.cpp:203-235) says the shapes overlap; triangle meshes on either side never
+ * do, and the box-convex test places the convex support at the box's position, as the
+ * reference's does (src/Overlap.cpp:200). Results: query i's hits are
+ * colliders[offsets[i] .. offsets[i+1]) (descriptor indices, ascending; the reference's
+ * OverlapHit {entity, colIndex} follows from col_actor and the collider's index within its
+ * actor; the reference lists them in its tree's order), offsets has n + 1 entries and
+ * *total = offsets[n]. When *total exceeds capacity, offsets and *total are set, colliders
+ * is not written and PGX_ERR_CAPACITY is returned: call again with capacity >= *total.
+ * device_us (may be NULL): the query's device time, tree build and sort included. */
+int pgx_world_overlap(pgx_world* w, int n, const int32_t* types, const float* params, const float* positions,
+                      const float* orientations, const int32_t* query_world, const int32_t* filters, int capacity,
+                      int32_t* offsets, int32_t* colliders, int32_t* total, double* device_us);
 /* In PGX_EXEC_COLORS, enabled != 0 launches each step's substep loop as one HIP graph:
  * captured every step (its launches follow the step's colours), the instantiated graph
  * updated in place when the launch sequence keeps its shape. Results are the same either

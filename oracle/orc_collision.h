@@ -68,6 +68,11 @@ bool collisionCapsuleConvex(V3 pos0, Q or0, float hh0, float r0, V3 pos1, Q or1,
 bool collisionBoxConvex(V3 pos0, Q or0, V3 he0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1, Manifold& res);
 bool collisionConvexConvex(V3 pos0, Q or0, const ConvexMesh* m0, V3 s0, V3 pos1, Q or1, const ConvexMesh* m1,
                            V3 s1, Manifold& res);
+// src/Overlap.cpp:141-200 (GJK only)
+bool overlapSphereConvex(V3 pos0, float r0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1);
+bool overlapCapsuleConvex(V3 pos0, Q or0, float hh0, float r0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1);
+bool overlapBoxConvex(V3 pos0, Q or0, V3 he0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1);
+bool overlapConvexConvex(V3 pos0, Q or0, const ConvexMesh* m0, V3 s0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1);
 // GJK + EPA of a convex mesh against one triangle (src/CollisionTriangleMesh.cpp:616-620)
 bool gjkEpaConvexTriangle(V3 pos, Q ori, const ConvexMesh* m, V3 scale, V3 a, V3 b, V3 c, V3 startDir, V3& normal,
                           V3& cpBody, V3& cpTri);

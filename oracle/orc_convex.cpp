@@ -382,6 +382,27 @@ bool collisionConvexConvex(V3 pos0, Q or0, const ConvexMesh* m0, V3 s0, V3 pos1,
     return true;
 }
 
+// The GJK-decided overlaps of src/Overlap.cpp:141-200. The box-convex case builds the
+// convex support function at the box's position (pos0 in both, Overlap.cpp:200), which is
+// kept.
+bool overlapSphereConvex(V3 pos0, float r0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1) {
+    GjkSimplex s;
+    return gjk(SupSphere{pos0, r0}, SupConvex{pos1, mat3_cast(or1), m1, s1}, pos1 - pos0, s);
+}
+bool overlapCapsuleConvex(V3 pos0, Q or0, float hh0, float r0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1) {
+    GjkSimplex s;
+    return gjk(SupCapsule{pos0, qrot(or0, v3(0, 1, 0)), hh0, r0}, SupConvex{pos1, mat3_cast(or1), m1, s1},
+               pos1 - pos0, s);
+}
+bool overlapBoxConvex(V3 pos0, Q or0, V3 he0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1) {
+    GjkSimplex s;
+    return gjk(SupBox{pos0, mat3_cast(or0), he0}, SupConvex{pos0, mat3_cast(or1), m1, s1}, pos1 - pos0, s);
+}
+bool overlapConvexConvex(V3 pos0, Q or0, const ConvexMesh* m0, V3 s0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1) {
+    GjkSimplex s;
+    return gjk(SupConvex{pos0, mat3_cast(or0), m0, s0}, SupConvex{pos1, mat3_cast(or1), m1, s1}, pos1 - pos0, s);
+}
+
 // src/Collision.cpp:488-499
 bool collisionSphereConvex(V3 pos0, float r0, V3 pos1, Q or1, const ConvexMesh* m1, V3 s1, Manifold& res) {
     SupSphere a{pos0, r0};

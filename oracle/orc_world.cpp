@@ -1414,9 +1414,170 @@ bool rayGeometry(V3 o, V3 d, V3 pos, Q ori, const Geom& g, float& t) {
     }
 }
 
+// src/Overlap.cpp:19-24
+bool ovSphereSphere(V3 pos0, float r0, V3 pos1, float r1) {
+    V3 d = pos1 - pos0;
+    float rs = r0 + r1;
+    return !(dot(d, d) > rs * rs);
+}
+
+// src/Overlap.cpp:26-35
+bool ovCapsuleCapsule(V3 pos0, Q or0, float hh0, float r0, V3 pos1, Q or1, float hh1, float r1) {
+    V3 up = v3(0, 1, 0);
+    V3 v0 = qrot(or0, up), v1 = qrot(or1, up);
+    V3 a, b;
+    closestPointsBetweenSegments(pos0, v0, -hh0, hh0, pos1, v1, -hh1, hh1, a, b);
+    V3 d = b - a;
+    float rs = r0 + r1;
+    return !(dot(d, d) > rs * rs);
+}
+
+// src/Overlap.cpp:37-139: 15-axis separating-axis test
+bool ovBoxBox(V3 pos0, Q or0, V3 he0, V3 pos1, Q or1, V3 he1) {
+    M3 u0 = mat3_cast(or0), u1 = mat3_cast(or1);
+    V3 t = pos1 - pos0;
+    t = v3(dot(t, u0[0]), dot(t, u0[1]), dot(t, u0[2]));
+    float r[3][3], ar[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) { r[i][j] = dot(u0[i], u1[j]); ar[i][j] = gabs(r[i][j]); }
+    const float a[3] = {he0.x, he0.y, he0.z}, b[3] = {he1.x, he1.y, he1.z}, tt[3] = {t.x, t.y, t.z};
+    for (int i = 0; i < 3; ++i) {
+        float ra = a[i], rb = b[0] * ar[i][0] + b[1] * ar[i][1] + b[2] * ar[i][2];
+        if (gabs(tt[i]) - ra - rb > 0) return false;
+    }
+    for (int i = 0; i < 3; ++i) {
+        float ra = a[0] * ar[0][i] + a[1] * ar[1][i] + a[2] * ar[2][i], rb = b[i];
+        if (gabs(tt[0] * r[0][i] + tt[1] * r[1][i] + tt[2] * r[2][i]) - ra - rb > 0) return false;
+    }
+    // L = Ai x Bj: the other two axes of A and B, in the reference's operand order
+    for (int i = 0; i < 3; ++i) {
+        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+        for (int j = 0; j < 3; ++j) {
+            const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            // ra = he0[lo] * absR[hi][j] + he0[hi] * absR[lo][j], lo < hi the two indices != i
+            const int lo = i1 < i2 ? i1 : i2, hi = i1 < i2 ? i2 : i1;
+            const int blo = j1 < j2 ? j1 : j2, bhi = j1 < j2 ? j2 : j1;
+            float ra = a[lo] * ar[hi][j] + a[hi] * ar[lo][j];
+            float rb = b[blo] * ar[i][bhi] + b[bhi] * ar[i][blo];
+            float l = gabs(tt[i2] * r[i1][j] - tt[i1] * r[i2][j]);
+            if (l - ra - rb > 0) return false;
+        }
+    }
+    return true;
+}
+
+// src/Overlap.cpp:145-153
+bool ovSphereCapsule(V3 pos0, float r0, V3 pos1, Q or1, float hh1, float r1) {
+    V3 dir = qrot(or1, v3(0, 1, 0));
+    V3 q = closestPointOnSegment(pos0, pos1, dir, -hh1, hh1);
+    V3 v = q - pos0;
+    float rs = r0 + r1;
+    return !(dot(v, v) > rs * rs);
+}
+
+// src/Overlap.cpp:155-169
+bool ovSphereBox(V3 pos0, float r0, V3 pos1, Q or1, V3 he1) {
+    M3 u1 = mat3_cast(or1);
+    V3 d = pos0 - pos1;
+    d = v3(dot(d, u1[0]), dot(d, u1[1]), dot(d, u1[2]));
+    V3 q = pos1 + mulmv(u1, vclamp(d, -he1, he1));
+    V3 v = q - pos0;
+    return !(dot(v, v) > r0 * r0);
+}
+
+// src/Overlap.cpp:175-193
+bool ovCapsuleBox(V3 pos0, Q or0, float hh0, float r0, V3 pos1, Q or1, V3 he1) {
+    M3 u1 = mat3_cast(or1);
+    V3 p = pos0 - pos1;
+    p = v3(dot(p, u1[0]), dot(p, u1[1]), dot(p, u1[2]));
+    V3 dir = qrot(or0, v3(0, 1, 0));
+    V3 dirLc = v3(dot(dir, u1[0]), dot(dir, u1[1]), dot(dir, u1[2]));
+    float t;
+    V3 q;
+    float sq = sqrDistSegmentAABB(p, dirLc, -hh0, hh0, he1, t, q);
+    return !(sq >= r0 * r0);
+}
+
+// physecs::overlap (src/Overlap.cpp:203-235); a triangle mesh on either side never overlaps
+bool overlapGeom(V3 p0, Q q0, const Geom& g0, V3 p1, Q q1, const Geom& g1) {
+    const V3 s0 = v3(g0.p[0], g0.p[1], g0.p[2]), s1 = v3(g1.p[0], g1.p[1], g1.p[2]);
+    switch (g0.type) {
+        case SPHERE:
+            switch (g1.type) {
+                case SPHERE: return ovSphereSphere(p0, g0.p[0], p1, g1.p[0]);
+                case CAPSULE: return ovSphereCapsule(p0, g0.p[0], p1, q1, g1.p[0], g1.p[1]);
+                case BOX: return ovSphereBox(p0, g0.p[0], p1, q1, s1);
+                case CONVEX_MESH: return overlapSphereConvex(p0, g0.p[0], p1, q1, g1.mesh, s1);
+            }
+            return false;
+        case CAPSULE:
+            switch (g1.type) {
+                case SPHERE: return ovSphereCapsule(p1, g1.p[0], p0, q0, g0.p[0], g0.p[1]);
+                case CAPSULE: return ovCapsuleCapsule(p0, q0, g0.p[0], g0.p[1], p1, q1, g1.p[0], g1.p[1]);
+                case BOX: return ovCapsuleBox(p0, q0, g0.p[0], g0.p[1], p1, q1, s1);
+                case CONVEX_MESH: return overlapCapsuleConvex(p0, q0, g0.p[0], g0.p[1], p1, q1, g1.mesh, s1);
+            }
+            return false;
+        case BOX:
+            switch (g1.type) {
+                case SPHERE: return ovSphereBox(p1, g1.p[0], p0, q0, s0);
+                case CAPSULE: return ovCapsuleBox(p1, q1, g1.p[0], g1.p[1], p0, q0, s0);
+                case BOX: return ovBoxBox(p0, q0, s0, p1, q1, s1);
+                case CONVEX_MESH: return overlapBoxConvex(p0, q0, s0, p1, q1, g1.mesh, s1);
+            }
+            return false;
+        case CONVEX_MESH:
+            switch (g1.type) {
+                case SPHERE: return overlapSphereConvex(p1, g1.p[0], p0, q0, g0.mesh, s0);
+                case CAPSULE: return overlapCapsuleConvex(p1, q1, g1.p[0], g1.p[1], p0, q0, g0.mesh, s0);
+                case BOX: return overlapBoxConvex(p1, q1, s1, p0, q0, g0.mesh, s0);
+                case CONVEX_MESH: return overlapConvexConvex(p0, q0, g0.mesh, s0, p1, q1, g1.mesh, s1);
+            }
+            return false;
+    }
+    return false;
+}
+
 }  // namespace
 
 extern "C" {
+
+int orc_overlap(int type0, const float* prm0, const float* pos0, const float* q0, int type1, const float* prm1,
+                const float* pos1, const float* q1) {
+    Geom g0 = pinGeom(type0, prm0), g1 = pinGeom(type1, prm1);
+    return overlapGeom(v3(pos0[0], pos0[1], pos0[2]), qmake(q0[3], q0[0], q0[1], q0[2]), g0,
+                       v3(pos1[0], pos1[1], pos1[2]), qmake(q1[3], q1[0], q1[1], q1[2]), g1) ? 1 : 0;
+}
+
+// Scene::overlap (src/Physecs.cpp:628-651) over the world's colliders: a leaf is reached
+// when the query's bounds intersect the leaf's bounds (every ancestor's box holds them), so
+// the hits are those of this loop; they are listed by collider index (the reference lists
+// them in its tree's order). filter != 0 keeps colliders whose data has a common bit with
+// it. A convex query's params[3] is one of the world's convex meshes. Returns the hit count
+// (at most cap written to out).
+int orc_world_overlap(const orc_world* w, int type, const float* prm, const float* pos, const float* q, int filter,
+                      int cap, int32_t* out) {
+    if (type == TRIANGLE_MESH) return 0;   // overlaps nothing (src/Overlap.cpp:203-235)
+    Geom g{type, {prm[0], prm[1], prm[2], prm[3]}};
+    if (type == CONVEX_MESH) g.mesh = &w->convex.at((size_t)prm[3]);   // one of the scene's convex meshes
+    V3 p = v3(pos[0], pos[1], pos[2]);
+    Q qq = qmake(q[3], q[0], q[1], q[2]);
+    Bounds b = getBounds(p, qq, g);
+    std::vector<int> hits;
+    for (const BPEntry& e : w->bp) {
+        const Bounds& eb = e.bounds;
+        if (b.max.x < eb.min.x || b.min.x > eb.max.x || b.max.y < eb.min.y || b.min.y > eb.max.y ||
+            b.max.z < eb.min.z || b.min.z > eb.max.z)
+            continue;   // intersects (src/BoundsUtil.cpp:87-92)
+        const Collider& c = w->cols[e.collider];
+        if (filter && !(c.data & filter)) continue;
+        const Actor& a = w->actors[c.actor];
+        if (overlapGeom(p, qq, g, a.pos + qrot(a.quat, c.lpos), qmul(a.quat, c.lquat), c.geom)) hits.push_back(e.collider);
+    }
+    std::sort(hits.begin(), hits.end());
+    for (size_t i = 0; i < hits.size() && (int)i < cap; ++i) out[i] = hits[i];
+    return (int)hits.size();
+}
 
 int orc_ray_aabb(const float* o, const float* d, const float* mn, const float* mx, float* t) {
     return rayAABB(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), v3(mn[0], mn[1], mn[2]), v3(mx[0], mx[1], mx[2]), *t) ? 1 : 0;

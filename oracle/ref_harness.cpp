@@ -26,6 +26,7 @@
 #include "GeomUtil.h"
 #include "MassUtil.h"
 #include "MathUtil.h"
+#include "Overlap.h"
 #include "Raycast.h"
 #include "SolverData.h"
 #include "TriangleMesh.h"
@@ -318,6 +319,13 @@ int ref_ray_geometry(const float* o, const float* d, int type, const float* prm,
                      float* t) {
     Geometry g = makeGeom(type, prm);
     return intersectRayGeometry(V(o), V(d), V(pos), Qt(q), g, *t) ? 1 : 0;
+}
+
+// physecs::overlap (src/Overlap.cpp:203-235)
+int ref_overlap(int type0, const float* prm0, const float* pos0, const float* q0, int type1, const float* prm1,
+                const float* pos1, const float* q1) {
+    Geometry g0 = makeGeom(type0, prm0), g1 = makeGeom(type1, prm1);
+    return overlap(V(pos0), Qt(q0), g0, V(pos1), Qt(q1), g1) ? 1 : 0;
 }
 
 }  // extern "C"

@@ -18,6 +18,7 @@ U32 = ctypes.POINTER(ctypes.c_uint32)
 U8 = ctypes.POINTER(ctypes.c_uint8)
 
 PGX_OK = 0
+ERR_CAPACITY = -3
 ERRORS = {-1: "PGX_ERR_INVALID", -2: "PGX_ERR_HIP", -3: "PGX_ERR_CAPACITY", -4: "PGX_ERR_UNSUPPORTED"}
 
 # every symbol declared in include/physecs_gpu.h
@@ -28,7 +29,7 @@ EXPORTS = [
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
-    "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast",
+    "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -131,6 +132,8 @@ def lib():
         L.pgx_world_get_exec.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.pgx_world_raycast.argtypes = [vp, ctypes.c_int, F, F, F, ctypes.POINTER(ctypes.c_int32),
                                         ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int32), F,
+                                        ctypes.POINTER(ctypes.c_double)]
+        L.pgx_world_overlap.argtypes = [vp, ctypes.c_int, I32, F, F, F, I32, I32, ctypes.c_int, I32, I32, I32,
                                         ctypes.POINTER(ctypes.c_double)]
         L.pgx_server_set_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
         L.pgx_server_create.argtypes = [ctypes.POINTER(PgxConfig), ctypes.POINTER(PgxServerConfig), ctypes.POINTER(vp)]
@@ -314,6 +317,38 @@ class World:
                                        None if ig is None else ig.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
                                        col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _f(t), ctypes.byref(us)))
         return col, t, us.value
+
+    def overlap(self, types, params, positions, orientations, world=None, filters=None, capacity=None):
+        """pgx_world_overlap: Scene::overlap for every query shape. Returns (list of
+        ascending collider-index arrays, one per query; device microseconds). Without a
+        capacity, the first call sizes the hit buffer from the returned total."""
+        t = np.ascontiguousarray(np.atleast_1d(types), np.int32)
+        n = len(t)
+        prm = np.zeros((n, 4), np.float32)
+        pa = np.asarray(params, np.float32).reshape(n, -1)
+        prm[:, : pa.shape[1]] = pa
+        pos = np.ascontiguousarray(positions, np.float32).reshape(n, 3)
+        q = np.ascontiguousarray(orientations, np.float32).reshape(n, 4)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        wd = None if world is None else np.ascontiguousarray(np.broadcast_to(np.asarray(world, np.int32), (n,)))
+        fl = None if filters is None else np.ascontiguousarray(np.broadcast_to(np.asarray(filters, np.int32), (n,)))
+        off = np.zeros(n + 1, np.int32)
+        total = ctypes.c_int32(0)
+        us = ctypes.c_double(0)
+        cap = 4 * n + 1024 if capacity is None else int(capacity)
+        while True:
+            cols = np.zeros(max(cap, 1), np.int32)
+            rc = self.L.pgx_world_overlap(self.h, n, t.ctypes.data_as(i32), _f(prm), _f(pos), _f(q),
+                                          None if wd is None else wd.ctypes.data_as(i32),
+                                          None if fl is None else fl.ctypes.data_as(i32), cap,
+                                          off.ctypes.data_as(i32), cols.ctypes.data_as(i32), ctypes.byref(total),
+                                          ctypes.byref(us))
+            if rc == ERR_CAPACITY and capacity is None and total.value > cap:
+                cap = total.value
+                continue
+            check(rc)
+            break
+        return [cols[off[i]:off[i + 1]].copy() for i in range(n)], us.value
 
     def exec_mode(self):
         m = ctypes.c_int(0)

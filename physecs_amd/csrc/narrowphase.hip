@@ -8,6 +8,7 @@
 // to the reference's for the same poses. The polygon clip buffers live in registers
 // (a quad clipped by four edges has at most 8 vertices).
 #include "pgx_internal.h"
+#include "shape_bounds.h"
 
 namespace pgx {
 
@@ -806,6 +807,215 @@ __global__ __launch_bounds__(64) void k_narrowphase_convex(
     }
 }
 
+// ------------------------------------------------------------------ overlap queries
+// physecs::overlap (src/Overlap.cpp): the pair tests as yes/no answers, no manifold.
+__device__ __forceinline__ bool ov_sphere_sphere(f3 p0, float r0, f3 p1, float r1) {   // Overlap.cpp:19-24
+    f3 d = sub(p1, p0);
+    float rs = r0 + r1;
+    return !(dot(d, d) > rs * rs);
+}
+
+__device__ bool ov_capsule_capsule(f3 p0, qf q0, float hh0, float r0, f3 p1, qf q1, float hh1,   // :26-35
+                                   float r1) {
+    const f3 up = mk3(0.f, 1.f, 0.f);
+    f3 a, b;
+    closest_points_segments(p0, qrot(q0, up), -hh0, hh0, p1, qrot(q1, up), -hh1, hh1, a, b);
+    f3 d = sub(b, a);
+    float rs = r0 + r1;
+    return !(dot(d, d) > rs * rs);
+}
+
+// Overlap.cpp:37-139: the 15 separating axes, separated when |t.L| - rA - rB > 0. For the
+// edge axes Ai x Bj the radius sums take the other two axes of each box, lower index first.
+__device__ bool ov_box_box(f3 p0, qf q0, f3 he0, f3 p1, qf q1, f3 he1) {
+    const m3 u0 = mat3_cast(q0), u1 = mat3_cast(q1);
+    f3 tv = sub(p1, p0);
+    tv = mk3(dot(tv, u0.c0), dot(tv, u0.c1), dot(tv, u0.c2));
+    float r[3][3], ar[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            r[i][j] = dot(col(u0, i), col(u1, j));
+            ar[i][j] = gabs(r[i][j]);
+        }
+    const float a[3] = {he0.x, he0.y, he0.z}, b[3] = {he1.x, he1.y, he1.z}, t[3] = {tv.x, tv.y, tv.z};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        if (gabs(t[i]) - a[i] - (b[0] * ar[i][0] + b[1] * ar[i][1] + b[2] * ar[i][2]) > 0) return false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        float ra = a[0] * ar[0][i] + a[1] * ar[1][i] + a[2] * ar[2][i];
+        if (gabs(t[0] * r[0][i] + t[1] * r[1][i] + t[2] * r[2][i]) - ra - b[i] > 0) return false;
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+        const int lo = i1 < i2 ? i1 : i2, hi = i1 < i2 ? i2 : i1;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            const int blo = j1 < j2 ? j1 : j2, bhi = j1 < j2 ? j2 : j1;
+            float ra = a[lo] * ar[hi][j] + a[hi] * ar[lo][j];
+            float rb = b[blo] * ar[i][bhi] + b[bhi] * ar[i][blo];
+            if (gabs(t[i2] * r[i1][j] - t[i1] * r[i2][j]) - ra - rb > 0) return false;
+        }
+    }
+    return true;
+}
+
+__device__ bool ov_sphere_capsule(f3 p0, float r0, f3 p1, qf q1, float hh1, float r1) {   // :145-153
+    f3 q = closest_point_on_segment(p0, p1, qrot(q1, mk3(0.f, 1.f, 0.f)), -hh1, hh1);
+    f3 v = sub(q, p0);
+    float rs = r0 + r1;
+    return !(dot(v, v) > rs * rs);
+}
+
+__device__ bool ov_sphere_box(f3 p0, float r0, f3 p1, qf q1, f3 he) {   // :155-169
+    m3 u = mat3_cast(q1);
+    f3 d = sub(p0, p1);
+    d = mk3(dot(d, u.c0), dot(d, u.c1), dot(d, u.c2));
+    f3 q = add(p1, mulmv(u, vclamp(d, neg(he), he)));
+    f3 v = sub(q, p0);
+    return !(dot(v, v) > r0 * r0);
+}
+
+// :175-193; separated when the squared distance reaches r^2 (>=, where the others use >)
+__device__ bool ov_capsule_box(f3 p0, qf q0, float hh0, float r0, f3 p1, qf q1, f3 he) {
+    m3 u = mat3_cast(q1);
+    f3 p = sub(p0, p1);
+    p = mk3(dot(p, u.c0), dot(p, u.c1), dot(p, u.c2));
+    f3 dir = qrot(q0, mk3(0.f, 1.f, 0.f));
+    f3 dl = mk3(dot(dir, u.c0), dot(dir, u.c1), dot(dir, u.c2));
+    float t = 0.f;
+    f3 q = s3(0.f);
+    return !(segment_aabb(p, dl, -hh0, hh0, he, t, q) >= r0 * r0);
+}
+
+__device__ __forceinline__ SupConvex ov_convex(const ConvexView& cv, float4 g, f3 pos, qf q) {
+    const int4 mi = cv.mesh[(int)g.w];
+    return SupConvex{pos, mat3_cast(q), mk3(g.x, g.y, g.z), cv.verts + mi.x, mi.y};
+}
+
+// physecs::overlap (src/Overlap.cpp:203-235). A triangle mesh on either side never
+// overlaps. The convex cases run GJK (Overlap.cpp:141-200); the box-convex case places the
+// convex support at the box's position, as the reference does (Overlap.cpp:200).
+__device__ bool overlap_shapes(int t0, float4 g0, f3 p0, qf q0, int t1, float4 g1, f3 p1, qf q1,
+                               const ConvexView& cv) {
+    if (t0 > PGX_CONVEX_MESH || t1 > PGX_CONVEX_MESH) return false;
+    const f3 he0 = mk3(g0.x, g0.y, g0.z), he1 = mk3(g1.x, g1.y, g1.z);
+    const f3 up = mk3(0.f, 1.f, 0.f);
+    GjkV s[4];
+    switch (t0) {
+        case PGX_SPHERE:
+            switch (t1) {
+                case PGX_SPHERE: return ov_sphere_sphere(p0, g0.x, p1, g1.x);
+                case PGX_CAPSULE: return ov_sphere_capsule(p0, g0.x, p1, q1, g1.x, g1.y);
+                case PGX_BOX: return ov_sphere_box(p0, g0.x, p1, q1, he1);
+                default: return gjk(SupSphere{p0, g0.x}, ov_convex(cv, g1, p1, q1), sub(p1, p0), s);
+            }
+        case PGX_CAPSULE:
+            switch (t1) {
+                case PGX_SPHERE: return ov_sphere_capsule(p1, g1.x, p0, q0, g0.x, g0.y);
+                case PGX_CAPSULE: return ov_capsule_capsule(p0, q0, g0.x, g0.y, p1, q1, g1.x, g1.y);
+                case PGX_BOX: return ov_capsule_box(p0, q0, g0.x, g0.y, p1, q1, he1);
+                default:
+                    return gjk(SupCapsule{p0, qrot(q0, up), g0.x, g0.y}, ov_convex(cv, g1, p1, q1), sub(p1, p0), s);
+            }
+        case PGX_BOX:
+            switch (t1) {
+                case PGX_SPHERE: return ov_sphere_box(p1, g1.x, p0, q0, he0);
+                case PGX_CAPSULE: return ov_capsule_box(p1, q1, g1.x, g1.y, p0, q0, he0);
+                case PGX_BOX: return ov_box_box(p0, q0, he0, p1, q1, he1);
+                default: return gjk(SupBox{p0, mat3_cast(q0), he0}, ov_convex(cv, g1, p0, q1), sub(p1, p0), s);
+            }
+        default:
+            switch (t1) {
+                case PGX_SPHERE: return gjk(SupSphere{p1, g1.x}, ov_convex(cv, g0, p0, q0), sub(p0, p1), s);
+                case PGX_CAPSULE:
+                    return gjk(SupCapsule{p1, qrot(q1, up), g1.x, g1.y}, ov_convex(cv, g0, p0, q0), sub(p0, p1), s);
+                case PGX_BOX: return gjk(SupBox{p1, mat3_cast(q1), he1}, ov_convex(cv, g0, p1, q0), sub(p0, p1), s);
+                default: return gjk(ov_convex(cv, g0, p0, q0), ov_convex(cv, g1, p1, q1), sub(p1, p0), s);
+            }
+    }
+}
+
+// intersects (src/BoundsUtil.cpp:87-92): inclusive on every axis
+__device__ __forceinline__ bool boxes_meet(f3 amn, f3 amx, float4 bmn, float4 bmx) {
+    return !(amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y || amx.z < bmn.z || amn.z > bmx.z);
+}
+
+// one collider of the query's world: the reference's leaf test (bounds, filter, shape)
+__device__ __forceinline__ bool overlap_collider(const OverlapArgs& A, int c, int t, float4 g, f3 pos, qf q, f3 mn,
+                                                 f3 mx, int filter) {
+    if (!boxes_meet(mn, mx, A.cbmin[c], A.cbmax[c])) return false;
+    if (filter && !(A.cdata[c] & filter)) return false;
+    const int b = A.cbody[c];
+    const qf bq = q4(A.bquat[b]);
+    const f3 cp = add(xyz(A.bpos[b]), qrot(bq, xyz(A.clpos[c])));
+    return overlap_shapes(t, g, pos, q, A.ctype[c], A.cparams[c], cp, qmul(bq, q4(A.clquat[c])), A.cv);
+}
+
+// One query per thread: its bounds, the walk of its world's LBVH (boxes tested inclusive,
+// as the reference's recursion does) and its world's static colliders. Each hit appends the
+// key (query << cbits | collider) with one atomic per wave; keys beyond the capacity are
+// counted, not written. count[i] = the query's hits.
+__global__ __launch_bounds__(64) void k_overlap(OverlapArgs A) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < A.n;
+    int t = PGX_TRIANGLE_MESH, filter = 0, wd = 0;
+    float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+    f3 pos = s3(0.f), mn = s3(0.f), mx = s3(0.f);
+    qf q = q4(make_float4(0.f, 0.f, 0.f, 1.f));
+    if (live) {
+        t = A.qType[i];
+        g = A.qPrm[i];
+        const float4 pw = A.qPos[i];
+        pos = xyz(pw);
+        wd = __float_as_int(pw.w);
+        q = q4(A.qQuat[i]);
+        filter = A.qFilter[i];
+        if (t <= PGX_CONVEX_MESH)
+            shape_bounds(t, g, pos, q, A.cv.verts, A.cv.mesh, nullptr, nullptr, mn, mx);
+    }
+    // a triangle-mesh query overlaps nothing (src/Overlap.cpp:203-235)
+    const bool active = live && t <= PGX_CONVEX_MESH;
+    int hits = 0;
+    int node = active ? A.worldRoot[wd] : -1;
+    const int stop = active && node != -1 ? A.worldStop[wd] : -1;
+    int visits = 0;
+    int st = active ? A.worldStaticStart[wd] : 0;
+    const int stEnd = active ? A.worldStaticStart[wd + 1] : 0;
+    // the walk and then the static list, in one loop so that the wave's lanes append together
+    while (true) {
+        int c = -1;
+        bool more = false;
+        if (node != -1 && node != stop && visits < 2 * A.nLeaves) {   // a walk visits at most 2n - 1 nodes
+            ++visits;
+            more = true;
+            if (node & LEAF_BIT) {
+                const int l = node & ~LEAF_BIT;
+                c = A.mvals[l];
+                node = __float_as_int(A.leafA[l].w);
+            } else {
+                const float4 na = A.nodeA[node], nb = A.nodeB[node];
+                node = boxes_meet(mn, mx, na, nb) ? __float_as_int(na.w) : __float_as_int(nb.w);
+            }
+        } else if (st < stEnd) {
+            more = true;
+            c = A.staticCols[st++];
+        }
+        const bool hit = c >= 0 && overlap_collider(A, c, t, g, pos, q, mn, mx, filter);
+        if (__ballot(hit)) {
+            const int slot = wave_append(A.total, hit);
+            if (hit && slot < A.cap) A.keys[slot] = ((uint64_t)(uint32_t)i << A.cbits) | (uint32_t)c;
+        }
+        hits += hit;
+        if (!__ballot(more)) break;
+    }
+    if (live) A.count[i] = hits;
+}
+
 // ------------------------------------------------------------------ triangle-mesh pairs
 // Everything the three mesh passes read.
 struct MeshArgs {
@@ -969,6 +1179,11 @@ __global__ __launch_bounds__(64) void k_mesh_emit(MeshArgs A, int32_t* tmWs, int
         meshFirst[m.p] = slot;
         meshCount[m.p] = out.n;
     }
+}
+
+void np_overlap_launch(hipStream_t s, const OverlapArgs& A) {
+    hipLaunchKernelGGL(k_overlap, dim3(div_up(A.n, 64)), dim3(64), 0, s, A);
+    PGX_LAUNCH_CHECK();
 }
 
 void np_run(World& w, int nPairs) {

@@ -18,15 +18,6 @@ namespace pgx {
 
 constexpr int CVX_MAXPOLY = 40;   // a clipped face polygon (faces have at most 16 vertices)
 
-// device view of the convex mesh tables
-struct ConvexView {
-    const float4* verts;    // per mesh, padded to a multiple of 4 with copies of the last vertex
-    const int4* mesh;       // {first padded vertex, vertex count, first face, face count}
-    const int2* face;       // {first index, index count}
-    const int32_t* faceIdx; // mesh-local vertex indices
-    const float4* faceN;    // outward unit normal
-    const float4* faceC;    // centroid
-};
 
 // ------------------------------------------------------------------ support maps
 struct SupSphere {                                  // src/GJK.h:13-20

@@ -277,7 +277,14 @@ struct World {
     DBuf<uint32_t> qIgnore;
     DBuf<int32_t> qHit;
     DBuf<float> qT;
-    hipEvent_t qEv[2] = {};
+    hipEvent_t qEv[4] = {};
+    // overlap queries: Collider::data (descriptor order), query shapes, per-query hit
+    // counts and offsets, the hit keys
+    DBuf<int32_t> cdata;
+    DBuf<float4> qShape;
+    DBuf<int32_t> qType, qFilter, qCount;
+    DBuf<uint64_t> qKeys;
+    DBuf<int32_t> qVals;
     std::vector<int32_t> worldListHost[2];   // worlds of at most WORLD_SMALL dynamic bodies, and the rest
     DBuf<int32_t> worldList[2];
     bool worldsContiguous = false;
@@ -316,9 +323,40 @@ void bp_build(World& w);              // the LBVH from the current bounds (no pa
 void qr_raycast(World& w, int n, const float* origins, const float* dirs, const float* maxDist,
                 const int32_t* rayWorld, const uint32_t* ignoreEntity, int32_t* hitCollider, float* hitDistance,
                 double* deviceUs);
+// queries.hip: Scene::overlap for n query shapes (host arrays); returns PGX_OK or
+// PGX_ERR_CAPACITY when the hits exceed `capacity` (offsets and *total are set either way)
+int qr_overlap(World& w, int n, const int32_t* types, const float* params, const float* positions,
+               const float* orientations, const int32_t* queryWorld, const int32_t* filters, int capacity,
+               int32_t* offsets, int32_t* colliders, int32_t* total, double* deviceUs);
+
+// device view of the convex mesh tables
+struct ConvexView {
+    const float4* verts;    // per mesh, padded to a multiple of 4 with copies of the last vertex
+    const int4* mesh;       // {first padded vertex, vertex count, first face, face count}
+    const int2* face;       // {first index, index count}
+    const int32_t* faceIdx; // mesh-local vertex indices
+    const float4* faceN;    // outward unit normal
+    const float4* faceC;    // centroid
+};
+
+// what the overlap query kernel reads and writes
+struct OverlapArgs {
+    int n, nLeaves, cap, cbits;             // keys: (query << cbits) | collider
+    const float4 *qPrm, *qPos, *qQuat;      // per query: params (.w mesh), position (.w world bits), orientation
+    const int32_t *qType, *qFilter;
+    const int32_t *worldRoot, *worldStop, *worldStaticStart, *staticCols, *mvals;
+    const float4 *leafA, *nodeA, *nodeB;
+    const float4 *cbmin, *cbmax, *cparams, *clpos, *clquat, *bpos, *bquat;
+    const int32_t *cbody, *ctype, *cdata;
+    ConvexView cv;
+    int32_t* count;                          // per query: its hits
+    int32_t* total;                          // all hits (device counter, zero before the launch)
+    uint64_t* keys;                          // the first `cap` hits' keys
+};
 
 // narrowphase.hip
 void np_run(World& w, int nPairs);
+void np_overlap_launch(hipStream_t s, const OverlapArgs& A);
 int upload_triangle_meshes(World& w, const pgx_scene_desc* sc, hipStream_t s);   // returns the BVH depth
 // device-layout arrays of a scene descriptor (world.hip)
 struct SceneArrays {

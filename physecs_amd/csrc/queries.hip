@@ -242,4 +242,82 @@ void qr_raycast(World& w, int n, const float* origins, const float* dirs, const 
     }
 }
 
+int qr_overlap(World& w, int n, const int32_t* types, const float* params, const float* positions,
+               const float* orientations, const int32_t* queryWorld, const int32_t* filters, int capacity,
+               int32_t* offsets, int32_t* colliders, int32_t* total, double* deviceUs) {
+    hipStream_t s = w.stream;
+    int cbits = 1;
+    while ((1L << cbits) < (long)w.C) ++cbits;
+    int qbits = 1;
+    while ((1L << qbits) < (long)n) ++qbits;
+    std::vector<float4> hs(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        int32_t wd = queryWorld ? queryWorld[i] : 0;
+        float wbits;
+        std::memcpy(&wbits, &wd, sizeof wbits);
+        hs[i] = make_float4(params[4 * i], params[4 * i + 1], params[4 * i + 2], params[4 * i + 3]);
+        hs[n + i] = make_float4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], wbits);
+        hs[2 * (size_t)n + i] = make_float4(orientations[4 * i], orientations[4 * i + 1], orientations[4 * i + 2],
+                                            orientations[4 * i + 3]);
+    }
+    std::vector<int32_t> hf(filters ? filters : nullptr, filters ? filters + n : nullptr);
+    if (!filters) hf.assign((size_t)n, 0);
+    w.qShape.alloc(3 * (size_t)n); w.qType.alloc(n); w.qFilter.alloc(n); w.qCount.alloc((size_t)n + 1);
+    w.qKeys.alloc(std::max(capacity, 1)); w.qVals.alloc(std::max(capacity, 1));
+    PGX_HIP(hipMemcpyAsync(w.qShape.p, hs.data(), sizeof(float4) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+    PGX_HIP(hipMemcpyAsync(w.qType.p, types, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    PGX_HIP(hipMemcpyAsync(w.qFilter.p, hf.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    if (!w.qEv[0])
+        for (auto& e : w.qEv) PGX_HIP(hipEventCreate(&e));
+    PGX_HIP(hipEventRecord(w.qEv[0], s));
+    fill_i32(s, w.counters.p + 7, 0, 1);   // the refit's work counter
+    fill_i32(s, w.qCount.p + n, 0, 1);     // the hit counter
+    bp_build(w);
+    if (w.nDynCol == 0) fill_i32(s, w.worldRoot.p, -1, w.W);   // no tree: static lists only
+    OverlapArgs A;
+    A.n = n; A.nLeaves = w.nDynCol; A.cap = capacity; A.cbits = cbits;
+    A.qPrm = w.qShape.p; A.qPos = w.qShape.p + n; A.qQuat = w.qShape.p + 2 * (size_t)n;
+    A.qType = w.qType.p; A.qFilter = w.qFilter.p;
+    A.worldRoot = w.worldRoot.p; A.worldStop = w.worldStop.p;
+    A.worldStaticStart = w.worldStaticStart.p; A.staticCols = w.staticCols.p; A.mvals = w.mvals.p;
+    A.leafA = w.leafA.p; A.nodeA = w.nodeA.p; A.nodeB = w.nodeB.p;
+    A.cbmin = w.cbmin.p; A.cbmax = w.cbmax.p; A.cparams = w.cparams.p; A.clpos = w.clpos.p; A.clquat = w.clquat.p;
+    A.bpos = w.bpos.p; A.bquat = w.bquat.p; A.cbody = w.cbody.p; A.ctype = w.ctype.p; A.cdata = w.cdata.p;
+    A.cv = ConvexView{w.cvVerts.p, w.cvMesh.p, w.cvFace.p, w.cvFaceIdx.p, w.cvFaceN.p, w.cvFaceC.p};
+    A.count = w.qCount.p; A.total = w.qCount.p + n; A.keys = w.qKeys.p;
+    if (n > 0) np_overlap_launch(s, A);
+    PGX_HIP(hipEventRecord(w.qEv[1], s));
+    std::vector<int32_t> cnt((size_t)n + 1);
+    PGX_HIP(hipMemcpyAsync(cnt.data(), w.qCount.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+    PGX_HIP(hipStreamSynchronize(s));
+    float ms0 = 0.f;
+    PGX_HIP(hipEventElapsedTime(&ms0, w.qEv[0], w.qEv[1]));
+    long sum = 0;
+    for (int i = 0; i < n; ++i) {
+        offsets[i] = (int32_t)sum;
+        sum += cnt[i];
+    }
+    offsets[n] = (int32_t)sum;
+    *total = (int32_t)sum;
+    if (sum != cnt[n]) throw PgxError{PGX_ERR_HIP, "overlap: the per-query hit counts disagree with the total"};
+    if (deviceUs) *deviceUs = 1000.0 * ms0;
+    if (sum > capacity) return PGX_ERR_CAPACITY;
+    if (sum == 0) return PGX_OK;
+    // the hits of each query ascending by collider: sort the keys (query, collider)
+    PGX_HIP(hipEventRecord(w.qEv[2], s));
+    radix_sort_u64(s, w.qKeys.p, w.qVals.p, (int)sum, qbits + cbits, w.sortT);
+    PGX_HIP(hipEventRecord(w.qEv[3], s));
+    std::vector<uint64_t> keys((size_t)sum);
+    PGX_HIP(hipMemcpyAsync(keys.data(), w.qKeys.p, sizeof(uint64_t) * sum, hipMemcpyDeviceToHost, s));
+    PGX_HIP(hipStreamSynchronize(s));
+    const uint64_t cmask = (1ull << cbits) - 1;
+    for (long k = 0; k < sum; ++k) colliders[k] = (int32_t)(keys[k] & cmask);
+    if (deviceUs) {
+        float ms1 = 0.f;
+        PGX_HIP(hipEventElapsedTime(&ms1, w.qEv[2], w.qEv[3]));
+        *deviceUs = 1000.0 * (ms0 + ms1);
+    }
+    return PGX_OK;
+}
+
 }  // namespace pgx

@@ -341,6 +341,9 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     upload(w.cflags, o.cflags, s); upload(w.ctype, o.ctype, s); upload(w.cindex, o.cindex, s);
     upload(w.centity, o.cent, s); upload(w.cparams, o.cparams, s); upload(w.clpos, o.clpos, s);
     upload(w.clquat, o.clquat, s); upload(w.cmat, o.cmat, s);
+    std::vector<int32_t> cdata((size_t)o.C, 0);   // lives until the stream is synchronised below
+    if (sc->col_data) std::copy(sc->col_data, sc->col_data + o.C, cdata.begin());
+    upload(w.cdata, cdata, s);
     upload(w.worldColStart, o.wstart, s); upload(w.colOfLocal, o.colOfLocal, s);
     upload(w.dynCols, o.dynCols, s); upload(w.staticCols, o.staticCols, s); upload(w.worldStaticStart, o.wss, s);
     world_alloc_derived(w, cfg, sc->n_trimesh);
@@ -1225,6 +1228,44 @@ int pgx_world_raycast(pgx_world* h, int n, const float* origins, const float* di
         PGX_HIP(hipSetDevice(w.device));
         qr_raycast(w, n, origins, dirs, max_distance, ray_world, ignore_entity, hit_collider, hit_distance, device_us);
         return PGX_OK;
+    });
+}
+
+int pgx_world_overlap(pgx_world* h, int n, const int32_t* types, const float* params, const float* positions,
+                      const float* orientations, const int32_t* query_world, const int32_t* filters, int capacity,
+                      int32_t* offsets, int32_t* colliders, int32_t* total, double* device_us) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    if (n < 0 || capacity < 0) return fail(PGX_ERR_INVALID, "query count and capacity must be >= 0");
+    if (!offsets || !total) return fail(PGX_ERR_INVALID, "NULL offsets or total");
+    if (n > 0 && (!types || !params || !positions || !orientations))
+        return fail(PGX_ERR_INVALID, "NULL query array");
+    if (capacity > 0 && !colliders) return fail(PGX_ERR_INVALID, "NULL collider array with capacity > 0");
+    World& w = h->w;
+    for (int i = 0; i < n; ++i) {
+        if (types[i] < PGX_SPHERE || types[i] > PGX_TRIANGLE_MESH)
+            return fail(PGX_ERR_INVALID, "query %d: shape type %d out of range", i, types[i]);
+        if (types[i] == PGX_CONVEX_MESH) {
+            const float m = params[4 * i + 3];
+            if (!(m >= 0.f && m < (float)w.nConvex) || m != (float)(int)m)
+                return fail(PGX_ERR_INVALID, "query %d: convex mesh %g is not one of the scene's %d", i, (double)m,
+                            w.nConvex);
+        }
+        if (query_world && (query_world[i] < 0 || query_world[i] >= w.W))
+            return fail(PGX_ERR_INVALID, "query %d: world %d out of range", i, query_world[i]);
+    }
+    if (n == 0) {
+        offsets[0] = 0;
+        *total = 0;
+        if (device_us) *device_us = 0.0;
+        return PGX_OK;
+    }
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(w.device));
+        int rc = qr_overlap(w, n, types, params, positions, orientations, query_world, filters, capacity, offsets,
+                            colliders, total, device_us);
+        if (rc == PGX_ERR_CAPACITY)
+            return fail(PGX_ERR_CAPACITY, "overlap: %d hits exceed the capacity %d", *total, capacity);
+        return rc;
     });
 }
 

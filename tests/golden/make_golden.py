@@ -392,6 +392,55 @@ def gen_raycast(ref, ids, n=4000):
                 aabb_hits=np.array(bhits, np.int32), aabb_outs=np.array(bouts))
 
 
+def shape_radius(t, p):
+    """A rough size of a shape, to place overlap partners near touching."""
+    if t == 0:
+        return p[0]
+    if t == 1:
+        return p[0] + p[1]
+    if t == 2:
+        return float(np.linalg.norm(p[:3])) * 0.8
+    return float(np.max(p[:3])) * 0.6
+
+
+def overlap_case(rng, ids):
+    """Two shapes (sphere, capsule, box, scaled convex mesh) at a distance near the sum of
+    their sizes, so that both answers occur for every type pair."""
+    out = []
+    for _ in range(2):
+        t = int(rng.integers(0, 4))
+        p = np.array(shape_params(t, rng) if t < 3 else [*rng.uniform(0.5, 1.5, 3), int(rng.integers(0, len(ids)))],
+                     np.float32)
+        out.append((t, p))
+    (t0, p0), (t1, p1) = out
+    pos0 = rng.uniform(-5, 5, 3).astype(np.float32)
+    d = rng.normal(size=3)
+    d /= np.linalg.norm(d)
+    dist = (shape_radius(t0, p0) + shape_radius(t1, p1)) * rng.uniform(0.4, 1.5)
+    pos1 = (pos0 + d * dist).astype(np.float32)
+    return t0, p0, pos0, rq(rng), t1, p1, pos1, rq(rng)
+
+
+def gen_overlap(ref, ids, n=6000):
+    """physecs::overlap (src/Overlap.cpp:203-235) for every pair of sphere, capsule, box and
+    scaled convex mesh."""
+    rng = np.random.default_rng(2027)
+    types, prm, poses, hits = [], [], [], []
+    for _ in range(n):
+        t0, p0, pos0, q0, t1, p1, pos1, q1 = overlap_case(rng, ids)
+        l0, l1 = p0.copy(), p1.copy()
+        if t0 == 3:
+            l0[3] = ids[int(p0[3])]
+        if t1 == 3:
+            l1[3] = ids[int(p1[3])]
+        hits.append(ol.overlap("ref", t0, l0, pos0, q0, t1, l1, pos1, q1))
+        types.append([t0, t1])
+        prm.append(np.concatenate([p0, p1]))
+        poses.append(np.concatenate([pos0, q0, pos1, q1]))
+    return dict(types=np.array(types, np.int32), params=np.array(prm), poses=np.array(poses),
+                hits=np.array(hits, np.int32))
+
+
 def main():
     if not ol.ref_available():
         raise SystemExit("oracle/_ref is not built: run `make -C oracle ref` where the reference sources are")
@@ -403,6 +452,11 @@ def main():
     m = ol.register_trimesh(verts, tris)
     idx, nodes = ol.trimesh_layout("ref", m, len(tris))
     only = sys.argv[1:]
+    if only == ["overlap"]:   # the overlap fixtures alone (the others are unchanged)
+        np.savez_compressed(os.path.join(HERE, "overlap.npz"),
+                            **{"ov_" + k: v for k, v in gen_overlap(ref, ids).items()}, **tables)
+        print("overlap.npz", os.path.getsize(os.path.join(HERE, "overlap.npz")), "bytes")
+        return
     if only == ["raycast"]:   # the ray fixtures alone (the others are unchanged)
         np.savez_compressed(os.path.join(HERE, "raycast.npz"),
                             **{"ray_" + k: v for k, v in gen_raycast(ref, ids).items()}, **tables)
@@ -410,6 +464,8 @@ def main():
         return
     np.savez_compressed(os.path.join(HERE, "raycast.npz"),
                         **{"ray_" + k: v for k, v in gen_raycast(ref, ids).items()}, **tables)
+    np.savez_compressed(os.path.join(HERE, "overlap.npz"),
+                        **{"ov_" + k: v for k, v in gen_overlap(ref, ids).items()}, **tables)
     np.savez_compressed(os.path.join(HERE, "collision.npz"),
                         **{"prim_" + k: v for k, v in gen_collision_primitive(ref).items()},
                         **{"cvx_" + k: v for k, v in gen_collision_convex(ref, ids).items()},

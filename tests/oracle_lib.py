@@ -131,6 +131,15 @@ def ray_geometry(which, o, d, t, prm, pos, q):
     return hit, out[0]
 
 
+def overlap(which, t0, p0, pos0, q0, t1, p1, pos1, q1):
+    """physecs::overlap of `which` ("orc" or "ref") for one pair: 0 or 1."""
+    lib = orc() if which == "orc" else ref()
+    fn = getattr(lib, which + "_overlap")
+    fn.argtypes = [ctypes.c_int, F, F, F, ctypes.c_int, F, F, F]
+    return fn(int(t0), fptr(_f32(p0, 4)), fptr(_f32(pos0, 3)), fptr(_f32(q0, 4)),
+              int(t1), fptr(_f32(p1, 4)), fptr(_f32(pos1, 3)), fptr(_f32(q1, 4)))
+
+
 def collision(lib, prefix, t0, p0, pos0, q0, t1, p1, pos1, q1):
     """Returns (hit, out[28]) for one narrowphase call of `lib` (orc or ref)."""
     out = np.zeros(28, np.float32)
@@ -236,6 +245,16 @@ class OracleWorld:
                         np.float32(md[i]), int(ig[i]), fptr(ti))
             t[i] = ti[0] if col[i] >= 0 else 0.0
         return col, t
+
+    def overlap(self, t, prm, pos, q, filt=0, cap=4096):
+        """Scene::overlap of one query shape: the overlapping collider indices, ascending."""
+        fn = self.lib.orc_world_overlap
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int, F, F, F, ctypes.c_int, ctypes.c_int, I32]
+        fn.restype = ctypes.c_int
+        out = np.zeros(cap, np.int32)
+        n = fn(self.h, int(t), fptr(_f32(prm, 4)), fptr(_f32(pos, 3)), fptr(_f32(q, 4)), int(filt), cap, iptr(out))
+        assert n <= cap
+        return out[:n].copy()
 
     def manifolds(self):
         n = self.lib.orc_world_num_manifolds(self.h)
