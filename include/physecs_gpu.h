@@ -289,6 +289,18 @@ int pgx_world_raycast(pgx_world* w, int n, const float* origins, const float* di
 int pgx_world_overlap(pgx_world* w, int n, const int32_t* types, const float* params, const float* positions,
                       const float* orientations, const int32_t* query_world, const int32_t* filters, int capacity,
                       int32_t* offsets, int32_t* colliders, int32_t* total, double* device_us);
+/* The last step's trigger events: Scene's onTriggerEnter / onTriggerExit listeners
+ * (include/Physecs/Physecs.h:30-40,220-223; src/Physecs.cpp:199-206,538-552). A pair with a
+ * trigger collider on either side (defaultContactFilter) makes no contact; when
+ * physecs::overlap says its shapes overlap it is in the step's trigger set. enter = pairs
+ * in this step's set but not the last step's, exit = the reverse. Each event is a pair
+ * (collider of the lower entity, collider of the other), descriptor indices, as the
+ * reference's ContactPair orders it; pairs ascend (the reference calls its listeners in
+ * unordered_set order). enter and exit hold 2 * capacity ints; when either list exceeds
+ * capacity, the counts are set and PGX_ERR_CAPACITY is returned. Events are of the last
+ * pgx_world_step only: a caller that wants every event reads them after every step. */
+int pgx_world_get_trigger_events(pgx_world* w, int capacity, int32_t* enter, int32_t* n_enter, int32_t* exit,
+                                 int32_t* n_exit);
 /* In PGX_EXEC_COLORS, enabled != 0 launches each step's substep loop as one HIP graph:
  * captured every step (its launches follow the step's colours), the instantiated graph
  * updated in place when the launch sequence keeps its shape. Results are the same either

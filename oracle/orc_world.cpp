@@ -11,9 +11,10 @@
 //   contact rows                 src/ContactConstraints.cpp:4-124
 //   cache swap + bounds          src/Physecs.cpp:553-559, 79-90
 //   joint rows are not restated here (no joints in the plain configurations).
-// Differences by design: trigger pairs are skipped (no trigger events), and the order of
-// contact constraints can be injected (set_order) so that a solver fed the GPU's graph
-// color order can be compared with it.
+//   trigger pairs + events       src/Physecs.cpp:199-206 (overlap), 538-552 (enter/exit, swap)
+// Differences by design: the order of contact constraints can be injected (set_order) so
+// that a solver fed the GPU's graph color order can be compared with it; trigger events are
+// kept as sorted lists (the reference calls its listeners in unordered_set order).
 #include <algorithm>
 #include <cmath>
 #include <limits>
@@ -23,6 +24,7 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <set>
 #include <unordered_set>
 #include <vector>
 #include "orc_collision.h"
@@ -112,6 +114,10 @@ struct orc_world {
     std::vector<CC> ccs;
     std::unordered_map<Key, CacheData, KeyHash> cache, cacheTemp;
     std::unordered_set<uint64_t> nonColliding;
+    // triggerCache / triggerCacheTemp (include/Physecs/Physecs.h:170-171) as collider pairs
+    // (collider of entity0, collider of entity1), and the last step's events
+    std::set<std::pair<int, int>> trig, trigTemp;
+    std::vector<std::pair<int, int>> trigEnter, trigExit;
     std::vector<int32_t> order;   // injected manifold order (5 ints per manifold)
     std::vector<Joint> joints;
     std::unordered_map<uint32_t, unsigned char> jointBitsets;   // per entity, src/Physecs.cpp:696-705
@@ -280,6 +286,10 @@ static void sap(orc_world* w) {
     }
 }
 
+namespace {
+bool overlapGeom(V3 p0, Q q0, const Geom& g0, V3 p1, Q q1, const Geom& g1);
+}
+
 // src/Physecs.cpp:183-316, one potential contact
 static void narrowTask(orc_world* w, int index, std::vector<CC>& outCC, std::vector<std::pair<Key, CacheData>>& outCache) {
     const Pair& cp = w->pairs[index];
@@ -291,7 +301,13 @@ static void narrowTask(orc_world* w, int index, std::vector<CC>& outCC, std::vec
     Q or0 = qmul(t0.quat, col0.lquat);
     V3 pos1 = t1.pos + qrot(t1.quat, col1.lpos);
     Q or1 = qmul(t1.quat, col1.lquat);
-    if (col0.trigger || col1.trigger) return;   // defaultContactFilter -> TRIGGER (no contact)
+    if (col0.trigger || col1.trigger) {   // defaultContactFilter -> TRIGGER: overlap only (:199-206)
+        if (overlapGeom(pos0, or0, col0.geom, pos1, or1, col1.geom)) {
+            std::lock_guard<std::mutex> lk(w->mtx);
+            w->trigTemp.insert({cp.g0, cp.g1});
+        }
+        return;
+    }
     uint64_t ek = ((uint64_t)cp.e0 << 32) | cp.e1;
     if (w->nonColliding.count(ek)) return;
     std::vector<Manifold> buf;
@@ -364,6 +380,7 @@ static void narrowTask(orc_world* w, int index, std::vector<CC>& outCC, std::vec
 static void narrowphase(orc_world* w) {
     w->ccs.clear();
     w->cacheTemp.clear();
+    w->trigTemp.clear();
     int count = (int)w->pairs.size();
     if (w->numThreads <= 0) {
         // ThreadPool with no workers: the caller drains indices in descending order
@@ -965,6 +982,14 @@ void orc_world_step(orc_world* w, float timeStep) {
     applyOrder(w);
     buildJointRows(w);
     substeps(w, timeStep);
+    // src/Physecs.cpp:538-552: enter = new pairs, exit = pairs no longer overlapping
+    w->trigEnter.clear();
+    w->trigExit.clear();
+    for (const auto& p : w->trigTemp)
+        if (!w->trig.count(p)) w->trigEnter.push_back(p);
+    for (const auto& p : w->trig)
+        if (!w->trigTemp.count(p)) w->trigExit.push_back(p);
+    w->trig.swap(w->trigTemp);
     w->cache.swap(w->cacheTemp);
     for (auto& e : w->bp) {
         const Collider& c = w->cols[e.collider];
@@ -1541,6 +1566,17 @@ bool overlapGeom(V3 p0, Q q0, const Geom& g0, V3 p1, Q q1, const Geom& g1) {
 }  // namespace
 
 extern "C" {
+
+// The last step's trigger events: which = 0 enter, 1 exit; pairs (collider of entity0,
+// collider of entity1) ascending. Returns the count (at most cap pairs written).
+int orc_world_trigger_events(const orc_world* w, int which, int cap, int32_t* out) {
+    const auto& v = which == 0 ? w->trigEnter : w->trigExit;
+    for (size_t i = 0; i < v.size() && (int)i < cap; ++i) {
+        out[2 * i] = v[i].first;
+        out[2 * i + 1] = v[i].second;
+    }
+    return (int)v.size();
+}
 
 int orc_overlap(int type0, const float* prm0, const float* pos0, const float* q0, int type1, const float* prm1,
                 const float* pos1, const float* q1) {

@@ -30,6 +30,7 @@ EXPORTS = [
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
+    "pgx_world_get_trigger_events",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -135,6 +136,7 @@ def lib():
                                         ctypes.POINTER(ctypes.c_double)]
         L.pgx_world_overlap.argtypes = [vp, ctypes.c_int, I32, F, F, F, I32, I32, ctypes.c_int, I32, I32, I32,
                                         ctypes.POINTER(ctypes.c_double)]
+        L.pgx_world_get_trigger_events.argtypes = [vp, ctypes.c_int, I32, I32, I32, I32]
         L.pgx_server_set_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
         L.pgx_server_create.argtypes = [ctypes.POINTER(PgxConfig), ctypes.POINTER(PgxServerConfig), ctypes.POINTER(vp)]
         L.pgx_server_destroy.argtypes = [vp]
@@ -349,6 +351,22 @@ class World:
             check(rc)
             break
         return [cols[off[i]:off[i + 1]].copy() for i in range(n)], us.value
+
+    def trigger_events(self, capacity=1024):
+        """pgx_world_get_trigger_events: the last step's (enter, exit) events, each an
+        (n, 2) array of (collider of the lower entity, collider of the other), ascending."""
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        while True:
+            en = np.zeros((max(capacity, 1), 2), np.int32)
+            ex = np.zeros((max(capacity, 1), 2), np.int32)
+            ne, nx = ctypes.c_int32(0), ctypes.c_int32(0)
+            rc = self.L.pgx_world_get_trigger_events(self.h, capacity, en.ctypes.data_as(i32), ctypes.byref(ne),
+                                                     ex.ctypes.data_as(i32), ctypes.byref(nx))
+            if rc == ERR_CAPACITY:
+                capacity = max(ne.value, nx.value)
+                continue
+            check(rc)
+            return en[: ne.value].copy(), ex[: nx.value].copy()
 
     def exec_mode(self):
         m = ctypes.c_int(0)

@@ -9,6 +9,7 @@
 // (a quad clipped by four edges has at most 8 vertices).
 #include "pgx_internal.h"
 #include "shape_bounds.h"
+#include <iterator>
 
 namespace pgx {
 
@@ -700,7 +701,8 @@ __global__ __launch_bounds__(256) void k_narrowphase(
     const float4* __restrict__ bquat, const uint64_t* __restrict__ nonColliding, int nNonColliding,
     int32_t* __restrict__ npNum, float4* __restrict__ npNormal, float4* __restrict__ npPts,
     int32_t* __restrict__ heavy, int32_t* __restrict__ nHeavy, int32_t* __restrict__ meshPairs,
-    int32_t* __restrict__ nMesh, int32_t* __restrict__ meshCount) {
+    int32_t* __restrict__ nMesh, int32_t* __restrict__ meshCount, int32_t* __restrict__ trigList,
+    int32_t* __restrict__ nTrig) {
     // pairs per lane = pairs / (grid size): a lane takes every grid-size'th pair of the
     // class order; results are stored at the pair's own slot
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nPairs; i += gridDim.x * blockDim.x) {
@@ -716,6 +718,12 @@ __global__ __launch_bounds__(256) void k_narrowphase(
         // defaultContactFilter: a trigger on either side makes a trigger pair, not a contact
         // (src/Physecs.cpp:20-23,200-207); joint-connected / setCanCollide pairs are skipped (:209)
         bool skip = ((cflags[c0] | cflags[c1]) & CF_TRIGGER) != 0;
+        // trigger pairs go to k_triggers (before the non-colliding test, as in the reference)
+        const bool toTrig = skip && trigList != nullptr;
+        if (__ballot(toTrig)) {
+            int i = wave_append(nTrig, toTrig);
+            if (toTrig) trigList[i] = p;
+        }
         if (!skip && nNonColliding) {
             uint64_t ek = ((uint64_t)centity[c0] << 32) | centity[c1];
             skip = in_sorted(nonColliding, nNonColliding, ek);
@@ -1016,6 +1024,41 @@ __global__ __launch_bounds__(64) void k_overlap(OverlapArgs A) {
     if (live) A.count[i] = hits;
 }
 
+// Trigger pairs: physecs::overlap of the two colliders (src/Physecs.cpp:199-206); the keys
+// of the overlapping pairs are appended with one atomic per wave. A fixed grid walks the
+// list, whose length is read on the device.
+__global__ __launch_bounds__(64) void k_triggers(
+    const int32_t* __restrict__ nTrigP, const int32_t* __restrict__ trigList, const uint64_t* __restrict__ pkeys,
+    int localBits, const int32_t* __restrict__ worldColStart, const int32_t* __restrict__ colOfLocal,
+    const int32_t* __restrict__ cbody, const int32_t* __restrict__ ctype, const float4* __restrict__ cparams,
+    const float4* __restrict__ clpos, const float4* __restrict__ clquat, const float4* __restrict__ bpos,
+    const float4* __restrict__ bquat, ConvexView cv, int32_t* __restrict__ nHits, uint64_t* __restrict__ hitKeys) {
+    const int nTrig = *nTrigP;
+    const uint64_t lmask = (1ull << localBits) - 1;
+    // blocks are one wave, so the trip count is wave-uniform and the appends see whole waves
+    for (int base = blockIdx.x * blockDim.x; base < nTrig; base += gridDim.x * blockDim.x) {
+        const int h = base + (int)threadIdx.x;
+        bool hit = false;
+        uint64_t key = 0;
+        if (h < nTrig) {
+            key = pkeys[trigList[h]];
+            const int b = worldColStart[(int)(key >> (2 * localBits))];
+            const int c0 = colOfLocal[b + (int)((key >> localBits) & lmask)];
+            const int c1 = colOfLocal[b + (int)(key & lmask)];
+            const int b0 = cbody[c0], b1 = cbody[c1];
+            const qf bq0 = q4(bquat[b0]), bq1 = q4(bquat[b1]);
+            const f3 p0 = add(xyz(bpos[b0]), qrot(bq0, xyz(clpos[c0])));
+            const f3 p1 = add(xyz(bpos[b1]), qrot(bq1, xyz(clpos[c1])));
+            hit = overlap_shapes(ctype[c0], cparams[c0], p0, qmul(bq0, q4(clquat[c0])), ctype[c1], cparams[c1], p1,
+                                 qmul(bq1, q4(clquat[c1])), cv);
+        }
+        if (__ballot(hit)) {
+            const int slot = wave_append(nHits, hit);
+            if (hit) hitKeys[slot] = key;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ triangle-mesh pairs
 // Everything the three mesh passes read.
 struct MeshArgs {
@@ -1187,6 +1230,14 @@ void np_overlap_launch(hipStream_t s, const OverlapArgs& A) {
 }
 
 void np_run(World& w, int nPairs) {
+    if (w.hasTriggers) {   // this step's list: its pair and hit counts start at zero
+        // every trigger pair and hit has a pair slot: the lists must hold pairCap entries
+        if (w.trigList.n < (size_t)w.pairCap || w.trigKeys[0].n < (size_t)w.pairCap ||
+            w.trigKeys[1].n < (size_t)w.pairCap || w.trigN.n < 4)
+            throw PgxError{PGX_ERR_HIP, "trigger lists smaller than the pair capacity"};
+        w.trigCur ^= 1;
+        fill_i32(w.stream, w.trigN.p + 2 * w.trigCur, 0, 2);
+    }
     if (nPairs == 0) return;
     const int ppl = std::max(1, w.tune.pairsPerLane);
     w.npClass.alloc(w.pairCap);
@@ -1200,8 +1251,17 @@ void np_run(World& w, int nPairs) {
                        w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.ctype.p, w.cflags.p,
                        w.cparams.p, w.clpos.p, w.clquat.p, w.centity.p, w.bpos.p, w.bquat.p, w.nonColliding.p,
                        w.nNonColliding, w.npNum.p, w.npNormal.p, w.npPts.p, w.heavy.p, w.counters.p + 8,
-                       w.meshPairs.p, w.counters.p + 10, w.meshCount.p);
+                       w.meshPairs.p, w.counters.p + 10, w.meshCount.p,
+                       w.hasTriggers ? w.trigList.p : nullptr, w.trigN.p + 2 * w.trigCur);
     PGX_LAUNCH_CHECK();
+    if (w.hasTriggers) {
+        ConvexView cv{w.cvVerts.p, w.cvMesh.p, w.cvFace.p, w.cvFaceIdx.p, w.cvFaceN.p, w.cvFaceC.p};
+        hipLaunchKernelGGL(k_triggers, dim3(std::min(div_up(nPairs, 64), 1024)), dim3(64), 0, w.stream,
+                           w.trigN.p + 2 * w.trigCur, w.trigList.p, w.pkeys.p, w.localBits, w.worldColStart.p,
+                           w.colOfLocal.p, w.cbody.p, w.ctype.p, w.cparams.p, w.clpos.p, w.clquat.p, w.bpos.p,
+                           w.bquat.p, cv, w.trigN.p + 2 * w.trigCur + 1, w.trigKeys[w.trigCur].p);
+        PGX_LAUNCH_CHECK();
+    }
     if (w.nConvex > 0) {
         ConvexView cv{w.cvVerts.p, w.cvMesh.p, w.cvFace.p, w.cvFaceIdx.p, w.cvFaceN.p, w.cvFaceC.p};
         hipLaunchKernelGGL(k_narrowphase_convex, dim3(div_up(w.epaLanes, 64)), dim3(64), 0, w.stream,
@@ -1231,6 +1291,44 @@ void np_run(World& w, int nPairs) {
                            w.npTri.p, w.meshFirst.p, w.meshCount.p, w.counters.p + 11, w.mmCap, w.pairCap);
         PGX_LAUNCH_CHECK();
     }
+}
+
+int np_trigger_events(World& w, int cap, int32_t* enter, int32_t* nEnter, int32_t* exit, int32_t* nExit) {
+    *nEnter = 0;
+    *nExit = 0;
+    if (!w.hasTriggers) return PGX_OK;
+    hipStream_t s = w.stream;
+    int32_t n[4];
+    PGX_HIP(hipMemcpyAsync(n, w.trigN.p, sizeof n, hipMemcpyDeviceToHost, s));
+    PGX_HIP(hipStreamSynchronize(s));
+    const int cur = w.trigCur, prev = cur ^ 1;
+    std::vector<uint64_t> a((size_t)n[2 * cur + 1]), b((size_t)n[2 * prev + 1]);
+    if (!a.empty()) PGX_HIP(hipMemcpyAsync(a.data(), w.trigKeys[cur].p, 8 * a.size(), hipMemcpyDeviceToHost, s));
+    if (!b.empty()) PGX_HIP(hipMemcpyAsync(b.data(), w.trigKeys[prev].p, 8 * b.size(), hipMemcpyDeviceToHost, s));
+    PGX_HIP(hipStreamSynchronize(s));
+    std::sort(a.begin(), a.end());
+    std::sort(b.begin(), b.end());
+    std::vector<uint64_t> en, ex;
+    std::set_difference(a.begin(), a.end(), b.begin(), b.end(), std::back_inserter(en));   // src/Physecs.cpp:538-544
+    std::set_difference(b.begin(), b.end(), a.begin(), a.end(), std::back_inserter(ex));   // :545-551
+    const uint64_t lmask = (1ull << w.localBits) - 1;
+    auto decode = [&](const std::vector<uint64_t>& keys) {
+        std::vector<std::pair<int32_t, int32_t>> out;
+        for (uint64_t k : keys) {
+            const int base = w.hostWorldColStart[(size_t)(k >> (2 * w.localBits))];
+            out.push_back({w.hostColOfLocal[base + (int)((k >> w.localBits) & lmask)],
+                           w.hostColOfLocal[base + (int)(k & lmask)]});
+        }
+        std::sort(out.begin(), out.end());
+        return out;
+    };
+    const auto pe = decode(en), px = decode(ex);
+    *nEnter = (int32_t)pe.size();
+    *nExit = (int32_t)px.size();
+    if ((int)pe.size() > cap || (int)px.size() > cap) return PGX_ERR_CAPACITY;
+    for (size_t i = 0; i < pe.size(); ++i) { enter[2 * i] = pe[i].first; enter[2 * i + 1] = pe[i].second; }
+    for (size_t i = 0; i < px.size(); ++i) { exit[2 * i] = px[i].first; exit[2 * i + 1] = px[i].second; }
+    return PGX_OK;
 }
 
 }  // namespace pgx

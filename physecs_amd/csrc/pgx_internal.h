@@ -278,6 +278,15 @@ struct World {
     DBuf<int32_t> qHit;
     DBuf<float> qT;
     hipEvent_t qEv[4] = {};
+    // trigger pairs (defaultContactFilter's TRIGGER: overlap only, src/Physecs.cpp:199-206):
+    // per step the trigger pairs' slots, and the keys of those that overlap, double-buffered
+    // (list trigCur is this step's, the other the last step's); trigN[2 * k] = trigger pairs
+    // found while list k was current, trigN[2 * k + 1] = list k's overlapping pairs
+    bool hasTriggers = false;
+    int trigCur = 0;
+    DBuf<int32_t> trigList, trigN;
+    DBuf<uint64_t> trigKeys[2];
+    std::vector<int32_t> hostColOfLocal, hostWorldColStart;
     // overlap queries: Collider::data (descriptor order), query shapes, per-query hit
     // counts and offsets, the hit keys
     DBuf<int32_t> cdata;
@@ -356,6 +365,9 @@ struct OverlapArgs {
 
 // narrowphase.hip
 void np_run(World& w, int nPairs);
+// the last step's trigger events as (collider of entity0, collider of entity1) pairs,
+// ascending; returns PGX_ERR_CAPACITY (counts set) when either list exceeds cap
+int np_trigger_events(World& w, int cap, int32_t* enter, int32_t* nEnter, int32_t* exit, int32_t* nExit);
 void np_overlap_launch(hipStream_t s, const OverlapArgs& A);
 int upload_triangle_meshes(World& w, const pgx_scene_desc* sc, hipStream_t s);   // returns the BVH depth
 // device-layout arrays of a scene descriptor (world.hip)

@@ -347,6 +347,17 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     upload(w.worldColStart, o.wstart, s); upload(w.colOfLocal, o.colOfLocal, s);
     upload(w.dynCols, o.dynCols, s); upload(w.staticCols, o.staticCols, s); upload(w.worldStaticStart, o.wss, s);
     world_alloc_derived(w, cfg, sc->n_trimesh);
+    // trigger lists: at most one entry per pair slot (pairCap is set by world_alloc_derived)
+    w.hasTriggers = false;
+    for (int c = 0; c < o.C && sc->col_trigger; ++c) w.hasTriggers = w.hasTriggers || sc->col_trigger[c] != 0;
+    w.trigCur = 0;
+    if (w.hasTriggers) {
+        w.trigList.alloc(w.pairCap); w.trigKeys[0].alloc(w.pairCap); w.trigKeys[1].alloc(w.pairCap);
+        w.trigN.alloc(4);
+        fill_i32(s, w.trigN.p, 0, 4);
+        w.hostColOfLocal = o.colOfLocal;
+        w.hostWorldColStart = o.wstart;
+    }
     if (sc->n_convex > 0) {        // ConvexMesh vertex buffers padded to a multiple of 4 with the last vertex
         // (src/ConvexMesh.cpp:3-11); faces, normals and centroids as given
         std::vector<float4> cvv, cvn, cvc;
@@ -681,8 +692,9 @@ namespace pgx {
 // counters and statistics. Autotuning steps the world from a snapshot and restores it.
 struct Snapshot {
     DBuf<float4> pos, quat, vel, ang, bmin, bmax, lp, gear;
-    DBuf<uint64_t> key;
-    DBuf<int32_t> tri, num, gearInit;
+    DBuf<uint64_t> key, trigKeys[2];
+    DBuf<int32_t> tri, num, gearInit, trigN;
+    int trigCur = 0;
     int cacheCur = 0, cacheCount = 0, lastRounds = 0, lastPairs = 0, lastManifolds = 0, lastColors = 0;
     int lastMeshSlots = 0;
     long statSteps = 0, statSolveLaunches = 0, lastPointsHost = 0;
@@ -708,6 +720,11 @@ static void take_snapshot(World& w, Snapshot& z) {
         z.gear.alloc(w.J); z.gearInit.alloc(w.J);
         d2d(z.gear.p, w.jGear.p, w.J, s); d2d(z.gearInit.p, w.jGearInit.p, w.J, s);
     }
+    if (w.hasTriggers) {   // the trigger lists, so that the events after a restore are unchanged
+        z.trigN.alloc(4); d2d(z.trigN.p, w.trigN.p, 4, s);
+        for (int k = 0; k < 2; ++k) { z.trigKeys[k].alloc(w.pairCap); d2d(z.trigKeys[k].p, w.trigKeys[k].p, w.pairCap, s); }
+        z.trigCur = w.trigCur;
+    }
     z.cacheCur = cur; z.cacheCount = n;
     z.lastRounds = w.lastRounds; z.lastPairs = w.lastPairs; z.lastManifolds = w.lastManifolds;
     z.lastColors = w.lastColors; z.lastMeshSlots = w.lastMeshSlots; z.lastPointsHost = w.lastPointsHost;
@@ -725,6 +742,11 @@ static void restore_snapshot(World& w, const Snapshot& z) {
     d2d(w.cacheKey[cur].p, z.key.p, n, s); d2d(w.cacheTri[cur].p, z.tri.p, n, s); d2d(w.cacheNum[cur].p, z.num.p, n, s);
     for (int j = 0; j < 4; ++j) d2d(w.cacheLp[cur].p + (size_t)j * w.manifoldCap, z.lp.p + (size_t)j * n, n, s);
     if (w.J > 0) { d2d(w.jGear.p, z.gear.p, w.J, s); d2d(w.jGearInit.p, z.gearInit.p, w.J, s); }
+    if (w.hasTriggers) {
+        d2d(w.trigN.p, z.trigN.p, 4, s);
+        for (int k = 0; k < 2; ++k) d2d(w.trigKeys[k].p, z.trigKeys[k].p, w.pairCap, s);
+        w.trigCur = z.trigCur;
+    }
     w.cacheCur = cur; w.cacheCount[cur] = n;
     w.lastRounds = z.lastRounds; w.lastPairs = z.lastPairs; w.lastManifolds = z.lastManifolds;
     w.lastColors = z.lastColors; w.lastMeshSlots = z.lastMeshSlots; w.lastPointsHost = z.lastPointsHost;
@@ -1265,6 +1287,22 @@ int pgx_world_overlap(pgx_world* h, int n, const int32_t* types, const float* pa
                             colliders, total, device_us);
         if (rc == PGX_ERR_CAPACITY)
             return fail(PGX_ERR_CAPACITY, "overlap: %d hits exceed the capacity %d", *total, capacity);
+        return rc;
+    });
+}
+
+int pgx_world_get_trigger_events(pgx_world* h, int capacity, int32_t* enter, int32_t* n_enter, int32_t* exit,
+                                 int32_t* n_exit) {
+    if (!h || !n_enter || !n_exit) return fail(PGX_ERR_INVALID, "NULL argument");
+    if (capacity < 0) return fail(PGX_ERR_INVALID, "capacity must be >= 0");
+    if (capacity > 0 && (!enter || !exit)) return fail(PGX_ERR_INVALID, "NULL event array with capacity > 0");
+    World& w = h->w;
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(w.device));
+        int rc = np_trigger_events(w, capacity, enter, n_enter, exit, n_exit);
+        if (rc == PGX_ERR_CAPACITY)
+            return fail(PGX_ERR_CAPACITY, "trigger events: %d enter / %d exit exceed the capacity %d", *n_enter,
+                        *n_exit, capacity);
         return rc;
     });
 }

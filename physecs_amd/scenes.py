@@ -263,17 +263,18 @@ class Scene:
         self._put("col_data", [data])
 
     def add_body(self, gtype, params, pos, quat=(0, 0, 0, 1), density=1000.0, vel=(0, 0, 0), angvel=(0, 0, 0),
-                 material=(0.6, 0.0, 0.0), world=0, data=0):
+                 material=(0.6, 0.0, 0.0), world=0, data=0, trigger=False):
         a = self.add_actor(pos, quat, world)
         mass = density * true_volume(gtype, params)
         im, com, invI = mass_props(gtype, params, mass)
         self.add_dynamic(a, im, com, invI, vel, angvel)
-        self.add_collider(a, gtype, params, material, data=data)
+        self.add_collider(a, gtype, params, material, data=data, trigger=trigger)
         return a
 
-    def add_static(self, gtype, params, pos, quat=(0, 0, 0, 1), material=(0.6, 0.0, 0.0), world=0, data=0):
+    def add_static(self, gtype, params, pos, quat=(0, 0, 0, 1), material=(0.6, 0.0, 0.0), world=0, data=0,
+                   trigger=False):
         a = self.add_actor(pos, quat, world)
-        self.add_collider(a, gtype, params, material, data=data)
+        self.add_collider(a, gtype, params, material, data=data, trigger=trigger)
         return a
 
     def add_bodies_bulk(self, gtypes, params, pos, quats, vels, worlds, density=1000.0, material=(0.6, 0.0, 0.0)):

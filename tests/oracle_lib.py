@@ -256,6 +256,19 @@ class OracleWorld:
         assert n <= cap
         return out[:n].copy()
 
+    def trigger_events(self):
+        """The last step's trigger (enter, exit) events: (n, 2) collider pairs, ascending."""
+        fn = self.lib.orc_world_trigger_events
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, I32]
+        fn.restype = ctypes.c_int
+        out = []
+        for which in (0, 1):
+            n = fn(self.h, which, 0, iptr(np.zeros(2, np.int32)))
+            buf = np.zeros((max(n, 1), 2), np.int32)
+            fn(self.h, which, n, iptr(buf))
+            out.append(buf[:n].copy())
+        return tuple(out)
+
     def manifolds(self):
         n = self.lib.orc_world_num_manifolds(self.h)
         keys = np.zeros((n, 5), np.int32)
