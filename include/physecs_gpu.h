@@ -301,6 +301,19 @@ int pgx_world_overlap(pgx_world* w, int n, const int32_t* types, const float* pa
  * pgx_world_step only: a caller that wants every event reads them after every step. */
 int pgx_world_get_trigger_events(pgx_world* w, int capacity, int32_t* enter, int32_t* n_enter, int32_t* exit,
                                  int32_t* n_exit);
+/* Save and resume a world's state between steps (no counterpart in the reference, which
+ * keeps its hidden state -- contactCache, GearJointData, triggerCache; include/Physecs/
+ * Physecs.h:167-171, include/Physecs/Joints/GearJoint.h:7-14 -- only inside a live Scene).
+ * The state is one byte string: poses, velocities, broadphase bounds, the contact cache, the
+ * gear joints' angles, the trigger lists and the last step's sizes. pgx_world_state_size
+ * gives its length now (it changes from step to step); pgx_world_save_state writes it
+ * (PGX_ERR_CAPACITY when bytes is too small); pgx_world_load_state replaces the world's
+ * state with one saved by a world created from the same descriptor and joints
+ * (PGX_ERR_INVALID otherwise, the world unchanged). Stepping after a load gives the same
+ * results, bit for bit, as stepping the world that saved it. */
+int pgx_world_state_size(pgx_world* w, size_t* bytes);
+int pgx_world_save_state(pgx_world* w, void* buf, size_t bytes);
+int pgx_world_load_state(pgx_world* w, const void* buf, size_t bytes);
 /* In PGX_EXEC_COLORS, enabled != 0 launches each step's substep loop as one HIP graph:
  * captured every step (its launches follow the step's colours), the instantiated graph
  * updated in place when the launch sequence keeps its shape. Results are the same either

@@ -30,7 +30,7 @@ EXPORTS = [
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
-    "pgx_world_get_trigger_events",
+    "pgx_world_get_trigger_events", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -137,6 +137,9 @@ def lib():
         L.pgx_world_overlap.argtypes = [vp, ctypes.c_int, I32, F, F, F, I32, I32, ctypes.c_int, I32, I32, I32,
                                         ctypes.POINTER(ctypes.c_double)]
         L.pgx_world_get_trigger_events.argtypes = [vp, ctypes.c_int, I32, I32, I32, I32]
+        L.pgx_world_state_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+        L.pgx_world_save_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
+        L.pgx_world_load_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
         L.pgx_server_set_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
         L.pgx_server_create.argtypes = [ctypes.POINTER(PgxConfig), ctypes.POINTER(PgxServerConfig), ctypes.POINTER(vp)]
         L.pgx_server_destroy.argtypes = [vp]
@@ -367,6 +370,19 @@ class World:
                 continue
             check(rc)
             return en[: ne.value].copy(), ex[: nx.value].copy()
+
+    def save_state(self):
+        """pgx_world_save_state: the world's state as bytes."""
+        n = ctypes.c_size_t(0)
+        check(self.L.pgx_world_state_size(self.h, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        check(self.L.pgx_world_save_state(self.h, buf, n.value))
+        return buf.raw
+
+    def load_state(self, state):
+        """pgx_world_load_state: replaces the world's state with bytes from save_state."""
+        buf = ctypes.create_string_buffer(bytes(state), len(state))
+        check(self.L.pgx_world_load_state(self.h, buf, len(state)))
 
     def exec_mode(self):
         m = ctypes.c_int(0)

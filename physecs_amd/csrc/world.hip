@@ -755,6 +755,109 @@ static void restore_snapshot(World& w, const Snapshot& z) {
     PGX_HIP(hipStreamSynchronize(s));
 }
 
+// ------------------------------------------------------------------ state save / load
+// Everything a world carries from one step to the next, as one host byte string: poses,
+// velocities, broadphase bounds (with their margins), the contact cache (the reference's
+// contactCache, include/Physecs/Physecs.h:167), the gear joints' angle state
+// (GearJointData, include/Physecs/Joints/GearJoint.h:7-14), the trigger lists
+// (triggerCache) and the last step's sizes that the next step's launches start from. The
+// structure (descriptor, joints, setCanCollide pairs, parameters) is not in it: a world
+// loads a state saved by a world created from the same descriptor and joints.
+namespace {
+constexpr uint32_t STATE_MAGIC = 0x53584750u;   // "PGXS"
+constexpr int32_t STATE_VERSION = 1;
+struct StateHeader {
+    uint32_t magic;
+    int32_t version, A, D, C, J, W, pairCap, manifoldCap;
+    int32_t cacheCount, hasTriggers, trigCur, trigN[4];
+    int32_t lastPairs, lastManifolds, lastColors, lastRounds, lastPoints, lastMeshSlots;
+};
+
+size_t state_bytes(const StateHeader& h) {
+    size_t n = sizeof(StateHeader);
+    n += 16 * (2 * (size_t)h.A + 2 * (size_t)h.D + 2 * (size_t)h.C);
+    n += (size_t)h.cacheCount * (8 + 4 + 4 + 4 * 16);
+    n += (size_t)h.J * (16 + 4);
+    if (h.hasTriggers) n += 8 * ((size_t)h.trigN[1] + (size_t)h.trigN[3]);
+    return n;
+}
+
+StateHeader state_header(World& w) {
+    StateHeader h{};
+    h.magic = STATE_MAGIC; h.version = STATE_VERSION;
+    h.A = w.A; h.D = w.D; h.C = w.C; h.J = w.J; h.W = w.W; h.pairCap = w.pairCap; h.manifoldCap = w.manifoldCap;
+    h.cacheCount = w.cacheCount[w.cacheCur];
+    h.hasTriggers = w.hasTriggers ? 1 : 0;
+    h.trigCur = w.trigCur;
+    if (w.hasTriggers) {
+        PGX_HIP(hipMemcpyAsync(h.trigN, w.trigN.p, sizeof h.trigN, hipMemcpyDeviceToHost, w.stream));
+        PGX_HIP(hipStreamSynchronize(w.stream));
+    }
+    h.lastPairs = w.lastPairs; h.lastManifolds = w.lastManifolds; h.lastColors = w.lastColors;
+    h.lastRounds = w.lastRounds; h.lastPoints = w.lastPoints; h.lastMeshSlots = w.lastMeshSlots;
+    return h;
+}
+
+// The sections in order: (device pointer, bytes); `save` copies them out, `load` in.
+template <class F>
+void state_sections(World& w, const StateHeader& h, F&& f) {
+    const int cur = w.cacheCur;
+    const size_t n = (size_t)h.cacheCount;
+    f(w.bpos.p, 16 * (size_t)h.A); f(w.bquat.p, 16 * (size_t)h.A);
+    f(w.dvel.p, 16 * (size_t)h.D); f(w.dang.p, 16 * (size_t)h.D);
+    f(w.cbmin.p, 16 * (size_t)h.C); f(w.cbmax.p, 16 * (size_t)h.C);
+    f(w.cacheKey[cur].p, 8 * n); f(w.cacheTri[cur].p, 4 * n); f(w.cacheNum[cur].p, 4 * n);
+    for (int j = 0; j < 4; ++j) f(w.cacheLp[cur].p + (size_t)j * w.manifoldCap, 16 * n);
+    f(w.jGear.p, 16 * (size_t)h.J); f(w.jGearInit.p, 4 * (size_t)h.J);
+    if (h.hasTriggers) { f(w.trigKeys[0].p, 8 * (size_t)h.trigN[1]); f(w.trigKeys[1].p, 8 * (size_t)h.trigN[3]); }
+}
+}  // namespace
+
+size_t world_state_size(World& w) { return state_bytes(state_header(w)); }
+
+void world_save_state(World& w, void* buf, size_t bytes) {
+    const StateHeader h = state_header(w);
+    if (bytes < state_bytes(h)) throw PgxError{PGX_ERR_CAPACITY, "state buffer of " + std::to_string(bytes) +
+                                                                     " bytes < " + std::to_string(state_bytes(h))};
+    char* out = static_cast<char*>(buf);
+    std::memcpy(out, &h, sizeof h);
+    size_t at = sizeof h;
+    state_sections(w, h, [&](void* dev, size_t nb) {
+        if (nb) PGX_HIP(hipMemcpyAsync(out + at, dev, nb, hipMemcpyDeviceToHost, w.stream));
+        at += nb;
+    });
+    PGX_HIP(hipStreamSynchronize(w.stream));
+}
+
+void world_load_state(World& w, const void* buf, size_t bytes) {
+    StateHeader h;
+    if (bytes < sizeof h) throw PgxError{PGX_ERR_INVALID, "state buffer shorter than its header"};
+    std::memcpy(&h, buf, sizeof h);
+    if (h.magic != STATE_MAGIC || h.version != STATE_VERSION)
+        throw PgxError{PGX_ERR_INVALID, "not a saved world state (magic or version)"};
+    if (h.A != w.A || h.D != w.D || h.C != w.C || h.J != w.J || h.W != w.W ||
+        h.hasTriggers != (w.hasTriggers ? 1 : 0))
+        throw PgxError{PGX_ERR_INVALID, "the state was saved by a world of another shape (actors, bodies, colliders, "
+                                        "joints, worlds or triggers differ)"};
+    if (h.cacheCount < 0 || h.cacheCount > w.manifoldCap || h.trigCur < 0 || h.trigCur > 1)
+        throw PgxError{PGX_ERR_INVALID, "state header out of range"};
+    for (int k = 0; k < 4; ++k)
+        if (h.trigN[k] < 0 || h.trigN[k] > w.pairCap) throw PgxError{PGX_ERR_INVALID, "trigger counts out of range"};
+    if (bytes < state_bytes(h)) throw PgxError{PGX_ERR_INVALID, "state buffer truncated"};
+    const char* in = static_cast<const char*>(buf);
+    size_t at = sizeof h;
+    state_sections(w, h, [&](void* dev, size_t nb) {
+        if (nb) PGX_HIP(hipMemcpyAsync(dev, in + at, nb, hipMemcpyHostToDevice, w.stream));
+        at += nb;
+    });
+    if (w.hasTriggers) PGX_HIP(hipMemcpyAsync(w.trigN.p, h.trigN, sizeof h.trigN, hipMemcpyHostToDevice, w.stream));
+    PGX_HIP(hipStreamSynchronize(w.stream));
+    w.cacheCount[w.cacheCur] = h.cacheCount;
+    w.trigCur = h.trigCur;
+    w.lastPairs = h.lastPairs; w.lastManifolds = h.lastManifolds; w.lastColors = h.lastColors;
+    w.lastRounds = h.lastRounds; w.lastPoints = h.lastPoints; w.lastMeshSlots = h.lastMeshSlots;
+}
+
 // Picks the launch shapes (and between per-colour launches and per-world passes) by
 // timing trial steps from the world's current state (one
 // untimed step, then trial_steps timed, wall clock with the stream drained; the better of
@@ -1304,6 +1407,33 @@ int pgx_world_get_trigger_events(pgx_world* h, int capacity, int32_t* enter, int
             return fail(PGX_ERR_CAPACITY, "trigger events: %d enter / %d exit exceed the capacity %d", *n_enter,
                         *n_exit, capacity);
         return rc;
+    });
+}
+
+int pgx_world_state_size(pgx_world* h, size_t* bytes) {
+    if (!h || !bytes) return fail(PGX_ERR_INVALID, "NULL argument");
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(h->w.device));
+        *bytes = world_state_size(h->w);
+        return PGX_OK;
+    });
+}
+
+int pgx_world_save_state(pgx_world* h, void* buf, size_t bytes) {
+    if (!h || !buf) return fail(PGX_ERR_INVALID, "NULL argument");
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(h->w.device));
+        world_save_state(h->w, buf, bytes);
+        return PGX_OK;
+    });
+}
+
+int pgx_world_load_state(pgx_world* h, const void* buf, size_t bytes) {
+    if (!h || !buf) return fail(PGX_ERR_INVALID, "NULL argument");
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(h->w.device));
+        world_load_state(h->w, buf, bytes);
+        return PGX_OK;
     });
 }
 
