@@ -289,6 +289,21 @@ int pgx_world_raycast(pgx_world* w, int n, const float* origins, const float* di
 int pgx_world_overlap(pgx_world* w, int n, const int32_t* types, const float* params, const float* positions,
                       const float* orientations, const int32_t* query_world, const int32_t* filters, int capacity,
                       int32_t* offsets, int32_t* colliders, int32_t* total, double* device_us);
+/* Scene::overlapWithMinTranslationalDistance (include/Physecs/Physecs.h:208,
+ * src/Physecs.cpp:653-688) for n query shapes at once, between steps. Queries as in
+ * pgx_world_overlap (sphere, capsule, box or convex mesh; a triangle-mesh query shape is
+ * PGX_ERR_UNSUPPORTED), no filter. Every collider whose broadphase bounds meet the query's
+ * bounds is tested with physecs::collision(collider, query); each manifold with points is a
+ * hit: the collider, the manifold's normal (from the collider toward the query) and its mtd,
+ * the largest dot(contact0 - contact1, normal) over its points and at least 0 (the
+ * reference's OverlapMtdHit {entity, colIndex, normal, mtd}). Query i's hits are
+ * [offsets[i], offsets[i+1]): colliders[k], normal_mtd[4k..4k+3] = normal xyz, mtd;
+ * ordered by collider index, then in collision()'s manifold order (the reference: its
+ * tree's order). When *total exceeds capacity, only *total is set and PGX_ERR_CAPACITY is
+ * returned. device_us (may be NULL): the query's device time. */
+int pgx_world_overlap_mtd(pgx_world* w, int n, const int32_t* types, const float* params, const float* positions,
+                          const float* orientations, const int32_t* query_world, int capacity, int32_t* offsets,
+                          int32_t* colliders, float* normal_mtd, int32_t* total, double* device_us);
 /* The last step's trigger events: Scene's onTriggerEnter / onTriggerExit listeners
  * (include/Physecs/Physecs.h:30-40,220-223; src/Physecs.cpp:199-206,538-552). A pair with a
  * trigger collider on either side (defaultContactFilter) makes no contact; when

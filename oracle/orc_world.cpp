@@ -1615,6 +1615,54 @@ int orc_world_overlap(const orc_world* w, int type, const float* prm, const floa
     return (int)hits.size();
 }
 
+// Scene::overlapWithMinTranslationalDistance (src/Physecs.cpp:653-688): every collider
+// whose bounds meet the query's bounds is tested with collision(collider, query); each
+// manifold with points gives {collider, normal, mtd}, mtd = the largest
+// dot(contact0 - contact1, normal) over its points, and at least 0. Listed by collider
+// index, then in collision()'s manifold order (the reference lists them in its tree's
+// order). A convex query's params[3] is one of the world's convex meshes. Writes at most
+// cap hits: out_col[i], out_nm[4i..] = normal xyz, mtd. Returns the hit count.
+int orc_world_overlap_mtd(const orc_world* w, int type, const float* prm, const float* pos, const float* q, int cap,
+                          int32_t* out_col, float* out_nm) {
+    Geom g{type, {prm[0], prm[1], prm[2], prm[3]}};
+    if (type == CONVEX_MESH) g.mesh = &w->convex.at((size_t)prm[3]);
+    if (type == TRIANGLE_MESH) return -1;   // not taken by the device boundary either
+    V3 p = v3(pos[0], pos[1], pos[2]);
+    Q qq = qmake(q[3], q[0], q[1], q[2]);
+    Bounds b = getBounds(p, qq, g);
+    std::vector<int> cands;
+    for (const BPEntry& e : w->bp) {
+        const Bounds& eb = e.bounds;
+        if (b.max.x < eb.min.x || b.min.x > eb.max.x || b.max.y < eb.min.y || b.min.y > eb.max.y ||
+            b.max.z < eb.min.z || b.min.z > eb.max.z)
+            continue;
+        cands.push_back(e.collider);
+    }
+    std::sort(cands.begin(), cands.end());
+    int n = 0;
+    for (int ci : cands) {
+        const Collider& c = w->cols[ci];
+        const Actor& a = w->actors[c.actor];
+        std::vector<Manifold> res;
+        if (!collision(a.pos + qrot(a.quat, c.lpos), qmul(a.quat, c.lquat), c.geom, p, qq, g, res)) continue;
+        for (const Manifold& m : res) {
+            if (!m.numPoints) continue;
+            float mtd = 0;
+            for (int i = 0; i < m.numPoints; i++) {
+                float dist = dot(m.points[i].p0 - m.points[i].p1, m.normal);
+                if (dist > mtd) mtd = dist;
+            }
+            if (n < cap) {
+                out_col[n] = ci;
+                out_nm[4 * n] = m.normal.x; out_nm[4 * n + 1] = m.normal.y; out_nm[4 * n + 2] = m.normal.z;
+                out_nm[4 * n + 3] = mtd;
+            }
+            ++n;
+        }
+    }
+    return n;
+}
+
 int orc_ray_aabb(const float* o, const float* d, const float* mn, const float* mx, float* t) {
     return rayAABB(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), v3(mn[0], mn[1], mn[2]), v3(mx[0], mx[1], mx[2]), *t) ? 1 : 0;
 }

@@ -30,7 +30,7 @@ EXPORTS = [
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
-    "pgx_world_get_trigger_events", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
+    "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -137,6 +137,8 @@ def lib():
         L.pgx_world_overlap.argtypes = [vp, ctypes.c_int, I32, F, F, F, I32, I32, ctypes.c_int, I32, I32, I32,
                                         ctypes.POINTER(ctypes.c_double)]
         L.pgx_world_get_trigger_events.argtypes = [vp, ctypes.c_int, I32, I32, I32, I32]
+        L.pgx_world_overlap_mtd.argtypes = [vp, ctypes.c_int, I32, F, F, F, I32, ctypes.c_int, I32, I32, F, I32,
+                                            ctypes.POINTER(ctypes.c_double)]
         L.pgx_world_state_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
         L.pgx_world_save_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
         L.pgx_world_load_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
@@ -354,6 +356,36 @@ class World:
             check(rc)
             break
         return [cols[off[i]:off[i + 1]].copy() for i in range(n)], us.value
+
+    def overlap_mtd(self, types, params, positions, orientations, world=None, capacity=None):
+        """pgx_world_overlap_mtd: Scene::overlapWithMinTranslationalDistance for every query.
+        Returns (per query: (collider indices, (k, 4) normal xyz + mtd); device microseconds)."""
+        t = np.ascontiguousarray(np.atleast_1d(types), np.int32)
+        n = len(t)
+        prm = np.zeros((n, 4), np.float32)
+        pa = np.asarray(params, np.float32).reshape(n, -1)
+        prm[:, : pa.shape[1]] = pa
+        pos = np.ascontiguousarray(positions, np.float32).reshape(n, 3)
+        q = np.ascontiguousarray(orientations, np.float32).reshape(n, 4)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        wd = None if world is None else np.ascontiguousarray(np.broadcast_to(np.asarray(world, np.int32), (n,)))
+        off = np.zeros(n + 1, np.int32)
+        total = ctypes.c_int32(0)
+        us = ctypes.c_double(0)
+        cap = 4 * n + 1024 if capacity is None else int(capacity)
+        while True:
+            cols = np.zeros(max(cap, 1), np.int32)
+            nm = np.zeros((max(cap, 1), 4), np.float32)
+            rc = self.L.pgx_world_overlap_mtd(self.h, n, t.ctypes.data_as(i32), _f(prm), _f(pos), _f(q),
+                                              None if wd is None else wd.ctypes.data_as(i32), cap,
+                                              off.ctypes.data_as(i32), cols.ctypes.data_as(i32), _f(nm),
+                                              ctypes.byref(total), ctypes.byref(us))
+            if rc == ERR_CAPACITY and capacity is None and total.value > cap:
+                cap = total.value
+                continue
+            check(rc)
+            break
+        return [(cols[off[i]:off[i + 1]].copy(), nm[off[i]:off[i + 1]].copy()) for i in range(n)], us.value
 
     def trigger_events(self, capacity=1024):
         """pgx_world_get_trigger_events: the last step's (enter, exit) events, each an

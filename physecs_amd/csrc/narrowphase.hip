@@ -1224,6 +1224,215 @@ __global__ __launch_bounds__(64) void k_mesh_emit(MeshArgs A, int32_t* tmWs, int
     }
 }
 
+// ------------------------------------------------------------------ MTD overlap queries
+// Scene::overlapWithMinTranslationalDistance (src/Physecs.cpp:653-688): collision(collider,
+// query) for every collider whose bounds meet the query's; each manifold with points gives
+// its normal and mtd. Results are appended with keys (query, collider, manifold index).
+constexpr int MTD_MBITS = 8;   // manifold index within a pair: at most TM_MAXCAND
+
+__device__ __forceinline__ float manifold_mtd(const Mf& m) {   // src/Physecs.cpp:663-670
+    float mtd = 0;
+    for (int i = 0; i < m.numPoints; i++) {
+        float dist = dot(sub(m.pts[i].p0, m.pts[i].p1), m.normal);
+        if (dist > mtd) mtd = dist;
+    }
+    return mtd;
+}
+
+__device__ __forceinline__ void mtd_put(const MtdArgs& A, int slot, int q, int c, int k, const Mf& m) {
+    if (slot >= A.cap) return;
+    A.keys[slot] = ((uint64_t)(uint32_t)q << (A.cbits + MTD_MBITS)) | ((uint64_t)(uint32_t)c << MTD_MBITS) |
+                   (uint64_t)k;
+    A.vals[slot] = slot;
+    A.res[slot] = f4(m.normal, manifold_mtd(m));
+}
+
+__device__ __forceinline__ void mtd_query(const MtdArgs& A, int i, int& t, float4& g, f3& pos, qf& q, int& wd) {
+    t = A.qType[i];
+    g = A.qPrm[i];
+    const float4 pw = A.qPos[i];
+    pos = xyz(pw);
+    wd = __float_as_int(pw.w);
+    q = q4(A.qQuat[i]);
+}
+
+__device__ __forceinline__ void mtd_collider_pose(const MtdArgs& A, int c, f3& cp, qf& cq) {
+    const int b = A.cbody[c];
+    const qf bq = q4(A.bquat[b]);
+    cp = add(xyz(A.bpos[b]), qrot(bq, xyz(A.clpos[c])));
+    cq = qmul(bq, q4(A.clquat[c]));
+}
+
+// pass 1, one lane per query: the walk of its world's LBVH and static list (as k_overlap);
+// pairs of primitives are solved here, pairs with a convex or triangle mesh are listed
+__global__ __launch_bounds__(64) void k_mtd_walk(MtdArgs A, ConvexView cv) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < A.n;
+    int t = PGX_SPHERE, wd = 0;
+    float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+    f3 pos = s3(0.f), mn = s3(0.f), mx = s3(0.f);
+    qf q = q4(make_float4(0.f, 0.f, 0.f, 1.f));
+    if (live) {
+        mtd_query(A, i, t, g, pos, q, wd);
+        shape_bounds(t, g, pos, q, cv.verts, cv.mesh, nullptr, nullptr, mn, mx);
+    }
+    int node = live ? A.worldRoot[wd] : -1;
+    const int stop = live && node != -1 ? A.worldStop[wd] : -1;
+    int visits = 0;
+    int st = live ? A.worldStaticStart[wd] : 0;
+    const int stEnd = live ? A.worldStaticStart[wd + 1] : 0;
+    while (true) {
+        int c = -1;
+        bool more = false;
+        if (node != -1 && node != stop && visits < 2 * A.nLeaves) {   // a walk visits at most 2n - 1 nodes
+            ++visits;
+            more = true;
+            if (node & LEAF_BIT) {
+                const int l = node & ~LEAF_BIT;
+                c = A.mvals[l];
+                node = __float_as_int(A.leafA[l].w);
+            } else {
+                const float4 na = A.nodeA[node], nb = A.nodeB[node];
+                node = boxes_meet(mn, mx, na, nb) ? __float_as_int(na.w) : __float_as_int(nb.w);
+            }
+        } else if (st < stEnd) {
+            more = true;
+            c = A.staticCols[st++];
+        }
+        bool hit = false, toCvx = false, toMesh = false;
+        Mf m;
+        m.numPoints = 0;
+        if (c >= 0 && boxes_meet(mn, mx, A.cbmin[c], A.cbmax[c])) {
+            const int ct = A.ctype[c];
+            if (ct == PGX_TRIANGLE_MESH) {
+                toMesh = true;
+            } else if (ct == PGX_CONVEX_MESH || t == PGX_CONVEX_MESH) {
+                toCvx = true;
+            } else {   // collision(collider, query), src/Physecs.cpp:659
+                f3 cp;
+                qf cq;
+                mtd_collider_pose(A, c, cp, cq);
+                hit = collide(ct, A.cparams[c], cp, cq, t, g, pos, q, m) && m.numPoints > 0;
+            }
+        }
+        if (__ballot(hit)) {
+            const int slot = wave_append(A.counts, hit);
+            if (hit) mtd_put(A, slot, i, c, 0, m);
+        }
+        if (__ballot(toCvx)) {
+            const int k = wave_append(A.counts + 1, toCvx);
+            if (toCvx && k < A.listCap) A.convexPairs[k] = make_int2(i, c);
+        }
+        if (__ballot(toMesh)) {
+            const int k = wave_append(A.counts + 2, toMesh);
+            if (toMesh && k < A.listCap) A.meshPairs[k] = make_int2(i, c);
+        }
+        if (!__ballot(more)) break;
+    }
+}
+
+// pass 2: the pairs with a convex mesh, a fixed grid of `lanes` threads; lane l owns EPA
+// workspace l
+__global__ __launch_bounds__(64) void k_mtd_convex(MtdArgs A, ConvexView cv, double* epaV, double* epaFN,
+                                                   int32_t* epaFI, int32_t* epaE, int lanes) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= lanes) return;
+    const int n = min(A.counts[1], A.listCap);
+    EpaWs ws;
+    ws.vd = epaV + lane; ws.fn = epaFN + lane; ws.fi = epaFI + lane; ws.ed = epaE + lane;
+    ws.L = (size_t)lanes;
+    ws.overflow = false;
+    for (int h = lane; h < n; h += lanes) {
+        const int2 qc = A.convexPairs[h];
+        int t, wd;
+        float4 g;
+        f3 pos, cp;
+        qf q, cq;
+        mtd_query(A, qc.x, t, g, pos, q, wd);
+        mtd_collider_pose(A, qc.y, cp, cq);
+        Mf m;
+        bool ovf = false;
+        const bool hit = collide_convex(A.ctype[qc.y], A.cparams[qc.y], cp, cq, t, g, pos, q, cv, ws, m, ovf);
+        if (ovf) atomicAdd(A.counts + 3, 1);
+        if (hit && m.numPoints > 0) mtd_put(A, atomicAdd(A.counts, 1), qc.x, qc.y, 0, m);
+    }
+}
+
+// the manifolds of one (query, mesh) pair: the mesh is the collider, geom0, so the pair is
+// solved as (query, mesh) and flipped (src/Collision.cpp:897-901)
+struct MtdOut {
+    const MtdArgs* A;
+    int q, c, n;
+    __device__ void emit(Mf& m, int) {
+        if (m.numPoints <= 0) return;
+        m.normal = neg(m.normal);
+        for (int i = 0; i < m.numPoints; ++i) { f3 t = m.pts[i].p0; m.pts[i].p0 = m.pts[i].p1; m.pts[i].p1 = t; }
+        mtd_put(*A, atomicAdd(A->counts, 1), q, c, n, m);
+        ++n;
+    }
+};
+
+// pass 3: the pairs with a triangle mesh, one lane per pair: the mesh BVH query, the
+// per-triangle tests into the lane's own block of TM_MAXCAND contact records, the emission
+// passes (the step's three mesh passes in one lane)
+__global__ __launch_bounds__(64) void k_mtd_mesh(MtdArgs A, ConvexView cv, TriMeshView tv, int32_t* tmWs,
+                                                 int wsLanes, MeshRecs recs, double* epaV, double* epaFN,
+                                                 int32_t* epaFI, int32_t* epaE, int epaLanes) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= A.recLanes) return;
+    const int n = min(A.counts[2], A.listCap);
+    MeshWs ws{tmWs + lane, (size_t)wsLanes};
+    EpaWs ews;
+    ews.vd = epaV + lane; ews.fn = epaFN + lane; ews.fi = epaFI + lane; ews.ed = epaE + lane;
+    ews.L = (size_t)epaLanes;
+    ews.overflow = false;
+    for (int h = lane; h < n; h += A.recLanes) {
+        const int2 qc = A.meshPairs[h];
+        int t, wd;
+        float4 g;
+        f3 pos, mpos, lpos;
+        qf q, mq, lor;
+        mtd_query(A, qc.x, t, g, pos, q, wd);
+        mtd_collider_pose(A, qc.y, mpos, mq);
+        SupConvex sc;
+        mesh_local(t, g, pos, q, mpos, mq, cv, lpos, lor, sc);
+        const MeshCtx mc{&tv, tv.mesh[(int)A.cparams[qc.y].w]};
+        const int nc = mesh_query(t, g, lpos, lor, cv, mc, ws);
+        if (nc < 0) { atomicAdd(A.counts + 3, 1); continue; }
+        const int base = lane * TM_MAXCAND;
+        int nct = 0;
+        for (int j = 0; j < nc; ++j) {
+            TCI ci;
+            bool epaErr = false;
+            ews.overflow = false;
+            const bool hit = mesh_test(t, g, lpos, lor, sc, mc, ws.cand(j), ews, ci, epaErr);
+            if (epaErr) atomicAdd(A.counts + 3, 1);
+            recs.store(base + j, hit, ci);
+            if (hit) ws.order(nct++) = base + j;
+        }
+        MtdOut out{&A, qc.x, qc.y, 0};
+        mesh_emit(t, g, lpos, lor, mpos, mq, mc, cv, ws, recs, nct, out);
+    }
+}
+
+void np_mtd_launch(World& w, const MtdArgs& A) {
+    const ConvexView cv{w.cvVerts.p, w.cvMesh.p, w.cvFace.p, w.cvFaceIdx.p, w.cvFaceN.p, w.cvFaceC.p};
+    hipLaunchKernelGGL(k_mtd_walk, dim3(div_up(A.n, 64)), dim3(64), 0, w.stream, A, cv);
+    PGX_LAUNCH_CHECK();
+    if (w.nConvex > 0) {
+        hipLaunchKernelGGL(k_mtd_convex, dim3(div_up(w.epaLanes, 64)), dim3(64), 0, w.stream, A, cv, w.epaV.p,
+                           w.epaFN.p, w.epaFI.p, w.epaE.p, w.epaLanes);
+        PGX_LAUNCH_CHECK();
+    }
+    if (w.nTriMesh > 0) {
+        const TriMeshView tv{w.tmVerts.p, w.tmTris.p, w.tmTriN.p, w.tmTriC.p, w.tmNodes.p, w.tmMesh.p};
+        const MeshRecs recs{w.tmRecs.p, (size_t)w.tmCandCap};
+        hipLaunchKernelGGL(k_mtd_mesh, dim3(div_up(A.recLanes, 64)), dim3(64), 0, w.stream, A, cv, tv, w.tmWs.p,
+                           w.meshLanes, recs, w.epaV.p, w.epaFN.p, w.epaFI.p, w.epaE.p, w.epaLanes);
+        PGX_LAUNCH_CHECK();
+    }
+}
+
 void np_overlap_launch(hipStream_t s, const OverlapArgs& A) {
     hipLaunchKernelGGL(k_overlap, dim3(div_up(A.n, 64)), dim3(64), 0, s, A);
     PGX_LAUNCH_CHECK();

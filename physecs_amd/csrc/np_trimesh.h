@@ -780,9 +780,10 @@ __device__ __forceinline__ bool is_voided(MeshWs& ws, int nv, int v) {
 }
 
 // the emission passes (:906-953) over the nct contact records ws.order(0 .. nct-1) lists
-// in candidate order
+// in candidate order; out.emit(manifold, triangle) takes each manifold in order
+template <class Out>
 __device__ void mesh_emit(int t, float4 g, f3 localPos, qf localOr, f3 pos1, qf or1, const MeshCtx& mc,
-                          const ConvexView& cv, MeshWs& ws, const MeshRecs& recs, int nct, MeshOut& out) {
+                          const ConvexView& cv, MeshWs& ws, const MeshRecs& recs, int nct, Out& out) {
     f3 he = mk3(g.x, g.y, g.z);
     MeshRef cm{&cv, make_int4(0, 0, 0, 0)};
     if (t == PGX_CONVEX_MESH) cm.info = cv.mesh[(int)g.w];

@@ -294,6 +294,10 @@ struct World {
     DBuf<int32_t> qType, qFilter, qCount;
     DBuf<uint64_t> qKeys;
     DBuf<int32_t> qVals;
+    // MTD queries: counters, results, the convex and triangle-mesh pair lists
+    DBuf<int32_t> qMtdN;
+    DBuf<float4> qRes;
+    DBuf<int2> qCvxPairs, qMeshPairs;
     std::vector<int32_t> worldListHost[2];   // worlds of at most WORLD_SMALL dynamic bodies, and the rest
     DBuf<int32_t> worldList[2];
     bool worldsContiguous = false;
@@ -337,6 +341,11 @@ void qr_raycast(World& w, int n, const float* origins, const float* dirs, const 
 int qr_overlap(World& w, int n, const int32_t* types, const float* params, const float* positions,
                const float* orientations, const int32_t* queryWorld, const int32_t* filters, int capacity,
                int32_t* offsets, int32_t* colliders, int32_t* total, double* deviceUs);
+// queries.hip: Scene::overlapWithMinTranslationalDistance for n query shapes; returns PGX_OK
+// or PGX_ERR_CAPACITY (only *total set) when the hits exceed `capacity`
+int qr_overlap_mtd(World& w, int n, const int32_t* types, const float* params, const float* positions,
+                   const float* orientations, const int32_t* queryWorld, int capacity, int32_t* offsets,
+                   int32_t* colliders, float* normalMtd, int32_t* total, double* deviceUs);
 
 // device view of the convex mesh tables
 struct ConvexView {
@@ -363,8 +372,28 @@ struct OverlapArgs {
     uint64_t* keys;                          // the first `cap` hits' keys
 };
 
+// what the MTD overlap query kernels read and write
+struct MtdArgs {
+    int n, nLeaves, cap, cbits, listCap, recLanes;
+    const float4 *qPrm, *qPos, *qQuat;      // per query: params (.w mesh), position (.w world bits), orientation
+    const int32_t* qType;
+    const int32_t *worldRoot, *worldStop, *worldStaticStart, *staticCols, *mvals;
+    const float4 *leafA, *nodeA, *nodeB;
+    const float4 *cbmin, *cbmax, *cparams, *clpos, *clquat, *bpos, *bquat;
+    const int32_t *cbody, *ctype;
+    int32_t* counts;      // [0] results, [1] convex pairs, [2] mesh pairs, [3] errors (device, zero before)
+    uint64_t* keys;       // (query, collider, manifold) keys of the first `cap` results
+    int32_t* vals;        // their result index (the sort's payload)
+    float4* res;          // normal xyz, mtd
+    int2* convexPairs;    // (query, collider) with a convex mesh on either side, listCap
+    int2* meshPairs;      // (query, triangle-mesh collider), listCap
+};
+
 // narrowphase.hip
 void np_run(World& w, int nPairs);
+// the MTD query's passes: the walk (primitive pairs solved, the others listed), the convex
+// pairs, the triangle-mesh pairs
+void np_mtd_launch(World& w, const MtdArgs& A);
 // the last step's trigger events as (collider of entity0, collider of entity1) pairs,
 // ascending; returns PGX_ERR_CAPACITY (counts set) when either list exceeds cap
 int np_trigger_events(World& w, int cap, int32_t* enter, int32_t* nEnter, int32_t* exit, int32_t* nExit);

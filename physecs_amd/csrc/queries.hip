@@ -242,6 +242,112 @@ void qr_raycast(World& w, int n, const float* origins, const float* dirs, const 
     }
 }
 
+// query shapes to the device: params, position (.w = world id bits), orientation; types
+static void upload_query_shapes(World& w, int n, const int32_t* types, const float* params, const float* positions,
+                                const float* orientations, const int32_t* queryWorld) {
+    std::vector<float4> hs(3 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        int32_t wd = queryWorld ? queryWorld[i] : 0;
+        float wbits;
+        std::memcpy(&wbits, &wd, sizeof wbits);
+        hs[i] = make_float4(params[4 * i], params[4 * i + 1], params[4 * i + 2], params[4 * i + 3]);
+        hs[n + i] = make_float4(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2], wbits);
+        hs[2 * (size_t)n + i] = make_float4(orientations[4 * i], orientations[4 * i + 1], orientations[4 * i + 2],
+                                            orientations[4 * i + 3]);
+    }
+    w.qShape.alloc(3 * (size_t)n); w.qType.alloc(n);
+    PGX_HIP(hipMemcpyAsync(w.qShape.p, hs.data(), sizeof(float4) * 3 * (size_t)n, hipMemcpyHostToDevice, w.stream));
+    PGX_HIP(hipMemcpyAsync(w.qType.p, types, sizeof(int32_t) * n, hipMemcpyHostToDevice, w.stream));
+    PGX_HIP(hipStreamSynchronize(w.stream));   // hs is released on return
+}
+
+int qr_overlap_mtd(World& w, int n, const int32_t* types, const float* params, const float* positions,
+                   const float* orientations, const int32_t* queryWorld, int capacity, int32_t* offsets,
+                   int32_t* colliders, float* normalMtd, int32_t* total, double* deviceUs) {
+    constexpr int MBITS = 8;   // manifold index within a pair (the kernels' MTD_MBITS)
+    hipStream_t s = w.stream;
+    int cbits = 1;
+    while ((1L << cbits) < (long)w.C) ++cbits;
+    int qbits = 1;
+    while ((1L << qbits) < (long)n) ++qbits;
+    if (qbits + cbits + MBITS > 64) throw PgxError{PGX_ERR_UNSUPPORTED, "too many queries and colliders for one call"};
+    upload_query_shapes(w, n, types, params, positions, orientations, queryWorld);
+    w.qMtdN.alloc(4);
+    w.qKeys.alloc(std::max(capacity, 1)); w.qVals.alloc(std::max(capacity, 1)); w.qRes.alloc(std::max(capacity, 1));
+    if (!w.qEv[0])
+        for (auto& e : w.qEv) PGX_HIP(hipEventCreate(&e));
+    MtdArgs A;
+    A.n = n; A.nLeaves = w.nDynCol; A.cap = capacity; A.cbits = cbits;
+    // triangle-mesh pairs: one lane each, with TM_MAXCAND contact records of its own
+    A.recLanes = std::max(1, std::min({w.meshLanes, w.epaLanes, w.tmCandCap / TM_MAXCAND}));
+    A.qPrm = w.qShape.p; A.qPos = w.qShape.p + n; A.qQuat = w.qShape.p + 2 * (size_t)n; A.qType = w.qType.p;
+    A.worldRoot = w.worldRoot.p; A.worldStop = w.worldStop.p;
+    A.worldStaticStart = w.worldStaticStart.p; A.staticCols = w.staticCols.p; A.mvals = w.mvals.p;
+    A.leafA = w.leafA.p; A.nodeA = w.nodeA.p; A.nodeB = w.nodeB.p;
+    A.cbmin = w.cbmin.p; A.cbmax = w.cbmax.p; A.cparams = w.cparams.p; A.clpos = w.clpos.p; A.clquat = w.clquat.p;
+    A.bpos = w.bpos.p; A.bquat = w.bquat.p; A.cbody = w.cbody.p; A.ctype = w.ctype.p;
+    A.counts = w.qMtdN.p; A.keys = w.qKeys.p; A.vals = w.qVals.p; A.res = w.qRes.p;
+    int cnt[4] = {0, 0, 0, 0};
+    int listCap = (int)std::max<size_t>(std::max<size_t>(w.qCvxPairs.n, 4096), 4 * (size_t)n);
+    float ms0 = 0.f;
+    for (int attempt = 0; attempt < 2; ++attempt) {   // the second with pair lists as long as the first needed
+        w.qCvxPairs.alloc(listCap); w.qMeshPairs.alloc(listCap);
+        A.listCap = listCap; A.convexPairs = w.qCvxPairs.p; A.meshPairs = w.qMeshPairs.p;
+        PGX_HIP(hipEventRecord(w.qEv[0], s));
+        fill_i32(s, w.qMtdN.p, 0, 4);
+        fill_i32(s, w.counters.p + 7, 0, 1);   // the refit's work counter
+        bp_build(w);
+        if (w.nDynCol == 0) fill_i32(s, w.worldRoot.p, -1, w.W);   // no tree: static lists only
+        np_mtd_launch(w, A);
+        PGX_HIP(hipEventRecord(w.qEv[1], s));
+        PGX_HIP(hipMemcpyAsync(cnt, w.qMtdN.p, sizeof cnt, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipStreamSynchronize(s));
+        PGX_HIP(hipEventElapsedTime(&ms0, w.qEv[0], w.qEv[1]));
+        if (std::max(cnt[1], cnt[2]) <= listCap) break;
+        if (attempt == 1) throw PgxError{PGX_ERR_HIP, "MTD query: pair lists overflowed twice"};
+        listCap = std::max(cnt[1], cnt[2]);
+    }
+    if (cnt[3]) throw PgxError{PGX_ERR_CAPACITY, "MTD query: EPA or triangle-candidate workspace overflow in " +
+                                                     std::to_string(cnt[3]) + " pairs"};
+    *total = cnt[0];
+    if (deviceUs) *deviceUs = 1000.0 * ms0;
+    if (cnt[0] > capacity) return PGX_ERR_CAPACITY;
+    const int m = cnt[0];
+    float ms1 = 0.f;
+    if (m > 1) {
+        PGX_HIP(hipEventRecord(w.qEv[2], s));
+        radix_sort_u64(s, w.qKeys.p, w.qVals.p, m, qbits + cbits + MBITS, w.sortT);
+        PGX_HIP(hipEventRecord(w.qEv[3], s));
+    }
+    std::vector<uint64_t> keys((size_t)m);
+    std::vector<int32_t> vals((size_t)m);
+    std::vector<float4> res((size_t)m);
+    if (m) {
+        PGX_HIP(hipMemcpyAsync(keys.data(), w.qKeys.p, 8 * (size_t)m, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(vals.data(), w.qVals.p, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(res.data(), w.qRes.p, 16 * (size_t)m, hipMemcpyDeviceToHost, s));
+    }
+    PGX_HIP(hipStreamSynchronize(s));
+    if (m > 1 && deviceUs) {
+        PGX_HIP(hipEventElapsedTime(&ms1, w.qEv[2], w.qEv[3]));
+        *deviceUs = 1000.0 * (ms0 + ms1);
+    }
+    const uint64_t cmask = (1ull << cbits) - 1;
+    int k = 0;
+    for (int i = 0; i < n; ++i) {
+        offsets[i] = k;
+        while (k < m && (int)(keys[k] >> (cbits + MBITS)) == i) {
+            colliders[k] = (int32_t)((keys[k] >> MBITS) & cmask);
+            const float4 r = res[vals[k]];
+            normalMtd[4 * k] = r.x; normalMtd[4 * k + 1] = r.y; normalMtd[4 * k + 2] = r.z; normalMtd[4 * k + 3] = r.w;
+            ++k;
+        }
+    }
+    offsets[n] = k;
+    if (k != m) throw PgxError{PGX_ERR_HIP, "MTD query: result keys out of order"};
+    return PGX_OK;
+}
+
 int qr_overlap(World& w, int n, const int32_t* types, const float* params, const float* positions,
                const float* orientations, const int32_t* queryWorld, const int32_t* filters, int capacity,
                int32_t* offsets, int32_t* colliders, int32_t* total, double* deviceUs) {

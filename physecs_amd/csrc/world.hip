@@ -1437,6 +1437,46 @@ int pgx_world_load_state(pgx_world* h, const void* buf, size_t bytes) {
     });
 }
 
+int pgx_world_overlap_mtd(pgx_world* h, int n, const int32_t* types, const float* params, const float* positions,
+                          const float* orientations, const int32_t* query_world, int capacity, int32_t* offsets,
+                          int32_t* colliders, float* normal_mtd, int32_t* total, double* device_us) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    if (n < 0 || capacity < 0) return fail(PGX_ERR_INVALID, "query count and capacity must be >= 0");
+    if (!offsets || !total) return fail(PGX_ERR_INVALID, "NULL offsets or total");
+    if (n > 0 && (!types || !params || !positions || !orientations))
+        return fail(PGX_ERR_INVALID, "NULL query array");
+    if (capacity > 0 && (!colliders || !normal_mtd)) return fail(PGX_ERR_INVALID, "NULL result arrays with capacity > 0");
+    World& w = h->w;
+    for (int i = 0; i < n; ++i) {
+        if (types[i] == PGX_TRIANGLE_MESH)
+            return fail(PGX_ERR_UNSUPPORTED, "query %d: triangle-mesh query shapes are not supported", i);
+        if (types[i] < PGX_SPHERE || types[i] > PGX_CONVEX_MESH)
+            return fail(PGX_ERR_INVALID, "query %d: shape type %d out of range", i, types[i]);
+        if (types[i] == PGX_CONVEX_MESH) {
+            const float m = params[4 * i + 3];
+            if (!(m >= 0.f && m < (float)w.nConvex) || m != (float)(int)m)
+                return fail(PGX_ERR_INVALID, "query %d: convex mesh %g is not one of the scene's %d", i, (double)m,
+                            w.nConvex);
+        }
+        if (query_world && (query_world[i] < 0 || query_world[i] >= w.W))
+            return fail(PGX_ERR_INVALID, "query %d: world %d out of range", i, query_world[i]);
+    }
+    if (n == 0) {
+        offsets[0] = 0;
+        *total = 0;
+        if (device_us) *device_us = 0.0;
+        return PGX_OK;
+    }
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(w.device));
+        int rc = qr_overlap_mtd(w, n, types, params, positions, orientations, query_world, capacity, offsets,
+                                colliders, normal_mtd, total, device_us);
+        if (rc == PGX_ERR_CAPACITY)
+            return fail(PGX_ERR_CAPACITY, "MTD overlap: %d hits exceed the capacity %d", *total, capacity);
+        return rc;
+    });
+}
+
 int pgx_world_get_exec(pgx_world* h, int* mode) {
     if (!h || !mode) return fail(PGX_ERR_INVALID, "NULL argument");
     *mode = h->w.execMode;

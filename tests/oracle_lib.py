@@ -256,6 +256,18 @@ class OracleWorld:
         assert n <= cap
         return out[:n].copy()
 
+    def overlap_mtd(self, t, prm, pos, q, cap=4096):
+        """Scene::overlapWithMinTranslationalDistance of one query shape: (collider indices,
+        (n, 4) normal xyz + mtd), by collider, then manifold order."""
+        fn = self.lib.orc_world_overlap_mtd
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int, F, F, F, ctypes.c_int, I32, F]
+        fn.restype = ctypes.c_int
+        cols = np.zeros(cap, np.int32)
+        nm = np.zeros((cap, 4), np.float32)
+        n = fn(self.h, int(t), fptr(_f32(prm, 4)), fptr(_f32(pos, 3)), fptr(_f32(q, 4)), cap, iptr(cols), fptr(nm))
+        assert 0 <= n <= cap
+        return cols[:n].copy(), nm[:n].copy()
+
     def trigger_events(self):
         """The last step's trigger (enter, exit) events: (n, 2) collider pairs, ascending."""
         fn = self.lib.orc_world_trigger_events
