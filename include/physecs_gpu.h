@@ -304,6 +304,29 @@ int pgx_world_overlap(pgx_world* w, int n, const int32_t* types, const float* pa
 int pgx_world_overlap_mtd(pgx_world* w, int n, const int32_t* types, const float* params, const float* positions,
                           const float* orientations, const int32_t* query_world, int capacity, int32_t* offsets,
                           int32_t* colliders, float* normal_mtd, int32_t* total, double* device_us);
+/* One character: the CharacterController's members (include/Physecs/CharacterController.h:
+ * 17-24) and its entity's pose. */
+typedef struct pgx_character {
+    float position[3];        /* the entity's TransformComponent::position, updated by a move */
+    float orientation[4];     /* its orientation (x, y, z, w) */
+    float height, radius;
+    float max_step_height;    /* kept for the reference's signature; see below */
+    float slope_limit;        /* cosine, the reference's default cos(45 degrees) */
+    int32_t num_iterations;   /* the reference's default 2 */
+    int32_t world;            /* the world the character moves in */
+} pgx_character;
+/* CharacterController::move (src/CharacterController.cpp:5-45) for n characters at once,
+ * between steps: position += translations[3i..], then num_iterations rounds of
+ * overlapWithMinTranslationalDistance with the character's capsule (half height
+ * height / 2, radius, centred height / 2 + radius above the position along its up axis),
+ * each hit pushing the position out: along up by mtd / (n . up) where n . up exceeds
+ * slope_limit, else along n by mtd, in hit order (collider index, then manifold order;
+ * the reference: its tree's order). flags[i] = CharacterCollisionFlags (DOWN 1, SIDES 2,
+ * UP 4). The reference's step-up branch (:13-23) casts its rays through the unfiltered
+ * raycastClosest, which never reports a hit, so it never changes the translation:
+ * max_step_height and delta_time have no effect, as there. */
+int pgx_world_character_move(pgx_world* w, int n, pgx_character* characters, const float* translations,
+                             float delta_time, uint8_t* flags);
 /* The last step's trigger events: Scene's onTriggerEnter / onTriggerExit listeners
  * (include/Physecs/Physecs.h:30-40,220-223; src/Physecs.cpp:199-206,538-552). A pair with a
  * trigger collider on either side (defaultContactFilter) makes no contact; when

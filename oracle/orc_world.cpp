@@ -1622,13 +1622,10 @@ int orc_world_overlap(const orc_world* w, int type, const float* prm, const floa
 // index, then in collision()'s manifold order (the reference lists them in its tree's
 // order). A convex query's params[3] is one of the world's convex meshes. Writes at most
 // cap hits: out_col[i], out_nm[4i..] = normal xyz, mtd. Returns the hit count.
-int orc_world_overlap_mtd(const orc_world* w, int type, const float* prm, const float* pos, const float* q, int cap,
-                          int32_t* out_col, float* out_nm) {
-    Geom g{type, {prm[0], prm[1], prm[2], prm[3]}};
-    if (type == CONVEX_MESH) g.mesh = &w->convex.at((size_t)prm[3]);
-    if (type == TRIANGLE_MESH) return -1;   // not taken by the device boundary either
-    V3 p = v3(pos[0], pos[1], pos[2]);
-    Q qq = qmake(q[3], q[0], q[1], q[2]);
+struct MtdHit { int col; V3 n; float mtd; };
+
+static std::vector<MtdHit> overlapMtd(const orc_world* w, V3 p, Q qq, const Geom& g) {
+    std::vector<MtdHit> out;
     Bounds b = getBounds(p, qq, g);
     std::vector<int> cands;
     for (const BPEntry& e : w->bp) {
@@ -1639,7 +1636,6 @@ int orc_world_overlap_mtd(const orc_world* w, int type, const float* prm, const 
         cands.push_back(e.collider);
     }
     std::sort(cands.begin(), cands.end());
-    int n = 0;
     for (int ci : cands) {
         const Collider& c = w->cols[ci];
         const Actor& a = w->actors[c.actor];
@@ -1652,15 +1648,49 @@ int orc_world_overlap_mtd(const orc_world* w, int type, const float* prm, const 
                 float dist = dot(m.points[i].p0 - m.points[i].p1, m.normal);
                 if (dist > mtd) mtd = dist;
             }
-            if (n < cap) {
-                out_col[n] = ci;
-                out_nm[4 * n] = m.normal.x; out_nm[4 * n + 1] = m.normal.y; out_nm[4 * n + 2] = m.normal.z;
-                out_nm[4 * n + 3] = mtd;
-            }
-            ++n;
+            out.push_back({ci, m.normal, mtd});
         }
     }
-    return n;
+    return out;
+}
+
+int orc_world_overlap_mtd(const orc_world* w, int type, const float* prm, const float* pos, const float* q, int cap,
+                          int32_t* out_col, float* out_nm) {
+    Geom g{type, {prm[0], prm[1], prm[2], prm[3]}};
+    if (type == CONVEX_MESH) g.mesh = &w->convex.at((size_t)prm[3]);
+    if (type == TRIANGLE_MESH) return -1;   // not taken by the device boundary either
+    const std::vector<MtdHit> hits = overlapMtd(w, v3(pos[0], pos[1], pos[2]), qmake(q[3], q[0], q[1], q[2]), g);
+    for (size_t k = 0; k < hits.size() && (int)k < cap; ++k) {
+        out_col[k] = hits[k].col;
+        out_nm[4 * k] = hits[k].n.x; out_nm[4 * k + 1] = hits[k].n.y; out_nm[4 * k + 2] = hits[k].n.z;
+        out_nm[4 * k + 3] = hits[k].mtd;
+    }
+    return (int)hits.size();
+}
+
+// CharacterController::move (src/CharacterController.cpp:5-45) of one character whose
+// entity stands at pos (updated) with orientation q. The step-up branch (:13-23) asks
+// raycastClosest's unfiltered overload, which reports no hit (its leaf test calls an
+// empty filter), so it never changes the translation. Returns CharacterCollisionFlags.
+int orc_character_move(const orc_world* w, float* pos, const float* q, float height, float radius,
+                       float slopeLimit, int numIterations, const float* translation) {
+    Q ori = qmake(q[3], q[0], q[1], q[2]);
+    V3 up = qrot(ori, v3(0, 1, 0));
+    V3 position = v3(pos[0], pos[1], pos[2]) + v3(translation[0], translation[1], translation[2]);
+    int flags = 0;
+    Geom geom{CAPSULE, {height * 0.5f, radius, 0, 0}};
+    for (int i = 0; i < numIterations; ++i) {
+        for (const MtdHit& h : overlapMtd(w, position + up * (height * 0.5f + radius), ori, geom)) {
+            float d = dot(h.n, up);
+            if (d > 0.001f) flags |= 1;          // DOWN
+            else if (d < -0.001f) flags |= 4;    // UP
+            else flags |= 2;                     // SIDES
+            if (d > slopeLimit) position = position + up * h.mtd / d;
+            else position = position + h.n * h.mtd;
+        }
+    }
+    pos[0] = position.x; pos[1] = position.y; pos[2] = position.z;
+    return flags;
 }
 
 int orc_ray_aabb(const float* o, const float* d, const float* mn, const float* mx, float* t) {

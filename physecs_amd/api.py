@@ -30,7 +30,7 @@ EXPORTS = [
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
-    "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
+    "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_character_move", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -89,6 +89,13 @@ class PgxTuning(ctypes.Structure):
                 ("color_rounds", ctypes.c_int), ("eager_rows_max", ctypes.c_int)]
 
 
+class PgxCharacter(ctypes.Structure):
+    """pgx_character (include/physecs_gpu.h)."""
+    _fields_ = [("position", ctypes.c_float * 3), ("orientation", ctypes.c_float * 4), ("height", ctypes.c_float),
+                ("radius", ctypes.c_float), ("max_step_height", ctypes.c_float), ("slope_limit", ctypes.c_float),
+                ("num_iterations", ctypes.c_int32), ("world", ctypes.c_int32)]
+
+
 class PgxError(RuntimeError):
     pass
 
@@ -139,6 +146,8 @@ def lib():
         L.pgx_world_get_trigger_events.argtypes = [vp, ctypes.c_int, I32, I32, I32, I32]
         L.pgx_world_overlap_mtd.argtypes = [vp, ctypes.c_int, I32, F, F, F, I32, ctypes.c_int, I32, I32, F, I32,
                                             ctypes.POINTER(ctypes.c_double)]
+        L.pgx_world_character_move.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PgxCharacter), F, ctypes.c_float,
+                                               ctypes.POINTER(ctypes.c_uint8)]
         L.pgx_world_state_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
         L.pgx_world_save_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
         L.pgx_world_load_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
@@ -386,6 +395,29 @@ class World:
             check(rc)
             break
         return [(cols[off[i]:off[i + 1]].copy(), nm[off[i]:off[i + 1]].copy()) for i in range(n)], us.value
+
+    def character_move(self, positions, orientations, translations, height, radius, slope_limit=np.cos(np.pi / 4),
+                       num_iterations=2, world=0, max_step_height=0.3, dt=1 / 60):
+        """pgx_world_character_move: CharacterController::move for n characters. Returns
+        (new positions (n, 3), CharacterCollisionFlags (n,))."""
+        p = np.asarray(positions, np.float32).reshape(-1, 3)
+        n = len(p)
+        q = np.broadcast_to(np.asarray(orientations, np.float32), (n, 4))
+        arr = (PgxCharacter * max(n, 1))()
+        bc = lambda v: np.broadcast_to(np.asarray(v), (n,))
+        h, r, sl, it, wd, ms = bc(height), bc(radius), bc(slope_limit), bc(num_iterations), bc(world), bc(max_step_height)
+        for i in range(n):
+            c = arr[i]
+            c.position[:] = [float(x) for x in p[i]]
+            c.orientation[:] = [float(x) for x in q[i]]
+            c.height, c.radius, c.slope_limit = float(h[i]), float(r[i]), float(sl[i])
+            c.max_step_height, c.num_iterations, c.world = float(ms[i]), int(it[i]), int(wd[i])
+        tr = np.ascontiguousarray(np.broadcast_to(np.asarray(translations, np.float32), (n, 3)))
+        flags = np.zeros(max(n, 1), np.uint8)
+        check(self.L.pgx_world_character_move(self.h, n, arr, _f(tr), float(dt),
+                                              flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        out = np.array([list(arr[i].position) for i in range(n)], np.float32).reshape(n, 3)
+        return out, flags[:n].copy()
 
     def trigger_events(self, capacity=1024):
         """pgx_world_get_trigger_events: the last step's (enter, exit) events, each an

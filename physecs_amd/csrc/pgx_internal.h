@@ -341,6 +341,8 @@ void qr_raycast(World& w, int n, const float* origins, const float* dirs, const 
 int qr_overlap(World& w, int n, const int32_t* types, const float* params, const float* positions,
                const float* orientations, const int32_t* queryWorld, const int32_t* filters, int capacity,
                int32_t* offsets, int32_t* colliders, int32_t* total, double* deviceUs);
+// queries.hip: CharacterController::move for n characters
+void qr_character_move(World& w, int n, pgx_character* chars, const float* translations, uint8_t* flags);
 // queries.hip: Scene::overlapWithMinTranslationalDistance for n query shapes; returns PGX_OK
 // or PGX_ERR_CAPACITY (only *total set) when the hits exceed `capacity`
 int qr_overlap_mtd(World& w, int n, const int32_t* types, const float* params, const float* positions,

@@ -348,6 +348,68 @@ int qr_overlap_mtd(World& w, int n, const int32_t* types, const float* params, c
     return PGX_OK;
 }
 
+// CharacterController::move (src/CharacterController.cpp:5-45) for every character: the
+// rounds of MTD queries run on the device for all characters still moving, and each
+// character's hits are applied on the host in order, as the reference's loop does.
+void qr_character_move(World& w, int n, pgx_character* chars, const float* translations, uint8_t* flags) {
+    std::vector<f3> pos((size_t)n), up((size_t)n);
+    int rounds = 0;
+    for (int i = 0; i < n; ++i) {
+        const pgx_character& c = chars[i];
+        const qf q{c.orientation[0], c.orientation[1], c.orientation[2], c.orientation[3]};
+        up[i] = qrot(q, mk3(0.f, 1.f, 0.f));
+        pos[i] = add(mk3(c.position[0], c.position[1], c.position[2]),
+                     mk3(translations[3 * i], translations[3 * i + 1], translations[3 * i + 2]));
+        flags[i] = 0;
+        rounds = std::max(rounds, (int)c.num_iterations);
+    }
+    std::vector<int32_t> idx, types, qworld, off, cols;
+    std::vector<float> prm, qpos, qori, nm;
+    int cap = 8 * n + 1024;
+    for (int r = 0; r < rounds; ++r) {
+        idx.clear(); types.clear(); qworld.clear(); prm.clear(); qpos.clear(); qori.clear();
+        for (int i = 0; i < n; ++i) {
+            const pgx_character& c = chars[i];
+            if (r >= c.num_iterations) continue;
+            idx.push_back(i);
+            types.push_back(PGX_CAPSULE);
+            qworld.push_back(c.world);
+            prm.insert(prm.end(), {c.height * 0.5f, c.radius, 0.f, 0.f});
+            const f3 centre = add(pos[i], mul(up[i], c.height * 0.5f + c.radius));
+            qpos.insert(qpos.end(), {centre.x, centre.y, centre.z});
+            qori.insert(qori.end(), {c.orientation[0], c.orientation[1], c.orientation[2], c.orientation[3]});
+        }
+        const int m = (int)idx.size();
+        if (m == 0) continue;
+        off.assign((size_t)m + 1, 0);
+        int32_t total = 0;
+        while (true) {
+            cols.resize((size_t)cap);
+            nm.resize(4 * (size_t)cap);
+            if (qr_overlap_mtd(w, m, types.data(), prm.data(), qpos.data(), qori.data(), qworld.data(), cap,
+                               off.data(), cols.data(), nm.data(), &total, nullptr) == PGX_OK)
+                break;
+            cap = total;   // PGX_ERR_CAPACITY: room for every hit
+        }
+        for (int j = 0; j < m; ++j) {
+            const int i = idx[j];
+            const float slope = chars[i].slope_limit;
+            for (int k = off[j]; k < off[j + 1]; ++k) {
+                const f3 nk = mk3(nm[4 * (size_t)k], nm[4 * (size_t)k + 1], nm[4 * (size_t)k + 2]);
+                const float mtd = nm[4 * (size_t)k + 3];
+                const float d = dot(nk, up[i]);
+                if (d > 0.001f) flags[i] |= 1;          // DOWN
+                else if (d < -0.001f) flags[i] |= 4;    // UP
+                else flags[i] |= 2;                     // SIDES
+                pos[i] = d > slope ? add(pos[i], divs(mul(up[i], mtd), d)) : add(pos[i], mul(nk, mtd));
+            }
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        chars[i].position[0] = pos[i].x; chars[i].position[1] = pos[i].y; chars[i].position[2] = pos[i].z;
+    }
+}
+
 int qr_overlap(World& w, int n, const int32_t* types, const float* params, const float* positions,
                const float* orientations, const int32_t* queryWorld, const int32_t* filters, int capacity,
                int32_t* offsets, int32_t* colliders, int32_t* total, double* deviceUs) {

@@ -1477,6 +1477,26 @@ int pgx_world_overlap_mtd(pgx_world* h, int n, const int32_t* types, const float
     });
 }
 
+int pgx_world_character_move(pgx_world* h, int n, pgx_character* characters, const float* translations,
+                             float delta_time, uint8_t* flags) {
+    (void)delta_time;   // only the reference's step-up branch reads it, and that branch never runs
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    if (n < 0) return fail(PGX_ERR_INVALID, "character count must be >= 0");
+    if (n > 0 && (!characters || !translations || !flags)) return fail(PGX_ERR_INVALID, "NULL argument");
+    World& w = h->w;
+    for (int i = 0; i < n; ++i) {
+        const pgx_character& c = characters[i];
+        if (c.world < 0 || c.world >= w.W) return fail(PGX_ERR_INVALID, "character %d: world %d out of range", i, c.world);
+        if (c.num_iterations < 0) return fail(PGX_ERR_INVALID, "character %d: num_iterations < 0", i);
+    }
+    if (n == 0) return PGX_OK;
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(w.device));
+        qr_character_move(w, n, characters, translations, flags);
+        return PGX_OK;
+    });
+}
+
 int pgx_world_get_exec(pgx_world* h, int* mode) {
     if (!h || !mode) return fail(PGX_ERR_INVALID, "NULL argument");
     *mode = h->w.execMode;

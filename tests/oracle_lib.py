@@ -268,6 +268,16 @@ class OracleWorld:
         assert 0 <= n <= cap
         return cols[:n].copy(), nm[:n].copy()
 
+    def character_move(self, pos, quat, translation, height, radius, slope_limit, num_iterations):
+        """CharacterController::move of one character: (new position, flags)."""
+        fn = self.lib.orc_character_move
+        fn.argtypes = [ctypes.c_void_p, F, F, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int, F]
+        fn.restype = ctypes.c_int
+        p = _f32(pos, 3)
+        flags = fn(self.h, fptr(p), fptr(_f32(quat, 4)), float(height), float(radius), float(slope_limit),
+                   int(num_iterations), fptr(_f32(translation, 3)))
+        return p, flags
+
     def trigger_events(self):
         """The last step's trigger (enter, exit) events: (n, 2) collider pairs, ascending."""
         fn = self.lib.orc_world_trigger_events
