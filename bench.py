@@ -1,6 +1,6 @@
 """Benchmark of the rigid-body step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain32k|pyramid|batched|convex|served|raycast|overlap]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain32k|pyramid|batched|convex|served|triggers|raycast|overlap|character|snapshot]
     python bench.py --copy-calibration      # a plain 1 GiB device copy, for scaling PMC byte counters
 
 A step is one Scene::simulate(1/60) with 4 substeps and 2 iterations over the whole scene,
@@ -88,6 +88,12 @@ def make_scene(name):
         return s, {"workload": "8192 convex hulls + 512 revolute capsule chains on a 131072-triangle fBm terrain",
                    "bodies": s.n_dyn, "joints": len(s.joints), "triangles": 131072, "substeps": 4,
                    "iterations": 2, "dt": 1 / 60}
+    if name == "triggers":
+        s = scenes.mixed_lattice(0, side=32)
+        s.arrays()["col_trigger"][5::10] = 1   # every tenth body collider a trigger (the container stays solid)
+        return s, {"workload": "mixed 32768 bodies, every tenth a trigger (overlap tests and enter/exit lists)",
+                   "bodies": 32768, "triggers": int(s.arrays()["col_trigger"].sum()), "substeps": 4, "iterations": 2,
+                   "dt": 1 / 60}
     raise SystemExit(f"unknown config {name}")
 
 
@@ -273,6 +279,84 @@ def run_overlap(args):
     print(json.dumps(line), flush=True)
 
 
+def run_character(args):
+    """CharacterController::move for 65,536 characters (pgx_world_character_move) on the
+    convex-terrain scene after 60 steps: each stands 5 cm into the terrain under it, yaw
+    random, and walks 0.1 m; 2 MTD rounds. value = characters
+    per second of wall time (the device MTD rounds plus the host pushes); the device time
+    of one round's MTD query of the same capsules is reported beside it."""
+    from physecs_amd import api, scenes
+    print("building the convex terrain scene", file=sys.stderr, flush=True)
+    sc = scenes.convex_terrain(0)
+    w = api.World(sc, substeps=4, iterations=2)
+    for k in range(60):
+        w.step(1 / 60)
+    w.sync()
+    rng = np.random.default_rng(0)
+    n = 1 << 16
+    # the terrain's surface under each character: the heightfield's two triangles per cell
+    _, _, hf = scenes.terrain_mesh(0, quads=256, amplitude=8.0)
+    xz = rng.uniform(-120, 120, (n, 2))
+    i0, j0 = np.floor(xz[:, 0] + 128).astype(int), np.floor(xz[:, 1] + 128).astype(int)
+    u, v = xz[:, 0] + 128 - i0, xz[:, 1] + 128 - j0
+    h00, h01, h10, h11 = hf[i0, j0], hf[i0, j0 + 1], hf[i0 + 1, j0], hf[i0 + 1, j0 + 1]
+    ground = np.where(u + v <= 1, h00 + (h10 - h00) * u + (h01 - h00) * v,
+                      h11 + (h01 - h11) * (1 - u) + (h10 - h11) * (1 - v))
+    pos = np.stack([xz[:, 0], ground - 0.05, xz[:, 1]], 1).astype(np.float32)   # 5 cm into the ground
+    yaw = rng.uniform(0, 2 * np.pi, n)
+    quat = np.zeros((n, 4), np.float32)
+    quat[:, 1], quat[:, 3] = np.sin(yaw / 2), np.cos(yaw / 2)
+    tr = np.stack([np.cos(yaw), np.zeros(n), np.sin(yaw)], 1).astype(np.float32) * 0.1
+    h, r = 1.6, 0.35
+    w.character_move(pos[:256], quat[:256], tr[:256], h, r)   # warm-up
+    wall, flags = [], None
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        _, flags = w.character_move(pos, quat, tr, h, r)
+        wall.append(time.perf_counter() - t0)
+    centre = (pos + tr + np.array([0, h * 0.5 + r, 0])).astype(np.float32)   # yaw-only: up is +y
+    caps = np.tile(np.array([h * 0.5, r, 0, 0], np.float32), (n, 1))
+    _, us = w.overlap_mtd(np.full(n, scenes.CAPSULE, np.int32), caps, centre, quat)
+    value = n / float(np.median(wall))
+    line = {"metric": "characters/s (CharacterController::move, 2 rounds, 1 GPU)", "value": value,
+            "unit": "characters/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+            "ms_per_step": 1000 * float(np.median(wall)), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene and characters)",
+            "config": {"workload": f"{n} capsule characters on the 131072-triangle terrain among 12288 bodies",
+                       "characters": n, "touching_ground": int((flags & 1).astype(bool).sum()),
+                       "mtd_round_device_ms": us / 1000.0, "parallelism": "single GPU"},
+            "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+
+
+def run_snapshot(args):
+    """pgx_world_save_state + pgx_world_load_state of the 32,768-body scene after 100 steps
+    (poses, velocities, bounds, contact cache). value = bytes per second through a save and a
+    load (device to pageable host memory and back, PCIe included)."""
+    from physecs_amd import api, scenes
+    sc = scenes.mixed_lattice(0, side=32)
+    w = api.World(sc, substeps=4, iterations=2)
+    for k in range(100):
+        w.step(1 / 60)
+    w.sync()
+    blob = w.save_state()
+    times = []
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        blob = w.save_state()
+        w.load_state(blob)
+        times.append(time.perf_counter() - t0)
+    value = 2 * len(blob) / float(np.median(times))
+    line = {"metric": "bytes/s (world state save + load, 32k bodies, 1 GPU)", "value": value, "unit": "B/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": 1000 * float(np.median(times)),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bytes",
+            "data": "synthetic (seeded scene)",
+            "config": {"workload": "save and load the state of 32768 mixed bodies after 100 steps",
+                       "state_bytes": len(blob), "parallelism": "single GPU"},
+            "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -309,6 +393,11 @@ def main():
         if args.steps == 590:
             args.steps = 10
         run_overlap(args)
+        return
+    if args.config in ("character", "snapshot"):
+        if args.steps == 590:
+            args.steps = 10
+        (run_character if args.config == "character" else run_snapshot)(args)
         return
     from physecs_amd import api
     print(f"building the {args.config} scene", file=sys.stderr, flush=True)

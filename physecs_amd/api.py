@@ -96,6 +96,12 @@ class PgxCharacter(ctypes.Structure):
                 ("num_iterations", ctypes.c_int32), ("world", ctypes.c_int32)]
 
 
+CHARACTER_DTYPE = np.dtype([("position", np.float32, 3), ("orientation", np.float32, 4), ("height", np.float32),
+                            ("radius", np.float32), ("max_step_height", np.float32), ("slope_limit", np.float32),
+                            ("num_iterations", np.int32), ("world", np.int32)])
+assert CHARACTER_DTYPE.itemsize == ctypes.sizeof(PgxCharacter)
+
+
 class PgxError(RuntimeError):
     pass
 
@@ -402,22 +408,16 @@ class World:
         (new positions (n, 3), CharacterCollisionFlags (n,))."""
         p = np.asarray(positions, np.float32).reshape(-1, 3)
         n = len(p)
-        q = np.broadcast_to(np.asarray(orientations, np.float32), (n, 4))
-        arr = (PgxCharacter * max(n, 1))()
-        bc = lambda v: np.broadcast_to(np.asarray(v), (n,))
-        h, r, sl, it, wd, ms = bc(height), bc(radius), bc(slope_limit), bc(num_iterations), bc(world), bc(max_step_height)
-        for i in range(n):
-            c = arr[i]
-            c.position[:] = [float(x) for x in p[i]]
-            c.orientation[:] = [float(x) for x in q[i]]
-            c.height, c.radius, c.slope_limit = float(h[i]), float(r[i]), float(sl[i])
-            c.max_step_height, c.num_iterations, c.world = float(ms[i]), int(it[i]), int(wd[i])
+        arr = np.zeros(max(n, 1), CHARACTER_DTYPE)   # the pgx_character layout
+        arr["position"][:n] = p
+        arr["orientation"][:n] = np.broadcast_to(np.asarray(orientations, np.float32), (n, 4))
+        arr["height"][:n], arr["radius"][:n], arr["slope_limit"][:n] = height, radius, slope_limit
+        arr["max_step_height"][:n], arr["num_iterations"][:n], arr["world"][:n] = max_step_height, num_iterations, world
         tr = np.ascontiguousarray(np.broadcast_to(np.asarray(translations, np.float32), (n, 3)))
         flags = np.zeros(max(n, 1), np.uint8)
-        check(self.L.pgx_world_character_move(self.h, n, arr, _f(tr), float(dt),
-                                              flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
-        out = np.array([list(arr[i].position) for i in range(n)], np.float32).reshape(n, 3)
-        return out, flags[:n].copy()
+        check(self.L.pgx_world_character_move(self.h, n, arr.ctypes.data_as(ctypes.POINTER(PgxCharacter)), _f(tr),
+                                              float(dt), flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return arr["position"][:n].copy(), flags[:n].copy()
 
     def trigger_events(self, capacity=1024):
         """pgx_world_get_trigger_events: the last step's (enter, exit) events, each an
