@@ -339,6 +339,20 @@ int pgx_world_character_move(pgx_world* w, int n, pgx_character* characters, con
  * pgx_world_step only: a caller that wants every event reads them after every step. */
 int pgx_world_get_trigger_events(pgx_world* w, int capacity, int32_t* enter, int32_t* n_enter, int32_t* exit,
                                  int32_t* n_exit);
+/* Host-side asset pipeline. physecs::setMassProps (include/Physecs/MassUtil.h:7-12,
+ * src/MassUtil.cpp:74-192) for one body of n colliders: types, params (4 each, as
+ * pgx_scene_desc.col_params), local poses (3 + 4 each), is_trigger (NULL: none; triggers
+ * add no mass). Convex colliders' meshes come from the tables given (as in
+ * pgx_scene_desc; NULL when there are none). Out: inv_mass, com[3], inv_inertia[9]
+ * (column-major), bit for bit as the reference computes them, quirks included: a box's
+ * volume is the product of its half extents, convex colliders' orientations are ignored,
+ * and the fan centre of a convex mesh sums its vertex buffer padded to a multiple of 4.
+ * Runs on the host; needs no device. */
+int pgx_mass_properties(int n, const int32_t* types, const float* params, const float* local_pos,
+                        const float* local_quat, const uint8_t* is_trigger, int n_convex,
+                        const int32_t* convex_vert_start, const float* convex_verts,
+                        const int32_t* convex_face_start, const int32_t* face_idx_start, const int32_t* face_idx,
+                        float mass, float* inv_mass, float* com, float* inv_inertia);
 /* Save and resume a world's state between steps (no counterpart in the reference, which
  * keeps its hidden state -- contactCache, GearJointData, triggerCache; include/Physecs/
  * Physecs.h:167-171, include/Physecs/Joints/GearJoint.h:7-14 -- only inside a live Scene).

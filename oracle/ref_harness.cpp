@@ -321,6 +321,30 @@ int ref_ray_geometry(const float* o, const float* d, int type, const float* prm,
     return intersectRayGeometry(V(o), V(d), V(pos), Qt(q), g, *t) ? 1 : 0;
 }
 
+// setMassProps (src/MassUtil.cpp:183-192) for a body of n colliders; convex params[3] is a
+// ref_set_convex_mesh index. out: invMass, com(3), invInertia (9, column-major)
+int ref_mass_props_multi(int n, const int32_t* types, const float* prm, const float* lpos, const float* lq,
+                         const uint8_t* trig, float mass, float* out) {
+    RigidBodyCollisionComponent col;
+    for (int i = 0; i < n; ++i) {
+        Collider c;
+        std::memset(&c, 0, sizeof(c));
+        c.position = V(lpos + 3 * i);
+        c.orientation = Qt(lq + 4 * i);
+        c.geometry = makeGeom(types[i], prm + 4 * i);
+        c.isTrigger = trig && trig[i];
+        c.enableSimulation = true;
+        col.colliders.push_back(c);
+    }
+    RigidBodyDynamicComponent d{};
+    setMassProps(d, col, mass);
+    out[0] = d.invMass;
+    out[1] = d.com.x; out[2] = d.com.y; out[3] = d.com.z;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) out[4 + 3 * i + j] = d.invInertiaTensor[i][j];
+    return 0;
+}
+
 // physecs::overlap (src/Overlap.cpp:203-235)
 int ref_overlap(int type0, const float* prm0, const float* pos0, const float* q0, int type1, const float* prm1,
                 const float* pos1, const float* q1) {

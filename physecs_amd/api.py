@@ -30,7 +30,7 @@ EXPORTS = [
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
-    "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_character_move", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
+    "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_character_move", "pgx_mass_properties", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -154,6 +154,8 @@ def lib():
                                             ctypes.POINTER(ctypes.c_double)]
         L.pgx_world_character_move.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PgxCharacter), F, ctypes.c_float,
                                                ctypes.POINTER(ctypes.c_uint8)]
+        L.pgx_mass_properties.argtypes = [ctypes.c_int, I32, F, F, F, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int,
+                                          I32, F, I32, I32, I32, ctypes.c_float, F, F, F]
         L.pgx_world_state_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
         L.pgx_world_save_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
         L.pgx_world_load_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
@@ -177,6 +179,40 @@ def check(rc):
     if rc != PGX_OK:
         msg = lib().pgx_last_error().decode(errors="replace")
         raise PgxError(f"{ERRORS.get(rc, rc)}: {msg}")
+
+
+def mass_properties(types, params, local_pos, local_quat, mass, is_trigger=None, meshes=()):
+    """pgx_mass_properties: physecs::setMassProps for one body of colliders (host code, no
+    device needed). meshes: convex meshes as (verts, faces, ...) tuples, indexed by a convex
+    collider's params[3]. Returns (inv_mass, com (3,), inv_inertia (9,) column-major)."""
+    t = np.ascontiguousarray(np.atleast_1d(types), np.int32)
+    n = len(t)
+    prm = np.zeros((n, 4), np.float32)
+    pa = np.asarray(params, np.float32).reshape(n, -1)
+    prm[:, : pa.shape[1]] = pa
+    lp = np.ascontiguousarray(local_pos, np.float32).reshape(n, 3)
+    lq = np.ascontiguousarray(local_quat, np.float32).reshape(n, 4)
+    tr = None if is_trigger is None else np.ascontiguousarray(is_trigger, np.uint8)
+    vs, fs, ist, idx, verts = [0], [0], [0], [], []
+    for m in meshes:
+        verts.append(np.asarray(m[0], np.float32).reshape(-1, 3))
+        vs.append(vs[-1] + len(verts[-1]))
+        fs.append(fs[-1] + len(m[1]))
+        for f in m[1]:
+            idx.extend(int(i) for i in f)
+            ist.append(ist[-1] + len(f))
+    i32 = lambda a: np.ascontiguousarray(a if len(a) else [0], np.int32)
+    cv = np.ascontiguousarray(np.concatenate(verts) if verts else np.zeros((1, 3)), np.float32)
+    vs, fs, ist, idx = i32(vs), i32(fs), i32(ist), i32(idx)
+    ip = lambda a: a.ctypes.data_as(I32)
+    im = np.zeros(1, np.float32)
+    com = np.zeros(3, np.float32)
+    inv = np.zeros(9, np.float32)
+    check(lib().pgx_mass_properties(n, ip(t), _f(prm), _f(lp), _f(lq),
+                                    None if tr is None else tr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                    len(meshes), ip(vs), _f(cv), ip(fs), ip(ist), ip(idx), float(mass), _f(im),
+                                    _f(com), _f(inv)))
+    return im[0], com, inv
 
 
 def device_count():

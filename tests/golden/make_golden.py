@@ -8,6 +8,7 @@ fixtures let the oracle be checked where the reference sources are absent.
 
 Every output is stored as raw float32 bits, so a check is bit for bit.
 """
+import ctypes
 import os
 import sys
 
@@ -441,6 +442,38 @@ def gen_overlap(ref, ids, n=6000):
                 hits=np.array(hits, np.int32))
 
 
+def massprops_case(rng, n_meshes):
+    """A body of 1-4 colliders of every type at random local poses, some of them triggers."""
+    k = int(rng.integers(1, 5))
+    types = rng.integers(0, 4 if n_meshes else 3, k).astype(np.int32)
+    prm = np.zeros((k, 4), np.float32)
+    for i, t in enumerate(types):
+        prm[i] = shape_params(int(t), rng) if t < 3 else [*rng.uniform(0.5, 1.5, 3), int(rng.integers(0, n_meshes))]
+    lpos = rng.uniform(-1, 1, (k, 3)).astype(np.float32)
+    lq = np.stack([rq(rng) for _ in range(k)])
+    trig = (rng.random(k) < 0.2).astype(np.uint8)
+    trig[0] = 0                                   # at least one collider with mass
+    return types, prm, lpos, lq, trig, np.float32(rng.uniform(0.5, 50.0))
+
+
+def gen_massprops(ref, ids, n=1500):
+    """setMassProps (src/MassUtil.cpp:183-192) for compound bodies."""
+    rng = np.random.default_rng(3033)
+    rows = []
+    for _ in range(n):
+        types, prm, lpos, lq, trig, mass = massprops_case(rng, len(ids))
+        live = prm.copy()
+        live[types == 3, 3] = [ids[int(m)] for m in prm[types == 3, 3]]
+        out = np.zeros(13, np.float32)
+        ref.ref_mass_props_multi(len(types), types.ctypes.data_as(ol.I32), ol.fptr(live), ol.fptr(lpos), ol.fptr(lq),
+                                 trig.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), mass, ol.fptr(out))
+        rows.append((types, prm, lpos, lq, trig, mass, out))
+    k = np.array([len(r[0]) for r in rows], np.int32)
+    cat = lambda i: np.concatenate([r[i] for r in rows])
+    return dict(count=k, types=cat(0), params=cat(1), lpos=cat(2), lquat=cat(3), trig=cat(4),
+                mass=np.array([r[5] for r in rows], np.float32), out=bits(np.stack([r[6] for r in rows])))
+
+
 def main():
     if not ol.ref_available():
         raise SystemExit("oracle/_ref is not built: run `make -C oracle ref` where the reference sources are")
@@ -452,6 +485,11 @@ def main():
     m = ol.register_trimesh(verts, tris)
     idx, nodes = ol.trimesh_layout("ref", m, len(tris))
     only = sys.argv[1:]
+    if only == ["massprops"]:   # the mass-property fixtures alone (the others are unchanged)
+        np.savez_compressed(os.path.join(HERE, "massprops.npz"),
+                            **{"mp_" + k: v for k, v in gen_massprops(ref, ids).items()}, **tables)
+        print("massprops.npz", os.path.getsize(os.path.join(HERE, "massprops.npz")), "bytes")
+        return
     if only == ["overlap"]:   # the overlap fixtures alone (the others are unchanged)
         np.savez_compressed(os.path.join(HERE, "overlap.npz"),
                             **{"ov_" + k: v for k, v in gen_overlap(ref, ids).items()}, **tables)
@@ -466,6 +504,8 @@ def main():
                         **{"ray_" + k: v for k, v in gen_raycast(ref, ids).items()}, **tables)
     np.savez_compressed(os.path.join(HERE, "overlap.npz"),
                         **{"ov_" + k: v for k, v in gen_overlap(ref, ids).items()}, **tables)
+    np.savez_compressed(os.path.join(HERE, "massprops.npz"),
+                        **{"mp_" + k: v for k, v in gen_massprops(ref, ids).items()}, **tables)
     np.savez_compressed(os.path.join(HERE, "collision.npz"),
                         **{"prim_" + k: v for k, v in gen_collision_primitive(ref).items()},
                         **{"cvx_" + k: v for k, v in gen_collision_convex(ref, ids).items()},

@@ -95,6 +95,8 @@ def ref():
         lib.ref_bounds.argtypes = [ctypes.c_int, F, F, F, F]
         lib.ref_sqr_dist_segment_aabb.argtypes = [F, F, ctypes.c_float, ctypes.c_float, F, F]
         lib.ref_mass_props.argtypes = [ctypes.c_int, F, F, F, ctypes.c_float, F]
+        lib.ref_mass_props_multi.argtypes = [ctypes.c_int, I32, F, F, F, ctypes.POINTER(ctypes.c_uint8),
+                                             ctypes.c_float, F]
         lib.ref_contact_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, F, F, F, F, ctypes.c_float,
                                           ctypes.c_int, ctypes.c_int, F]
         lib.ref_integrate_velocity.argtypes = [F, F, F, F, ctypes.c_float, ctypes.c_float, F]
