@@ -315,6 +315,12 @@ typedef struct pgx_character {
     int32_t num_iterations;   /* the reference's default 2 */
     int32_t world;            /* the world the character moves in */
 } pgx_character;
+/* registry.patch<TransformComponent> of n entities between steps: the Scene's on_update
+ * hook (src/Physecs.cpp:51-54,95) recomputes the bounds of every collider of a moved entity,
+ * static or dynamic, with the 0.01 margin (Scene::updateBounds, :79-90). actors[i] (descriptor
+ * order) takes positions pos[3i..] and orientations quat[4i..] (x, y, z, w). Velocities are
+ * unchanged. (pgx_world_write_state writes poses as the step does, in place, without a hook.) */
+int pgx_world_patch_transforms(pgx_world* w, int n, const int32_t* actors, const float* pos, const float* quat);
 /* CharacterController::move (src/CharacterController.cpp:5-45) for n characters at once,
  * between steps: position += translations[3i..], then num_iterations rounds of
  * overlapWithMinTranslationalDistance with the character's capsule (half height

@@ -1567,6 +1567,23 @@ bool overlapGeom(V3 p0, Q q0, const Geom& g0, V3 p1, Q q1, const Geom& g1) {
 
 extern "C" {
 
+// registry.patch<TransformComponent> of one entity: Scene::onRigidBodyMove ->
+// updateBounds (src/Physecs.cpp:51-54,79-90): the pose, then every collider's bounds of the
+// entity with the 0.01 margin, static or dynamic.
+void orc_world_patch_transform(orc_world* w, int actor, const float* pos, const float* q) {
+    Actor& a = w->actors.at((size_t)actor);
+    a.pos = v3(pos[0], pos[1], pos[2]);
+    a.quat = qmake(q[3], q[0], q[1], q[2]);
+    for (auto& e : w->bp) {
+        const Collider& c = w->cols[e.collider];
+        if (c.actor != actor) continue;
+        Bounds b = getBounds(a.pos + qrot(a.quat, c.lpos), qmul(a.quat, c.lquat), c.geom);
+        b.max = b.max + v3s(0.01f);
+        b.min = b.min - v3s(0.01f);
+        e.bounds = b;
+    }
+}
+
 // The last step's trigger events: which = 0 enter, 1 exit; pairs (collider of entity0,
 // collider of entity1) ascending. Returns the count (at most cap pairs written).
 int orc_world_trigger_events(const orc_world* w, int which, int cap, int32_t* out) {

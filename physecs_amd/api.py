@@ -30,7 +30,8 @@ EXPORTS = [
     "pgx_world_get_stats", "pgx_world_set_exec", "pgx_world_add_joints", "pgx_world_set_joint_params",
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
-    "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_character_move", "pgx_mass_properties", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
+    "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_character_move", "pgx_mass_properties",
+    "pgx_world_patch_transforms", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -156,6 +157,7 @@ def lib():
                                                ctypes.POINTER(ctypes.c_uint8)]
         L.pgx_mass_properties.argtypes = [ctypes.c_int, I32, F, F, F, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int,
                                           I32, F, I32, I32, I32, ctypes.c_float, F, F, F]
+        L.pgx_world_patch_transforms.argtypes = [vp, ctypes.c_int, I32, F, F]
         L.pgx_world_state_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
         L.pgx_world_save_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
         L.pgx_world_load_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
@@ -274,6 +276,14 @@ class World:
         c = lambda a: None if a is None else np.ascontiguousarray(a, np.float32)
         pos, quat, vel, ang = c(pos), c(quat), c(vel), c(ang)
         check(self.L.pgx_world_write_state(self.h, _f(pos), _f(quat), _f(vel), _f(ang)))
+
+    def patch_transforms(self, actors, pos, quat):
+        """pgx_world_patch_transforms: registry.patch<TransformComponent> of these actors
+        (their poses, and their colliders' bounds with the margin)."""
+        a = np.ascontiguousarray(np.atleast_1d(actors), np.int32)
+        p = np.ascontiguousarray(pos, np.float32).reshape(len(a), 3)
+        q = np.ascontiguousarray(quat, np.float32).reshape(len(a), 4)
+        check(self.L.pgx_world_patch_transforms(self.h, len(a), a.ctypes.data_as(I32), _f(p), _f(q)))
 
     def set_joint_params(self, index, **params):
         """The joint's setters: updates the data members named in params (JOINT_FIELDS) of joint

@@ -28,19 +28,20 @@ __global__ void k_bounds(int C, const int32_t* __restrict__ cbody, const int32_t
                          const float4* __restrict__ clpos, const float4* __restrict__ clquat,
                          const float4* __restrict__ bpos, const float4* __restrict__ bquat, int dynamicOnly,
                          const int32_t* __restrict__ cworld, const uint8_t* __restrict__ worldMask,
-                         const float4* __restrict__ cvVerts, const int4* __restrict__ cvMesh,
-                         const float4* __restrict__ tmVerts, const int4* __restrict__ tmMesh,
-                         float4* __restrict__ bmin, float4* __restrict__ bmax) {
+                         const uint8_t* __restrict__ colMask, const float4* __restrict__ cvVerts,
+                         const int4* __restrict__ cvMesh, const float4* __restrict__ tmVerts,
+                         const int4* __restrict__ tmMesh, float4* __restrict__ bmin, float4* __restrict__ bmax) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
     if (dynamicOnly && !(cflags[c] & CF_DYNAMIC)) return;
     if (worldMask && !worldMask[cworld[c]]) return;
+    if (colMask && !colMask[c]) return;   // the colliders of moved entities only, with the margin
     f3 pos;
     qf q;
     collider_world_pose(bpos, bquat, clpos, clquat, cbody[c], c, pos, q);
     f3 mn, mx;
     shape_bounds(ctype[c], cparams[c], pos, q, cvVerts, cvMesh, tmVerts, tmMesh, mn, mx);
-    if (dynamicOnly) {   // Bounds::addMargin(vec3(0.01f)), src/Bounds.cpp:23-26
+    if (dynamicOnly || colMask) {   // Bounds::addMargin(vec3(0.01f)), src/Bounds.cpp:23-26
         mx = add(mx, s3(0.01f));
         mn = sub(mn, s3(0.01f));
     }
@@ -48,12 +49,12 @@ __global__ void k_bounds(int C, const int32_t* __restrict__ cbody, const int32_t
     bmax[c] = f4(mx, 0.f);
 }
 
-void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin, const uint8_t* worldMask) {
+void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin, const uint8_t* worldMask, const uint8_t* colMask) {
     if (w.C == 0) return;
     hipLaunchKernelGGL(k_bounds, dim3(div_up(w.C, 256)), dim3(256), 0, w.stream, w.C, w.cbody.p, w.ctype.p,
                        w.cflags.p, w.cparams.p, w.clpos.p, w.clquat.p, w.bpos.p, w.bquat.p,
-                       dynamicOnlyWithMargin ? 1 : 0, w.cworld.p, worldMask, w.cvVerts.p, w.cvMesh.p, w.tmVerts.p, w.tmMesh.p, w.cbmin.p,
-                       w.cbmax.p);
+                       dynamicOnlyWithMargin ? 1 : 0, w.cworld.p, worldMask, colMask, w.cvVerts.p, w.cvMesh.p,
+                       w.tmVerts.p, w.tmMesh.p, w.cbmin.p, w.cbmax.p);
     PGX_LAUNCH_CHECK();
 }
 

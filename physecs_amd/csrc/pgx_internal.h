@@ -290,6 +290,7 @@ struct World {
     // overlap queries: Collider::data (descriptor order), query shapes, per-query hit
     // counts and offsets, the hit keys
     DBuf<int32_t> cdata;
+    DBuf<uint8_t> patchMask;             // colliders of the entities a transform patch moved
     DBuf<float4> qShape;
     DBuf<int32_t> qType, qFilter, qCount;
     DBuf<uint64_t> qKeys;
@@ -328,8 +329,10 @@ struct World {
 
 // broadphase.hip
 // bounds of every collider (creation: no margin) or, with the margin, of the dynamic
-// colliders; worldMask (device, one byte per world) limits the update to the marked worlds
-void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin, const uint8_t* worldMask = nullptr);
+// colliders; worldMask (device, one byte per world) limits the update to the marked worlds;
+// colMask (device, one byte per collider) updates the marked colliders, with the margin
+void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin, const uint8_t* worldMask = nullptr,
+                       const uint8_t* colMask = nullptr);
 int bp_find_pairs(World& w);          // returns pair count (host)
 void bp_build(World& w);              // the LBVH from the current bounds (no pairs)
 // queries.hip: Scene::raycastClosest for n rays (host arrays)

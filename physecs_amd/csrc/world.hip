@@ -1110,6 +1110,35 @@ int pgx_world_write_state(pgx_world* h, const float* pos, const float* quat, con
     });
 }
 
+int pgx_world_patch_transforms(pgx_world* h, int n, const int32_t* actors, const float* pos, const float* quat) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    if (n < 0 || (n > 0 && (!actors || !pos || !quat))) return fail(PGX_ERR_INVALID, "NULL argument");
+    World& w = h->w;
+    for (int i = 0; i < n; ++i)
+        if (actors[i] < 0 || actors[i] >= w.A) return fail(PGX_ERR_INVALID, "actor %d out of range", actors[i]);
+    if (n == 0) return PGX_OK;
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(w.device));
+        hipStream_t s = w.stream;
+        std::vector<float4> bp(w.A), bq(w.A);
+        PGX_HIP(hipMemcpyAsync(bp.data(), w.bpos.p, sizeof(float4) * w.A, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(bq.data(), w.bquat.p, sizeof(float4) * w.A, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipStreamSynchronize(s));
+        std::vector<uint8_t> moved((size_t)w.A, 0), mask((size_t)std::max(w.C, 1), 0);
+        for (int i = 0; i < n; ++i) {
+            const int a = actors[i], b = w.actorToBody[a];
+            bp[b] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.f);
+            bq[b] = make_float4(quat[4 * i], quat[4 * i + 1], quat[4 * i + 2], quat[4 * i + 3]);
+            moved[b] = 1;
+        }
+        for (int c = 0; c < w.C; ++c) mask[c] = moved[w.hostColBody[c]];
+        upload(w.bpos, bp, s); upload(w.bquat, bq, s); upload(w.patchMask, mask, s);
+        bp_compute_bounds(w, false, nullptr, w.patchMask.p);   // Scene::updateBounds (src/Physecs.cpp:79-90)
+        PGX_HIP(hipStreamSynchronize(s));
+        return PGX_OK;
+    });
+}
+
 int pgx_world_get_counts(pgx_world* h, pgx_counts* out) {
     if (!h || !out) return fail(PGX_ERR_INVALID, "NULL argument");
     out->pairs = h->w.lastPairs;

@@ -280,6 +280,12 @@ class OracleWorld:
                    int(num_iterations), fptr(_f32(translation, 3)))
         return p, flags
 
+    def patch_transform(self, actor, pos, quat):
+        """registry.patch<TransformComponent>: the pose and the entity's bounds (with margin)."""
+        fn = self.lib.orc_world_patch_transform
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int, F, F]
+        fn(self.h, int(actor), fptr(_f32(pos, 3)), fptr(_f32(quat, 4)))
+
     def trigger_events(self):
         """The last step's trigger (enter, exit) events: (n, 2) collider pairs, ascending."""
         fn = self.lib.orc_world_trigger_events
