@@ -1584,6 +1584,18 @@ void orc_world_patch_transform(orc_world* w, int actor, const float* pos, const 
     }
 }
 
+// Scene::setIsKinematic (src/Physecs.cpp:753-770): the flag, zero velocities when it turns
+// kinematic, and the broadphase entries' isDynamic.
+void orc_world_set_kinematic(orc_world* w, int actor, int kinematic) {
+    const Actor& a = w->actors.at((size_t)actor);
+    if (a.dyn < 0) return;
+    Dyn& d = w->dyn[a.dyn];
+    d.kinematic = kinematic != 0;
+    if (d.kinematic) { d.v = v3s(0); d.w = v3s(0); }
+    for (auto& e : w->bp)
+        if (w->cols[e.collider].actor == actor) e.isDynamic = !d.kinematic;
+}
+
 // The last step's trigger events: which = 0 enter, 1 exit; pairs (collider of entity0,
 // collider of entity1) ascending. Returns the count (at most cap pairs written).
 int orc_world_trigger_events(const orc_world* w, int which, int cap, int32_t* out) {

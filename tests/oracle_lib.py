@@ -286,6 +286,12 @@ class OracleWorld:
         fn.argtypes = [ctypes.c_void_p, ctypes.c_int, F, F]
         fn(self.h, int(actor), fptr(_f32(pos, 3)), fptr(_f32(quat, 4)))
 
+    def set_kinematic(self, actor, kinematic):
+        """Scene::setIsKinematic."""
+        fn = self.lib.orc_world_set_kinematic
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        fn(self.h, int(actor), int(bool(kinematic)))
+
     def trigger_events(self):
         """The last step's trigger (enter, exit) events: (n, 2) collider pairs, ascending."""
         fn = self.lib.orc_world_trigger_events

@@ -42,3 +42,33 @@ def test_patch_matches_oracle(api):
         assert np.array_equal(bits(g.bounds()), bits(o.bounds()))
 
     lockstep(sc, 30, check_every=5, on_step=on_step)
+
+
+def test_set_kinematic_by_rebuild_carries_state(api):
+    """Scene::setIsKinematic as a shim does it over this boundary: the descriptor's flag,
+    a new world of the same shape, the saved state loaded into it and the body's velocities
+    zeroed; then in lockstep with the oracle's setIsKinematic."""
+    sc = scenes.mixed_lattice(11, side=6)
+    arr = sc.arrays()
+    dyn_actors = np.nonzero(arr["dyn_of_actor"] >= 0)[0]
+    targets = dyn_actors[[3, 40, 100, 150]]
+    log("[kinematic] 20 lockstep steps, 4 bodies made kinematic by a rebuild with the state carried, 20 more")
+    g, o = lockstep(sc, 20, check_every=10)
+    blob = g.save_state()
+    d = arr["dyn_of_actor"][targets]
+    arr["kinematic"][d] = 1
+    g2 = api.World(sc, substeps=4, iterations=2)
+    g2.load_state(blob)
+    pos, quat, vel, ang = g2.state()
+    vel[d] = 0
+    ang[d] = 0
+    g2.write_state(pos, quat, vel, ang)
+    for a in targets:
+        o.set_kinematic(a, True)
+    for k in range(20):
+        g2.step(1 / 60)
+        o.set_order(g2.manifolds()[0])
+        o.step(1 / 60)
+    for x, y in zip(g2.state(), o.state()):
+        assert np.array_equal(bits(x), bits(y))
+    assert np.array_equal(bits(g2.state()[0][targets]), bits(pos[targets]))   # they stayed put
