@@ -393,7 +393,7 @@ class World:
         t = np.ascontiguousarray(np.atleast_1d(types), np.int32)
         n = len(t)
         prm = np.zeros((n, 4), np.float32)
-        pa = np.asarray(params, np.float32).reshape(n, -1)
+        pa = np.asarray(params, np.float32).reshape(n, -1) if n else np.zeros((0, 4), np.float32)
         prm[:, : pa.shape[1]] = pa
         pos = np.ascontiguousarray(positions, np.float32).reshape(n, 3)
         q = np.ascontiguousarray(orientations, np.float32).reshape(n, 4)
@@ -424,7 +424,7 @@ class World:
         t = np.ascontiguousarray(np.atleast_1d(types), np.int32)
         n = len(t)
         prm = np.zeros((n, 4), np.float32)
-        pa = np.asarray(params, np.float32).reshape(n, -1)
+        pa = np.asarray(params, np.float32).reshape(n, -1) if n else np.zeros((0, 4), np.float32)
         prm[:, : pa.shape[1]] = pa
         pos = np.ascontiguousarray(positions, np.float32).reshape(n, 3)
         q = np.ascontiguousarray(orientations, np.float32).reshape(n, 4)
