@@ -948,20 +948,43 @@ __device__ bool overlap_shapes(int t0, float4 g0, f3 p0, qf q0, int t1, float4 g
     }
 }
 
+// the primitive pairs of physecs::overlap (sphere, capsule, box on both sides), no GJK
+__device__ bool overlap_prims(int t0, float4 g0, f3 p0, qf q0, int t1, float4 g1, f3 p1, qf q1) {
+    const f3 he0 = mk3(g0.x, g0.y, g0.z), he1 = mk3(g1.x, g1.y, g1.z);
+    if (t0 == PGX_SPHERE) {
+        if (t1 == PGX_SPHERE) return ov_sphere_sphere(p0, g0.x, p1, g1.x);
+        if (t1 == PGX_CAPSULE) return ov_sphere_capsule(p0, g0.x, p1, q1, g1.x, g1.y);
+        return ov_sphere_box(p0, g0.x, p1, q1, he1);
+    }
+    if (t0 == PGX_CAPSULE) {
+        if (t1 == PGX_SPHERE) return ov_sphere_capsule(p1, g1.x, p0, q0, g0.x, g0.y);
+        if (t1 == PGX_CAPSULE) return ov_capsule_capsule(p0, q0, g0.x, g0.y, p1, q1, g1.x, g1.y);
+        return ov_capsule_box(p0, q0, g0.x, g0.y, p1, q1, he1);
+    }
+    if (t1 == PGX_SPHERE) return ov_sphere_box(p1, g1.x, p0, q0, he0);
+    if (t1 == PGX_CAPSULE) return ov_capsule_box(p1, q1, g1.x, g1.y, p0, q0, he0);
+    return ov_box_box(p0, q0, he0, p1, q1, he1);
+}
+
 // intersects (src/BoundsUtil.cpp:87-92): inclusive on every axis
 __device__ __forceinline__ bool boxes_meet(f3 amn, f3 amx, float4 bmn, float4 bmx) {
     return !(amx.x < bmn.x || amn.x > bmx.x || amx.y < bmn.y || amn.y > bmx.y || amx.z < bmn.z || amn.z > bmx.z);
 }
 
-// one collider of the query's world: the reference's leaf test (bounds, filter, shape)
-__device__ __forceinline__ bool overlap_collider(const OverlapArgs& A, int c, int t, float4 g, f3 pos, qf q, f3 mn,
-                                                 f3 mx, int filter) {
-    if (!boxes_meet(mn, mx, A.cbmin[c], A.cbmax[c])) return false;
-    if (filter && !(A.cdata[c] & filter)) return false;
+// one collider of the query's world: the reference's leaf test (bounds, filter, shape).
+// 0: no overlap; 1: overlap; 2: a pair with a convex mesh, left to k_overlap_convex (the
+// walk keeps no GJK state, so it needs far fewer registers)
+__device__ __forceinline__ int overlap_collider(const OverlapArgs& A, int c, int t, float4 g, f3 pos, qf q, f3 mn,
+                                                f3 mx, int filter) {
+    if (!boxes_meet(mn, mx, A.cbmin[c], A.cbmax[c])) return 0;
+    if (filter && !(A.cdata[c] & filter)) return 0;
+    const int ct = A.ctype[c];
+    if (ct == PGX_TRIANGLE_MESH) return 0;   // overlaps nothing (src/Overlap.cpp:203-235)
+    if (ct == PGX_CONVEX_MESH || t == PGX_CONVEX_MESH) return 2;
     const int b = A.cbody[c];
     const qf bq = q4(A.bquat[b]);
     const f3 cp = add(xyz(A.bpos[b]), qrot(bq, xyz(A.clpos[c])));
-    return overlap_shapes(t, g, pos, q, A.ctype[c], A.cparams[c], cp, qmul(bq, q4(A.clquat[c])), A.cv);
+    return overlap_prims(t, g, pos, q, ct, A.cparams[c], cp, qmul(bq, q4(A.clquat[c]))) ? 1 : 0;
 }
 
 // One query per thread: its bounds, the walk of its world's LBVH (boxes tested inclusive,
@@ -1013,10 +1036,15 @@ __global__ __launch_bounds__(64) void k_overlap(OverlapArgs A) {
             more = true;
             c = A.staticCols[st++];
         }
-        const bool hit = c >= 0 && overlap_collider(A, c, t, g, pos, q, mn, mx, filter);
+        const int r = c >= 0 ? overlap_collider(A, c, t, g, pos, q, mn, mx, filter) : 0;
+        const bool hit = r == 1, toCvx = r == 2;
         if (__ballot(hit)) {
             const int slot = wave_append(A.total, hit);
             if (hit && slot < A.cap) A.keys[slot] = ((uint64_t)(uint32_t)i << A.cbits) | (uint32_t)c;
+        }
+        if (__ballot(toCvx)) {
+            const int k = wave_append(A.nCvx, toCvx);
+            if (toCvx && k < A.listCap) A.cvxPairs[k] = make_int2(i, c);
         }
         hits += hit;
         if (!__ballot(more)) break;
@@ -1433,9 +1461,36 @@ void np_mtd_launch(World& w, const MtdArgs& A) {
     }
 }
 
-void np_overlap_launch(hipStream_t s, const OverlapArgs& A) {
+// the overlap queries' pairs with a convex mesh (GJK), a fixed grid of `lanes` threads;
+// each hit appends its key and adds to its query's count
+__global__ __launch_bounds__(64) void k_overlap_convex(OverlapArgs A, int lanes) {
+    const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= lanes) return;
+    const int n = min(*A.nCvx, A.listCap);
+    for (int h = lane; h < n; h += lanes) {
+        const int2 qc = A.cvxPairs[h];
+        const int i = qc.x, c = qc.y;
+        const float4 pw = A.qPos[i];
+        const int b = A.cbody[c];
+        const qf bq = q4(A.bquat[b]);
+        const f3 cp = add(xyz(A.bpos[b]), qrot(bq, xyz(A.clpos[c])));
+        if (!overlap_shapes(A.qType[i], A.qPrm[i], xyz(pw), q4(A.qQuat[i]), A.ctype[c], A.cparams[c], cp,
+                            qmul(bq, q4(A.clquat[c])), A.cv))
+            continue;
+        const int slot = atomicAdd(A.total, 1);
+        if (slot < A.cap) A.keys[slot] = ((uint64_t)(uint32_t)i << A.cbits) | (uint32_t)c;
+        atomicAdd(A.count + i, 1);
+    }
+}
+
+void np_overlap_launch(hipStream_t s, const OverlapArgs& A, bool convex) {
     hipLaunchKernelGGL(k_overlap, dim3(div_up(A.n, 64)), dim3(64), 0, s, A);
     PGX_LAUNCH_CHECK();
+    if (convex) {
+        constexpr int lanes = 16384;
+        hipLaunchKernelGGL(k_overlap_convex, dim3(lanes / 64), dim3(64), 0, s, A, lanes);
+        PGX_LAUNCH_CHECK();
+    }
 }
 
 void np_run(World& w, int nPairs) {

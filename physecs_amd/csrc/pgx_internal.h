@@ -375,6 +375,9 @@ struct OverlapArgs {
     int32_t* count;                          // per query: its hits
     int32_t* total;                          // all hits (device counter, zero before the launch)
     uint64_t* keys;                          // the first `cap` hits' keys
+    int32_t* nCvx;                           // pairs with a convex mesh (device counter, zero before)
+    int2* cvxPairs;                          // those pairs (query, collider), listCap of them
+    int listCap;
 };
 
 // what the MTD overlap query kernels read and write
@@ -402,7 +405,9 @@ void np_mtd_launch(World& w, const MtdArgs& A);
 // the last step's trigger events as (collider of entity0, collider of entity1) pairs,
 // ascending; returns PGX_ERR_CAPACITY (counts set) when either list exceeds cap
 int np_trigger_events(World& w, int cap, int32_t* enter, int32_t* nEnter, int32_t* exit, int32_t* nExit);
-void np_overlap_launch(hipStream_t s, const OverlapArgs& A);
+// the overlap query kernel, then (convex: the scene or a query has convex meshes) its
+// convex pairs
+void np_overlap_launch(hipStream_t s, const OverlapArgs& A, bool convex);
 int upload_triangle_meshes(World& w, const pgx_scene_desc* sc, hipStream_t s);   // returns the BVH depth
 // device-layout arrays of a scene descriptor (world.hip)
 struct SceneArrays {

@@ -430,18 +430,13 @@ int qr_overlap(World& w, int n, const int32_t* types, const float* params, const
     }
     std::vector<int32_t> hf(filters ? filters : nullptr, filters ? filters + n : nullptr);
     if (!filters) hf.assign((size_t)n, 0);
-    w.qShape.alloc(3 * (size_t)n); w.qType.alloc(n); w.qFilter.alloc(n); w.qCount.alloc((size_t)n + 1);
+    w.qShape.alloc(3 * (size_t)n); w.qType.alloc(n); w.qFilter.alloc(n); w.qCount.alloc((size_t)n + 2);
     w.qKeys.alloc(std::max(capacity, 1)); w.qVals.alloc(std::max(capacity, 1));
     PGX_HIP(hipMemcpyAsync(w.qShape.p, hs.data(), sizeof(float4) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
     PGX_HIP(hipMemcpyAsync(w.qType.p, types, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
     PGX_HIP(hipMemcpyAsync(w.qFilter.p, hf.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
     if (!w.qEv[0])
         for (auto& e : w.qEv) PGX_HIP(hipEventCreate(&e));
-    PGX_HIP(hipEventRecord(w.qEv[0], s));
-    fill_i32(s, w.counters.p + 7, 0, 1);   // the refit's work counter
-    fill_i32(s, w.qCount.p + n, 0, 1);     // the hit counter
-    bp_build(w);
-    if (w.nDynCol == 0) fill_i32(s, w.worldRoot.p, -1, w.W);   // no tree: static lists only
     OverlapArgs A;
     A.n = n; A.nLeaves = w.nDynCol; A.cap = capacity; A.cbits = cbits;
     A.qPrm = w.qShape.p; A.qPos = w.qShape.p + n; A.qQuat = w.qShape.p + 2 * (size_t)n;
@@ -452,14 +447,27 @@ int qr_overlap(World& w, int n, const int32_t* types, const float* params, const
     A.cbmin = w.cbmin.p; A.cbmax = w.cbmax.p; A.cparams = w.cparams.p; A.clpos = w.clpos.p; A.clquat = w.clquat.p;
     A.bpos = w.bpos.p; A.bquat = w.bquat.p; A.cbody = w.cbody.p; A.ctype = w.ctype.p; A.cdata = w.cdata.p;
     A.cv = ConvexView{w.cvVerts.p, w.cvMesh.p, w.cvFace.p, w.cvFaceIdx.p, w.cvFaceN.p, w.cvFaceC.p};
-    A.count = w.qCount.p; A.total = w.qCount.p + n; A.keys = w.qKeys.p;
-    if (n > 0) np_overlap_launch(s, A);
-    PGX_HIP(hipEventRecord(w.qEv[1], s));
-    std::vector<int32_t> cnt((size_t)n + 1);
-    PGX_HIP(hipMemcpyAsync(cnt.data(), w.qCount.p, sizeof(int32_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
-    PGX_HIP(hipStreamSynchronize(s));
+    A.count = w.qCount.p; A.total = w.qCount.p + n; A.keys = w.qKeys.p; A.nCvx = w.qCount.p + n + 1;
+    std::vector<int32_t> cnt((size_t)n + 2);
+    int listCap = (int)std::max<size_t>(std::max<size_t>(w.qCvxPairs.n, 4096), (size_t)n);
     float ms0 = 0.f;
-    PGX_HIP(hipEventElapsedTime(&ms0, w.qEv[0], w.qEv[1]));
+    for (int attempt = 0; attempt < 2; ++attempt) {   // the second with a convex-pair list as long as needed
+        w.qCvxPairs.alloc(listCap);
+        A.cvxPairs = w.qCvxPairs.p; A.listCap = listCap;
+        PGX_HIP(hipEventRecord(w.qEv[0], s));
+        fill_i32(s, w.counters.p + 7, 0, 1);   // the refit's work counter
+        fill_i32(s, w.qCount.p + n, 0, 2);     // the hit and convex-pair counters
+        bp_build(w);
+        if (w.nDynCol == 0) fill_i32(s, w.worldRoot.p, -1, w.W);   // no tree: static lists only
+        np_overlap_launch(s, A, w.nConvex > 0);
+        PGX_HIP(hipEventRecord(w.qEv[1], s));
+        PGX_HIP(hipMemcpyAsync(cnt.data(), w.qCount.p, sizeof(int32_t) * ((size_t)n + 2), hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipStreamSynchronize(s));
+        PGX_HIP(hipEventElapsedTime(&ms0, w.qEv[0], w.qEv[1]));
+        if (cnt[n + 1] <= listCap) break;
+        if (attempt == 1) throw PgxError{PGX_ERR_HIP, "overlap: the convex-pair list overflowed twice"};
+        listCap = cnt[n + 1];
+    }
     long sum = 0;
     for (int i = 0; i < n; ++i) {
         offsets[i] = (int32_t)sum;
