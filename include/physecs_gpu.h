@@ -359,6 +359,15 @@ int pgx_mass_properties(int n, const int32_t* types, const float* params, const 
                         const int32_t* convex_vert_start, const float* convex_verts,
                         const int32_t* convex_face_start, const int32_t* face_idx_start, const int32_t* face_idx,
                         float mass, float* inv_mass, float* com, float* inv_inertia);
+/* Host-side asset pipeline: ConvexMesh data (include/Physecs/ConvexMesh.h:7-34) for the
+ * convex hull of n >= 4 points (3 floats each; the reference has no hull builder and takes
+ * vertices and faces as given). Out: the hull's vertices (those input points, ascending
+ * input order; 3 * n floats of room), faces as polygons of coplanar hull triangles,
+ * clockwise seen from outside: face f is face_idx[face_start[f] .. face_start[f+1]) (room:
+ * 2n + 1 and 6n ints), outward unit normals and vertex-mean centroids (3 floats per face,
+ * room for 2n faces). PGX_ERR_INVALID for fewer than 4 points or a flat cloud. Host code. */
+int pgx_convex_hull(const float* points, int n, float* verts, int* n_verts, int* face_start, int* face_idx,
+                    float* normals, float* centroids, int* n_faces);
 /* Save and resume a world's state between steps (no counterpart in the reference, which
  * keeps its hidden state -- contactCache, GearJointData, triggerCache; include/Physecs/
  * Physecs.h:167-171, include/Physecs/Joints/GearJoint.h:7-14 -- only inside a live Scene).

@@ -31,7 +31,7 @@ EXPORTS = [
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
     "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_character_move", "pgx_mass_properties",
-    "pgx_world_patch_transforms", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
+    "pgx_world_patch_transforms", "pgx_convex_hull", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -158,6 +158,7 @@ def lib():
         L.pgx_mass_properties.argtypes = [ctypes.c_int, I32, F, F, F, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int,
                                           I32, F, I32, I32, I32, ctypes.c_float, F, F, F]
         L.pgx_world_patch_transforms.argtypes = [vp, ctypes.c_int, I32, F, F]
+        L.pgx_convex_hull.argtypes = [F, ctypes.c_int, F, I32, I32, I32, F, F, I32]
         L.pgx_world_state_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
         L.pgx_world_save_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
         L.pgx_world_load_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
@@ -215,6 +216,23 @@ def mass_properties(types, params, local_pos, local_quat, mass, is_trigger=None,
                                     len(meshes), ip(vs), _f(cv), ip(fs), ip(ist), ip(idx), float(mass), _f(im),
                                     _f(com), _f(inv)))
     return im[0], com, inv
+
+
+def convex_hull(points):
+    """pgx_convex_hull: (verts (k, 3), faces [lists of vertex indices], normals, centroids)
+    of the convex hull of points (host code, no device needed)."""
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    n = len(p)
+    verts = np.zeros((max(n, 1), 3), np.float32)
+    fs = np.zeros(2 * n + 1, np.int32)
+    fi = np.zeros(max(6 * n, 1), np.int32)
+    nrm = np.zeros((max(2 * n, 1), 3), np.float32)
+    cen = np.zeros((max(2 * n, 1), 3), np.float32)
+    nv, nf = ctypes.c_int32(0), ctypes.c_int32(0)
+    check(lib().pgx_convex_hull(_f(p), n, _f(verts), ctypes.byref(nv), fs.ctypes.data_as(I32), fi.ctypes.data_as(I32),
+                                _f(nrm), _f(cen), ctypes.byref(nf)))
+    faces = [list(fi[fs[f]:fs[f + 1]]) for f in range(nf.value)]
+    return verts[: nv.value].copy(), faces, nrm[: nf.value].copy(), cen[: nf.value].copy()
 
 
 def device_count():

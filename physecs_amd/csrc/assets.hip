@@ -4,6 +4,8 @@
 // library is built with -ffp-contract=off. The volume terms keep the reference's
 // double-precision detour: glm::pow(float, int) resolves to std::pow(double, double).
 #include <cmath>
+#include <map>
+#include <numeric>
 #include "pgx_internal.h"
 
 namespace {
@@ -206,5 +208,167 @@ extern "C" int pgx_mass_properties(int n, const int32_t* types, const float* par
     for (int k = 0; k < 3; ++k) {
         inv_inertia[3 * k] = cols[k].x; inv_inertia[3 * k + 1] = cols[k].y; inv_inertia[3 * k + 2] = cols[k].z;
     }
+    return PGX_OK;
+}
+
+// ------------------------------------------------------------------ convex hulls
+// ConvexMesh data (include/Physecs/ConvexMesh.h:7-34) for the convex hull of a point cloud;
+// the reference builds none and takes vertices and faces as given. Incremental hull in
+// double precision: a tetrahedron of extreme points, then every other point, farthest from
+// the hull first, replaces the triangles it sees by a fan to their horizon. Coplanar
+// neighbouring triangles are merged into polygons.
+namespace {
+struct D3 { double x, y, z; };
+D3 dsub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+D3 dcross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+double dlen(D3 a) { return std::sqrt(ddot(a, a)); }
+
+struct Tri {
+    int v[3];
+    D3 n;          // outward unit normal
+    double d;      // plane offset: dot(n, x) = d
+    bool alive;
+};
+
+Tri make_tri(const std::vector<D3>& P, int a, int b, int c, D3 inside) {
+    Tri t{{a, b, c}, {0, 0, 0}, 0, true};
+    D3 n = dcross(dsub(P[b], P[a]), dsub(P[c], P[a]));
+    double l = dlen(n);
+    n = {n.x / l, n.y / l, n.z / l};
+    if (ddot(n, dsub(inside, P[a])) > 0) {   // keep the normal outward: swap the winding
+        std::swap(t.v[1], t.v[2]);
+        n = {-n.x, -n.y, -n.z};
+    }
+    t.n = n;
+    t.d = ddot(n, P[a]);
+    return t;
+}
+}  // namespace
+
+extern "C" int pgx_convex_hull(const float* points, int n, float* verts, int* n_verts, int* face_start,
+                               int* face_idx, float* normals, float* centroids, int* n_faces) {
+    if (n < 4 || !points || !verts || !n_verts || !face_start || !face_idx || !normals || !centroids || !n_faces)
+        return PGX_ERR_INVALID;
+    std::vector<D3> P((size_t)n);
+    for (int i = 0; i < n; ++i) P[i] = {points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+    double ext = 0;
+    for (int i = 0; i < n; ++i) ext = std::max({ext, std::fabs(P[i].x), std::fabs(P[i].y), std::fabs(P[i].z)});
+    const double eps = 1e-10 * std::max(ext, 1e-30);
+    // the initial tetrahedron: extreme x, farthest from their line, farthest from their plane
+    int i0 = 0, i1 = 0;
+    for (int i = 1; i < n; ++i) {
+        if (P[i].x < P[i0].x) i0 = i;
+        if (P[i].x > P[i1].x) i1 = i;
+    }
+    if (i0 == i1) return PGX_ERR_INVALID;
+    int i2 = -1, i3 = -1;
+    double best = eps;
+    for (int i = 0; i < n; ++i) {
+        double d = dlen(dcross(dsub(P[i], P[i0]), dsub(P[i1], P[i0])));
+        if (d > best) { best = d; i2 = i; }
+    }
+    if (i2 < 0) return PGX_ERR_INVALID;   // collinear
+    const D3 pn = dcross(dsub(P[i1], P[i0]), dsub(P[i2], P[i0]));
+    best = eps * dlen(pn);
+    for (int i = 0; i < n; ++i) {
+        double d = std::fabs(ddot(pn, dsub(P[i], P[i0])));
+        if (d > best) { best = d; i3 = i; }
+    }
+    if (i3 < 0) return PGX_ERR_INVALID;   // coplanar
+    const D3 inside = {(P[i0].x + P[i1].x + P[i2].x + P[i3].x) / 4, (P[i0].y + P[i1].y + P[i2].y + P[i3].y) / 4,
+                       (P[i0].z + P[i1].z + P[i2].z + P[i3].z) / 4};
+    std::vector<Tri> T = {make_tri(P, i0, i1, i2, inside), make_tri(P, i0, i1, i3, inside),
+                          make_tri(P, i0, i2, i3, inside), make_tri(P, i1, i2, i3, inside)};
+    // the other points, farthest from the tetrahedron's centre first
+    std::vector<int> order;
+    for (int i = 0; i < n; ++i)
+        if (i != i0 && i != i1 && i != i2 && i != i3) order.push_back(i);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return ddot(dsub(P[a], inside), dsub(P[a], inside)) > ddot(dsub(P[b], inside), dsub(P[b], inside));
+    });
+    for (int p : order) {
+        std::vector<int> vis;
+        for (int t = 0; t < (int)T.size(); ++t)
+            if (T[t].alive && ddot(T[t].n, P[p]) - T[t].d > eps) vis.push_back(t);
+        if (vis.empty()) continue;   // inside the hull
+        // horizon: directed edges of visible triangles whose reverse is on no visible one
+        std::map<std::pair<int, int>, int> edges;
+        for (int t : vis)
+            for (int k = 0; k < 3; ++k) edges[{T[t].v[k], T[t].v[(k + 1) % 3]}] = t;
+        for (int t : vis) T[t].alive = false;
+        for (const auto& e : edges) {
+            if (edges.count({e.first.second, e.first.first})) continue;
+            Tri nt{{e.first.first, e.first.second, p}, {0, 0, 0}, 0, true};
+            D3 nn = dcross(dsub(P[nt.v[1]], P[nt.v[0]]), dsub(P[nt.v[2]], P[nt.v[0]]));
+            double l = dlen(nn);
+            if (l <= eps * eps) continue;
+            nt.n = {nn.x / l, nn.y / l, nn.z / l};
+            nt.d = ddot(nt.n, P[nt.v[0]]);
+            T.push_back(nt);
+        }
+    }
+    // merge coplanar triangles that share an edge (union-find over the live triangles)
+    std::vector<int> live;
+    for (int t = 0; t < (int)T.size(); ++t)
+        if (T[t].alive) live.push_back(t);
+    std::vector<int> parent(T.size());
+    std::iota(parent.begin(), parent.end(), 0);
+    auto find = [&](int x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    std::map<std::pair<int, int>, int> owner;
+    for (int t : live)
+        for (int k = 0; k < 3; ++k) owner[{T[t].v[k], T[t].v[(k + 1) % 3]}] = t;
+    for (int t : live)
+        for (int k = 0; k < 3; ++k) {
+            auto it = owner.find({T[t].v[(k + 1) % 3], T[t].v[k]});
+            if (it == owner.end()) continue;
+            const Tri& u = T[it->second];
+            if (dlen(dsub(T[t].n, u.n)) < 1e-6 && std::fabs(T[t].d - u.d) < 1e-6 * std::max(ext, 1e-30))
+                parent[find(t)] = find(it->second);
+        }
+    // hull vertices in ascending input order
+    std::vector<int> used((size_t)n, 0), remap((size_t)n, -1);
+    for (int t : live)
+        for (int k = 0; k < 3; ++k) used[T[t].v[k]] = 1;
+    int nv = 0;
+    for (int i = 0; i < n; ++i)
+        if (used[i]) {
+            remap[i] = nv;
+            verts[3 * nv] = points[3 * i]; verts[3 * nv + 1] = points[3 * i + 1]; verts[3 * nv + 2] = points[3 * i + 2];
+            ++nv;
+        }
+    // one polygon per group: its vertices ordered clockwise seen from outside
+    std::map<int, std::vector<int>> groups;
+    for (int t : live) groups[find(t)].push_back(t);
+    int nf = 0, ni = 0;
+    face_start[0] = 0;
+    for (const auto& g : groups) {
+        const Tri& t0 = T[g.first];
+        std::vector<int> vs;
+        for (int t : g.second)
+            for (int k = 0; k < 3; ++k)
+                if (std::find(vs.begin(), vs.end(), T[t].v[k]) == vs.end()) vs.push_back(T[t].v[k]);
+        D3 c = {0, 0, 0};
+        for (int v : vs) c = {c.x + P[v].x, c.y + P[v].y, c.z + P[v].z};
+        c = {c.x / vs.size(), c.y / vs.size(), c.z / vs.size()};
+        D3 u = dsub(P[vs[0]], c);
+        const double ul = dlen(u);
+        u = {u.x / ul, u.y / ul, u.z / ul};
+        const D3 w = dcross(t0.n, u);
+        std::vector<std::pair<double, int>> ang;
+        for (int v : vs) ang.push_back({std::atan2(ddot(dsub(P[v], c), w), ddot(dsub(P[v], c), u)), v});
+        std::sort(ang.begin(), ang.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+        for (const auto& a : ang) face_idx[ni++] = remap[a.second];
+        face_start[++nf] = ni;
+        normals[3 * (nf - 1)] = (float)t0.n.x; normals[3 * (nf - 1) + 1] = (float)t0.n.y;
+        normals[3 * (nf - 1) + 2] = (float)t0.n.z;
+        centroids[3 * (nf - 1)] = (float)c.x; centroids[3 * (nf - 1) + 1] = (float)c.y;
+        centroids[3 * (nf - 1) + 2] = (float)c.z;
+    }
+    *n_verts = nv;
+    *n_faces = nf;
     return PGX_OK;
 }
