@@ -1,6 +1,6 @@
 """Benchmark of the rigid-body step on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain32k|pyramid|batched|convex|served|triggers|raycast|overlap|character|snapshot]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config plain32k|pyramid|batched|convex|served|triggers|raycast|overlap|character|snapshot|meshcache]
     python bench.py --copy-calibration      # a plain 1 GiB device copy, for scaling PMC byte counters
 
 A step is one Scene::simulate(1/60) with 4 substeps and 2 iterations over the whole scene,
@@ -329,6 +329,39 @@ def run_character(args):
     print(json.dumps(line), flush=True)
 
 
+def run_meshcache(args):
+    """World creation for the convex-terrain scene (131,072-triangle terrain, 12,288 bodies,
+    3,584 joints) with the TriangleMesh build done at creation and loaded from
+    pgx_trimesh_cache bytes. value = creation time with the cache, ms (median)."""
+    from physecs_amd import api, scenes
+    sc = scenes.convex_terrain(0)
+    api.World(sc, substeps=4, iterations=2).sync()   # warm-up: device, library, scene arrays
+    t0 = time.perf_counter()
+    caches = [api.trimesh_cache(v, t) for v, t in sc.trimesh]
+    build_ms = 1000 * (time.perf_counter() - t0)
+    times = {}
+    for mode in ("build", "cached"):
+        sc.tri_bvh_cache = caches if mode == "cached" else None
+        ts = []
+        for k in range(args.steps):
+            t0 = time.perf_counter()
+            w = api.World(sc, substeps=4, iterations=2)
+            w.sync()
+            ts.append(time.perf_counter() - t0)
+            del w
+        times[mode] = 1000 * float(np.median(ts))
+    line = {"metric": "world creation ms (convex terrain, TriangleMesh build cached)", "value": times["cached"],
+            "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": times["cached"],
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded scene)",
+            "config": {"workload": "create the world of 12288 bodies on a 131072-triangle terrain",
+                       "creation_ms_building": times["build"], "creation_ms_cached": times["cached"],
+                       "cache_build_ms": build_ms, "cache_bytes": sum(len(c) for c in caches),
+                       "parallelism": "single GPU"},
+            "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+
+
 def run_snapshot(args):
     """pgx_world_save_state + pgx_world_load_state of the 32,768-body scene after 100 steps
     (poses, velocities, bounds, contact cache). value = bytes per second through a save and a
@@ -394,10 +427,10 @@ def main():
             args.steps = 10
         run_overlap(args)
         return
-    if args.config in ("character", "snapshot"):
+    if args.config in ("character", "snapshot", "meshcache"):
         if args.steps == 590:
-            args.steps = 10
-        (run_character if args.config == "character" else run_snapshot)(args)
+            args.steps = 10 if args.config != "meshcache" else 5
+        {"character": run_character, "snapshot": run_snapshot, "meshcache": run_meshcache}[args.config](args)
         return
     from physecs_amd import api
     print(f"building the {args.config} scene", file=sys.stderr, flush=True)

@@ -123,6 +123,10 @@ typedef struct pgx_scene_desc {
     const float* tri_verts;            /* 3 x vertices */
     const int32_t* tri_start;          /* n_trimesh + 1 */
     const int32_t* tri_idx;            /* 3 x triangles */
+    /* Optional (NULL, or NULL entries: build at creation): per triangle mesh, the bytes
+     * pgx_trimesh_cache wrote for that same mesh; creation then loads its triangle order
+     * and BVH instead of building them. A cache of another mesh is rejected. */
+    const void* const* tri_bvh_cache;
 } pgx_scene_desc;
 
 /* Built-in joint types (include/Physecs/Joints/*.h) */
@@ -368,6 +372,13 @@ int pgx_mass_properties(int n, const int32_t* types, const float* params, const 
  * room for 2n faces). PGX_ERR_INVALID for fewer than 4 points or a flat cloud. Host code. */
 int pgx_convex_hull(const float* points, int n, float* verts, int* n_verts, int* face_start, int* face_idx,
                     float* normals, float* centroids, int* n_faces);
+/* Host-side asset pipeline: the TriangleMesh constructor's build (src/TriangleMesh.cpp:7-164:
+ * triangle order, normals, centroids, SAH BVH) of one mesh as bytes, for
+ * pgx_scene_desc.tri_bvh_cache, so worlds with this mesh skip the build at creation.
+ * *bytes = the cache's size; with out NULL only the size is computed; PGX_ERR_CAPACITY when
+ * cap < *bytes. The bytes hold a hash of the vertices and indices they were built from. */
+int pgx_trimesh_cache(const float* verts, int n_verts, const int32_t* tris, int n_tris, void* out, size_t cap,
+                      size_t* bytes);
 /* Save and resume a world's state between steps (no counterpart in the reference, which
  * keeps its hidden state -- contactCache, GearJointData, triggerCache; include/Physecs/
  * Physecs.h:167-171, include/Physecs/Joints/GearJoint.h:7-14 -- only inside a live Scene).

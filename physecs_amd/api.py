@@ -31,7 +31,7 @@ EXPORTS = [
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
     "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_character_move", "pgx_mass_properties",
-    "pgx_world_patch_transforms", "pgx_convex_hull", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
+    "pgx_world_patch_transforms", "pgx_convex_hull", "pgx_trimesh_cache", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -54,6 +54,7 @@ class PgxSceneDesc(ctypes.Structure):
         ("n_convex", ctypes.c_int), ("convex_vert_start", I32), ("convex_verts", F), ("convex_face_start", I32),
         ("face_idx_start", I32), ("face_idx", I32), ("face_normal", F), ("face_centroid", F),
         ("n_trimesh", ctypes.c_int), ("tri_vert_start", I32), ("tri_verts", F), ("tri_start", I32), ("tri_idx", I32),
+        ("tri_bvh_cache", ctypes.POINTER(ctypes.c_void_p)),
     ]
 
 
@@ -159,6 +160,8 @@ def lib():
                                           I32, F, I32, I32, I32, ctypes.c_float, F, F, F]
         L.pgx_world_patch_transforms.argtypes = [vp, ctypes.c_int, I32, F, F]
         L.pgx_convex_hull.argtypes = [F, ctypes.c_int, F, I32, I32, I32, F, F, I32]
+        L.pgx_trimesh_cache.argtypes = [F, ctypes.c_int, I32, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                        ctypes.POINTER(ctypes.c_size_t)]
         L.pgx_world_state_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
         L.pgx_world_save_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
         L.pgx_world_load_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
@@ -233,6 +236,18 @@ def convex_hull(points):
                                 _f(nrm), _f(cen), ctypes.byref(nf)))
     faces = [list(fi[fs[f]:fs[f + 1]]) for f in range(nf.value)]
     return verts[: nv.value].copy(), faces, nrm[: nf.value].copy(), cen[: nf.value].copy()
+
+
+def trimesh_cache(verts, tris):
+    """pgx_trimesh_cache: the TriangleMesh build of one mesh as bytes (host code), for
+    Scene.tri_bvh_cache."""
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    n = ctypes.c_size_t(0)
+    check(lib().pgx_trimesh_cache(_f(v), len(v), t.ctypes.data_as(I32), len(t), None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value)
+    check(lib().pgx_trimesh_cache(_f(v), len(v), t.ctypes.data_as(I32), len(t), buf, n.value, ctypes.byref(n)))
+    return buf.raw
 
 
 def device_count():

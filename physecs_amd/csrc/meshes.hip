@@ -102,6 +102,95 @@ struct Builder {
 
 // Builds every mesh of the descriptor and uploads vertices, triangles (post-build order),
 // triangle normals / centroids and BVH nodes. Returns the deepest BVH path (stack bound).
+// One mesh as the constructor leaves it: triangles in post-build order with their normals
+// and centroids, the BVH nodes ({min, triCount bits}, {max, index bits}), the tree depth.
+struct BuiltMesh {
+    std::vector<int4> tris;
+    std::vector<float4> triN, triC, nodes;
+    int depth = 0;
+};
+
+BuiltMesh build_mesh(const float* vp, int nv, const int32_t* ip, int nt) {
+    std::vector<f3> v(nv);
+    for (int i = 0; i < nv; ++i) v[i] = mk3(vp[3 * i], vp[3 * i + 1], vp[3 * i + 2]);
+    std::vector<HTri> ht(nt);
+    for (int t = 0; t < nt; ++t) {
+        HTri& h = ht[t];
+        for (int k = 0; k < 3; ++k) h.idx[k] = ip[3 * t + k];
+        f3 a = v[h.idx[0]], b = v[h.idx[1]], c = v[h.idx[2]];
+        h.b = {vmin(vmin(a, b), c), vmax(vmax(a, b), c)};        // getBoundsTriangle
+        h.n = normalize(cross(sub(b, a), sub(c, a)));
+        h.c = divs(add(add(a, b), c), 3.f);
+    }
+    std::vector<HNode> bvh;
+    bvh.reserve(2 * (size_t)nt + 1);
+    bvh.push_back({hempty(), nt, 0});
+    Builder bld{ht, bvh};
+    bld.node_bounds(0);
+    bld.subdivide(0);
+    BuiltMesh out;
+    // depth of the tree (bounds the device traversal stack)
+    std::vector<int> depth(bvh.size(), 0);
+    for (size_t n = 0; n < bvh.size(); ++n)
+        if (!bvh[n].triCount) {
+            depth[bvh[n].index] = depth[bvh[n].index + 1] = depth[n] + 1;
+            out.depth = std::max(out.depth, depth[n] + 1);
+        }
+    for (const HTri& h : ht) {
+        out.tris.push_back(make_int4(h.idx[0], h.idx[1], h.idx[2], 0));
+        out.triN.push_back(f4(h.n, 0.f));
+        out.triC.push_back(f4(h.c, 0.f));
+    }
+    auto asf = [](int i) { float f; std::memcpy(&f, &i, sizeof f); return f; };
+    for (const HNode& n : bvh) {
+        out.nodes.push_back(make_float4(n.b.mn.x, n.b.mn.y, n.b.mn.z, asf(n.triCount)));
+        out.nodes.push_back(make_float4(n.b.mx.x, n.b.mx.y, n.b.mx.z, asf(n.index)));
+    }
+    return out;
+}
+
+// the cache's layout: header, then tris, triN, triC (nt each), nodes (2 per node)
+constexpr uint32_t MESH_MAGIC = 0x4d584750u;   // "PGXM"
+struct MeshCacheHeader {
+    uint32_t magic, version;
+    int32_t nv, nt, nNodes, depth;
+    uint64_t hash;   // FNV-1a of the vertex and index bytes the cache was built from
+};
+
+uint64_t mesh_hash(const float* vp, int nv, const int32_t* ip, int nt) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
+    };
+    mix(vp, 12 * (size_t)nv);
+    mix(ip, 12 * (size_t)nt);
+    return h;
+}
+
+size_t cache_bytes(int nt, int nNodes) {
+    return sizeof(MeshCacheHeader) + 16 * (3 * (size_t)nt + 2 * (size_t)nNodes);
+}
+
+// the mesh from a cache written for these vertices and indices (throws if it is not one)
+BuiltMesh load_mesh(const void* cache, const float* vp, int nv, const int32_t* ip, int nt) {
+    MeshCacheHeader h;
+    std::memcpy(&h, cache, sizeof h);
+    if (h.magic != MESH_MAGIC || h.version != 1 || h.nv != nv || h.nt != nt || h.nNodes < 1 || h.nNodes > 2 * nt ||
+        h.hash != mesh_hash(vp, nv, ip, nt))
+        throw PgxError{PGX_ERR_INVALID, "triangle-mesh cache was not built from this mesh"};
+    BuiltMesh m;
+    m.depth = h.depth;
+    const char* p = static_cast<const char*>(cache) + sizeof h;
+    auto take = [&](auto& vec, size_t n) {
+        vec.resize(n);
+        std::memcpy(vec.data(), p, sizeof(vec[0]) * n);
+        p += sizeof(vec[0]) * n;
+    };
+    take(m.tris, nt); take(m.triN, nt); take(m.triC, nt); take(m.nodes, 2 * (size_t)h.nNodes);
+    return m;
+}
+
 int upload_triangle_meshes(World& w, const pgx_scene_desc* sc, hipStream_t s) {
     std::vector<float4> verts, triN, triC, nodes;
     std::vector<int4> tris, meshes;
@@ -109,44 +198,18 @@ int upload_triangle_meshes(World& w, const pgx_scene_desc* sc, hipStream_t s) {
     for (int m = 0; m < sc->n_trimesh; ++m) {
         int v0 = sc->tri_vert_start[m], nv = sc->tri_vert_start[m + 1] - v0;
         int t0 = sc->tri_start[m], nt = sc->tri_start[m + 1] - t0;
-        std::vector<f3> v(nv);
-        for (int i = 0; i < nv; ++i)
-            v[i] = mk3(sc->tri_verts[3 * (v0 + i)], sc->tri_verts[3 * (v0 + i) + 1], sc->tri_verts[3 * (v0 + i) + 2]);
-        std::vector<HTri> ht(nt);
-        for (int t = 0; t < nt; ++t) {
-            HTri& h = ht[t];
-            for (int k = 0; k < 3; ++k) h.idx[k] = sc->tri_idx[3 * (t0 + t) + k];
-            f3 a = v[h.idx[0]], b = v[h.idx[1]], c = v[h.idx[2]];
-            h.b = {vmin(vmin(a, b), c), vmax(vmax(a, b), c)};        // getBoundsTriangle
-            h.n = normalize(cross(sub(b, a), sub(c, a)));
-            h.c = divs(add(add(a, b), c), 3.f);
-        }
-        std::vector<HNode> bvh;
-        bvh.reserve(2 * (size_t)nt + 1);
-        bvh.push_back({hempty(), nt, 0});
-        Builder bld{ht, bvh};
-        bld.node_bounds(0);
-        bld.subdivide(0);
-        // depth of the tree (bounds the device traversal stack)
-        std::vector<int> depth(bvh.size(), 0);
-        for (size_t n = 0; n < bvh.size(); ++n)
-            if (!bvh[n].triCount) {
-                depth[bvh[n].index] = depth[bvh[n].index + 1] = depth[n] + 1;
-                maxDepth = std::max(maxDepth, depth[n] + 1);
-            }
+        const float* vp = sc->tri_verts + 3 * (size_t)v0;
+        const int32_t* ip = sc->tri_idx + 3 * (size_t)t0;
+        const void* cache = sc->tri_bvh_cache ? sc->tri_bvh_cache[m] : nullptr;
+        const BuiltMesh bm = cache ? load_mesh(cache, vp, nv, ip, nt) : build_mesh(vp, nv, ip, nt);
+        maxDepth = std::max(maxDepth, bm.depth);
         int vbase = (int)verts.size(), tbase = (int)tris.size(), nbase = (int)nodes.size() / 2;
         meshes.push_back(make_int4(vbase, nv, tbase, nbase));
-        for (const f3& p : v) verts.push_back(f4(p, 0.f));
-        for (const HTri& h : ht) {
-            tris.push_back(make_int4(h.idx[0], h.idx[1], h.idx[2], 0));
-            triN.push_back(f4(h.n, 0.f));
-            triC.push_back(f4(h.c, 0.f));
-        }
-        auto asf = [](int i) { float f; std::memcpy(&f, &i, sizeof f); return f; };
-        for (const HNode& n : bvh) {   // {min, triCount bits}, {max, index bits}
-            nodes.push_back(make_float4(n.b.mn.x, n.b.mn.y, n.b.mn.z, asf(n.triCount)));
-            nodes.push_back(make_float4(n.b.mx.x, n.b.mx.y, n.b.mx.z, asf(n.index)));
-        }
+        for (int i = 0; i < nv; ++i) verts.push_back(make_float4(vp[3 * i], vp[3 * i + 1], vp[3 * i + 2], 0.f));
+        tris.insert(tris.end(), bm.tris.begin(), bm.tris.end());
+        triN.insert(triN.end(), bm.triN.begin(), bm.triN.end());
+        triC.insert(triC.end(), bm.triC.begin(), bm.triC.end());
+        nodes.insert(nodes.end(), bm.nodes.begin(), bm.nodes.end());
     }
     auto up = [&](auto& buf, const auto& vec) {
         buf.alloc(vec.size());
@@ -161,3 +224,30 @@ int upload_triangle_meshes(World& w, const pgx_scene_desc* sc, hipStream_t s) {
 }
 
 }  // namespace pgx
+
+extern "C" int pgx_trimesh_cache(const float* verts, int n_verts, const int32_t* tris, int n_tris, void* out,
+                                 size_t cap, size_t* bytes) {
+    using namespace pgx;
+    if (!verts || !tris || !bytes || n_verts < 3 || n_tris < 1) return PGX_ERR_INVALID;
+    for (int t = 0; t < 3 * n_tris; ++t)
+        if (tris[t] < 0 || tris[t] >= n_verts) return PGX_ERR_INVALID;
+    try {
+        const BuiltMesh m = build_mesh(verts, n_verts, tris, n_tris);
+        const int nNodes = (int)m.nodes.size() / 2;
+        *bytes = cache_bytes(n_tris, nNodes);
+        if (!out) return PGX_OK;
+        if (cap < *bytes) return PGX_ERR_CAPACITY;
+        MeshCacheHeader h{MESH_MAGIC, 1, n_verts, n_tris, nNodes, m.depth, mesh_hash(verts, n_verts, tris, n_tris)};
+        char* p = static_cast<char*>(out);
+        std::memcpy(p, &h, sizeof h);
+        p += sizeof h;
+        auto put = [&](const auto& vec) {
+            std::memcpy(p, vec.data(), sizeof(vec[0]) * vec.size());
+            p += sizeof(vec[0]) * vec.size();
+        };
+        put(m.tris); put(m.triN); put(m.triC); put(m.nodes);
+        return PGX_OK;
+    } catch (...) {
+        return PGX_ERR_INVALID;
+    }
+}

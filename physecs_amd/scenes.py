@@ -11,6 +11,7 @@ density x true volume, inertia from getInertiaSphere/Capsule/Box (the box formul
 the reference's use of half-extents where full extents belong, MassUtil.cpp:22-28).
 Random draws use numpy's PCG64 with the given seed, in the order written below.
 """
+import ctypes
 import math
 
 import numpy as np
@@ -477,6 +478,17 @@ class Scene:
                 kw[name] = len(self.trimesh)
             elif name == "actor_world":
                 kw[name] = arr["world"].ctypes.data_as(ctype)
+            elif name == "tri_bvh_cache":   # per triangle mesh: cached build bytes, or None
+                caches = getattr(self, "tri_bvh_cache", None)
+                if caches:
+                    self._cache_bufs = [None if c is None else ctypes.create_string_buffer(bytes(c), len(c))
+                                        for c in caches]
+                    ptrs = (ctypes.c_void_p * len(caches))(
+                        *[None if b is None else ctypes.cast(b, ctypes.c_void_p).value for b in self._cache_bufs])
+                    self._cache_ptrs = ptrs
+                    kw[name] = ctypes.cast(ptrs, ctype)
+                else:
+                    kw[name] = None
             else:
                 kw[name] = arr[name].ctypes.data_as(ctype)
         return cls(**kw)
