@@ -196,7 +196,8 @@ def test_struct_layouts_match_header(tmp_path):
     from physecs_amd import api
     mirrors = {"pgx_config": api.PgxConfig, "pgx_scene_desc": api.PgxSceneDesc, "pgx_joint_desc": api.PgxJointDesc,
                "pgx_counts": api.PgxCounts, "pgx_stats": api.PgxStats, "pgx_server_config": api.PgxServerConfig,
-               "pgx_server_step_info": api.PgxServerStepInfo, "pgx_tuning": api.PgxTuning}
+               "pgx_server_step_info": api.PgxServerStepInfo, "pgx_tuning": api.PgxTuning,
+               "pgx_character": api.PgxCharacter}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "physecs_gpu.h"', "int main(void) {"]
     for cname, cls in mirrors.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
