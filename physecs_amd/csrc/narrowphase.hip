@@ -1401,8 +1401,9 @@ struct MtdOut {
 };
 
 // pass 3: the pairs with a triangle mesh, one lane per pair: the mesh BVH query, the
-// per-triangle tests into the lane's own block of TM_MAXCAND contact records, the emission
-// passes (the step's three mesh passes in one lane)
+// per-triangle tests into the lane's own block of TM_MAXCAND contact records (a buffer of
+// recLanes * TM_MAXCAND records), the emission passes (the step's three mesh passes in one
+// lane)
 __global__ __launch_bounds__(64) void k_mtd_mesh(MtdArgs A, ConvexView cv, TriMeshView tv, int32_t* tmWs,
                                                  int wsLanes, MeshRecs recs, double* epaV, double* epaFN,
                                                  int32_t* epaFI, int32_t* epaE, int epaLanes) {
@@ -1454,7 +1455,7 @@ void np_mtd_launch(World& w, const MtdArgs& A) {
     }
     if (w.nTriMesh > 0) {
         const TriMeshView tv{w.tmVerts.p, w.tmTris.p, w.tmTriN.p, w.tmTriC.p, w.tmNodes.p, w.tmMesh.p};
-        const MeshRecs recs{w.tmRecs.p, (size_t)w.tmCandCap};
+        const MeshRecs recs{A.recs, (size_t)A.recLanes * TM_MAXCAND};
         hipLaunchKernelGGL(k_mtd_mesh, dim3(div_up(A.recLanes, 64)), dim3(64), 0, w.stream, A, cv, tv, w.tmWs.p,
                            w.meshLanes, recs, w.epaV.p, w.epaFN.p, w.epaFI.p, w.epaE.p, w.epaLanes);
         PGX_LAUNCH_CHECK();

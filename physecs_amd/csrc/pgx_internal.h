@@ -299,6 +299,7 @@ struct World {
     DBuf<int32_t> qMtdN;
     DBuf<float4> qRes;
     DBuf<int2> qCvxPairs, qMeshPairs;
+    DBuf<int32_t> qRecs;                 // MTD triangle-mesh pairs: TM_MAXCAND contact records per lane
     std::vector<int32_t> worldListHost[2];   // worlds of at most WORLD_SMALL dynamic bodies, and the rest
     DBuf<int32_t> worldList[2];
     bool worldsContiguous = false;
@@ -383,6 +384,7 @@ struct OverlapArgs {
 // what the MTD overlap query kernels read and write
 struct MtdArgs {
     int n, nLeaves, cap, cbits, listCap, recLanes;
+    int32_t* recs;        // the triangle-mesh pass's contact records, TM_MAXCAND per lane
     const float4 *qPrm, *qPos, *qQuat;      // per query: params (.w mesh), position (.w world bits), orientation
     const int32_t* qType;
     const int32_t *worldRoot, *worldStop, *worldStaticStart, *staticCols, *mvals;

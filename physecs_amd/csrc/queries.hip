@@ -279,7 +279,14 @@ int qr_overlap_mtd(World& w, int n, const int32_t* types, const float* params, c
     MtdArgs A;
     A.n = n; A.nLeaves = w.nDynCol; A.cap = capacity; A.cbits = cbits;
     // triangle-mesh pairs: one lane each, with TM_MAXCAND contact records of its own
-    A.recLanes = std::max(1, std::min({w.meshLanes, w.epaLanes, w.tmCandCap / TM_MAXCAND}));
+    // (a buffer of the query's own, allocated on first use: 8192 lanes, 126 MB)
+    A.recLanes = 1;
+    A.recs = nullptr;
+    if (w.nTriMesh > 0) {
+        A.recLanes = std::max(1, std::min({w.meshLanes, w.epaLanes, 8192}));
+        w.qRecs.alloc((size_t)(TM_CI_WORDS + 1) * TM_MAXCAND * A.recLanes);
+        A.recs = w.qRecs.p;
+    }
     A.qPrm = w.qShape.p; A.qPos = w.qShape.p + n; A.qQuat = w.qShape.p + 2 * (size_t)n; A.qType = w.qType.p;
     A.worldRoot = w.worldRoot.p; A.worldStop = w.worldStop.p;
     A.worldStaticStart = w.worldStaticStart.p; A.staticCols = w.staticCols.p; A.mvals = w.mvals.p;
