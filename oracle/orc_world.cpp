@@ -74,6 +74,24 @@ enum JointType { J_SPHERICAL = 0, J_UNIVERSAL = 1, J_REVOLUTE = 2, J_FIXED = 3, 
 enum RowFlags { F_NONE = 0, F_SOFT = 1, F_ANGULAR = 2, F_LIMITED = 4 };
 // flag-type order of Constraint1DContainer's collections (Constraint1DContainer.h:117)
 static const int kFlagOrder[6] = {F_NONE, F_ANGULAR, F_SOFT, F_LIMITED, F_ANGULAR | F_SOFT, F_ANGULAR | F_LIMITED};
+
+// Planted mistakes, for tests that show a pin catches them (0 = none, the restatement as is).
+// Each one is a plausible misreading of the reference at the place it names.
+enum Mutation {
+    MUT_NONE = 0,
+    MUT_SIMD_WARM_LOCAL = 1,        // Constraint1DW.cpp:137-161: warm impulse kept out of the written velocities
+    MUT_SIMD_LIMIT_UNSCALED = 2,    // Constraint1DW.cpp:196-200: clamp to [min, max] instead of [min*h, max*h]
+    MUT_SIMD_MASK_NONSTRICT = 3,    // Constraint1DW.cpp:140-142: <= instead of <
+    MUT_SIMD_BIAS_FLOAT = 4,        // Constraint1DW.cpp:175: 0.2f * c / h instead of (float)(0.2 / h) * c
+    MUT_SIMD_ZERO_WRITEBACK = 5,    // Constraint1DW.cpp:164,213-215: lanes with invEff 0 written back
+    MUT_FRICTION_SOLVER_VEL = 6,    // ContactConstraints.cpp:92-101: friction from the solver velocities
+    MUT_NO_RESTITUTION_CACHE = 7,   // Physecs.cpp:291-304: cached target velocities ignored
+    MUT_RELAX_SOFT = 8,             // Physecs.cpp:515-520: soft contacts relaxed too
+    MUT_JOINT_AXIS_ORDER = 9,       // Joints/*.cpp: angular row axis cross(a, b) instead of cross(b, a)
+    MUT_JOINT_NO_COM = 10,          // Physecs.cpp:343-344: anchor not taken relative to the COM
+    MUT_JOINT_UNIVERSAL_AXIS = 11,  // UniversalJoint.cpp:102-105: u1[1] instead of u1[2]
+};
+static int g_mutation = MUT_NONE;
 struct Joint {
     int type, a0, a1;                  // actor indices
     V3 p0; Q q0; V3 p1; Q q1;          // anchors in body space
@@ -360,7 +378,7 @@ static void narrowTask(orc_world* w, int index, std::vector<CC>& outCC, std::vec
             r1 = qrot(qinverse(t1.quat), r1);
             float tv = 0;
             bool found = false;
-            if (prev) {
+            if (prev && g_mutation != MUT_NO_RESTITUTION_CACHE) {
                 for (int i = 0; i < prev->numPoints; ++i) {
                     // glm::distance(r0, prev) < 0.1 compares in double (Physecs.cpp:294)
                     if ((double)distance(r0, prev->lp[i]) < 0.1) { found = true; tv = prev->tv[i]; break; }
@@ -515,8 +533,8 @@ static void buildJointRows(orc_world* w) {
         const Dyn* d1 = t1.dyn >= 0 ? &w->dyn[t1.dyn] : nullptr;
         int b0 = d0 && !d0->kinematic ? t0.dyn : -1;
         int b1 = d1 && !d1->kinematic ? t1.dyn : -1;
-        jr.r0 = b0 >= 0 ? j.p0 - d0->com : j.p0;
-        jr.r1 = b1 >= 0 ? j.p1 - d1->com : j.p1;
+        jr.r0 = b0 >= 0 && g_mutation != MUT_JOINT_NO_COM ? j.p0 - d0->com : j.p0;
+        jr.r1 = b1 >= 0 && g_mutation != MUT_JOINT_NO_COM ? j.p1 - d1->com : j.p1;
         int fl[7];
         jr.nRows = rowLayout(w, j, fl);
         for (int r = 0; r < jr.nRows; ++r) {
@@ -555,7 +573,7 @@ static void updateJointRows(orc_world* w, JointRows& jr) {
     V3 d = p1 - p0;
     Row* R = jr.rows;
     auto angularRow = [&](Row& row, V3 a, V3 b) {   // c = dot(a, b), axis = cross(b, a)
-        V3 ax = cross(b, a);
+        V3 ax = g_mutation == MUT_JOINT_AXIS_ORDER ? cross(a, b) : cross(b, a);
         row.angular0 = ax; row.angular1 = ax; row.c = dot(a, b);
     };
     if (j.type == J_PRISMATIC) {                    // PrismaticJoint.cpp:6-92
@@ -617,7 +635,7 @@ static void updateJointRows(orc_world* w, JointRows& jr) {
     R[0].linear = v3(1, 0, 0); R[0].angular0 = v3(0, r0.z, -r0.y); R[0].angular1 = v3(0, r1.z, -r1.y); R[0].c = d.x;
     R[1].linear = v3(0, 1, 0); R[1].angular0 = v3(-r0.z, 0, r0.x); R[1].angular1 = v3(-r1.z, 0, r1.x); R[1].c = d.y;
     R[2].linear = v3(0, 0, 1); R[2].angular0 = v3(r0.y, -r0.x, 0); R[2].angular1 = v3(r1.y, -r1.x, 0); R[2].c = d.z;
-    if (j.type == J_UNIVERSAL) angularRow(R[3], u0[2], u1[2]);
+    if (j.type == J_UNIVERSAL) angularRow(R[3], u0[2], g_mutation == MUT_JOINT_UNIVERSAL_AXIS ? u1[1] : u1[2]);
     if (j.type == J_REVOLUTE || j.type == J_FIXED || j.type == J_SERVO) {
         angularRow(R[3], u0[0], u1[1]);
         angularRow(R[4], u0[0], u1[2]);
@@ -689,9 +707,12 @@ static void rowSolveSimd(Row& r, VelocityData* vel, float h, bool warm) {
     V3 v0 = v3s(0), w0 = v3s(0), v1 = v3s(0), w1 = v3s(0);
     if (r.b0 >= 0) { if (!ang) v0 = vel[r.b0].v; w0 = vel[r.b0].w; }
     if (r.b1 >= 0) { if (!ang) v1 = vel[r.b1].v; w1 = vel[r.b1].w; }
+    const V3 v0in = v0, w0in = w0, v1in = v1, w1in = w1;
     if (!(r.flags & F_SOFT) && warm) {
         float ac = std::fabs(r.c), al = std::fabs(r.lambda);
-        if (ac < 1e-4f && al < 10000.f) {
+        const bool inMask = g_mutation == MUT_SIMD_MASK_NONSTRICT ? (ac <= 1e-4f && al <= 10000.f)
+                                                                  : (ac < 1e-4f && al < 10000.f);
+        if (inMask) {
             float lam = r.lambda * 0.5f;
             if (!ang) { v0 = v0 + lam * r.linear0t; v1 = v1 - lam * r.linear1t; }
             w0 = w0 + lam * r.angular0t;
@@ -699,7 +720,22 @@ static void rowSolveSimd(Row& r, VelocityData* vel, float h, bool warm) {
             r.lambda = lam;
         }
     }
-    if (r.invEff == 0.f) return;
+    if (r.invEff == 0.f) {
+        if (g_mutation == MUT_SIMD_ZERO_WRITEBACK) {
+            if (r.b0 >= 0) { if (!ang) vel[r.b0].v = v0; vel[r.b0].w = w0; }
+            if (r.b1 >= 0) { if (!ang) vel[r.b1].v = v1; vel[r.b1].w = w1; }
+        }
+        return;
+    }
+    if (g_mutation == MUT_SIMD_WARM_LOCAL && !(r.flags & (F_SOFT | F_LIMITED))) {
+        // the relative velocity still sees the warm impulse, the write-back does not
+        const float rvw = dot(r.angular1, w1) - dot(r.angular0, w0) + (ang ? 0.f : dot(r.linear, v1) - dot(r.linear, v0));
+        const float lam = (rvw - r.tv + (float)(0.2 / (double)h) * r.c) * (1.f / r.invEff);
+        r.lambda = r.lambda + lam;
+        if (r.b0 >= 0) { if (!ang) vel[r.b0].v = v0in + lam * r.linear0t; vel[r.b0].w = w0in + lam * r.angular0t; }
+        if (r.b1 >= 0) { if (!ang) vel[r.b1].v = v1in - lam * r.linear1t; vel[r.b1].w = w1in - lam * r.angular1t; }
+        return;
+    }
     float rv = dot(r.angular1, w1) - dot(r.angular0, w0);
     if (!ang) rv = rv + (dot(r.linear, v1) - dot(r.linear, v0));
     float effMass = 1.f / r.invEff;
@@ -714,12 +750,14 @@ static void rowSolveSimd(Row& r, VelocityData* vel, float h, bool warm) {
         lambda = (rv + beta * r.c / h) / (r.invEff + gamma / h);
     } else {
         float bias = (float)(0.2 / (double)h);
-        lambda = (rv - r.tv + bias * r.c) * effMass;
+        if (g_mutation == MUT_SIMD_BIAS_FLOAT) lambda = (rv - r.tv + 0.2f * r.c / h) * effMass;
+        else lambda = (rv - r.tv + bias * r.c) * effMass;
     }
     float prev = r.lambda;
     if (r.flags & F_LIMITED) {
         float t = r.lambda + lambda;
         float lo = r.mn * h, hi = r.mx * h;
+        if (g_mutation == MUT_SIMD_LIMIT_UNSCALED) { lo = r.mn; hi = r.mx; }
         t = t > lo ? t : lo;
         t = t < hi ? t : hi;
         r.lambda = t;
@@ -846,6 +884,10 @@ static void solveContact(orc_world* w, CC& c, VelocityData* vel, bool useBias, f
         V3 v0 = v3s(0), w0 = v3s(0), v1 = v3s(0), w1 = v3s(0);
         if (c.b0 >= 0) { v0 = dyn0->v; w0 = dyn0->w; }     // component velocities (quirk)
         if (c.b1 >= 0) { v1 = dyn1->v; w1 = dyn1->w; }
+        if (g_mutation == MUT_FRICTION_SOLVER_VEL) {
+            if (c.b0 >= 0) { v0 = vel[c.b0].v; w0 = vel[c.b0].w; }
+            if (c.b1 >= 0) { v1 = vel[c.b1].v; w1 = vel[c.b1].w; }
+        }
         float rv = dot(-p.t, v0) + dot(-p.r0xt, w0) + dot(p.t, v1) + dot(p.r1xt, w1);
         float lambda = rv / p.invEffT;
         float limit = c.friction * p.lN;
@@ -960,8 +1002,8 @@ static void substeps(orc_world* w, float timeStep) {
         }
         // relaxation, src/Physecs.cpp:516-519
         for (auto& c : w->ccs) {
-            if (c.isSoft) continue;
-            solveContact(w, c, vel.data(), false, 0.f);
+            if (c.isSoft && g_mutation != MUT_RELAX_SOFT) continue;
+            solveContact(w, c, vel.data(), false, c.isSoft ? h : 0.f);
         }
         // write back, src/Physecs.cpp:523-530
         for (size_t i = 0; i < nd; ++i) {
@@ -1285,10 +1327,67 @@ int orc_world_destroy_joint(orc_world* w, int i) {
 
 int orc_world_joint_color(const orc_world* w, int i) { return w->joints[i].color; }
 
+// Plants one of the Mutation mistakes in every later call (0 restores the restatement).
+void orc_set_mutation(int m) { g_mutation = m; }
+int orc_get_mutation(void) { return g_mutation; }
+
+// The rows joint i would get at the world's current poses: buildJointRows's layout for that
+// joint, then updateJointRows (the same two steps a step runs). Per row, 18 floats: flags, b0,
+// b1, linear(3), angular0(3), angular1(3), c, targetVelocity, min, max, frequency, damping.
+// The joint's own state (gear angles, prismatic limit choice) is left as it was. Returns the
+// row count, or -1 for a destroyed or unknown joint.
+int orc_world_joint_rows(orc_world* w, int i, float* out) {
+    if (i < 0 || i >= (int)w->joints.size() || !w->joints[i].alive) return -1;
+    const Joint saved = w->joints[i];
+    Joint& j = w->joints[i];
+    JointRows jr{};
+    jr.joint = i;
+    const Actor& t0 = w->actors[j.a0];
+    const Actor& t1 = w->actors[j.a1];
+    const Dyn* d0 = t0.dyn >= 0 ? &w->dyn[t0.dyn] : nullptr;
+    const Dyn* d1 = t1.dyn >= 0 ? &w->dyn[t1.dyn] : nullptr;
+    int b0 = d0 && !d0->kinematic ? t0.dyn : -1;
+    int b1 = d1 && !d1->kinematic ? t1.dyn : -1;
+    jr.r0 = b0 >= 0 && g_mutation != MUT_JOINT_NO_COM ? j.p0 - d0->com : j.p0;
+    jr.r1 = b1 >= 0 && g_mutation != MUT_JOINT_NO_COM ? j.p1 - d1->com : j.p1;
+    int fl[7];
+    jr.nRows = rowLayout(w, j, fl);
+    for (int r = 0; r < jr.nRows; ++r) {
+        Row& row = jr.rows[r];
+        std::memset(&row, 0, sizeof(row));
+        row.flags = fl[r];
+        row.b0 = b0; row.b1 = b1;
+        row.mn = std::numeric_limits<float>::lowest();
+        row.mx = std::numeric_limits<float>::max();
+    }
+    updateJointRows(w, jr);
+    for (int r = 0; r < jr.nRows; ++r) {
+        const Row& row = jr.rows[r];
+        float* o = out + 18 * r;
+        o[0] = (float)row.flags; o[1] = (float)row.b0; o[2] = (float)row.b1;
+        o[3] = row.linear.x; o[4] = row.linear.y; o[5] = row.linear.z;
+        o[6] = row.angular0.x; o[7] = row.angular0.y; o[8] = row.angular0.z;
+        o[9] = row.angular1.x; o[10] = row.angular1.y; o[11] = row.angular1.z;
+        o[12] = row.c; o[13] = row.tv; o[14] = row.mn; o[15] = row.mx; o[16] = row.freq; o[17] = row.damp;
+    }
+    w->joints[i] = saved;
+    return jr.nRows;
+}
+
 // One joint row through this restatement, same layout as ref_row_scalar (oracle/ref_harness.cpp);
 // simd != 0 runs the per-lane Constraint1DW semantics instead of Constraint1D's.
+int orc_row_ex(int simd, int flags, int b0, int b1, const float* in, const float* masses, float* vel, float* pv,
+               float h, int iters, float lambda0, int warmMask, float* out);
+
 int orc_row(int simd, int flags, int b0, int b1, const float* in, const float* masses, float* vel, float* pv, float h,
             int iters, float* out) {
+    return orc_row_ex(simd, flags, b0, b1, in, masses, vel, pv, h, iters, 0.f, 1, out);
+}
+
+// orc_row with a starting accumulated lambda (it carries over between the substeps of a step)
+// and a per-solve warm-start mask (bit i: warm start on solve i)
+int orc_row_ex(int simd, int flags, int b0, int b1, const float* in, const float* masses, float* vel, float* pv,
+               float h, int iters, float lambda0, int warmMask, float* out) {
     Row r{};
     r.flags = flags; r.b0 = b0 >= 0 ? 0 : -1; r.b1 = b1 >= 0 ? 1 : -1;
     r.linear = v3(in[0], in[1], in[2]); r.angular0 = v3(in[3], in[4], in[5]); r.angular1 = v3(in[6], in[7], in[8]);
@@ -1306,9 +1405,11 @@ int orc_row(int simd, int flags, int b0, int b1, const float* in, const float* m
         pd[k].count = (int)pv[7 * k + 6];
     }
     rowPreSolve(r, md, pd, simd != 0);
+    r.lambda = lambda0;
     for (int i = 0; i < iters; ++i) {
-        if (simd) rowSolveSimd(r, vd, h, i == 0);
-        else rowSolveScalar(r, vd, h, i == 0);
+        const bool warm = ((unsigned)warmMask >> i) & 1u;
+        if (simd) rowSolveSimd(r, vd, h, warm);
+        else rowSolveScalar(r, vd, h, warm);
     }
     for (int k = 0; k < 2; ++k) {
         vel[6 * k] = vd[k].v.x; vel[6 * k + 1] = vd[k].v.y; vel[6 * k + 2] = vd[k].v.z;

@@ -67,9 +67,11 @@ static glm::quat Qt(const float* q) { return glm::quat(q[3], q[0], q[1], q[2]); 
 // first with warm start. in: linear(3) angular0(3) angular1(3) tv c min max freq damp;
 // masses: invMass0, invI0(9), invMass1, invI1(9); vel: v0 w0 v1 w1 (12, updated);
 // pv: pv0 pw0 count0 pv1 pw1 count1 (14, updated). out: invEff, lambda.
+// lambda0 is the accumulated lambda the row starts with (totalLambda, which carries over from
+// the previous substep within a step); bit i of warmMask asks for warm start on solve i.
 template <int F>
 static void runRow(int b0, int b1, const float* in, const float* masses, float* vel, float* pv, float h, int iters,
-                   float* out) {
+                   float* out, float lambda0 = 0.f, unsigned warmMask = 1u) {
     Constraint1D<F> r(b0 >= 0 ? 0 : -1, b1 >= 0 ? 1 : -1);
     r.linear = V(in); r.angular0 = V(in + 3); r.angular1 = V(in + 6);
     r.targetVelocity = in[9]; r.c = in[10]; r.min = in[11]; r.max = in[12]; r.frequency = in[13]; r.dampingRatio = in[14];
@@ -84,7 +86,8 @@ static void runRow(int b0, int b1, const float* in, const float* masses, float* 
         pd[k].constraintCount = (int)pv[7 * k + 6];
     }
     r.preSolve(md, pd);
-    for (int i = 0; i < iters; ++i) r.solve(vd, h, i == 0);
+    r.totalLambda = lambda0;
+    for (int i = 0; i < iters; ++i) r.solve(vd, h, ((warmMask >> i) & 1u) != 0);
     for (int k = 0; k < 2; ++k) {
         vel[6 * k] = vd[k].velocity.x; vel[6 * k + 1] = vd[k].velocity.y; vel[6 * k + 2] = vd[k].velocity.z;
         vel[6 * k + 3] = vd[k].angularVelocity.x; vel[6 * k + 4] = vd[k].angularVelocity.y; vel[6 * k + 5] = vd[k].angularVelocity.z;
@@ -305,6 +308,21 @@ int ref_row_scalar(int flags, int b0, int b1, const float* in, const float* mass
         case LIMITED: runRow<LIMITED>(b0, b1, in, masses, vel, pv, h, iters, out); return 0;
         case ANGULAR | SOFT: runRow<ANGULAR | SOFT>(b0, b1, in, masses, vel, pv, h, iters, out); return 0;
         case ANGULAR | LIMITED: runRow<ANGULAR | LIMITED>(b0, b1, in, masses, vel, pv, h, iters, out); return 0;
+    }
+    return -1;
+}
+
+// ref_row_scalar with a starting accumulated lambda and a per-solve warm-start mask
+int ref_row_scalar_ex(int flags, int b0, int b1, const float* in, const float* masses, float* vel, float* pv, float h,
+                      int iters, float lambda0, int warmMask, float* out) {
+    const unsigned m = (unsigned)warmMask;
+    switch (flags) {
+        case NONE: runRow<NONE>(b0, b1, in, masses, vel, pv, h, iters, out, lambda0, m); return 0;
+        case ANGULAR: runRow<ANGULAR>(b0, b1, in, masses, vel, pv, h, iters, out, lambda0, m); return 0;
+        case SOFT: runRow<SOFT>(b0, b1, in, masses, vel, pv, h, iters, out, lambda0, m); return 0;
+        case LIMITED: runRow<LIMITED>(b0, b1, in, masses, vel, pv, h, iters, out, lambda0, m); return 0;
+        case ANGULAR | SOFT: runRow<ANGULAR | SOFT>(b0, b1, in, masses, vel, pv, h, iters, out, lambda0, m); return 0;
+        case ANGULAR | LIMITED: runRow<ANGULAR | LIMITED>(b0, b1, in, masses, vel, pv, h, iters, out, lambda0, m); return 0;
     }
     return -1;
 }

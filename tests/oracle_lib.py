@@ -197,6 +197,15 @@ class OracleWorld:
         self.lib.orc_world_joint_color.argtypes = [ctypes.c_void_p, ctypes.c_int]
         return [self.lib.orc_world_joint_color(self.h, i) for i in range(len(self.scene.joints))]
 
+    def joint_rows(self, index):
+        """The rows joint `index` gets at the current poses, as an (n, 18) array: flags, b0, b1,
+        linear, angular0, angular1, c, target velocity, min, max, frequency, damping."""
+        self.lib.orc_world_joint_rows.argtypes = [ctypes.c_void_p, ctypes.c_int, F]
+        buf = np.zeros((7, 18), np.float32)
+        n = self.lib.orc_world_joint_rows(self.h, index, fptr(buf))
+        assert n >= 0
+        return buf[:n].copy()
+
     def __del__(self):
         if getattr(self, "h", None):
             self.lib.orc_world_destroy(self.h)
@@ -336,6 +345,51 @@ def row_solve(which, flags, b0, b1, inp, masses, vel, pv, h, iters):
                        ctypes.c_int, F]
         fn(1 if which == "simd" else 0, flags, b0, b1, fptr(inp), fptr(masses), fptr(vel), fptr(pv), np.float32(h),
            iters, fptr(out))
+    return vel, pv, out
+
+
+# Planted mistakes in the oracle (oracle/orc_world.cpp, enum Mutation), for tests that show
+# a pin fails on a plausible misreading of the reference.
+MUT_SIMD_WARM_LOCAL, MUT_SIMD_LIMIT_UNSCALED, MUT_SIMD_MASK_NONSTRICT, MUT_SIMD_BIAS_FLOAT = 1, 2, 3, 4
+MUT_SIMD_ZERO_WRITEBACK, MUT_FRICTION_SOLVER_VEL, MUT_NO_RESTITUTION_CACHE, MUT_RELAX_SOFT = 5, 6, 7, 8
+MUT_JOINT_AXIS_ORDER, MUT_JOINT_NO_COM, MUT_JOINT_UNIVERSAL_AXIS = 9, 10, 11
+
+
+class mutation:
+    """`with mutation(m):` runs the oracle with planted mistake m, then restores it."""
+
+    def __init__(self, m):
+        self.m = m
+
+    def __enter__(self):
+        orc().orc_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        orc().orc_set_mutation(0)
+        return False
+
+
+def row_solve_ex(which, flags, b0, b1, inp, masses, vel, pv, h, iters, lambda0, warm_mask):
+    """row_solve with a starting accumulated lambda and a per-solve warm-start mask (bit i:
+    warm start on solve i); which = 'ref' (reference Constraint1D), 'scalar' or 'simd'."""
+    vel = np.array(vel, np.float32)
+    pv = np.array(pv, np.float32)
+    out = np.zeros(2, np.float32)
+    inp = np.ascontiguousarray(inp, np.float32)
+    masses = np.ascontiguousarray(masses, np.float32)
+    if which == "ref":
+        fn = ref().ref_row_scalar_ex
+        fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, F, F, F, F, ctypes.c_float, ctypes.c_int,
+                       ctypes.c_float, ctypes.c_int, F]
+        fn(flags, b0, b1, fptr(inp), fptr(masses), fptr(vel), fptr(pv), np.float32(h), iters, np.float32(lambda0),
+           warm_mask, fptr(out))
+    else:
+        fn = orc().orc_row_ex
+        fn.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, F, F, F, F, ctypes.c_float,
+                       ctypes.c_int, ctypes.c_float, ctypes.c_int, F]
+        fn(1 if which == "simd" else 0, flags, b0, b1, fptr(inp), fptr(masses), fptr(vel), fptr(pv), np.float32(h),
+           iters, np.float32(lambda0), warm_mask, fptr(out))
     return vel, pv, out
 
 
