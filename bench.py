@@ -8,6 +8,11 @@ inputs resident in HBM. The default workload is the 32,768-body mixed scene (1/3
 capsules, boxes on a jittered 32^3 lattice at 1.5 m in an open 5-box container, 600 steps:
 10 warm-up + 590 timed). Prints one JSON line (metric body-substeps/s, higher is better).
 
+The timed window is always the end of the workload's run: with --warmup W and --steps K
+adding up to less than the workload's length (600 steps for the default workload), the
+first 600 - W - K steps run untimed (config.settle_steps), then W warm-up steps, then the K
+timed ones, so a short run times the settled pile, not the first fall.
+
 roofline: the dominant kernel is k_solve, the contact solve of one graph colour (one launch
 per colour and pass: numIterations biased passes + 1 relaxation, every substep). Its
 algorithmic bytes per launch are 284 B per contact point + 12 B per manifold of the colour
@@ -19,8 +24,9 @@ k_solve_flow per pass instead (every colour in one launch: a manifold solves onc
 bodies have taken the lower colours' solves), so a launch covers all of the pass's points
 and manifolds; --exec cooperative times the single-launch substep kernel k_step_persistent
 with the per-substep phase model.
-traffic: HBM bytes per k_solve launch from the committed PMC summary (profiles/), default
-workload only.
+traffic: HBM bytes per k_solve launch from the committed PMC summary (profiles/), which
+records the window its counters were taken over; it is reported only when this run times
+the same window of the same workload (config.window), and is null otherwise.
 
 cpu_baseline: the CPU oracle (a restatement of the reference's Scene::simulate with the
 reference's ThreadPool narrowphase chunking, nproc-1 workers + caller) on the same scene,
@@ -38,23 +44,31 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_SUMMARY = {"colors": os.path.join(ROOT, "profiles", "k_solve_plain32k_traffic.md")}
+TRAFFIC_SUMMARY = {("plain32k", "colors"): os.path.join(ROOT, "profiles", "k_solve_plain32k_traffic.md")}
+# steps of each workload's run; the timed window is its end
+RUN_STEPS = {"plain32k": 600, "pyramid": 300, "batched": 300, "convex": 600, "triggers": 200}
 
 
-def measured_traffic(mode):
-    """HBM bytes per launch of the roofline kernel on the default workload, from the
-    committed PMC summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the same
-    counter's reading for a copy of known size); None when there is no summary."""
-    if mode not in TRAFFIC_SUMMARY:
+def measured_traffic(config, mode, window):
+    """HBM bytes per launch of the roofline kernel, from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch over the timed window's launches, each scaled
+    by the same counter's reading for a copy of known size). None when there is no summary
+    or when the summary's counters were taken over another window."""
+    path = TRAFFIC_SUMMARY.get((config, mode))
+    if path is None:
         return None
+    fields = {}
     try:
-        with open(TRAFFIC_SUMMARY[mode]) as f:
+        with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+                if ":" in line and not line.startswith(("#", "|", " ")):
+                    k, v = line.split(":", 1)
+                    fields[k.strip()] = v.strip()
     except OSError:
-        pass
-    return None
+        return None
+    if fields.get("window") != window or "traffic_bytes_per_launch" not in fields:
+        return None
+    return float(fields["traffic_bytes_per_launch"])
 
 
 def copy_calibration():
@@ -414,6 +428,14 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    try:
+        dispatch(args)
+    finally:
+        if "physecs_amd.api" in sys.modules:
+            sys.modules["physecs_amd.api"]._close_open_handles()   # explicit teardown before shutdown
+
+
+def dispatch(args):
     if args.config == "served":
         run_served(args)
         return
@@ -432,14 +454,27 @@ def main():
             args.steps = 10 if args.config != "meshcache" else 5
         {"character": run_character, "snapshot": run_snapshot, "meshcache": run_meshcache}[args.config](args)
         return
+    run_world(args)
+
+
+def run_world(args):
     from physecs_amd import api
     print(f"building the {args.config} scene", file=sys.stderr, flush=True)
     scene, cfg = make_scene(args.config)
     print(f"scene built: {scene.n_dyn} dynamic bodies", file=sys.stderr, flush=True)
     sub, it, dt = cfg["substeps"], cfg["iterations"], cfg["dt"]
+    total = RUN_STEPS.get(args.config, args.warmup + args.steps)
+    settle = max(0, total - args.warmup - args.steps)
+    first = settle + args.warmup + 1
+    window = f"steps {first}-{first + args.steps - 1} of {max(total, settle + args.warmup + args.steps)}"
     w = api.World(scene, substeps=sub, iterations=it)
     w.set_exec(args.exec, blocks=args.blocks)
     w.set_graph(args.graph)
+    for k in range(settle):
+        w.step(dt)
+        if (k + 1) % 100 == 0:
+            w.sync()
+            print(f"settle step {k + 1}/{settle}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         w.step(dt)
     w.sync()
@@ -478,6 +513,8 @@ def main():
         w.sync()
     st = w.stats()
     counts = w.counts()
+    tuning, exec_mode, phase = w.tuning(), w.exec_mode(), w.timing()
+    w.close()
     bss = scene.n_dyn * sub * args.steps / el
     achieved = st["solve_bytes"] / (st["solve_us"] * 1e-6) / 1e9 if st["solve_us"] > 0 else 0.0
     line = {
@@ -486,15 +523,16 @@ def main():
         "ms_per_step": 1000 * el / args.steps, "p50_step_ms": 1000 * float(np.percentile(lat, 50)),
         "p99_step_ms": 1000 * float(np.percentile(lat, 99)), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
-        "config": dict(cfg, parallelism="single GPU", graph=args.graph, tuning=w.tuning(), tuned=args.tune,
-                       exec=w.exec_mode(),
+        "config": dict(cfg, parallelism="single GPU", graph=args.graph, tuning=tuning, tuned=args.tune,
+                       exec=exec_mode, settle_steps=settle, window=window,
+                       points_per_body=counts["points"] / scene.n_dyn,
                        last_step_counts=counts,
-                       phase_us_last_step=w.timing()),
+                       phase_us_last_step=phase),
         "roofline": {"bound": "hbm", "kernel": {"flow": "k_solve_flow", "xcd": "k_solve_flow", "colors": "k_solve",
                                                     "cooperative": "k_step_persistent",
-                                                    "worlds": "k_world_pass"}[w.exec_mode()], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                                    "worlds": "k_world_pass"}[exec_mode], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(w.exec_mode()) if args.config == "plain32k" else None,
+                     "traffic": measured_traffic(args.config, exec_mode, window),
                      "avg_launch_us": st["solve_us"] / max(1, st["solve_launches"]),
                      "bytes_per_launch": st["solve_bytes"] / max(1, st["solve_launches"]),
                      "launches_timed": st["solve_launches"]},

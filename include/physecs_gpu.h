@@ -124,9 +124,12 @@ typedef struct pgx_scene_desc {
     const int32_t* tri_start;          /* n_trimesh + 1 */
     const int32_t* tri_idx;            /* 3 x triangles */
     /* Optional (NULL, or NULL entries: build at creation): per triangle mesh, the bytes
-     * pgx_trimesh_cache wrote for that same mesh; creation then loads its triangle order
-     * and BVH instead of building them. A cache of another mesh is rejected. */
+     * pgx_trimesh_cache wrote for that same mesh, and their lengths (required with
+     * tri_bvh_cache); creation then loads its triangle order and BVH instead of building
+     * them. A cache of another mesh, of another length, or with a corrupted payload is
+     * rejected (pgx_trimesh_cache_validate). */
     const void* const* tri_bvh_cache;
+    const size_t* tri_bvh_cache_bytes;
 } pgx_scene_desc;
 
 /* Built-in joint types (include/Physecs/Joints/*.h) */
@@ -172,6 +175,12 @@ typedef struct pgx_counts {
 /* Library / device */
 const char* pgx_last_error(void);
 int pgx_device_count(int* count);
+
+/* Diagnostic, no counterpart in the reference: the library's stable radix sort of (keys,
+ * vals) by the low `bits` bits of the keys, the sort the step uses for Morton codes, pair
+ * keys, shape-pair classes and colours. Host arrays in and out; runs on `device`. */
+int pgx_sort_u64(int device, const uint64_t* keys, const int32_t* vals, int n, int bits, uint64_t* keys_out,
+                 int32_t* vals_out);
 
 /* World lifetime. The descriptor is copied to the device; creation-time bounds carry no
  * margin (src/Physecs.cpp:32). cfg may be NULL for the defaults. */
@@ -379,6 +388,12 @@ int pgx_convex_hull(const float* points, int n, float* verts, int* n_verts, int*
  * cap < *bytes. The bytes hold a hash of the vertices and indices they were built from. */
 int pgx_trimesh_cache(const float* verts, int n_verts, const int32_t* tris, int n_tris, void* out, size_t cap,
                       size_t* bytes);
+/* Host code, no counterpart in the reference: checks `bytes` bytes of a pgx_trimesh_cache
+ * cache against the mesh it is meant for, as world creation does before the device sees
+ * it: length, the hashes of the mesh and of the payload, child and leaf ranges, vertex
+ * indices, depth. PGX_OK or PGX_ERR_INVALID. */
+int pgx_trimesh_cache_validate(const float* verts, int n_verts, const int32_t* tris, int n_tris, const void* cache,
+                               size_t bytes);
 /* Save and resume a world's state between steps (no counterpart in the reference, which
  * keeps its hidden state -- contactCache, GearJointData, triggerCache; include/Physecs/
  * Physecs.h:167-171, include/Physecs/Joints/GearJoint.h:7-14 -- only inside a live Scene).

@@ -4,8 +4,11 @@ The library is built in-tree (physecs_amd/lib/libphysecs_gpu.so) by physecs_amd.
 There is no CPU fallback: loading fails loudly when the library is missing, and creating
 a world fails with the HIP error when no GPU is present.
 """
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -23,7 +26,7 @@ ERRORS = {-1: "PGX_ERR_INVALID", -2: "PGX_ERR_HIP", -3: "PGX_ERR_CAPACITY", -4: 
 
 # every symbol declared in include/physecs_gpu.h
 EXPORTS = [
-    "pgx_last_error", "pgx_device_count", "pgx_world_create", "pgx_world_destroy", "pgx_world_set_params",
+    "pgx_last_error", "pgx_device_count", "pgx_sort_u64", "pgx_world_create", "pgx_world_destroy", "pgx_world_set_params",
     "pgx_world_set_can_collide", "pgx_world_step", "pgx_world_sync", "pgx_world_read_state",
     "pgx_world_write_state", "pgx_world_get_counts", "pgx_world_get_bounds", "pgx_world_get_pairs",
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
@@ -31,7 +34,7 @@ EXPORTS = [
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
     "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_character_move", "pgx_mass_properties",
-    "pgx_world_patch_transforms", "pgx_convex_hull", "pgx_trimesh_cache", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
+    "pgx_world_patch_transforms", "pgx_convex_hull", "pgx_trimesh_cache", "pgx_trimesh_cache_validate", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -54,7 +57,7 @@ class PgxSceneDesc(ctypes.Structure):
         ("n_convex", ctypes.c_int), ("convex_vert_start", I32), ("convex_verts", F), ("convex_face_start", I32),
         ("face_idx_start", I32), ("face_idx", I32), ("face_normal", F), ("face_centroid", F),
         ("n_trimesh", ctypes.c_int), ("tri_vert_start", I32), ("tri_verts", F), ("tri_start", I32), ("tri_idx", I32),
-        ("tri_bvh_cache", ctypes.POINTER(ctypes.c_void_p)),
+        ("tri_bvh_cache", ctypes.POINTER(ctypes.c_void_p)), ("tri_bvh_cache_bytes", ctypes.POINTER(ctypes.c_size_t)),
     ]
 
 
@@ -106,6 +109,24 @@ assert CHARACTER_DTYPE.itemsize == ctypes.sizeof(PgxCharacter)
 
 class PgxError(RuntimeError):
     pass
+
+
+# Handles still open. They are closed by close(), by the garbage collector, or at the latest
+# by _close_open_handles, which atexit runs at the start of interpreter shutdown while the
+# HIP runtime and any profiler attached to it are fully alive. Destructors that run later,
+# during finalization, make no HIP call: the process is ending and the driver reclaims the
+# device memory.
+_open_handles = weakref.WeakSet()
+
+
+@atexit.register
+def _close_open_handles():
+    for obj in list(_open_handles):
+        obj.close()
+
+
+def _finalizing():
+    return sys.is_finalizing()
 
 
 _lib = None
@@ -250,10 +271,34 @@ def trimesh_cache(verts, tris):
     return buf.raw
 
 
+def trimesh_cache_validate(verts, tris, cache):
+    """pgx_trimesh_cache_validate: PGX_OK (0) or PGX_ERR_INVALID for `cache` against the mesh."""
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(tris, np.int32).reshape(-1, 3)
+    L = lib()
+    L.pgx_trimesh_cache_validate.argtypes = [F, ctypes.c_int, I32, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t]
+    return L.pgx_trimesh_cache_validate(_f(v), len(v), t.ctypes.data_as(I32), len(t), bytes(cache), len(cache))
+
+
 def device_count():
     n = ctypes.c_int(0)
     check(lib().pgx_device_count(ctypes.byref(n)))
     return n.value
+
+
+def sort_u64(keys, vals, bits, device=0):
+    """pgx_sort_u64: the library's stable radix sort of (keys, vals) by the low `bits` bits."""
+    keys = np.ascontiguousarray(keys, np.uint64)
+    vals = np.ascontiguousarray(vals, np.int32)
+    ko = np.zeros_like(keys)
+    vo = np.zeros_like(vals)
+    L = lib()
+    L.pgx_sort_u64.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), I32, ctypes.c_int, ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_uint64), I32]
+    P64 = ctypes.POINTER(ctypes.c_uint64)
+    check(L.pgx_sort_u64(device, keys.ctypes.data_as(P64), vals.ctypes.data_as(I32), len(keys), bits,
+                         ko.ctypes.data_as(P64), vo.ctypes.data_as(I32)))
+    return ko, vo
 
 
 def _f(a):
@@ -275,6 +320,7 @@ class World:
         h = ctypes.c_void_p()
         check(self.L.pgx_world_create(ctypes.byref(cfg), ctypes.byref(desc), ctypes.byref(h)))
         self.h = h
+        _open_handles.add(self)
         check(self.L.pgx_world_set_params(self.h, substeps, iterations, gravity))
         for e0, e1 in getattr(scene, "non_colliding", []):
             check(self.L.pgx_world_set_can_collide(self.h, e0, e1, 0))
@@ -283,12 +329,15 @@ class World:
             check(self.L.pgx_world_add_joints(self.h, len(scene.joints), arr))
 
     def close(self):
+        """pgx_world_destroy: waits for the world's stream and frees its device memory."""
         if getattr(self, "h", None):
             self.L.pgx_world_destroy(self.h)
             self.h = None
+        _open_handles.discard(self)
 
     def __del__(self):
-        self.close()
+        if not _finalizing():
+            self.close()
 
     def step(self, dt):
         check(self.L.pgx_world_step(self.h, dt))
@@ -567,15 +616,19 @@ class Server:
         h = ctypes.c_void_p()
         check(self.L.pgx_server_create(ctypes.byref(cfg), ctypes.byref(scfg), ctypes.byref(h)))
         self.h = h
+        _open_handles.add(self)
         self.scenes = {}
 
     def close(self):
+        """pgx_server_destroy: waits for the server's stream and frees its device memory."""
         if getattr(self, "h", None):
             self.L.pgx_server_destroy(self.h)
             self.h = None
+        _open_handles.discard(self)
 
     def __del__(self):
-        self.close()
+        if not _finalizing():
+            self.close()
 
     @staticmethod
     def footprint(scene):

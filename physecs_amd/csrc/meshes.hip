@@ -151,43 +151,73 @@ BuiltMesh build_mesh(const float* vp, int nv, const int32_t* ip, int nt) {
 
 // the cache's layout: header, then tris, triN, triC (nt each), nodes (2 per node)
 constexpr uint32_t MESH_MAGIC = 0x4d584750u;   // "PGXM"
+constexpr uint32_t MESH_VERSION = 2;
 struct MeshCacheHeader {
     uint32_t magic, version;
     int32_t nv, nt, nNodes, depth;
-    uint64_t hash;   // FNV-1a of the vertex and index bytes the cache was built from
+    uint64_t bytes;         // the whole cache, header included
+    uint64_t hash;          // FNV-1a of the vertex and index bytes the cache was built from
+    uint64_t payloadHash;   // FNV-1a of everything after the header
 };
 
-uint64_t mesh_hash(const float* vp, int nv, const int32_t* ip, int nt) {
-    uint64_t h = 1469598103934665603ull;
-    auto mix = [&](const void* p, size_t n) {
-        const unsigned char* b = static_cast<const unsigned char*>(p);
-        for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
-    };
-    mix(vp, 12 * (size_t)nv);
-    mix(ip, 12 * (size_t)nt);
+static uint64_t fnv(uint64_t h, const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) { h ^= b[i]; h *= 1099511628211ull; }
     return h;
+}
+constexpr uint64_t FNV_BASIS = 1469598103934665603ull;
+
+uint64_t mesh_hash(const float* vp, int nv, const int32_t* ip, int nt) {
+    return fnv(fnv(FNV_BASIS, vp, 12 * (size_t)nv), ip, 12 * (size_t)nt);
 }
 
 size_t cache_bytes(int nt, int nNodes) {
     return sizeof(MeshCacheHeader) + 16 * (3 * (size_t)nt + 2 * (size_t)nNodes);
 }
 
-// the mesh from a cache written for these vertices and indices (throws if it is not one)
-BuiltMesh load_mesh(const void* cache, const float* vp, int nv, const int32_t* ip, int nt) {
+// The mesh from a cache of `len` bytes written for these vertices and indices. Throws
+// PGX_ERR_INVALID unless the cache is one: its length, the hashes of the mesh it was built
+// from and of its payload, and the structure the device walks -- every internal node's
+// children inside the node list and after it, every leaf's triangle range inside the
+// triangles, every triangle's vertices inside the mesh, the depth recomputed from the nodes.
+BuiltMesh load_mesh(const void* cache, size_t len, const float* vp, int nv, const int32_t* ip, int nt) {
+    const auto bad = [](const char* why) {
+        return PgxError{PGX_ERR_INVALID, std::string("triangle-mesh cache rejected: ") + why};
+    };
     MeshCacheHeader h;
+    if (len < sizeof h) throw bad("shorter than its header");
     std::memcpy(&h, cache, sizeof h);
-    if (h.magic != MESH_MAGIC || h.version != 1 || h.nv != nv || h.nt != nt || h.nNodes < 1 || h.nNodes > 2 * nt ||
-        h.hash != mesh_hash(vp, nv, ip, nt))
-        throw PgxError{PGX_ERR_INVALID, "triangle-mesh cache was not built from this mesh"};
-    BuiltMesh m;
-    m.depth = h.depth;
+    if (h.magic != MESH_MAGIC || h.version != MESH_VERSION) throw bad("not a cache of this version");
+    if (h.nv != nv || h.nt != nt || h.hash != mesh_hash(vp, nv, ip, nt)) throw bad("built from another mesh");
+    if (h.nNodes < 1 || h.nNodes > 2 * nt) throw bad("node count out of range");
+    if (h.bytes != cache_bytes(nt, h.nNodes) || len != h.bytes) throw bad("length does not match its contents");
     const char* p = static_cast<const char*>(cache) + sizeof h;
+    if (fnv(FNV_BASIS, p, len - sizeof h) != h.payloadHash) throw bad("payload hash differs (corrupted)");
+    BuiltMesh m;
     auto take = [&](auto& vec, size_t n) {
         vec.resize(n);
         std::memcpy(vec.data(), p, sizeof(vec[0]) * n);
         p += sizeof(vec[0]) * n;
     };
     take(m.tris, nt); take(m.triN, nt); take(m.triC, nt); take(m.nodes, 2 * (size_t)h.nNodes);
+    for (const int4& t : m.tris)
+        if (t.x < 0 || t.x >= nv || t.y < 0 || t.y >= nv || t.z < 0 || t.z >= nv) throw bad("vertex index out of range");
+    auto asi = [](float f) { int i; std::memcpy(&i, &f, sizeof i); return i; };
+    std::vector<int> depth(h.nNodes, 0);
+    int maxDepth = 0;
+    for (int n = 0; n < h.nNodes; ++n) {
+        const int count = asi(m.nodes[2 * n].w), index = asi(m.nodes[2 * n + 1].w);
+        if (count < 0) throw bad("negative triangle count");
+        if (count == 0) {       // internal: children index and index + 1, stored after the node
+            if (index <= n || index + 1 >= h.nNodes) throw bad("child index out of range");
+            depth[index] = depth[index + 1] = depth[n] + 1;
+            maxDepth = std::max(maxDepth, depth[n] + 1);
+        } else if (index < 0 || (long)index + count > nt) {
+            throw bad("leaf triangle range out of range");
+        }
+    }
+    if (maxDepth != h.depth) throw bad("depth differs from its nodes");
+    m.depth = h.depth;
     return m;
 }
 
@@ -201,7 +231,10 @@ int upload_triangle_meshes(World& w, const pgx_scene_desc* sc, hipStream_t s) {
         const float* vp = sc->tri_verts + 3 * (size_t)v0;
         const int32_t* ip = sc->tri_idx + 3 * (size_t)t0;
         const void* cache = sc->tri_bvh_cache ? sc->tri_bvh_cache[m] : nullptr;
-        const BuiltMesh bm = cache ? load_mesh(cache, vp, nv, ip, nt) : build_mesh(vp, nv, ip, nt);
+        if (cache && !sc->tri_bvh_cache_bytes)
+            throw PgxError{PGX_ERR_INVALID, "tri_bvh_cache given without tri_bvh_cache_bytes"};
+        const BuiltMesh bm = cache ? load_mesh(cache, sc->tri_bvh_cache_bytes[m], vp, nv, ip, nt)
+                                   : build_mesh(vp, nv, ip, nt);
         maxDepth = std::max(maxDepth, bm.depth);
         int vbase = (int)verts.size(), tbase = (int)tris.size(), nbase = (int)nodes.size() / 2;
         meshes.push_back(make_int4(vbase, nv, tbase, nbase));
@@ -237,15 +270,29 @@ extern "C" int pgx_trimesh_cache(const float* verts, int n_verts, const int32_t*
         *bytes = cache_bytes(n_tris, nNodes);
         if (!out) return PGX_OK;
         if (cap < *bytes) return PGX_ERR_CAPACITY;
-        MeshCacheHeader h{MESH_MAGIC, 1, n_verts, n_tris, nNodes, m.depth, mesh_hash(verts, n_verts, tris, n_tris)};
-        char* p = static_cast<char*>(out);
-        std::memcpy(p, &h, sizeof h);
-        p += sizeof h;
+        MeshCacheHeader h{MESH_MAGIC, MESH_VERSION, n_verts, n_tris, nNodes, m.depth, *bytes,
+                          mesh_hash(verts, n_verts, tris, n_tris), 0};
+        char* p = static_cast<char*>(out) + sizeof h;
+        char* const payload = p;
         auto put = [&](const auto& vec) {
             std::memcpy(p, vec.data(), sizeof(vec[0]) * vec.size());
             p += sizeof(vec[0]) * vec.size();
         };
         put(m.tris); put(m.triN); put(m.triC); put(m.nodes);
+        h.payloadHash = fnv(FNV_BASIS, payload, (size_t)(p - payload));
+        std::memcpy(out, &h, sizeof h);
+        return PGX_OK;
+    } catch (...) {
+        return PGX_ERR_INVALID;
+    }
+}
+
+extern "C" int pgx_trimesh_cache_validate(const float* verts, int n_verts, const int32_t* tris, int n_tris,
+                                          const void* cache, size_t bytes) {
+    using namespace pgx;
+    if (!verts || !tris || !cache || n_verts < 3 || n_tris < 1) return PGX_ERR_INVALID;
+    try {
+        (void)load_mesh(cache, bytes, verts, n_verts, tris, n_tris);
         return PGX_OK;
     } catch (...) {
         return PGX_ERR_INVALID;

@@ -82,18 +82,18 @@ constexpr int TM_LANES = 16384;
 inline int div_up(long a, long b) { return (int)((a + b - 1) / b); }
 
 // ------------------------------------------------------------------ utilities (util.hip)
-struct SortTemp {
-    DBuf<uint8_t> temp;                  // library sort workspace
-    DBuf<uint64_t> kalt;
-    DBuf<int32_t> valt;
-};
-// Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys (8-bit digits).
-// The result is left in keys/vals.
-void radix_sort_u64(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t);
-
 struct ScanTemp {
     DBuf<int32_t> sums;
 };
+struct SortTemp {
+    DBuf<int32_t> hist;                  // per pass: digit x block counts
+    DBuf<uint64_t> kalt;
+    DBuf<int32_t> valt;
+    ScanTemp scan;
+};
+// Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys (8-bit digits, one
+// scatter launch per digit after one histogram launch). The result is left in keys/vals.
+void radix_sort_u64(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t);
 // out[i] = sum(in[0..i-1]); *total (device) = sum(in[0..n-1])
 void exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int n, int32_t* total, ScanTemp& t);
 

@@ -1,25 +1,178 @@
-// Device-wide primitives: stable radix sort of 64-bit keys with 32-bit values (rocPRIM's
-// onesweep radix sort through hipCUB, which is stable: equal keys keep their input order),
-// and an exclusive scan written for 64-lane CDNA wavefronts.
-#include <hipcub/hipcub.hpp>
+// Device-wide primitives: a stable LSD radix sort of 64-bit keys with 32-bit values, and an
+// exclusive scan, both written for 64-lane CDNA wavefronts.
 #include "pgx_internal.h"
 
 namespace pgx {
 
-void radix_sort_u64(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t) {
-    if (n <= 1 || bits <= 0) return;
+// ------------------------------------------------------------------ radix sort
+// One pass sorts by one digit of up to 8 bits. A block owns a tile of RS_THREADS * ITEMS
+// consecutive elements; wave w holds the tile's w-th run of 64 * ITEMS elements and walks it
+// in item order, so ranks follow input order and each pass is stable.
+//
+// Ranking: for item k, eight ballots over the digit bits give the lanes holding the same
+// digit; the lane's rank is the wave's running count of that digit (in LDS) plus the
+// same-digit lanes below it, and the lowest such lane adds the group's size to the count.
+// The wave counts are then scanned across the block's waves per digit.
+//
+// Offsets: H[d * nb + b] holds digit d's count in tile b (one table per pass). A block's
+// output offset for digit d is sum(H[d' < d][*]) + sum(H[d][b' < b]): the exclusive scan
+// of the table read digit-major. With few tiles each block sums its own from the table;
+// with many, the table is scanned in place first and the block reads its offsets. The
+// first pass's table comes from k_rs_hist; every scatter adds its outputs to the next
+// pass's table with global atomics (each output lands in the next pass's tile pos / TILE),
+// which k_rs_hist cleared.
+constexpr int RS_THREADS = 256;
+constexpr int RS_WAVES = RS_THREADS / WAVE;
+constexpr int RS_BINS = 256;
+
+__device__ __forceinline__ uint32_t rs_digit(uint64_t k, int shift, int dbits) {
+    return (uint32_t)(k >> shift) & ((1u << dbits) - 1u);
+}
+
+template <int ITEMS>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys, int n, int shift, int dbits,
+                                                        int nb, int32_t* __restrict__ H, int nextTables) {
+    __shared__ int32_t cnt[RS_BINS];
+    constexpr int TILE = RS_THREADS * ITEMS;
+    cnt[threadIdx.x] = 0;
+    // clear this tile's column of the later passes' tables
+    for (int t = 1; t <= nextTables; ++t) H[(size_t)t * RS_BINS * nb + (size_t)threadIdx.x * nb + blockIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * TILE;
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = base + k * RS_THREADS + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[rs_digit(keys[i], shift, dbits)], 1);
+    }
+    __syncthreads();
+    H[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
+}
+
+template <int ITEMS>
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin,
+                                                           uint64_t* __restrict__ kout, int32_t* __restrict__ vout, int n,
+                                                           int shift, int dbits, int nb, const int32_t* __restrict__ H,
+                                                           int scanned, int nshift, int ndbits,
+                                                           int32_t* __restrict__ Hnext) {
+    constexpr int TILE = RS_THREADS * ITEMS;
+    __shared__ int32_t waveCnt[RS_WAVES][RS_BINS];
+    __shared__ int32_t off[RS_BINS];
+    __shared__ int32_t wsum[RS_WAVES];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+    for (int w = 0; w < RS_WAVES; ++w) waveCnt[w][tid] = 0;
+    // this tile's offset for digit tid: all tiles' smaller digits + earlier tiles' digit tid
+    if (scanned) {
+        off[tid] = H[(size_t)tid * nb + blockIdx.x];
+        __syncthreads();
+    } else {
+        const int32_t* row = H + (size_t)tid * nb;
+        int32_t total = 0, pre = 0;
+        for (int b = 0; b < nb; ++b) {
+            const int32_t v = row[b];
+            total += v;
+            pre += b < (int)blockIdx.x ? v : 0;
+        }
+        int32_t x = total;
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const int32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == WAVE - 1) wsum[wv] = x;
+        __syncthreads();
+        int32_t before = 0;
+        for (int w = 0; w < RS_WAVES; ++w) before += w < wv ? wsum[w] : 0;
+        off[tid] = before + x - total + pre;
+    }
+    // the wave's elements in input order: item k covers [k * 64, k * 64 + 64) of its run
+    const int runBase = blockIdx.x * TILE + wv * WAVE * ITEMS;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint64_t key[ITEMS];
+    int32_t val[ITEMS];
+    int32_t rank[ITEMS];
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = runBase + k * WAVE + lane;
+        const bool ok = i < n;
+        key[k] = ok ? kin[i] : 0ull;
+        val[k] = ok ? vin[i] : 0;
+    }
+    for (int k = 0; k < ITEMS; ++k) {
+        const bool ok = runBase + k * WAVE + lane < n;
+        const uint32_t d = rs_digit(key[k], shift, dbits);
+        uint64_t same = __ballot(ok);
+        for (int b = 0; b < dbits; ++b) {
+            const uint64_t bal = __ballot((d >> b) & 1u);
+            same &= ((d >> b) & 1u) ? bal : ~bal;
+        }
+        const int32_t prior = waveCnt[wv][d];
+        rank[k] = prior + __popcll(same & lt);
+        if (ok && (same & lt) == 0ull) waveCnt[wv][d] = prior + __popcll(same);
+    }
+    __syncthreads();
+    {   // exclusive scan of the wave counts across waves, per digit
+        int32_t run = 0;
+        for (int w = 0; w < RS_WAVES; ++w) {
+            const int32_t c = waveCnt[w][tid];
+            waveCnt[w][tid] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int k = 0; k < ITEMS; ++k) {
+        if (runBase + k * WAVE + lane >= n) continue;
+        const uint32_t d = rs_digit(key[k], shift, dbits);
+        const int pos = off[d] + waveCnt[wv][d] + rank[k];
+        kout[pos] = key[k];
+        vout[pos] = val[k];
+        if (Hnext) atomicAdd(&Hnext[(size_t)rs_digit(key[k], nshift, ndbits) * nb + pos / TILE], 1);
+    }
+}
+
+__global__ void k_rs_copy(const uint64_t* __restrict__ ks, const int32_t* __restrict__ vs, uint64_t* __restrict__ kd,
+                          int32_t* __restrict__ vd, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        kd[i] = ks[i];
+        vd[i] = vs[i];
+    }
+}
+
+// up to this many tiles, every block sums its offsets from the histogram table itself
+constexpr int RS_SUM_TILES = 64;
+
+template <int ITEMS>
+static void radix_sort_tiles(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t) {
+    constexpr int TILE = RS_THREADS * ITEMS;
+    const int nb = div_up(n, TILE);
+    const int passes = div_up(bits, 8);
+    const bool scanned = nb > RS_SUM_TILES;
     t.kalt.alloc(n);
     t.valt.alloc(n);
-    hipcub::DoubleBuffer<uint64_t> dk(keys, t.kalt.p);
-    hipcub::DoubleBuffer<int32_t> dv(vals, t.valt.p);
-    size_t bytes = 0;
-    PGX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, dk, dv, n, 0, bits, s));
-    t.temp.alloc(std::max<size_t>(bytes, 1));
-    PGX_HIP(hipcub::DeviceRadixSort::SortPairs(t.temp.p, bytes, dk, dv, n, 0, bits, s));
-    if (dk.Current() != keys) {
-        PGX_HIP(hipMemcpyAsync(keys, dk.Current(), sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, s));
-        PGX_HIP(hipMemcpyAsync(vals, dv.Current(), sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s));
+    t.hist.alloc((size_t)passes * RS_BINS * nb);
+    auto dbitsOf = [&](int p) { return std::min(8, bits - 8 * p); };
+    hipLaunchKernelGGL(k_rs_hist<ITEMS>, dim3(nb), dim3(RS_THREADS), 0, s, keys, n, 0, dbitsOf(0), nb, t.hist.p,
+                       passes - 1);
+    uint64_t* kb[2] = {keys, t.kalt.p};
+    int32_t* vb[2] = {vals, t.valt.p};
+    for (int p = 0; p < passes; ++p) {
+        const bool last = p + 1 == passes;
+        int32_t* Hp = t.hist.p + (size_t)p * RS_BINS * nb;
+        int32_t* Hn = last ? nullptr : t.hist.p + (size_t)(p + 1) * RS_BINS * nb;
+        if (scanned) exclusive_scan_i32(s, Hp, Hp, RS_BINS * nb, nullptr, t.scan);
+        hipLaunchKernelGGL(k_rs_scatter<ITEMS>, dim3(nb), dim3(RS_THREADS), 0, s, kb[p & 1], vb[p & 1],
+                           kb[(p + 1) & 1], vb[(p + 1) & 1], n, 8 * p, dbitsOf(p), nb, Hp, scanned ? 1 : 0,
+                           last ? 0 : 8 * (p + 1), last ? 1 : dbitsOf(p + 1), Hn);
     }
+    if (passes & 1)
+        hipLaunchKernelGGL(k_rs_copy, dim3(div_up(n, 256)), dim3(256), 0, s, t.kalt.p, t.valt.p, keys, vals, n);
+    PGX_LAUNCH_CHECK();
+}
+
+void radix_sort_u64(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t) {
+    if (n <= 1 || bits <= 0) return;
+    bits = std::min(bits, 64);
+    // small inputs in small tiles (more blocks); the offset sums read one table row of nb
+    // entries per digit, which stays short with 4096-element tiles up to a few million keys
+    if (n <= 64 * 1024) radix_sort_tiles<4>(s, keys, vals, n, bits, t);
+    else radix_sort_tiles<16>(s, keys, vals, n, bits, t);
 }
 
 // ------------------------------------------------------------------ exclusive scan

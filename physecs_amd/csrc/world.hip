@@ -68,6 +68,37 @@ int pgx_device_count(int* count) {
     });
 }
 
+int pgx_sort_u64(int device, const uint64_t* keys, const int32_t* vals, int n, int bits, uint64_t* keys_out,
+                 int32_t* vals_out) {
+    if (n < 0 || bits < 0 || bits > 64) return fail(PGX_ERR_INVALID, "n must be >= 0 and bits in [0, 64]");
+    if (n > 0 && (!keys || !vals || !keys_out || !vals_out)) return fail(PGX_ERR_INVALID, "array is NULL");
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(device));
+        hipStream_t s = nullptr;
+        PGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } guard{s};
+        DBuf<uint64_t> k;
+        DBuf<int32_t> v;
+        SortTemp t;
+        k.alloc(n);
+        v.alloc(n);
+        if (n) {
+            PGX_HIP(hipMemcpyAsync(k.p, keys, sizeof(uint64_t) * n, hipMemcpyHostToDevice, s));
+            PGX_HIP(hipMemcpyAsync(v.p, vals, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+        }
+        radix_sort_u64(s, k.p, v.p, n, bits, t);
+        if (n) {
+            PGX_HIP(hipMemcpyAsync(keys_out, k.p, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, s));
+            PGX_HIP(hipMemcpyAsync(vals_out, v.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+        }
+        PGX_HIP(hipStreamSynchronize(s));
+        return PGX_OK;
+    });
+}
+
 }  // extern "C"
 
 static int validate_scene(const pgx_scene_desc* sc) {
@@ -459,7 +490,7 @@ static void release_world(World& w) {
     DBuf<uint64_t>* u64s[] = {&w.mkeys, &w.pkeys, &w.mfKey, &w.claim, &w.usedMask, &w.colorKeys, &w.cacheKey[0],
                               &w.cacheKey[1], &w.nonColliding, &w.sortT.kalt};
     for (auto* b : u64s) b->release();
-    w.bentity.release(); w.centity.release(); w.worldBoundsI.release(); w.sortT.temp.release();
+    w.bentity.release(); w.centity.release(); w.worldBoundsI.release(); w.sortT.hist.release(); w.sortT.scan.sums.release();
     w.nodeRange.release(); w.sSoft.release();
     DBuf<int32_t>* ji[] = {&w.jType, &w.jA0, &w.jA1, &w.jB0, &w.jB1, &w.jRowFlags, &w.jNRows, &w.jOrder};
     for (auto* b : ji) b->release();
@@ -854,8 +885,13 @@ void world_load_state(World& w, const void* buf, size_t bytes) {
     PGX_HIP(hipStreamSynchronize(w.stream));
     w.cacheCount[w.cacheCur] = h.cacheCount;
     w.trigCur = h.trigCur;
-    w.lastPairs = h.lastPairs; w.lastManifolds = h.lastManifolds; w.lastColors = h.lastColors;
-    w.lastRounds = h.lastRounds; w.lastPoints = h.lastPoints; w.lastMeshSlots = h.lastMeshSlots;
+    // The last step's pair, manifold and mesh-slot lists live in buffers the state does not
+    // carry: after a load there is no last step to report (counts and manifolds read empty
+    // until the next step). The colouring-round count only sizes the next step's first
+    // batch of rounds.
+    w.lastPairs = 0; w.lastManifolds = 0; w.lastColors = 0; w.lastPoints = 0; w.lastMeshSlots = 0;
+    w.lastPointsHost = 0;
+    w.lastRounds = std::max(0, std::min(h.lastRounds, 1024));
 }
 
 // Picks the launch shapes (and between per-colour launches and per-world passes) by

@@ -489,6 +489,13 @@ class Scene:
                     kw[name] = ctypes.cast(ptrs, ctype)
                 else:
                     kw[name] = None
+            elif name == "tri_bvh_cache_bytes":
+                caches = getattr(self, "tri_bvh_cache", None)
+                if caches:
+                    self._cache_lens = (ctypes.c_size_t * len(caches))(*[0 if c is None else len(c) for c in caches])
+                    kw[name] = ctypes.cast(self._cache_lens, ctype)
+                else:
+                    kw[name] = None
             else:
                 kw[name] = arr[name].ctypes.data_as(ctype)
         return cls(**kw)

@@ -11,7 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_symbols():
     txt = open(os.path.join(ROOT, "include", "physecs_gpu.h")).read()
-    return sorted(set(re.findall(r"\b(pgx_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(pgx_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():

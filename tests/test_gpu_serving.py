@@ -1,11 +1,12 @@
 """The served form: requests admitted while others run, stepped in the same launches,
 retired at their horizon. Each request's result is bit-identical to the same scene
 stepped alone for its horizon (continuing requests keep their device state and contact
-cache across admissions and retirements)."""
+cache across admissions and retirements), and that alone-run is stepped in lockstep with
+the oracle."""
 import numpy as np
 import pytest
 
-from gpu_util import bits, log
+from gpu_util import bits, lockstep, log
 from physecs_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -19,9 +20,9 @@ def api():
 
 
 def alone(api, scene, steps):
-    w = api.World(scene, substeps=4, iterations=2)
-    for _ in range(steps):
-        w.step(1 / 60)
+    """The request's scene stepped alone for its horizon, in lockstep with the oracle, so a
+    served result equal to it equals the oracle's too."""
+    w, _ = lockstep(scene, steps, check_every=5)
     return w.state()
 
 

@@ -91,3 +91,48 @@ def test_load_rejects_foreign_states(api):
     with pytest.raises(api.PgxError, match="INVALID"):
         a.load_state(b"\0" * len(blob))                   # not a state
     assert same(other.state(), before)
+
+
+def test_counts_and_manifolds_right_after_load(api):
+    """A load leaves no last step to report: counts and manifolds read empty until the next
+    step, whatever the loading world's own last step held, and the next step is the saver's."""
+    sc = scenes.mixed_lattice(4, side=8)
+    a = api.World(sc, substeps=4, iterations=2)
+    for _ in range(10):
+        a.step(DT)
+    blob = a.save_state()
+    a.step(DT)
+    want = a.state()
+    b = api.World(sc, substeps=4, iterations=2)
+    for _ in range(40):                                   # a fuller contact list than the saver's
+        b.step(DT)
+    assert b.counts()["manifolds"] > 0
+    b.load_state(blob)
+    cnt = b.counts()
+    assert cnt["pairs"] == 0 and cnt["manifolds"] == 0 and cnt["points"] == 0, cnt
+    keys, ints, f = b.manifolds()
+    assert len(keys) == 0 and len(ints) == 0 and len(f) == 0
+    b.step(DT)
+    assert same(b.state(), want)
+
+
+def test_load_from_a_world_with_larger_capacity(api):
+    """A state saved by a world with a larger pair (and so manifold) capacity loads where its
+    contact cache fits, and is rejected, with nothing changed, where it does not."""
+    sc = scenes.mixed_lattice(4, side=8)
+    big = api.World(sc, substeps=4, iterations=2, pair_capacity=1 << 16)
+    for _ in range(60):
+        big.step(DT)
+    blob = big.save_state()
+    n_cache = big.counts()["manifolds"]
+    big.step(DT)
+    want = big.state()
+    fits = api.World(sc, substeps=4, iterations=2, pair_capacity=max(4 * n_cache, 4096))
+    fits.load_state(blob)
+    fits.step(DT)
+    assert same(fits.state(), want)
+    small = api.World(sc, substeps=4, iterations=2, pair_capacity=max(1, n_cache // 4))
+    before = small.state()
+    with pytest.raises(api.PgxError, match="INVALID"):
+        small.load_state(blob)
+    assert same(small.state(), before)
