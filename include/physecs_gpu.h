@@ -257,12 +257,18 @@ int pgx_world_reset_stats(pgx_world* w);
  *                            solver velocities in LDS, its colours in order with
  *                            workgroup barriers. Needs each world's dynamic bodies to be
  *                            contiguous and to fit in LDS (5120 bodies on gfx950);
- *                            otherwise the step uses PGX_EXEC_COLORS. */
+ *                            otherwise the step uses PGX_EXEC_COLORS.
+ *   PGX_EXEC_PASS            one cooperative launch per pass, every colour in it with a
+ *                            grid barrier between colours; the solver velocities go
+ *                            through the device's coherence point, so a barrier needs no
+ *                            cache writeback. blocks = its grid size (0 = enough for the
+ *                            largest colour, at most one workgroup per CU). */
 #define PGX_EXEC_COLORS 0
 #define PGX_EXEC_COOPERATIVE 1
 #define PGX_EXEC_FLOW 2
 #define PGX_EXEC_FLOW_XCD 3
 #define PGX_EXEC_WORLDS 4
+#define PGX_EXEC_PASS 5
 int pgx_world_set_exec(pgx_world* w, int mode, int blocks);
 int pgx_world_get_exec(pgx_world* w, int* mode);
 int pgx_world_get_stats(pgx_world* w, pgx_stats* out);

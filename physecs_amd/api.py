@@ -419,13 +419,14 @@ class World:
     def enable_timing(self, on=True):
         check(self.L.pgx_world_enable_timing(self.h, 1 if on else 0))
 
-    EXEC_MODES = {"colors": 0, "cooperative": 1, "flow": 2, "xcd": 3, "worlds": 4}
+    EXEC_MODES = {"colors": 0, "cooperative": 1, "flow": 2, "xcd": 3, "worlds": 4, "pass": 5}
 
     def set_exec(self, mode="colors", blocks=0):
         """How the contact passes launch: "colors" (one launch per colour, the default),
         "flow" (one dependency-driven launch per pass), "xcd" (the same with each world's
-        manifolds on one XCD), "cooperative" (one launch per step) or "worlds" (one
-        workgroup per world per pass, bodies in LDS)."""
+        manifolds on one XCD), "cooperative" (one launch per step), "worlds" (one
+        workgroup per world per pass, bodies in LDS) or "pass" (one launch per pass, grid
+        barriers between colours)."""
         check(self.L.pgx_world_set_exec(self.h, self.EXEC_MODES[mode], blocks))
 
     def set_graph(self, enabled=True):

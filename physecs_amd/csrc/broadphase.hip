@@ -306,7 +306,15 @@ __global__ void k_world_stop(int W, const int32_t* __restrict__ worldRoot, const
     worldStop[wd] = (r & LEAF_BIT) ? leafEsc[r & ~LEAF_BIT] : nodeEsc[r];
 }
 
-// Traversal records: a visited node (or leaf) is one round of independent loads.
+// a child or escape link (a leaf carries LEAF_BIT, -1 ends the walk) as an index into the
+// pair walk's one array of n - 1 internal nodes followed by n leaves
+__device__ __forceinline__ int trav_index(int link, int n) {
+    return link < 0 ? -1 : (link & LEAF_BIT) ? n - 1 + (link & ~LEAF_BIT) : link;
+}
+
+// Traversal records: a visited node (or leaf) is one round of independent loads. The
+// queries walk nodeA/nodeB/leafA/leafB/leafI; the pair walk walks travA/travB/travC, and
+// tests the non-dynamic colliders from statA/statB/statL.
 __global__ void k_pack(int n, const float4* __restrict__ nmin, const float4* __restrict__ nmax,
                        const int32_t* __restrict__ child, const int32_t* __restrict__ nodeEsc,
                        const int2* __restrict__ range, const float4* __restrict__ lmin, const float4* __restrict__ lmax,
@@ -314,13 +322,20 @@ __global__ void k_pack(int n, const float4* __restrict__ nmin, const float4* __r
                        const int32_t* __restrict__ order, const uint32_t* __restrict__ centity,
                        const int32_t* __restrict__ clocal, float4* __restrict__ nodeA, float4* __restrict__ nodeB,
                        int32_t* __restrict__ nodeLast, float4* __restrict__ leafA, float4* __restrict__ leafB,
-                       int2* __restrict__ leafI) {
+                       int2* __restrict__ leafI, float4* __restrict__ travA, float4* __restrict__ travB,
+                       int4* __restrict__ travC, int nStatic, const int32_t* __restrict__ staticCols,
+                       const float4* __restrict__ cbmin, const float4* __restrict__ cbmax,
+                       const int32_t* __restrict__ cflags, float4* __restrict__ statA, float4* __restrict__ statB,
+                       int32_t* __restrict__ statL) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n - 1) {
         float4 a = nmin[t], b = nmax[t];
         nodeA[t] = make_float4(a.x, a.y, a.z, __int_as_float(child[2 * t]));
         nodeB[t] = make_float4(b.x, b.y, b.z, __int_as_float(nodeEsc[t]));
         nodeLast[t] = range[t].y;
+        travA[t] = make_float4(a.x, a.y, a.z, __int_as_float(trav_index(child[2 * t], n)));
+        travB[t] = make_float4(b.x, b.y, b.z, __int_as_float(trav_index(nodeEsc[t], n)));
+        travC[t] = make_int4(range[t].y, 0, 0, CF_ENABLE);
     }
     if (t < n) {
         float4 a = lmin[t], b = lmax[t];
@@ -328,6 +343,17 @@ __global__ void k_pack(int n, const float4* __restrict__ nmin, const float4* __r
         leafB[t] = make_float4(b.x, b.y, b.z, __int_as_float(lflags[t]));
         int c = order[t];
         leafI[t] = make_int2((int)centity[c], clocal[c]);
+        const float esc = __int_as_float(trav_index(leafEsc[t], n));
+        travA[n - 1 + t] = make_float4(a.x, a.y, a.z, esc);
+        travB[n - 1 + t] = make_float4(b.x, b.y, b.z, esc);
+        travC[n - 1 + t] = make_int4(t, (int)centity[c], clocal[c], lflags[t]);
+    }
+    if (t < nStatic) {
+        const int c = staticCols[t];
+        const float4 a = cbmin[c], b = cbmax[c];
+        statA[t] = make_float4(a.x, a.y, a.z, __int_as_float(cflags[c]));
+        statB[t] = make_float4(b.x, b.y, b.z, __int_as_float((int)centity[c]));
+        statL[t] = clocal[c];
     }
 }
 
@@ -340,19 +366,19 @@ __device__ __forceinline__ bool overlaps(float4 amin, float4 amax, float4 bmin, 
 
 constexpr int TRAV_SLOTS = 8;   // pairs a lane holds before its wave appends them
 
-// Each enabled dynamic leaf s walks its world's subtree over escape links and takes the
-// dynamic leaves after it (so each dynamic pair is emitted once), then tests the world's
-// non-dynamic colliders (static and kinematic; their bounds never move during a step).
-
+// Each enabled dynamic leaf s first tests its world's non-dynamic colliders (static and
+// kinematic; their bounds never move during a step), two per round of loads, then walks its
+// world's subtree over escape links and takes the dynamic leaves after it (so each dynamic
+// pair is emitted once). Every walk step is the same three loads of one record, internal
+// node or leaf alike, and one select of the next record: a record is taken when it holds a
+// leaf after s, is enabled and overlaps; a taken node leads to its left child, a taken leaf
+// is a pair; anything else leads over the escape link.
 __global__ __launch_bounds__(256) void k_traverse(
-    int n, const uint64_t* __restrict__ mkeys, int worldShift, const int32_t* __restrict__ order,
-    const float4* __restrict__ leafA, const float4* __restrict__ leafB, const int2* __restrict__ leafI,
-    const float4* __restrict__ nodeA, const float4* __restrict__ nodeB, const int32_t* __restrict__ nodeLast,
-    const int32_t* __restrict__ worldRoot, const int32_t* __restrict__ worldStop,
-    const int32_t* __restrict__ worldStaticStart, const int32_t* __restrict__ staticCols,
-    const float4* __restrict__ cbmin, const float4* __restrict__ cbmax, const int32_t* __restrict__ cflags,
-    const uint32_t* __restrict__ centity, const int32_t* __restrict__ clocal, int localBits,
-    int32_t* __restrict__ pairCount, int cap, uint64_t* __restrict__ pkeys) {
+    int n, const uint64_t* __restrict__ mkeys, int worldShift, const float4* __restrict__ travA,
+    const float4* __restrict__ travB, const int4* __restrict__ travC, const int32_t* __restrict__ worldRoot,
+    const int32_t* __restrict__ worldStop, const int32_t* __restrict__ worldStaticStart,
+    const float4* __restrict__ statA, const float4* __restrict__ statB, const int32_t* __restrict__ statL,
+    int localBits, int32_t* __restrict__ pairCount, int cap, uint64_t* __restrict__ pkeys) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     bool active = false;
     float4 qmin = make_float4(0.f, 0.f, 0.f, 0.f), qmax = qmin;
@@ -361,19 +387,20 @@ __global__ __launch_bounds__(256) void k_traverse(
     uint32_t myEnt = 0;
     int wd = 0;
     if (s < n) {
-        float4 a = leafA[s], b = leafB[s];
-        if (__float_as_int(b.w) & CF_ENABLE) {
+        const int self = n - 1 + s;
+        float4 a = travA[self], b = travB[self];
+        int4 c = travC[self];
+        if (c.w & CF_ENABLE) {
             active = true;
             qmin = a;
             qmax = b;
+            myEnt = (uint32_t)c.y;
+            myLocal = c.z;
             wd = (int)(mkeys[s] >> worldShift);
-            node = worldRoot[wd];
-            stop = worldStop[wd];
-            st = worldStaticStart[wd];   // then the world's non-dynamic colliders
+            node = trav_index(worldRoot[wd], n);
+            stop = trav_index(worldStop[wd], n);
+            st = worldStaticStart[wd];
             stEnd = worldStaticStart[wd + 1];
-            int2 me = leafI[s];
-            myEnt = (uint32_t)me.x;
-            myLocal = me.y;
         }
     }
     const int lane = threadIdx.x & (WAVE - 1);
@@ -396,9 +423,40 @@ __global__ __launch_bounds__(256) void k_traverse(
             if (base + j < cap) pkeys[base + j] = mine[j];
         held = 0;
     };
+    auto pairKey = [&](uint32_t e2, int l2) {
+        const int l0 = myEnt < e2 ? myLocal : l2;
+        const int l1 = myEnt < e2 ? l2 : myLocal;
+        return ((uint64_t)wd << (2 * localBits)) | ((uint64_t)l0 << localBits) | (uint64_t)l1;
+    };
+    // the world's non-dynamic colliders, two per round of loads
+    while (__ballot(active && st < stEnd)) {
+        bool e0 = false, e1 = false;
+        uint64_t k0 = 0, k1 = 0;
+        if (active && st < stEnd) {
+            const bool two = st + 1 < stEnd;
+            const float4 a0 = statA[st], b0 = statB[st];
+            const int l0 = statL[st];
+            float4 a1 = a0, b1 = b0;
+            int l1 = l0;
+            if (two) { a1 = statA[st + 1]; b1 = statB[st + 1]; l1 = statL[st + 1]; }
+            const uint32_t en0 = (uint32_t)__float_as_int(b0.w), en1 = (uint32_t)__float_as_int(b1.w);
+            if ((__float_as_int(a0.w) & CF_ENABLE) && en0 != myEnt && overlaps(qmin, qmax, a0, b0)) {
+                e0 = true;
+                k0 = pairKey(en0, l0);
+            }
+            if (two && (__float_as_int(a1.w) & CF_ENABLE) && en1 != myEnt && overlaps(qmin, qmax, a1, b1)) {
+                e1 = true;
+                k1 = pairKey(en1, l1);
+            }
+            st += two ? 2 : 1;
+        }
+        if (__ballot(held + 2 > TRAV_SLOTS)) flush();
+        if (e0) mine[held++] = k0;
+        if (e1) mine[held++] = k1;
+    }
     int visits = 0;
-    const int maxVisits = 2 * n;   // a depth-first walk visits at most 2n - 1 nodes
-    while (__ballot(active && (node != stop || st < stEnd))) {
+    const int maxVisits = 2 * n;   // a depth-first walk visits at most 2n - 1 records
+    while (__ballot(active && node != stop)) {
         bool emit = false;
         uint64_t key = 0;
         if (active && node != stop && ++visits > maxVisits) {
@@ -406,37 +464,15 @@ __global__ __launch_bounds__(256) void k_traverse(
             node = stop;
         }
         if (active && node != stop) {
-            if (node & LEAF_BIT) {
-                int s2 = node & ~LEAF_BIT;
-                float4 a = leafA[s2], b = leafB[s2];
-                if (s2 > s && (__float_as_int(b.w) & CF_ENABLE) && overlaps(qmin, qmax, a, b)) {
-                    int2 o = leafI[s2];
-                    uint32_t e2 = (uint32_t)o.x;
-                    if (e2 != myEnt) {
-                        int l0 = myEnt < e2 ? myLocal : o.y;
-                        int l1 = myEnt < e2 ? o.y : myLocal;
-                        key = ((uint64_t)wd << (2 * localBits)) | ((uint64_t)l0 << localBits) | (uint64_t)l1;
-                        emit = true;
-                    }
-                }
-                node = __float_as_int(a.w);
-            } else {
-                // subtrees whose leaves all sit at or before s hold no pair this leaf owns
-                float4 a = nodeA[node], b = nodeB[node];
-                int last = nodeLast[node];
-                node = (last > s && overlaps(qmin, qmax, a, b)) ? __float_as_int(a.w) : __float_as_int(b.w);
+            const float4 a = travA[node], b = travB[node];
+            const int4 c = travC[node];
+            // subtrees whose leaves all sit at or before s hold no pair this leaf owns
+            const bool take = c.x > s && (c.w & CF_ENABLE) && overlaps(qmin, qmax, a, b);
+            if (take && node >= n - 1 && (uint32_t)c.y != myEnt) {
+                key = pairKey((uint32_t)c.y, c.z);
+                emit = true;
             }
-        } else if (active && st < stEnd) {
-            int c2 = staticCols[st++];
-            if ((cflags[c2] & CF_ENABLE) && overlaps(qmin, qmax, cbmin[c2], cbmax[c2])) {
-                uint32_t e2 = centity[c2];
-                if (e2 != myEnt) {
-                    int l0 = myEnt < e2 ? myLocal : clocal[c2];
-                    int l1 = myEnt < e2 ? clocal[c2] : myLocal;
-                    key = ((uint64_t)wd << (2 * localBits)) | ((uint64_t)l0 << localBits) | (uint64_t)l1;
-                    emit = true;
-                }
-            }
+            node = take ? __float_as_int(a.w) : __float_as_int(b.w);
         }
         if (emit) mine[held++] = key;
         if (__ballot(held == TRAV_SLOTS)) flush();
@@ -490,10 +526,12 @@ void bp_build(World& w) {
         }
         hipLaunchKernelGGL(k_world_stop, dim3(div_up(W, 256)), dim3(256), 0, s, W, w.worldRoot.p, w.nodeEsc.p,
                            w.leafEsc.p, w.worldStop.p);
-        hipLaunchKernelGGL(k_pack, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.nodeMin.p, w.nodeMax.p, w.nodeChild.p,
-                           w.nodeEsc.p, w.nodeRange.p, w.leafMin.p, w.leafMax.p, w.leafEsc.p, w.leafFlags.p, w.mvals.p,
-                           w.centity.p, w.clocal.p, w.nodeA.p, w.nodeB.p, w.nodeLast.p, w.leafA.p, w.leafB.p,
-                           w.leafI.p);
+        const int nStatic = w.C - w.nDynCol;
+        hipLaunchKernelGGL(k_pack, dim3(div_up(std::max(n, nStatic), 256)), dim3(256), 0, s, n, w.nodeMin.p,
+                           w.nodeMax.p, w.nodeChild.p, w.nodeEsc.p, w.nodeRange.p, w.leafMin.p, w.leafMax.p,
+                           w.leafEsc.p, w.leafFlags.p, w.mvals.p, w.centity.p, w.clocal.p, w.nodeA.p, w.nodeB.p,
+                           w.nodeLast.p, w.leafA.p, w.leafB.p, w.leafI.p, w.travA.p, w.travB.p, w.travC.p, nStatic,
+                           w.staticCols.p, w.cbmin.p, w.cbmax.p, w.cflags.p, w.statA.p, w.statB.p, w.statL.p);
         PGX_LAUNCH_CHECK();
     }
 }
@@ -508,10 +546,9 @@ int bp_find_pairs(World& w) {
     fill_i32(s, w.counters.p, 0, 14);
     bp_build(w);
     if (n > 0) {
-        hipLaunchKernelGGL(k_traverse, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift,
-                           w.mvals.p, w.leafA.p, w.leafB.p, w.leafI.p, w.nodeA.p, w.nodeB.p, w.nodeLast.p,
-                           w.worldRoot.p, w.worldStop.p, w.worldStaticStart.p, w.staticCols.p, w.cbmin.p, w.cbmax.p,
-                           w.cflags.p, w.centity.p, w.clocal.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
+        hipLaunchKernelGGL(k_traverse, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift, w.travA.p,
+                           w.travB.p, w.travC.p, w.worldRoot.p, w.worldStop.p, w.worldStaticStart.p, w.statA.p,
+                           w.statB.p, w.statL.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
         PGX_LAUNCH_CHECK();
     }
     int cnt[8] = {0};

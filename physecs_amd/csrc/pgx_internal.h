@@ -64,7 +64,7 @@ constexpr int JOINT_MAX_ROWS = 7;      // a prismatic joint with its limit and d
 // (grid barriers between colours); one dependency-driven launch per pass
 // (one device-wide queue), the same with a queue per XCD
 // (one launch per pass); one workgroup per world per pass, bodies in LDS
-enum { EXEC_COLORS = 0, EXEC_COOPERATIVE = 1, EXEC_FLOW = 2, EXEC_FLOW_XCD = 3, EXEC_WORLDS = 4 };
+enum { EXEC_COLORS = 0, EXEC_COOPERATIVE = 1, EXEC_FLOW = 2, EXEC_FLOW_XCD = 3, EXEC_WORLDS = 4, EXEC_PASS = 5 };
 constexpr int FLOW_QUEUES = 8;         // queues of a dependency-driven pass: at most one per XCD
 constexpr int WORLD_SMALL = 256;       // per-world passes: worlds up to this many dynamic bodies share a launch
 // convex narrowphase: EPA polytope capacity per lane and lanes of the convex kernel
@@ -91,11 +91,24 @@ struct SortTemp {
     DBuf<int32_t> valt;
     ScanTemp scan;
 };
-// Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys (8-bit digits, one
-// scatter launch per digit after one histogram launch). The result is left in keys/vals.
+// Stable LSD radix sort of (keys, vals) by the low `bits` bits of keys (8-bit digits, a
+// histogram and a scatter launch per digit). The result is left in keys/vals.
 void radix_sort_u64(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t);
-// out[i] = sum(in[0..i-1]); *total (device) = sum(in[0..n-1])
+// The sort's scratch, in elements: alternate keys and values (n each), digit histograms,
+// scan block sums (0 when the histograms are summed in the scatter itself).
+struct RadixScratch {
+    size_t keys = 0, vals = 0, hist = 0, sums = 0;
+};
+RadixScratch radix_sort_scratch(int n, int bits);
+// The same sort with caller-provided scratch of radix_sort_scratch(n, bits) elements; it
+// needs no zeroed memory (every table it reads it has written first) and allocates nothing.
+void radix_sort_u64_into(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, uint64_t* kalt,
+                         int32_t* valt, int32_t* hist, int32_t* sums);
+// out[i] = sum(in[0..i-1]); *total (device) = sum(in[0..n-1]); in == out is allowed
 void exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int n, int32_t* total, ScanTemp& t);
+// the same with caller-provided block sums (scan_blocks(n) of them)
+int scan_blocks(int n);
+void exclusive_scan_i32_into(hipStream_t s, const int32_t* in, int32_t* out, int n, int32_t* total, int32_t* sums);
 
 void fill_i32(hipStream_t s, int32_t* p, int32_t v, int n);
 void fill_u64(hipStream_t s, uint64_t* p, uint64_t v, int n);
@@ -162,6 +175,14 @@ struct World {
     DBuf<int32_t> nodeLast;              // last leaf of the node's subtree
     DBuf<float4> leafA, leafB;           // {min, escape bits}, {max, flags bits}
     DBuf<int2> leafI;                    // {entity, world-local collider index}
+    // The pair walk's records: internal nodes [0, n-1), leaves [n-1, 2n-1) of one array, so
+    // every visit is the same three loads. A: {min, next if it overlaps}, B: {max, escape},
+    // C: {last leaf of the subtree (a leaf's own index), entity, world-local index, flags}.
+    DBuf<float4> travA, travB;
+    DBuf<int4> travC;
+    // the non-dynamic colliders in static-list order: {min, flags}, {max, entity}, local index
+    DBuf<float4> statA, statB;
+    DBuf<int32_t> statL;
     DBuf<int32_t> worldRoot, worldStop;
     DBuf<uint64_t> pkeys;
     DBuf<int32_t> pvals;
@@ -307,6 +328,8 @@ struct World {
     hipGraphExec_t subExec = nullptr;    // the last instantiated substep-loop graph
     long graphInstantiations = 0, graphUpdates = 0;
     int persistentBlocks = 0, persistentBlocksWanted = 0;
+    int passBlocksMax = 0;               // co-resident workgroups of the one-launch-per-pass kernel
+    DBuf<int32_t> passBarrier;           // its grid barriers' arrival count
     // dependency-driven passes
     int nXcd = 1;                        // XCDs of the device (CUs / 32)
     uint32_t flowEpoch = 0;              // per launch: versions of the published velocities

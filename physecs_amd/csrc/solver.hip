@@ -324,6 +324,7 @@ struct SubArgs {
     int32_t* tickets;
     int32_t* flowErr;
     int32_t* qDone;                       // manifolds finished per queue in this step
+    int32_t* barrier;                     // arrivals at the pass kernel's grid barriers (this substep)
 };
 
 __device__ __forceinline__ m3 world_inv_inertia(const SubArgs& a, int b) {
@@ -587,6 +588,7 @@ __device__ void write_back(const SubArgs& a, int d) {
 __global__ __launch_bounds__(256) void k_prepare(SubArgs a, int sub) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < (a.iterations + 1) * FLOW_QUEUES) a.tickets[i] = 0;
+    if (i == 0) a.barrier[0] = 0;
     if (sub == 0 && i < FLOW_QUEUES) a.qDone[i] = 0;
     if (i < a.M) prepare_manifold(a, i);
     else if (i - a.M < a.D) integrate_velocity(a, i - a.M);
@@ -595,6 +597,91 @@ template <int BLOCK, bool EAGER>
 __global__ __launch_bounds__(BLOCK) void k_solve(SubArgs a, int begin, int end, int useBias, float h, int skipSoft) {
     int s = begin + blockIdx.x * BLOCK + threadIdx.x;
     if (s < end) solve_manifold<EAGER>(a, s, useBias != 0, h, skipSoft != 0);
+}
+
+// ------------------------------------------------------------------ one launch per pass
+// k_solve_pass runs every colour of one contact pass in one launch, a grid barrier between
+// colours. Within it, the only data that passes from one colour to the next across CUs (and
+// XCDs, whose L2 caches are not coherent with each other) is the bodies' solver velocities:
+// they are read and written with relaxed agent-scope atomic accesses, which go to the
+// device's coherence point, so the barrier needs no cache writeback. Each thread's
+// accesses are complete (s_waitcnt 0) before its workgroup arrives; one lane per workgroup
+// adds the arrival and polls the count with agent-scope loads. Every other input of a
+// manifold is fixed for the pass and read with plain loads: a thread loads the next colour's
+// manifold before it waits at the barrier. Lambdas are read and written by the same thread
+// within the launch.
+constexpr int PASS_THREADS = 256;
+constexpr int PASS_SPIN_LIMIT = 1 << 22;   // ~seconds; never expected with co-resident blocks
+
+__device__ __forceinline__ f3 ld_agent3(const float4* p) {
+    uint32_t* q = reinterpret_cast<uint32_t*>(const_cast<float4*>(p));
+    const uint32_t x = __hip_atomic_load(q + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t y = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t z = __hip_atomic_load(q + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return mk3(__uint_as_float(x), __uint_as_float(y), __uint_as_float(z));
+}
+__device__ __forceinline__ void st_agent3(float4* p, f3 v) {
+    uint32_t* q = reinterpret_cast<uint32_t*>(p);
+    __hip_atomic_store(q + 0, __float_as_uint(v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, __float_as_uint(v.y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 2, __float_as_uint(v.z), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// the solve of one manifold whose fixed inputs are loaded, on velocities at the coherence point
+__device__ __forceinline__ void solve_coherent(const SubArgs& a, int s, const MInputs& in, bool useBias, float h,
+                                              bool skipSoft) {
+    if (skipSoft && in.soft) return;
+    f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
+    if (in.b0 >= 0) { v0 = ld_agent3(&a.bvel[bv(in.b0, BV_V)]); w0 = ld_agent3(&a.bvel[bv(in.b0, BV_W)]); }
+    if (in.b1 >= 0) { v1 = ld_agent3(&a.bvel[bv(in.b1, BV_V)]); w1 = ld_agent3(&a.bvel[bv(in.b1, BV_W)]); }
+    solve_core(a, s, in, useBias, h, v0, w0, v1, w1);
+    if (in.b0 >= 0) { st_agent3(&a.bvel[bv(in.b0, BV_V)], v0); st_agent3(&a.bvel[bv(in.b0, BV_W)], w0); }
+    if (in.b1 >= 0) { st_agent3(&a.bvel[bv(in.b1, BV_V)], v1); st_agent3(&a.bvel[bv(in.b1, BV_W)], w1); }
+}
+
+// grid barrier number `target` / gridDim.x of the substep: every workgroup has arrived
+// `target` times in all. A wait that runs out (or that finds another one ran out) sets the
+// error flag and lets the grid go on, so every wave still reaches the end of the kernel.
+__device__ __forceinline__ void pass_barrier(int32_t* count, int target, int32_t* err) {
+    __builtin_amdgcn_s_waitcnt(0);   // this thread's velocity stores are at the coherence point
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(count, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int spins = 0;
+        while (__hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > PASS_SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(PASS_THREADS) void k_solve_pass(SubArgs a, int nColors, int useBias, float h,
+                                                            int skipSoft, int barrierBase) {
+    const int tid = blockIdx.x * PASS_THREADS + threadIdx.x;
+    const int nth = gridDim.x * PASS_THREADS;
+    const bool bias = useBias != 0, skip = skipSoft != 0;
+    int s = a.colorStart[0] + tid, e = a.colorStart[1];
+    MInputs in;
+    bool have = s < e;
+    if (have) load_inputs<true>(a, s, in);
+    for (int c = 0; c < nColors; ++c) {
+        if (have) solve_coherent(a, s, in, bias, h, skip);
+        for (int s2 = s + nth; s2 < e; s2 += nth) {
+            MInputs in2;
+            load_inputs<true>(a, s2, in2);
+            solve_coherent(a, s2, in2, bias, h, skip);
+        }
+        if (c + 1 == nColors) break;
+        s = e + tid;                 // the next colour starts where this one ends
+        e = a.colorStart[c + 2];
+        have = s < e;
+        if (have) load_inputs<true>(a, s, in);
+        pass_barrier(a.barrier, barrierBase + (c + 1) * (int)gridDim.x, a.flowErr);
+    }
 }
 
 // Consecutive small colours c0..c1-1 in one launch of one workgroup: the colours are solved
@@ -920,7 +1007,8 @@ int sv_setup_manifolds(World& w, int nPairs) {
     PGX_LAUNCH_CHECK();
     PGX_HIP(hipStreamSynchronize(s));
     if (flowErr)
-        throw PgxError{PGX_ERR_HIP, "contact solve: a dependency wait of the previous step ran out"};
+        throw PgxError{PGX_ERR_HIP, "contact solve: a wait of the previous step ran out (dependency-driven pass or "
+                                    "grid barrier of the one-launch pass)"};
     for (int q = 0; q < lastQueues; ++q)
         if (qDone[q] != lastPasses * (qStart[q + 1] - qStart[q]))
             throw PgxError{PGX_ERR_HIP, "contact solve: queue " + std::to_string(q) + " of the previous step finished " +
@@ -1028,6 +1116,7 @@ static SubArgs make_args(World& w, int M, float dt) {
     a.bvel = w.bvel.p; a.dcom = w.dcom.p; a.dinvI = w.dinvI.p;
     a.pvel = w.pvel.p; a.pang = w.pang.p;
     a.used = w.usedMask.p; a.tickets = w.tickets.p; a.flowErr = w.flowErr.p; a.qDone = w.qDone.p;
+    a.barrier = w.passBarrier.p;
     return a;
 }
 
@@ -1040,6 +1129,19 @@ static int persistent_blocks(World& w) {
     int target = w.persistentBlocksWanted > 0 ? w.persistentBlocksWanted : cus;   // one block per CU by default
     w.persistentBlocks = std::max(1, std::min(target, perCU * cus));
     return w.persistentBlocks;
+}
+
+// Grid of the one-launch-per-pass kernel: enough threads for the largest colour in one
+// round, at most one workgroup per CU, and never more than can be co-resident.
+static int pass_blocks(World& w, int maxColour) {
+    if (w.passBlocksMax == 0) {
+        int perCU = 0, cus = 0;
+        PGX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void*)k_solve_pass, PASS_THREADS, 0));
+        PGX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, w.device));
+        w.passBlocksMax = std::max(1, std::min(cus, perCU * cus));
+    }
+    const int want = w.persistentBlocksWanted > 0 ? w.persistentBlocksWanted : div_up(std::max(maxColour, 1), PASS_THREADS);
+    return std::max(1, std::min(want, w.passBlocksMax));
 }
 
 // The start/stop events of one timed k_solve launch (timing mode, first substep of the
@@ -1165,9 +1267,32 @@ void sv_substeps(World& w, int M, float dt) {
         ++c;
     }
     int passIndex = 0;
+    const bool onePass = w.execMode == EXEC_PASS && M > 0 && nColors > 0;
+    int passGrid = 0;
+    if (onePass) {
+        int maxColour = 0;
+        for (int c = 0; c < nColors; ++c) maxColour = std::max(maxColour, w.colorStartHost[c + 1] - w.colorStartHost[c]);
+        passGrid = pass_blocks(w, maxColour);
+    }
     auto pass = [&](bool rec, int useBias, float h, int skipSoft) {
         const int p = passIndex++;
         if (M == 0) return;
+        if (onePass) {
+            // barriers of this substep so far: (nColors - 1) per earlier pass, each counted
+            // once per workgroup; the count is cleared by k_prepare
+            int base = p * (nColors - 1) * passGrid;
+            int nc = nColors;
+            void* args[] = {&a, &nc, &useBias, &h, &skipSoft, &base};
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (rec) {
+                solve_events(w, -1, &e0, &e1);
+                PGX_HIP(hipEventRecord(e0, s));
+            }
+            PGX_HIP(hipLaunchCooperativeKernel((const void*)k_solve_pass, dim3(passGrid), dim3(PASS_THREADS), args, 0,
+                                               s));
+            if (rec) PGX_HIP(hipEventRecord(e1, s));
+            return;
+        }
         if (flow) {
             f.pass = p; f.useBias = useBias; f.skipSoft = skipSoft; f.h = h;
             f.epoch = ++w.flowEpoch;

@@ -5,38 +5,49 @@
 namespace pgx {
 
 // ------------------------------------------------------------------ radix sort
-// One pass sorts by one digit of up to 8 bits. A block owns a tile of RS_THREADS * ITEMS
-// consecutive elements; wave w holds the tile's w-th run of 64 * ITEMS elements and walks it
-// in item order, so ranks follow input order and each pass is stable.
+// LSD radix sort, one pass per digit of up to 8 bits, each pass stable.
 //
-// Ranking: for item k, eight ballots over the digit bits give the lanes holding the same
-// digit; the lane's rank is the wave's running count of that digit (in LDS) plus the
-// same-digit lanes below it, and the lowest such lane adds the group's size to the count.
-// The wave counts are then scanned across the block's waves per digit.
+// Ranking (both paths): a wave walks its run of elements in item order; for item k, eight
+// ballots over the digit bits give the lanes holding the same digit; the lane's rank is the
+// wave's running count of that digit (in LDS) plus the same-digit lanes below it, and the
+// lowest such lane adds the group's size to the count. The wave counts are then scanned
+// across the block's waves per digit, so ranks follow input order.
 //
-// Offsets: H[d * nb + b] holds digit d's count in tile b (one table per pass). A block's
-// output offset for digit d is sum(H[d' < d][*]) + sum(H[d][b' < b]): the exclusive scan
-// of the table read digit-major. With few tiles each block sums its own from the table;
-// with many, the table is scanned in place first and the block reads its offsets. The
-// first pass's table comes from k_rs_hist; every scatter adds its outputs to the next
-// pass's table with global atomics (each output lands in the next pass's tile pos / TILE),
-// which k_rs_hist cleared.
+// Per pass, k_rs_hist counts each tile's digits in LDS into H, and k_rs_scatter gives tile
+// b's digit d the output offset sum(H[d' < d][*]) + sum(H[d][b' < b]). With up to
+// RS_SUM_TILES tiles, H is tile-major (H[b * 256 + d]) and each block sums its own offsets,
+// every thread loading its digit's count in every tile at once (coalesced across the
+// block's digits, one memory round trip); with more tiles, H is digit-major (H[d * nb + b]),
+// whose exclusive scan is the offset table, scanned in place before the scatter reads it.
 constexpr int RS_THREADS = 256;
 constexpr int RS_WAVES = RS_THREADS / WAVE;
 constexpr int RS_BINS = 256;
+// up to this many tiles, every block sums its offsets from the histogram table itself
+constexpr int RS_SUM_TILES = 64;
 
 __device__ __forceinline__ uint32_t rs_digit(uint64_t k, int shift, int dbits) {
     return (uint32_t)(k >> shift) & ((1u << dbits) - 1u);
 }
 
+// rank of the lane's element among the wave's elements of the same digit so far; bumps the
+// wave's count of the digit (counts: this wave's RS_BINS counters in LDS)
+__device__ __forceinline__ int32_t rs_rank(uint32_t d, bool ok, int dbits, uint64_t lt, int32_t* counts) {
+    uint64_t same = __ballot(ok);
+    for (int b = 0; b < dbits; ++b) {
+        const uint64_t bal = __ballot((d >> b) & 1u);
+        same &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    const int32_t prior = counts[d];
+    if (ok && (same & lt) == 0ull) counts[d] = prior + (int32_t)__popcll(same);
+    return prior + (int32_t)__popcll(same & lt);
+}
+
 template <int ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys, int n, int shift, int dbits,
-                                                        int nb, int32_t* __restrict__ H, int nextTables) {
+                                                        int nb, int32_t* __restrict__ H, int digitMajor) {
     __shared__ int32_t cnt[RS_BINS];
     constexpr int TILE = RS_THREADS * ITEMS;
     cnt[threadIdx.x] = 0;
-    // clear this tile's column of the later passes' tables
-    for (int t = 1; t <= nextTables; ++t) H[(size_t)t * RS_BINS * nb + (size_t)threadIdx.x * nb + blockIdx.x] = 0;
     __syncthreads();
     const int base = blockIdx.x * TILE;
     for (int k = 0; k < ITEMS; ++k) {
@@ -44,15 +55,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restri
         if (i < n) atomicAdd(&cnt[rs_digit(keys[i], shift, dbits)], 1);
     }
     __syncthreads();
-    H[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
+    if (digitMajor) H[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
+    else H[(size_t)blockIdx.x * RS_BINS + threadIdx.x] = cnt[threadIdx.x];
 }
 
 template <int ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __restrict__ kin, const int32_t* __restrict__ vin,
                                                            uint64_t* __restrict__ kout, int32_t* __restrict__ vout, int n,
                                                            int shift, int dbits, int nb, const int32_t* __restrict__ H,
-                                                           int scanned, int nshift, int ndbits,
-                                                           int32_t* __restrict__ Hnext) {
+                                                           int scanned) {
     constexpr int TILE = RS_THREADS * ITEMS;
     __shared__ int32_t waveCnt[RS_WAVES][RS_BINS];
     __shared__ int32_t off[RS_BINS];
@@ -64,12 +75,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
         off[tid] = H[(size_t)tid * nb + blockIdx.x];
         __syncthreads();
     } else {
-        const int32_t* row = H + (size_t)tid * nb;
+        // the digit's count in each of the nb <= RS_SUM_TILES tiles (tile-major), all loads
+        // issued before the sums
+        int32_t v[RS_SUM_TILES];
+#pragma unroll
+        for (int b = 0; b < RS_SUM_TILES; ++b) v[b] = b < nb ? H[(size_t)b * RS_BINS + tid] : 0;
         int32_t total = 0, pre = 0;
-        for (int b = 0; b < nb; ++b) {
-            const int32_t v = row[b];
-            total += v;
-            pre += b < (int)blockIdx.x ? v : 0;
+#pragma unroll
+        for (int b = 0; b < RS_SUM_TILES; ++b) {
+            total += v[b];
+            pre += b < (int)blockIdx.x ? v[b] : 0;
         }
         int32_t x = total;
         for (int o = 1; o < WAVE; o <<= 1) {
@@ -94,18 +109,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
         key[k] = ok ? kin[i] : 0ull;
         val[k] = ok ? vin[i] : 0;
     }
-    for (int k = 0; k < ITEMS; ++k) {
-        const bool ok = runBase + k * WAVE + lane < n;
-        const uint32_t d = rs_digit(key[k], shift, dbits);
-        uint64_t same = __ballot(ok);
-        for (int b = 0; b < dbits; ++b) {
-            const uint64_t bal = __ballot((d >> b) & 1u);
-            same &= ((d >> b) & 1u) ? bal : ~bal;
-        }
-        const int32_t prior = waveCnt[wv][d];
-        rank[k] = prior + __popcll(same & lt);
-        if (ok && (same & lt) == 0ull) waveCnt[wv][d] = prior + __popcll(same);
-    }
+    for (int k = 0; k < ITEMS; ++k)
+        rank[k] = rs_rank(rs_digit(key[k], shift, dbits), runBase + k * WAVE + lane < n, dbits, lt, waveCnt[wv]);
     __syncthreads();
     {   // exclusive scan of the wave counts across waves, per digit
         int32_t run = 0;
@@ -122,7 +127,6 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
         const int pos = off[d] + waveCnt[wv][d] + rank[k];
         kout[pos] = key[k];
         vout[pos] = val[k];
-        if (Hnext) atomicAdd(&Hnext[(size_t)rs_digit(key[k], nshift, ndbits) * nb + pos / TILE], 1);
     }
 }
 
@@ -135,44 +139,64 @@ __global__ void k_rs_copy(const uint64_t* __restrict__ ks, const int32_t* __rest
     }
 }
 
-// up to this many tiles, every block sums its offsets from the histogram table itself
-constexpr int RS_SUM_TILES = 64;
+// tiles: 512 keys up to 32 Ki keys, 1024 up to 64 Ki, 4096 above (small inputs in more
+// blocks: a pass is a short dependent chain per block)
+static int rs_items(int n) { return n <= 32 * 1024 ? 2 : n <= 64 * 1024 ? 4 : 16; }
+
+RadixScratch radix_sort_scratch(int n, int bits) {
+    RadixScratch r;
+    if (n <= 1 || bits <= 0) return r;
+    r.keys = (size_t)n;
+    r.vals = (size_t)n;
+    const int nb = div_up(n, RS_THREADS * rs_items(n));
+    r.hist = (size_t)RS_BINS * nb;
+    r.sums = nb > RS_SUM_TILES ? (size_t)scan_blocks(RS_BINS * nb) : 0;
+    return r;
+}
 
 template <int ITEMS>
-static void radix_sort_tiles(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t) {
+static void radix_sort_tiles(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, uint64_t* kalt,
+                             int32_t* valt, int32_t* hist, int32_t* sums) {
     constexpr int TILE = RS_THREADS * ITEMS;
     const int nb = div_up(n, TILE);
     const int passes = div_up(bits, 8);
     const bool scanned = nb > RS_SUM_TILES;
-    t.kalt.alloc(n);
-    t.valt.alloc(n);
-    t.hist.alloc((size_t)passes * RS_BINS * nb);
-    auto dbitsOf = [&](int p) { return std::min(8, bits - 8 * p); };
-    hipLaunchKernelGGL(k_rs_hist<ITEMS>, dim3(nb), dim3(RS_THREADS), 0, s, keys, n, 0, dbitsOf(0), nb, t.hist.p,
-                       passes - 1);
-    uint64_t* kb[2] = {keys, t.kalt.p};
-    int32_t* vb[2] = {vals, t.valt.p};
+    uint64_t* kb[2] = {keys, kalt};
+    int32_t* vb[2] = {vals, valt};
     for (int p = 0; p < passes; ++p) {
-        const bool last = p + 1 == passes;
-        int32_t* Hp = t.hist.p + (size_t)p * RS_BINS * nb;
-        int32_t* Hn = last ? nullptr : t.hist.p + (size_t)(p + 1) * RS_BINS * nb;
-        if (scanned) exclusive_scan_i32(s, Hp, Hp, RS_BINS * nb, nullptr, t.scan);
+        const int dbits = std::min(8, bits - 8 * p);
+        hipLaunchKernelGGL(k_rs_hist<ITEMS>, dim3(nb), dim3(RS_THREADS), 0, s, kb[p & 1], n, 8 * p, dbits, nb, hist,
+                           scanned ? 1 : 0);
+        if (scanned) exclusive_scan_i32_into(s, hist, hist, RS_BINS * nb, nullptr, sums);
         hipLaunchKernelGGL(k_rs_scatter<ITEMS>, dim3(nb), dim3(RS_THREADS), 0, s, kb[p & 1], vb[p & 1],
-                           kb[(p + 1) & 1], vb[(p + 1) & 1], n, 8 * p, dbitsOf(p), nb, Hp, scanned ? 1 : 0,
-                           last ? 0 : 8 * (p + 1), last ? 1 : dbitsOf(p + 1), Hn);
+                           kb[(p + 1) & 1], vb[(p + 1) & 1], n, 8 * p, dbits, nb, hist, scanned ? 1 : 0);
     }
     if (passes & 1)
-        hipLaunchKernelGGL(k_rs_copy, dim3(div_up(n, 256)), dim3(256), 0, s, t.kalt.p, t.valt.p, keys, vals, n);
+        hipLaunchKernelGGL(k_rs_copy, dim3(div_up(n, 256)), dim3(256), 0, s, kalt, valt, keys, vals, n);
     PGX_LAUNCH_CHECK();
 }
 
-void radix_sort_u64(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t) {
+void radix_sort_u64_into(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, uint64_t* kalt,
+                         int32_t* valt, int32_t* hist, int32_t* sums) {
     if (n <= 1 || bits <= 0) return;
     bits = std::min(bits, 64);
-    // small inputs in small tiles (more blocks); the offset sums read one table row of nb
-    // entries per digit, which stays short with 4096-element tiles up to a few million keys
-    if (n <= 64 * 1024) radix_sort_tiles<4>(s, keys, vals, n, bits, t);
-    else radix_sort_tiles<16>(s, keys, vals, n, bits, t);
+    if (rs_items(n) == 2) {
+        radix_sort_tiles<2>(s, keys, vals, n, bits, kalt, valt, hist, sums);
+    } else if (rs_items(n) == 4) {
+        radix_sort_tiles<4>(s, keys, vals, n, bits, kalt, valt, hist, sums);
+    } else {
+        radix_sort_tiles<16>(s, keys, vals, n, bits, kalt, valt, hist, sums);
+    }
+}
+
+void radix_sort_u64(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t) {
+    const RadixScratch r = radix_sort_scratch(n, bits);
+    if (!r.keys) return;
+    t.kalt.alloc(r.keys);
+    t.valt.alloc(r.vals);
+    t.hist.alloc(std::max<size_t>(r.hist, 1));
+    t.scan.sums.alloc(std::max<size_t>(r.sums, 1));
+    radix_sort_u64_into(s, keys, vals, n, bits, t.kalt.p, t.valt.p, t.hist.p, t.scan.sums.p);
 }
 
 // ------------------------------------------------------------------ exclusive scan
@@ -249,17 +273,23 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int32_t* __re
     }
 }
 
-void exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int n, int32_t* total, ScanTemp& t) {
+int scan_blocks(int n) { return div_up(std::max(n, 1), SCAN_TILE); }
+
+void exclusive_scan_i32_into(hipStream_t s, const int32_t* in, int32_t* out, int n, int32_t* total, int32_t* sums) {
     if (n <= 0) {
         if (total) fill_i32(s, total, 0, 1);
         return;
     }
-    int nb = div_up(n, SCAN_TILE);
-    t.sums.alloc(nb);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, t.sums.p);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_THREADS), 0, s, t.sums.p, nb, total);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, t.sums.p, out);
+    int nb = scan_blocks(n);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SCAN_THREADS), 0, s, sums, nb, total);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(SCAN_THREADS), 0, s, in, n, sums, out);
     PGX_LAUNCH_CHECK();
+}
+
+void exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int n, int32_t* total, ScanTemp& t) {
+    if (n > 0) t.sums.alloc(scan_blocks(n));
+    exclusive_scan_i32_into(s, in, out, n, total, t.sums.p);
 }
 
 __global__ void k_fill_i32(int32_t* p, int32_t v, int n) {

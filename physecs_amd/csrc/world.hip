@@ -312,6 +312,8 @@ void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh) {
     w.nodeCounter.alloc(ni); w.nodeRange.alloc(ni); w.nodeMin.alloc(ni); w.nodeMax.alloc(ni);
     w.nodeA.alloc(ni); w.nodeB.alloc(ni); w.nodeLast.alloc(ni);
     w.leafA.alloc(C); w.leafB.alloc(C); w.leafI.alloc(C);
+    w.travA.alloc(ni + C); w.travB.alloc(ni + C); w.travC.alloc(ni + C);
+    w.statA.alloc(std::max(1, C)); w.statB.alloc(std::max(1, C)); w.statL.alloc(std::max(1, C));
     w.worldRoot.alloc(W); w.worldStop.alloc(W); w.worldMStart.alloc(W);
     w.pairCap = (cfg && cfg->pair_capacity > 0) ? cfg->pair_capacity : std::max(1024, 12 * C);
     // manifolds of triangle-mesh pairs are stored after the pair slots
@@ -328,6 +330,7 @@ void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh) {
     w.mfSrc.alloc(M); w.mfTri.alloc(M); w.sWorld.alloc(M);
     w.claim.alloc(std::max(D, 1)); w.usedMask.alloc(2 * (size_t)std::max(D, 1));
     w.flowPub.alloc(6 * (size_t)std::max(D, 1));
+    w.passBarrier.alloc(1);
     // versions start at zero; a launch's versions carry its epoch, which starts at one
     PGX_HIP(hipMemsetAsync(w.flowPub.p, 0, sizeof(uint64_t) * 6 * std::max(D, 1), w.stream));
     w.flowErr.alloc(1);
@@ -1336,7 +1339,7 @@ int pgx_world_get_stats(pgx_world* h, pgx_stats* out) {
 int pgx_world_set_exec(pgx_world* h, int mode, int blocks) {
     if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
     if (blocks < 0) return fail(PGX_ERR_INVALID, "blocks must be >= 0");
-    if (mode < PGX_EXEC_COLORS || mode > PGX_EXEC_WORLDS) return fail(PGX_ERR_INVALID, "unknown execution mode %d", mode);
+    if (mode < PGX_EXEC_COLORS || mode > PGX_EXEC_PASS) return fail(PGX_ERR_INVALID, "unknown execution mode %d", mode);
     h->w.execMode = mode;
     h->w.persistentBlocksWanted = blocks;
     h->w.persistentBlocks = 0;

@@ -69,10 +69,31 @@ def test_served_request_lockstep(api, seed, n):
     assert g.counts()["manifolds"] > 0
 
 
-def test_mixed_32k_lockstep(api):
+@pytest.mark.parametrize("mode", ["colors", "pass"])
+def test_mixed_32k_lockstep(api, mode):
     sc = scenes.mixed_lattice(0, side=32)
-    log(f"[C2] mixed lattice: {sc.n_dyn} bodies, 12 steps")
-    g, _ = lockstep(sc, 12, check_every=4)
+    log(f"[C2] mixed lattice: {sc.n_dyn} bodies, 12 steps, exec {mode}")
+    g, _ = lockstep(sc, 12, check_every=4, mode=mode)
     cnt = g.counts()
     log(f"  counts {cnt}")
     assert sc.n_dyn == 32768 and cnt["manifolds"] > 1000
+
+
+def test_pass_mode_equals_colour_mode_settled(api):
+    """The 32,768-body scene well into its pile (about 20 colours of thousands of manifolds,
+    so many grid barriers per pass): one launch per pass equals one launch per colour bit
+    for bit at every checked step (the colour mode is the one the locksteps above pin)."""
+    sc = scenes.mixed_lattice(0, side=32)
+    log("[C2] pass vs colours, 150 steps")
+    a = api.World(sc, substeps=4, iterations=2)
+    b = api.World(sc, substeps=4, iterations=2)
+    b.set_exec("pass")
+    for k in range(150):
+        a.step(1 / 60)
+        b.step(1 / 60)
+        if (k + 1) % 25 == 0:
+            for x, y in zip(a.state(), b.state()):
+                assert np.array_equal(bits(x), bits(y)), k
+    cnt = b.counts()
+    log(f"  counts {cnt}")
+    assert cnt["colors"] >= 8 and cnt["manifolds"] > 20000
