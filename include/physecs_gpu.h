@@ -181,6 +181,13 @@ int pgx_device_count(int* count);
  * keys, shape-pair classes and colours. Host arrays in and out; runs on `device`. */
 int pgx_sort_u64(int device, const uint64_t* keys, const int32_t* vals, int n, int bits, uint64_t* keys_out,
                  int32_t* vals_out);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, serving loops that must not call hipMalloc): pgx_sort_u64 on
+ * device arrays, sorted in place, asynchronously on `stream` (a hipStream_t; NULL for the
+ * null stream), allocating nothing. The sort's scratch is a static 12.3 MiB array each GPU
+ * holds until the process ends, so n is at most 1048576 (PGX_ERR_INVALID beyond), and calls
+ * that overlap on one GPU race for it. */
+int pgx_sort_u64_noalloc(int device, uint64_t* keys, int32_t* vals, int n, int bits, void* stream);
 
 /* World lifetime. The descriptor is copied to the device; creation-time bounds carry no
  * margin (src/Physecs.cpp:32). cfg may be NULL for the defaults. */

@@ -26,7 +26,7 @@ ERRORS = {-1: "PGX_ERR_INVALID", -2: "PGX_ERR_HIP", -3: "PGX_ERR_CAPACITY", -4: 
 
 # every symbol declared in include/physecs_gpu.h
 EXPORTS = [
-    "pgx_last_error", "pgx_device_count", "pgx_sort_u64", "pgx_world_create", "pgx_world_destroy", "pgx_world_set_params",
+    "pgx_last_error", "pgx_device_count", "pgx_sort_u64", "pgx_sort_u64_noalloc", "pgx_world_create", "pgx_world_destroy", "pgx_world_set_params",
     "pgx_world_set_can_collide", "pgx_world_step", "pgx_world_sync", "pgx_world_read_state",
     "pgx_world_write_state", "pgx_world_get_counts", "pgx_world_get_bounds", "pgx_world_get_pairs",
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",

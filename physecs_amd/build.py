@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(OUT_DIR, "libphysecs_gpu.so")
-SOURCES = ["util.hip", "broadphase.hip", "narrowphase.hip", "solver.hip", "joints.hip", "meshes.hip", "world.hip", "server.hip", "queries.hip", "assets.hip"]
+SOURCES = ["util.hip", "broadphase.hip", "narrowphase.hip", "solver.hip", "joints.hip", "meshes.hip", "world.hip", "server.hip", "queries.hip", "assets.hip", "noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PGX_OFFLOAD_ARCH", "gfx950")
