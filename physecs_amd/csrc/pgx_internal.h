@@ -204,6 +204,8 @@ struct World {
     DBuf<int32_t> mfB0, mfB1, mfColor;
     DBuf<uint64_t> claim, claim2;        // D coloring claims, by round parity
     DBuf<int32_t> colorUncol;            // manifolds left uncoloured after each round
+    DBuf<int32_t> mfPrev;                // per manifold: its previous step's cache entry, or -1
+    DBuf<int32_t> colorFlag;             // set when inherited colours clash on a body
     DBuf<uint64_t> usedMask;             // 2 x D color masks
     DBuf<uint64_t> colorKeys;
     DBuf<int32_t> colorPerm;
@@ -283,7 +285,7 @@ struct World {
         int solveBlock = 64;             // threads per k_solve workgroup (64, 128, 256)
         int colorBatch = 0;              // runs of colours of at most this many manifolds: one launch
         int pairsPerLane = 1;            // narrowphase pairs per lane (1, 2, 4)
-        int colorRounds = 8;             // least colouring rounds queued per read-back
+        int colorRounds = 4;             // least colouring rounds queued per read-back
         int eagerMax = 32768;            // colours of at most this many manifolds load rows eagerly
     } tune;
     bool graph = false;                  // substep loop as one HIP graph launch (colour mode)

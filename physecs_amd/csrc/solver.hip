@@ -142,6 +142,59 @@ __global__ void k_color_round(int M, int round, const uint64_t* __restrict__ mfK
     if (b >= 0) { used[2 * b] |= bl; used[2 * b + 1] |= bh; }
 }
 
+// The previous step's colour of each manifold that persists (same pair key and triangle,
+// found in the previous step's contact cache as k_setup finds it), and its bodies' used
+// colours. Persisting manifolds were coloured properly among themselves last step, so
+// their colours stay valid unless a body changed from kinematic to dynamic in between: a
+// colour set twice on one body raises the clash flag, and k_color_clash then drops every
+// inherited colour (the step colours from scratch). Only new manifolds go through the
+// claim rounds.
+constexpr int CACHE_NUM_MASK = 255, CACHE_COLOR_SHIFT = 8, CACHE_HAS_COLOR = 0x8000;
+__global__ void k_color_inherit(int M, const uint64_t* __restrict__ mfKey, const int32_t* __restrict__ mfTri,
+                                const uint64_t* __restrict__ prevKey, const int32_t* __restrict__ prevTri,
+                                const int32_t* __restrict__ prevNum, int prevCount, const int32_t* __restrict__ b0,
+                                const int32_t* __restrict__ b1, int32_t* __restrict__ color,
+                                int32_t* __restrict__ mfPrev, unsigned long long* __restrict__ used,
+                                int32_t* __restrict__ clash) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const uint64_t key = mfKey[m];
+    const int tri = mfTri[m];
+    int prev = -1;
+    {   // the cache is in manifold order, so a pair's manifolds are one run of equal keys
+        int lo = 0, hi = prevCount;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (prevKey[mid] < key) lo = mid + 1; else hi = mid;
+        }
+        for (; lo < prevCount && prevKey[lo] == key; ++lo)
+            if (prevTri[lo] == tri) { prev = lo; break; }
+    }
+    mfPrev[m] = prev;
+    int c = -1;
+    if (prev >= 0) {
+        const int pn = prevNum[prev];
+        if (pn & CACHE_HAS_COLOR) c = (pn >> CACHE_COLOR_SHIFT) & 127;
+    }
+    color[m] = c;
+    if (c < 0) return;
+    const unsigned long long bit = 1ull << (c & 63);
+    const int word = c >> 6;
+    const int a = b0[m], b = b1[m];
+    bool twice = false;
+    if (a >= 0) twice |= (atomicOr(&used[2 * a + word], bit) & bit) != 0;
+    if (b >= 0) twice |= (atomicOr(&used[2 * b + word], bit) & bit) != 0;
+    if (twice) *clash = 1;
+}
+
+__global__ void k_color_clash(int M, int D, const int32_t* __restrict__ clash, int32_t* __restrict__ color,
+                              uint64_t* __restrict__ used) {
+    if (!*clash) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) color[i] = -1;
+    if (i < 2 * D) used[i] = 0;
+}
+
 __global__ void k_color_keys(int M, const int32_t* __restrict__ color, uint64_t* __restrict__ keys,
                              int32_t* __restrict__ vals) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -150,12 +203,17 @@ __global__ void k_color_keys(int M, const int32_t* __restrict__ color, uint64_t*
     vals[m] = m;
 }
 
-__global__ void k_color_start(int M, const uint64_t* __restrict__ keys, int32_t* __restrict__ start) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    int c = (int)keys[s];
-    if (s == 0 || (int)keys[s - 1] != c) start[c] = s;
-    if (s == M - 1) start[c + 1] = M;
+// start[c] = the first solve position with colour >= c, for every c in [0, n): colours with
+// no manifold (inherited colours leave gaps) get empty ranges
+__global__ void k_color_start(int M, int n, const uint64_t* __restrict__ keys, int32_t* __restrict__ start) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int lo = 0, hi = M;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (keys[mid] < (uint64_t)c) lo = mid + 1; else hi = mid;
+    }
+    start[c] = lo;
 }
 
 // ------------------------------------------------------------------ setup (solve order)
@@ -168,8 +226,8 @@ __global__ __launch_bounds__(256) void k_setup(
     const float4* __restrict__ npPts, const int32_t* __restrict__ mfColor, const int32_t* __restrict__ mfB0,
     const int32_t* __restrict__ mfB1, const float4* __restrict__ bpos, const float4* __restrict__ bquat,
     const float4* __restrict__ dvel, const float4* __restrict__ dang, const float4* __restrict__ dcom, int D,
-    const uint64_t* __restrict__ prevKey, const int32_t* __restrict__ prevTri, const int32_t* __restrict__ prevNum,
-    const float4* __restrict__ prevLp, int prevCount, int prevCap, uint64_t* __restrict__ curKey,
+    const int32_t* __restrict__ mfPrev, const int32_t* __restrict__ prevNum,
+    const float4* __restrict__ prevLp, int prevCap, uint64_t* __restrict__ curKey,
     int32_t* __restrict__ curTri, int32_t* __restrict__ curNum,
     float4* __restrict__ curLp, int32_t* __restrict__ sB0, int32_t* __restrict__ sB1, int32_t* __restrict__ sA0,
     int32_t* __restrict__ sA1, int32_t* __restrict__ sFlags, int32_t* __restrict__ sCanon, float4* __restrict__ sN,
@@ -205,18 +263,9 @@ __global__ __launch_bounds__(256) void k_setup(
     f3 com0 = s3(0.f), v0 = s3(0.f), w0 = s3(0.f), com1 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
     if (b0 >= 0) { com0 = add(xyz(bpos[a0]), qrot(q0, xyz(dcom[b0]))); v0 = xyz(dvel[b0]); w0 = xyz(dang[b0]); }
     if (b1 >= 0) { com1 = add(xyz(bpos[a1]), qrot(q1, xyz(dcom[b1]))); v1 = xyz(dvel[b1]); w1 = xyz(dang[b1]); }
-    // previous step's manifold of this (pair, triangle), src/Physecs.cpp:237: the cache is
-    // in manifold order, so a pair's manifolds are one run of equal keys
-    int prev = -1;
-    {
-        int lo = 0, hi = prevCount;
-        while (lo < hi) {
-            int mid = (lo + hi) >> 1;
-            if (prevKey[mid] < key) lo = mid + 1; else hi = mid;
-        }
-        for (; lo < prevCount && prevKey[lo] == key; ++lo)
-            if (prevTri[lo] == tri) { prev = lo; break; }
-    }
+    // previous step's manifold of this (pair, triangle), src/Physecs.cpp:237, as
+    // k_color_inherit found it
+    const int prev = mfPrev[m];
     qf iq0 = qinverse(q0), iq1 = qinverse(q1);
     for (int k = 0; k < 4; ++k) {
         float4 r0o = make_float4(0.f, 0.f, 0.f, 0.f), r1o = r0o;
@@ -230,7 +279,7 @@ __global__ __launch_bounds__(256) void k_setup(
             float tv = 0.f;
             bool found = false;
             if (prev >= 0) {
-                int pn = prevNum[prev];
+                int pn = prevNum[prev] & CACHE_NUM_MASK;
                 for (int i = 0; i < pn; ++i) {
                     float4 lp = prevLp[(size_t)i * prevCap + prev];
                     f3 d = sub(xyz(lp), r0);
@@ -248,7 +297,7 @@ __global__ __launch_bounds__(256) void k_setup(
     }
     curKey[m] = key;
     curTri[m] = tri;
-    curNum[m] = num;
+    curNum[m] = num | (mfColor[m] << CACHE_COLOR_SHIFT) | CACHE_HAS_COLOR;
     sB0[s] = b0;
     sB1[s] = b1;
     sA0[s] = a0;
@@ -1045,6 +1094,18 @@ int sv_color(World& w, int M) {
     w.claim2.alloc(std::max(w.D, 1));
     fill_u64(s, w.claim2.p, 0ull, w.D);
     unsigned long long* claims[2] = {(unsigned long long*)w.claim.p, (unsigned long long*)w.claim2.p};
+    // colours of the manifolds that persist from the previous step; only new ones are
+    // coloured in the rounds below
+    const int prevCache = w.cacheCur;
+    w.mfPrev.alloc(w.manifoldCap);
+    w.colorFlag.alloc(1);
+    fill_i32(s, w.colorFlag.p, 0, 1);
+    hipLaunchKernelGGL(k_color_inherit, dim3(nb), dim3(256), 0, s, M, w.mfKey.p, w.mfTri.p, w.cacheKey[prevCache].p,
+                       w.cacheTri[prevCache].p, w.cacheNum[prevCache].p, w.cacheCount[prevCache], w.mfB0.p, w.mfB1.p,
+                       w.mfColor.p,
+                       w.mfPrev.p, (unsigned long long*)w.usedMask.p, w.colorFlag.p);
+    hipLaunchKernelGGL(k_color_clash, dim3(div_up(std::max(M, 2 * w.D), 256)), dim3(256), 0, s, M, w.D, w.colorFlag.p,
+                       w.mfColor.p, w.usedMask.p);
     // round-1 claims
     hipLaunchKernelGGL(k_color_claim, dim3(nb), dim3(256), 0, s, M, 1, w.mfKey.p, worldShift, w.worldMStart.p,
                        w.mfColor.p, w.mfB0.p, w.mfB1.p, claims[1]);
@@ -1067,7 +1128,7 @@ int sv_color(World& w, int M) {
         // stable sort by colour -> solve order
         hipLaunchKernelGGL(k_color_keys, dim3(nb), dim3(256), 0, s, M, w.mfColor.p, w.colorKeys.p, w.colorPerm.p);
         radix_sort_u64(s, w.colorKeys.p, w.colorPerm.p, M, 8, w.sortT);
-        hipLaunchKernelGGL(k_color_start, dim3(nb), dim3(256), 0, s, M, w.colorKeys.p, w.colorCount.p);
+        hipLaunchKernelGGL(k_color_start, dim3(1), dim3(256), 0, s, M, 130, w.colorKeys.p, w.colorCount.p);
         PGX_LAUNCH_CHECK();
         int ovf = 0;
         uc.assign(round + 1, 0);
@@ -1094,8 +1155,8 @@ int sv_color(World& w, int M) {
                        w.mfKey.p,
                        w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.cflags.p, w.cmat.p, w.npNum.p,
                        w.npNormal.p, w.npPts.p, w.mfColor.p, w.mfB0.p, w.mfB1.p, w.bpos.p, w.bquat.p, w.dvel.p,
-                       w.dang.p, w.dcom.p, w.D, w.cacheKey[prv].p, w.cacheTri[prv].p, w.cacheNum[prv].p,
-                       w.cacheLp[prv].p, w.cacheCount[prv], w.manifoldCap, w.cacheKey[cur].p, w.cacheTri[cur].p,
+                       w.dang.p, w.dcom.p, w.D, w.mfPrev.p, w.cacheNum[prv].p, w.cacheLp[prv].p, w.manifoldCap,
+                       w.cacheKey[cur].p, w.cacheTri[cur].p,
                        w.cacheNum[cur].p, w.cacheLp[cur].p,
                        w.sB0.p, w.sB1.p, w.sA0.p, w.sA1.p, w.sFlags.p, w.sCanon.p, w.sN.p, w.sSoft.p, w.sR0.p,
                        w.sR1.p, w.sWorld.p);

@@ -1067,7 +1067,10 @@ void world_step(World& w, float dt) {
     }
     w.lastPairs = nPairs;
     w.lastManifolds = M;
-    w.lastColors = nColors;
+    // colours that hold manifolds (inherited colours can leave gaps below the highest)
+    int used = 0;
+    for (int c = 0; c < nColors; ++c) used += w.colorStartHost[c + 1] > w.colorStartHost[c] ? 1 : 0;
+    w.lastColors = used;
 }
 
 int report_error(int code, const std::string& msg) { return fail(code, "%s", msg.c_str()); }
