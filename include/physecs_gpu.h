@@ -194,6 +194,18 @@ int pgx_sort_u64_noalloc(int device, uint64_t* keys, int32_t* vals, int n, int b
 int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* scene, pgx_world** out);
 int pgx_world_destroy(pgx_world* w);
 int pgx_world_set_params(pgx_world* w, int num_substeps, int num_iterations, float gravity);
+/* Scene::setContactFilter (include/Physecs/Physecs.h:174,224; src/Physecs.cpp:200,795-796)
+ * for a filter function ContactType(*)(bool isTrigger0, int data0, bool isTrigger1, int
+ * data1), tabulated by the caller: collider_class[c] (descriptor order) in [0, n_classes)
+ * names each collider's (isTrigger, data) value, and table[k0 * n_classes + k1] != 0 is
+ * the filter returning TRIGGER for a pair whose lower entity's collider has class k0 and
+ * whose other collider has class k1 (the argument order of src/Physecs.cpp:200). TRIGGER
+ * pairs get the overlap test and enter/exit events instead of contacts. n_classes 0 or
+ * table NULL restores defaultContactFilter (a trigger on either side). Takes effect from
+ * the next step. A table that makes trigger pairs gives the world trigger lists, which
+ * its saved states then carry. */
+#define PGX_MAX_FILTER_CLASSES 1024
+int pgx_world_set_contact_filter(pgx_world* w, int n_classes, const uint8_t* table, const int32_t* collider_class);
 int pgx_world_set_can_collide(pgx_world* w, uint32_t entity0, uint32_t entity1, int can_collide);
 /* Appends joints in creation order. Each takes the lowest joint colour free on both of its
  * entities (8 colours, then the overflow colour), and the pair stops colliding. */

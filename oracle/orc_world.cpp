@@ -90,6 +90,7 @@ enum Mutation {
     MUT_JOINT_AXIS_ORDER = 9,       // Joints/*.cpp: angular row axis cross(a, b) instead of cross(b, a)
     MUT_JOINT_NO_COM = 10,          // Physecs.cpp:343-344: anchor not taken relative to the COM
     MUT_JOINT_UNIVERSAL_AXIS = 11,  // UniversalJoint.cpp:102-105: u1[1] instead of u1[2]
+    MUT_FILTER_ARG_ORDER = 12,      // Physecs.cpp:200: the filter called with entity1's collider first
 };
 static int g_mutation = MUT_NONE;
 struct Joint {
@@ -132,6 +133,12 @@ struct orc_world {
     std::vector<CC> ccs;
     std::unordered_map<Key, CacheData, KeyHash> cache, cacheTemp;
     std::unordered_set<uint64_t> nonColliding;
+    // Scene::contactFilter (include/Physecs/Physecs.h:174) tabulated over collider classes:
+    // filterTable[k0 * filterK + k1] != 0 is TRIGGER for (collider of entity0, collider of
+    // entity1); filterK = 0 is defaultContactFilter (src/Physecs.cpp:20-23)
+    int filterK = 0;
+    std::vector<uint8_t> filterTable;
+    std::vector<int> filterClass;
     // triggerCache / triggerCacheTemp (include/Physecs/Physecs.h:170-171) as collider pairs
     // (collider of entity0, collider of entity1), and the last step's events
     std::set<std::pair<int, int>> trig, trigTemp;
@@ -260,6 +267,13 @@ void orc_world_set_params(orc_world* w, int substeps, int iterations, float grav
 }
 
 // src/Physecs.cpp:788-793 (setCanCollide(e0, e1, false))
+void orc_world_set_contact_filter(orc_world* w, int k, const uint8_t* table, const int32_t* classes) {
+    if (k <= 0 || !table) { w->filterK = 0; return; }
+    w->filterK = k;
+    w->filterTable.assign(table, table + (size_t)k * k);
+    w->filterClass.assign(classes, classes + w->cols.size());
+}
+
 void orc_world_set_non_colliding(orc_world* w, uint32_t e0, uint32_t e1) {
     uint32_t a = e0 < e1 ? e0 : e1, b = e0 < e1 ? e1 : e0;
     w->nonColliding.insert(((uint64_t)a << 32) | b);
@@ -319,7 +333,13 @@ static void narrowTask(orc_world* w, int index, std::vector<CC>& outCC, std::vec
     Q or0 = qmul(t0.quat, col0.lquat);
     V3 pos1 = t1.pos + qrot(t1.quat, col1.lpos);
     Q or1 = qmul(t1.quat, col1.lquat);
-    if (col0.trigger || col1.trigger) {   // defaultContactFilter -> TRIGGER: overlap only (:199-206)
+    // contactFilter(col0.isTrigger, col0.data, col1.isTrigger, col1.data), col0 the collider of
+    // entity0 (src/Physecs.cpp:200); TRIGGER: overlap only (:199-206)
+    const int k0 = w->filterK ? w->filterClass[g_mutation == MUT_FILTER_ARG_ORDER ? cp.g1 : cp.g0] : 0;
+    const int k1 = w->filterK ? w->filterClass[g_mutation == MUT_FILTER_ARG_ORDER ? cp.g0 : cp.g1] : 0;
+    const bool trigger = w->filterK ? w->filterTable[(size_t)k0 * w->filterK + k1] != 0
+                                    : (col0.trigger || col1.trigger);
+    if (trigger) {
         if (overlapGeom(pos0, or0, col0.geom, pos1, or1, col1.geom)) {
             std::lock_guard<std::mutex> lk(w->mtx);
             w->trigTemp.insert({cp.g0, cp.g1});

@@ -26,7 +26,7 @@ ERRORS = {-1: "PGX_ERR_INVALID", -2: "PGX_ERR_HIP", -3: "PGX_ERR_CAPACITY", -4: 
 
 # every symbol declared in include/physecs_gpu.h
 EXPORTS = [
-    "pgx_last_error", "pgx_device_count", "pgx_sort_u64", "pgx_sort_u64_noalloc", "pgx_world_create", "pgx_world_destroy", "pgx_world_set_params",
+    "pgx_last_error", "pgx_device_count", "pgx_sort_u64", "pgx_sort_u64_noalloc", "pgx_world_create", "pgx_world_destroy", "pgx_world_set_params", "pgx_world_set_contact_filter",
     "pgx_world_set_can_collide", "pgx_world_step", "pgx_world_sync", "pgx_world_read_state",
     "pgx_world_write_state", "pgx_world_get_counts", "pgx_world_get_bounds", "pgx_world_get_pairs",
     "pgx_world_get_manifolds", "pgx_world_enable_timing", "pgx_world_get_timing", "pgx_world_reset_stats",
@@ -428,6 +428,20 @@ class World:
         workgroup per world per pass, bodies in LDS) or "pass" (one launch per pass, grid
         barriers between colours)."""
         check(self.L.pgx_world_set_exec(self.h, self.EXEC_MODES[mode], blocks))
+
+    def set_contact_filter(self, fn):
+        """Scene::setContactFilter: fn(is_trigger0, data0, is_trigger1, data1) -> True for
+        TRIGGER, tabulated over the scene's colliders (scenes.contact_filter_table); None
+        restores the default filter."""
+        L = self.L
+        L.pgx_world_set_contact_filter.argtypes = [ctypes.c_void_p, ctypes.c_int, U8, I32]
+        if fn is None:
+            check(L.pgx_world_set_contact_filter(self.h, 0, None, None))
+            return
+        from physecs_amd import scenes
+        k, table, classes = scenes.contact_filter_table(self.scene, fn)
+        self._filter = (table, classes)
+        check(L.pgx_world_set_contact_filter(self.h, k, table.ctypes.data_as(U8), classes.ctypes.data_as(I32)))
 
     def set_graph(self, enabled=True):
         """Launch each step's substep loop as one HIP graph (colour mode); same results."""

@@ -702,7 +702,8 @@ __global__ __launch_bounds__(256) void k_narrowphase(
     int32_t* __restrict__ npNum, float4* __restrict__ npNormal, float4* __restrict__ npPts,
     int32_t* __restrict__ heavy, int32_t* __restrict__ nHeavy, int32_t* __restrict__ meshPairs,
     int32_t* __restrict__ nMesh, int32_t* __restrict__ meshCount, int32_t* __restrict__ trigList,
-    int32_t* __restrict__ nTrig) {
+    int32_t* __restrict__ nTrig, const int32_t* __restrict__ filterClass, const uint8_t* __restrict__ filterTable,
+    int filterK) {
     // pairs per lane = pairs / (grid size): a lane takes every grid-size'th pair of the
     // class order; results are stored at the pair's own slot
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nPairs; i += gridDim.x * blockDim.x) {
@@ -715,9 +716,11 @@ __global__ __launch_bounds__(256) void k_narrowphase(
         int c0 = colOfLocal[base + (int)((key >> localBits) & lmask)];
         int c1 = colOfLocal[base + (int)(key & lmask)];
         int num = 0;
-        // defaultContactFilter: a trigger on either side makes a trigger pair, not a contact
-        // (src/Physecs.cpp:20-23,200-207); joint-connected / setCanCollide pairs are skipped (:209)
-        bool skip = ((cflags[c0] | cflags[c1]) & CF_TRIGGER) != 0;
+        // the contact filter on (collider of the lower entity, other collider): TRIGGER makes
+        // a trigger pair, not a contact (src/Physecs.cpp:200-207); defaultContactFilter is a
+        // trigger on either side (:20-23). Joint-connected / setCanCollide pairs are skipped (:209)
+        bool skip = filterK ? filterTable[filterClass[c0] * filterK + filterClass[c1]] != 0
+                            : ((cflags[c0] | cflags[c1]) & CF_TRIGGER) != 0;
         // trigger pairs go to k_triggers (before the non-colliding test, as in the reference)
         const bool toTrig = skip && trigList != nullptr;
         if (__ballot(toTrig)) {
@@ -1517,7 +1520,8 @@ void np_run(World& w, int nPairs) {
                        w.cparams.p, w.clpos.p, w.clquat.p, w.centity.p, w.bpos.p, w.bquat.p, w.nonColliding.p,
                        w.nNonColliding, w.npNum.p, w.npNormal.p, w.npPts.p, w.heavy.p, w.counters.p + 8,
                        w.meshPairs.p, w.counters.p + 10, w.meshCount.p,
-                       w.hasTriggers ? w.trigList.p : nullptr, w.trigN.p + 2 * w.trigCur);
+                       w.hasTriggers ? w.trigList.p : nullptr, w.trigN.p + 2 * w.trigCur, w.filterClass.p,
+                       w.filterTable.p, w.filterK);
     PGX_LAUNCH_CHECK();
     if (w.hasTriggers) {
         ConvexView cv{w.cvVerts.p, w.cvMesh.p, w.cvFace.p, w.cvFaceIdx.p, w.cvFaceN.p, w.cvFaceC.p};

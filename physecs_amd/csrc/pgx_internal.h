@@ -310,6 +310,13 @@ struct World {
     DBuf<int32_t> trigList, trigN;
     DBuf<uint64_t> trigKeys[2];
     std::vector<int32_t> hostColOfLocal, hostWorldColStart;
+    // contact filter (Scene::setContactFilter), tabulated: filterK classes of colliders
+    // (their distinct (isTrigger, data)), filterTable[k0 * filterK + k1] != 0 makes a pair of
+    // classes (collider of the lower entity first) a trigger pair; filterK = 0: the default
+    // filter (a trigger on either side)
+    int filterK = 0;
+    DBuf<int32_t> filterClass;
+    DBuf<uint8_t> filterTable;
     // overlap queries: Collider::data (descriptor order), query shapes, per-query hit
     // counts and offsets, the hit keys
     DBuf<int32_t> cdata;
@@ -458,6 +465,7 @@ void world_step(World& w, float dt);
 void world_release_graph(World& w);
 void world_autotune(World& w, float dt, int trialSteps);
 int report_error(int code, const std::string& msg);   // sets pgx_last_error(), returns code
+void world_enable_triggers(World& w);                  // allocates the trigger lists (once)
 
 // solver.hip
 int sv_setup_manifolds(World& w, int nPairs);

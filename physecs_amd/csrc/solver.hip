@@ -486,9 +486,10 @@ __device__ __forceinline__ m3 body_inertia(const SubArgs& a, int b) {
     return mk_m3(xyz(a.bvel[bv(b, BV_I0)]), xyz(a.bvel[bv(b, BV_I1)]), xyz(a.bvel[bv(b, BV_I2)]));
 }
 
-// EAGER loads the rows of all four point slots at once, without waiting for the point
-// count (slots beyond it hold stale data and are not used): one dependent memory round
-// trip less for colours too small to be bandwidth-bound, at the cost of unused loads.
+// EAGER loads point slot 0's rows (every manifold has a first point) with the manifold's
+// flags and body indices, without waiting for the point count; slots 1-3 wait for the
+// count, in the same round of loads as the body records (which wait for the body indices
+// anyway), so unused slots cost no traffic and the solve still takes two dependent rounds.
 template <bool EAGER = false>
 __device__ __forceinline__ void load_inputs(const SubArgs& a, int s, MInputs& in) {
     int flags = a.sFlags[s];
@@ -517,7 +518,7 @@ __device__ __forceinline__ void load_inputs(const SubArgs& a, int s, MInputs& in
     // parallelism); unused point slots are never read back
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        if (EAGER || k < in.num) {
+        if ((EAGER && k == 0) || k < in.num) {
             in.rA[k] = ROW(R_R0XN, k); in.rB[k] = ROW(R_R1XN, k); in.rT[k] = ROW(R_T, k);
             in.rC[k] = ROW(R_R0XT, k); in.rE[k] = ROW(R_R1XT, k);
             in.lam[k] = LAM(k);

@@ -383,15 +383,12 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     world_alloc_derived(w, cfg, sc->n_trimesh);
     // trigger lists: at most one entry per pair slot (pairCap is set by world_alloc_derived)
     w.hasTriggers = false;
-    for (int c = 0; c < o.C && sc->col_trigger; ++c) w.hasTriggers = w.hasTriggers || sc->col_trigger[c] != 0;
+    w.hostColOfLocal = o.colOfLocal;
+    w.hostWorldColStart = o.wstart;
+    bool anyTrigger = false;
+    for (int c = 0; c < o.C && sc->col_trigger; ++c) anyTrigger = anyTrigger || sc->col_trigger[c] != 0;
     w.trigCur = 0;
-    if (w.hasTriggers) {
-        w.trigList.alloc(w.pairCap); w.trigKeys[0].alloc(w.pairCap); w.trigKeys[1].alloc(w.pairCap);
-        w.trigN.alloc(4);
-        fill_i32(s, w.trigN.p, 0, 4);
-        w.hostColOfLocal = o.colOfLocal;
-        w.hostWorldColStart = o.wstart;
-    }
+    if (anyTrigger) world_enable_triggers(w);
     if (sc->n_convex > 0) {        // ConvexMesh vertex buffers padded to a multiple of 4 with the last vertex
         // (src/ConvexMesh.cpp:3-11); faces, normals and centroids as given
         std::vector<float4> cvv, cvn, cvc;
@@ -448,6 +445,16 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     // creation-time bounds: no margin (src/Physecs.cpp:32)
     bp_compute_bounds(w, false);
     PGX_HIP(hipStreamSynchronize(s));
+}
+
+// trigger lists: at most one entry per pair slot (pairCap is set by world_alloc_derived)
+void world_enable_triggers(World& w) {
+    if (w.hasTriggers) return;
+    w.trigList.alloc(w.pairCap); w.trigKeys[0].alloc(w.pairCap); w.trigKeys[1].alloc(w.pairCap);
+    w.trigN.alloc(4);
+    fill_i32(w.stream, w.trigN.p, 0, 4);
+    w.trigCur = 0;
+    w.hasTriggers = true;
 }
 
 }  // namespace pgx
@@ -521,6 +528,40 @@ int pgx_world_destroy(pgx_world* h) {
     release_world(h->w);
     delete h;
     return rc;
+}
+
+int pgx_world_set_contact_filter(pgx_world* h, int n_classes, const uint8_t* table, const int32_t* collider_class) {
+    if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
+    World& w = h->w;
+    if (n_classes == 0 || !table) {
+        w.filterK = 0;
+        return PGX_OK;
+    }
+    if (n_classes < 0 || n_classes > PGX_MAX_FILTER_CLASSES)
+        return fail(PGX_ERR_INVALID, "n_classes must be in [0, %d]", PGX_MAX_FILTER_CLASSES);
+    if (!collider_class && w.C > 0) return fail(PGX_ERR_INVALID, "collider_class is NULL");
+    for (int c = 0; c < w.C; ++c)
+        if (collider_class[c] < 0 || collider_class[c] >= n_classes)
+            return fail(PGX_ERR_INVALID, "collider %d has class %d outside [0, %d)", c, collider_class[c], n_classes);
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(w.device));
+        const size_t nt = (size_t)n_classes * n_classes;
+        std::vector<uint8_t> t(table, table + nt);
+        bool anyTrigger = false;
+        for (uint8_t& v : t) { v = v ? 1 : 0; anyTrigger = anyTrigger || v; }
+        // colliders keep descriptor order on the device
+        std::vector<int32_t> cls(std::max(w.C, 1), 0);
+        for (int c = 0; c < w.C; ++c) cls[c] = collider_class[c];
+        w.filterClass.alloc(cls.size());
+        w.filterTable.alloc(nt);
+        PGX_HIP(hipMemcpyAsync(w.filterClass.p, cls.data(), sizeof(int32_t) * cls.size(), hipMemcpyHostToDevice,
+                               w.stream));
+        PGX_HIP(hipMemcpyAsync(w.filterTable.p, t.data(), nt, hipMemcpyHostToDevice, w.stream));
+        if (anyTrigger) world_enable_triggers(w);
+        PGX_HIP(hipStreamSynchronize(w.stream));
+        w.filterK = n_classes;
+        return PGX_OK;
+    });
 }
 
 int pgx_world_set_params(pgx_world* h, int substeps, int iterations, float gravity) {

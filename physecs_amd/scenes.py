@@ -527,6 +527,25 @@ class Scene:
 
 
 # ---------------------------------------------------------------- configurations
+def contact_filter_table(scene, fn):
+    """A contact filter fn(is_trigger0, data0, is_trigger1, data1) -> True for TRIGGER (the
+    reference's ContactType (*)(bool, int, bool, int), include/Physecs/Physecs.h:43,174),
+    tabulated over the scene's distinct (isTrigger, data) values: (classes, table, per-
+    collider class) for pgx_world_set_contact_filter. Argument 0 is the collider of the
+    pair's lower entity, as src/Physecs.cpp:200 calls it."""
+    arr = scene.arrays()
+    keys = list(zip([bool(t) for t in arr["col_trigger"].tolist()], [int(d) for d in arr["col_data"].tolist()]))
+    distinct = sorted(set(keys))
+    index = {k: i for i, k in enumerate(distinct)}
+    classes = np.array([index[k] for k in keys], np.int32)
+    k = len(distinct)
+    table = np.zeros((k, k), np.uint8)
+    for i, (t0, d0) in enumerate(distinct):
+        for j, (t1, d1) in enumerate(distinct):
+            table[i, j] = 1 if fn(t0, d0, t1, d1) else 0
+    return k, np.ascontiguousarray(table), classes
+
+
 def box_pyramid(seed=0, rows=20, jitter=0.01):
     """20-layer box pyramid: 210 dynamic 1x1x1 m boxes (density 1000, friction 0.6,
     restitution 0) on a static 100x1x100 m ground box; per-box xz jitter U(-j, j)."""

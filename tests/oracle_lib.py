@@ -177,6 +177,19 @@ class OracleWorld:
             self.lib.orc_world_add_joints.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(OrcJoint)]
             self.lib.orc_world_add_joints(self.h, len(scene.joints), arr)
 
+    def set_contact_filter(self, fn):
+        """The oracle's contactFilter, tabulated as for the device (scenes.contact_filter_table)."""
+        from physecs_amd import scenes
+        self.lib.orc_world_set_contact_filter.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                                          ctypes.POINTER(ctypes.c_uint8), I32]
+        if fn is None:
+            self.lib.orc_world_set_contact_filter(self.h, 0, None, None)
+            return
+        k, table, classes = scenes.contact_filter_table(self.scene, fn)
+        self._filter = (table, classes)
+        self.lib.orc_world_set_contact_filter(self.h, k, table.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                              iptr(classes))
+
     def set_joint_params(self, index, **params):
         """The joint's setters on the oracle (see api.World.set_joint_params)."""
         from physecs_amd import scenes
@@ -352,7 +365,7 @@ def row_solve(which, flags, b0, b1, inp, masses, vel, pv, h, iters):
 # a pin fails on a plausible misreading of the reference.
 MUT_SIMD_WARM_LOCAL, MUT_SIMD_LIMIT_UNSCALED, MUT_SIMD_MASK_NONSTRICT, MUT_SIMD_BIAS_FLOAT = 1, 2, 3, 4
 MUT_SIMD_ZERO_WRITEBACK, MUT_FRICTION_SOLVER_VEL, MUT_NO_RESTITUTION_CACHE, MUT_RELAX_SOFT = 5, 6, 7, 8
-MUT_JOINT_AXIS_ORDER, MUT_JOINT_NO_COM, MUT_JOINT_UNIVERSAL_AXIS = 9, 10, 11
+MUT_JOINT_AXIS_ORDER, MUT_JOINT_NO_COM, MUT_JOINT_UNIVERSAL_AXIS, MUT_FILTER_ARG_ORDER = 9, 10, 11, 12
 
 
 class mutation:
