@@ -44,31 +44,31 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_SUMMARY = {("plain32k", "colors"): os.path.join(ROOT, "profiles", "k_solve_plain32k_traffic.md")}
+# PMC summaries of the roofline kernel, one per timed window they were summarised over
+TRAFFIC_SUMMARIES = {("plain32k", "colors"): "k_solve_plain32k_traffic_*.md"}
 # steps of each workload's run; the timed window is its end
 RUN_STEPS = {"plain32k": 600, "pyramid": 300, "batched": 300, "convex": 600, "triggers": 200}
 
 
 def measured_traffic(config, mode, window):
-    """HBM bytes per launch of the roofline kernel, from the committed PMC summary
-    (FETCH_SIZE and WRITE_SIZE per dispatch over the timed window's launches, each scaled
-    by the same counter's reading for a copy of known size). None when there is no summary
-    or when the summary's counters were taken over another window."""
-    path = TRAFFIC_SUMMARY.get((config, mode))
-    if path is None:
+    """HBM bytes per launch of the roofline kernel, from the committed PMC summary of this
+    window (FETCH_SIZE and WRITE_SIZE per dispatch over the timed window's launches, each
+    scaled by the same counter's reading for a copy of known size). None when no summary
+    was taken over this window."""
+    import glob
+    pattern = TRAFFIC_SUMMARIES.get((config, mode))
+    if pattern is None:
         return None
-    fields = {}
-    try:
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", pattern))):
+        fields = {}
         with open(path) as f:
             for line in f:
                 if ":" in line and not line.startswith(("#", "|", " ")):
                     k, v = line.split(":", 1)
                     fields[k.strip()] = v.strip()
-    except OSError:
-        return None
-    if fields.get("window") != window or "traffic_bytes_per_launch" not in fields:
-        return None
-    return float(fields["traffic_bytes_per_launch"])
+        if fields.get("window") == window and "traffic_bytes_per_launch" in fields:
+            return float(fields["traffic_bytes_per_launch"])
+    return None
 
 
 def copy_calibration():
@@ -136,6 +136,45 @@ def cpu_baseline(scene, state, substeps, iterations, dt, budget_s=15.0):
     return {"value": scene.n_dyn * substeps * steps / el, "unit": "body-substeps/s", "cores": threads + 1,
             "kind": "port", "sample": f"{steps} steps of the same scene from the device state after warm-up, "
                                       f"{el:.1f} s; narrowphase on {threads} workers + caller, solver single-threaded"}
+
+
+def cpu_baseline_worlds(scene, state, substeps, iterations, dt, budget_s=15.0):
+    """The batched workload's CPU baseline: a sequential loop over per-world Scenes. Each
+    sampled world is its own oracle world (the world's bodies, colliders and joints, from the
+    device state after warm-up), stepped on its own; worlds are taken in a fixed stride over
+    the batch so the sample spans the size range, each stepped 5 times, until the budget."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    pos, quat, vel, ang = state
+    arr = scene.arrays()
+    saved = {k: arr[k].copy() for k in ("pos", "quat", "vel", "angvel")}
+    arr["pos"][:] = pos
+    arr["quat"][:] = quat
+    arr["vel"][:] = vel
+    arr["angvel"][:] = ang
+    threads = max(0, (os.cpu_count() or 1) - 1)
+    order = [w for k in range(8) for w in range(k, scene.n_worlds, 8)]   # strided over the batch
+    body_substeps, worlds, el = 0, 0, 0.0
+    try:
+        for w in order:
+            sub = scene.world_subscene(w)
+            o = ol.OracleWorld(sub, substeps=substeps, iterations=iterations, threads=threads)
+            t0 = time.perf_counter()
+            for _ in range(5):
+                o.step(dt)
+            el += time.perf_counter() - t0
+            body_substeps += sub.n_dyn * substeps * 5
+            worlds += 1
+            del o
+            if el >= budget_s:
+                break
+    finally:
+        for k, v in saved.items():
+            arr[k][:] = v
+    return {"value": body_substeps / el, "unit": "body-substeps/s", "cores": threads + 1, "kind": "port",
+            "sample": f"{worlds} of {scene.n_worlds} worlds (stride 8 over the batch), each its own oracle Scene "
+                      f"from the device state after warm-up, 5 steps each, one world after another, {el:.1f} s of "
+                      f"steps; narrowphase on {threads} workers + caller, solver single-threaded"}
 
 
 def run_served(args):
@@ -538,7 +577,8 @@ def run_world(args):
                      "launches_timed": st["solve_launches"]},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(scene, warm_state, sub, it, dt)
+        base = cpu_baseline_worlds if scene.n_worlds > 1 else cpu_baseline
+        line["cpu_baseline"] = base(scene, warm_state, sub, it, dt)
     print(json.dumps(line), flush=True)
 
 

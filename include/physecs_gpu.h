@@ -22,7 +22,7 @@
  *   pgx_world_set_can_collide  Scene::setCanCollide (include/Physecs/Physecs.h:223)
  *   pgx_world_add_joints    Scene::createJoint<T> for the built-in joint types
  *                           (include/Physecs/Physecs.h:209-214, src/Physecs.cpp:690-710)
- *   pgx_world_set_joint_params  the joints' setters (include/Physecs/Joints/*.h)
+ *   pgx_world_set_joint_params  the joints' setters (include/Physecs/Joints/<name>Joint.h)
  *   pgx_world_destroy_joint Scene::destroyJoint (include/Physecs/Physecs.h:215,
  *                           src/Physecs.cpp:712-725)
  *   pgx_world_read_state / pgx_world_write_state
@@ -35,6 +35,7 @@
 #ifndef PHYSECS_GPU_H
 #define PHYSECS_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -62,7 +63,8 @@ typedef struct pgx_config {
     float gravity;          /* Scene::g, default 9.81 along -y (Physecs.h:159) */
     int device;             /* HIP device ordinal */
     int pair_capacity;      /* candidate pairs per step; 0 = 12 x colliders */
-    int block_size;         /* solver block size (autotuned; 0 = default 256) */
+    int block_size;         /* threads per contact-solve workgroup at creation: 64, 128 or 256;
+                               0 = 64 (pgx_tuning.solve_block; the autotuner may change it) */
 } pgx_config;
 
 /* Scene descriptor. Actors are entities with a TransformComponent; the dynamic storage
@@ -132,7 +134,7 @@ typedef struct pgx_scene_desc {
     const size_t* tri_bvh_cache_bytes;
 } pgx_scene_desc;
 
-/* Built-in joint types (include/Physecs/Joints/*.h) */
+/* Built-in joint types (include/Physecs/Joints/<name>Joint.h) */
 #define PGX_JOINT_SPHERICAL 0
 #define PGX_JOINT_UNIVERSAL 1
 #define PGX_JOINT_REVOLUTE 2
@@ -419,6 +421,15 @@ int pgx_trimesh_cache(const float* verts, int n_verts, const int32_t* tris, int 
  * indices, depth. PGX_OK or PGX_ERR_INVALID. */
 int pgx_trimesh_cache_validate(const float* verts, int n_verts, const int32_t* tris, int n_tris, const void* cache,
                                size_t bytes);
+/* Host code: the cache of a TriangleMesh that is already built (src/TriangleMesh.cpp:7-164
+ * left its triangles in build order with normals and centroids, and its BVH nodes as
+ * {bounds min, max, triCount, index}): the drop-in's path for the app's TriangleMesh
+ * objects, whose input order is gone. `tris` (build order) are the descriptor's tri_idx for
+ * this mesh. The same size and capacity rules as pgx_trimesh_cache; PGX_ERR_INVALID for a
+ * node list that does not describe a tree over the triangles. */
+int pgx_trimesh_cache_pack(const float* verts, int n_verts, const int32_t* tris, int n_tris, const float* tri_normals,
+                           const float* tri_centroids, const float* node_bounds, const int32_t* node_tri_count,
+                           const int32_t* node_index, int n_nodes, void* out, size_t cap, size_t* bytes);
 /* Save and resume a world's state between steps (no counterpart in the reference, which
  * keeps its hidden state -- contactCache, GearJointData, triggerCache; include/Physecs/
  * Physecs.h:167-171, include/Physecs/Joints/GearJoint.h:7-14 -- only inside a live Scene).

@@ -1,0 +1,225 @@
+// TEST INFRASTRUCTURE ONLY: a C ABI over an entt::registry and the drop-in physecs::Scene
+// (physecs_amd/shim/physecs_gpu_scene.cpp), compiled against the reference's own headers,
+// entt and glm, and its TriangleMesh / ConvexMesh / MassUtil sources (oracle/_ref). A test
+// builds a registry through it -- entities, TransformComponent, RigidBodyDynamicComponent,
+// RigidBodyCollisionComponent, meshes built by the reference's constructors, mass properties
+// from the reference's setMassProps -- and reads back the descriptor the Scene gathers.
+// Gathering is host code; sc_simulate steps the device world through Scene::simulate.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ConvexMesh.cpp"     // ConvexMeshVertices<4>'s constructor template (not instantiated there)
+#include "MassUtil.h"
+#include "physecs_gpu_scene.h"
+
+namespace {
+
+// every trigger callback the Scene made: enter (1) or exit (0), entity0, colIndex0, entity1, colIndex1
+struct TriggerLog : physecs::OnTriggerEnterListener, physecs::OnTriggerExitListener {
+    std::vector<uint32_t> events;
+    void onTriggerEnter(entt::entity e0, int c0, entt::entity e1, int c1) override {
+        events.insert(events.end(), {1u, (uint32_t)e0, (uint32_t)c0, (uint32_t)e1, (uint32_t)c1});
+    }
+    void onTriggerExit(entt::entity e0, int c0, entt::entity e1, int c1) override {
+        events.insert(events.end(), {0u, (uint32_t)e0, (uint32_t)c0, (uint32_t)e1, (uint32_t)c1});
+    }
+};
+
+struct Check {
+    entt::registry registry;
+    std::unique_ptr<physecs::Scene> scene;
+    std::vector<std::unique_ptr<physecs::ConvexMesh>> convex;
+    std::vector<std::unique_ptr<physecs::TriangleMesh>> trimesh;
+    TriggerLog triggers;
+};
+
+// a filter an app might set: colliders whose data share a bit, or a trigger on either side,
+// make trigger pairs
+physecs::ContactType shareBitFilter(bool isTrigger0, int data0, bool isTrigger1, int data1) {
+    return isTrigger0 || isTrigger1 || (data0 & data1) != 0 ? physecs::TRIGGER : physecs::COLLISION;
+}
+
+glm::vec3 v3(const float* p) { return glm::vec3(p[0], p[1], p[2]); }
+glm::quat q4(const float* p) { return glm::quat(p[3], p[0], p[1], p[2]); }   // x, y, z, w in
+
+}  // namespace
+
+extern "C" {
+
+void* sc_new() {
+    auto* c = new Check;
+    c->scene = std::make_unique<physecs::Scene>(c->registry, 1);
+    return c;
+}
+
+void sc_free(void* h) {
+    auto* c = static_cast<Check*>(h);
+    c->scene.reset();   // disconnects its hooks before the registry goes
+    delete c;
+}
+
+// Scene::simulate through the drop-in (device 0); 0, or -1 with the error in sc_error
+static std::string g_error;
+int sc_simulate(void* h, float dt, int substeps, int iterations, int steps) {
+    auto* c = static_cast<Check*>(h);
+    try {
+        c->scene->setNumSubSteps(substeps);
+        c->scene->setNumIterations(iterations);
+        for (int k = 0; k < steps; ++k) c->scene->simulate(dt);
+        return 0;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+const char* sc_error() { return g_error.c_str(); }
+
+// an entity's pose and, when it has a dynamic component, velocities
+void sc_read(void* h, uint32_t e, float* pos, float* quat, float* vel, float* angvel) {
+    auto& r = static_cast<Check*>(h)->registry;
+    const auto& t = r.get<TransformComponent>((entt::entity)e);
+    std::memcpy(pos, &t.position, 12);
+    quat[0] = t.orientation.x; quat[1] = t.orientation.y; quat[2] = t.orientation.z; quat[3] = t.orientation.w;
+    if (const auto* d = r.try_get<physecs::RigidBodyDynamicComponent>((entt::entity)e)) {
+        std::memcpy(vel, &d->velocity, 12);
+        std::memcpy(angvel, &d->angularVelocity, 12);
+    }
+}
+
+void sc_listen_triggers(void* h) {
+    auto* c = static_cast<Check*>(h);
+    c->scene->addOnTriggerEnterCallback(&c->triggers);
+    c->scene->addOnTriggerExitCallback(&c->triggers);
+}
+
+// the callbacks made since the last call, 5 values each; returns their count
+int sc_take_triggers(void* h, uint32_t* out, int cap) {
+    auto& ev = static_cast<Check*>(h)->triggers.events;
+    const int n = (int)ev.size() / 5;
+    std::memcpy(out, ev.data(), sizeof(uint32_t) * 5 * (size_t)std::min(n, cap));
+    ev.clear();
+    return n;
+}
+
+// 0: defaultContactFilter, 1: shareBitFilter
+void sc_set_contact_filter(void* h, int which) {
+    static_cast<Check*>(h)->scene->setContactFilter(which ? shareBitFilter : physecs::defaultContactFilter);
+}
+
+int sc_create_joint(void* h, const pgx_joint_desc* d, uint32_t e0, uint32_t e1) {
+    auto* c = static_cast<Check*>(h);
+    return c->scene->createJoint(d->type, (entt::entity)e0, v3(d->anchor0), q4(d->anchor0 + 3), (entt::entity)e1,
+                                 v3(d->anchor1), q4(d->anchor1 + 3), d);
+}
+
+void sc_set_can_collide(void* h, uint32_t e0, uint32_t e1, int can) {
+    static_cast<Check*>(h)->scene->setCanCollide((entt::entity)e0, (entt::entity)e1, can != 0);
+}
+
+uint32_t sc_create(void* h) { return (uint32_t)static_cast<Check*>(h)->registry.create(); }
+
+void sc_transform(void* h, uint32_t e, const float* pos, const float* quat) {
+    auto& r = static_cast<Check*>(h)->registry;
+    r.emplace<TransformComponent>((entt::entity)e, TransformComponent{v3(pos), q4(quat), glm::vec3(1.f)});
+}
+
+// inv_inertia_rows: the matrix by rows (element (r, c) at 3r + c)
+void sc_dynamic(void* h, uint32_t e, int kinematic, const float* vel, const float* angvel, float inv_mass,
+                const float* com, const float* inv_inertia_rows) {
+    auto& r = static_cast<Check*>(h)->registry;
+    physecs::RigidBodyDynamicComponent d{};
+    d.isKinematic = kinematic != 0;
+    d.velocity = v3(vel);
+    d.angularVelocity = v3(angvel);
+    d.invMass = inv_mass;
+    d.com = v3(com);
+    for (int row = 0; row < 3; ++row)
+        for (int col = 0; col < 3; ++col) d.invInertiaTensor[col][row] = inv_inertia_rows[3 * row + col];   // glm: [column][row]
+    r.emplace<physecs::RigidBodyDynamicComponent>((entt::entity)e, d);
+}
+
+// A ConvexMesh as the app makes one: vertices, faces (face f = face_idx[face_start[f] ..
+// face_start[f+1])), normals, centroids. Returns its handle.
+int sc_convex_mesh(void* h, const float* verts, int nv, const int* face_start, const int* face_idx,
+                   const float* normals, const float* centroids, int nf) {
+    auto* c = static_cast<Check*>(h);
+    std::vector<glm::vec3> vs;
+    for (int i = 0; i < nv; ++i) vs.push_back(v3(verts + 3 * i));
+    std::vector<physecs::ConvexMeshFace> fs;
+    for (int f = 0; f < nf; ++f) {
+        physecs::ConvexMeshFace face;
+        face.indices.assign(face_idx + face_start[f], face_idx + face_start[f + 1]);
+        face.normal = v3(normals + 3 * f);
+        face.centroid = v3(centroids + 3 * f);
+        fs.push_back(face);
+    }
+    c->convex.push_back(std::make_unique<physecs::ConvexMesh>(std::move(vs), std::move(fs)));
+    return (int)c->convex.size() - 1;
+}
+
+// A TriangleMesh built by the reference's constructor. Returns its handle.
+int sc_triangle_mesh(void* h, const float* verts, int nv, const uint32_t* idx, int nt) {
+    auto* c = static_cast<Check*>(h);
+    std::vector<glm::vec3> vs;
+    for (int i = 0; i < nv; ++i) vs.push_back(v3(verts + 3 * i));
+    std::vector<unsigned int> is(idx, idx + 3 * (size_t)nt);
+    c->trimesh.push_back(std::make_unique<physecs::TriangleMesh>(vs, is));
+    return (int)c->trimesh.size() - 1;
+}
+
+// The entity's RigidBodyCollisionComponent with n colliders; params as the descriptor's
+// col_params, a mesh collider's fourth parameter naming a handle from sc_*_mesh.
+void sc_collision(void* h, uint32_t e, int n, const int32_t* types, const float* params, const float* lpos,
+                  const float* lquat, const float* material, const uint8_t* trigger, const uint8_t* enable,
+                  const int32_t* data) {
+    auto* c = static_cast<Check*>(h);
+    physecs::RigidBodyCollisionComponent cc;
+    for (int k = 0; k < n; ++k) {
+        physecs::Collider col{};
+        col.position = v3(lpos + 3 * k);
+        col.orientation = q4(lquat + 4 * k);
+        const float* p = params + 4 * k;
+        col.geometry.type = (physecs::GeometryType)types[k];
+        switch (types[k]) {
+            case physecs::SPHERE: col.geometry.sphere.radius = p[0]; break;
+            case physecs::CAPSULE: col.geometry.capsule = {p[0], p[1]}; break;
+            case physecs::BOX: col.geometry.box.halfExtents = v3(p); break;
+            case physecs::CONVEX_MESH: col.geometry.convex = {c->convex.at((size_t)p[3]).get(), v3(p)}; break;
+            case physecs::TRIANGLE_MESH: col.geometry.triangleMesh.mesh = c->trimesh.at((size_t)p[3]).get(); break;
+        }
+        col.material = {material[3 * k], material[3 * k + 1], material[3 * k + 2]};
+        col.isTrigger = trigger[k] != 0;
+        col.enableSimulation = enable[k] != 0;
+        col.data = data[k];
+        cc.colliders.push_back(col);
+    }
+    c->registry.emplace<physecs::RigidBodyCollisionComponent>((entt::entity)e, cc);
+}
+
+// physecs::setMassProps (src/MassUtil.cpp) on the entity's components
+void sc_set_mass_props(void* h, uint32_t e, float mass) {
+    auto& r = static_cast<Check*>(h)->registry;
+    physecs::setMassProps(r.get<physecs::RigidBodyDynamicComponent>((entt::entity)e),
+                          r.get<physecs::RigidBodyCollisionComponent>((entt::entity)e), mass);
+}
+
+void sc_erase_dynamic(void* h, uint32_t e) {
+    static_cast<Check*>(h)->registry.erase<physecs::RigidBodyDynamicComponent>((entt::entity)e);
+}
+
+void sc_destroy(void* h, uint32_t e) { static_cast<Check*>(h)->registry.destroy((entt::entity)e); }
+
+// The dynamic storage's packed entity order, the reference's body index (src/Physecs.cpp:117)
+int sc_dynamic_order(void* h, uint32_t* out, int cap) {
+    auto& s = static_cast<Check*>(h)->registry.storage<physecs::RigidBodyDynamicComponent>();
+    const int n = (int)s.size();
+    for (int i = 0; i < n && i < cap; ++i) out[i] = (uint32_t)s.data()[i];
+    return n;
+}
+
+const pgx_scene_desc* sc_gather(void* h) { return &static_cast<Check*>(h)->scene->gatherDescriptor(); }
+
+}  // extern "C"

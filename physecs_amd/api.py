@@ -34,7 +34,7 @@ EXPORTS = [
     "pgx_world_destroy_joint", "pgx_world_set_graph", "pgx_world_set_tuning", "pgx_world_get_tuning",
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
     "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_character_move", "pgx_mass_properties",
-    "pgx_world_patch_transforms", "pgx_convex_hull", "pgx_trimesh_cache", "pgx_trimesh_cache_validate", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
+    "pgx_world_patch_transforms", "pgx_convex_hull", "pgx_trimesh_cache", "pgx_trimesh_cache_validate", "pgx_trimesh_cache_pack", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",

@@ -298,3 +298,52 @@ extern "C" int pgx_trimesh_cache_validate(const float* verts, int n_verts, const
         return PGX_ERR_INVALID;
     }
 }
+
+extern "C" int pgx_trimesh_cache_pack(const float* verts, int n_verts, const int32_t* tris, int n_tris,
+                                      const float* tri_normals, const float* tri_centroids, const float* node_bounds,
+                                      const int32_t* node_tri_count, const int32_t* node_index, int n_nodes, void* out,
+                                      size_t cap, size_t* bytes) {
+    using namespace pgx;
+    if (!verts || !tris || !tri_normals || !tri_centroids || !node_bounds || !node_tri_count || !node_index || !bytes ||
+        n_verts < 3 || n_tris < 1 || n_nodes < 1 || n_nodes > 2 * n_tris)
+        return PGX_ERR_INVALID;
+    *bytes = cache_bytes(n_tris, n_nodes);
+    if (!out) return PGX_OK;
+    if (cap < *bytes) return PGX_ERR_CAPACITY;
+    try {
+        BuiltMesh m;
+        for (int t = 0; t < n_tris; ++t) {
+            m.tris.push_back(make_int4(tris[3 * t], tris[3 * t + 1], tris[3 * t + 2], 0));
+            m.triN.push_back(make_float4(tri_normals[3 * t], tri_normals[3 * t + 1], tri_normals[3 * t + 2], 0.f));
+            m.triC.push_back(make_float4(tri_centroids[3 * t], tri_centroids[3 * t + 1], tri_centroids[3 * t + 2], 0.f));
+        }
+        auto asf = [](int i) { float f; std::memcpy(&f, &i, sizeof f); return f; };
+        std::vector<int> depth(n_nodes, 0);
+        for (int n = 0; n < n_nodes; ++n) {
+            const float* b = node_bounds + 6 * (size_t)n;
+            m.nodes.push_back(make_float4(b[0], b[1], b[2], asf(node_tri_count[n])));
+            m.nodes.push_back(make_float4(b[3], b[4], b[5], asf(node_index[n])));
+            if (node_tri_count[n] == 0) {
+                const int c = node_index[n];
+                if (c <= n || c + 1 >= n_nodes) return PGX_ERR_INVALID;
+                depth[c] = depth[c + 1] = depth[n] + 1;
+                m.depth = std::max(m.depth, depth[n] + 1);
+            }
+        }
+        MeshCacheHeader h{MESH_MAGIC, MESH_VERSION, n_verts, n_tris, n_nodes, m.depth, *bytes,
+                          mesh_hash(verts, n_verts, tris, n_tris), 0};
+        char* p = static_cast<char*>(out) + sizeof h;
+        char* const payload = p;
+        auto put = [&](const auto& vec) {
+            std::memcpy(p, vec.data(), sizeof(vec[0]) * vec.size());
+            p += sizeof(vec[0]) * vec.size();
+        };
+        put(m.tris); put(m.triN); put(m.triC); put(m.nodes);
+        h.payloadHash = fnv(FNV_BASIS, payload, (size_t)(p - payload));
+        std::memcpy(out, &h, sizeof h);
+        (void)load_mesh(out, *bytes, verts, n_verts, tris, n_tris);   // the structure checks world creation runs
+        return PGX_OK;
+    } catch (...) {
+        return PGX_ERR_INVALID;
+    }
+}

@@ -466,10 +466,13 @@ int pgx_world_create(const pgx_config* cfg, const pgx_scene_desc* sc, pgx_world*
     *out = nullptr;
     int rc = validate_scene(sc);
     if (rc != PGX_OK) return rc;
+    if (cfg && cfg->block_size != 0 && cfg->block_size != 64 && cfg->block_size != 128 && cfg->block_size != 256)
+        return fail(PGX_ERR_INVALID, "block_size must be 0, 64, 128 or 256 (got %d)", cfg->block_size);
     return guarded([&] {
         auto* h = new pgx_world();
         World& w = h->w;
         try {
+            if (cfg && cfg->block_size) w.tune.solveBlock = cfg->block_size;
             w.device = cfg ? cfg->device : 0;
             PGX_HIP(hipSetDevice(w.device));
             PGX_HIP(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking));

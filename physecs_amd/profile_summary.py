@@ -10,7 +10,14 @@ kernel's dispatch count, total and average duration from the kernel trace
 (*kernel_trace.csv), and, where a counter run's *counter_collection.csv is in the folder,
 each counter's mean per dispatch over the window. FETCH_SIZE and WRITE_SIZE (KiB per
 dispatch) are also given in bytes after scaling by the factors a copy of known size reads
-(--fetch-scale, --write-scale: true bytes per counted byte).
+(--fetch-scale, --write-scale: true bytes per counted byte). Where a run collected the SQ
+counters for them, each kernel also gets, over the window:
+- occupancy: mean resident waves per CU while the kernel ran, 4 * SQ_WAVE_CYCLES (quad
+  cycles) / (GRBM_GUI_ACTIVE / 8, the counter is summed over the 8 XCDs) / 256 CUs, and
+  that as a share of the 32 wave slots of a CU;
+- LDS bank conflicts per LDS access cycle, SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE -
+  SQ_LDS_BANK_CONFLICT);
+- the share of wave time spent waiting to issue, SQ_WAIT_INST_ANY / SQ_WAVE_CYCLES.
 """
 import argparse
 import collections
@@ -80,6 +87,30 @@ def read_counters(path, first_dispatch):
     return per
 
 
+CUS, XCDS, WAVE_SLOTS = 256, 8, 32
+
+
+def derived(per):
+    """Table lines of occupancy, LDS conflict rate and issue-wait share per kernel."""
+    out = []
+    for kern in sorted(per):
+        c = {name: sum(v.values()) for name, v in per[kern].items()}
+        n = max(len(v) for v in per[kern].values())
+        occ = lds = wait = ""
+        if c.get("SQ_WAVE_CYCLES") and c.get("GRBM_GUI_ACTIVE"):
+            waves = 4.0 * c["SQ_WAVE_CYCLES"] / (c["GRBM_GUI_ACTIVE"] / XCDS) / CUS
+            occ = f"{waves:.2f} ({100.0 * waves / WAVE_SLOTS:.1f}%)"
+        if "SQ_LDS_BANK_CONFLICT" in c and c.get("SQ_LDS_IDX_ACTIVE"):
+            busy = c["SQ_LDS_IDX_ACTIVE"] - c["SQ_LDS_BANK_CONFLICT"]
+            lds = f"{c['SQ_LDS_BANK_CONFLICT'] / busy:.3f}" if busy > 0 else "0"
+        if c.get("SQ_WAIT_INST_ANY") is not None and c.get("SQ_WAVE_CYCLES"):
+            wait = f"{100.0 * c['SQ_WAIT_INST_ANY'] / c['SQ_WAVE_CYCLES']:.1f}%"
+        if occ or lds or wait:
+            waves_per = c.get("SQ_WAVES", 0) / n
+            out.append(f"| `{kern}` | {n} | {waves_per:.0f} | {occ} | {lds} | {wait} |")
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--dir", required=True, help="rocprofv3 output folder (-d)")
@@ -120,6 +151,7 @@ def main(argv=None):
               f"span_us_per_step: {span / a.window_steps:.1f}",
               f"first_window_dispatch: {first_dispatch}", ""]
     counter_lines = []
+    derived_lines = []
     traffic = None
     for cdir in a.counters_dir:
         cpath = _find(cdir, "counter_collection.csv")
@@ -140,6 +172,7 @@ def main(argv=None):
             ks = sorted(ids)
             cfirst = ks[-a.window_steps]
         per = read_counters(cpath, cfirst)
+        derived_lines += derived(per)
         for kern in sorted(per):
             for cname, vals in sorted(per[kern].items()):
                 v = list(vals.values())
@@ -165,6 +198,10 @@ def main(argv=None):
     if counter_lines:
         lines += ["## Counters over the window (mean per dispatch)", "",
                   "| kernel | counter | dispatches | mean | scaled |", "|---|---|---:|---:|---:|"] + counter_lines + [""]
+    if derived_lines:
+        lines += ["## Occupancy, LDS bank conflicts and issue waits over the window", "",
+                  "| kernel | dispatches | waves/dispatch | resident waves per CU (of 32) | LDS conflicts per access "
+                  "| wave time waiting to issue |", "|---|---:|---:|---:|---:|---:|"] + derived_lines + [""]
     at, atotal, acalls = table(rows, max(1, n_steps_all))
     lines += [f"## Whole run ({n_steps_all} steps, {acalls} dispatches, {atotal / 1e3:.1f} us of kernels)", ""] + at[:32]
     with open(a.out, "w") as f:

@@ -510,12 +510,11 @@ class Scene:
         amap = {}
         for a in np.nonzero(arr["world"] == w)[0]:
             amap[int(a)] = s.add_actor(arr["pos"][a], arr["quat"][a], 0, entity=int(arr["entity"][a]))
-        for d in range(self.n_dyn):
+        for d in np.nonzero(arr["world"][arr["actor_of_dyn"]] == w)[0]:
             a = int(arr["actor_of_dyn"][d])
-            if a in amap:
-                s.add_dynamic(amap[a], arr["inv_mass"][d], arr["com"][d], arr["inv_inertia"][d], arr["vel"][d],
-                              arr["angvel"][d], bool(arr["kinematic"][d]))
-        for c in np.nonzero(np.isin(arr["col_actor"], list(amap)))[0]:
+            s.add_dynamic(amap[a], arr["inv_mass"][d], arr["com"][d], arr["inv_inertia"][d], arr["vel"][d],
+                          arr["angvel"][d], bool(arr["kinematic"][d]))
+        for c in np.nonzero(arr["world"][arr["col_actor"]] == w)[0]:
             a = int(arr["col_actor"][c])
             s.add_collider(amap[a], int(arr["col_type"][c]), arr["col_params"][c][:3], arr["col_material"][c],
                            arr["col_pos"][c], arr["col_quat"][c], bool(arr["col_trigger"][c]),

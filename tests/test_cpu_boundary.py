@@ -46,6 +46,19 @@ def test_invalid_descriptor_rejected_on_host():
     assert b"geometry type 7" in api.lib().pgx_last_error()
 
 
+def test_config_block_size_checked_on_host():
+    """pgx_config.block_size: 0, 64, 128 or 256 threads per contact-solve workgroup; any
+    other value is rejected before anything touches a device."""
+    from physecs_amd import api, scenes
+    sc = scenes.box_pyramid(0, rows=2)
+    desc = sc.struct(api.PgxSceneDesc)
+    desc.actor_world = ctypes.cast(None, api.I32)
+    h = ctypes.c_void_p()
+    cfg = api.PgxConfig(4, 2, 9.81, 0, 0, 96)
+    assert api.lib().pgx_world_create(ctypes.byref(cfg), ctypes.byref(desc), ctypes.byref(h)) == -1
+    assert b"block_size" in api.lib().pgx_last_error()
+
+
 def test_scene_generators_shapes():
     from physecs_amd import scenes
     p = scenes.box_pyramid(0)
@@ -214,3 +227,15 @@ def test_struct_layouts_match_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(cls), cname
         for fname, _ in cls._fields_:
             assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+@pytest.mark.parametrize("lang,cc", [("c", "gcc"), ("c++", "g++")])
+def test_header_is_self_contained(tmp_path, lang, cc):
+    """include/physecs_gpu.h compiles as the first and only include, as C99 and C++17, with
+    warnings as errors (a caller's binding includes nothing before it)."""
+    import subprocess
+    std = "-std=c99" if lang == "c" else "-std=c++17"
+    src = tmp_path / ("only." + ("c" if lang == "c" else "cpp"))
+    src.write_text('#include "physecs_gpu.h"\nint main(void) { return PGX_OK; }\n')
+    subprocess.run([cc, std, "-Wall", "-Wextra", "-pedantic", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    "-c", str(src), "-o", str(tmp_path / "only.o")], check=True)

@@ -87,6 +87,33 @@ def test_autotune_restores_the_world(api):
             assert np.array_equal(bits(x), bits(y)), k
 
 
+def test_config_block_size_sets_solve_block(api):
+    """pgx_config.block_size is the world's first solve_block; 0 keeps the default 64. Both
+    worlds step to the same state."""
+    import ctypes
+    sc = scenes.box_pyramid(0, rows=4)
+    states, blocks = [], []
+    for bs in (0, 128):
+        desc = sc.struct(api.PgxSceneDesc)
+        desc.actor_world = ctypes.cast(None, api.I32)
+        h = ctypes.c_void_p()
+        cfg = api.PgxConfig(4, 2, 9.81, 0, 0, bs)
+        api.check(api.lib().pgx_world_create(ctypes.byref(cfg), ctypes.byref(desc), ctypes.byref(h)))
+        try:
+            t = api.PgxTuning()
+            api.check(api.lib().pgx_world_get_tuning(h, ctypes.byref(t)))
+            blocks.append(t.solve_block)
+            for _ in range(30):
+                api.check(api.lib().pgx_world_step(h, 1 / 60))
+            pos = np.zeros((sc.n_actors, 3), np.float32)
+            api.check(api.lib().pgx_world_read_state(h, pos.ctypes.data_as(api.F), None, None, None))
+            states.append(pos)
+        finally:
+            api.lib().pgx_world_destroy(h)
+    assert blocks == [64, 128]
+    assert np.array_equal(states[0].view(np.uint32), states[1].view(np.uint32))
+
+
 def test_tuning_rejects_bad_values(api):
     w = api.World(scenes.box_pyramid(0, rows=3), substeps=4, iterations=2)
     start = w.tuning()
