@@ -446,7 +446,8 @@ def run_snapshot(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=590)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the workload's run after "
+                    "the warm-up, 590 for the 32k scene)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="plain32k")
     ap.add_argument("--no-cpu", action="store_true")
@@ -475,6 +476,8 @@ def main():
 
 
 def dispatch(args):
+    if args.steps is None:   # the workload's whole run after the warm-up
+        args.steps = RUN_STEPS.get(args.config, 600) - args.warmup
     if args.config == "served":
         run_served(args)
         return

@@ -344,8 +344,13 @@ __global__ void k_pack(int n, const float4* __restrict__ nmin, const float4* __r
         int c = order[t];
         leafI[t] = make_int2((int)centity[c], clocal[c]);
         const float esc = __int_as_float(trav_index(leafEsc[t], n));
-        travA[n - 1 + t] = make_float4(a.x, a.y, a.z, esc);
-        travB[n - 1 + t] = make_float4(b.x, b.y, b.z, esc);
+        if (lflags[t] & CF_ENABLE) {
+            travA[n - 1 + t] = make_float4(a.x, a.y, a.z, esc);
+            travB[n - 1 + t] = make_float4(b.x, b.y, b.z, esc);
+        } else {   // a disabled leaf overlaps nothing
+            travA[n - 1 + t] = make_float4(INFINITY, INFINITY, INFINITY, esc);
+            travB[n - 1 + t] = make_float4(-INFINITY, -INFINITY, -INFINITY, esc);
+        }
         travC[n - 1 + t] = make_int4(t, (int)centity[c], clocal[c], lflags[t]);
     }
     if (t < nStatic) {
@@ -366,19 +371,23 @@ __device__ __forceinline__ bool overlaps(float4 amin, float4 amax, float4 bmin, 
 
 constexpr int TRAV_SLOTS = 8;   // pairs a lane holds before its wave appends them
 
-// Each enabled dynamic leaf s first tests its world's non-dynamic colliders (static and
-// kinematic; their bounds never move during a step), two per round of loads, then walks its
-// world's subtree over escape links and takes the dynamic leaves after it (so each dynamic
-// pair is emitted once). Every walk step is the same three loads of one record, internal
-// node or leaf alike, and one select of the next record: a record is taken when it holds a
-// leaf after s, is enabled and overlaps; a taken node leads to its left child, a taken leaf
-// is a pair; anything else leads over the escape link.
+// Each enabled dynamic leaf s walks the records after it in depth-first pre-order, starting
+// at its own escape link (the pre-order successor of a leaf), up to its world's stop: those
+// are exactly the subtrees whose leaves come after s, so each dynamic pair is found once,
+// by its first leaf, and no walk descends from the root to its own leaf. Every walk step is
+// two loads of one record ({min, next if taken}, {max, escape}), internal node or leaf
+// alike, and one select: a record is taken when it overlaps; a taken node leads to its left
+// child, a taken leaf is a candidate and leads over its escape link. A disabled leaf's
+// record holds an empty box. Alongside the walk, one of the world's non-dynamic colliders
+// (static and kinematic; their bounds never move during a step) per step is tested from its
+// own records. Candidates from the walk are kept as leaf indices and resolved to pair keys
+// (and same-entity colliders dropped) when the lane's slots are flushed.
 __global__ __launch_bounds__(256) void k_traverse(
     int n, const uint64_t* __restrict__ mkeys, int worldShift, const float4* __restrict__ travA,
-    const float4* __restrict__ travB, const int4* __restrict__ travC, const int32_t* __restrict__ worldRoot,
-    const int32_t* __restrict__ worldStop, const int32_t* __restrict__ worldStaticStart,
-    const float4* __restrict__ statA, const float4* __restrict__ statB, const int32_t* __restrict__ statL,
-    int localBits, int32_t* __restrict__ pairCount, int cap, uint64_t* __restrict__ pkeys) {
+    const float4* __restrict__ travB, const int4* __restrict__ travC, const int32_t* __restrict__ worldStop,
+    const int32_t* __restrict__ worldStaticStart, const float4* __restrict__ statA,
+    const float4* __restrict__ statB, const int32_t* __restrict__ statL, int localBits,
+    int32_t* __restrict__ pairCount, int cap, uint64_t* __restrict__ pkeys) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     bool active = false;
     float4 qmin = make_float4(0.f, 0.f, 0.f, 0.f), qmax = qmin;
@@ -388,29 +397,53 @@ __global__ __launch_bounds__(256) void k_traverse(
     int wd = 0;
     if (s < n) {
         const int self = n - 1 + s;
-        float4 a = travA[self], b = travB[self];
-        int4 c = travC[self];
+        const int4 c = travC[self];
         if (c.w & CF_ENABLE) {
             active = true;
-            qmin = a;
-            qmax = b;
+            qmin = travA[self];
+            qmax = travB[self];
+            node = __float_as_int(qmax.w);   // the leaf's escape link: its pre-order successor
             myEnt = (uint32_t)c.y;
             myLocal = c.z;
             wd = (int)(mkeys[s] >> worldShift);
-            node = trav_index(worldRoot[wd], n);
             stop = trav_index(worldStop[wd], n);
             st = worldStaticStart[wd];
             stEnd = worldStaticStart[wd + 1];
         }
     }
     const int lane = threadIdx.x & (WAVE - 1);
-    // found pairs wait in the lane's own LDS slots; when a lane's slots are full (or the
-    // walk ends) the wave appends all of its lanes' pairs with one atomic on the counter
+    // candidates wait in the lane's own LDS slots (a static pair as its key, a dynamic leaf
+    // as its index, marked in treeMask); when a lane's slots are full (or the walk ends) the
+    // wave resolves and appends all of its lanes' pairs with one atomic on the counter
     __shared__ uint64_t found[256 * TRAV_SLOTS];
     uint64_t* mine = found + threadIdx.x * TRAV_SLOTS;
     int held = 0;
+    uint32_t treeMask = 0;
+    auto pairKey = [&](uint32_t e2, int l2) {
+        const int l0 = myEnt < e2 ? myLocal : l2;
+        const int l1 = myEnt < e2 ? l2 : myLocal;
+        return ((uint64_t)wd << (2 * localBits)) | ((uint64_t)l0 << localBits) | (uint64_t)l1;
+    };
     auto flush = [&]() {
-        int incl = held;   // inclusive prefix sum of held over the wave
+        // resolve the walk's leaves: entity and local index from their records (independent
+        // loads), colliders of the lane's own entity dropped
+        int4 rec[TRAV_SLOTS];
+#pragma unroll
+        for (int j = 0; j < TRAV_SLOTS; ++j)
+            if ((treeMask >> j) & 1u) rec[j] = travC[n - 1 + (int)mine[j]];
+        int kept = 0;
+#pragma unroll
+        for (int j = 0; j < TRAV_SLOTS; ++j) {
+            if (j >= held) break;
+            uint64_t key = mine[j];
+            bool keep = true;
+            if ((treeMask >> j) & 1u) {
+                keep = (uint32_t)rec[j].y != myEnt;
+                key = pairKey((uint32_t)rec[j].y, rec[j].z);
+            }
+            if (keep) mine[kept++] = key;
+        }
+        int incl = kept;   // inclusive prefix sum of kept over the wave
         for (int o = 1; o < WAVE; o <<= 1) {
             int t = __shfl_up(incl, o);
             if (lane >= o) incl += t;
@@ -418,64 +451,47 @@ __global__ __launch_bounds__(256) void k_traverse(
         int total = __shfl(incl, WAVE - 1);
         int base = 0;
         if (lane == WAVE - 1 && total) base = atomicAdd(pairCount, total);
-        base = __shfl(base, WAVE - 1) + incl - held;
-        for (int j = 0; j < held; ++j)
+        base = __shfl(base, WAVE - 1) + incl - kept;
+        for (int j = 0; j < kept; ++j)
             if (base + j < cap) pkeys[base + j] = mine[j];
         held = 0;
+        treeMask = 0;
     };
-    auto pairKey = [&](uint32_t e2, int l2) {
-        const int l0 = myEnt < e2 ? myLocal : l2;
-        const int l1 = myEnt < e2 ? l2 : myLocal;
-        return ((uint64_t)wd << (2 * localBits)) | ((uint64_t)l0 << localBits) | (uint64_t)l1;
-    };
-    // the world's non-dynamic colliders, two per round of loads
-    while (__ballot(active && st < stEnd)) {
-        bool e0 = false, e1 = false;
-        uint64_t k0 = 0, k1 = 0;
-        if (active && st < stEnd) {
-            const bool two = st + 1 < stEnd;
-            const float4 a0 = statA[st], b0 = statB[st];
-            const int l0 = statL[st];
-            float4 a1 = a0, b1 = b0;
-            int l1 = l0;
-            if (two) { a1 = statA[st + 1]; b1 = statB[st + 1]; l1 = statL[st + 1]; }
-            const uint32_t en0 = (uint32_t)__float_as_int(b0.w), en1 = (uint32_t)__float_as_int(b1.w);
-            if ((__float_as_int(a0.w) & CF_ENABLE) && en0 != myEnt && overlaps(qmin, qmax, a0, b0)) {
-                e0 = true;
-                k0 = pairKey(en0, l0);
-            }
-            if (two && (__float_as_int(a1.w) & CF_ENABLE) && en1 != myEnt && overlaps(qmin, qmax, a1, b1)) {
-                e1 = true;
-                k1 = pairKey(en1, l1);
-            }
-            st += two ? 2 : 1;
-        }
-        if (__ballot(held + 2 > TRAV_SLOTS)) flush();
-        if (e0) mine[held++] = k0;
-        if (e1) mine[held++] = k1;
-    }
     int visits = 0;
     const int maxVisits = 2 * n;   // a depth-first walk visits at most 2n - 1 records
-    while (__ballot(active && node != stop)) {
-        bool emit = false;
-        uint64_t key = 0;
+    while (__ballot(active && (node != stop || st < stEnd))) {
+        bool emitLeaf = false, emitStatic = false;
+        int leaf = 0;
+        uint64_t skey = 0;
         if (active && node != stop && ++visits > maxVisits) {
             atomicOr(&pairCount[6], 1);   // malformed tree: stop and report
             node = stop;
         }
         if (active && node != stop) {
             const float4 a = travA[node], b = travB[node];
-            const int4 c = travC[node];
-            // subtrees whose leaves all sit at or before s hold no pair this leaf owns
-            const bool take = c.x > s && (c.w & CF_ENABLE) && overlaps(qmin, qmax, a, b);
-            if (take && node >= n - 1 && (uint32_t)c.y != myEnt) {
-                key = pairKey((uint32_t)c.y, c.z);
-                emit = true;
+            const bool take = overlaps(qmin, qmax, a, b);
+            if (take && node >= n - 1) {
+                emitLeaf = true;
+                leaf = node - (n - 1);
             }
             node = take ? __float_as_int(a.w) : __float_as_int(b.w);
         }
-        if (emit) mine[held++] = key;
-        if (__ballot(held == TRAV_SLOTS)) flush();
+        if (active && st < stEnd) {
+            const float4 a = statA[st], b = statB[st];
+            const int l = statL[st];
+            const uint32_t en = (uint32_t)__float_as_int(b.w);
+            if ((__float_as_int(a.w) & CF_ENABLE) && en != myEnt && overlaps(qmin, qmax, a, b)) {
+                emitStatic = true;
+                skey = pairKey(en, l);
+            }
+            ++st;
+        }
+        if (__ballot(held + 2 > TRAV_SLOTS)) flush();
+        if (emitLeaf) {
+            treeMask |= 1u << held;
+            mine[held++] = (uint64_t)leaf;
+        }
+        if (emitStatic) mine[held++] = skey;
     }
     if (__ballot(held > 0)) flush();
 }
@@ -547,8 +563,8 @@ int bp_find_pairs(World& w) {
     bp_build(w);
     if (n > 0) {
         hipLaunchKernelGGL(k_traverse, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift, w.travA.p,
-                           w.travB.p, w.travC.p, w.worldRoot.p, w.worldStop.p, w.worldStaticStart.p, w.statA.p,
-                           w.statB.p, w.statL.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
+                           w.travB.p, w.travC.p, w.worldStop.p, w.worldStaticStart.p, w.statA.p, w.statB.p,
+                           w.statL.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
         PGX_LAUNCH_CHECK();
     }
     int cnt[8] = {0};
