@@ -452,7 +452,7 @@ def main():
     ap.add_argument("--config", default="plain32k")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--blocks", type=int, default=0, help="cooperative grid size (0 = one block per CU)")
-    ap.add_argument("--exec", default="colors", choices=["flow", "xcd", "colors", "cooperative", "worlds", "pass"],
+    ap.add_argument("--exec", default="colors", choices=["flow", "xcd", "colors", "cooperative", "worlds", "pass", "xpass"],
                     help="how the contact passes launch")
     ap.add_argument("--graph", action="store_true", help="launch each step's substep loop as one HIP graph")
     ap.add_argument("--tune", default=None, help="launch shapes: 'auto' (pgx_world_autotune after the warm-up), "
@@ -572,7 +572,8 @@ def run_world(args):
                        phase_us_last_step=phase),
         "roofline": {"bound": "hbm", "kernel": {"flow": "k_solve_flow", "xcd": "k_solve_flow", "colors": "k_solve",
                                                     "cooperative": "k_step_persistent",
-                                                    "worlds": "k_world_pass", "pass": "k_solve_pass"}[exec_mode], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                                    "worlds": "k_world_pass", "pass": "k_solve_pass",
+                                                    "xpass": "k_solve_xpass"}[exec_mode], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(args.config, exec_mode, window),
                      "avg_launch_us": st["solve_us"] / max(1, st["solve_launches"]),

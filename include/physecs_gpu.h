@@ -283,13 +283,19 @@ int pgx_world_reset_stats(pgx_world* w);
  *                            grid barrier between colours; the solver velocities go
  *                            through the device's coherence point, so a barrier needs no
  *                            cache writeback. blocks = its grid size (0 = enough for the
- *                            largest colour, at most one workgroup per CU). */
+ *                            largest colour, at most one workgroup per CU).
+ *   PGX_EXEC_XPASS           one cooperative launch per pass on the CUs of one XCD (the
+ *                            one most of the launch's workgroups landed on), every colour
+ *                            in it with a barrier between colours; the solver velocities
+ *                            stay in that XCD's L2. blocks = its grid size over all XCDs
+ *                            (0 = 8 x enough for the largest colour; at most 248). */
 #define PGX_EXEC_COLORS 0
 #define PGX_EXEC_COOPERATIVE 1
 #define PGX_EXEC_FLOW 2
 #define PGX_EXEC_FLOW_XCD 3
 #define PGX_EXEC_WORLDS 4
 #define PGX_EXEC_PASS 5
+#define PGX_EXEC_XPASS 6
 int pgx_world_set_exec(pgx_world* w, int mode, int blocks);
 int pgx_world_get_exec(pgx_world* w, int* mode);
 int pgx_world_get_stats(pgx_world* w, pgx_stats* out);

@@ -419,14 +419,15 @@ class World:
     def enable_timing(self, on=True):
         check(self.L.pgx_world_enable_timing(self.h, 1 if on else 0))
 
-    EXEC_MODES = {"colors": 0, "cooperative": 1, "flow": 2, "xcd": 3, "worlds": 4, "pass": 5}
+    EXEC_MODES = {"colors": 0, "cooperative": 1, "flow": 2, "xcd": 3, "worlds": 4, "pass": 5, "xpass": 6}
 
     def set_exec(self, mode="colors", blocks=0):
         """How the contact passes launch: "colors" (one launch per colour, the default),
         "flow" (one dependency-driven launch per pass), "xcd" (the same with each world's
         manifolds on one XCD), "cooperative" (one launch per step), "worlds" (one
-        workgroup per world per pass, bodies in LDS) or "pass" (one launch per pass, grid
-        barriers between colours)."""
+        workgroup per world per pass, bodies in LDS), "pass" (one launch per pass, grid
+        barriers between colours) or "xpass" (one launch per pass on the CUs of one XCD,
+        barriers between colours, velocities in that XCD's L2)."""
         check(self.L.pgx_world_set_exec(self.h, self.EXEC_MODES[mode], blocks))
 
     def set_contact_filter(self, fn):

@@ -64,7 +64,7 @@ constexpr int JOINT_MAX_ROWS = 7;      // a prismatic joint with its limit and d
 // (grid barriers between colours); one dependency-driven launch per pass
 // (one device-wide queue), the same with a queue per XCD
 // (one launch per pass); one workgroup per world per pass, bodies in LDS
-enum { EXEC_COLORS = 0, EXEC_COOPERATIVE = 1, EXEC_FLOW = 2, EXEC_FLOW_XCD = 3, EXEC_WORLDS = 4, EXEC_PASS = 5 };
+enum { EXEC_COLORS = 0, EXEC_COOPERATIVE = 1, EXEC_FLOW = 2, EXEC_FLOW_XCD = 3, EXEC_WORLDS = 4, EXEC_PASS = 5, EXEC_XPASS = 6 };
 constexpr int FLOW_QUEUES = 8;         // queues of a dependency-driven pass: at most one per XCD
 constexpr int WORLD_SMALL = 256;       // per-world passes: worlds up to this many dynamic bodies share a launch
 // convex narrowphase: EPA polytope capacity per lane and lanes of the convex kernel
@@ -339,6 +339,8 @@ struct World {
     int persistentBlocks = 0, persistentBlocksWanted = 0;
     int passBlocksMax = 0;               // co-resident workgroups of the one-launch-per-pass kernel
     DBuf<int32_t> passBarrier;           // its grid barriers' arrival count
+    int xpassBlocksMax = 0;              // co-resident workgroups of the one-XCD pass kernel
+    DBuf<int32_t> xpassCtl;              // its control blocks, one per pass of a substep
     // dependency-driven passes
     int nXcd = 1;                        // XCDs of the device (CUs / 32)
     uint32_t flowEpoch = 0;              // per launch: versions of the published velocities

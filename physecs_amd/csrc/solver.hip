@@ -374,6 +374,8 @@ struct SubArgs {
     int32_t* flowErr;
     int32_t* qDone;                       // manifolds finished per queue in this step
     int32_t* barrier;                     // arrivals at the pass kernel's grid barriers (this substep)
+    int32_t* xctl;                        // control blocks of the one-XCD pass kernel, one per pass
+    int xctlWords;                        // words of xctl cleared by k_prepare
 };
 
 __device__ __forceinline__ m3 world_inv_inertia(const SubArgs& a, int b) {
@@ -639,6 +641,7 @@ __global__ __launch_bounds__(256) void k_prepare(SubArgs a, int sub) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < (a.iterations + 1) * FLOW_QUEUES) a.tickets[i] = 0;
     if (i == 0) a.barrier[0] = 0;
+    for (int j = i; j < a.xctlWords; j += gridDim.x * blockDim.x) a.xctl[j] = 0;
     if (sub == 0 && i < FLOW_QUEUES) a.qDone[i] = 0;
     if (i < a.M) prepare_manifold(a, i);
     else if (i - a.M < a.D) integrate_velocity(a, i - a.M);
@@ -731,6 +734,94 @@ __global__ __launch_bounds__(PASS_THREADS) void k_solve_pass(SubArgs a, int nCol
         have = s < e;
         if (have) load_inputs<true>(a, s, in);
         pass_barrier(a.barrier, barrierBase + (c + 1) * (int)gridDim.x, a.flowErr);
+    }
+}
+
+// ------------------------------------------------------------------ one launch per pass, on one XCD
+// k_solve_xpass runs every colour of one contact pass in one launch, like k_solve_pass, but
+// only on the CUs of one XCD, so the solver velocities handed from one colour to the next
+// stay in that XCD's L2 instead of crossing to the device's coherence point:
+// - every workgroup registers the XCD it runs on (hardware register XCC_ID) with one
+//   agent-scope atomic add on a 64-bit word holding one byte per XCD; the byte's old value
+//   is the workgroup's rank on its XCD. When all gridDim.x have registered, every workgroup
+//   reads the word and the XCD with the most workgroups (the lowest on a tie) is chosen;
+//   the workgroups of the other XCDs leave;
+// - the participants solve the colours in order, a barrier between colours: velocities are
+//   loaded with agent-scope loads, which skip the CU's L1 and are served by the XCD's L2,
+//   where the other participants' plain stores land; stores are plain and complete
+//   (s_waitcnt 0) before a workgroup arrives at the barrier (pass_barrier's count);
+// - every wait is bounded and reports through flowErr.
+// ctl (per launch, cleared by k_prepare): [0..1] the registration word, [2] the barrier count.
+constexpr int XPASS_THREADS = 256;
+constexpr int XPASS_CTL = 4;        // int32 words of one launch's control block
+constexpr int XPASS_MAX_GRID = 248; // at most 31 workgroups per XCD: a byte never overflows
+
+__device__ __forceinline__ f3 ld_l2_3(const float4* p) { return ld_agent3(p); }
+
+__device__ __forceinline__ void solve_xcd(const SubArgs& a, int s, const MInputs& in, bool useBias, float h,
+                                          bool skipSoft) {
+    if (skipSoft && in.soft) return;
+    f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
+    if (in.b0 >= 0) { v0 = ld_l2_3(&a.bvel[bv(in.b0, BV_V)]); w0 = ld_l2_3(&a.bvel[bv(in.b0, BV_W)]); }
+    if (in.b1 >= 0) { v1 = ld_l2_3(&a.bvel[bv(in.b1, BV_V)]); w1 = ld_l2_3(&a.bvel[bv(in.b1, BV_W)]); }
+    solve_core(a, s, in, useBias, h, v0, w0, v1, w1);
+    if (in.b0 >= 0) { a.bvel[bv(in.b0, BV_V)] = f4(v0, 0.f); a.bvel[bv(in.b0, BV_W)] = f4(w0, 0.f); }
+    if (in.b1 >= 0) { a.bvel[bv(in.b1, BV_V)] = f4(v1, 0.f); a.bvel[bv(in.b1, BV_W)] = f4(w1, 0.f); }
+}
+
+__global__ __launch_bounds__(XPASS_THREADS) void k_solve_xpass(SubArgs a, int nColors, int useBias, float h,
+                                                              int skipSoft, int32_t* ctl) {
+    __shared__ int sRank, sP;
+    unsigned long long* reg = reinterpret_cast<unsigned long long*>(ctl);
+    if (threadIdx.x == 0) {
+        const int x = (int)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7);   // hwreg(HW_REG_XCC_ID)
+        const unsigned long long old =
+            __hip_atomic_fetch_add(reg, 1ull << (8 * x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int spins = 0;
+        unsigned long long v = 0;
+        for (;;) {
+            v = __hip_atomic_load(reg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int total = 0;
+            for (int k = 0; k < 8; ++k) total += (int)((v >> (8 * k)) & 255);
+            if (total >= (int)gridDim.x) break;
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > PASS_SPIN_LIMIT || __hip_atomic_load(a.flowErr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                __hip_atomic_store(a.flowErr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                v = 0;
+                break;
+            }
+        }
+        int best = 0, bestN = 0;
+        for (int k = 0; k < 8; ++k) {
+            const int c = (int)((v >> (8 * k)) & 255);
+            if (c > bestN) { best = k; bestN = c; }
+        }
+        sRank = (int)((old >> (8 * x)) & 255);
+        sP = (x == best) ? bestN : 0;
+    }
+    __syncthreads();
+    const int P = sP;
+    if (P == 0) return;            // another XCD's workgroup (or registration failed)
+    const int tid = sRank * XPASS_THREADS + threadIdx.x;
+    const int nth = P * XPASS_THREADS;
+    const bool bias = useBias != 0, skip = skipSoft != 0;
+    int s = a.colorStart[0] + tid, e = a.colorStart[1];
+    MInputs in;
+    bool have = s < e;
+    if (have) load_inputs<true>(a, s, in);
+    for (int c = 0; c < nColors; ++c) {
+        if (have) solve_xcd(a, s, in, bias, h, skip);
+        for (int s2 = s + nth; s2 < e; s2 += nth) {
+            MInputs in2;
+            load_inputs<true>(a, s2, in2);
+            solve_xcd(a, s2, in2, bias, h, skip);
+        }
+        if (c + 1 == nColors) break;
+        s = e + tid;
+        e = a.colorStart[c + 2];
+        have = s < e;
+        if (have) load_inputs<true>(a, s, in);
+        pass_barrier(ctl + 2, (c + 1) * P, a.flowErr);
     }
 }
 
@@ -1179,6 +1270,9 @@ static SubArgs make_args(World& w, int M, float dt) {
     a.pvel = w.pvel.p; a.pang = w.pang.p;
     a.used = w.usedMask.p; a.tickets = w.tickets.p; a.flowErr = w.flowErr.p; a.qDone = w.qDone.p;
     a.barrier = w.passBarrier.p;
+    if (w.execMode == EXEC_XPASS) w.xpassCtl.alloc(XPASS_CTL * (size_t)(w.iterations + 1));
+    a.xctl = w.xpassCtl.p;
+    a.xctlWords = w.execMode == EXEC_XPASS ? XPASS_CTL * (w.iterations + 1) : 0;
     return a;
 }
 
@@ -1204,6 +1298,21 @@ static int pass_blocks(World& w, int maxColour) {
     }
     const int want = w.persistentBlocksWanted > 0 ? w.persistentBlocksWanted : div_up(std::max(maxColour, 1), PASS_THREADS);
     return std::max(1, std::min(want, w.passBlocksMax));
+}
+
+// Grid of the one-XCD pass kernel: about an eighth of it lands on each XCD, so 8 x enough
+// workgroups for the largest colour in one round on one XCD, never more than can be
+// co-resident, and at most XPASS_MAX_GRID so no XCD's registration byte can overflow.
+static int xpass_blocks(World& w, int maxColour) {
+    if (w.xpassBlocksMax == 0) {
+        int perCU = 0, cus = 0;
+        PGX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void*)k_solve_xpass, XPASS_THREADS, 0));
+        PGX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, w.device));
+        w.xpassBlocksMax = std::max(8, std::min(XPASS_MAX_GRID, perCU * cus));
+    }
+    const int perXcd = w.persistentBlocksWanted > 0 ? div_up(w.persistentBlocksWanted, 8)
+                                                    : div_up(std::max(maxColour, 1), XPASS_THREADS);
+    return std::max(8, std::min(8 * perXcd, w.xpassBlocksMax));
 }
 
 // The start/stop events of one timed k_solve launch (timing mode, first substep of the
@@ -1330,15 +1439,30 @@ void sv_substeps(World& w, int M, float dt) {
     }
     int passIndex = 0;
     const bool onePass = w.execMode == EXEC_PASS && M > 0 && nColors > 0;
+    const bool xcdPass = w.execMode == EXEC_XPASS && M > 0 && nColors > 0;
     int passGrid = 0;
-    if (onePass) {
+    if (onePass || xcdPass) {
         int maxColour = 0;
         for (int c = 0; c < nColors; ++c) maxColour = std::max(maxColour, w.colorStartHost[c + 1] - w.colorStartHost[c]);
-        passGrid = pass_blocks(w, maxColour);
+        passGrid = onePass ? pass_blocks(w, maxColour) : xpass_blocks(w, maxColour);
     }
     auto pass = [&](bool rec, int useBias, float h, int skipSoft) {
         const int p = passIndex++;
         if (M == 0) return;
+        if (xcdPass) {
+            int nc = nColors;
+            int32_t* ctl = w.xpassCtl.p + XPASS_CTL * p;
+            void* args[] = {&a, &nc, &useBias, &h, &skipSoft, &ctl};
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (rec) {
+                solve_events(w, -1, &e0, &e1);
+                PGX_HIP(hipEventRecord(e0, s));
+            }
+            PGX_HIP(hipLaunchCooperativeKernel((const void*)k_solve_xpass, dim3(passGrid), dim3(XPASS_THREADS), args,
+                                               0, s));
+            if (rec) PGX_HIP(hipEventRecord(e1, s));
+            return;
+        }
         if (onePass) {
             // barriers of this substep so far: (nColors - 1) per earlier pass, each counted
             // once per workgroup; the count is cleared by k_prepare

@@ -1386,7 +1386,7 @@ int pgx_world_get_stats(pgx_world* h, pgx_stats* out) {
 int pgx_world_set_exec(pgx_world* h, int mode, int blocks) {
     if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
     if (blocks < 0) return fail(PGX_ERR_INVALID, "blocks must be >= 0");
-    if (mode < PGX_EXEC_COLORS || mode > PGX_EXEC_PASS) return fail(PGX_ERR_INVALID, "unknown execution mode %d", mode);
+    if (mode < PGX_EXEC_COLORS || mode > PGX_EXEC_XPASS) return fail(PGX_ERR_INVALID, "unknown execution mode %d", mode);
     h->w.execMode = mode;
     h->w.persistentBlocksWanted = blocks;
     h->w.persistentBlocks = 0;

@@ -153,6 +153,7 @@ def main(argv=None):
     counter_lines = []
     derived_lines = []
     traffic = None
+    fetch_vals, write_vals = [], []
     for cdir in a.counters_dir:
         cpath = _find(cdir, "counter_collection.csv")
         if not cpath:
@@ -183,13 +184,14 @@ def main(argv=None):
                 elif cname.startswith("WRITE_SIZE") and a.write_scale:
                     extra = f" | {mean * 1024 * a.write_scale:.0f} B |"
                 counter_lines.append(f"| `{kern}` | {cname} | {len(v)} | {mean:.4g}{extra or ' | |'}")
-        if a.traffic_kernel and a.fetch_scale and a.write_scale:
-            tk = per.get(_short(a.traffic_kernel)) or per.get(a.traffic_kernel)
-            if tk:
-                f_ = [v for c, d in tk.items() if c.startswith("FETCH_SIZE") for v in d.values()]
-                w_ = [v for c, d in tk.items() if c.startswith("WRITE_SIZE") for v in d.values()]
-                if f_ and w_:
-                    traffic = (sum(f_) / len(f_) * 1024 * a.fetch_scale, sum(w_) / len(w_) * 1024 * a.write_scale)
+        if a.traffic_kernel:
+            # FETCH_SIZE and WRITE_SIZE come from separate runs: gather them over the runs
+            tk = per.get(_short(a.traffic_kernel)) or per.get(a.traffic_kernel) or {}
+            fetch_vals += [v for c, d in tk.items() if c.startswith("FETCH_SIZE") for v in d.values()]
+            write_vals += [v for c, d in tk.items() if c.startswith("WRITE_SIZE") for v in d.values()]
+    if fetch_vals and write_vals and a.fetch_scale and a.write_scale:
+        traffic = (sum(fetch_vals) / len(fetch_vals) * 1024 * a.fetch_scale,
+                   sum(write_vals) / len(write_vals) * 1024 * a.write_scale)
     if traffic:
         lines += [f"traffic_fetch_bytes_per_launch: {traffic[0]:.0f}",
                   f"traffic_write_bytes_per_launch: {traffic[1]:.0f}",

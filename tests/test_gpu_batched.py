@@ -25,7 +25,7 @@ def world_rows(scene, w):
     return actors, dyn
 
 
-@pytest.mark.parametrize("mode", ["colors", "xcd", "worlds", "pass"])
+@pytest.mark.parametrize("mode", ["colors", "xcd", "worlds", "pass", "xpass"])
 def test_world_same_alone_and_in_mix(api, mode):
     """A world steps bit-identically alone and among 23 others, with the contact passes
     launched per colour, on per-XCD queues (worlds spread over the XCDs) or one workgroup

@@ -69,7 +69,7 @@ def test_served_request_lockstep(api, seed, n):
     assert g.counts()["manifolds"] > 0
 
 
-@pytest.mark.parametrize("mode", ["colors", "pass"])
+@pytest.mark.parametrize("mode", ["colors", "pass", "xpass"])
 def test_mixed_32k_lockstep(api, mode):
     sc = scenes.mixed_lattice(0, side=32)
     log(f"[C2] mixed lattice: {sc.n_dyn} bodies, 12 steps, exec {mode}")
@@ -79,15 +79,17 @@ def test_mixed_32k_lockstep(api, mode):
     assert sc.n_dyn == 32768 and cnt["manifolds"] > 1000
 
 
-def test_pass_mode_equals_colour_mode_settled(api):
+@pytest.mark.parametrize("mode", ["pass", "xpass"])
+def test_pass_mode_equals_colour_mode_settled(api, mode):
     """The 32,768-body scene well into its pile (about 20 colours of thousands of manifolds,
-    so many grid barriers per pass): one launch per pass equals one launch per colour bit
-    for bit at every checked step (the colour mode is the one the locksteps above pin)."""
+    so many barriers per pass): one launch per pass (on every XCD, or on one) equals one
+    launch per colour bit for bit at every checked step (the colour mode is the one the
+    locksteps above pin)."""
     sc = scenes.mixed_lattice(0, side=32)
-    log("[C2] pass vs colours, 150 steps")
+    log(f"[C2] {mode} vs colours, 150 steps")
     a = api.World(sc, substeps=4, iterations=2)
     b = api.World(sc, substeps=4, iterations=2)
-    b.set_exec("pass")
+    b.set_exec(mode)
     for k in range(150):
         a.step(1 / 60)
         b.step(1 / 60)

@@ -78,7 +78,7 @@ def test_free_fall_closed_form(api):
     assert abs(gs[2][0, 1] - (-9.81 * 30 / 60)) < 1e-3
 
 
-@pytest.mark.parametrize("mode", ["xcd", "flow", "colors", "cooperative", "worlds", "pass"])
+@pytest.mark.parametrize("mode", ["xcd", "flow", "colors", "cooperative", "worlds", "pass", "xpass"])
 def test_pyramid_lockstep_bitexact(api, mode):
     """20-layer box pyramid, dt 1/60, 4 substeps, 2 iterations: 120 steps in lockstep, in
     every execution mode of the contact passes (dependency-driven, one launch per colour,
@@ -110,7 +110,7 @@ def test_cluster_lockstep_with_restitution_and_soft(api):
     lockstep(sc, 60)
 
 
-@pytest.mark.parametrize("seed,mode", [(0, "colors"), (1, "colors"), (0, "worlds"), (1, "pass")])
+@pytest.mark.parametrize("seed,mode", [(0, "colors"), (1, "colors"), (0, "worlds"), (1, "pass"), (0, "xpass")])
 def test_joints_lockstep_bitexact(api, seed, mode):
     """Revolute chains and ragdolls (spherical / universal / revolute rows, SIMD colours);
     contact passes per colour or per world with the joint colours between them."""
