@@ -469,6 +469,10 @@ typedef struct pgx_tuning {
     int eager_rows_max;   /* colours of at most this many manifolds issue all four point
                              slots' row loads at once instead of after the point count:
                              shorter latency, more bytes (>= 0) */
+    int pair_search;      /* how the broadphase finds a step's pairs: 0 = a uniform grid of
+                             the dynamic colliders (cells as large as the largest one; a
+                             step whose fullest cell holds more than 512 colliders uses the
+                             tree walk instead), 1 = a walk of the LBVH over escape links */
 } pgx_tuning;
 int pgx_world_set_tuning(pgx_world* w, const pgx_tuning* t);
 int pgx_world_get_tuning(pgx_world* w, pgx_tuning* t);

@@ -91,7 +91,7 @@ class PgxServerStepInfo(ctypes.Structure):
 
 class PgxTuning(ctypes.Structure):
     _fields_ = [("solve_block", ctypes.c_int), ("color_batch", ctypes.c_int), ("pairs_per_lane", ctypes.c_int),
-                ("color_rounds", ctypes.c_int), ("eager_rows_max", ctypes.c_int)]
+                ("color_rounds", ctypes.c_int), ("eager_rows_max", ctypes.c_int), ("pair_search", ctypes.c_int)]
 
 
 class PgxCharacter(ctypes.Structure):
@@ -455,7 +455,7 @@ class World:
 
     def set_tuning(self, **values):
         """Launch shapes (solve_block, color_batch, pairs_per_lane, color_rounds,
-        eager_rows_max); the results do not depend on them."""
+        eager_rows_max, pair_search); the results do not depend on them."""
         t = PgxTuning(**dict(self.tuning(), **values))
         check(self.L.pgx_world_set_tuning(self.h, ctypes.byref(t)))
 

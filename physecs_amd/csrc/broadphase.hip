@@ -69,35 +69,62 @@ __device__ __forceinline__ float ord_dec(uint32_t u) {
     return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
-__global__ void k_scene_bounds_init(uint32_t* wb) {
-    if (threadIdx.x < 3) { wb[threadIdx.x] = 0xffffffffu; wb[3 + threadIdx.x] = 0u; }
+__global__ void k_scene_bounds_init(uint32_t* wb, double* mom) {
+    if (threadIdx.x < 3) { wb[threadIdx.x] = 0xffffffffu; wb[3 + threadIdx.x] = 0u; wb[6 + threadIdx.x] = 0u; }
+    if (threadIdx.x < 6) mom[threadIdx.x] = 0.0;
 }
 
-// one block of 1024 threads reduces its share in LDS, then 6 atomics per block
+// The dynamic colliders' centres: their box (min, max per axis) and their sums and sums of
+// squares per axis (the Morton box and the grid below); and the colliders' largest extent
+// per axis (wb[6..8], as float bits: extents are >= 0, so the bits order like the values). One block of 1024 threads reduces its share with
+// wave shuffles and per-wave partials in LDS, then 6 integer and 6 double atomics per block.
 __global__ __launch_bounds__(1024) void k_scene_bounds(int n, const int32_t* __restrict__ dynCols,
                                                        const float4* __restrict__ bmin,
-                                                       const float4* __restrict__ bmax, uint32_t* __restrict__ wb) {
-    __shared__ uint32_t red[6];
-    if (threadIdx.x < 3) { red[threadIdx.x] = 0xffffffffu; red[3 + threadIdx.x] = 0u; }
+                                                       const float4* __restrict__ bmax, uint32_t* __restrict__ wb,
+                                                       double* __restrict__ mom) {
+    __shared__ uint32_t red[9];
+    __shared__ double part[1024 / WAVE][6];
+    if (threadIdx.x < 3) { red[threadIdx.x] = 0xffffffffu; red[3 + threadIdx.x] = 0u; red[6 + threadIdx.x] = 0u; }
     __syncthreads();
-    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+    uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u}, ex[3] = {0u, 0u, 0u};
+    double sm[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         int c = dynCols[i];
         float4 a = bmin[c], b = bmax[c];
-        uint32_t e[3] = {ord_enc(0.5f * (a.x + b.x)), ord_enc(0.5f * (a.y + b.y)), ord_enc(0.5f * (a.z + b.z))};
-        for (int k = 0; k < 3; ++k) { mn[k] = min(mn[k], e[k]); mx[k] = max(mx[k], e[k]); }
+        const float cc[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+        const float ee[3] = {b.x - a.x, b.y - a.y, b.z - a.z};
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t e = ord_enc(cc[k]);
+            mn[k] = min(mn[k], e);
+            mx[k] = max(mx[k], e);
+            ex[k] = max(ex[k], __float_as_uint(fmaxf(ee[k], 0.f)));
+            sm[k] += (double)cc[k];
+            sm[3 + k] += (double)cc[k] * (double)cc[k];
+        }
     }
-    for (int off = WAVE / 2; off >= 1; off >>= 1)
+    for (int off = WAVE / 2; off >= 1; off >>= 1) {
         for (int k = 0; k < 3; ++k) {
             mn[k] = min(mn[k], (uint32_t)__shfl_xor((int)mn[k], off));
             mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], off));
+            ex[k] = max(ex[k], (uint32_t)__shfl_xor((int)ex[k], off));
         }
-    if ((threadIdx.x & (WAVE - 1)) == 0)
-        for (int k = 0; k < 3; ++k) { atomicMin(&red[k], mn[k]); atomicMax(&red[3 + k], mx[k]); }
+        for (int k = 0; k < 6; ++k) sm[k] += __shfl_xor(sm[k], off);
+    }
+    const int wv = threadIdx.x / WAVE;
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        for (int k = 0; k < 3; ++k) { atomicMin(&red[k], mn[k]); atomicMax(&red[3 + k], mx[k]); atomicMax(&red[6 + k], ex[k]); }
+        for (int k = 0; k < 6; ++k) part[wv][k] = sm[k];
+    }
     __syncthreads();
     if (threadIdx.x < 3) {
         atomicMin(&wb[threadIdx.x], red[threadIdx.x]);
         atomicMax(&wb[3 + threadIdx.x], red[3 + threadIdx.x]);
+        atomicMax(&wb[6 + threadIdx.x], red[6 + threadIdx.x]);
+    }
+    if (threadIdx.x < 6) {
+        double t = 0.0;
+        for (int w = 0; w < (int)blockDim.x / WAVE; ++w) t += part[w][threadIdx.x];
+        atomicAdd(&mom[threadIdx.x], t);
     }
 }
 
@@ -109,9 +136,15 @@ __device__ __forceinline__ uint32_t spread10(uint32_t v) {
     return v;
 }
 
+// Morton code of each dynamic collider's centre, 10 bits per axis, over the centres' box cut
+// to mean +- 4 standard deviations per axis: a few bodies far from the rest (thrown out of a
+// pile) would otherwise stretch the box until thousands of the pile's bodies share a code
+// and the tree over them loses its shape. Centres outside the cut box get the edge's code.
+// The codes order the tree's leaves only; the pairs the walk finds do not depend on them.
 __global__ void k_morton(int n, const int32_t* __restrict__ dynCols, const int32_t* __restrict__ cworld,
                          const float4* __restrict__ bmin, const float4* __restrict__ bmax,
-                         const uint32_t* __restrict__ wb, uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+                         const uint32_t* __restrict__ wb, const double* __restrict__ mom, uint64_t* __restrict__ keys,
+                         int32_t* __restrict__ vals) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int c = dynCols[i];
@@ -120,6 +153,10 @@ __global__ void k_morton(int n, const int32_t* __restrict__ dynCols, const int32
     uint32_t q[3];
     for (int k = 0; k < 3; ++k) {
         float lo = ord_dec(wb[k]), hi = ord_dec(wb[3 + k]);
+        const double mean = mom[k] / n, var = fmax(mom[3 + k] / n - mean * mean, 0.0), sd = sqrt(var);
+        const float clo = (float)(mean - 4.0 * sd), chi = (float)(mean + 4.0 * sd);
+        if (clo > lo && clo < hi) lo = clo;
+        if (chi < hi && chi > lo) hi = chi;
         float ext = hi - lo;
         float t = ext > 0.f ? (cc[k] - lo) / ext : 0.5f;
         t = fminf(fmaxf(t, 0.f), 1.f);
@@ -128,6 +165,302 @@ __global__ void k_morton(int n, const int32_t* __restrict__ dynCols, const int32
     uint32_t m = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
     keys[i] = ((uint64_t)cworld[c] << 30) | m;
     vals[i] = c;
+}
+
+__device__ __forceinline__ bool overlaps(float4 amin, float4 amax, float4 bmin, float4 bmax) {
+    // inclusive, as the sweep's `max < min` rejections (src/Physecs.cpp:152-154)
+    return !(amax.x < bmin.x || amin.x > bmax.x || amax.y < bmin.y || amin.y > bmax.y || amax.z < bmin.z ||
+             amin.z > bmax.z);
+}
+
+// Workgroups are dispatched round-robin to the 8 XCDs (workgroup b on XCD b % 8), each
+// with its own L2. The searches of Morton-adjacent leaves touch the same part of the tree (or the
+// same cells), so workgroup b takes logical block xcd_block(b): XCD x searches for one
+// contiguous eighth of the leaves and its L2 holds that region's data. (Any other dispatch order only costs L2 hits.)
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int x = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
+    return x * q + min(x, r) + i;
+}
+
+// ------------------------------------------------------------------ uniform grid of the dynamic colliders
+// The step's pair search (pgx_tuning.pair_search 0). Cells are boxes of side h per axis, h
+// at least the largest dynamic collider extent along that axis (with 1% and a few float
+// ulps of slack), laid over the centres' box cut to mean +- 4 standard deviations (the
+// Morton box); a collider sits in the cell of its centre, clamped into the grid. Two
+// overlapping AABBs have centres at most h apart per axis, so their cells differ by at most
+// one per axis; clamping only merges cells, which keeps that. A cell is a key
+// world << 30 | Morton(cell x, y, z) (at most 1024 cells per axis), the colliders are sorted
+// by it, and a hash table maps each occupied cell to its run [start, end) of the sorted
+// colliders. A collider then tests the colliders of its own cell that come after it and all
+// colliders of the 13 neighbour cells that lie forward of its cell in (dx, dy, dz) order:
+// of two distinct neighbour cells exactly one is forward of the other, so each pair is found
+// once. The tests are the sweep's inclusive AABB overlap, so the pair set is the SAP's.
+constexpr int GRID_LANES = 16;      // per collider: its own cell, 13 forward cells, 2 halves of the statics
+constexpr int GRID_SLOTS = 8;       // pairs a lane holds before its wave appends them
+constexpr int GRID_MAX_RUN = 512;   // a fuller cell hands the step to the tree walk
+constexpr uint64_t GRID_EMPTY = ~0ull;
+
+__device__ __forceinline__ uint32_t compact10(uint32_t v) {   // every third bit, 10 of them
+    uint32_t r = 0;
+    for (int b = 0; b < 10; ++b) r |= ((v >> (3 * b)) & 1u) << b;
+    return r;
+}
+
+// murmur3's 64-bit finaliser (fmix64)
+__device__ __forceinline__ uint32_t cell_hash(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return (uint32_t)k;
+}
+
+// the grid's origin and cell size per axis, the same in every thread of a step
+__device__ __forceinline__ void grid_frame(const uint32_t* __restrict__ wb, const double* __restrict__ mom, int n,
+                                           float lo[3], float h[3]) {
+    for (int k = 0; k < 3; ++k) {
+        float l = ord_dec(wb[k]), hi = ord_dec(wb[3 + k]);
+        const float mag = fmaxf(fabsf(l), fabsf(hi));
+        const double mean = mom[k] / n, var = fmax(mom[3 + k] / n - mean * mean, 0.0), sd = sqrt(var);
+        const float clo = (float)(mean - 4.0 * sd), chi = (float)(mean + 4.0 * sd);
+        if (clo > l && clo < hi) l = clo;
+        if (chi < hi && chi > l) hi = chi;
+        float cell = __uint_as_float(wb[6 + k]) * 1.01f + 1e-5f * mag + 1e-6f;
+        const float span = hi - l;
+        if (span > 1000.f * cell) cell = span / 1000.f;   // at most ~1000 cells per axis, each still >= the extent
+        lo[k] = l;
+        h[k] = cell;
+    }
+}
+
+__global__ void k_cell_codes(int n, const int32_t* __restrict__ dynCols, const int32_t* __restrict__ cworld,
+                             const float4* __restrict__ bmin, const float4* __restrict__ bmax,
+                             const uint32_t* __restrict__ wb, const double* __restrict__ mom, uint64_t* __restrict__ keys,
+                             int32_t* __restrict__ vals) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float lo[3], h[3];
+    grid_frame(wb, mom, n, lo, h);
+    int c = dynCols[i];
+    float4 a = bmin[c], b = bmax[c];
+    const float cc[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) q[k] = (uint32_t)fminf(fmaxf(floorf((cc[k] - lo[k]) / h[k]), 0.f), 1023.f);
+    keys[i] = ((uint64_t)cworld[c] << 30) | (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    vals[i] = c;
+}
+
+// the first collider of each cell's run claims a table slot (linear probing)
+__global__ void k_cell_insert(int n, const uint64_t* __restrict__ keys, uint32_t mask, uint64_t* __restrict__ tabKey,
+                              int32_t* __restrict__ tabStart) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    if (i > 0 && keys[i - 1] == k) return;
+    uint32_t slot = cell_hash(k) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {   // the table is at least twice the cells
+        const unsigned long long prev = atomicCAS((unsigned long long*)&tabKey[slot], GRID_EMPTY, k);
+        if (prev == GRID_EMPTY) {
+            tabStart[slot] = i;
+            return;
+        }
+        slot = (slot + 1) & mask;
+    }
+}
+
+__device__ __forceinline__ int cell_find(uint64_t k, uint32_t mask, const uint64_t* __restrict__ tabKey) {
+    uint32_t slot = cell_hash(k) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+        const uint64_t t = tabKey[slot];
+        if (t == k) return (int)slot;
+        if (t == GRID_EMPTY) return -1;
+        slot = (slot + 1) & mask;
+    }
+    return -1;
+}
+
+// the last collider of each run records its end, and the longest run (counters[15])
+__global__ void k_cell_ends(int n, const uint64_t* __restrict__ keys, uint32_t mask, const uint64_t* __restrict__ tabKey,
+                            const int32_t* __restrict__ tabStart, int32_t* __restrict__ tabEnd,
+                            int32_t* __restrict__ counters) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    if (i + 1 < n && keys[i + 1] == k) return;
+    const int slot = cell_find(k, mask, tabKey);
+    if (slot < 0) { atomicOr(&counters[14], 2); return; }   // never expected: the run start inserted it
+    tabEnd[slot] = i + 1;
+    atomicMax(&counters[15], i + 1 - tabStart[slot]);
+}
+
+// The grid search's record of each dynamic collider in sorted order, one 32-byte line:
+// {min xyz, world-local index}, {max xyz, entity}; a disabled collider gets an empty box, so
+// it overlaps nothing. And the world's non-dynamic colliders, as k_pack writes them for the
+// tree walk.
+__global__ void k_grid_pack(int n, const int32_t* __restrict__ order, const float4* __restrict__ cbmin,
+                            const float4* __restrict__ cbmax, const int32_t* __restrict__ cflags,
+                            const uint32_t* __restrict__ centity, const int32_t* __restrict__ clocal,
+                            float4* __restrict__ gridRec, int nStatic, const int32_t* __restrict__ staticCols,
+                            float4* __restrict__ statA, float4* __restrict__ statB, int32_t* __restrict__ statL) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        const int c = order[t];
+        float4 a = cbmin[c], b = cbmax[c];
+        if (!(cflags[c] & CF_ENABLE)) {
+            a = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+            b = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+        }
+        gridRec[2 * t] = make_float4(a.x, a.y, a.z, __int_as_float(clocal[c]));
+        gridRec[2 * t + 1] = make_float4(b.x, b.y, b.z, __int_as_float((int)centity[c]));
+    }
+    if (t < nStatic) {
+        const int c = staticCols[t];
+        const float4 a = cbmin[c], b = cbmax[c];
+        statA[t] = make_float4(a.x, a.y, a.z, __int_as_float(cflags[c]));
+        statB[t] = make_float4(b.x, b.y, b.z, __int_as_float((int)centity[c]));
+        statL[t] = clocal[c];
+    }
+}
+
+// GRID_LANES lanes per enabled dynamic collider s (sorted order): lane 0 its own cell after
+// s, lanes 1-13 the forward neighbour cells, lanes 14-15 the two halves of its world's
+// non-dynamic colliders. Two candidates per round of loads (more registers would cost the
+// occupancy that keeps every wave of the 32k-collider grid resident at once). Found pairs
+// wait in the lane's
+// LDS slots; at the end the workgroup scans its lanes' counts and appends all its pairs with
+// one atomic on the pair count (a lane whose slots fill before that appends with its wave).
+// When the fullest cell holds more than GRID_MAX_RUN colliders (cells made large by a few
+// large colliders) every lane leaves at once and counters[14] hands the step to the tree walk.
+constexpr int GRID_BLOCK = 1024;
+__global__ __launch_bounds__(GRID_BLOCK) void k_grid_pairs(
+    int n, const uint64_t* __restrict__ keys, const float4* __restrict__ rec, uint32_t mask,
+    const uint64_t* __restrict__ tabKey, const int32_t* __restrict__ tabStart, const int32_t* __restrict__ tabEnd,
+    const int32_t* __restrict__ worldStaticStart, const float4* __restrict__ statA, const float4* __restrict__ statB,
+    const int32_t* __restrict__ statL, int localBits, int32_t* __restrict__ counters, int cap,
+    uint64_t* __restrict__ pkeys) {
+    if (counters[15] > GRID_MAX_RUN) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&counters[14], 1);
+        return;
+    }
+    __shared__ uint64_t found[GRID_SLOTS][GRID_BLOCK];
+    __shared__ int32_t waveTotal[GRID_BLOCK / WAVE];
+    __shared__ int32_t blockBase;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+    const long long g = (long long)xcd_block(blockIdx.x, gridDim.x) * GRID_BLOCK + tid;
+    const int s = (int)(g / GRID_LANES), k = (int)(g % GRID_LANES);
+    bool active = false, statics = false;
+    int j = 0, end = 0, wd = 0, myLocal = 0;
+    uint32_t myEnt = 0;
+    float4 qmin = make_float4(0.f, 0.f, 0.f, 0.f), qmax = qmin;
+    if (s < n) {
+        qmin = rec[2 * s];
+        qmax = rec[2 * s + 1];
+    }
+    if (s < n && qmin.x <= qmax.x) {   // an enabled collider (a disabled one has an empty box)
+        const uint64_t key = keys[s];
+        wd = (int)(key >> 30);
+        myLocal = __float_as_int(qmin.w);
+        myEnt = (uint32_t)__float_as_int(qmax.w);
+        if (k < 14) {
+            int dx = 0, dy = 0, dz = 0;
+            if (k >= 1 && k <= 9) { dx = 1; dy = (k - 1) / 3 - 1; dz = (k - 1) % 3 - 1; }
+            else if (k >= 10 && k <= 12) { dy = 1; dz = k - 11; }
+            else if (k == 13) { dz = 1; }
+            const uint32_t m = (uint32_t)(key & 0x3fffffffu);
+            const int x = (int)compact10(m >> 2) + dx, y = (int)compact10(m >> 1) + dy, z = (int)compact10(m) + dz;
+            if (x >= 0 && x < 1024 && y >= 0 && y < 1024 && z >= 0 && z < 1024) {
+                const uint64_t nk = ((uint64_t)wd << 30) | (spread10((uint32_t)x) << 2) | (spread10((uint32_t)y) << 1) |
+                                    spread10((uint32_t)z);
+                const int slot = cell_find(nk, mask, tabKey);
+                if (slot >= 0) {
+                    j = k == 0 ? s + 1 : tabStart[slot];
+                    end = tabEnd[slot];
+                    active = j < end;
+                }
+            }
+        } else {
+            const int a = worldStaticStart[wd], b = worldStaticStart[wd + 1], mid = a + (b - a + 1) / 2;
+            j = k == 14 ? a : mid;
+            end = k == 14 ? mid : b;
+            active = j < end;
+            statics = true;
+        }
+    }
+    int held = 0;
+    auto pairKey = [&](uint32_t e2, int l2) {
+        const int l0 = myEnt < e2 ? myLocal : l2;
+        const int l1 = myEnt < e2 ? l2 : myLocal;
+        return ((uint64_t)wd << (2 * localBits)) | ((uint64_t)l0 << localBits) | (uint64_t)l1;
+    };
+    auto waveFlush = [&]() {   // a lane's slots are full: the wave appends its pairs now
+        int incl = held;
+        for (int o = 1; o < WAVE; o <<= 1) {
+            int t = __shfl_up(incl, o);
+            if (lane >= o) incl += t;
+        }
+        const int total = __shfl(incl, WAVE - 1);
+        int base = 0;
+        if (lane == WAVE - 1 && total) base = atomicAdd(counters, total);
+        base = __shfl(base, WAVE - 1) + incl - held;
+        for (int q = 0; q < held; ++q)
+            if (base + q < cap) pkeys[base + q] = found[q][tid];
+        held = 0;
+    };
+    // two candidates per round of loads: the cell's dynamic colliders (lanes 0-13) or the
+    // world's non-dynamic ones (lanes 14-15)
+    while (__ballot(active)) {
+        bool hit0 = false, hit1 = false;
+        uint64_t c0 = 0, c1 = 0;
+        if (active) {
+            const bool two = j + 1 < end;
+            float4 a0, b0, a1, b1;
+            bool ok0 = true, ok1 = true;
+            if (statics) {
+                a0 = statA[j]; b0 = statB[j];
+                a1 = two ? statA[j + 1] : a0; b1 = two ? statB[j + 1] : b0;
+                const int l0 = statL[j], l1 = two ? statL[j + 1] : l0;
+                ok0 = (__float_as_int(a0.w) & CF_ENABLE) != 0;
+                ok1 = (__float_as_int(a1.w) & CF_ENABLE) != 0;
+                a0.w = __int_as_float(l0);
+                a1.w = __int_as_float(l1);
+            } else {
+                a0 = rec[2 * j]; b0 = rec[2 * j + 1];
+                a1 = two ? rec[2 * j + 2] : a0; b1 = two ? rec[2 * j + 3] : b0;
+            }
+            const uint32_t e0 = (uint32_t)__float_as_int(b0.w), e1 = (uint32_t)__float_as_int(b1.w);
+            hit0 = ok0 && e0 != myEnt && overlaps(qmin, qmax, a0, b0);
+            hit1 = two && ok1 && e1 != myEnt && overlaps(qmin, qmax, a1, b1);
+            if (hit0) c0 = pairKey(e0, __float_as_int(a0.w));
+            if (hit1) c1 = pairKey(e1, __float_as_int(a1.w));
+            j += two ? 2 : 1;
+            active = j < end;
+        }
+        if (__ballot(held + 2 > GRID_SLOTS)) waveFlush();
+        if (hit0) found[held++][tid] = c0;
+        if (hit1) found[held++][tid] = c1;
+    }
+    // the workgroup's remaining pairs: one atomic for all of them
+    int incl = held;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        int t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
+    }
+    if (lane == WAVE - 1) waveTotal[wv] = incl;
+    __syncthreads();
+    if (tid == 0) {
+        int total = 0;
+        for (int w = 0; w < GRID_BLOCK / WAVE; ++w) {
+            const int t = waveTotal[w];
+            waveTotal[w] = total;
+            total += t;
+        }
+        blockBase = total ? atomicAdd(counters, total) : 0;
+    }
+    __syncthreads();
+    const int base = blockBase + waveTotal[wv] + incl - held;
+    for (int q = 0; q < held; ++q)
+        if (base + q < cap) pkeys[base + q] = found[q][tid];
 }
 
 // ------------------------------------------------------------------ LBVH build
@@ -363,11 +696,6 @@ __global__ void k_pack(int n, const float4* __restrict__ nmin, const float4* __r
 }
 
 // ------------------------------------------------------------------ traversal
-__device__ __forceinline__ bool overlaps(float4 amin, float4 amax, float4 bmin, float4 bmax) {
-    // inclusive, as the sweep's `max < min` rejections (src/Physecs.cpp:152-154)
-    return !(amax.x < bmin.x || amin.x > bmax.x || amax.y < bmin.y || amin.y > bmax.y || amax.z < bmin.z ||
-             amin.z > bmax.z);
-}
 
 constexpr int TRAV_SLOTS = 8;   // pairs a lane holds before its wave appends them
 
@@ -388,7 +716,7 @@ __global__ __launch_bounds__(256) void k_traverse(
     const int32_t* __restrict__ worldStaticStart, const float4* __restrict__ statA,
     const float4* __restrict__ statB, const int32_t* __restrict__ statL, int localBits,
     int32_t* __restrict__ pairCount, int cap, uint64_t* __restrict__ pkeys) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     bool active = false;
     float4 qmin = make_float4(0.f, 0.f, 0.f, 0.f), qmax = qmin;
     int node = -1, stop = -1, st = 0, stEnd = 0;
@@ -415,8 +743,7 @@ __global__ __launch_bounds__(256) void k_traverse(
     // candidates wait in the lane's own LDS slots (a static pair as its key, a dynamic leaf
     // as its index, marked in treeMask); when a lane's slots are full (or the walk ends) the
     // wave resolves and appends all of its lanes' pairs with one atomic on the counter
-    __shared__ uint64_t found[256 * TRAV_SLOTS];
-    uint64_t* mine = found + threadIdx.x * TRAV_SLOTS;
+    __shared__ uint64_t found[TRAV_SLOTS][256];   // slot-major: a wave's lanes hit consecutive words
     int held = 0;
     uint32_t treeMask = 0;
     auto pairKey = [&](uint32_t e2, int l2) {
@@ -430,18 +757,18 @@ __global__ __launch_bounds__(256) void k_traverse(
         int4 rec[TRAV_SLOTS];
 #pragma unroll
         for (int j = 0; j < TRAV_SLOTS; ++j)
-            if ((treeMask >> j) & 1u) rec[j] = travC[n - 1 + (int)mine[j]];
+            if ((treeMask >> j) & 1u) rec[j] = travC[n - 1 + (int)found[j][threadIdx.x]];
         int kept = 0;
 #pragma unroll
         for (int j = 0; j < TRAV_SLOTS; ++j) {
             if (j >= held) break;
-            uint64_t key = mine[j];
+            uint64_t key = found[j][threadIdx.x];
             bool keep = true;
             if ((treeMask >> j) & 1u) {
                 keep = (uint32_t)rec[j].y != myEnt;
                 key = pairKey((uint32_t)rec[j].y, rec[j].z);
             }
-            if (keep) mine[kept++] = key;
+            if (keep) found[kept++][threadIdx.x] = key;
         }
         int incl = kept;   // inclusive prefix sum of kept over the wave
         for (int o = 1; o < WAVE; o <<= 1) {
@@ -453,7 +780,7 @@ __global__ __launch_bounds__(256) void k_traverse(
         if (lane == WAVE - 1 && total) base = atomicAdd(pairCount, total);
         base = __shfl(base, WAVE - 1) + incl - kept;
         for (int j = 0; j < kept; ++j)
-            if (base + j < cap) pkeys[base + j] = mine[j];
+            if (base + j < cap) pkeys[base + j] = found[j][threadIdx.x];
         held = 0;
         treeMask = 0;
     };
@@ -489,9 +816,9 @@ __global__ __launch_bounds__(256) void k_traverse(
         if (__ballot(held + 2 > TRAV_SLOTS)) flush();
         if (emitLeaf) {
             treeMask |= 1u << held;
-            mine[held++] = (uint64_t)leaf;
+            found[held++][threadIdx.x] = (uint64_t)leaf;
         }
-        if (emitStatic) mine[held++] = skey;
+        if (emitStatic) found[held++][threadIdx.x] = skey;
     }
     if (__ballot(held > 0)) flush();
 }
@@ -510,11 +837,11 @@ void bp_build(World& w) {
     int worldShift = 30;
     if (n > 0) {
         // quantisation box -> Morton codes of the dynamic colliders -> sort
-        hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p);
+        hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p, w.sceneMoments.p);
         hipLaunchKernelGGL(k_scene_bounds, dim3(std::min(64, div_up(n, 1024))), dim3(1024), 0, s, n, w.dynCols.p,
-                           w.cbmin.p, w.cbmax.p, w.worldBoundsI.p);
+                           w.cbmin.p, w.cbmax.p, w.worldBoundsI.p, w.sceneMoments.p);
         hipLaunchKernelGGL(k_morton, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.dynCols.p, w.cworld.p, w.cbmin.p,
-                           w.cbmax.p, w.worldBoundsI.p, w.mkeys.p, w.mvals.p);
+                           w.cbmax.p, w.worldBoundsI.p, w.sceneMoments.p, w.mkeys.p, w.mvals.p);
         PGX_LAUNCH_CHECK();
         radix_sort_u64(s, w.mkeys.p, w.mvals.p, n, 30 + w.worldBits, w.sortT);
         fill_i32(s, w.worldRoot.p, -1, W);
@@ -552,24 +879,75 @@ void bp_build(World& w) {
     }
 }
 
+// The grid's hash table: at least twice the dynamic colliders (so at least twice the cells)
+static uint32_t grid_table_size(int n) {
+    uint32_t t = 64;
+    while (t < 2u * (uint32_t)std::max(n, 1)) t <<= 1;
+    return t;
+}
+
+// The step's pairs on the uniform grid: cell codes of the dynamic colliders, sorted; the
+// cells' runs in the hash table; the grid search. (The tree walk's inputs are not built.)
+static void bp_grid(World& w) {
+    hipStream_t s = w.stream;
+    const int n = w.nDynCol;
+    const uint32_t T = grid_table_size(n);
+    hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p, w.sceneMoments.p);
+    hipLaunchKernelGGL(k_scene_bounds, dim3(std::min(64, div_up(n, 1024))), dim3(1024), 0, s, n, w.dynCols.p,
+                       w.cbmin.p, w.cbmax.p, w.worldBoundsI.p, w.sceneMoments.p);
+    hipLaunchKernelGGL(k_cell_codes, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.dynCols.p, w.cworld.p, w.cbmin.p,
+                       w.cbmax.p, w.worldBoundsI.p, w.sceneMoments.p, w.mkeys.p, w.mvals.p);
+    PGX_LAUNCH_CHECK();
+    radix_sort_u64(s, w.mkeys.p, w.mvals.p, n, 30 + w.worldBits, w.sortT);
+    const int nStatic = w.C - w.nDynCol;
+    hipLaunchKernelGGL(k_grid_pack, dim3(div_up(std::max(n, nStatic), 256)), dim3(256), 0, s, n, w.mvals.p,
+                       w.cbmin.p, w.cbmax.p, w.cflags.p, w.centity.p, w.clocal.p, w.gridRec.p, nStatic, w.staticCols.p,
+                       w.statA.p, w.statB.p, w.statL.p);
+    fill_u64(s, w.gridKey.p, GRID_EMPTY, T);
+    hipLaunchKernelGGL(k_cell_insert, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, T - 1, w.gridKey.p,
+                       w.gridStart.p);
+    hipLaunchKernelGGL(k_cell_ends, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, T - 1, w.gridKey.p,
+                       w.gridStart.p, w.gridEnd.p, w.counters.p);
+    const long long lanes = (long long)n * GRID_LANES;
+    hipLaunchKernelGGL(k_grid_pairs, dim3((unsigned)div_up(lanes, GRID_BLOCK)), dim3(GRID_BLOCK), 0, s, n, w.mkeys.p,
+                       w.gridRec.p, T - 1, w.gridKey.p, w.gridStart.p, w.gridEnd.p, w.worldStaticStart.p, w.statA.p,
+                       w.statB.p, w.statL.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
+    PGX_LAUNCH_CHECK();
+}
+
 int bp_find_pairs(World& w) {
     hipStream_t s = w.stream;
     int n = w.nDynCol;
     int worldShift = 30;
     // [6] traversal visit bound, [7] refit, [8] convex pairs, [9] EPA capacity errors,
     // [10] mesh pairs, [11] mesh manifold slots, [12] mesh capacity errors, [13] mesh
-    // candidate triangles
-    fill_i32(s, w.counters.p, 0, 14);
-    bp_build(w);
-    if (n > 0) {
-        hipLaunchKernelGGL(k_traverse, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift, w.travA.p,
-                           w.travB.p, w.travC.p, w.worldStop.p, w.worldStaticStart.p, w.statA.p, w.statB.p,
-                           w.statL.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
-        PGX_LAUNCH_CHECK();
-    }
-    int cnt[8] = {0};
+    // candidate triangles, [14] the grid handed the step to the tree walk (1) or lost a cell
+    // (2, never expected), [15] the grid's fullest cell
+    fill_i32(s, w.counters.p, 0, 16);
+    auto tree_walk = [&]() {
+        bp_build(w);
+        if (n > 0) {
+            hipLaunchKernelGGL(k_traverse, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, worldShift, w.travA.p,
+                               w.travB.p, w.travC.p, w.worldStop.p, w.worldStaticStart.p, w.statA.p, w.statB.p,
+                               w.statL.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
+            PGX_LAUNCH_CHECK();
+        }
+    };
+    const bool grid = n > 0 && w.tune.pairSearch == 0;
+    if (grid) bp_grid(w);
+    else tree_walk();
+    int cnt[16] = {0};
     PGX_HIP(hipMemcpyAsync(cnt, w.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
     PGX_HIP(hipStreamSynchronize(s));
+    if (grid && (cnt[14] & 2)) throw PgxError{PGX_ERR_HIP, "broadphase grid lost a cell of its hash table"};
+    if (grid && cnt[14]) {
+        // a cell too full for the grid: the tree walk finds the step's pairs instead
+        ++w.gridFallbacks;
+        fill_i32(s, w.counters.p, 0, 1);
+        tree_walk();
+        PGX_HIP(hipMemcpyAsync(cnt, w.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipStreamSynchronize(s));
+    }
     int nPairs = cnt[0];
     if (cnt[6]) throw PgxError{PGX_ERR_INVALID, "broadphase traversal exceeded its visit bound (malformed tree)"};
     if (nPairs > w.pairCap) throw PgxError{PGX_ERR_CAPACITY, "pair capacity exceeded: " + std::to_string(nPairs) +

@@ -179,6 +179,11 @@ struct World {
     // every visit is the same three loads. A: {min, next if it overlaps}, B: {max, escape},
     // C: {last leaf of the subtree (a leaf's own index), entity, world-local index, flags}.
     DBuf<float4> travA, travB;
+    DBuf<double> sceneMoments;           // sums and sums of squares of the dynamic centres (Morton box)
+    long gridFallbacks = 0;              // steps whose grid had a cell run too long (tree walk used)
+    DBuf<uint64_t> gridKey;              // the grid's hash table: cell key, run start and end
+    DBuf<int32_t> gridStart, gridEnd;
+    DBuf<float4> gridRec;                // the grid search's collider records: 2 float4 per dynamic collider
     DBuf<int4> travC;
     // the non-dynamic colliders in static-list order: {min, flags}, {max, entity}, local index
     DBuf<float4> statA, statB;
@@ -287,6 +292,7 @@ struct World {
         int pairsPerLane = 1;            // narrowphase pairs per lane (1, 2, 4)
         int colorRounds = 4;             // least colouring rounds queued per read-back
         int eagerMax = 32768;            // colours of at most this many manifolds load rows eagerly
+        int pairSearch = 0;              // broadphase pair search: 0 uniform grid, 1 tree walk
     } tune;
     bool graph = false;                  // substep loop as one HIP graph launch (colour mode)
     // per-world passes: first dynamic body of each world (W + 1), whether every world's

@@ -684,13 +684,14 @@ int pgx_server_set_tuning(pgx_server* S, const pgx_tuning* t) {
     if (!S || !t) return srv_fail(PGX_ERR_INVALID, "NULL argument");
     if ((t->solve_block != 64 && t->solve_block != 128 && t->solve_block != 256) || t->color_batch < 0 ||
         (t->pairs_per_lane != 1 && t->pairs_per_lane != 2 && t->pairs_per_lane != 4) || t->color_rounds < 1 ||
-        t->color_rounds > 1024 || t->eager_rows_max < 0)
+        t->color_rounds > 1024 || t->eager_rows_max < 0 || (t->pair_search != 0 && t->pair_search != 1))
         return srv_fail(PGX_ERR_INVALID, "tuning value out of range");
     S->w.tune.solveBlock = t->solve_block;
     S->w.tune.colorBatch = t->color_batch;
     S->w.tune.pairsPerLane = t->pairs_per_lane;
     S->w.tune.colorRounds = t->color_rounds;
     S->w.tune.eagerMax = t->eager_rows_max;
+    S->w.tune.pairSearch = t->pair_search;
     return PGX_OK;
 }
 

@@ -304,7 +304,14 @@ void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh) {
     w.bvel.alloc(BV_WORDS * (size_t)D); w.pvel.alloc(D); w.pang.alloc(D); w.winvI.alloc(3 * (size_t)D);
     w.cbmin.alloc(C); w.cbmax.alloc(C);
     // broadphase buffers
-    w.worldBoundsI.alloc(8);
+    w.worldBoundsI.alloc(16);
+    w.sceneMoments.alloc(6);
+    {
+        uint32_t t = 64;   // the grid's hash table: at least twice the dynamic colliders
+        while (t < 2u * (uint32_t)std::max(w.nDynCol, 1)) t <<= 1;
+        w.gridKey.alloc(t); w.gridStart.alloc(t); w.gridEnd.alloc(t);
+        w.gridRec.alloc(2 * (size_t)std::max(w.nDynCol, 1));
+    }
     w.mkeys.alloc(C); w.mvals.alloc(C);
     w.leafMin.alloc(C); w.leafMax.alloc(C); w.leafFlags.alloc(C); w.leafParent.alloc(C); w.leafEsc.alloc(C);
     size_t ni = C > 1 ? C - 1 : 1;
@@ -972,7 +979,8 @@ void world_autotune(World& w, float dt, int trialSteps) {
                                    {&w.tune.solveBlock, {64, 128, 256}},
                                    {&w.tune.eagerMax, {0, 8192, 32768, 131072, 1 << 30}},
                                    {&w.tune.pairsPerLane, {1, 2, 4}},
-                                   {&w.tune.colorRounds, {8, 16, 32}}};
+                                   {&w.tune.colorRounds, {8, 16, 32}},
+                                   {&w.tune.pairSearch, {0, 1}}};
         // per-colour launches or per-world passes (the same floats either way)
         if ((w.execMode == EXEC_COLORS || w.execMode == EXEC_WORLDS) && sv_world_pass_fits(w))
             knobs.insert(knobs.begin(), Knob{&w.execMode, {EXEC_COLORS, EXEC_WORLDS}});
@@ -1402,6 +1410,8 @@ static int check_tuning(const pgx_tuning* t) {
         return fail(PGX_ERR_INVALID, "pairs_per_lane must be 1, 2 or 4 (got %d)", t->pairs_per_lane);
     if (t->color_rounds < 1 || t->color_rounds > 1024) return fail(PGX_ERR_INVALID, "color_rounds must be in 1..1024");
     if (t->eager_rows_max < 0) return fail(PGX_ERR_INVALID, "eager_rows_max must be >= 0");
+    if (t->pair_search != 0 && t->pair_search != 1)
+        return fail(PGX_ERR_INVALID, "pair_search must be 0 or 1 (got %d)", t->pair_search);
     return PGX_OK;
 }
 
@@ -1411,6 +1421,7 @@ static void tuning_in(World& w, const pgx_tuning* t) {
     w.tune.pairsPerLane = t->pairs_per_lane;
     w.tune.colorRounds = t->color_rounds;
     w.tune.eagerMax = t->eager_rows_max;
+    w.tune.pairSearch = t->pair_search;
 }
 
 static void tuning_out(const World& w, pgx_tuning* t) {
@@ -1419,6 +1430,7 @@ static void tuning_out(const World& w, pgx_tuning* t) {
     t->pairs_per_lane = w.tune.pairsPerLane;
     t->color_rounds = w.tune.colorRounds;
     t->eager_rows_max = w.tune.eagerMax;
+    t->pair_search = w.tune.pairSearch;
 }
 
 int pgx_world_set_tuning(pgx_world* h, const pgx_tuning* t) {

@@ -11,10 +11,14 @@ from physecs_amd import scenes
 
 pytestmark = pytest.mark.gpu
 
+# pair_search: 0 is the uniform-grid pair search (the default), 1 the walk of the LBVH
 TUNINGS = [
-    {"solve_block": 256, "color_batch": 1 << 30, "pairs_per_lane": 4, "color_rounds": 1, "eager_rows_max": 0},
-    {"solve_block": 128, "color_batch": 512, "pairs_per_lane": 2, "color_rounds": 32, "eager_rows_max": 1 << 30},
-    {"solve_block": 64, "color_batch": 64, "pairs_per_lane": 1, "color_rounds": 3, "eager_rows_max": 100},
+    {"solve_block": 256, "color_batch": 1 << 30, "pairs_per_lane": 4, "color_rounds": 1, "eager_rows_max": 0,
+     "pair_search": 1},
+    {"solve_block": 128, "color_batch": 512, "pairs_per_lane": 2, "color_rounds": 32, "eager_rows_max": 1 << 30,
+     "pair_search": 0},
+    {"solve_block": 64, "color_batch": 64, "pairs_per_lane": 1, "color_rounds": 3, "eager_rows_max": 100,
+     "pair_search": 1},
 ]
 
 
@@ -118,7 +122,53 @@ def test_tuning_rejects_bad_values(api):
     w = api.World(scenes.box_pyramid(0, rows=3), substeps=4, iterations=2)
     start = w.tuning()
     for bad in ({"solve_block": 96}, {"pairs_per_lane": 3}, {"color_rounds": 0}, {"color_batch": -1},
-                {"eager_rows_max": -1}):
+                {"eager_rows_max": -1}, {"pair_search": -1}, {"pair_search": 2}):
         with pytest.raises(api.PgxError):
             w.set_tuning(**bad)
     assert w.tuning() == start
+
+
+def _giants(seed=9):
+    """Small bodies in a pile and, far from it, a few dynamic boxes 40 m wide: the grid's cells
+    would be as large as the boxes, its fullest cell far beyond its limit, so the step uses the
+    tree walk."""
+    rng = np.random.default_rng(seed)
+    s = scenes.Scene()
+    s.add_static(scenes.BOX, [400, 0.5, 400], [0, -0.5, 0])
+    for i in range(3):
+        s.add_body(scenes.BOX, [20.0, 1.0, 20.0], [300.0 * (i - 1) + (150.0 if i == 1 else 0.0), 1.2, 250.0],
+                   density=10.0)
+    for i in range(4000):   # about three cells of 40 m: over a thousand bodies in each
+        k = int(rng.integers(0, 3))
+        prm = [0.3] if k == 0 else ([0.3, 0.2] if k == 1 else [0.25, 0.3, 0.2])
+        s.add_body(k, prm, [rng.uniform(-40, 40), rng.uniform(2.5, 12.0), rng.uniform(-15, 15)],
+                   scenes.random_quats(rng, 1)[0])
+    return s
+
+
+@pytest.mark.parametrize("name", ["mixed32k", "batched", "terrain", "giants"])
+def test_pair_searches_find_the_same_pairs(api, name):
+    """The broadphase's pair set from the uniform grid and from the LBVH walk is the same set,
+    after a step and well into the pile; with a few giant bodies the grid hands its steps to
+    the tree walk and the sets still agree."""
+    from gpu_util import pair_set
+    sc = {"mixed32k": lambda: scenes.mixed_lattice(0, side=32),
+          "batched": lambda: scenes.batched_worlds(3, n_worlds=256, lo=8, hi=400, ragdoll_frac=0.25),
+          "terrain": lambda: scenes.convex_terrain(5, n_convex=48, n_chains=4, quads=16, amplitude=2.0,
+                                                   drop=(0.3, 2.0)),
+          "giants": _giants}[name]()
+    log(f"[pairs] {name}: {sc.n_dyn} bodies, grid vs tree walk")
+    worlds = [api.World(sc, substeps=4, iterations=2) for _ in range(2)]
+    for w, mode in zip(worlds, (0, 1)):
+        w.set_tuning(pair_search=mode)
+    for k in range(60):
+        for w in worlds:
+            w.step(1 / 60)
+        if k in (0, 59):
+            sets = [pair_set(w.pairs()) for w in worlds]
+            assert len(sets[0]) > 0
+            assert sets[0] == sets[1], (name, k, [len(x) for x in sets])
+            st = [x.state() for x in worlds]
+            for a, b in zip(st[0], st[1]):
+                assert np.array_equal(bits(a), bits(b)), (name, k)
+    log(f"  {len(sets[0])} pairs after 60 steps")
