@@ -42,42 +42,29 @@ __device__ __forceinline__ int32_t rs_rank(uint32_t d, bool ok, int dbits, uint6
     return prior + (int32_t)__popcll(same & lt);
 }
 
-// The tile's digit counts: each wave counts its keys in a private row of LDS counts, one
-// add per distinct digit of a round (wave ballots group the lanes with equal digits, as in
-// rs_rank, so nearly sorted keys -- the usual input, the previous step's order -- do not
-// pile atomics onto one bin); the rows are summed at the end.
+// The tile's digit counts with LDS atomics, every key load issued before the counting. (A
+// ballot-ranked count per wave, free of the atomics' bank conflicts on nearly sorted keys,
+// measured slower: 11.9 against 6.4 us per pass on the 32k scene's 16-key tiles.)
 template <int ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t* __restrict__ keys, int n, int shift, int dbits,
                                                         int nb, int32_t* __restrict__ H, int digitMajor) {
-    __shared__ int32_t waveCnt[RS_WAVES][RS_BINS];
+    __shared__ int32_t cnt[RS_BINS];
     constexpr int TILE = RS_THREADS * ITEMS;
-    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
-    for (int w = 0; w < RS_WAVES; ++w) waveCnt[w][tid] = 0;
+    cnt[threadIdx.x] = 0;
     const int base = blockIdx.x * TILE;
     uint32_t d[ITEMS];
 #pragma unroll
-    for (int k = 0; k < ITEMS; ++k) {   // every load issued before the counting
-        const int i = base + k * RS_THREADS + tid;
-        d[k] = i < n ? rs_digit(keys[i], shift, dbits) : 0u;
-    }
-    __syncthreads();
-    const uint64_t lt = (1ull << lane) - 1ull;
-    int32_t* counts = waveCnt[wv];
-#pragma unroll
     for (int k = 0; k < ITEMS; ++k) {
-        const bool ok = base + k * RS_THREADS + tid < n;
-        uint64_t same = __ballot(ok);
-        for (int b = 0; b < dbits; ++b) {
-            const uint64_t bal = __ballot((d[k] >> b) & 1u);
-            same &= ((d[k] >> b) & 1u) ? bal : ~bal;
-        }
-        if (ok && (same & lt) == 0ull) counts[d[k]] += (int32_t)__popcll(same);
+        const int i = base + k * RS_THREADS + threadIdx.x;
+        d[k] = i < n ? rs_digit(keys[i], shift, dbits) : RS_BINS;
     }
     __syncthreads();
-    int32_t c = 0;
-    for (int w = 0; w < RS_WAVES; ++w) c += waveCnt[w][tid];
-    if (digitMajor) H[(size_t)tid * nb + blockIdx.x] = c;
-    else H[(size_t)blockIdx.x * RS_BINS + tid] = c;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k)
+        if (d[k] < RS_BINS) atomicAdd(&cnt[d[k]], 1);
+    __syncthreads();
+    if (digitMajor) H[(size_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
+    else H[(size_t)blockIdx.x * RS_BINS + threadIdx.x] = cnt[threadIdx.x];
 }
 
 template <int ITEMS>
