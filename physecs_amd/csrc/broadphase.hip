@@ -195,7 +195,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // colliders of the 13 neighbour cells that lie forward of its cell in (dx, dy, dz) order:
 // of two distinct neighbour cells exactly one is forward of the other, so each pair is found
 // once. The tests are the sweep's inclusive AABB overlap, so the pair set is the SAP's.
-constexpr int GRID_LANES = 16;      // most lanes per collider: its own cell, 13 forward cells, 2 halves of the statics
+constexpr int GRID_LANES = 16;      // per collider: its own cell, 13 forward cells, 2 halves of the statics
 constexpr int GRID_SLOTS = 8;       // pairs a lane holds before its wave appends them
 constexpr int GRID_MAX_RUN = 512;   // a fuller cell hands the step to the tree walk
 constexpr uint64_t GRID_EMPTY = ~0ull;
@@ -351,7 +351,7 @@ __global__ void k_grid_pack(int n, const int32_t* __restrict__ order, const floa
 // large colliders) every lane leaves at once and counters[14] hands the step to the tree walk.
 constexpr int GRID_BLOCK = 1024;
 __global__ __launch_bounds__(GRID_BLOCK) void k_grid_pairs(
-    int n, int lanes, const uint64_t* __restrict__ keys, const float4* __restrict__ rec, uint32_t mask,
+    int n, const uint64_t* __restrict__ keys, const float4* __restrict__ rec, uint32_t mask,
     const uint64_t* __restrict__ tabKey, const int32_t* __restrict__ tabStart, const int32_t* __restrict__ tabEnd,
     const int32_t* __restrict__ worldStaticStart, const float4* __restrict__ statA, const float4* __restrict__ statB,
     const int32_t* __restrict__ statL, int localBits, int32_t* __restrict__ counters, int cap,
@@ -365,13 +365,10 @@ __global__ __launch_bounds__(GRID_BLOCK) void k_grid_pairs(
     __shared__ int32_t blockBase;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
     const long long g = (long long)xcd_block(blockIdx.x, gridDim.x) * GRID_BLOCK + tid;
-    const int s = (int)(g / lanes), k = (int)(g % lanes);
-    // lanes per collider: 16 = 14 cell lanes + 2 halves of the statics; 8 or 4 = 7 or 3 cell
-    // lanes (cells k, k + 7, ... or k, k + 3, ...) + 1 lane for the statics
-    const int cellLanes = lanes == GRID_LANES ? 14 : lanes - 1;
-    bool active = false, statics = false, live = false;
-    int j = 0, end = 0, wd = 0, myLocal = 0, cell = k;
-    uint32_t myEnt = 0, mcode = 0;
+    const int s = (int)(g / GRID_LANES), k = (int)(g % GRID_LANES);
+    bool active = false, statics = false;
+    int j = 0, end = 0, wd = 0, myLocal = 0;
+    uint32_t myEnt = 0;
     float4 qmin = make_float4(0.f, 0.f, 0.f, 0.f), qmax = qmin;
     if (s < n) {
         qmin = rec[2 * s];
@@ -380,48 +377,33 @@ __global__ __launch_bounds__(GRID_BLOCK) void k_grid_pairs(
     if (s < n && qmin.x <= qmax.x) {   // an enabled collider (a disabled one has an empty box)
         const uint64_t key = keys[s];
         wd = (int)(key >> 30);
-        mcode = (uint32_t)(key & 0x3fffffffu);
         myLocal = __float_as_int(qmin.w);
         myEnt = (uint32_t)__float_as_int(qmax.w);
-        live = true;
-        if (k >= cellLanes) {
-            const int a = worldStaticStart[wd], b = worldStaticStart[wd + 1];
-            if (lanes == GRID_LANES) {
-                const int mid = a + (b - a + 1) / 2;
-                j = k == 14 ? a : mid;
-                end = k == 14 ? mid : b;
-            } else {
-                j = a;
-                end = b;
+        if (k < 14) {
+            int dx = 0, dy = 0, dz = 0;
+            if (k >= 1 && k <= 9) { dx = 1; dy = (k - 1) / 3 - 1; dz = (k - 1) % 3 - 1; }
+            else if (k >= 10 && k <= 12) { dy = 1; dz = k - 11; }
+            else if (k == 13) { dz = 1; }
+            const uint32_t m = (uint32_t)(key & 0x3fffffffu);
+            const int x = (int)compact10(m >> 2) + dx, y = (int)compact10(m >> 1) + dy, z = (int)compact10(m) + dz;
+            if (x >= 0 && x < 1024 && y >= 0 && y < 1024 && z >= 0 && z < 1024) {
+                const uint64_t nk = ((uint64_t)wd << 30) | (spread10((uint32_t)x) << 2) | (spread10((uint32_t)y) << 1) |
+                                    spread10((uint32_t)z);
+                const int slot = cell_find(nk, mask, tabKey);
+                if (slot >= 0) {
+                    j = k == 0 ? s + 1 : tabStart[slot];
+                    end = tabEnd[slot];
+                    active = j < end;
+                }
             }
+        } else {
+            const int a = worldStaticStart[wd], b = worldStaticStart[wd + 1], mid = a + (b - a + 1) / 2;
+            j = k == 14 ? a : mid;
+            end = k == 14 ? mid : b;
             active = j < end;
             statics = true;
         }
     }
-    // the next of this lane's cells with colliders to test: own cell (after s) or a forward
-    // neighbour cell
-    auto open_next = [&]() {
-        while (cell < 14) {
-            const int c = cell;
-            cell += cellLanes;
-            int dx = 0, dy = 0, dz = 0;
-            if (c >= 1 && c <= 9) { dx = 1; dy = (c - 1) / 3 - 1; dz = (c - 1) % 3 - 1; }
-            else if (c >= 10 && c <= 12) { dy = 1; dz = c - 11; }
-            else if (c == 13) { dz = 1; }
-            const int x = (int)compact10(mcode >> 2) + dx, y = (int)compact10(mcode >> 1) + dy,
-                      z = (int)compact10(mcode) + dz;
-            if (x < 0 || x >= 1024 || y < 0 || y >= 1024 || z < 0 || z >= 1024) continue;
-            const uint64_t nk = ((uint64_t)wd << 30) | (spread10((uint32_t)x) << 2) | (spread10((uint32_t)y) << 1) |
-                                spread10((uint32_t)z);
-            const int slot = cell_find(nk, mask, tabKey);
-            if (slot < 0) continue;
-            j = c == 0 ? s + 1 : tabStart[slot];
-            end = tabEnd[slot];
-            if (j < end) return true;
-        }
-        return false;
-    };
-    if (live && !statics) active = open_next();
     int held = 0;
     auto pairKey = [&](uint32_t e2, int l2) {
         const int l0 = myEnt < e2 ? myLocal : l2;
@@ -469,7 +451,7 @@ __global__ __launch_bounds__(GRID_BLOCK) void k_grid_pairs(
             if (hit0) c0 = pairKey(e0, __float_as_int(a0.w));
             if (hit1) c1 = pairKey(e1, __float_as_int(a1.w));
             j += two ? 2 : 1;
-            active = j < end || (!statics && open_next());
+            active = j < end;
         }
         if (__ballot(held + 2 > GRID_SLOTS)) waveFlush();
         if (hit0) found[held++][tid] = c0;
@@ -943,11 +925,10 @@ static void bp_grid(World& w) {
                        w.gridStart.p);
     hipLaunchKernelGGL(k_cell_ends, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, T - 1, w.gridKey.p,
                        w.gridStart.p, w.gridEnd.p, w.counters.p);
-    // 16 lanes per collider while every wave fits on the GPU at once, fewer lanes (each
-    // walking several cells) beyond that
-    const int perCollider = n <= 32768 ? GRID_LANES : n <= 65536 ? 8 : 4;
-    const long long lanes = (long long)n * perCollider;
-    hipLaunchKernelGGL(k_grid_pairs, dim3((unsigned)div_up(lanes, GRID_BLOCK)), dim3(GRID_BLOCK), 0, s, n, perCollider, w.mkeys.p,
+    // 16 lanes per collider at every size: fewer lanes, each opening several cells one after
+    // another, measured slower on the 857k-collider batch (506 against 390 us)
+    const long long lanes = (long long)n * GRID_LANES;
+    hipLaunchKernelGGL(k_grid_pairs, dim3((unsigned)div_up(lanes, GRID_BLOCK)), dim3(GRID_BLOCK), 0, s, n, w.mkeys.p,
                        w.gridRec.p, T - 1, w.gridKey.p, w.gridStart.p, w.gridEnd.p, w.worldStaticStart.p, w.statA.p,
                        w.statB.p, w.statL.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
     PGX_LAUNCH_CHECK();

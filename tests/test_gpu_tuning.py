@@ -149,12 +149,11 @@ def _giants(seed=9):
 @pytest.mark.parametrize("name", ["mixed32k", "batched", "batched_mid", "batched_large", "terrain", "giants"])
 def test_pair_searches_find_the_same_pairs(api, name):
     """The broadphase's pair set from the uniform grid and from the LBVH walk is the same set,
-    after a step and well into the pile, for grids of 16, 8 and 4 lanes per collider; with a
-    few giant bodies the grid hands its steps to the tree walk and the sets still agree."""
+    after a step and well into the pile, from 4k to 113k dynamic colliders; with a few giant
+    bodies the grid hands its steps to the tree walk and the sets still agree."""
     from gpu_util import pair_set
     sc = {"mixed32k": lambda: scenes.mixed_lattice(0, side=32),
           "batched": lambda: scenes.batched_worlds(3, n_worlds=256, lo=8, hi=400, ragdoll_frac=0.25),
-          # 32,769..65,536 and over 65,536 dynamic colliders: the grid search's 8- and 4-lane shapes
           "batched_mid": lambda: scenes.batched_worlds(5, n_worlds=300, lo=60, hi=400, ragdoll_frac=0.1),
           "batched_large": lambda: scenes.batched_worlds(4, n_worlds=600, lo=60, hi=400, ragdoll_frac=0.0),
           "terrain": lambda: scenes.convex_terrain(5, n_convex=48, n_chains=4, quads=16, amplitude=2.0,
