@@ -449,6 +449,24 @@ int pgx_trimesh_cache_pack(const float* verts, int n_verts, const int32_t* tris,
 int pgx_world_state_size(pgx_world* w, size_t* bytes);
 int pgx_world_save_state(pgx_world* w, void* buf, size_t bytes);
 int pgx_world_load_state(pgx_world* w, const void* buf, size_t bytes);
+/* Loads a state saved by a world whose colliders differ from this one's: the contact cache
+ * kept through Scene::addCollider and Scene::clearColliders (src/Physecs.cpp:725-752; the
+ * reference keys contactCache and triggerCache by entity and collider index,
+ * include/Physecs/Physecs.h:112-125,165-171, and neither edit changes those of the other
+ * colliders). Every cache entry, trigger pair and broadphase box goes to this world's
+ * collider with the same world, entity and index within its entity; entries without one
+ * are dropped. Poses and velocities stay as this world was created (from the descriptor).
+ * The gear joints' state is loaded when both worlds have the same number of joints. The
+ * last step's trigger pairs that lose a collider (all of them when this world has no
+ * trigger colliders and no trigger-making contact filter) are written to `gone`, four
+ * values each: entity0, index0, entity1, index1 (the lower entity first). The reference
+ * reports them as exits at the next step (src/Physecs.cpp:545-551), this world does not:
+ * the caller does. *n_gone gets their count; PGX_ERR_CAPACITY when it exceeds
+ * gone_capacity, PGX_ERR_INVALID for a state of another number of worlds or a damaged one;
+ * the world is unchanged on error. Load after pgx_world_set_contact_filter, which decides
+ * whether the world keeps trigger lists. */
+int pgx_world_load_state_matched(pgx_world* w, const void* buf, size_t bytes, int gone_capacity, uint32_t* gone,
+                                 int32_t* n_gone);
 /* In PGX_EXEC_COLORS, enabled != 0 launches each step's substep loop as one HIP graph:
  * captured every step (its launches follow the step's colours), the instantiated graph
  * updated in place when the launch sequence keeps its shape. Results are the same either

@@ -24,7 +24,9 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <map>
 #include <set>
+#include <tuple>
 #include <unordered_set>
 #include <vector>
 #include "orc_collision.h"
@@ -38,7 +40,11 @@ struct Dyn { int actor; bool kinematic; V3 v, w; float invMass; V3 com; M3 invI;
 struct Collider {
     int actor, index; V3 lpos; Q lquat; Geom geom;
     float friction, restitution, damping; bool trigger, enable; int data;
+    bool alive = true;   // false after Scene::clearColliders
 };
+// ContactPair (include/Physecs/Physecs.h:90-99): entity0 < entity1 and each collider's
+// index within its entity, the key of triggerCache
+using TrigKey = std::tuple<uint32_t, int, uint32_t, int>;
 struct BPEntry { int collider; uint32_t entity; int colliderIndex; Bounds bounds; bool enable; bool isDynamic; };
 struct Pair { uint32_t e0; int c0; uint32_t e1; int c1; int g0, g1; };
 struct PointC {
@@ -91,6 +97,7 @@ enum Mutation {
     MUT_JOINT_NO_COM = 10,          // Physecs.cpp:343-344: anchor not taken relative to the COM
     MUT_JOINT_UNIVERSAL_AXIS = 11,  // UniversalJoint.cpp:102-105: u1[1] instead of u1[2]
     MUT_FILTER_ARG_ORDER = 12,      // Physecs.cpp:200: the filter called with entity1's collider first
+    MUT_CLEAR_DROPS_CACHE = 13,     // Physecs.cpp:725-737: the entity's contact cache entries erased too
 };
 static int g_mutation = MUT_NONE;
 struct Joint {
@@ -139,10 +146,10 @@ struct orc_world {
     int filterK = 0;
     std::vector<uint8_t> filterTable;
     std::vector<int> filterClass;
-    // triggerCache / triggerCacheTemp (include/Physecs/Physecs.h:170-171) as collider pairs
-    // (collider of entity0, collider of entity1), and the last step's events
-    std::set<std::pair<int, int>> trig, trigTemp;
-    std::vector<std::pair<int, int>> trigEnter, trigExit;
+    // triggerCache / triggerCacheTemp (include/Physecs/Physecs.h:170-171), and the last
+    // step's events
+    std::set<TrigKey> trig, trigTemp;
+    std::vector<TrigKey> trigEnter, trigExit;
     std::vector<int32_t> order;   // injected manifold order (5 ints per manifold)
     std::vector<Joint> joints;
     std::unordered_map<uint32_t, unsigned char> jointBitsets;   // per entity, src/Physecs.cpp:696-705
@@ -342,7 +349,7 @@ static void narrowTask(orc_world* w, int index, std::vector<CC>& outCC, std::vec
     if (trigger) {
         if (overlapGeom(pos0, or0, col0.geom, pos1, or1, col1.geom)) {
             std::lock_guard<std::mutex> lk(w->mtx);
-            w->trigTemp.insert({cp.g0, cp.g1});
+            w->trigTemp.insert(TrigKey{cp.e0, cp.c0, cp.e1, cp.c1});
         }
         return;
     }
@@ -1718,14 +1725,98 @@ void orc_world_set_kinematic(orc_world* w, int actor, int kinematic) {
 }
 
 // The last step's trigger events: which = 0 enter, 1 exit; pairs (collider of entity0,
-// collider of entity1) ascending. Returns the count (at most cap pairs written).
+// collider of entity1) as the oracle's collider handles (the scene's collider order, then
+// added colliders in the order added), ascending; -1 for a collider that is gone (an exit
+// after Scene::clearColliders). Returns the count (at most cap pairs written).
 int orc_world_trigger_events(const orc_world* w, int which, int cap, int32_t* out) {
     const auto& v = which == 0 ? w->trigEnter : w->trigExit;
+    std::map<std::pair<uint32_t, int>, int> live;
+    for (size_t c = 0; c < w->cols.size(); ++c)
+        if (w->cols[c].alive) live[{w->actors[w->cols[c].actor].entity, w->cols[c].index}] = (int)c;
+    auto handle = [&](uint32_t e, int i) {
+        auto it = live.find({e, i});
+        return it == live.end() ? -1 : it->second;
+    };
+    std::vector<std::pair<int, int>> pairs;
+    for (const TrigKey& k : v)
+        pairs.push_back({handle(std::get<0>(k), std::get<1>(k)), handle(std::get<2>(k), std::get<3>(k))});
+    std::sort(pairs.begin(), pairs.end());
+    for (size_t i = 0; i < pairs.size() && (int)i < cap; ++i) {
+        out[2 * i] = pairs[i].first;
+        out[2 * i + 1] = pairs[i].second;
+    }
+    return (int)pairs.size();
+}
+
+// The same events as the listeners get them (src/Physecs.cpp:538-551): rows (entity0,
+// colliderIndex0, entity1, colliderIndex1), ascending.
+int orc_world_trigger_events_ec(const orc_world* w, int which, int cap, uint32_t* out) {
+    const auto& v = which == 0 ? w->trigEnter : w->trigExit;
     for (size_t i = 0; i < v.size() && (int)i < cap; ++i) {
-        out[2 * i] = v[i].first;
-        out[2 * i + 1] = v[i].second;
+        out[4 * i] = std::get<0>(v[i]);
+        out[4 * i + 1] = (uint32_t)std::get<1>(v[i]);
+        out[4 * i + 2] = std::get<2>(v[i]);
+        out[4 * i + 3] = (uint32_t)std::get<3>(v[i]);
     }
     return (int)v.size();
+}
+
+// Scene::addCollider (src/Physecs.cpp:739-752): the collider takes its entity's next index
+// (the count of the entity's colliders), and a broadphase entry with its bounds at the
+// actor's current pose, without margin, isDynamic from the body's kinematic flag. The
+// caches are untouched. Returns the new collider's handle.
+int orc_world_add_collider(orc_world* w, int actor, int type, const float* prm, const float* lpos, const float* lquat,
+                           const float* material, int trigger, int enable, int data) {
+    const Actor& a = w->actors.at((size_t)actor);
+    Collider c;
+    c.actor = actor;
+    c.index = 0;
+    for (const Collider& o : w->cols)
+        if (o.alive && o.actor == actor) ++c.index;
+    c.lpos = v3(lpos[0], lpos[1], lpos[2]);
+    c.lquat = {lquat[0], lquat[1], lquat[2], lquat[3]};
+    c.geom.type = type;
+    for (int k = 0; k < 4; ++k) c.geom.p[k] = prm[k];
+    if (type == CONVEX_MESH) c.geom.mesh = &w->convex.at((size_t)prm[3]);
+    if (type == TRIANGLE_MESH) c.geom.tmesh = &w->trimesh.at((size_t)prm[3]);
+    c.friction = material[0];
+    c.restitution = material[1];
+    c.damping = material[2];
+    c.trigger = trigger != 0;
+    c.enable = enable != 0;
+    c.data = data;
+    const int handle = (int)w->cols.size();
+    w->cols.push_back(c);
+    if (w->filterK) w->filterClass.push_back(0);   // class 0 until the filter is set again
+    const bool isDynamic = a.dyn >= 0 ? !w->dyn[a.dyn].kinematic : false;
+    Bounds b = getBounds(a.pos + qrot(a.quat, c.lpos), qmul(a.quat, c.lquat), c.geom);
+    w->bp.push_back({handle, a.entity, c.index, b, c.enable, isDynamic});
+    return handle;
+}
+
+// Scene::clearColliders (src/Physecs.cpp:725-737): each of the entity's broadphase entries,
+// in collider-index order, replaced by the last entry; the colliders are gone. The contact
+// and trigger caches keep their keys, so a collider added later under the same index takes
+// them over, as in the reference.
+void orc_world_clear_colliders(orc_world* w, int actor) {
+    std::vector<int> mine;
+    for (size_t c = 0; c < w->cols.size(); ++c)
+        if (w->cols[c].alive && w->cols[c].actor == actor) mine.push_back((int)c);
+    std::sort(mine.begin(), mine.end(), [&](int x, int y) { return w->cols[x].index < w->cols[y].index; });
+    for (int c : mine) {
+        for (size_t i = 0; i < w->bp.size(); ++i) {
+            if (w->bp[i].collider != c) continue;
+            w->bp[i] = w->bp.back();
+            w->bp.pop_back();
+            break;
+        }
+        w->cols[c].alive = false;
+    }
+    if (g_mutation == MUT_CLEAR_DROPS_CACHE) {
+        const uint32_t e = w->actors.at((size_t)actor).entity;
+        for (auto it = w->cache.begin(); it != w->cache.end();)
+            it = (it->first.e0 == e || it->first.e1 == e) ? w->cache.erase(it) : std::next(it);
+    }
 }
 
 int orc_overlap(int type0, const float* prm0, const float* pos0, const float* q0, int type1, const float* prm1,

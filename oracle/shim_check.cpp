@@ -170,6 +170,26 @@ int sc_triangle_mesh(void* h, const float* verts, int nv, const uint32_t* idx, i
     return (int)c->trimesh.size() - 1;
 }
 
+static physecs::Collider make_collider(Check* c, int type, const float* p, const float* lpos, const float* lquat,
+                                       const float* material, uint8_t trigger, uint8_t enable, int32_t data) {
+    physecs::Collider col{};
+    col.position = v3(lpos);
+    col.orientation = q4(lquat);
+    col.geometry.type = (physecs::GeometryType)type;
+    switch (type) {
+        case physecs::SPHERE: col.geometry.sphere.radius = p[0]; break;
+        case physecs::CAPSULE: col.geometry.capsule = {p[0], p[1]}; break;
+        case physecs::BOX: col.geometry.box.halfExtents = v3(p); break;
+        case physecs::CONVEX_MESH: col.geometry.convex = {c->convex.at((size_t)p[3]).get(), v3(p)}; break;
+        case physecs::TRIANGLE_MESH: col.geometry.triangleMesh.mesh = c->trimesh.at((size_t)p[3]).get(); break;
+    }
+    col.material = {material[0], material[1], material[2]};
+    col.isTrigger = trigger != 0;
+    col.enableSimulation = enable != 0;
+    col.data = data;
+    return col;
+}
+
 // The entity's RigidBodyCollisionComponent with n colliders; params as the descriptor's
 // col_params, a mesh collider's fourth parameter naming a handle from sc_*_mesh.
 void sc_collision(void* h, uint32_t e, int n, const int32_t* types, const float* params, const float* lpos,
@@ -177,26 +197,25 @@ void sc_collision(void* h, uint32_t e, int n, const int32_t* types, const float*
                   const int32_t* data) {
     auto* c = static_cast<Check*>(h);
     physecs::RigidBodyCollisionComponent cc;
-    for (int k = 0; k < n; ++k) {
-        physecs::Collider col{};
-        col.position = v3(lpos + 3 * k);
-        col.orientation = q4(lquat + 4 * k);
-        const float* p = params + 4 * k;
-        col.geometry.type = (physecs::GeometryType)types[k];
-        switch (types[k]) {
-            case physecs::SPHERE: col.geometry.sphere.radius = p[0]; break;
-            case physecs::CAPSULE: col.geometry.capsule = {p[0], p[1]}; break;
-            case physecs::BOX: col.geometry.box.halfExtents = v3(p); break;
-            case physecs::CONVEX_MESH: col.geometry.convex = {c->convex.at((size_t)p[3]).get(), v3(p)}; break;
-            case physecs::TRIANGLE_MESH: col.geometry.triangleMesh.mesh = c->trimesh.at((size_t)p[3]).get(); break;
-        }
-        col.material = {material[3 * k], material[3 * k + 1], material[3 * k + 2]};
-        col.isTrigger = trigger[k] != 0;
-        col.enableSimulation = enable[k] != 0;
-        col.data = data[k];
-        cc.colliders.push_back(col);
-    }
+    for (int k = 0; k < n; ++k)
+        cc.colliders.push_back(make_collider(c, types[k], params + 4 * k, lpos + 3 * k, lquat + 4 * k,
+                                             material + 3 * k, trigger[k], enable[k], data[k]));
     c->registry.emplace<physecs::RigidBodyCollisionComponent>((entt::entity)e, cc);
+}
+
+// Scene::addCollider and Scene::clearColliders on a live scene
+void sc_add_collider(void* h, uint32_t e, int type, const float* params, const float* lpos, const float* lquat,
+                     const float* material, int trigger, int enable, int data) {
+    auto* c = static_cast<Check*>(h);
+    c->scene->addCollider((entt::entity)e, make_collider(c, type, params, lpos, lquat, material, (uint8_t)trigger,
+                                                         (uint8_t)enable, data));
+}
+
+void sc_clear_colliders(void* h, uint32_t e) { static_cast<Check*>(h)->scene->clearColliders((entt::entity)e); }
+
+// Scene::setIsKinematic
+void sc_set_kinematic(void* h, uint32_t e, int kinematic) {
+    static_cast<Check*>(h)->scene->setIsKinematic((entt::entity)e, kinematic != 0);
 }
 
 // physecs::setMassProps (src/MassUtil.cpp) on the entity's components

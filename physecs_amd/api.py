@@ -35,6 +35,7 @@ EXPORTS = [
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
     "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_character_move", "pgx_mass_properties",
     "pgx_world_patch_transforms", "pgx_convex_hull", "pgx_trimesh_cache", "pgx_trimesh_cache_validate", "pgx_trimesh_cache_pack", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
+    "pgx_world_load_state_matched",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -186,6 +187,8 @@ def lib():
         L.pgx_world_state_size.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
         L.pgx_world_save_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
         L.pgx_world_load_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t]
+        L.pgx_world_load_state_matched.argtypes = [vp, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                                                   ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_int32)]
         L.pgx_server_set_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
         L.pgx_server_create.argtypes = [ctypes.POINTER(PgxConfig), ctypes.POINTER(PgxServerConfig), ctypes.POINTER(vp)]
         L.pgx_server_destroy.argtypes = [vp]
@@ -591,6 +594,25 @@ class World:
         """pgx_world_load_state: replaces the world's state with bytes from save_state."""
         buf = ctypes.create_string_buffer(bytes(state), len(state))
         check(self.L.pgx_world_load_state(self.h, buf, len(state)))
+
+    def load_state_matched(self, state):
+        """pgx_world_load_state_matched: the contact cache, trigger lists and bounds of a state
+        saved by a world whose colliders differ, carried to the colliders with the same world,
+        entity and index. Returns the last step's trigger pairs that lost a collider, as rows
+        (entity0, index0, entity1, index1)."""
+        buf = ctypes.create_string_buffer(bytes(state), len(state))
+        cap = 64
+        while True:
+            gone = np.zeros((cap, 4), np.uint32)
+            n = ctypes.c_int32(0)
+            rc = self.L.pgx_world_load_state_matched(self.h, buf, len(state), cap,
+                                                     gone.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                                     ctypes.byref(n))
+            if rc == ERR_CAPACITY and n.value > cap:
+                cap = n.value
+                continue
+            check(rc)
+            return gone[: n.value].copy()
 
     def exec_mode(self):
         m = ctypes.c_int(0)

@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <unordered_map>
 #include "pgx_internal.h"
 
 using namespace pgx;
@@ -845,25 +846,52 @@ static void restore_snapshot(World& w, const Snapshot& z) {
 // velocities, broadphase bounds (with their margins), the contact cache (the reference's
 // contactCache, include/Physecs/Physecs.h:167), the gear joints' angle state
 // (GearJointData, include/Physecs/Joints/GearJoint.h:7-14), the trigger lists
-// (triggerCache) and the last step's sizes that the next step's launches start from. The
-// structure (descriptor, joints, setCanCollide pairs, parameters) is not in it: a world
-// loads a state saved by a world created from the same descriptor and joints.
+// (triggerCache) and the last step's sizes that the next step's launches start from, plus
+// each collider's identity (world, entity, index within its entity). The structure
+// (descriptor, joints, setCanCollide pairs, parameters) is not in it: world_load_state
+// loads a state saved by a world created from the same descriptor and joints, and
+// world_load_state_matched one saved by a world whose colliders differ.
 namespace {
 constexpr uint32_t STATE_MAGIC = 0x53584750u;   // "PGXS"
-constexpr int32_t STATE_VERSION = 1;
+constexpr int32_t STATE_VERSION = 2;
 struct StateHeader {
     uint32_t magic;
     int32_t version, A, D, C, J, W, pairCap, manifoldCap;
     int32_t cacheCount, hasTriggers, trigCur, trigN[4];
     int32_t lastPairs, lastManifolds, lastColors, lastRounds, lastPoints, lastMeshSlots;
+    int32_t localBits, reserved;
+};
+// a collider's identity: the reference keys its contact and trigger caches by entity and
+// collider index (CollisionPair / ContactPair, include/Physecs/Physecs.h:112-125)
+struct ColId {
+    int32_t world;
+    uint32_t entity;
+    int32_t index;
 };
 
+// the device sections in state order
+enum Section { S_POS, S_QUAT, S_VEL, S_ANG, S_BMIN, S_BMAX, S_KEY, S_TRI, S_NUM, S_LP0, S_LP1, S_LP2, S_LP3, S_GEAR,
+               S_GEAR_INIT, S_TRIG0, S_TRIG1, S_COUNT };
+
+void section_bytes(const StateHeader& h, size_t out[S_COUNT]) {
+    const size_t n = (size_t)h.cacheCount;
+    out[S_POS] = out[S_QUAT] = 16 * (size_t)h.A;
+    out[S_VEL] = out[S_ANG] = 16 * (size_t)h.D;
+    out[S_BMIN] = out[S_BMAX] = 16 * (size_t)h.C;
+    out[S_KEY] = 8 * n; out[S_TRI] = 4 * n; out[S_NUM] = 4 * n;
+    out[S_LP0] = out[S_LP1] = out[S_LP2] = out[S_LP3] = 16 * n;
+    out[S_GEAR] = 16 * (size_t)h.J; out[S_GEAR_INIT] = 4 * (size_t)h.J;
+    out[S_TRIG0] = h.hasTriggers ? 8 * (size_t)h.trigN[1] : 0;
+    out[S_TRIG1] = h.hasTriggers ? 8 * (size_t)h.trigN[3] : 0;
+}
+
+// byte offset of the first device section
+size_t ids_end(const StateHeader& h) { return sizeof(StateHeader) + sizeof(ColId) * (size_t)h.C; }
+
 size_t state_bytes(const StateHeader& h) {
-    size_t n = sizeof(StateHeader);
-    n += 16 * (2 * (size_t)h.A + 2 * (size_t)h.D + 2 * (size_t)h.C);
-    n += (size_t)h.cacheCount * (8 + 4 + 4 + 4 * 16);
-    n += (size_t)h.J * (16 + 4);
-    if (h.hasTriggers) n += 8 * ((size_t)h.trigN[1] + (size_t)h.trigN[3]);
+    size_t sz[S_COUNT], n = ids_end(h);
+    section_bytes(h, sz);
+    for (size_t b : sz) n += b;
     return n;
 }
 
@@ -880,21 +908,60 @@ StateHeader state_header(World& w) {
     }
     h.lastPairs = w.lastPairs; h.lastManifolds = w.lastManifolds; h.lastColors = w.lastColors;
     h.lastRounds = w.lastRounds; h.lastPoints = w.lastPoints; h.lastMeshSlots = w.lastMeshSlots;
+    h.localBits = w.localBits;
     return h;
 }
 
-// The sections in order: (device pointer, bytes); `save` copies them out, `load` in.
+// The device sections in order: (device pointer, bytes); `save` copies them out, `load` in.
 template <class F>
 void state_sections(World& w, const StateHeader& h, F&& f) {
     const int cur = w.cacheCur;
-    const size_t n = (size_t)h.cacheCount;
-    f(w.bpos.p, 16 * (size_t)h.A); f(w.bquat.p, 16 * (size_t)h.A);
-    f(w.dvel.p, 16 * (size_t)h.D); f(w.dang.p, 16 * (size_t)h.D);
-    f(w.cbmin.p, 16 * (size_t)h.C); f(w.cbmax.p, 16 * (size_t)h.C);
-    f(w.cacheKey[cur].p, 8 * n); f(w.cacheTri[cur].p, 4 * n); f(w.cacheNum[cur].p, 4 * n);
-    for (int j = 0; j < 4; ++j) f(w.cacheLp[cur].p + (size_t)j * w.manifoldCap, 16 * n);
-    f(w.jGear.p, 16 * (size_t)h.J); f(w.jGearInit.p, 4 * (size_t)h.J);
-    if (h.hasTriggers) { f(w.trigKeys[0].p, 8 * (size_t)h.trigN[1]); f(w.trigKeys[1].p, 8 * (size_t)h.trigN[3]); }
+    void* dev[S_COUNT] = {w.bpos.p, w.bquat.p, w.dvel.p, w.dang.p, w.cbmin.p, w.cbmax.p, w.cacheKey[cur].p,
+                          w.cacheTri[cur].p, w.cacheNum[cur].p, w.cacheLp[cur].p,
+                          w.cacheLp[cur].p + (size_t)w.manifoldCap, w.cacheLp[cur].p + 2 * (size_t)w.manifoldCap,
+                          w.cacheLp[cur].p + 3 * (size_t)w.manifoldCap, w.jGear.p, w.jGearInit.p,
+                          h.hasTriggers ? (void*)w.trigKeys[0].p : nullptr,
+                          h.hasTriggers ? (void*)w.trigKeys[1].p : nullptr};
+    size_t sz[S_COUNT];
+    section_bytes(h, sz);
+    for (int k = 0; k < S_COUNT; ++k) f(dev[k], sz[k]);
+}
+
+// each descriptor collider's identity, from the host copies of the world's layout
+std::vector<ColId> collider_ids(const World& w) {
+    std::vector<ColId> ids((size_t)w.C);
+    for (int wd = 0; wd < w.W; ++wd)
+        for (int l = w.hostWorldColStart[wd]; l < w.hostWorldColStart[wd + 1]; ++l) ids[w.hostColOfLocal[l]].world = wd;
+    for (int c = 0; c < w.C; ++c) {
+        ids[c].entity = w.hostEntity[w.hostColBody[c]];
+        ids[c].index = w.hostColIndex[c];
+    }
+    return ids;
+}
+
+StateHeader read_header(const void* buf, size_t bytes) {
+    StateHeader h;
+    if (bytes < sizeof h) throw PgxError{PGX_ERR_INVALID, "state buffer shorter than its header"};
+    std::memcpy(&h, buf, sizeof h);
+    if (h.magic != STATE_MAGIC || h.version != STATE_VERSION)
+        throw PgxError{PGX_ERR_INVALID, "not a saved world state (magic or version)"};
+    if (h.A < 0 || h.D < 0 || h.C < 0 || h.J < 0 || h.W < 1 || h.cacheCount < 0 || h.cacheCount > h.manifoldCap ||
+        h.trigCur < 0 || h.trigCur > 1 || h.localBits < 1 || h.localBits > 31)
+        throw PgxError{PGX_ERR_INVALID, "state header out of range"};
+    for (int k = 0; k < 4; ++k)
+        if (h.trigN[k] < 0 || h.trigN[k] > h.pairCap) throw PgxError{PGX_ERR_INVALID, "trigger counts out of range"};
+    if (bytes < state_bytes(h)) throw PgxError{PGX_ERR_INVALID, "state buffer truncated"};
+    return h;
+}
+
+// The last step's pair, manifold and mesh-slot lists live in buffers the state does not
+// carry: after a load there is no last step to report (counts and manifolds read empty
+// until the next step). The colouring-round count only sizes the next step's first batch
+// of rounds.
+void reset_last_step(World& w, const StateHeader& h) {
+    w.lastPairs = 0; w.lastManifolds = 0; w.lastColors = 0; w.lastPoints = 0; w.lastMeshSlots = 0;
+    w.lastPointsHost = 0;
+    w.lastRounds = std::max(0, std::min(h.lastRounds, 1024));
 }
 }  // namespace
 
@@ -906,7 +973,9 @@ void world_save_state(World& w, void* buf, size_t bytes) {
                                                                      " bytes < " + std::to_string(state_bytes(h))};
     char* out = static_cast<char*>(buf);
     std::memcpy(out, &h, sizeof h);
-    size_t at = sizeof h;
+    const std::vector<ColId> ids = collider_ids(w);
+    if (!ids.empty()) std::memcpy(out + sizeof h, ids.data(), sizeof(ColId) * ids.size());
+    size_t at = ids_end(h);
     state_sections(w, h, [&](void* dev, size_t nb) {
         if (nb) PGX_HIP(hipMemcpyAsync(out + at, dev, nb, hipMemcpyDeviceToHost, w.stream));
         at += nb;
@@ -915,22 +984,14 @@ void world_save_state(World& w, void* buf, size_t bytes) {
 }
 
 void world_load_state(World& w, const void* buf, size_t bytes) {
-    StateHeader h;
-    if (bytes < sizeof h) throw PgxError{PGX_ERR_INVALID, "state buffer shorter than its header"};
-    std::memcpy(&h, buf, sizeof h);
-    if (h.magic != STATE_MAGIC || h.version != STATE_VERSION)
-        throw PgxError{PGX_ERR_INVALID, "not a saved world state (magic or version)"};
+    const StateHeader h = read_header(buf, bytes);
     if (h.A != w.A || h.D != w.D || h.C != w.C || h.J != w.J || h.W != w.W ||
-        h.hasTriggers != (w.hasTriggers ? 1 : 0))
+        h.hasTriggers != (w.hasTriggers ? 1 : 0) || h.localBits != w.localBits)
         throw PgxError{PGX_ERR_INVALID, "the state was saved by a world of another shape (actors, bodies, colliders, "
                                         "joints, worlds or triggers differ)"};
-    if (h.cacheCount < 0 || h.cacheCount > w.manifoldCap || h.trigCur < 0 || h.trigCur > 1)
-        throw PgxError{PGX_ERR_INVALID, "state header out of range"};
-    for (int k = 0; k < 4; ++k)
-        if (h.trigN[k] < 0 || h.trigN[k] > w.pairCap) throw PgxError{PGX_ERR_INVALID, "trigger counts out of range"};
-    if (bytes < state_bytes(h)) throw PgxError{PGX_ERR_INVALID, "state buffer truncated"};
+    if (h.cacheCount > w.manifoldCap) throw PgxError{PGX_ERR_INVALID, "state header out of range"};
     const char* in = static_cast<const char*>(buf);
-    size_t at = sizeof h;
+    size_t at = ids_end(h);
     state_sections(w, h, [&](void* dev, size_t nb) {
         if (nb) PGX_HIP(hipMemcpyAsync(dev, in + at, nb, hipMemcpyHostToDevice, w.stream));
         at += nb;
@@ -939,13 +1000,193 @@ void world_load_state(World& w, const void* buf, size_t bytes) {
     PGX_HIP(hipStreamSynchronize(w.stream));
     w.cacheCount[w.cacheCur] = h.cacheCount;
     w.trigCur = h.trigCur;
-    // The last step's pair, manifold and mesh-slot lists live in buffers the state does not
-    // carry: after a load there is no last step to report (counts and manifolds read empty
-    // until the next step). The colouring-round count only sizes the next step's first
-    // batch of rounds.
-    w.lastPairs = 0; w.lastManifolds = 0; w.lastColors = 0; w.lastPoints = 0; w.lastMeshSlots = 0;
-    w.lastPointsHost = 0;
-    w.lastRounds = std::max(0, std::min(h.lastRounds, 1024));
+    reset_last_step(w, h);
+}
+
+// A state saved by a world whose colliders differ from this one's (Scene::addCollider and
+// Scene::clearColliders, src/Physecs.cpp:725-752, and any other rebuild of a scene): the
+// reference's caches are keyed by (entity, collider index), which neither edit changes for
+// the other colliders, so every cache entry, trigger pair and bounds box goes to the
+// collider of this world with the same world, entity and index, and what has no such
+// collider is dropped. The contact cache stays sorted by its (new) keys, a pair's entries
+// in their saved order. Poses and velocities stay as this world was created. The gear
+// joints' state is loaded when both worlds have the same number of joints. Trigger pairs of
+// the last step that lose a collider are written to `gone` (entity0, index0, entity1,
+// index1; the lower entity first), as are all of them when this world keeps no trigger
+// lists: the reference reports them as exits at the next step (src/Physecs.cpp:545-551).
+// Nothing is changed when the state is rejected or `gone` is too small.
+void world_load_state_matched(World& w, const void* buf, size_t bytes, int goneCap, uint32_t* gone, int32_t* nGone) {
+    const StateHeader h = read_header(buf, bytes);
+    if (h.W != w.W) throw PgxError{PGX_ERR_INVALID, "the state was saved by a world with another number of worlds"};
+    const char* in = static_cast<const char*>(buf);
+    std::vector<ColId> old((size_t)h.C);
+    if (h.C) std::memcpy(old.data(), in + sizeof h, sizeof(ColId) * old.size());
+    size_t sz[S_COUNT], off[S_COUNT];
+    section_bytes(h, sz);
+    off[0] = ids_end(h);
+    for (int k = 1; k < S_COUNT; ++k) off[k] = off[k - 1] + sz[k - 1];
+
+    // the saver's world-local collider order (descriptor order within each world)
+    std::vector<int32_t> oldStart(h.W + 1, 0), oldOfLocal((size_t)h.C);
+    for (const ColId& c : old) {
+        if (c.world < 0 || c.world >= h.W) throw PgxError{PGX_ERR_INVALID, "state collider world out of range"};
+        ++oldStart[c.world + 1];
+    }
+    for (int i = 0; i < h.W; ++i) oldStart[i + 1] += oldStart[i];
+    {
+        std::vector<int32_t> fillp(oldStart.begin(), oldStart.end() - 1);
+        for (int c = 0; c < h.C; ++c) oldOfLocal[fillp[old[c].world]++] = c;
+    }
+    // this world's colliders by identity, and their world-local indices
+    const std::vector<ColId> now = collider_ids(w);
+    auto idKey = [](const ColId& c) {
+        return ((uint64_t)(uint32_t)c.world << 32) ^ ((uint64_t)c.entity << 8) ^ (uint64_t)(uint32_t)c.index;
+    };
+    std::unordered_multimap<uint64_t, int32_t> byId;
+    byId.reserve(now.size());
+    for (int c = 0; c < w.C; ++c) byId.emplace(idKey(now[c]), c);
+    std::vector<int32_t> local((size_t)w.C);
+    for (int wd = 0; wd < w.W; ++wd)
+        for (int l = w.hostWorldColStart[wd]; l < w.hostWorldColStart[wd + 1]; ++l)
+            local[w.hostColOfLocal[l]] = l - w.hostWorldColStart[wd];
+    std::vector<int32_t> map((size_t)h.C, -1);
+    for (int c = 0; c < h.C; ++c) {
+        auto r = byId.equal_range(idKey(old[c]));
+        for (auto it = r.first; it != r.second; ++it) {
+            const ColId& n = now[it->second];
+            if (n.world == old[c].world && n.entity == old[c].entity && n.index == old[c].index) {
+                map[c] = it->second;
+                break;
+            }
+        }
+    }
+    // a saved key (world, local of the lower entity's collider, other local) in the saver's
+    // layout to this world's: false when a collider is gone
+    const uint64_t oldMask = (1ull << h.localBits) - 1;
+    auto decode = [&](uint64_t key, int& c0, int& c1) {
+        const uint64_t wd = key >> (2 * h.localBits);
+        const uint64_t l0 = (key >> h.localBits) & oldMask, l1 = key & oldMask;
+        if (wd >= (uint64_t)h.W || l0 >= (uint64_t)(oldStart[wd + 1] - oldStart[wd]) ||
+            l1 >= (uint64_t)(oldStart[wd + 1] - oldStart[wd]))
+            throw PgxError{PGX_ERR_INVALID, "state key out of range"};
+        c0 = oldOfLocal[oldStart[wd] + l0];
+        c1 = oldOfLocal[oldStart[wd] + l1];
+    };
+    auto translate = [&](uint64_t key, uint64_t& out) {
+        int c0, c1;
+        decode(key, c0, c1);
+        const int n0 = map[c0], n1 = map[c1];
+        if (n0 < 0 || n1 < 0) return false;
+        out = ((uint64_t)now[n0].world << (2 * w.localBits)) | ((uint64_t)local[n0] << w.localBits) | (uint64_t)local[n1];
+        return true;
+    };
+
+    // contact cache: the entries whose colliders both remain, stably sorted by their new keys
+    const size_t n = (size_t)h.cacheCount;
+    std::vector<uint64_t> keyIn(n);
+    std::vector<int32_t> triIn(n), numIn(n);
+    std::vector<float4> lpIn(4 * n);
+    if (n) {
+        std::memcpy(keyIn.data(), in + off[S_KEY], sz[S_KEY]);
+        std::memcpy(triIn.data(), in + off[S_TRI], sz[S_TRI]);
+        std::memcpy(numIn.data(), in + off[S_NUM], sz[S_NUM]);
+        for (int j = 0; j < 4; ++j) std::memcpy(lpIn.data() + j * n, in + off[S_LP0 + j], sz[S_LP0 + j]);
+    }
+    std::vector<std::pair<uint64_t, int32_t>> kept;   // (new key, saved entry)
+    kept.reserve(n);
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t k;
+        if (translate(keyIn[i], k)) kept.push_back({k, (int32_t)i});
+    }
+    std::stable_sort(kept.begin(), kept.end(),
+                     [](const std::pair<uint64_t, int32_t>& a, const std::pair<uint64_t, int32_t>& b) {
+                         return a.first < b.first;
+                     });
+    if (kept.size() > (size_t)w.manifoldCap)
+        throw PgxError{PGX_ERR_CAPACITY, "the carried contact cache exceeds the world's manifold capacity"};
+    const size_t m = kept.size(), mc = (size_t)w.manifoldCap;
+    std::vector<uint64_t> keyOut(m);
+    std::vector<int32_t> triOut(m), numOut(m);
+    std::vector<float4> lpOut(4 * std::max<size_t>(m, 1));
+    for (size_t i = 0; i < m; ++i) {
+        const int32_t s = kept[i].second;
+        keyOut[i] = kept[i].first; triOut[i] = triIn[s]; numOut[i] = numIn[s];
+        for (int j = 0; j < 4; ++j) lpOut[j * m + i] = lpIn[j * n + s];
+    }
+
+    // trigger lists, and the last step's trigger pairs this world will not see again
+    std::vector<uint64_t> trigOut[2];
+    std::vector<uint32_t> goneIds;
+    if (h.hasTriggers) {
+        for (int k = 0; k < 2; ++k) {
+            const size_t cnt = sz[S_TRIG0 + k] / 8;
+            std::vector<uint64_t> keys(cnt);
+            if (cnt) std::memcpy(keys.data(), in + off[S_TRIG0 + k], sz[S_TRIG0 + k]);
+            for (uint64_t key : keys) {
+                uint64_t t;
+                const bool ok = translate(key, t);
+                if (ok && w.hasTriggers) trigOut[k].push_back(t);
+                if (k == h.trigCur && !(ok && w.hasTriggers)) {
+                    int c0, c1;
+                    decode(key, c0, c1);
+                    goneIds.insert(goneIds.end(), {old[c0].entity, (uint32_t)old[c0].index, old[c1].entity,
+                                                   (uint32_t)old[c1].index});
+                }
+            }
+        }
+    }
+    const int32_t ng = (int32_t)(goneIds.size() / 4);
+    if (nGone) *nGone = ng;
+    if (ng > 0 && (ng > goneCap || !gone))
+        throw PgxError{PGX_ERR_CAPACITY, std::to_string(ng) + " trigger pairs lose a collider; the gone list holds " +
+                                             std::to_string(std::max(goneCap, 0))};
+
+    // bounds: the carried boxes of the colliders that remain, the rest as created
+    std::vector<float4> bmin((size_t)w.C), bmax((size_t)w.C);
+    hipStream_t s = w.stream;
+    if (w.C) {
+        PGX_HIP(hipMemcpyAsync(bmin.data(), w.cbmin.p, 16 * (size_t)w.C, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(bmax.data(), w.cbmax.p, 16 * (size_t)w.C, hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipStreamSynchronize(s));
+    }
+    for (int c = 0; c < h.C; ++c) {
+        if (map[c] < 0) continue;
+        std::memcpy(&bmin[map[c]], in + off[S_BMIN] + 16 * (size_t)c, 16);
+        std::memcpy(&bmax[map[c]], in + off[S_BMAX] + 16 * (size_t)c, 16);
+    }
+
+    // commit
+    if (w.C) {
+        PGX_HIP(hipMemcpyAsync(w.cbmin.p, bmin.data(), 16 * (size_t)w.C, hipMemcpyHostToDevice, s));
+        PGX_HIP(hipMemcpyAsync(w.cbmax.p, bmax.data(), 16 * (size_t)w.C, hipMemcpyHostToDevice, s));
+    }
+    const int cur = w.cacheCur;
+    if (m) {
+        PGX_HIP(hipMemcpyAsync(w.cacheKey[cur].p, keyOut.data(), 8 * m, hipMemcpyHostToDevice, s));
+        PGX_HIP(hipMemcpyAsync(w.cacheTri[cur].p, triOut.data(), 4 * m, hipMemcpyHostToDevice, s));
+        PGX_HIP(hipMemcpyAsync(w.cacheNum[cur].p, numOut.data(), 4 * m, hipMemcpyHostToDevice, s));
+        for (int j = 0; j < 4; ++j)
+            PGX_HIP(hipMemcpyAsync(w.cacheLp[cur].p + j * mc, lpOut.data() + j * m, 16 * m, hipMemcpyHostToDevice, s));
+    }
+    if (h.J == w.J && w.J > 0) {
+        PGX_HIP(hipMemcpyAsync(w.jGear.p, in + off[S_GEAR], sz[S_GEAR], hipMemcpyHostToDevice, s));
+        PGX_HIP(hipMemcpyAsync(w.jGearInit.p, in + off[S_GEAR_INIT], sz[S_GEAR_INIT], hipMemcpyHostToDevice, s));
+    }
+    int32_t trigN[4] = {0, 0, 0, 0};
+    if (w.hasTriggers) {
+        for (int k = 0; k < 2; ++k) {
+            trigN[2 * k + 1] = (int32_t)trigOut[k].size();
+            if (!trigOut[k].empty())
+                PGX_HIP(hipMemcpyAsync(w.trigKeys[k].p, trigOut[k].data(), 8 * trigOut[k].size(),
+                                       hipMemcpyHostToDevice, s));
+        }
+        PGX_HIP(hipMemcpyAsync(w.trigN.p, trigN, sizeof trigN, hipMemcpyHostToDevice, s));
+    }
+    PGX_HIP(hipStreamSynchronize(s));
+    if (ng > 0) std::memcpy(gone, goneIds.data(), sizeof(uint32_t) * goneIds.size());
+    w.cacheCount[cur] = (int32_t)m;
+    w.trigCur = h.hasTriggers ? h.trigCur : 0;
+    reset_last_step(w, h);
 }
 
 // Picks the launch shapes (and between per-colour launches and per-world passes) by
@@ -1560,6 +1801,17 @@ int pgx_world_load_state(pgx_world* h, const void* buf, size_t bytes) {
     return guarded([&] {
         PGX_HIP(hipSetDevice(h->w.device));
         world_load_state(h->w, buf, bytes);
+        return PGX_OK;
+    });
+}
+
+int pgx_world_load_state_matched(pgx_world* h, const void* buf, size_t bytes, int gone_capacity, uint32_t* gone,
+                                 int32_t* n_gone) {
+    if (!h || !buf) return fail(PGX_ERR_INVALID, "NULL argument");
+    if (n_gone) *n_gone = 0;
+    return guarded([&] {
+        PGX_HIP(hipSetDevice(h->w.device));
+        world_load_state_matched(h->w, buf, bytes, gone_capacity, gone, n_gone);
         return PGX_OK;
     });
 }

@@ -395,6 +395,57 @@ class Scene:
             self.fill_joint(arr[i], *j)
         return arr
 
+    def rebuilt(self, state=None, clear=(), add=()):
+        """The scene as Scene::clearColliders and Scene::addCollider leave it
+        (src/Physecs.cpp:725-752), for a world created anew: poses and velocities from `state`
+        ((pos, quat, vel, angvel) as World.state() gives them), the colliders of each actor in
+        `clear` removed, then each (actor, gtype, params, keyword arguments of add_collider) of
+        `add` appended with its entity's next collider index. The colliders are then ordered
+        as an entt::registry holds them when each actor's RigidBodyCollisionComponent was
+        emplaced in the order the actors first appear here: by that order, then by index
+        within the entity. Actors, bodies, joints, non-colliding pairs and meshes are kept.
+        Returns (scene, collider map): the map gives each old collider's index in the new
+        scene, or -1."""
+        import copy
+        arr = self.arrays()
+        s = copy.copy(self)
+        s._chunks = {k: [np.array(v, copy=True)] for k, v in arr.items() if k in _FIELDS}
+        s._dyn_patch = {}
+        s._frozen = None
+        s._n = dict(self._n)
+        s.non_colliding = list(self.non_colliding)
+        s.joints = list(self.joints)
+        if state is not None:
+            for name, v in zip(("pos", "quat", "vel", "angvel"), state):
+                s._chunks[name] = [np.ascontiguousarray(v, np.float32).reshape(s._chunks[name][0].shape)]
+        keep = ~np.isin(arr["col_actor"], np.asarray(list(clear), np.int32))
+        for k in _FIELDS:
+            if k.startswith("col_"):
+                s._chunks[k] = [s._chunks[k][0][keep]]
+        s._n["col"] = int(keep.sum())
+        colmap = np.full(self.n_colliders, -1, np.int64)
+        colmap[keep] = np.arange(s._n["col"])
+        actors = s._chunks["col_actor"][0]
+        s._ncol = {int(a): int(c) for a, c in zip(*np.unique(actors, return_counts=True))}
+        for a in clear:
+            s._ncol[int(a)] = 0
+        for actor, gtype, params, kw in add:
+            s.add_collider(actor, gtype, params, **kw)
+        new = s.arrays()
+        rank = np.full(self.n_actors, self.n_actors, np.int64)
+        first = np.unique(arr["col_actor"], return_index=True)
+        rank[first[0]] = np.argsort(np.argsort(first[1]))
+        order = np.lexsort((new["col_index"], np.where(rank[new["col_actor"]] < self.n_actors,
+                                                        rank[new["col_actor"]], self.n_actors + new["col_actor"])))
+        for k in _FIELDS:
+            if k.startswith("col_"):
+                s._chunks[k] = [new[k][order].copy()]
+        s._frozen = None
+        inv = np.empty_like(order)
+        inv[order] = np.arange(len(order))
+        colmap[keep] = inv[colmap[keep]]
+        return s, colmap
+
     # -- views
     @property
     def n_actors(self):

@@ -229,7 +229,15 @@ const pgx_scene_desc& Scene::gatherDescriptor() {
 }
 
 void Scene::rebuild() {
+    // The old world's state goes to the new world's colliders by entity and collider index,
+    // the keys of the reference's contactCache and triggerCache, which survive addCollider,
+    // clearColliders and the creation or destruction of other bodies (src/Physecs.cpp:725-752).
+    std::vector<char> carried;
     if (world) {
+        size_t n = 0;
+        check(pgx_world_state_size(world, &n), "state size");
+        carried.resize(n);
+        check(pgx_world_save_state(world, carried.data(), n), "save state");
         pgx_world_destroy(world);
         world = nullptr;
     }
@@ -252,6 +260,17 @@ void Scene::rebuild() {
         check(pgx_world_add_joints(world, 1, &d), "createJoint");
     }
     applyContactFilter();
+    if (!carried.empty()) {
+        int32_t nGone = 0;
+        std::vector<uint32_t> gone(4 * 64);
+        int rc = pgx_world_load_state_matched(world, carried.data(), carried.size(), 64, gone.data(), &nGone);
+        if (rc == PGX_ERR_CAPACITY && nGone > 64) {
+            gone.resize(4 * (size_t)nGone);
+            rc = pgx_world_load_state_matched(world, carried.data(), carried.size(), nGone, gone.data(), &nGone);
+        }
+        check(rc, "carry the world's state");
+        pendingExits.insert(pendingExits.end(), gone.begin(), gone.begin() + 4 * (size_t)nGone);
+    }
     dirty = false;
 }
 
@@ -311,6 +330,8 @@ void Scene::pullState() {
 }
 
 void Scene::fireTriggers() {
+    std::vector<uint32_t> gone;
+    gone.swap(pendingExits);
     if (enterListeners.empty() && exitListeners.empty()) return;
     int cap = 256;
     std::vector<int32_t> en, ex;
@@ -332,6 +353,10 @@ void Scene::fireTriggers() {
         for (OnTriggerExitListener* l : exitListeners)
             l->onTriggerExit(actors[colActor[ex[2 * i]]], colIndex[ex[2 * i]], actors[colActor[ex[2 * i + 1]]],
                              colIndex[ex[2 * i + 1]]);
+    // pairs whose collider went in a rebuild since the last step
+    for (size_t i = 0; i + 3 < gone.size(); i += 4)
+        for (OnTriggerExitListener* l : exitListeners)
+            l->onTriggerExit((entt::entity)gone[i], (int)gone[i + 1], (entt::entity)gone[i + 2], (int)gone[i + 3]);
 }
 
 void Scene::simulate(float timeStep) {
@@ -423,15 +448,9 @@ void Scene::setIsKinematic(entt::entity e, bool isKinematic) {
         d.angularVelocity = glm::vec3(0);
     }
     if (!world || dirty) return;
-    // the world keeps its shape, so its state (contact cache, gear angles, trigger lists)
-    // goes through the rebuild that carries the new flag
-    size_t n = 0;
-    check(pgx_world_state_size(world, &n), "state size");
-    std::vector<char> state(n);
-    check(pgx_world_save_state(world, state.data(), n), "save state");
+    // the flag changes the world's broadphase classes: rebuilt now, its contact cache, gear
+    // angles and trigger lists carried across
     rebuild();
-    check(pgx_world_load_state(world, state.data(), n), "load state");
-    pushState();
 }
 
 void Scene::setCanCollide(entt::entity e0, entt::entity e1, bool canCollide) {

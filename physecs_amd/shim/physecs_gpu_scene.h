@@ -123,12 +123,16 @@ private:
     int subSteps = 8, iterations = 2;            // Physecs.h:156-157
     float gravity = 9.81f;                       // Physecs.h:159
     pgx_world* world = nullptr;
-    bool dirty = true;                           // bodies or colliders added or removed
+    bool dirty = true;                           // bodies or colliders added or removed: rebuild
+                                                 // the world, carrying its caches across
     bool syncFromRegistry = true;
     ContactType (*contactFilter)(bool, int, bool, int) = defaultContactFilter;
     std::vector<OnTriggerEnterListener*> enterListeners;
     std::vector<OnTriggerExitListener*> exitListeners;
     std::vector<glm::vec3> contactPoints;
+    // trigger pairs (entity0, index0, entity1, index1) that lost a collider in a rebuild: the
+    // reference reports them as exits at the next step
+    std::vector<uint32_t> pendingExits;
 
     struct JointRecord {
         pgx_joint_desc desc;

@@ -327,6 +327,39 @@ class OracleWorld:
             out.append(buf[:n].copy())
         return tuple(out)
 
+    def trigger_events_ec(self):
+        """The last step's trigger (enter, exit) events as the listeners get them: (n, 4) rows
+        (entity0, colliderIndex0, entity1, colliderIndex1)."""
+        fn = self.lib.orc_world_trigger_events_ec
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)]
+        fn.restype = ctypes.c_int
+        out = []
+        for which in (0, 1):
+            n = fn(self.h, which, 0, np.zeros(4, np.uint32).ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+            buf = np.zeros((max(n, 1), 4), np.uint32)
+            fn(self.h, which, n, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+            out.append(buf[:n].copy())
+        return tuple(out)
+
+    def add_collider(self, actor, gtype, params, material=(0.6, 0.0, 0.0), lpos=(0, 0, 0), lquat=(0, 0, 0, 1),
+                     trigger=False, enable=True, data=0):
+        """Scene::addCollider: returns the new collider's handle (its place after the scene's
+        colliders)."""
+        fn = self.lib.orc_world_add_collider
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, F, F, F, F, ctypes.c_int, ctypes.c_int,
+                       ctypes.c_int]
+        fn.restype = ctypes.c_int
+        p = np.zeros(4, np.float32)
+        p[: len(params)] = params
+        return fn(self.h, int(actor), int(gtype), fptr(p), fptr(_f32(lpos, 3)), fptr(_f32(lquat, 4)),
+                  fptr(_f32(material, 3)), int(bool(trigger)), int(bool(enable)), int(data))
+
+    def clear_colliders(self, actor):
+        """Scene::clearColliders."""
+        fn = self.lib.orc_world_clear_colliders
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        fn(self.h, int(actor))
+
     def manifolds(self):
         n = self.lib.orc_world_num_manifolds(self.h)
         keys = np.zeros((n, 5), np.int32)
@@ -366,6 +399,7 @@ def row_solve(which, flags, b0, b1, inp, masses, vel, pv, h, iters):
 MUT_SIMD_WARM_LOCAL, MUT_SIMD_LIMIT_UNSCALED, MUT_SIMD_MASK_NONSTRICT, MUT_SIMD_BIAS_FLOAT = 1, 2, 3, 4
 MUT_SIMD_ZERO_WRITEBACK, MUT_FRICTION_SOLVER_VEL, MUT_NO_RESTITUTION_CACHE, MUT_RELAX_SOFT = 5, 6, 7, 8
 MUT_JOINT_AXIS_ORDER, MUT_JOINT_NO_COM, MUT_JOINT_UNIVERSAL_AXIS, MUT_FILTER_ARG_ORDER = 9, 10, 11, 12
+MUT_CLEAR_DROPS_CACHE = 13
 
 
 class mutation:

@@ -63,6 +63,10 @@ def lib():
         L.sc_set_contact_filter.argtypes = [vp, ctypes.c_int]
         L.sc_create_joint.argtypes = [vp, ctypes.POINTER(api.PgxJointDesc), ctypes.c_uint32, ctypes.c_uint32]
         L.sc_set_can_collide.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+        L.sc_add_collider.argtypes = [vp, ctypes.c_uint32, ctypes.c_int, F, F, F, F, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int]
+        L.sc_clear_colliders.argtypes = [vp, ctypes.c_uint32]
+        L.sc_set_kinematic.argtypes = [vp, ctypes.c_uint32, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -203,6 +207,20 @@ class Registry:
 
     def set_can_collide(self, e0, e1, can):
         self.L.sc_set_can_collide(self.h, e0, e1, int(can))
+
+    def add_collider(self, e, gtype, params, material=(0.6, 0.0, 0.0), lpos=(0, 0, 0), lquat=(0, 0, 0, 1),
+                     trigger=False, enable=True, data=0):
+        """Scene::addCollider (a mesh collider's fourth parameter names a registry mesh)."""
+        p = np.zeros(4, np.float32)
+        p[: len(params)] = params
+        self.L.sc_add_collider(self.h, e, int(gtype), _f(p), _f(lpos), _f(lquat), _f(material), int(bool(trigger)),
+                               int(bool(enable)), int(data))
+
+    def clear_colliders(self, e):
+        self.L.sc_clear_colliders(self.h, e)
+
+    def set_kinematic(self, e, kinematic):
+        self.L.sc_set_kinematic(self.h, e, int(bool(kinematic)))
 
 
 def replay(scene):
