@@ -479,8 +479,9 @@ int pgx_world_set_graph(pgx_world* w, int enabled);
 typedef struct pgx_tuning {
     int solve_block;      /* threads per workgroup of the per-colour contact solve: 64, 128, 256 */
     int color_batch;      /* a run of two or more consecutive colours of at most this many
-                             manifolds is solved by one single-workgroup launch that steps
-                             through them in order (0: one launch per colour) */
+                             manifolds (and at most 1264 in all) is solved by one
+                             single-workgroup launch that steps through them in order, its
+                             bodies' velocities in LDS (0: one launch per colour; default 512) */
     int pairs_per_lane;   /* narrowphase pairs per lane: 1, 2, 4 */
     int color_rounds;     /* least graph-colouring rounds queued before the host reads back
                              whether colouring finished: 1..1024 */

@@ -58,6 +58,7 @@ constexpr int WAVE = 64;
 // inertia of the substep (massTemp's R * invI * R^T). Bodies are gathered by index, so
 // one line per body instead of one per array.
 enum { BV_V = 0, BV_W = 1, BV_CV = 2, BV_CW = 3, BV_I0 = 4, BV_I1 = 5, BV_I2 = 6, BV_WORDS = 8 };
+constexpr int CONTACT_ROW_FIELDS = 9;   // float4 rows per contact point (sRow)
 __host__ __device__ __forceinline__ int bv(int body, int field) { return BV_WORDS * body + field; }
 constexpr int JOINT_MAX_ROWS = 7;      // a prismatic joint with its limit and drive rows
 // substep-loop execution: one launch per colour and pass; one cooperative launch per step
@@ -220,6 +221,8 @@ struct World {
     // manifolds, solve (colour) order
     DBuf<int32_t> sB0, sB1, sFlags, sA0, sA1;   // sFlags: numPoints | soft<<8 | color<<16
     DBuf<int32_t> sCanon;                // canonical index
+    DBuf<int32_t> batchLoc, batchList, batchN;   // runs of small colours: LDS slots of their bodies
+    bool batchLdsSet = false;            // k_solve_batch's LDS limit raised on this world's device
     DBuf<float4> sN;                     // normal, friction
     DBuf<float2> sSoft;                  // frequency, damping
     DBuf<float4> sR0, sR1;               // 4 x M: r0 local + targetVelocity, r1 local
@@ -288,7 +291,7 @@ struct World {
     // depend on them
     struct Tuning {
         int solveBlock = 64;             // threads per k_solve workgroup (64, 128, 256)
-        int colorBatch = 0;              // runs of colours of at most this many manifolds: one launch
+        int colorBatch = 512;            // runs of colours of at most this many manifolds: one launch
         int pairsPerLane = 1;            // narrowphase pairs per lane (1, 2, 4)
         int colorRounds = 4;             // least colouring rounds queued per read-back
         int eagerMax = 32768;            // colours of at most this many manifolds load rows eagerly
@@ -479,6 +482,7 @@ void world_enable_triggers(World& w);                  // allocates the trigger 
 int sv_setup_manifolds(World& w, int nPairs);
 void sv_color_points(World& w, int nColors);        // contact points per colour -> colorPoints          // returns manifold count
 int sv_color(World& w, int nManifolds);                 // returns colour count
+void sv_presize(World& w);
 void sv_substeps(World& w, int nManifolds, float dt);
 void sv_finish_step(World& w, int nManifolds);
 double sv_step_bytes(const World& w, int M, long points);

@@ -195,11 +195,18 @@ __global__ void k_color_clash(int M, int D, const int32_t* __restrict__ clash, i
     if (i < 2 * D) used[i] = 0;
 }
 
-__global__ void k_color_keys(int M, const int32_t* __restrict__ color, uint64_t* __restrict__ keys,
+// Solve order: by colour, and within a colour by point count (byPoints), so the lanes of a
+// wave solve manifolds of one point count and run the same number of rows. A colour's
+// manifolds share no dynamic body, so their order within the colour changes no result. The
+// per-world passes keep each world's manifolds of a colour together instead (byPoints 0).
+constexpr int COLOR_KEY_SHIFT = 2;
+__global__ void k_color_keys(int M, const int32_t* __restrict__ color, const int32_t* __restrict__ npNum,
+                             const int32_t* __restrict__ mfSrc, int byPoints, uint64_t* __restrict__ keys,
                              int32_t* __restrict__ vals) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
-    keys[m] = (uint64_t)color[m];
+    const int num = byPoints ? min(max(npNum[mfSrc[m]], 1), 4) : 1;
+    keys[m] = ((uint64_t)color[m] << COLOR_KEY_SHIFT) | (uint64_t)(num - 1);
     vals[m] = m;
 }
 
@@ -211,7 +218,7 @@ __global__ void k_color_start(int M, int n, const uint64_t* __restrict__ keys, i
     int lo = 0, hi = M;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (keys[mid] < (uint64_t)c) lo = mid + 1; else hi = mid;
+        if (keys[mid] < ((uint64_t)c << COLOR_KEY_SHIFT)) lo = mid + 1; else hi = mid;
     }
     start[c] = lo;
 }
@@ -346,11 +353,15 @@ void sv_color_points(World& w, int nColors) {
 // ------------------------------------------------------------------ substep pieces
 // Contact rows of point k of manifold slot s, field f at ((k * R_NF + f) * Mcap + s):
 // r0 x n (.w = c), r1 x n (.w = inverse effective mass N), tangent (.w = inverse
-// effective mass T), r0 x t, r1 x t. The rows
-// times the bodies' world inverse inertia are recomputed in the solve from the body
-// records, with the operations that produced them here. Per point also (lambda N,
-// lambda T, target velocity) in sLam.
-enum { R_R0XN = 0, R_R1XN, R_T, R_R0XT, R_R1XT, R_NF };
+// effective mass T), r0 x t (.w = effective mass N, 1 / invN), r1 x t (.w = the bias
+// 0.1 c / h), and the rows times the bodies' world inverse inertia, as preSolve keeps them
+// (src/ContactConstraints.cpp:21-24): r0xnt (.w = beta c / h of a soft contact), r1xnt
+// (.w = invN + gamma / h of a soft contact), r0xtt, r1xtt. Everything here is fixed for the
+// substep, so the solve's arithmetic on it is done once, with the operations and in the
+// order ContactConstraints::solve does it (src/ContactConstraints.cpp:54-68). Per point
+// also (lambda N, lambda T, target velocity) in sLam.
+enum { R_R0XN = 0, R_R1XN, R_T, R_R0XT, R_R1XT, R_R0XNT, R_R1XNT, R_R0XTT, R_R1XTT, R_NF };
+static_assert(R_NF == CONTACT_ROW_FIELDS, "sRow holds CONTACT_ROW_FIELDS float4 per point");
 #define ROW(f, k) a.sRow[((size_t)((k) * R_NF + (f))) * a.Mcap + s]
 #define LAM(k) a.sLam[(size_t)(k) * a.Mcap + s]
 
@@ -407,6 +418,10 @@ __device__ void prepare_manifold(const SubArgs& a, int s) {
         I1 = world_inv_inertia(a, b1);
     }
     float nn = dot(n, n);
+    const int sflags = a.sFlags[s];
+    const bool soft = (sflags & 256) != 0;
+    const float2 sp = a.sSoft[s];
+    const float h = a.h;
     for (int k = 0; k < num; ++k) {
         f3 r0 = qrot(q0, xyz(a.sR0[(size_t)k * a.Mcap + s]));
         f3 r1 = qrot(q1, xyz(a.sR1[(size_t)k * a.Mcap + s]));
@@ -422,11 +437,28 @@ __device__ void prepare_manifold(const SubArgs& a, int s) {
         f3 r0xnt = mulmv(I0, r0xn), r1xnt = mulmv(I1, r1xn), r0xtt = mulmv(I0, r0xt), r1xtt = mulmv(I1, r1xt);
         float invN = nn * (im0 + im1) + dot(r0xn, r0xnt) + dot(r1xn, r1xnt);
         float invT = dot(t, t) * (im0 + im1) + dot(r0xt, r0xtt) + dot(r1xt, r1xtt);
+        // src/ContactConstraints.cpp:54-68 on the substep's fixed values
+        const float effMass = 1.f / invN;
+        const float bias = 0.1f * cn / h;
+        float softA = 0.f, softB = 0.f;
+        if (soft) {
+            float af = 2.f * 3.14159265358979323846f * sp.x;
+            float stiffness = af * af * effMass;
+            float damping = 2.f * af * sp.y * effMass;
+            float gamma = 1.f / (damping + h * stiffness);
+            float beta = h * stiffness / (damping + h * stiffness);
+            softA = beta * cn / h;
+            softB = invN + gamma / h;
+        }
         ROW(R_R0XN, k) = f4(r0xn, cn);
         ROW(R_R1XN, k) = f4(r1xn, invN);
         ROW(R_T, k) = f4(t, invT);
-        ROW(R_R0XT, k) = f4(r0xt, 0.f);
-        ROW(R_R1XT, k) = f4(r1xt, 0.f);
+        ROW(R_R0XT, k) = f4(r0xt, effMass);
+        ROW(R_R1XT, k) = f4(r1xt, bias);
+        ROW(R_R0XNT, k) = f4(r0xnt, softA);
+        ROW(R_R1XNT, k) = f4(r1xnt, softB);
+        ROW(R_R0XTT, k) = f4(r0xtt, 0.f);
+        ROW(R_R1XTT, k) = f4(r1xtt, 0.f);
         LAM(k) = make_float4(0.f, 0.f, a.sR0[(size_t)k * a.Mcap + s].w, 0.f);   // lambdas start at 0
     }
 }
@@ -471,21 +503,37 @@ __device__ void integrate_velocity(const SubArgs& a, int d) {
 // one colour touch disjoint dynamic bodies, so a colour is solved in parallel exactly as a
 // sequential pass over it would be.
 // Everything one manifold's solve reads except the bodies' solver velocities: the
-// manifold's rows, normal, material, cached target velocities, and both bodies' inverse
-// masses and component velocities. All of it is fixed for the whole pass.
+// manifold's rows, normal, friction, cached target velocities and lambdas, and both
+// bodies' inverse masses and component velocities. All of it is fixed for the whole pass.
 struct MInputs {
     int num, b0, b1;
     bool soft;
     f3 n;
     float friction, im0, im1;
     f3 cv0, cw0, cv1, cw1;
-    m3 I0, I1;                            // the bodies' world inverse inertia
-    float2 sp;
-    float4 rA[4], rB[4], rT[4], rC[4], rE[4], lam[4];
+    float4 row[4][R_NF], lam[4];
 };
 
-__device__ __forceinline__ m3 body_inertia(const SubArgs& a, int b) {
-    return mk_m3(xyz(a.bvel[bv(b, BV_I0)]), xyz(a.bvel[bv(b, BV_I1)]), xyz(a.bvel[bv(b, BV_I2)]));
+// the manifold's own inputs: flags, bodies, normal; point slots' rows and lambdas (all four
+// slots when ALL, else slot 0 with the flags and slots 1-3 once the point count is known)
+template <bool ALL>
+__device__ __forceinline__ void load_own(const SubArgs& a, int s, MInputs& in) {
+    const int flags = a.sFlags[s];
+    in.soft = (flags & 256) != 0;
+    in.num = flags & 255;
+    in.b0 = a.sB0[s];
+    in.b1 = a.sB1[s];
+    const float4 nf = a.sN[s];
+    in.n = xyz(nf);
+    in.friction = nf.w;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (ALL || k == 0 || k < in.num) {
+#pragma unroll
+            for (int f = 0; f < R_NF; ++f) in.row[k][f] = ROW(f, k);
+            in.lam[k] = LAM(k);
+        }
+    }
 }
 
 // EAGER loads point slot 0's rows (every manifold has a first point) with the manifold's
@@ -504,25 +552,21 @@ __device__ __forceinline__ void load_inputs(const SubArgs& a, int s, MInputs& in
     in.friction = nf.w;
     in.im0 = 0.f; in.im1 = 0.f;
     in.cv0 = s3(0.f); in.cw0 = s3(0.f); in.cv1 = s3(0.f); in.cw1 = s3(0.f);
-    in.I0 = mzero(); in.I1 = mzero();
     if (in.b0 >= 0) {
         float4 c = a.bvel[bv(in.b0, BV_CV)];
         in.im0 = c.w; in.cv0 = xyz(c); in.cw0 = xyz(a.bvel[bv(in.b0, BV_CW)]);
-        in.I0 = body_inertia(a, in.b0);
     }
     if (in.b1 >= 0) {
         float4 c = a.bvel[bv(in.b1, BV_CV)];
         in.im1 = c.w; in.cv1 = xyz(c); in.cw1 = xyz(a.bvel[bv(in.b1, BV_CW)]);
-        in.I1 = body_inertia(a, in.b1);
     }
-    in.sp = a.sSoft[s];
     // all rows of the manifold are loaded before any dependent arithmetic (memory-level
     // parallelism); unused point slots are never read back
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if ((EAGER && k == 0) || k < in.num) {
-            in.rA[k] = ROW(R_R0XN, k); in.rB[k] = ROW(R_R1XN, k); in.rT[k] = ROW(R_T, k);
-            in.rC[k] = ROW(R_R0XT, k); in.rE[k] = ROW(R_R1XT, k);
+#pragma unroll
+            for (int f = 0; f < R_NF; ++f) in.row[k][f] = ROW(f, k);
             in.lam[k] = LAM(k);
         }
     }
@@ -530,9 +574,11 @@ __device__ __forceinline__ void load_inputs(const SubArgs& a, int s, MInputs& in
 
 // ContactConstraints::solve (src/ContactConstraints.cpp:33-124) of one manifold on the
 // bodies' solver velocities v0, w0, v1, w1 (updated in place); stores the accumulated
-// lambdas in the manifold's rows.
+// lambdas in the manifold's rows. The effective mass, the bias and a soft contact's terms
+// come from the rows (computed once per substep with the same operations).
 __device__ __forceinline__ void solve_core(const SubArgs& a, int s, const MInputs& in, bool useBias, float h, f3& v0,
                                            f3& w0, f3& v1, f3& w1) {
+    (void)h;
     const int num = in.num, b0 = in.b0, b1 = in.b1;
     const f3 n = in.n;
     const float im0 = in.im0, im1 = in.im1;
@@ -540,41 +586,31 @@ __device__ __forceinline__ void solve_core(const SubArgs& a, int s, const MInput
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= num) break;
-        float4 r0 = in.rA[k], r1 = in.rB[k];
+        const float4* r = in.row[k];
         lN[k] = in.lam[k].x;
-        float invN = r1.w;
+        float invN = r[R_R1XN].w;
         if (invN == 0.f) continue;
-        f3 r0xn = xyz(r0), r1xn = xyz(r1);
-        float c = r0.w;
+        f3 r0xn = xyz(r[R_R0XN]), r1xn = xyz(r[R_R1XN]);
         float rv = dot(neg(n), v0) + dot(neg(r0xn), w0) + dot(n, v1) + dot(r1xn, w1);
-        float effMass = 1.f / invN;
         float lambda;
-        if (in.soft) {
-            float af = 2.f * 3.14159265358979323846f * in.sp.x;
-            float stiffness = af * af * effMass;
-            float damping = 2.f * af * in.sp.y * effMass;
-            float gamma = 1.f / (damping + h * stiffness);
-            float beta = h * stiffness / (damping + h * stiffness);
-            lambda = (rv + beta * c / h) / (invN + gamma / h);
-        } else {
-            lambda = (rv - in.lam[k].z + (useBias ? 0.1f * c / h : 0.f)) * effMass;
-        }
+        if (in.soft) lambda = (rv + r[R_R0XNT].w) / r[R_R1XNT].w;
+        else lambda = (rv - in.lam[k].z + (useBias ? r[R_R1XT].w : 0.f)) * r[R_R0XT].w;
         float prev = lN[k];
         lN[k] = lN[k] + lambda;
         lN[k] = gmin(lN[k], 0.f);
         lambda = lN[k] - prev;
-        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, n)); w0 = add(w0, mul(lambda, mulmv(in.I0, r0xn))); }
-        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, n)); w1 = sub(w1, mul(lambda, mulmv(in.I1, r1xn))); }
+        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, n)); w0 = add(w0, mul(lambda, xyz(r[R_R0XNT]))); }
+        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, n)); w1 = sub(w1, mul(lambda, xyz(r[R_R1XNT]))); }
     }
     // friction rows read the component velocities (src/ContactConstraints.cpp:142-152)
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k >= num) break;
-        float4 t4 = in.rT[k];
+        const float4* r = in.row[k];
         lT[k] = in.lam[k].y;
-        float invT = t4.w;
+        float invT = r[R_T].w;
         if (invT == 0.f) continue;
-        f3 t = xyz(t4), r0xt = xyz(in.rC[k]), r1xt = xyz(in.rE[k]);
+        f3 t = xyz(r[R_T]), r0xt = xyz(r[R_R0XT]), r1xt = xyz(r[R_R1XT]);
         float rv = dot(neg(t), in.cv0) + dot(neg(r0xt), in.cw0) + dot(t, in.cv1) + dot(r1xt, in.cw1);
         float lambda = rv / invT;
         float limit = in.friction * lN[k];
@@ -582,8 +618,8 @@ __device__ __forceinline__ void solve_core(const SubArgs& a, int s, const MInput
         lT[k] = lT[k] + lambda;
         lT[k] = gclamp(lT[k], limit, -limit);
         lambda = lT[k] - prev;
-        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, t)); w0 = add(w0, mul(lambda, mulmv(in.I0, r0xt))); }
-        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, t)); w1 = sub(w1, mul(lambda, mulmv(in.I1, r1xt))); }
+        if (b0 >= 0) { v0 = add(v0, mul(lambda * im0, t)); w0 = add(w0, mul(lambda, xyz(r[R_R0XTT]))); }
+        if (b1 >= 0) { v1 = sub(v1, mul(lambda * im1, t)); w1 = sub(w1, mul(lambda, xyz(r[R_R1XTT]))); }
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -825,17 +861,150 @@ __global__ __launch_bounds__(XPASS_THREADS) void k_solve_xpass(SubArgs a, int nC
     }
 }
 
-// Consecutive small colours c0..c1-1 in one launch of one workgroup: the colours are solved
-// in order with a workgroup barrier between them, so the solve order is the colour order,
-// exactly as with one launch per colour. The barrier makes one colour's global stores
-// visible to every thread of the workgroup before the next colour reads them.
-constexpr int BATCH_THREADS = 512;   // k_solve's register footprint allows 2 waves per SIMD
-__global__ __launch_bounds__(BATCH_THREADS) void k_solve_batch(SubArgs a, int c0, int c1, int useBias, float h,
-                                                               int skipSoft) {
-    for (int c = c0; c < c1; ++c) {
-        const int b = a.colorStart[c], e = a.colorStart[c + 1];
-        for (int s = b + (int)threadIdx.x; s < e; s += BATCH_THREADS) solve_manifold(a, s, useBias != 0, h, skipSoft != 0);
+// Consecutive small colours c0..c1-1 (solve slots [S, E)) in one launch of one workgroup:
+// the colours are solved in order with a workgroup barrier between them, so the solve order
+// is the colour order, exactly as with one launch per colour. The solver velocities of the
+// run's bodies stay in LDS for the whole run: one gather before the first colour, one
+// scatter after the last, and between colours only the manifolds' fixed inputs come from
+// global memory. k_batch_map gives each run its bodies (batchList, from 2S on) and each
+// manifold side its LDS slot (batchLoc, 2 per solve slot; -1 for a static side), once per
+// step: a run's slots never change within the step.
+constexpr int BATCH_THREADS = 512;      // k_solve's register footprint allows 2 waves per SIMD
+constexpr int BATCH_MAP_THREADS = 1024;
+constexpr int BATCH_HASH = 8192;        // LDS hash of body -> slot: at least twice a run's bodies
+constexpr int BATCH_MAX_GROUPS = 16;
+constexpr int BATCH_MAX_MANIFOLDS = 1264;   // 2 x 1264 bodies x 64 B of records in LDS
+constexpr int BATCH_MAX_COLOURS = 128;
+
+struct BatchGroups {
+    int n;
+    int S[BATCH_MAX_GROUPS], E[BATCH_MAX_GROUPS];
+};
+
+__device__ __forceinline__ uint32_t batch_hash(int b) { return ((uint32_t)b * 2654435761u) >> (32 - 13); }
+
+// One workgroup, every run in turn: the run's bodies get slots 0..n-1 in first-claim order
+// (the order does not reach the arithmetic) through an LDS hash table with linear probing.
+__global__ __launch_bounds__(BATCH_MAP_THREADS) void k_batch_map(BatchGroups g, const int32_t* __restrict__ sB0,
+                                                                 const int32_t* __restrict__ sB1,
+                                                                 int32_t* __restrict__ batchLoc,
+                                                                 int32_t* __restrict__ batchList,
+                                                                 int32_t* __restrict__ batchN) {
+    static_assert(BATCH_HASH == 1 << 13, "batch_hash gives 13 bits");
+    __shared__ int32_t hk[BATCH_HASH], hv[BATCH_HASH];
+    __shared__ int32_t count;
+    for (int q = 0; q < g.n; ++q) {
+        const int S = g.S[q], E = g.E[q], sides = 2 * (E - S);
+        for (int i = threadIdx.x; i < BATCH_HASH; i += BATCH_MAP_THREADS) hk[i] = -1;
+        if (threadIdx.x == 0) count = 0;
         __syncthreads();
+        for (int i = threadIdx.x; i < sides; i += BATCH_MAP_THREADS) {
+            const int sl = S + (i >> 1);
+            const int b = (i & 1) ? sB1[sl] : sB0[sl];
+            if (b < 0) continue;
+            uint32_t h = batch_hash(b);
+            for (int probe = 0; probe < BATCH_HASH; ++probe) {   // the table is never full
+                const int prev = atomicCAS(&hk[h], -1, b);
+                if (prev == -1) {
+                    const int slot = atomicAdd(&count, 1);
+                    hv[h] = slot;
+                    batchList[2 * S + slot] = b;
+                    break;
+                }
+                if (prev == b) break;
+                h = (h + 1) & (BATCH_HASH - 1);
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < sides; i += BATCH_MAP_THREADS) {
+            const int sl = S + (i >> 1);
+            const int b = (i & 1) ? sB1[sl] : sB0[sl];
+            int slot = -1;
+            if (b >= 0) {
+                uint32_t h = batch_hash(b);
+                for (int probe = 0; probe < BATCH_HASH && hk[h] != b; ++probe) h = (h + 1) & (BATCH_HASH - 1);
+                slot = hv[h];
+            }
+            batchLoc[2 * sl + (i & 1)] = slot;
+        }
+        if (threadIdx.x == 0) batchN[q] = count;
+        __syncthreads();
+    }
+}
+
+// A workgroup barrier that orders LDS only: the global loads a thread has in flight stay in
+// flight across it (s_waitcnt lgkmcnt(0); s_barrier).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The run's bodies' records (solver v and w, component v and w with the inverse mass:
+// BV_V..BV_CW) are gathered into LDS once per launch, so a manifold's solve needs only its
+// own rows from global memory: slot 0's with the flags, slots 1-3 once the point count is
+// known (one CU's load bandwidth bounds these launches, so unused slots are not read). Each
+// colour's rows are loaded before the barrier that waits for the previous colour's
+// velocities, so their latency overlaps the wait.
+constexpr int BATCH_BODY_F4 = BV_CW + 1;
+__global__ __launch_bounds__(BATCH_THREADS) void k_solve_batch(SubArgs a, int c0, int c1, int S,
+                                                               const int32_t* __restrict__ batchLoc,
+                                                               const int32_t* __restrict__ batchList,
+                                                               const int32_t* __restrict__ batchN, int useBias, float h,
+                                                               int skipSoft) {
+    extern __shared__ float4 rec[];       // [BATCH_BODY_F4][n]: record field f of body slot i at f * n + i
+    __shared__ int32_t cs[BATCH_MAX_COLOURS + 1];   // the run's colour starts
+    const int n = *batchN;
+    const int32_t* list = batchList + 2 * S;
+    for (int i = threadIdx.x; i <= c1 - c0; i += BATCH_THREADS) cs[i] = a.colorStart[c0 + i];
+    for (int i = threadIdx.x; i < n; i += BATCH_THREADS) {
+        const int b = list[i];
+#pragma unroll
+        for (int f = 0; f < BATCH_BODY_F4; ++f) rec[f * n + i] = a.bvel[bv(b, f)];
+    }
+    __syncthreads();
+    auto solve = [&](int s, MInputs& in, int2 l) {
+        if (skipSoft && in.soft) return;
+        f3 v0 = s3(0.f), w0 = s3(0.f), v1 = s3(0.f), w1 = s3(0.f);
+        in.im0 = 0.f; in.im1 = 0.f;
+        in.cv0 = s3(0.f); in.cw0 = s3(0.f); in.cv1 = s3(0.f); in.cw1 = s3(0.f);
+        if (in.b0 >= 0) {
+            v0 = xyz(rec[BV_V * n + l.x]); w0 = xyz(rec[BV_W * n + l.x]);
+            const float4 c = rec[BV_CV * n + l.x];
+            in.im0 = c.w; in.cv0 = xyz(c); in.cw0 = xyz(rec[BV_CW * n + l.x]);
+        }
+        if (in.b1 >= 0) {
+            v1 = xyz(rec[BV_V * n + l.y]); w1 = xyz(rec[BV_W * n + l.y]);
+            const float4 c = rec[BV_CV * n + l.y];
+            in.im1 = c.w; in.cv1 = xyz(c); in.cw1 = xyz(rec[BV_CW * n + l.y]);
+        }
+        solve_core(a, s, in, useBias != 0, h, v0, w0, v1, w1);
+        if (in.b0 >= 0) { rec[BV_V * n + l.x] = f4(v0, 0.f); rec[BV_W * n + l.x] = f4(w0, 0.f); }
+        if (in.b1 >= 0) { rec[BV_V * n + l.y] = f4(v1, 0.f); rec[BV_W * n + l.y] = f4(w1, 0.f); }
+    };
+    for (int c = 0; c < c1 - c0; ++c) {
+        const int e = cs[c + 1];
+        int s = cs[c] + (int)threadIdx.x;
+        MInputs in;
+        int2 l = make_int2(-1, -1);
+        const bool have = s < e;
+        if (have) {
+            load_own<false>(a, s, in);
+            l = *reinterpret_cast<const int2*>(batchLoc + 2 * s);
+        }
+        lds_barrier();   // the previous colour's velocities are in LDS
+        if (have) solve(s, in, l);
+        // colours larger than the workgroup: a thread's manifolds of one colour share no body
+        for (s += BATCH_THREADS; s < e; s += BATCH_THREADS) {
+            load_own<false>(a, s, in);
+            solve(s, in, *reinterpret_cast<const int2*>(batchLoc + 2 * s));
+        }
+    }
+    lds_barrier();
+    for (int i = threadIdx.x; i < n; i += BATCH_THREADS) {
+        const int b = list[i];
+        a.bvel[bv(b, BV_V)] = rec[BV_V * n + i];
+        a.bvel[bv(b, BV_W)] = rec[BV_W * n + i];
     }
 }
 
@@ -1217,9 +1386,10 @@ int sv_color(World& w, int M) {
                                (const unsigned long long*)claims[round & 1], claims[(round + 1) & 1], w.usedMask.p,
                                w.colorUncol.p, w.counters.p);
         }
-        // stable sort by colour -> solve order
-        hipLaunchKernelGGL(k_color_keys, dim3(nb), dim3(256), 0, s, M, w.mfColor.p, w.colorKeys.p, w.colorPerm.p);
-        radix_sort_u64(s, w.colorKeys.p, w.colorPerm.p, M, 8, w.sortT);
+        // stable sort by colour and point count -> solve order
+        hipLaunchKernelGGL(k_color_keys, dim3(nb), dim3(256), 0, s, M, w.mfColor.p, w.npNum.p, w.mfSrc.p,
+                           w.execMode == EXEC_WORLDS ? 0 : 1, w.colorKeys.p, w.colorPerm.p);
+        radix_sort_u64(s, w.colorKeys.p, w.colorPerm.p, M, 7 + COLOR_KEY_SHIFT, w.sortT);
         hipLaunchKernelGGL(k_color_start, dim3(1), dim3(256), 0, s, M, 130, w.colorKeys.p, w.colorCount.p);
         PGX_LAUNCH_CHECK();
         int ovf = 0;
@@ -1238,7 +1408,7 @@ int sv_color(World& w, int M) {
     int used = round;
     for (int r = 1; r <= round; ++r)
         if (uc[r] == 0) { used = r; break; }
-    int nColors = (int)lastKey + 1;
+    int nColors = (int)(lastKey >> COLOR_KEY_SHIFT) + 1;
     w.colorStartHost.assign(starts.begin(), starts.begin() + nColors + 1);
     w.lastRounds = used;
     // contact setup, written in solve order
@@ -1361,10 +1531,26 @@ static void launch_solve(int block, bool eager, hipStream_t s, hipEvent_t e0, hi
     }
 }
 
+// Buffers and launch attributes of the substep loop, set before a graph capture of it (no
+// allocation under capture); idempotent.
+void sv_presize(World& w) {
+    w.tickets.alloc((size_t)(w.iterations + 1) * FLOW_QUEUES);   // k_prepare clears them
+    if (w.tune.colorBatch > 0) {
+        w.batchLoc.alloc(2 * (size_t)w.manifoldCap);
+        w.batchList.alloc(2 * (size_t)w.manifoldCap);
+        w.batchN.alloc(BATCH_MAX_GROUPS);
+        if (!w.batchLdsSet) {
+            PGX_HIP(hipFuncSetAttribute((const void*)k_solve_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)(2 * BATCH_BODY_F4 * sizeof(float4) * BATCH_MAX_MANIFOLDS)));
+            w.batchLdsSet = true;
+        }
+    }
+}
+
 void sv_substeps(World& w, int M, float dt) {
     hipStream_t s = w.stream;
     w.solveEvUsed = 0;
-    w.tickets.alloc((size_t)(w.iterations + 1) * FLOW_QUEUES);   // k_prepare clears them
+    sv_presize(w);
     SubArgs a = make_args(w, M, dt);
     if (w.execMode == EXEC_COOPERATIVE && w.J == 0) {
         int blocks = persistent_blocks(w);
@@ -1421,21 +1607,34 @@ void sv_substeps(World& w, int M, float dt) {
     // waves of the dependency-driven pass: enough to hold every manifold at once, at most
     // what the chip keeps resident comfortably; on XCD queues, 64 workgroups per XCD
     const int flowBlocks = f.xcdScope ? 64 * w.nXcd : std::max(1, std::min(div_up(std::max(M, 1), 256), 2048));
-    // launch groups of the colour passes: (first colour, end colour, batched)
+    // launch groups of the colour passes: (first colour, end colour, batched run index + 1);
+    // a run holds consecutive colours of at most colorBatch manifolds each, at most
+    // BATCH_MAX_MANIFOLDS in all (its bodies' velocities fit in LDS), two colours or more
     std::vector<int3> groups;
+    BatchGroups bg{};
     for (int c = 0; c < nColors;) {
         const int n = w.colorStartHost[c + 1] - w.colorStartHost[c];
-        if (w.tune.colorBatch > 0 && n <= w.tune.colorBatch) {
+        if (w.tune.colorBatch > 0 && n <= w.tune.colorBatch && bg.n < BATCH_MAX_GROUPS) {
             int c1 = c + 1;
-            while (c1 < nColors && w.colorStartHost[c1 + 1] - w.colorStartHost[c1] <= w.tune.colorBatch) ++c1;
+            while (c1 < nColors && w.colorStartHost[c1 + 1] - w.colorStartHost[c1] <= w.tune.colorBatch &&
+                   w.colorStartHost[c1 + 1] - w.colorStartHost[c] <= BATCH_MAX_MANIFOLDS)
+                ++c1;
             if (c1 - c >= 2) {
-                groups.push_back(make_int3(c, c1, 1));
+                bg.S[bg.n] = w.colorStartHost[c];
+                bg.E[bg.n] = w.colorStartHost[c1];
+                groups.push_back(make_int3(c, c1, ++bg.n));
                 c = c1;
                 continue;
             }
         }
         if (n > 0) groups.push_back(make_int3(c, c + 1, 0));
         ++c;
+    }
+    if (bg.n > 0) {   // the runs' body slots, fixed for the step
+        sv_presize(w);
+        hipLaunchKernelGGL(k_batch_map, dim3(1), dim3(BATCH_MAP_THREADS), 0, s, bg, w.sB0.p, w.sB1.p, w.batchLoc.p,
+                           w.batchList.p, w.batchN.p);
+        PGX_LAUNCH_CHECK();
     }
     int passIndex = 0;
     const bool onePass = w.execMode == EXEC_PASS && M > 0 && nColors > 0;
@@ -1514,7 +1713,11 @@ void sv_substeps(World& w, int M, float dt) {
         // colours in one single-workgroup launch
         for (const int3& g : groups) {
             if (g.z) {
-                hipLaunchKernelGGL(k_solve_batch, dim3(1), dim3(BATCH_THREADS), 0, s, a, g.x, g.y, useBias, h, skipSoft);
+                const int q = g.z - 1;
+                // 2 bodies per manifold at most
+                const size_t lds = 2 * BATCH_BODY_F4 * sizeof(float4) * (size_t)(bg.E[q] - bg.S[q]);
+                hipLaunchKernelGGL(k_solve_batch, dim3(1), dim3(BATCH_THREADS), lds, s, a, g.x, g.y, bg.S[q],
+                                   w.batchLoc.p, w.batchList.p, w.batchN.p + q, useBias, h, skipSoft);
                 continue;
             }
             const int c = g.x, b = w.colorStartHost[c], e = w.colorStartHost[c + 1];

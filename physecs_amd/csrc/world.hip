@@ -353,7 +353,7 @@ void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh) {
 
     w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorCount.alloc(160); w.colorPoints.alloc(160);
     w.sB0.alloc(M); w.sB1.alloc(M); w.sFlags.alloc(M); w.sA0.alloc(M); w.sA1.alloc(M); w.sCanon.alloc(M);
-    w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(20 * M); w.sLam.alloc(4 * M);
+    w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(4 * CONTACT_ROW_FIELDS * M); w.sLam.alloc(4 * M);
     for (int k = 0; k < 2; ++k) {
         w.cacheKey[k].alloc(M); w.cacheTri[k].alloc(M); w.cacheNum[k].alloc(M); w.cacheLp[k].alloc(4 * M);
     }
@@ -1250,7 +1250,7 @@ void world_autotune(World& w, float dt, int trialSteps) {
 // the sequence has the shape of the previous step's and instantiated anew otherwise.
 static void substeps_graph(World& w, int M, float dt) {
     hipStream_t s = w.stream;
-    w.tickets.alloc((size_t)(w.iterations + 1) * FLOW_QUEUES);   // no allocation under capture
+    sv_presize(w);   // no allocation under capture
     PGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     hipGraph_t g = nullptr;
     try {

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 TUNINGS = [
     {"solve_block": 256, "color_batch": 1 << 30, "pairs_per_lane": 4, "color_rounds": 1, "eager_rows_max": 0,
      "pair_search": 1},
-    {"solve_block": 128, "color_batch": 512, "pairs_per_lane": 2, "color_rounds": 32, "eager_rows_max": 1 << 30,
+    {"solve_block": 128, "color_batch": 0, "pairs_per_lane": 2, "color_rounds": 32, "eager_rows_max": 1 << 30,
      "pair_search": 0},
     {"solve_block": 64, "color_batch": 64, "pairs_per_lane": 1, "color_rounds": 3, "eager_rows_max": 100,
      "pair_search": 1},
