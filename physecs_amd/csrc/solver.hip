@@ -682,10 +682,26 @@ __global__ __launch_bounds__(256) void k_prepare(SubArgs a, int sub) {
     if (i < a.M) prepare_manifold(a, i);
     else if (i - a.M < a.D) integrate_velocity(a, i - a.M);
 }
+// XCD-homed slots. Workgroups are dispatched round-robin to the 8 XCDs (workgroup b on XCD
+// b % 8), each with its own L2. k_prepare writes slot s's rows from workgroup s / 256, so
+// from XCD (s / 256) % 8, the slot's home; the colour launches give each workgroup a chunk
+// of BLOCK slots homed on its own XCD, so the rows (and the lambdas, rewritten in place every
+// pass) are read from that XCD's L2 rather than from memory. Chunk k holds slots
+// [k * BLOCK, (k + 1) * BLOCK); its home is (k / q) % 8 with q = 256 / BLOCK chunks per
+// prepare workgroup, so home-x chunks come in runs of q, one run every 8q chunks. Workgroup
+// b (XCD x = b % 8) takes the (b / 8)-th home-x chunk at or after chunk k0.
+__device__ __forceinline__ int homed_chunk(int b, int k0, int q) {
+    const int x = b & 7, i = b >> 3, period = 8 * q;
+    const int g = k0 / period, r = k0 - g * period;
+    const int m = q * g + min(max(r - q * x, 0), q) + i;   // home-x chunks below k0, plus i
+    return period * (m / q) + q * x + (m % q);
+}
+
 template <int BLOCK, bool EAGER>
 __global__ __launch_bounds__(BLOCK) void k_solve(SubArgs a, int begin, int end, int useBias, float h, int skipSoft) {
-    int s = begin + blockIdx.x * BLOCK + threadIdx.x;
-    if (s < end) solve_manifold<EAGER>(a, s, useBias != 0, h, skipSoft != 0);
+    const int k = homed_chunk(blockIdx.x, begin / BLOCK, 256 / BLOCK);
+    int s = k * BLOCK + threadIdx.x;
+    if (s >= begin && s < end) solve_manifold<EAGER>(a, s, useBias != 0, h, skipSoft != 0);
 }
 
 // ------------------------------------------------------------------ one launch per pass
@@ -1513,12 +1529,25 @@ bool sv_world_pass_fits(World& w) {
 
 // one colour [b, e) in workgroups of `block` threads; with events, stamped from the
 // launch's own dispatch packet
+// Workgroups of a colour launch over slots [b, e) in chunks of bs homed on their XCD
+// (homed_chunk): 8 per round, as many rounds as the XCD with the most chunks needs.
+static int homed_blocks(int b, int e, int bs) {
+    const int q = 256 / bs, period = 8 * q, k0 = b / bs, k1 = (e - 1) / bs + 1;   // chunks [k0, k1)
+    auto below = [&](int k, int x) {   // home-x chunks below chunk k
+        const int g = k / period, r = k - g * period;
+        return q * g + std::min(std::max(r - q * x, 0), q);
+    };
+    int most = 0;
+    for (int x = 0; x < 8; ++x) most = std::max(most, below(k1, x) - below(k0, x));
+    return 8 * most;
+}
+
 static void launch_solve(int block, bool eager, hipStream_t s, hipEvent_t e0, hipEvent_t e1, const SubArgs& a, int b,
                          int e, int useBias, float h, int skipSoft) {
     auto go = [&](auto kernel, int bs) {
-        if (e0) hipExtLaunchKernelGGL(kernel, dim3(div_up(e - b, bs)), dim3(bs), 0, s, e0, e1, 0, a, b, e, useBias, h,
-                                      skipSoft);
-        else hipLaunchKernelGGL(kernel, dim3(div_up(e - b, bs)), dim3(bs), 0, s, a, b, e, useBias, h, skipSoft);
+        const int nb = homed_blocks(b, e, bs);
+        if (e0) hipExtLaunchKernelGGL(kernel, dim3(nb), dim3(bs), 0, s, e0, e1, 0, a, b, e, useBias, h, skipSoft);
+        else hipLaunchKernelGGL(kernel, dim3(nb), dim3(bs), 0, s, a, b, e, useBias, h, skipSoft);
     };
     if (eager) {
         if (block == 256) go(k_solve<256, true>, 256);
