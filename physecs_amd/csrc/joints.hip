@@ -452,13 +452,10 @@ __device__ void row_solve_scalar(const JointArgs& a, int j, int r, int flags, bo
     }
 }
 
-// one SIMD colour: a thread per joint, its rows in flag-type order. The joint's bodies
-// belong to no other joint of the colour, so their velocities (or pseudo velocities) stay
-// in registers across the joint's rows: one load and one store per body.
-__global__ void k_joint_color(JointArgs a, int begin, int end, int phase, int warm) {
-    int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= end) return;
-    int j = a.order[k];
+// one joint of a SIMD colour, its rows in flag-type order. The joint's bodies belong to no
+// other joint of the colour, so their velocities (or pseudo velocities) stay in registers
+// across the joint's rows: one load and one store per body.
+__device__ void joint_color_one(const JointArgs& a, int j, int phase, int warm) {
     int n = a.jNRows[j];
     int b0 = a.jB0[j], b1 = a.jB1[j];
     // the joint's rows in solve order (flag type by flag type, creation order inside a
@@ -499,6 +496,31 @@ __global__ void k_joint_color(JointArgs a, int begin, int end, int phase, int wa
     }
     if (b0 >= 0) { set_xyz(tv0, v0); set_xyz(tw0, w0); a.bvel[bv(b0, BV_V)] = tv0; a.bvel[bv(b0, BV_W)] = tw0; }
     if (b1 >= 0) { set_xyz(tv1, v1); set_xyz(tw1, w1); a.bvel[bv(b1, BV_V)] = tv1; a.bvel[bv(b1, BV_W)] = tw1; }
+}
+
+// one SIMD colour: a thread per joint
+__global__ void k_joint_color(JointArgs a, int begin, int end, int phase, int warm) {
+    int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < end) joint_color_one(a, a.order[k], phase, warm);
+}
+
+// Every SIMD colour (0-7) of a joint pass in one launch: joints that share a dynamic body,
+// directly or through other joints, form an island, and one workgroup takes each island,
+// its joints colour by colour with a workgroup barrier between colours. Islands share no
+// dynamic body, so each body sees its joints in colour order, as with one launch per
+// colour, and the floats are the same. islandCS: per island, 9 offsets into islandJoint
+// (the start of each colour 0-7, then the island's end).
+constexpr int ISLAND_THREADS = 64;
+__global__ __launch_bounds__(ISLAND_THREADS) void k_joint_islands(JointArgs a, const int32_t* __restrict__ islandCS,
+                                                                  const int32_t* __restrict__ islandJoint, int phase,
+                                                                  int warm) {
+    const int32_t* cs = islandCS + 9 * blockIdx.x;
+    for (int c = 0; c < 8; ++c) {
+        const int b = cs[c], e = cs[c + 1];
+        if (b == e) continue;   // the same for the whole workgroup
+        for (int k = b + (int)threadIdx.x; k < e; k += ISLAND_THREADS) joint_color_one(a, islandJoint[k], phase, warm);
+        __syncthreads();        // this colour's velocity stores are seen by the next colour's loads
+    }
 }
 
 // the overflow colour: one thread, flag type by flag type over all its joints (which may
@@ -559,6 +581,15 @@ void jt_update(World& w, float h) {
 void jt_colors(World& w, float h, int phase, bool warm) {
     if (!w.J) return;
     JointArgs a = make_jargs(w, h);
+    if (w.jIslands > 0) {   // colours 0-7 by island in one launch, then the overflow colour
+        hipLaunchKernelGGL(k_joint_islands, dim3(w.jIslands), dim3(ISLAND_THREADS), 0, w.stream, a, w.jIslandCS.p,
+                           w.jIslandJoint.p, phase, warm ? 1 : 0);
+        const int b = w.jColorStart[8], e = w.jColorStart[9];
+        if (e > b)
+            hipLaunchKernelGGL(k_joint_overflow, dim3(1), dim3(64), 0, w.stream, a, b, e, phase, warm ? 1 : 0);
+        PGX_LAUNCH_CHECK();
+        return;
+    }
     for (int c = 0; c < 9; ++c) {
         int b = w.jColorStart[c], e = w.jColorStart[c + 1];
         if (e <= b) continue;

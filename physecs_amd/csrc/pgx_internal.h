@@ -277,6 +277,8 @@ struct World {
     std::vector<unsigned char> hjBits;           // per internal body: colours taken (src/Physecs.cpp:696-705)
     std::vector<int> jColorStart;                // 10 offsets into jOrder
     DBuf<int32_t> jType, jA0, jA1, jB0, jB1, jRowFlags, jNRows, jOrder;
+    int jIslands = 0;                    // islands of the one-launch joint passes (0: per-colour launches)
+    DBuf<int32_t> jIslandCS, jIslandJoint;   // per island 9 colour offsets; joints by island and colour
     DBuf<float4> jP0, jQ0, jP1, jQ1, jDrive, jR0, jR1, jRow;
     DBuf<float4> jPar0, jPar1;           // (lower, upper, target, stiffness), (damping, gear ratio)
     DBuf<float4> jGear;                  // gear state: persistent angle 0, 1, virtual angle 0, 1

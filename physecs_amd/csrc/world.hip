@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <numeric>
 #include <unordered_map>
 #include "pgx_internal.h"
@@ -645,6 +646,9 @@ static void grow_keep(DBuf<T>& b, int keep, int n, hipStream_t s) {
     b.n = (size_t)std::max(n, 1);
 }
 
+constexpr int ISLAND_MAX = 64;   // joints of one colour in one island: one wave's lanes
+constexpr int ISLAND_MIN = 1;
+
 // The live joints grouped by colour, creation order inside a colour, and the per-joint
 // arrays, to the device.
 static void upload_joints(World& w, int keepState) {
@@ -659,6 +663,56 @@ static void upload_joints(World& w, int keepState) {
     std::vector<int> fillp(w.jColorStart.begin(), w.jColorStart.end() - 1);
     for (int j = 0; j < w.J; ++j)
         if (w.hjAlive[j]) order[fillp[w.hjColor[j]]++] = j;
+    // islands of the SIMD colours (0-7): joints joined through dynamic bodies; each island's
+    // joints by colour, in the colour order above inside a colour. One launch per joint pass
+    // solves them all (k_joint_islands) when no island holds more than ISLAND_MAX joints of
+    // one colour; per-colour launches otherwise.
+    {
+        std::vector<int> parent(w.D > 0 ? w.D : 1);
+        std::iota(parent.begin(), parent.end(), 0);
+        std::function<int(int)> root = [&](int x) {
+            while (parent[x] != x) x = parent[x] = parent[parent[x]];
+            return x;
+        };
+        for (int k = 0; k < w.jColorStart[8]; ++k) {
+            const int j = order[k], b0 = w.hjB0[j], b1 = w.hjB1[j];
+            if (b0 >= 0 && b1 >= 0) parent[root(b0)] = root(b1);
+        }
+        // island of each joint: its dynamic body's root, or a fresh island for a joint
+        // between two non-dynamic bodies
+        std::vector<int> islandOf(w.jColorStart[8]);
+        std::unordered_map<int, int> idOfRoot;
+        int nIsl = 0;
+        for (int k = 0; k < w.jColorStart[8]; ++k) {
+            const int j = order[k], b = w.hjB0[j] >= 0 ? w.hjB0[j] : w.hjB1[j];
+            if (b < 0) { islandOf[k] = nIsl++; continue; }
+            auto it = idOfRoot.emplace(root(b), nIsl);
+            if (it.second) ++nIsl;
+            islandOf[k] = it.first->second;
+        }
+        std::vector<int32_t> cnt((size_t)nIsl * 8, 0);   // joints per island and colour
+        for (int k = 0; k < w.jColorStart[8]; ++k) ++cnt[(size_t)islandOf[k] * 8 + w.hjColor[order[k]]];
+        const int most = cnt.empty() ? 0 : *std::max_element(cnt.begin(), cnt.end());
+        w.jIslands = 0;
+        if (nIsl >= ISLAND_MIN && most <= ISLAND_MAX) {
+            std::vector<int32_t> cs((size_t)nIsl * 9), cursor((size_t)nIsl * 8), list(w.jColorStart[8]);
+            int at = 0;
+            for (int i = 0; i < nIsl; ++i) {
+                for (int c = 0; c < 8; ++c) {
+                    cs[(size_t)i * 9 + c] = cursor[(size_t)i * 8 + c] = at;
+                    at += cnt[(size_t)i * 8 + c];
+                }
+                cs[(size_t)i * 9 + 8] = at;
+            }
+            for (int k = 0; k < w.jColorStart[8]; ++k) {   // colour order, creation order inside a colour
+                const int j = order[k];
+                list[cursor[(size_t)islandOf[k] * 8 + w.hjColor[j]]++] = j;
+            }
+            w.jIslands = nIsl;
+            upload(w.jIslandCS, cs, w.stream);
+            upload(w.jIslandJoint, list, w.stream);
+        }
+    }
     hipStream_t s = w.stream;
     w.jCap = w.J;
     upload(w.jType, w.hjType, s); upload(w.jA0, w.hjA0, s); upload(w.jA1, w.hjA1, s);
