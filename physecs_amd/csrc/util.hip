@@ -197,13 +197,22 @@ void radix_sort_u64_into(hipStream_t s, uint64_t* keys, int32_t* vals, int n, in
     }
 }
 
+// Scratch sizes rounded up to a power of two: the sorts' sizes change from step to step, and
+// growing a buffer to the exact size would free and allocate again (and hipFree waits for the
+// device) every time a step sets a new maximum.
+static size_t scratch_size(size_t n) {
+    size_t p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
 void radix_sort_u64(hipStream_t s, uint64_t* keys, int32_t* vals, int n, int bits, SortTemp& t) {
     const RadixScratch r = radix_sort_scratch(n, bits);
     if (!r.keys) return;
-    t.kalt.alloc(r.keys);
-    t.valt.alloc(r.vals);
-    t.hist.alloc(std::max<size_t>(r.hist, 1));
-    t.scan.sums.alloc(std::max<size_t>(r.sums, 1));
+    t.kalt.alloc(scratch_size(r.keys));
+    t.valt.alloc(scratch_size(r.vals));
+    t.hist.alloc(scratch_size(std::max<size_t>(r.hist, 1)));
+    t.scan.sums.alloc(scratch_size(std::max<size_t>(r.sums, 1)));
     radix_sort_u64_into(s, keys, vals, n, bits, t.kalt.p, t.valt.p, t.hist.p, t.scan.sums.p);
 }
 
@@ -296,7 +305,7 @@ void exclusive_scan_i32_into(hipStream_t s, const int32_t* in, int32_t* out, int
 }
 
 void exclusive_scan_i32(hipStream_t s, const int32_t* in, int32_t* out, int n, int32_t* total, ScanTemp& t) {
-    if (n > 0) t.sums.alloc(scan_blocks(n));
+    if (n > 0) t.sums.alloc(scratch_size(scan_blocks(n)));
     exclusive_scan_i32_into(s, in, out, n, total, t.sums.p);
 }
 
