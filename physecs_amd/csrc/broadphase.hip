@@ -251,6 +251,24 @@ __global__ void k_cell_codes(int n, const int32_t* __restrict__ dynCols, const i
     vals[i] = c;
 }
 
+// the first collider of each cell's run claims a table slot (linear probing)
+__global__ void k_cell_insert(int n, const uint64_t* __restrict__ keys, uint32_t mask, uint64_t* __restrict__ tabKey,
+                              int32_t* __restrict__ tabStart) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    if (i > 0 && keys[i - 1] == k) return;
+    uint32_t slot = cell_hash(k) & mask;
+    for (uint32_t probe = 0; probe <= mask; ++probe) {   // the table is at least twice the cells
+        const unsigned long long prev = atomicCAS((unsigned long long*)&tabKey[slot], GRID_EMPTY, k);
+        if (prev == GRID_EMPTY) {
+            tabStart[slot] = i;
+            return;
+        }
+        slot = (slot + 1) & mask;
+    }
+}
+
 __device__ __forceinline__ int cell_find(uint64_t k, uint32_t mask, const uint64_t* __restrict__ tabKey) {
     uint32_t slot = cell_hash(k) & mask;
     for (uint32_t probe = 0; probe <= mask; ++probe) {
@@ -262,55 +280,24 @@ __device__ __forceinline__ int cell_find(uint64_t k, uint32_t mask, const uint64
     return -1;
 }
 
-// The dynamic colliders grouped by cell with a counting sort over the hash table's slots:
-// each collider finds or claims its cell's slot (linear probing) and counts itself there;
-// the counts are scanned into run starts; each collider then takes a place in its cell's run.
-// The order inside a run is whatever order the places were taken in: the search tests every
-// pair of a run once (each collider against those after it) whatever the order, and the
-// pairs are put in canonical order afterwards.
-__global__ void k_cell_count(int n, const uint64_t* __restrict__ keys, uint32_t mask, uint64_t* __restrict__ tabKey,
-                             int32_t* __restrict__ tabCount, int32_t* __restrict__ cellSlot,
-                             int32_t* __restrict__ counters) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t k = keys[i];
-    uint32_t slot = cell_hash(k) & mask;
-    for (uint32_t probe = 0; probe <= mask; ++probe) {   // the table is at least twice the cells
-        const unsigned long long prev = atomicCAS((unsigned long long*)&tabKey[slot], GRID_EMPTY, k);
-        if (prev == GRID_EMPTY || prev == k) {
-            atomicAdd(&tabCount[slot], 1);
-            cellSlot[i] = (int)slot;
-            return;
-        }
-        slot = (slot + 1) & mask;
-    }
-    atomicOr(&counters[14], 2);   // never expected: a full table
-    cellSlot[i] = -1;
-}
-
-// each collider's place in its cell's run; the collider that takes a run's first place
-// records the run's end and its length for the longest run (counters[15]: one atomic per
-// workgroup, after a wave and workgroup maximum)
-__global__ __launch_bounds__(256) void k_cell_place(int n, const uint64_t* __restrict__ keys,
-                                                    const int32_t* __restrict__ vals,
-                                                    const int32_t* __restrict__ cellSlot,
-                                                    const int32_t* __restrict__ tabCount,
-                                                    const int32_t* __restrict__ tabStart, int32_t* __restrict__ tabFill,
-                                                    int32_t* __restrict__ tabEnd, uint64_t* __restrict__ outKeys,
-                                                    int32_t* __restrict__ outVals, int32_t* __restrict__ counters) {
+// the last collider of each run records its end, and the longest run (counters[15]: one
+// atomic per workgroup, after a wave and workgroup maximum)
+__global__ __launch_bounds__(256) void k_cell_ends(int n, const uint64_t* __restrict__ keys, uint32_t mask,
+                                                   const uint64_t* __restrict__ tabKey,
+                                                   const int32_t* __restrict__ tabStart, int32_t* __restrict__ tabEnd,
+                                                   int32_t* __restrict__ counters) {
     __shared__ int32_t waveMax[256 / WAVE];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int run = 0;
     if (i < n) {
-        const int slot = cellSlot[i];
-        if (slot >= 0) {
-            const int start = tabStart[slot];
-            const int at = atomicAdd(&tabFill[slot], 1);
-            outKeys[start + at] = keys[i];
-            outVals[start + at] = vals[i];
-            if (at == 0) {
-                run = tabCount[slot];
-                tabEnd[slot] = start + run;
+        const uint64_t k = keys[i];
+        if (i + 1 == n || keys[i + 1] != k) {
+            const int slot = cell_find(k, mask, tabKey);
+            if (slot < 0) {
+                atomicOr(&counters[14], 2);   // never expected: the run start inserted it
+            } else {
+                tabEnd[slot] = i + 1;
+                run = i + 1 - tabStart[slot];
             }
         }
     }
@@ -324,7 +311,7 @@ __global__ __launch_bounds__(256) void k_cell_place(int n, const uint64_t* __res
     }
 }
 
-// The grid search's record of each dynamic collider in run order, one 32-byte line:
+// The grid search's record of each dynamic collider in sorted order, one 32-byte line:
 // {min xyz, world-local index}, {max xyz, entity}; a disabled collider gets an empty box, so
 // it overlaps nothing. And the world's non-dynamic colliders, as k_pack writes them for the
 // tree walk.
@@ -916,9 +903,8 @@ static uint32_t grid_table_size(int n) {
     return t;
 }
 
-// The step's pairs on the uniform grid: cell codes of the dynamic colliders, grouped by cell
-// into runs found through the hash table; the grid search. (The tree walk's inputs are not
-// built.)
+// The step's pairs on the uniform grid: cell codes of the dynamic colliders, sorted; the
+// cells' runs in the hash table; the grid search. (The tree walk's inputs are not built.)
 static void bp_grid(World& w) {
     hipStream_t s = w.stream;
     const int n = w.nDynCol;
@@ -929,24 +915,20 @@ static void bp_grid(World& w) {
     hipLaunchKernelGGL(k_cell_codes, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.dynCols.p, w.cworld.p, w.cbmin.p,
                        w.cbmax.p, w.worldBoundsI.p, w.sceneMoments.p, w.mkeys.p, w.mvals.p);
     PGX_LAUNCH_CHECK();
-    // group the colliders by cell: count per table slot, scan, place
-    fill_u64(s, w.gridKey.p, GRID_EMPTY, T);
-    fill_i32(s, w.gridCount.p, 0, 2 * (int)T);   // counts, then places taken
-    hipLaunchKernelGGL(k_cell_count, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, T - 1, w.gridKey.p,
-                       w.gridCount.p, w.gridSlot.p, w.counters.p);
-    exclusive_scan_i32(s, w.gridCount.p, w.gridStart.p, (int)T, nullptr, w.scanT);
-    hipLaunchKernelGGL(k_cell_place, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, w.mvals.p, w.gridSlot.p,
-                       w.gridCount.p, w.gridStart.p, w.gridCount.p + T, w.gridEnd.p, w.gridCellKey.p,
-                       w.gridCellCol.p, w.counters.p);
+    radix_sort_u64(s, w.mkeys.p, w.mvals.p, n, 30 + w.worldBits, w.sortT);
     const int nStatic = w.C - w.nDynCol;
-    hipLaunchKernelGGL(k_grid_pack, dim3(div_up(std::max(n, nStatic), 256)), dim3(256), 0, s, n, w.gridCellCol.p,
+    hipLaunchKernelGGL(k_grid_pack, dim3(div_up(std::max(n, nStatic), 256)), dim3(256), 0, s, n, w.mvals.p,
                        w.cbmin.p, w.cbmax.p, w.cflags.p, w.centity.p, w.clocal.p, w.gridRec.p, nStatic, w.staticCols.p,
                        w.statA.p, w.statB.p, w.statL.p);
+    fill_u64(s, w.gridKey.p, GRID_EMPTY, T);
+    hipLaunchKernelGGL(k_cell_insert, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, T - 1, w.gridKey.p,
+                       w.gridStart.p);
+    hipLaunchKernelGGL(k_cell_ends, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, T - 1, w.gridKey.p,
+                       w.gridStart.p, w.gridEnd.p, w.counters.p);
     // 16 lanes per collider at every size: fewer lanes, each opening several cells one after
     // another, measured slower on the 857k-collider batch (506 against 390 us)
     const long long lanes = (long long)n * GRID_LANES;
-    hipLaunchKernelGGL(k_grid_pairs, dim3((unsigned)div_up(lanes, GRID_BLOCK)), dim3(GRID_BLOCK), 0, s, n,
-                       w.gridCellKey.p,
+    hipLaunchKernelGGL(k_grid_pairs, dim3((unsigned)div_up(lanes, GRID_BLOCK)), dim3(GRID_BLOCK), 0, s, n, w.mkeys.p,
                        w.gridRec.p, T - 1, w.gridKey.p, w.gridStart.p, w.gridEnd.p, w.worldStaticStart.p, w.statA.p,
                        w.statB.p, w.statL.p, w.localBits, w.counters.p, w.pairCap, w.pkeys.p);
     PGX_LAUNCH_CHECK();

@@ -184,9 +184,6 @@ struct World {
     long gridFallbacks = 0;              // steps whose grid had a cell run too long (tree walk used)
     DBuf<uint64_t> gridKey;              // the grid's hash table: cell key, run start and end
     DBuf<int32_t> gridStart, gridEnd;
-    DBuf<int32_t> gridCount, gridSlot;   // per slot: colliders, places taken (2 x table); each collider's slot
-    DBuf<uint64_t> gridCellKey;          // the dynamic colliders' cell keys and colliders, grouped by cell
-    DBuf<int32_t> gridCellCol;
     DBuf<float4> gridRec;                // the grid search's collider records: 2 float4 per dynamic collider
     DBuf<int4> travC;
     // the non-dynamic colliders in static-list order: {min, flags}, {max, entity}, local index

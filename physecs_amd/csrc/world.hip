@@ -311,9 +311,7 @@ void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh) {
     {
         uint32_t t = 64;   // the grid's hash table: at least twice the dynamic colliders
         while (t < 2u * (uint32_t)std::max(w.nDynCol, 1)) t <<= 1;
-        w.gridKey.alloc(t); w.gridStart.alloc(t); w.gridEnd.alloc(t); w.gridCount.alloc(2 * (size_t)t);
-        w.gridSlot.alloc(std::max(w.nDynCol, 1)); w.gridCellKey.alloc(std::max(w.nDynCol, 1));
-        w.gridCellCol.alloc(std::max(w.nDynCol, 1));
+        w.gridKey.alloc(t); w.gridStart.alloc(t); w.gridEnd.alloc(t);
         w.gridRec.alloc(2 * (size_t)std::max(w.nDynCol, 1));
     }
     w.mkeys.alloc(C); w.mvals.alloc(C);
