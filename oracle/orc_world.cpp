@@ -19,6 +19,9 @@
 #include <cmath>
 #include <limits>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
+#include <memory>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -129,6 +132,56 @@ struct JointRows {                     // one joint's rows, in solve order
 
 }  // namespace
 
+// The reference's ThreadPool (src/ThreadPool.cpp:13-46): workers created once, started for
+// each parallelFor, the caller taking tasks too and then spinning until every worker is done.
+// (C++17 here: workers wait on a condition variable instead of std::atomic::wait.)
+struct Pool {
+    std::vector<std::thread> threads;
+    std::mutex m;
+    std::condition_variable cv;
+    uint64_t gen = 0;
+    bool quit = false;
+    std::function<void()> job;
+    std::atomic<int> running{0};
+    explicit Pool(int n) {
+        for (int t = 0; t < n; ++t) threads.emplace_back([this] { loop(); });
+    }
+    void loop() {
+        uint64_t seen = 0;
+        while (true) {
+            std::function<void()> f;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv.wait(lk, [&] { return quit || gen != seen; });
+                if (quit) return;
+                seen = gen;
+                f = job;
+            }
+            f();
+            running.fetch_sub(1);
+        }
+    }
+    void run(const std::function<void()>& f) {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            job = f;
+            running.store((int)threads.size());
+            ++gen;
+        }
+        cv.notify_all();
+        f();
+        while (running.load() != 0) std::this_thread::yield();
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            quit = true;
+        }
+        cv.notify_all();
+        for (auto& t : threads) t.join();
+    }
+};
+
 struct orc_world {
     std::vector<ConvexMesh> convex;
     std::vector<TriMesh> trimesh;
@@ -155,6 +208,7 @@ struct orc_world {
     std::unordered_map<uint32_t, unsigned char> jointBitsets;   // per entity, src/Physecs.cpp:696-705
     std::vector<JointRows> jrows[9];                           // per colour, joints in creation order
     int numSubSteps = 8, numIterations = 2, numThreads = 0;
+    std::unique_ptr<Pool> pool;   // numThreads workers, made when the thread count is set
     float g = 9.81f;
     std::mutex mtx;
 };
@@ -270,6 +324,10 @@ void orc_world_set_params(orc_world* w, int substeps, int iterations, float grav
     w->numSubSteps = substeps;
     w->numIterations = iterations;
     w->g = gravity;
+    if (threads != w->numThreads || (threads > 0 && !w->pool)) {
+        w->pool.reset();
+        if (threads > 0) w->pool.reset(new Pool(threads));
+    }
     w->numThreads = threads;
 }
 
@@ -460,10 +518,7 @@ static void narrowphase(orc_world* w) {
             }
         }
     };
-    std::vector<std::thread> th;
-    for (int t = 0; t < T; ++t) th.emplace_back(worker);
-    worker();
-    for (auto& t : th) t.join();
+    w->pool->run(worker);
 }
 
 static void applyOrder(orc_world* w) {
