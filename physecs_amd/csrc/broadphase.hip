@@ -30,9 +30,11 @@ __global__ void k_bounds(int C, const int32_t* __restrict__ cbody, const int32_t
                          const int32_t* __restrict__ cworld, const uint8_t* __restrict__ worldMask,
                          const uint8_t* __restrict__ colMask, const float4* __restrict__ cvVerts,
                          const int4* __restrict__ cvMesh, const float4* __restrict__ tmVerts,
-                         const int4* __restrict__ tmMesh, float4* __restrict__ bmin, float4* __restrict__ bmax) {
+                         const int4* __restrict__ tmMesh, int meshesApart, float4* __restrict__ bmin,
+                         float4* __restrict__ bmax) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= C) return;
+    if (meshesApart && (ctype[c] == PGX_CONVEX_MESH || ctype[c] == PGX_TRIANGLE_MESH)) return;   // k_bounds_mesh
     if (dynamicOnly && !(cflags[c] & CF_DYNAMIC)) return;
     if (worldMask && !worldMask[cworld[c]]) return;
     if (colMask && !colMask[c]) return;   // the colliders of moved entities only, with the margin
@@ -49,12 +51,84 @@ __global__ void k_bounds(int C, const int32_t* __restrict__ cbody, const int32_t
     bmax[c] = f4(mx, 0.f);
 }
 
+// The box of one convex or triangle-mesh collider, one wave per collider: the lanes take
+// every 64th vertex and the wave reduces. BoundsUtil folds min/max over the vertices in
+// order and keeps the earlier of equal values (glm::min / max), which decides the sign of a
+// zero; the reduction carries each candidate's vertex index and keeps the lower one on ties,
+// so the bits are those of the sequential fold.
+struct MinMax {
+    float v;
+    int i;
+};
+__device__ __forceinline__ MinMax pick_min(MinMax a, MinMax b) { return (b.v < a.v || (b.v == a.v && b.i < a.i)) ? b : a; }
+__device__ __forceinline__ MinMax pick_max(MinMax a, MinMax b) { return (a.v < b.v || (b.v == a.v && b.i < a.i)) ? b : a; }
+
+__global__ __launch_bounds__(WAVE) void k_bounds_mesh(int nMesh, const int32_t* __restrict__ meshCols,
+                                                      const int32_t* __restrict__ cbody, const int32_t* __restrict__ ctype,
+                                                      const int32_t* __restrict__ cflags, const float4* __restrict__ cparams,
+                                                      const float4* __restrict__ clpos, const float4* __restrict__ clquat,
+                                                      const float4* __restrict__ bpos, const float4* __restrict__ bquat,
+                                                      int dynamicOnly, const int32_t* __restrict__ cworld,
+                                                      const uint8_t* __restrict__ worldMask,
+                                                      const uint8_t* __restrict__ colMask,
+                                                      const float4* __restrict__ cvVerts, const int4* __restrict__ cvMesh,
+                                                      const float4* __restrict__ tmVerts, const int4* __restrict__ tmMesh,
+                                                      float4* __restrict__ bmin, float4* __restrict__ bmax) {
+    const int c = meshCols[blockIdx.x];
+    if (dynamicOnly && !(cflags[c] & CF_DYNAMIC)) return;
+    if (worldMask && !worldMask[cworld[c]]) return;
+    if (colMask && !colMask[c]) return;
+    f3 pos;
+    qf q;
+    collider_world_pose(bpos, bquat, clpos, clquat, cbody[c], c, pos, q);
+    // src/BoundsUtil.cpp:61-69 (convex: the padded vertex buffer) and :77-85 (triangle mesh)
+    const float4 prm = cparams[c];
+    const bool cvx = ctype[c] == PGX_CONVEX_MESH;
+    const int4 mi = cvx ? cvMesh[(int)prm.w] : tmMesh[(int)prm.w];
+    const float4* v = (cvx ? cvVerts : tmVerts) + mi.x;
+    const int n = cvx ? ((mi.y + 3) & ~3) : mi.y;
+    const f3 sc = cvx ? mk3(prm.x, prm.y, prm.z) : s3(1.f);
+    const int lane = threadIdx.x;
+    MinMax lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = {3.402823466e+38f, 0x7fffffff};
+        hi[k] = {-3.402823466e+38f, 0x7fffffff};
+    }
+    for (int i = lane; i < n; i += WAVE) {   // each lane's vertices in order: its first extreme kept
+        const f3 t = add(pos, qrot(q, cvx ? mulv(sc, xyz(v[i])) : xyz(v[i])));
+        const float tv[3] = {t.x, t.y, t.z};
+        for (int k = 0; k < 3; ++k) {
+            if (tv[k] < lo[k].v) lo[k] = {tv[k], i};
+            if (hi[k].v < tv[k]) hi[k] = {tv[k], i};
+        }
+    }
+    for (int o = WAVE / 2; o >= 1; o >>= 1)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = pick_min(lo[k], MinMax{__shfl_xor(lo[k].v, o), __shfl_xor(lo[k].i, o)});
+            hi[k] = pick_max(hi[k], MinMax{__shfl_xor(hi[k].v, o), __shfl_xor(hi[k].i, o)});
+        }
+    if (lane != 0) return;
+    f3 mn = mk3(lo[0].v, lo[1].v, lo[2].v), mx = mk3(hi[0].v, hi[1].v, hi[2].v);
+    if (dynamicOnly || colMask) {   // Bounds::addMargin(vec3(0.01f)), src/Bounds.cpp:23-26
+        mx = add(mx, s3(0.01f));
+        mn = sub(mn, s3(0.01f));
+    }
+    bmin[c] = f4(mn, 0.f);
+    bmax[c] = f4(mx, 0.f);
+}
+
 void bp_compute_bounds(World& w, bool dynamicOnlyWithMargin, const uint8_t* worldMask, const uint8_t* colMask) {
     if (w.C == 0) return;
+    const int nMesh = (int)w.meshColsHost.size();
     hipLaunchKernelGGL(k_bounds, dim3(div_up(w.C, 256)), dim3(256), 0, w.stream, w.C, w.cbody.p, w.ctype.p,
                        w.cflags.p, w.cparams.p, w.clpos.p, w.clquat.p, w.bpos.p, w.bquat.p,
                        dynamicOnlyWithMargin ? 1 : 0, w.cworld.p, worldMask, colMask, w.cvVerts.p, w.cvMesh.p,
-                       w.tmVerts.p, w.tmMesh.p, w.cbmin.p, w.cbmax.p);
+                       w.tmVerts.p, w.tmMesh.p, nMesh > 0 ? 1 : 0, w.cbmin.p, w.cbmax.p);
+    if (nMesh > 0)
+        hipLaunchKernelGGL(k_bounds_mesh, dim3(nMesh), dim3(WAVE), 0, w.stream, nMesh, w.meshCols.p, w.cbody.p,
+                           w.ctype.p, w.cflags.p, w.cparams.p, w.clpos.p, w.clquat.p, w.bpos.p, w.bquat.p,
+                           dynamicOnlyWithMargin ? 1 : 0, w.cworld.p, worldMask, colMask, w.cvVerts.p, w.cvMesh.p,
+                           w.tmVerts.p, w.tmMesh.p, w.cbmin.p, w.cbmax.p);
     PGX_LAUNCH_CHECK();
 }
 

@@ -158,6 +158,8 @@ struct World {
     int nDynCol = 0;                     // colliders of non-kinematic dynamic bodies (the LBVH leaves)
     DBuf<int32_t> dynCols;               // those colliders
     DBuf<int32_t> staticCols;            // the other colliders, grouped by world
+    DBuf<int32_t> meshCols;              // convex and triangle-mesh colliders: their boxes one wave each
+    std::vector<int32_t> meshColsHost;
     DBuf<int32_t> worldStaticStart;      // W + 1 offsets into staticCols
     DBuf<int32_t> refitReady;
 

@@ -242,6 +242,7 @@ void convert_scene(const pgx_scene_desc* sc, uint32_t entityBase, SceneArrays& o
 // is cleared.
 void world_reset_host(World& w) {
     w.J = 0; w.jCap = 0; w.jRowsTotal = 0;
+    w.meshColsHost.clear();   // the server's worlds bound meshes in k_bounds
     for (auto* v : {&w.hjType, &w.hjA0, &w.hjA1, &w.hjB0, &w.hjB1, &w.hjRowFlags, &w.hjNRows, &w.hjColor, &w.hjAlive})
         v->clear();
     for (auto* v : {&w.hjP0, &w.hjQ0, &w.hjP1, &w.hjQ1, &w.hjDrive, &w.hjPar0, &w.hjPar1}) v->clear();
@@ -389,6 +390,10 @@ void world_init(World& w, const pgx_config* cfg, const pgx_scene_desc* sc) {
     upload(w.cdata, cdata, s);
     upload(w.worldColStart, o.wstart, s); upload(w.colOfLocal, o.colOfLocal, s);
     upload(w.dynCols, o.dynCols, s); upload(w.staticCols, o.staticCols, s); upload(w.worldStaticStart, o.wss, s);
+    w.meshColsHost.clear();
+    for (int c = 0; c < o.C; ++c)
+        if (sc->col_type[c] == PGX_CONVEX_MESH || sc->col_type[c] == PGX_TRIANGLE_MESH) w.meshColsHost.push_back(c);
+    if (!w.meshColsHost.empty()) upload(w.meshCols, w.meshColsHost, s);
     world_alloc_derived(w, cfg, sc->n_trimesh);
     // trigger lists: at most one entry per pair slot (pairCap is set by world_alloc_derived)
     w.hasTriggers = false;
