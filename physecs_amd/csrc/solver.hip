@@ -84,11 +84,22 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     return x;
 }
 
-__global__ void k_world_mstart(int M, const uint64_t* __restrict__ mfKey, int worldShift, int32_t* __restrict__ wstart) {
-    int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= M) return;
-    int wd = (int)(mfKey[m] >> worldShift);
-    if (m == 0 || (int)(mfKey[m - 1] >> worldShift) != wd) wstart[wd] = m;
+// The colouring's start in one launch: each world's first manifold, and the colours, body
+// claims, colour masks, per-round counts and flags cleared.
+__global__ void k_color_begin(int M, int D, int rounds, const uint64_t* __restrict__ mfKey, int worldShift,
+                              int32_t* __restrict__ wstart, int32_t* __restrict__ color,
+                              uint64_t* __restrict__ claim, uint64_t* __restrict__ claim2, uint64_t* __restrict__ used,
+                              int32_t* __restrict__ uncol, int32_t* __restrict__ overflow, int32_t* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < M) {
+        const int wd = (int)(mfKey[i] >> worldShift);
+        if (i == 0 || (int)(mfKey[i - 1] >> worldShift) != wd) wstart[wd] = i;
+        color[i] = -1;
+    }
+    if (i < D) { claim[i] = 0ull; claim2[i] = 0ull; }
+    if (i < 2 * D) used[i] = 0ull;
+    if (i < rounds) uncol[i] = 0;
+    if (i == 0) { *overflow = 0; *flag = 0; }
 }
 
 __global__ void k_color_claim(int M, int round, const uint64_t* __restrict__ mfKey, int worldShift,
@@ -1360,23 +1371,18 @@ int sv_color(World& w, int M) {
     }
     int worldShift = 2 * w.localBits;
     int nb = div_up(M, 256);
-    hipLaunchKernelGGL(k_world_mstart, dim3(nb), dim3(256), 0, s, M, w.mfKey.p, worldShift, w.worldMStart.p);
-    fill_i32(s, w.mfColor.p, -1, M);
-    fill_u64(s, w.claim.p, 0ull, w.D);
-    fill_u64(s, w.usedMask.p, 0ull, 2 * w.D);
-    fill_i32(s, w.counters.p + 5, 0, 1);
     constexpr int MAX_ROUNDS = 1024;
     w.colorUncol.alloc(MAX_ROUNDS + 1);
-    fill_i32(s, w.colorUncol.p, 0, MAX_ROUNDS + 1);
     w.claim2.alloc(std::max(w.D, 1));
-    fill_u64(s, w.claim2.p, 0ull, w.D);
+    w.mfPrev.alloc(w.manifoldCap);
+    w.colorFlag.alloc(1);
+    hipLaunchKernelGGL(k_color_begin, dim3(div_up(std::max({M, 2 * w.D, MAX_ROUNDS + 1}), 256)), dim3(256), 0, s, M,
+                       w.D, MAX_ROUNDS + 1, w.mfKey.p, worldShift, w.worldMStart.p, w.mfColor.p, w.claim.p,
+                       w.claim2.p, w.usedMask.p, w.colorUncol.p, w.counters.p + 5, w.colorFlag.p);
     unsigned long long* claims[2] = {(unsigned long long*)w.claim.p, (unsigned long long*)w.claim2.p};
     // colours of the manifolds that persist from the previous step; only new ones are
     // coloured in the rounds below
     const int prevCache = w.cacheCur;
-    w.mfPrev.alloc(w.manifoldCap);
-    w.colorFlag.alloc(1);
-    fill_i32(s, w.colorFlag.p, 0, 1);
     hipLaunchKernelGGL(k_color_inherit, dim3(nb), dim3(256), 0, s, M, w.mfKey.p, w.mfTri.p, w.cacheKey[prevCache].p,
                        w.cacheTri[prevCache].p, w.cacheNum[prevCache].p, w.cacheCount[prevCache], w.mfB0.p, w.mfB1.p,
                        w.mfColor.p,
