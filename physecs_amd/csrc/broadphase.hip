@@ -1030,16 +1030,19 @@ int bp_find_pairs(World& w) {
     if (grid) bp_grid(w);
     else tree_walk();
     int cnt[16] = {0};
-    PGX_HIP(hipMemcpyAsync(cnt, w.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    w.readHost.alloc(READ_HOST_INTS);   // pinned: the copy is a plain DMA
+    PGX_HIP(hipMemcpyAsync(w.readHost.p, w.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
     PGX_HIP(hipStreamSynchronize(s));
+    std::copy(w.readHost.p, w.readHost.p + 16, cnt);
     if (grid && (cnt[14] & 2)) throw PgxError{PGX_ERR_HIP, "broadphase grid lost a cell of its hash table"};
     if (grid && cnt[14]) {
         // a cell too full for the grid: the tree walk finds the step's pairs instead
         ++w.gridFallbacks;
         fill_i32(s, w.counters.p, 0, 1);
         tree_walk();
-        PGX_HIP(hipMemcpyAsync(cnt, w.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        PGX_HIP(hipMemcpyAsync(w.readHost.p, w.counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
         PGX_HIP(hipStreamSynchronize(s));
+        std::copy(w.readHost.p, w.readHost.p + 16, cnt);
     }
     int nPairs = cnt[0];
     if (cnt[6]) throw PgxError{PGX_ERR_INVALID, "broadphase traversal exceeded its visit bound (malformed tree)"};

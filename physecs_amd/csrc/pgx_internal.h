@@ -51,6 +51,20 @@ struct DBuf {
     ~DBuf() { release(); }
 };
 
+// Pinned host memory for read-backs (one allocation, freed with its owner).
+struct PinnedInts {
+    int32_t* p = nullptr;
+    void alloc(size_t n) {
+        if (!p) PGX_HIP(hipHostMalloc((void**)&p, sizeof(int32_t) * n, hipHostMallocDefault));
+    }
+    PinnedInts() = default;
+    PinnedInts(const PinnedInts&) = delete;
+    PinnedInts& operator=(const PinnedInts&) = delete;
+    ~PinnedInts() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 constexpr int WAVE = 64;
 // What a contact or joint solve reads and writes for one dynamic body, in one 128-byte
 // record (one cache line): solver v and w (velocityTemp), component v after the velocity
@@ -224,6 +238,8 @@ struct World {
     DBuf<int32_t> sB0, sB1, sFlags, sA0, sA1;   // sFlags: numPoints | soft<<8 | color<<16
     DBuf<int32_t> sCanon;                // canonical index
     DBuf<int32_t> batchLoc, batchList, batchN;   // runs of small colours: LDS slots of their bodies
+    DBuf<int32_t> colorPack;             // the colouring's read-back, packed by k_color_start
+    PinnedInts readHost;                 // pinned host memory for the step's read-backs (READ_HOST_INTS)
     bool batchLdsSet = false;            // k_solve_batch's LDS limit raised on this world's device
     DBuf<float4> sN;                     // normal, friction
     DBuf<float2> sSoft;                  // frequency, damping
@@ -486,6 +502,7 @@ void world_enable_triggers(World& w);                  // allocates the trigger 
 int sv_setup_manifolds(World& w, int nPairs);
 void sv_color_points(World& w, int nColors);        // contact points per colour -> colorPoints          // returns manifold count
 int sv_color(World& w, int nManifolds);                 // returns colour count
+constexpr int READ_HOST_INTS = 2048;
 void sv_presize(World& w);
 void sv_substeps(World& w, int nManifolds, float dt);
 void sv_finish_step(World& w, int nManifolds);

@@ -223,15 +223,29 @@ __global__ void k_color_keys(int M, const int32_t* __restrict__ color, const int
 
 // start[c] = the first solve position with colour >= c, for every c in [0, n): colours with
 // no manifold (inherited colours leave gaps) get empty ranges
-__global__ void k_color_start(int M, int n, const uint64_t* __restrict__ keys, int32_t* __restrict__ start) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    int lo = 0, hi = M;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (keys[mid] < ((uint64_t)c << COLOR_KEY_SHIFT)) lo = mid + 1; else hi = mid;
+// One workgroup. Also packs what the host reads back after the colouring rounds into one
+// block, so a single copy brings it: [0, n) the starts, [n] the colour-overflow flag,
+// [n+1, n+2] the last manifold's key (low, high word), [n+3, n+3+rounds] the count of
+// uncoloured manifolds after each round.
+__global__ void k_color_start(int M, int n, const uint64_t* __restrict__ keys, int32_t* __restrict__ start,
+                              int rounds, const int32_t* __restrict__ uncol, const int32_t* __restrict__ overflow,
+                              int32_t* __restrict__ pack) {
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        int lo = 0, hi = M;
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (keys[mid] < ((uint64_t)c << COLOR_KEY_SHIFT)) lo = mid + 1; else hi = mid;
+        }
+        start[c] = lo;
+        pack[c] = lo;
     }
-    start[c] = lo;
+    for (int r = threadIdx.x; r <= rounds; r += blockDim.x) pack[n + 3 + r] = uncol[r];
+    if (threadIdx.x == 0) {
+        pack[n] = *overflow;
+        const uint64_t last = M > 0 ? keys[M - 1] : 0ull;
+        pack[n + 1] = (int32_t)(uint32_t)last;
+        pack[n + 2] = (int32_t)(uint32_t)(last >> 32);
+    }
 }
 
 // ------------------------------------------------------------------ setup (solve order)
@@ -1326,23 +1340,28 @@ int sv_setup_manifolds(World& w, int nPairs) {
     exclusive_scan_i32(s, w.pvals.p, w.npIndex.p, nPairs, w.counters.p + 1, w.scanT);
     int M = 0, epaErrors = 0, flowErr = 0;
     int meshInfo[3] = {0, 0, 0};   // mesh pairs, mesh manifold slots, mesh capacity errors
-    PGX_HIP(hipMemcpyAsync(&M, w.counters.p + 1, sizeof(int), hipMemcpyDeviceToHost, s));
-    // the previous step's dependency-driven passes: a wait that ran out, and every queue
-    // served by waves of its XCD
+    // the counters (manifold total, EPA and mesh counts) and the previous step's
+    // dependency-driven passes: a wait that ran out, and every queue served by waves of its
+    // XCD; to pinned host memory
     int qDone[FLOW_QUEUES] = {}, qStart[FLOW_QUEUES + 1] = {};
     const int lastQueues = w.lastQueues, lastPasses = w.lastFlowPasses;
-    PGX_HIP(hipMemcpyAsync(&flowErr, w.flowErr.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    w.readHost.alloc(READ_HOST_INTS);
+    int32_t* rb = w.readHost.p;
+    PGX_HIP(hipMemcpyAsync(rb, w.counters.p, 16 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    PGX_HIP(hipMemcpyAsync(rb + 16, w.flowErr.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     if (lastQueues) {
         PGX_HIP(hipMemcpyAsync(qDone, w.qDone.p, sizeof(qDone), hipMemcpyDeviceToHost, s));
         PGX_HIP(hipMemcpyAsync(qStart, w.qStart.p, sizeof(int) * (lastQueues + 1), hipMemcpyDeviceToHost, s));
     }
-    if (w.nConvex) PGX_HIP(hipMemcpyAsync(&epaErrors, w.counters.p + 9, sizeof(int), hipMemcpyDeviceToHost, s));
-    if (w.nTriMesh) PGX_HIP(hipMemcpyAsync(meshInfo, w.counters.p + 10, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
     hipLaunchKernelGGL(k_compact, dim3(div_up(nPairs, 256)), dim3(256), 0, s, nPairs, w.npNum.p, w.npIndex.p,
                        w.meshFirst.p, w.meshCount.p, w.npTri.p, w.pairCap, w.pkeys.p, w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.cflags.p,
                        w.mfPair.p, w.mfKey.p, w.mfB0.p, w.mfB1.p, w.mfSrc.p, w.mfTri.p);
     PGX_LAUNCH_CHECK();
     PGX_HIP(hipStreamSynchronize(s));
+    M = rb[1];
+    flowErr = rb[16];
+    if (w.nConvex) epaErrors = rb[9];
+    if (w.nTriMesh) std::copy(rb + 10, rb + 13, meshInfo);
     if (flowErr)
         throw PgxError{PGX_ERR_HIP, "contact solve: a wait of the previous step ran out (dependency-driven pass or "
                                     "grid barrier of the one-launch pass)"};
@@ -1372,7 +1391,10 @@ int sv_color(World& w, int M) {
     int worldShift = 2 * w.localBits;
     int nb = div_up(M, 256);
     constexpr int MAX_ROUNDS = 1024;
+    static_assert(133 + MAX_ROUNDS + 1 <= READ_HOST_INTS, "the colouring's read-back fits the pinned buffer");
     w.colorUncol.alloc(MAX_ROUNDS + 1);
+    w.colorPack.alloc(133 + MAX_ROUNDS + 1);
+    w.readHost.alloc(READ_HOST_INTS);
     w.claim2.alloc(std::max(w.D, 1));
     w.mfPrev.alloc(w.manifoldCap);
     w.colorFlag.alloc(1);
@@ -1412,15 +1434,19 @@ int sv_color(World& w, int M) {
         hipLaunchKernelGGL(k_color_keys, dim3(nb), dim3(256), 0, s, M, w.mfColor.p, w.npNum.p, w.mfSrc.p,
                            w.execMode == EXEC_WORLDS ? 0 : 1, w.colorKeys.p, w.colorPerm.p);
         radix_sort_u64(s, w.colorKeys.p, w.colorPerm.p, M, 7 + COLOR_KEY_SHIFT, w.sortT);
-        hipLaunchKernelGGL(k_color_start, dim3(1), dim3(256), 0, s, M, 130, w.colorKeys.p, w.colorCount.p);
+        hipLaunchKernelGGL(k_color_start, dim3(1), dim3(256), 0, s, M, 130, w.colorKeys.p, w.colorCount.p, round,
+                           w.colorUncol.p, w.counters.p + 5, w.colorPack.p);
         PGX_LAUNCH_CHECK();
-        int ovf = 0;
-        uc.assign(round + 1, 0);
-        PGX_HIP(hipMemcpyAsync(uc.data(), w.colorUncol.p, sizeof(int) * (round + 1), hipMemcpyDeviceToHost, s));
-        PGX_HIP(hipMemcpyAsync(&ovf, w.counters.p + 5, sizeof(int), hipMemcpyDeviceToHost, s));
-        PGX_HIP(hipMemcpyAsync(&lastKey, w.colorKeys.p + (M - 1), sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-        PGX_HIP(hipMemcpyAsync(starts.data(), w.colorCount.p, sizeof(int32_t) * 130, hipMemcpyDeviceToHost, s));
+        // one copy to pinned host memory brings the starts, the overflow flag, the last key
+        // and the per-round counts
+        const size_t packed = 133 + (size_t)round + 1;
+        PGX_HIP(hipMemcpyAsync(w.readHost.p, w.colorPack.p, sizeof(int32_t) * packed, hipMemcpyDeviceToHost, s));
         PGX_HIP(hipStreamSynchronize(s));
+        const int32_t* rb = w.readHost.p;
+        const int ovf = rb[130];
+        lastKey = (uint64_t)(uint32_t)rb[131] | ((uint64_t)(uint32_t)rb[132] << 32);
+        std::copy(rb, rb + 130, starts.begin());
+        uc.assign(rb + 133, rb + 133 + round + 1);
         if (ovf) throw PgxError{PGX_ERR_CAPACITY, "a body touches more than 127 contact colours"};
         if (uc[round] == 0) break;
         if (round >= MAX_ROUNDS) throw PgxError{PGX_ERR_INVALID, "graph colouring did not converge"};
