@@ -143,9 +143,11 @@ __device__ __forceinline__ float ord_dec(uint32_t u) {
     return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
-__global__ void k_scene_bounds_init(uint32_t* wb, double* mom) {
+// (and, when given, the step's 16 broadphase counters cleared)
+__global__ void k_scene_bounds_init(uint32_t* wb, double* mom, int32_t* counters) {
     if (threadIdx.x < 3) { wb[threadIdx.x] = 0xffffffffu; wb[3 + threadIdx.x] = 0u; wb[6 + threadIdx.x] = 0u; }
     if (threadIdx.x < 6) mom[threadIdx.x] = 0.0;
+    if (counters && threadIdx.x < 16) counters[threadIdx.x] = 0;
 }
 
 // The dynamic colliders' centres: their box (min, max per axis) and their sums and sums of
@@ -311,8 +313,9 @@ __device__ __forceinline__ void grid_frame(const uint32_t* __restrict__ wb, cons
 __global__ void k_cell_codes(int n, const int32_t* __restrict__ dynCols, const int32_t* __restrict__ cworld,
                              const float4* __restrict__ bmin, const float4* __restrict__ bmax,
                              const uint32_t* __restrict__ wb, const double* __restrict__ mom, uint64_t* __restrict__ keys,
-                             int32_t* __restrict__ vals) {
+                             int32_t* __restrict__ vals, int tableSize, uint64_t* __restrict__ tabKey) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < tableSize) tabKey[i] = GRID_EMPTY;   // the hash table starts empty (k_cell_insert fills it)
     if (i >= n) return;
     float lo[3], h[3];
     grid_frame(wb, mom, n, lo, h);
@@ -928,7 +931,8 @@ void bp_build(World& w) {
     int worldShift = 30;
     if (n > 0) {
         // quantisation box -> Morton codes of the dynamic colliders -> sort
-        hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p, w.sceneMoments.p);
+        hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p, w.sceneMoments.p,
+                           nullptr);
         hipLaunchKernelGGL(k_scene_bounds, dim3(std::min(64, div_up(n, 1024))), dim3(1024), 0, s, n, w.dynCols.p,
                            w.cbmin.p, w.cbmax.p, w.worldBoundsI.p, w.sceneMoments.p);
         hipLaunchKernelGGL(k_morton, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.dynCols.p, w.cworld.p, w.cbmin.p,
@@ -983,18 +987,19 @@ static void bp_grid(World& w) {
     hipStream_t s = w.stream;
     const int n = w.nDynCol;
     const uint32_t T = grid_table_size(n);
-    hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p, w.sceneMoments.p);
+    hipLaunchKernelGGL(k_scene_bounds_init, dim3(1), dim3(64), 0, s, w.worldBoundsI.p, w.sceneMoments.p,
+                       w.counters.p);
     hipLaunchKernelGGL(k_scene_bounds, dim3(std::min(64, div_up(n, 1024))), dim3(1024), 0, s, n, w.dynCols.p,
                        w.cbmin.p, w.cbmax.p, w.worldBoundsI.p, w.sceneMoments.p);
-    hipLaunchKernelGGL(k_cell_codes, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.dynCols.p, w.cworld.p, w.cbmin.p,
-                       w.cbmax.p, w.worldBoundsI.p, w.sceneMoments.p, w.mkeys.p, w.mvals.p);
+    hipLaunchKernelGGL(k_cell_codes, dim3(div_up(std::max<int>(n, (int)T), 256)), dim3(256), 0, s, n, w.dynCols.p,
+                       w.cworld.p, w.cbmin.p, w.cbmax.p, w.worldBoundsI.p, w.sceneMoments.p, w.mkeys.p, w.mvals.p,
+                       (int)T, w.gridKey.p);
     PGX_LAUNCH_CHECK();
     radix_sort_u64(s, w.mkeys.p, w.mvals.p, n, 30 + w.worldBits, w.sortT);
     const int nStatic = w.C - w.nDynCol;
     hipLaunchKernelGGL(k_grid_pack, dim3(div_up(std::max(n, nStatic), 256)), dim3(256), 0, s, n, w.mvals.p,
                        w.cbmin.p, w.cbmax.p, w.cflags.p, w.centity.p, w.clocal.p, w.gridRec.p, nStatic, w.staticCols.p,
                        w.statA.p, w.statB.p, w.statL.p);
-    fill_u64(s, w.gridKey.p, GRID_EMPTY, T);
     hipLaunchKernelGGL(k_cell_insert, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, T - 1, w.gridKey.p,
                        w.gridStart.p);
     hipLaunchKernelGGL(k_cell_ends, dim3(div_up(n, 256)), dim3(256), 0, s, n, w.mkeys.p, T - 1, w.gridKey.p,
@@ -1016,7 +1021,8 @@ int bp_find_pairs(World& w) {
     // [10] mesh pairs, [11] mesh manifold slots, [12] mesh capacity errors, [13] mesh
     // candidate triangles, [14] the grid handed the step to the tree walk (1) or lost a cell
     // (2, never expected), [15] the grid's fullest cell
-    fill_i32(s, w.counters.p, 0, 16);
+    const bool grid = n > 0 && w.tune.pairSearch == 0;
+    if (!grid) fill_i32(s, w.counters.p, 0, 16);   // the grid's first kernel clears them
     auto tree_walk = [&]() {
         bp_build(w);
         if (n > 0) {
@@ -1026,7 +1032,6 @@ int bp_find_pairs(World& w) {
             PGX_LAUNCH_CHECK();
         }
     };
-    const bool grid = n > 0 && w.tune.pairSearch == 0;
     if (grid) bp_grid(w);
     else tree_walk();
     int cnt[16] = {0};
