@@ -454,7 +454,10 @@ __device__ void row_solve_scalar(const JointArgs& a, int j, int r, int flags, bo
 
 // one joint of a SIMD colour, its rows in flag-type order. The joint's bodies belong to no
 // other joint of the colour, so their velocities (or pseudo velocities) stay in registers
-// across the joint's rows: one load and one store per body.
+// across the joint's rows: one load and one store per body. UPFRONT loads every row of the
+// joint at once (the loads overlap; about 200 registers, for a kernel bounded to few
+// threads); otherwise each row is loaded while the one before it is solved.
+template <bool UPFRONT>
 __device__ void joint_color_one(const JointArgs& a, int j, int phase, int warm) {
     int n = a.jNRows[j];
     int b0 = a.jB0[j], b1 = a.jB1[j];
@@ -467,18 +470,36 @@ __device__ void joint_color_one(const JointArgs& a, int j, int phase, int warm) 
         for (int r = 0; r < n; ++r)
             if (((rowFlags >> (4 * r)) & 15) == kFlagOrder[f]) seq |= (uint32_t)r << (4 * cnt++);
     float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-    JRow cur, nxt;
-    if (cnt) cur = load_jrow(a, j, seq & 15);
+    // UPFRONT: rows[i] holds row i of the solve order; otherwise rows[0] the row being solved
+    // and rows[1] the next one, in flight meanwhile
+    constexpr int NR = UPFRONT ? JOINT_MAX_ROWS : 2;
+    JRow rows[NR];
+    if constexpr (UPFRONT) {
+#pragma unroll
+        for (int i = 0; i < NR; ++i)
+            if (i < cnt) rows[i] = load_jrow(a, j, (seq >> (4 * i)) & 15);
+    } else {
+        if (cnt) rows[0] = load_jrow(a, j, seq & 15);
+    }
     if (phase == 0) {
         const JBodies B = joint_bodies(a, j);
         float4 pv0 = z, pw0 = z, pv1 = z, pw1 = z;
         if (b0 >= 0) { pv0 = a.pvel[b0]; pw0 = a.pang[b0]; }
         if (b1 >= 0) { pv1 = a.pvel[b1]; pw1 = a.pang[b1]; }
-        for (int i = 0; i < cnt; ++i) {
-            const int r = (seq >> (4 * i)) & 15;
-            if (i + 1 < cnt) nxt = load_jrow(a, j, (seq >> (4 * (i + 1))) & 15);
-            row_presolve_d(a, j, r, (rowFlags >> (4 * r)) & 15, true, cur, B, pv0, pw0, pv1, pw1);
-            cur = nxt;
+        if constexpr (UPFRONT) {
+#pragma unroll
+            for (int i = 0; i < NR; ++i) {
+                if (i >= cnt) break;
+                const int r = (seq >> (4 * i)) & 15;
+                row_presolve_d(a, j, r, (rowFlags >> (4 * r)) & 15, true, rows[i], B, pv0, pw0, pv1, pw1);
+            }
+        } else {
+            for (int i = 0; i < cnt; ++i) {
+                const int r = (seq >> (4 * i)) & 15;
+                if (i + 1 < cnt) rows[1] = load_jrow(a, j, (seq >> (4 * (i + 1))) & 15);
+                row_presolve_d(a, j, r, (rowFlags >> (4 * r)) & 15, true, rows[0], B, pv0, pw0, pv1, pw1);
+                rows[0] = rows[1];
+            }
         }
         if (b0 >= 0) { a.pvel[b0] = pv0; a.pang[b0] = pw0; }
         if (b1 >= 0) { a.pvel[b1] = pv1; a.pang[b1] = pw1; }
@@ -488,11 +509,20 @@ __device__ void joint_color_one(const JointArgs& a, int j, int phase, int warm) 
     if (b0 >= 0) { tv0 = a.bvel[bv(b0, BV_V)]; tw0 = a.bvel[bv(b0, BV_W)]; }
     if (b1 >= 0) { tv1 = a.bvel[bv(b1, BV_V)]; tw1 = a.bvel[bv(b1, BV_W)]; }
     f3 v0 = xyz(tv0), w0 = xyz(tw0), v1 = xyz(tv1), w1 = xyz(tw1);
-    for (int i = 0; i < cnt; ++i) {
-        const int r = (seq >> (4 * i)) & 15;
-        if (i + 1 < cnt) nxt = load_jrow(a, j, (seq >> (4 * (i + 1))) & 15);
-        row_solve_simd(a, j, r, (rowFlags >> (4 * r)) & 15, warm != 0, cur, b0, b1, v0, w0, v1, w1);
-        cur = nxt;
+    if constexpr (UPFRONT) {
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            if (i >= cnt) break;
+            const int r = (seq >> (4 * i)) & 15;
+            row_solve_simd(a, j, r, (rowFlags >> (4 * r)) & 15, warm != 0, rows[i], b0, b1, v0, w0, v1, w1);
+        }
+    } else {
+        for (int i = 0; i < cnt; ++i) {
+            const int r = (seq >> (4 * i)) & 15;
+            if (i + 1 < cnt) rows[1] = load_jrow(a, j, (seq >> (4 * (i + 1))) & 15);
+            row_solve_simd(a, j, r, (rowFlags >> (4 * r)) & 15, warm != 0, rows[0], b0, b1, v0, w0, v1, w1);
+            rows[0] = rows[1];
+        }
     }
     if (b0 >= 0) { set_xyz(tv0, v0); set_xyz(tw0, w0); a.bvel[bv(b0, BV_V)] = tv0; a.bvel[bv(b0, BV_W)] = tw0; }
     if (b1 >= 0) { set_xyz(tv1, v1); set_xyz(tw1, w1); a.bvel[bv(b1, BV_V)] = tv1; a.bvel[bv(b1, BV_W)] = tw1; }
@@ -501,7 +531,7 @@ __device__ void joint_color_one(const JointArgs& a, int j, int phase, int warm) 
 // one SIMD colour: a thread per joint
 __global__ void k_joint_color(JointArgs a, int begin, int end, int phase, int warm) {
     int k = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < end) joint_color_one(a, a.order[k], phase, warm);
+    if (k < end) joint_color_one<false>(a, a.order[k], phase, warm);
 }
 
 // Every SIMD colour (0-7) of a joint pass in one launch: joints that share a dynamic body,
@@ -518,7 +548,8 @@ __global__ __launch_bounds__(ISLAND_THREADS) void k_joint_islands(JointArgs a, c
     for (int c = 0; c < 8; ++c) {
         const int b = cs[c], e = cs[c + 1];
         if (b == e) continue;   // the same for the whole workgroup
-        for (int k = b + (int)threadIdx.x; k < e; k += ISLAND_THREADS) joint_color_one(a, islandJoint[k], phase, warm);
+        for (int k = b + (int)threadIdx.x; k < e; k += ISLAND_THREADS)
+            joint_color_one<true>(a, islandJoint[k], phase, warm);
         __syncthreads();        // this colour's velocity stores are seen by the next colour's loads
     }
 }
