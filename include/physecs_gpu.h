@@ -23,6 +23,9 @@
  *   pgx_world_add_joints    Scene::createJoint<T> for the built-in joint types
  *                           (include/Physecs/Physecs.h:209-214, src/Physecs.cpp:690-710)
  *   pgx_world_set_joint_params  the joints' setters (include/Physecs/Joints/<name>Joint.h)
+ *   pgx_world_set_joint_callbacks  user Joint subclasses (include/Physecs/Joint.h:42-70):
+ *                           getSolverDesc per step, MakeConstraintsFunc per substep
+ *                           (src/Physecs.cpp:321-355,430-439)
  *   pgx_world_destroy_joint Scene::destroyJoint (include/Physecs/Physecs.h:215,
  *                           src/Physecs.cpp:712-725)
  *   pgx_world_read_state / pgx_world_write_state
@@ -142,6 +145,46 @@ typedef struct pgx_scene_desc {
 #define PGX_JOINT_PRISMATIC 4
 #define PGX_JOINT_GEAR 5
 #define PGX_JOINT_SERVO 6
+/* A user Joint subclass (include/Physecs/Joint.h:42-70): its rows come from the world's
+ * joint callbacks (pgx_world_set_joint_callbacks); only type, actors and anchors are read. */
+#define PGX_JOINT_CUSTOM 7
+
+/* Constraint1D flags (include/Physecs/Constraint1D.h:11-16). A row has one of NONE,
+ * ANGULAR, SOFT, LIMITED, ANGULAR | SOFT, ANGULAR | LIMITED: the combinations the
+ * reference's constraint containers hold (include/Physecs/Constraint1DContainer.h:117-118). */
+#define PGX_ROW_NONE 0
+#define PGX_ROW_SOFT 1
+#define PGX_ROW_ANGULAR 2
+#define PGX_ROW_LIMITED 4
+#define PGX_JOINT_MAX_ROWS 7    /* rows of one joint on the device */
+
+/* JointWorldSpaceData (include/Physecs/Joint.h:16-23), the same memory layout: the anchors
+ * in world space (p0, p1), the anchors relative to the bodies' centres of mass rotated into
+ * world space (r0, r1), and the anchor frames (u0, u1; 3x3, column-major as glm::mat3),
+ * computed as JointSolverData::calculateWorldSpaceData does (src/Joint.cpp:5-14). */
+typedef struct pgx_joint_world_space {
+    float p0[3], p1[3], r0[3], r1[3];
+    float u0[9], u1[9];
+} pgx_joint_world_space;
+
+/* The fields of one Constraint1D that MakeConstraintsFunc writes through Constraint1DView
+ * (include/Physecs/Constraint1DContainer.h:22-91). */
+typedef struct pgx_joint_row {
+    float linear[3], angular0[3], angular1[3];
+    float target_velocity, c, min, max, frequency, damping_ratio;
+} pgx_joint_row;
+
+/* Joint::getSolverDesc, for joint `joint` (creation order) at the start of each step
+ * (src/Physecs.cpp:321-355): writes its rows' flags (PGX_ROW_*, in creation order) into
+ * row_flags and returns their number, 0 to PGX_JOINT_MAX_ROWS; a negative return fails
+ * the step. `ws` holds the joint's world-space data at the step's starting poses. */
+typedef int (*pgx_joint_layout_fn)(void* user, int joint, const pgx_joint_world_space* ws, int32_t* row_flags);
+/* MakeConstraintsFunc, for joint `joint` at every substep (src/Physecs.cpp:430-439): `rows`
+ * holds the joint's n_rows rows in creation order, as the previous call of this step left
+ * them (Constraint1D's defaults at the step's first substep: zeros, min = -FLT_MAX,
+ * max = FLT_MAX; include/Physecs/Constraint1D.h:19-35); the function sets what it writes. */
+typedef void (*pgx_joint_rows_fn)(void* user, int joint, const pgx_joint_world_space* ws, pgx_joint_row* rows,
+                                  int n_rows);
 
 /* One Scene::createJoint<T>(entity0, anchor0Pos, anchor0Or, entity1, anchor1Pos, anchor1Or).
  * Anchors are in body space: position xyz, orientation xyzw. The other fields are the
@@ -220,6 +263,16 @@ int pgx_world_set_joint_params(pgx_world* w, int index, const pgx_joint_desc* jo
  * entities and the entity pair collides again (even where another joint still joins
  * them, as the reference does). Other joints keep their indices. */
 int pgx_world_destroy_joint(pgx_world* w, int index);
+/* The host functions of the world's PGX_JOINT_CUSTOM joints (user Joint subclasses,
+ * include/Physecs/Joint.h:42-70; the calls of src/Physecs.cpp:321-355,430-439). Each step
+ * calls `layout` for every live custom joint, joint colour by colour and in creation order
+ * inside a colour, as the reference walks its joint colours; each substep calls `rows` in the
+ * same order. The device waits for these calls: the step stops at its start and at each
+ * substep to hand the joints' world-space data to the host and take their rows back, so a
+ * world with custom joints does not run its substep loop as a HIP graph. A step of a world
+ * with custom joints and no callbacks fails with PGX_ERR_INVALID. NULL functions remove the
+ * callbacks. */
+int pgx_world_set_joint_callbacks(pgx_world* w, pgx_joint_layout_fn layout, pgx_joint_rows_fn rows, void* user);
 
 /* One Scene::simulate(time_step) on the device. Asynchronous; pgx_world_sync waits. */
 int pgx_world_step(pgx_world* w, float time_step);

@@ -79,7 +79,15 @@ struct KeyHash {
 struct CacheData { int numPoints; V3 lp[4]; float tv[4]; };
 
 // ---- joints (include/Physecs/Joint.h, src/Joints/*.cpp, include/Physecs/Constraint1D*.h)
-enum JointType { J_SPHERICAL = 0, J_UNIVERSAL = 1, J_REVOLUTE = 2, J_FIXED = 3, J_PRISMATIC = 4, J_GEAR = 5, J_SERVO = 6 };
+enum JointType { J_SPHERICAL = 0, J_UNIVERSAL = 1, J_REVOLUTE = 2, J_FIXED = 3, J_PRISMATIC = 4, J_GEAR = 5, J_SERVO = 6,
+                 J_CUSTOM = 7 };
+// A user Joint subclass (include/Physecs/Joint.h:42-70) through two host functions:
+// getSolverDesc's layout (row flags in creation order, returns the row count) and
+// MakeConstraintsFunc (writes the rows' fields). World-space data: p0, p1, r0, r1, u0, u1
+// (30 floats, matrices column-major); a row: linear, angular0, angular1, targetVelocity,
+// c, min, max, frequency, dampingRatio (15 floats).
+typedef int (*CustomLayoutFn)(void* user, int joint, const float* ws, int32_t* flags);
+typedef void (*CustomRowsFn)(void* user, int joint, const float* ws, float* rows, int n);
 enum RowFlags { F_NONE = 0, F_SOFT = 1, F_ANGULAR = 2, F_LIMITED = 4 };
 // flag-type order of Constraint1DContainer's collections (Constraint1DContainer.h:117)
 static const int kFlagOrder[6] = {F_NONE, F_ANGULAR, F_SOFT, F_LIMITED, F_ANGULAR | F_SOFT, F_ANGULAR | F_LIMITED};
@@ -208,6 +216,10 @@ struct orc_world {
     std::unordered_map<uint32_t, unsigned char> jointBitsets;   // per entity, src/Physecs.cpp:696-705
     std::vector<JointRows> jrows[9];                           // per colour, joints in creation order
     int numSubSteps = 8, numIterations = 2, numThreads = 0;
+    CustomLayoutFn custLayout = nullptr;
+    CustomRowsFn custRows = nullptr;
+    void* custUser = nullptr;
+    int jointError = 0;                 // a custom layout that was refused (0: none)
     std::unique_ptr<Pool> pool;   // numThreads workers, made when the thread count is set
     float g = 9.81f;
     std::mutex mtx;
@@ -600,13 +612,42 @@ static int rowLayout(const orc_world* w, Joint& j, int* flags) {
     return n;
 }
 
+// JointSolverData::calculateWorldSpaceData (src/Joint.cpp:5-14) as 30 floats: p0, p1, r0,
+// r1, then u0 and u1 by columns
+static void jointWorldSpace(const orc_world* w, const Joint& j, const JointRows& jr, float* o) {
+    const Actor& t0 = w->actors[j.a0];
+    const Actor& t1 = w->actors[j.a1];
+    M3 u0 = mat3_cast(qmul(t0.quat, j.q0));
+    M3 u1 = mat3_cast(qmul(t1.quat, j.q1));
+    V3 v[10] = {t0.pos + qrot(t0.quat, j.p0), t1.pos + qrot(t1.quat, j.p1), qrot(t0.quat, jr.r0), qrot(t1.quat, jr.r1),
+                u0[0], u0[1], u0[2], u1[0], u1[1], u1[2]};
+    for (int i = 0; i < 10; ++i) { o[3 * i] = v[i].x; o[3 * i + 1] = v[i].y; o[3 * i + 2] = v[i].z; }
+}
+
+// getSolverDesc of a custom joint: its rows' flags from the layout function
+static int customLayout(orc_world* w, int index, const JointRows& jr, int* flags) {
+    float ws[30];
+    jointWorldSpace(w, w->joints[index], jr, ws);
+    int32_t fl[7] = {0};
+    const int n = w->custLayout ? w->custLayout(w->custUser, index, ws, fl) : -1;
+    bool ok = n >= 0 && n <= 7;
+    for (int r = 0; ok && r < n; ++r) ok = fl[r] == 0 || fl[r] == 1 || fl[r] == 2 || fl[r] == 4 || fl[r] == 3 || fl[r] == 6;
+    if (!ok) { w->jointError = 1; return 0; }
+    for (int r = 0; r < n; ++r) flags[r] = fl[r];
+    return n;
+}
+
 // "create joint constraints", src/Physecs.cpp:321-355: rows are rebuilt every step (defaults
-// from include/Physecs/Constraint1D.h:17-31, so the accumulated lambda restarts at 0).
+// from include/Physecs/Constraint1D.h:19-35, so the accumulated lambda restarts at 0). The
+// joints are visited colour by colour, creation order inside a colour, as the reference
+// walks jointGraph.colors (destroyJoint erases in place, Physecs.cpp:712-717): the order in
+// which a custom joint's layout function is called.
 static void buildJointRows(orc_world* w) {
     for (auto& c : w->jrows) c.clear();
+    for (int col = 0; col < 9; ++col)
     for (size_t i = 0; i < w->joints.size(); ++i) {
         Joint& j = w->joints[i];
-        if (!j.alive) continue;
+        if (!j.alive || j.color != col) continue;
         JointRows jr{};
         jr.joint = (int)i;
         const Actor& t0 = w->actors[j.a0];
@@ -618,7 +659,7 @@ static void buildJointRows(orc_world* w) {
         jr.r0 = b0 >= 0 && g_mutation != MUT_JOINT_NO_COM ? j.p0 - d0->com : j.p0;
         jr.r1 = b1 >= 0 && g_mutation != MUT_JOINT_NO_COM ? j.p1 - d1->com : j.p1;
         int fl[7];
-        jr.nRows = rowLayout(w, j, fl);
+        jr.nRows = j.type == J_CUSTOM ? customLayout(w, (int)i, jr, fl) : rowLayout(w, j, fl);
         for (int r = 0; r < jr.nRows; ++r) {
             Row& row = jr.rows[r];
             std::memset(&row, 0, sizeof(row));
@@ -644,6 +685,25 @@ static M3 mcols(V3 a, V3 b, V3 c) { M3 m; m[0] = a; m[1] = b; m[2] = c; return m
 // (src/JointsUtil.h:8-28, src/Joints/*.cpp)
 static void updateJointRows(orc_world* w, JointRows& jr) {
     Joint& j = w->joints[jr.joint];
+    if (j.type == J_CUSTOM) {   // MakeConstraintsFunc through the rows function
+        float ws[30], q[7 * 15];
+        jointWorldSpace(w, j, jr, ws);
+        for (int r = 0; r < jr.nRows; ++r) {
+            const Row& R = jr.rows[r];
+            float* o = q + 15 * r;
+            const float f[15] = {R.linear.x, R.linear.y, R.linear.z, R.angular0.x, R.angular0.y, R.angular0.z,
+                                 R.angular1.x, R.angular1.y, R.angular1.z, R.tv, R.c, R.mn, R.mx, R.freq, R.damp};
+            std::memcpy(o, f, sizeof(f));
+        }
+        if (w->custRows) w->custRows(w->custUser, jr.joint, ws, q, jr.nRows);
+        for (int r = 0; r < jr.nRows; ++r) {
+            Row& R = jr.rows[r];
+            const float* o = q + 15 * r;
+            R.linear = v3(o[0], o[1], o[2]); R.angular0 = v3(o[3], o[4], o[5]); R.angular1 = v3(o[6], o[7], o[8]);
+            R.tv = o[9]; R.c = o[10]; R.mn = o[11]; R.mx = o[12]; R.freq = o[13]; R.damp = o[14];
+        }
+        return;
+    }
     const Actor& t0 = w->actors[j.a0];
     const Actor& t1 = w->actors[j.a1];
     M3 u0 = mat3_cast(qmul(t0.quat, j.q0));
@@ -1408,6 +1468,19 @@ int orc_world_destroy_joint(orc_world* w, int i) {
 }
 
 int orc_world_joint_color(const orc_world* w, int i) { return w->joints[i].color; }
+
+// the host functions of the world's custom joints (J_CUSTOM)
+void orc_world_set_joint_callbacks(orc_world* w, CustomLayoutFn layout, CustomRowsFn rows, void* user) {
+    w->custLayout = layout;
+    w->custRows = rows;
+    w->custUser = user;
+}
+// 1 when a custom joint's layout was refused since the last call (then it had no rows)
+int orc_world_joint_error(orc_world* w) {
+    const int e = w->jointError;
+    w->jointError = 0;
+    return e;
+}
 
 // Plants one of the Mutation mistakes in every later call (0 restores the restatement).
 void orc_set_mutation(int m) { g_mutation = m; }

@@ -35,7 +35,7 @@ EXPORTS = [
     "pgx_world_autotune", "pgx_world_get_exec", "pgx_world_raycast", "pgx_world_overlap",
     "pgx_world_get_trigger_events", "pgx_world_overlap_mtd", "pgx_world_character_move", "pgx_mass_properties",
     "pgx_world_patch_transforms", "pgx_convex_hull", "pgx_trimesh_cache", "pgx_trimesh_cache_validate", "pgx_trimesh_cache_pack", "pgx_world_state_size", "pgx_world_save_state", "pgx_world_load_state",
-    "pgx_world_load_state_matched",
+    "pgx_world_load_state_matched", "pgx_world_set_joint_callbacks",
     "pgx_server_create", "pgx_server_destroy", "pgx_server_submit", "pgx_server_step", "pgx_server_done",
     "pgx_server_take_result", "pgx_server_footprint", "pgx_server_set_graph",
     "pgx_server_set_tuning",
@@ -68,6 +68,47 @@ class PgxJointDesc(ctypes.Structure):
                 ("drive_velocity", ctypes.c_float), ("drive_max_torque", ctypes.c_float),
                 ("lower_limit", ctypes.c_float), ("upper_limit", ctypes.c_float), ("target", ctypes.c_float),
                 ("drive_stiffness", ctypes.c_float), ("drive_damping", ctypes.c_float), ("gear_ratio", ctypes.c_float)]
+
+
+# User joint types (PGX_JOINT_CUSTOM): getSolverDesc's layout and MakeConstraintsFunc as host
+# functions. The world-space data is 30 floats (p0, p1, r0, r1, then u0 and u1 by columns:
+# pgx_joint_world_space); a row is 15 floats (linear, angular0, angular1, target velocity, c,
+# min, max, frequency, damping ratio: pgx_joint_row). The oracle takes the same functions.
+JOINT_LAYOUT_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, F, I32)
+JOINT_ROWS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, F, F, ctypes.c_int)
+JOINT_MAX_ROWS = 7
+JOINT_WS_FLOATS, JOINT_ROW_FLOATS = 30, 15
+
+
+def joint_callbacks(layout, rows):
+    """C function pointers for a custom joint's Python functions:
+    layout(joint, ws) -> row flags in creation order (PGX_ROW_* values), called once per step;
+    rows(joint, ws, rows) writes into rows, an (n, 15) float32 array holding the joint's rows as
+    the previous call of the step left them; called every substep. ws is a (30,) float32 array.
+    An exception in either function fails the step (it is printed)."""
+    import traceback
+
+    def lay(user, joint, ws, flags):
+        try:
+            fl = [int(f) for f in layout(joint, np.ctypeslib.as_array(ws, shape=(JOINT_WS_FLOATS,)))]
+            if len(fl) > JOINT_MAX_ROWS:
+                return len(fl)   # refused by the caller, nothing written
+            for i, f in enumerate(fl):
+                flags[i] = f
+            return len(fl)
+        except Exception:
+            traceback.print_exc()
+            return -1
+
+    def wr(user, joint, ws, out, n):
+        try:
+            if n > 0:
+                rows(joint, np.ctypeslib.as_array(ws, shape=(JOINT_WS_FLOATS,)),
+                     np.ctypeslib.as_array(out, shape=(n, JOINT_ROW_FLOATS)))
+        except Exception:
+            traceback.print_exc()
+
+    return JOINT_LAYOUT_FN(lay), JOINT_ROWS_FN(wr)
 
 
 class PgxCounts(ctypes.Structure):
@@ -162,6 +203,7 @@ def lib():
         L.pgx_world_add_joints.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PgxJointDesc)]
         L.pgx_world_set_joint_params.argtypes = [vp, ctypes.c_int, ctypes.POINTER(PgxJointDesc)]
         L.pgx_world_destroy_joint.argtypes = [vp, ctypes.c_int]
+        L.pgx_world_set_joint_callbacks.argtypes = [vp, JOINT_LAYOUT_FN, JOINT_ROWS_FN, vp]
         L.pgx_world_set_graph.argtypes = [vp, ctypes.c_int]
         L.pgx_server_set_graph.argtypes = [vp, ctypes.c_int]
         L.pgx_world_set_tuning.argtypes = [vp, ctypes.POINTER(PgxTuning)]
@@ -388,6 +430,16 @@ class World:
     def destroy_joint(self, index):
         """Scene::destroyJoint for joint `index` (creation order)."""
         check(self.L.pgx_world_destroy_joint(self.h, index))
+
+    def set_joint_callbacks(self, layout, rows):
+        """The host functions of the world's custom joints (joint_callbacks); None, None
+        removes them."""
+        if layout is None:
+            self._joint_fns = None
+            check(self.L.pgx_world_set_joint_callbacks(self.h, JOINT_LAYOUT_FN(), JOINT_ROWS_FN(), None))
+            return
+        self._joint_fns = joint_callbacks(layout, rows)   # kept alive with the world
+        check(self.L.pgx_world_set_joint_callbacks(self.h, self._joint_fns[0], self._joint_fns[1], None))
 
     def counts(self):
         c = PgxCounts()

@@ -13,12 +13,14 @@
 // flag-type order; that is exactly what the reference's 4-wide rows compute lane by lane.
 // Colour 8 may share bodies and is walked by one thread, flag type by flag type over all
 // its joints, as the reference's overflow container does.
+#include <cstring>
 #include "pgx_internal.h"
 
 namespace pgx {
 
 enum { JF_NONE = 0, JF_SOFT = 1, JF_ANGULAR = 2, JF_LIMITED = 4 };
-enum { JT_SPHERICAL = 0, JT_UNIVERSAL = 1, JT_REVOLUTE = 2, JT_FIXED = 3, JT_PRISMATIC = 4, JT_GEAR = 5, JT_SERVO = 6 };
+enum { JT_SPHERICAL = 0, JT_UNIVERSAL = 1, JT_REVOLUTE = 2, JT_FIXED = 3, JT_PRISMATIC = 4, JT_GEAR = 5, JT_SERVO = 6,
+       JT_CUSTOM = 7 };
 // jRowFlags: 4 flag bits per row (7 rows), then the prismatic limit row's kind
 constexpr int JL_SHIFT = 28, JL_UPPER = 1, JL_LOWER = 2;
 __constant__ int kFlagOrder[6] = {JF_NONE, JF_ANGULAR, JF_SOFT, JF_LIMITED, JF_ANGULAR | JF_SOFT,
@@ -118,16 +120,28 @@ __device__ __forceinline__ void write_soft(const JointArgs& a, int j, int r, flo
     JROW(r, JR_ANG1T) = t1; JROW(r, JR_EXTRA) = ex;
 }
 
-// world-space data and the rows' values for one joint, every substep
-__device__ void joint_update(const JointArgs& a, int j) {
+// JointSolverData::calculateWorldSpaceData (src/Joint.cpp:5-14) at the current poses
+struct JWorld { m3 u0, u1; f3 r0, r1, p0, p1; };
+__device__ __forceinline__ JWorld joint_world(const JointArgs& a, int j) {
+    JWorld s;
     int A0 = a.jA0[j], A1 = a.jA1[j];
     qf t0 = q4(a.bquat[A0]), t1 = q4(a.bquat[A1]);
-    m3 u0 = mat3_cast(qmul(t0, q4(a.jQ0[j])));
-    m3 u1 = mat3_cast(qmul(t1, q4(a.jQ1[j])));
-    f3 r0 = qrot(t0, xyz(a.jR0[j]));
-    f3 r1 = qrot(t1, xyz(a.jR1[j]));
-    f3 p0 = add(xyz(a.bpos[A0]), qrot(t0, xyz(a.jP0[j])));
-    f3 p1 = add(xyz(a.bpos[A1]), qrot(t1, xyz(a.jP1[j])));
+    s.u0 = mat3_cast(qmul(t0, q4(a.jQ0[j])));
+    s.u1 = mat3_cast(qmul(t1, q4(a.jQ1[j])));
+    s.r0 = qrot(t0, xyz(a.jR0[j]));
+    s.r1 = qrot(t1, xyz(a.jR1[j]));
+    s.p0 = add(xyz(a.bpos[A0]), qrot(t0, xyz(a.jP0[j])));
+    s.p1 = add(xyz(a.bpos[A1]), qrot(t1, xyz(a.jP1[j])));
+    return s;
+}
+
+// world-space data and the rows' values for one joint, every substep
+__device__ void joint_update(const JointArgs& a, int j) {
+    int type = a.jType[j];
+    if (type == JT_CUSTOM) return;   // rows from the host (k_custom_rows)
+    const JWorld ws = joint_world(a, j);
+    const m3 &u0 = ws.u0, &u1 = ws.u1;
+    const f3 r0 = ws.r0, r1 = ws.r1, p0 = ws.p0, p1 = ws.p1;
     f3 d = sub(p1, p0);
     auto angular_row = [&](int r, f3 x, f3 y) {   // c = dot(x, y), axis = cross(y, x)
         f3 ax = cross(y, x);
@@ -135,7 +149,6 @@ __device__ void joint_update(const JointArgs& a, int j) {
         set_xyz(g0, ax); g0.w = dot(x, y); set_xyz(g1, ax);
         JROW(r, JR_ANG0) = g0; JROW(r, JR_ANG1) = g1;
     };
-    int type = a.jType[j];
     if (type == JT_PRISMATIC) {                    // PrismaticJoint.cpp:6-92
         write_row(a, j, 0, u0.c1, cross(r0, u0.c1), cross(r1, u0.c1), dot(d, u0.c1));
         write_row(a, j, 1, u0.c2, cross(r0, u0.c2), cross(r1, u0.c2), dot(d, u0.c2));
@@ -225,6 +238,59 @@ __device__ void joint_update(const JointArgs& a, int j) {
 __global__ void k_joint_update(JointArgs a) {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j < a.J) joint_update(a, j);
+}
+
+// ---- user joint types: rows declared and written by host functions
+// Staging of n custom joints (device and pinned host, in floats): world-space data
+// (CUST_WS per joint), then the layouts (2 ints per joint: rows, packed flags), then the
+// rows (JOINT_MAX_ROWS x CUST_ROW per joint, pgx_joint_row's fields).
+constexpr int CUST_WS = 30, CUST_LAYOUT = 2, CUST_ROW = 15;
+static_assert(sizeof(pgx_joint_world_space) == CUST_WS * sizeof(float), "JointWorldSpaceData layout");
+static_assert(sizeof(pgx_joint_row) == CUST_ROW * sizeof(float), "row layout");
+constexpr size_t cust_floats(size_t n) { return n * (CUST_WS + CUST_LAYOUT + (size_t)JOINT_MAX_ROWS * CUST_ROW); }
+
+// the listed joints' JointWorldSpaceData, in pgx_joint_world_space's layout
+__global__ void k_custom_ws(JointArgs a, const int32_t* __restrict__ list, int n, float* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const JWorld s = joint_world(a, list[k]);
+    float* o = out + (size_t)CUST_WS * k;
+    const f3 v[10] = {s.p0, s.p1, s.r0, s.r1, s.u0.c0, s.u0.c1, s.u0.c2, s.u1.c0, s.u1.c1, s.u1.c2};
+    for (int i = 0; i < 10; ++i) { o[3 * i] = v[i].x; o[3 * i + 1] = v[i].y; o[3 * i + 2] = v[i].z; }
+}
+
+// the step's layouts of the listed joints: row count and flags, and fresh rows
+// (Constraint1D's defaults, include/Physecs/Constraint1D.h:19-35), as k_joint_setup
+__global__ void k_custom_layout(JointArgs a, const int32_t* __restrict__ list, int n, const int32_t* __restrict__ lay) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int j = list[k], nr = lay[CUST_LAYOUT * k];
+    a.jNRows[j] = nr;
+    a.jRowFlags[j] = lay[CUST_LAYOUT * k + 1];
+    for (int r = 0; r < nr; ++r) {
+        for (int f = 0; f < JR_NF; ++f) JROW(r, f) = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 e = JROW(r, JR_LIN1T); e.w = -3.402823466e+38f; JROW(r, JR_LIN1T) = e;
+        float4 m = JROW(r, JR_ANG0T); m.w = 3.402823466e+38f; JROW(r, JR_ANG0T) = m;
+    }
+}
+
+// the rows the host functions wrote this substep, into the fields the solve reads (lambda
+// and the preSolve products untouched)
+__global__ void k_custom_rows(JointArgs a, const int32_t* __restrict__ list, int n, const float* __restrict__ rows) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int j = list[k], nr = a.jNRows[j];
+    for (int r = 0; r < nr; ++r) {
+        const float* q = rows + ((size_t)k * JOINT_MAX_ROWS + r) * CUST_ROW;
+        float4 l = JROW(r, JR_LIN), g0 = JROW(r, JR_ANG0), g1 = JROW(r, JR_ANG1);
+        float4 mn = JROW(r, JR_LIN1T), mx = JROW(r, JR_ANG0T), fr = JROW(r, JR_ANG1T), ex = JROW(r, JR_EXTRA);
+        set_xyz(l, mk3(q[0], q[1], q[2])); l.w = q[9];
+        set_xyz(g0, mk3(q[3], q[4], q[5])); g0.w = q[10];
+        set_xyz(g1, mk3(q[6], q[7], q[8]));
+        mn.w = q[11]; mx.w = q[12]; fr.w = q[13]; ex.x = q[14];
+        JROW(r, JR_LIN) = l; JROW(r, JR_ANG0) = g0; JROW(r, JR_ANG1) = g1;
+        JROW(r, JR_LIN1T) = mn; JROW(r, JR_ANG0T) = mx; JROW(r, JR_ANG1T) = fr; JROW(r, JR_EXTRA) = ex;
+    }
 }
 
 __device__ __forceinline__ m3 world_inv_inertia_j(const JointArgs& a, int b) {
@@ -594,11 +660,77 @@ static JointArgs make_jargs(World& w, float h) {
     return a;
 }
 
+// The custom joints' world-space data at the current poses, on the host (waits for the device).
+static const pgx_joint_world_space* custom_world_space(World& w, const JointArgs& a) {
+    const int n = (int)w.custList.size();
+    w.custDev.alloc(cust_floats(n));
+    w.custHost.alloc(sizeof(float) * cust_floats(n));
+    hipLaunchKernelGGL(k_custom_ws, dim3(div_up(n, 64)), dim3(64), 0, w.stream, a, w.custListDev.p, n, w.custDev.p);
+    PGX_LAUNCH_CHECK();
+    PGX_HIP(hipMemcpyAsync(w.custHost.p, w.custDev.p, sizeof(float) * CUST_WS * n, hipMemcpyDeviceToHost, w.stream));
+    PGX_HIP(hipStreamSynchronize(w.stream));
+    return (const pgx_joint_world_space*)w.custHost.p;
+}
+
+// Joint::getSolverDesc of every custom joint (src/Physecs.cpp:321-355): the step's rows and
+// their flags, fresh rows on the device and on the host.
+static void custom_layouts(World& w, const JointArgs& a) {
+    if (!w.custLayout) throw PgxError{PGX_ERR_INVALID, "the world has custom joints and no joint callbacks"};
+    const int n = (int)w.custList.size();
+    const pgx_joint_world_space* ws = custom_world_space(w, a);
+    int32_t* lay = (int32_t*)((float*)w.custHost.p + (size_t)CUST_WS * n);
+    pgx_joint_row fresh{};
+    fresh.min = -3.402823466e+38f;
+    fresh.max = 3.402823466e+38f;
+    for (int k = 0; k < n; ++k) {
+        const int j = w.custList[k];
+        int32_t fl[JOINT_MAX_ROWS] = {0};
+        const int nr = w.custLayout(w.custUser, j, &ws[k], fl);
+        if (nr < 0 || nr > JOINT_MAX_ROWS)
+            throw PgxError{PGX_ERR_INVALID, "joint " + std::to_string(j) + ": the layout function returned " +
+                                                std::to_string(nr) + " rows"};
+        int packed = 0;
+        for (int r = 0; r < nr; ++r) {
+            const int f = fl[r];
+            if (f != 0 && f != 1 && f != 2 && f != 4 && f != 3 && f != 6)
+                throw PgxError{PGX_ERR_INVALID, "joint " + std::to_string(j) + ": row " + std::to_string(r) +
+                                                    " has flags " + std::to_string(f)};
+            packed |= f << (4 * r);
+        }
+        lay[CUST_LAYOUT * k] = nr;
+        lay[CUST_LAYOUT * k + 1] = packed;
+        w.jRowsTotal += nr - w.hjNRows[j];
+        w.hjNRows[j] = nr;
+        w.hjRowFlags[j] = packed;
+        for (int r = 0; r < JOINT_MAX_ROWS; ++r) w.custRowsHost[(size_t)k * JOINT_MAX_ROWS + r] = fresh;
+    }
+    int32_t* layDev = (int32_t*)(w.custDev.p + (size_t)CUST_WS * n);
+    PGX_HIP(hipMemcpyAsync(layDev, lay, sizeof(int32_t) * CUST_LAYOUT * n, hipMemcpyHostToDevice, w.stream));
+    hipLaunchKernelGGL(k_custom_layout, dim3(div_up(n, 64)), dim3(64), 0, w.stream, a, w.custListDev.p, n, layDev);
+    PGX_LAUNCH_CHECK();
+}
+
+// MakeConstraintsFunc of every custom joint (src/Physecs.cpp:430-439), its rows to the device
+static void custom_rows(World& w, const JointArgs& a) {
+    const int n = (int)w.custList.size();
+    const pgx_joint_world_space* ws = custom_world_space(w, a);
+    for (int k = 0; k < n; ++k)
+        w.custRows(w.custUser, w.custList[k], &ws[k], &w.custRowsHost[(size_t)k * JOINT_MAX_ROWS],
+                   w.hjNRows[w.custList[k]]);
+    const size_t off = (size_t)(CUST_WS + CUST_LAYOUT) * n, bytes = sizeof(pgx_joint_row) * JOINT_MAX_ROWS * n;
+    std::memcpy((float*)w.custHost.p + off, w.custRowsHost.data(), bytes);
+    PGX_HIP(hipMemcpyAsync(w.custDev.p + off, (float*)w.custHost.p + off, bytes, hipMemcpyHostToDevice, w.stream));
+    hipLaunchKernelGGL(k_custom_rows, dim3(div_up(n, 64)), dim3(64), 0, w.stream, a, w.custListDev.p, n,
+                       w.custDev.p + off);
+    PGX_LAUNCH_CHECK();
+}
+
 void jt_step_setup(World& w) {
     if (!w.J) return;
     JointArgs a = make_jargs(w, 0.f);
     hipLaunchKernelGGL(k_joint_setup, dim3(div_up(w.J, 256)), dim3(256), 0, w.stream, a);
     PGX_LAUNCH_CHECK();
+    if (!w.custList.empty()) custom_layouts(w, a);
 }
 
 void jt_update(World& w, float h) {
@@ -606,6 +738,7 @@ void jt_update(World& w, float h) {
     JointArgs a = make_jargs(w, h);
     hipLaunchKernelGGL(k_joint_update, dim3(div_up(w.J, 256)), dim3(256), 0, w.stream, a);
     PGX_LAUNCH_CHECK();
+    if (!w.custList.empty()) custom_rows(w, a);
 }
 
 // phase 0 = preSolve, 1 = solve (warm start on the first iteration)

@@ -52,6 +52,26 @@ struct DBuf {
 };
 
 // Pinned host memory for read-backs (one allocation, freed with its owner).
+// pinned host memory that grows by reallocation (contents not kept)
+struct PinnedBytes {
+    void* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t bytes) {
+        if (bytes <= n) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        PGX_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+        n = bytes;
+    }
+    PinnedBytes() = default;
+    PinnedBytes(const PinnedBytes&) = delete;
+    PinnedBytes& operator=(const PinnedBytes&) = delete;
+    ~PinnedBytes() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 struct PinnedInts {
     int32_t* p = nullptr;
     void alloc(size_t n) {
@@ -301,6 +321,15 @@ struct World {
     DBuf<float4> jPar0, jPar1;           // (lower, upper, target, stiffness), (damping, gear ratio)
     DBuf<float4> jGear;                  // gear state: persistent angle 0, 1, virtual angle 0, 1
     DBuf<int32_t> jGearInit;
+    // user joint types (PGX_JOINT_CUSTOM): the host functions that declare and write their rows
+    pgx_joint_layout_fn custLayout = nullptr;
+    pgx_joint_rows_fn custRows = nullptr;
+    void* custUser = nullptr;
+    std::vector<int32_t> custList;               // live custom joints, colour order, creation order inside
+    std::vector<pgx_joint_row> custRowsHost;     // JOINT_MAX_ROWS per custom joint, kept across a step's substeps
+    DBuf<int32_t> custListDev;
+    DBuf<float> custDev;                         // world-space data out, layouts and rows in
+    PinnedBytes custHost;                        // the same, on the host
 
     // last step stats
     int lastPairs = 0, lastManifolds = 0, lastColors = 0, lastRounds = 0, lastPoints = 0;

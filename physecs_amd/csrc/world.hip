@@ -242,6 +242,7 @@ void convert_scene(const pgx_scene_desc* sc, uint32_t entityBase, SceneArrays& o
 // is cleared.
 void world_reset_host(World& w) {
     w.J = 0; w.jCap = 0; w.jRowsTotal = 0;
+    w.custList.clear(); w.custRowsHost.clear();
     w.meshColsHost.clear();   // the server's worlds bound meshes in k_bounds
     for (auto* v : {&w.hjType, &w.hjA0, &w.hjA1, &w.hjB0, &w.hjB1, &w.hjRowFlags, &w.hjNRows, &w.hjColor, &w.hjAlive})
         v->clear();
@@ -611,6 +612,10 @@ namespace pgx {
 // host layout holds its fixed rows and drive.
 static int joint_layout(int type, bool drive, int* packed) {
     int fl[7], nr = 0;
+    if (type == PGX_JOINT_CUSTOM) {   // declared by the layout callback at each step
+        *packed = 0;
+        return 0;
+    }
     if (type == PGX_JOINT_PRISMATIC) {
         fl[nr++] = 0; fl[nr++] = 0; fl[nr++] = 2; fl[nr++] = 2; fl[nr++] = 2;
         if (drive) fl[nr++] = 1;                               // SOFT
@@ -731,6 +736,13 @@ static void upload_joints(World& w, int keepState) {
     }
     w.jR0.alloc(w.J); w.jR1.alloc(w.J);
     w.jRow.alloc((size_t)JOINT_MAX_ROWS * 8 * w.J);
+    // custom joints in the order the reference calls them: colour by colour, creation
+    // order inside a colour (src/Physecs.cpp:322-325,431-433)
+    w.custList.clear();
+    for (int32_t j : order)
+        if (w.hjType[j] == PGX_JOINT_CUSTOM) w.custList.push_back(j);
+    w.custRowsHost.assign(w.custList.size() * JOINT_MAX_ROWS, pgx_joint_row{});
+    if (!w.custList.empty()) upload(w.custListDev, w.custList, s);
     PGX_HIP(hipStreamSynchronize(s));
 }
 
@@ -779,7 +791,7 @@ int pgx_world_add_joints(pgx_world* h, int n, const pgx_joint_desc* js) {
     World& w = h->w;
     for (int i = 0; i < n; ++i) {
         const pgx_joint_desc& d = js[i];
-        if (d.type < PGX_JOINT_SPHERICAL || d.type > PGX_JOINT_SERVO)
+        if (d.type < PGX_JOINT_SPHERICAL || d.type > PGX_JOINT_CUSTOM)
             return fail(PGX_ERR_INVALID, "joint %d: unknown type %d", i, d.type);
         if (d.actor0 < 0 || d.actor0 >= w.A || d.actor1 < 0 || d.actor1 >= w.A)
             return fail(PGX_ERR_INVALID, "joint %d: actor out of range", i);
@@ -803,6 +815,16 @@ int pgx_world_set_joint_params(pgx_world* h, int index, const pgx_joint_desc* d)
         upload_joints(w, w.J);
         return PGX_OK;
     });
+}
+
+int pgx_world_set_joint_callbacks(pgx_world* h, pgx_joint_layout_fn layout, pgx_joint_rows_fn rows, void* user) {
+    if (!h) return fail(PGX_ERR_INVALID, "NULL argument");
+    if ((layout == nullptr) != (rows == nullptr)) return fail(PGX_ERR_INVALID, "give both joint functions or neither");
+    World& w = h->w;
+    w.custLayout = layout;
+    w.custRows = rows;
+    w.custUser = layout ? user : nullptr;
+    return PGX_OK;
 }
 
 int pgx_world_destroy_joint(pgx_world* h, int index) {
@@ -1373,7 +1395,8 @@ void world_step(World& w, float dt) {
     if (w.timing) PGX_HIP(hipEventRecord(w.ev[2], s));
     int nColors = sv_color(w, M);
     if (w.timing) PGX_HIP(hipEventRecord(w.ev[3], s));
-    if (w.graph && !w.timing && w.execMode == EXEC_COLORS) substeps_graph(w, M, dt);
+    // custom joints hand their rows over at every substep, which a graph cannot wait for
+    if (w.graph && !w.timing && w.execMode == EXEC_COLORS && w.custList.empty()) substeps_graph(w, M, dt);
     else sv_substeps(w, M, dt);
     if (w.timing) PGX_HIP(hipEventRecord(w.ev[4], s));
     sv_finish_step(w, M);

@@ -17,7 +17,9 @@ import math
 import numpy as np
 
 SPHERE, CAPSULE, BOX, CONVEX_MESH, TRIANGLE_MESH = range(5)
-SPHERICAL, UNIVERSAL, REVOLUTE, FIXED, PRISMATIC, GEAR, SERVO = range(7)
+SPHERICAL, UNIVERSAL, REVOLUTE, FIXED, PRISMATIC, GEAR, SERVO, CUSTOM = range(8)
+# Constraint1D flags (include/Physecs/Constraint1D.h:11-16)
+ROW_NONE, ROW_SOFT, ROW_ANGULAR, ROW_LIMITED = 0, 1, 2, 4
 
 # Defaults of the joints' data members (include/Physecs/Joints/*.h)
 JOINT_DEFAULTS = {

@@ -206,6 +206,18 @@ class OracleWorld:
         self.lib.orc_world_destroy_joint.argtypes = [ctypes.c_void_p, ctypes.c_int]
         assert self.lib.orc_world_destroy_joint(self.h, index) == 0
 
+    def set_joint_callbacks(self, layout, rows):
+        """The custom joints' host functions, as api.World.set_joint_callbacks."""
+        from physecs_amd.api import JOINT_LAYOUT_FN, JOINT_ROWS_FN, joint_callbacks
+        self._joint_fns = joint_callbacks(layout, rows)
+        self.lib.orc_world_set_joint_callbacks.argtypes = [ctypes.c_void_p, JOINT_LAYOUT_FN, JOINT_ROWS_FN,
+                                                            ctypes.c_void_p]
+        self.lib.orc_world_set_joint_callbacks(self.h, self._joint_fns[0], self._joint_fns[1], None)
+
+    def joint_error(self):
+        self.lib.orc_world_joint_error.argtypes = [ctypes.c_void_p]
+        return self.lib.orc_world_joint_error(self.h)
+
     def joint_colors(self):
         self.lib.orc_world_joint_color.argtypes = [ctypes.c_void_p, ctypes.c_int]
         return [self.lib.orc_world_joint_color(self.h, i) for i in range(len(self.scene.joints))]
