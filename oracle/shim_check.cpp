@@ -28,9 +28,103 @@ struct TriggerLog : physecs::OnTriggerEnterListener, physecs::OnTriggerExitListe
     }
 };
 
+// User joints, written as a user of the reference writes a Joint subclass
+// (include/Physecs/Joint.h:42-70): rows declared in getSolverDesc, written in the
+// MakeConstraintsFunc it returns. PointJoint, UniversalPair and HingeJoint restate the rows
+// of the built-in spherical, universal and revolute joints (src/JointsUtil.h:8-28,
+// src/Joints/UniversalJoint.cpp, src/Joints/RevoluteJoint.cpp without its drive); a
+// TetherJoint holds its bodies within a length with a limit row declared only while the
+// registry's poses put them farther apart.
+using physecs::Constraint1DLayout;
+using physecs::Constraint1DWriter;
+using physecs::JointSolverDesc;
+using physecs::JointWorldSpaceData;
+
+void point_rows(const JointWorldSpaceData& w, Constraint1DWriter& rows) {
+    const glm::vec3 d = w.p1 - w.p0;
+    rows.next().setLinear({1, 0, 0}).setAngular0({0, w.r0.z, -w.r0.y}).setAngular1({0, w.r1.z, -w.r1.y}).setC(d.x);
+    rows.next().setLinear({0, 1, 0}).setAngular0({-w.r0.z, 0, w.r0.x}).setAngular1({-w.r1.z, 0, w.r1.x}).setC(d.y);
+    rows.next().setLinear({0, 0, 1}).setAngular0({w.r0.y, -w.r0.x, 0}).setAngular1({w.r1.y, -w.r1.x, 0}).setC(d.z);
+}
+
+void angular_row(Constraint1DWriter& rows, const glm::vec3& x, const glm::vec3& y) {
+    const glm::vec3 axis = glm::cross(y, x);
+    rows.next<physecs::ANGULAR>().setAngular0(axis).setAngular1(axis).setC(glm::dot(x, y));
+}
+
+struct PointJoint final : physecs::Joint {
+    PointJoint(entt::entity e0, glm::vec3 p0, glm::quat q0, entt::entity e1, glm::vec3 p1, glm::quat q1)
+        : Joint(e0, p0, q0, e1, p1, q1) {}
+    static void make(JointWorldSpaceData& w, void*, Constraint1DWriter& rows) { point_rows(w, rows); }
+    JointSolverDesc getSolverDesc(entt::registry&, Constraint1DLayout& layout) override {
+        layout.createConstraints<physecs::NONE, 3>();
+        return {nullptr, make};
+    }
+};
+
+struct UniversalPair final : physecs::Joint {
+    UniversalPair(entt::entity e0, glm::vec3 p0, glm::quat q0, entt::entity e1, glm::vec3 p1, glm::quat q1)
+        : Joint(e0, p0, q0, e1, p1, q1) {}
+    static void make(JointWorldSpaceData& w, void*, Constraint1DWriter& rows) {
+        point_rows(w, rows);
+        angular_row(rows, w.u0[2], w.u1[2]);
+    }
+    JointSolverDesc getSolverDesc(entt::registry&, Constraint1DLayout& layout) override {
+        layout.createConstraints<physecs::NONE, 3>();
+        layout.createConstraints<physecs::ANGULAR>();
+        return {nullptr, make};
+    }
+};
+
+struct HingeJoint final : physecs::Joint {
+    HingeJoint(entt::entity e0, glm::vec3 p0, glm::quat q0, entt::entity e1, glm::vec3 p1, glm::quat q1)
+        : Joint(e0, p0, q0, e1, p1, q1) {}
+    static void make(JointWorldSpaceData& w, void*, Constraint1DWriter& rows) {
+        point_rows(w, rows);
+        angular_row(rows, w.u0[0], w.u1[1]);
+        angular_row(rows, w.u0[0], w.u1[2]);
+    }
+    JointSolverDesc getSolverDesc(entt::registry&, Constraint1DLayout& layout) override {
+        layout.createConstraints<physecs::NONE, 3>();
+        layout.createConstraints<physecs::ANGULAR, 2>();
+        return {nullptr, make};
+    }
+};
+
+struct TetherJoint final : physecs::Joint {
+    float length = 1.f;
+    bool taut = false;
+    int tautSteps = 0;
+    TetherJoint(entt::entity e0, glm::vec3 p0, glm::quat q0, entt::entity e1, glm::vec3 p1, glm::quat q1)
+        : Joint(e0, p0, q0, e1, p1, q1) {}
+    static void make(JointWorldSpaceData& w, void* data, Constraint1DWriter& rows) {
+        const auto* t = static_cast<const TetherJoint*>(data);
+        if (!t->taut) return;
+        const glm::vec3 d = w.p1 - w.p0;
+        const float len = glm::length(d);
+        const glm::vec3 n = d / len;
+        rows.next<physecs::LIMITED>().setLinear(n).setAngular0(glm::cross(w.r0, n)).setAngular1(glm::cross(w.r1, n))
+            .setC(len - t->length).setMin(0.f);
+    }
+    JointSolverDesc getSolverDesc(entt::registry& registry, Constraint1DLayout& layout) override {
+        // the anchors in world space from the registry, as PrismaticJoint::getSolverDesc reads it
+        const auto& t0 = registry.get<TransformComponent>(entity0);
+        const auto& t1 = registry.get<TransformComponent>(entity1);
+        const glm::vec3 p0 = t0.position + t0.orientation * anchor0Pos;
+        const glm::vec3 p1 = t1.position + t1.orientation * anchor1Pos;
+        taut = glm::length(p1 - p0) > length;
+        if (taut) {
+            layout.createConstraints<physecs::LIMITED>();
+            ++tautSteps;
+        }
+        return {this, make};
+    }
+};
+
 struct Check {
     entt::registry registry;
     std::unique_ptr<physecs::Scene> scene;
+    std::vector<physecs::Joint*> userJoints;
     std::vector<std::unique_ptr<physecs::ConvexMesh>> convex;
     std::vector<std::unique_ptr<physecs::TriangleMesh>> trimesh;
     TriggerLog triggers;
@@ -113,6 +207,39 @@ int sc_create_joint(void* h, const pgx_joint_desc* d, uint32_t e0, uint32_t e1) 
     auto* c = static_cast<Check*>(h);
     return c->scene->createJoint(d->type, (entt::entity)e0, v3(d->anchor0), q4(d->anchor0 + 3), (entt::entity)e1,
                                  v3(d->anchor1), q4(d->anchor1 + 3), d);
+}
+
+// a user joint: kind 0 PointJoint, 1 UniversalPair, 2 HingeJoint, 3 TetherJoint (param: its
+// length); anchors as position xyz, orientation xyzw. Returns its handle.
+int sc_create_user_joint(void* h, int kind, uint32_t e0, const float* a0, uint32_t e1, const float* a1, float param) {
+    auto* c = static_cast<Check*>(h);
+    const auto E0 = (entt::entity)e0, E1 = (entt::entity)e1;
+    physecs::Joint* j = nullptr;
+    if (kind == 0) j = c->scene->createJoint<PointJoint>(E0, v3(a0), q4(a0 + 3), E1, v3(a1), q4(a1 + 3));
+    else if (kind == 1) j = c->scene->createJoint<UniversalPair>(E0, v3(a0), q4(a0 + 3), E1, v3(a1), q4(a1 + 3));
+    else if (kind == 2) j = c->scene->createJoint<HingeJoint>(E0, v3(a0), q4(a0 + 3), E1, v3(a1), q4(a1 + 3));
+    else if (kind == 3) {
+        auto* t = c->scene->createJoint<TetherJoint>(E0, v3(a0), q4(a0 + 3), E1, v3(a1), q4(a1 + 3));
+        t->length = param;
+        j = t;
+    } else {
+        return -1;
+    }
+    c->userJoints.push_back(j);
+    return (int)c->userJoints.size() - 1;
+}
+
+// Scene::destroyJoint(Joint*) of a user joint
+void sc_destroy_user_joint(void* h, int handle) {
+    auto* c = static_cast<Check*>(h);
+    c->scene->destroyJoint(c->userJoints.at(handle));
+    c->userJoints[handle] = nullptr;
+}
+
+// steps in which a tether's getSolverDesc declared its limit row
+int sc_tether_taut_steps(void* h, int handle) {
+    auto* t = dynamic_cast<TetherJoint*>(static_cast<Check*>(h)->userJoints.at(handle));
+    return t ? t->tautSteps : -1;
 }
 
 void sc_set_can_collide(void* h, uint32_t e0, uint32_t e1, int can) {

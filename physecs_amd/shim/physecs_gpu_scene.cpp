@@ -259,6 +259,7 @@ void Scene::rebuild() {
         d.actor1 = actorIndex(joints[op].e1);
         check(pgx_world_add_joints(world, 1, &d), "createJoint");
     }
+    check(pgx_world_set_joint_callbacks(world, &Scene::userJointLayout, &Scene::userJointRows, this), "joint callbacks");
     applyContactFilter();
     if (!carried.empty()) {
         int32_t nGone = 0;
@@ -428,6 +429,63 @@ void Scene::destroyJoint(int joint) {
     joints[joint].alive = false;
     jointLog.push_back(-1 - joint);
     if (world && !dirty) check(pgx_world_destroy_joint(world, joint), "destroyJoint");
+}
+
+void Scene::addUserJoint(Joint* joint) {
+    JointRecord r{};
+    r.desc.type = PGX_JOINT_CUSTOM;
+    const glm::vec3 p0 = joint->getAnchor0Pos(), p1 = joint->getAnchor1Pos();
+    const glm::quat q0 = joint->getAnchor0Or(), q1 = joint->getAnchor1Or();
+    const float a0[7] = {p0.x, p0.y, p0.z, q0.x, q0.y, q0.z, q0.w};
+    const float a1[7] = {p1.x, p1.y, p1.z, q1.x, q1.y, q1.z, q1.w};
+    std::memcpy(r.desc.anchor0, a0, sizeof a0);
+    std::memcpy(r.desc.anchor1, a1, sizeof a1);
+    r.e0 = joint->getEntity0();
+    r.e1 = joint->getEntity1();
+    r.alive = true;
+    r.user = joint;
+    joints.push_back(r);
+    jointLog.push_back((int)joints.size() - 1);
+    if (world && !dirty) {
+        pgx_joint_desc d = r.desc;
+        d.actor0 = actorIndex(r.e0);
+        d.actor1 = actorIndex(r.e1);
+        check(pgx_world_add_joints(world, 1, &d), "createJoint");
+    }
+}
+
+void Scene::destroyJoint(Joint* joint) {
+    for (size_t j = 0; j < joints.size(); ++j)
+        if (joints[j].alive && joints[j].user == joint) {
+            destroyJoint((int)j);
+            joints[j].user = nullptr;
+            delete joint;
+            return;
+        }
+}
+
+// Joint::getSolverDesc of a user joint (src/Physecs.cpp:338-339), on the registry as the
+// last step left it; the step fails when the joint declares more rows than the device holds
+int Scene::userJointLayout(void* scene, int joint, const pgx_joint_world_space*, int32_t* flags) {
+    Scene& s = *static_cast<Scene*>(scene);
+    JointRecord& r = s.joints.at(joint);
+    std::vector<int32_t> declared;
+    Constraint1DLayout layout(declared);
+    r.solver = r.user->getSolverDesc(s.registry, layout);
+    if (declared.size() <= (size_t)PGX_JOINT_MAX_ROWS) std::copy(declared.begin(), declared.end(), flags);
+    return (int)declared.size();
+}
+
+// the MakeConstraintsFunc that getSolverDesc returned, on JointWorldSpaceData (the same layout
+// as pgx_joint_world_space), src/Physecs.cpp:431-437
+void Scene::userJointRows(void* scene, int joint, const pgx_joint_world_space* ws, pgx_joint_row* rows, int n) {
+    static_assert(sizeof(JointWorldSpaceData) == sizeof(pgx_joint_world_space), "JointWorldSpaceData layout");
+    Scene& s = *static_cast<Scene*>(scene);
+    JointRecord& r = s.joints.at(joint);
+    JointWorldSpaceData data;
+    std::memcpy(&data, ws, sizeof data);
+    Constraint1DWriter writer(rows, n);
+    if (r.solver.makeConstraintsFunc) r.solver.makeConstraintsFunc(data, r.solver.additionalData, writer);
 }
 
 void Scene::clearColliders(entt::entity e) {

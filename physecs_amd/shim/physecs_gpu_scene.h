@@ -11,10 +11,13 @@
 // colliders were added or removed, steps on the GPU, and writes poses and velocities back
 // into the components in place, as the reference's simulate does (src/Physecs.cpp:501-530).
 //
-// Joints: the reference's Joint subclasses derive from include/Physecs/Joint.h, whose
-// Constraint1DContainer.h is MSVC-only; here a joint is created by its built-in type
-// (PGX_JOINT_*) and its data members (pgx_joint_desc), which is what createJoint<T> and the
-// joints' setters carry (src/Joints/*.cpp).
+// Joints: a built-in joint is created by its type (PGX_JOINT_*) and its data members
+// (pgx_joint_desc), which is what createJoint<T> and the joints' setters carry
+// (src/Joints/*.cpp), and runs entirely on the device. A user's own Joint subclass
+// (include/Physecs/Joint.h) is created with createJoint<T> as in the reference; its
+// getSolverDesc and MakeConstraintsFunc write rows through this folder's
+// Constraint1DContainer.h and run on the host at every step and substep
+// (pgx_world_set_joint_callbacks).
 #pragma once
 
 #include <cstdint>
@@ -27,6 +30,8 @@
 #include <glm/gtc/quaternion.hpp>
 
 #include "Components.h"
+#include "Constraint1DContainer.h"
+#include "Joint.h"
 #include "Transform.h"
 #include "physecs_gpu.h"
 
@@ -93,6 +98,21 @@ public:
     void setJointParams(int joint, const pgx_joint_desc& params);   // the joints' setters
     void destroyJoint(int joint);                                    // Physecs.h:215
 
+    // createJoint<T> (Physecs.h:209-214) for a user Joint subclass T: the Scene calls T's
+    // getSolverDesc at every step and the MakeConstraintsFunc it returns at every substep
+    // (src/Physecs.cpp:321-355,430-439) with the world-space data of the device's poses.
+    template <typename T>
+    T* createJoint(entt::entity entity0, glm::vec3 anchor0Pos, glm::quat anchor0Or, entt::entity entity1,
+                   glm::vec3 anchor1Pos, glm::quat anchor1Or) {
+        T* joint = new T(entity0, anchor0Pos, anchor0Or, entity1, anchor1Pos, anchor1Or);
+        addUserJoint(joint);
+        return joint;
+    }
+    // Scene::destroyJoint(Joint*) (Physecs.h:215, src/Physecs.cpp:712-723) for a joint made by
+    // createJoint<T>: the joint leaves the world and is deleted. Joints still alive when the
+    // Scene is destroyed are not deleted, as in the reference.
+    void destroyJoint(Joint* joint);
+
     // Physecs.h:216-217: collider edits of one entity (the world is rebuilt from the
     // registry at the next simulate; the contact cache starts over)
     void clearColliders(entt::entity entity);
@@ -138,6 +158,8 @@ private:
         pgx_joint_desc desc;
         entt::entity e0, e1;
         bool alive;
+        Joint* user;             // a user subclass (PGX_JOINT_CUSTOM), or NULL
+        JointSolverDesc solver;  // what its getSolverDesc returned for this step
     };
     std::vector<JointRecord> joints;
     std::vector<int> jointLog;                   // creations (index) and destructions (-1 - index)
@@ -160,6 +182,9 @@ private:
     std::vector<const void*> triCachePtr;
     std::vector<size_t> triCacheBytes;
 
+    void addUserJoint(Joint* joint);
+    static int userJointLayout(void* scene, int joint, const pgx_joint_world_space* ws, int32_t* flags);
+    static void userJointRows(void* scene, int joint, const pgx_joint_world_space* ws, pgx_joint_row* rows, int n);
     void connectSignals();
     void markDirty(entt::registry&, entt::entity) { dirty = true; }
     void onTransformPatched(entt::registry&, entt::entity e);

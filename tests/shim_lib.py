@@ -67,6 +67,9 @@ def lib():
                                       ctypes.c_int]
         L.sc_clear_colliders.argtypes = [vp, ctypes.c_uint32]
         L.sc_set_kinematic.argtypes = [vp, ctypes.c_uint32, ctypes.c_int]
+        L.sc_create_user_joint.argtypes = [vp, ctypes.c_int, ctypes.c_uint32, F, ctypes.c_uint32, F, ctypes.c_float]
+        L.sc_destroy_user_joint.argtypes = [vp, ctypes.c_int]
+        L.sc_tether_taut_steps.argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -204,6 +207,20 @@ class Registry:
 
     def create_joint(self, desc, e0, e1):
         return int(self.L.sc_create_joint(self.h, ctypes.byref(desc), e0, e1))
+
+    # user Joint subclasses in the harness: createJoint<T> on the drop-in Scene
+    POINT, UNIVERSAL, HINGE, TETHER = range(4)
+
+    def create_user_joint(self, kind, e0, anchor0, e1, anchor1, param=0.0):
+        h = int(self.L.sc_create_user_joint(self.h, kind, e0, _f(anchor0), e1, _f(anchor1), param))
+        assert h >= 0
+        return h
+
+    def destroy_user_joint(self, handle):
+        self.L.sc_destroy_user_joint(self.h, handle)
+
+    def tether_taut_steps(self, handle):
+        return int(self.L.sc_tether_taut_steps(self.h, handle))
 
     def set_can_collide(self, e0, e1, can):
         self.L.sc_set_can_collide(self.h, e0, e1, int(can))

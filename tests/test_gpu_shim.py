@@ -8,8 +8,9 @@ are the trigger events. Scenes: every joint type (joints created through
 Scene::createJoint, pairs through setCanCollide), a convex pile on a triangle-mesh terrain
 (the terrain's BVH from the reference's own TriangleMesh constructor, carried by a cache),
 and colliders whose data bits make trigger pairs under a contact filter set through
-Scene::setContactFilter. Needs oracle/_ref/libshimcheck.so (built where the reference's
-headers are)."""
+Scene::setContactFilter. User Joint subclasses made by createJoint<T> step as the built-in
+joints they restate, and one that reads the registry in getSolverDesc holds its bodies.
+Needs oracle/_ref/libshimcheck.so (built where the reference's headers are)."""
 import numpy as np
 import pytest
 
@@ -111,3 +112,57 @@ def test_scene_simulate_contact_filter_triggers(api):
     seen = _run(api, sc, 80, check_every=10, filter_fn=share_bit, filter_id=1, triggers=True)
     log(f"  {seen} trigger callbacks")
     assert seen > 10
+
+
+_USER_KIND = {scenes.SPHERICAL: "POINT", scenes.UNIVERSAL: "UNIVERSAL", scenes.REVOLUTE: "HINGE"}
+
+
+def test_scene_user_joint_subclasses(api):
+    """createJoint<T> with user Joint subclasses (the harness's PointJoint, UniversalPair and
+    HingeJoint, written against the drop-in's Constraint1DContainer.h as against the
+    reference's) in place of the built-in spherical, universal and revolute joints: their
+    getSolverDesc and MakeConstraintsFunc run on the host every step and substep, and the
+    bodies move bit for bit as with the built-in joints, through the same Scene."""
+    sc = scenes.joint_scene(1)
+    assert {j[0] for j in sc.joints} <= set(_USER_KIND) and not any(j[5].get("drive_enabled") for j in sc.joints)
+    log(f"[shim] user joint subclasses: {len(sc.joints)} joints, 40 steps")
+    ref, ents0 = shim_lib.replay(sc)
+    usr, ents1 = shim_lib.replay(sc)
+    arr = sc.arrays()
+    with ref, usr:
+        _joints_through_scene(api, ref, ents0, sc)
+        for (t, a0, a1, an0, an1, p) in sc.joints:
+            usr.create_user_joint(getattr(usr, _USER_KIND[t]), ents1[a0], an0, ents1[a1], an1)
+        for k in range(40):
+            ref.simulate(1 / 60, 4, 2)
+            usr.simulate(1 / 60, 4, 2)
+            for d in range(sc.n_dyn):
+                a = int(arr["actor_of_dyn"][d])
+                for x, y in zip(ref.read(ents0[a]), usr.read(ents1[a])):
+                    assert np.array_equal(bits(x), bits(y)), (k, d)
+
+
+def test_scene_user_tether_and_destroy(api):
+    """A user joint whose getSolverDesc reads the registry (as PrismaticJoint's does) and
+    declares its limit row only while the rope is taut; destroyJoint(Joint*) lets go."""
+    sc = scenes.Scene()
+    sc.add_static(scenes.BOX, [10, 0.5, 10], [0, -0.5, 0])
+    post = sc.add_static(scenes.SPHERE, [0.05], [0, 5, 0])
+    box = sc.add_body(scenes.BOX, [0.2, 0.2, 0.2], [0.5, 4.5, 0], vel=(3.0, 0.0, 0.0))
+    log("[shim] user tether: 120 steps, then destroyed, 60 steps")
+    reg, ents = shim_lib.replay(sc)
+    with reg:
+        h = reg.create_user_joint(reg.TETHER, ents[post], (0, 0, 0, 0, 0, 0, 1), ents[box], (0, 0, 0, 0, 0, 0, 1),
+                                  1.0)
+        top = np.array([0, 5, 0], np.float32)
+        far = 0.0
+        for k in range(120):
+            reg.simulate(1 / 60, 4, 2)
+            far = max(far, float(np.linalg.norm(reg.read(ents[box])[0] - top)))
+        assert reg.tether_taut_steps(h) > 10
+        assert far < 1.05, far                     # held within its length
+        y0 = float(reg.read(ents[box])[0][1])
+        reg.destroy_user_joint(h)
+        for k in range(60):
+            reg.simulate(1 / 60, 4, 2)
+        assert float(reg.read(ents[box])[0][1]) < y0 - 2.0   # falls once let go
