@@ -295,13 +295,15 @@ int pgx_world_get_pairs(pgx_world* w, int32_t* out, int cap, int* n);
  * points, 4 x (r0 local, r1 local, targetVelocity). Any pointer may be NULL. */
 int pgx_world_get_manifolds(pgx_world* w, int32_t* keys, int32_t* ints, float* floats, int cap, int* n);
 
-/* Timing of the phases of the last step on the device, in microseconds (HIP events on
- * the world's stream; enabled by pgx_world_enable_timing). */
+/* Timing of the phases of the last timed step on the device, in microseconds (HIP events
+ * on the world's stream). pgx_world_enable_timing(w, n): 0 off, 1 every step, n > 1 every
+ * n-th step from the next one on (the steps between record no events and run as without
+ * timing: a timed step is slower, by its per-launch events and its wait at the end). */
 int pgx_world_enable_timing(pgx_world* w, int enable);
 int pgx_world_get_timing(pgx_world* w, float* broadphase_us, float* narrowphase_us, float* coloring_us,
                          float* solver_us, float* total_us);
 
-/* Cumulative timing since pgx_world_reset_stats, collected while timing is enabled.
+/* Cumulative timing since pgx_world_reset_stats, collected on the timed steps.
  * Default (per-colour) mode: every k_solve launch of the first substep of each step is
  * bracketed by HIP events on the world's stream; solve_bytes is their algorithmic traffic,
  * 284 B per contact point + 12 B per manifold of the launched colour (rows 108 + scalars

@@ -471,8 +471,9 @@ class World:
                                                  _f(f), m, ctypes.byref(n)))
         return keys, ints, f
 
-    def enable_timing(self, on=True):
-        check(self.L.pgx_world_enable_timing(self.h, 1 if on else 0))
+    def enable_timing(self, on=True, period=1):
+        """Per-launch HIP events on every `period`-th step (off with on=False)."""
+        check(self.L.pgx_world_enable_timing(self.h, int(period) if on else 0))
 
     EXEC_MODES = {"colors": 0, "cooperative": 1, "flow": 2, "xcd": 3, "worlds": 4, "pass": 5, "xpass": 6}
 

@@ -334,7 +334,9 @@ struct World {
     // last step stats
     int lastPairs = 0, lastManifolds = 0, lastColors = 0, lastRounds = 0, lastPoints = 0;
     std::vector<int> colorStartHost;
-    bool timing = false;
+    bool timing = false;                 // this step records events (set at each step from the period)
+    int timingPeriod = 0;                // events on every timingPeriod-th step (0: none)
+    long timingCounter = 0;
     int execMode = EXEC_COLORS;          // how the substep loop's contact passes are launched
     // launch shapes, picked by pgx_world_autotune or set by the caller; results do not
     // depend on them

@@ -1280,8 +1280,8 @@ void world_load_state_matched(World& w, const void* buf, size_t bytes, int goneC
 void world_autotune(World& w, float dt, int trialSteps) {
     Snapshot z;
     take_snapshot(w, z);
-    const bool timing = w.timing;
-    w.timing = false;
+    const int period = w.timingPeriod;
+    w.timingPeriod = 0;
     auto trial = [&]() {
         double best = 1e30;
         for (int rep = 0; rep < 2; ++rep) {
@@ -1319,10 +1319,10 @@ void world_autotune(World& w, float dt, int trialSteps) {
             *k.field = bestValue;
         }
     } catch (...) {
-        w.timing = timing;
+        w.timingPeriod = period;
         throw;
     }
-    w.timing = timing;
+    w.timingPeriod = period;
     restore_snapshot(w, z);
 }
 
@@ -1385,6 +1385,8 @@ void world_step(World& w, float dt) {
         w.nNonColliding = (int)w.hostNonColliding.size();
         w.nonCollidingDirty = false;
     }
+    // per-launch events on every timingPeriod-th step only: the steps between them run bare
+    w.timing = w.timingPeriod > 0 && w.timingCounter++ % w.timingPeriod == 0;
     if (w.timing && !w.ev[0])
         for (auto& e : w.ev) PGX_HIP(hipEventCreate(&e));
     if (w.timing) PGX_HIP(hipEventRecord(w.ev[0], s));
@@ -1966,7 +1968,9 @@ int pgx_world_get_exec(pgx_world* h, int* mode) {
 
 int pgx_world_enable_timing(pgx_world* h, int enable) {
     if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
-    h->w.timing = enable != 0;
+    if (enable < 0) return fail(PGX_ERR_INVALID, "negative timing period");
+    h->w.timingPeriod = enable;
+    h->w.timingCounter = 0;
     return PGX_OK;
 }
 
