@@ -463,7 +463,7 @@ def main():
     ap.add_argument("--tune", default=None, help="launch shapes: 'auto' (pgx_world_autotune after the warm-up), "
                     "'default', or a JSON object of pgx_tuning fields; by default 'auto' for the batched worlds "
                     "and 'default' otherwise")
-    ap.add_argument("--seconds", type=float, default=20.0, help="served: length of the request stream")
+    ap.add_argument("--seconds", type=float, default=60.0, help="served: length of the request stream")
     ap.add_argument("--admit-interval", type=int, default=8, help="served: steps between admission boundaries")
     ap.add_argument("--copy-calibration", action="store_true", help="run only the 1 GiB copy used to scale PMC counters")
     args = ap.parse_args()
@@ -526,7 +526,9 @@ def run_world(args):
         w.step(dt)
     w.sync()
     if args.tune is None:
-        args.tune = "auto" if args.config == "batched" else "default"
+        # the library's autotuner where the default launch shapes are not the fastest: the
+        # batch's worlds and the 210-box pyramid (one small world: per-world passes)
+        args.tune = "auto" if args.config in ("batched", "pyramid") else "default"
     if args.tune == "auto":
         chosen = w.autotune(dt, trial_steps=4)   # restores the world's state
         print(f"autotune chose {chosen}, exec {w.exec_mode()}", file=sys.stderr, flush=True)

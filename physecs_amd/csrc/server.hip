@@ -49,7 +49,10 @@ struct Request {
     uint32_t entityBase = 0;
     int horizon = 0, stepsDone = 0;
     double footprint = 0;
-    int state = 0;                      // 0 queued, 1 active, 2 done (may still ride in the batch)
+    // 0 queued, 1 active, 2 done (may still ride in the batch), 3 done and its result taken
+    // while it still rides in the batch (the next boundary retires and erases it)
+    int state = 0;
+    bool inBatch = false;               // admitted and not yet retired at a boundary
     bool placed = false;                // has a placement in the current batched world
     // placement in the current batched world: world id, first dynamic body, first other
     // body (counted after all dynamic bodies), first collider, first entry of the dynamic
@@ -613,10 +616,13 @@ int pgx_server_step(pgx_server* S, float dt, pgx_server_step_info* info) {
         if (S->sinceBoundary >= S->admitInterval || !anyRunning) {
             std::vector<int> keep;
             for (size_t i = 0; i < S->active.size(); ++i) {
-                Request& r = S->reqs[S->active[i]];
-                if (r.state == 2) {
+                auto it = S->reqs.find(S->active[i]);
+                Request& r = it->second;
+                if (r.state == 2 || r.state == 3) {
                     S->footprint -= r.footprint;
+                    r.inBatch = false;
                     ++retired;
+                    if (r.state == 3) S->reqs.erase(it);   // its result was taken already
                 } else {
                     keep.push_back(r.id);
                 }
@@ -626,6 +632,7 @@ int pgx_server_step(pgx_server* S, float dt, pgx_server_step_info* info) {
                 if (!keep.empty() && S->footprint + r.footprint > S->budget) break;
                 S->queue.pop_front();
                 r.state = 1;
+                r.inBatch = true;
                 S->footprint += r.footprint;
                 keep.push_back(r.id);
                 ++admitted;
@@ -699,6 +706,7 @@ int pgx_server_done(pgx_server* S, int id) {
     if (!S) return srv_fail(PGX_ERR_INVALID, "server is NULL");
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) return srv_fail(PGX_ERR_INVALID, "unknown request");
+    if (it->second.state == 3) return srv_fail(PGX_ERR_INVALID, "request already taken");
     return it->second.state == 2 ? 1 : 0;
 }
 
@@ -723,7 +731,16 @@ int pgx_server_take_result(pgx_server* S, int id, float* pos, float* quat, float
         if (vel) { vel[3 * d] = r.rvel[d].x; vel[3 * d + 1] = r.rvel[d].y; vel[3 * d + 2] = r.rvel[d].z; }
         if (angvel) { angvel[3 * d] = r.rang[d].x; angvel[3 * d + 1] = r.rang[d].y; angvel[3 * d + 2] = r.rang[d].z; }
     }
-    S->reqs.erase(it);
+    if (r.inBatch) {
+        // still in the batched world until the next boundary, which retires it (its
+        // footprint leaves the budget there) and erases it; the result is gone already
+        Request& q = it->second;
+        q.state = 3;
+        std::vector<float4>().swap(q.rpos); std::vector<float4>().swap(q.rquat);
+        std::vector<float4>().swap(q.rvel); std::vector<float4>().swap(q.rang);
+    } else {
+        S->reqs.erase(it);
+    }
     return PGX_OK;
 }
 

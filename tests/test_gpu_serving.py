@@ -89,3 +89,34 @@ def test_server_budget_limits_admission(api):
         srv.step(1 / 60)
     assert all(srv.done(r) for r in ids)
     srv.close()
+
+
+def test_results_taken_before_their_boundary_free_the_budget(api):
+    """A result taken as soon as its request finishes, before the next admission boundary
+    retires the request: the boundary still retires it and returns its footprint to the
+    budget, so the queue behind it is admitted and drains, and the results equal the
+    requests' alone-runs. (Taking it used to leave the request's footprint counted for good.)"""
+    scs = [scenes.served_request(40 + k, n_bodies=48) for k in range(6)]
+    fp = max(api.Server.footprint(sc) for sc in scs)
+    srv = api.Server(substeps=4, iterations=2, budget_bytes=2.5 * fp, admit_interval=8)
+    horizons = [5, 9, 6, 11, 7, 5]
+    ids = [srv.submit(sc, h) for sc, h in zip(scs, horizons)]
+    log("[served] 6 requests, results taken at once, budget for two")
+    got = {}
+    for k in range(200):
+        info = srv.step(1 / 60)
+        for r in ids:
+            if r not in got and srv.done(r):
+                got[r] = srv.take_result(r)
+                with pytest.raises(api.PgxError):
+                    srv.done(r)          # taken: gone for the caller
+        if len(got) == len(ids) and info["active"] == 0:
+            break
+    assert len(got) == len(ids), (len(got), info)
+    info = srv.step(1 / 60)
+    assert info["footprint_bytes"] == 0 and info["queued"] == 0, info
+    for r, sc, h in zip(ids, scs, horizons):
+        want = alone(api, sc, h)
+        for x, y in zip(got[r], want):
+            assert np.array_equal(bits(x), bits(y)), r
+    srv.close()
