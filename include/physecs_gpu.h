@@ -319,6 +319,11 @@ typedef struct pgx_stats {
     double solve_us;
     double solve_bytes;
     double step_us;
+    /* the runs of small colours solved in one workgroup (k_solve_batch), bracketed and
+     * counted the same way over their colours' points and manifolds */
+    long batch_launches;
+    double batch_us;
+    double batch_bytes;
 } pgx_stats;
 int pgx_world_reset_stats(pgx_world* w);
 /* Execution of the substep loop's contact passes (the results are the same in every mode):

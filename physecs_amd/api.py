@@ -118,7 +118,8 @@ class PgxCounts(ctypes.Structure):
 
 class PgxStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_long), ("solve_launches", ctypes.c_long), ("solve_us", ctypes.c_double),
-                ("solve_bytes", ctypes.c_double), ("step_us", ctypes.c_double)]
+                ("solve_bytes", ctypes.c_double), ("step_us", ctypes.c_double), ("batch_launches", ctypes.c_long),
+                ("batch_us", ctypes.c_double), ("batch_bytes", ctypes.c_double)]
 
 
 class PgxServerConfig(ctypes.Structure):

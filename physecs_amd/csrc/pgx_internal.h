@@ -415,10 +415,11 @@ struct World {
     // per-launch timing of the contact solve (first substep of each timed step)
     DBuf<int32_t> colorPoints;           // contact points per colour (this step)
     std::vector<hipEvent_t> solveEv;
-    std::vector<int> solveEvColor;
+    std::vector<int> solveEvColor;       // the launch's colour (-1: every colour of the pass)
+    std::vector<int> solveEvColorEnd;    // a k_solve_batch run: one past its last colour (else 0)
     int solveEvUsed = 0;
-    long statSteps = 0, statSolveLaunches = 0;
-    double statSolveUs = 0, statSolveBytes = 0, statStepUs = 0;
+    long statSteps = 0, statSolveLaunches = 0, statBatchLaunches = 0;
+    double statSolveUs = 0, statSolveBytes = 0, statStepUs = 0, statBatchUs = 0, statBatchBytes = 0;
     long lastPointsHost = 0;
 };
 

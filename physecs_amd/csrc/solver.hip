@@ -1537,7 +1537,7 @@ static int xpass_blocks(World& w, int maxColour) {
 // step). They go to hipExtLaunchKernelGGL, which stamps them from the kernel's own dispatch
 // packet: the kernel's execution time, without the launch latency an event recorded on the
 // stream before and after the launch would add.
-static void solve_events(World& w, int color, hipEvent_t* start, hipEvent_t* stop) {
+static void solve_events(World& w, int color, hipEvent_t* start, hipEvent_t* stop, int colorEnd = 0) {
     int idx = 2 * w.solveEvUsed;
     while ((int)w.solveEv.size() <= idx + 1) {
         hipEvent_t e;
@@ -1546,8 +1546,12 @@ static void solve_events(World& w, int color, hipEvent_t* start, hipEvent_t* sto
     }
     *start = w.solveEv[idx];
     *stop = w.solveEv[idx + 1];
-    if ((int)w.solveEvColor.size() <= w.solveEvUsed) w.solveEvColor.resize(w.solveEvUsed + 1);
+    if ((int)w.solveEvColor.size() <= w.solveEvUsed) {
+        w.solveEvColor.resize(w.solveEvUsed + 1);
+        w.solveEvColorEnd.resize(w.solveEvUsed + 1);
+    }
     w.solveEvColor[w.solveEvUsed] = color;
+    w.solveEvColorEnd[w.solveEvUsed] = colorEnd;
     ++w.solveEvUsed;
 }
 
@@ -1777,8 +1781,15 @@ void sv_substeps(World& w, int M, float dt) {
                 const int q = g.z - 1;
                 // 2 bodies per manifold at most
                 const size_t lds = 2 * BATCH_BODY_F4 * sizeof(float4) * (size_t)(bg.E[q] - bg.S[q]);
-                hipLaunchKernelGGL(k_solve_batch, dim3(1), dim3(BATCH_THREADS), lds, s, a, g.x, g.y, bg.S[q],
-                                   w.batchLoc.p, w.batchList.p, w.batchN.p + q, useBias, h, skipSoft);
+                if (rec) {   // a run of colours [g.x, g.y), stamped from its dispatch packet
+                    hipEvent_t e0, e1;
+                    solve_events(w, g.x, &e0, &e1, g.y);
+                    hipExtLaunchKernelGGL(k_solve_batch, dim3(1), dim3(BATCH_THREADS), lds, s, e0, e1, 0, a, g.x, g.y,
+                                          bg.S[q], w.batchLoc.p, w.batchList.p, w.batchN.p + q, useBias, h, skipSoft);
+                } else {
+                    hipLaunchKernelGGL(k_solve_batch, dim3(1), dim3(BATCH_THREADS), lds, s, a, g.x, g.y, bg.S[q],
+                                       w.batchLoc.p, w.batchList.p, w.batchN.p + q, useBias, h, skipSoft);
+                }
                 continue;
             }
             const int c = g.x, b = w.colorStartHost[c], e = w.colorStartHost[c + 1];

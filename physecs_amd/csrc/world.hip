@@ -864,8 +864,8 @@ struct Snapshot {
     int trigCur = 0;
     int cacheCur = 0, cacheCount = 0, lastRounds = 0, lastPairs = 0, lastManifolds = 0, lastColors = 0;
     int lastMeshSlots = 0;
-    long statSteps = 0, statSolveLaunches = 0, lastPointsHost = 0;
-    double statSolveUs = 0, statSolveBytes = 0, statStepUs = 0;
+    long statSteps = 0, statSolveLaunches = 0, lastPointsHost = 0, statBatchLaunches = 0;
+    double statSolveUs = 0, statSolveBytes = 0, statStepUs = 0, statBatchUs = 0, statBatchBytes = 0;
 };
 
 template <class T>
@@ -897,6 +897,7 @@ static void take_snapshot(World& w, Snapshot& z) {
     z.lastColors = w.lastColors; z.lastMeshSlots = w.lastMeshSlots; z.lastPointsHost = w.lastPointsHost;
     z.statSteps = w.statSteps; z.statSolveLaunches = w.statSolveLaunches; z.statSolveUs = w.statSolveUs;
     z.statSolveBytes = w.statSolveBytes; z.statStepUs = w.statStepUs;
+    z.statBatchLaunches = w.statBatchLaunches; z.statBatchUs = w.statBatchUs; z.statBatchBytes = w.statBatchBytes;
     PGX_HIP(hipStreamSynchronize(s));
 }
 
@@ -919,6 +920,7 @@ static void restore_snapshot(World& w, const Snapshot& z) {
     w.lastColors = z.lastColors; w.lastMeshSlots = z.lastMeshSlots; w.lastPointsHost = z.lastPointsHost;
     w.statSteps = z.statSteps; w.statSolveLaunches = z.statSolveLaunches; w.statSolveUs = z.statSolveUs;
     w.statSolveBytes = z.statSolveBytes; w.statStepUs = z.statStepUs;
+    w.statBatchLaunches = z.statBatchLaunches; w.statBatchUs = z.statBatchUs; w.statBatchBytes = z.statBatchBytes;
     PGX_HIP(hipStreamSynchronize(s));
 }
 
@@ -1432,7 +1434,15 @@ void world_step(World& w, float dt) {
             // k_solve_flow (colour -1): the same over every colour of the pass
             float ms = 0.f;
             PGX_HIP(hipEventElapsedTime(&ms, w.solveEv[2 * j], w.solveEv[2 * j + 1]));
-            int c = w.solveEvColor[j];
+            int c = w.solveEvColor[j], ce = w.solveEvColorEnd[j];
+            if (ce > 0) {   // k_solve_batch: the same per-point model over its run of colours
+                double bytes = 0;
+                for (int k = c; k < ce; ++k) bytes += 284.0 * pts[k] + 12.0 * (w.colorStartHost[k + 1] - w.colorStartHost[k]);
+                w.statBatchUs += 1000.0 * ms;
+                w.statBatchBytes += bytes;
+                ++w.statBatchLaunches;
+                continue;
+            }
             double bytes = c >= 0 ? 284.0 * pts[c] + 12.0 * (w.colorStartHost[c + 1] - w.colorStartHost[c])
                                   : 284.0 * points + 12.0 * M;
             w.statSolveUs += 1000.0 * ms;
@@ -1700,8 +1710,8 @@ int pgx_world_get_manifolds(pgx_world* h, int32_t* keys, int32_t* ints, float* f
 int pgx_world_reset_stats(pgx_world* h) {
     if (!h) return fail(PGX_ERR_INVALID, "world is NULL");
     World& w = h->w;
-    w.statSteps = w.statSolveLaunches = 0;
-    w.statSolveUs = w.statSolveBytes = w.statStepUs = 0;
+    w.statSteps = w.statSolveLaunches = w.statBatchLaunches = 0;
+    w.statSolveUs = w.statSolveBytes = w.statStepUs = w.statBatchUs = w.statBatchBytes = 0;
     return PGX_OK;
 }
 
@@ -1713,6 +1723,9 @@ int pgx_world_get_stats(pgx_world* h, pgx_stats* out) {
     out->solve_us = w.statSolveUs;
     out->solve_bytes = w.statSolveBytes;
     out->step_us = w.statStepUs;
+    out->batch_launches = w.statBatchLaunches;
+    out->batch_us = w.statBatchUs;
+    out->batch_bytes = w.statBatchBytes;
     return PGX_OK;
 }
 
