@@ -122,7 +122,8 @@ def test_scene_user_joint_subclasses(api):
     HingeJoint, written against the drop-in's Constraint1DContainer.h as against the
     reference's) in place of the built-in spherical, universal and revolute joints: their
     getSolverDesc and MakeConstraintsFunc run on the host every step and substep, and the
-    bodies move bit for bit as with the built-in joints, through the same Scene."""
+    bodies move bit for bit as with the built-in joints, through the same Scene, also after a
+    collider edit has rebuilt the world and replayed the joint creations."""
     sc = scenes.joint_scene(1)
     assert {j[0] for j in sc.joints} <= set(_USER_KIND) and not any(j[5].get("drive_enabled") for j in sc.joints)
     log(f"[shim] user joint subclasses: {len(sc.joints)} joints, 40 steps")
@@ -134,6 +135,12 @@ def test_scene_user_joint_subclasses(api):
         for (t, a0, a1, an0, an1, p) in sc.joints:
             usr.create_user_joint(getattr(usr, _USER_KIND[t]), ents1[a0], an0, ents1[a1], an1)
         for k in range(40):
+            if k == 20:   # a collider edit rebuilds both worlds; the joints are created again
+                for reg, ents in ((ref, ents0), (usr, ents1)):
+                    reg.clear_colliders(ents[0])
+                    reg.add_collider(ents[0], int(arr["col_type"][0]), arr["col_params"][0],
+                                     material=tuple(arr["col_material"][0]), lpos=tuple(arr["col_pos"][0]),
+                                     lquat=tuple(arr["col_quat"][0]))
             ref.simulate(1 / 60, 4, 2)
             usr.simulate(1 / 60, 4, 2)
             for d in range(sc.n_dyn):
