@@ -222,3 +222,17 @@ def rope_scene(seed=3):
         s.add_body(scenes.BOX, list(rng.uniform(0.15, 0.3, 3)), [rng.uniform(-5, 5), rng.uniform(0.3, 2.0), rng.uniform(-2, 2)],
                    scenes.random_quats(rng, 1)[0])
     return s, Restated(kinds)
+
+
+def hub_scene(n=10):
+    """A dynamic hub with n boxes joined to it by spherical joints: more than 8 joints on one
+    body, so the last ones take the scalar overflow colour."""
+    s = scenes.Scene()
+    s.add_static(scenes.BOX, [10, 0.5, 10], [0, -0.5, 0])
+    hub = s.add_body(scenes.SPHERE, [0.3], [0, 3.0, 0], vel=(0.5, 0.0, 0.2))
+    for i in range(n):
+        a = 2 * np.pi * i / n
+        d = np.array([np.cos(a), 0.2 * (i % 3), np.sin(a)])
+        b = s.add_body(scenes.BOX, [0.1, 0.1, 0.1], [0, 3.0, 0] + 0.8 * d)
+        s.add_joint(scenes.SPHERICAL if i % 2 else scenes.REVOLUTE, hub, b, tuple(0.4 * d), (0, 0, 0))
+    return s

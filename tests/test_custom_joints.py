@@ -116,3 +116,16 @@ def test_refused_layouts(bad):
     o.set_joint_callbacks(lambda j, ws: bad, lambda j, ws, r: None)
     o.step(DT)
     assert o.joint_error() == 1
+
+
+def test_overflow_colour_joints():
+    """Ten joints on one hub: joints 9 and 10 take the scalar overflow colour (8), as custom
+    joints as well as built-in ones."""
+    ref = cj.hub_scene()
+    cus, fns = cj.customised(cj.hub_scene(), BUILTIN)
+    o0, a = _run(ref, 40)
+    o1, b = _run(cus, 40, fns)
+    assert 8 in o1.joint_colors()
+    for k, (x, y) in enumerate(zip(a, b)):
+        for u, v in zip(x, y):
+            assert np.array_equal(_bits(u), _bits(v)), k

@@ -4,8 +4,9 @@ once per step and MakeConstraintsFunc's rows function once per substep (src/Phys
 321-355,430-439), and solves the rows it gets back with the built-in joints' kernels.
 
 - built-in joints restated as custom joints (tests/custom_joint_rows.py) step bit for bit as
-  the built-in ones on the device, through the island launches and, for a chain with more
-  than 64 joints of one colour, the per-colour launches;
+  the built-in ones on the device, through the island launches, the per-colour launches (a
+  chain with more than 64 joints of one colour) and the scalar overflow colour (a hub with
+  ten joints);
 - ropes (a layout that changes from step to step) and springs step bit for bit as the oracle
   with the same functions, and the functions are called in the same order;
 - a refused layout fails the step; a world with custom joints and the graph option on steps
@@ -37,6 +38,8 @@ def _scene(name):
         scenes.capsule_chain(s, [2.0, 3.0, 7.0], n=5, drive=(2.0, 50.0))
         cj.fixed_pairs(s, [-4.0, 1.0, 8.0])
         return s
+    if name == "hub":          # ten joints on one body: two in the scalar overflow colour
+        return cj.hub_scene()
     if name == "long_chain":   # 139 revolute joints in one island: colours 0 and 1 hold ~70 each
         s = scenes.Scene()
         s.add_static(scenes.BOX, [40, 0.5, 40], [0, -0.5, 0])
@@ -50,7 +53,7 @@ def _same(a, b, what):
         assert np.array_equal(bits(x), bits(y)), (what, n)
 
 
-@pytest.mark.parametrize("name", ["chains", "mechanism", "long_chain"])
+@pytest.mark.parametrize("name", ["chains", "mechanism", "long_chain", "hub"])
 def test_restated_builtins_match_the_builtins(api, name):
     ref = _scene(name)
     cus, fns = cj.customised(_scene(name), BUILTIN)
