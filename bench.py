@@ -48,7 +48,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # PMC summaries of the roofline kernel, one per timed window they were summarised over
-TRAFFIC_SUMMARIES = {("plain32k", "colors"): "k_solve_plain32k_traffic_*.md"}
+TRAFFIC_SUMMARIES = {("plain32k", "colors"): "k_solve_plain32k_traffic_*.md",
+                     ("batched", "colors"): "k_solve_batched4096_traffic.md"}
 # steps of each workload's run; the timed window is its end
 RUN_STEPS = {"plain32k": 600, "pyramid": 300, "batched": 300, "convex": 600, "triggers": 200}
 

@@ -121,8 +121,9 @@ def main(argv=None):
     ap.add_argument("--window", default="", help="the window as the bench names it (config.window)")
     ap.add_argument("--fetch-scale", type=float, default=None)
     ap.add_argument("--write-scale", type=float, default=None)
-    ap.add_argument("--traffic-kernel", default="", help="kernel whose scaled FETCH+WRITE bytes per launch "
-                    "become the traffic_bytes_per_launch line")
+    ap.add_argument("--traffic-kernel", action="append", default=[],
+                    help="kernel whose scaled FETCH+WRITE bytes per launch become the traffic_bytes_per_launch "
+                    "line; given more than once, the mean over the launches of all of them")
     ap.add_argument("--counters-dir", action="append", default=[], help="folders of counter runs")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
@@ -184,9 +185,9 @@ def main(argv=None):
                 elif cname.startswith("WRITE_SIZE") and a.write_scale:
                     extra = f" | {mean * 1024 * a.write_scale:.0f} B |"
                 counter_lines.append(f"| `{kern}` | {cname} | {len(v)} | {mean:.4g}{extra or ' | |'}")
-        if a.traffic_kernel:
+        for name in a.traffic_kernel:
             # FETCH_SIZE and WRITE_SIZE come from separate runs: gather them over the runs
-            tk = per.get(_short(a.traffic_kernel)) or per.get(a.traffic_kernel) or {}
+            tk = per.get(_short(name)) or per.get(name) or {}
             fetch_vals += [v for c, d in tk.items() if c.startswith("FETCH_SIZE") for v in d.values()]
             write_vals += [v for c, d in tk.items() if c.startswith("WRITE_SIZE") for v in d.values()]
     if fetch_vals and write_vals and a.fetch_scale and a.write_scale:
