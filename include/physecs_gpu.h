@@ -271,7 +271,8 @@ int pgx_world_destroy_joint(pgx_world* w, int index);
  * substep to hand the joints' world-space data to the host and take their rows back, so a
  * world with custom joints does not run its substep loop as a HIP graph. A step of a world
  * with custom joints and no callbacks fails with PGX_ERR_INVALID. NULL functions remove the
- * callbacks. */
+ * callbacks. The functions run inside pgx_world_step, on the calling thread, and must not
+ * call into the same world. */
 int pgx_world_set_joint_callbacks(pgx_world* w, pgx_joint_layout_fn layout, pgx_joint_rows_fn rows, void* user);
 
 /* One Scene::simulate(time_step) on the device. Asynchronous; pgx_world_sync waits. */
