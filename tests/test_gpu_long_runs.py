@@ -1,6 +1,6 @@
-"""Long runs of the configured workloads, judged as the task judges the reference's
-trajectories (stack stability, penetration, energy), with the device's trajectory held
-against the oracle's, which is pinned to the reference's own compiled code.
+"""Long runs of the configured workloads, measured on stack stability, penetration and
+energy, with the device's trajectory held against the oracle's, which is pinned to the
+reference's own compiled code.
 
 - The 210-box pyramid for its whole 300-step run, bit-identical to the oracle at every tenth
   step. At this configuration (dt 1/60, 4 substeps, 2 iterations) the reference's solver does
