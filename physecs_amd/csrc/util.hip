@@ -138,6 +138,81 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint64_t* __res
     }
 }
 
+// An input of at most one tile (RS_SINGLE keys): every digit pass in one workgroup, the keys
+// and values held in LDS between passes, each pass ranked and placed as k_rs_scatter places
+// a single tile (no other tile's counts), so the output is the same stable order in one
+// launch instead of two per digit.
+constexpr int RS_SINGLE_ITEMS = 16;
+constexpr int RS_SINGLE = RS_THREADS * RS_SINGLE_ITEMS;
+__global__ __launch_bounds__(RS_THREADS) void k_rs_single(uint64_t* __restrict__ keys, int32_t* __restrict__ vals,
+                                                          int n, int bits) {
+    constexpr int ITEMS = RS_SINGLE_ITEMS;
+    __shared__ uint64_t sk[RS_SINGLE];
+    __shared__ int32_t sv[RS_SINGLE];
+    __shared__ int32_t waveCnt[RS_WAVES][RS_BINS];
+    __shared__ int32_t off[RS_BINS];
+    __shared__ int32_t wsum[RS_WAVES];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wv = tid / WAVE;
+    for (int i = tid; i < n; i += RS_THREADS) {
+        sk[i] = keys[i];
+        sv[i] = vals[i];
+    }
+    const int runBase = wv * WAVE * ITEMS;   // the wave's elements: item k covers [k * 64, k * 64 + 64) of its run
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int passes = (bits + 7) / 8;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p, dbits = min(8, bits - 8 * p);
+        __syncthreads();   // the last pass's placement (or the load) is done, and its reads of the counts
+        for (int w = 0; w < RS_WAVES; ++w) waveCnt[w][tid] = 0;
+        uint64_t key[ITEMS];
+        int32_t val[ITEMS];
+        int32_t rank[ITEMS];
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            const int i = runBase + k * WAVE + lane;
+            const bool ok = i < n;
+            key[k] = ok ? sk[i] : 0ull;
+            val[k] = ok ? sv[i] : 0;
+        }
+        __syncthreads();   // counters cleared
+        for (int k = 0; k < ITEMS; ++k)
+            rank[k] = rs_rank(rs_digit(key[k], shift, dbits), runBase + k * WAVE + lane < n, dbits, lt, waveCnt[wv]);
+        __syncthreads();
+        // digit tid: the waves' counts to exclusive prefixes across waves, and the digit's total
+        int32_t total = 0;
+        for (int w = 0; w < RS_WAVES; ++w) {
+            const int32_t c = waveCnt[w][tid];
+            waveCnt[w][tid] = total;
+            total += c;
+        }
+        // the digit's offset: the exclusive scan of the totals over the digits
+        int32_t x = total;
+        for (int o = 1; o < WAVE; o <<= 1) {
+            const int32_t y = __shfl_up(x, o);
+            if (lane >= o) x += y;
+        }
+        if (lane == WAVE - 1) wsum[wv] = x;
+        __syncthreads();
+        int32_t before = 0;
+        for (int w = 0; w < RS_WAVES; ++w) before += w < wv ? wsum[w] : 0;
+        off[tid] = before + x - total;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; ++k) {
+            if (runBase + k * WAVE + lane >= n) continue;
+            const uint32_t d = rs_digit(key[k], shift, dbits);
+            const int pos = off[d] + waveCnt[wv][d] + rank[k];
+            sk[pos] = key[k];
+            sv[pos] = val[k];
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += RS_THREADS) {
+        keys[i] = sk[i];
+        vals[i] = sv[i];
+    }
+}
+
 __global__ void k_rs_copy(const uint64_t* __restrict__ ks, const int32_t* __restrict__ vs, uint64_t* __restrict__ kd,
                           int32_t* __restrict__ vd, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,6 +263,11 @@ void radix_sort_u64_into(hipStream_t s, uint64_t* keys, int32_t* vals, int n, in
                          int32_t* valt, int32_t* hist, int32_t* sums) {
     if (n <= 1 || bits <= 0) return;
     bits = std::min(bits, 64);
+    if (n <= RS_SINGLE) {   // one tile: every pass in one launch
+        hipLaunchKernelGGL(k_rs_single, dim3(1), dim3(RS_THREADS), 0, s, keys, vals, n, bits);
+        PGX_LAUNCH_CHECK();
+        return;
+    }
     if (rs_items(n) == 2) {
         radix_sort_tiles<2>(s, keys, vals, n, bits, kalt, valt, hist, sums);
     } else if (rs_items(n) == 4) {

@@ -28,7 +28,7 @@ def _check(api, keys, bits):
     assert np.array_equal(ko, keys[order]), bits
 
 
-@pytest.mark.parametrize("n", [2, 63, 64, 1023, 1024, 1025, 4097, 65535, 65536, 65537, 300001])
+@pytest.mark.parametrize("n", [2, 63, 64, 1023, 1024, 1025, 4095, 4096, 4097, 65535, 65536, 65537, 300001])
 @pytest.mark.parametrize("bits", [4, 8, 30, 32])
 def test_sort_matches_stable_argsort(api, n, bits):
     log(f"[sort] n={n} bits={bits}")
