@@ -251,6 +251,7 @@ struct World {
     DBuf<uint64_t> usedMask;             // 2 x D color masks
     DBuf<uint64_t> colorKeys;
     DBuf<int32_t> colorPerm;
+    DBuf<int32_t> colorRank;            // per manifold: its slot in solve order
     DBuf<int32_t> colorCount;            // colour start offsets (129 + 1)
     DBuf<int32_t> worldMStart;           // W first manifold of each world
 

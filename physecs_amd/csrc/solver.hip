@@ -250,8 +250,16 @@ __global__ void k_color_start(int M, int n, const uint64_t* __restrict__ keys, i
 
 // ------------------------------------------------------------------ setup (solve order)
 
+// each manifold's slot in solve order (the inverse of the colour sort's permutation)
+__global__ void k_color_rank(int M, const int32_t* __restrict__ perm, int32_t* __restrict__ rank) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < M) rank[perm[s]] = s;
+}
+
+// One thread per manifold in manifold order, so its reads of the manifold arrays are
+// coalesced; the solve-order arrays it writes are scattered instead (stores do not stall).
 __global__ __launch_bounds__(256) void k_setup(
-    int M, int Mcap, const int32_t* __restrict__ perm, const int32_t* __restrict__ mfSrc,
+    int M, int Mcap, const int32_t* __restrict__ rank, const int32_t* __restrict__ mfSrc,
     const int32_t* __restrict__ mfTri, const uint64_t* __restrict__ mfKey, int localBits, const int32_t* __restrict__ worldColStart,
     const int32_t* __restrict__ colOfLocal, const int32_t* __restrict__ cbody, const int32_t* __restrict__ cflags,
     const float4* __restrict__ cmat, const int32_t* __restrict__ npNum, const float4* __restrict__ npNormal,
@@ -264,9 +272,9 @@ __global__ __launch_bounds__(256) void k_setup(
     float4* __restrict__ curLp, int32_t* __restrict__ sB0, int32_t* __restrict__ sB1, int32_t* __restrict__ sA0,
     int32_t* __restrict__ sA1, int32_t* __restrict__ sFlags, int32_t* __restrict__ sCanon, float4* __restrict__ sN,
     float2* __restrict__ sSoft, float4* __restrict__ sR0, float4* __restrict__ sR1, int32_t* __restrict__ sWorld) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= M) return;
-    int m = perm[s];
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    const int s = rank[m];
     int p = mfSrc[m];
     int tri = mfTri[m];
     uint64_t key = mfKey[m];
@@ -1461,7 +1469,8 @@ int sv_color(World& w, int M) {
     w.lastRounds = used;
     // contact setup, written in solve order
     int cur = w.cacheCur ^ 1, prv = w.cacheCur;
-    hipLaunchKernelGGL(k_setup, dim3(nb), dim3(256), 0, s, M, w.manifoldCap, w.colorPerm.p, w.mfSrc.p, w.mfTri.p,
+    hipLaunchKernelGGL(k_color_rank, dim3(nb), dim3(256), 0, s, M, w.colorPerm.p, w.colorRank.p);
+    hipLaunchKernelGGL(k_setup, dim3(nb), dim3(256), 0, s, M, w.manifoldCap, w.colorRank.p, w.mfSrc.p, w.mfTri.p,
                        w.mfKey.p,
                        w.localBits, w.worldColStart.p, w.colOfLocal.p, w.cbody.p, w.cflags.p, w.cmat.p, w.npNum.p,
                        w.npNormal.p, w.npPts.p, w.mfColor.p, w.mfB0.p, w.mfB1.p, w.bpos.p, w.bquat.p, w.dvel.p,

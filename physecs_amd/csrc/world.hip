@@ -354,7 +354,7 @@ void world_alloc_derived(World& w, const pgx_config* cfg, int nTrimesh) {
         w.nXcd = std::max(1, std::min(FLOW_QUEUES, cus / 32));   // 32 CUs per XCD on CDNA3 and CDNA4
     }
 
-    w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorCount.alloc(160); w.colorPoints.alloc(160);
+    w.colorKeys.alloc(M); w.colorPerm.alloc(M); w.colorRank.alloc(M); w.colorCount.alloc(160); w.colorPoints.alloc(160);
     w.sB0.alloc(M); w.sB1.alloc(M); w.sFlags.alloc(M); w.sA0.alloc(M); w.sA1.alloc(M); w.sCanon.alloc(M);
     w.sN.alloc(M); w.sSoft.alloc(M); w.sR0.alloc(4 * M); w.sR1.alloc(4 * M); w.sRow.alloc(4 * CONTACT_ROW_FIELDS * M); w.sLam.alloc(4 * M);
     for (int k = 0; k < 2; ++k) {
@@ -511,7 +511,7 @@ static void release_world(World& w) {
                              &w.colOfLocal, &w.mvals, &w.leafFlags, &w.leafParent, &w.leafEsc, &w.nodeChild,
                              &w.nodeParent, &w.nodeEsc, &w.nodeFlags, &w.nodeCounter, &w.worldRoot, &w.worldStop,
                              &w.pvals, &w.counters, &w.npNum, &w.npIndex, &w.mfPair, &w.mfB0, &w.mfB1, &w.mfColor,
-                             &w.colorPerm, &w.colorCount, &w.worldMStart, &w.sB0, &w.sB1, &w.sFlags, &w.sA0,
+                             &w.colorPerm, &w.colorRank, &w.colorCount, &w.worldMStart, &w.sB0, &w.sB1, &w.sFlags, &w.sA0,
                              &w.sA1, &w.sCanon, &w.cacheNum[0], &w.cacheNum[1], &w.sortT.valt, &w.scanT.sums,
                              &w.dynCols, &w.staticCols, &w.worldStaticStart, &w.refitReady};
     for (auto* b : i32s) b->release();
