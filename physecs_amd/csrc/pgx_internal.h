@@ -106,7 +106,7 @@ constexpr int WORLD_SMALL = 256;       // per-world passes: worlds up to this ma
 constexpr int EPA_MAXV = 104;          // 4 simplex vertices + at most 100 insertions
 constexpr int EPA_MAXF = 256;          // > 2 * EPA_MAXV - 4, the face count of a closed triangulation
 constexpr int EPA_MAXE = 128;          // horizon (loose) edges of one insertion
-constexpr int EPA_LANES = 32768;
+constexpr int EPA_LANES = 65536;
 constexpr int CVX_MAX_FACE_VERTS = 16;
 // triangle-mesh narrowphase: per-lane capacities and lanes
 constexpr int TM_MAXCAND = 256;        // triangles overlapping one body's local bounds
